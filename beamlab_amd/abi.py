@@ -1,0 +1,169 @@
+"""ctypes view of the C ABI (include/beamlab_gpu.h) and numpy dtypes of its records.
+
+The record dtypes are layout-identical to the reference's Node / NodeInfo / Beam / Triangle / Capsule
+(src/phys/softbody.h:43-196), as the C header is.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbeamlab_gpu.so")
+
+NODE = np.dtype([("p", "<f4", 3), ("inv_mass", "<f4"), ("v", "<f4", 3), ("mass", "<f4")])
+NODE_INFO = np.dtype([("flags", "<u2"), ("surface", "u1"), ("pad", "u1"), ("friction", "<f4"), ("radius", "<f4"),
+                      ("frame", "<i4"), ("ror_id", "<i4")])
+BEAM = np.dtype([("a", "<u4"), ("b", "<u4"), ("L", "<f4"), ("k", "<f4"), ("d", "<f4"), ("max_pos", "<f4"),
+                 ("max_neg", "<f4"), ("strength", "<f4"), ("plastic", "<f4"), ("stress", "<f4"), ("L0", "<f4"),
+                 ("support_limit", "<f4"), ("type", "u1"), ("flags", "u1"), ("group", "<u2")])
+TRIANGLE = np.dtype([("a", "<u4"), ("b", "<u4"), ("c", "<u4"), ("surface", "u1"), ("two_sided", "u1"),
+                     ("torn", "u1"), ("pad", "u1"), ("rest_edge2", "<f4")])
+CAPSULE = np.dtype([("a", "<u4"), ("b", "<u4"), ("radius", "<f4"), ("joint", "<i4")])
+BOX = np.dtype([("center", "<f4", 3), ("rot", "<f4", 9), ("half", "<f4", 3), ("aabb_mn", "<f4", 3),
+                ("aabb_mx", "<f4", 3), ("max_force", "<f4"), ("surface", "<u4")])
+assert NODE.itemsize == 32 and NODE_INFO.itemsize == 20 and BEAM.itemsize == 52
+assert TRIANGLE.itemsize == 20 and CAPSULE.itemsize == 16 and BOX.itemsize == 92
+
+# NodeFlag (src/phys/softbody.h:29-40)
+NF_GROUND, NF_FIXED, NF_TYRE, NF_RIM, NF_CONTACTER = 1, 2, 4, 8, 16
+NF_FRAME, NF_LOAD, NF_NO_MOUSE, NF_SELF_COLL = 32, 64, 128, 256
+# BeamType / BeamFlag (src/phys/softbody.h:60-69)
+BT_NORMAL, BT_ROPE, BT_SUPPORT = 0, 1, 2
+BF_BROKEN, BF_INVISIBLE, BF_NO_DEFORM, BF_NO_BREAK, BF_WHEEL, BF_HYDRO, BF_SHOCK = 1, 2, 4, 8, 16, 32, 64
+# Surface (src/phys/static_world.h:59-62)
+SURF_GRASS, SURF_DIRT, SURF_ASPHALT, SURF_CONCRETE, SURF_GRAVEL, SURF_MUD, SURF_SAND = range(7)
+SURF_ROCK, SURF_WOOD, SURF_METAL, SURF_ICE = range(7, 11)
+
+BL_OK, BL_ERR_ARG, BL_ERR_HIP, BL_ERR_UNSUPPORTED, BL_ERR_STATE = 0, -1, -2, -3, -4
+MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT = 0, 1, 2
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_float * 3), ("dt", ctypes.c_float), ("collision_hz", ctypes.c_float),
+                ("frame_theta", ctypes.c_float), ("frame_dissipation", ctypes.c_float),
+                ("inter_body_collisions", ctypes.c_int32), ("sleeping", ctypes.c_int32), ("kill_y", ctypes.c_float)]
+
+
+class BodyOpts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("collision_radius", "ground_friction", "bounce", "contact_push_max",
+                                               "contact_slop", "hull_depth", "sphere_ball", "contact_friction")] + \
+               [(n, ctypes.c_int32) for n in ("allow_break", "allow_deform", "ductile", "self_collision",
+                                               "collision_group", "faces_only")] + \
+               [("air_drag", ctypes.c_float), ("aero_cda", ctypes.c_float), ("rest_damp", ctypes.c_float),
+                ("energy_guard", ctypes.c_int32)]
+
+
+class BodyDesc(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("info", ctypes.c_void_p), ("n_nodes", ctypes.c_uint32),
+                ("beams", ctypes.c_void_p), ("n_beams", ctypes.c_uint32), ("tris", ctypes.c_void_p),
+                ("n_tris", ctypes.c_uint32), ("capsules", ctypes.c_void_p), ("n_capsules", ctypes.c_uint32),
+                ("opts", BodyOpts)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("substeps", "broken_beams", "static_contacts", "vel_clamps",
+                                               "nan_resets", "pair_rebuilds", "contact_pairs", "narrow_tests",
+                                               "contacts", "solve_failures")] + \
+               [("aabb_mn", ctypes.c_float * 3), ("aabb_mx", ctypes.c_float * 3), ("max_speed", ctypes.c_float),
+                ("pad", ctypes.c_float)]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_ if n not in ("aabb_mn", "aabb_mx", "pad")}
+        d["aabb_mn"] = list(self.aabb_mn)
+        d["aabb_mx"] = list(self.aabb_mx)
+        return d
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("substeps_per_launch", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32)]
+
+
+# every entry point the header declares, with its argument types
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "bl_finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, _P, ctypes.c_uint32]),
+    "bl_stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.POINTER(ctypes.c_int32)]),
+    "bl_box_from_quat": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
+    "bl_default_settings": (None, [ctypes.POINTER(Settings)]),
+    "bl_default_body_opts": (None, [ctypes.POINTER(BodyOpts)]),
+    "bl_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "bl_abi_version": (ctypes.c_int, []),
+    "bl_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(Settings)]),
+    "bl_destroy": (None, [_P]),
+    "bl_set_statics": (ctypes.c_int, [_P, _P, ctypes.c_int32]),
+    "bl_add_body": (ctypes.c_int, [_P, ctypes.POINTER(BodyDesc), ctypes.POINTER(ctypes.c_uint32)]),
+    "bl_step": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "bl_step_async": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "bl_sync": (ctypes.c_int, [_P]),
+    "bl_read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+    "bl_write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+    "bl_body_broken_beams": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64)]),
+    "bl_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "bl_reset_stats": (ctypes.c_int, [_P]),
+    "bl_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "bl_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "bl_time_step": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+class BeamlabError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libbeamlab_gpu.so (built by beamlab_amd.build); raises if it is missing - there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BeamlabError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != BL_OK:
+        msg = load().bl_error_string(rc).decode()
+        raise BeamlabError(f"{what}: status {rc}: {msg}")
+    return rc
+
+
+def ptr(a):
+    return None if a is None or len(a) == 0 else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_settings():
+    s = Settings()
+    s.gravity[:] = (0.0, -9.81, 0.0)
+    s.dt = 0.0005
+    s.collision_hz = 120.0
+    s.frame_theta = 0.5
+    s.frame_dissipation = 0.02
+    s.inter_body_collisions = 1
+    s.sleeping = 0
+    s.kill_y = -200.0
+    return s
+
+
+def default_body_opts():
+    o = BodyOpts()
+    o.collision_radius = 0.05
+    o.ground_friction = 1.0
+    o.bounce = -1.0
+    o.contact_push_max = 2.0
+    o.contact_slop = 0.0
+    o.hull_depth = 0.3
+    o.contact_friction = 1.0
+    o.allow_break = 1
+    o.allow_deform = 1
+    return o

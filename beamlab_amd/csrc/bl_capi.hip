@@ -1,0 +1,789 @@
+// The C ABI (include/beamlab_gpu.h): context, device memory, uploads, launches and read-back.
+#include "bl_world.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+using namespace blg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define HIPCHK(x)                                                                                                   \
+    do {                                                                                                            \
+        hipError_t e_ = (x);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                     \
+            g_last_error = std::string(#x) + ": " + hipGetErrorString(e_);                                          \
+            return BL_ERR_HIP;                                                                                      \
+        }                                                                                                           \
+    } while (0)
+
+struct HostBody {
+    std::vector<bl_node> nodes;
+    std::vector<bl_node_info> info;
+    std::vector<bl_beam> beams;
+    std::vector<bl_triangle> tris;
+    std::vector<bl_capsule> caps;
+    bl_body_opts opts;
+    uint32_t node0 = 0, beam0 = 0;
+};
+
+template <class T>
+struct DevArr {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return BL_OK;
+        HIPCHK(hipMalloc((void**)&p, sizeof(T) * count));
+        return BL_OK;
+    }
+    int upload(const std::vector<T>& v) {
+        int rc = alloc(v.size());
+        if (rc) return rc;
+        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return BL_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+} // namespace
+
+struct bl_ctx {
+    bl_settings s{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostBody> bodies;
+    std::vector<bl_box> boxes;
+    bool built = false;
+    int parity = 0;
+    bl_tuning tune{};
+    int64_t substeps = 0;
+    // device
+    DevArr<float4> pm[2], vm[2], iA, iM, bcold;
+    DevArr<uint32_t> ninfo, noff, ch_n0, body_c0;
+    DevArr<float2> naux;
+    DevArr<uint2> iAB;
+    DevArr<float> stress;
+    DevArr<BodyParam> bparam;
+    DevArr<bl_box> dboxes;
+    DevArr<unsigned long long> counters, broken;
+    DevArr<unsigned int> bounds, body_frame;
+    bool frame_pending = false;   // a frame enqueued whose post-frame check has not run
+    int64_t repairs = 0;
+    std::vector<uint32_t> a_slot;   // global beam -> incidence of its a end
+    std::vector<uint32_t> h_noff;
+    KArgs args{};
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    void release() {
+        drop_graphs();
+        for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
+        iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release();
+        body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
+        counters.release(), broken.release(), bounds.release(), body_frame.release();
+        frame_pending = false;
+        built = false;
+    }
+};
+
+namespace {
+
+int effective_block(const bl_ctx* c) { return c->tune.block == 512 || c->tune.block == 1024 ? c->tune.block : 256; }
+
+bool unsupported_opts(const bl_body_opts& o) {
+    return o.air_drag != 0 || o.aero_cda != 0 || o.rest_damp != 0 || o.energy_guard != 0;
+}
+
+// Device state -> host copies of every body (nodes, beam state), so that a rebuild starts from the current state.
+int pull_state(bl_ctx* c) {
+    if (!c->built) return BL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t N = c->pm[0].n;
+    std::vector<float4> P(N), V(N);
+    if (N) {
+        HIPCHK(hipMemcpy(P.data(), c->pm[c->parity].p, sizeof(float4) * N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(V.data(), c->vm[c->parity].p, sizeof(float4) * N, hipMemcpyDeviceToHost));
+    }
+    const size_t E = c->iA.n;
+    std::vector<float4> A(E), M(E);
+    if (E) {
+        HIPCHK(hipMemcpy(A.data(), c->iA.p, sizeof(float4) * E, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(M.data(), c->iM.p, sizeof(float4) * E, hipMemcpyDeviceToHost));
+    }
+    const size_t B = c->stress.n;
+    std::vector<float> S(B);
+    if (B) HIPCHK(hipMemcpy(S.data(), c->stress.p, sizeof(float) * B, hipMemcpyDeviceToHost));
+    for (HostBody& hb : c->bodies) {
+        for (size_t i = 0; i < hb.nodes.size(); i++) {
+            const float4 p = P[hb.node0 + i], v = V[hb.node0 + i];
+            bl_node& n = hb.nodes[i];
+            n.p[0] = p.x, n.p[1] = p.y, n.p[2] = p.z, n.inv_mass = p.w;
+            n.v[0] = v.x, n.v[1] = v.y, n.v[2] = v.z, n.mass = v.w;
+        }
+        for (size_t j = 0; j < hb.beams.size(); j++) {
+            const uint32_t e = c->a_slot[hb.beam0 + j];
+            bl_beam& b = hb.beams[j];
+            b.stress = S[hb.beam0 + j];
+            if (e == 0xffffffffu) continue;
+            b.L = A[e].x, b.strength = A[e].w, b.max_pos = M[e].x, b.max_neg = M[e].y;
+            if (f2u(M[e].w) & TF_BROKEN) b.flags |= BL_BF_BROKEN;
+        }
+    }
+    return BL_OK;
+}
+
+int build(bl_ctx* c) {
+    c->release();
+    const int T = effective_block(c);
+    const uint32_t CAP = (uint32_t)chunk_capacity(T);
+    uint32_t N = 0, B = 0;
+    for (HostBody& hb : c->bodies) {
+        hb.node0 = N, hb.beam0 = B;
+        N += (uint32_t)hb.nodes.size();
+        B += (uint32_t)hb.beams.size();
+    }
+    std::vector<float4> P(N), V(N);
+    std::vector<uint32_t> ninfo(N), deg(N + 1, 0);
+    std::vector<float2> naux(N);
+    std::vector<BodyParam> bp(c->bodies.size());
+    for (size_t k = 0; k < c->bodies.size(); k++) {
+        const HostBody& hb = c->bodies[k];
+        bp[k] = {hb.opts.bounce, hb.opts.contact_push_max, hb.opts.contact_slop, 0};
+        const float fric_body = hb.opts.ground_friction * 1.0f; // (resting is off: src/phys/world.cpp:792)
+        for (size_t i = 0; i < hb.nodes.size(); i++) {
+            const bl_node& n = hb.nodes[i];
+            const uint32_t g = hb.node0 + (uint32_t)i;
+            P[g] = make_float4(n.p[0], n.p[1], n.p[2], n.inv_mass);
+            V[g] = make_float4(n.v[0], n.v[1], n.v[2], n.mass);
+            ninfo[g] = (uint32_t)hb.info[i].flags | ((uint32_t)k << 16);
+            naux[g] = make_float2(hb.info[i].radius, hb.info[i].friction * fric_body);
+        }
+        for (const bl_beam& b : hb.beams) deg[hb.node0 + b.a]++, deg[hb.node0 + b.b]++;
+    }
+    std::vector<uint32_t> noff(N + 1, 0);
+    for (uint32_t i = 0; i < N; i++) noff[i + 1] = noff[i] + deg[i];
+    const uint32_t E = noff[N];
+    std::vector<uint32_t> fill(noff.begin(), noff.end() - 1);
+    std::vector<float4> iA(E), iM(E), bcold(B);
+    std::vector<uint2> iAB(E);
+    std::vector<float> stress(B);
+    c->a_slot.assign(B, 0xffffffffu);
+    for (size_t k = 0; k < c->bodies.size(); k++) {
+        const HostBody& hb = c->bodies[k];
+        const bl_body_opts& o = hb.opts;
+        for (size_t j = 0; j < hb.beams.size(); j++) {
+            const bl_beam& b = hb.beams[j];
+            const uint32_t gb = hb.beam0 + (uint32_t)j, ga = hb.node0 + b.a, gbn = hb.node0 + b.b;
+            uint32_t tf = 0;
+            if (b.flags & BL_BF_BROKEN) tf |= TF_BROKEN;
+            if (b.flags & BL_BF_SHOCK) tf |= TF_SKIP;
+            if ((b.flags & BL_BF_NO_DEFORM) || !o.allow_deform) tf |= TF_NODEFORM;
+            if (b.flags & BL_BF_NO_BREAK) tf |= TF_NOBREAK;
+            if (o.allow_break) tf |= TF_BRK;
+            if (o.ductile) tf |= TF_DUCTILE;
+            tf |= (uint32_t)b.type << TF_TYPE_SHIFT;
+            bcold[gb] = make_float4(b.plastic, b.L0, b.support_limit, 0);
+            stress[gb] = b.stress;
+            for (int side = 0; side < 2; side++) {
+                const uint32_t node = side ? gbn : ga;
+                const uint32_t e = fill[node]++;
+                iA[e] = make_float4(b.L, b.k, b.d, b.strength);
+                iM[e] = make_float4(b.max_pos, b.max_neg, u2f(gb), u2f(tf | (side ? TF_ISB : 0u)));
+                iAB[e] = make_uint2(ga, gbn);
+                if (!side) c->a_slot[gb] = e;
+            }
+        }
+    }
+    // chunks: contiguous node ranges of one body with at most CAP incidences and CAP nodes
+    std::vector<uint32_t> ch_n0, body_c0;
+    for (const HostBody& hb : c->bodies) {
+        body_c0.push_back((uint32_t)ch_n0.size());
+        const uint32_t n0 = hb.node0, n1 = hb.node0 + (uint32_t)hb.nodes.size();
+        uint32_t start = n0;
+        for (uint32_t i = n0; i < n1; i++) {
+            if (deg[i] > CAP) {
+                g_last_error = "a node has more incidences than a workgroup holds";
+                return BL_ERR_UNSUPPORTED;
+            }
+            if (noff[i + 1] - noff[start] > CAP || i + 1 - start > CAP) {
+                ch_n0.push_back(start);
+                start = i;
+            }
+        }
+        if (n1 > start) ch_n0.push_back(start);
+    }
+    body_c0.push_back((uint32_t)ch_n0.size());
+    const uint32_t nchunks = (uint32_t)ch_n0.size();
+    ch_n0.push_back(N);
+    c->h_noff = noff;
+
+    int rc;
+    if ((rc = c->pm[0].upload(P)) || (rc = c->vm[0].upload(V)) || (rc = c->pm[1].alloc(N)) || (rc = c->vm[1].alloc(N)) ||
+        (rc = c->ninfo.upload(ninfo)) || (rc = c->naux.upload(naux)) || (rc = c->noff.upload(noff)) ||
+        (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iAB.upload(iAB)) || (rc = c->bcold.upload(bcold)) ||
+        (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->body_c0.upload(body_c0)) ||
+        (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
+        (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
+        (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
+        return rc;
+    HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
+    HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
+    KArgs& a = c->args;
+    a = KArgs{};
+    a.pm[0] = c->pm[0].p, a.pm[1] = c->pm[1].p, a.vm[0] = c->vm[0].p, a.vm[1] = c->vm[1].p;
+    a.ninfo = c->ninfo.p, a.naux = c->naux.p, a.noff = c->noff.p;
+    a.iA = c->iA.p, a.iM = c->iM.p, a.iAB = c->iAB.p, a.bcold = c->bcold.p, a.stress = c->stress.p;
+    a.ch_n0 = c->ch_n0.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
+    a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
+    a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
+    a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
+    a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
+    a.xcd_remap = 1;
+    c->parity = 0;
+    c->built = true;
+    return BL_OK;
+}
+
+int ensure_built(bl_ctx* c) {
+    if (c->built) return BL_OK;
+    if (c->bodies.empty()) {
+        g_last_error = "no body added";
+        return BL_ERR_STATE;
+    }
+    return build(c);
+}
+
+// One workgroup per body (many substeps per launch, workgroup barriers only) when every body is a few chunks;
+// otherwise one launch per substep spreads the chunks over the chip.
+int resolved_mode(const bl_ctx* c) {
+    if (c->tune.mode == BL_MODE_STEPPED || c->tune.mode == BL_MODE_PERSISTENT) return c->tune.mode;
+    return c->args.nchunks <= 16u * c->args.nbodies ? BL_MODE_PERSISTENT : BL_MODE_STEPPED;
+}
+
+int enqueue(bl_ctx* c, int substeps) {
+    if (substeps <= 0) return BL_OK;
+    bl_tuning t = c->tune;
+    t.block = effective_block(c);
+    t.mode = resolved_mode(c);
+    int rc;
+    if (t.mode == BL_MODE_PERSISTENT) {
+        const int per = t.substeps_per_launch > 0 ? t.substeps_per_launch : 64;
+        for (int s = 0; s < substeps; s += per) {
+            const int k = std::min(per, substeps - s);
+            if ((rc = launch_substeps(c->args, c->stream, c->parity, k, t))) return rc;
+            c->parity ^= (k & 1);
+        }
+    } else if (t.use_graph) {
+        const int per = t.substeps_per_launch > 0 ? t.substeps_per_launch : 32;
+        for (int s = 0; s < substeps; s += per) {
+            const int k = std::min(per, substeps - s);
+            auto key = std::make_pair(k, c->parity);
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                rc = launch_substeps(c->args, c->stream, c->parity, k, t);
+                hipError_t ce = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                HIPCHK(ce);
+                hipGraphExec_t ex;
+                HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                it = c->graphs.emplace(key, ex).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second, c->stream));
+            c->parity ^= (k & 1);
+        }
+    } else {
+        if ((rc = launch_substeps(c->args, c->stream, c->parity, substeps, t))) return rc;
+        c->parity ^= (substeps & 1);
+    }
+    c->substeps += substeps;
+    return BL_OK;
+}
+
+// World::repair_nodes (src/phys/world.cpp:4022-4111), beams only: bad nodes (not finite, faster than 360 m/s,
+// further than 300 m from the median) move to the mean of their good neighbours over up to 8 passes, leftovers
+// rest at the median. Returns the nodes reset, or -1 when more than half of the body is bad.
+int repair_body(HostBody& b) {
+    const size_t n = b.nodes.size();
+    std::vector<float> xs, ys, zs;
+    for (const bl_node& x : b.nodes)
+        if (std::isfinite(x.p[0]) && std::isfinite(x.p[1]) && std::isfinite(x.p[2]))
+            xs.push_back(x.p[0]), ys.push_back(x.p[1]), zs.push_back(x.p[2]);
+    if (xs.empty()) return -1;
+    auto median = [](std::vector<float>& v) {
+        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+        return v[v.size() / 2];
+    };
+    const f3 med = mk(median(xs), median(ys), median(zs));
+    const float reach = 300.0f, vmax = 0.9f * 400.0f;
+    std::vector<uint8_t> bad(n, 0);
+    int nbad = 0;
+    for (size_t i = 0; i < n; i++) {
+        const bl_node& x = b.nodes[i];
+        const bool finite = std::isfinite(x.p[0]) && std::isfinite(x.p[1]) && std::isfinite(x.p[2]) &&
+                            std::isfinite(x.v[0]) && std::isfinite(x.v[1]) && std::isfinite(x.v[2]);
+        const f3 v = mk(x.v[0], x.v[1], x.v[2]);
+        if (!finite || dot(v, v) > vmax * vmax || length(sub(mk(x.p[0], x.p[1], x.p[2]), med)) > reach) bad[i] = 1, nbad++;
+    }
+    if (nbad == 0) return 0;
+    if (nbad * 2 > (int)n) return -1;
+    const int reset = nbad;
+    for (int pass = 0; pass < 8 && nbad > 0; pass++) {
+        std::vector<f3> sp(n, mk(0, 0, 0)), sv(n, mk(0, 0, 0));
+        std::vector<int> cnt(n, 0);
+        auto link = [&](uint32_t x, uint32_t y) {
+            const bl_node &nx = b.nodes[x], &ny = b.nodes[y];
+            if (bad[x] && !bad[y]) {
+                sp[x] = add(sp[x], mk(ny.p[0], ny.p[1], ny.p[2])), sv[x] = add(sv[x], mk(ny.v[0], ny.v[1], ny.v[2])), cnt[x]++;
+            } else if (bad[y] && !bad[x]) {
+                sp[y] = add(sp[y], mk(nx.p[0], nx.p[1], nx.p[2])), sv[y] = add(sv[y], mk(nx.v[0], nx.v[1], nx.v[2])), cnt[y]++;
+            }
+        };
+        for (const bl_beam& bm : b.beams)
+            if (!(bm.flags & BL_BF_BROKEN)) link(bm.a, bm.b);
+        for (size_t i = 0; i < n; i++)
+            if (bad[i] && cnt[i] > 0) {
+                const f3 p = divs(sp[i], (float)cnt[i]), v = divs(sv[i], (float)cnt[i]);
+                b.nodes[i].p[0] = p.x, b.nodes[i].p[1] = p.y, b.nodes[i].p[2] = p.z;
+                b.nodes[i].v[0] = v.x, b.nodes[i].v[1] = v.y, b.nodes[i].v[2] = v.z;
+                bad[i] = 0;
+                nbad--;
+            }
+    }
+    for (size_t i = 0; i < n; i++)
+        if (bad[i]) {
+            b.nodes[i].p[0] = med.x, b.nodes[i].p[1] = med.y, b.nodes[i].p[2] = med.z;
+            b.nodes[i].v[0] = b.nodes[i].v[1] = b.nodes[i].v[2] = 0;
+        }
+    return reset;
+}
+
+// Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.
+int enqueue_frame(bl_ctx* c, int substeps) {
+    if (substeps <= 0) return BL_OK;
+    int rc;
+    if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
+    if ((rc = enqueue(c, substeps))) return rc;
+    if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
+    c->frame_pending = true;
+    return BL_OK;
+}
+
+// The post-frame check of World::simulate_island (src/phys/world.cpp:3962-3979) on the frame just run: a body whose
+// bounds are not finite or wider than 800 m, or whose top speed passed 360 m/s, is repaired on the host, or frozen
+// when it cannot be.
+int post_frame(bl_ctx* c) {
+    if (!c->frame_pending) return BL_OK;
+    c->frame_pending = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t nb = c->bodies.size();
+    std::vector<unsigned int> w(nb * BF_N);
+    HIPCHK(hipMemcpy(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost));
+    std::vector<size_t> trouble;
+    const float lim = 400.0f * 0.9f;
+    for (size_t k = 0; k < nb; k++) {
+        if (c->bodies[k].nodes.empty()) continue;
+        const unsigned int* x = &w[k * BF_N];
+        const f3 mn = mk(fkey_inv(x[BF_MN]), fkey_inv(x[BF_MN + 1]), fkey_inv(x[BF_MN + 2]));
+        const f3 mx = mk(fkey_inv(x[BF_MX]), fkey_inv(x[BF_MX + 1]), fkey_inv(x[BF_MX + 2]));
+        const f3 e = sub(mx, mn);
+        const float ms = std::sqrt(u2f(x[BF_V2]));
+        if (!std::isfinite(mn.x) || !std::isfinite(mx.y) || smax(e.x, smax(e.y, e.z)) > 800.0f || ms > lim)
+            trouble.push_back(k);
+    }
+    if (trouble.empty()) return BL_OK;
+    int rc = pull_state(c);
+    if (rc) return rc;
+    for (size_t k : trouble) {
+        HostBody& hb = c->bodies[k];
+        const int fixed = repair_body(hb);
+        if (fixed < 0)
+            for (bl_node& x : hb.nodes) {
+                if (!std::isfinite(x.p[0]) || !std::isfinite(x.p[1]) || !std::isfinite(x.p[2])) x.p[0] = 0, x.p[1] = -1000, x.p[2] = 0;
+                x.v[0] = x.v[1] = x.v[2] = 0;
+                x.inv_mass = 0;
+            }
+        c->repairs++;
+    }
+    // the repaired state goes back; the counters survive the rebuild
+    std::vector<unsigned long long> cnt(CNT_N, 0), brk(nb, 0);
+    HIPCHK(hipMemcpy(cnt.data(), c->counters.p, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(brk.data(), c->broken.p, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost));
+    c->release();
+    if ((rc = build(c))) return rc;
+    HIPCHK(hipMemcpy(c->counters.p, cnt.data(), sizeof(unsigned long long) * CNT_N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->broken.p, brk.data(), sizeof(unsigned long long) * nb, hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int bl_abi_version(void) { return BL_ABI_VERSION; }
+
+const char* bl_error_string(int status) {
+    if (!g_last_error.empty() && status != BL_OK) return g_last_error.c_str();
+    switch (status) {
+    case BL_OK: return "ok";
+    case BL_ERR_ARG: return "invalid argument";
+    case BL_ERR_HIP: return "HIP runtime error";
+    case BL_ERR_UNSUPPORTED: return "unsupported feature";
+    case BL_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+    }
+}
+
+void bl_default_settings(bl_settings* s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->gravity[1] = -9.81f;
+    s->dt = 0.0005f;
+    s->collision_hz = 120.0f;
+    s->frame_theta = 0.5f;
+    s->frame_dissipation = 0.02f;
+    s->inter_body_collisions = 1;
+    s->sleeping = 0;
+    s->kill_y = -200.0f;
+}
+
+void bl_default_body_opts(bl_body_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->collision_radius = 0.05f;
+    o->ground_friction = 1.0f;
+    o->bounce = -1.0f;
+    o->contact_push_max = 2.0f;
+    o->contact_slop = 0.0f;
+    o->hull_depth = 0.3f;
+    o->contact_friction = 1.0f;
+    o->allow_break = 1;
+    o->allow_deform = 1;
+}
+
+// SoftBody::finalize (src/phys/softbody.cpp:76-83): L = L0 = max(1e-3, |pa - pb|), triangles' longest rest edge.
+int bl_finalize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb, bl_triangle* tris, uint32_t nt) {
+    if ((nb && (!nodes || !beams)) || (nt && (!nodes || !tris))) return BL_ERR_ARG;
+    for (uint32_t j = 0; j < nb; j++) {
+        if (beams[j].a >= n || beams[j].b >= n) return BL_ERR_ARG;
+        const bl_node &pa = nodes[beams[j].a], &pb = nodes[beams[j].b];
+        const float d = length(sub(mk(pa.p[0], pa.p[1], pa.p[2]), mk(pb.p[0], pb.p[1], pb.p[2])));
+        beams[j].L = beams[j].L0 = smax(1e-3f, d);
+    }
+    for (uint32_t t = 0; t < nt; t++) {
+        if (tris[t].a >= n || tris[t].b >= n || tris[t].c >= n) return BL_ERR_ARG;
+        const f3 a = mk(nodes[tris[t].a].p[0], nodes[tris[t].a].p[1], nodes[tris[t].a].p[2]);
+        const f3 b = mk(nodes[tris[t].b].p[0], nodes[tris[t].b].p[1], nodes[tris[t].b].p[2]);
+        const f3 cc = mk(nodes[tris[t].c].p[0], nodes[tris[t].c].p[1], nodes[tris[t].c].p[2]);
+        const f3 ba = sub(b, a), cb = sub(cc, b), ac = sub(a, cc);
+        // max_edge2 (src/phys/softbody.h:154)
+        tris[t].rest_edge2 = smax(dot(ba, ba), smax(dot(cb, cb), dot(ac, ac)));
+    }
+    return BL_OK;
+}
+
+// SoftBody::stabilize's beam part (src/phys/softbody.cpp:85-110).
+int bl_stabilize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb, float dt, float k_budget,
+                 float d_budget, int32_t* changed) {
+    if (nb && (!nodes || !beams)) return BL_ERR_ARG;
+    std::vector<float> ksum(n, 0.0f), dsum(n, 0.0f), ks(n, 1.0f), ds(n, 1.0f);
+    for (uint32_t j = 0; j < nb; j++) {
+        if (beams[j].a >= n || beams[j].b >= n) return BL_ERR_ARG;
+        ksum[beams[j].a] += beams[j].k;
+        ksum[beams[j].b] += beams[j].k;
+        dsum[beams[j].a] += beams[j].d;
+        dsum[beams[j].b] += beams[j].d;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const float m = nodes[i].mass;
+        if (m <= 0) continue;
+        const float kl = ksum[i] * dt * dt / m;
+        if (kl > k_budget) ks[i] = k_budget / kl;
+        const float dl = dsum[i] * dt / m;
+        if (dl > d_budget) ds[i] = d_budget / dl;
+    }
+    int32_t ch = 0;
+    for (uint32_t j = 0; j < nb; j++) {
+        const float a = smin(ks[beams[j].a], ks[beams[j].b]);
+        const float d = smin(ds[beams[j].a], ds[beams[j].b]);
+        if (a < 1.0f || d < 1.0f) ch++;
+        beams[j].k *= a;
+        beams[j].d *= d;
+    }
+    if (changed) *changed = ch;
+    return BL_OK;
+}
+
+// StaticWorld::add_box (src/phys/static_world.cpp:298-306): to_mat3 (src/core/math.h:163-170) and
+// StaticBox::update_aabb (src/phys/static_world.cpp:242-246).
+int bl_box_from_quat(const float center[3], const float half[3], const float q[4], uint32_t surface, bl_box* out) {
+    if (!center || !half || !q || !out) return BL_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    const float x = q[0], y = q[1], z = q[2], w = q[3];
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y, wz = w * z;
+    const float m[9] = {1 - 2 * (yy + zz), 2 * (xy + wz), 2 * (xz - wy), 2 * (xy - wz), 1 - 2 * (xx + zz),
+                        2 * (yz + wx), 2 * (xz + wy), 2 * (yz - wx), 1 - 2 * (xx + yy)};
+    std::memcpy(out->rot, m, sizeof(m));
+    const f3 e = add(add(mul(mk(std::fabs(m[0]), std::fabs(m[1]), std::fabs(m[2])), half[0]),
+                         mul(mk(std::fabs(m[3]), std::fabs(m[4]), std::fabs(m[5])), half[1])),
+                     mul(mk(std::fabs(m[6]), std::fabs(m[7]), std::fabs(m[8])), half[2]));
+    for (int k = 0; k < 3; k++) out->center[k] = center[k], out->half[k] = half[k];
+    out->aabb_mn[0] = center[0] - e.x, out->aabb_mn[1] = center[1] - e.y, out->aabb_mn[2] = center[2] - e.z;
+    out->aabb_mx[0] = center[0] + e.x, out->aabb_mx[1] = center[1] + e.y, out->aabb_mx[2] = center[2] + e.z;
+    out->surface = surface;
+    out->max_force = 0;
+    return BL_OK;
+}
+
+int bl_create(bl_ctx** out, const bl_settings* s) {
+    if (!out || !s) return BL_ERR_ARG;
+    *out = nullptr;
+    if (s->sleeping) {
+        g_last_error = "sleeping bodies are not stepped on the GPU (settings.sleeping must be 0)";
+        return BL_ERR_UNSUPPORTED;
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) {
+        g_last_error = "no HIP device";
+        return BL_ERR_HIP;
+    }
+    bl_ctx* c = new bl_ctx();
+    c->s = *s;
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        g_last_error = "stream/event creation failed";
+        delete c;
+        return BL_ERR_HIP;
+    }
+    c->tune.mode = BL_MODE_AUTO;
+    c->tune.block = 256;
+    c->tune.substeps_per_launch = 0;
+    c->tune.use_graph = 1;
+    *out = c;
+    return BL_OK;
+}
+
+void bl_destroy(bl_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int bl_set_statics(bl_ctx* c, const bl_box* boxes, int32_t n) {
+    if (!c || n < 0 || (n > 0 && !boxes)) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    c->boxes.assign(boxes, boxes + n);
+    for (const bl_box& b : c->boxes)
+        if (b.surface >= BL_SURF_COUNT) return BL_ERR_ARG;
+    c->release();
+    return BL_OK;
+}
+
+int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
+    if (!c || !d) return BL_ERR_ARG;
+    if ((d->n_nodes && (!d->nodes || !d->info)) || (d->n_beams && !d->beams) || (d->n_tris && !d->tris) ||
+        (d->n_capsules && !d->capsules))
+        return BL_ERR_ARG;
+    if (c->bodies.size() >= 65535) return BL_ERR_ARG;
+    if (unsupported_opts(d->opts)) {
+        g_last_error = "air drag, aerodynamic drag, rest damping and the energy guard are not stepped on the GPU";
+        return BL_ERR_UNSUPPORTED;
+    }
+    for (uint32_t j = 0; j < d->n_beams; j++)
+        if (d->beams[j].a >= d->n_nodes || d->beams[j].b >= d->n_nodes || d->beams[j].type > BL_BT_SUPPORT) return BL_ERR_ARG;
+    for (uint32_t i = 0; i < d->n_nodes; i++)
+        if (d->info[i].flags & BL_NF_TYRE) {
+            g_last_error = "tyre nodes are not stepped on the GPU";
+            return BL_ERR_UNSUPPORTED;
+        }
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    HostBody hb;
+    hb.nodes.assign(d->nodes, d->nodes + d->n_nodes);
+    hb.info.assign(d->info, d->info + d->n_nodes);
+    hb.beams.assign(d->beams, d->beams + d->n_beams);
+    if (d->n_tris) hb.tris.assign(d->tris, d->tris + d->n_tris);
+    if (d->n_capsules) hb.caps.assign(d->capsules, d->capsules + d->n_capsules);
+    hb.opts = d->opts;
+    c->bodies.push_back(std::move(hb));
+    c->release();
+    if (id) *id = (uint32_t)c->bodies.size() - 1;
+    return BL_OK;
+}
+
+int bl_step_async(bl_ctx* c, int32_t substeps) {
+    if (!c || substeps < 0) return BL_ERR_ARG;
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if ((rc = post_frame(c))) return rc; // (the previous frame's check first)
+    return enqueue_frame(c, substeps);
+}
+
+int bl_sync(bl_ctx* c) {
+    if (!c) return BL_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return post_frame(c);
+}
+
+int bl_step(bl_ctx* c, int32_t substeps) {
+    int rc = bl_step_async(c, substeps);
+    if (rc) return rc;
+    return bl_sync(c);
+}
+
+int bl_read_body(bl_ctx* c, uint32_t id, bl_node* nodes_out, bl_beam* beams_out) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    const HostBody& hb = c->bodies[id];
+    if (nodes_out && !hb.nodes.empty()) std::memcpy(nodes_out, hb.nodes.data(), sizeof(bl_node) * hb.nodes.size());
+    if (beams_out && !hb.beams.empty()) std::memcpy(beams_out, hb.beams.data(), sizeof(bl_beam) * hb.beams.size());
+    return BL_OK;
+}
+
+int bl_write_body(bl_ctx* c, uint32_t id, const bl_node* nodes, const bl_beam* beams) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    HostBody& hb = c->bodies[id];
+    if (beams)
+        for (size_t j = 0; j < hb.beams.size(); j++)
+            if (beams[j].a != hb.beams[j].a || beams[j].b != hb.beams[j].b || beams[j].type > BL_BT_SUPPORT) return BL_ERR_ARG;
+    if (nodes && !hb.nodes.empty()) std::memcpy(hb.nodes.data(), nodes, sizeof(bl_node) * hb.nodes.size());
+    if (beams && !hb.beams.empty()) std::memcpy(hb.beams.data(), beams, sizeof(bl_beam) * hb.beams.size());
+    // counters survive the rebuild
+    std::vector<unsigned long long> cnt(CNT_N, 0), brk(c->bodies.size(), 0);
+    if (c->built) {
+        HIPCHK(hipMemcpy(cnt.data(), c->counters.p, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(brk.data(), c->broken.p, sizeof(unsigned long long) * brk.size(), hipMemcpyDeviceToHost));
+    }
+    c->release();
+    if ((rc = build(c))) return rc;
+    HIPCHK(hipMemcpy(c->counters.p, cnt.data(), sizeof(unsigned long long) * CNT_N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->broken.p, brk.data(), sizeof(unsigned long long) * brk.size(), hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_body_broken_beams(bl_ctx* c, uint32_t id, int64_t* out) {
+    if (!c || !out || id >= c->bodies.size()) return BL_ERR_ARG;
+    *out = 0;
+    if (!c->built) return BL_OK;
+    unsigned long long v = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&v, c->broken.p + id, sizeof(v), hipMemcpyDeviceToHost));
+    *out = (int64_t)v;
+    return BL_OK;
+}
+
+int bl_get_stats(bl_ctx* c, bl_stats* out) {
+    if (!c || !out) return BL_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->substeps = c->substeps;
+    if (!c->built) return BL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long cnt[CNT_N];
+    HIPCHK(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> brk(c->broken.n);
+    HIPCHK(hipMemcpy(brk.data(), c->broken.p, sizeof(unsigned long long) * brk.size(), hipMemcpyDeviceToHost));
+    out->static_contacts = (int64_t)cnt[CNT_STATIC];
+    out->vel_clamps = (int64_t)cnt[CNT_CLAMPS];
+    out->nan_resets = (int64_t)cnt[CNT_NANS];
+    for (unsigned long long b : brk) out->broken_beams += (int64_t)b;
+    unsigned int init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpy(c->bounds.p, init, sizeof(init), hipMemcpyHostToDevice));
+    int rc = launch_bounds(c->args, c->stream, c->parity, c->bounds.p);
+    if (rc) return rc;
+    unsigned int res[8];
+    HIPCHK(hipMemcpy(res, c->bounds.p, sizeof(res), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++) {
+        const uint32_t lo = res[k], hi = res[3 + k];
+        out->aabb_mn[k] = u2f((lo & 0x80000000u) ? (lo & 0x7fffffffu) : ~lo);
+        out->aabb_mx[k] = u2f((hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi);
+    }
+    out->max_speed = std::sqrt(u2f(res[6]));
+    return BL_OK;
+}
+
+int bl_reset_stats(bl_ctx* c) {
+    if (!c) return BL_ERR_ARG;
+    c->substeps = 0;
+    if (!c->built) return BL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
+    HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
+    return BL_OK;
+}
+
+int bl_set_tuning(bl_ctx* c, const bl_tuning* t) {
+    if (!c || !t) return BL_ERR_ARG;
+    if (t->mode < 0 || t->mode > BL_MODE_PERSISTENT || t->substeps_per_launch < 0) return BL_ERR_ARG;
+    if (t->block != 0 && t->block != 256 && t->block != 512 && t->block != 1024) return BL_ERR_ARG;
+    const bool rechunk = effective_block(c) != (t->block ? t->block : 256);
+    int rc = rechunk ? pull_state(c) : BL_OK;
+    if (rc) return rc;
+    c->tune = *t;
+    if (!c->tune.block) c->tune.block = 256;
+    c->drop_graphs();
+    if (rechunk && c->built) {
+        c->release();
+        return build(c);
+    }
+    return BL_OK;
+}
+
+int bl_get_tuning(bl_ctx* c, bl_tuning* t) {
+    if (!c || !t) return BL_ERR_ARG;
+    *t = c->tune;
+    if (c->built) t->mode = resolved_mode(c);
+    return BL_OK;
+}
+
+int bl_time_step(bl_ctx* c, int32_t substeps, float* ms) {
+    if (!c || !ms || substeps < 0) return BL_ERR_ARG;
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if ((rc = post_frame(c))) return rc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if ((rc = enqueue_frame(c, substeps))) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return post_frame(c);
+}
+
+} // extern "C"

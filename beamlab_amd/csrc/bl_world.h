@@ -1,0 +1,72 @@
+// Device-side world layout shared by the kernels and the host code of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/beamlab_gpu.h"
+#include "bl_devmath.h"
+
+namespace blg {
+
+// Per-incidence flags (iM[e].w): the beam's flags that the force loop reads, folded with its body's settings.
+enum : uint32_t {
+    TF_BROKEN = 1u << 0,   // BF_BROKEN
+    TF_SKIP = 1u << 1,     // BF_SHOCK: left to the shock loop, skipped by the beam loop
+    TF_NODEFORM = 1u << 2, // BF_NO_DEFORM, or the body's allow_deform off
+    TF_NOBREAK = 1u << 3,  // BF_NO_BREAK
+    TF_BRK = 1u << 4,      // the body's allow_break
+    TF_DUCTILE = 1u << 5,  // the body's ductile
+    TF_ISB = 1u << 8,      // this incidence is the beam's b end
+};
+constexpr int TF_TYPE_SHIFT = 6;
+
+enum { CNT_STATIC = 0, CNT_CLAMPS = 1, CNT_NANS = 2, CNT_N = 4 };
+// per-body frame words: [0] top speed squared (float bits), [1..3] min key x y z, [4..6] max key x y z
+enum { BF_V2 = 0, BF_MN = 1, BF_MX = 4, BF_N = 8 };
+
+// order-preserving map of a float onto unsigned integers (for atomic min / max of floats)
+BL_HD uint32_t fkey(float f) {
+    const uint32_t u = f2u(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+BL_HD float fkey_inv(uint32_t k) { return u2f((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+
+struct BodyParam {
+    float bounce, push_max, slop, pad;
+};
+
+struct KArgs {
+    float4* pm[2];
+    float4* vm[2];
+    const uint32_t* ninfo; // node flags (low 16 bits) | body index (high 16 bits)
+    const float2* naux;    // radius, friction x ground friction
+    const uint32_t* noff;  // CSR offsets of the incidences, [N + 1]
+    float4* iA;            // L, k, d, strength
+    float4* iM;            // max_pos, max_neg, beam index (bits), flags (bits)
+    const uint2* iAB;      // a, b
+    const float4* bcold;   // per beam: plastic, L0, support_limit, 0
+    float* stress;         // per beam
+    const uint32_t* ch_n0; // chunk node starts, [chunks + 1]
+    const uint32_t* body_c0; // first chunk of each body, [bodies + 1]
+    const BodyParam* bodies;
+    const bl_box* boxes;
+    unsigned long long* counters; // CNT_*
+    unsigned long long* broken;   // per body
+    unsigned int* body_frame;     // per body, BF_N words: top |v|^2 of the frame (float bits), bounds keys
+    f3 g;
+    float dt;
+    float kill_y;
+    int nbox;
+    uint32_t nnodes;
+    uint32_t nchunks;
+    uint32_t nbodies;
+    int xcd_remap;
+};
+
+int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
+int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out);
+int launch_frame_begin(const KArgs& a, hipStream_t st, int parity);
+int launch_frame_end(const KArgs& a, hipStream_t st, int parity);
+int chunk_capacity(int block);
+
+} // namespace blg

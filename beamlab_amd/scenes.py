@@ -1,0 +1,184 @@
+"""Synthetic scenes of the benchmark configurations, built through the SoftBody interface.
+
+The body recipe is the reference's soft box (build_soft_box, src/world/objects.cpp:892-966): a node lattice with
+beams to neighbours, each pair once in the lattice's loop order, then finalize and stabilize(dt)
+(src/phys/softbody.cpp:76-110). The beam neighbourhood is the 18-neighbourhood (every neighbour closer than
+sqrt(3) spacings on the unjittered lattice). Breaking follows the reference's force rule (|force| > strength,
+src/phys/softbody.cpp:458): a break strain eps means strength = eps * k * L0 with k after stabilize. Seeds drive the
+reference tests' xorshift32 generator (TRng, tests/test_physics.cpp:43-55).
+"""
+import math
+
+import numpy as np
+
+from . import abi
+from .world import SoftBody, World
+
+DT = np.float32(0.0005)
+
+
+class TRng:
+    """xorshift32 as the reference tests' TRng (tests/test_physics.cpp:43-55)."""
+
+    def __init__(self, seed):
+        self.s = (seed * 2654435761 + 1) & 0xFFFFFFFF
+
+    def next(self):
+        s = self.s
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        self.s = s
+        return s
+
+    def uni(self):
+        return np.float32(self.next() & 0xFFFFFF) / np.float32(0x1000000)
+
+    def range(self, a, b):
+        a, b = np.float32(a), np.float32(b)
+        return a + (b - a) * self.uni()
+
+    def uniforms(self, n):
+        """n draws of uni() as a float32 array."""
+        out = np.empty(n, np.float32)
+        s = self.s
+        for i in range(n):
+            s ^= (s << 13) & 0xFFFFFFFF
+            s ^= s >> 17
+            s ^= (s << 5) & 0xFFFFFFFF
+            out[i] = s & 0xFFFFFF
+        self.s = s
+        return out / np.float32(0x1000000)
+
+
+# the forward neighbour offsets (b > a) within sqrt(2) spacings, in build_soft_box's dz, dy, dx loop order
+_OFFSETS = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+            if (dz, dy, dx) > (0, 0, 0) and dx * dx + dy * dy + dz * dz <= 2]
+
+
+def lattice_body(nx, ny, nz, spacing=0.25, node_mass=2.0, origin=(0.0, 1.0, 0.0), k=2e6, d=500.0, eps=0.05,
+                 seed=1, jitter=0.01, k_range=None, weak_frac=0.0, weak_eps=0.01, deform_factor=1.0,
+                 dt=DT, faces=False, name="lattice"):
+    """A node lattice body; origin is the (x, z) centre and the bottom y."""
+    rng = TRng(seed)
+    n = nx * ny * nz
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    s = np.float32(spacing)
+    x0 = np.float32(origin[0]) - np.float32((nx - 1) * spacing * 0.5)
+    y0 = np.float32(origin[1])
+    z0 = np.float32(origin[2]) - np.float32((nz - 1) * spacing * 0.5)
+    p = np.stack([x0 + ix.astype(np.float32) * s, y0 + iy.astype(np.float32) * s, z0 + iz.astype(np.float32) * s], 1)
+    p = p.astype(np.float32)
+    if jitter > 0:
+        u = rng.uniforms(6 * n).reshape(n, 6)  # 3 position draws, then 3 (zero) velocity draws per node
+        jt = np.float32(jitter)
+        p = (p + (-jt + (jt - (-jt)) * u[:, :3])).astype(np.float32)
+    body = SoftBody(name)
+    nodes = np.zeros(n, abi.NODE)
+    nodes["p"] = p
+    nodes["mass"] = np.float32(node_mass)
+    nodes["inv_mass"] = np.float32(1.0) / np.float32(node_mass)
+    info = np.zeros(n, abi.NODE_INFO)
+    info["flags"] = abi.NF_GROUND | abi.NF_CONTACTER
+    info["friction"] = 1.0
+    info["frame"] = -1
+    info["ror_id"] = -1
+    body.nodes, body.info = nodes, info
+
+    ids = (iz * ny + iy) * nx + ix
+    cols_b, cols_m = [], []
+    for dx, dy, dz in _OFFSETS:
+        x2, y2, z2 = ix + dx, iy + dy, iz + dz
+        ok = (x2 >= 0) & (y2 >= 0) & (z2 >= 0) & (x2 < nx) & (y2 < ny) & (z2 < nz)
+        cols_b.append(np.where(ok, (z2 * ny + y2) * nx + x2, -1))
+        cols_m.append(ok)
+    bmat = np.stack(cols_b, 1)
+    mmat = np.stack(cols_m, 1)
+    a_idx = np.repeat(ids, len(_OFFSETS)).reshape(n, len(_OFFSETS))
+    sel = mmat.ravel()
+    beams = np.zeros(int(sel.sum()), abi.BEAM)
+    beams["a"] = a_idx.ravel()[sel]
+    beams["b"] = bmat.ravel()[sel]
+    nb = len(beams)
+    if k_range is not None:
+        u = rng.uniforms(nb).astype(np.float64)
+        lo, hi = math.log(k_range[0]), math.log(k_range[1])
+        beams["k"] = np.exp(lo + u * (hi - lo)).astype(np.float32)
+    else:
+        beams["k"] = np.float32(k)
+    beams["d"] = np.float32(d)
+    beams["support_limit"] = np.float32(4.0)
+    body.beams = beams
+    if faces:
+        body.tris = box_faces(nx, ny, nz)
+    body.finalize()
+    body.stabilize(float(dt))
+    eps_arr = np.full(nb, np.float32(eps), np.float32)
+    if weak_frac > 0:
+        u = rng.uniforms(nb)
+        eps_arr[u < np.float32(weak_frac)] = np.float32(weak_eps)
+    strength = (eps_arr * body.beams["k"]) * body.beams["L0"]
+    body.beams["strength"] = strength.astype(np.float32)
+    deform = (strength * np.float32(deform_factor)).astype(np.float32)
+    body.beams["max_pos"] = deform
+    body.beams["max_neg"] = -deform
+    return body
+
+
+def box_faces(nx, ny, nz):
+    """The lattice's 6 faces as triangles, wound outwards (build_soft_box's faces, src/world/objects.cpp:923-958)."""
+    dims = (nx, ny, nz)
+
+    def nid(c):
+        return (c[2] * ny + c[1]) * nx + c[0]
+
+    tris = []
+    for axis in range(3):
+        for side in range(2):
+            u_ax, v_ax = (axis + 1) % 3, (axis + 2) % 3
+            fixed = dims[axis] - 1 if side else 0
+            for v in range(dims[v_ax] - 1):
+                for u in range(dims[u_ax] - 1):
+                    q = []
+                    for du, dv in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        c = [0, 0, 0]
+                        c[axis], c[u_ax], c[v_ax] = fixed, u + du, v + dv
+                        q.append(nid(c))
+                    if side:
+                        tris += [(q[0], q[1], q[2]), (q[0], q[2], q[3])]
+                    else:
+                        tris += [(q[0], q[2], q[1]), (q[0], q[3], q[2])]
+    t = np.zeros(len(tris), abi.TRIANGLE)
+    arr = np.array(tris, np.uint32)
+    t["a"], t["b"], t["c"] = arr[:, 0], arr[:, 1], arr[:, 2]
+    t["two_sided"] = 1
+    return t
+
+
+def ground_box(world_or_statics, half=100.0):
+    """A concrete ground slab with its top at y = 0."""
+    st = world_or_statics.statics if hasattr(world_or_statics, "statics") else world_or_statics
+    st.add_box((0.0, -0.5, 0.0), (half, 0.5, half), (0.0, 0.0, 0.0, 1.0), abi.SURF_CONCRETE)
+
+
+# ---------------------------------------------------------------------------------------------- the configurations
+def config1_body(seed=1):
+    """20x8x4 lattice, 0.25 m, 2 kg/node, k = 2e6, d = 500, break strain 0.05, dropped from 1 m, +-1 cm jitter."""
+    return lattice_body(20, 8, 4, 0.25, 2.0, (0.0, 1.0, 0.0), k=2e6, d=500.0, eps=0.05, seed=seed, jitter=0.01,
+                        name="c1_lattice")
+
+
+def config2_body(seed=2, faces=False):
+    """64x32x24 lattice, 0.25 m, 2 kg/node, k log-uniform in [1e5, 5e6], 2 % of beams at break strain 0.01."""
+    return lattice_body(64, 32, 24, 0.25, 2.0, (0.0, 0.5, 0.0), k=None, d=500.0, eps=0.05, seed=seed, jitter=0.01,
+                        k_range=(1e5, 5e6), weak_frac=0.02, weak_eps=0.01, faces=faces, name="c2_lattice")
+
+
+def make_world(bodies, ground=True, settings=None):
+    w = World(settings)
+    if ground:
+        ground_box(w)
+    for b in bodies:
+        w.add_body(b)
+    return w

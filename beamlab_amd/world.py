@@ -1,0 +1,204 @@
+"""Host-side mirror of the reference's physics interface over the C ABI.
+
+Names and meaning follow src/phys: SoftBody's add_node / add_beam / add_triangle / finalize / stabilize
+(src/phys/softbody.cpp:8-110), StaticWorld::add_box (src/phys/static_world.cpp:298-306) and World's add_body /
+step_substeps (src/phys/world.h:160-169). Bodies hold their arrays as numpy records laid out like the reference's
+Node / NodeInfo / Beam / Triangle / Capsule, so they go through the C ABI unchanged.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class SoftBody:
+    """A node/beam body under construction (the reference's SoftBody builder methods)."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self.nodes = np.zeros(0, abi.NODE)
+        self.info = np.zeros(0, abi.NODE_INFO)
+        self.beams = np.zeros(0, abi.BEAM)
+        self.tris = np.zeros(0, abi.TRIANGLE)
+        self.capsules = np.zeros(0, abi.CAPSULE)
+        self.opts = abi.default_body_opts()
+        self._pending_nodes, self._pending_beams, self._pending_tris = [], [], []
+
+    # SoftBody::add_node (src/phys/softbody.cpp:8-20)
+    def add_node(self, p, mass, flags=abi.NF_GROUND | abi.NF_CONTACTER):
+        m = np.float32(mass)
+        inv = np.float32(0.0) if (flags & abi.NF_FIXED) or m <= 0 else np.float32(1.0) / m
+        self._pending_nodes.append((tuple(np.float32(x) for x in p), inv, m, flags))
+        return len(self.nodes) + len(self._pending_nodes) - 1
+
+    # SoftBody::add_beam (src/phys/softbody.cpp:22-40); L and L0 are set by finalize
+    def add_beam(self, a, b, k, d, strength, deform, type_=abi.BT_NORMAL, flags=0):
+        self._pending_beams.append((a, b, k, d, strength, deform, type_, flags))
+        return len(self.beams) + len(self._pending_beams) - 1
+
+    def add_triangle(self, a, b, c, two_sided=True):
+        self._pending_tris.append((a, b, c, two_sided))
+        return len(self.tris) + len(self._pending_tris) - 1
+
+    def _flush(self):
+        if self._pending_nodes:
+            n = np.zeros(len(self._pending_nodes), abi.NODE)
+            inf = np.zeros(len(n), abi.NODE_INFO)
+            for i, (p, inv, m, fl) in enumerate(self._pending_nodes):
+                n[i]["p"] = p
+                n[i]["inv_mass"] = inv
+                n[i]["mass"] = m
+                inf[i]["flags"] = fl
+            inf["friction"] = 1.0
+            inf["frame"] = -1
+            inf["ror_id"] = -1
+            self.nodes = np.concatenate([self.nodes, n])
+            self.info = np.concatenate([self.info, inf])
+            self._pending_nodes = []
+        if self._pending_beams:
+            b = np.zeros(len(self._pending_beams), abi.BEAM)
+            for j, (a, bb, k, d, s, df, t, fl) in enumerate(self._pending_beams):
+                b[j]["a"], b[j]["b"], b[j]["k"], b[j]["d"] = a, bb, k, d
+                b[j]["strength"], b[j]["max_pos"], b[j]["max_neg"] = s, df, -np.float32(df)
+                b[j]["type"], b[j]["flags"], b[j]["support_limit"] = t, fl, 4.0
+            self.beams = np.concatenate([self.beams, b])
+            self._pending_beams = []
+        if self._pending_tris:
+            t = np.zeros(len(self._pending_tris), abi.TRIANGLE)
+            for j, (a, b, c, ts) in enumerate(self._pending_tris):
+                t[j]["a"], t[j]["b"], t[j]["c"], t[j]["two_sided"] = a, b, c, int(ts)
+            self.tris = np.concatenate([self.tris, t])
+            self._pending_tris = []
+
+    # SoftBody::finalize (src/phys/softbody.cpp:76-83)
+    def finalize(self):
+        self._flush()
+        lib = abi.load()
+        abi.check(lib.bl_finalize(abi.ptr(self.nodes), len(self.nodes), abi.ptr(self.beams), len(self.beams),
+                                  abi.ptr(self.tris), len(self.tris)), "bl_finalize")
+
+    # SoftBody::stabilize (src/phys/softbody.cpp:85-110); returns the number of beams scaled
+    def stabilize(self, dt, k_budget=0.6, d_budget=0.5):
+        self._flush()
+        lib = abi.load()
+        changed = ctypes.c_int32(0)
+        abi.check(lib.bl_stabilize(abi.ptr(self.nodes), len(self.nodes), abi.ptr(self.beams), len(self.beams),
+                                   dt, k_budget, d_budget, ctypes.byref(changed)), "bl_stabilize")
+        return changed.value
+
+    def desc(self):
+        """The bl_body_desc of this body (it points into the body's arrays: keep the body alive)."""
+        self._flush()
+        d = abi.BodyDesc()
+        d.nodes, d.info, d.n_nodes = abi.ptr(self.nodes), abi.ptr(self.info), len(self.nodes)
+        d.beams, d.n_beams = abi.ptr(self.beams), len(self.beams)
+        d.tris, d.n_tris = abi.ptr(self.tris), len(self.tris)
+        d.capsules, d.n_capsules = abi.ptr(self.capsules), len(self.capsules)
+        d.opts = self.opts
+        return d
+
+
+def make_box(center, half, quat=(0.0, 0.0, 0.0, 1.0), surface=abi.SURF_CONCRETE):
+    """StaticWorld::add_box's record (src/phys/static_world.cpp:298-306)."""
+    out = np.zeros(1, abi.BOX)
+    c = np.asarray(center, np.float32)
+    h = np.asarray(half, np.float32)
+    q = np.asarray(quat, np.float32)
+    abi.check(abi.load().bl_box_from_quat(abi.ptr(c), abi.ptr(h), abi.ptr(q), surface, abi.ptr(out)),
+              "bl_box_from_quat")
+    return out[0]
+
+
+class StaticWorld:
+    def __init__(self):
+        self.boxes = np.zeros(0, abi.BOX)
+
+    def add_box(self, center, half, quat=(0.0, 0.0, 0.0, 1.0), surface=abi.SURF_CONCRETE):
+        self.boxes = np.concatenate([self.boxes, np.array([make_box(center, half, quat, surface)], abi.BOX)])
+
+
+class World:
+    """The GPU world: World::add_body / step_substeps over bl_create / bl_add_body / bl_step."""
+
+    def __init__(self, settings=None):
+        self.settings = settings or abi.default_settings()
+        self.statics = StaticWorld()
+        self.bodies = []
+        self._ctx = None
+        self._statics_sent = None
+
+    def _lib(self):
+        return abi.load()
+
+    def _context(self):
+        if self._ctx is None:
+            h = ctypes.c_void_p()
+            abi.check(self._lib().bl_create(ctypes.byref(h), ctypes.byref(self.settings)), "bl_create")
+            self._ctx = h
+        return self._ctx
+
+    def add_body(self, body):
+        ctx = self._context()
+        bid = ctypes.c_uint32(0)
+        d = body.desc()
+        abi.check(self._lib().bl_add_body(ctx, ctypes.byref(d), ctypes.byref(bid)), "bl_add_body")
+        self.bodies.append(body)
+        return bid.value
+
+    def _sync_statics(self):
+        boxes = self.statics.boxes
+        key = boxes.tobytes()
+        if key != self._statics_sent:
+            abi.check(self._lib().bl_set_statics(self._context(), abi.ptr(boxes), len(boxes)), "bl_set_statics")
+            self._statics_sent = key
+
+    def set_tuning(self, mode=abi.MODE_AUTO, block=256, substeps_per_launch=0, use_graph=1):
+        t = abi.Tuning(mode, block, substeps_per_launch, use_graph)
+        abi.check(self._lib().bl_set_tuning(self._context(), ctypes.byref(t)), "bl_set_tuning")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        abi.check(self._lib().bl_get_tuning(self._context(), ctypes.byref(t)), "bl_get_tuning")
+        return t
+
+    def step_substeps(self, n):
+        self._sync_statics()
+        abi.check(self._lib().bl_step(self._context(), int(n)), "bl_step")
+
+    def time_substeps(self, n):
+        self._sync_statics()
+        ms = ctypes.c_float(0)
+        abi.check(self._lib().bl_time_step(self._context(), int(n), ctypes.byref(ms)), "bl_time_step")
+        return ms.value
+
+    def read_body(self, bid):
+        b = self.bodies[bid]
+        nodes = np.zeros(len(b.nodes), abi.NODE)
+        beams = np.zeros(len(b.beams), abi.BEAM)
+        abi.check(self._lib().bl_read_body(self._context(), bid, abi.ptr(nodes), abi.ptr(beams)), "bl_read_body")
+        return nodes, beams
+
+    def write_body(self, bid, nodes=None, beams=None):
+        abi.check(self._lib().bl_write_body(self._context(), bid, abi.ptr(nodes), abi.ptr(beams)), "bl_write_body")
+
+    def broken_beams(self, bid):
+        v = ctypes.c_int64(0)
+        abi.check(self._lib().bl_body_broken_beams(self._context(), bid, ctypes.byref(v)), "bl_body_broken_beams")
+        return v.value
+
+    def stats(self):
+        s = abi.Stats()
+        abi.check(self._lib().bl_get_stats(self._context(), ctypes.byref(s)), "bl_get_stats")
+        return s.as_dict()
+
+    def close(self):
+        if self._ctx is not None:
+            self._lib().bl_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,287 @@
+/*
+ * beamlab_gpu.h - C ABI of the MI355X (gfx950) soft-body step.
+ *
+ * A drop-in for the CPU step of BeamLab's node/beam/shell physics (reference: zenkovich/beamlab, src/phys).
+ * The host keeps its own SoftBody/World objects and hands their arrays over through plain pointers and sizes;
+ * the records below are layout-identical to the reference's types so that a caller can pass
+ * `body.nodes.data()`, `body.beams.data()` ... unchanged (the layout is asserted against the reference headers
+ * in oracle/ref_capi.cpp).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: BL_OK (0) or a negative BL_ERR_* code. No exceptions cross the ABI.
+ *     Numerical events (broken beams, clamps, failed pivots) are counters in bl_stats, not status codes, as the
+ *     reference reports them (src/phys/softbody.h:588-593, src/phys/frame_fem.h:342-344).
+ *   - Ownership: the caller owns every host array; bl_* copies in and out. Device memory belongs to the context.
+ *   - Threading: one host thread per context (the reference has a single caller of World::step_frame); a context
+ *     owns one HIP stream.
+ */
+#ifndef BEAMLAB_GPU_H
+#define BEAMLAB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BL_ABI_VERSION 1
+
+enum {
+    BL_OK = 0,
+    BL_ERR_ARG = -1,         /* a null pointer, an index out of range, a size that does not fit */
+    BL_ERR_HIP = -2,         /* a HIP runtime call failed (no device, out of device memory, ...) */
+    BL_ERR_UNSUPPORTED = -3, /* a feature of the body that this build does not step on the GPU */
+    BL_ERR_STATE = -4,       /* a call out of order (e.g. stepping before any body was added) */
+};
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Records, layout-identical to the reference's
+ * ------------------------------------------------------------------------------------------------------------- */
+
+/* == bl::phys::Node (src/phys/softbody.h:43-48): 32 B, p, inv_mass, v, mass */
+typedef struct bl_node {
+    float p[3];
+    float inv_mass;
+    float v[3];
+    float mass;
+} bl_node;
+
+/* == bl::phys::NodeInfo (src/phys/softbody.h:50-58): 20 B */
+typedef struct bl_node_info {
+    uint16_t flags; /* BL_NF_* == NodeFlag (src/phys/softbody.h:29-40) */
+    uint8_t surface;
+    uint8_t pad;
+    float friction;
+    float radius;
+    int32_t frame;
+    int32_t ror_id;
+} bl_node_info;
+
+enum {
+    BL_NF_GROUND = 1 << 0,
+    BL_NF_FIXED = 1 << 1,
+    BL_NF_TYRE = 1 << 2,
+    BL_NF_RIM = 1 << 3,
+    BL_NF_CONTACTER = 1 << 4,
+    BL_NF_FRAME = 1 << 5,
+    BL_NF_LOAD = 1 << 6,
+    BL_NF_NO_MOUSE = 1 << 7,
+    BL_NF_SELF_COLL = 1 << 8,
+};
+
+/* == bl::phys::Beam (src/phys/softbody.h:71-85): 52 B */
+typedef struct bl_beam {
+    uint32_t a, b;
+    float L;             /* current rest length (plastic) */
+    float k, d;          /* spring N/m, damping N s/m */
+    float max_pos;       /* compressive yield force (> 0) */
+    float max_neg;       /* tensile yield force (< 0) */
+    float strength;      /* break force */
+    float plastic;
+    float stress;        /* last force (positive = compression) */
+    float L0;
+    float support_limit;
+    uint8_t type;        /* BL_BT_* == BeamType (src/phys/softbody.h:60) */
+    uint8_t flags;       /* BL_BF_* == BeamFlag (src/phys/softbody.h:61-69) */
+    uint16_t group;
+} bl_beam;
+
+enum { BL_BT_NORMAL = 0, BL_BT_ROPE = 1, BL_BT_SUPPORT = 2 };
+enum {
+    BL_BF_BROKEN = 1 << 0,
+    BL_BF_INVISIBLE = 1 << 1,
+    BL_BF_NO_DEFORM = 1 << 2,
+    BL_BF_NO_BREAK = 1 << 3,
+    BL_BF_WHEEL = 1 << 4,
+    BL_BF_HYDRO = 1 << 5,
+    BL_BF_SHOCK = 1 << 6,
+};
+
+/* == bl::phys::Triangle (src/phys/softbody.h:126-137): 20 B */
+typedef struct bl_triangle {
+    uint32_t a, b, c;
+    uint8_t surface;
+    uint8_t two_sided;
+    uint8_t torn;
+    uint8_t pad;
+    float rest_edge2;
+} bl_triangle;
+
+/* == bl::phys::Capsule (src/phys/softbody.h:192-196): 16 B */
+typedef struct bl_capsule {
+    uint32_t a, b;
+    float radius;
+    int32_t joint;
+} bl_capsule;
+
+/* A static oriented box (== StaticBox, src/phys/static_world.h:115-126, without hull planes). `rot` holds the
+ * columns of the rotation (the box's local axes in world space), as bl::mat3 does; aabb_mn/aabb_mx as
+ * StaticBox::update_aabb (src/phys/static_world.cpp:242-246) computes them (bl_box_from_quat fills both). */
+typedef struct bl_box {
+    float center[3];
+    float rot[9];
+    float half[3];
+    float aabb_mn[3];
+    float aabb_mx[3];
+    float max_force;  /* > 0: a yielding obstacle, the per-node contact force is capped */
+    uint32_t surface; /* BL_SURF_* == Surface (src/phys/static_world.h:59-62) */
+} bl_box;
+
+enum {
+    BL_SURF_GRASS = 0, BL_SURF_DIRT, BL_SURF_ASPHALT, BL_SURF_CONCRETE, BL_SURF_GRAVEL, BL_SURF_MUD, BL_SURF_SAND,
+    BL_SURF_ROCK, BL_SURF_WOOD, BL_SURF_METAL, BL_SURF_ICE, BL_SURF_COUNT
+};
+
+/* The world settings the path reads (a subset of WorldSettings, src/phys/world.h:24-77). */
+typedef struct bl_settings {
+    float gravity[3];
+    float dt;                  /* substep, default 0.0005 (2 kHz) */
+    float collision_hz;        /* pair-search rate, default 120 */
+    float frame_theta;         /* FEM implicit step theta, default 0.5 */
+    float frame_dissipation;   /* FEM numerical dissipation, default 0.02 */
+    int32_t inter_body_collisions;
+    int32_t sleeping;          /* must be 0: sleeping is not stepped on the GPU */
+    float kill_y;              /* StaticWorld::kill_y, default -200 */
+} bl_settings;
+
+/* The per-body settings the path reads (SoftBody, src/phys/softbody.h:649-716). */
+typedef struct bl_body_opts {
+    float collision_radius;    /* default 0.05 */
+    float ground_friction;     /* default 1 */
+    float bounce;              /* default -1 (RoR: 80 % of the approach taken) */
+    float contact_push_max;    /* default 2 */
+    float contact_slop;        /* default 0 */
+    float hull_depth;          /* default 0.3 */
+    float sphere_ball;         /* default 0 */
+    float contact_friction;    /* default 1 */
+    int32_t allow_break;       /* default 1 */
+    int32_t allow_deform;      /* default 1 */
+    int32_t ductile;           /* default 0 */
+    int32_t self_collision;    /* default 0 */
+    int32_t collision_group;   /* default 0 */
+    int32_t faces_only;        /* default 0 */
+    /* features the GPU path does not step; non-zero values are rejected with BL_ERR_UNSUPPORTED */
+    float air_drag;
+    float aero_cda;
+    float rest_damp;
+    int32_t energy_guard;
+} bl_body_opts;
+
+typedef struct bl_body_desc {
+    const bl_node* nodes;
+    const bl_node_info* info;
+    uint32_t n_nodes;
+    const bl_beam* beams;
+    uint32_t n_beams;
+    const bl_triangle* tris;
+    uint32_t n_tris;
+    const bl_capsule* capsules;
+    uint32_t n_capsules;
+    bl_body_opts opts;
+} bl_body_desc;
+
+/* Counters (WorldStats / BodyStats / FemFrame counters: src/phys/world.h:79-132, src/phys/softbody.h:588-593). */
+typedef struct bl_stats {
+    int64_t substeps;
+    int64_t broken_beams;
+    int64_t static_contacts;   /* node-vs-static contacts, summed over substeps */
+    int64_t vel_clamps;        /* nodes clamped to 400 m/s */
+    int64_t nan_resets;        /* non-finite velocities reset to 0 */
+    int64_t pair_rebuilds;
+    int64_t contact_pairs;     /* candidate pairs at the last rebuild */
+    int64_t narrow_tests;      /* candidate pairs tested, summed over substeps */
+    int64_t contacts;          /* inter-body contacts, summed over substeps */
+    int64_t solve_failures;
+    float aabb_mn[3];          /* bounds and top speed of the last substep, all bodies */
+    float aabb_mx[3];
+    float max_speed;
+    float pad;
+} bl_stats;
+
+/* Execution choices (written by the autotuner, or by hand). Zero fields take the defaults. */
+typedef struct bl_tuning {
+    int32_t mode;              /* BL_MODE_* */
+    int32_t block;             /* threads per workgroup */
+    int32_t substeps_per_launch;
+    int32_t use_graph;         /* capture a launch sequence in a hipGraph */
+} bl_tuning;
+
+enum {
+    BL_MODE_AUTO = 0,
+    BL_MODE_STEPPED = 1,       /* one node-phase launch per substep */
+    BL_MODE_PERSISTENT = 2,    /* one workgroup per body, many substeps per launch */
+};
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Host-side setup helpers (no device needed)
+ * ------------------------------------------------------------------------------------------------------------- */
+
+/* SoftBody::finalize's beam part (src/phys/softbody.cpp:76-83): L = L0 = max(1e-3, |pa - pb|); and each
+ * triangle's rest_edge2. */
+int bl_finalize(const bl_node* nodes, uint32_t n_nodes, bl_beam* beams, uint32_t n_beams, bl_triangle* tris,
+                uint32_t n_tris);
+
+/* SoftBody::stabilize's beam part (src/phys/softbody.cpp:85-110): per node sum(k) dt^2 / m <= k_budget and
+ * sum(d) dt / m <= d_budget, each beam scaled by the smaller factor of its two nodes. *changed = beams scaled. */
+int bl_stabilize(const bl_node* nodes, uint32_t n_nodes, bl_beam* beams, uint32_t n_beams, float dt, float k_budget,
+                 float d_budget, int32_t* changed);
+
+/* A box from its centre, half extents and rotation quaternion (x, y, z, w), as StaticWorld::add_box does
+ * (src/phys/static_world.cpp:298-306, to_mat3 src/core/math.h:163-170). */
+int bl_box_from_quat(const float center[3], const float half[3], const float quat_xyzw[4], uint32_t surface,
+                     bl_box* out);
+
+void bl_default_settings(bl_settings* s);
+void bl_default_body_opts(bl_body_opts* o);
+const char* bl_error_string(int status);
+int bl_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * The step
+ * ------------------------------------------------------------------------------------------------------------- */
+
+typedef struct bl_ctx bl_ctx;
+
+/* A context on the current HIP device: owns the device copies of the world and one stream. */
+int bl_create(bl_ctx** out, const bl_settings* settings);
+void bl_destroy(bl_ctx* ctx);
+
+/* The static world (replaces the previous boxes). World::statics (src/phys/world.h:158). */
+int bl_set_statics(bl_ctx* ctx, const bl_box* boxes, int32_t n_boxes);
+
+/* Adds a body (World::add_body, src/phys/world.cpp:283-289). Bodies are stepped together. *id = its index. */
+int bl_add_body(bl_ctx* ctx, const bl_body_desc* desc, uint32_t* id);
+
+/* Runs `substeps` substeps of every body (World::step_substeps, src/phys/world.cpp:368) and waits for them. */
+int bl_step(bl_ctx* ctx, int32_t substeps);
+
+/* Enqueues `substeps` substeps on the context's stream without waiting (bl_sync waits). */
+int bl_step_async(bl_ctx* ctx, int32_t substeps);
+int bl_sync(bl_ctx* ctx);
+
+/* Copies a body's state out (either pointer may be null). Beams come back with L, max_pos, max_neg, strength,
+ * stress and flags as the step left them; the other fields as they were added. */
+int bl_read_body(bl_ctx* ctx, uint32_t id, bl_node* nodes_out, bl_beam* beams_out);
+
+/* Overwrites a body's node and beam state (a snapshot restore; sizes as added; either pointer may be null). */
+int bl_write_body(bl_ctx* ctx, uint32_t id, const bl_node* nodes, const bl_beam* beams);
+
+/* A body's broken beams since it was added (BodyStats::broken_beams, src/phys/softbody.h:589). */
+int bl_body_broken_beams(bl_ctx* ctx, uint32_t id, int64_t* out);
+
+/* Counters since bl_create (or the last bl_reset_stats). */
+int bl_get_stats(bl_ctx* ctx, bl_stats* out);
+int bl_reset_stats(bl_ctx* ctx);
+
+/* Execution choices; bl_get_tuning returns the ones in force. */
+int bl_set_tuning(bl_ctx* ctx, const bl_tuning* t);
+int bl_get_tuning(bl_ctx* ctx, bl_tuning* t);
+
+/* Times `substeps` substeps (already resident inputs) with HIP events on the context's stream. *ms = elapsed. */
+int bl_time_step(bl_ctx* ctx, int32_t substeps, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAMLAB_GPU_H */

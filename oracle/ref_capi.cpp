@@ -1,0 +1,226 @@
+// Test infrastructure only: a C shim over the compiled reference (zenkovich/beamlab, src/phys + src/core), built
+// from the reference's own sources where they lie by oracle/Makefile into oracle/_ref/libblref.so. Only the tests,
+// the smoke check and bench.py's CPU-baseline leg load it, as a checker and a baseline; the product never does.
+//
+// It takes the same plain records as include/beamlab_gpu.h, builds the reference's SoftBody / StaticWorld from
+// them (copying the records as they are, so both sides start from identical bits) and runs the reference's own
+// World::step_substeps (src/phys/world.cpp:368).
+#include "phys/world.h"
+#include "phys/softbody.h"
+#include "phys/static_world.h"
+#include "core/jobs.h"
+
+#include "beamlab_gpu.h"
+
+#include <chrono>
+#include <cstddef>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using namespace bl;
+using namespace bl::phys;
+
+// The C records are layout-identical to the reference types: the drop-in passes the reference's arrays as they are.
+static_assert(sizeof(bl_node) == sizeof(Node), "bl_node vs Node");
+static_assert(offsetof(bl_node, inv_mass) == offsetof(Node, inv_mass), "Node.inv_mass");
+static_assert(offsetof(bl_node, v) == offsetof(Node, v), "Node.v");
+static_assert(offsetof(bl_node, mass) == offsetof(Node, mass), "Node.mass");
+static_assert(sizeof(bl_node_info) == sizeof(NodeInfo), "bl_node_info vs NodeInfo");
+static_assert(offsetof(bl_node_info, friction) == offsetof(NodeInfo, friction), "NodeInfo.friction");
+static_assert(offsetof(bl_node_info, radius) == offsetof(NodeInfo, radius), "NodeInfo.radius");
+static_assert(offsetof(bl_node_info, frame) == offsetof(NodeInfo, frame), "NodeInfo.frame");
+static_assert(sizeof(bl_beam) == sizeof(Beam), "bl_beam vs Beam");
+static_assert(offsetof(bl_beam, L) == offsetof(Beam, L), "Beam.L");
+static_assert(offsetof(bl_beam, strength) == offsetof(Beam, strength), "Beam.strength");
+static_assert(offsetof(bl_beam, support_limit) == offsetof(Beam, support_limit), "Beam.support_limit");
+static_assert(offsetof(bl_beam, type) == offsetof(Beam, type), "Beam.type");
+static_assert(offsetof(bl_beam, flags) == offsetof(Beam, flags), "Beam.flags");
+static_assert(offsetof(bl_beam, group) == offsetof(Beam, group), "Beam.group");
+static_assert(sizeof(bl_triangle) == sizeof(Triangle), "bl_triangle vs Triangle");
+static_assert(offsetof(bl_triangle, two_sided) == offsetof(Triangle, two_sided), "Triangle.two_sided");
+static_assert(offsetof(bl_triangle, rest_edge2) == offsetof(Triangle, rest_edge2), "Triangle.rest_edge2");
+static_assert(sizeof(bl_capsule) == sizeof(Capsule), "bl_capsule vs Capsule");
+
+namespace {
+
+struct RefWorld {
+    World w;
+    std::vector<SoftBody*> bodies;
+};
+
+int g_threads_inited = 0;
+
+vec3 v3(const float* f) { return vec3(f[0], f[1], f[2]); }
+
+void apply_opts(SoftBody& b, const bl_body_opts& o) {
+    b.collision_radius = o.collision_radius;
+    b.ground_friction = o.ground_friction;
+    b.bounce = o.bounce;
+    b.contact_push_max = o.contact_push_max;
+    b.contact_slop = o.contact_slop;
+    b.hull_depth = o.hull_depth;
+    b.sphere_ball = o.sphere_ball;
+    b.contact_friction = o.contact_friction;
+    b.allow_break = o.allow_break != 0;
+    b.allow_deform = o.allow_deform != 0;
+    b.ductile = o.ductile != 0;
+    b.self_collision = o.self_collision != 0;
+    b.collision_group = o.collision_group;
+    b.faces_only = o.faces_only != 0;
+    b.air_drag = o.air_drag;
+    b.aero_cda = o.aero_cda;
+    b.rest_damp = o.rest_damp;
+    b.energy_guard = o.energy_guard != 0;
+}
+
+} // namespace
+
+extern "C" {
+
+// threads: JobSystem threads (workers + caller); the first call fixes the pool for the process.
+void* blref_create(const bl_settings* s, int threads) {
+    if (!g_threads_inited) {
+        JobSystem::get().init(threads > 1 ? threads - 1 : 0);
+        g_threads_inited = threads > 1 ? threads : 1;
+    }
+    auto* rw = new RefWorld();
+    WorldSettings& ws = rw->w.settings;
+    ws.gravity = v3(s->gravity);
+    ws.dt = s->dt;
+    ws.collision_hz = s->collision_hz;
+    ws.frame_theta = s->frame_theta;
+    ws.frame_dissipation = s->frame_dissipation;
+    ws.inter_body_collisions = s->inter_body_collisions != 0;
+    ws.sleeping = s->sleeping != 0;
+    ws.multithreaded = threads > 1;
+    rw->w.statics.kill_y = s->kill_y;
+    return rw;
+}
+
+void blref_destroy(void* h) { delete static_cast<RefWorld*>(h); }
+
+int blref_threads(void) { return JobSystem::get().num_threads(); }
+
+int blref_set_statics(void* h, const bl_box* boxes, int n) {
+    auto* rw = static_cast<RefWorld*>(h);
+    rw->w.statics.boxes.clear();
+    for (int i = 0; i < n; i++) {
+        StaticBox b;
+        b.center = v3(boxes[i].center);
+        b.rot = mat3(v3(boxes[i].rot), v3(boxes[i].rot + 3), v3(boxes[i].rot + 6));
+        b.half = v3(boxes[i].half);
+        b.surface = (uint8_t)boxes[i].surface;
+        b.max_force = boxes[i].max_force;
+        b.update_aabb();
+        rw->w.statics.boxes.push_back(b);
+    }
+    rw->w.statics.build_grid();
+    return 0;
+}
+
+// The reference's own box (StaticWorld::add_box): to check bl_box_from_quat bit for bit.
+int blref_box_from_quat(const float* c, const float* half, const float* q, uint32_t surface, bl_box* out) {
+    StaticWorld sw;
+    sw.add_box(v3(c), v3(half), quat(q[0], q[1], q[2], q[3]), (uint8_t)surface);
+    const StaticBox& b = sw.boxes[0];
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < 3; k++) {
+        out->center[k] = b.center[k];
+        out->half[k] = b.half[k];
+        out->aabb_mn[k] = b.aabb.mn[k];
+        out->aabb_mx[k] = b.aabb.mx[k];
+        for (int r = 0; r < 3; r++) out->rot[3 * k + r] = b.rot.c[k][r];
+    }
+    out->surface = b.surface;
+    out->max_force = b.max_force;
+    return 0;
+}
+
+int blref_add_body(void* h, const bl_body_desc* d) {
+    auto* rw = static_cast<RefWorld*>(h);
+    auto b = std::make_unique<SoftBody>();
+    b->nodes.resize(d->n_nodes);
+    if (d->n_nodes) std::memcpy(b->nodes.data(), d->nodes, sizeof(Node) * d->n_nodes);
+    b->info.resize(d->n_nodes);
+    for (uint32_t i = 0; i < d->n_nodes; i++) {
+        NodeInfo& in = b->info[i];
+        in.flags = d->info[i].flags;
+        in.surface = d->info[i].surface;
+        in.friction = d->info[i].friction;
+        in.radius = d->info[i].radius;
+        in.frame = d->info[i].frame;
+        in.ror_id = d->info[i].ror_id;
+    }
+    b->force.assign(d->n_nodes, vec3(0));
+    b->beams.resize(d->n_beams);
+    if (d->n_beams) std::memcpy(b->beams.data(), d->beams, sizeof(Beam) * d->n_beams);
+    for (uint32_t i = 0; i < d->n_tris; i++) {
+        Triangle t;
+        t.a = d->tris[i].a, t.b = d->tris[i].b, t.c = d->tris[i].c;
+        t.surface = d->tris[i].surface;
+        t.two_sided = d->tris[i].two_sided != 0;
+        t.torn = d->tris[i].torn != 0;
+        t.rest_edge2 = d->tris[i].rest_edge2;
+        b->tris.push_back(t);
+    }
+    for (uint32_t i = 0; i < d->n_capsules; i++) {
+        Capsule c;
+        c.a = d->capsules[i].a, c.b = d->capsules[i].b, c.radius = d->capsules[i].radius, c.joint = d->capsules[i].joint;
+        b->capsules.push_back(c);
+    }
+    apply_opts(*b, d->opts);
+    SoftBody* p = rw->w.add_body(std::move(b));
+    rw->bodies.push_back(p);
+    return (int)rw->bodies.size() - 1;
+}
+
+int blref_step(void* h, int n) {
+    static_cast<RefWorld*>(h)->w.step_substeps(n);
+    return 0;
+}
+
+// Steps `calls` times `n` substeps; returns the wall time in seconds.
+double blref_time_step(void* h, int calls, int n) {
+    auto* rw = static_cast<RefWorld*>(h);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int c = 0; c < calls; c++) rw->w.step_substeps(n);
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int blref_read_body(void* h, uint32_t id, bl_node* nodes, bl_beam* beams, int64_t* broken) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    const SoftBody& b = *rw->bodies[id];
+    if (nodes && !b.nodes.empty()) std::memcpy(nodes, b.nodes.data(), sizeof(Node) * b.nodes.size());
+    if (beams && !b.beams.empty()) std::memcpy(beams, b.beams.data(), sizeof(Beam) * b.beams.size());
+    if (broken) *broken = b.stats.broken_beams;
+    return 0;
+}
+
+int blref_write_body(void* h, uint32_t id, const bl_node* nodes, const bl_beam* beams) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    SoftBody& b = *rw->bodies[id];
+    if (nodes && !b.nodes.empty()) std::memcpy(b.nodes.data(), nodes, sizeof(Node) * b.nodes.size());
+    if (beams && !b.beams.empty()) std::memcpy(b.beams.data(), beams, sizeof(Beam) * b.beams.size());
+    return 0;
+}
+
+// The reference's SoftBody::finalize + stabilize on a copy of the arrays (src/phys/softbody.cpp:76-110).
+int blref_finalize_stabilize(bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb, int do_finalize, float dt,
+                             float kb, float db, int* changed) {
+    SoftBody b;
+    b.nodes.resize(n);
+    std::memcpy(b.nodes.data(), nodes, sizeof(Node) * n);
+    b.info.resize(n);
+    b.beams.resize(nb);
+    std::memcpy(b.beams.data(), beams, sizeof(Beam) * nb);
+    if (do_finalize) b.finalize();
+    int c = dt > 0 ? b.stabilize(dt, kb, db) : 0;
+    if (changed) *changed = c;
+    std::memcpy(beams, b.beams.data(), sizeof(Beam) * nb);
+    return 0;
+}
+
+} // extern "C"
