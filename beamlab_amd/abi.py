@@ -36,7 +36,7 @@ SURF_GRASS, SURF_DIRT, SURF_ASPHALT, SURF_CONCRETE, SURF_GRAVEL, SURF_MUD, SURF_
 SURF_ROCK, SURF_WOOD, SURF_METAL, SURF_ICE = range(7, 11)
 
 BL_OK, BL_ERR_ARG, BL_ERR_HIP, BL_ERR_UNSUPPORTED, BL_ERR_STATE = 0, -1, -2, -3, -4
-MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT = 0, 1, 2
+MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT, MODE_RESIDENT = 0, 1, 2, 3
 
 
 class Settings(ctypes.Structure):
@@ -77,7 +77,7 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("block", ctypes.c_int32), ("substeps_per_launch", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("incidences_per_thread", ctypes.c_int32)]
 
 
 # every entry point the header declares, with its argument types
@@ -106,6 +106,8 @@ SIGNATURES = {
     "bl_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "bl_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "bl_time_step": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    "bl_time_kernels": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _lib = None

@@ -69,19 +69,22 @@ struct bl_ctx {
     std::vector<HostBody> bodies;
     std::vector<bl_box> boxes;
     bool built = false;
+    int built_block = 0, built_per = 0; // the chunking of the device arrays
     int parity = 0;
     bl_tuning tune{};
     int64_t substeps = 0;
     // device
     DevArr<float4> pm[2], vm[2], iA, iM, bcold;
-    DevArr<uint32_t> ninfo, noff, ch_n0, body_c0;
+    DevArr<uint32_t> ninfo, noff, ch_n0, body_c0, chunk_body;
+    DevArr<uint4> cstat;
     DevArr<float2> naux;
     DevArr<uint2> iAB;
     DevArr<float> stress;
     DevArr<BodyParam> bparam;
     DevArr<bl_box> dboxes;
     DevArr<unsigned long long> counters, broken;
-    DevArr<unsigned int> bounds, body_frame;
+    DevArr<unsigned int> bounds, body_frame, bar;
+    DevArr<int> err;
     bool frame_pending = false;   // a frame enqueued whose post-frame check has not run
     int64_t repairs = 0;
     std::vector<uint32_t> a_slot;   // global beam -> incidence of its a end
@@ -89,6 +92,7 @@ struct bl_ctx {
     KArgs args{};
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev;  // per-launch events of bl_time_kernels
 
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -99,7 +103,8 @@ struct bl_ctx {
         for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
         iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release();
         body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
-        counters.release(), broken.release(), bounds.release(), body_frame.release();
+        counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
+        chunk_body.release(), cstat.release();
         frame_pending = false;
         built = false;
     }
@@ -108,6 +113,24 @@ struct bl_ctx {
 namespace {
 
 int effective_block(const bl_ctx* c) { return c->tune.block == 512 || c->tune.block == 1024 ? c->tune.block : 256; }
+int effective_per(const bl_ctx* c) {
+    const int p = c->tune.incidences_per_thread;
+    return p == 1 || p == 2 || p == 8 ? p : 4;
+}
+
+// The (block, incidences per thread) pairs each form is instantiated for.
+bool tuning_valid(const bl_tuning& t) {
+    const int b = t.block ? t.block : 256, p = t.incidences_per_thread ? t.incidences_per_thread : 4;
+    switch (t.mode) {
+    case BL_MODE_AUTO: return (b == 256 && (p == 1 || p == 2 || p == 4 || p == 8)) || (b == 512 && (p == 2 || p == 4));
+    case BL_MODE_STEPPED: return (b == 256 && (p == 1 || p == 2 || p == 4 || p == 8)) || (b == 512 && (p == 2 || p == 4));
+    case BL_MODE_PERSISTENT: return (b == 256 || b == 1024) && p == 4;
+    case BL_MODE_RESIDENT: return (b == 256 && (p == 2 || p == 4 || p == 8)) || (b == 512 && p == 4);
+    default: return false;
+    }
+}
+
+constexpr uint32_t kSpinLimit = 1u << 20; // grid-barrier polls (about a second) before a resident launch gives up
 
 bool unsupported_opts(const bl_body_opts& o) {
     return o.air_drag != 0 || o.aero_cda != 0 || o.rest_damp != 0 || o.energy_guard != 0;
@@ -154,7 +177,8 @@ int pull_state(bl_ctx* c) {
 int build(bl_ctx* c) {
     c->release();
     const int T = effective_block(c);
-    const uint32_t CAP = (uint32_t)chunk_capacity(T);
+    const uint32_t CAP = (uint32_t)chunk_capacity(T, effective_per(c));
+    const uint32_t NCAP = (uint32_t)T; // nodes per chunk: one per thread
     uint32_t N = 0, B = 0;
     for (HostBody& hb : c->bodies) {
         hb.node0 = N, hb.beam0 = B;
@@ -214,8 +238,9 @@ int build(bl_ctx* c) {
         }
     }
     // chunks: contiguous node ranges of one body with at most CAP incidences and CAP nodes
-    std::vector<uint32_t> ch_n0, body_c0;
+    std::vector<uint32_t> ch_n0, body_c0, chunk_body;
     for (const HostBody& hb : c->bodies) {
+        const uint32_t bidx = (uint32_t)body_c0.size();
         body_c0.push_back((uint32_t)ch_n0.size());
         const uint32_t n0 = hb.node0, n1 = hb.node0 + (uint32_t)hb.nodes.size();
         uint32_t start = n0;
@@ -224,12 +249,12 @@ int build(bl_ctx* c) {
                 g_last_error = "a node has more incidences than a workgroup holds";
                 return BL_ERR_UNSUPPORTED;
             }
-            if (noff[i + 1] - noff[start] > CAP || i + 1 - start > CAP) {
-                ch_n0.push_back(start);
+            if (noff[i + 1] - noff[start] > CAP || i + 1 - start > NCAP) {
+                ch_n0.push_back(start), chunk_body.push_back(bidx);
                 start = i;
             }
         }
-        if (n1 > start) ch_n0.push_back(start);
+        if (n1 > start) ch_n0.push_back(start), chunk_body.push_back(bidx);
     }
     body_c0.push_back((uint32_t)ch_n0.size());
     const uint32_t nchunks = (uint32_t)ch_n0.size();
@@ -243,10 +268,14 @@ int build(bl_ctx* c) {
         (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->body_c0.upload(body_c0)) ||
         (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
-        (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
+        (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)) || (rc = c->bar.alloc(1)) ||
+        (rc = c->err.alloc(1)) || (rc = c->chunk_body.upload(chunk_body)) ||
+        (rc = c->cstat.alloc(std::max<size_t>(1, nchunks))))
         return rc;
     HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
     HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
+    HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
+    HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
     KArgs& a = c->args;
     a = KArgs{};
     a.pm[0] = c->pm[0].p, a.pm[1] = c->pm[1].p, a.vm[0] = c->vm[0].p, a.vm[1] = c->vm[1].p;
@@ -254,11 +283,13 @@ int build(bl_ctx* c) {
     a.iA = c->iA.p, a.iM = c->iM.p, a.iAB = c->iAB.p, a.bcold = c->bcold.p, a.stress = c->stress.p;
     a.ch_n0 = c->ch_n0.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
+    a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
     a.xcd_remap = 1;
     c->parity = 0;
+    c->built_block = T, c->built_per = effective_per(c);
     c->built = true;
     return BL_OK;
 }
@@ -272,20 +303,41 @@ int ensure_built(bl_ctx* c) {
     return build(c);
 }
 
-// One workgroup per body (many substeps per launch, workgroup barriers only) when every body is a few chunks;
-// otherwise one launch per substep spreads the chunks over the chip.
+// AUTO: the resident form when the chip holds every chunk's workgroup at once, else one launch per substep.
 int resolved_mode(const bl_ctx* c) {
-    if (c->tune.mode == BL_MODE_STEPPED || c->tune.mode == BL_MODE_PERSISTENT) return c->tune.mode;
-    return c->args.nchunks <= 16u * c->args.nbodies ? BL_MODE_PERSISTENT : BL_MODE_STEPPED;
+    if (c->tune.mode != BL_MODE_AUTO) return c->tune.mode;
+    const int T = effective_block(c), per = effective_per(c);
+    bl_tuning t{BL_MODE_RESIDENT, T, 0, 0, per};
+    if (tuning_valid(t) && resident_fits(T, per, c->args.nchunks)) return BL_MODE_RESIDENT;
+    return BL_MODE_STEPPED;
 }
 
 int enqueue(bl_ctx* c, int substeps) {
     if (substeps <= 0) return BL_OK;
     bl_tuning t = c->tune;
     t.block = effective_block(c);
+    t.incidences_per_thread = effective_per(c);
     t.mode = resolved_mode(c);
+    if (t.block != c->built_block || t.incidences_per_thread != c->built_per) {
+        g_last_error = "execution form differs from the chunking the world was built for";
+        return BL_ERR_STATE;
+    }
     int rc;
-    if (t.mode == BL_MODE_PERSISTENT) {
+    if (t.mode == BL_MODE_RESIDENT) {
+        if (!resident_fits(t.block, t.incidences_per_thread, c->args.nchunks)) {
+            g_last_error = "the resident form needs every chunk's workgroup resident at once: too many chunks";
+            return BL_ERR_UNSUPPORTED;
+        }
+        const int per = t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps;
+        for (int s = 0; s < substeps; s += per) {
+            const int k = std::min(per, substeps - s);
+            if ((rc = launch_resident(c->args, c->stream, c->parity, k, t, c->bar.p, c->err.p, kSpinLimit))) {
+                g_last_error = "cooperative launch of the resident form failed";
+                return rc;
+            }
+            c->parity ^= (k & 1);
+        }
+    } else if (t.mode == BL_MODE_PERSISTENT) {
         const int per = t.substeps_per_launch > 0 ? t.substeps_per_launch : 64;
         for (int s = 0; s < substeps; s += per) {
             const int k = std::min(per, substeps - s);
@@ -397,6 +449,14 @@ int post_frame(bl_ctx* c) {
     if (!c->frame_pending) return BL_OK;
     c->frame_pending = false;
     HIPCHK(hipStreamSynchronize(c->stream));
+    int errw = 0;
+    HIPCHK(hipMemcpy(&errw, c->err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (errw) {
+        HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
+    HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
+        g_last_error = "a resident launch's grid barrier timed out: the state of that frame is not valid";
+        return BL_ERR_HIP;
+    }
     const size_t nb = c->bodies.size();
     std::vector<unsigned int> w(nb * BF_N);
     HIPCHK(hipMemcpy(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost));
@@ -581,6 +641,7 @@ int bl_create(bl_ctx** out, const bl_settings* s) {
     c->tune.block = 256;
     c->tune.substeps_per_launch = 0;
     c->tune.use_graph = 1;
+    c->tune.incidences_per_thread = 4;
     *out = c;
     return BL_OK;
 }
@@ -589,6 +650,7 @@ void bl_destroy(bl_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->release();
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -751,13 +813,18 @@ int bl_reset_stats(bl_ctx* c) {
 
 int bl_set_tuning(bl_ctx* c, const bl_tuning* t) {
     if (!c || !t) return BL_ERR_ARG;
-    if (t->mode < 0 || t->mode > BL_MODE_PERSISTENT || t->substeps_per_launch < 0) return BL_ERR_ARG;
-    if (t->block != 0 && t->block != 256 && t->block != 512 && t->block != 1024) return BL_ERR_ARG;
-    const bool rechunk = effective_block(c) != (t->block ? t->block : 256);
-    int rc = rechunk ? pull_state(c) : BL_OK;
+    if (t->substeps_per_launch < 0 || !tuning_valid(*t)) {
+        g_last_error = "no such execution form (mode, block, incidences per thread)";
+        return BL_ERR_ARG;
+    }
+    bl_tuning nt = *t;
+    if (!nt.block) nt.block = 256;
+    if (!nt.incidences_per_thread) nt.incidences_per_thread = 4;
+    const bool rechunk = nt.block != effective_block(c) || nt.incidences_per_thread != effective_per(c);
+    int rc = post_frame(c);
     if (rc) return rc;
-    c->tune = *t;
-    if (!c->tune.block) c->tune.block = 256;
+    if (rechunk && (rc = pull_state(c))) return rc;
+    c->tune = nt;
     c->drop_graphs();
     if (rechunk && c->built) {
         c->release();
@@ -783,6 +850,54 @@ int bl_time_step(bl_ctx* c, int32_t substeps, float* ms) {
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return post_frame(c);
+}
+
+// Times one frame of `substeps` with an event pair around every kernel launch of the substeps (no graph), so that
+// *kernel_ms is the summed duration of the substep kernels alone and *total_ms the frame's wall time on the stream.
+int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_ms, int32_t* launches) {
+    if (!c || !total_ms || !kernel_ms || !launches || substeps <= 0) return BL_ERR_ARG;
+    int rc = ensure_built(c);
+    if (rc) return rc;
+    if ((rc = post_frame(c))) return rc;
+    bl_tuning t = c->tune;
+    t.block = effective_block(c);
+    t.mode = resolved_mode(c);
+    t.incidences_per_thread = effective_per(c);
+    if (t.block != c->built_block || t.incidences_per_thread != c->built_per) return BL_ERR_STATE;
+    const int per = t.mode == BL_MODE_PERSISTENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : 64)
+                    : t.mode == BL_MODE_RESIDENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps) : 1;
+    const int nl = (substeps + per - 1) / per;
+    while ((int)c->kev.size() < 2 * nl) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
+    for (int l = 0; l < nl; l++) {
+        const int k = std::min(per, substeps - l * per);
+        HIPCHK(hipEventRecord(c->kev[2 * l], c->stream));
+        if (t.mode == BL_MODE_RESIDENT) rc = launch_resident(c->args, c->stream, c->parity, k, t, c->bar.p, c->err.p, kSpinLimit);
+        else rc = launch_substeps(c->args, c->stream, c->parity, k, t);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(c->kev[2 * l + 1], c->stream));
+        c->parity ^= (k & 1);
+    }
+    c->substeps += substeps;
+    if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
+    c->frame_pending = true;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(total_ms, c->ev0, c->ev1));
+    float sum = 0;
+    for (int l = 0; l < nl; l++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->kev[2 * l], c->kev[2 * l + 1]));
+        sum += ms;
+    }
+    *kernel_ms = sum;
+    *launches = nl;
     return post_frame(c);
 }
 

@@ -13,11 +13,18 @@
 //              iA float4 {L, k, d, strength}, iM float4 {max_pos, max_neg, beam index, flags}, iAB uint2 {a, b}.
 //              Each end keeps its own copy of the beam's mutable state; both copies see the same inputs and run
 //              the same arithmetic, so they stay identical and no end ever reads what the other writes.
-//   chunks     contiguous node ranges whose incidences fit in LDS (ch_n0[]): one workgroup each.
+//   chunks     contiguous node ranges of one body whose incidences fit in LDS (ch_n0[]): one workgroup each.
 //
 // A chunk's workgroup evaluates its incidences in parallel (one thread per incidence, forces into LDS), then one
 // thread per node adds them up in beam-index order - the order in which the reference's serial loop scatters
 // f[a] += fv, f[b] -= fv - so that every node force is bit-identical without atomics.
+//
+// Three launch forms share that chunk step:
+//   k_substep          one launch per substep, one workgroup per chunk (captured in a hipGraph by the host);
+//   k_body_persistent  one workgroup per body, many substeps per launch (small bodies);
+//   k_resident         one workgroup per chunk, many substeps per launch, the incidences' state and the chunk's
+//                      nodes held in registers across substeps, a grid barrier between substeps (cooperative
+//                      launch, so every workgroup is resident).
 #include "bl_world.h"
 
 #pragma clang fp contract(off)
@@ -26,21 +33,21 @@ namespace blg {
 
 __constant__ ground_model k_gm[BL_SURF_N] = BL_GROUND_MODELS;
 
-// ---------------------------------------------------------------------------------------------- incidence phase
-// One incidence of compute_beam_forces (src/phys/softbody.cpp:416-468): the beam's force, signed for this end,
-// and the beam's yield / break update on this end's copy of its state.
-__device__ __forceinline__ bool incidence_force(uint32_t e, const KArgs& a, const float4* __restrict__ pin,
-                                                const float4* __restrict__ vin, f3& out) {
-    const float4 M = a.iM[e];
+// ---------------------------------------------------------------------------------------------- one beam end
+enum : uint32_t { EV_CHANGED = 1, EV_BROKE = 2 };
+
+// One incidence of compute_beam_forces (src/phys/softbody.cpp:416-468) on this end's copy of the beam state
+// (A = L, k, d, strength; M = max_pos, max_neg, beam, flags). Returns true with the force signed for this end, or
+// false when the beam gives none this substep; `ev` reports a state change (yield, break) and a break.
+__device__ __forceinline__ bool beam_eval(float4& A, float4& M, const float4 Pa, const float4 Pb, const float4 Va,
+                                          const float4 Vb, const float4* __restrict__ bcold, f3& out, float& slen_out,
+                                          uint32_t& ev) {
+    ev = 0;
     uint32_t tf = f2u(M.w);
     if (tf & (TF_BROKEN | TF_SKIP)) return false;
-    const float4 A = a.iA[e];
-    const uint2 ab = a.iAB[e];
-    const float4 Pa = pin[ab.x], Pb = pin[ab.y];
     const f3 dis = sub(mk(Pa.x, Pa.y, Pa.z), mk(Pb.x, Pb.y, Pb.z));
     const float len2 = dot(dis, dis);
     if (len2 < 1e-12f) return false;
-    const float4 Va = vin[ab.x], Vb = vin[ab.y];
     const float inv_len = 1.0f / fsqrt(len2);
     const float len = len2 * inv_len;
     float L = A.x, strength = A.w, max_pos = M.x, max_neg = M.y;
@@ -50,34 +57,32 @@ __device__ __forceinline__ bool incidence_force(uint32_t e, const KArgs& a, cons
     const uint32_t type = (tf >> TF_TYPE_SHIFT) & 3u;
     const bool brk = (tf & TF_BRK) != 0;
     const uint32_t beam = f2u(M.z);
-    const bool owner = (tf & TF_ISB) == 0;
     if (type == BL_BT_ROPE) {
         if (diff < 0) { k = 0; d *= 0.1f; }
     } else if (type == BL_BT_SUPPORT) {
         if (diff > 0) {
             k = 0;
             d *= 0.1f;
-            if (brk && diff > L * a.bcold[beam].z) {
-                a.iM[e] = make_float4(max_pos, max_neg, M.z, u2f(tf | TF_BROKEN));
-                if (owner) atomicAdd(&a.broken[a.ninfo[ab.x] >> 16], 1ull);
+            if (brk && diff > L * bcold[beam].z) {
+                M.w = u2f(tf | TF_BROKEN);
+                ev = EV_CHANGED | EV_BROKE;
                 return false;
             }
         }
     }
     float slen = -k * diff - d * dot(dv, dis) * inv_len;
-    bool changed = false;
     if (!(tf & TF_NODEFORM) && k != 0) {
         if (slen > max_pos && diff < 0) {
-            const float4 cold = a.bcold[beam]; // plastic, L0, support_limit
+            const float4 cold = bcold[beam]; // plastic, L0, support_limit
             const float yield_len = max_pos / k;
             const float def = diff + yield_len * (1.0f - cold.x);
             const float Lold = L;
             L = smax(smin(0.1f, 0.3f * cold.y), L + def);
             slen = slen - (slen - max_pos) * 0.5f;
             if (L < Lold) max_pos *= Lold / L;
-            changed = true;
+            ev = EV_CHANGED;
         } else if (slen < max_neg && diff > 0) {
-            const float4 cold = a.bcold[beam];
+            const float4 cold = bcold[beam];
             const float yield_len = max_neg / k;
             const float def = diff + yield_len * (1.0f - cold.x);
             const float Lold = L;
@@ -85,23 +90,22 @@ __device__ __forceinline__ bool incidence_force(uint32_t e, const KArgs& a, cons
             slen = slen - (slen - max_neg) * 0.5f;
             if (Lold > 0 && L > Lold) max_neg *= L / Lold;
             if (!(tf & TF_DUCTILE)) strength -= def * k;
-            changed = true;
+            ev = EV_CHANGED;
         }
     }
+    if (ev) {
+        A.x = L, A.w = strength;
+        M.x = max_pos, M.y = max_neg;
+    }
     if (brk && ffabs(slen) > strength && !(tf & TF_NOBREAK)) {
-        a.iA[e] = make_float4(L, A.y, A.z, strength);
-        a.iM[e] = make_float4(max_pos, max_neg, M.z, u2f(tf | TF_BROKEN));
-        if (owner) atomicAdd(&a.broken[a.ninfo[ab.x] >> 16], 1ull);
+        M.w = u2f(tf | TF_BROKEN);
+        ev = EV_CHANGED | EV_BROKE;
         return false;
     }
-    if (changed) {
-        a.iA[e] = make_float4(L, A.y, A.z, strength);
-        a.iM[e] = make_float4(max_pos, max_neg, M.z, M.w);
-    }
-    if (owner) a.stress[beam] = slen;
+    slen_out = slen;
     const f3 fv = mul(dis, slen * inv_len);
     // f[b] -= fv is f[b] + (-fv) to the bit: the b end stores the negated force and the gather only adds
-    out = owner ? fv : mk(-fv.x, -fv.y, -fv.z);
+    out = (tf & TF_ISB) ? mk(-fv.x, -fv.y, -fv.z) : fv;
     return true;
 }
 
@@ -157,19 +161,17 @@ struct NodeCounts {
 };
 
 // One node after its force sum: World::collide_static (src/phys/world.cpp:799-869, boxes only) then
-// SoftBody::integrate_nodes (src/phys/softbody.cpp:773-791).
-__device__ __forceinline__ void node_finish(uint32_t i, const KArgs& a, float4 P, float4 Vv, f3 f, float4* __restrict__ pout,
-                                            float4* __restrict__ vout, NodeCounts& cnt) {
-    f3 p = mk(P.x, P.y, P.z), v = mk(Vv.x, Vv.y, Vv.z);
+// SoftBody::integrate_nodes (src/phys/softbody.cpp:773-791). P, V are updated in place.
+__device__ __forceinline__ void node_finish(const KArgs& a, uint32_t info, float2 aux, float4& P, float4& V, f3 f,
+                                            NodeCounts& cnt) {
+    f3 p = mk(P.x, P.y, P.z), v = mk(V.x, V.y, V.z);
     float inv_mass = P.w;
-    const float mass = Vv.w;
-    const uint32_t info = a.ninfo[i];
+    const float mass = V.w;
     if ((info & BL_NF_GROUND) && inv_mass > 0) {
         if (p.y < a.kill_y) {
             v = mk(0, 0, 0);
             inv_mass = 0;
         } else if (a.nbox > 0) {
-            const float2 aux = a.naux[i]; // radius, friction x the body's ground friction
             f3 n;
             float depth, max_force;
             uint32_t surf;
@@ -204,12 +206,24 @@ __device__ __forceinline__ void node_finish(uint32_t i, const KArgs& a, float4 P
         p = add(p, mul(v, a.dt));
         cnt.max_v2 = v2 > cnt.max_v2 ? v2 : cnt.max_v2;
     }
-    pout[i] = make_float4(p.x, p.y, p.z, inv_mass);
-    vout[i] = make_float4(v.x, v.y, v.z, mass);
+    P = make_float4(p.x, p.y, p.z, inv_mass);
+    V = make_float4(v.x, v.y, v.z, mass);
 }
 
-// ---------------------------------------------------------------------------------------------- one chunk
-template <int T, int CAP>
+// The force sum of one node from the chunk's incidence forces in LDS, in beam-index order; starts from
+// clear_forces' f = g m (src/phys/softbody.cpp:381).
+__device__ __forceinline__ f3 node_force(const KArgs& a, float mass, uint32_t b0, uint32_t b1, const float* sfx,
+                                         const float* sfy, const float* sfz, const uint8_t* sok) {
+    f3 f = mul(a.g, mass);
+    for (uint32_t s = b0; s < b1; s++)
+        if (sok[s]) f = add(f, mk(sfx[s], sfy[s], sfz[s]));
+    return f;
+}
+
+// ---------------------------------------------------------------------------------------------- streamed chunk
+// One chunk's substep with the incidence records streamed from memory: all of a thread's loads are issued before
+// its arithmetic (PER incidences per thread).
+template <int T, int PER>
 __device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int parity, float* sfx, float* sfy,
                                               float* sfz, uint8_t* sok, NodeCounts& cnt) {
     const float4* __restrict__ pin = a.pm[parity];
@@ -218,96 +232,301 @@ __device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int pa
     float4* __restrict__ vout = a.vm[parity ^ 1];
     const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
     const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
-    for (uint32_t e = e0 + threadIdx.x; e < e1; e += T) {
-        f3 fv;
-        const bool ok = incidence_force(e, a, pin, vin, fv);
-        const uint32_t s = e - e0;
+    float4 A[PER], M[PER];
+    uint2 ab[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + threadIdx.x + k * T;
+        if (e < e1) A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = a.iAB[e];
+    }
+    float4 Pa[PER], Pb[PER], Va[PER], Vb[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + threadIdx.x + k * T;
+        if (e < e1) Pa[k] = pin[ab[k].x], Pb[k] = pin[ab[k].y], Va[k] = vin[ab[k].x], Vb[k] = vin[ab[k].y];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + threadIdx.x + k * T;
+        if (e >= e1) continue;
+        f3 fv = mk(0, 0, 0);
+        float slen = 0;
+        uint32_t ev;
+        const bool ok = beam_eval(A[k], M[k], Pa[k], Pb[k], Va[k], Vb[k], a.bcold, fv, slen, ev);
+        const uint32_t tf = f2u(M[k].w);
+        if (ev & EV_CHANGED) a.iA[e] = A[k], a.iM[e] = M[k];
+        if ((ev & EV_BROKE) && !(tf & TF_ISB)) atomicAdd(&a.broken[a.ninfo[ab[k].x] >> 16], 1ull);
+        if (ok && !(tf & TF_ISB)) a.stress[f2u(M[k].z)] = slen;
+        const uint32_t s = threadIdx.x + k * T;
         sfx[s] = fv.x, sfy[s] = fv.y, sfz[s] = fv.z;
         sok[s] = ok;
     }
     __syncthreads();
     for (uint32_t i = n0 + threadIdx.x; i < n1; i += T) {
-        const float4 P = pin[i], Vv = vin[i];
-        // clear_forces (src/phys/softbody.cpp:381): f = g m
-        f3 f = mul(a.g, Vv.w);
+        float4 P = pin[i], V = vin[i];
         const uint32_t b0 = a.noff[i] - e0, b1 = a.noff[i + 1] - e0;
-        for (uint32_t s = b0; s < b1; s++)
-            if (sok[s]) f = add(f, mk(sfx[s], sfy[s], sfz[s]));
-        node_finish(i, a, P, Vv, f, pout, vout, cnt);
+        const f3 f = node_force(a, V.w, b0, b1, sfx, sfy, sfz, sok);
+        node_finish(a, a.ninfo[i], a.naux[i], P, V, f, cnt);
+        pout[i] = P, vout[i] = V;
     }
 }
 
-__device__ __forceinline__ void flush_counts(const KArgs& a, NodeCounts cnt) {
+// The workgroup's counts onto its chunk's slot (a plain read-modify-write: one workgroup owns a chunk at a time;
+// the slots are folded into the counters once a frame, so the substeps need no atomics).
+template <int T>
+__device__ __forceinline__ void flush_chunk(const KArgs& a, NodeCounts cnt, uint32_t c, uint4* sred) {
     uint32_t c0 = cnt.contacts, c1 = cnt.clamps, c2 = cnt.nans;
+    float v2 = cnt.max_v2;
     for (int off = 32; off > 0; off >>= 1) {
         c0 += __shfl_xor(c0, off);
         c1 += __shfl_xor(c1, off);
         c2 += __shfl_xor(c2, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (c0) atomicAdd(&a.counters[CNT_STATIC], (unsigned long long)c0);
-        if (c1) atomicAdd(&a.counters[CNT_CLAMPS], (unsigned long long)c1);
-        if (c2) atomicAdd(&a.counters[CNT_NANS], (unsigned long long)c2);
-    }
-}
-
-// The workgroup's top |v|^2 onto its body's frame word (all nodes of a workgroup belong to one body).
-template <int T>
-__device__ __forceinline__ void flush_speed(const KArgs& a, float v2, uint32_t body, float* sred) {
-    for (int off = 32; off > 0; off >>= 1) {
         const float o = __shfl_xor(v2, off);
         v2 = o > v2 ? o : v2;
     }
-    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = v2;
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = make_uint4(c0, c1, c2, f2u(v2));
     __syncthreads();
     if (threadIdx.x == 0) {
-        float m = sred[0];
-        for (int w = 1; w < T / 64; w++) m = sred[w] > m ? sred[w] : m;
-        if (m > 0) atomicMax(&a.body_frame[body * BF_N + BF_V2], f2u(m));
+        uint4 t = a.cstat[c];
+        for (int w = 0; w < T / 64; w++) {
+            const uint4 x = sred[w];
+            t.x += x.x, t.y += x.y, t.z += x.z;
+            t.w = u2f(x.w) > u2f(t.w) ? x.w : t.w;
+        }
+        a.cstat[c] = t;
     }
 }
 
-// One substep over all chunks: one workgroup per chunk, chunks laid out contiguously per XCD (the hardware deals
-// workgroup b to XCD b % 8; logical chunk ranges per XCD keep a node's neighbours in that XCD's L2).
-template <int T, int CAP>
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t C, int remap) {
+    if (!remap) return b;
+    // the hardware deals workgroup b to XCD b % 8: give each XCD a contiguous range of chunks
+    const uint32_t per = C / 8, rem = C % 8, x = b % 8, idx = b / 8;
+    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+template <int T, int PER>
 __global__ __launch_bounds__(T) void k_substep(KArgs a, int parity) {
-    __shared__ float sfx[CAP], sfy[CAP], sfz[CAP];
-    __shared__ uint8_t sok[CAP];
-    const uint32_t C = a.nchunks;
-    uint32_t c = blockIdx.x;
-    if (a.xcd_remap) {
-        const uint32_t per = C / 8, rem = C % 8, x = blockIdx.x % 8, idx = blockIdx.x / 8;
-        c = x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
-    }
+    __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
+    __shared__ uint8_t sok[T * PER];
+    __shared__ uint4 sred[T / 64];
+    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    chunk_substep<T, CAP>(c, a, parity, sfx, sfy, sfz, sok, cnt);
-    flush_counts(a, cnt);
-    __shared__ float sred[T / 64];
-    flush_speed<T>(a, cnt.max_v2, a.ninfo[a.ch_n0[c]] >> 16, sred);
+    chunk_substep<T, PER>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+    flush_chunk<T>(a, cnt, c, sred);
 }
 
 // Many substeps per launch, one workgroup per body: the body's chunks one after another, a workgroup barrier
 // between phases (bodies are independent without inter-body contacts).
-template <int T, int CAP>
+template <int T, int PER>
 __global__ __launch_bounds__(T) void k_body_persistent(KArgs a, int parity0, int substeps) {
-    __shared__ float sfx[CAP], sfy[CAP], sfz[CAP];
-    __shared__ uint8_t sok[CAP];
+    __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
+    __shared__ uint8_t sok[T * PER];
+    __shared__ uint4 sred[T / 64];
     const uint32_t body = blockIdx.x;
     const uint32_t c0 = a.body_c0[body], c1 = a.body_c0[body + 1];
     NodeCounts cnt = {0, 0, 0, 0.0f};
     for (int s = 0; s < substeps; s++) {
         const int parity = parity0 ^ (s & 1);
         for (uint32_t c = c0; c < c1; c++) {
-            chunk_substep<T, CAP>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+            chunk_substep<T, PER>(c, a, parity, sfx, sfy, sfz, sok, cnt);
             __syncthreads();
         }
     }
-    flush_counts(a, cnt);
-    __shared__ float sred[T / 64];
-    flush_speed<T>(a, cnt.max_v2, body, sred);
+    if (c1 > c0) flush_chunk<T>(a, cnt, c0, sred);
 }
 
-// Bounds and top speed of the current state (SoftBody::integrate_nodes' mn/mx/max_v2 over all nodes).
+// ---------------------------------------------------------------------------------------------- resident form
+// Grid-wide barrier of a cooperative launch: a monotonic arrival counter (cleared before the launch). Bounded:
+// after `spin_limit` polls, or once another workgroup has given up, it raises the error word and the caller ends
+// its loop, so the grid always drains.
+__device__ __forceinline__ bool grid_barrier(unsigned int* count, int* err, unsigned int target, uint32_t spin_limit,
+                                             int* sflag) {
+    __builtin_amdgcn_s_waitcnt(0); // this wave's stores of the substep acknowledged
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence(); // the workgroup's stores out to the device scope
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        int bad = 0;
+        while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { bad = 1; break; }
+            if (++spins > spin_limit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bad = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        *sflag = bad;
+    }
+    __syncthreads();
+    return *sflag == 0;
+}
+
+// Many substeps per launch over the whole chip: each workgroup owns one chunk for the launch, keeps its
+// incidences' state (PER per thread) and its nodes (at most one per thread) in registers, and reads the other
+// chunks' nodes from the previous substep's buffer after the grid barrier. Writes the state back at the end.
+template <int T, int PER, int WAVES>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void k_resident(
+    KArgs a, int parity0, int substeps, unsigned int* bar, int* err, uint32_t spin_limit) {
+    __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
+    __shared__ uint8_t sok[T * PER];
+    __shared__ uint4 sred[T / 64];
+    __shared__ int sflag;
+    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
+    const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
+    float4 A[PER], M[PER];
+    uint2 ab[PER];
+    float stress[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + threadIdx.x + k * T;
+        A[k] = make_float4(0, 0, 0, 0), M[k] = make_float4(0, 0, 0, u2f(TF_SKIP)), ab[k] = make_uint2(0, 0);
+        stress[k] = 0;
+        if (e < e1) {
+            A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = a.iAB[e];
+            stress[k] = (f2u(M[k].w) & TF_ISB) ? 0.0f : a.stress[f2u(M[k].z)];
+        }
+    }
+    const uint32_t i = n0 + threadIdx.x;
+    const bool has_node = i < n1;
+    float4 P = make_float4(0, 0, 0, 0), V = make_float4(0, 0, 0, 0);
+    uint32_t info = 0, b0 = 0, b1 = 0;
+    float2 aux = make_float2(0, 0);
+    if (has_node) {
+        P = a.pm[parity0][i], V = a.vm[parity0][i];
+        info = a.ninfo[i], aux = a.naux[i];
+        b0 = a.noff[i] - e0, b1 = a.noff[i + 1] - e0;
+    }
+    NodeCounts cnt = {0, 0, 0, 0.0f};
+    const unsigned int nb = gridDim.x;
+    for (int s = 0; s < substeps; s++) {
+        const int parity = parity0 ^ (s & 1);
+        const float4* __restrict__ pin = a.pm[parity];
+        const float4* __restrict__ vin = a.vm[parity];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const uint32_t e = e0 + threadIdx.x + k * T;
+            if (e >= e1) continue;
+            const float4 Pa = pin[ab[k].x], Pb = pin[ab[k].y], Va = vin[ab[k].x], Vb = vin[ab[k].y];
+            f3 fv = mk(0, 0, 0);
+            float slen = 0;
+            uint32_t ev;
+            const bool ok = beam_eval(A[k], M[k], Pa, Pb, Va, Vb, a.bcold, fv, slen, ev);
+            if ((ev & EV_BROKE) && !(f2u(M[k].w) & TF_ISB)) atomicAdd(&a.broken[a.ninfo[ab[k].x] >> 16], 1ull);
+            if (ok) stress[k] = slen;
+            const uint32_t sl = threadIdx.x + k * T;
+            sfx[sl] = fv.x, sfy[sl] = fv.y, sfz[sl] = fv.z;
+            sok[sl] = ok;
+        }
+        __syncthreads();
+        if (has_node) {
+            const f3 f = node_force(a, V.w, b0, b1, sfx, sfy, sfz, sok);
+            node_finish(a, info, aux, P, V, f, cnt);
+            a.pm[parity ^ 1][i] = P, a.vm[parity ^ 1][i] = V;
+        }
+        if (!grid_barrier(bar, err, (unsigned int)(s + 1) * nb, spin_limit, &sflag)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + threadIdx.x + k * T;
+        if (e < e1) {
+            a.iA[e] = A[k], a.iM[e] = M[k];
+            if (!(f2u(M[k].w) & TF_ISB)) a.stress[f2u(M[k].z)] = stress[k];
+        }
+    }
+    flush_chunk<T>(a, cnt, c, sred);
+}
+
+// ---------------------------------------------------------------------------------------------- frame kernels
+// The workgroup's nodes all belong to one body when its first and last do (bodies are contiguous): then a
+// workgroup reduction and one atomic per word, else one atomic per node (small bodies: little contention).
+template <int T>
+__device__ __forceinline__ void body_reduce(const KArgs& a, uint32_t i, bool valid, float v2, float px, float py,
+                                            float pz, bool want_v2, bool want_box, uint32_t* sred) {
+    const uint32_t first = blockIdx.x * T;
+    const uint32_t last = (first + T < a.nnodes ? first + T : a.nnodes) - 1;
+    const uint32_t bfirst = a.ninfo[first] >> 16, blast = a.ninfo[last] >> 16;
+    uint32_t kv = valid && want_v2 && v2 > 0 ? f2u(v2) : 0u;
+    const float p[3] = {px, py, pz};
+    uint32_t kmn[3], kmx[3];
+    for (int k = 0; k < 3; k++) {
+        const bool ok = valid && want_box && p[k] == p[k];
+        kmn[k] = ok ? fkey(p[k]) : 0xffffffffu;
+        kmx[k] = ok ? fkey(p[k]) : 0u;
+    }
+    if (bfirst != blast) {
+        if (!valid) return;
+        unsigned int* w = a.body_frame + (a.ninfo[i] >> 16) * BF_N;
+        if (kv) atomicMax(&w[BF_V2], kv);
+        for (int k = 0; k < 3; k++)
+            if (want_box && p[k] == p[k]) atomicMin(&w[BF_MN + k], kmn[k]), atomicMax(&w[BF_MX + k], kmx[k]);
+        return;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        kv = max(kv, (uint32_t)__shfl_xor((int)kv, off));
+        for (int k = 0; k < 3; k++) {
+            kmn[k] = min(kmn[k], (uint32_t)__shfl_xor((int)kmn[k], off));
+            kmx[k] = max(kmx[k], (uint32_t)__shfl_xor((int)kmx[k], off));
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sred[wv * 7] = kv;
+        for (int k = 0; k < 3; k++) sred[wv * 7 + 1 + k] = kmn[k], sred[wv * 7 + 4 + k] = kmx[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w2 = 1; w2 < T / 64; w2++) {
+            sred[0] = max(sred[0], sred[w2 * 7]);
+            for (int k = 0; k < 3; k++)
+                sred[1 + k] = min(sred[1 + k], sred[w2 * 7 + 1 + k]), sred[4 + k] = max(sred[4 + k], sred[w2 * 7 + 4 + k]);
+        }
+        unsigned int* w = a.body_frame + bfirst * BF_N;
+        if (want_v2 && sred[0]) atomicMax(&w[BF_V2], sred[0]);
+        if (want_box)
+            for (int k = 0; k < 3; k++) atomicMin(&w[BF_MN + k], sred[1 + k]), atomicMax(&w[BF_MX + k], sred[4 + k]);
+    }
+}
+
+// The frame's start (src/phys/world.cpp:3217-3222): every body's top |v|^2 over all its nodes (NaN left out, as
+// std::max leaves it).
+__global__ __launch_bounds__(256) void k_frame_begin(KArgs a, int parity) {
+    __shared__ uint32_t sred[4 * 7];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < a.nnodes;
+    float v2 = 0;
+    if (valid) {
+        const float4 V = a.vm[parity][i];
+        const f3 v = mk(V.x, V.y, V.z);
+        v2 = dot(v, v);
+    }
+    body_reduce<256>(a, i, valid, v2, 0, 0, 0, true, false, sred);
+}
+
+// The frame's end: each body's bounds (vmin / vmax over its nodes: NaN coordinates leave them unchanged).
+__global__ __launch_bounds__(256) void k_frame_end(KArgs a, int parity) {
+    __shared__ uint32_t sred[4 * 7];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < a.nnodes;
+    float4 P = make_float4(0, 0, 0, 0);
+    if (valid) P = a.pm[parity][i];
+    body_reduce<256>(a, i, valid, 0, P.x, P.y, P.z, false, true, sred);
+}
+
+// Resets the per-body frame words: top speed 0, bounds keys to the empty box (1e30 / -1e30, as the reference's
+// integrate_nodes starts them).
+__global__ void k_frame_clear(KArgs a) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.nbodies) return;
+    unsigned int* w = a.body_frame + b * BF_N;
+    w[BF_V2] = 0;
+    for (int k = 0; k < 3; k++) w[BF_MN + k] = fkey(1e30f), w[BF_MX + k] = fkey(-1e30f);
+}
+
+// Bounds and top speed of the current state, all bodies (for bl_get_stats).
 __global__ void k_bounds(KArgs a, int parity, unsigned int* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nnodes) return;
@@ -322,78 +541,114 @@ __global__ void k_bounds(KArgs a, int parity, unsigned int* out) {
     }
 }
 
-// The frame's start (src/phys/world.cpp:3217-3222): every body's top |v|^2 over all its nodes (NaN left out, as
-// std::max leaves it), and its bounds keys reset.
-__global__ void k_frame_begin(KArgs a, int parity) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.nbodies) {
-        unsigned int* w = a.body_frame + i * BF_N;
-        w[BF_MN] = w[BF_MN + 1] = w[BF_MN + 2] = fkey(1e30f);
-        w[BF_MX] = w[BF_MX + 1] = w[BF_MX + 2] = fkey(-1e30f);
-    }
-    if (i >= a.nnodes) return;
-    const float4 V = a.vm[parity][i];
-    const f3 v = mk(V.x, V.y, V.z);
-    const float v2 = dot(v, v);
-    if (v2 > 0) atomicMax(&a.body_frame[(a.ninfo[i] >> 16) * BF_N + BF_V2], f2u(v2));
+// The frame's end for the chunk slots: counts into the counters, top speed into the body's frame word; slots cleared.
+__global__ void k_chunk_fold(KArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nchunks) return;
+    const uint4 t = a.cstat[c];
+    if (t.x) atomicAdd(&a.counters[CNT_STATIC], (unsigned long long)t.x);
+    if (t.y) atomicAdd(&a.counters[CNT_CLAMPS], (unsigned long long)t.y);
+    if (t.z) atomicAdd(&a.counters[CNT_NANS], (unsigned long long)t.z);
+    if (u2f(t.w) > 0) atomicMax(&a.body_frame[a.chunk_body[c] * BF_N + BF_V2], t.w);
+    a.cstat[c] = make_uint4(0, 0, 0, 0);
 }
 
-// The frame's end: each body's bounds (vmin / vmax over its nodes: NaN coordinates leave them unchanged).
-__global__ void k_frame_end(KArgs a, int parity) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nnodes) return;
-    const float4 P = a.pm[parity][i];
-    unsigned int* w = a.body_frame + (a.ninfo[i] >> 16) * BF_N;
-    const float p[3] = {P.x, P.y, P.z};
-    for (int k = 0; k < 3; k++) {
-        if (p[k] != p[k]) continue;
-        atomicMin(&w[BF_MN + k], fkey(p[k]));
-        atomicMax(&w[BF_MX + k], fkey(p[k]));
-    }
+__global__ void k_chunk_clear(KArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.nchunks) a.cstat[c] = make_uint4(0, 0, 0, 0);
 }
 
-__global__ void k_clear_u32(unsigned int* p, uint32_t n) {
+__global__ void k_clear_words(unsigned int* p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------------------- launches
+#define BL_OK_OR_HIP() (hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP)
+
+template <int T, int PER, int WAVES>
+static int launch_resident_t(const KArgs& a, hipStream_t st, int parity, int substeps, unsigned int* bar, int* err,
+                             uint32_t spin_limit) {
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, st, bar, 1u);
+    KArgs aa = a;
+    int p = parity, s = substeps;
+    void* args[] = {&aa, &p, &s, &bar, &err, &spin_limit};
+    if (hipLaunchCooperativeKernel((const void*)k_resident<T, PER, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st) != hipSuccess)
+        return BL_ERR_HIP;
+    return BL_OK_OR_HIP();
+}
+
+int resident_fits(int block, int per, uint32_t nchunks) {
+    int dev = 0, cus = 0, nbpm = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (block == 256 && per == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_resident<256, 4, 4>, 256, 0);
+    else if (block == 256 && per == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_resident<256, 2, 6>, 256, 0);
+    else if (block == 256 && per == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_resident<256, 8, 2>, 256, 0);
+    else if (block == 512 && per == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_resident<512, 4, 4>, 512, 0);
+    if (e != hipSuccess) return 0;
+    return (uint64_t)nbpm * (uint64_t)cus >= nchunks ? 1 : 0;
+}
+
+int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,
+                    int* err, uint32_t spin_limit) {
+    const int per = t.incidences_per_thread;
+    if (t.block == 256 && per == 4) return launch_resident_t<256, 4, 4>(a, st, parity, substeps, bar, err, spin_limit);
+    if (t.block == 256 && per == 2) return launch_resident_t<256, 2, 6>(a, st, parity, substeps, bar, err, spin_limit);
+    if (t.block == 256 && per == 8) return launch_resident_t<256, 8, 2>(a, st, parity, substeps, bar, err, spin_limit);
+    if (t.block == 512 && per == 4) return launch_resident_t<512, 4, 4>(a, st, parity, substeps, bar, err, spin_limit);
+    return BL_ERR_ARG;
+}
+
+// The launch of one form; every (block, incidences per thread) pair the host may chunk for has its own instance
+// (the chunks hold at most block x per incidences and block nodes), any other pair is refused.
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t) {
+    const int per = t.incidences_per_thread, T = t.block;
     if (t.mode == BL_MODE_PERSISTENT) {
-        if (t.block == 512) hipLaunchKernelGGL((k_body_persistent<512, 2048>), dim3(a.nbodies), dim3(512), 0, st, a, parity, substeps);
-        else if (t.block == 1024) hipLaunchKernelGGL((k_body_persistent<1024, 4096>), dim3(a.nbodies), dim3(1024), 0, st, a, parity, substeps);
-        else hipLaunchKernelGGL((k_body_persistent<256, 1024>), dim3(a.nbodies), dim3(256), 0, st, a, parity, substeps);
-        return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+        if (T == 1024 && per == 4) hipLaunchKernelGGL((k_body_persistent<1024, 4>), dim3(a.nbodies), dim3(1024), 0, st, a, parity, substeps);
+        else if (T == 256 && per == 4) hipLaunchKernelGGL((k_body_persistent<256, 4>), dim3(a.nbodies), dim3(256), 0, st, a, parity, substeps);
+        else return BL_ERR_ARG;
+        return BL_OK_OR_HIP();
     }
     for (int s = 0; s < substeps; s++) {
         const int p = parity ^ (s & 1);
-        if (t.block == 512) hipLaunchKernelGGL((k_substep<512, 2048>), dim3(a.nchunks), dim3(512), 0, st, a, p);
-        else if (t.block == 1024) hipLaunchKernelGGL((k_substep<1024, 4096>), dim3(a.nchunks), dim3(1024), 0, st, a, p);
-        else hipLaunchKernelGGL((k_substep<256, 1024>), dim3(a.nchunks), dim3(256), 0, st, a, p);
+        if (T == 256 && per == 1) hipLaunchKernelGGL((k_substep<256, 1>), dim3(a.nchunks), dim3(256), 0, st, a, p);
+        else if (T == 256 && per == 2) hipLaunchKernelGGL((k_substep<256, 2>), dim3(a.nchunks), dim3(256), 0, st, a, p);
+        else if (T == 256 && per == 4) hipLaunchKernelGGL((k_substep<256, 4>), dim3(a.nchunks), dim3(256), 0, st, a, p);
+        else if (T == 256 && per == 8) hipLaunchKernelGGL((k_substep<256, 8>), dim3(a.nchunks), dim3(256), 0, st, a, p);
+        else if (T == 512 && per == 2) hipLaunchKernelGGL((k_substep<512, 2>), dim3(a.nchunks), dim3(512), 0, st, a, p);
+        else if (T == 512 && per == 4) hipLaunchKernelGGL((k_substep<512, 4>), dim3(a.nchunks), dim3(512), 0, st, a, p);
+        else return BL_ERR_ARG;
     }
-    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+    return BL_OK_OR_HIP();
 }
 
 int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out) {
     if (a.nnodes == 0) return BL_OK;
     hipLaunchKernelGGL(k_bounds, dim3((a.nnodes + 255) / 256), dim3(256), 0, st, a, parity, out);
-    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+    return BL_OK_OR_HIP();
 }
 
 int launch_frame_begin(const KArgs& a, hipStream_t st, int parity) {
-    const uint32_t nw = a.nbodies * BF_N;
-    hipLaunchKernelGGL(k_clear_u32, dim3((nw + 255) / 256), dim3(256), 0, st, a.body_frame, nw);
-    const uint32_t n = a.nnodes > a.nbodies ? a.nnodes : a.nbodies;
-    hipLaunchKernelGGL(k_frame_begin, dim3((n + 255) / 256), dim3(256), 0, st, a, parity);
-    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+    hipLaunchKernelGGL(k_frame_clear, dim3((a.nbodies + 63) / 64), dim3(64), 0, st, a);
+    if (a.nnodes) hipLaunchKernelGGL(k_frame_begin, dim3((a.nnodes + 255) / 256), dim3(256), 0, st, a, parity);
+    return BL_OK_OR_HIP();
 }
 
 int launch_frame_end(const KArgs& a, hipStream_t st, int parity) {
     if (a.nnodes == 0) return BL_OK;
     hipLaunchKernelGGL(k_frame_end, dim3((a.nnodes + 255) / 256), dim3(256), 0, st, a, parity);
-    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+    if (a.nchunks) hipLaunchKernelGGL(k_chunk_fold, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
+    return BL_OK_OR_HIP();
 }
 
-int chunk_capacity(int block) { return block == 512 ? 2048 : block == 1024 ? 4096 : 1024; }
+int launch_chunk_clear(const KArgs& a, hipStream_t st) {
+    if (a.nchunks) hipLaunchKernelGGL(k_chunk_clear, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
+    return BL_OK_OR_HIP();
+}
+
+// Incidences of a chunk: block x incidences per thread (the node cap of a chunk is the block).
+int chunk_capacity(int block, int per) { return block * per; }
 
 } // namespace blg

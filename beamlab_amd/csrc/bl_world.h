@@ -53,6 +53,8 @@ struct KArgs {
     unsigned long long* counters; // CNT_*
     unsigned long long* broken;   // per body
     unsigned int* body_frame;     // per body, BF_N words: top |v|^2 of the frame (float bits), bounds keys
+    uint4* cstat;                 // per chunk: static contacts, clamps, NaN resets, top |v|^2 (float bits) of the frame
+    const uint32_t* chunk_body;   // per chunk: its body
     f3 g;
     float dt;
     float kill_y;
@@ -64,9 +66,13 @@ struct KArgs {
 };
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
+int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,
+                    int* err, uint32_t spin_limit);
+int resident_fits(int block, int per, uint32_t nchunks);
 int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out);
 int launch_frame_begin(const KArgs& a, hipStream_t st, int parity);
 int launch_frame_end(const KArgs& a, hipStream_t st, int parity);
-int chunk_capacity(int block);
+int launch_chunk_clear(const KArgs& a, hipStream_t st);
+int chunk_capacity(int block, int per);
 
 } // namespace blg

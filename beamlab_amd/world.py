@@ -153,8 +153,8 @@ class World:
             abi.check(self._lib().bl_set_statics(self._context(), abi.ptr(boxes), len(boxes)), "bl_set_statics")
             self._statics_sent = key
 
-    def set_tuning(self, mode=abi.MODE_AUTO, block=256, substeps_per_launch=0, use_graph=1):
-        t = abi.Tuning(mode, block, substeps_per_launch, use_graph)
+    def set_tuning(self, mode=abi.MODE_AUTO, block=256, substeps_per_launch=0, use_graph=1, incidences_per_thread=4):
+        t = abi.Tuning(mode, block, substeps_per_launch, use_graph, incidences_per_thread)
         abi.check(self._lib().bl_set_tuning(self._context(), ctypes.byref(t)), "bl_set_tuning")
 
     def get_tuning(self):
@@ -171,6 +171,14 @@ class World:
         ms = ctypes.c_float(0)
         abi.check(self._lib().bl_time_step(self._context(), int(n), ctypes.byref(ms)), "bl_time_step")
         return ms.value
+
+    def time_kernels(self, n):
+        """(frame ms, summed substep-kernel ms, launches) of one frame of n substeps, events around each launch."""
+        self._sync_statics()
+        tot, ker, nl = ctypes.c_float(0), ctypes.c_float(0), ctypes.c_int32(0)
+        abi.check(self._lib().bl_time_kernels(self._context(), int(n), ctypes.byref(tot), ctypes.byref(ker),
+                                              ctypes.byref(nl)), "bl_time_kernels")
+        return tot.value, ker.value, nl.value
 
     def read_body(self, bid):
         b = self.bodies[bid]

@@ -200,16 +200,18 @@ typedef struct bl_stats {
 
 /* Execution choices (written by the autotuner, or by hand). Zero fields take the defaults. */
 typedef struct bl_tuning {
-    int32_t mode;              /* BL_MODE_* */
-    int32_t block;             /* threads per workgroup */
-    int32_t substeps_per_launch;
-    int32_t use_graph;         /* capture a launch sequence in a hipGraph */
+    int32_t mode;                  /* BL_MODE_* */
+    int32_t block;                 /* threads per workgroup: 256, 512 or 1024 */
+    int32_t substeps_per_launch;   /* persistent forms: substeps per launch (0: a whole bl_step) */
+    int32_t use_graph;             /* stepped form: capture the launches in a hipGraph */
+    int32_t incidences_per_thread; /* beam ends per thread: 1, 2, 4 or 8 (a chunk holds block x this many) */
 } bl_tuning;
 
 enum {
     BL_MODE_AUTO = 0,
-    BL_MODE_STEPPED = 1,       /* one node-phase launch per substep */
-    BL_MODE_PERSISTENT = 2,    /* one workgroup per body, many substeps per launch */
+    BL_MODE_STEPPED = 1,    /* one launch per substep, one workgroup per chunk */
+    BL_MODE_PERSISTENT = 2, /* one workgroup per body, many substeps per launch */
+    BL_MODE_RESIDENT = 3,   /* one workgroup per chunk, state in registers, many substeps per launch (cooperative) */
 };
 
 /* ---------------------------------------------------------------------------------------------------------------
@@ -279,6 +281,10 @@ int bl_get_tuning(bl_ctx* ctx, bl_tuning* t);
 
 /* Times `substeps` substeps (already resident inputs) with HIP events on the context's stream. *ms = elapsed. */
 int bl_time_step(bl_ctx* ctx, int32_t substeps, float* ms);
+
+/* Times one frame of `substeps` with an event pair around every kernel launch of the substeps (no graph capture):
+ * *kernel_ms = summed duration of the substep kernels, *total_ms = the frame on the stream, *launches = launches. */
+int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kernel_ms, int32_t* launches);
 
 #ifdef __cplusplus
 }

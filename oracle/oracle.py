@@ -5,6 +5,7 @@ Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
 Both worlds take the same records as the C ABI (include/beamlab_gpu.h), so one input feeds the GPU, the oracle
 and the reference alike.
 """
+import atexit
 import ctypes
 import os
 import subprocess
@@ -76,6 +77,7 @@ def ref_lib():
             "create": (_P, [_P, ctypes.c_int]),
             "destroy": (None, [_P]),
             "threads": (ctypes.c_int, []),
+            "shutdown": (None, []),
             "set_statics": (ctypes.c_int, [_P, _P, ctypes.c_int]),
             "box_from_quat": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
             "add_body": (ctypes.c_int, [_P, _P]),
@@ -87,6 +89,7 @@ def ref_lib():
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
         })
+        atexit.register(_ref.blref_shutdown)
     return _ref
 
 

@@ -102,6 +102,9 @@ void blref_destroy(void* h) { delete static_cast<RefWorld*>(h); }
 
 int blref_threads(void) { return JobSystem::get().num_threads(); }
 
+// Joins the job system's workers (before the process exits: a parked worker otherwise holds the exit up).
+void blref_shutdown(void) { JobSystem::get().shutdown(); }
+
 int blref_set_statics(void* h, const bl_box* boxes, int n) {
     auto* rw = static_cast<RefWorld*>(h);
     rw->w.statics.boxes.clear();

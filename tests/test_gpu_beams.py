@@ -1,0 +1,124 @@
+"""The HIP beam path against the oracle, bit for bit (node p and v, beam state and break events), on every case,
+in every execution mode; the full-size lattice (64x32x24) through the oracle too; and the compiled reference's
+golden fixtures."""
+import os
+
+import numpy as np
+import pytest
+
+from _cases import CPU_CASES, assert_same, run_world
+from beamlab_amd import abi
+from beamlab_amd.world import World
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = {
+    "auto": dict(mode=abi.MODE_AUTO),
+    "stepped_graph": dict(mode=abi.MODE_STEPPED, block=256, use_graph=1, substeps_per_launch=16, incidences_per_thread=1),
+    "stepped_nograph": dict(mode=abi.MODE_STEPPED, block=512, use_graph=0, incidences_per_thread=2),
+    "stepped_4": dict(mode=abi.MODE_STEPPED, block=256, use_graph=1, substeps_per_launch=32, incidences_per_thread=4),
+    "persistent": dict(mode=abi.MODE_PERSISTENT, block=1024, substeps_per_launch=50),
+    "persistent_256": dict(mode=abi.MODE_PERSISTENT, block=256, substeps_per_launch=7),
+    "resident_8": dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=8),
+    "resident_4": dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=4, substeps_per_launch=33),
+    "resident_2": dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=2),
+    "resident_512": dict(mode=abi.MODE_RESIDENT, block=512, incidences_per_thread=4),
+}
+
+
+def gpu_world(settings, boxes, tuning):
+    w = World(settings)
+    w.statics.boxes = boxes
+    w.set_tuning(**tuning)
+    return w
+
+
+@pytest.mark.parametrize("name", sorted(CPU_CASES))
+def test_gpu_matches_oracle(name, oracle_mod):
+    print(f"case {name}", flush=True)
+    settings, boxes, bodies, n = CPU_CASES[name]()
+    cps = sorted({1, 7, n // 3, n})
+    so = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, cps)
+    sg = run_world(gpu_world(settings, boxes, TUNINGS["auto"]), bodies, n, cps)
+    assert_same(sg, so, f"gpu vs oracle [{name}]")
+
+
+@pytest.mark.parametrize("tuning", sorted(TUNINGS))
+@pytest.mark.parametrize("name", ["c1", "multi", "plastic"])
+def test_modes_identical(name, tuning, oracle_mod):
+    print(f"case {name} tuning {tuning}", flush=True)
+    settings, boxes, bodies, n = CPU_CASES[name]()
+    n = min(n, 600)
+    cps = [n // 2, n]
+    so = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, cps)
+    sg = run_world(gpu_world(settings, boxes, TUNINGS[tuning]), bodies, n, cps)
+    assert_same(sg, so, f"gpu[{tuning}] vs oracle [{name}]")
+
+
+@pytest.mark.parametrize("tuning", ["auto", "resident_8", "stepped_4"])
+def test_full_size_lattice(tuning, oracle_mod):
+    """The 64x32x24 lattice (49 152 nodes, 420 848 beams): 300 substeps, bit-identical to the oracle."""
+    from beamlab_amd import scenes
+    print(f"full-size lattice {tuning}", flush=True)
+    body = scenes.config2_body(seed=2)
+    # lowered onto the ground so that contacts, yields and breaks happen within the run
+    body.nodes["p"][:, 1] -= np.float32(0.45)
+    body.nodes["v"][:, 1] = np.float32(-3.0)
+    settings = abi.default_settings()
+    from _cases import GROUND, _boxes
+    boxes = _boxes(GROUND)
+    cps = [150, 300]
+    so = run_world(oracle_mod.OracleWorld(settings, boxes), [body], 300, cps)
+    sg = run_world(gpu_world(settings, boxes, TUNINGS[tuning]), [body], 300, cps)
+    assert_same(sg, so, f"gpu[{tuning}] vs oracle [64x32x24]")
+    assert so[-1][-1][0] > 0, "the run should break beams"
+
+
+def test_write_read_roundtrip(oracle_mod):
+    settings, boxes, bodies, n = CPU_CASES["c1"]()
+    w = gpu_world(settings, boxes, TUNINGS["auto"])
+    w.add_body(bodies[0])
+    w.step_substeps(950)
+    nodes, beams = w.read_body(0)
+    # restore into a fresh oracle world and a fresh GPU world: both continue identically
+    o = oracle_mod.OracleWorld(settings, boxes)
+    o.add_body(bodies[0])
+    o.write_body(0, nodes, beams)
+    w.write_body(0, nodes, beams)
+    o.step_substeps(300)
+    w.step_substeps(300)
+    gn, gb = w.read_body(0)
+    on, ob = o.read_body(0)
+    assert gn.tobytes() == on.tobytes() and gb.tobytes() == ob.tobytes()
+
+
+def test_golden_c1(oracle_mod):
+    """The compiled reference's own state of the config-1 lattice (tests/golden/c1_ref.npz) reproduced on the GPU."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "c1_ref.npz")
+    g = np.load(path)
+    settings, boxes, bodies, _ = CPU_CASES["c1"]()
+    w = gpu_world(settings, boxes, TUNINGS["auto"])
+    w.add_body(bodies[0])
+    done = 0
+    for k, cp in enumerate(g["checkpoints"]):
+        w.step_substeps(int(cp) - done)
+        done = int(cp)
+        nodes, beams = w.read_body(0)
+        assert nodes.tobytes() == g[f"nodes_{k}"].tobytes(), f"nodes differ from the reference at {cp}"
+        assert beams.tobytes() == g[f"beams_{k}"].tobytes(), f"beams differ from the reference at {cp}"
+        assert w.broken_beams(0) == int(g["broken"][k])
+
+
+def test_stats_counters(oracle_mod):
+    settings, boxes, bodies, n = CPU_CASES["runaway"]()
+    w = gpu_world(settings, boxes, TUNINGS["auto"])
+    o = oracle_mod.OracleWorld(settings, boxes)
+    for b in bodies:
+        w.add_body(b)
+        o.add_body(b)
+    w.step_substeps(20)
+    o.step_substeps(20)
+    sg, so = w.stats(), o.stats()
+    for key in ("vel_clamps", "nan_resets", "static_contacts", "broken_beams"):
+        assert sg[key] == so[key], (key, sg[key], so[key])
+    assert sg["vel_clamps"] > 0 and sg["nan_resets"] > 0
