@@ -36,7 +36,7 @@ SURF_GRASS, SURF_DIRT, SURF_ASPHALT, SURF_CONCRETE, SURF_GRAVEL, SURF_MUD, SURF_
 SURF_ROCK, SURF_WOOD, SURF_METAL, SURF_ICE = range(7, 11)
 
 BL_OK, BL_ERR_ARG, BL_ERR_HIP, BL_ERR_UNSUPPORTED, BL_ERR_STATE = 0, -1, -2, -3, -4
-MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT, MODE_RESIDENT = 0, 1, 2, 3
+MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT, MODE_RESIDENT, MODE_HALO = 0, 1, 2, 3, 4
 
 
 class Settings(ctypes.Structure):

@@ -11,6 +11,8 @@ from . import abi
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
 CANDIDATES = [
+    dict(mode=abi.MODE_HALO, block=256, incidences_per_thread=8, substeps_per_launch=0, use_graph=0),
+    dict(mode=abi.MODE_HALO, block=256, incidences_per_thread=4, substeps_per_launch=0, use_graph=0),
     dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=8, substeps_per_launch=0, use_graph=0),
     dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=4, substeps_per_launch=0, use_graph=0),
     dict(mode=abi.MODE_RESIDENT, block=512, incidences_per_thread=4, substeps_per_launch=0, use_graph=0),
@@ -19,6 +21,7 @@ CANDIDATES = [
     dict(mode=abi.MODE_STEPPED, block=256, incidences_per_thread=2, substeps_per_launch=32, use_graph=1),
     dict(mode=abi.MODE_STEPPED, block=256, incidences_per_thread=4, substeps_per_launch=32, use_graph=1),
     dict(mode=abi.MODE_STEPPED, block=512, incidences_per_thread=2, substeps_per_launch=64, use_graph=1),
+    dict(mode=abi.MODE_STEPPED, block=256, incidences_per_thread=8, substeps_per_launch=32, use_graph=1),
     dict(mode=abi.MODE_PERSISTENT, block=256, incidences_per_thread=4, substeps_per_launch=100, use_graph=0),
     dict(mode=abi.MODE_PERSISTENT, block=1024, incidences_per_thread=4, substeps_per_launch=100, use_graph=0),
 ]

@@ -77,6 +77,7 @@ struct bl_ctx {
     DevArr<float4> pm[2], vm[2], iA, iM, bcold;
     DevArr<uint32_t> ninfo, noff, ch_n0, body_c0, chunk_body;
     DevArr<uint4> cstat;
+    DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags;
     DevArr<float2> naux;
     DevArr<uint2> iAB;
     DevArr<float> stress;
@@ -105,6 +106,7 @@ struct bl_ctx {
         body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
+        halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
         frame_pending = false;
         built = false;
     }
@@ -126,6 +128,7 @@ bool tuning_valid(const bl_tuning& t) {
     case BL_MODE_STEPPED: return (b == 256 && (p == 1 || p == 2 || p == 4 || p == 8)) || (b == 512 && (p == 2 || p == 4));
     case BL_MODE_PERSISTENT: return (b == 256 || b == 1024) && p == 4;
     case BL_MODE_RESIDENT: return (b == 256 && (p == 2 || p == 4 || p == 8)) || (b == 512 && p == 4);
+    case BL_MODE_HALO: return b == 256 && (p == 4 || p == 8);
     default: return false;
     }
 }
@@ -237,22 +240,47 @@ int build(bl_ctx* c) {
             }
         }
     }
-    // chunks: contiguous node ranges of one body with at most CAP incidences and CAP nodes
+    // chunks: contiguous node ranges of one body with at most CAP incidences, NCAP nodes and HCAP nodes read (its
+    // own and its halo together)
+    const uint32_t HCAP = (uint32_t)halo_capacity(T, effective_per(c));
     std::vector<uint32_t> ch_n0, body_c0, chunk_body;
+    std::vector<uint32_t> mark(N, 0xffffffffu);
+    uint32_t stamp = 0, reach = 0;
+    auto count_new = [&](uint32_t node, bool commit) {
+        uint32_t add = 0;
+        auto touch = [&](uint32_t g) {
+            if (mark[g] != stamp) {
+                add++;
+                if (commit) mark[g] = stamp;
+            }
+        };
+        if (!commit) {
+            // (a dry count may see a node twice: an upper bound, enough for the cap)
+            add += mark[node] != stamp;
+            for (uint32_t e = noff[node]; e < noff[node + 1]; e++) add += (mark[iAB[e].x] != stamp) + (mark[iAB[e].y] != stamp);
+            return add;
+        }
+        touch(node);
+        for (uint32_t e = noff[node]; e < noff[node + 1]; e++) touch(iAB[e].x), touch(iAB[e].y);
+        return add;
+    };
     for (const HostBody& hb : c->bodies) {
         const uint32_t bidx = (uint32_t)body_c0.size();
         body_c0.push_back((uint32_t)ch_n0.size());
         const uint32_t n0 = hb.node0, n1 = hb.node0 + (uint32_t)hb.nodes.size();
         uint32_t start = n0;
+        stamp++, reach = 0;
         for (uint32_t i = n0; i < n1; i++) {
-            if (deg[i] > CAP) {
-                g_last_error = "a node has more incidences than a workgroup holds";
+            if (deg[i] > CAP || 1 + 2 * deg[i] > HCAP) {
+                g_last_error = "a node has more beams than a workgroup holds";
                 return BL_ERR_UNSUPPORTED;
             }
-            if (noff[i + 1] - noff[start] > CAP || i + 1 - start > NCAP) {
+            if (i > start && (noff[i + 1] - noff[start] > CAP || i + 1 - start > NCAP || reach + count_new(i, false) > HCAP)) {
                 ch_n0.push_back(start), chunk_body.push_back(bidx);
                 start = i;
+                stamp++, reach = 0;
             }
+            reach += count_new(i, true);
         }
         if (n1 > start) ch_n0.push_back(start), chunk_body.push_back(bidx);
     }
@@ -260,6 +288,38 @@ int build(bl_ctx* c) {
     const uint32_t nchunks = (uint32_t)ch_n0.size();
     ch_n0.push_back(N);
     c->h_noff = noff;
+    // halos: per chunk the outside nodes its incidences read (sorted), the chunks owning them, each incidence's
+    // two LDS slots
+    std::vector<uint32_t> node_chunk(N);
+    for (uint32_t k = 0; k < nchunks; k++)
+        for (uint32_t i = ch_n0[k]; i < ch_n0[k + 1]; i++) node_chunk[i] = k;
+    std::vector<uint32_t> halo_off(nchunks + 1, 0), halo, nbr_off(nchunks + 1, 0), nbr, iloc(E);
+    std::vector<uint32_t> ext;
+    std::vector<uint32_t> slot(N, 0);
+    for (uint32_t k = 0; k < nchunks; k++) {
+        const uint32_t a0 = ch_n0[k], a1 = ch_n0[k + 1];
+        ext.clear();
+        for (uint32_t e = noff[a0]; e < noff[a1]; e++)
+            for (uint32_t g : {iAB[e].x, iAB[e].y})
+                if (g < a0 || g >= a1) ext.push_back(g);
+        std::sort(ext.begin(), ext.end());
+        ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+        if ((a1 - a0) + ext.size() > HCAP) {
+            g_last_error = "chunk halo over capacity";
+            return BL_ERR_UNSUPPORTED;
+        }
+        for (uint32_t i = a0; i < a1; i++) slot[i] = i - a0;
+        for (size_t j = 0; j < ext.size(); j++) slot[ext[j]] = (a1 - a0) + (uint32_t)j;
+        for (uint32_t e = noff[a0]; e < noff[a1]; e++) iloc[e] = slot[iAB[e].x] | (slot[iAB[e].y] << 16);
+        halo.insert(halo.end(), ext.begin(), ext.end());
+        halo_off[k + 1] = (uint32_t)halo.size();
+        std::vector<uint32_t> nb;
+        for (uint32_t g : ext) nb.push_back(node_chunk[g]);
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        nbr.insert(nbr.end(), nb.begin(), nb.end());
+        nbr_off[k + 1] = (uint32_t)nbr.size();
+    }
 
     int rc;
     if ((rc = c->pm[0].upload(P)) || (rc = c->vm[0].upload(V)) || (rc = c->pm[1].alloc(N)) || (rc = c->vm[1].alloc(N)) ||
@@ -270,7 +330,9 @@ int build(bl_ctx* c) {
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
         (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)) || (rc = c->bar.alloc(1)) ||
         (rc = c->err.alloc(1)) || (rc = c->chunk_body.upload(chunk_body)) ||
-        (rc = c->cstat.alloc(std::max<size_t>(1, nchunks))))
+        (rc = c->cstat.alloc(std::max<size_t>(1, nchunks))) || (rc = c->halo_off.upload(halo_off)) ||
+        (rc = c->halo.upload(halo)) || (rc = c->nbr_off.upload(nbr_off)) || (rc = c->nbr.upload(nbr)) ||
+        (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))))
         return rc;
     HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
     HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
@@ -284,6 +346,7 @@ int build(bl_ctx* c) {
     a.ch_n0 = c->ch_n0.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
     a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;
+    a.halo_off = c->halo_off.p, a.halo = c->halo.p, a.nbr_off = c->nbr_off.p, a.nbr = c->nbr.p, a.iloc = c->iloc.p;
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
@@ -307,6 +370,8 @@ int ensure_built(bl_ctx* c) {
 int resolved_mode(const bl_ctx* c) {
     if (c->tune.mode != BL_MODE_AUTO) return c->tune.mode;
     const int T = effective_block(c), per = effective_per(c);
+    bl_tuning h{BL_MODE_HALO, T, 0, 0, per};
+    if (tuning_valid(h) && halo_fits(T, per, c->args.nchunks)) return BL_MODE_HALO;
     bl_tuning t{BL_MODE_RESIDENT, T, 0, 0, per};
     if (tuning_valid(t) && resident_fits(T, per, c->args.nchunks)) return BL_MODE_RESIDENT;
     return BL_MODE_STEPPED;
@@ -323,7 +388,21 @@ int enqueue(bl_ctx* c, int substeps) {
         return BL_ERR_STATE;
     }
     int rc;
-    if (t.mode == BL_MODE_RESIDENT) {
+    if (t.mode == BL_MODE_HALO) {
+        if (!halo_fits(t.block, t.incidences_per_thread, c->args.nchunks)) {
+            g_last_error = "the halo form needs every chunk's workgroup resident at once: too many chunks";
+            return BL_ERR_UNSUPPORTED;
+        }
+        const int per = t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps;
+        for (int s = 0; s < substeps; s += per) {
+            const int k = std::min(per, substeps - s);
+            if ((rc = launch_halo(c->args, c->stream, c->parity, k, t, c->flags.p, c->err.p, kSpinLimit))) {
+                g_last_error = "cooperative launch of the halo form failed";
+                return rc;
+            }
+            c->parity ^= (k & 1);
+        }
+    } else if (t.mode == BL_MODE_RESIDENT) {
         if (!resident_fits(t.block, t.incidences_per_thread, c->args.nchunks)) {
             g_last_error = "the resident form needs every chunk's workgroup resident at once: too many chunks";
             return BL_ERR_UNSUPPORTED;
@@ -866,7 +945,8 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
     t.incidences_per_thread = effective_per(c);
     if (t.block != c->built_block || t.incidences_per_thread != c->built_per) return BL_ERR_STATE;
     const int per = t.mode == BL_MODE_PERSISTENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : 64)
-                    : t.mode == BL_MODE_RESIDENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps) : 1;
+                    : (t.mode == BL_MODE_RESIDENT || t.mode == BL_MODE_HALO)
+                        ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps) : 1;
     const int nl = (substeps + per - 1) / per;
     while ((int)c->kev.size() < 2 * nl) {
         hipEvent_t e;
@@ -879,6 +959,7 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
         const int k = std::min(per, substeps - l * per);
         HIPCHK(hipEventRecord(c->kev[2 * l], c->stream));
         if (t.mode == BL_MODE_RESIDENT) rc = launch_resident(c->args, c->stream, c->parity, k, t, c->bar.p, c->err.p, kSpinLimit);
+        else if (t.mode == BL_MODE_HALO) rc = launch_halo(c->args, c->stream, c->parity, k, t, c->flags.p, c->err.p, kSpinLimit);
         else rc = launch_substeps(c->args, c->stream, c->parity, k, t);
         if (rc) return rc;
         HIPCHK(hipEventRecord(c->kev[2 * l + 1], c->stream));

@@ -440,6 +440,130 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)
     flush_chunk<T>(a, cnt, c, sred);
 }
 
+
+// ---------------------------------------------------------------------------------------------- halo form
+// Device-coherent 16-byte load / store (two 64-bit agent-scope accesses): they bypass the XCD-private caches, so a
+// node written by a workgroup on another XCD is seen without cache maintenance.
+__device__ __forceinline__ float4 ld_dev(const float4* p) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(const_cast<float4*>(p));
+    const uint64_t lo = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t hi = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(u2f((uint32_t)lo), u2f((uint32_t)(lo >> 32)), u2f((uint32_t)hi), u2f((uint32_t)(hi >> 32)));
+}
+__device__ __forceinline__ void st_dev(float4* p, float4 v) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    __hip_atomic_store(q, (uint64_t)f2u(v.x) | ((uint64_t)f2u(v.y) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (uint64_t)f2u(v.z) | ((uint64_t)f2u(v.w) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Many substeps per launch; each workgroup owns one chunk, keeps its incidences' state in registers and its nodes
+// in LDS, and per substep: waits until the chunks it reads from (and that read from it) have finished the previous
+// substep (their progress words), copies its halo - the outside nodes its beams reach - into LDS, evaluates its
+// incidences from LDS, integrates its nodes, publishes them and its progress. No grid-wide barrier; every wait is
+// bounded (spin_limit polls, or another workgroup's error) so the grid always drains.
+template <int T, int PER, int HCAP, int WAVES>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void k_halo(
+    KArgs a, int parity0, int substeps, unsigned int* flags, int* err, uint32_t spin_limit) {
+    __shared__ float4 hp[HCAP], hv[HCAP];
+    __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
+    __shared__ uint8_t sok[T * PER];
+    __shared__ uint4 sred[T / 64];
+    __shared__ int sbad;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1], nown = n1 - n0;
+    const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
+    const uint32_t h0 = a.halo_off[c], next = a.halo_off[c + 1] - h0;
+    const uint32_t q0 = a.nbr_off[c], nnbr = a.nbr_off[c + 1] - q0;
+    const uint32_t body = a.chunk_body[c];
+    if (tid == 0) sbad = 0;
+    float4 A[PER], M[PER];
+    uint32_t loc[PER];
+    float stress[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + tid + k * T;
+        A[k] = make_float4(0, 0, 0, 0), M[k] = make_float4(0, 0, 0, u2f(TF_SKIP)), loc[k] = 0, stress[k] = 0;
+        if (e < e1) {
+            A[k] = a.iA[e], M[k] = a.iM[e], loc[k] = a.iloc[e];
+            stress[k] = (f2u(M[k].w) & TF_ISB) ? 0.0f : a.stress[f2u(M[k].z)];
+        }
+    }
+    const uint32_t i = n0 + tid;
+    const bool has_node = tid < nown;
+    float4 P = make_float4(0, 0, 0, 0), V = make_float4(0, 0, 0, 0);
+    uint32_t info = 0, b0 = 0, b1 = 0;
+    float2 aux = make_float2(0, 0);
+    if (has_node) {
+        P = a.pm[parity0][i], V = a.vm[parity0][i];
+        info = a.ninfo[i], aux = a.naux[i];
+        b0 = a.noff[i] - e0, b1 = a.noff[i + 1] - e0;
+        hp[tid] = P, hv[tid] = V;
+    }
+    NodeCounts cnt = {0, 0, 0, 0.0f};
+    for (int s = 0; s < substeps; s++) {
+        const int parity = parity0 ^ (s & 1);
+        const float4* __restrict__ pin = a.pm[parity];
+        const float4* __restrict__ vin = a.vm[parity];
+        if (s > 0) {
+            for (uint32_t j = tid; j < nnbr; j += T) {
+                const uint32_t nb = a.nbr[q0 + j];
+                uint32_t spins = 0;
+                while (__hip_atomic_load(&flags[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned int)s) {
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > spin_limit) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        sbad = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __syncthreads(); // (the halo is read only after every neighbour has published)
+        for (uint32_t j = tid; j < next; j += T) {
+            const uint32_t g = a.halo[h0 + j];
+            hp[nown + j] = ld_dev(&pin[g]), hv[nown + j] = ld_dev(&vin[g]);
+        }
+        __syncthreads();
+        if (sbad) break;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const uint32_t e = e0 + tid + k * T;
+            if (e >= e1) continue;
+            const uint32_t la = loc[k] & 0xffffu, lb = loc[k] >> 16;
+            f3 fv = mk(0, 0, 0);
+            float slen = 0;
+            uint32_t ev;
+            const bool ok = beam_eval(A[k], M[k], hp[la], hp[lb], hv[la], hv[lb], a.bcold, fv, slen, ev);
+            if ((ev & EV_BROKE) && !(f2u(M[k].w) & TF_ISB)) atomicAdd(&a.broken[body], 1ull);
+            if (ok) stress[k] = slen;
+            const uint32_t sl = tid + k * T;
+            sfx[sl] = fv.x, sfy[sl] = fv.y, sfz[sl] = fv.z;
+            sok[sl] = ok;
+        }
+        __syncthreads();
+        if (has_node) {
+            const f3 f = node_force(a, V.w, b0, b1, sfx, sfy, sfz, sok);
+            node_finish(a, info, aux, P, V, f, cnt);
+            hp[tid] = P, hv[tid] = V;
+            st_dev(&a.pm[parity ^ 1][i], P);
+            st_dev(&a.vm[parity ^ 1][i], V);
+        }
+        __builtin_amdgcn_s_waitcnt(0); // the nodes' device-scope stores acknowledged before the progress word
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&flags[c], (unsigned int)(s + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = e0 + tid + k * T;
+        if (e < e1) {
+            a.iA[e] = A[k], a.iM[e] = M[k];
+            if (!(f2u(M[k].w) & TF_ISB)) a.stress[f2u(M[k].z)] = stress[k];
+        }
+    }
+    flush_chunk<T>(a, cnt, c, sred);
+}
+
 // ---------------------------------------------------------------------------------------------- frame kernels
 // The workgroup's nodes all belong to one body when its first and last do (bodies are contiguous): then a
 // workgroup reduction and one atomic per word, else one atomic per node (small bodies: little contention).
@@ -646,6 +770,41 @@ int launch_frame_end(const KArgs& a, hipStream_t st, int parity) {
 int launch_chunk_clear(const KArgs& a, hipStream_t st) {
     if (a.nchunks) hipLaunchKernelGGL(k_chunk_clear, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
     return BL_OK_OR_HIP();
+}
+
+
+template <int T, int PER, int HCAP, int WAVES>
+static int launch_halo_t(const KArgs& a, hipStream_t st, int parity, int substeps, unsigned int* flags, int* err,
+                         uint32_t spin_limit) {
+    hipLaunchKernelGGL(k_clear_words, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, flags, a.nchunks);
+    KArgs aa = a;
+    int p = parity, s = substeps;
+    void* args[] = {&aa, &p, &s, &flags, &err, &spin_limit};
+    if (hipLaunchCooperativeKernel((const void*)k_halo<T, PER, HCAP, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st) !=
+        hipSuccess)
+        return BL_ERR_HIP;
+    return BL_OK_OR_HIP();
+}
+
+int halo_capacity(int block, int per) { return block == 256 && per == 8 ? 1024 : 512; }
+
+int halo_fits(int block, int per, uint32_t nchunks) {
+    int dev = 0, cus = 0, nbpm = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (block == 256 && per == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_halo<256, 8, 1024, 2>, 256, 0);
+    else if (block == 256 && per == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpm, k_halo<256, 4, 512, 4>, 256, 0);
+    if (e != hipSuccess) return 0;
+    return (uint64_t)nbpm * (uint64_t)cus >= nchunks ? 1 : 0;
+}
+
+int launch_halo(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* flags,
+                int* err, uint32_t spin_limit) {
+    const int per = t.incidences_per_thread;
+    if (t.block == 256 && per == 8) return launch_halo_t<256, 8, 1024, 2>(a, st, parity, substeps, flags, err, spin_limit);
+    if (t.block == 256 && per == 4) return launch_halo_t<256, 4, 512, 4>(a, st, parity, substeps, flags, err, spin_limit);
+    return BL_ERR_ARG;
 }
 
 // Incidences of a chunk: block x incidences per thread (the node cap of a chunk is the block).

@@ -55,6 +55,14 @@ struct KArgs {
     unsigned int* body_frame;     // per body, BF_N words: top |v|^2 of the frame (float bits), bounds keys
     uint4* cstat;                 // per chunk: static contacts, clamps, NaN resets, top |v|^2 (float bits) of the frame
     const uint32_t* chunk_body;   // per chunk: its body
+    // the halo form: per chunk, the nodes outside it that its incidences read (halo[halo_off[c] ..]), the chunks
+    // owning them (nbr[nbr_off[c] ..]), and per incidence the LDS slots of its two nodes (a | b << 16; the chunk's
+    // own nodes first, then its halo)
+    const uint32_t* halo_off;
+    const uint32_t* halo;
+    const uint32_t* nbr_off;
+    const uint32_t* nbr;
+    const uint32_t* iloc;
     f3 g;
     float dt;
     float kill_y;
@@ -69,6 +77,10 @@ int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, co
 int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,
                     int* err, uint32_t spin_limit);
 int resident_fits(int block, int per, uint32_t nchunks);
+int launch_halo(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* flags,
+                int* err, uint32_t spin_limit);
+int halo_fits(int block, int per, uint32_t nchunks);
+int halo_capacity(int block, int per);
 int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out);
 int launch_frame_begin(const KArgs& a, hipStream_t st, int parity);
 int launch_frame_end(const KArgs& a, hipStream_t st, int parity);

@@ -130,12 +130,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "64x32x24 beam lattice on a ground box (no self or sphere contacts)",
                    "nodes": nn, "beams": nb, "substep_s": 0.0005, "substeps": args.steps,
-                   "mode": {1: "stepped", 2: "persistent", 3: "resident"}.get(tun.mode, str(tun.mode)),
+                   "mode": {1: "stepped", 2: "persistent", 3: "resident", 4: "halo"}.get(tun.mode, str(tun.mode)),
                    "block": tun.block, "incidences_per_thread": tun.incidences_per_thread,
                    "substeps_per_launch": tun.substeps_per_launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": {1: "k_substep", 2: "k_body_persistent", 3: "k_resident"}.get(tun.mode, "?"),
+                     "kernel": {1: "k_substep", 2: "k_body_persistent", 3: "k_resident", 4: "k_halo"}.get(tun.mode, "?"),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_launch_s * 1e6},
     }
     if not args.no_cpu:

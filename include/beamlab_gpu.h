@@ -212,6 +212,7 @@ enum {
     BL_MODE_STEPPED = 1,    /* one launch per substep, one workgroup per chunk */
     BL_MODE_PERSISTENT = 2, /* one workgroup per body, many substeps per launch */
     BL_MODE_RESIDENT = 3,   /* one workgroup per chunk, state in registers, many substeps per launch (cooperative) */
+    BL_MODE_HALO = 4,       /* as RESIDENT, each chunk's neighbour nodes staged in LDS, neighbour-only waits */
 };
 
 /* ---------------------------------------------------------------------------------------------------------------

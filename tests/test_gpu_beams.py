@@ -23,6 +23,8 @@ TUNINGS = {
     "resident_4": dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=4, substeps_per_launch=33),
     "resident_2": dict(mode=abi.MODE_RESIDENT, block=256, incidences_per_thread=2),
     "resident_512": dict(mode=abi.MODE_RESIDENT, block=512, incidences_per_thread=4),
+    "halo_8": dict(mode=abi.MODE_HALO, block=256, incidences_per_thread=8),
+    "halo_4": dict(mode=abi.MODE_HALO, block=256, incidences_per_thread=4, substeps_per_launch=33),
 }
 
 
@@ -55,7 +57,7 @@ def test_modes_identical(name, tuning, oracle_mod):
     assert_same(sg, so, f"gpu[{tuning}] vs oracle [{name}]")
 
 
-@pytest.mark.parametrize("tuning", ["auto", "resident_8", "stepped_4"])
+@pytest.mark.parametrize("tuning", ["auto", "halo_8", "halo_4", "resident_8", "stepped_4"])
 def test_full_size_lattice(tuning, oracle_mod):
     """The 64x32x24 lattice (49 152 nodes, 420 848 beams): 300 substeps, bit-identical to the oracle."""
     from beamlab_amd import scenes
