@@ -156,6 +156,31 @@ def box_faces(nx, ny, nz):
     return t
 
 
+def ball_body(center, radius, mass, friction=1.0, bounce=-1.0, name="ball"):
+    """A ball (build_ball, src/world/objects.cpp:1213-1232): one node, a capsule of no length, the node's static
+    contact radius the ball's."""
+    b = SoftBody(name)
+    nodes = np.zeros(1, abi.NODE)
+    nodes["p"][0] = np.asarray(center, np.float32)
+    nodes["mass"] = np.float32(mass)
+    nodes["inv_mass"] = np.float32(1.0) / np.float32(mass)
+    info = np.zeros(1, abi.NODE_INFO)
+    info["flags"] = abi.NF_GROUND | abi.NF_CONTACTER
+    info["friction"] = 1.0
+    info["radius"] = np.float32(radius)
+    info["frame"] = -1
+    info["ror_id"] = -1
+    b.nodes, b.info = nodes, info
+    caps = np.zeros(1, abi.CAPSULE)
+    caps["a"], caps["b"], caps["radius"], caps["joint"] = 0, 0, np.float32(radius), -1
+    b.capsules = caps
+    b.opts.collision_radius = np.float32(0.01)
+    b.opts.sphere_ball = np.float32(radius)
+    b.opts.ground_friction = np.float32(friction)
+    b.opts.bounce = np.float32(bounce)
+    return b
+
+
 def ground_box(world_or_statics, half=100.0):
     """A concrete ground slab with its top at y = 0."""
     st = world_or_statics.statics if hasattr(world_or_statics, "statics") else world_or_statics

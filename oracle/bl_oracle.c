@@ -110,6 +110,10 @@ typedef struct {
     float max_speed;
     v3 mn, mx;
     int64_t repairs;
+    bl_triangle* tris;
+    uint32_t ntri;
+    bl_capsule* caps;
+    uint32_t ncap;
 } orc_body;
 
 struct orc_world {
@@ -119,6 +123,7 @@ struct orc_world {
     orc_body* bodies;
     int nbodies;
     bl_stats st;
+    int unsupported; /* a feature met that the restatement leaves out (fast bodies' pairs) */
 };
 
 orc_world* orc_create(const bl_settings* s) {
@@ -134,6 +139,8 @@ void orc_destroy(orc_world* w) {
         free(w->bodies[i].info);
         free(w->bodies[i].beams);
         free(w->bodies[i].force);
+        free(w->bodies[i].tris);
+        free(w->bodies[i].caps);
     }
     free(w->bodies);
     free(w->boxes);
@@ -160,6 +167,14 @@ int orc_add_body(orc_world* w, const bl_body_desc* d) {
     if (b->n) memcpy(b->nodes, d->nodes, sizeof(bl_node) * b->n), memcpy(b->info, d->info, sizeof(bl_node_info) * b->n);
     if (b->nb) memcpy(b->beams, d->beams, sizeof(bl_beam) * b->nb);
     b->opts = d->opts;
+    b->ntri = d->n_tris, b->ncap = d->n_capsules;
+    b->tris = (bl_triangle*)malloc(sizeof(bl_triangle) * (b->ntri ? b->ntri : 1));
+    b->caps = (bl_capsule*)malloc(sizeof(bl_capsule) * (b->ncap ? b->ncap : 1));
+    if (b->ntri) memcpy(b->tris, d->tris, sizeof(bl_triangle) * b->ntri);
+    if (b->ncap) memcpy(b->caps, d->capsules, sizeof(bl_capsule) * b->ncap);
+    /* SoftBody::compute_aabb at World::add_body (src/phys/world.cpp:283-289, src/phys/softbody.cpp:271-275) */
+    b->mn = V(1e30f, 1e30f, 1e30f), b->mx = V(-1e30f, -1e30f, -1e30f);
+    for (uint32_t i = 0; i < b->n; i++) b->mn = vmin3(b->mn, ld3(b->nodes[i].p)), b->mx = vmax3(b->mx, ld3(b->nodes[i].p));
     return w->nbodies++;
 }
 
@@ -390,6 +405,13 @@ static int repair_nodes(orc_body* B) {
     for (uint32_t i = 0; i < n; i++)
         if (bad[i]) st3(B->nodes[i].p, med), st3(B->nodes[i].v, V(0, 0, 0));
     free(sp), free(sv), free(cnt), free(bad);
+    /* bounds and top speed of the repaired body (src/phys/world.cpp:4106-4109) */
+    B->mn = V(1e30f, 1e30f, 1e30f), B->mx = V(-1e30f, -1e30f, -1e30f);
+    B->max_speed = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        B->mn = vmin3(B->mn, ld3(B->nodes[i].p)), B->mx = vmax3(B->mx, ld3(B->nodes[i].p));
+        B->max_speed = smax(B->max_speed, vlen(ld3(B->nodes[i].v)));
+    }
     return reset;
 }
 
@@ -407,41 +429,768 @@ static void post_frame(orc_world* w, orc_body* B) {
                 x->v[0] = x->v[1] = x->v[2] = 0;
                 x->inv_mass = 0;
             }
+            B->mn = V(1e30f, 1e30f, 1e30f), B->mx = V(-1e30f, -1e30f, -1e30f);
+            for (uint32_t i = 0; i < B->n; i++) B->mn = vmin3(B->mn, ld3(B->nodes[i].p)), B->mx = vmax3(B->mx, ld3(B->nodes[i].p));
         }
         B->repairs += 1;
     }
 }
 
+/* ------------------------------------------------------------------ inter-body contacts (slow bodies) */
+typedef struct { uint16_t bn, bt; uint32_t node, tri; } orc_nt;
+typedef struct { uint16_t bn, bc; uint32_t node, cap; } orc_nc;
+typedef struct { uint16_t bc, bt; uint32_t cap, tri; } orc_ct;
+typedef struct { uint32_t pair; float s; v3 bary, nrm; float pen; } orc_hit;
+typedef struct { uint16_t body; uint32_t node; int x, y, z; v3 p; } orc_pt;
+typedef struct { uint32_t key, idx; } orc_entry;
+
+#define VEC_PUSH(arr, n, cap, val)                                                                                     \
+    do {                                                                                                               \
+        if ((n) == (cap)) {                                                                                            \
+            (cap) = (cap) ? 2 * (cap) : 64;                                                                            \
+            (arr) = realloc((arr), sizeof(*(arr)) * (cap));                                                            \
+        }                                                                                                              \
+        (arr)[(n)++] = (val);                                                                                          \
+    } while (0)
+
+typedef struct {
+    int* bodies;  /* world indices, in body order */
+    int nb;
+    int needs_pairs;
+    int interval, next_rebuild;
+    float margin, inv_cell;
+    orc_nt* nt; size_t nnt, cnt;
+    orc_nc* nc; size_t nnc, cnc;
+    orc_ct* ct; size_t nct, cct;
+    orc_hit* hits; size_t nhit, chit;
+} orc_island;
+
+/* cell_hash (src/phys/world.cpp:23-25): int products wrap as uint32 */
+static inline uint32_t cell_hash(int x, int y, int z) {
+    return ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+}
+
+static inline float max_edge2(v3 a, v3 b, v3 c) {
+    v3 ba = vsub(b, a), cb = vsub(c, b), ac = vsub(a, c);
+    return smax(vdot(ba, ba), smax(vdot(cb, cb), vdot(ac, ac)));
+}
+static inline v3 vcross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline v3 vnormalize(v3 a) { float l = vlen(a); return l > 1e-12f ? vmul(a, 1.0f / l) : V(0, 0, 0); }
+static inline float clampf01(float v) { return clampf_(v, 0, 1); }
+
+/* closest_on_triangle (src/phys/softbody.h:158-190, Ericson 5.1.5) */
+static v3 closest_on_triangle(v3 p, v3 a, v3 b, v3 c, v3* bary) {
+    v3 ab = vsub(b, a), ac = vsub(c, a), ap = vsub(p, a);
+    float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { *bary = V(1, 0, 0); return a; }
+    v3 bp = vsub(p, b);
+    float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { *bary = V(0, 1, 0); return b; }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        float v = d1 / (d1 - d3);
+        *bary = V(1 - v, v, 0);
+        return vadd(a, vmul(ab, v));
+    }
+    v3 cp = vsub(p, c);
+    float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { *bary = V(0, 0, 1); return c; }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        float ww = d2 / (d2 - d6);
+        *bary = V(1 - ww, 0, ww);
+        return vadd(a, vmul(ac, ww));
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        float ww = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *bary = V(0, 1 - ww, ww);
+        return vadd(b, vmul(vsub(c, b), ww));
+    }
+    float denom = 1.0f / (va + vb + vc);
+    float v = vb * denom, ww = vc * denom;
+    *bary = V(1 - v - ww, v, ww);
+    return vadd(vadd(a, vmul(ab, v)), vmul(ac, ww));
+}
+
+/* closest_on_segment, closest_seg_seg (src/phys/world.cpp:27-47) */
+static float closest_on_segment(v3 p, v3 a, v3 b) {
+    v3 ab = vsub(b, a);
+    float l2 = vdot(ab, ab);
+    if (l2 < 1e-12f) return 0;
+    return clampf01(vdot(vsub(p, a), ab) / l2);
+}
+static void closest_seg_seg(v3 p1, v3 q1, v3 p2, v3 q2, float* s, float* t) {
+    v3 d1 = vsub(q1, p1), d2 = vsub(q2, p2), r = vsub(p1, p2);
+    float a = vdot(d1, d1), e = vdot(d2, d2), f = vdot(d2, r);
+    if (a <= 1e-12f && e <= 1e-12f) { *s = *t = 0; return; }
+    if (a <= 1e-12f) { *s = 0; *t = clampf01(f / e); return; }
+    float c = vdot(d1, r);
+    if (e <= 1e-12f) { *t = 0; *s = clampf01(-c / a); return; }
+    float b = vdot(d1, d2), denom = a * e - b * b;
+    *s = denom != 0 ? clampf01((b * f - c * e) / denom) : 0;
+    *t = (b * *s + f) / e;
+    if (*t < 0) { *t = 0; *s = clampf01(-c / a); }
+    else if (*t > 1) { *t = 1; *s = clampf01((b - c) / a); }
+}
+
+/* near_triangle (src/phys/world.cpp:1135-1142), two-sided triangles */
+static int near_triangle(v3 p, v3 a, v3 b, v3 c, v3 tu, float r) {
+    const float s = vdot(vsub(p, a), tu);
+    v3 bary;
+    v3 q = closest_on_triangle(p, a, b, c, &bary);
+    v3 d = vsub(q, p);
+    return fabsf(s) <= r && vdot(d, d) <= r * r;
+}
+
+static inline int cell_of(float v, float inv_cell) { return (int)floorf(v * inv_cell); }
+
+static int has_contacters(const orc_body* B) {
+    for (uint32_t i = 0; i < B->n; i++)
+        if (B->info[i].flags & BL_NF_CONTACTER) return 1;
+    return 0;
+}
+
+typedef struct { v3 mn, mx; } aabb3;
+static inline int aabb_overlaps(aabb3 a, aabb3 b) {
+    return a.mn.x <= b.mx.x && a.mx.x >= b.mn.x && a.mn.y <= b.mx.y && a.mx.y >= b.mn.y && a.mn.z <= b.mx.z && a.mx.z >= b.mn.z;
+}
+static inline aabb3 aabb_expand(aabb3 a, float r) {
+    a.mn = vsub(a.mn, V(r, r, r)), a.mx = vadd(a.mx, V(r, r, r));
+    return a;
+}
+
+static int uf_find(int* parent, int x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+}
+
+/* World::build_islands (src/phys/world.cpp:636-762), for awake bodies of node/beam type: union-find over the
+ * expanded bounds of bodies that can touch (contacters against triangles or capsules, capsules against triangles). */
+static int build_islands(orc_world* w, float frame_time, orc_island** out) {
+    const int nb = w->nbodies;
+    int* parent = malloc(sizeof(int) * (nb ? nb : 1));
+    aabb3* boxes = malloc(sizeof(aabb3) * (nb ? nb : 1));
+    int* hc = malloc(sizeof(int) * (nb ? nb : 1));
+    for (int i = 0; i < nb; i++) {
+        orc_body* B = &w->bodies[i];
+        parent[i] = i;
+        aabb3 a = {B->mn, B->mx};
+        float m = smin(B->max_speed, 200.0f) * frame_time * 2.0f + 0.3f;
+        boxes[i] = aabb_expand(a, m + 0.0f /* hull_reach: two-sided triangles only */);
+        hc[i] = has_contacters(B);
+    }
+    if (w->s.inter_body_collisions)
+        for (int i = 0; i < nb; i++)
+            for (int j = i + 1; j < nb; j++) {
+                if (!aabb_overlaps(boxes[i], boxes[j])) continue;
+                orc_body *bi = &w->bodies[i], *bj = &w->bodies[j];
+                if (bi->opts.collision_group != 0 && bi->opts.collision_group == bj->opts.collision_group) continue;
+                int ti = bi->ntri > 0, tj = bj->ntri > 0, ci = bi->ncap > 0, cj = bj->ncap > 0;
+                int can = (hc[i] && (tj || cj)) || (hc[j] && (ti || ci)) || (ci && tj) || (cj && ti);
+                if (!can) continue;
+                int a = uf_find(parent, i), b = uf_find(parent, j);
+                if (a != b) parent[a] = b;
+            }
+    int* island_of = malloc(sizeof(int) * (nb ? nb : 1));
+    for (int i = 0; i < nb; i++) island_of[i] = -1;
+    orc_island* isl = calloc(nb ? nb : 1, sizeof(orc_island));
+    int nisl = 0;
+    for (int i = 0; i < nb; i++) {
+        int r = uf_find(parent, i);
+        if (island_of[r] < 0) island_of[r] = nisl++;
+        orc_island* I = &isl[island_of[r]];
+        I->bodies = realloc(I->bodies, sizeof(int) * (I->nb + 1));
+        I->bodies[I->nb++] = i;
+    }
+    for (int k = 0; k < nisl; k++)
+        isl[k].needs_pairs = w->s.inter_body_collisions && (isl[k].nb > 1 || w->bodies[isl[k].bodies[0]].opts.self_collision);
+    free(parent), free(boxes), free(hc), free(island_of);
+    *out = isl;
+    return nisl;
+}
+
+static void free_island(orc_island* I) {
+    free(I->bodies), free(I->nt), free(I->nc), free(I->ct), free(I->hits);
+}
+
+typedef struct {
+    orc_world* w;
+    orc_island* I;
+    aabb3* boxes;
+    int** partners;
+    int* npart;
+} bp_ctx;
+
+static int near_partner(const bp_ctx* c, int bi, v3 mn, v3 mx) {
+    orc_body* B = &c->w->bodies[c->I->bodies[bi]];
+    if (B->opts.self_collision) return 1;
+    for (int k = 0; k < c->npart[bi]; k++) {
+        aabb3 q = c->boxes[c->partners[bi][k]];
+        if (mn.x <= q.mx.x && mx.x >= q.mn.x && mn.y <= q.mx.y && mx.y >= q.mn.y && mn.z <= q.mx.z && mx.z >= q.mn.z) return 1;
+    }
+    return 0;
+}
+
+static int same_group(orc_world* w, orc_island* I, int a, int b) {
+    int ga = w->bodies[I->bodies[a]].opts.collision_group, gb = w->bodies[I->bodies[b]].opts.collision_group;
+    return ga != 0 && ga == gb;
+}
+
+/* build_buckets of rebuild_pairs (src/phys/world.cpp:1985-1995): power-of-two buckets, stable counting fill */
+static uint32_t build_buckets(const orc_entry* ent, size_t n, uint32_t** bstart, uint32_t** bitems) {
+    uint32_t nbk = 64;
+    while (nbk < n * 2) nbk <<= 1;
+    uint32_t mask = nbk - 1;
+    uint32_t* st = calloc(nbk + 1, sizeof(uint32_t));
+    for (size_t e = 0; e < n; e++) st[(ent[e].key & mask) + 1]++;
+    for (uint32_t i = 0; i < nbk; i++) st[i + 1] += st[i];
+    uint32_t* items = malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* fill = malloc(sizeof(uint32_t) * nbk);
+    memcpy(fill, st, sizeof(uint32_t) * nbk);
+    for (size_t e = 0; e < n; e++) items[fill[ent[e].key & mask]++] = ent[e].idx;
+    free(fill);
+    *bstart = st, *bitems = items;
+    return mask;
+}
+
+static int cmp_ct(const void* x, const void* y) {
+    const orc_ct *a = x, *b = y;
+    if (a->bt != b->bt) return a->bt < b->bt ? -1 : 1;
+    return a->tri < b->tri ? -1 : a->tri > b->tri ? 1 : 0;
+}
+
+/* World::rebuild_pairs (src/phys/world.cpp:1851-2308) without fast bodies, hull triangles and FEM mid points */
+static void rebuild_pairs(orc_world* w, orc_island* I) {
+    w->st.pair_rebuilds++;
+    I->nnt = I->nnc = I->nct = 0;
+    const int nb = I->nb;
+    float rmax = 0;
+    int any_caps = 0, any_tris = 0;
+    for (int k = 0; k < nb; k++) {
+        orc_body* B = &w->bodies[I->bodies[k]];
+        rmax = smax(rmax, B->opts.collision_radius);
+        any_caps |= B->ncap > 0;
+        any_tris |= B->ntri > 0;
+        for (uint32_t t = 0; t < B->ntri; t++) {
+            bl_triangle* tr = &B->tris[t];
+            if (!tr->torn && tr->rest_edge2 > 0 &&
+                max_edge2(ld3(B->nodes[tr->a].p), ld3(B->nodes[tr->b].p), ld3(B->nodes[tr->c].p)) > 2.5f * 2.5f * tr->rest_edge2)
+                tr->torn = 1;
+        }
+    }
+    float slow_speed = 0;
+    for (int k = 0; k < nb; k++) {
+        orc_body* B = &w->bodies[I->bodies[k]];
+        if (B->max_speed <= 8.0f) slow_speed = smax(slow_speed, B->max_speed);
+        else w->unsupported = 1; /* a fast body: its own pair schedule (fast_pairs) is not restated */
+    }
+    const float speed = slow_speed + 0.5f;
+    const double hz = (double)smax(1.0f, w->s.collision_hz);
+    int interval = (int)ceil(1.0 / (hz * (double)w->s.dt) - 1e-6);
+    if (interval < 1) interval = 1;
+    I->interval = interval;
+    const float margin = 2.0f * speed * (float)interval * w->s.dt + 0.02f;
+    I->margin = margin;
+    {
+        double tri_size = 0;
+        int samples = 0;
+        for (int k = 0; k < nb; k++) {
+            orc_body* B = &w->bodies[I->bodies[k]];
+            for (uint32_t t = 0; t < B->ntri; t += 7) {
+                bl_triangle* tr = &B->tris[t];
+                tri_size += sqrtf(max_edge2(ld3(B->nodes[tr->a].p), ld3(B->nodes[tr->b].p), ld3(B->nodes[tr->c].p)));
+                samples++;
+            }
+        }
+        const float mean = samples ? (float)(tri_size / samples) : 1.0f;
+        I->inv_cell = 1.0f / clampf_(mean + 2.0f * (rmax + margin), 0.25f, 1.0f);
+    }
+    const float inv_cell = I->inv_cell;
+    const float reach = rmax + margin;
+    bp_ctx c;
+    c.w = w, c.I = I;
+    c.boxes = malloc(sizeof(aabb3) * nb);
+    c.partners = calloc(nb, sizeof(int*));
+    c.npart = calloc(nb, sizeof(int));
+    for (int k = 0; k < nb; k++) {
+        orc_body* B = &w->bodies[I->bodies[k]];
+        aabb3 a = {B->mn, B->mx};
+        c.boxes[k] = aabb_expand(a, reach + 0.0f + 0.0f);
+        c.partners[k] = malloc(sizeof(int) * nb);
+    }
+    for (int a = 0; a < nb; a++)
+        for (int b = a + 1; b < nb; b++) {
+            if (same_group(w, I, a, b) || !aabb_overlaps(c.boxes[a], c.boxes[b])) continue;
+            c.partners[a][c.npart[a]++] = b;
+            c.partners[b][c.npart[b]++] = a;
+        }
+    /* 1) the spatial hash of contacter nodes */
+    orc_pt* pts = NULL;
+    size_t npts = 0, cpts = 0;
+    orc_entry* ent = NULL;
+    size_t nent = 0, cent = 0;
+    for (int bi = 0; bi < nb; bi++) {
+        orc_body* B = &w->bodies[I->bodies[bi]];
+        if (c.npart[bi] == 0 && !B->opts.self_collision) continue;
+        for (uint32_t ni = 0; ni < B->n; ni++) {
+            if (!(B->info[ni].flags & BL_NF_CONTACTER)) continue;
+            v3 p = ld3(B->nodes[ni].p);
+            if (!near_partner(&c, bi, p, p)) continue;
+            orc_pt pt = {(uint16_t)bi, ni, cell_of(p.x, inv_cell), cell_of(p.y, inv_cell), cell_of(p.z, inv_cell), p};
+            orc_entry e = {cell_hash(pt.x, pt.y, pt.z), (uint32_t)npts};
+            VEC_PUSH(ent, nent, cent, e);
+            VEC_PUSH(pts, npts, cpts, pt);
+        }
+    }
+    if (npts && (any_tris || any_caps)) {
+        uint32_t *bstart, *bitems;
+        const uint32_t mask = build_buckets(ent, nent, &bstart, &bitems);
+        uint32_t* run_end = malloc(sizeof(uint32_t) * (npts ? npts : 1));
+        for (uint32_t bk = 0; bk <= mask; bk++) {
+            uint32_t e = bstart[bk + 1];
+            for (uint32_t k = e; k-- > bstart[bk];) {
+                int same = k + 1 < e && pts[bitems[k + 1]].body == pts[bitems[k]].body;
+                run_end[k] = same ? run_end[k + 1] : k + 1;
+            }
+        }
+        /* 2) triangles, then capsules, of each body query the node hash */
+        for (int bt = 0; bt < nb; bt++) {
+            orc_body* O = &w->bodies[I->bodies[bt]];
+            if (c.npart[bt] == 0 && !O->opts.self_collision) continue;
+            const float rq0 = rmax + margin + 0.0f;
+            for (uint32_t t = 0; t < O->ntri; t++) {
+                const bl_triangle* tr = &O->tris[t];
+                if (tr->torn) continue;
+                const float depth = 0.0f, rq = rq0 + depth;
+                v3 a = ld3(O->nodes[tr->a].p), bb = ld3(O->nodes[tr->b].p), cc = ld3(O->nodes[tr->c].p);
+                v3 tmn = vmin3(a, vmin3(bb, cc)), tmx = vmax3(a, vmax3(bb, cc));
+                if (!near_partner(&c, bt, vsub(tmn, V(rq, rq, rq)), vadd(tmx, V(rq, rq, rq)))) continue;
+                const v3 tn = vcross(vsub(bb, a), vsub(cc, a));
+                const float tn2 = vdot(tn, tn);
+                const v3 tu = tn2 > 1e-20f ? vmul(tn, 1.0f / sqrtf(tn2)) : V(0, 0, 0);
+                int x0 = cell_of(tmn.x - rq, inv_cell), x1 = cell_of(tmx.x + rq, inv_cell);
+                int y0 = cell_of(tmn.y - rq, inv_cell), y1 = cell_of(tmx.y + rq, inv_cell);
+                int z0 = cell_of(tmn.z - rq, inv_cell), z1 = cell_of(tmx.z + rq, inv_cell);
+                if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 512) continue;
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++)
+                        for (int x = x0; x <= x1; x++) {
+                            uint32_t bk = cell_hash(x, y, z) & mask;
+                            for (uint32_t k = bstart[bk]; k < bstart[bk + 1]; k++) {
+                                const orc_pt* pt = &pts[bitems[k]];
+                                if (pt->body == bt && !O->opts.self_collision) { k = run_end[k] - 1; continue; }
+                                if (pt->x != x || pt->y != y || pt->z != z) continue;
+                                if (pt->p.x < tmn.x - rq || pt->p.y < tmn.y - rq || pt->p.z < tmn.z - rq || pt->p.x > tmx.x + rq ||
+                                    pt->p.y > tmx.y + rq || pt->p.z > tmx.z + rq)
+                                    continue;
+                                const int bn = pt->body;
+                                orc_body* Bn = &w->bodies[I->bodies[bn]];
+                                int self = bn == bt;
+                                if (self && (!Bn->opts.self_collision || tr->a == pt->node || tr->b == pt->node || tr->c == pt->node)) continue;
+                                if (!self && same_group(w, I, bn, bt)) continue;
+                                v3 p = ld3(Bn->nodes[pt->node].p);
+                                float r = Bn->opts.collision_radius + margin + 0.0f;
+                                const float rb = r + depth;
+                                if (p.x < tmn.x - rb || p.y < tmn.y - rb || p.z < tmn.z - rb || p.x > tmx.x + rb || p.y > tmx.y + rb || p.z > tmx.z + rb) continue;
+                                if (!near_triangle(p, a, bb, cc, tu, r)) continue;
+                                orc_nt q = {(uint16_t)bn, (uint16_t)bt, pt->node, t};
+                                VEC_PUSH(I->nt, I->nnt, I->cnt, q);
+                            }
+                        }
+            }
+            for (uint32_t ci = 0; ci < O->ncap; ci++) {
+                const bl_capsule* cp = &O->caps[ci];
+                v3 a = ld3(O->nodes[cp->a].p), bb = ld3(O->nodes[cp->b].p);
+                float rq2 = cp->radius + rmax + margin + 0.0f;
+                v3 mn = vsub(vmin3(a, bb), V(rq2, rq2, rq2)), mx = vadd(vmax3(a, bb), V(rq2, rq2, rq2));
+                if (!near_partner(&c, bt, mn, mx)) continue;
+                int x0 = cell_of(mn.x, inv_cell), x1 = cell_of(mx.x, inv_cell), y0 = cell_of(mn.y, inv_cell), y1 = cell_of(mx.y, inv_cell);
+                int z0 = cell_of(mn.z, inv_cell), z1 = cell_of(mx.z, inv_cell);
+                if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 512) continue;
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++)
+                        for (int x = x0; x <= x1; x++) {
+                            uint32_t bk = cell_hash(x, y, z) & mask;
+                            for (uint32_t k = bstart[bk]; k < bstart[bk + 1]; k++) {
+                                const orc_pt* pt = &pts[bitems[k]];
+                                if (pt->body == bt) { k = run_end[k] - 1; continue; }
+                                if (pt->x != x || pt->y != y || pt->z != z) continue;
+                                const int bn = pt->body;
+                                if (same_group(w, I, bn, bt)) continue;
+                                orc_body* Bn = &w->bodies[I->bodies[bn]];
+                                v3 p = ld3(Bn->nodes[pt->node].p);
+                                float r = cp->radius + Bn->opts.collision_radius + margin + 0.0f;
+                                float sg = closest_on_segment(p, a, bb);
+                                v3 d = vsub(vadd(a, vmul(vsub(bb, a), sg)), p);
+                                if (vdot(d, d) > r * r) continue;
+                                orc_nc q = {(uint16_t)bn, (uint16_t)bt, pt->node, ci};
+                                VEC_PUSH(I->nc, I->nnc, I->cnc, q);
+                            }
+                        }
+            }
+        }
+        free(bstart), free(bitems), free(run_end);
+    }
+    /* 3) capsules against triangles: the triangles hashed into every cell of their boxes */
+    if (any_caps && any_tris) {
+        typedef struct { uint16_t body; uint32_t idx; } prim;
+        prim* prims = NULL;
+        size_t np = 0, cpn = 0;
+        nent = 0;
+        for (int bi = 0; bi < nb; bi++) {
+            orc_body* B = &w->bodies[I->bodies[bi]];
+            if (c.npart[bi] == 0 && !B->opts.self_collision) continue;
+            const float r = B->opts.collision_radius + margin + 0.0f;
+            for (uint32_t t = 0; t < B->ntri; t++) {
+                const bl_triangle* tr = &B->tris[t];
+                if (tr->torn) continue;
+                v3 a = ld3(B->nodes[tr->a].p), bb = ld3(B->nodes[tr->b].p), cc = ld3(B->nodes[tr->c].p);
+                v3 mn = vsub(vmin3(a, vmin3(bb, cc)), V(r, r, r)), mx = vadd(vmax3(a, vmax3(bb, cc)), V(r, r, r));
+                if (!near_partner(&c, bi, mn, mx)) continue;
+                int x0 = cell_of(mn.x, inv_cell), x1 = cell_of(mx.x, inv_cell), y0 = cell_of(mn.y, inv_cell), y1 = cell_of(mx.y, inv_cell);
+                int z0 = cell_of(mn.z, inv_cell), z1 = cell_of(mx.z, inv_cell);
+                if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 512) continue;
+                uint32_t pid = (uint32_t)np;
+                prim pr = {(uint16_t)bi, t};
+                VEC_PUSH(prims, np, cpn, pr);
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++)
+                        for (int x = x0; x <= x1; x++) {
+                            orc_entry e = {cell_hash(x, y, z), pid};
+                            VEC_PUSH(ent, nent, cent, e);
+                        }
+            }
+        }
+        uint32_t *bstart, *bitems;
+        const uint32_t mask = build_buckets(ent, nent, &bstart, &bitems);
+        for (int bc = 0; bc < nb; bc++) {
+            orc_body* B = &w->bodies[I->bodies[bc]];
+            for (uint32_t ci = 0; ci < B->ncap; ci++) {
+                const bl_capsule* cp = &B->caps[ci];
+                v3 a = ld3(B->nodes[cp->a].p), bb = ld3(B->nodes[cp->b].p);
+                float rr = cp->radius + B->opts.collision_radius + margin + 0.0f;
+                v3 mn = vsub(vmin3(a, bb), V(rr, rr, rr)), mx = vadd(vmax3(a, bb), V(rr, rr, rr));
+                int x0 = cell_of(mn.x, inv_cell), x1 = cell_of(mx.x, inv_cell), y0 = cell_of(mn.y, inv_cell), y1 = cell_of(mx.y, inv_cell);
+                int z0 = cell_of(mn.z, inv_cell), z1 = cell_of(mx.z, inv_cell);
+                if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) > 512) continue;
+                size_t start = I->nct;
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++)
+                        for (int x = x0; x <= x1; x++) {
+                            uint32_t bk = cell_hash(x, y, z) & mask;
+                            for (uint32_t k = bstart[bk]; k < bstart[bk + 1]; k++) {
+                                const prim* pr = &prims[bitems[k]];
+                                if (pr->body == bc || same_group(w, I, bc, pr->body)) continue;
+                                orc_body* O = &w->bodies[I->bodies[pr->body]];
+                                const bl_triangle* t = &O->tris[pr->idx];
+                                v3 ta = ld3(O->nodes[t->a].p), tb = ld3(O->nodes[t->b].p), tc = ld3(O->nodes[t->c].p);
+                                v3 tmn = vmin3(ta, vmin3(tb, tc)), tmx = vmax3(ta, vmax3(tb, tc));
+                                if (tmx.x < mn.x || tmx.y < mn.y || tmx.z < mn.z || tmn.x > mx.x || tmn.y > mx.y || tmn.z > mx.z) continue;
+                                orc_ct q = {(uint16_t)bc, pr->body, ci, pr->idx};
+                                VEC_PUSH(I->ct, I->nct, I->cct, q);
+                            }
+                        }
+                /* dedupe: a capsule sees a triangle through several cells (sorted by body, triangle) */
+                size_t m = I->nct - start;
+                if (m > 1) {
+                    qsort(I->ct + start, m, sizeof(orc_ct), cmp_ct);
+                    size_t wr = start + 1;
+                    for (size_t k = start + 1; k < I->nct; k++)
+                        if (I->ct[k].bt != I->ct[wr - 1].bt || I->ct[k].tri != I->ct[wr - 1].tri) I->ct[wr++] = I->ct[k];
+                    I->nct = wr;
+                }
+            }
+        }
+        free(bstart), free(bitems), free(prims);
+    }
+    free(pts), free(ent);
+    for (int k = 0; k < nb; k++) free(c.partners[k]);
+    free(c.partners), free(c.npart), free(c.boxes);
+    w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
+}
+
+/* World::near_pairs (src/phys/world.cpp:2525-2596): the node-triangle candidates within reach of their relative
+ * travel until the next rebuild, and not receding across the triangle's plane; compacted in order */
+static void near_pairs(orc_world* w, orc_island* I) {
+    const size_t n = I->nnt;
+    if (n == 0) return;
+    const float T = (float)(I->interval > 1 ? I->interval : 1) * w->s.dt;
+    size_t wr = 0;
+    for (size_t i = 0; i < n; i++) {
+        const orc_nt pr = I->nt[i];
+        orc_body* Bn = &w->bodies[I->bodies[pr.bn]];
+        orc_body* Bt = &w->bodies[I->bodies[pr.bt]];
+        const bl_node* x = &Bn->nodes[pr.node];
+        const bl_triangle* t = &Bt->tris[pr.tri];
+        if (t->torn) continue;
+        const bl_node *a = &Bt->nodes[t->a], *b = &Bt->nodes[t->b], *c3 = &Bt->nodes[t->c];
+        v3 bary;
+        const v3 q = closest_on_triangle(ld3(x->p), ld3(a->p), ld3(b->p), ld3(c3->p), &bary);
+        const float r0 = Bn->opts.collision_radius + 0.0f;
+        const v3 da = vsub(ld3(x->v), ld3(a->v)), db = vsub(ld3(x->v), ld3(b->v)), dc = vsub(ld3(x->v), ld3(c3->v));
+        const float vrel = sqrtf(smax(vdot(da, da), smax(vdot(db, db), vdot(dc, dc))));
+        const float reach = r0 + 1.25f * vrel * T + 0.02f;
+        v3 xq = vsub(ld3(x->p), q);
+        if (vdot(xq, xq) >= reach * reach) continue;
+        const v3 nn = vcross(vsub(ld3(b->p), ld3(a->p)), vsub(ld3(c3->p), ld3(a->p)));
+        const float nl = vlen(nn);
+        if (nl > 1e-12f) {
+            const v3 n0 = vdivs(nn, nl);
+            const float s0 = vdot(n0, vsub(ld3(x->p), ld3(a->p)));
+            const float sg = s0 >= 0 ? 1.0f : -1.0f;
+            const float close = smax(0.0f, smax(-sg * vdot(n0, da), smax(-sg * vdot(n0, db), -sg * vdot(n0, dc))));
+            if (fabsf(s0) - 1.25f * close * T - 0.02f >= Bn->opts.collision_radius) continue;
+        }
+        I->nt[wr++] = pr;
+    }
+    I->nnt = wr;
+    w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
+}
+
+/* World::pair_detect (src/phys/world.cpp:2807-2962), two-sided triangles */
+static void pair_detect(orc_world* w, orc_island* I) {
+    I->nhit = 0;
+    for (size_t i = 0; i < I->nnt; i++) {
+        const orc_nt pr = I->nt[i];
+        orc_body* Bn = &w->bodies[I->bodies[pr.bn]];
+        orc_body* Bt = &w->bodies[I->bodies[pr.bt]];
+        const bl_node* n = &Bn->nodes[pr.node];
+        const bl_triangle* t = &Bt->tris[pr.tri];
+        if (t->torn) continue;
+        const bl_node *a = &Bt->nodes[t->a], *b = &Bt->nodes[t->b], *cc = &Bt->nodes[t->c];
+        const float r = Bn->opts.collision_radius;
+        v3 np = ld3(n->p), ap = ld3(a->p), bp = ld3(b->p), cp3 = ld3(cc->p);
+        const v3 lo = vsub(vmin3(ap, vmin3(bp, cp3)), V(r, r, r)), hi = vadd(vmax3(ap, vmax3(bp, cp3)), V(r, r, r));
+        if (np.x < lo.x || np.y < lo.y || np.z < lo.z || np.x > hi.x || np.y > hi.y || np.z > hi.z) continue;
+        v3 bary;
+        v3 cp = closest_on_triangle(np, ap, bp, cp3, &bary);
+        v3 d = vsub(np, cp);
+        float dist2 = vdot(d, d);
+        if (dist2 >= r * r) continue;
+        float dist = sqrtf(dist2);
+        v3 nrm;
+        if (dist > 1e-5f) nrm = vdivs(d, dist);
+        else {
+            nrm = vnormalize(vcross(vsub(bp, ap), vsub(cp3, ap)));
+            v3 vt = vadd(vadd(vmul(ld3(a->v), bary.x), vmul(ld3(b->v), bary.y)), vmul(ld3(cc->v), bary.z));
+            if (vdot(vsub(ld3(n->v), vt), nrm) > 0) nrm = V(-nrm.x, -nrm.y, -nrm.z);
+        }
+        orc_hit h = {(uint32_t)i, 0.0f, bary, nrm, r - dist};
+        VEC_PUSH(I->hits, I->nhit, I->chit, h);
+    }
+    for (size_t i = 0; i < I->nnc; i++) {
+        const orc_nc pr = I->nc[i];
+        orc_body* Bn = &w->bodies[I->bodies[pr.bn]];
+        orc_body* Bc = &w->bodies[I->bodies[pr.bc]];
+        const bl_capsule* cap = &Bc->caps[pr.cap];
+        v3 np = ld3(Bn->nodes[pr.node].p), ap = ld3(Bc->nodes[cap->a].p), bp = ld3(Bc->nodes[cap->b].p);
+        float t = closest_on_segment(np, ap, bp);
+        v3 cp = vadd(ap, vmul(vsub(bp, ap), t));
+        v3 d = vsub(np, cp);
+        float r = Bn->opts.collision_radius + cap->radius;
+        float dist2 = vdot(d, d);
+        if (dist2 >= r * r) continue;
+        float dist = sqrtf(dist2);
+        v3 nrm = dist > 1e-5f ? vdivs(d, dist) : V(0, 1, 0);
+        orc_hit h = {(uint32_t)(I->nnt + i), t, V(0, 0, 0), nrm, r - dist};
+        VEC_PUSH(I->hits, I->nhit, I->chit, h);
+    }
+    for (size_t i = 0; i < I->nct; i++) {
+        const orc_ct pr = I->ct[i];
+        orc_body* Bc = &w->bodies[I->bodies[pr.bc]];
+        orc_body* Bt = &w->bodies[I->bodies[pr.bt]];
+        const bl_capsule* cap = &Bc->caps[pr.cap];
+        const v3 p0 = ld3(Bc->nodes[cap->a].p), p1 = ld3(Bc->nodes[cap->b].p);
+        const bl_triangle* tri = &Bt->tris[pr.tri];
+        const v3 a = ld3(Bt->nodes[tri->a].p), b = ld3(Bt->nodes[tri->b].p), cc = ld3(Bt->nodes[tri->c].p);
+        float best = 1e30f, best_s = 0;
+        v3 best_bary = V(0, 0, 0), best_seg = V(0, 0, 0), best_tri = V(0, 0, 0);
+#define CONSIDER(S, SP, TP, BARY)                                                                                      \
+    do {                                                                                                               \
+        v3 dd_ = vsub((SP), (TP));                                                                                     \
+        float d2_ = vdot(dd_, dd_);                                                                                    \
+        if (d2_ < best) { best = d2_; best_s = (S); best_seg = (SP); best_tri = (TP); best_bary = (BARY); }            \
+    } while (0)
+        v3 bary;
+        v3 q = closest_on_triangle(p0, a, b, cc, &bary);
+        CONSIDER(0.0f, p0, q, bary);
+        q = closest_on_triangle(p1, a, b, cc, &bary);
+        CONSIDER(1.0f, p1, q, bary);
+        const v3 tv[3] = {a, b, cc};
+        for (int e = 0; e < 3; e++) {
+            float sg, u;
+            closest_seg_seg(p0, p1, tv[e], tv[(e + 1) % 3], &sg, &u);
+            v3 sp = vadd(p0, vmul(vsub(p1, p0), sg));
+            v3 tp = vadd(tv[e], vmul(vsub(tv[(e + 1) % 3], tv[e]), u));
+            float bb3[3] = {0, 0, 0};
+            bb3[e] = 1 - u;
+            bb3[(e + 1) % 3] = u;
+            CONSIDER(sg, sp, tp, V(bb3[0], bb3[1], bb3[2]));
+        }
+#undef CONSIDER
+        v3 tn = vcross(vsub(b, a), vsub(cc, a));
+        float d0 = vdot(vsub(p0, a), tn), d1 = vdot(vsub(p1, a), tn);
+        int crossing = 0;
+        if ((d0 < 0) != (d1 < 0) && fabsf(d0 - d1) > 1e-12f) {
+            float sg = d0 / (d0 - d1);
+            v3 ip = vadd(p0, vmul(vsub(p1, p0), sg));
+            q = closest_on_triangle(ip, a, b, cc, &bary);
+            v3 qi = vsub(q, ip);
+            if (vdot(qi, qi) < 1e-8f) {
+                crossing = 1;
+                best = 0, best_s = sg, best_seg = ip, best_tri = q, best_bary = bary;
+            }
+        }
+        float r = cap->radius + Bc->opts.collision_radius;
+        if (best >= r * r) continue;
+        float dist = sqrtf(best);
+        v3 nrm;
+        float pen;
+        if (!crossing && dist > 1e-5f) {
+            nrm = vdivs(vsub(best_seg, best_tri), dist);
+            pen = r - dist;
+        } else {
+            nrm = vnormalize(tn);
+            v3 mid = vmul(vadd(p0, p1), 0.5f);
+            if (vdot(vsub(mid, a), nrm) < 0) nrm = V(-nrm.x, -nrm.y, -nrm.z);
+            float dm = smin(fabsf(d0), fabsf(d1)) / smax(1e-6f, vlen(tn));
+            pen = r + smin(dm, 0.2f);
+        }
+        orc_hit h = {(uint32_t)(I->nnt + I->nnc + i), best_s, best_bary, nrm, pen};
+        VEC_PUSH(I->hits, I->nhit, I->chit, h);
+    }
+}
+
+/* contact_force<NA, NB> (src/phys/world.cpp:59-91) with both sides movable and no force snapshots */
+static int contact_force(orc_world* w, bl_node** A, v3** FA, const float* wa, int na, bl_node** B, v3** FB,
+                         const float* wb, int nbn, v3 n, float pen, float dt, float fric) {
+    float W = 0;
+    for (int i = 0; i < na; i++) W += wa[i] * wa[i] * A[i]->inv_mass;
+    for (int i = 0; i < nbn; i++) W += wb[i] * wb[i] * B[i]->inv_mass;
+    if (W <= 1e-12f) return 0;
+    float m = 1.0f / W;
+    v3 va = V(0, 0, 0), vb = V(0, 0, 0), acc = V(0, 0, 0);
+    for (int i = 0; i < na; i++) {
+        va = vadd(va, vmul(ld3(A[i]->v), wa[i]));
+        acc = vadd(acc, vmul(*FA[i], wa[i] * A[i]->inv_mass));
+    }
+    for (int i = 0; i < nbn; i++) {
+        vb = vadd(vb, vmul(ld3(B[i]->v), wb[i]));
+        acc = vsub(acc, vmul(*FB[i], wb[i] * B[i]->inv_mass));
+    }
+    const v3 f_rel = vmul(acc, m);
+    v3 f = prim_coll(f_rel, vsub(va, vb), m, n, dt, &k_gm[BL_SURF_CONCRETE], pen, fric, 2.0f, 0.0f, -1.0f);
+    for (int i = 0; i < na; i++) *FA[i] = vadd(*FA[i], vmul(f, wa[i]));
+    for (int i = 0; i < nbn; i++) *FB[i] = vsub(*FB[i], vmul(f, wb[i]));
+    (void)w;
+    return 1;
+}
+
+/* World::collide_pairs (src/phys/world.cpp:2964-3079): the narrow phase, then the response in pair order */
+static void collide_pairs(orc_world* w, orc_island* I) {
+    w->st.narrow_tests += (int64_t)(I->nnt + I->nnc + I->nct);
+    if (I->nnt + I->nnc + I->nct == 0) return;
+    pair_detect(w, I);
+    const float dt = w->s.dt;
+    for (size_t k = 0; k < I->nhit; k++) {
+        const orc_hit* h = &I->hits[k];
+        if (h->pair < I->nnt) {
+            const orc_nt pr = I->nt[h->pair];
+            orc_body *Bn = &w->bodies[I->bodies[pr.bn]], *Bt = &w->bodies[I->bodies[pr.bt]];
+            const bl_triangle* t = &Bt->tris[pr.tri];
+            bl_node* A[1] = {&Bn->nodes[pr.node]};
+            v3* FA[1] = {&Bn->force[pr.node]};
+            float wa[1] = {1.0f};
+            bl_node* B[3] = {&Bt->nodes[t->a], &Bt->nodes[t->b], &Bt->nodes[t->c]};
+            v3* FB[3] = {&Bt->force[t->a], &Bt->force[t->b], &Bt->force[t->c]};
+            float wb[3] = {h->bary.x, h->bary.y, h->bary.z};
+            float fric = (Bn->info[pr.node].flags & BL_NF_TYRE) ? Bn->info[pr.node].friction
+                                                                 : 0.8f * smin(Bn->opts.contact_friction, Bt->opts.contact_friction);
+            if (contact_force(w, A, FA, wa, 1, B, FB, wb, 3, h->nrm, h->pen, dt, fric)) w->st.contacts++;
+        } else if (h->pair < I->nnt + I->nnc) {
+            const orc_nc pr = I->nc[h->pair - I->nnt];
+            orc_body *Bn = &w->bodies[I->bodies[pr.bn]], *Bc = &w->bodies[I->bodies[pr.bc]];
+            const bl_capsule* cap = &Bc->caps[pr.cap];
+            bl_node* A[1] = {&Bn->nodes[pr.node]};
+            v3* FA[1] = {&Bn->force[pr.node]};
+            float wa[1] = {1.0f};
+            bl_node* B[2] = {&Bc->nodes[cap->a], &Bc->nodes[cap->b]};
+            v3* FB[2] = {&Bc->force[cap->a], &Bc->force[cap->b]};
+            float wb[2] = {1 - h->s, h->s};
+            if (contact_force(w, A, FA, wa, 1, B, FB, wb, 2, h->nrm, h->pen, dt,
+                              0.8f * smin(Bn->opts.contact_friction, Bc->opts.contact_friction)))
+                w->st.contacts++;
+        } else {
+            const orc_ct pr = I->ct[h->pair - I->nnt - I->nnc];
+            orc_body *Bc = &w->bodies[I->bodies[pr.bc]], *Bt = &w->bodies[I->bodies[pr.bt]];
+            const bl_capsule* cap = &Bc->caps[pr.cap];
+            const bl_triangle* tri = &Bt->tris[pr.tri];
+            bl_node* A[2] = {&Bc->nodes[cap->a], &Bc->nodes[cap->b]};
+            v3* FA[2] = {&Bc->force[cap->a], &Bc->force[cap->b]};
+            float wa[2] = {1 - h->s, h->s};
+            bl_node* B[3] = {&Bt->nodes[tri->a], &Bt->nodes[tri->b], &Bt->nodes[tri->c]};
+            v3* FB[3] = {&Bt->force[tri->a], &Bt->force[tri->b], &Bt->force[tri->c]};
+            float wb[3] = {h->bary.x, h->bary.y, h->bary.z};
+            if (contact_force(w, A, FA, wa, 2, B, FB, wb, 3, h->nrm, h->pen, dt,
+                              0.8f * smin(Bc->opts.contact_friction, Bt->opts.contact_friction)))
+                w->st.contacts++;
+        }
+    }
+}
+
+/* World::step_substeps (src/phys/world.cpp:368-585) -> build_islands, then simulate_island per island
+ * (src/phys/world.cpp:3211-4020): per substep the forces (clear_forces src/phys/softbody.cpp:381 with no drag:
+ * f = g m; the beams), the inter-body contacts (pairs rebuilt at the first substep and every `interval`
+ * substeps), then per node the static contact and the integration; after the frame, the post-frame check. */
 int orc_step(orc_world* w, int substeps) {
     const float dt = w->s.dt;
     const v3 g = ld3(w->s.gravity);
-    /* the top speed at the island's start (src/phys/world.cpp:3217-3222) */
-    for (int k = 0; k < w->nbodies; k++) {
-        orc_body* B = &w->bodies[k];
-        float m2 = 0;
-        for (uint32_t i = 0; i < B->n; i++) {
-            v3 v = ld3(B->nodes[i].v);
-            m2 = smax(m2, vdot(v, v));
+    if (substeps <= 0) return 0;
+    orc_island* isl;
+    const int nisl = build_islands(w, (float)substeps * dt, &isl);
+    for (int k = 0; k < nisl; k++) {
+        orc_island* I = &isl[k];
+        for (int q = 0; q < I->nb; q++) {
+            orc_body* B = &w->bodies[I->bodies[q]];
+            float m2 = 0;
+            for (uint32_t i = 0; i < B->n; i++) {
+                v3 v = ld3(B->nodes[i].v);
+                m2 = smax(m2, vdot(v, v));
+            }
+            B->max_speed = sqrtf(m2);
         }
-        B->max_speed = sqrtf(m2);
-    }
-    for (int s = 0; s < substeps; s++) {
-        for (int k = 0; k < w->nbodies; k++) {
-            orc_body* B = &w->bodies[k];
-            for (uint32_t i = 0; i < B->n; i++) B->force[i] = vmul(g, B->nodes[i].mass);
-            beam_forces(B);
-            w->st.static_contacts += collide_static(w, B, dt);
-            v3 mn = V(1e30f, 1e30f, 1e30f), mx = V(-1e30f, -1e30f, -1e30f);
-            float v2 = 0;
-            integrate(w, B, dt, &mn, &mx, &v2);
-            B->mn = mn, B->mx = mx;
-            float ms = sqrtf(v2);
-            if (ms > B->max_speed) B->max_speed = ms;
+        I->next_rebuild = 0;
+        for (int s = 0; s < substeps; s++) {
+            for (int q = 0; q < I->nb; q++) {
+                orc_body* B = &w->bodies[I->bodies[q]];
+                for (uint32_t i = 0; i < B->n; i++) B->force[i] = vmul(g, B->nodes[i].mass);
+                beam_forces(B);
+            }
+            if (I->needs_pairs) {
+                if (s == 0 || s >= I->next_rebuild) {
+                    rebuild_pairs(w, I);
+                    near_pairs(w, I);
+                    I->next_rebuild = s + I->interval;
+                }
+                collide_pairs(w, I);
+            }
+            for (int q = 0; q < I->nb; q++) {
+                orc_body* B = &w->bodies[I->bodies[q]];
+                w->st.static_contacts += collide_static(w, B, dt);
+                v3 mn = V(1e30f, 1e30f, 1e30f), mx = V(-1e30f, -1e30f, -1e30f);
+                float v2 = 0;
+                integrate(w, B, dt, &mn, &mx, &v2);
+                B->mn = mn, B->mx = mx;
+                float ms = sqrtf(v2);
+                if (ms > B->max_speed) B->max_speed = ms;
+            }
         }
-        w->st.substeps++;
+        free_island(I);
     }
-    if (substeps > 0)
-        for (int k = 0; k < w->nbodies; k++) post_frame(w, &w->bodies[k]);
+    free(isl);
+    w->st.substeps += substeps;
+    for (int k = 0; k < w->nbodies; k++) post_frame(w, &w->bodies[k]);
     return 0;
 }
 

@@ -19,6 +19,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from _cases import CPU_CASES, run_world  # noqa: E402
+from _contact_cases import CONTACT_CASES  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 
@@ -35,8 +36,8 @@ def main():
     out["broken"] = np.array(out["broken"])
     np.savez_compressed(os.path.join(HERE, "c1_ref.npz"), **out)
     digests = {}
-    for name in sorted(CPU_CASES):
-        settings, boxes, bodies, n = CPU_CASES[name]()
+    for name in sorted(CPU_CASES) + sorted(CONTACT_CASES):
+        settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
         st = run_world(oracle.RefWorld(settings, boxes), bodies, n, sorted({1, 7, n // 3, n}))
         h = hashlib.sha256()
         for nodes, beams in st[-1][:-1]:

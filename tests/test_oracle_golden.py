@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from _cases import CPU_CASES, run_world
+from _contact_cases import CONTACT_CASES
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -24,10 +25,10 @@ def test_c1_states(oracle_mod):
     assert int(g["broken"][-1]) > 0, "the fixture covers breaking"
 
 
-@pytest.mark.parametrize("name", sorted(CPU_CASES))
+@pytest.mark.parametrize("name", sorted(CPU_CASES) + sorted(CONTACT_CASES))
 def test_case_digests(name, oracle_mod):
     g = np.load(os.path.join(GOLD, "cases_ref.npz"))
-    settings, boxes, bodies, n = CPU_CASES[name]()
+    settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
     st = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, sorted({1, 7, n // 3, n}))
     h = hashlib.sha256()
     for nodes, beams in st[-1][:-1]:
