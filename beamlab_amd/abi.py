@@ -114,7 +114,11 @@ _lib = None
 
 
 class BeamlabError(RuntimeError):
-    pass
+    """A non-zero status of an entry point (status: the BL_ERR_* code, None when the library is missing)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def load():
@@ -136,7 +140,7 @@ def load():
 def check(rc, what=""):
     if rc != BL_OK:
         msg = load().bl_error_string(rc).decode()
-        raise BeamlabError(f"{what}: status {rc}: {msg}")
+        raise BeamlabError(f"{what}: status {rc}: {msg}", rc)
     return rc
 
 

@@ -1,5 +1,5 @@
 // The C ABI (include/beamlab_gpu.h): context, device memory, uploads, launches and read-back.
-#include "bl_world.h"
+#include "bl_contacts.h"
 
 #include <algorithm>
 #include <cmath>
@@ -33,7 +33,14 @@ struct HostBody {
     std::vector<bl_triangle> tris;
     std::vector<bl_capsule> caps;
     bl_body_opts opts;
-    uint32_t node0 = 0, beam0 = 0;
+    uint32_t node0 = 0, beam0 = 0, tri0 = 0, cap0 = 0;
+    // as of the next frame's start: top speed of the last frame and bounds (World::build_islands reads them)
+    float max_speed = 0;
+    f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
+    void compute_bounds() {
+        mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
+        for (const bl_node& x : nodes) mn = vmin(mn, mk(x.p[0], x.p[1], x.p[2])), mx = vmax(mx, mk(x.p[0], x.p[1], x.p[2]));
+    }
 };
 
 template <class T>
@@ -86,6 +93,15 @@ struct bl_ctx {
     DevArr<unsigned long long> counters, broken;
     DevArr<unsigned int> bounds, body_frame, bar;
     DevArr<int> err;
+    // inter-body contacts
+    DevArr<float4> F;
+    DevArr<uint4> tri_abc, cap_data;
+    DevArr<float> tri_rest2;
+    DevArr<uint32_t> tri_torn;
+    DevArr<unsigned int> cbox;
+    ContactEngine contacts;
+    HostView hv;
+    bool contact_geometry = false; // some body has triangles, capsules or self collision
     bool frame_pending = false;   // a frame enqueued whose post-frame check has not run
     int64_t repairs = 0;
     std::vector<uint32_t> a_slot;   // global beam -> incidence of its a end
@@ -107,6 +123,8 @@ struct bl_ctx {
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
         halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
+        F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), cbox.release();
+        contacts.release();
         frame_pending = false;
         built = false;
     }
@@ -158,7 +176,10 @@ int pull_state(bl_ctx* c) {
     const size_t B = c->stress.n;
     std::vector<float> S(B);
     if (B) HIPCHK(hipMemcpy(S.data(), c->stress.p, sizeof(float) * B, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> torn(c->tri_torn.n);
+    if (!torn.empty()) HIPCHK(hipMemcpy(torn.data(), c->tri_torn.p, 4 * torn.size(), hipMemcpyDeviceToHost));
     for (HostBody& hb : c->bodies) {
+        for (size_t t = 0; t < hb.tris.size() && hb.tri0 + t < torn.size(); t++) hb.tris[t].torn = torn[hb.tri0 + t] ? 1 : 0;
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const float4 p = P[hb.node0 + i], v = V[hb.node0 + i];
             bl_node& n = hb.nodes[i];
@@ -182,19 +203,52 @@ int build(bl_ctx* c) {
     const int T = effective_block(c);
     const uint32_t CAP = (uint32_t)chunk_capacity(T, effective_per(c));
     const uint32_t NCAP = (uint32_t)T; // nodes per chunk: one per thread
-    uint32_t N = 0, B = 0;
+    uint32_t N = 0, B = 0, NT = 0, NC = 0;
     for (HostBody& hb : c->bodies) {
-        hb.node0 = N, hb.beam0 = B;
+        hb.node0 = N, hb.beam0 = B, hb.tri0 = NT, hb.cap0 = NC;
         N += (uint32_t)hb.nodes.size();
         B += (uint32_t)hb.beams.size();
+        NT += (uint32_t)hb.tris.size();
+        NC += (uint32_t)hb.caps.size();
     }
+    // collision triangles and capsules with global node indices; the island builder's view of the bodies
+    std::vector<uint4> tri_abc(NT), cap_data(NC);
+    std::vector<float> tri_rest2(NT);
+    std::vector<uint32_t> tri_torn(NT);
+    c->hv = HostView{};
+    c->hv.dt = c->s.dt, c->hv.inter_body = c->s.inter_body_collisions;
+    c->contact_geometry = false;
+    for (size_t k = 0; k < c->bodies.size(); k++) {
+        const HostBody& hb = c->bodies[k];
+        for (size_t t = 0; t < hb.tris.size(); t++) {
+            const bl_triangle& tr = hb.tris[t];
+            tri_abc[hb.tri0 + t] = make_uint4(hb.node0 + tr.a, hb.node0 + tr.b, hb.node0 + tr.c, (uint32_t)k);
+            tri_rest2[hb.tri0 + t] = tr.rest_edge2;
+            tri_torn[hb.tri0 + t] = tr.torn;
+        }
+        for (size_t q = 0; q < hb.caps.size(); q++) {
+            const bl_capsule& cp = hb.caps[q];
+            cap_data[hb.cap0 + q] = make_uint4(hb.node0 + cp.a, hb.node0 + cp.b, (uint32_t)k, f2u(cp.radius));
+        }
+        BodyView v{};
+        v.group = hb.opts.collision_group, v.self = hb.opts.self_collision != 0;
+        v.ntri = (uint32_t)hb.tris.size(), v.ncap = (uint32_t)hb.caps.size(), v.tri0 = hb.tri0;
+        v.collision_radius = hb.opts.collision_radius;
+        for (const bl_node_info& inf : hb.info) v.contacters |= (inf.flags & BL_NF_CONTACTER) ? 1u : 0u;
+        c->hv.bodies.push_back(v);
+        c->contact_geometry |= v.ntri > 0 || v.ncap > 0 || v.self;
+    }
+    c->hv.max_speed.assign(c->bodies.size(), 0.0f);
+    c->hv.mn.assign(c->bodies.size(), mk(0, 0, 0));
+    c->hv.mx.assign(c->bodies.size(), mk(0, 0, 0));
     std::vector<float4> P(N), V(N);
     std::vector<uint32_t> ninfo(N), deg(N + 1, 0);
     std::vector<float2> naux(N);
     std::vector<BodyParam> bp(c->bodies.size());
     for (size_t k = 0; k < c->bodies.size(); k++) {
         const HostBody& hb = c->bodies[k];
-        bp[k] = {hb.opts.bounce, hb.opts.contact_push_max, hb.opts.contact_slop, 0};
+        bp[k] = BodyParam{hb.opts.bounce, hb.opts.contact_push_max, hb.opts.contact_slop, hb.opts.collision_radius,
+                          hb.opts.contact_friction, hb.opts.collision_group, hb.opts.self_collision, 0};
         const float fric_body = hb.opts.ground_friction * 1.0f; // (resting is off: src/phys/world.cpp:792)
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const bl_node& n = hb.nodes[i];
@@ -332,8 +386,16 @@ int build(bl_ctx* c) {
         (rc = c->err.alloc(1)) || (rc = c->chunk_body.upload(chunk_body)) ||
         (rc = c->cstat.alloc(std::max<size_t>(1, nchunks))) || (rc = c->halo_off.upload(halo_off)) ||
         (rc = c->halo.upload(halo)) || (rc = c->nbr_off.upload(nbr_off)) || (rc = c->nbr.upload(nbr)) ||
-        (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))))
+        (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))) ||
+        (rc = c->tri_abc.upload(tri_abc)) || (rc = c->tri_rest2.upload(tri_rest2)) || (rc = c->tri_torn.upload(tri_torn)) ||
+        (rc = c->cap_data.upload(cap_data)) ||
+        (rc = c->F.alloc(c->contact_geometry ? N : 0)) ||
+        (rc = c->cbox.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
         return rc;
+    if (c->contact_geometry && (rc = c->contacts.setup(c->hv))) {
+        g_last_error = "contact buffers: device allocation failed";
+        return rc;
+    }
     HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
     HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
     HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
@@ -347,6 +409,8 @@ int build(bl_ctx* c) {
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
     a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;
     a.halo_off = c->halo_off.p, a.halo = c->halo.p, a.nbr_off = c->nbr_off.p, a.nbr = c->nbr.p, a.iloc = c->iloc.p;
+    a.F = c->F.p, a.tri_abc = c->tri_abc.p, a.tri_rest2 = c->tri_rest2.p, a.tri_torn = c->tri_torn.p;
+    a.cap_data = c->cap_data.p, a.ntris = NT, a.ncaps = NC;
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
@@ -510,10 +574,84 @@ int repair_body(HostBody& b) {
     return reset;
 }
 
+// A frame with inter-body contacts (World::simulate_island, src/phys/world.cpp:3211-3591): per substep the forces
+// into F, the pair lists rebuilt at the first substep and every `interval` after (from the state at that point),
+// the narrow phase and the response on F, then the static contacts and the integration. The host waits at each
+// rebuild (list sizes) - about one substep in 17 at the default 120 Hz pair rate.
+int contact_frame(bl_ctx* c, int substeps) {
+    bl_tuning t = c->tune;
+    t.block = c->built_block, t.incidences_per_thread = c->built_per;
+    const double hz = (double)smax(1.0f, c->s.collision_hz);
+    int interval = (int)std::ceil(1.0 / (hz * (double)c->s.dt) - 1e-6);
+    if (interval < 1) interval = 1;
+    c->contacts.interval = interval;
+    const size_t nb = c->bodies.size();
+    KArgs& a = c->args;
+    KArgs ab = a;
+    ab.body_frame = c->cbox.p;
+    int rc;
+    if ((rc = launch_frame_begin(a, c->stream, c->parity))) return rc;
+    std::vector<unsigned int> w(nb * BF_N), wb(nb * BF_N);
+    std::vector<float> v2(nb), box(nb * 6);
+    int next = 0;
+    for (int s = 0; s < substeps; s++) {
+        if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
+        if (s == 0 || s >= next) {
+            // each body's top speed so far (the frame's start included) and bounds now (at the frame's start:
+            // as the last frame left them)
+            if (s > 0 && ((rc = launch_fold_speed(a, c->stream)) || (rc = launch_box(ab, c->stream, c->parity)))) return rc;
+            HIPCHK(hipMemcpyAsync(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost, c->stream));
+            if (s > 0) HIPCHK(hipMemcpyAsync(wb.data(), c->cbox.p, sizeof(unsigned int) * wb.size(), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (size_t k = 0; k < nb; k++) {
+                v2[k] = u2f(w[k * BF_N + BF_V2]);
+                const unsigned int* x = &wb[k * BF_N];
+                const f3 mn = s > 0 ? mk(fkey_inv(x[BF_MN]), fkey_inv(x[BF_MN + 1]), fkey_inv(x[BF_MN + 2])) : c->bodies[k].mn;
+                const f3 mx = s > 0 ? mk(fkey_inv(x[BF_MX]), fkey_inv(x[BF_MX + 1]), fkey_inv(x[BF_MX + 2])) : c->bodies[k].mx;
+                box[6 * k] = mn.x, box[6 * k + 1] = mn.y, box[6 * k + 2] = mn.z;
+                box[6 * k + 3] = mx.x, box[6 * k + 4] = mx.y, box[6 * k + 5] = mx.z;
+            }
+            if ((rc = c->contacts.rebuild(a, c->hv, v2, box, c->stream, c->parity))) {
+                g_last_error = "pair rebuild failed";
+                return rc;
+            }
+            if (c->contacts.unsupported) {
+                g_last_error = "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not stepped "
+                               "on the GPU), or a capsule meets more than 128 triangles";
+                return BL_ERR_UNSUPPORTED;
+            }
+            next = s + interval;
+        }
+        if ((rc = c->contacts.collide(a, c->stream, c->parity))) {
+            g_last_error = "contact narrow phase or response failed";
+            return rc;
+        }
+        if ((rc = launch_integrate(a, c->stream, c->parity, t))) return rc;
+        c->parity ^= 1;
+    }
+    c->substeps += substeps;
+    if ((rc = launch_frame_end(a, c->stream, c->parity))) return rc;
+    c->frame_pending = true;
+    return BL_OK;
+}
+
+// Whether this frame has an island that searches for pairs (World::build_islands on the bodies as the last frame
+// left them).
+bool needs_contacts(bl_ctx* c, int substeps) {
+    if (!c->contact_geometry || !c->s.inter_body_collisions) return false;
+    for (size_t k = 0; k < c->bodies.size(); k++) {
+        c->hv.max_speed[k] = c->bodies[k].max_speed;
+        c->hv.mn[k] = c->bodies[k].mn, c->hv.mx[k] = c->bodies[k].mx;
+    }
+    c->contacts.build_islands(c->hv, (float)substeps * c->s.dt);
+    return !c->contacts.pair_islands.empty();
+}
+
 // Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.
 int enqueue_frame(bl_ctx* c, int substeps) {
     if (substeps <= 0) return BL_OK;
     int rc;
+    if (needs_contacts(c, substeps)) return contact_frame(c, substeps);
     if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
     if ((rc = enqueue(c, substeps))) return rc;
     if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
@@ -542,12 +680,14 @@ int post_frame(bl_ctx* c) {
     std::vector<size_t> trouble;
     const float lim = 400.0f * 0.9f;
     for (size_t k = 0; k < nb; k++) {
-        if (c->bodies[k].nodes.empty()) continue;
+        HostBody& hb = c->bodies[k];
+        if (hb.nodes.empty()) continue;
         const unsigned int* x = &w[k * BF_N];
         const f3 mn = mk(fkey_inv(x[BF_MN]), fkey_inv(x[BF_MN + 1]), fkey_inv(x[BF_MN + 2]));
         const f3 mx = mk(fkey_inv(x[BF_MX]), fkey_inv(x[BF_MX + 1]), fkey_inv(x[BF_MX + 2]));
         const f3 e = sub(mx, mn);
         const float ms = std::sqrt(u2f(x[BF_V2]));
+        hb.max_speed = ms, hb.mn = mn, hb.mx = mx; // (the next frame's islands start from these)
         if (!std::isfinite(mn.x) || !std::isfinite(mx.y) || smax(e.x, smax(e.y, e.z)) > 800.0f || ms > lim)
             trouble.push_back(k);
     }
@@ -557,12 +697,20 @@ int post_frame(bl_ctx* c) {
     for (size_t k : trouble) {
         HostBody& hb = c->bodies[k];
         const int fixed = repair_body(hb);
-        if (fixed < 0)
+        if (fixed > 0) {
+            // bounds and top speed of the repaired body (src/phys/world.cpp:4106-4109)
+            hb.compute_bounds();
+            hb.max_speed = 0;
+            for (const bl_node& x : hb.nodes) hb.max_speed = smax(hb.max_speed, length(mk(x.v[0], x.v[1], x.v[2])));
+        }
+        if (fixed < 0) {
             for (bl_node& x : hb.nodes) {
                 if (!std::isfinite(x.p[0]) || !std::isfinite(x.p[1]) || !std::isfinite(x.p[2])) x.p[0] = 0, x.p[1] = -1000, x.p[2] = 0;
                 x.v[0] = x.v[1] = x.v[2] = 0;
                 x.inv_mass = 0;
             }
+            hb.compute_bounds();
+        }
         c->repairs++;
     }
     // the repaired state goes back; the counters survive the rebuild
@@ -766,6 +914,22 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
             g_last_error = "tyre nodes are not stepped on the GPU";
             return BL_ERR_UNSUPPORTED;
         }
+    for (uint32_t t = 0; t < d->n_tris; t++) {
+        const bl_triangle& tr = d->tris[t];
+        if (tr.a >= d->n_nodes || tr.b >= d->n_nodes || tr.c >= d->n_nodes) return BL_ERR_ARG;
+        if (!tr.two_sided) {
+            g_last_error = "hull (one-sided) triangles are not stepped on the GPU";
+            return BL_ERR_UNSUPPORTED;
+        }
+    }
+    for (uint32_t q = 0; q < d->n_capsules; q++) {
+        const bl_capsule& cp = d->capsules[q];
+        if (cp.a >= d->n_nodes || cp.b >= d->n_nodes) return BL_ERR_ARG;
+        if (cp.joint >= 0) {
+            g_last_error = "capsules tied to joints are not stepped on the GPU";
+            return BL_ERR_UNSUPPORTED;
+        }
+    }
     int rc = post_frame(c);
     if (rc) return rc;
     rc = pull_state(c);
@@ -777,6 +941,7 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     if (d->n_tris) hb.tris.assign(d->tris, d->tris + d->n_tris);
     if (d->n_capsules) hb.caps.assign(d->capsules, d->capsules + d->n_capsules);
     hb.opts = d->opts;
+    hb.compute_bounds(); // (SoftBody::compute_aabb at World::add_body, src/phys/world.cpp:283-289)
     c->bodies.push_back(std::move(hb));
     c->release();
     if (id) *id = (uint32_t)c->bodies.size() - 1;
@@ -864,6 +1029,10 @@ int bl_get_stats(bl_ctx* c, bl_stats* out) {
     out->static_contacts = (int64_t)cnt[CNT_STATIC];
     out->vel_clamps = (int64_t)cnt[CNT_CLAMPS];
     out->nan_resets = (int64_t)cnt[CNT_NANS];
+    out->contacts = (int64_t)cnt[CNT_CONTACTS];
+    out->pair_rebuilds = c->contacts.pair_rebuilds;
+    out->contact_pairs = c->contacts.contact_pairs;
+    out->narrow_tests = c->contacts.narrow_tests;
     for (unsigned long long b : brk) out->broken_beams += (int64_t)b;
     unsigned int init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0, 0, 0};
     HIPCHK(hipMemcpy(c->bounds.p, init, sizeof(init), hipMemcpyHostToDevice));
@@ -883,6 +1052,7 @@ int bl_get_stats(bl_ctx* c, bl_stats* out) {
 int bl_reset_stats(bl_ctx* c) {
     if (!c) return BL_ERR_ARG;
     c->substeps = 0;
+    c->contacts.pair_rebuilds = c->contacts.contact_pairs = c->contacts.narrow_tests = 0;
     if (!c->built) return BL_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
@@ -944,6 +1114,17 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
     t.mode = resolved_mode(c);
     t.incidences_per_thread = effective_per(c);
     if (t.block != c->built_block || t.incidences_per_thread != c->built_per) return BL_ERR_STATE;
+    if (needs_contacts(c, substeps)) {
+        // (a contact frame waits on the host at its rebuilds: its wall time is the only figure)
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        if ((rc = contact_frame(c, substeps))) return rc;
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        HIPCHK(hipEventElapsedTime(total_ms, c->ev0, c->ev1));
+        *kernel_ms = *total_ms;
+        *launches = substeps;
+        return post_frame(c);
+    }
     const int per = t.mode == BL_MODE_PERSISTENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : 64)
                     : (t.mode == BL_MODE_RESIDENT || t.mode == BL_MODE_HALO)
                         ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : substeps) : 1;

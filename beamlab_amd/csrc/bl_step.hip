@@ -223,7 +223,7 @@ __device__ __forceinline__ f3 node_force(const KArgs& a, float mass, uint32_t b0
 // ---------------------------------------------------------------------------------------------- streamed chunk
 // One chunk's substep with the incidence records streamed from memory: all of a thread's loads are issued before
 // its arithmetic (PER incidences per thread).
-template <int T, int PER>
+template <int T, int PER, bool FORCES_ONLY = false>
 __device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int parity, float* sfx, float* sfy,
                                               float* sfz, uint8_t* sok, NodeCounts& cnt) {
     const float4* __restrict__ pin = a.pm[parity];
@@ -266,6 +266,10 @@ __device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int pa
         float4 P = pin[i], V = vin[i];
         const uint32_t b0 = a.noff[i] - e0, b1 = a.noff[i + 1] - e0;
         const f3 f = node_force(a, V.w, b0, b1, sfx, sfy, sfz, sok);
+        if (FORCES_ONLY) {
+            a.F[i] = make_float4(f.x, f.y, f.z, 0.0f);
+            continue;
+        }
         node_finish(a, a.ninfo[i], a.naux[i], P, V, f, cnt);
         pout[i] = P, vout[i] = V;
     }
@@ -767,6 +771,12 @@ int launch_frame_end(const KArgs& a, hipStream_t st, int parity) {
     return BL_OK_OR_HIP();
 }
 
+int launch_box(const KArgs& a, hipStream_t st, int parity) {
+    hipLaunchKernelGGL(k_frame_clear, dim3((a.nbodies + 63) / 64), dim3(64), 0, st, a);
+    if (a.nnodes) hipLaunchKernelGGL(k_frame_end, dim3((a.nnodes + 255) / 256), dim3(256), 0, st, a, parity);
+    return BL_OK_OR_HIP();
+}
+
 int launch_chunk_clear(const KArgs& a, hipStream_t st) {
     if (a.nchunks) hipLaunchKernelGGL(k_chunk_clear, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
     return BL_OK_OR_HIP();
@@ -805,6 +815,69 @@ int launch_halo(const KArgs& a, hipStream_t st, int parity, int substeps, const 
     if (t.block == 256 && per == 8) return launch_halo_t<256, 8, 1024, 2>(a, st, parity, substeps, flags, err, spin_limit);
     if (t.block == 256 && per == 4) return launch_halo_t<256, 4, 512, 4>(a, st, parity, substeps, flags, err, spin_limit);
     return BL_ERR_ARG;
+}
+
+
+// ---------------------------------------------------------------------------------------------- split substep
+// The force phase alone (the inter-body contacts come between it and the integration): F = g m + beams.
+template <int T, int PER>
+__global__ __launch_bounds__(T) void k_forces(KArgs a, int parity) {
+    __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
+    __shared__ uint8_t sok[T * PER];
+    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    NodeCounts cnt = {0, 0, 0, 0.0f};
+    chunk_substep<T, PER, true>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+}
+
+// The static contacts and the integration of a chunk's nodes from F.
+template <int T>
+__global__ __launch_bounds__(T) void k_integrate(KArgs a, int parity) {
+    __shared__ uint4 sred[T / 64];
+    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
+    NodeCounts cnt = {0, 0, 0, 0.0f};
+    for (uint32_t i = n0 + threadIdx.x; i < n1; i += T) {
+        float4 P = a.pm[parity][i], V = a.vm[parity][i];
+        const float4 F = a.F[i];
+        node_finish(a, a.ninfo[i], a.naux[i], P, V, mk(F.x, F.y, F.z), cnt);
+        a.pm[parity ^ 1][i] = P, a.vm[parity ^ 1][i] = V;
+    }
+    flush_chunk<T>(a, cnt, c, sred);
+}
+
+// The chunks' top speeds so far onto their bodies (the running top speed the pair margins need mid-frame; the
+// counts stay for the frame's end).
+__global__ void k_fold_speed(KArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nchunks) return;
+    const uint4 t = a.cstat[c];
+    if (u2f(t.w) > 0) atomicMax(&a.body_frame[a.chunk_body[c] * BF_N + BF_V2], t.w);
+}
+
+int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
+    const int per = t.incidences_per_thread, T = t.block;
+    if (T == 256 && per == 1) hipLaunchKernelGGL((k_forces<256, 1>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 2) hipLaunchKernelGGL((k_forces<256, 2>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 4) hipLaunchKernelGGL((k_forces<256, 4>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 8) hipLaunchKernelGGL((k_forces<256, 8>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
+    else if (T == 512 && per == 2) hipLaunchKernelGGL((k_forces<512, 2>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
+    else if (T == 512 && per == 4) hipLaunchKernelGGL((k_forces<512, 4>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
+    else if (T == 1024 && per == 4) hipLaunchKernelGGL((k_forces<1024, 4>), dim3(a.nchunks), dim3(1024), 0, st, a, parity);
+    else return BL_ERR_ARG;
+    return BL_OK_OR_HIP();
+}
+
+int launch_integrate(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
+    if (t.block == 256) hipLaunchKernelGGL((k_integrate<256>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
+    else if (t.block == 512) hipLaunchKernelGGL((k_integrate<512>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
+    else if (t.block == 1024) hipLaunchKernelGGL((k_integrate<1024>), dim3(a.nchunks), dim3(1024), 0, st, a, parity);
+    else return BL_ERR_ARG;
+    return BL_OK_OR_HIP();
+}
+
+int launch_fold_speed(const KArgs& a, hipStream_t st) {
+    if (a.nchunks) hipLaunchKernelGGL(k_fold_speed, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
+    return BL_OK_OR_HIP();
 }
 
 // Incidences of a chunk: block x incidences per thread (the node cap of a chunk is the block).

@@ -20,7 +20,7 @@ enum : uint32_t {
 };
 constexpr int TF_TYPE_SHIFT = 6;
 
-enum { CNT_STATIC = 0, CNT_CLAMPS = 1, CNT_NANS = 2, CNT_N = 4 };
+enum { CNT_STATIC = 0, CNT_CLAMPS = 1, CNT_NANS = 2, CNT_CONTACTS = 3, CNT_N = 4 };
 // per-body frame words: [0] top speed squared (float bits), [1..3] min key x y z, [4..6] max key x y z
 enum { BF_V2 = 0, BF_MN = 1, BF_MX = 4, BF_N = 8 };
 
@@ -32,7 +32,9 @@ BL_HD uint32_t fkey(float f) {
 BL_HD float fkey_inv(uint32_t k) { return u2f((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
 
 struct BodyParam {
-    float bounce, push_max, slop, pad;
+    float bounce, push_max, slop;
+    float collision_radius, contact_friction; // inter-body contacts
+    int32_t group, self_coll, pad;
 };
 
 struct KArgs {
@@ -63,6 +65,14 @@ struct KArgs {
     const uint32_t* nbr_off;
     const uint32_t* nbr;
     const uint32_t* iloc;
+    // inter-body contacts: the node forces between the force phase and the integration, the collision triangles
+    // and capsules with global node indices
+    float4* F;
+    const uint4* tri_abc;  // a, b, c, body
+    const float* tri_rest2;
+    uint32_t* tri_torn;
+    const uint4* cap_data; // a, b, body, radius (bits)
+    uint32_t ntris, ncaps;
     f3 g;
     float dt;
     float kill_y;
@@ -85,6 +95,12 @@ int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out)
 int launch_frame_begin(const KArgs& a, hipStream_t st, int parity);
 int launch_frame_end(const KArgs& a, hipStream_t st, int parity);
 int launch_chunk_clear(const KArgs& a, hipStream_t st);
+// each body's bounds of the current state into a.body_frame's bound words (the caller points it at its own buffer)
+int launch_box(const KArgs& a, hipStream_t st, int parity);
+// the substep split for inter-body contacts: forces into F, then the static contacts and the integration from F
+int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t);
+int launch_integrate(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t);
+int launch_fold_speed(const KArgs& a, hipStream_t st);
 int chunk_capacity(int block, int per);
 
 } // namespace blg

@@ -1,0 +1,85 @@
+"""Inter-body and self contacts on the GPU against the oracle, bit for bit (node p and v, beam state, break
+counts) at several checkpoints, with the contact counters (pair rebuilds, candidate pairs, narrow-phase tests,
+contacts) equal too; and the refusals of what the GPU path does not step."""
+import numpy as np
+import pytest
+
+from _cases import assert_same, run_world
+from _contact_cases import CONTACT_CASES
+from beamlab_amd import abi, scenes
+from beamlab_amd.world import World
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = {
+    "auto": dict(mode=abi.MODE_AUTO),
+    "stepped_1": dict(mode=abi.MODE_STEPPED, block=256, use_graph=1, incidences_per_thread=1),
+    "persistent": dict(mode=abi.MODE_PERSISTENT, block=1024, substeps_per_launch=50),
+}
+COUNTERS = ("contacts", "pair_rebuilds", "contact_pairs", "narrow_tests", "static_contacts", "broken_beams")
+
+
+def gpu_world(settings, boxes, tuning):
+    w = World(settings)
+    w.statics.boxes = boxes
+    w.set_tuning(**tuning)
+    return w
+
+
+@pytest.mark.parametrize("tuning", sorted(TUNINGS))
+@pytest.mark.parametrize("name", sorted(CONTACT_CASES))
+def test_gpu_contacts_match_oracle(name, tuning, oracle_mod):
+    print(f"contact case {name} tuning {tuning}", flush=True)
+    settings, boxes, bodies, n = CONTACT_CASES[name]()
+    cps = sorted({1, 17, 40, n // 2, n})
+    ow = oracle_mod.OracleWorld(settings, boxes)
+    so = run_world(ow, bodies, n, cps)
+    gw = gpu_world(settings, boxes, TUNINGS[tuning])
+    sg = run_world(gw, bodies, n, cps)
+    assert_same(sg, so, f"gpu[{tuning}] vs oracle [{name}]")
+    st_o, st_g = ow.stats(), gw.stats()
+    for key in COUNTERS:
+        assert st_g[key] == st_o[key], (key, st_g[key], st_o[key])
+    assert st_g["contacts"] > 100
+
+
+def test_contacts_long_frames(oracle_mod):
+    """Frames longer than the pair interval (several rebuilds inside one call) and a frame of one substep."""
+    settings, boxes, bodies, _ = CONTACT_CASES["lattice_pile"]()
+    cps = [1, 2, 100, 333, 700]
+    ow = oracle_mod.OracleWorld(settings, boxes)
+    so = run_world(ow, bodies, 700, cps)
+    gw = gpu_world(settings, boxes, TUNINGS["auto"])
+    sg = run_world(gw, bodies, 700, cps)
+    assert_same(sg, so, "gpu vs oracle [lattice_pile, long frames]")
+    assert gw.stats()["pair_rebuilds"] == ow.stats()["pair_rebuilds"]
+
+
+def test_fast_body_refused():
+    """A body faster than 8 m/s in a contact island has its own pair schedule in the reference: refused."""
+    settings = abi.default_settings()
+    w = World(settings)
+    scenes.ground_box(w)
+    lat = scenes.lattice_body(4, 3, 3, 0.25, 1.0, (0.0, 0.3, 0.0), seed=5, faces=True)
+    ball = scenes.ball_body((0.0, 1.5, 0.0), 0.1, 2.0)
+    ball.nodes["v"][0] = (0.0, -20.0, 0.0)
+    w.add_body(lat)
+    w.add_body(ball)
+    with pytest.raises(abi.BeamlabError) as e:
+        for _ in range(3):  # (the first frame's islands see the bodies apart: top speed 0 at add_body)
+            w.step_substeps(40)
+    assert e.value.status == abi.BL_ERR_UNSUPPORTED
+
+
+def test_hull_triangles_and_jointed_capsules_refused():
+    w = World(abi.default_settings())
+    lat = scenes.lattice_body(3, 2, 2, 0.25, 1.0, (0.0, 0.3, 0.0), seed=5, faces=True)
+    lat.tris["two_sided"] = 0
+    with pytest.raises(abi.BeamlabError) as e:
+        w.add_body(lat)
+    assert e.value.status == abi.BL_ERR_UNSUPPORTED
+    ball = scenes.ball_body((0.0, 1.5, 0.0), 0.1, 2.0)
+    ball.capsules["joint"] = 0
+    with pytest.raises(abi.BeamlabError) as e:
+        w.add_body(ball)
+    assert e.value.status == abi.BL_ERR_UNSUPPORTED
