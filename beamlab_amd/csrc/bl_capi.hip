@@ -84,7 +84,7 @@ struct bl_ctx {
     DevArr<float4> pm[2], vm[2], iA, iM, bcold;
     DevArr<uint32_t> ninfo, noff, ch_n0, body_c0, chunk_body;
     DevArr<uint4> cstat;
-    DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags;
+    DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags, wg_chunk;
     DevArr<float2> naux;
     DevArr<uint2> iAB;
     DevArr<float> stress;
@@ -99,6 +99,7 @@ struct bl_ctx {
     DevArr<float> tri_rest2;
     DevArr<uint32_t> tri_torn;
     DevArr<unsigned int> cbox;
+    DevArr<uint4> wstat; // per workgroup of the node-parallel integration: its counts of the frame
     ContactEngine contacts;
     HostView hv;
     bool contact_geometry = false; // some body has triangles, capsules or self collision
@@ -123,7 +124,9 @@ struct bl_ctx {
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
         halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
+        wg_chunk.release();
         F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), cbox.release();
+        wstat.release();
         contacts.release();
         frame_pending = false;
         built = false;
@@ -340,6 +343,38 @@ int build(bl_ctx* c) {
     }
     body_c0.push_back((uint32_t)ch_n0.size());
     const uint32_t nchunks = (uint32_t)ch_n0.size();
+    // workgroup -> chunk: XCD x runs workgroups x, x + 8, ...; a body of 8 chunks or more is split into 8
+    // contiguous ranges, one per XCD (the longer ranges rotating over the XCDs from body to body), smaller bodies
+    // are dealt chunk by chunk; lists longer than their XCD's share hand their tails to the short ones
+    std::vector<std::vector<uint32_t>> xl(8);
+    {
+        uint32_t rot = 0;
+        for (size_t k = 0; k + 1 < body_c0.size(); k++) {
+            const uint32_t c0 = body_c0[k], n = body_c0[k + 1] - c0;
+            if (n >= 8) {
+                uint32_t at = c0;
+                for (uint32_t q = 0; q < 8; q++) {
+                    const uint32_t x = (rot + q) % 8, len = n / 8 + (q < n % 8 ? 1u : 0u);
+                    for (uint32_t j = 0; j < len; j++) xl[x].push_back(at++);
+                }
+                rot = (rot + n % 8) % 8;
+            } else {
+                for (uint32_t j = 0; j < n; j++) xl[rot].push_back(c0 + j), rot = (rot + 1) % 8;
+            }
+        }
+        std::vector<uint32_t> spill;
+        for (uint32_t x = 0; x < 8; x++) {
+            const size_t share = nchunks > x ? (nchunks - x + 7) / 8 : 0;
+            while (xl[x].size() > share) spill.push_back(xl[x].back()), xl[x].pop_back();
+        }
+        for (uint32_t x = 0; x < 8; x++) {
+            const size_t share = nchunks > x ? (nchunks - x + 7) / 8 : 0;
+            while (xl[x].size() < share) xl[x].push_back(spill.back()), spill.pop_back();
+        }
+    }
+    std::vector<uint32_t> wg_chunk(nchunks);
+    for (uint32_t x = 0; x < 8; x++)
+        for (size_t j = 0; j < xl[x].size(); j++) wg_chunk[j * 8 + x] = xl[x][j];
     ch_n0.push_back(N);
     c->h_noff = noff;
     // halos: per chunk the outside nodes its incidences read (sorted), the chunks owning them, each incidence's
@@ -388,11 +423,11 @@ int build(bl_ctx* c) {
         (rc = c->halo.upload(halo)) || (rc = c->nbr_off.upload(nbr_off)) || (rc = c->nbr.upload(nbr)) ||
         (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))) ||
         (rc = c->tri_abc.upload(tri_abc)) || (rc = c->tri_rest2.upload(tri_rest2)) || (rc = c->tri_torn.upload(tri_torn)) ||
-        (rc = c->cap_data.upload(cap_data)) ||
-        (rc = c->F.alloc(c->contact_geometry ? N : 0)) ||
+        (rc = c->cap_data.upload(cap_data)) || (rc = c->wg_chunk.upload(wg_chunk)) ||
+        (rc = c->F.alloc(c->contact_geometry ? N : 0)) || (rc = c->wstat.alloc((N + 255) / 256 + 1)) ||
         (rc = c->cbox.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
         return rc;
-    if (c->contact_geometry && (rc = c->contacts.setup(c->hv))) {
+    if (c->contact_geometry && (rc = c->contacts.setup(c->hv, N, NT, NC))) {
         g_last_error = "contact buffers: device allocation failed";
         return rc;
     }
@@ -400,6 +435,7 @@ int build(bl_ctx* c) {
     HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
     HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
     HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
+    HIPCHK(hipMemset(c->wstat.p, 0, sizeof(uint4) * c->wstat.n));
     KArgs& a = c->args;
     a = KArgs{};
     a.pm[0] = c->pm[0].p, a.pm[1] = c->pm[1].p, a.vm[0] = c->vm[0].p, a.vm[1] = c->vm[1].p;
@@ -414,7 +450,7 @@ int build(bl_ctx* c) {
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
-    a.xcd_remap = 1;
+    a.wg_chunk = c->wg_chunk.p;
     c->parity = 0;
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;
@@ -578,7 +614,7 @@ int repair_body(HostBody& b) {
 // into F, the pair lists rebuilt at the first substep and every `interval` after (from the state at that point),
 // the narrow phase and the response on F, then the static contacts and the integration. The host waits at each
 // rebuild (list sizes) - about one substep in 17 at the default 120 Hz pair rate.
-int contact_frame(bl_ctx* c, int substeps) {
+int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     bl_tuning t = c->tune;
     t.block = c->built_block, t.incidences_per_thread = c->built_per;
     const double hz = (double)smax(1.0f, c->s.collision_hz);
@@ -590,34 +626,33 @@ int contact_frame(bl_ctx* c, int substeps) {
     KArgs ab = a;
     ab.body_frame = c->cbox.p;
     int rc;
+    if ((rc = c->contacts.begin_frame(c->stream))) return rc;
+    // the bounds the first rebuild reads: as the last frame left them
+    std::vector<unsigned int> w0(nb * BF_N, 0);
+    for (size_t k = 0; k < nb; k++) {
+        const HostBody& hb = c->bodies[k];
+        const float mn[3] = {hb.mn.x, hb.mn.y, hb.mn.z}, mx[3] = {hb.mx.x, hb.mx.y, hb.mx.z};
+        for (int q = 0; q < 3; q++) w0[k * BF_N + BF_MN + q] = fkey(mn[q]), w0[k * BF_N + BF_MX + q] = fkey(mx[q]);
+    }
+    HIPCHK(hipMemcpyAsync(c->cbox.p, w0.data(), sizeof(unsigned int) * w0.size(), hipMemcpyHostToDevice, c->stream));
     if ((rc = launch_frame_begin(a, c->stream, c->parity))) return rc;
-    std::vector<unsigned int> w(nb * BF_N), wb(nb * BF_N);
-    std::vector<float> v2(nb), box(nb * 6);
     int next = 0;
     for (int s = 0; s < substeps; s++) {
+        if (kev) HIPCHK(hipEventRecord(kev[2 * s], c->stream));
         if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
+        if (kev) HIPCHK(hipEventRecord(kev[2 * s + 1], c->stream));
         if (s == 0 || s >= next) {
-            // each body's top speed so far (the frame's start included) and bounds now (at the frame's start:
-            // as the last frame left them)
-            if (s > 0 && ((rc = launch_fold_speed(a, c->stream)) || (rc = launch_box(ab, c->stream, c->parity)))) return rc;
-            HIPCHK(hipMemcpyAsync(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost, c->stream));
-            if (s > 0) HIPCHK(hipMemcpyAsync(wb.data(), c->cbox.p, sizeof(unsigned int) * wb.size(), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            for (size_t k = 0; k < nb; k++) {
-                v2[k] = u2f(w[k * BF_N + BF_V2]);
-                const unsigned int* x = &wb[k * BF_N];
-                const f3 mn = s > 0 ? mk(fkey_inv(x[BF_MN]), fkey_inv(x[BF_MN + 1]), fkey_inv(x[BF_MN + 2])) : c->bodies[k].mn;
-                const f3 mx = s > 0 ? mk(fkey_inv(x[BF_MX]), fkey_inv(x[BF_MX + 1]), fkey_inv(x[BF_MX + 2])) : c->bodies[k].mx;
-                box[6 * k] = mn.x, box[6 * k + 1] = mn.y, box[6 * k + 2] = mn.z;
-                box[6 * k + 3] = mx.x, box[6 * k + 4] = mx.y, box[6 * k + 5] = mx.z;
-            }
-            if ((rc = c->contacts.rebuild(a, c->hv, v2, box, c->stream, c->parity))) {
+            // each body's top speed so far (the frame's start included) and, after the first substep, its bounds now
+            if (s > 0 && (rc = launch_box(ab, c->stream, c->parity))) return rc;
+            if ((rc = c->contacts.rebuild(a, c->cbox.p, c->stream, c->parity))) {
                 g_last_error = "pair rebuild failed";
                 return rc;
             }
-            if (c->contacts.unsupported) {
-                g_last_error = "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not stepped "
-                               "on the GPU), or a capsule meets more than 128 triangles";
+            if (c->contacts.last.unsupported) {
+                g_last_error = (c->contacts.last.unsupported & 1)
+                                   ? "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not "
+                                     "stepped on the GPU)"
+                                   : "a capsule meets more than 128 triangles in one pair rebuild";
                 return BL_ERR_UNSUPPORTED;
             }
             next = s + interval;
@@ -626,11 +661,11 @@ int contact_frame(bl_ctx* c, int substeps) {
             g_last_error = "contact narrow phase or response failed";
             return rc;
         }
-        if ((rc = launch_integrate(a, c->stream, c->parity, t))) return rc;
+        if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p))) return rc;
         c->parity ^= 1;
     }
     c->substeps += substeps;
-    if ((rc = launch_frame_end(a, c->stream, c->parity))) return rc;
+    if ((rc = launch_wstat_fold(a, c->stream, c->wstat.p)) || (rc = launch_frame_end(a, c->stream, c->parity))) return rc;
     c->frame_pending = true;
     return BL_OK;
 }
@@ -1115,13 +1150,24 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
     t.incidences_per_thread = effective_per(c);
     if (t.block != c->built_block || t.incidences_per_thread != c->built_per) return BL_ERR_STATE;
     if (needs_contacts(c, substeps)) {
-        // (a contact frame waits on the host at its rebuilds: its wall time is the only figure)
+        // a contact frame: the beam kernel (the force phase) of every substep between events
+        while ((int)c->kev.size() < 2 * substeps) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c->kev.push_back(e);
+        }
         HIPCHK(hipEventRecord(c->ev0, c->stream));
-        if ((rc = contact_frame(c, substeps))) return rc;
+        if ((rc = contact_frame(c, substeps, c->kev.data()))) return rc;
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         HIPCHK(hipEventSynchronize(c->ev1));
         HIPCHK(hipEventElapsedTime(total_ms, c->ev0, c->ev1));
-        *kernel_ms = *total_ms;
+        float sum = 0;
+        for (int l = 0; l < substeps; l++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, c->kev[2 * l], c->kev[2 * l + 1]));
+            sum += ms;
+        }
+        *kernel_ms = sum;
         *launches = substeps;
         return post_frame(c);
     }

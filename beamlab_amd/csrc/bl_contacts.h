@@ -17,29 +17,74 @@ struct Hit {
     float pen;
 };
 
-// The device tables of one island's pair rebuild.
+// A pair island's search parameters of the current rebuild (rebuild_pairs, src/phys/world.cpp:1879-1935).
+struct IslandPar {
+    float margin, inv_cell, rmax, reach;
+};
+
+// What a rebuild leaves for the host: list sizes (needed sizes when a list overflowed its buffer) and flags.
+struct RStat {
+    uint32_t npts, nnt_raw, nnc, nct, nent, nnt;
+    uint32_t overflow;    // bit 0 NT, 1 NC, 2 CT, 3 entries, 4 partners
+    uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles
+    uint32_t last_pairs;  // candidate pairs kept in the last pair island
+    uint32_t npart;       // partner entries (when they overflowed their buffer)
+    uint32_t nbig;        // hash buckets of more than 16 points
+};
+
+// The device tables of a rebuild (all pair islands at once: one hash table, its keys salted by island).
 struct CArgs {
     int parity;
-    float margin, inv_cell, rmax, horizon;
-    const int* island_local;     // per body: its index in the island, -1 outside
-    const uint32_t* query;       // per island body: has a partner or collides with itself
-    const uint32_t* self;        // per island body: self collision
-    const uint32_t* island_body; // per island body: its body
-    const float* boxes;          // per island body: the partner-search box (mn xyz, mx xyz)
-    const uint32_t* part_off;    // per island body: its partners (island indices), CSR
-    const uint32_t* part;
-    const uint32_t* body_tri0;   // per body: its first triangle
-    const uint32_t* sample_base; // per body: its first size sample
-    float* samples;              // every 7th triangle's longest edge
-    uint32_t* overflow;          // set when a capsule meets more triangles than it keeps
-    uint32_t *flag, *scan, *scan2, *count;
-    uint32_t npts, nprims, mask;
-    uint64_t* keys;
-    int4* pt_cell;               // hashed nodes: cell x y z, node
-    float4* pt_pos;              // hashed nodes: position, island body (bits)
-    uint32_t *bstart, *items, *run_end;
-    uint32_t *entry_pid, *prim_tri;
-    uint4 *nt, *nc, *ct;
+    float horizon;
+    const int* bisl;          // per body: its pair island, -1 when none
+    const uint32_t* imem_off; // per pair island: its bodies (CSR, ascending)
+    const uint32_t* imem;
+    uint32_t nisl, nmem;
+    const int* bmem;          // per body: its index in imem, -1 outside the pair islands
+    uint32_t *pcnt, *poff, *plist; // per island body: its partners (bodies), CSR
+    uint32_t cap_part;
+    int interval;
+    float dt;
+    IslandPar* ipar;
+    float* bbox;              // per body: the partner box (bounds +- the island's reach)
+    uint32_t* bquery;         // per body: has a partner or collides with itself
+    const uint32_t* body_box; // per body: bounds keys (BF_MN, BF_MX words of a frame record)
+    const uint32_t* body_tri0;
+    const uint32_t* sample_base;
+    float* samples;           // every 7th triangle's longest edge, in body order
+    const uint32_t* sbody;    // per sample: its body
+    uint32_t nsamples;
+    RStat* st;
+    uint32_t* icount;         // per pair island: candidate pairs kept (NT after the filter, NC, CT)
+    // hashed nodes
+    uint32_t *pflag, *pscan, *pbucket;
+    int4* pcell;   // cell x y z, node
+    float4* ppos;  // position, body (bits)
+    uint32_t tmask;
+    uint32_t *tcount, *tstart, *tfill, *titems;
+    int4* scell;   // the hashed nodes' records in bucket order
+    float4* spos;
+    uint32_t* big; // buckets left to the wave sort
+    // node-triangle, node-capsule candidates
+    uint32_t *tcnt, *tscan, *ccnt, *cscan;
+    uint4 *ntraw, *nc;
+    uint32_t cap_nt, cap_nc;
+    // capsule-triangle: triangle cells, their table, the candidates
+    uint32_t *ecnt, *escan, *eb;
+    uint4* ea;     // per entry: triangle, body, group, island
+    float4* ebb;   // per entry: raw bounds (min x y z, max x)
+    float2* ec;    // per entry: raw bounds (max y z)
+    uint4* sa;     // the entries in bucket order
+    float4* sb;
+    float2* sc;
+    uint32_t cap_ent, emask;
+    uint32_t *ecount, *estart, *efill;
+    uint32_t *ctcnt, *ctscan, *ctslot;
+    uint4* ct;
+    uint32_t cap_ct;
+    // the near-pair filter
+    uint32_t *nflag, *nscan;
+    uint4* nt;
 };
 
 // The candidate lists of a frame (all pair islands) and the narrow phase's buffers.
@@ -48,12 +93,12 @@ struct PArgs {
     const uint4* nc; // node body, capsule body, node, capsule
     const uint4* ct; // capsule body, triangle body, capsule, triangle
     uint32_t nnt, nnc, nct;
-    uint32_t *hflag, *hscan;
-    Hit *hraw, *hits;
-    uint32_t* nhits;
+    Hit* hraw;       // per block of 256 candidates: its hits first, in order
+    uint32_t* bcnt;  // per block: its hit count
+    uint32_t* bpre;  // per block: the hits before it (the response fills it)
 };
 
-// What the island and pair building read of a body on the host.
+// What the island building reads of a body on the host.
 struct BodyView {
     int32_t group;
     uint32_t self, contacters, ntri, ncap, tri0;
@@ -89,42 +134,52 @@ struct Buf {
 
 class ContactEngine {
 public:
-    // per frame
-    std::vector<std::vector<uint32_t>> islands, pair_islands;
+    std::vector<std::vector<uint32_t>> islands, pair_islands; // this frame's
     int interval = 1;
-    bool unsupported = false;
+    RStat last{};
     // counters (bl_stats)
     int64_t pair_rebuilds = 0, contact_pairs = 0, narrow_tests = 0;
 
-    int setup(const HostView& hv);
+    int setup(const HostView& hv, uint32_t nnodes, uint32_t ntris, uint32_t ncaps);
     void build_islands(const HostView& hv, float frame_time);
-    // the pair lists anew; body_v2 / body_box: each body's top speed squared so far and its current bounds
-    int rebuild(const KArgs& a, const HostView& hv, const std::vector<float>& body_v2, const std::vector<float>& body_box,
-                hipStream_t st, int parity);
+    // the frame's pair-island tables to the device (after build_islands)
+    int begin_frame(hipStream_t st);
+    // the pair lists anew from the state of `parity`: bounds keys per body in `body_box`, top speeds in
+    // a.body_frame; waits for the device once (and again when a list outgrew its buffer)
+    int rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t st, int parity);
     int collide(const KArgs& a, hipStream_t st, int parity);
     void release();
     ~ContactEngine() { release(); }
 
 private:
+    uint32_t nnodes = 0, ntris = 0, ncaps = 0, nbodies = 0, nsamples = 0;
     size_t nnt = 0, nnc = 0, nct = 0;
-    uint32_t nsamples = 0;
+    uint32_t tsize = 0; // node table buckets
     Buf<uint8_t> temp;
-    Buf<uint32_t> flag, scanv, scan2, count, bstart, items, run_end, entry_pid, prim_tri, hflag, hscan, nhits, overflow;
-    Buf<uint32_t> isl_query, isl_self, isl_body, part_off, part, sample_base, body_tri0;
-    Buf<int> isl_local;
-    Buf<float> boxes, samples;
-    Buf<uint64_t> keys, sorted;
-    Buf<int4> pt_cell;
-    Buf<float4> pt_pos;
-    Buf<uint4> nt, nt2, nc, ct, all_nt, all_nc, all_ct;
-    Buf<Hit> hraw, hits;
+    Buf<int> bisl, bmem;
+    Buf<uint32_t> pcnt, poff, plist;
+    uint32_t nmem = 0, cap_part = 0;
+    Buf<uint32_t> imem_off, imem, bquery, body_tri0, sample_base, icount;
+    Buf<float> bbox, samples;
+    Buf<IslandPar> ipar;
+    Buf<RStat> stat;
+    Buf<uint32_t> pflag, pscan, pbucket, tcount, tstart, tfill, titems, sbody;
+    Buf<int4> pcell;
+    Buf<float4> ppos;
+    Buf<uint32_t> tcnt, tscan, ccnt, cscan, ecnt, escan, eb, ecount, estart, efill, ctcnt, ctscan;
+    Buf<int4> scell;
+    Buf<uint32_t> big, ctslot;
+    Buf<float4> spos, ebb, sb;
+    Buf<uint4> ea, sa;
+    Buf<float2> ec, sc;
+    Buf<uint32_t> nflag, nscan, bcnt, bpre;
+    Buf<uint4> ntraw, nc, ct, nt;
+    Buf<Hit> hraw;
+    uint32_t cap_nt = 0, cap_nc = 0, cap_ct = 0, cap_ent = 0, esize = 0;
 
     int scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t st);
-    int sort(const uint64_t* in, uint64_t* out, size_t n, int end_bit, hipStream_t st);
-    int total(const uint32_t* scan_out, const uint32_t* in, size_t n, uint32_t& tot, hipStream_t st);
-    int rebuild_island(const KArgs& a, const HostView& hv, const std::vector<uint32_t>& isl,
-                       const std::vector<float>& body_v2, const std::vector<float>& body_box,
-                       const std::vector<float>& samples, hipStream_t st, int parity, int64_t& island_pairs);
+    int ensure_lists();
+    int enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st);
 };
 
 } // namespace blg

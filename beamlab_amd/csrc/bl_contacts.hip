@@ -8,15 +8,15 @@
 // src/phys/world.cpp:23-47,1135-1142.
 //
 // Order is part of the result: the response of a contact cancels the forces accumulated before it. Every list is
-// built in the reference's order - candidates by triangle (body, then index), then hash cell (z, y, x), then
-// bucket position; a bucket holds its nodes by insertion (body, then node): a stable radix sort of (bucket,
-// insertion index) keys gives the same buckets; all compactions are order-preserving scans. The response runs in
-// one workgroup: contacts that share no node are independent, so each contact is placed at the dependency level
-// one past the latest earlier contact sharing a node with it (a sort of (node, contact) keys in LDS) and each level
-// runs in parallel - the same arithmetic, in the same per-node order, as the serial loop.
+// built in the reference's order - candidates by triangle (body, then index), then hash cell (z, y, x), then the
+// cell's nodes in insertion order (body, then node): each bucket is filled by atomics, then sorted by insertion
+// index; all compactions are order-preserving scans. A rebuild runs on the device end to end (island parameters,
+// partner boxes, hash, queries, filter) and the host waits once, for the list sizes. The response runs in one
+// workgroup: contacts that share no node are independent, so each contact is placed at the dependency level one past
+// the latest earlier contact sharing a node with it (a sort of (node, contact) keys in LDS) and each level runs in
+// parallel - the same arithmetic, in the same per-node order, as the serial loop.
 #include "bl_contacts.h"
 
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -112,28 +112,86 @@ __device__ __forceinline__ bool near_triangle(f3 p, f3 a, f3 b, f3 c, f3 tu, flo
     return ffabs(s) <= r && dot(d, d) <= r * r;
 }
 
-__device__ __forceinline__ bool near_partner(const CArgs& c, uint32_t lb, f3 mn, f3 mx) {
-    if (c.self[lb]) return true;
-    for (uint32_t k = c.part_off[lb]; k < c.part_off[lb + 1]; k++) {
-        const float* q = c.boxes + 6 * c.part[k];
-        if (mn.x <= q[3] && mx.x >= q[0] && mn.y <= q[4] && mx.y >= q[1] && mn.z <= q[5] && mx.z >= q[2]) return true;
-    }
-    return false;
-}
-
 __device__ __forceinline__ bool same_group(const KArgs& a, uint32_t ga, uint32_t gb) {
     const int x = a.bodies[ga].group, y = a.bodies[gb].group;
     return x != 0 && x == y;
 }
 
+// One table for every pair island: a cell's bucket is its cell_hash salted with the island and mixed. The candidate lists
+// do not depend on the table: a triangle's (or capsule's) candidates are, cell by cell in z, y, x order, the points
+// of that exact cell in insertion order, whatever else shares the bucket; the capsule-triangle candidates are the
+// set of triangles whose boxes meet the capsule's, sorted.
+__device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, int isl, uint32_t mask) {
+    // (cell_hash's low bits collide in runs on lattice-shaped cell sets: a finalizing mix spreads them)
+    uint32_t h = cell_hash(x, y, z) ^ ((uint32_t)isl * 0x9E3779B1u);
+    h ^= h >> 16, h *= 0x85EBCA6Bu, h ^= h >> 13, h *= 0xC2B2AE35u, h ^= h >> 16;
+    return h & mask;
+}
+
+__device__ __forceinline__ bool box_meets(const float* q, f3 mn, f3 mx) {
+    return mn.x <= q[3] && mx.x >= q[0] && mn.y <= q[4] && mx.y >= q[1] && mn.z <= q[5] && mx.z >= q[2];
+}
+
+// Partners (src/phys/world.cpp:1936-1950): two bodies of one island, not of one collision group, whose bounds
+// grown by the island's reach meet.
+__device__ __forceinline__ bool partners(const KArgs& a, const CArgs& c, uint32_t g, uint32_t y) {
+    if (y == g || same_group(a, g, y)) return false;
+    const float* q = c.bbox + 6 * y;
+    const float* p = c.bbox + 6 * g;
+    return box_meets(q, mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]));
+}
+
+// near_partner: a self-colliding body, or one of its partners' boxes meets [mn, mx].
+__device__ bool near_partner(const KArgs& a, const CArgs& c, uint32_t g, f3 mn, f3 mx) {
+    if (a.bodies[g].self_coll) return true;
+    const int m = c.bmem[g];
+    const uint32_t k1 = c.poff[m] + c.pcnt[m] < c.cap_part ? c.poff[m] + c.pcnt[m] : c.cap_part;
+    for (uint32_t k = c.poff[m]; k < k1; k++)
+        if (box_meets(c.bbox + 6 * c.plist[k], mn, mx)) return true;
+    return false;
+}
+
+// wave-aggregated add of the counts of the last pair island (the reference's pair counter keeps the last island's);
+// every lane of the wave calls it
+__device__ __forceinline__ void add_last(const CArgs& c, uint32_t n) {
+    uint32_t s = n;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&c.st->last_pairs, s);
+}
+
+// exclusive prefix over the wave's lanes (every lane calls it); *total: the wave's sum
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// LDS traffic of one wave is processed in order: a code-motion barrier with wave-scope fences is enough between
+// one lane's write and another lane's read
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------------------------- rebuild stages
+__global__ void k_rb_clear(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // Triangles stretched past 2.5 times their longest rest edge are torn (src/phys/world.cpp:1874-1877); and every
-// 7th triangle's size sample for the cell size (:1916-1929), in the island's order.
-__global__ void k_tris_torn(KArgs a, CArgs c) {
+// 7th triangle's size sample for the cell size (:1916-1929).
+__global__ void k_rb_torn(KArgs a, CArgs c) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntris) return;
     const uint4 tr = a.tri_abc[t];
-    if (c.island_local[tr.w] < 0) return;
+    if (c.bisl[tr.w] < 0) return;
     const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
     const float e2 = max_edge2(pa, pb, pc);
     if (!a.tri_torn[t] && a.tri_rest2[t] > 0 && e2 > 2.5f * 2.5f * a.tri_rest2[t]) a.tri_torn[t] = 1;
@@ -141,290 +199,668 @@ __global__ void k_tris_torn(KArgs a, CArgs c) {
     if (k % 7 == 0) c.samples[c.sample_base[tr.w] + k / 7] = fsqrt(e2);
 }
 
-// 1) the contacter nodes of the island's querying bodies near a partner: flag, then (after a scan) their records
-__global__ void k_pts(KArgs a, CArgs c, int write) {
+template <class T, class Op>
+__device__ __forceinline__ T block_reduce(T v, Op op, T* sh) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off));
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    T r = sh[0];
+    for (uint32_t w = 1; w < blockDim.x / 64; w++) r = op(r, sh[w]);
+    return r;
+}
+
+// Each pair island's search parameters (src/phys/world.cpp:1879-1935), one workgroup per island: the largest
+// collision radius, the margin from the top speed of its slow bodies (a body faster than 8 m/s is flagged: its own
+// pair schedule is not stepped here), the cell size from the mean size sample. The reference sums the samples in
+// double, one after another. When every partial sum is exact in double - all samples are multiples of 2^emin and
+// their total stays below 2^(emin + 53) - that sequential sum is the exact sum, which an integer sum in any order
+// gives; otherwise one thread adds them in the reference's order.
+__global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
+    __shared__ float shf[4];
+    __shared__ int shi[4];
+    __shared__ unsigned long long shu[4];
+    const uint32_t i = blockIdx.x, tid = threadIdx.x;
+    const uint32_t k0 = c.imem_off[i], k1 = c.imem_off[i + 1];
+    float rmax = 0, slow = 0;
+    int fast = 0;
+    for (uint32_t k = k0 + tid; k < k1; k += 256) {
+        const uint32_t g = c.imem[k];
+        rmax = fmaxf(rmax, a.bodies[g].collision_radius);
+        const float ms = fsqrt(u2f(a.body_frame[g * BF_N + BF_V2]));
+        if (ms <= 8.0f) slow = fmaxf(slow, ms);
+        else fast = 1;
+    }
+    auto fmx = [](float x, float y) { return fmaxf(x, y); };
+    auto imx = [](int x, int y) { return x > y ? x : y; };
+    auto imn = [](int x, int y) { return x < y ? x : y; };
+    auto iad = [](int x, int y) { return x + y; };
+    auto uad = [](unsigned long long x, unsigned long long y) { return x + y; };
+    rmax = block_reduce(rmax, fmx, shf);
+    slow = block_reduce(slow, fmx, shf);
+    fast = block_reduce(fast, imx, shi);
+    // the samples (the island's, in the global order: body, then triangle): count, smallest unit in the last place,
+    // largest value, anything not finite
+    int n = 0, emin = 1 << 20, bad = 0;
+    float xmax = 0;
+    for (uint32_t j = tid; j < c.nsamples; j += 256) {
+        if (c.bisl[c.sbody[j]] != (int)i) continue;
+        const float x = c.samples[j];
+        n++;
+        const uint32_t b = f2u(x), ex = (b >> 23) & 0xff;
+        if (ex == 0xff || (b >> 31)) bad = 1;
+        else if (b) emin = imn(emin, ex ? (int)ex - 150 : -149), xmax = fmaxf(xmax, x);
+    }
+    n = block_reduce(n, iad, shi);
+    emin = block_reduce(emin, imn, shi);
+    bad = block_reduce(bad, imx, shi);
+    xmax = block_reduce(xmax, fmx, shf);
+    double sum = 0;
+    bool exact = false;
+    if (!bad && n > 0 && xmax > 0) {
+        int ehi;
+        (void)frexp((double)n * (double)xmax * (1.0 + 1e-9), &ehi); // every partial sum is below 2^ehi
+        exact = ehi - emin <= 53;
+    } else if (!bad) {
+        exact = true; // (no samples, or all zero)
+    }
+    if (exact && xmax > 0) {
+        unsigned long long acc = 0;
+        for (uint32_t j = tid; j < c.nsamples; j += 256) {
+            if (c.bisl[c.sbody[j]] != (int)i) continue;
+            const uint32_t b = f2u(c.samples[j]), ex = (b >> 23) & 0xff;
+            if (!b) continue;
+            const unsigned long long mant = (b & 0x7fffffu) | (ex ? 0x800000u : 0u);
+            const int e = ex ? (int)ex - 150 : -149;
+            acc += mant << (e - emin);
+        }
+        acc = block_reduce(acc, uad, shu);
+        sum = ldexp((double)acc, emin);
+    }
+    if (tid != 0) return;
+    if (!exact) {
+        n = 0;
+        for (uint32_t j = 0; j < c.nsamples; j++)
+            if (c.bisl[c.sbody[j]] == (int)i) sum += (double)c.samples[j], n++;
+    }
+    if (fast) atomicOr(&c.st->unsupported, 1u);
+    const float speed = slow + 0.5f;
+    const float margin = 2.0f * speed * (float)c.interval * c.dt + 0.02f;
+    const float mean = n ? (float)(sum / (double)n) : 1.0f;
+    IslandPar ip;
+    ip.margin = margin, ip.rmax = rmax, ip.reach = rmax + margin;
+    ip.inv_cell = 1.0f / clampf(mean + 2.0f * (rmax + margin), 0.25f, 1.0f);
+    c.ipar[i] = ip;
+}
+
+// Each island body's partner box: its bounds grown by the island's reach.
+__global__ void k_rb_boxes(KArgs a, CArgs c) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nbodies || c.bisl[g] < 0) return;
+    const float r = c.ipar[c.bisl[g]].reach + 0.0f + 0.0f;
+    const uint32_t* w = c.body_box + g * BF_N;
+    for (int k = 0; k < 3; k++) c.bbox[6 * g + k] = fkey_inv(w[BF_MN + k]) - r, c.bbox[6 * g + 3 + k] = fkey_inv(w[BF_MX + k]) + r;
+}
+
+// Each island body's partners (one wave per body, lanes over its island; count, then write in body order) and
+// whether it searches (a partner, or self collision).
+__global__ __launch_bounds__(256) void k_rb_partners(KArgs a, CArgs c, int write) {
+    const uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (m >= c.nmem) return;
+    const uint32_t g = c.imem[m];
+    const int isl = c.bisl[g];
+    const uint32_t k0 = c.imem_off[isl], k1 = c.imem_off[isl + 1];
+    uint32_t total = 0;
+    const uint32_t base = write ? c.poff[m] : 0;
+    for (uint32_t y0 = k0; y0 < k1; y0 += 64) {
+        const uint32_t k = y0 + lane;
+        const bool ok = k < k1 && partners(a, c, g, c.imem[k]);
+        uint32_t chunk;
+        const uint32_t before = wave_prefix(ok ? 1u : 0u, chunk);
+        if (write && ok && base + total + before < c.cap_part) c.plist[base + total + before] = c.imem[k];
+        total += chunk;
+    }
+    if (lane != 0) return;
+    if (!write) {
+        c.pcnt[m] = total;
+        c.bquery[g] = (total > 0 || a.bodies[g].self_coll) ? 1u : 0u;
+    } else if (m == c.nmem - 1) {
+        const uint32_t tot = c.poff[m] + c.pcnt[m];
+        if (tot > c.cap_part) atomicOr(&c.st->overflow, 16u), c.st->npart = tot;
+    }
+}
+
+// 1) the contacter nodes of the searching bodies near a partner (flag; after the scan their records, in node order)
+__global__ void k_rb_pts(KArgs a, CArgs c, int write) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nnodes) return;
-    const uint32_t info = a.ninfo[i], gb = info >> 16;
-    const int lb = c.island_local[gb];
-    bool ok = lb >= 0 && c.query[lb] && (info & BL_NF_CONTACTER);
+    const uint32_t g = a.ninfo[i] >> 16;
+    const int isl = c.bisl[g];
+    bool ok = isl >= 0 && c.bquery[g] && (a.ninfo[i] & BL_NF_CONTACTER);
     f3 p = mk(0, 0, 0);
     if (ok) {
         p = ld3(a.pm[c.parity][i]);
-        ok = near_partner(c, (uint32_t)lb, p, p);
+        ok = near_partner(a, c, g, p, p);
     }
     if (!write) {
-        c.flag[i] = ok;
+        c.pflag[i] = ok;
         return;
     }
+    if (i == a.nnodes - 1) c.st->npts = c.pscan[i] + c.pflag[i];
     if (!ok) return;
-    const uint32_t k = c.scan[i];
-    const int x = cell_of(p.x, c.inv_cell), y = cell_of(p.y, c.inv_cell), z = cell_of(p.z, c.inv_cell);
-    c.pt_cell[k] = make_int4(x, y, z, (int)i);
-    c.pt_pos[k] = make_float4(p.x, p.y, p.z, u2f((uint32_t)lb));
-    c.keys[k] = ((uint64_t)(cell_hash(x, y, z) & c.mask) << 32) | k;
+    const uint32_t k = c.pscan[i];
+    const float ic = c.ipar[isl].inv_cell;
+    const int x = cell_of(p.x, ic), y = cell_of(p.y, ic), z = cell_of(p.z, ic);
+    c.pcell[k] = make_int4(x, y, z, (int)i);
+    c.ppos[k] = make_float4(p.x, p.y, p.z, u2f(g));
+    const uint32_t b = bucket_of(x, y, z, isl, c.tmask);
+    c.pbucket[k] = b;
+    atomicAdd(&c.tcount[b], 1u);
 }
 
-// buckets from the sorted (bucket, index) keys: bucket_start, the items, and run_end (the end of the run of one
-// body's nodes in a bucket: a querying body skips its own nodes in one step)
-__global__ void k_buckets(CArgs c, uint32_t n, uint32_t nbk, const uint64_t* sorted, uint32_t* bstart, uint32_t* items,
-                          const float4* pos, uint32_t* run_end) {
+__global__ void k_rb_fill(KArgs a, CArgs c) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > n) return;
-    const uint32_t cur = k < n ? (uint32_t)(sorted[k] >> 32) : nbk;
-    const uint32_t prev = k > 0 ? (uint32_t)(sorted[k - 1] >> 32) : 0xffffffffu;
-    if (k == 0) {
-        for (uint32_t b = 0; b <= cur; b++) bstart[b] = 0;
-    } else if (cur != prev) {
-        for (uint32_t b = prev + 1; b <= cur; b++) bstart[b] = k;
-    }
-    if (k < n) items[k] = (uint32_t)sorted[k];
-    if (pos && k < n) {
-        const uint32_t body = f2u(pos[(uint32_t)sorted[k]].w);
-        uint32_t e = k + 1;
-        while (e < n && (uint32_t)(sorted[e] >> 32) == cur && f2u(pos[(uint32_t)sorted[e]].w) == body) e++;
-        run_end[k] = e;
-    }
+    if (k >= c.st->npts) return;
+    const uint32_t b = c.pbucket[k];
+    c.titems[c.tstart[b] + atomicAdd(&c.tfill[b], 1u)] = k;
 }
 
-// 2) each triangle of the querying bodies scans its cells for nodes in reach (count, then write in order)
-__global__ void k_query_tris(KArgs a, CArgs c, int write) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ntris) return;
-    const uint4 tr = a.tri_abc[t];
-    const int lbt = c.island_local[tr.w];
-    uint32_t cnt = 0, out = write ? c.scan[t] : 0;
-    if (lbt >= 0 && c.query[lbt] && !a.tri_torn[t] && c.npts > 0) {
-        const BodyParam& O = a.bodies[tr.w];
-        const float rq = c.rmax + c.margin + 0.0f + 0.0f;
-        const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
-        const f3 tmn = vmin(pa, vmin(pb, pc)), tmx = vmax(pa, vmax(pb, pc));
-        if (near_partner(c, lbt, sub(tmn, mk(rq, rq, rq)), add(tmx, mk(rq, rq, rq)))) {
-            const f3 tn = cross(sub(pb, pa), sub(pc, pa));
-            const float tn2 = dot(tn, tn);
-            const f3 tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);
-            const int x0 = cell_of(tmn.x - rq, c.inv_cell), x1 = cell_of(tmx.x + rq, c.inv_cell);
-            const int y0 = cell_of(tmn.y - rq, c.inv_cell), y1 = cell_of(tmx.y + rq, c.inv_cell);
-            const int z0 = cell_of(tmn.z - rq, c.inv_cell), z1 = cell_of(tmx.z + rq, c.inv_cell);
-            if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) <= 512) {
-                for (int z = z0; z <= z1; z++)
-                    for (int y = y0; y <= y1; y++)
-                        for (int x = x0; x <= x1; x++) {
-                            const uint32_t bk = cell_hash(x, y, z) & c.mask;
-                            for (uint32_t k = c.bstart[bk]; k < c.bstart[bk + 1]; k++) {
-                                const uint32_t pi = c.items[k];
-                                const int4 cell = c.pt_cell[pi];
-                                const float4 pp = c.pt_pos[pi];
-                                const uint32_t lbn = f2u(pp.w);
-                                if (lbn == (uint32_t)lbt && !O.self_coll) { k = c.run_end[k] - 1; continue; }
-                                if (cell.x != x || cell.y != y || cell.z != z) continue;
-                                if (pp.x < tmn.x - rq || pp.y < tmn.y - rq || pp.z < tmn.z - rq || pp.x > tmx.x + rq ||
-                                    pp.y > tmx.y + rq || pp.z > tmx.z + rq)
-                                    continue;
-                                const uint32_t node = (uint32_t)cell.w, gbn = c.island_body[lbn];
-                                const BodyParam& B = a.bodies[gbn];
-                                const bool self = lbn == (uint32_t)lbt;
-                                if (self && (!B.self_coll || tr.x == node || tr.y == node || tr.z == node)) continue;
-                                if (!self && same_group(a, gbn, tr.w)) continue;
-                                const f3 p = mk(pp.x, pp.y, pp.z);
-                                const float r = B.collision_radius + c.margin + 0.0f;
-                                const float rb = r + 0.0f;
-                                if (p.x < tmn.x - rb || p.y < tmn.y - rb || p.z < tmn.z - rb || p.x > tmx.x + rb || p.y > tmx.y + rb || p.z > tmx.z + rb) continue;
-                                if (!near_triangle(p, pa, pb, pc, tu, r)) continue;
-                                if (write) c.nt[out++] = make_uint4(gbn, tr.w, node, t);
-                                cnt++;
-                            }
-                        }
+// each bucket in insertion order (its points by index; short buckets sorted in registers), and the points' records
+// copied into bucket order, so that a query reads them in one pass without indirection
+__global__ void k_rb_sortb(CArgs c, uint32_t nbk) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbk) return;
+    const uint32_t s = c.tstart[b], e = c.tstart[b + 1], n = e - s;
+    if (n == 0) return;
+    if (n <= 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) v[k] = k < n ? c.titems[s + k] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t i = 1; i < 16; i++)
+#pragma unroll
+            for (uint32_t j = i; j > 0; j--) {
+                const uint32_t lo = v[j - 1] < v[j] ? v[j - 1] : v[j], hi = v[j - 1] < v[j] ? v[j] : v[j - 1];
+                v[j - 1] = lo, v[j] = hi;
             }
-        }
-    }
-    if (!write) c.count[t] = cnt;
-}
-
-// ... and each capsule of those bodies scans for nodes within its radius
-__global__ void k_query_caps(KArgs a, CArgs c, int write) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= a.ncaps) return;
-    const uint4 cp = a.cap_data[ci];
-    const int lbt = c.island_local[cp.z];
-    uint32_t cnt = 0, out = write ? c.scan[ci] : 0;
-    if (lbt >= 0 && c.query[lbt] && c.npts > 0) {
-        const f3 pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
-        const float rq2 = u2f(cp.w) + c.rmax + c.margin + 0.0f;
-        const f3 mn = sub(vmin(pa, pb), mk(rq2, rq2, rq2)), mx = add(vmax(pa, pb), mk(rq2, rq2, rq2));
-        if (near_partner(c, lbt, mn, mx)) {
-            const int x0 = cell_of(mn.x, c.inv_cell), x1 = cell_of(mx.x, c.inv_cell), y0 = cell_of(mn.y, c.inv_cell);
-            const int y1 = cell_of(mx.y, c.inv_cell), z0 = cell_of(mn.z, c.inv_cell), z1 = cell_of(mx.z, c.inv_cell);
-            if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) <= 512)
-                for (int z = z0; z <= z1; z++)
-                    for (int y = y0; y <= y1; y++)
-                        for (int x = x0; x <= x1; x++) {
-                            const uint32_t bk = cell_hash(x, y, z) & c.mask;
-                            for (uint32_t k = c.bstart[bk]; k < c.bstart[bk + 1]; k++) {
-                                const uint32_t pi = c.items[k];
-                                const int4 cell = c.pt_cell[pi];
-                                const float4 pp = c.pt_pos[pi];
-                                const uint32_t lbn = f2u(pp.w);
-                                if (lbn == (uint32_t)lbt) { k = c.run_end[k] - 1; continue; }
-                                if (cell.x != x || cell.y != y || cell.z != z) continue;
-                                const uint32_t gbn = c.island_body[lbn];
-                                if (same_group(a, gbn, cp.z)) continue;
-                                const f3 p = mk(pp.x, pp.y, pp.z);
-                                const float r = u2f(cp.w) + a.bodies[gbn].collision_radius + c.margin + 0.0f;
-                                const float s = closest_on_segment(p, pa, pb);
-                                const f3 d = sub(add(pa, mul(sub(pb, pa), s)), p);
-                                if (dot(d, d) > r * r) continue;
-                                if (write) c.nc[out++] = make_uint4(gbn, cp.z, (uint32_t)cell.w, ci);
-                                cnt++;
-                            }
-                        }
-        }
-    }
-    if (!write) c.count[ci] = cnt;
-}
-
-// 3) capsules against triangles: the triangles as primitives in every cell of their (radius-expanded) boxes
-__global__ void k_prims(KArgs a, CArgs c, int write) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ntris) return;
-    const uint4 tr = a.tri_abc[t];
-    const int lb = c.island_local[tr.w];
-    uint32_t ncell = 0;
-    int x0 = 0, x1 = -1, y0 = 0, y1 = -1, z0 = 0, z1 = -1;
-    if (lb >= 0 && c.query[lb] && !a.tri_torn[t]) {
-        const float r = a.bodies[tr.w].collision_radius + c.margin + 0.0f;
-        const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
-        const f3 mn = sub(vmin(pa, vmin(pb, pc)), mk(r, r, r)), mx = add(vmax(pa, vmax(pb, pc)), mk(r, r, r));
-        if (near_partner(c, lb, mn, mx)) {
-            x0 = cell_of(mn.x, c.inv_cell), x1 = cell_of(mx.x, c.inv_cell), y0 = cell_of(mn.y, c.inv_cell);
-            y1 = cell_of(mx.y, c.inv_cell), z0 = cell_of(mn.z, c.inv_cell), z1 = cell_of(mx.z, c.inv_cell);
-            const int nc = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-            if (nc <= 512) ncell = (uint32_t)nc;
-        }
-    }
-    if (write == 0) {
-        c.flag[t] = ncell > 0;
-        c.count[t] = ncell;
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++)
+            if (k < n) c.scell[s + k] = c.pcell[v[k]], c.spos[s + k] = c.ppos[v[k]];
         return;
     }
-    if (!ncell) return;
-    const uint32_t pid = c.scan[t], e0 = c.scan2[t];
-    c.prim_tri[pid] = t;
-    uint32_t e = e0;
-    for (int z = z0; z <= z1; z++)
-        for (int y = y0; y <= y1; y++)
-            for (int x = x0; x <= x1; x++) {
-                c.entry_pid[e] = pid;
-                c.keys[e] = ((uint64_t)(cell_hash(x, y, z) & c.mask) << 32) | e;
-                e++;
-            }
+    c.big[atomicAdd(&c.st->nbig, 1u)] = b; // (longer buckets: a wave each, below)
 }
 
-// each capsule of the island collects the triangles of the other bodies whose boxes meet its own, once each, sorted
-// by (body, triangle) (count, then write)
-constexpr int kCtMax = 128;
-__global__ void k_query_ct(KArgs a, CArgs c, int write) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= a.ncaps) return;
-    const uint4 cp = a.cap_data[ci];
-    const int lbc = c.island_local[cp.z];
-    uint32_t found[kCtMax];
-    int nf = 0;
-    if (lbc >= 0 && c.nprims > 0) {
-        const f3 pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
-        const float rr = u2f(cp.w) + a.bodies[cp.z].collision_radius + c.margin + 0.0f;
-        const f3 mn = sub(vmin(pa, pb), mk(rr, rr, rr)), mx = add(vmax(pa, pb), mk(rr, rr, rr));
-        const int x0 = cell_of(mn.x, c.inv_cell), x1 = cell_of(mx.x, c.inv_cell), y0 = cell_of(mn.y, c.inv_cell);
-        const int y1 = cell_of(mx.y, c.inv_cell), z0 = cell_of(mn.z, c.inv_cell), z1 = cell_of(mx.z, c.inv_cell);
-        if ((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1) <= 512)
-            for (int z = z0; z <= z1; z++)
-                for (int y = y0; y <= y1; y++)
-                    for (int x = x0; x <= x1; x++) {
-                        const uint32_t bk = cell_hash(x, y, z) & c.mask;
-                        for (uint32_t k = c.bstart[bk]; k < c.bstart[bk + 1]; k++) {
-                            const uint32_t t = c.prim_tri[c.entry_pid[c.items[k]]];
-                            const uint4 tr = a.tri_abc[t];
-                            if (tr.w == cp.z || same_group(a, cp.z, tr.w)) continue;
-                            const f3 ta = ld3(a.pm[c.parity][tr.x]), tb = ld3(a.pm[c.parity][tr.y]), tc = ld3(a.pm[c.parity][tr.z]);
-                            const f3 tmn = vmin(ta, vmin(tb, tc)), tmx = vmax(ta, vmax(tb, tc));
-                            if (tmx.x < mn.x || tmx.y < mn.y || tmx.z < mn.z || tmn.x > mx.x || tmn.y > mx.y || tmn.z > mx.z) continue;
-                            // (global triangle order is (body, triangle) order: keep the distinct ones sorted)
-                            int pos = nf;
-                            while (pos > 0 && found[pos - 1] > t) pos--;
-                            if (pos > 0 && found[pos - 1] == t) continue;
-                            if (nf == kCtMax) {
-                                atomicOr(c.overflow, 1u);
-                                continue;
-                            }
-                            for (int q = nf; q > pos; q--) found[q] = found[q - 1];
-                            found[pos] = t;
-                            nf++;
-                        }
-                    }
+// the buckets of more than 16 points, one wave each: a bitonic sort in LDS (a serial one past its capacity)
+constexpr int kBig = 512;
+__global__ __launch_bounds__(256) void k_rb_sortbig(CArgs c) {
+    __shared__ uint32_t lst[4][kBig];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t nw = gridDim.x * 4;
+    uint32_t* L = lst[wv];
+    for (uint32_t w = blockIdx.x * 4 + wv; w < c.st->nbig; w += nw) {
+        const uint32_t b = c.big[w];
+        const uint32_t s = c.tstart[b], e = c.tstart[b + 1], n = e - s;
+        if (n > (uint32_t)kBig) {
+            if (lane == 0) {
+                for (uint32_t k = s + 1; k < e; k++) {
+                    const uint32_t x = c.titems[k];
+                    uint32_t j = k;
+                    while (j > s && c.titems[j - 1] > x) c.titems[j] = c.titems[j - 1], j--;
+                    c.titems[j] = x;
+                }
+                for (uint32_t k = s; k < e; k++) c.scell[k] = c.pcell[c.titems[k]], c.spos[k] = c.ppos[c.titems[k]];
+            }
+            continue;
+        }
+        uint32_t np = 1;
+        while (np < n) np <<= 1;
+        for (uint32_t k = lane; k < np; k += 64) L[k] = k < n ? c.titems[s + k] : 0xffffffffu;
+        wave_sync();
+        for (uint32_t size = 2; size <= np; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t q = lane; q < np / 2; q += 64) {
+                    const uint32_t i = 2 * q - (q & (stride - 1)), j = i + stride;
+                    const bool up = (i & size) == 0;
+                    const uint32_t x = L[i], y = L[j];
+                    if ((x > y) == up) L[i] = y, L[j] = x;
+                }
+                wave_sync();
+            }
+        for (uint32_t k = lane; k < n; k += 64) c.scell[s + k] = c.pcell[L[k]], c.spos[s + k] = c.ppos[L[k]];
+        wave_sync();
+    }
+}
+
+// A query's cell range: index ci in z, y, x order.
+struct CellRange {
+    int x0, y0, z0, nx, ny;
+    uint32_t n;
+    __device__ void set(f3 mn, f3 mx, float ic) {
+        x0 = cell_of(mn.x, ic), y0 = cell_of(mn.y, ic), z0 = cell_of(mn.z, ic);
+        nx = cell_of(mx.x, ic) - x0 + 1, ny = cell_of(mx.y, ic) - y0 + 1;
+        const int nz = cell_of(mx.z, ic) - z0 + 1;
+        // (over 512 cells: no query, as the reference; an empty range, from coordinates that are not finite: none)
+        n = nx > 0 && ny > 0 && nz > 0 && nx * ny * nz <= 512 ? (uint32_t)(nx * ny * nz) : 0u;
+    }
+    __device__ void cell(uint32_t ci, int& x, int& y, int& z) const {
+        x = x0 + (int)(ci % (uint32_t)nx);
+        y = y0 + (int)((ci / (uint32_t)nx) % (uint32_t)ny);
+        z = z0 + (int)(ci / (uint32_t)(nx * ny));
+    }
+};
+
+// exclusive prefix over a group of G lanes (G divides 64; every lane of the group calls it); total: the group's sum
+template <int G>
+__device__ __forceinline__ uint32_t group_prefix(uint32_t v, uint32_t& total) {
+    const uint32_t gl = threadIdx.x & (G - 1);
+    uint32_t x = v;
+    for (int off = 1; off < G; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, G);
+        if (gl >= (uint32_t)off) x += y;
+    }
+    total = __shfl(x, G - 1, G);
+    return x - v;
+}
+
+// 2) each non-torn triangle of a searching body scans its cells for nodes in reach: a group of 8 lanes per
+// triangle, a lane per cell; the lanes' matches concatenate in cell order (count, then write).
+struct NtQuery {
+    uint4 tr;
+    uint32_t t;
+    int isl;
+    f3 pa, pb, pc, tmn, tmx, tu;
+    float rq, margin;
+    bool self_coll;
+};
+
+__device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, const NtQuery& q, int x, int y, int z,
+                                            uint4* out, uint32_t at, uint32_t cap) {
+    uint32_t cnt = 0;
+    const uint32_t bk = bucket_of(x, y, z, q.isl, c.tmask);
+    for (uint32_t k = c.tstart[bk]; k < c.tstart[bk + 1]; k++) {
+        const int4 cell = c.scell[k];
+        const float4 pp = c.spos[k];
+        if (cell.x != x || cell.y != y || cell.z != z) continue;
+        const uint32_t gbn = f2u(pp.w);
+        if (gbn == q.tr.w && !q.self_coll) continue;
+        if (c.bisl[gbn] != q.isl) continue;
+        const float rq = q.rq;
+        if (pp.x < q.tmn.x - rq || pp.y < q.tmn.y - rq || pp.z < q.tmn.z - rq || pp.x > q.tmx.x + rq ||
+            pp.y > q.tmx.y + rq || pp.z > q.tmx.z + rq)
+            continue;
+        const uint32_t node = (uint32_t)cell.w;
+        const BodyParam& B = a.bodies[gbn];
+        const bool self = gbn == q.tr.w;
+        if (self && (!B.self_coll || q.tr.x == node || q.tr.y == node || q.tr.z == node)) continue;
+        if (!self && same_group(a, gbn, q.tr.w)) continue;
+        const f3 p = mk(pp.x, pp.y, pp.z);
+        const float r = B.collision_radius + q.margin + 0.0f;
+        const float rb = r + 0.0f;
+        if (p.x < q.tmn.x - rb || p.y < q.tmn.y - rb || p.z < q.tmn.z - rb || p.x > q.tmx.x + rb ||
+            p.y > q.tmx.y + rb || p.z > q.tmx.z + rb)
+            continue;
+        if (!near_triangle(p, q.pa, q.pb, q.pc, q.tu, r)) continue;
+        if (out && at + cnt < cap) out[at + cnt] = make_uint4(gbn, q.tr.w, node, q.t);
+        cnt++;
+    }
+    return cnt;
+}
+
+constexpr int kGnt = 8;
+__global__ __launch_bounds__(256) void k_rb_nt(KArgs a, CArgs c, int write) {
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
+    if (t >= a.ntris) return; // (whole groups)
+    NtQuery q;
+    q.tr = a.tri_abc[t];
+    q.t = t;
+    q.isl = c.bisl[q.tr.w];
+    CellRange cr;
+    cr.n = 0;
+    if (q.isl >= 0 && c.bquery[q.tr.w] && !a.tri_torn[t] && c.st->npts > 0) {
+        const IslandPar ip = c.ipar[q.isl];
+        q.self_coll = a.bodies[q.tr.w].self_coll != 0;
+        q.margin = ip.margin;
+        const float rq0 = ip.rmax + ip.margin + 0.0f;
+        q.rq = rq0 + 0.0f;
+        q.pa = ld3(a.pm[c.parity][q.tr.x]), q.pb = ld3(a.pm[c.parity][q.tr.y]), q.pc = ld3(a.pm[c.parity][q.tr.z]);
+        q.tmn = vmin(q.pa, vmin(q.pb, q.pc)), q.tmx = vmax(q.pa, vmax(q.pb, q.pc));
+        const f3 r3 = mk(q.rq, q.rq, q.rq);
+        if (near_partner(a, c, q.tr.w, sub(q.tmn, r3), add(q.tmx, r3))) {
+            const f3 tn = cross(sub(q.pb, q.pa), sub(q.pc, q.pa));
+            const float tn2 = dot(tn, tn);
+            q.tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);
+            cr.set(sub(q.tmn, r3), add(q.tmx, r3), ip.inv_cell);
+        }
+    }
+    const uint32_t base = write ? c.tscan[t] : 0;
+    uint32_t total = 0;
+    for (uint32_t c0 = 0; c0 < cr.n; c0 += kGnt) {
+        const uint32_t ci = c0 + gl;
+        int x = 0, y = 0, z = 0;
+        uint32_t cnt = 0;
+        if (ci < cr.n) {
+            cr.cell(ci, x, y, z);
+            cnt = nt_cell(a, c, q, x, y, z, nullptr, 0, 0);
+        }
+        uint32_t chunk;
+        const uint32_t before = group_prefix<kGnt>(cnt, chunk);
+        if (write && cnt) nt_cell(a, c, q, x, y, z, c.ntraw, base + total + before, c.cap_nt);
+        total += chunk;
+    }
+    if (gl != 0) return;
+    if (!write) {
+        c.tcnt[t] = total;
+    } else if (t == a.ntris - 1) {
+        const uint32_t tot = c.tscan[t] + c.tcnt[t];
+        c.st->nnt_raw = tot;
+        if (tot > c.cap_nt) atomicOr(&c.st->overflow, 1u);
+    }
+}
+
+// ... and each capsule of a searching body scans for nodes within its radius (a group of 16 lanes per capsule)
+__device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint4 cp, int isl, f3 pa, f3 pb,
+                                            float margin, int x, int y, int z, uint4* out, uint32_t at, uint32_t cap,
+                                            uint32_t ci) {
+    uint32_t cnt = 0;
+    const uint32_t bk = bucket_of(x, y, z, isl, c.tmask);
+    for (uint32_t k = c.tstart[bk]; k < c.tstart[bk + 1]; k++) {
+        const int4 cell = c.scell[k];
+        const float4 pp = c.spos[k];
+        if (cell.x != x || cell.y != y || cell.z != z) continue;
+        const uint32_t gbn = f2u(pp.w);
+        if (gbn == cp.z || c.bisl[gbn] != isl) continue;
+        if (same_group(a, gbn, cp.z)) continue;
+        const f3 p = mk(pp.x, pp.y, pp.z);
+        const float r = u2f(cp.w) + a.bodies[gbn].collision_radius + margin + 0.0f;
+        const float s = closest_on_segment(p, pa, pb);
+        const f3 d = sub(add(pa, mul(sub(pb, pa), s)), p);
+        if (dot(d, d) > r * r) continue;
+        if (out && at + cnt < cap) out[at + cnt] = make_uint4(gbn, cp.z, (uint32_t)cell.w, ci);
+        cnt++;
+    }
+    return cnt;
+}
+
+constexpr int kGcap = 16;
+__global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c, int write) {
+    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    const bool live = ci < a.ncaps;
+    uint4 cp = make_uint4(0, 0, 0, 0);
+    int isl = -1;
+    CellRange cr;
+    cr.n = 0;
+    f3 pa = mk(0, 0, 0), pb = mk(0, 0, 0);
+    float margin = 0;
+    if (live) {
+        cp = a.cap_data[ci];
+        isl = c.bisl[cp.z];
+        if (isl >= 0 && c.bquery[cp.z] && c.st->npts > 0) {
+            const IslandPar ip = c.ipar[isl];
+            margin = ip.margin;
+            pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
+            const float rq2 = u2f(cp.w) + ip.rmax + ip.margin + 0.0f;
+            const f3 mn = sub(vmin(pa, pb), mk(rq2, rq2, rq2)), mx = add(vmax(pa, pb), mk(rq2, rq2, rq2));
+            if (near_partner(a, c, cp.z, mn, mx)) cr.set(mn, mx, ip.inv_cell);
+        }
+    }
+    const uint32_t base = write && live ? c.cscan[ci] : 0;
+    uint32_t total = 0;
+    for (uint32_t c0 = 0; c0 < cr.n; c0 += kGcap) {
+        const uint32_t cc = c0 + gl;
+        int x = 0, y = 0, z = 0;
+        uint32_t cnt = 0;
+        if (cc < cr.n) {
+            cr.cell(cc, x, y, z);
+            cnt = nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
+        }
+        uint32_t chunk;
+        const uint32_t before = group_prefix<kGcap>(cnt, chunk);
+        if (write && cnt) nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, c.nc, base + total + before, c.cap_nc, ci);
+        total += chunk;
+    }
+    if (write) add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
+    if (!live || gl != 0) return;
+    if (!write) {
+        c.ccnt[ci] = total;
+    } else if (ci == a.ncaps - 1) {
+        const uint32_t tot = c.cscan[ci] + c.ccnt[ci];
+        c.st->nnc = tot;
+        if (tot > c.cap_nc) atomicOr(&c.st->overflow, 2u);
+    }
+}
+
+// 3) capsules against triangles: the triangles of the searching bodies into every cell of their grown boxes, each
+// entry carrying the triangle's body, group, island and raw bounds
+__global__ void k_rb_prims(KArgs a, CArgs c, int write) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntris) return;
+    const uint4 tr = a.tri_abc[t];
+    const int isl = c.bisl[tr.w];
+    // A prim is a triangle whose grown box spans at most 512 cells (the reference enters it into all of them). The
+    // capsules keep the triangles whose raw box meets theirs, and such a triangle shares with the capsule a cell of
+    // its raw box: entering those cells alone finds the same set.
+    CellRange cr;
+    cr.n = 0;
+    f3 tmn = mk(0, 0, 0), tmx = mk(0, 0, 0);
+    if (isl >= 0 && c.bquery[tr.w] && !a.tri_torn[t]) {
+        const IslandPar ip = c.ipar[isl];
+        const float r = a.bodies[tr.w].collision_radius + ip.margin + 0.0f;
+        const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
+        tmn = vmin(pa, vmin(pb, pc)), tmx = vmax(pa, vmax(pb, pc));
+        const f3 mn = sub(tmn, mk(r, r, r)), mx = add(tmx, mk(r, r, r));
+        if (near_partner(a, c, tr.w, mn, mx)) {
+            cr.set(mn, mx, ip.inv_cell);
+            if (cr.n) cr.set(tmn, tmx, ip.inv_cell);
+        }
     }
     if (!write) {
-        c.count[ci] = (uint32_t)nf;
+        c.ecnt[t] = cr.n;
         return;
     }
-    uint32_t out = c.scan[ci];
-    for (int q = 0; q < nf; q++) c.ct[out + q] = make_uint4(cp.z, a.tri_abc[found[q]].w, ci, found[q]);
+    if (t == a.ntris - 1) {
+        const uint32_t tot = c.escan[t] + c.ecnt[t];
+        c.st->nent = tot;
+        if (tot > c.cap_ent) atomicOr(&c.st->overflow, 8u);
+    }
+    const uint32_t e0 = c.escan[t];
+    const uint4 ra = make_uint4(t, tr.w, (uint32_t)a.bodies[tr.w].group, (uint32_t)isl);
+    const float4 rb = make_float4(tmn.x, tmn.y, tmn.z, tmx.x);
+    const float2 rc = make_float2(tmx.y, tmx.z);
+    for (uint32_t ci = 0; ci < cr.n && e0 + ci < c.cap_ent; ci++) {
+        int x, y, z;
+        cr.cell(ci, x, y, z);
+        const uint32_t b = bucket_of(x, y, z, isl, c.emask);
+        c.eb[e0 + ci] = b;
+        c.ea[e0 + ci] = ra, c.ebb[e0 + ci] = rb, c.ec[e0 + ci] = rc;
+        atomicAdd(&c.ecount[b], 1u);
+    }
+}
+
+__global__ void k_rb_efill(CArgs c) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c.st->nent < c.cap_ent ? c.st->nent : c.cap_ent;
+    if (e >= n) return;
+    const uint32_t b = c.eb[e];
+    const uint32_t at = c.estart[b] + atomicAdd(&c.efill[b], 1u);
+    c.sa[at] = c.ea[e], c.sb[at] = c.ebb[e], c.sc[at] = c.ec[e];
+}
+
+// each capsule of an island collects the triangles of its other bodies whose boxes meet its own, once each, in
+// (body, triangle) order: a group of 16 lanes per capsule, a lane per cell, the finds gathered in LDS, sorted, made
+// unique into the capsule's slot; after the scan of the counts, k_rb_ct_out moves them into the list
+constexpr int kCtCap = 256;
+__global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
+    __shared__ uint32_t lst[256 / kGcap][kCtCap];
+    __shared__ uint32_t lcnt[256 / kGcap];
+    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    const uint32_t gi = threadIdx.x / kGcap;
+    const bool live = ci < a.ncaps;
+    uint32_t* L = lst[gi];
+    uint4 cp = make_uint4(0, 0, 0, 0);
+    int isl = -1;
+    CellRange cr;
+    cr.n = 0;
+    f3 mn = mk(0, 0, 0), mx = mk(0, 0, 0);
+    if (live) {
+        cp = a.cap_data[ci];
+        isl = c.bisl[cp.z];
+        if (isl >= 0) {
+            const IslandPar ip = c.ipar[isl];
+            const f3 pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
+            const float rr = u2f(cp.w) + a.bodies[cp.z].collision_radius + ip.margin + 0.0f;
+            mn = sub(vmin(pa, pb), mk(rr, rr, rr)), mx = add(vmax(pa, pb), mk(rr, rr, rr));
+            cr.set(mn, mx, ip.inv_cell);
+        }
+    }
+    const uint32_t cg = live ? (uint32_t)a.bodies[cp.z].group : 0u;
+    if (gl == 0) lcnt[gi] = 0;
+    wave_sync();
+    for (uint32_t c0 = 0; c0 < cr.n; c0 += kGcap) {
+        const uint32_t cc = c0 + gl;
+        if (cc < cr.n) {
+            int x, y, z;
+            cr.cell(cc, x, y, z);
+            const uint32_t bk = bucket_of(x, y, z, isl, c.emask);
+            for (uint32_t k = c.estart[bk]; k < c.estart[bk + 1]; k++) {
+                const uint4 ra = c.sa[k];
+                if ((int)ra.w != isl || ra.y == cp.z || (cg != 0 && cg == ra.z)) continue;
+                const float4 rb = c.sb[k];
+                const float2 rc = c.sc[k];
+                if (rb.w < mn.x || rc.x < mn.y || rc.y < mn.z || rb.x > mx.x || rb.y > mx.y || rb.z > mx.z) continue;
+                const uint32_t at = atomicAdd(&lcnt[gi], 1u);
+                if (at < (uint32_t)kCtCap) L[at] = ra.x;
+            }
+        }
+    }
+    wave_sync();
+    const uint32_t raw = lcnt[gi];
+    if (raw > (uint32_t)kCtCap && gl == 0) atomicOr(&c.st->unsupported, 2u);
+    const uint32_t n = raw < (uint32_t)kCtCap ? raw : (uint32_t)kCtCap;
+    // bitonic sort of the finds (padded to a power of two)
+    uint32_t np = 1;
+    while (np < n) np <<= 1;
+    for (uint32_t k = n + gl; k < np; k += kGcap) L[k] = 0xffffffffu;
+    wave_sync();
+    for (uint32_t size = 2; size <= np; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t q = gl; q < np / 2; q += kGcap) {
+                const uint32_t i = 2 * q - (q & (stride - 1)), j = i + stride;
+                const bool up = (i & size) == 0;
+                const uint32_t x = L[i], y = L[j];
+                if ((x > y) == up) L[i] = y, L[j] = x;
+            }
+            wave_sync();
+        }
+    // the distinct ones, in order, into the capsule's slot
+    uint32_t total = 0;
+    for (uint32_t j0 = 0; j0 < n; j0 += kGcap) {
+        const uint32_t j = j0 + gl;
+        const bool keep = j < n && (j == 0 || L[j] != L[j - 1]);
+        uint32_t chunk;
+        const uint32_t before = group_prefix<kGcap>(keep ? 1u : 0u, chunk);
+        if (keep) c.ctslot[(size_t)ci * kCtCap + total + before] = L[j];
+        total += chunk;
+    }
+    if (live && gl == 0) c.ctcnt[ci] = total;
+}
+
+__global__ __launch_bounds__(256) void k_rb_ct_out(KArgs a, CArgs c) {
+    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    const bool live = ci < a.ncaps;
+    uint32_t n = 0;
+    int isl = -1;
+    if (live) {
+        n = c.ctcnt[ci];
+        const uint4 cp = a.cap_data[ci];
+        isl = c.bisl[cp.z];
+        const uint32_t base = c.ctscan[ci];
+        for (uint32_t j = gl; j < n; j += kGcap) {
+            const uint32_t t = c.ctslot[(size_t)ci * kCtCap + j];
+            if (base + j < c.cap_ct) c.ct[base + j] = make_uint4(cp.z, a.tri_abc[t].w, ci, t);
+        }
+        if (gl == 0 && ci == a.ncaps - 1) {
+            const uint32_t tot = base + n;
+            c.st->nct = tot;
+            if (tot > c.cap_ct) atomicOr(&c.st->overflow, 4u);
+        }
+    }
+    add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? n : 0u);
 }
 
 // near_pairs (src/phys/world.cpp:2525-2596): the node-triangle candidates within reach of their relative travel
-// until the next rebuild and not receding across the plane (flag; compaction by scan)
-__global__ void k_near(KArgs a, CArgs c, const uint4* nt, uint32_t n, int write, uint4* out) {
+// until the next rebuild and not receding across the plane (flag over the whole buffer; compaction by scan)
+__global__ void k_rb_near(KArgs a, CArgs c, int write) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint4 pr = nt[i];
+    const uint32_t n = c.st->nnt_raw < c.cap_nt ? c.st->nnt_raw : c.cap_nt;
     if (write) {
-        if (c.flag[i]) out[c.scan[i]] = pr;
+        // (every lane reaches the wave sum: no early exit)
+        uint32_t last = 0;
+        if (i < c.cap_nt) {
+            if (i == c.cap_nt - 1) c.st->nnt = c.nscan[i] + c.nflag[i];
+            if (i < n && c.nflag[i]) {
+                const uint4 pr = c.ntraw[i];
+                c.nt[c.nscan[i]] = pr;
+                last = c.bisl[pr.y] == (int)c.nisl - 1;
+            }
+        }
+        add_last(c, last);
         return;
     }
-    bool keep = !a.tri_torn[pr.w];
-    if (keep) {
-        const uint4 tr = a.tri_abc[pr.w];
-        const float4* P = a.pm[c.parity];
-        const float4* Vv = a.vm[c.parity];
-        const f3 xp = ld3(P[pr.z]), xv = ld3(Vv[pr.z]);
-        const f3 ap = ld3(P[tr.x]), bp = ld3(P[tr.y]), cp = ld3(P[tr.z]);
-        const f3 av = ld3(Vv[tr.x]), bv = ld3(Vv[tr.y]), cv = ld3(Vv[tr.z]);
-        f3 bary;
-        const f3 q = closest_on_triangle(xp, ap, bp, cp, bary);
-        const float cr = a.bodies[pr.x].collision_radius;
-        const float r0 = cr + 0.0f;
-        const f3 da = sub(xv, av), db = sub(xv, bv), dc = sub(xv, cv);
-        const float vrel = fsqrt(smax(dot(da, da), smax(dot(db, db), dot(dc, dc))));
-        const float reach = r0 + 1.25f * vrel * c.horizon + 0.02f;
-        const f3 xq = sub(xp, q);
-        if (dot(xq, xq) >= reach * reach) keep = false;
-        else {
-            const f3 nn = cross(sub(bp, ap), sub(cp, ap));
-            const float nl = length(nn);
-            if (nl > 1e-12f) {
-                const f3 n0 = divs(nn, nl);
-                const float s0 = dot(n0, sub(xp, ap));
-                const float sg = s0 >= 0 ? 1.0f : -1.0f;
-                const float close = smax(0.0f, smax(-sg * dot(n0, da), smax(-sg * dot(n0, db), -sg * dot(n0, dc))));
-                if (ffabs(s0) - 1.25f * close * c.horizon - 0.02f >= cr) keep = false;
+    if (i >= c.cap_nt) return;
+    bool keep = false;
+    if (i < n) {
+        const uint4 pr = c.ntraw[i];
+        keep = !a.tri_torn[pr.w];
+        if (keep) {
+            const uint4 tr = a.tri_abc[pr.w];
+            const float4* P = a.pm[c.parity];
+            const float4* Vv = a.vm[c.parity];
+            const f3 xp = ld3(P[pr.z]), xv = ld3(Vv[pr.z]);
+            const f3 ap = ld3(P[tr.x]), bp = ld3(P[tr.y]), cp = ld3(P[tr.z]);
+            const f3 av = ld3(Vv[tr.x]), bv = ld3(Vv[tr.y]), cv = ld3(Vv[tr.z]);
+            f3 bary;
+            const f3 q = closest_on_triangle(xp, ap, bp, cp, bary);
+            const float cr = a.bodies[pr.x].collision_radius;
+            const float r0 = cr + 0.0f;
+            const f3 da = sub(xv, av), db = sub(xv, bv), dc = sub(xv, cv);
+            const float vrel = fsqrt(smax(dot(da, da), smax(dot(db, db), dot(dc, dc))));
+            const float reach = r0 + 1.25f * vrel * c.horizon + 0.02f;
+            const f3 xq = sub(xp, q);
+            if (dot(xq, xq) >= reach * reach) keep = false;
+            else {
+                const f3 nn = cross(sub(bp, ap), sub(cp, ap));
+                const float nl = length(nn);
+                if (nl > 1e-12f) {
+                    const f3 n0 = divs(nn, nl);
+                    const float s0 = dot(n0, sub(xp, ap));
+                    const float sg = s0 >= 0 ? 1.0f : -1.0f;
+                    const float close = smax(0.0f, smax(-sg * dot(n0, da), smax(-sg * dot(n0, db), -sg * dot(n0, dc))));
+                    if (ffabs(s0) - 1.25f * close * c.horizon - 0.02f >= cr) keep = false;
+                }
             }
         }
     }
-    c.flag[i] = keep;
+    c.nflag[i] = keep;
 }
 
 // ---------------------------------------------------------------------------------------------- narrow phase
 // pair_detect (src/phys/world.cpp:2807-2962) on the concatenated candidates (node-triangle, node-capsule,
-// capsule-triangle): a hit flag and its record (compacted by scan, in pair order)
-__global__ void k_narrow(KArgs a, PArgs p, int parity) {
+// capsule-triangle), 256 per block: each block's hits packed to the front of its 256 slots, in candidate order,
+// and its count
+__global__ __launch_bounds__(256) void k_narrow(KArgs a, PArgs p, int parity) {
+    __shared__ uint32_t wcnt[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = p.nnt + p.nnc + p.nct;
-    if (i >= n) return;
     const float4* P = a.pm[parity];
     const float4* Vv = a.vm[parity];
     Hit h;
     h.pair = i, h.s = 0, h.bary = mk(0, 0, 0), h.nrm = mk(0, 0, 0), h.pen = 0;
     bool hit = false;
-    if (i < p.nnt) {
+    if (i >= n) {
+        // (past the end: no hit; the block's compaction below needs every thread)
+    } else if (i < p.nnt) {
         const uint4 pr = p.nt[i];
         if (!a.tri_torn[pr.w]) {
             const uint4 tr = a.tri_abc[pr.w];
@@ -527,17 +963,17 @@ __global__ void k_narrow(KArgs a, PArgs p, int parity) {
             hit = true;
         }
     }
-    p.hflag[i] = hit;
-    p.hraw[i] = h;
+    // the block's hits to the front of its slot range, in candidate order
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t m = __ballot(hit);
+    if (lane == 0) wcnt[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < 4; w++) off += w < wv ? wcnt[w] : 0u, tot += wcnt[w];
+    if (hit) p.hraw[blockIdx.x * 256 + off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = h;
+    if (threadIdx.x == 0) p.bcnt[blockIdx.x] = tot;
 }
 
-__global__ void k_hits_compact(PArgs p) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = p.nnt + p.nnc + p.nct;
-    if (i >= n) return;
-    if (p.hflag[i]) p.hits[p.hscan[i]] = p.hraw[i];
-    if (i == n - 1) *p.nhits = p.hscan[i] + p.hflag[i];
-}
 
 // ---------------------------------------------------------------------------------------------- response
 // The nodes and weights of a contact (collide_pairs, src/phys/world.cpp:2997-3076): side A then side B.
@@ -609,38 +1045,71 @@ __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn
     return true;
 }
 
-// One workgroup: windows of 1024 contacts; per window the (node, contact) keys sorted in LDS give each contact its
-// predecessors (the latest earlier contact of the window on each of its nodes), the longest chain to it its level;
-// the levels run one after another, the contacts of a level (no shared node) in parallel.
+// One workgroup. The hits' positions first (a prefix over the narrow phase's per-block counts), then windows of
+// 1024 hits: the window's (node, hit) keys sorted in LDS give each hit its predecessors (the latest earlier hit of
+// the window on each of its nodes), the longest chain to it its level; the levels run one after another, the hits
+// of a level (no shared node) in parallel.
 constexpr int kRespW = 1024, kRespKeys = 8192;
-__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parity) {
+__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parity, uint32_t nblk) {
     __shared__ uint64_t keys[kRespKeys];
     __shared__ int16_t preds[kRespW * 5];
     __shared__ uint16_t level[kRespW];
     __shared__ uint32_t sred[kRespW / 64];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t nh = *p.nhits;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // exclusive prefix of the block counts
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nblk; c0 += kRespW) {
+        const uint32_t i = c0 + tid;
+        const uint32_t v = i < nblk ? p.bcnt[i] : 0u;
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) sred[wv] = x;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < kRespW / 64; w++) {
+            before += w < wv ? sred[w] : 0u;
+            total += sred[w];
+        }
+        if (i < nblk) p.bpre[i] = carry + before + x - v;
+        carry += total;
+        __syncthreads();
+    }
+    const uint32_t nh = carry;
     uint32_t contacts = 0;
     for (uint32_t w0 = 0; w0 < nh; w0 += kRespW) {
+        const uint32_t m = nh - w0 < (uint32_t)kRespW ? nh - w0 : (uint32_t)kRespW;
+        uint32_t nk = 64;
+        while (nk < 5 * m) nk <<= 1;
         const uint32_t h = w0 + tid;
-        const bool valid = h < nh;
+        const bool valid = tid < m;
         Hit hit;
         ContactNodes cn;
         cn.na = cn.nb = 0;
         if (valid) {
-            hit = p.hits[h];
+            // the block holding hit h: the last one whose prefix is at most h
+            uint32_t lo = 0, hi = nblk - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (p.bpre[mid] <= h) lo = mid;
+                else hi = mid - 1;
+            }
+            hit = p.hraw[lo * 256 + (h - p.bpre[lo])];
             contact_nodes(a, p, hit, cn);
         }
+        for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
+        __syncthreads();
         for (int j = 0; j < 5; j++) {
-            keys[tid * 5 + j] = (valid && j < cn.na + cn.nb) ? (((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j) : ~0ull;
+            if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
             preds[tid * 5 + j] = -1;
         }
-        for (uint32_t k = kRespW * 5 + tid; k < kRespKeys; k += kRespW) keys[k] = ~0ull;
         __syncthreads();
-        // bitonic sort of the keys
-        for (uint32_t size = 2; size <= kRespKeys; size <<= 1)
+        // bitonic sort of the window's keys
+        for (uint32_t size = 2; size <= nk; size <<= 1)
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t q = tid; q < kRespKeys / 2; q += kRespW) {
+                for (uint32_t q = tid; q < nk / 2; q += kRespW) {
                     const uint32_t i = 2 * q - (q & (stride - 1));
                     const uint32_t j = i + stride;
                     const bool up = (i & size) == 0;
@@ -649,8 +1118,8 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parit
                 }
                 __syncthreads();
             }
-        // predecessors: the previous key of the same node from another (earlier) contact
-        for (uint32_t k = tid; k < kRespKeys; k += kRespW) {
+        // predecessors: the previous key of the same node from another (earlier) hit
+        for (uint32_t k = tid; k < nk; k += kRespW) {
             const uint64_t key = keys[k];
             if (key == ~0ull) continue;
             const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
@@ -660,7 +1129,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parit
         }
         level[tid] = 0;
         __syncthreads();
-        // levels: one past the deepest predecessor (a few rounds: the depth of the chains)
+        // levels: one past the deepest predecessor (as many rounds as the longest chain)
         int changed = 1;
         while (__syncthreads_or(changed)) {
             changed = 0;
@@ -672,19 +1141,20 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parit
             __syncthreads();
             if (L != level[tid]) level[tid] = (uint16_t)L, changed = 1;
         }
-        uint32_t my = valid ? level[tid] : 0, top = my;
+        const uint32_t my = valid ? level[tid] : 0u;
+        uint32_t top = my;
         for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
-        if ((tid & 63) == 0) sred[tid >> 6] = top;
+        if (lane == 0) sred[wv] = top;
         __syncthreads();
         uint32_t maxl = 0;
-        for (int w = 0; w < kRespW / 64; w++) maxl = max(maxl, sred[w]);
+        for (uint32_t w = 0; w < kRespW / 64; w++) maxl = max(maxl, sred[w]);
         for (uint32_t L = 0; L <= maxl; L++) {
             if (valid && my == L && contact_force(a, parity, cn, hit.nrm, hit.pen)) contacts++;
             __syncthreads();
         }
     }
     for (int off = 32; off > 0; off >>= 1) contacts += __shfl_xor(contacts, off);
-    if ((tid & 63) == 0 && contacts) atomicAdd(&a.counters[CNT_CONTACTS], (unsigned long long)contacts);
+    if (lane == 0 && contacts) atomicAdd(&a.counters[CNT_CONTACTS], (unsigned long long)contacts);
 }
 
 // ---------------------------------------------------------------------------------------------- host side
@@ -702,17 +1172,10 @@ namespace {
 
 inline dim3 grid_for(size_t n, int block = 256) { return dim3((unsigned)((n + block - 1) / block)); }
 
-inline int bits_for(uint32_t nbk) {
-    int b = 0;
-    while ((1u << b) < nbk) b++;
-    return b;
-}
-
-// build_buckets' table size (src/phys/world.cpp:1985-1995): a power of two, at least 64 and twice the entries
-inline uint32_t bucket_count(size_t entries) {
-    uint32_t nbk = 64;
-    while (nbk < entries * 2) nbk <<= 1;
-    return nbk;
+inline uint32_t pow2_at_least(size_t n) {
+    uint32_t v = 64;
+    while (v < n) v <<= 1;
+    return v;
 }
 
 } // namespace
@@ -726,70 +1189,106 @@ int ContactEngine::scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t
     return BL_OK;
 }
 
-int ContactEngine::sort(const uint64_t* in, uint64_t* out, size_t n, int end_bit, hipStream_t st) {
-    if (n == 0) return BL_OK;
-    size_t bytes = 0;
-    CHK(rocprim::radix_sort_keys(nullptr, bytes, in, out, n, 0u, (unsigned)end_bit, st));
-    RC(temp.ensure(bytes));
-    CHK(rocprim::radix_sort_keys(temp.p, bytes, in, out, n, 0u, (unsigned)end_bit, st));
-    return BL_OK;
-}
-
-// The total of an exclusive scan over n flags or counts (waits for the stream).
-int ContactEngine::total(const uint32_t* scan_out, const uint32_t* in, size_t n, uint32_t& tot, hipStream_t st) {
-    tot = 0;
-    if (n == 0) return BL_OK;
-    uint32_t x[2] = {0, 0};
-    CHK(hipMemcpyAsync(&x[0], scan_out + n - 1, 4, hipMemcpyDeviceToHost, st));
-    CHK(hipMemcpyAsync(&x[1], in + n - 1, 4, hipMemcpyDeviceToHost, st));
-    CHK(hipStreamSynchronize(st));
-    tot = x[0] + x[1];
-    return BL_OK;
-}
-
 void ContactEngine::release() {
     temp.release();
-    for (Buf<uint32_t>* b : {&flag, &scanv, &scan2, &count, &bstart, &items, &run_end, &entry_pid, &prim_tri, &hflag, &hscan,
-                             &nhits, &overflow, &isl_query, &isl_self, &isl_body, &part_off, &part, &sample_base, &body_tri0})
+    bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
+    nmem = cap_part = 0;
+    for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pflag, &pscan, &pbucket, &tcount,
+                             &tstart, &tfill, &titems, &sbody, &tcnt, &tscan, &ccnt, &cscan, &ecnt, &escan, &eb,
+                             &ecount, &estart, &efill, &ctcnt, &ctscan, &nflag, &nscan, &bcnt, &bpre})
         b->release();
-    isl_local.release(), boxes.release(), samples.release(), keys.release(), sorted.release();
-    pt_cell.release(), pt_pos.release();
-    for (Buf<uint4>* b : {&nt, &nt2, &nc, &ct, &all_nt, &all_nc, &all_ct}) b->release();
-    hraw.release(), hits.release();
+    bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
+    scell.release(), spos.release(), big.release(), ctslot.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
+    for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
+    hraw.release();
     nnt = nnc = nct = 0;
+    cap_nt = cap_nc = cap_ct = cap_ent = esize = 0;
 }
 
-// The per-body tables the rebuilds read (first triangle, first size sample); the flags cleared.
-int ContactEngine::setup(const HostView& hv) {
-    const size_t nb = hv.bodies.size();
-    std::vector<uint32_t> tri0(nb), s0(nb);
-    uint32_t acc = 0;
-    for (size_t i = 0; i < nb; i++) {
-        tri0[i] = hv.bodies[i].tri0;
-        s0[i] = acc;
-        acc += (hv.bodies[i].ntri + 6) / 7;
+// The buffers whose sizes follow the candidate counts (grown when a rebuild reports an overflow).
+int ContactEngine::ensure_lists() {
+    RC(ntraw.ensure(cap_nt));
+    RC(nt.ensure(cap_nt));
+    RC(nflag.ensure(cap_nt));
+    RC(nscan.ensure(cap_nt));
+    RC(nc.ensure(cap_nc));
+    RC(ct.ensure(cap_ct));
+    RC(eb.ensure(cap_ent));
+    RC(ea.ensure(cap_ent));
+    RC(ebb.ensure(cap_ent));
+    RC(ec.ensure(cap_ent));
+    RC(sa.ensure(cap_ent));
+    RC(sb.ensure(cap_ent));
+    RC(sc.ensure(cap_ent));
+    esize = pow2_at_least(2 * (size_t)cap_ent);
+    RC(ecount.ensure((size_t)esize + 1));
+    RC(estart.ensure((size_t)esize + 1));
+    RC(efill.ensure((size_t)esize + 1));
+    return BL_OK;
+}
+
+// The per-body tables (first triangle, first size sample), the node hash and the first list sizes.
+int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C) {
+    nnodes = N, ntris = T, ncaps = C, nbodies = (uint32_t)hv.bodies.size();
+    std::vector<uint32_t> tri0(nbodies + 1, 0), s0(nbodies + 1, 0);
+    for (uint32_t i = 0; i < nbodies; i++) {
+        tri0[i + 1] = tri0[i] + hv.bodies[i].ntri;
+        s0[i + 1] = s0[i] + (hv.bodies[i].ntri + 6) / 7;
     }
-    nsamples = acc;
-    RC(body_tri0.ensure(nb));
-    RC(sample_base.ensure(nb));
-    RC(samples.ensure(acc ? acc : 1));
-    RC(overflow.ensure(1));
-    RC(nhits.ensure(1));
-    if (nb) {
-        CHK(hipMemcpy(body_tri0.p, tri0.data(), 4 * nb, hipMemcpyHostToDevice));
-        CHK(hipMemcpy(sample_base.p, s0.data(), 4 * nb, hipMemcpyHostToDevice));
-    }
-    CHK(hipMemset(overflow.p, 0, 4));
-    CHK(hipMemset(nhits.p, 0, 4));
+    nsamples = s0[nbodies];
+    std::vector<uint32_t> sb(nsamples + 1, 0);
+    for (uint32_t i = 0; i < nbodies; i++)
+        for (uint32_t j = s0[i]; j < s0[i + 1]; j++) sb[j] = i;
+    RC(sbody.ensure(nsamples + 1));
+    CHK(hipMemcpy(sbody.p, sb.data(), 4 * (nsamples + 1), hipMemcpyHostToDevice));
+    RC(body_tri0.ensure(nbodies + 1));
+    RC(sample_base.ensure(nbodies + 1));
+    RC(samples.ensure(nsamples + 1));
+    CHK(hipMemcpy(body_tri0.p, tri0.data(), 4 * (nbodies + 1), hipMemcpyHostToDevice));
+    CHK(hipMemcpy(sample_base.p, s0.data(), 4 * (nbodies + 1), hipMemcpyHostToDevice));
+    RC(bisl.ensure(nbodies + 1));
+    RC(bmem.ensure(nbodies + 1));
+    RC(bbox.ensure(6 * (size_t)nbodies + 6));
+    RC(bquery.ensure(nbodies + 1));
+    RC(icount.ensure(nbodies + 1));
+    RC(ipar.ensure(nbodies + 1));
+    RC(stat.ensure(1));
+    RC(pflag.ensure(N + 1));
+    RC(pscan.ensure(N + 1));
+    RC(pbucket.ensure(N + 1));
+    RC(pcell.ensure(N + 1));
+    RC(ppos.ensure(N + 1));
+    RC(titems.ensure(N + 1));
+    RC(scell.ensure(N + 1));
+    RC(spos.ensure(N + 1));
+    tsize = pow2_at_least(4 * (size_t)N);
+    RC(big.ensure((size_t)tsize + 1));
+    RC(tcount.ensure((size_t)tsize + 1));
+    RC(tstart.ensure((size_t)tsize + 1));
+    RC(tfill.ensure((size_t)tsize + 1));
+    RC(tcnt.ensure(T + 1));
+    RC(tscan.ensure(T + 1));
+    RC(ecnt.ensure(T + 1));
+    RC(escan.ensure(T + 1));
+    RC(ccnt.ensure(C + 1));
+    RC(cscan.ensure(C + 1));
+    RC(ctcnt.ensure(C + 1));
+    RC(ctscan.ensure(C + 1));
+    RC(ctslot.ensure((size_t)C * kCtCap + 1));
+    cap_nt = std::max<uint32_t>(4096, 2 * T);
+    cap_nc = std::max<uint32_t>(4096, 4 * C);
+    cap_ct = std::max<uint32_t>(4096, 4 * C);
+    cap_ent = std::max<uint32_t>(4096, 8 * T);
+    RC(ensure_lists());
     nnt = nnc = nct = 0;
-    unsupported = false;
     return BL_OK;
 }
 
 // World::build_islands (src/phys/world.cpp:636-762) for awake node/beam bodies: union-find over the expanded bounds
 // of bodies that can touch (contacters against triangles or capsules, capsules against triangles); an island's
 // bodies in body order, islands by their first body. Islands of several bodies, or of one self-colliding body,
-// search for pairs.
+// search for pairs. The candidate pairs of bodies are found by sorting the boxes along x (a sweep; the union order
+// does not change the islands).
 void ContactEngine::build_islands(const HostView& hv, float frame_time) {
     const size_t nb = hv.bodies.size();
     std::vector<int> parent(nb);
@@ -805,19 +1304,32 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         const float mn[3] = {hv.mn[i].x, hv.mn[i].y, hv.mn[i].z}, mx[3] = {hv.mx[i].x, hv.mx[i].y, hv.mx[i].z};
         for (int k = 0; k < 3; k++) bx[6 * i + k] = mn[k] - r, bx[6 * i + 3 + k] = mx[k] + r;
     }
-    if (hv.inter_body)
-        for (size_t i = 0; i < nb; i++)
-            for (size_t j = i + 1; j < nb; j++) {
-                const float *p = &bx[6 * i], *q = &bx[6 * j];
-                if (!(p[0] <= q[3] && p[3] >= q[0] && p[1] <= q[4] && p[4] >= q[1] && p[2] <= q[5] && p[5] >= q[2])) continue;
-                const BodyView &bi = hv.bodies[i], &bj = hv.bodies[j];
-                if (bi.group != 0 && bi.group == bj.group) continue;
-                const bool ti = bi.ntri > 0, tj = bj.ntri > 0, ci = bi.ncap > 0, cj = bj.ncap > 0;
-                const bool can = (bi.contacters && (tj || cj)) || (bj.contacters && (ti || ci)) || (ci && tj) || (cj && ti);
-                if (!can) continue;
-                const int x = find((int)i), y = find((int)j);
-                if (x != y) parent[x] = y;
-            }
+    auto overlaps = [&](size_t i, size_t j) {
+        const float *p = &bx[6 * i], *q = &bx[6 * j];
+        return p[0] <= q[3] && p[3] >= q[0] && p[1] <= q[4] && p[4] >= q[1] && p[2] <= q[5] && p[5] >= q[2];
+    };
+    auto consider = [&](size_t i, size_t j) {
+        if (!overlaps(i, j)) return;
+        const BodyView &bi = hv.bodies[i], &bj = hv.bodies[j];
+        if (bi.group != 0 && bi.group == bj.group) return;
+        const bool ti = bi.ntri > 0, tj = bj.ntri > 0, ci = bi.ncap > 0, cj = bj.ncap > 0;
+        const bool can = (bi.contacters && (tj || cj)) || (bj.contacters && (ti || ci)) || (ci && tj) || (cj && ti);
+        if (!can) return;
+        const int x = find((int)i), y = find((int)j);
+        if (x != y) parent[x] = y;
+    };
+    if (hv.inter_body) {
+        // bodies whose boxes are not finite take the plain all-pairs test
+        std::vector<uint32_t> order, odd;
+        for (size_t i = 0; i < nb; i++) (std::isfinite(bx[6 * i]) && std::isfinite(bx[6 * i + 3]) ? order : odd).push_back((uint32_t)i);
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return bx[6 * a] < bx[6 * b] || (bx[6 * a] == bx[6 * b] && a < b); });
+        for (size_t u = 0; u < order.size(); u++)
+            for (size_t v = u + 1; v < order.size() && bx[6 * order[v]] <= bx[6 * order[u] + 3]; v++)
+                consider(std::min(order[u], order[v]), std::max(order[u], order[v]));
+        for (uint32_t i : odd)
+            for (size_t j = 0; j < nb; j++)
+                if (j != i) consider(std::min<size_t>(i, j), std::max<size_t>(i, j));
+    }
     islands.clear();
     std::vector<int> island_of(nb, -1);
     for (size_t i = 0; i < nb; i++) {
@@ -830,255 +1342,146 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         if (hv.inter_body && (is.size() > 1 || hv.bodies[is[0]].self)) pair_islands.push_back(is);
 }
 
-// One island's pair lists (rebuild_pairs, then near_pairs), appended to the frame's.
-int ContactEngine::rebuild_island(const KArgs& a, const HostView& hv, const std::vector<uint32_t>& isl,
-                                  const std::vector<float>& body_v2, const std::vector<float>& body_box,
-                                  const std::vector<float>& smp, hipStream_t st, int parity, int64_t& island_pairs) {
-    const uint32_t nb = (uint32_t)isl.size();
-    float rmax = 0, slow_speed = 0;
-    bool any_caps = false, any_tris = false;
-    for (uint32_t g : isl) {
-        const BodyView& b = hv.bodies[g];
-        rmax = smax(rmax, b.collision_radius);
-        any_caps |= b.ncap > 0;
-        any_tris |= b.ntri > 0;
+int ContactEngine::begin_frame(hipStream_t st) {
+    std::vector<int> isl(nbodies, -1), mi(nbodies, -1);
+    std::vector<uint32_t> off(pair_islands.size() + 1, 0), mem;
+    for (size_t k = 0; k < pair_islands.size(); k++) {
+        for (uint32_t g : pair_islands[k]) isl[g] = (int)k, mi[g] = (int)mem.size(), mem.push_back(g);
+        off[k + 1] = (uint32_t)mem.size();
     }
-    for (uint32_t g : isl) {
-        const float ms = std::sqrt(body_v2[g]);
-        if (ms <= 8.0f) slow_speed = smax(slow_speed, ms);
-        else unsupported = true; // a fast body: its own pair schedule (fast_pairs) is not stepped here
-    }
-    const float speed = slow_speed + 0.5f;
-    const float margin = 2.0f * speed * (float)interval * hv.dt + 0.02f;
-    // the cell size from every 7th triangle's longest edge (a double sum in body, triangle order)
-    double tri_size = 0;
-    int nsmp = 0;
-    for (uint32_t g : isl) {
-        const BodyView& b = hv.bodies[g];
-        uint32_t base = 0;
-        for (uint32_t q = 0; q < g; q++) base += (hv.bodies[q].ntri + 6) / 7;
-        for (uint32_t t = 0; t < b.ntri; t += 7) tri_size += smp[base + t / 7], nsmp++;
-    }
-    const float mean = nsmp ? (float)(tri_size / nsmp) : 1.0f;
-    const float inv_cell = 1.0f / clampf(mean + 2.0f * (rmax + margin), 0.25f, 1.0f);
-    const float reach = rmax + margin;
-    // partners: island bodies of other groups whose expanded boxes meet
-    std::vector<float> bxs(nb * 6);
-    for (uint32_t k = 0; k < nb; k++)
-        for (int q = 0; q < 3; q++) {
-            bxs[6 * k + q] = body_box[6 * isl[k] + q] - (reach + 0.0f + 0.0f);
-            bxs[6 * k + 3 + q] = body_box[6 * isl[k] + 3 + q] + (reach + 0.0f + 0.0f);
-        }
-    std::vector<std::vector<uint32_t>> partners(nb);
-    for (uint32_t x = 0; x < nb; x++)
-        for (uint32_t y = x + 1; y < nb; y++) {
-            const BodyView &bx = hv.bodies[isl[x]], &by = hv.bodies[isl[y]];
-            if (bx.group != 0 && bx.group == by.group) continue;
-            const float *p = &bxs[6 * x], *q = &bxs[6 * y];
-            if (!(p[0] <= q[3] && p[3] >= q[0] && p[1] <= q[4] && p[4] >= q[1] && p[2] <= q[5] && p[5] >= q[2])) continue;
-            partners[x].push_back(y), partners[y].push_back(x);
-        }
-    std::vector<uint32_t> poff(nb + 1, 0), plist, query(nb), self(nb);
-    for (uint32_t k = 0; k < nb; k++) {
-        plist.insert(plist.end(), partners[k].begin(), partners[k].end());
-        poff[k + 1] = (uint32_t)plist.size();
-        self[k] = hv.bodies[isl[k]].self;
-        query[k] = !partners[k].empty() || self[k];
-    }
-    std::vector<int> local(hv.bodies.size(), -1);
-    for (uint32_t k = 0; k < nb; k++) local[isl[k]] = (int)k;
-    RC(isl_local.ensure(local.size()));
-    RC(isl_query.ensure(nb));
-    RC(isl_self.ensure(nb));
-    RC(isl_body.ensure(nb));
-    RC(boxes.ensure(bxs.size()));
-    RC(part_off.ensure(nb + 1));
-    RC(part.ensure(plist.size() + 1));
-    CHK(hipMemcpyAsync(isl_local.p, local.data(), sizeof(int) * local.size(), hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(isl_query.p, query.data(), 4 * nb, hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(isl_self.p, self.data(), 4 * nb, hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(isl_body.p, isl.data(), 4 * nb, hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(boxes.p, bxs.data(), sizeof(float) * bxs.size(), hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(part_off.p, poff.data(), 4 * (nb + 1), hipMemcpyHostToDevice, st));
-    if (!plist.empty()) CHK(hipMemcpyAsync(part.p, plist.data(), 4 * plist.size(), hipMemcpyHostToDevice, st));
-    const size_t N = a.nnodes, T = a.ntris, C = a.ncaps;
-    const size_t big = std::max(N, std::max(T, C)) + 1;
-    RC(flag.ensure(big));
-    RC(scanv.ensure(big));
-    RC(scan2.ensure(big));
-    RC(count.ensure(big));
-    CArgs c{};
-    c.parity = parity, c.margin = margin, c.inv_cell = inv_cell, c.rmax = rmax;
-    c.horizon = (float)(interval > 1 ? interval : 1) * hv.dt;
-    c.island_local = isl_local.p, c.query = isl_query.p, c.self = isl_self.p, c.island_body = isl_body.p;
-    c.boxes = boxes.p, c.part_off = part_off.p, c.part = part.p;
-    c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p, c.overflow = overflow.p;
-    c.flag = flag.p, c.scan = scanv.p, c.scan2 = scan2.p, c.count = count.p;
-    uint32_t nnt_i = 0, nnc_i = 0, nct_i = 0;
-    // 1) the contacter nodes near a partner into the hash
-    hipLaunchKernelGGL(k_pts, grid_for(N), dim3(256), 0, st, a, c, 0);
-    RC(scan(flag.p, scanv.p, N, st));
-    RC(total(scanv.p, flag.p, N, c.npts, st));
-    if (c.npts && (any_tris || any_caps)) {
-        const uint32_t nbk = bucket_count(c.npts);
-        c.mask = nbk - 1;
-        RC(keys.ensure(c.npts));
-        RC(sorted.ensure(c.npts));
-        RC(pt_cell.ensure(c.npts));
-        RC(pt_pos.ensure(c.npts));
-        RC(bstart.ensure((size_t)nbk + 1));
-        RC(items.ensure(c.npts));
-        RC(run_end.ensure(c.npts));
-        c.keys = keys.p, c.pt_cell = pt_cell.p, c.pt_pos = pt_pos.p;
-        hipLaunchKernelGGL(k_pts, grid_for(N), dim3(256), 0, st, a, c, 1);
-        RC(sort(keys.p, sorted.p, c.npts, 32 + bits_for(nbk), st));
-        hipLaunchKernelGGL(k_buckets, grid_for((size_t)c.npts + 1), dim3(256), 0, st, c, c.npts, nbk, sorted.p, bstart.p,
-                           items.p, (const float4*)pt_pos.p, run_end.p);
-        c.bstart = bstart.p, c.items = items.p, c.run_end = run_end.p;
-        // 2) the triangles, then the capsules, of the querying bodies against the hash (count, scan, write)
-        if (T) {
-            hipLaunchKernelGGL(k_query_tris, grid_for(T), dim3(256), 0, st, a, c, 0);
-            RC(scan(count.p, scanv.p, T, st));
-            RC(total(scanv.p, count.p, T, nnt_i, st));
-            if (nnt_i) {
-                RC(nt.ensure(nnt_i));
-                c.nt = nt.p;
-                hipLaunchKernelGGL(k_query_tris, grid_for(T), dim3(256), 0, st, a, c, 1);
-            }
-        }
-        if (C) {
-            hipLaunchKernelGGL(k_query_caps, grid_for(C), dim3(256), 0, st, a, c, 0);
-            RC(scan(count.p, scanv.p, C, st));
-            RC(total(scanv.p, count.p, C, nnc_i, st));
-            if (nnc_i) {
-                RC(nc.ensure(nnc_i));
-                c.nc = nc.p;
-                hipLaunchKernelGGL(k_query_caps, grid_for(C), dim3(256), 0, st, a, c, 1);
-            }
-        }
-    }
-    // 3) capsules against triangles: the triangles into every cell of their boxes, the capsules query
-    if (any_caps && any_tris) {
-        hipLaunchKernelGGL(k_prims, grid_for(T), dim3(256), 0, st, a, c, 0);
-        uint32_t nprims = 0, nentries = 0;
-        RC(scan(flag.p, scanv.p, T, st));
-        RC(total(scanv.p, flag.p, T, nprims, st));
-        RC(scan(count.p, scan2.p, T, st));
-        RC(total(scan2.p, count.p, T, nentries, st));
-        c.nprims = nprims;
-        if (nprims) {
-            const uint32_t nbk = bucket_count(nentries);
-            c.mask = nbk - 1;
-            RC(keys.ensure(nentries));
-            RC(sorted.ensure(nentries));
-            RC(entry_pid.ensure(nentries));
-            RC(prim_tri.ensure(nprims));
-            RC(bstart.ensure((size_t)nbk + 1));
-            RC(items.ensure(nentries));
-            c.keys = keys.p, c.entry_pid = entry_pid.p, c.prim_tri = prim_tri.p;
-            hipLaunchKernelGGL(k_prims, grid_for(T), dim3(256), 0, st, a, c, 1);
-            RC(sort(keys.p, sorted.p, nentries, 32 + bits_for(nbk), st));
-            hipLaunchKernelGGL(k_buckets, grid_for((size_t)nentries + 1), dim3(256), 0, st, c, nentries, nbk, sorted.p,
-                               bstart.p, items.p, (const float4*)nullptr, (uint32_t*)nullptr);
-            c.bstart = bstart.p, c.items = items.p;
-            if (C) {
-                hipLaunchKernelGGL(k_query_ct, grid_for(C), dim3(256), 0, st, a, c, 0);
-                RC(scan(count.p, scanv.p, C, st));
-                RC(total(scanv.p, count.p, C, nct_i, st));
-                if (nct_i) {
-                    RC(ct.ensure(nct_i));
-                    c.ct = ct.p;
-                    hipLaunchKernelGGL(k_query_ct, grid_for(C), dim3(256), 0, st, a, c, 1);
-                }
-            }
-        }
-    }
-    // near_pairs on the node-triangle candidates, then the island's lists onto the frame's
-    uint32_t nkeep = 0;
-    if (nnt_i) {
-        RC(nt2.ensure(nnt_i));
-        RC(flag.ensure(nnt_i));
-        RC(scanv.ensure(nnt_i));
-        c.flag = flag.p, c.scan = scanv.p;
-        hipLaunchKernelGGL(k_near, grid_for(nnt_i), dim3(256), 0, st, a, c, (const uint4*)nt.p, nnt_i, 0, nt2.p);
-        RC(scan(flag.p, scanv.p, nnt_i, st));
-        RC(total(scanv.p, flag.p, nnt_i, nkeep, st));
-        hipLaunchKernelGGL(k_near, grid_for(nnt_i), dim3(256), 0, st, a, c, (const uint4*)nt.p, nnt_i, 1, nt2.p);
-    }
-    if (nkeep || nnc_i || nct_i) {
-        // (the lists grow: keep what earlier islands wrote)
-        auto grow = [&](Buf<uint4>& b, size_t have, size_t need) -> int {
-            if (b.p && need <= b.cap) return BL_OK;
-            Buf<uint4> nb2;
-            RC(nb2.ensure(need));
-            if (have) CHK(hipMemcpyAsync(nb2.p, b.p, sizeof(uint4) * have, hipMemcpyDeviceToDevice, st));
-            CHK(hipStreamSynchronize(st));
-            b.release();
-            b = nb2;
-            nb2.p = nullptr;
-            return BL_OK;
-        };
-        RC(grow(all_nt, nnt, nnt + nkeep));
-        RC(grow(all_nc, nnc, nnc + nnc_i));
-        RC(grow(all_ct, nct, nct + nct_i));
-        if (nkeep) CHK(hipMemcpyAsync(all_nt.p + nnt, nt2.p, sizeof(uint4) * nkeep, hipMemcpyDeviceToDevice, st));
-        if (nnc_i) CHK(hipMemcpyAsync(all_nc.p + nnc, nc.p, sizeof(uint4) * nnc_i, hipMemcpyDeviceToDevice, st));
-        if (nct_i) CHK(hipMemcpyAsync(all_ct.p + nct, ct.p, sizeof(uint4) * nct_i, hipMemcpyDeviceToDevice, st));
-    }
-    nnt += nkeep, nnc += nnc_i, nct += nct_i;
-    island_pairs = (int64_t)nkeep + nnc_i + nct_i;
-    CHK(hipStreamSynchronize(st));
-    CHK(hipGetLastError());
-    uint32_t ov = 0;
-    CHK(hipMemcpy(&ov, overflow.p, 4, hipMemcpyDeviceToHost));
-    if (ov) unsupported = true;
+    nmem = (uint32_t)mem.size();
+    RC(imem_off.ensure(off.size()));
+    RC(imem.ensure(mem.size() + 1));
+    RC(pcnt.ensure(nmem + 1));
+    RC(poff.ensure(nmem + 1));
+    cap_part = std::max<uint32_t>(cap_part, std::max<uint32_t>(4096, 8 * nmem));
+    RC(plist.ensure(cap_part));
+    if (nbodies) CHK(hipMemcpyAsync(bisl.p, isl.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
+    if (nbodies) CHK(hipMemcpyAsync(bmem.p, mi.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
+    CHK(hipMemcpyAsync(imem_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
+    if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
+    CHK(hipStreamSynchronize(st)); // (the host vectors go out of scope)
+    nnt = nnc = nct = 0;
     return BL_OK;
 }
 
-// The frame's pair lists anew (its first substep and every `interval` after): torn triangles and size samples
-// first, then each pair island's rebuild, in island order.
-int ContactEngine::rebuild(const KArgs& a, const HostView& hv, const std::vector<float>& body_v2,
-                           const std::vector<float>& body_box, hipStream_t st, int parity) {
+// The rebuild's launches: no host wait between them; the counts stay on the device.
+int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
+    const uint32_t N = nnodes, T = ntris, C = ncaps, NB = nbodies;
+    c.cap_nt = cap_nt, c.cap_nc = cap_nc, c.cap_ct = cap_ct, c.cap_ent = cap_ent, c.cap_part = cap_part;
+    c.plist = plist.p;
+    c.ntraw = ntraw.p, c.nc = nc.p, c.ct = ct.p, c.nt = nt.p, c.nflag = nflag.p, c.nscan = nscan.p;
+    c.eb = eb.p, c.ea = ea.p, c.ebb = ebb.p, c.ec = ec.p, c.sa = sa.p, c.sb = sb.p, c.sc = sc.p;
+    c.ecount = ecount.p, c.estart = estart.p, c.efill = efill.p;
+    c.emask = esize - 1;
+    hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, (uint32_t*)stat.p, (uint32_t)(sizeof(RStat) / 4));
+    if (T) hipLaunchKernelGGL(k_rb_torn, grid_for(T), dim3(256), 0, st, a, c);
+    hipLaunchKernelGGL(k_rb_island, dim3(c.nisl), dim3(256), 0, st, a, c);
+    hipLaunchKernelGGL(k_rb_boxes, grid_for(NB), dim3(256), 0, st, a, c);
+    hipLaunchKernelGGL(k_rb_partners, grid_for((size_t)nmem * 64), dim3(256), 0, st, a, c, 0);
+    RC(scan(pcnt.p, poff.p, nmem, st));
+    hipLaunchKernelGGL(k_rb_partners, grid_for((size_t)nmem * 64), dim3(256), 0, st, a, c, 1);
+    // 1) the node hash
+    hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tcount.p, tsize + 1);
+    hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tfill.p, tsize + 1);
+    hipLaunchKernelGGL(k_rb_pts, grid_for(N), dim3(256), 0, st, a, c, 0);
+    RC(scan(pflag.p, pscan.p, N, st));
+    hipLaunchKernelGGL(k_rb_pts, grid_for(N), dim3(256), 0, st, a, c, 1);
+    RC(scan(tcount.p, tstart.p, (size_t)tsize + 1, st));
+    hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
+    hipLaunchKernelGGL(k_rb_sortb, grid_for(tsize), dim3(256), 0, st, c, tsize);
+    hipLaunchKernelGGL(k_rb_sortbig, dim3(64), dim3(256), 0, st, c);
+    // 2) triangles, then capsules, against the node hash
+    if (T) {
+        hipLaunchKernelGGL(k_rb_nt, grid_for((size_t)T * kGnt), dim3(256), 0, st, a, c, 0);
+        RC(scan(tcnt.p, tscan.p, T, st));
+        hipLaunchKernelGGL(k_rb_nt, grid_for((size_t)T * kGnt), dim3(256), 0, st, a, c, 1);
+    }
+    if (C) {
+        hipLaunchKernelGGL(k_rb_nc, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c, 0);
+        RC(scan(ccnt.p, cscan.p, C, st));
+        hipLaunchKernelGGL(k_rb_nc, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c, 1);
+    }
+    // 3) capsules against the triangles' cells
+    if (C && T) {
+        hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, ecount.p, esize + 1);
+        hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, efill.p, esize + 1);
+        hipLaunchKernelGGL(k_rb_prims, grid_for(T), dim3(256), 0, st, a, c, 0);
+        RC(scan(ecnt.p, escan.p, T, st));
+        hipLaunchKernelGGL(k_rb_prims, grid_for(T), dim3(256), 0, st, a, c, 1);
+        RC(scan(ecount.p, estart.p, (size_t)esize + 1, st));
+        hipLaunchKernelGGL(k_rb_efill, grid_for(cap_ent), dim3(256), 0, st, c);
+        hipLaunchKernelGGL(k_rb_ct, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c);
+        RC(scan(ctcnt.p, ctscan.p, C, st));
+        hipLaunchKernelGGL(k_rb_ct_out, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c);
+    }
+    // the near-pair filter on the node-triangle candidates
+    if (T) {
+        hipLaunchKernelGGL(k_rb_near, grid_for(cap_nt), dim3(256), 0, st, a, c, 0);
+        RC(scan(nflag.p, nscan.p, cap_nt, st));
+        hipLaunchKernelGGL(k_rb_near, grid_for(cap_nt), dim3(256), 0, st, a, c, 1);
+    }
+    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
+}
+
+// The frame's pair lists anew (its first substep and every `interval` after), all pair islands at once. One wait
+// for the device at the end; a list that outgrew its buffer grows it and the rebuild runs again (the rebuild only
+// reads the state, so a second run finds the same lists).
+int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t st, int parity) {
     nnt = nnc = nct = 0;
     if (pair_islands.empty()) return BL_OK;
-    std::vector<int> local(hv.bodies.size(), -1);
-    for (auto& is : pair_islands)
-        for (uint32_t g : is) local[g] = 0;
-    RC(isl_local.ensure(local.size()));
-    CHK(hipMemcpyAsync(isl_local.p, local.data(), sizeof(int) * local.size(), hipMemcpyHostToDevice, st));
     CArgs c{};
-    c.parity = parity, c.island_local = isl_local.p, c.samples = samples.p, c.sample_base = sample_base.p;
-    c.body_tri0 = body_tri0.p;
-    if (a.ntris) hipLaunchKernelGGL(k_tris_torn, grid_for(a.ntris), dim3(256), 0, st, a, c);
-    std::vector<float> smp(nsamples ? nsamples : 1, 0.0f);
-    if (nsamples) CHK(hipMemcpyAsync(smp.data(), samples.p, 4 * nsamples, hipMemcpyDeviceToHost, st));
-    CHK(hipStreamSynchronize(st));
-    for (auto& is : pair_islands) {
-        int64_t pairs = 0;
-        RC(rebuild_island(a, hv, is, body_v2, body_box, smp, st, parity, pairs));
-        pair_rebuilds++;
-        contact_pairs = pairs; // (the last island's count, as the reference's counter keeps)
+    c.parity = parity;
+    c.horizon = (float)(interval > 1 ? interval : 1) * a.dt;
+    c.interval = interval, c.dt = a.dt;
+    c.bisl = bisl.p, c.imem_off = imem_off.p, c.imem = imem.p, c.nisl = (uint32_t)pair_islands.size();
+    c.nmem = nmem, c.bmem = bmem.p, c.pcnt = pcnt.p, c.poff = poff.p;
+    c.ipar = ipar.p, c.bbox = bbox.p, c.bquery = bquery.p, c.body_box = body_box;
+    c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
+    c.sbody = sbody.p, c.nsamples = nsamples;
+    c.st = stat.p, c.icount = icount.p;
+    c.pflag = pflag.p, c.pscan = pscan.p, c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
+    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p, c.tfill = tfill.p, c.titems = titems.p;
+    c.scell = scell.p, c.spos = spos.p, c.big = big.p;
+    c.tcnt = tcnt.p, c.tscan = tscan.p, c.ccnt = ccnt.p, c.cscan = cscan.p;
+    c.ecnt = ecnt.p, c.escan = escan.p, c.ctcnt = ctcnt.p, c.ctscan = ctscan.p, c.ctslot = ctslot.p;
+    for (int attempt = 0;; attempt++) {
+        RC(enqueue_rebuild(a, c, st));
+        CHK(hipMemcpyAsync(&last, stat.p, sizeof(RStat), hipMemcpyDeviceToHost, st));
+        CHK(hipStreamSynchronize(st));
+        if (!last.overflow) break;
+        if (attempt > 4) return BL_ERR_HIP;
+        if (last.overflow & 1) cap_nt = std::max<uint32_t>(cap_nt, last.nnt_raw + last.nnt_raw / 2 + 1);
+        if (last.overflow & 2) cap_nc = std::max<uint32_t>(cap_nc, last.nnc + last.nnc / 2 + 1);
+        if (last.overflow & 4) cap_ct = std::max<uint32_t>(cap_ct, last.nct + last.nct / 2 + 1);
+        if (last.overflow & 8) cap_ent = std::max<uint32_t>(cap_ent, last.nent + last.nent / 2 + 1);
+        if (last.overflow & 16) {
+            cap_part = std::max<uint32_t>(cap_part, last.npart + last.npart / 2 + 1);
+            RC(plist.ensure(cap_part));
+        }
+        RC(ensure_lists());
     }
+    nnt = last.nnt, nnc = last.nnc, nct = last.nct;
+    pair_rebuilds += (int64_t)pair_islands.size();
+    contact_pairs = last.last_pairs;
     return BL_OK;
 }
 
-// The substep's contacts: narrow phase over the frame's pairs, order-preserving compaction, the response.
+// The substep's contacts: the narrow phase over the frame's candidates, then the response in candidate order.
 int ContactEngine::collide(const KArgs& a, hipStream_t st, int parity) {
     const size_t n = nnt + nnc + nct;
     narrow_tests += (int64_t)n;
     if (n == 0) return BL_OK;
-    RC(hflag.ensure(n));
-    RC(hscan.ensure(n));
-    RC(hraw.ensure(n));
-    RC(hits.ensure(n));
+    const uint32_t nblk = (uint32_t)((n + 255) / 256);
+    RC(hraw.ensure((size_t)nblk * 256));
+    RC(bcnt.ensure(nblk));
+    RC(bpre.ensure((size_t)nblk + 1));
     PArgs p{};
-    p.nt = all_nt.p, p.nc = all_nc.p, p.ct = all_ct.p;
+    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p;
     p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct;
-    p.hflag = hflag.p, p.hscan = hscan.p, p.hraw = hraw.p, p.hits = hits.p, p.nhits = nhits.p;
-    hipLaunchKernelGGL(k_narrow, grid_for(n), dim3(256), 0, st, a, p, parity);
-    RC(scan(hflag.p, hscan.p, n, st));
-    hipLaunchKernelGGL(k_hits_compact, grid_for(n), dim3(256), 0, st, p);
-    hipLaunchKernelGGL(k_response, dim3(1), dim3(kRespW), 0, st, a, p, parity);
+    p.hraw = hraw.p, p.bcnt = bcnt.p, p.bpre = bpre.p;
+    hipLaunchKernelGGL(k_narrow, dim3(nblk), dim3(256), 0, st, a, p, parity);
+    hipLaunchKernelGGL(k_response, dim3(1), dim3(kRespW), 0, st, a, p, parity, nblk);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 

@@ -301,19 +301,16 @@ __device__ __forceinline__ void flush_chunk(const KArgs& a, NodeCounts cnt, uint
     }
 }
 
-__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t C, int remap) {
-    if (!remap) return b;
-    // the hardware deals workgroup b to XCD b % 8: give each XCD a contiguous range of chunks
-    const uint32_t per = C / 8, rem = C % 8, x = b % 8, idx = b / 8;
-    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
-}
+// The hardware deals workgroup b to XCD b % 8; the host's table gives each XCD an equal share of every large body
+// as one contiguous range of chunks (neighbours share an L2) and deals small bodies round the XCDs.
+__device__ __forceinline__ uint32_t xcd_chunk(const KArgs& a, uint32_t b) { return a.wg_chunk ? a.wg_chunk[b] : b; }
 
 template <int T, int PER>
 __global__ __launch_bounds__(T) void k_substep(KArgs a, int parity) {
     __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
     __shared__ uint8_t sok[T * PER];
     __shared__ uint4 sred[T / 64];
-    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t c = xcd_chunk(a, blockIdx.x);
     NodeCounts cnt = {0, 0, 0, 0.0f};
     chunk_substep<T, PER>(c, a, parity, sfx, sfy, sfz, sok, cnt);
     flush_chunk<T>(a, cnt, c, sred);
@@ -378,7 +375,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)
     __shared__ uint8_t sok[T * PER];
     __shared__ uint4 sred[T / 64];
     __shared__ int sflag;
-    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t c = xcd_chunk(a, blockIdx.x);
     const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
     const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
     float4 A[PER], M[PER];
@@ -474,7 +471,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)
     __shared__ uint4 sred[T / 64];
     __shared__ int sbad;
     const uint32_t tid = threadIdx.x;
-    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t c = xcd_chunk(a, blockIdx.x);
     const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1], nown = n1 - n0;
     const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
     const uint32_t h0 = a.halo_off[c], next = a.halo_off[c + 1] - h0;
@@ -824,34 +821,46 @@ template <int T, int PER>
 __global__ __launch_bounds__(T) void k_forces(KArgs a, int parity) {
     __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
     __shared__ uint8_t sok[T * PER];
-    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
+    const uint32_t c = xcd_chunk(a, blockIdx.x);
     NodeCounts cnt = {0, 0, 0, 0.0f};
     chunk_substep<T, PER, true>(c, a, parity, sfx, sfy, sfz, sok, cnt);
 }
 
-// The static contacts and the integration of a chunk's nodes from F.
-template <int T>
-__global__ __launch_bounds__(T) void k_integrate(KArgs a, int parity) {
-    __shared__ uint4 sred[T / 64];
-    const uint32_t c = xcd_chunk(blockIdx.x, a.nchunks, a.xcd_remap);
-    const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
+// The static contacts and the integration from F, one thread per node (a workgroup's nodes may belong to several
+// bodies): the counts into the workgroup's slot of `wstat` (folded at the frame's end), each body's top speed
+// straight into its frame word, so that it is current at every pair rebuild.
+__global__ __launch_bounds__(256) void k_integrate(KArgs a, int parity, uint4* wstat) {
+    __shared__ uint32_t sred[4 * 7];
+    __shared__ uint4 scnt[4];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < a.nnodes;
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    for (uint32_t i = n0 + threadIdx.x; i < n1; i += T) {
+    if (valid) {
         float4 P = a.pm[parity][i], V = a.vm[parity][i];
         const float4 F = a.F[i];
         node_finish(a, a.ninfo[i], a.naux[i], P, V, mk(F.x, F.y, F.z), cnt);
         a.pm[parity ^ 1][i] = P, a.vm[parity ^ 1][i] = V;
     }
-    flush_chunk<T>(a, cnt, c, sred);
+    uint32_t c0 = cnt.contacts, c1 = cnt.clamps, c2 = cnt.nans;
+    for (int off = 32; off > 0; off >>= 1) c0 += __shfl_xor(c0, off), c1 += __shfl_xor(c1, off), c2 += __shfl_xor(c2, off);
+    if ((threadIdx.x & 63) == 0) scnt[threadIdx.x >> 6] = make_uint4(c0, c1, c2, 0);
+    body_reduce<256>(a, i, valid, cnt.max_v2, 0, 0, 0, true, false, sred);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint4 t = wstat[blockIdx.x];
+        for (int w = 0; w < 4; w++) t.x += scnt[w].x, t.y += scnt[w].y, t.z += scnt[w].z;
+        wstat[blockIdx.x] = t;
+    }
 }
 
-// The chunks' top speeds so far onto their bodies (the running top speed the pair margins need mid-frame; the
-// counts stay for the frame's end).
-__global__ void k_fold_speed(KArgs a) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.nchunks) return;
-    const uint4 t = a.cstat[c];
-    if (u2f(t.w) > 0) atomicMax(&a.body_frame[a.chunk_body[c] * BF_N + BF_V2], t.w);
+__global__ void k_wstat_fold(KArgs a, uint4* wstat, uint32_t n) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const uint4 t = wstat[w];
+    if (t.x) atomicAdd(&a.counters[CNT_STATIC], (unsigned long long)t.x);
+    if (t.y) atomicAdd(&a.counters[CNT_CLAMPS], (unsigned long long)t.y);
+    if (t.z) atomicAdd(&a.counters[CNT_NANS], (unsigned long long)t.z);
+    wstat[w] = make_uint4(0, 0, 0, 0);
 }
 
 int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
@@ -867,16 +876,14 @@ int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t
     return BL_OK_OR_HIP();
 }
 
-int launch_integrate(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
-    if (t.block == 256) hipLaunchKernelGGL((k_integrate<256>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
-    else if (t.block == 512) hipLaunchKernelGGL((k_integrate<512>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
-    else if (t.block == 1024) hipLaunchKernelGGL((k_integrate<1024>), dim3(a.nchunks), dim3(1024), 0, st, a, parity);
-    else return BL_ERR_ARG;
+int launch_integrate(const KArgs& a, hipStream_t st, int parity, uint4* wstat) {
+    if (a.nnodes) hipLaunchKernelGGL(k_integrate, dim3((a.nnodes + 255) / 256), dim3(256), 0, st, a, parity, wstat);
     return BL_OK_OR_HIP();
 }
 
-int launch_fold_speed(const KArgs& a, hipStream_t st) {
-    if (a.nchunks) hipLaunchKernelGGL(k_fold_speed, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
+int launch_wstat_fold(const KArgs& a, hipStream_t st, uint4* wstat) {
+    const uint32_t n = (a.nnodes + 255) / 256;
+    if (n) hipLaunchKernelGGL(k_wstat_fold, dim3((n + 255) / 256), dim3(256), 0, st, a, wstat, n);
     return BL_OK_OR_HIP();
 }
 

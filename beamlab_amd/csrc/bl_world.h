@@ -80,7 +80,7 @@ struct KArgs {
     uint32_t nnodes;
     uint32_t nchunks;
     uint32_t nbodies;
-    int xcd_remap;
+    const uint32_t* wg_chunk; // workgroup -> chunk (XCD placement), or null for the identity
 };
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
@@ -99,8 +99,8 @@ int launch_chunk_clear(const KArgs& a, hipStream_t st);
 int launch_box(const KArgs& a, hipStream_t st, int parity);
 // the substep split for inter-body contacts: forces into F, then the static contacts and the integration from F
 int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t);
-int launch_integrate(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t);
-int launch_fold_speed(const KArgs& a, hipStream_t st);
+int launch_integrate(const KArgs& a, hipStream_t st, int parity, uint4* wstat); // wstat: ceil(nodes / 256) slots
+int launch_wstat_fold(const KArgs& a, hipStream_t st, uint4* wstat);
 int chunk_capacity(int block, int per);
 
 } // namespace blg

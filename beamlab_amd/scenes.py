@@ -200,6 +200,44 @@ def config2_body(seed=2, faces=False):
                         k_range=(1e5, 5e6), weak_frac=0.02, weak_eps=0.01, faces=faces, name="c2_lattice")
 
 
+def config2_scene(seed=2, n_balls=2000):
+    """The 64x32x24 lattice with its faces as triangles and self collision on (collision radius 0.04, as
+    build_soft_box sets), resting 5 cm above the ground, and `n_balls` rigid balls (radius uniform in [0.1, 0.3] m,
+    5 kg): one layer on the block's top, 1 m in from its edges, the rest on the ground around it, nearest first,
+    the first ring 5 cm from its sides (0.7 m grid, +-2 cm jitter). Every body stays below 8 m/s (the slow bodies'
+    pair schedule) over 4000 substeps."""
+    body = lattice_body(64, 32, 24, 0.25, 2.0, (0.0, 0.05, 0.0), k=None, d=500.0, eps=0.05, seed=seed, jitter=0.01,
+                        k_range=(1e5, 5e6), weak_frac=0.02, weak_eps=0.01, faces=True, name="c2_lattice")
+    body.opts.self_collision = 1
+    body.opts.collision_radius = np.float32(0.04)
+    rng = TRng(seed + 1000)
+    hx, hz, top = 7.875, 2.875, 0.05 + 31 * 0.25
+    spots = [(-6.65 + 0.7 * ix, top + 0.4, -1.75 + 0.7 * iz) for iz in range(6) for ix in range(20)]
+    xs = sorted({round(s * (hx + 0.35 + 0.7 * i), 4) for s in (-1, 1) for i in range(40)} |
+                {round(-7.7 + 0.7 * j, 4) for j in range(23)})
+    zs = sorted({round(s * (hz + 0.35 + 0.7 * i), 4) for s in (-1, 1) for i in range(40)} |
+                {round(-2.8 + 0.7 * j, 4) for j in range(9)})
+    ring = []
+    for x in xs:
+        for z in zs:
+            if abs(x) < hx + 0.3 and abs(z) < hz + 0.3:
+                continue
+            d = max(abs(x) - hx, abs(z) - hz, 0.0)
+            ring.append((round(d, 4), abs(z), abs(x), x, z))
+    ring.sort()
+    balls = []
+    for k in range(n_balls):
+        r = rng.range(0.1, 0.3)
+        jx, jz = rng.range(-0.02, 0.02), rng.range(-0.02, 0.02)
+        if k < len(spots):
+            x, y, z = spots[k]
+        else:
+            x, z = ring[k - len(spots)][3:]
+            y = float(r) + 0.02
+        balls.append(ball_body((np.float32(x) + jx, np.float32(y), np.float32(z) + jz), r, 5.0, name=f"ball{k}"))
+    return [body] + balls
+
+
 def make_world(bodies, ground=True, settings=None):
     w = World(settings)
     if ground:

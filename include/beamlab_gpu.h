@@ -284,7 +284,8 @@ int bl_get_tuning(bl_ctx* ctx, bl_tuning* t);
 int bl_time_step(bl_ctx* ctx, int32_t substeps, float* ms);
 
 /* Times one frame of `substeps` with an event pair around every kernel launch of the substeps (no graph capture):
- * *kernel_ms = summed duration of the substep kernels, *total_ms = the frame on the stream, *launches = launches. */
+ * *kernel_ms = summed duration of the substep kernels, *total_ms = the frame on the stream, *launches = launches.
+ * A frame with inter-body contacts times the force-phase kernel (beams and gather) of each substep. */
 int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kernel_ms, int32_t* launches);
 
 #ifdef __cplusplus
