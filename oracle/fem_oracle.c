@@ -1,0 +1,729 @@
+/* Test infrastructure only: the checker of the HIP shell-FEM path. Only tests/, the smoke check and bench.py's
+ * CPU-baseline leg load it; the product never does.
+ *
+ * A restatement in C of the reference's implicit shell step on a plate (zenkovich/beamlab src/phys/frame_fem.cpp,
+ * fem_shell.cpp, core/math.h), elastic sections (no yield, tears or refinement), no members, mounts, contacts or
+ * held steps:
+ *   ShellMesher::grid / tri                 fem_shell.cpp:17-46   (the plate, its masses)
+ *   FemFrame::finalize (triangles)          frame_fem.cpp:259-276 (rest shape, rest frames)
+ *   FemFrame::node_inertia                  frame_fem.cpp:3075-3086
+ *   tri_grad, dkt_b, tri_local_k            frame_fem.cpp:1482-1579 (the element stiffness, FP64)
+ *   tri_build_k                             frame_fem.cpp:1625-1647 (stored as floats)
+ *   tri_kinematics, eval_tris (elastic)     frame_fem.cpp:1584-1622, 1661-1831
+ *   gather_part (triangles, no contacts)    frame_fem.cpp:916-1086 (6x6 blocks, FP64)
+ *   chol6, right_solve_lt, sub_abt, sub_mv, sub_mtv, forward6, backward6   frame_fem.cpp:2193-2317
+ *   the factorization and substitution loops   frame_fem.cpp:2899-2943 (here over the nodes' natural order, dense)
+ *   the update of v, w, positions, rotations   frame_fem.cpp:3003-3027, 3407-3416
+ * The element stiffness is pinned bit for bit against the compiled reference's FemFrame::tri_stiffness; the rest by
+ * closed forms (rigid-body modes, plate theory) and by residuals.
+ */
+#include "fem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+
+
+/* ------------------------------------------------------------------ float vectors and quaternions (core/math.h) */
+typedef struct { float x, y, z; } f3;
+typedef struct { float x, y, z, w; } qf;
+static inline f3 F3(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static inline f3 add3(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 sub3(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 mul3(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
+static inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross3(f3 a, f3 b) { return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline float len3(f3 a) { return sqrtf(dot3(a, a)); }
+static inline f3 div3(f3 a, float s) { float i = 1.0f / s; return F3(a.x * i, a.y * i, a.z * i); }
+static inline f3 norm3(f3 a) { float l = len3(a); return l > 1e-12f ? mul3(a, 1.0f / l) : F3(0, 0, 0); }
+static inline qf Q(float x, float y, float z, float w) { qf q = {x, y, z, w}; return q; }
+static inline qf qmul(qf a, qf b) {
+    return Q(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+static inline qf qconj(qf q) { return Q(-q.x, -q.y, -q.z, q.w); }
+static inline qf qscale(qf q, float s) { return Q(q.x * s, q.y * s, q.z * s, q.w * s); }
+static inline qf qnorm(qf q) {
+    float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    return l > 1e-12f ? qscale(q, 1.0f / l) : Q(0, 0, 0, 1);
+}
+/* from_mat3 with columns c0 c1 c2 (core/math.h:171-202) */
+static qf from_cols(f3 c0, f3 c1, f3 c2) {
+    float m00 = c0.x, m11 = c1.y, m22 = c2.z;
+    float tr = m00 + m11 + m22;
+    qf q;
+    if (tr > 0) {
+        float s = sqrtf(tr + 1.0f) * 2;
+        q.w = 0.25f * s;
+        q.x = (c1.z - c2.y) / s;
+        q.y = (c2.x - c0.z) / s;
+        q.z = (c0.y - c1.x) / s;
+    } else if (m00 > m11 && m00 > m22) {
+        float s = sqrtf(1.0f + m00 - m11 - m22) * 2;
+        q.w = (c1.z - c2.y) / s;
+        q.x = 0.25f * s;
+        q.y = (c1.x + c0.y) / s;
+        q.z = (c2.x + c0.z) / s;
+    } else if (m11 > m22) {
+        float s = sqrtf(1.0f + m11 - m00 - m22) * 2;
+        q.w = (c2.x - c0.z) / s;
+        q.x = (c1.x + c0.y) / s;
+        q.y = 0.25f * s;
+        q.z = (c2.y + c1.z) / s;
+    } else {
+        float s = sqrtf(1.0f + m22 - m00 - m11) * 2;
+        q.w = (c0.y - c1.x) / s;
+        q.x = (c2.x + c0.z) / s;
+        q.y = (c2.y + c1.z) / s;
+        q.z = 0.25f * s;
+    }
+    return qnorm(q);
+}
+/* quat_log / quat_exp (core/math.h:214-226) */
+static f3 qlog(qf qin) {
+    qf q = qin.w < 0 ? qscale(qin, -1.0f) : qin;
+    f3 v = F3(q.x, q.y, q.z);
+    float s = len3(v);
+    if (s < 1e-8f) return mul3(v, 2.0f);
+    float angle = 2.0f * atan2f(s, q.w);
+    return mul3(v, angle / s);
+}
+static qf qexp(f3 rv) {
+    float a = len3(rv);
+    if (a < 1e-8f) return qnorm(Q(rv.x * 0.5f, rv.y * 0.5f, rv.z * 0.5f, 1.0f));
+    f3 axis = norm3(div3(rv, a));
+    float hh = a * 0.5f, s = sinf(hh);
+    return Q(axis.x * s, axis.y * s, axis.z * s, cosf(hh));
+}
+
+/* ------------------------------------------------------------------ the element (frame_fem.cpp:1482-1579) */
+static void dkt_b(const double x[3], const double y[3], double xi, double eta, double B[3][9]) {
+    const double x23 = x[1] - x[2], x31 = x[2] - x[0], x12 = x[0] - x[1];
+    const double y23 = y[1] - y[2], y31 = y[2] - y[0], y12 = y[0] - y[1];
+    double P[7], q[7], r[7], t[7];
+    const double xs[3] = {x23, x31, x12}, ys[3] = {y23, y31, y12};
+    for (int k = 4; k <= 6; k++) {
+        const double xij = xs[k - 4], yij = ys[k - 4], l2 = xij * xij + yij * yij;
+        P[k] = -6 * xij / l2, q[k] = 3 * xij * yij / l2, t[k] = -6 * yij / l2, r[k] = 3 * yij * yij / l2;
+    }
+    const double a = 1 - 2 * xi, bb = 1 - 2 * eta;
+    const double Hx_xi[9] = {P[6] * a + (P[5] - P[6]) * eta, q[6] * a - (q[5] + q[6]) * eta,
+                             -4 + 6 * (xi + eta) + r[6] * a - eta * (r[5] + r[6]),
+                             -P[6] * a + eta * (P[4] + P[6]), q[6] * a - eta * (q[6] - q[4]),
+                             -2 + 6 * xi + r[6] * a + eta * (r[4] - r[6]),
+                             -eta * (P[5] + P[4]), eta * (q[4] - q[5]), -eta * (r[5] - r[4])};
+    const double Hy_xi[9] = {t[6] * a + eta * (t[5] - t[6]), 1 + r[6] * a - eta * (r[5] + r[6]),
+                             -q[6] * a + eta * (q[5] + q[6]),
+                             -t[6] * a + eta * (t[4] + t[6]), -1 + r[6] * a + eta * (r[4] - r[6]),
+                             -q[6] * a - eta * (q[4] - q[6]),
+                             -eta * (t[4] + t[5]), eta * (r[4] - r[5]), -eta * (q[4] - q[5])};
+    const double Hx_eta[9] = {-P[5] * bb - xi * (P[6] - P[5]), q[5] * bb - xi * (q[5] + q[6]),
+                              -4 + 6 * (xi + eta) + r[5] * bb - xi * (r[5] + r[6]),
+                              xi * (P[4] + P[6]), xi * (q[4] - q[6]), -xi * (r[6] - r[4]),
+                              P[5] * bb - xi * (P[4] + P[5]), q[5] * bb + xi * (q[4] - q[5]),
+                              -2 + 6 * eta + r[5] * bb + xi * (r[4] - r[5])};
+    const double Hy_eta[9] = {-t[5] * bb - xi * (t[6] - t[5]), 1 + r[5] * bb - xi * (r[5] + r[6]),
+                              -q[5] * bb + xi * (q[5] + q[6]),
+                              xi * (t[4] + t[6]), xi * (r[4] - r[6]), -xi * (q[4] - q[6]),
+                              t[5] * bb - xi * (t[4] + t[5]), -1 + r[5] * bb + xi * (r[4] - r[5]),
+                              -q[5] * bb - xi * (q[4] - q[5])};
+    const double A2 = x31 * y12 - x12 * y31, iA = 1.0 / A2;
+    for (int j = 0; j < 9; j++) {
+        B[0][j] = (y31 * Hx_xi[j] + y12 * Hx_eta[j]) * iA;
+        B[1][j] = (-x31 * Hy_xi[j] - x12 * Hy_eta[j]) * iA;
+        B[2][j] = (-x31 * Hx_xi[j] - x12 * Hx_eta[j] + y31 * Hy_xi[j] + y12 * Hy_eta[j]) * iA;
+    }
+}
+static const double kGauss[3][2] = {{1.0 / 6, 1.0 / 6}, {2.0 / 3, 1.0 / 6}, {1.0 / 6, 2.0 / 3}};
+static const int kMemDof[3] = {0, 1, 5}, kPlateDof[3] = {2, 3, 4};
+
+static void tri_grad(const float X[3][2], double b[3], double c[3], double* A2) {
+    for (int i = 0; i < 3; i++) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        b[i] = (double)X[j][1] - X[k][1];
+        c[i] = (double)X[k][0] - X[j][0];
+    }
+    *A2 = ((double)X[1][0] - X[0][0]) * ((double)X[2][1] - X[0][1]) - ((double)X[2][0] - X[0][0]) * ((double)X[1][1] - X[0][1]);
+}
+
+void orc_fem_tri_local_k(const float Xf[6], float E, float nuf, float tf, float drill, double* Kout) {
+    double K[18][18];
+    const float X[3][2] = {{Xf[0], Xf[1]}, {Xf[2], Xf[3]}, {Xf[4], Xf[5]}};
+    for (int i = 0; i < 18; i++)
+        for (int j = 0; j < 18; j++) K[i][j] = 0;
+    double b[3], c[3], A2;
+    tri_grad(X, b, c, &A2);
+    if (!(A2 > 0)) { memcpy(Kout, K, sizeof(K)); return; }
+    const double A = 0.5 * A2, nu = nuf;
+    const double Dm = (double)E * tf / (1 - nu * nu);
+    double Bm[3][6];
+    for (int i = 0; i < 3; i++) {
+        Bm[0][2 * i] = b[i] / A2, Bm[1][2 * i] = 0, Bm[2][2 * i] = c[i] / A2;
+        Bm[0][2 * i + 1] = 0, Bm[1][2 * i + 1] = c[i] / A2, Bm[2][2 * i + 1] = b[i] / A2;
+    }
+    const double D3[3][3] = {{1, nu, 0}, {nu, 1, 0}, {0, 0, 0.5 * (1 - nu)}};
+    for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++) {
+            double v = 0;
+            for (int r = 0; r < 3; r++)
+                for (int t = 0; t < 3; t++) v += Bm[r][p] * D3[r][t] * Bm[t][q];
+            K[6 * (p / 2) + p % 2][6 * (q / 2) + q % 2] += A * Dm * v;
+        }
+    const double x[3] = {X[0][0], X[1][0], X[2][0]}, y[3] = {X[0][1], X[1][1], X[2][1]};
+    const float Dbf = E * tf * tf * tf / (12.0f * (1.0f - nuf * nuf)); /* ShellSection::D() in float */
+    const double Db = Dbf;
+    for (int gq = 0; gq < 3; gq++) {
+        double B[3][9];
+        dkt_b(x, y, kGauss[gq][0], kGauss[gq][1], B);
+        for (int p = 0; p < 9; p++)
+            for (int q = 0; q < 9; q++) {
+                double v = 0;
+                for (int r = 0; r < 3; r++)
+                    for (int t = 0; t < 3; t++) v += B[r][p] * D3[r][t] * B[t][q];
+                K[6 * (p / 3) + 2 + p % 3][6 * (q / 3) + 2 + q % 3] += A2 / 6.0 * Db * v;
+            }
+    }
+    const double G = (double)E / (2 * (1 + nu)), kd = (double)drill * G * tf * A;
+    double gu[3], gv[3];
+    for (int i = 0; i < 3; i++) gu[i] = -c[i] / (2 * A2), gv[i] = b[i] / (2 * A2);
+    for (int i = 0; i < 3; i++) {
+        K[6 * i + 5][6 * i + 5] += kd;
+        for (int j = 0; j < 3; j++) {
+            K[6 * i + 5][6 * j] -= kd * gu[j], K[6 * j][6 * i + 5] -= kd * gu[j];
+            K[6 * i + 5][6 * j + 1] -= kd * gv[j], K[6 * j + 1][6 * i + 5] -= kd * gv[j];
+            K[6 * i][6 * j] += 3 * kd * gu[i] * gu[j], K[6 * i][6 * j + 1] += 3 * kd * gu[i] * gv[j];
+            K[6 * i + 1][6 * j] += 3 * kd * gv[i] * gu[j], K[6 * i + 1][6 * j + 1] += 3 * kd * gv[i] * gv[j];
+        }
+    }
+    memcpy(Kout, K, sizeof(K));
+}
+
+/* ------------------------------------------------------------------ the plate */
+struct orc_fem {
+    uint32_t nn, nt;
+    double* xd;
+    float *v, *w, *q, *mass, *inertia, *pos0;
+    uint8_t* fixed;
+    uint32_t* tn;
+    float *X, *r0, *Kf, *tmass, *area0;
+    float E, nu, rho, t, drill, damping;
+};
+
+int orc_fem_rest_shape(const float p[9], float X[6]) {
+    const f3 x0 = F3(p[0], p[1], p[2]), x1 = F3(p[3], p[4], p[5]), x2 = F3(p[6], p[7], p[8]);
+    const f3 nn = cross3(sub3(x1, x0), sub3(x2, x0));
+    const float a2 = len3(nn);
+    if (!(a2 > 1e-8f)) return -1;
+    const f3 e3 = div3(nn, a2), e1 = norm3(sub3(x1, x0)), e2 = cross3(e3, e1);
+    const f3 c = div3(add3(add3(x0, x1), x2), 3.0f);
+    const f3 xs[3] = {x0, x1, x2};
+    for (int i = 0; i < 3; i++) X[2 * i] = dot3(sub3(xs[i], c), e1), X[2 * i + 1] = dot3(sub3(xs[i], c), e2);
+    return 0;
+}
+
+orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6]) {
+    orc_fem* f = (orc_fem*)calloc(1, sizeof(orc_fem));
+    f->E = sec[0], f->nu = sec[1], f->rho = sec[2], f->t = sec[3], f->drill = sec[4], f->damping = sec[5];
+    const uint32_t nn = (uint32_t)((nu + 1) * (nv + 1)), nt = (uint32_t)(2 * nu * nv);
+    f->nn = nn, f->nt = nt;
+    f->xd = (double*)calloc(3 * nn, sizeof(double));
+    f->pos0 = (float*)calloc(3 * nn, sizeof(float));
+    f->v = (float*)calloc(3 * nn, sizeof(float));
+    f->w = (float*)calloc(3 * nn, sizeof(float));
+    f->q = (float*)calloc(4 * nn, sizeof(float));
+    f->mass = (float*)calloc(nn, sizeof(float));
+    f->inertia = (float*)calloc(nn, sizeof(float));
+    f->fixed = (uint8_t*)calloc(nn, 1);
+    f->tn = (uint32_t*)calloc(3 * nt, sizeof(uint32_t));
+    f->X = (float*)calloc(6 * nt, sizeof(float));
+    f->r0 = (float*)calloc(12 * nt, sizeof(float));
+    f->Kf = (float*)calloc(162 * (size_t)nt, sizeof(float));
+    f->tmass = (float*)calloc(nt, sizeof(float));
+    f->area0 = (float*)calloc(nt, sizeof(float));
+    /* ShellMesher::grid: origin + du i + dv j (float vector arithmetic) */
+    const f3 origin = F3(-0.5f * size, y0, -0.5f * size), du = F3(size / (float)nu, 0, 0), dv = F3(0, 0, size / (float)nv);
+    for (int j = 0; j <= nv; j++)
+        for (int i = 0; i <= nu; i++) {
+            const uint32_t id = (uint32_t)(j * (nu + 1) + i);
+            const f3 p = add3(add3(origin, mul3(du, (float)i)), mul3(dv, (float)j));
+            f->pos0[3 * id] = p.x, f->pos0[3 * id + 1] = p.y, f->pos0[3 * id + 2] = p.z;
+            f->xd[3 * id] = p.x, f->xd[3 * id + 1] = p.y, f->xd[3 * id + 2] = p.z;
+            f->q[4 * id + 3] = 1.0f;
+            f->fixed[id] = (i == 0 || j == 0 || i == nu || j == nv);
+        }
+    uint32_t t = 0;
+    const float mpa = f->rho * f->t; /* ShellSection::mass_per_m2 */
+    for (int j = 0; j < nv; j++)
+        for (int i = 0; i < nu; i++) {
+            const uint32_t n00 = (uint32_t)(j * (nu + 1) + i), n10 = n00 + 1, n01 = n00 + (uint32_t)(nu + 1), n11 = n01 + 1;
+            uint32_t tri2[2][3];
+            if ((i + j) & 1) {
+                uint32_t a[2][3] = {{n00, n10, n11}, {n00, n11, n01}};
+                memcpy(tri2, a, sizeof(a));
+            } else {
+                uint32_t a[2][3] = {{n00, n10, n01}, {n10, n11, n01}};
+                memcpy(tri2, a, sizeof(a));
+            }
+            for (int k = 0; k < 2; k++, t++) {
+                memcpy(&f->tn[3 * t], tri2[k], sizeof(tri2[k]));
+                const float* pa = &f->pos0[3 * tri2[k][0]];
+                const float* pb = &f->pos0[3 * tri2[k][1]];
+                const float* pc = &f->pos0[3 * tri2[k][2]];
+                const f3 A = F3(pa[0], pa[1], pa[2]), B = F3(pb[0], pb[1], pb[2]), C = F3(pc[0], pc[1], pc[2]);
+                const float area = 0.5f * len3(cross3(sub3(B, A), sub3(C, A)));
+                const float m = mpa * area;
+                f->tmass[t] = m;
+                for (int c = 0; c < 3; c++) f->mass[tri2[k][c]] += m / 3.0f;
+            }
+        }
+    /* finalize: rest shapes, rest frames, stored stiffness (tri_build_k) */
+    for (uint32_t k = 0; k < nt; k++) {
+        float p[9];
+        for (int c = 0; c < 3; c++) memcpy(&p[3 * c], &f->pos0[3 * f->tn[3 * k + c]], 3 * sizeof(float));
+        orc_fem_rest_shape(p, &f->X[6 * k]);
+        const f3 x0 = F3(p[0], p[1], p[2]), x1 = F3(p[3], p[4], p[5]), x2 = F3(p[6], p[7], p[8]);
+        const f3 nn3 = cross3(sub3(x1, x0), sub3(x2, x0));
+        const float a2 = len3(nn3);
+        const f3 e3 = div3(nn3, a2), e1 = norm3(sub3(x1, x0)), e2 = cross3(e3, e1);
+        f->area0[k] = 0.5f * a2;
+        const qf E0 = from_cols(e1, e2, e3);
+        for (int c = 0; c < 3; c++) {
+            const float* qn = &f->q[4 * f->tn[3 * k + c]];
+            const qf r = qnorm(qmul(qconj(Q(qn[0], qn[1], qn[2], qn[3])), E0));
+            f->r0[12 * k + 4 * c] = r.x, f->r0[12 * k + 4 * c + 1] = r.y, f->r0[12 * k + 4 * c + 2] = r.z, f->r0[12 * k + 4 * c + 3] = r.w;
+        }
+        double K[18][18];
+        orc_fem_tri_local_k(&f->X[6 * k], f->E, f->nu, f->t, f->drill, &K[0][0]);
+        float* P = &f->Kf[162 * (size_t)k];
+        for (int a = 0; a < 9; a++)
+            for (int c = 0; c < 9; c++) {
+                const int im = 6 * (a / 3) + kMemDof[a % 3], jm = 6 * (c / 3) + kMemDof[c % 3];
+                const int ip = 6 * (a / 3) + kPlateDof[a % 3], jp = 6 * (c / 3) + kPlateDof[c % 3];
+                P[a * 9 + c] = (float)K[im < jm ? im : jm][im < jm ? jm : im];
+                P[81 + a * 9 + c] = (float)K[ip < jp ? ip : jp][ip < jp ? jp : ip];
+            }
+    }
+    /* node_inertia: a corner's share of its triangles, with the node's floor */
+    for (uint32_t i = 0; i < nn; i++) f->inertia[i] = 0;
+    for (uint32_t k = 0; k < nt; k++)
+        for (int c = 0; c < 3; c++) f->inertia[f->tn[3 * k + c]] += f->tmass[k] * f->area0[k] / 108.0f;
+    for (uint32_t i = 0; i < nn; i++) {
+        const float fl = 1e-6f > f->mass[i] * 1e-4f ? 1e-6f : f->mass[i] * 1e-4f;
+        f->inertia[i] = f->inertia[i] > fl ? f->inertia[i] : fl;
+    }
+    return f;
+}
+
+void orc_fem_destroy(orc_fem* f) {
+    if (!f) return;
+    free(f->xd), free(f->pos0), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
+    free(f->tn), free(f->X), free(f->r0), free(f->Kf), free(f->tmass), free(f->area0);
+    free(f);
+}
+uint32_t orc_fem_nodes(const orc_fem* f) { return f->nn; }
+uint32_t orc_fem_tris(const orc_fem* f) { return f->nt; }
+void orc_fem_topology(const orc_fem* f, uint32_t* tri_nodes, double* xd, uint8_t* fixed, float* mass, float* inertia) {
+    if (tri_nodes) memcpy(tri_nodes, f->tn, 3 * sizeof(uint32_t) * f->nt);
+    if (xd) memcpy(xd, f->xd, 3 * sizeof(double) * f->nn);
+    if (fixed) memcpy(fixed, f->fixed, f->nn);
+    if (mass) memcpy(mass, f->mass, sizeof(float) * f->nn);
+    if (inertia) memcpy(inertia, f->inertia, sizeof(float) * f->nn);
+}
+void orc_fem_get_state(const orc_fem* f, double* xd, float* v, float* w, float* q) {
+    if (xd) memcpy(xd, f->xd, 3 * sizeof(double) * f->nn);
+    if (v) memcpy(v, f->v, 3 * sizeof(float) * f->nn);
+    if (w) memcpy(w, f->w, 3 * sizeof(float) * f->nn);
+    if (q) memcpy(q, f->q, 4 * sizeof(float) * f->nn);
+}
+void orc_fem_set_state(orc_fem* f, const double* xd, const float* v, const float* w, const float* q) {
+    if (xd) memcpy(f->xd, xd, 3 * sizeof(double) * f->nn);
+    if (v) memcpy(f->v, v, 3 * sizeof(float) * f->nn);
+    if (w) memcpy(f->w, w, 3 * sizeof(float) * f->nn);
+    if (q) memcpy(f->q, q, 4 * sizeof(float) * f->nn);
+}
+
+/* ------------------------------------------------------------------ the element pass (eval_tris, elastic) */
+typedef struct {
+    int on;
+    f3 e1, e2, e3;
+    f3 fo[3], to[3]; /* corner forces and moments (world) */
+    float g[3][3];   /* membrane tension's geometric stiffness between corners */
+    double kv[18];   /* K v in world axes */
+} tri_out;
+
+static int tri_kin(const orc_fem* f, uint32_t k, double d[18], f3* E1, f3* E2, f3* E3) {
+    const uint32_t* n = &f->tn[3 * k];
+    const float* X = &f->X[6 * k];
+    double x[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) x[i][j] = f->xd[n[i] * 3 + j];
+    const double a[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+    const double c[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+    double nv[3] = {a[1] * c[2] - a[2] * c[1], a[2] * c[0] - a[0] * c[2], a[0] * c[1] - a[1] * c[0]};
+    const double nl = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]), al = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (!(nl > 1e-12) || !(al > 1e-9) || !isfinite(nl)) return 0;
+    for (int j = 0; j < 3; j++) nv[j] /= nl;
+    const double t1[3] = {a[0] / al, a[1] / al, a[2] / al};
+    const double t2[3] = {nv[1] * t1[2] - nv[2] * t1[1], nv[2] * t1[0] - nv[0] * t1[2], nv[0] * t1[1] - nv[1] * t1[0]};
+    double cen[3];
+    for (int j = 0; j < 3; j++) cen[j] = (x[0][j] + x[1][j] + x[2][j]) / 3.0;
+    double p[3][2];
+    for (int i = 0; i < 3; i++) {
+        const double r[3] = {x[i][0] - cen[0], x[i][1] - cen[1], x[i][2] - cen[2]};
+        p[i][0] = r[0] * t1[0] + r[1] * t1[1] + r[2] * t1[2];
+        p[i][1] = r[0] * t2[0] + r[1] * t2[1] + r[2] * t2[2];
+    }
+    double sn = 0, cs = 0;
+    for (int i = 0; i < 3; i++) sn += X[2 * i] * p[i][1] - X[2 * i + 1] * p[i][0], cs += X[2 * i] * p[i][0] + X[2 * i + 1] * p[i][1];
+    const double phi = atan2(sn, cs), cp = cos(phi), sp = sin(phi);
+    double e1[3], e2[3];
+    for (int j = 0; j < 3; j++) e1[j] = cp * t1[j] + sp * t2[j], e2[j] = -sp * t1[j] + cp * t2[j];
+    *E1 = F3((float)e1[0], (float)e1[1], (float)e1[2]);
+    *E2 = F3((float)e2[0], (float)e2[1], (float)e2[2]);
+    *E3 = F3((float)nv[0], (float)nv[1], (float)nv[2]);
+    const qf Ec = qconj(from_cols(*E1, *E2, *E3));
+    for (int i = 0; i < 3; i++) {
+        d[6 * i] = cp * p[i][0] + sp * p[i][1] - X[2 * i];
+        d[6 * i + 1] = -sp * p[i][0] + cp * p[i][1] - X[2 * i + 1];
+        d[6 * i + 2] = 0;
+        const float* qn = &f->q[4 * n[i]];
+        const float* r0 = &f->r0[12 * k + 4 * i];
+        const f3 th = qlog(qmul(Ec, qnorm(qmul(Q(qn[0], qn[1], qn[2], qn[3]), Q(r0[0], r0[1], r0[2], r0[3])))));
+        d[6 * i + 3] = th.x, d[6 * i + 4] = th.y, d[6 * i + 5] = th.z;
+    }
+    return 1;
+}
+
+static void eval_tri(const orc_fem* f, uint32_t k, tri_out* o) {
+    o->on = 0;
+    double d[18];
+    f3 e1, e2, e3;
+    if (!tri_kin(f, k, d, &e1, &e2, &e3)) return;
+    const float Xa[3][2] = {{f->X[6 * k], f->X[6 * k + 1]}, {f->X[6 * k + 2], f->X[6 * k + 3]}, {f->X[6 * k + 4], f->X[6 * k + 5]}};
+    double b3[3], c3[3], A2;
+    tri_grad(Xa, b3, c3, &A2);
+    if (!(A2 > 0)) return;
+    const double nu = f->nu, Dm = (double)f->E / (1 - nu * nu);
+    double eps[3] = {0, 0, 0}, sig[3];
+    for (int i = 0; i < 3; i++) {
+        eps[0] += b3[i] * d[6 * i] / A2;
+        eps[1] += c3[i] * d[6 * i + 1] / A2;
+        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
+    }
+    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+    const float* Km = &f->Kf[162 * (size_t)k];
+    const float* Kp = Km + 81;
+    double vl[18];
+    {
+        const double E[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
+        for (int i = 0; i < 3; i++) {
+            const uint32_t nd = f->tn[3 * k + i];
+            const f3 v = F3(f->v[3 * nd], f->v[3 * nd + 1], f->v[3 * nd + 2]), om = F3(f->w[3 * nd], f->w[3 * nd + 1], f->w[3 * nd + 2]);
+            for (int a2 = 0; a2 < 3; a2++) {
+                vl[6 * i + a2] = E[0][a2] * v.x + E[1][a2] * v.y + E[2][a2] * v.z;
+                vl[6 * i + 3 + a2] = E[0][a2] * om.x + E[1][a2] * om.y + E[2][a2] * om.z;
+            }
+        }
+    }
+    double fo[18], fl[18];
+    {
+        double dm[9], dp[9], vm[9], vp[9];
+        for (int a = 0; a < 9; a++) {
+            const int ci = 6 * (a / 3);
+            dm[a] = d[ci + kMemDof[a % 3]], dp[a] = d[ci + kPlateDof[a % 3]];
+            vm[a] = vl[ci + kMemDof[a % 3]], vp[a] = vl[ci + kPlateDof[a % 3]];
+        }
+        double fm[9] = {0}, fp[9] = {0}, um[9] = {0}, up[9] = {0};
+        for (int c = 0; c < 9; c++)
+            for (int a = 0; a < 9; a++) {
+                const double km = (double)Km[c * 9 + a], kp = (double)Kp[c * 9 + a];
+                fm[a] += km * dm[c], um[a] += km * vm[c];
+                fp[a] += kp * dp[c], up[a] += kp * vp[c];
+            }
+        for (int a = 0; a < 9; a++) {
+            const int ci = 6 * (a / 3);
+            fo[ci + kMemDof[a % 3]] = fm[a], fl[ci + kMemDof[a % 3]] = um[a];
+            fo[ci + kPlateDof[a % 3]] = fp[a], fl[ci + kPlateDof[a % 3]] = up[a];
+        }
+    }
+    o->on = 1;
+    o->e1 = e1, o->e2 = e2, o->e3 = e3;
+    for (int i = 0; i < 3; i++) {
+        const f3 fw = add3(add3(mul3(e1, (float)fo[6 * i]), mul3(e2, (float)fo[6 * i + 1])), mul3(e3, (float)fo[6 * i + 2]));
+        const f3 tw = add3(add3(mul3(e1, (float)fo[6 * i + 3]), mul3(e2, (float)fo[6 * i + 4])), mul3(e3, (float)fo[6 * i + 5]));
+        o->fo[i] = F3(-fw.x, -fw.y, -fw.z);
+        o->to[i] = F3(-tw.x, -tw.y, -tw.z);
+    }
+    {
+        const double sx = sig[0] * f->t, sy = sig[1] * f->t, sxy = sig[2] * f->t;
+        const double tr = 0.5 * (sx + sy), df = sqrt(0.25 * (sx - sy) * (sx - sy) + sxy * sxy);
+        const double l1 = fmax(0.0, tr + df), l2 = fmax(0.0, tr - df);
+        double vx = sxy, vy = l1 - sx;
+        if (fabs(vx) + fabs(vy) < 1e-12 * (fabs(sx) + fabs(sy) + 1)) vx = sx >= sy ? 1 : 0, vy = sx >= sy ? 0 : 1;
+        const double vlen = sqrt(vx * vx + vy * vy);
+        vx /= vlen, vy /= vlen;
+        const double S[2][2] = {{l1 * vx * vx + l2 * vy * vy, (l1 - l2) * vx * vy}, {(l1 - l2) * vx * vy, l1 * vy * vy + l2 * vx * vx}};
+        const double A = 0.5 * A2;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double gi[2] = {b3[i] / A2, c3[i] / A2}, gj[2] = {b3[j] / A2, c3[j] / A2};
+                o->g[i][j] = (float)(A * (gi[0] * (S[0][0] * gj[0] + S[0][1] * gj[1]) + gi[1] * (S[1][0] * gj[0] + S[1][1] * gj[1])));
+            }
+    }
+    {
+        const double E[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
+        for (int i = 0; i < 3; i++)
+            for (int p = 0; p < 3; p++) {
+                o->kv[6 * i + p] = E[p][0] * fl[6 * i] + E[p][1] * fl[6 * i + 1] + E[p][2] * fl[6 * i + 2];
+                o->kv[6 * i + 3 + p] = E[p][0] * fl[6 * i + 3] + E[p][1] * fl[6 * i + 4] + E[p][2] * fl[6 * i + 5];
+            }
+    }
+}
+
+/* gather_part's tri_block (frame_fem.cpp:948-985): corners (i, j) in world axes, times c, tension on the diagonal */
+static void tri_block(const orc_fem* f, uint32_t k, const tri_out* o, int i, int j, double c, double h2, double* D) {
+    const double E[3][3] = {{o->e1.x, o->e2.x, o->e3.x}, {o->e1.y, o->e2.y, o->e3.y}, {o->e1.z, o->e2.z, o->e3.z}};
+    const float* Km = &f->Kf[162 * (size_t)k];
+    const float* Kp = Km + 81;
+#define M_(s2, t2) ((double)Km[(3 * i + (s2)) * 9 + 3 * j + (t2)])
+#define P_(s2, t2) ((double)Kp[(3 * i + (s2)) * 9 + 3 * j + (t2)])
+    for (int ra = 0; ra < 2; ra++)
+        for (int cb = 0; cb < 2; cb++) {
+            double T[3][3];
+            for (int p = 0; p < 3; p++) {
+                const double E0 = E[p][0], E1 = E[p][1], E2 = E[p][2];
+                if (ra == 0 && cb == 0) {
+                    T[p][0] = E0 * M_(0, 0) + E1 * M_(1, 0);
+                    T[p][1] = E0 * M_(0, 1) + E1 * M_(1, 1);
+                    T[p][2] = E2 * P_(0, 0);
+                } else if (ra == 0) {
+                    T[p][0] = E2 * P_(0, 1);
+                    T[p][1] = E2 * P_(0, 2);
+                    T[p][2] = E0 * M_(0, 2) + E1 * M_(1, 2);
+                } else if (cb == 0) {
+                    T[p][0] = E2 * M_(2, 0);
+                    T[p][1] = E2 * M_(2, 1);
+                    T[p][2] = E0 * P_(1, 0) + E1 * P_(2, 0);
+                } else {
+                    T[p][0] = E0 * P_(1, 1) + E1 * P_(2, 1);
+                    T[p][1] = E0 * P_(1, 2) + E1 * P_(2, 2);
+                    T[p][2] = E2 * M_(2, 2);
+                }
+            }
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) D[(3 * ra + p) * 6 + 3 * cb + q] += c * (T[p][0] * E[q][0] + T[p][1] * E[q][1] + T[p][2] * E[q][2]);
+        }
+#undef M_
+#undef P_
+    const double gg = h2 * o->g[i][j];
+    if (gg != 0)
+        for (int p = 0; p < 3; p++) D[p * 7] += gg;
+}
+
+/* ------------------------------------------------------------------ the reference's 6x6 block kernels */
+static int chol6(double* A, double* dinv) {
+    for (int j = 0; j < 6; j++) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; k++) s -= A[j * 6 + k] * A[j * 6 + k];
+        if (!(s > 0)) return 0;
+        const double d = sqrt(s), id = 1.0 / d;
+        A[j * 6 + j] = d;
+        dinv[j] = id;
+        for (int i = j + 1; i < 6; i++) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; k++) t -= A[i * 6 + k] * A[j * 6 + k];
+            A[i * 6 + j] = t * id;
+        }
+        for (int i = 0; i < j; i++) A[i * 6 + j] = 0;
+    }
+    return 1;
+}
+static void right_solve_lt(const double* L, const double* dinv, double* X) {
+    for (int r = 0; r < 6; r++) {
+        double* x = X + r * 6;
+        for (int j = 0; j < 6; j++) {
+            double s = x[j];
+            for (int m = 0; m < j; m++) s -= L[j * 6 + m] * x[m];
+            x[j] = s * dinv[j];
+        }
+    }
+}
+static void sub_abt(double* T, const double* P, const double* Qm) {
+    double Qt[36];
+    for (int c = 0; c < 6; c++)
+        for (int m = 0; m < 6; m++) Qt[m * 6 + c] = Qm[c * 6 + m];
+    for (int r = 0; r < 6; r++) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int m = 0; m < 6; m++) {
+            const double p = P[r * 6 + m];
+            for (int c = 0; c < 6; c++) acc[c] += p * Qt[m * 6 + c];
+        }
+        for (int c = 0; c < 6; c++) T[r * 6 + c] -= acc[c];
+    }
+}
+static void sub_mv(const double* L, const double* y, double* r) {
+    for (int i = 0; i < 6; i++) {
+        double s = 0;
+        for (int j = 0; j < 6; j++) s += L[i * 6 + j] * y[j];
+        r[i] -= s;
+    }
+}
+static void sub_mtv(const double* L, const double* xr, double* x) {
+    for (int j = 0; j < 6; j++) {
+        double s = 0;
+        for (int i = 0; i < 6; i++) s += L[i * 6 + j] * xr[i];
+        x[j] -= s;
+    }
+}
+static void forward6(const double* L, const double* dinv, double* y) {
+    for (int j = 0; j < 6; j++) {
+        double s = y[j];
+        for (int m = 0; m < j; m++) s -= L[j * 6 + m] * y[m];
+        y[j] = s * dinv[j];
+    }
+}
+static void backward6(const double* L, const double* dinv, double* x) {
+    for (int j = 5; j >= 0; j--) {
+        double s = x[j];
+        for (int m = j + 1; m < 6; m++) s -= L[m * 6 + j] * x[m];
+        x[j] = s * dinv[j];
+    }
+}
+
+/* dense (6n)^2 row-major <-> blocks */
+static void get_block(const double* A, int n, int r, int c, double* B) {
+    for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++) B[p * 6 + q] = A[(size_t)(6 * r + p) * 6 * n + 6 * c + q];
+}
+static void put_block(double* A, int n, int r, int c, const double* B) {
+    for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++) A[(size_t)(6 * r + p) * 6 * n + 6 * c + q] = B[p * 6 + q];
+}
+
+int orc_fem_block_solve(int n, double* A, double* b) {
+    /* blocks of the lower triangle, right-looking over the columns (frame_fem.cpp:2899-2919), then L L^T x = b
+     * (:2933-2942) */
+    double* L = (double*)malloc(sizeof(double) * 36 * (size_t)n * n);
+    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c <= r; c++) get_block(A, n, r, c, &L[((size_t)r * n + c) * 36]);
+    int ok = 1;
+    for (int k = 0; k < n && ok; k++) {
+        double* Lkk = &L[((size_t)k * n + k) * 36];
+        if (!chol6(Lkk, &dinv[6 * k])) { ok = 0; break; }
+        for (int r = k + 1; r < n; r++) right_solve_lt(Lkk, &dinv[6 * k], &L[((size_t)r * n + k) * 36]);
+        for (int r = k + 1; r < n; r++)
+            for (int c = k + 1; c <= r; c++)
+                sub_abt(&L[((size_t)r * n + c) * 36], &L[((size_t)r * n + k) * 36], &L[((size_t)c * n + k) * 36]);
+    }
+    if (ok) {
+        for (int k = 0; k < n; k++) {
+            forward6(&L[((size_t)k * n + k) * 36], &dinv[6 * k], &b[6 * k]);
+            for (int r = k + 1; r < n; r++) sub_mv(&L[((size_t)r * n + k) * 36], &b[6 * k], &b[6 * r]);
+        }
+        for (int k = n - 1; k >= 0; k--) {
+            for (int r = k + 1; r < n; r++) sub_mtv(&L[((size_t)r * n + k) * 36], &b[6 * r], &b[6 * k]);
+            backward6(&L[((size_t)k * n + k) * 36], &dinv[6 * k], &b[6 * k]);
+        }
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c <= r; c++) put_block(A, n, r, c, &L[((size_t)r * n + c) * 36]);
+    }
+    free(L), free(dinv);
+    return ok ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------ the step */
+int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* Aout,
+                 double* rhs_out, double* x_out, uint32_t* free_map, uint32_t* nfree_out) {
+    const uint32_t nn = f->nn, nt = f->nt;
+    tri_out* o = (tri_out*)calloc(nt, sizeof(tri_out));
+    for (uint32_t k = 0; k < nt; k++) eval_tri(f, k, &o[k]);
+    /* the node forces: gravity m g, the loads, then the triangles' corners in triangle order */
+    f3* F = (f3*)malloc(sizeof(f3) * nn);
+    f3* T = (f3*)calloc(nn, sizeof(f3));
+    for (uint32_t i = 0; i < nn; i++) {
+        F[i] = mul3(F3(g[0], g[1], g[2]), f->mass[i]);
+        if (loads) F[i] = add3(F[i], F3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]));
+    }
+    for (uint32_t k = 0; k < nt; k++)
+        if (o[k].on)
+            for (int c = 0; c < 3; c++) {
+                const uint32_t nd = f->tn[3 * k + c];
+                F[nd] = add3(F[nd], o[k].fo[c]);
+                T[nd] = add3(T[nd], o[k].to[c]);
+            }
+    /* the free nodes in natural order */
+    int* fi = (int*)malloc(sizeof(int) * nn);
+    int nf = 0;
+    for (uint32_t i = 0; i < nn; i++) fi[i] = f->fixed[i] ? -1 : nf++;
+    if (free_map)
+        for (uint32_t i = 0; i < nn; i++)
+            if (fi[i] >= 0) free_map[fi[i]] = i;
+    if (nfree_out) *nfree_out = (uint32_t)nf;
+    const size_t N6 = 6 * (size_t)nf;
+    double* A = (double*)calloc(N6 * N6, sizeof(double));
+    double* r = (double*)calloc(N6, sizeof(double));
+    const double h2 = (double)theta * h * h, hd = (double)dissipation * h * h;
+    const double c = h2 + (double)f->damping * h, cr = hd + (double)f->damping * h;
+    for (uint32_t i = 0; i < nn; i++) {
+        if (fi[i] < 0) continue;
+        double D[36] = {0}, rd[6] = {0};
+        D[0] = D[7] = D[14] = f->mass[i];
+        D[21] = D[28] = D[35] = f->inertia[i];
+        for (uint32_t k = 0; k < nt; k++) {
+            if (!o[k].on) continue;
+            for (int ci = 0; ci < 3; ci++)
+                if (f->tn[3 * k + ci] == i) {
+                    tri_block(f, k, &o[k], ci, ci, c, h2, D);
+                    if (cr != 0)
+                        for (int j = 0; j < 6; j++) rd[j] -= cr * o[k].kv[6 * ci + j];
+                }
+        }
+        put_block(A, nf, fi[i], fi[i], D);
+        double* rr = &r[6 * (size_t)fi[i]];
+        rr[0] = h * (double)F[i].x, rr[1] = h * (double)F[i].y, rr[2] = h * (double)F[i].z;
+        rr[3] = h * (double)T[i].x, rr[4] = h * (double)T[i].y, rr[5] = h * (double)T[i].z;
+        for (int j = 0; j < 6; j++) rr[j] += rd[j];
+    }
+    /* the off-diagonal blocks: corners (i, j) of every triangle, both free */
+    for (uint32_t k = 0; k < nt; k++) {
+        if (!o[k].on) continue;
+        for (int ci = 0; ci < 3; ci++)
+            for (int cj = 0; cj < 3; cj++) {
+                if (ci == cj) continue;
+                const int a = fi[f->tn[3 * k + ci]], b = fi[f->tn[3 * k + cj]];
+                if (a < 0 || b < 0) continue;
+                double B[36];
+                get_block(A, nf, a, b, B);
+                tri_block(f, k, &o[k], ci, cj, c, h2, B);
+                put_block(A, nf, a, b, B);
+            }
+    }
+    if (Aout) memcpy(Aout, A, sizeof(double) * N6 * N6);
+    if (rhs_out) memcpy(rhs_out, r, sizeof(double) * N6);
+    int rc = orc_fem_block_solve(nf, A, r);
+    if (rc == 0) {
+        if (x_out) memcpy(x_out, r, sizeof(double) * N6);
+        /* the new velocities and rotations, then the positions and orientations */
+        for (uint32_t i = 0; i < nn; i++) {
+            if (fi[i] < 0) {
+                f->w[3 * i] = f->w[3 * i + 1] = f->w[3 * i + 2] = 0;
+                continue;
+            }
+            const double* x = &r[6 * (size_t)fi[i]];
+            for (int j = 0; j < 3; j++) f->v[3 * i + j] += (float)x[j], f->w[3 * i + j] += (float)x[3 + j];
+        }
+        for (uint32_t i = 0; i < nn; i++) {
+            if (f->fixed[i]) continue;
+            for (int j = 0; j < 3; j++) f->xd[3 * i + j] += (double)h * f->v[3 * i + j];
+            const f3 wv = F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]);
+            if (dot3(wv, wv) > 0) {
+                const qf qn = qnorm(qmul(qexp(mul3(wv, h)), Q(f->q[4 * i], f->q[4 * i + 1], f->q[4 * i + 2], f->q[4 * i + 3])));
+                f->q[4 * i] = qn.x, f->q[4 * i + 1] = qn.y, f->q[4 * i + 2] = qn.z, f->q[4 * i + 3] = qn.w;
+            }
+        }
+    }
+    free(o), free(F), free(T), free(fi), free(A), free(r);
+    return rc;
+}

@@ -1,0 +1,45 @@
+/* Test infrastructure only: the FP64 restatement of the reference's shell FEM step used as the checker of the HIP
+ * FEM path (see fem_oracle.c). Never linked into the product. */
+#ifndef BL_FEM_ORACLE_H
+#define BL_FEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_fem orc_fem;
+
+/* the element stiffness of one triangle (tri_local_k, src/phys/frame_fem.cpp:1524-1579), 18 x 18 row-major */
+void orc_fem_tri_local_k(const float X[6], float E, float nu, float t, float drill, double* K);
+/* the rest shape of a triangle from its corners (FemFrame::finalize, src/phys/frame_fem.cpp:259-276): X[6] */
+int orc_fem_rest_shape(const float p[9], float X[6]);
+
+/* A plate of nu x nv cells (ShellMesher::grid, src/phys/fem_shell.cpp:34-46) of `size` metres in the x-z plane,
+ * centred on the origin at height y0, its border nodes clamped. Section: E, nu, rho, t, drill, damping. */
+orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float section[6]);
+void orc_fem_destroy(orc_fem* f);
+uint32_t orc_fem_nodes(const orc_fem* f);
+uint32_t orc_fem_tris(const orc_fem* f);
+/* node arrays: tri corners (3 per triangle), positions (3 per node, double), fixed flags, mass, inertia */
+void orc_fem_topology(const orc_fem* f, uint32_t* tri_nodes, double* xd, uint8_t* fixed, float* mass, float* inertia);
+/* the state: v, w (3 per node), q (4 per node: x y z w) */
+void orc_fem_get_state(const orc_fem* f, double* xd, float* v, float* w, float* q);
+void orc_fem_set_state(orc_fem* f, const double* xd, const float* v, const float* w, const float* q);
+
+/* One implicit step of h: element pass, gather, assembly, block Cholesky (chol6 / right_solve_lt / sub_abt in the
+ * reference's right-looking order over the nodes' natural order, dense), substitution, update. loads: 3 per node
+ * (N), added to gravity m g. If `A` is non-null it receives the assembled system of the free nodes in the natural
+ * order: dense (nf*6)^2 row-major, `rhs` nf*6, `x` the solution; `free_map` the free nodes. Returns 0, or -1 when a
+ * pivot block is not positive definite. */
+int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* A,
+                 double* rhs, double* x, uint32_t* free_map, uint32_t* nfree);
+
+/* the reference's dense block Cholesky on an n x n block SPD matrix (row-major (6n)^2): solves A x = b in place */
+int orc_fem_block_solve(int n, double* A, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

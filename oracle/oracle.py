@@ -64,6 +64,18 @@ def oracle_lib():
             "box_from_quat": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
             "primitive_collision": (None, [_P, _P, ctypes.c_float, _P, ctypes.c_float, ctypes.c_int, ctypes.c_float,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_tri_local_k": (None, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_rest_shape": (ctypes.c_int, [_P, _P]),
+            "fem_plate": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_destroy": (None, [_P]),
+            "fem_nodes": (ctypes.c_uint32, [_P]),
+            "fem_tris": (ctypes.c_uint32, [_P]),
+            "fem_topology": (None, [_P, _P, _P, _P, _P, _P]),
+            "fem_get_state": (None, [_P, _P, _P, _P, _P]),
+            "fem_set_state": (None, [_P, _P, _P, _P, _P]),
+            "fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P,
+                                        ctypes.POINTER(ctypes.c_uint32)]),
+            "fem_block_solve": (ctypes.c_int, [ctypes.c_int, _P, _P]),
         })
     return _oracle
 
@@ -88,6 +100,7 @@ def ref_lib():
             "finalize_stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_int,
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
+            "tri_stiffness": (ctypes.c_int, [_P, _P, _P, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -210,3 +223,109 @@ def primitive_collision(F, vel, mass, normal, dt, surface, pen, fric, push_max=2
     oracle_lib().orc_primitive_collision(_ptr(F), _ptr(vel), mass, _ptr(normal), dt, surface, pen, fric, push_max,
                                          slop, restitution, _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------------------------- the shell FEM
+SECTION_C3 = (70e9, 0.33, 2700.0, 1.5e-3, 0.05, 3e-4)  # E, nu, rho, t, drill, damping
+
+
+def fem_tri_local_k(X, section=SECTION_C3):
+    """The oracle's element stiffness (18 x 18, float64) of a triangle of rest shape X (3 x 2)."""
+    X = np.ascontiguousarray(X, np.float32).reshape(6)
+    K = np.zeros((18, 18), np.float64)
+    E, nu, _, t, drill, _ = section
+    oracle_lib().orc_fem_tri_local_k(_ptr(X), E, nu, t, drill, _ptr(K))
+    return K
+
+
+def fem_rest_shape(p):
+    p = np.ascontiguousarray(p, np.float32).reshape(9)
+    X = np.zeros(6, np.float32)
+    if oracle_lib().orc_fem_rest_shape(_ptr(p), _ptr(X)) != 0:
+        raise ValueError("degenerate triangle")
+    return X.reshape(3, 2)
+
+
+def ref_tri_stiffness(p, section=SECTION_C3):
+    """The compiled reference's FemFrame::tri_stiffness of the triangle with corners p (3 x 3): (K, X)."""
+    p = np.ascontiguousarray(p, np.float32).reshape(9)
+    sec = np.asarray(section, np.float32)
+    K = np.zeros((18, 18), np.float64)
+    X = np.zeros(6, np.float32)
+    if ref_lib().blref_tri_stiffness(_ptr(p), _ptr(sec), _ptr(K), _ptr(X)) != 0:
+        raise ValueError("degenerate triangle")
+    return K, X.reshape(3, 2)
+
+
+class FemPlate:
+    """The oracle's plate (ShellMesher::grid) and its implicit step (see fem_oracle.c)."""
+
+    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3):
+        self._lib = oracle_lib()
+        sec = np.asarray(section, np.float32)
+        self._h = self._lib.orc_fem_plate(int(nu), int(nv), float(size), float(y0), _ptr(sec))
+        self.nn = self._lib.orc_fem_nodes(self._h)
+        self.nt = self._lib.orc_fem_tris(self._h)
+
+    def topology(self):
+        tn = np.zeros((self.nt, 3), np.uint32)
+        xd = np.zeros((self.nn, 3), np.float64)
+        fixed = np.zeros(self.nn, np.uint8)
+        mass = np.zeros(self.nn, np.float32)
+        inertia = np.zeros(self.nn, np.float32)
+        self._lib.orc_fem_topology(self._h, _ptr(tn), _ptr(xd), _ptr(fixed), _ptr(mass), _ptr(inertia))
+        return tn, xd, fixed, mass, inertia
+
+    def state(self):
+        xd = np.zeros((self.nn, 3), np.float64)
+        v = np.zeros((self.nn, 3), np.float32)
+        w = np.zeros((self.nn, 3), np.float32)
+        q = np.zeros((self.nn, 4), np.float32)
+        self._lib.orc_fem_get_state(self._h, _ptr(xd), _ptr(v), _ptr(w), _ptr(q))
+        return xd, v, w, q
+
+    def set_state(self, xd, v, w, q):
+        self._lib.orc_fem_set_state(self._h, *[_ptr(np.ascontiguousarray(a, t)) for a, t in
+                                              ((xd, np.float64), (v, np.float32), (w, np.float32), (q, np.float32))])
+
+    def step(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0), system=False):
+        """One implicit step; with system=True also (A, rhs, x, free nodes) of the step's linear system."""
+        loads = np.ascontiguousarray(loads, np.float32).reshape(self.nn, 3)
+        gg = np.asarray(g, np.float32)
+        nf = ctypes.c_uint32(0)
+        if system:
+            fixed = self.topology()[2]
+            n = int((fixed == 0).sum())
+            A = np.zeros((6 * n, 6 * n), np.float64)
+            r = np.zeros(6 * n, np.float64)
+            x = np.zeros(6 * n, np.float64)
+            fm = np.zeros(n, np.uint32)
+            rc = self._lib.orc_fem_step(self._h, _ptr(loads), h, theta, dissipation, _ptr(gg), _ptr(A), _ptr(r), _ptr(x),
+                                        _ptr(fm), ctypes.byref(nf))
+            if rc != 0:
+                raise RuntimeError("oracle FEM step: a pivot block is not positive definite")
+            return A, r, x, fm
+        rc = self._lib.orc_fem_step(self._h, _ptr(loads), h, theta, dissipation, _ptr(gg), None, None, None, None,
+                                    ctypes.byref(nf))
+        if rc != 0:
+            raise RuntimeError("oracle FEM step: a pivot block is not positive definite")
+
+    def close(self):
+        if self._h:
+            self._lib.orc_fem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fem_block_solve(A, b):
+    """The reference's dense block Cholesky (chol6, right_solve_lt, sub_abt; forward6, backward6) on A x = b."""
+    A = np.array(A, np.float64, order="C", copy=True)
+    x = np.array(b, np.float64, copy=True)
+    if oracle_lib().orc_fem_block_solve(len(x) // 6, _ptr(A), _ptr(x)) != 0:
+        raise RuntimeError("not positive definite")
+    return x

@@ -226,4 +226,21 @@ int blref_finalize_stabilize(bl_node* nodes, uint32_t n, bl_beam* beams, uint32_
     return 0;
 }
 
+// One shell triangle through the reference's own FemFrame: its rest shape (FemFrame::finalize) and its element
+// stiffness (FemFrame::tri_stiffness -> tri_local_k, src/phys/frame_fem.cpp:1524-1579, 1649-1651), 18 x 18 row-major.
+// sec: E, nu, rho, t, drill, damping.
+int blref_tri_stiffness(const float p[9], const float sec[6], double* K, float* X) {
+    SoftBody b;
+    for (int i = 0; i < 3; i++) b.add_node(vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), 1.0f);
+    ShellSection s;
+    s.E = sec[0], s.nu = sec[1], s.rho = sec[2], s.t = sec[3], s.drill = sec[4], s.damping = sec[5];
+    const uint16_t si = b.fem.add_shell_section(s);
+    b.fem.add_tri(0, 1, 2, si);
+    b.fem.finalize(b);
+    if (b.fem.tris.empty() || b.fem.tris[0].broken) return -1;
+    b.fem.tri_stiffness(0, reinterpret_cast<double(*)[18]>(K));
+    for (int i = 0; i < 3; i++) X[2 * i] = b.fem.tris[0].X[i][0], X[2 * i + 1] = b.fem.tris[0].X[i][1];
+    return 0;
+}
+
 } // extern "C"

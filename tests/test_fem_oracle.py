@@ -1,0 +1,94 @@
+"""The FEM oracle (oracle/fem_oracle.c) pinned: the element stiffness bit for bit against the compiled reference's
+FemFrame::tri_stiffness, the element's rigid-body modes, the plate against Kirchhoff plate theory, and the
+reference's block Cholesky by its residual."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+
+def _corners(rng):
+    while True:
+        p = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        if np.linalg.norm(n) > 1e-2:
+            return p
+
+
+def test_tri_stiffness_matches_reference_bitwise(ref_mod):
+    rng = np.random.default_rng(7)
+    sections = [oracle.SECTION_C3, (2.1e11, 0.3, 7850.0, 1e-3, 0.05, 3e-4), (5e9, 0.45, 1200.0, 4e-3, 0.2, 0.0)]
+    for trial in range(120):
+        sec = sections[trial % len(sections)]
+        p = _corners(rng) * np.float32(0.01 + trial / 60)
+        Kr, Xr = oracle.ref_tri_stiffness(p, sec)
+        X = oracle.fem_rest_shape(p)
+        assert X.tobytes() == Xr.tobytes(), "rest shape"
+        K = oracle.fem_tri_local_k(X, sec)
+        assert K.tobytes() == Kr.tobytes(), f"element stiffness differs from the reference (trial {trial})"
+
+
+def test_plate_triangles_match_reference(ref_mod):
+    P = oracle.FemPlate(4, 4)
+    tn, xd, *_ = P.topology()
+    for k in range(P.nt):
+        p = xd[tn[k]].astype(np.float32)
+        Kr, _ = oracle.ref_tri_stiffness(p)
+        assert oracle.fem_tri_local_k(oracle.fem_rest_shape(p)).tobytes() == Kr.tobytes()
+
+
+def test_element_symmetric_and_rigid_modes():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        X = oracle.fem_rest_shape(_corners(rng))
+        K = oracle.fem_tri_local_k(X)
+        scale = np.abs(K).max()
+        assert np.abs(K - K.T).max() <= 1e-12 * scale
+        # rigid-body modes of the flat element (dofs per corner u v w theta_x theta_y theta_z): three translations,
+        # the turn about the normal (u = -omega y, v = omega x, theta_z = omega), the tilts (w = y, theta_x = 1 and
+        # w = -x, theta_y = 1: theta_x = w_y, theta_y = -w_x)
+        modes = []
+        for d in range(3):
+            m = np.zeros(18)
+            m[d::6] = 1
+            modes.append(m)
+        m = np.zeros(18)
+        m[0::6], m[1::6], m[5::6] = -X[:, 1], X[:, 0], 1
+        modes.append(m)
+        m = np.zeros(18)
+        m[2::6], m[3::6] = X[:, 1], 1
+        modes.append(m)
+        m = np.zeros(18)
+        m[2::6], m[4::6] = -X[:, 0], 1
+        modes.append(m)
+        for m in modes:
+            assert np.abs(K @ m).max() <= 1e-9 * scale * np.abs(m).max()
+
+
+def test_plate_matches_kirchhoff():
+    """Clamped square plate, central load: w = 0.00560 P a^2 / D (Timoshenko & Woinowsky-Krieger, table 55); a
+    quasi-static step (theta 1, h large, no damping, no gravity) from rest is K^-1 f."""
+    n = 24
+    sec = list(oracle.SECTION_C3)
+    sec[5] = 0.0
+    P = oracle.FemPlate(n, n, section=sec)
+    loads = np.zeros((P.nn, 3), np.float32)
+    c = (n // 2) * (n + 1) + n // 2
+    loads[c, 1] = -1.0
+    x0 = P.state()[0].copy()
+    P.step(loads, 1e3, theta=1.0, dissipation=0.0, g=(0, 0, 0))
+    w = (P.state()[0] - x0)[c, 1]
+    E, nu, t = 70e9, 0.33, 1.5e-3
+    D = E * t ** 3 / (12 * (1 - nu * nu))
+    assert w == pytest.approx(-0.0056 / D, rel=0.01)
+
+
+def test_step_system_symmetric_and_solved():
+    P = oracle.FemPlate(6, 5)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        loads = rng.normal(0, 50, (P.nn, 3)).astype(np.float32)
+        A, r, x, fm = P.step(loads, 0.002, system=True)
+        assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+        assert np.linalg.norm(A @ x - r) <= 1e-12 * np.linalg.norm(r)
+        assert np.allclose(x, np.linalg.solve(A, r), rtol=1e-9, atol=1e-12 * np.abs(x).max())
