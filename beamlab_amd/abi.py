@@ -80,6 +80,25 @@ class Tuning(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("incidences_per_thread", ctypes.c_int32)]
 
 
+class ShellSection(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("E", "nu", "rho", "t", "drill", "damping")]
+
+
+class FemDesc(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_uint32), ("n_tris", ctypes.c_uint32), ("p", ctypes.c_void_p),
+                ("tris", ctypes.c_void_p), ("fixed", ctypes.c_void_p), ("mass", ctypes.c_void_p),
+                ("section", ShellSection)]
+
+
+class FemReport(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("factor_flops", ctypes.c_double)] + \
+               [(n, ctypes.c_float) for n in ("assemble_ms", "factor_ms", "solve_ms", "update_ms")] + \
+               [(n, ctypes.c_int32) for n in ("failures", "supernodes", "fronts_max")] + [("factor_nnz", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # every entry point the header declares, with its argument types
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -108,6 +127,15 @@ SIGNATURES = {
     "bl_time_step": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "bl_time_kernels": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
+    "bl_fem_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(FemDesc)]),
+    "bl_fem_destroy": (None, [_P]),
+    "bl_fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P,
+                                   ctypes.POINTER(FemReport)]),
+    "bl_fem_get_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "bl_fem_set_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "bl_fem_last_system": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
+    "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),
+    "bl_fem_tri_k": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
 }
 
 _lib = None

@@ -761,6 +761,8 @@ int post_frame(bl_ctx* c) {
 
 } // namespace
 
+void blg::set_last_error(const char* msg) { g_last_error = msg; }
+
 extern "C" {
 
 int bl_abi_version(void) { return BL_ABI_VERSION; }

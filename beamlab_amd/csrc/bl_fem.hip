@@ -1,0 +1,1502 @@
+// The implicit co-rotational shell step on gfx950: element pass, 6x6-block assembly, supernodal (multifrontal)
+// Cholesky over a nested-dissection order with the dense fronts' trailing updates on FP64 matrix cores, the
+// substitutions, the update of the state.
+//
+// Reference semantics (src/phys/frame_fem.cpp): FemFrame::finalize for triangles (:259-276), node_inertia
+// (:3075-3086), tri_grad / dkt_b / tri_local_k / tri_build_k (:1482-1579, 1625-1647; FP64, bit-identical),
+// tri_kinematics and eval_tris for elastic sections (:1584-1622, 1661-1831), gather_part's triangle blocks and right
+// side (:916-1086), the update of velocities, rotations and positions (:3003-3027, 3407-3416). The factorization
+// solves the same system as the reference's chol6 / right_solve_lt / sub_abt over its own order (:2193-2317,
+// 2899-2943): the results agree to the rounding of FP64 (the residual is measured every step).
+//
+// Layout in HBM:
+//   nodes       xd double[3n] positions, v / w float[3n], q float4 (x y z w), mass, inertia, fixed
+//   triangles   corners uint4, rest shape X float[6], rest frames r0 float4[3], stiffness float[162] (membrane 9x9,
+//               plate 9x9, as the reference stores it), the element pass's outputs (axes, corner forces and
+//               moments, tension stiffness, K v)
+//   system      per free node its diagonal block, per pair of free nodes sharing a triangle the lower block
+//               (FP64 36 each), the right side
+//   fronts      one dense column-major m x m FP64 matrix per supernode (its columns, then the rows below them),
+//               holding L11 / L21 after the factorization and the update matrix for its parent in the trailing part
+#include "bl_world.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+using blg::f3;
+
+namespace blf {
+
+using blg::mk;
+
+constexpr int kMemDof[3] = {0, 1, 5}, kPlateDof[3] = {2, 3, 4};
+__constant__ int c_mem_dof[3] = {0, 1, 5};
+__constant__ int c_plate_dof[3] = {2, 3, 4};
+
+// ------------------------------------------------------------------------------------------ float quaternions
+// (src/core/math.h:131-226, the same operations in the same order)
+struct qf {
+    float x, y, z, w;
+};
+BL_HD qf Q(float x, float y, float z, float w) { qf q; q.x = x, q.y = y, q.z = z, q.w = w; return q; }
+BL_HD qf qmul(qf a, qf b) {
+    return Q(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+BL_HD qf qconj(qf q) { return Q(-q.x, -q.y, -q.z, q.w); }
+BL_HD qf qscale(qf q, float s) { return Q(q.x * s, q.y * s, q.z * s, q.w * s); }
+BL_HD qf qnorm(qf q) {
+    const float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    return l > 1e-12f ? qscale(q, 1.0f / l) : Q(0, 0, 0, 1);
+}
+BL_HD qf from_cols(f3 c0, f3 c1, f3 c2) {
+    const float m00 = c0.x, m11 = c1.y, m22 = c2.z;
+    const float tr = m00 + m11 + m22;
+    qf q;
+    if (tr > 0) {
+        const float s = sqrtf(tr + 1.0f) * 2;
+        q.w = 0.25f * s, q.x = (c1.z - c2.y) / s, q.y = (c2.x - c0.z) / s, q.z = (c0.y - c1.x) / s;
+    } else if (m00 > m11 && m00 > m22) {
+        const float s = sqrtf(1.0f + m00 - m11 - m22) * 2;
+        q.w = (c1.z - c2.y) / s, q.x = 0.25f * s, q.y = (c1.x + c0.y) / s, q.z = (c2.x + c0.z) / s;
+    } else if (m11 > m22) {
+        const float s = sqrtf(1.0f + m11 - m00 - m22) * 2;
+        q.w = (c2.x - c0.z) / s, q.x = (c1.x + c0.y) / s, q.y = 0.25f * s, q.z = (c2.y + c1.z) / s;
+    } else {
+        const float s = sqrtf(1.0f + m22 - m00 - m11) * 2;
+        q.w = (c0.y - c1.x) / s, q.x = (c2.x + c0.z) / s, q.y = (c2.y + c1.z) / s, q.z = 0.25f * s;
+    }
+    return qnorm(q);
+}
+__device__ f3 qlog(qf qin) {
+    const qf q = qin.w < 0 ? qscale(qin, -1.0f) : qin;
+    const f3 v = mk(q.x, q.y, q.z);
+    const float s = blg::length(v);
+    if (s < 1e-8f) return blg::mul(v, 2.0f);
+    const float angle = 2.0f * atan2f(s, q.w);
+    return blg::mul(v, angle / s);
+}
+__device__ qf qexp(f3 rv) {
+    const float a = blg::length(rv);
+    if (a < 1e-8f) return qnorm(Q(rv.x * 0.5f, rv.y * 0.5f, rv.z * 0.5f, 1.0f));
+    const f3 d = blg::divs(rv, a);
+    const float l = blg::length(d);
+    const f3 axis = l > 1e-12f ? blg::mul(d, 1.0f / l) : mk(0, 0, 0);
+    const float hh = a * 0.5f, s = sinf(hh);
+    return Q(axis.x * s, axis.y * s, axis.z * s, cosf(hh));
+}
+
+// ------------------------------------------------------------------------------------------ the element, FP64
+__device__ void dkt_b(const double x[3], const double y[3], double xi, double eta, double B[3][9]) {
+    const double x23 = x[1] - x[2], x31 = x[2] - x[0], x12 = x[0] - x[1];
+    const double y23 = y[1] - y[2], y31 = y[2] - y[0], y12 = y[0] - y[1];
+    double P[7], q[7], r[7], t[7];
+    const double xs[3] = {x23, x31, x12}, ys[3] = {y23, y31, y12};
+    for (int k = 4; k <= 6; k++) {
+        const double xij = xs[k - 4], yij = ys[k - 4], l2 = xij * xij + yij * yij;
+        P[k] = -6 * xij / l2, q[k] = 3 * xij * yij / l2, t[k] = -6 * yij / l2, r[k] = 3 * yij * yij / l2;
+    }
+    const double a = 1 - 2 * xi, bb = 1 - 2 * eta;
+    const double Hx_xi[9] = {P[6] * a + (P[5] - P[6]) * eta, q[6] * a - (q[5] + q[6]) * eta,
+                             -4 + 6 * (xi + eta) + r[6] * a - eta * (r[5] + r[6]),
+                             -P[6] * a + eta * (P[4] + P[6]), q[6] * a - eta * (q[6] - q[4]),
+                             -2 + 6 * xi + r[6] * a + eta * (r[4] - r[6]),
+                             -eta * (P[5] + P[4]), eta * (q[4] - q[5]), -eta * (r[5] - r[4])};
+    const double Hy_xi[9] = {t[6] * a + eta * (t[5] - t[6]), 1 + r[6] * a - eta * (r[5] + r[6]),
+                             -q[6] * a + eta * (q[5] + q[6]),
+                             -t[6] * a + eta * (t[4] + t[6]), -1 + r[6] * a + eta * (r[4] - r[6]),
+                             -q[6] * a - eta * (q[4] - q[6]),
+                             -eta * (t[4] + t[5]), eta * (r[4] - r[5]), -eta * (q[4] - q[5])};
+    const double Hx_eta[9] = {-P[5] * bb - xi * (P[6] - P[5]), q[5] * bb - xi * (q[5] + q[6]),
+                              -4 + 6 * (xi + eta) + r[5] * bb - xi * (r[5] + r[6]),
+                              xi * (P[4] + P[6]), xi * (q[4] - q[6]), -xi * (r[6] - r[4]),
+                              P[5] * bb - xi * (P[4] + P[5]), q[5] * bb + xi * (q[4] - q[5]),
+                              -2 + 6 * eta + r[5] * bb + xi * (r[4] - r[5])};
+    const double Hy_eta[9] = {-t[5] * bb - xi * (t[6] - t[5]), 1 + r[5] * bb - xi * (r[5] + r[6]),
+                              -q[5] * bb + xi * (q[5] + q[6]),
+                              xi * (t[4] + t[6]), xi * (r[4] - r[6]), -xi * (q[4] - q[6]),
+                              t[5] * bb - xi * (t[4] + t[5]), -1 + r[5] * bb + xi * (r[4] - r[5]),
+                              -q[5] * bb - xi * (q[4] - q[5])};
+    const double A2 = x31 * y12 - x12 * y31, iA = 1.0 / A2;
+    for (int j = 0; j < 9; j++) {
+        B[0][j] = (y31 * Hx_xi[j] + y12 * Hx_eta[j]) * iA;
+        B[1][j] = (-x31 * Hy_xi[j] - x12 * Hy_eta[j]) * iA;
+        B[2][j] = (-x31 * Hx_xi[j] - x12 * Hx_eta[j] + y31 * Hy_xi[j] + y12 * Hy_eta[j]) * iA;
+    }
+}
+
+__device__ __forceinline__ void tri_grad(const float* X, double b[3], double c[3], double& A2) {
+    for (int i = 0; i < 3; i++) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        b[i] = (double)X[2 * j + 1] - X[2 * k + 1];
+        c[i] = (double)X[2 * k] - X[2 * j];
+    }
+    A2 = ((double)X[2] - X[0]) * ((double)X[5] - X[1]) - ((double)X[4] - X[0]) * ((double)X[3] - X[1]);
+}
+
+struct Section {
+    float E, nu, rho, t, drill, damping;
+};
+
+// tri_local_k (src/phys/frame_fem.cpp:1524-1579): K (18 x 18, row-major) of the rest shape X
+__device__ void tri_local_k(const float* X, const Section& s, double* K) {
+    for (int i = 0; i < 324; i++) K[i] = 0;
+    double b[3], c[3], A2;
+    tri_grad(X, b, c, A2);
+    if (!(A2 > 0)) return;
+    const double A = 0.5 * A2, nu = s.nu;
+    const double Dm = (double)s.E * s.t / (1 - nu * nu);
+    double Bm[3][6];
+    for (int i = 0; i < 3; i++) {
+        Bm[0][2 * i] = b[i] / A2, Bm[1][2 * i] = 0, Bm[2][2 * i] = c[i] / A2;
+        Bm[0][2 * i + 1] = 0, Bm[1][2 * i + 1] = c[i] / A2, Bm[2][2 * i + 1] = b[i] / A2;
+    }
+    const double D3[3][3] = {{1, nu, 0}, {nu, 1, 0}, {0, 0, 0.5 * (1 - nu)}};
+    for (int p = 0; p < 6; p++)
+        for (int q = 0; q < 6; q++) {
+            double v = 0;
+            for (int r = 0; r < 3; r++)
+                for (int t = 0; t < 3; t++) v += Bm[r][p] * D3[r][t] * Bm[t][q];
+            K[(6 * (p / 2) + p % 2) * 18 + 6 * (q / 2) + q % 2] += A * Dm * v;
+        }
+    const double x[3] = {X[0], X[2], X[4]}, y[3] = {X[1], X[3], X[5]};
+    const float Dbf = s.E * s.t * s.t * s.t / (12.0f * (1.0f - s.nu * s.nu)); // (ShellSection::D(), in float)
+    const double Db = Dbf;
+    const double gauss[3][2] = {{1.0 / 6, 1.0 / 6}, {2.0 / 3, 1.0 / 6}, {1.0 / 6, 2.0 / 3}};
+    for (int g = 0; g < 3; g++) {
+        double B[3][9];
+        dkt_b(x, y, gauss[g][0], gauss[g][1], B);
+        for (int p = 0; p < 9; p++)
+            for (int q = 0; q < 9; q++) {
+                double v = 0;
+                for (int r = 0; r < 3; r++)
+                    for (int t = 0; t < 3; t++) v += B[r][p] * D3[r][t] * B[t][q];
+                K[(6 * (p / 3) + 2 + p % 3) * 18 + 6 * (q / 3) + 2 + q % 3] += A2 / 6.0 * Db * v;
+            }
+    }
+    const double G = (double)s.E / (2 * (1 + nu)), kd = (double)s.drill * G * s.t * A;
+    double gu[3], gv[3];
+    for (int i = 0; i < 3; i++) gu[i] = -c[i] / (2 * A2), gv[i] = b[i] / (2 * A2);
+    for (int i = 0; i < 3; i++) {
+        K[(6 * i + 5) * 18 + 6 * i + 5] += kd;
+        for (int j = 0; j < 3; j++) {
+            K[(6 * i + 5) * 18 + 6 * j] -= kd * gu[j], K[(6 * j) * 18 + 6 * i + 5] -= kd * gu[j];
+            K[(6 * i + 5) * 18 + 6 * j + 1] -= kd * gv[j], K[(6 * j + 1) * 18 + 6 * i + 5] -= kd * gv[j];
+            K[(6 * i) * 18 + 6 * j] += 3 * kd * gu[i] * gu[j], K[(6 * i) * 18 + 6 * j + 1] += 3 * kd * gu[i] * gv[j];
+            K[(6 * i + 1) * 18 + 6 * j] += 3 * kd * gv[i] * gu[j], K[(6 * i + 1) * 18 + 6 * j + 1] += 3 * kd * gv[i] * gv[j];
+        }
+    }
+}
+
+struct FemArgs {
+    uint32_t nn, nt, nf;
+    Section sec;
+    double* xd;
+    float *v, *w;
+    float4* q;
+    const float *mass, *inertia;
+    const uint8_t* fixed;
+    const uint4* tri;   // corners, 0
+    const float* X;     // 6 per triangle
+    const float4* r0;   // 3 per triangle
+    float* Kf;          // 162 per triangle
+    double* Kd;         // 324 per triangle (kept for export)
+    // element pass outputs
+    float* te;          // 9 per triangle: e1 e2 e3
+    float* tfo;         // 9: the corners' forces
+    float* tto;         // 9: the corners' moments
+    float* tg;          // 9: tension stiffness g[i][j]
+    double* tkv;        // 18: K v, world axes
+    uint8_t* ton;
+    // node -> (triangle, corner) incidences in triangle order
+    const uint32_t* nc_off;
+    const uint32_t* nc;
+    // free nodes: order index and its front, position in the front
+    const int* fidx;     // per node: its free index, -1 pinned
+    const uint32_t* ford; // per free index: its position in the order
+    // the system
+    double* Dsys;        // 36 per free index
+    double* rhs;         // 6 per free index
+    double* Osys;        // 36 per pair
+    const uint4* pairs;  // (row node, column node, first contribution, end)
+    const uint4* pcon;   // (triangle, corner of the row node, corner of the column node, 0)
+    uint32_t npairs;
+    const float* loads;  // 3 per node
+    float g[3];
+    float h, theta, dissipation;
+};
+
+// FemFrame::tri_build_k on the device: the element stiffness (FP64) and its float copy
+__global__ void k_tri_k(FemArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nt) return;
+    double* K = a.Kd + (size_t)t * 324;
+    tri_local_k(a.X + 6 * (size_t)t, a.sec, K);
+    float* P = a.Kf + (size_t)t * 162;
+    for (int i = 0; i < 9; i++)
+        for (int j = 0; j < 9; j++) {
+            const int im = 6 * (i / 3) + kMemDof[i % 3], jm = 6 * (j / 3) + kMemDof[j % 3];
+            const int ip = 6 * (i / 3) + kPlateDof[i % 3], jp = 6 * (j / 3) + kPlateDof[j % 3];
+            P[i * 9 + j] = (float)K[(im < jm ? im : jm) * 18 + (im < jm ? jm : im)];
+            P[81 + i * 9 + j] = (float)K[(ip < jp ? ip : jp) * 18 + (ip < jp ? jp : ip)];
+        }
+}
+
+// tri_kinematics + eval_tris (elastic): the co-rotated frame, the corners' forces and moments, the tangent
+__global__ void k_tris(FemArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nt) return;
+    a.ton[t] = 0;
+    const uint4 tr = a.tri[t];
+    const uint32_t n[3] = {tr.x, tr.y, tr.z};
+    const float* X = a.X + 6 * (size_t)t;
+    double x[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) x[i][j] = a.xd[n[i] * 3 + j];
+    const double av[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+    const double cv[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+    double nv[3] = {av[1] * cv[2] - av[2] * cv[1], av[2] * cv[0] - av[0] * cv[2], av[0] * cv[1] - av[1] * cv[0]};
+    const double nl = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]), al = sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+    if (!(nl > 1e-12) || !(al > 1e-9) || !isfinite(nl)) return;
+    for (int j = 0; j < 3; j++) nv[j] /= nl;
+    const double t1[3] = {av[0] / al, av[1] / al, av[2] / al};
+    const double t2[3] = {nv[1] * t1[2] - nv[2] * t1[1], nv[2] * t1[0] - nv[0] * t1[2], nv[0] * t1[1] - nv[1] * t1[0]};
+    double cen[3];
+    for (int j = 0; j < 3; j++) cen[j] = (x[0][j] + x[1][j] + x[2][j]) / 3.0;
+    double p[3][2];
+    for (int i = 0; i < 3; i++) {
+        const double r[3] = {x[i][0] - cen[0], x[i][1] - cen[1], x[i][2] - cen[2]};
+        p[i][0] = r[0] * t1[0] + r[1] * t1[1] + r[2] * t1[2];
+        p[i][1] = r[0] * t2[0] + r[1] * t2[1] + r[2] * t2[2];
+    }
+    double sn = 0, cs = 0;
+    for (int i = 0; i < 3; i++) sn += X[2 * i] * p[i][1] - X[2 * i + 1] * p[i][0], cs += X[2 * i] * p[i][0] + X[2 * i + 1] * p[i][1];
+    const double phi = atan2(sn, cs), cp = cos(phi), sp = sin(phi);
+    double e1d[3], e2d[3];
+    for (int j = 0; j < 3; j++) e1d[j] = cp * t1[j] + sp * t2[j], e2d[j] = -sp * t1[j] + cp * t2[j];
+    const f3 e1 = mk((float)e1d[0], (float)e1d[1], (float)e1d[2]), e2 = mk((float)e2d[0], (float)e2d[1], (float)e2d[2]);
+    const f3 e3 = mk((float)nv[0], (float)nv[1], (float)nv[2]);
+    const qf Ec = qconj(from_cols(e1, e2, e3));
+    double d[18];
+    for (int i = 0; i < 3; i++) {
+        d[6 * i] = cp * p[i][0] + sp * p[i][1] - X[2 * i];
+        d[6 * i + 1] = -sp * p[i][0] + cp * p[i][1] - X[2 * i + 1];
+        d[6 * i + 2] = 0;
+        const float4 qn = a.q[n[i]], r0 = a.r0[3 * (size_t)t + i];
+        const f3 th = qlog(qmul(Ec, qnorm(qmul(Q(qn.x, qn.y, qn.z, qn.w), Q(r0.x, r0.y, r0.z, r0.w)))));
+        d[6 * i + 3] = th.x, d[6 * i + 4] = th.y, d[6 * i + 5] = th.z;
+    }
+    double b3[3], c3[3], A2;
+    tri_grad(X, b3, c3, A2);
+    if (!(A2 > 0)) return;
+    const double nu = a.sec.nu, Dm = (double)a.sec.E / (1 - nu * nu);
+    double eps[3] = {0, 0, 0}, sig[3];
+    for (int i = 0; i < 3; i++) {
+        eps[0] += b3[i] * d[6 * i] / A2;
+        eps[1] += c3[i] * d[6 * i + 1] / A2;
+        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
+    }
+    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+    const float* Km = a.Kf + (size_t)t * 162;
+    const float* Kp = Km + 81;
+    const double E[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
+    double vl[18];
+    for (int i = 0; i < 3; i++) {
+        const f3 vv = mk(a.v[3 * n[i]], a.v[3 * n[i] + 1], a.v[3 * n[i] + 2]);
+        const f3 om = mk(a.w[3 * n[i]], a.w[3 * n[i] + 1], a.w[3 * n[i] + 2]);
+        for (int k = 0; k < 3; k++) {
+            vl[6 * i + k] = E[0][k] * vv.x + E[1][k] * vv.y + E[2][k] * vv.z;
+            vl[6 * i + 3 + k] = E[0][k] * om.x + E[1][k] * om.y + E[2][k] * om.z;
+        }
+    }
+    double fo[18], fl[18];
+    {
+        double dm[9], dp[9], vm[9], vp[9];
+        for (int k = 0; k < 9; k++) {
+            const int ci = 6 * (k / 3);
+            dm[k] = d[ci + c_mem_dof[k % 3]], dp[k] = d[ci + c_plate_dof[k % 3]];
+            vm[k] = vl[ci + c_mem_dof[k % 3]], vp[k] = vl[ci + c_plate_dof[k % 3]];
+        }
+        double fm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, fp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double um[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, up[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 9; c++)
+            for (int k = 0; k < 9; k++) {
+                const double km = (double)Km[c * 9 + k], kp = (double)Kp[c * 9 + k];
+                fm[k] += km * dm[c], um[k] += km * vm[c];
+                fp[k] += kp * dp[c], up[k] += kp * vp[c];
+            }
+        for (int k = 0; k < 9; k++) {
+            const int ci = 6 * (k / 3);
+            fo[ci + c_mem_dof[k % 3]] = fm[k], fl[ci + c_mem_dof[k % 3]] = um[k];
+            fo[ci + c_plate_dof[k % 3]] = fp[k], fl[ci + c_plate_dof[k % 3]] = up[k];
+        }
+    }
+    float* te = a.te + 9 * (size_t)t;
+    te[0] = e1.x, te[1] = e1.y, te[2] = e1.z, te[3] = e2.x, te[4] = e2.y, te[5] = e2.z, te[6] = e3.x, te[7] = e3.y, te[8] = e3.z;
+    for (int i = 0; i < 3; i++) {
+        const f3 fw = blg::add(blg::add(blg::mul(e1, (float)fo[6 * i]), blg::mul(e2, (float)fo[6 * i + 1])), blg::mul(e3, (float)fo[6 * i + 2]));
+        const f3 tw = blg::add(blg::add(blg::mul(e1, (float)fo[6 * i + 3]), blg::mul(e2, (float)fo[6 * i + 4])), blg::mul(e3, (float)fo[6 * i + 5]));
+        a.tfo[9 * (size_t)t + 3 * i] = -fw.x, a.tfo[9 * (size_t)t + 3 * i + 1] = -fw.y, a.tfo[9 * (size_t)t + 3 * i + 2] = -fw.z;
+        a.tto[9 * (size_t)t + 3 * i] = -tw.x, a.tto[9 * (size_t)t + 3 * i + 1] = -tw.y, a.tto[9 * (size_t)t + 3 * i + 2] = -tw.z;
+    }
+    {
+        const double sx = sig[0] * a.sec.t, sy = sig[1] * a.sec.t, sxy = sig[2] * a.sec.t;
+        const double tr2 = 0.5 * (sx + sy), df = sqrt(0.25 * (sx - sy) * (sx - sy) + sxy * sxy);
+        const double l1 = fmax(0.0, tr2 + df), l2 = fmax(0.0, tr2 - df);
+        double vx = sxy, vy = l1 - sx;
+        if (fabs(vx) + fabs(vy) < 1e-12 * (fabs(sx) + fabs(sy) + 1)) vx = sx >= sy ? 1 : 0, vy = sx >= sy ? 0 : 1;
+        const double vlen = sqrt(vx * vx + vy * vy);
+        vx /= vlen, vy /= vlen;
+        const double S00 = l1 * vx * vx + l2 * vy * vy, S01 = (l1 - l2) * vx * vy, S11 = l1 * vy * vy + l2 * vx * vx;
+        const double A = 0.5 * A2;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double gi0 = b3[i] / A2, gi1 = c3[i] / A2, gj0 = b3[j] / A2, gj1 = c3[j] / A2;
+                a.tg[9 * (size_t)t + 3 * i + j] = (float)(A * (gi0 * (S00 * gj0 + S01 * gj1) + gi1 * (S01 * gj0 + S11 * gj1)));
+            }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int p2 = 0; p2 < 3; p2++) {
+            a.tkv[18 * (size_t)t + 6 * i + p2] = E[p2][0] * fl[6 * i] + E[p2][1] * fl[6 * i + 1] + E[p2][2] * fl[6 * i + 2];
+            a.tkv[18 * (size_t)t + 6 * i + 3 + p2] = E[p2][0] * fl[6 * i + 3] + E[p2][1] * fl[6 * i + 4] + E[p2][2] * fl[6 * i + 5];
+        }
+    a.ton[t] = 1;
+}
+
+// gather_part's triangle block (src/phys/frame_fem.cpp:948-985): corners (i, j) in world axes, times c, the
+// membrane tension's stiffness on the translations' diagonal
+__device__ void tri_block(const FemArgs& a, uint32_t t, int i, int j, double c, double h2, double* D) {
+    const float* te = a.te + 9 * (size_t)t;
+    const double E[3][3] = {{te[0], te[3], te[6]}, {te[1], te[4], te[7]}, {te[2], te[5], te[8]}};
+    const float* Km = a.Kf + (size_t)t * 162;
+    const float* Kp = Km + 81;
+    for (int ra = 0; ra < 2; ra++)
+        for (int cb = 0; cb < 2; cb++) {
+            double T[3][3];
+            for (int p = 0; p < 3; p++) {
+                const double E0 = E[p][0], E1 = E[p][1], E2 = E[p][2];
+#define M_(s2, t2) ((double)Km[(3 * i + (s2)) * 9 + 3 * j + (t2)])
+#define P_(s2, t2) ((double)Kp[(3 * i + (s2)) * 9 + 3 * j + (t2)])
+                if (ra == 0 && cb == 0) {
+                    T[p][0] = E0 * M_(0, 0) + E1 * M_(1, 0);
+                    T[p][1] = E0 * M_(0, 1) + E1 * M_(1, 1);
+                    T[p][2] = E2 * P_(0, 0);
+                } else if (ra == 0) {
+                    T[p][0] = E2 * P_(0, 1);
+                    T[p][1] = E2 * P_(0, 2);
+                    T[p][2] = E0 * M_(0, 2) + E1 * M_(1, 2);
+                } else if (cb == 0) {
+                    T[p][0] = E2 * M_(2, 0);
+                    T[p][1] = E2 * M_(2, 1);
+                    T[p][2] = E0 * P_(1, 0) + E1 * P_(2, 0);
+                } else {
+                    T[p][0] = E0 * P_(1, 1) + E1 * P_(2, 1);
+                    T[p][1] = E0 * P_(1, 2) + E1 * P_(2, 2);
+                    T[p][2] = E2 * M_(2, 2);
+                }
+#undef M_
+#undef P_
+            }
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) D[(3 * ra + p) * 6 + 3 * cb + q] += c * (T[p][0] * E[q][0] + T[p][1] * E[q][1] + T[p][2] * E[q][2]);
+        }
+    const double gg = h2 * a.tg[9 * (size_t)t + 3 * i + j];
+    if (gg != 0)
+        for (int p = 0; p < 3; p++) D[p * 7] += gg;
+}
+
+// Per node: its force (gravity, the load, the triangles' corners in triangle order), and for a free node its
+// diagonal block and its right side (gather_part, parts 0 and 1).
+__global__ void k_nodes(FemArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nn) return;
+    const int fi = a.fidx[i];
+    if (fi < 0) return;
+    f3 F = blg::mul(mk(a.g[0], a.g[1], a.g[2]), a.mass[i]);
+    if (a.loads) F = blg::add(F, mk(a.loads[3 * i], a.loads[3 * i + 1], a.loads[3 * i + 2]));
+    f3 T = mk(0, 0, 0);
+    const double h = a.h;
+    const double h2 = (double)a.theta * a.h * a.h, hd = (double)a.dissipation * a.h * a.h;
+    const double c = h2 + (double)a.sec.damping * h, cr = hd + (double)a.sec.damping * h;
+    double D[36];
+    for (int k = 0; k < 36; k++) D[k] = 0;
+    double rd[6] = {0, 0, 0, 0, 0, 0};
+    D[0] = D[7] = D[14] = a.mass[i];
+    D[21] = D[28] = D[35] = a.inertia[i];
+    for (uint32_t k = a.nc_off[i]; k < a.nc_off[i + 1]; k++) {
+        const uint32_t t = a.nc[k] / 3, corner = a.nc[k] % 3;
+        if (!a.ton[t]) continue;
+        F = blg::add(F, mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]));
+        T = blg::add(T, mk(a.tto[9 * t + 3 * corner], a.tto[9 * t + 3 * corner + 1], a.tto[9 * t + 3 * corner + 2]));
+        tri_block(a, t, (int)corner, (int)corner, c, h2, D);
+        if (cr != 0)
+            for (int j = 0; j < 6; j++) rd[j] -= cr * a.tkv[18 * (size_t)t + 6 * corner + j];
+    }
+    double* Dd = a.Dsys + 36 * (size_t)fi;
+    for (int k = 0; k < 36; k++) Dd[k] = D[k];
+    double* r = a.rhs + 6 * (size_t)fi;
+    r[0] = h * (double)F.x, r[1] = h * (double)F.y, r[2] = h * (double)F.z;
+    r[3] = h * (double)T.x, r[4] = h * (double)T.y, r[5] = h * (double)T.z;
+    for (int j = 0; j < 6; j++) r[j] += rd[j];
+}
+
+// Per pair of free nodes sharing a triangle: the lower block (later node's row, earlier node's column), its
+// triangles in triangle order.
+__global__ void k_pairs(FemArgs a) {
+    const uint32_t pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= a.npairs) return;
+    const uint4 pr = a.pairs[pi];
+    const double h2 = (double)a.theta * a.h * a.h;
+    const double c = h2 + (double)a.sec.damping * a.h;
+    double O[36];
+    for (int k = 0; k < 36; k++) O[k] = 0;
+    for (uint32_t k = pr.z; k < pr.w; k++) {
+        const uint4 cn = a.pcon[k];
+        if (a.ton[cn.x]) tri_block(a, cn.x, (int)cn.y, (int)cn.z, c, h2, O);
+    }
+    double* Od = a.Osys + 36 * (size_t)pi;
+    for (int k = 0; k < 36; k++) Od[k] = O[k];
+}
+
+// ------------------------------------------------------------------------------------------ the multifrontal factor
+struct Front {
+    uint64_t off;      // first double of its m x m matrix (column-major)
+    uint32_t m, c;     // dimension, pivot columns (6 per own node)
+    uint32_t own0, ns; // its own nodes: order positions own0 .. own0 + ns
+    uint32_t row0, nr; // its row nodes: rows[row0 .. row0 + nr] (order positions, ascending)
+    int32_t parent;
+    uint32_t level;
+    uint64_t woff;     // forward substitution: its 6 nr update of the right side
+};
+
+struct FacArgs {
+    double* F;             // the fronts
+    const Front* fr;
+    double* W;             // the fronts' right-side updates
+    int* fail;
+};
+
+// Scatter of the system into the fronts: entry = (source, 0 diagonal / 1 pair, front, row position, column position)
+struct Scat {
+    uint32_t src, kind, front, rpos, cpos;
+};
+
+__global__ void k_zero(double* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void k_clear_int(int* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+__global__ void k_scatter(FemArgs a, FacArgs f, const Scat* sc, uint32_t n) {
+    const uint32_t e = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, l = threadIdx.x & 63;
+    if (e >= n || l >= 36) return;
+    const Scat s = sc[e];
+    const Front fr = f.fr[s.front];
+    const double* B = s.kind == 0 ? a.Dsys + 36 * (size_t)s.src : a.Osys + 36 * (size_t)s.src;
+    const int p = l / 6, q = l % 6;
+    const uint64_t r = 6 * (uint64_t)s.rpos + p, c = 6 * (uint64_t)s.cpos + q;
+    if (s.kind == 0 && p < q) return; // (the lower triangle of a diagonal block)
+    f.F[fr.off + c * fr.m + r] = B[l];
+}
+
+// Extend-add: a child's update matrix (the trailing (6 nr)^2 of its front) into its parent's front. Task = (child,
+// row block of 64, column block of 64); map: the child's row nodes -> positions in the parent's front.
+struct EaTask {
+    uint32_t child, r0, c0, pad;
+};
+__global__ void k_extend_add(FacArgs f, const EaTask* tk, uint32_t n, const uint32_t* cmap) {
+    const uint32_t ti = blockIdx.x;
+    if (ti >= n) return;
+    const EaTask t = tk[ti];
+    const Front ch = f.fr[t.child];
+    const Front pa = f.fr[ch.parent];
+    const uint32_t M = 6 * ch.nr;
+    for (uint32_t k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+        const uint32_t r = t.r0 + k % 64, c = t.c0 + k / 64;
+        if (r >= M || c >= M || r < c) continue;
+        const double u = f.F[ch.off + (uint64_t)(ch.c + c) * ch.m + ch.c + r];
+        const uint32_t pr = 6 * cmap[ch.row0 + r / 6] + r % 6, pc = 6 * cmap[ch.row0 + c / 6] + c % 6;
+        f.F[pa.off + (uint64_t)pc * pa.m + pr] += u;
+    }
+}
+
+// Panel step k of a front: the diagonal tile's Cholesky (T x T, in LDS). Task = front.
+constexpr int kT = 32;
+__global__ __launch_bounds__(256) void k_potrf(FacArgs f, const uint32_t* tk, uint32_t n, uint32_t k) {
+    __shared__ double L[kT][kT + 1];
+    const uint32_t ti = blockIdx.x;
+    if (ti >= n) return;
+    const Front fr = f.fr[tk[ti]];
+    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
+    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
+        const uint32_t r = e % kT, c = e / kT;
+        L[r][c] = (r < w && c < w && r >= c) ? f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] : 0.0;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < w; j++) {
+        if (threadIdx.x == 0) {
+            const double s = L[j][j];
+            if (!(s > 0)) atomicOr(f.fail, 1);
+            L[j][j] = sqrt(s > 0 ? s : 1.0);
+        }
+        __syncthreads();
+        const double d = L[j][j];
+        for (uint32_t r = j + 1 + threadIdx.x; r < w; r += blockDim.x) L[r][j] /= d;
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
+            const uint32_t r = e % kT, c = e / kT;
+            if (c > j && r >= c && r < w) L[r][c] -= L[r][j] * L[c][j];
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
+        const uint32_t r = e % kT, c = e / kT;
+        if (r < w && c < w && r >= c) f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] = L[r][c];
+    }
+}
+
+// ... the panel below it: X := X L^-T, 256 rows a task. Task = (front, first row).
+__global__ __launch_bounds__(256) void k_trsm(FacArgs f, const uint2* tk, uint32_t n, uint32_t k) {
+    __shared__ double L[kT][kT + 1];
+    const uint32_t ti = blockIdx.x;
+    if (ti >= n) return;
+    const uint2 t = tk[ti];
+    const Front fr = f.fr[t.x];
+    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
+    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
+        const uint32_t r = e % kT, c = e / kT;
+        L[r][c] = (r < w && c < w && r >= c) ? f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] : 0.0;
+    }
+    __syncthreads();
+    const uint32_t row = t.y + threadIdx.x;
+    if (row >= fr.m) return;
+    double x[kT];
+#pragma unroll
+    for (int j = 0; j < kT; j++) x[j] = j < (int)w ? f.F[fr.off + (uint64_t)(k + j) * fr.m + row] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kT; j++) {
+        double s = x[j];
+#pragma unroll
+        for (int m = 0; m < j; m++) s -= L[j][m] * x[m];
+        x[j] = j < (int)w ? s / L[j][j] : 0.0; // (past the panel's width: L is zero there, the result unused)
+    }
+#pragma unroll
+    for (int j = 0; j < kT; j++)
+        if (j < (int)w) f.F[fr.off + (uint64_t)(k + j) * fr.m + row] = x[j];
+}
+
+// ... and the trailing update F[i, j] -= P_i P_j^T (the lower triangle of the rest of the front) on FP64 matrix
+// cores: a 64 x 64 tile a workgroup, 32 x 32 a wave (2 x 2 tiles of v_mfma_f64_16x16x4f64). Task = (front, row
+// tile start, column tile start). The instruction's lanes (measured on gfx950): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; item r of its result is D[l / 16 + 4 r][l % 16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_syrk(FacArgs f, const uint4* tk, uint32_t n, uint32_t k) {
+    __shared__ double Pa[64][kT + 1];
+    __shared__ double Pb[64][kT + 1];
+    const uint32_t ti = blockIdx.x;
+    if (ti >= n) return;
+    const uint4 t = tk[ti];
+    const Front fr = f.fr[t.x];
+    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
+    const uint32_t r0 = t.y, c0 = t.z;
+    for (uint32_t e = threadIdx.x; e < 64 * kT; e += blockDim.x) {
+        const uint32_t rr = e % 64, kk = e / 64;
+        const bool okk = kk < w;
+        Pa[rr][kk] = (okk && r0 + rr < fr.m) ? f.F[fr.off + (uint64_t)(k + kk) * fr.m + r0 + rr] : 0.0;
+        Pb[rr][kk] = (okk && c0 + rr < fr.m) ? f.F[fr.off + (uint64_t)(k + kk) * fr.m + c0 + rr] : 0.0;
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    dbl4 acc[2][2];
+    for (int a0 = 0; a0 < 2; a0++)
+        for (int b0 = 0; b0 < 2; b0++) acc[a0][b0] = (dbl4){0, 0, 0, 0};
+    const uint32_t li = lane & 15, lk = lane >> 4;
+    for (uint32_t kk = 0; kk < (uint32_t)kT; kk += 4) {
+        double av[2], bv[2];
+        for (int a0 = 0; a0 < 2; a0++) av[a0] = Pa[wr + a0 * 16 + li][kk + lk];
+        for (int b0 = 0; b0 < 2; b0++) bv[b0] = Pb[wc + b0 * 16 + li][kk + lk];
+        for (int a0 = 0; a0 < 2; a0++)
+            for (int b0 = 0; b0 < 2; b0++) acc[a0][b0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a0], bv[b0], acc[a0][b0], 0, 0, 0);
+    }
+    const uint32_t lo = k + w; // (the trailing region starts past the panel)
+    for (int a0 = 0; a0 < 2; a0++)
+        for (int b0 = 0; b0 < 2; b0++)
+            for (int r = 0; r < 4; r++) {
+                const uint32_t row = r0 + wr + a0 * 16 + lk + 4 * r, col = c0 + wc + b0 * 16 + li;
+                if (row < fr.m && col < fr.m && row >= col && col >= lo) f.F[fr.off + (uint64_t)col * fr.m + row] -= acc[a0][b0][r];
+            }
+}
+
+// ------------------------------------------------------------------------------------------ substitutions
+// Forward, one workgroup a front (a level at a time, children first): b = the own rows' right side plus the
+// children's updates, y = L11^-1 b into the global y, w = b_rows - L21 y for the parent.
+struct SolArgs {
+    const double* F;
+    const Front* fr;
+    double* W;
+    const uint32_t* rows;      // row nodes (order positions) of every front
+    const uint32_t* children;  // children of every front (CSR)
+    const uint32_t* ch_off;
+    const uint32_t* cmap;      // per child row node: its position in the parent's front
+    const uint32_t* ord2f;     // order position -> free index
+    const double* b;           // right side, 6 per free index
+    double* y;                 // 6 per free index
+    double* x;                 // 6 per free index
+};
+
+__global__ __launch_bounds__(1024) void k_forward(SolArgs s, const uint32_t* list, uint32_t n) {
+    extern __shared__ double sb[]; // m doubles
+    if (blockIdx.x >= n) return;
+    const uint32_t fi = list[blockIdx.x];
+    const Front fr = s.fr[fi];
+    const uint32_t m = fr.m, c = fr.c;
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        double v = 0;
+        if (r < c) v = s.b[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6];
+        sb[r] = v;
+    }
+    __syncthreads();
+    for (uint32_t k = s.ch_off[fi]; k < s.ch_off[fi + 1]; k++) {
+        const Front ch = s.fr[s.children[k]];
+        for (uint32_t r = threadIdx.x; r < 6 * ch.nr; r += blockDim.x)
+            sb[6 * s.cmap[ch.row0 + r / 6] + r % 6] += s.W[ch.woff + r];
+        __syncthreads();
+    }
+    // L11 y = b (column blocks of 32: the block's columns one after another, then the rows below)
+    for (uint32_t j0 = 0; j0 < c; j0 += 32) {
+        const uint32_t je = j0 + 32 < c ? j0 + 32 : c;
+        for (uint32_t j = j0; j < je; j++) {
+            if (threadIdx.x == 0) sb[j] = sb[j] / s.F[fr.off + (uint64_t)j * m + j];
+            __syncthreads();
+            const double yj = sb[j];
+            for (uint32_t r = j + 1 + threadIdx.x; r < je; r += blockDim.x) sb[r] -= s.F[fr.off + (uint64_t)j * m + r] * yj;
+            __syncthreads();
+        }
+        for (uint32_t r = je + threadIdx.x; r < m; r += blockDim.x) {
+            double acc = 0;
+            for (uint32_t j = j0; j < je; j++) acc += s.F[fr.off + (uint64_t)j * m + r] * sb[j];
+            sb[r] -= acc;
+        }
+        __syncthreads();
+    }
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        if (r < c) s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = sb[r];
+        else s.W[fr.woff + r - c] = sb[r];
+    }
+}
+
+// Backward, one workgroup a front (parents first): z = y - L21^T x_rows, x = L11^-T z.
+__global__ __launch_bounds__(1024) void k_backward(SolArgs s, const uint32_t* list, uint32_t n) {
+    extern __shared__ double sx[]; // m doubles: x of the own rows (solved here) and of the row nodes (known)
+    if (blockIdx.x >= n) return;
+    const uint32_t fi = list[blockIdx.x];
+    const Front fr = s.fr[fi];
+    const uint32_t m = fr.m, c = fr.c;
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        if (r < c) sx[r] = s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6];
+        else sx[r] = s.x[6 * (size_t)s.ord2f[s.rows[fr.row0 + (r - c) / 6]] + (r - c) % 6];
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < c; j += blockDim.x) {
+        double acc = 0;
+        for (uint32_t r = c; r < m; r++) acc += s.F[fr.off + (uint64_t)j * m + r] * sx[r];
+        sx[j] -= acc;
+    }
+    __syncthreads();
+    // L11^T x = z, column blocks from the last: the block's unknowns, then their share off the columns before
+    for (int j1 = (int)c; j1 > 0; j1 -= 32) {
+        const int j0 = j1 - 32 > 0 ? j1 - 32 : 0;
+        for (int j = j1 - 1; j >= j0; j--) {
+            if (threadIdx.x == 0) sx[j] = sx[j] / s.F[fr.off + (uint64_t)j * m + j];
+            __syncthreads();
+            const double xj = sx[j];
+            for (int r = j0 + (int)threadIdx.x; r < j; r += blockDim.x) sx[r] -= s.F[fr.off + (uint64_t)r * m + j] * xj;
+            __syncthreads();
+        }
+        for (int r = (int)threadIdx.x; r < j0; r += blockDim.x) {
+            double acc = 0;
+            for (int j = j0; j < j1; j++) acc += s.F[fr.off + (uint64_t)r * m + j] * sx[j];
+            sx[r] -= acc;
+        }
+        __syncthreads();
+    }
+    for (uint32_t r = threadIdx.x; r < c; r += blockDim.x) s.x[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = sx[r];
+}
+
+// ------------------------------------------------------------------------------------------ residual and update
+// r = A x - b over the assembled system (diagonal blocks, and each pair block both ways): per free node, its
+// pairs (CSR, both directions) - then the partial sums of |r|^2 and |b|^2 per workgroup.
+__global__ void k_residual(FemArgs a, const double* x, const uint32_t* adj_off, const uint2* adj, double* part) {
+    __shared__ double s1[256], s2[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0, bb = 0;
+    if (i < a.nf) {
+        double y[6];
+        const double* D = a.Dsys + 36 * (size_t)i;
+        for (int p = 0; p < 6; p++) {
+            double acc = 0;
+            for (int q = 0; q < 6; q++) acc += (p >= q ? D[p * 6 + q] : D[q * 6 + p]) * x[6 * i + q];
+            y[p] = acc;
+        }
+        for (uint32_t k = adj_off[i]; k < adj_off[i + 1]; k++) {
+            const uint2 e = adj[k]; // (pair, other free index | direction << 31)
+            const double* O = a.Osys + 36 * (size_t)e.x;
+            const uint32_t o = e.y & 0x7fffffffu;
+            const bool row_side = (e.y >> 31) != 0; // this node is the pair's row node: O x_other; else O^T x_other
+            for (int p = 0; p < 6; p++) {
+                double acc = 0;
+                for (int q = 0; q < 6; q++) acc += (row_side ? O[p * 6 + q] : O[q * 6 + p]) * x[6 * o + q];
+                y[p] += acc;
+            }
+        }
+        for (int p = 0; p < 6; p++) {
+            const double r = y[p] - a.rhs[6 * i + p];
+            rr += r * r;
+            bb += a.rhs[6 * i + p] * a.rhs[6 * i + p];
+        }
+    }
+    s1[threadIdx.x] = rr, s2[threadIdx.x] = bb;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) s1[threadIdx.x] += s1[threadIdx.x + off], s2[threadIdx.x] += s2[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = s1[0], part[2 * blockIdx.x + 1] = s2[0];
+}
+
+__global__ void k_sum_parts(const double* part, uint32_t n, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0, b = 0;
+    for (uint32_t k = 0; k < n; k++) a += part[2 * k], b += part[2 * k + 1];
+    out[0] = a, out[1] = b;
+}
+
+// the new velocities and rotations, then positions and orientations (src/phys/frame_fem.cpp:3003-3027, 3407-3416)
+__global__ void k_update(FemArgs a, const double* x) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nn) return;
+    const int fi = a.fidx[i];
+    if (fi < 0) {
+        a.w[3 * i] = a.w[3 * i + 1] = a.w[3 * i + 2] = 0;
+        return;
+    }
+    for (int j = 0; j < 3; j++) a.v[3 * i + j] += (float)x[6 * fi + j], a.w[3 * i + j] += (float)x[6 * fi + 3 + j];
+    for (int j = 0; j < 3; j++) a.xd[3 * i + j] += (double)a.h * a.v[3 * i + j];
+    const f3 wv = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
+    if (blg::dot(wv, wv) > 0) {
+        const float4 q0 = a.q[i];
+        const qf qn = qnorm(qmul(qexp(blg::mul(wv, a.h)), Q(q0.x, q0.y, q0.z, q0.w)));
+        a.q[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
+    }
+}
+
+} // namespace blf
+
+// ============================================================================================== host: the engine
+namespace {
+
+using namespace blf;
+
+#define FCHK(x)                                                                                                     \
+    do {                                                                                                            \
+        hipError_t e_ = (x);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                     \
+            blg::set_last_error((std::string(#x) + ": " + hipGetErrorString(e_)).c_str());                          \
+            return BL_ERR_HIP;                                                                                      \
+        }                                                                                                           \
+    } while (0)
+
+template <class T>
+struct Dv {
+    T* p = nullptr;
+    size_t n = 0;
+    int put(const std::vector<T>& v) {
+        if (p) (void)hipFree(p), p = nullptr;
+        n = v.size();
+        if (!n) return BL_OK;
+        if (hipMalloc((void**)&p, sizeof(T) * n) != hipSuccess) return BL_ERR_HIP;
+        return hipMemcpy(p, v.data(), sizeof(T) * n, hipMemcpyHostToDevice) == hipSuccess ? BL_OK : BL_ERR_HIP;
+    }
+    int alloc(size_t cnt) {
+        if (p) (void)hipFree(p), p = nullptr;
+        n = cnt;
+        if (!n) return BL_OK;
+        return hipMalloc((void**)&p, sizeof(T) * n) == hipSuccess ? BL_OK : BL_ERR_HIP;
+    }
+    ~Dv() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct Level {
+    std::vector<uint32_t> fronts;
+    uint32_t max_c = 0;
+};
+
+} // namespace
+
+struct bl_fem {
+    hipStream_t st = nullptr;
+    hipEvent_t ev[5] = {};
+    uint32_t nn = 0, nt = 0, nf = 0, npairs = 0;
+    Section sec{};
+    // state and topology
+    Dv<double> xd;
+    Dv<float> v, w, mass, inertia, X, Kf, te, tfo, tto, tg, loads;
+    Dv<float4> q, r0;
+    Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, part, sums;
+    Dv<uint8_t> fixed, ton;
+    Dv<uint4> tri, pairs, pcon;
+    Dv<uint32_t> nc_off, nc, ford, ord2f, rows, children, ch_off, cmap, adj_off;
+    Dv<int> fidx, fail;
+    Dv<uint2> adj;
+    // the factorization
+    Dv<Front> fronts;
+    Dv<double> F, W;
+    Dv<Scat> scat;
+    std::vector<Front> hfronts;
+    std::vector<Level> levels; // bottom-up
+    // task lists: per level the extend-add rounds, per (level, panel) the potrf / trsm / syrk tasks
+    struct Step {
+        uint32_t k;
+        uint64_t potrf, npotrf, trsm, ntrsm, syrk, nsyrk; // offsets into the task arrays
+    };
+    struct LevelPlan {
+        std::vector<std::pair<uint64_t, uint32_t>> ea; // per round: (offset, count)
+        std::vector<Step> steps;
+        uint64_t list_off;                              // the level's fronts in `flist`
+        uint32_t nlist, max_m;
+    };
+    std::vector<LevelPlan> plan;
+    Dv<EaTask> ea_tasks;
+    Dv<uint32_t> potrf_tasks, flist;
+    Dv<uint2> trsm_tasks;
+    Dv<uint4> syrk_tasks;
+    uint32_t nparts = 0;
+    double flops = 0;
+    int64_t nnz = 0;
+    int fronts_max = 0;
+    bool have_system = false;
+
+    ~bl_fem() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    FemArgs args(const float* dloads, float h, float theta, float dis, const float g[3]) const {
+        FemArgs a{};
+        a.nn = nn, a.nt = nt, a.nf = nf, a.sec = sec;
+        a.xd = xd.p, a.v = v.p, a.w = w.p, a.q = q.p, a.mass = mass.p, a.inertia = inertia.p, a.fixed = fixed.p;
+        a.tri = tri.p, a.X = X.p, a.r0 = r0.p, a.Kf = Kf.p, a.Kd = Kd.p;
+        a.te = te.p, a.tfo = tfo.p, a.tto = tto.p, a.tg = tg.p, a.tkv = tkv.p, a.ton = ton.p;
+        a.nc_off = nc_off.p, a.nc = nc.p, a.fidx = fidx.p, a.ford = ford.p;
+        a.Dsys = Dsys.p, a.rhs = rhs.p, a.Osys = Osys.p, a.pairs = pairs.p, a.pcon = pcon.p, a.npairs = npairs;
+        a.loads = dloads;
+        if (g) a.g[0] = g[0], a.g[1] = g[1], a.g[2] = g[2];
+        a.h = h, a.theta = theta, a.dissipation = dis;
+        return a;
+    }
+    FacArgs fac() const {
+        FacArgs f{};
+        f.F = F.p, f.fr = fronts.p, f.W = W.p, f.fail = fail.p;
+        return f;
+    }
+    SolArgs sol(const double* b) const {
+        SolArgs s{};
+        s.F = F.p, s.fr = fronts.p, s.W = W.p, s.rows = rows.p, s.children = children.p, s.ch_off = ch_off.p;
+        s.cmap = cmap.p, s.ord2f = ord2f.p, s.b = b, s.y = y.p, s.x = x.p;
+        return s;
+    }
+};
+
+namespace {
+
+// Nested dissection of the free nodes by their rest positions: the set split at the median of its longest extent,
+// the nodes of the smaller side touching the other side taken out as the separator; sets of at most `leaf` nodes
+// are leaves. Returns the supernodes (own nodes) with their children, in post-order.
+struct SNode {
+    std::vector<uint32_t> own;
+    std::vector<uint32_t> children;
+};
+
+void dissect(const std::vector<uint32_t>& S, const std::vector<float>& pos, const std::vector<std::vector<uint32_t>>& adj,
+             std::vector<int>& side, std::vector<SNode>& out, std::vector<uint32_t>& roots, uint32_t leaf) {
+    if (S.size() <= leaf) {
+        SNode s;
+        s.own = S;
+        out.push_back(s);
+        roots.push_back((uint32_t)out.size() - 1);
+        return;
+    }
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t i : S)
+        for (int k = 0; k < 3; k++) mn[k] = std::min(mn[k], pos[3 * i + k]), mx[k] = std::max(mx[k], pos[3 * i + k]);
+    int ax = 0;
+    for (int k = 1; k < 3; k++)
+        if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
+    std::vector<uint32_t> sorted = S;
+    std::stable_sort(sorted.begin(), sorted.end(), [&](uint32_t a, uint32_t b) { return pos[3 * a + ax] < pos[3 * b + ax]; });
+    const size_t half = sorted.size() / 2;
+    std::vector<uint32_t> L(sorted.begin(), sorted.begin() + half), R(sorted.begin() + half, sorted.end());
+    for (uint32_t i : L) side[i] = 1;
+    for (uint32_t i : R) side[i] = 2;
+    std::vector<uint32_t> bl, br;
+    for (uint32_t i : L)
+        for (uint32_t j : adj[i])
+            if (side[j] == 2) { bl.push_back(i); break; }
+    for (uint32_t i : R)
+        for (uint32_t j : adj[i])
+            if (side[j] == 1) { br.push_back(i); break; }
+    for (uint32_t i : S) side[i] = 0;
+    const bool use_left = bl.size() <= br.size();
+    std::vector<uint32_t>& sep = use_left ? bl : br;
+    std::vector<uint8_t> in_sep(0);
+    std::vector<uint32_t> L2, R2;
+    {
+        std::vector<uint32_t> sp = sep;
+        std::sort(sp.begin(), sp.end());
+        auto in = [&](uint32_t i) { return std::binary_search(sp.begin(), sp.end(), i); };
+        for (uint32_t i : L)
+            if (!(use_left && in(i))) L2.push_back(i);
+        for (uint32_t i : R)
+            if (!(!use_left && in(i))) R2.push_back(i);
+    }
+    std::vector<uint32_t> rl, rr;
+    if (!L2.empty()) dissect(L2, pos, adj, side, out, rl, leaf);
+    if (!R2.empty()) dissect(R2, pos, adj, side, out, rr, leaf);
+    if (sep.empty()) {
+        roots.insert(roots.end(), rl.begin(), rl.end());
+        roots.insert(roots.end(), rr.begin(), rr.end());
+        return;
+    }
+    SNode s;
+    s.own = sep;
+    s.children = rl;
+    s.children.insert(s.children.end(), rr.begin(), rr.end());
+    out.push_back(s);
+    roots.push_back((uint32_t)out.size() - 1);
+}
+
+int fem_build(bl_fem* f, const bl_fem_desc* d) {
+    const uint32_t nn = d->n_nodes, nt = d->n_tris;
+    f->nn = nn, f->nt = nt;
+    f->sec = Section{d->section.E, d->section.nu, d->section.rho, d->section.t, d->section.drill, d->section.damping};
+    // masses (ShellMesher::tri), rest shapes and frames (FemFrame::finalize), inertia (node_inertia)
+    std::vector<float> mass(nn, 0.0f), inertia(nn, 0.0f), X(6 * (size_t)nt, 0.0f), tmass(nt), area0(nt);
+    std::vector<float4> r0(3 * (size_t)nt);
+    std::vector<uint4> tri(nt);
+    const float mpa = d->section.rho * d->section.t;
+    auto P = [&](uint32_t i) { return mk(d->p[3 * i], d->p[3 * i + 1], d->p[3 * i + 2]); };
+    for (uint32_t k = 0; k < nt; k++) {
+        const uint32_t a = d->tris[3 * k], b = d->tris[3 * k + 1], c = d->tris[3 * k + 2];
+        if (a >= nn || b >= nn || c >= nn || a == b || b == c || a == c) {
+            blg::set_last_error("a triangle's corners are out of range or repeated");
+            return BL_ERR_ARG;
+        }
+        tri[k] = make_uint4(a, b, c, 0);
+        const f3 x0 = P(a), x1 = P(b), x2 = P(c);
+        const f3 u = blg::sub(x1, x0), vv = blg::sub(x2, x0);
+        const f3 nn3 = mk(u.y * vv.z - u.z * vv.y, u.z * vv.x - u.x * vv.z, u.x * vv.y - u.y * vv.x);
+        const float a2 = blg::length(nn3);
+        tmass[k] = mpa * (0.5f * a2);
+        if (!d->mass)
+            for (uint32_t v2 : {a, b, c}) mass[v2] += tmass[k] / 3.0f;
+        if (!(a2 > 1e-8f)) {
+            blg::set_last_error("a degenerate triangle");
+            return BL_ERR_ARG;
+        }
+        const f3 e3 = blg::divs(nn3, a2);
+        const f3 d01 = blg::sub(x1, x0);
+        const float l01 = blg::length(d01);
+        const f3 e1 = l01 > 1e-12f ? blg::mul(d01, 1.0f / l01) : mk(0, 0, 0);
+        const f3 e2 = mk(e3.y * e1.z - e3.z * e1.y, e3.z * e1.x - e3.x * e1.z, e3.x * e1.y - e3.y * e1.x);
+        const f3 cc = blg::divs(blg::add(blg::add(x0, x1), x2), 3.0f);
+        const f3 xs[3] = {x0, x1, x2};
+        for (int i = 0; i < 3; i++) X[6 * k + 2 * i] = blg::dot(blg::sub(xs[i], cc), e1), X[6 * k + 2 * i + 1] = blg::dot(blg::sub(xs[i], cc), e2);
+        area0[k] = 0.5f * a2;
+        const qf E0 = from_cols(e1, e2, e3);
+        for (int i = 0; i < 3; i++) {
+            const qf r = qnorm(qmul(qconj(Q(0, 0, 0, 1)), E0)); // (the nodes' orientations start at identity)
+            r0[3 * (size_t)k + i] = make_float4(r.x, r.y, r.z, r.w);
+        }
+    }
+    if (d->mass) std::memcpy(mass.data(), d->mass, sizeof(float) * nn);
+    for (uint32_t k = 0; k < nt; k++)
+        for (int c = 0; c < 3; c++) inertia[d->tris[3 * k + c]] += tmass[k] * area0[k] / 108.0f;
+    for (uint32_t i = 0; i < nn; i++) inertia[i] = std::max(inertia[i], std::max(1e-6f, mass[i] * 1e-4f));
+    // node -> incidences in triangle order
+    std::vector<uint32_t> nc_off(nn + 1, 0), nc(3 * (size_t)nt);
+    for (uint32_t k = 0; k < nt; k++)
+        for (int c = 0; c < 3; c++) nc_off[d->tris[3 * k + c] + 1]++;
+    for (uint32_t i = 0; i < nn; i++) nc_off[i + 1] += nc_off[i];
+    {
+        std::vector<uint32_t> fill(nc_off.begin(), nc_off.end() - 1);
+        for (uint32_t k = 0; k < nt; k++)
+            for (int c = 0; c < 3; c++) nc[fill[d->tris[3 * k + c]]++] = 3 * k + (uint32_t)c;
+    }
+    // the free nodes, their graph
+    std::vector<int> fidx(nn, -1);
+    std::vector<uint32_t> freen;
+    for (uint32_t i = 0; i < nn; i++)
+        if (!(d->fixed && d->fixed[i])) fidx[i] = (int)freen.size(), freen.push_back(i);
+    const uint32_t nf = (uint32_t)freen.size();
+    f->nf = nf;
+    std::vector<std::vector<uint32_t>> adj(nf);
+    for (uint32_t k = 0; k < nt; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const int a = fidx[d->tris[3 * k + i]], b = fidx[d->tris[3 * k + j]];
+                if (i != j && a >= 0 && b >= 0) adj[a].push_back((uint32_t)b);
+            }
+    for (auto& l : adj) std::sort(l.begin(), l.end()), l.erase(std::unique(l.begin(), l.end()), l.end());
+    // nested dissection, post-order: the order positions
+    std::vector<float> fpos(3 * (size_t)nf);
+    for (uint32_t i = 0; i < nf; i++)
+        for (int k = 0; k < 3; k++) fpos[3 * i + k] = d->p[3 * freen[i] + k];
+    std::vector<SNode> sn;
+    std::vector<uint32_t> roots;
+    std::vector<int> side(nf, 0);
+    std::vector<uint32_t> all(nf);
+    for (uint32_t i = 0; i < nf; i++) all[i] = i;
+    if (nf) dissect(all, fpos, adj, side, sn, roots, 8);
+    // (dissect appends children before parents: the vector is a post-order)
+    std::vector<uint32_t> order, opos(nf);
+    std::vector<uint32_t> sown0(sn.size());
+    for (size_t s = 0; s < sn.size(); s++) {
+        sown0[s] = (uint32_t)order.size();
+        for (uint32_t i : sn[s].own) opos[i] = (uint32_t)order.size(), order.push_back(i);
+    }
+    // the structure: a supernode's rows - later nodes adjacent to its own, and its children's rows past it
+    std::vector<int> parent(sn.size(), -1);
+    for (size_t s = 0; s < sn.size(); s++)
+        for (uint32_t c : sn[s].children) parent[c] = (int)s;
+    std::vector<std::vector<uint32_t>> srows(sn.size());
+    for (size_t s = 0; s < sn.size(); s++) {
+        const uint32_t last = sown0[s] + (uint32_t)sn[s].own.size();
+        std::vector<uint32_t> r;
+        for (uint32_t i : sn[s].own)
+            for (uint32_t j : adj[i])
+                if (opos[j] >= last) r.push_back(opos[j]);
+        for (uint32_t c : sn[s].children)
+            for (uint32_t o : srows[c])
+                if (o >= last) r.push_back(o);
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        srows[s] = r;
+    }
+    // fronts, their storage and levels (depth from the leaves)
+    std::vector<Front> fr(sn.size());
+    std::vector<uint32_t> rows, cmap;
+    std::vector<uint32_t> level(sn.size(), 0);
+    uint64_t off = 0, woff = 0;
+    double flops = 0;
+    int64_t nnz = 0;
+    int maxm = 0;
+    for (size_t s = 0; s < sn.size(); s++) {
+        Front& x = fr[s];
+        x.ns = (uint32_t)sn[s].own.size(), x.nr = (uint32_t)srows[s].size();
+        x.c = 6 * x.ns, x.m = 6 * (x.ns + x.nr);
+        x.own0 = sown0[s], x.row0 = (uint32_t)rows.size();
+        rows.insert(rows.end(), srows[s].begin(), srows[s].end());
+        x.parent = parent[s];
+        x.off = off, off += (uint64_t)x.m * x.m;
+        x.woff = woff, woff += 6 * (uint64_t)x.nr;
+        for (uint32_t c : sn[s].children) level[s] = std::max(level[s], level[c] + 1);
+        x.level = level[s];
+        const double c = x.c, r = 6.0 * x.nr;
+        flops += c * c * c / 3.0 + c * c * r + c * r * r;
+        nnz += (int64_t)(c * (c + 1) / 2 + c * r);
+        maxm = std::max(maxm, (int)x.m);
+    }
+    // each child's row nodes -> positions in its parent's front (own nodes first, then rows)
+    cmap.assign(rows.size(), 0);
+    for (size_t s = 0; s < sn.size(); s++) {
+        if (fr[s].parent < 0) continue;
+        const Front& p = fr[fr[s].parent];
+        for (uint32_t k = 0; k < fr[s].nr; k++) {
+            const uint32_t o = rows[fr[s].row0 + k];
+            uint32_t pos;
+            if (o >= p.own0 && o < p.own0 + p.ns) pos = o - p.own0;
+            else {
+                const auto it = std::lower_bound(rows.begin() + p.row0, rows.begin() + p.row0 + p.nr, o);
+                if (it == rows.begin() + p.row0 + p.nr || *it != o) {
+                    blg::set_last_error("symbolic factorization: a child's row is missing from its parent");
+                    return BL_ERR_STATE;
+                }
+                pos = p.ns + (uint32_t)(it - (rows.begin() + p.row0));
+            }
+            cmap[fr[s].row0 + k] = pos;
+        }
+    }
+    f->flops = flops, f->nnz = nnz, f->fronts_max = maxm;
+    // the pairs (lower blocks) and their triangles, in triangle order
+    std::vector<uint4> pairs, pcon;
+    {
+        std::vector<std::vector<uint4>> con; // per pair
+        std::vector<uint64_t> keys;
+        std::vector<uint4> raw; // (row free, col free, tri, corners)
+        for (uint32_t k = 0; k < nt; k++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    if (i == j) continue;
+                    const int a = fidx[d->tris[3 * k + i]], b = fidx[d->tris[3 * k + j]];
+                    if (a < 0 || b < 0) continue;
+                    if (opos[a] <= opos[b]) continue; // (row: the later node in the order)
+                    raw.push_back(make_uint4((uint32_t)a, (uint32_t)b, k, (uint32_t)(i | (j << 8))));
+                }
+        std::stable_sort(raw.begin(), raw.end(), [](const uint4& x, const uint4& y) {
+            return x.x != y.x ? x.x < y.x : x.y != y.y ? x.y < y.y : x.z < y.z;
+        });
+        for (size_t e = 0; e < raw.size();) {
+            size_t e2 = e;
+            while (e2 < raw.size() && raw[e2].x == raw[e].x && raw[e2].y == raw[e].y) e2++;
+            pairs.push_back(make_uint4(raw[e].x, raw[e].y, (uint32_t)pcon.size(), (uint32_t)(pcon.size() + (e2 - e))));
+            for (size_t q2 = e; q2 < e2; q2++) pcon.push_back(make_uint4(raw[q2].z, raw[q2].w & 0xff, raw[q2].w >> 8, 0));
+            e = e2;
+        }
+    }
+    f->npairs = (uint32_t)pairs.size();
+    // the scatter list: diagonal blocks and pair blocks into the fronts
+    std::vector<Scat> scat;
+    std::vector<uint32_t> front_of(nf);
+    for (size_t s = 0; s < sn.size(); s++)
+        for (uint32_t i : sn[s].own) front_of[i] = (uint32_t)s;
+    for (uint32_t i = 0; i < nf; i++) {
+        const uint32_t s = front_of[i];
+        scat.push_back(Scat{i, 0, s, opos[i] - fr[s].own0, opos[i] - fr[s].own0});
+    }
+    for (size_t pi = 0; pi < pairs.size(); pi++) {
+        const uint32_t rnode = pairs[pi].x, cnode = pairs[pi].y;
+        const uint32_t s = front_of[cnode];
+        const Front& x = fr[s];
+        const uint32_t cpos = opos[cnode] - x.own0;
+        uint32_t rpos;
+        if (opos[rnode] < x.own0 + x.ns) rpos = opos[rnode] - x.own0;
+        else {
+            const auto it = std::lower_bound(rows.begin() + x.row0, rows.begin() + x.row0 + x.nr, opos[rnode]);
+            rpos = x.ns + (uint32_t)(it - (rows.begin() + x.row0));
+        }
+        scat.push_back(Scat{(uint32_t)pi, 1, s, rpos, cpos});
+    }
+    // residual adjacency: per free node its pairs, both directions
+    std::vector<uint32_t> adj_off(nf + 1, 0);
+    std::vector<uint2> adjl;
+    {
+        std::vector<std::vector<uint2>> per(nf);
+        for (size_t pi = 0; pi < pairs.size(); pi++) {
+            per[pairs[pi].x].push_back(make_uint2((uint32_t)pi, pairs[pi].y | 0x80000000u));
+            per[pairs[pi].y].push_back(make_uint2((uint32_t)pi, pairs[pi].x));
+        }
+        for (uint32_t i = 0; i < nf; i++) {
+            adjl.insert(adjl.end(), per[i].begin(), per[i].end());
+            adj_off[i + 1] = (uint32_t)adjl.size();
+        }
+    }
+    // levels and task lists
+    uint32_t nlev = 0;
+    for (const Front& x : fr) nlev = std::max(nlev, x.level + 1);
+    f->levels.assign(nlev, Level());
+    for (size_t s = 0; s < fr.size(); s++) {
+        f->levels[fr[s].level].fronts.push_back((uint32_t)s);
+        f->levels[fr[s].level].max_c = std::max(f->levels[fr[s].level].max_c, fr[s].c);
+    }
+    std::vector<EaTask> ea;
+    std::vector<uint32_t> potrf, flist;
+    std::vector<uint2> trsm;
+    std::vector<uint4> syrk;
+    std::vector<uint32_t> children, ch_off(fr.size() + 1, 0);
+    for (size_t s = 0; s < fr.size(); s++) {
+        children.insert(children.end(), sn[s].children.begin(), sn[s].children.end());
+        ch_off[s + 1] = (uint32_t)children.size();
+    }
+    f->plan.assign(nlev, bl_fem::LevelPlan());
+    for (uint32_t L = 0; L < nlev; L++) {
+        bl_fem::LevelPlan& lp = f->plan[L];
+        const Level& lv = f->levels[L];
+        lp.list_off = flist.size(), lp.nlist = (uint32_t)lv.fronts.size(), lp.max_m = 0;
+        for (uint32_t s : lv.fronts) flist.push_back(s), lp.max_m = std::max(lp.max_m, fr[s].m);
+        // extend-add rounds: the r-th child of every front of the level
+        for (uint32_t rnd = 0;; rnd++) {
+            const uint64_t o = ea.size();
+            for (uint32_t s : lv.fronts) {
+                if (rnd >= sn[s].children.size()) continue;
+                const uint32_t ch = sn[s].children[rnd];
+                const uint32_t M = 6 * fr[ch].nr;
+                for (uint32_t r0 = 0; r0 < M; r0 += 64)
+                    for (uint32_t c0 = 0; c0 <= r0; c0 += 64) ea.push_back(EaTask{ch, r0, c0, 0});
+            }
+            if (ea.size() == o) {
+                bool any = false;
+                for (uint32_t s : lv.fronts) any |= rnd < sn[s].children.size();
+                if (!any) break;
+            }
+            lp.ea.push_back({o, (uint32_t)(ea.size() - o)});
+        }
+        for (uint32_t k = 0; k < lv.max_c; k += kT) {
+            bl_fem::Step stp{};
+            stp.k = k;
+            stp.potrf = potrf.size(), stp.trsm = trsm.size(), stp.syrk = syrk.size();
+            for (uint32_t s : lv.fronts) {
+                const Front& x = fr[s];
+                if (k >= x.c) continue;
+                potrf.push_back(s);
+                const uint32_t wdt = std::min<uint32_t>(kT, x.c - k), lo = k + wdt;
+                for (uint32_t r = lo; r < x.m; r += 256) trsm.push_back(make_uint2(s, r));
+                for (uint32_t r0 = lo; r0 < x.m; r0 += 64)
+                    for (uint32_t c0 = lo; c0 <= r0; c0 += 64) syrk.push_back(make_uint4(s, r0, c0, 0));
+            }
+            stp.npotrf = potrf.size() - stp.potrf, stp.ntrsm = trsm.size() - stp.trsm, stp.nsyrk = syrk.size() - stp.syrk;
+            lp.steps.push_back(stp);
+        }
+    }
+    // ord2f: order position -> free index
+    std::vector<uint32_t> ord2f(order.begin(), order.end()), ford(nf);
+    for (uint32_t i = 0; i < nf; i++) ford[i] = opos[i];
+    f->hfronts = fr;
+    // device arrays
+    std::vector<double> xd(3 * (size_t)nn);
+    for (uint32_t i = 0; i < 3 * nn; i++) xd[i] = d->p[i];
+    std::vector<float4> q(nn, make_float4(0, 0, 0, 1));
+    std::vector<uint8_t> fixed(nn, 0);
+    if (d->fixed) std::memcpy(fixed.data(), d->fixed, nn);
+    int rc;
+    if ((rc = f->xd.put(xd)) || (rc = f->v.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
+        (rc = f->w.put(std::vector<float>(3 * (size_t)nn, 0.0f))) || (rc = f->q.put(q)) || (rc = f->mass.put(mass)) ||
+        (rc = f->inertia.put(inertia)) || (rc = f->fixed.put(fixed)) || (rc = f->tri.put(tri)) || (rc = f->X.put(X)) ||
+        (rc = f->r0.put(r0)) || (rc = f->Kf.alloc(162 * (size_t)nt)) || (rc = f->Kd.alloc(324 * (size_t)nt)) ||
+        (rc = f->te.alloc(9 * (size_t)nt)) || (rc = f->tfo.alloc(9 * (size_t)nt)) || (rc = f->tto.alloc(9 * (size_t)nt)) ||
+        (rc = f->tg.alloc(9 * (size_t)nt)) || (rc = f->tkv.alloc(18 * (size_t)nt)) || (rc = f->ton.alloc(nt)) ||
+        (rc = f->nc_off.put(nc_off)) || (rc = f->nc.put(nc)) || (rc = f->fidx.put(fidx)) || (rc = f->ford.put(ford)) ||
+        (rc = f->ord2f.put(ord2f)) || (rc = f->Dsys.alloc(36 * (size_t)nf)) || (rc = f->rhs.alloc(6 * (size_t)nf)) ||
+        (rc = f->Osys.alloc(36 * pairs.size())) || (rc = f->pairs.put(pairs)) || (rc = f->pcon.put(pcon)) ||
+        (rc = f->y.alloc(6 * (size_t)nf)) || (rc = f->x.alloc(6 * (size_t)nf)) || (rc = f->loads.alloc(3 * (size_t)nn)) ||
+        (rc = f->rows.put(rows)) || (rc = f->cmap.put(cmap)) || (rc = f->children.put(children)) ||
+        (rc = f->ch_off.put(ch_off)) || (rc = f->fronts.put(fr)) || (rc = f->F.alloc(off)) || (rc = f->W.alloc(woff + 1)) ||
+        (rc = f->scat.put(scat)) || (rc = f->ea_tasks.put(ea)) || (rc = f->potrf_tasks.put(potrf)) ||
+        (rc = f->trsm_tasks.put(trsm)) || (rc = f->syrk_tasks.put(syrk)) || (rc = f->flist.put(flist)) ||
+        (rc = f->adj_off.put(adj_off)) || (rc = f->adj.put(adjl)) || (rc = f->fail.alloc(1)))
+        return rc;
+    f->nparts = (nf + 255) / 256;
+    if ((rc = f->part.alloc(2 * (size_t)f->nparts + 2)) || (rc = f->sums.alloc(2))) return rc;
+    FemArgs a = f->args(nullptr, 0, 0, 0, nullptr);
+    if (nt) hipLaunchKernelGGL(k_tri_k, dim3((nt + 63) / 64), dim3(64), 0, f->st, a);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(f->st));
+    return BL_OK;
+}
+
+int fem_factor_solve(bl_fem* f, float* ms_factor, float* ms_solve) {
+    FacArgs fa = f->fac();
+    const hipStream_t st = f->st;
+    FCHK(hipEventRecord(f->ev[1], st));
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, f->F.p, (uint64_t)f->F.n);
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, f->fail.p, 1u);
+    FemArgs a = f->args(nullptr, 0, 0, 0, nullptr);
+    const uint32_t nsc = (uint32_t)f->scat.n;
+    if (nsc) hipLaunchKernelGGL(k_scatter, dim3((nsc + 3) / 4), dim3(256), 0, st, a, fa, f->scat.p, nsc);
+    for (size_t L = 0; L < f->plan.size(); L++) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        for (const auto& r : lp.ea)
+            if (r.second) hipLaunchKernelGGL(k_extend_add, dim3(r.second), dim3(256), 0, st, fa, f->ea_tasks.p + r.first, r.second, f->cmap.p);
+        for (const bl_fem::Step& s : lp.steps) {
+            if (s.npotrf) hipLaunchKernelGGL(k_potrf, dim3((uint32_t)s.npotrf), dim3(256), 0, st, fa, f->potrf_tasks.p + s.potrf, (uint32_t)s.npotrf, s.k);
+            if (s.ntrsm) hipLaunchKernelGGL(k_trsm, dim3((uint32_t)s.ntrsm), dim3(256), 0, st, fa, f->trsm_tasks.p + s.trsm, (uint32_t)s.ntrsm, s.k);
+            if (s.nsyrk) hipLaunchKernelGGL(k_syrk, dim3((uint32_t)s.nsyrk), dim3(256), 0, st, fa, f->syrk_tasks.p + s.syrk, (uint32_t)s.nsyrk, s.k);
+        }
+    }
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(f->ev[2], st));
+    // substitutions: forward bottom-up, backward top-down
+    SolArgs so = f->sol(f->rhs.p);
+    for (size_t L = 0; L < f->plan.size(); L++) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_forward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
+    }
+    for (size_t L = f->plan.size(); L-- > 0;) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_backward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
+    }
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(f->ev[3], st));
+    FCHK(hipEventSynchronize(f->ev[3]));
+    if (ms_factor) FCHK(hipEventElapsedTime(ms_factor, f->ev[1], f->ev[2]));
+    if (ms_solve) FCHK(hipEventElapsedTime(ms_solve, f->ev[2], f->ev[3]));
+    return BL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
+    if (!out || !d || (d->n_nodes && !d->p) || (d->n_tris && !d->tris)) return BL_ERR_ARG;
+    *out = nullptr;
+    if (d->n_nodes == 0 || d->n_tris == 0) return BL_ERR_ARG;
+    bl_fem* f = new bl_fem();
+    if (hipStreamCreate(&f->st) != hipSuccess) {
+        delete f;
+        return BL_ERR_HIP;
+    }
+    for (hipEvent_t& e : f->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete f;
+            return BL_ERR_HIP;
+        }
+    const int rc = fem_build(f, d);
+    if (rc) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return BL_OK;
+}
+
+void bl_fem_destroy(bl_fem* f) { delete f; }
+
+int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
+                bl_fem_report* rep) {
+    if (!f || !g || !(h > 0)) return BL_ERR_ARG;
+    const hipStream_t st = f->st;
+    const float* dl = nullptr;
+    if (loads) {
+        FCHK(hipMemcpyAsync(f->loads.p, loads, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice, st));
+        dl = f->loads.p;
+    }
+    FemArgs a = f->args(dl, h, theta, dissipation, g);
+    FCHK(hipEventRecord(f->ev[0], st));
+    hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
+    if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
+    FCHK(hipGetLastError());
+    float msf = 0, mss = 0, msa = 0, msu = 0;
+    int rc = fem_factor_solve(f, &msf, &mss);
+    if (rc) return rc;
+    FCHK(hipEventElapsedTime(&msa, f->ev[0], f->ev[1]));
+    int fail = 0;
+    FCHK(hipMemcpy(&fail, f->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+    // the residual of the solve against the assembled system
+    hipLaunchKernelGGL(k_residual, dim3(f->nparts), dim3(256), 0, st, a, f->x.p, f->adj_off.p, f->adj.p, f->part.p);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, f->part.p, f->nparts, f->sums.p);
+    double sums[2] = {0, 0};
+    FCHK(hipMemcpyAsync(sums, f->sums.p, sizeof(sums), hipMemcpyDeviceToHost, st));
+    FCHK(hipEventRecord(f->ev[3], st));
+    if (!fail) hipLaunchKernelGGL(k_update, dim3((f->nn + 255) / 256), dim3(256), 0, st, a, f->x.p);
+    FCHK(hipEventRecord(f->ev[4], st));
+    FCHK(hipEventSynchronize(f->ev[4]));
+    FCHK(hipEventElapsedTime(&msu, f->ev[3], f->ev[4]));
+    f->have_system = true;
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->residual = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
+        rep->factor_flops = f->flops;
+        rep->assemble_ms = msa, rep->factor_ms = msf, rep->solve_ms = mss, rep->update_ms = msu;
+        rep->failures = fail ? 1 : 0;
+        rep->supernodes = (int32_t)f->hfronts.size();
+        rep->fronts_max = f->fronts_max;
+        rep->factor_nnz = f->nnz;
+    }
+    if (fail) {
+        blg::set_last_error("a pivot of the factorization is not positive");
+        return BL_ERR_STATE;
+    }
+    return BL_OK;
+}
+
+int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q) {
+    if (!f) return BL_ERR_ARG;
+    FCHK(hipStreamSynchronize(f->st));
+    if (x) FCHK(hipMemcpy(x, f->xd.p, sizeof(double) * 3 * f->nn, hipMemcpyDeviceToHost));
+    if (v) FCHK(hipMemcpy(v, f->v.p, sizeof(float) * 3 * f->nn, hipMemcpyDeviceToHost));
+    if (w) FCHK(hipMemcpy(w, f->w.p, sizeof(float) * 3 * f->nn, hipMemcpyDeviceToHost));
+    if (q) FCHK(hipMemcpy(q, f->q.p, sizeof(float) * 4 * f->nn, hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q) {
+    if (!f) return BL_ERR_ARG;
+    FCHK(hipStreamSynchronize(f->st));
+    if (x) FCHK(hipMemcpy(f->xd.p, x, sizeof(double) * 3 * f->nn, hipMemcpyHostToDevice));
+    if (v) FCHK(hipMemcpy(f->v.p, v, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
+    if (w) FCHK(hipMemcpy(f->w.p, w, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
+    if (q) FCHK(hipMemcpy(f->q.p, q, sizeof(float) * 4 * f->nn, hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_fem_last_system(bl_fem* f, double* A, double* rhs, double* x, uint32_t* nfo) {
+    if (!f) return BL_ERR_ARG;
+    if (!f->have_system) return BL_ERR_STATE;
+    FCHK(hipStreamSynchronize(f->st));
+    const uint32_t nf = f->nf;
+    if (nfo) *nfo = nf;
+    if (A) {
+        std::vector<double> D(36 * (size_t)nf), O(36 * (size_t)f->npairs);
+        std::vector<uint4> pairs(f->npairs);
+        FCHK(hipMemcpy(D.data(), f->Dsys.p, sizeof(double) * D.size(), hipMemcpyDeviceToHost));
+        if (f->npairs) {
+            FCHK(hipMemcpy(O.data(), f->Osys.p, sizeof(double) * O.size(), hipMemcpyDeviceToHost));
+            FCHK(hipMemcpy(pairs.data(), f->pairs.p, sizeof(uint4) * pairs.size(), hipMemcpyDeviceToHost));
+        }
+        const size_t N = 6 * (size_t)nf;
+        std::memset(A, 0, sizeof(double) * N * N);
+        for (uint32_t i = 0; i < nf; i++)
+            for (int p = 0; p < 6; p++)
+                for (int qq = 0; qq < 6; qq++) {
+                    const double vv = p >= qq ? D[36 * (size_t)i + p * 6 + qq] : D[36 * (size_t)i + qq * 6 + p];
+                    A[(6 * (size_t)i + p) * N + 6 * i + qq] = vv;
+                }
+        for (uint32_t k = 0; k < f->npairs; k++) {
+            const uint32_t r = pairs[k].x, c = pairs[k].y;
+            for (int p = 0; p < 6; p++)
+                for (int qq = 0; qq < 6; qq++) {
+                    A[(6 * (size_t)r + p) * N + 6 * c + qq] = O[36 * (size_t)k + p * 6 + qq];
+                    A[(6 * (size_t)c + qq) * N + 6 * r + p] = O[36 * (size_t)k + p * 6 + qq];
+                }
+        }
+    }
+    if (rhs) FCHK(hipMemcpy(rhs, f->rhs.p, sizeof(double) * 6 * nf, hipMemcpyDeviceToHost));
+    if (x) FCHK(hipMemcpy(x, f->x.p, sizeof(double) * 6 * nf, hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+int bl_fem_solve(bl_fem* f, const double* b, double* x) {
+    if (!f || !b || !x) return BL_ERR_ARG;
+    if (!f->have_system) return BL_ERR_STATE;
+    const hipStream_t st = f->st;
+    Dv<double> bd;
+    std::vector<double> hb(b, b + 6 * (size_t)f->nf);
+    int rc;
+    if ((rc = bd.put(hb))) return rc;
+    SolArgs so = f->sol(bd.p);
+    for (size_t L = 0; L < f->plan.size(); L++) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_forward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
+    }
+    for (size_t L = f->plan.size(); L-- > 0;) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_backward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
+    }
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(st));
+    FCHK(hipMemcpy(x, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K) {
+    if (!f || !K || tri >= f->nt) return BL_ERR_ARG;
+    FCHK(hipStreamSynchronize(f->st));
+    FCHK(hipMemcpy(K, f->Kd.p + 324 * (size_t)tri, sizeof(double) * 324, hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+} // extern "C"

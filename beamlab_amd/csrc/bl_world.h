@@ -102,5 +102,7 @@ int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t
 int launch_integrate(const KArgs& a, hipStream_t st, int parity, uint4* wstat); // wstat: ceil(nodes / 256) slots
 int launch_wstat_fold(const KArgs& a, hipStream_t st, uint4* wstat);
 int chunk_capacity(int block, int per);
+// the message bl_error_string returns for the calling thread's next error status
+void set_last_error(const char* msg);
 
 } // namespace blg

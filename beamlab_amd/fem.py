@@ -1,0 +1,119 @@
+"""Host-side mirror of the reference's shell FEM over the C ABI (bl_fem_*, include/beamlab_gpu.h).
+
+ShellMesher.grid follows src/phys/fem_shell.cpp:34-46 (a grid of nu x nv cells, two triangles a cell, the diagonals
+alternating with (i + j) & 1, each triangle's mass rho t A in thirds on its corners); ShellSection the elastic
+fields of src/phys/frame_fem.h:117-149; ShellBody.step one implicit step of FemFrame::solve
+(src/phys/frame_fem.cpp:2514-3027) for a body of shell triangles, computed on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+# the plate section of the implicit-FEM workload: aluminium, 1.5 mm (E, nu, rho, t, drill, damping)
+SECTION_PLATE = (70e9, 0.33, 2700.0, 1.5e-3, 0.05, 3e-4)
+
+
+def plate_mesh(nu, nv, size=1.0, y0=0.0):
+    """ShellMesher::grid over a size x size square in the x-z plane centred on the origin at height y0, its border
+    nodes clamped: (positions float32 [n, 3], triangles uint32 [t, 3], fixed uint8 [n])."""
+    origin = np.array([-0.5 * size, y0, -0.5 * size], np.float32)
+    du = np.array([np.float32(size) / np.float32(nu), 0, 0], np.float32)
+    dv = np.array([0, 0, np.float32(size) / np.float32(nv)], np.float32)
+    j, i = np.meshgrid(np.arange(nv + 1, dtype=np.float32), np.arange(nu + 1, dtype=np.float32), indexing="ij")
+    # origin + du i + dv j, float arithmetic in the reference's order
+    p = (origin[None, None, :] + du[None, None, :] * i[..., None]) + dv[None, None, :] * j[..., None]
+    p = p.reshape(-1, 3).astype(np.float32)
+    tris = []
+    for jj in range(nv):
+        for ii in range(nu):
+            n00 = jj * (nu + 1) + ii
+            n10, n01 = n00 + 1, n00 + nu + 1
+            n11 = n01 + 1
+            if (ii + jj) & 1:
+                tris += [(n00, n10, n11), (n00, n11, n01)]
+            else:
+                tris += [(n00, n10, n01), (n10, n11, n01)]
+    ii, jj = np.meshgrid(np.arange(nu + 1), np.arange(nv + 1))
+    fixed = ((ii == 0) | (jj == 0) | (ii == nu) | (jj == nv)).reshape(-1).astype(np.uint8)
+    return p, np.asarray(tris, np.uint32), fixed
+
+
+class ShellBody:
+    """A shell body stepped implicitly on the GPU: bl_fem_create / bl_fem_step / bl_fem_get_state."""
+
+    def __init__(self, p, tris, fixed=None, section=SECTION_PLATE, mass=None):
+        self._lib = abi.load()
+        self.p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        self.tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        self.nn, self.nt = len(self.p), len(self.tris)
+        self.fixed = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        self.mass = None if mass is None else np.ascontiguousarray(mass, np.float32)
+        d = abi.FemDesc()
+        d.n_nodes, d.n_tris = self.nn, self.nt
+        d.p, d.tris = abi.ptr(self.p.reshape(-1)), abi.ptr(self.tris.reshape(-1))
+        d.fixed = abi.ptr(self.fixed)
+        d.mass = abi.ptr(self.mass)
+        d.section = abi.ShellSection(*[float(x) for x in section])
+        h = ctypes.c_void_p()
+        abi.check(self._lib.bl_fem_create(ctypes.byref(h), ctypes.byref(d)), "bl_fem_create")
+        self._h = h
+        self.report = abi.FemReport()
+
+    @classmethod
+    def plate(cls, nu, nv, size=1.0, y0=0.0, section=SECTION_PLATE):
+        p, t, fx = plate_mesh(nu, nv, size, y0)
+        return cls(p, t, fx, section)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bl_fem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, loads=None, h=0.002, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """One implicit step; returns the report (residual, timings, factorization figures) as a dict."""
+        ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
+        gg = np.asarray(g, np.float32)
+        abi.check(self._lib.bl_fem_step(self._h, abi.ptr(ld), h, theta, dissipation, abi.ptr(gg),
+                                        ctypes.byref(self.report)), "bl_fem_step")
+        return self.report.as_dict()
+
+    def state(self):
+        x = np.zeros((self.nn, 3), np.float64)
+        v = np.zeros((self.nn, 3), np.float32)
+        w = np.zeros((self.nn, 3), np.float32)
+        q = np.zeros((self.nn, 4), np.float32)
+        abi.check(self._lib.bl_fem_get_state(self._h, abi.ptr(x.reshape(-1)), abi.ptr(v.reshape(-1)),
+                                             abi.ptr(w.reshape(-1)), abi.ptr(q.reshape(-1))), "bl_fem_get_state")
+        return x, v, w, q
+
+    def set_state(self, x=None, v=None, w=None, q=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dt).reshape(-1)
+                for a, dt in ((x, np.float64), (v, np.float32), (w, np.float32), (q, np.float32))]
+        abi.check(self._lib.bl_fem_set_state(self._h, *[abi.ptr(a) for a in arrs]), "bl_fem_set_state")
+
+    def last_system(self, dense=True):
+        nf = ctypes.c_uint32()
+        abi.check(self._lib.bl_fem_last_system(self._h, None, None, None, ctypes.byref(nf)), "bl_fem_last_system")
+        n6 = 6 * nf.value
+        A = np.zeros((n6, n6), np.float64) if dense else None
+        r = np.zeros(n6, np.float64)
+        x = np.zeros(n6, np.float64)
+        abi.check(self._lib.bl_fem_last_system(self._h, abi.ptr(None if A is None else A.reshape(-1)), abi.ptr(r),
+                                               abi.ptr(x), ctypes.byref(nf)), "bl_fem_last_system")
+        return A, r, x
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, np.float64).reshape(-1)
+        x = np.zeros_like(b)
+        abi.check(self._lib.bl_fem_solve(self._h, abi.ptr(b), abi.ptr(x)), "bl_fem_solve")
+        return x
+
+    def tri_k(self, k):
+        K = np.zeros((18, 18), np.float64)
+        abi.check(self._lib.bl_fem_tri_k(self._h, k, abi.ptr(K.reshape(-1))), "bl_fem_tri_k")
+        return K

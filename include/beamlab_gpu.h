@@ -288,6 +288,58 @@ int bl_time_step(bl_ctx* ctx, int32_t substeps, float* ms);
  * A frame with inter-body contacts times the force-phase kernel (beams and gather) of each substep. */
 int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kernel_ms, int32_t* launches);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Shell FEM: the implicit co-rotational step of a triangle shell (FemFrame's triangles: src/phys/frame_fem.h:154-181,
+ * FemFrame::solve src/phys/frame_fem.cpp:2514-3027). Replaces SoftBody::fem for a body made of shell triangles
+ * (elastic sections; no members, mounts, contacts or held steps). The system - 6x6 blocks per node pair, FP64 - is
+ * factored each step by a supernodal (multifrontal) Cholesky over a nested-dissection order, its dense fronts on
+ * FP64 matrix cores.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct bl_fem bl_fem;
+
+/* == ShellSection (src/phys/frame_fem.h:117-149), the fields the elastic step reads */
+typedef struct bl_shell_section {
+    float E, nu, rho, t;
+    float drill;   /* drilling spring share of G t A per corner (default 0.05) */
+    float damping; /* stiffness-proportional damping (s, default 3e-4) */
+} bl_shell_section;
+
+typedef struct bl_fem_desc {
+    uint32_t n_nodes, n_tris;
+    const float* p;         /* 3 per node: rest positions (the body's Node::p) */
+    const uint32_t* tris;   /* 3 per triangle (FrameTri::n), counter-clockwise about the rest normal */
+    const uint8_t* fixed;   /* per node: 1 = pinned (inv_mass 0) */
+    const float* mass;      /* per node (ShellMesher::tri's thirds), or null: from the section's rho t per area */
+    bl_shell_section section;
+} bl_fem_desc;
+
+typedef struct bl_fem_report {
+    double residual;        /* || A x - b || / || b || of the step's solve, FP64 on the device */
+    double factor_flops;    /* floating-point operations of the numeric factorization */
+    float assemble_ms, factor_ms, solve_ms, update_ms; /* device time of the phases (HIP events) */
+    int32_t failures;       /* a pivot block not positive definite (the step then leaves the state as it was) */
+    int32_t supernodes, fronts_max; /* the symbolic factorization: supernodes, largest front (scalar rows) */
+    int64_t factor_nnz;     /* nonzeros of L (scalars) */
+} bl_fem_report;
+
+/* FemFrame::finalize for the triangles (src/phys/frame_fem.cpp:259-276, 3075-3086) and the symbolic analysis. */
+int bl_fem_create(bl_fem** out, const bl_fem_desc* d);
+void bl_fem_destroy(bl_fem* f);
+/* One implicit step of h (theta, numerical dissipation, gravity g): element pass, assembly, factorization,
+ * substitution, update of v, w, positions and orientations. loads: 3 per node (N, host memory) or null. */
+int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
+                bl_fem_report* rep);
+/* The state: positions (double, 3 per node), v, w (3 per node), q (4 per node, x y z w); any pointer may be null. */
+int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q);
+int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q);
+/* The last step's linear system over the free nodes in node order: dense (6 nf)^2 row-major A (lower triangle and
+ * its mirror), rhs and solution (6 nf each); *nf the free nodes (testing; small meshes). */
+int bl_fem_last_system(bl_fem* f, double* A, double* rhs, double* x, uint32_t* nf);
+/* Solves A x = b with the last step's factorization (b, x: 6 per free node, node order, host memory). */
+int bl_fem_solve(bl_fem* f, const double* b, double* x);
+/* The element stiffness of triangle `tri` as the device built it (18 x 18 row-major, FP64). */
+int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
+
 #ifdef __cplusplus
 }
 #endif

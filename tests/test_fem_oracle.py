@@ -92,3 +92,14 @@ def test_step_system_symmetric_and_solved():
         assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
         assert np.linalg.norm(A @ x - r) <= 1e-12 * np.linalg.norm(r)
         assert np.allclose(x, np.linalg.solve(A, r), rtol=1e-9, atol=1e-12 * np.abs(x).max())
+
+
+def test_plate_mesh_matches_oracle():
+    """The package's plate builder (ShellMesher::grid) gives the oracle's nodes, triangles and clamps exactly."""
+    from beamlab_amd import fem
+    for nu, nv in ((6, 5), (9, 7), (4, 4)):
+        P = oracle.FemPlate(nu, nv)
+        tn, xd, fixed, *_ = P.topology()
+        p, t, fx = fem.plate_mesh(nu, nv)
+        assert np.array_equal(t, tn) and np.array_equal(fx, fixed)
+        assert np.array_equal(p.astype(np.float64), xd)
