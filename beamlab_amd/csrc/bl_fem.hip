@@ -472,14 +472,18 @@ struct Front {
     uint32_t row0, nr; // its row nodes: rows[row0 .. row0 + nr] (order positions, ascending)
     int32_t parent;
     uint32_t level;
-    uint64_t woff;     // forward substitution: its 6 nr update of the right side
+    uint64_t voff;     // its m-vector in the substitutions' buffer: the right side, then y (own rows) and the parent's
+                       // update (the rows below); in the backward pass, x of its own rows
+    uint32_t fb;       // its first flag: one per 64 own columns
+    uint32_t pad;
+    uint64_t off_l;    // first double of its factor columns (m x c, column-major) in L
 };
 
 struct FacArgs {
     double* F;             // the fronts
+    double* L;             // the factor's columns of every front (m x c, column-major)
     const Front* fr;
-    double* W;             // the fronts' right-side updates
-    int* fail;
+    int* fail;             // bit 1: a pivot not positive; bit 2: a substitution's wait timed out
 };
 
 // Scatter of the system into the fronts: entry = (source, 0 diagonal / 1 pair, front, row position, column position)
@@ -528,93 +532,143 @@ __global__ void k_extend_add(FacArgs f, const EaTask* tk, uint32_t n, const uint
     }
 }
 
-// Panel step k of a front: the diagonal tile's Cholesky (T x T, in LDS). Task = front.
+// The panel steps of a level, with a look-ahead of one: launch s holds, for every front of the level, the panel
+// factorization of columns k = 32 s .. k + w (w <= 32) and the trailing update of the panel before it (columns
+// k - 32 ..). A panel task (front, rows r0 .. r0 + 64 of the panel, r0 >= k)
+//   1. takes the previous panel's update off its diagonal tile and its rows (from L, the factor's stored columns:
+//      that update is the one part of the trailing update of the previous panel not left to the update tasks),
+//   2. factors the w x w diagonal tile in one wave (the reference's chol6 order: pivot, the column times 1 / pivot,
+//      the rank-1 update; a row per lane in registers, the finished column through LDS),
+//   3. solves its rows below the tile against it (X := X L^-T, a row per thread),
+//   4. stores its rows (the first task also the tile) as the factor's columns in L, a storage apart from the fronts.
+// Every panel task of a front computes the same tile from the same inputs with the same code, bit for bit. An update
+// task (front, 64 x 64 tile of columns >= lo2: past the next panel) takes P_r P_c^T of the previous panel off its
+// tile on the FP64 matrix cores. Panel tasks write L's columns k.. and read L's columns k - 32.., update tasks read
+// L's columns k - 32.. and write the front's columns >= lo2: no task of a launch reads what another writes.
+// (The factorization and the substitutions are not bit-exact with the reference - its order of the updates differs -
+// so their multiply-adds are fused.)
 constexpr int kT = 32;
-__global__ __launch_bounds__(256) void k_potrf(FacArgs f, const uint32_t* tk, uint32_t n, uint32_t k) {
-    __shared__ double L[kT][kT + 1];
-    const uint32_t ti = blockIdx.x;
-    if (ti >= n) return;
-    const Front fr = f.fr[tk[ti]];
-    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
-    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
-        const uint32_t r = e % kT, c = e / kT;
-        L[r][c] = (r < w && c < w && r >= c) ? f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] : 0.0;
-    }
-    __syncthreads();
-    for (uint32_t j = 0; j < w; j++) {
-        if (threadIdx.x == 0) {
-            const double s = L[j][j];
-            if (!(s > 0)) atomicOr(f.fail, 1);
-            L[j][j] = sqrt(s > 0 ? s : 1.0);
-        }
-        __syncthreads();
-        const double d = L[j][j];
-        for (uint32_t r = j + 1 + threadIdx.x; r < w; r += blockDim.x) L[r][j] /= d;
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
-            const uint32_t r = e % kT, c = e / kT;
-            if (c > j && r >= c && r < w) L[r][c] -= L[r][j] * L[c][j];
-        }
-        __syncthreads();
-    }
-    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
-        const uint32_t r = e % kT, c = e / kT;
-        if (r < w && c < w && r >= c) f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] = L[r][c];
-    }
-}
-
-// ... the panel below it: X := X L^-T, 256 rows a task. Task = (front, first row).
-__global__ __launch_bounds__(256) void k_trsm(FacArgs f, const uint2* tk, uint32_t n, uint32_t k) {
-    __shared__ double L[kT][kT + 1];
-    const uint32_t ti = blockIdx.x;
-    if (ti >= n) return;
-    const uint2 t = tk[ti];
-    const Front fr = f.fr[t.x];
-    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
-    for (uint32_t e = threadIdx.x; e < kT * kT; e += blockDim.x) {
-        const uint32_t r = e % kT, c = e / kT;
-        L[r][c] = (r < w && c < w && r >= c) ? f.F[fr.off + (uint64_t)(k + c) * fr.m + k + r] : 0.0;
-    }
-    __syncthreads();
-    const uint32_t row = t.y + threadIdx.x;
-    if (row >= fr.m) return;
-    double x[kT];
-#pragma unroll
-    for (int j = 0; j < kT; j++) x[j] = j < (int)w ? f.F[fr.off + (uint64_t)(k + j) * fr.m + row] : 0.0;
-#pragma unroll
-    for (int j = 0; j < kT; j++) {
-        double s = x[j];
-#pragma unroll
-        for (int m = 0; m < j; m++) s -= L[j][m] * x[m];
-        x[j] = j < (int)w ? s / L[j][j] : 0.0; // (past the panel's width: L is zero there, the result unused)
-    }
-#pragma unroll
-    for (int j = 0; j < kT; j++)
-        if (j < (int)w) f.F[fr.off + (uint64_t)(k + j) * fr.m + row] = x[j];
-}
-
-// ... and the trailing update F[i, j] -= P_i P_j^T (the lower triangle of the rest of the front) on FP64 matrix
-// cores: a 64 x 64 tile a workgroup, 32 x 32 a wave (2 x 2 tiles of v_mfma_f64_16x16x4f64). Task = (front, row
-// tile start, column tile start). The instruction's lanes (measured on gfx950): lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16]; item r of its result is D[l / 16 + 4 r][l % 16].
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void k_syrk(FacArgs f, const uint4* tk, uint32_t n, uint32_t k) {
-    __shared__ double Pa[64][kT + 1];
-    __shared__ double Pb[64][kT + 1];
-    const uint32_t ti = blockIdx.x;
-    if (ti >= n) return;
-    const uint4 t = tk[ti];
+
+__device__ __forceinline__ uint32_t panel_w(const Front& fr, uint32_t k) { return fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT; }
+
+__device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
+    double (*Td)[kT + 1] = (double (*)[kT + 1])sh;               // 32 x 33: the diagonal tile
+    double (*X)[kT + 1] = (double (*)[kT + 1])(sh + 32 * 33);    // 64 x 33: the task's rows
+    double (*Lr)[kT + 1] = (double (*)[kT + 1])(sh + 96 * 33);   // 64 x 33: their previous-panel factor
+    double (*Ld)[kT + 1] = (double (*)[kT + 1])(sh + 160 * 33);  // 32 x 33: the tile rows' previous-panel factor
+    double* inv = sh + 192 * 33;
+    double* colj = inv + kT;
     const Front fr = f.fr[t.x];
-    const uint32_t w = fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT;
+    const uint32_t m = fr.m, w = panel_w(fr, k), lo = k + w, r0 = t.y;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double* A = f.F + fr.off;
+    double* Lf = f.L + fr.off_l;
+    const bool prev = k > 0;
+    const uint32_t kp = k - kT;
+    // 1. loads: the tile (lower), the rows below it, and their previous-panel factor rows
+    for (uint32_t e = tid; e < kT * kT; e += 256) {
+        const uint32_t ii = e / kT, jj = e % kT;
+        Td[ii][jj] = (ii < w && jj <= ii) ? A[(uint64_t)(k + jj) * m + k + ii] : 0.0;
+        if (prev) Ld[ii][jj] = ii < w ? Lf[(uint64_t)(kp + jj) * m + k + ii] : 0.0;
+    }
+    for (uint32_t e = tid; e < 64 * kT; e += 256) {
+        const uint32_t ii = e % 64, jj = e / 64, r = r0 + ii;
+        const bool below = r < m && r >= lo;
+        X[ii][jj] = (below && jj < w) ? A[(uint64_t)(k + jj) * m + r] : 0.0;
+        if (prev) Lr[ii][jj] = below ? Lf[(uint64_t)(kp + jj) * m + r] : 0.0;
+    }
+    __syncthreads();
+    if (prev) {
+        // the tile's lower triangle and the rows, less the previous panel's part (4 partial sums a product)
+        for (uint32_t e = tid; e < 96 * kT; e += 256) {
+            const uint32_t ii = e / kT, jj = e % kT;
+            const bool tile = ii < kT;
+            const uint32_t i2 = tile ? ii : ii - kT;
+            if (jj >= w || (tile && (jj > i2 || i2 >= w))) continue;
+            const double* u = tile ? Ld[i2] : Lr[i2];
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+            for (int q = 0; q < kT; q += 4)
+                s0 += u[q] * Ld[jj][q], s1 += u[q + 1] * Ld[jj][q + 1], s2 += u[q + 2] * Ld[jj][q + 2], s3 += u[q + 3] * Ld[jj][q + 3];
+            const double sub = (s0 + s1) + (s2 + s3);
+            if (tile) Td[i2][jj] -= sub;
+            else X[i2][jj] -= sub;
+        }
+        __syncthreads();
+    }
+    // 2. the tile's Cholesky in wave 0
+    if (wv == 0) {
+        const uint32_t ri = lane & 31;
+        double a[kT];
+#pragma unroll
+        for (int j = 0; j < kT; j++) a[j] = Td[ri][j];
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < kT; j++) {
+            if (j < (int)w) {
+                const uint64_t u = __builtin_bit_cast(uint64_t, a[j]);
+                const double piv = __builtin_bit_cast(
+                    double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j));
+                bad |= !(piv > 0);
+                const double d = sqrt(piv > 0 ? piv : 1.0), id = 1.0 / d;
+                a[j] = (int)ri == j ? d : a[j] * id;
+                if (lane == 0) inv[j] = id;
+                if (lane < kT) colj[lane] = a[j];
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int c = j + 1; c < kT; c++) a[c] -= a[j] * colj[c];
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (lane < kT)
+#pragma unroll
+            for (int j = 0; j < kT; j++) Td[ri][j] = j <= (int)ri ? a[j] : 0.0;
+        if (bad && lane == 0 && r0 == k) atomicOr(f.fail, 1);
+    }
+    __syncthreads();
+    // 3. the rows: X := X L^-T, a row per thread (4 partial sums a dot product)
+    if (tid < 64 && r0 + tid < m && r0 + tid >= lo) {
+        double* x = X[tid];
+        for (uint32_t jj = 0; jj < w; jj++) {
+            double s0 = x[jj], s1 = 0, s2 = 0, s3 = 0;
+            uint32_t mm = 0;
+            for (; mm + 4 <= jj; mm += 4)
+                s0 -= Td[jj][mm] * x[mm], s1 -= Td[jj][mm + 1] * x[mm + 1], s2 -= Td[jj][mm + 2] * x[mm + 2], s3 -= Td[jj][mm + 3] * x[mm + 3];
+            for (; mm < jj; mm++) s0 -= Td[jj][mm] * x[mm];
+            x[jj] = ((s0 + s1) + (s2 + s3)) * inv[jj];
+        }
+    }
+    __syncthreads();
+    // 4. store
+    for (uint32_t e = tid; e < 64 * kT; e += 256) {
+        const uint32_t ii = e % 64, jj = e / 64, r = r0 + ii;
+        if (jj >= w || r >= m || r < k) continue;
+        if (r >= lo) Lf[(uint64_t)(k + jj) * m + r] = X[ii][jj];
+        else if (jj <= r - k) Lf[(uint64_t)(k + jj) * m + r] = Td[r - k][jj];
+    }
+}
+
+// C -= P_r P_c^T on the matrix cores: a 32 x 32 quarter a wave, 2 x 2 blocks of v_mfma_f64_16x16x4f64. The
+// instruction's lanes (measured on gfx950): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; item r of its
+// result is D[l / 16 + 4 r][l % 16].
+__device__ void update_task(const FacArgs& f, uint4 t, uint32_t kp, double* sh) {
+    double (*Pa)[kT + 1] = (double (*)[kT + 1])sh;
+    double (*Pb)[kT + 1] = (double (*)[kT + 1])(sh + 64 * 33);
+    const Front fr = f.fr[t.x];
+    const uint32_t m = fr.m, w = panel_w(fr, kp);
     const uint32_t r0 = t.y, c0 = t.z;
-    for (uint32_t e = threadIdx.x; e < 64 * kT; e += blockDim.x) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double* Lf = f.L + fr.off_l;
+    for (uint32_t e = tid; e < 64 * kT; e += 256) {
         const uint32_t rr = e % 64, kk = e / 64;
         const bool okk = kk < w;
-        Pa[rr][kk] = (okk && r0 + rr < fr.m) ? f.F[fr.off + (uint64_t)(k + kk) * fr.m + r0 + rr] : 0.0;
-        Pb[rr][kk] = (okk && c0 + rr < fr.m) ? f.F[fr.off + (uint64_t)(k + kk) * fr.m + c0 + rr] : 0.0;
+        Pa[rr][kk] = (okk && r0 + rr < m) ? Lf[(uint64_t)(kp + kk) * m + r0 + rr] : 0.0;
+        Pb[rr][kk] = (okk && c0 + rr < m) ? Lf[(uint64_t)(kp + kk) * m + c0 + rr] : 0.0;
     }
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
     dbl4 acc[2][2];
     for (int a0 = 0; a0 < 2; a0++)
@@ -627,22 +681,36 @@ __global__ __launch_bounds__(256) void k_syrk(FacArgs f, const uint4* tk, uint32
         for (int a0 = 0; a0 < 2; a0++)
             for (int b0 = 0; b0 < 2; b0++) acc[a0][b0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a0], bv[b0], acc[a0][b0], 0, 0, 0);
     }
-    const uint32_t lo = k + w; // (the trailing region starts past the panel)
+    double* C = f.F + fr.off;
     for (int a0 = 0; a0 < 2; a0++)
         for (int b0 = 0; b0 < 2; b0++)
             for (int r = 0; r < 4; r++) {
                 const uint32_t row = r0 + wr + a0 * 16 + lk + 4 * r, col = c0 + wc + b0 * 16 + li;
-                if (row < fr.m && col < fr.m && row >= col && col >= lo) f.F[fr.off + (uint64_t)col * fr.m + row] -= acc[a0][b0][r];
+                if (row < m && col < m && row >= col) C[(uint64_t)col * m + row] -= acc[a0][b0][r];
             }
 }
 
+// launch s of a level: panel tasks of panel k = 32 s (first), then the update tasks of panel k - 32
+__global__ __launch_bounds__(256) void k_fstep(FacArgs f, const uint2* pt, uint32_t np, const uint4* ut, uint32_t nu, uint32_t k) {
+#pragma clang fp contract(fast)
+    __shared__ double sh[192 * 33 + 2 * kT];
+    if (blockIdx.x < np) panel_task(f, pt[blockIdx.x], k, sh);
+    else if (blockIdx.x < np + nu) update_task(f, ut[blockIdx.x - np], k - kT, sh);
+}
+
 // ------------------------------------------------------------------------------------------ substitutions
-// Forward, one workgroup a front (a level at a time, children first): b = the own rows' right side plus the
-// children's updates, y = L11^-1 b into the global y, w = b_rows - L21 y for the parent.
+// Per level, the forward pass (children first) gathers each front's right side (its own rows, then its children's
+// updates in child order), then solves it in blocks of 64 rows: a workgroup per block, which takes the solved blocks
+// of the front's own columns in order as their flags come up (y of block jb is published with a device-scope flag),
+// solves its diagonal block in one wave and publishes its y - or, below the own columns, leaves the parent's update.
+// The backward pass (parents first) solves L11^T x = y - L21^T x_rows in blocks of 64 columns, the last block first.
+// Every sum runs in a fixed order: the result does not depend on the timing. Waits are bounded (spin_limit polls,
+// or another workgroup's error) and raise the error word, so every grid drains.
 struct SolArgs {
-    const double* F;
+    const double* F;           // the factor's columns (L)
     const Front* fr;
-    double* W;
+    double* V;                 // per front m: right side / y / parent's update (forward)
+    double* X;                 // per front m: x of the own rows (backward)
     const uint32_t* rows;      // row nodes (order positions) of every front
     const uint32_t* children;  // children of every front (CSR)
     const uint32_t* ch_off;
@@ -651,85 +719,195 @@ struct SolArgs {
     const double* b;           // right side, 6 per free index
     double* y;                 // 6 per free index
     double* x;                 // 6 per free index
+    uint32_t* fflag;           // per front per 64 own columns: forward y published (= epoch)
+    uint32_t* bflag;           // ... backward x published
+    const uint32_t* epoch;
+    int* err;
+    uint32_t spin_limit;
 };
 
-__global__ __launch_bounds__(1024) void k_forward(SolArgs s, const uint32_t* list, uint32_t n) {
-    extern __shared__ double sb[]; // m doubles
+__device__ __forceinline__ double ld_dev(const double* p) {
+    return __builtin_bit_cast(double, __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_dev(double* p, double v) {
+    __hip_atomic_store((uint64_t*)p, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// thread 0 of the workgroup: wait for *flag == epoch; false on an error (its own time-out or another workgroup's)
+__device__ bool wait_flag(const SolArgs& s, const uint32_t* flag, uint32_t epoch) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > s.spin_limit) {
+            __hip_atomic_fetch_or(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__global__ void k_epoch(uint32_t* e) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t v = *e + 1;
+        *e = v ? v : 1;
+    }
+}
+
+// the right side of every front of the level: own rows from b, zero below, plus the children's updates in order
+__global__ __launch_bounds__(256) void k_fgather(SolArgs s, const uint32_t* list, uint32_t n) {
     if (blockIdx.x >= n) return;
     const uint32_t fi = list[blockIdx.x];
     const Front fr = s.fr[fi];
-    const uint32_t m = fr.m, c = fr.c;
-    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
-        double v = 0;
-        if (r < c) v = s.b[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6];
-        sb[r] = v;
-    }
+    double* V = s.V + fr.voff;
+    for (uint32_t r = threadIdx.x; r < fr.m; r += blockDim.x)
+        V[r] = r < fr.c ? s.b[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] : 0.0;
     __syncthreads();
     for (uint32_t k = s.ch_off[fi]; k < s.ch_off[fi + 1]; k++) {
         const Front ch = s.fr[s.children[k]];
         for (uint32_t r = threadIdx.x; r < 6 * ch.nr; r += blockDim.x)
-            sb[6 * s.cmap[ch.row0 + r / 6] + r % 6] += s.W[ch.woff + r];
+            V[6 * s.cmap[ch.row0 + r / 6] + r % 6] += s.V[ch.voff + ch.c + r];
         __syncthreads();
-    }
-    // L11 y = b (column blocks of 32: the block's columns one after another, then the rows below)
-    for (uint32_t j0 = 0; j0 < c; j0 += 32) {
-        const uint32_t je = j0 + 32 < c ? j0 + 32 : c;
-        for (uint32_t j = j0; j < je; j++) {
-            if (threadIdx.x == 0) sb[j] = sb[j] / s.F[fr.off + (uint64_t)j * m + j];
-            __syncthreads();
-            const double yj = sb[j];
-            for (uint32_t r = j + 1 + threadIdx.x; r < je; r += blockDim.x) sb[r] -= s.F[fr.off + (uint64_t)j * m + r] * yj;
-            __syncthreads();
-        }
-        for (uint32_t r = je + threadIdx.x; r < m; r += blockDim.x) {
-            double acc = 0;
-            for (uint32_t j = j0; j < je; j++) acc += s.F[fr.off + (uint64_t)j * m + r] * sb[j];
-            sb[r] -= acc;
-        }
-        __syncthreads();
-    }
-    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
-        if (r < c) s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = sb[r];
-        else s.W[fr.woff + r - c] = sb[r];
     }
 }
 
-// Backward, one workgroup a front (parents first): z = y - L21^T x_rows, x = L11^-T z.
-__global__ __launch_bounds__(1024) void k_backward(SolArgs s, const uint32_t* list, uint32_t n) {
-    extern __shared__ double sx[]; // m doubles: x of the own rows (solved here) and of the row nodes (known)
+// task = (front, row block): rows R .. R + 64 of the front
+__global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, uint32_t n) {
+#pragma clang fp contract(fast)
+    __shared__ double ys[64];
+    __shared__ double red[4][64];
+    __shared__ double Ld[64][65];
+    __shared__ int sbad;
     if (blockIdx.x >= n) return;
-    const uint32_t fi = list[blockIdx.x];
-    const Front fr = s.fr[fi];
-    const uint32_t m = fr.m, c = fr.c;
-    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
-        if (r < c) sx[r] = s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6];
-        else sx[r] = s.x[6 * (size_t)s.ord2f[s.rows[fr.row0 + (r - c) / 6]] + (r - c) % 6];
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        double acc = 0;
-        for (uint32_t r = c; r < m; r++) acc += s.F[fr.off + (uint64_t)j * m + r] * sx[r];
-        sx[j] -= acc;
-    }
-    __syncthreads();
-    // L11^T x = z, column blocks from the last: the block's unknowns, then their share off the columns before
-    for (int j1 = (int)c; j1 > 0; j1 -= 32) {
-        const int j0 = j1 - 32 > 0 ? j1 - 32 : 0;
-        for (int j = j1 - 1; j >= j0; j--) {
-            if (threadIdx.x == 0) sx[j] = sx[j] / s.F[fr.off + (uint64_t)j * m + j];
-            __syncthreads();
-            const double xj = sx[j];
-            for (int r = j0 + (int)threadIdx.x; r < j; r += blockDim.x) sx[r] -= s.F[fr.off + (uint64_t)r * m + j] * xj;
-            __syncthreads();
-        }
-        for (int r = (int)threadIdx.x; r < j0; r += blockDim.x) {
-            double acc = 0;
-            for (int j = j0; j < j1; j++) acc += s.F[fr.off + (uint64_t)r * m + j] * sx[j];
-            sx[r] -= acc;
+    const uint2 t = tasks[blockIdx.x];
+    const Front fr = s.fr[t.x];
+    const uint32_t m = fr.m, c = fr.c, R = t.y * 64, nrow = m - R < 64u ? m - R : 64u;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, part = tid >> 6;
+    const uint32_t epoch = *s.epoch;
+    const double* F = s.F + fr.off_l;
+    double* V = s.V + fr.voff;
+    const double vinit = lane < nrow ? V[R + lane] : 0.0;
+    double pacc = 0;
+    const uint32_t ncol = R < c ? R : c, nblk = (ncol + 63) / 64;
+    if (tid == 0) sbad = 0;
+    for (uint32_t jb = 0; jb < nblk; jb++) {
+        if (tid == 0 && !sbad && !wait_flag(s, &s.fflag[fr.fb + jb], epoch)) sbad = 1;
+        __syncthreads();
+        if (sbad) break;
+        const uint32_t J = jb * 64, w = c - J < 64u ? c - J : 64u;
+        if (tid < w) ys[tid] = ld_dev(&V[J + tid]);
+        __syncthreads();
+        if (lane < nrow) {
+            const uint32_t j0 = part * 16, j1 = j0 + 16 < w ? j0 + 16 : w;
+            for (uint32_t j = j0; j < j1; j++) pacc += F[(uint64_t)(J + j) * m + R + lane] * ys[j];
         }
         __syncthreads();
     }
-    for (uint32_t r = threadIdx.x; r < c; r += blockDim.x) s.x[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = sx[r];
+    red[part][lane] = pacc;
+    if (R < c) {
+        const uint32_t w = c - R < 64u ? c - R : 64u;
+        for (uint32_t e = tid; e < 64 * 64; e += 256) {
+            const uint32_t i = e % 64, j = e / 64;
+            Ld[i][j] = (i < nrow && j < w && i >= j) ? F[(uint64_t)(R + j) * m + R + i] : 0.0;
+        }
+    }
+    __syncthreads();
+    if (part == 0 && !sbad) {
+        double acc = vinit - (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
+        if (R < c) {
+            const uint32_t w = c - R < 64u ? c - R : 64u;
+            for (uint32_t j = 0; j < w; j++) {
+                const double yj = __shfl(acc, (int)j) / Ld[j][j];
+                if (lane == j) acc = yj;
+                else if (lane > j) acc -= Ld[lane][j] * yj;
+            }
+        }
+        if (lane < nrow) {
+            const uint32_t r = R + lane;
+            st_dev(&V[r], acc);
+            if (r < c) s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = acc;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    if (tid == 0 && R < c && !sbad) __hip_atomic_store(&s.fflag[fr.fb + t.y], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// task = (front, own column block jb), the front's last block first
+__global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, uint32_t n) {
+#pragma clang fp contract(fast)
+    __shared__ double xs[256];
+    __shared__ double dots[64];
+    __shared__ double Ld[64][65];
+    __shared__ int sbad;
+    if (blockIdx.x >= n) return;
+    const uint2 t = tasks[blockIdx.x];
+    const Front fr = s.fr[t.x];
+    const uint32_t m = fr.m, c = fr.c, J = t.y * 64, w = c - J < 64u ? c - J : 64u;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, part = tid >> 6;
+    const uint32_t epoch = *s.epoch;
+    const double* F = s.F + fr.off_l;
+    double* X = s.X + fr.voff;
+    if (tid == 0) sbad = 0;
+    // this wave's 16 columns: partial sums of L[r][j] x_r per lane
+    double pj[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) pj[q] = 0;
+    // the rows below the own columns: x known (the ancestors'), 256 at a time
+    for (uint32_t r0 = c; r0 < m; r0 += 256) {
+        const uint32_t r = r0 + tid;
+        if (r < m) xs[tid] = s.x[6 * (size_t)s.ord2f[s.rows[fr.row0 + (r - c) / 6]] + (r - c) % 6];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint32_t j = part * 16 + q;
+            if (j < w)
+                for (uint32_t k = lane; k < 256 && r0 + k < m; k += 64) pj[q] += F[(uint64_t)(J + j) * m + r0 + k] * xs[k];
+        }
+        __syncthreads();
+    }
+    // the later own blocks, the last first, as their flags come up
+    const uint32_t nb = (c + 63) / 64;
+    for (uint32_t kb = nb; kb-- > t.y + 1;) {
+        if (tid == 0 && !sbad && !wait_flag(s, &s.bflag[fr.fb + kb], epoch)) sbad = 1;
+        __syncthreads();
+        if (sbad) break;
+        const uint32_t K = kb * 64, wk = c - K < 64u ? c - K : 64u;
+        if (tid < wk) xs[tid] = ld_dev(&X[K + tid]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint32_t j = part * 16 + q;
+            if (j < w && lane < wk) pj[q] += F[(uint64_t)(J + j) * m + K + lane] * xs[lane];
+        }
+        __syncthreads();
+    }
+    // one sum per column (a fixed tree over the lanes)
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        double v = pj[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) dots[part * 16 + q] = v;
+    }
+    for (uint32_t e = tid; e < 64 * 64; e += 256) {
+        const uint32_t i = e % 64, j = e / 64;
+        Ld[i][j] = (i < w && j < w && i >= j) ? F[(uint64_t)(J + j) * m + J + i] : 0.0;
+    }
+    __syncthreads();
+    if (part == 0 && !sbad) {
+        double acc = lane < w ? s.V[fr.voff + J + lane] - dots[lane] : 0.0;
+        for (uint32_t j = w; j-- > 0;) {
+            const double xj = __shfl(acc, (int)j) / Ld[j][j];
+            if (lane == j) acc = xj;
+            else if (lane < j) acc -= Ld[j][lane] * xj;
+        }
+        if (lane < w) {
+            const uint32_t r = J + lane;
+            st_dev(&X[r], acc);
+            s.x[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = acc;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    if (tid == 0 && !sbad) __hip_atomic_store(&s.bflag[fr.fb + t.y], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------------------------------------ residual and update
@@ -861,25 +1039,29 @@ struct bl_fem {
     Dv<uint2> adj;
     // the factorization
     Dv<Front> fronts;
-    Dv<double> F, W;
+    Dv<double> F, Lbuf, V, Xs;
+    Dv<uint32_t> fflag, bflag, epoch;
+    Dv<uint2> fs_tasks, bs_tasks;
     Dv<Scat> scat;
     std::vector<Front> hfronts;
     std::vector<Level> levels; // bottom-up
     // task lists: per level the extend-add rounds, per (level, panel) the potrf / trsm / syrk tasks
     struct Step {
         uint32_t k;
-        uint64_t potrf, npotrf, trsm, ntrsm, syrk, nsyrk; // offsets into the task arrays
+        uint64_t pfac, npfac, syrk, nsyrk; // its tasks in pfac_tasks, syrk_tasks
     };
     struct LevelPlan {
         std::vector<std::pair<uint64_t, uint32_t>> ea; // per round: (offset, count)
         std::vector<Step> steps;
         uint64_t list_off;                              // the level's fronts in `flist`
         uint32_t nlist, max_m;
+        uint64_t fs_off, bs_off;                        // the substitutions' block tasks
+        uint32_t nfs, nbs;
     };
     std::vector<LevelPlan> plan;
     Dv<EaTask> ea_tasks;
-    Dv<uint32_t> potrf_tasks, flist;
-    Dv<uint2> trsm_tasks;
+    Dv<uint32_t> flist;
+    Dv<uint2> pfac_tasks;
     Dv<uint4> syrk_tasks;
     uint32_t nparts = 0;
     double flops = 0;
@@ -909,13 +1091,14 @@ struct bl_fem {
     }
     FacArgs fac() const {
         FacArgs f{};
-        f.F = F.p, f.fr = fronts.p, f.W = W.p, f.fail = fail.p;
+        f.F = F.p, f.L = Lbuf.p, f.fr = fronts.p, f.fail = fail.p;
         return f;
     }
     SolArgs sol(const double* b) const {
         SolArgs s{};
-        s.F = F.p, s.fr = fronts.p, s.W = W.p, s.rows = rows.p, s.children = children.p, s.ch_off = ch_off.p;
-        s.cmap = cmap.p, s.ord2f = ord2f.p, s.b = b, s.y = y.p, s.x = x.p;
+        s.F = Lbuf.p, s.fr = fronts.p, s.V = V.p, s.X = Xs.p, s.rows = rows.p, s.children = children.p;
+        s.ch_off = ch_off.p, s.cmap = cmap.p, s.ord2f = ord2f.p, s.b = b, s.y = y.p, s.x = x.p;
+        s.fflag = fflag.p, s.bflag = bflag.p, s.epoch = epoch.p, s.err = fail.p, s.spin_limit = 1u << 20;
         return s;
     }
 };
@@ -1099,7 +1282,8 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     std::vector<Front> fr(sn.size());
     std::vector<uint32_t> rows, cmap;
     std::vector<uint32_t> level(sn.size(), 0);
-    uint64_t off = 0, woff = 0;
+    uint64_t off = 0, voff = 0, loff = 0;
+    uint32_t fb = 0;
     double flops = 0;
     int64_t nnz = 0;
     int maxm = 0;
@@ -1111,7 +1295,9 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         rows.insert(rows.end(), srows[s].begin(), srows[s].end());
         x.parent = parent[s];
         x.off = off, off += (uint64_t)x.m * x.m;
-        x.woff = woff, woff += 6 * (uint64_t)x.nr;
+        x.off_l = loff, loff += (uint64_t)x.m * x.c;
+        x.voff = voff, voff += x.m;
+        x.fb = fb, fb += (x.c + 63) / 64;
         for (uint32_t c : sn[s].children) level[s] = std::max(level[s], level[c] + 1);
         x.level = level[s];
         const double c = x.c, r = 6.0 * x.nr;
@@ -1212,9 +1398,10 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         f->levels[fr[s].level].max_c = std::max(f->levels[fr[s].level].max_c, fr[s].c);
     }
     std::vector<EaTask> ea;
-    std::vector<uint32_t> potrf, flist;
-    std::vector<uint2> trsm;
+    std::vector<uint32_t> flist;
+    std::vector<uint2> pfac;
     std::vector<uint4> syrk;
+    std::vector<uint2> fs_tasks, bs_tasks;
     std::vector<uint32_t> children, ch_off(fr.size() + 1, 0);
     for (size_t s = 0; s < fr.size(); s++) {
         children.insert(children.end(), sn[s].children.begin(), sn[s].children.end());
@@ -1243,21 +1430,30 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
             }
             lp.ea.push_back({o, (uint32_t)(ea.size() - o)});
         }
-        for (uint32_t k = 0; k < lv.max_c; k += kT) {
+        lp.fs_off = fs_tasks.size(), lp.bs_off = bs_tasks.size();
+        for (uint32_t s : lv.fronts) {
+            for (uint32_t rb = 0; rb * 64 < fr[s].m; rb++) fs_tasks.push_back(make_uint2(s, rb));
+            for (uint32_t jb = (fr[s].c + 63) / 64; jb-- > 0;) bs_tasks.push_back(make_uint2(s, jb));
+        }
+        lp.nfs = (uint32_t)(fs_tasks.size() - lp.fs_off), lp.nbs = (uint32_t)(bs_tasks.size() - lp.bs_off);
+        // launch s: the panel tasks of panel k = 32 s, the update tasks of panel k - 32 (columns past panel k)
+        for (uint32_t k = 0; k < lv.max_c + kT; k += kT) {
             bl_fem::Step stp{};
             stp.k = k;
-            stp.potrf = potrf.size(), stp.trsm = trsm.size(), stp.syrk = syrk.size();
+            stp.pfac = pfac.size(), stp.syrk = syrk.size();
             for (uint32_t s : lv.fronts) {
                 const Front& x = fr[s];
-                if (k >= x.c) continue;
-                potrf.push_back(s);
-                const uint32_t wdt = std::min<uint32_t>(kT, x.c - k), lo = k + wdt;
-                for (uint32_t r = lo; r < x.m; r += 256) trsm.push_back(make_uint2(s, r));
-                for (uint32_t r0 = lo; r0 < x.m; r0 += 64)
-                    for (uint32_t c0 = lo; c0 <= r0; c0 += 64) syrk.push_back(make_uint4(s, r0, c0, 0));
+                if (k < x.c)
+                    for (uint32_t r0 = k; r0 < x.m; r0 += 64) pfac.push_back(make_uint2(s, r0));
+                if (k >= kT && k - kT < x.c) {
+                    const uint32_t kp = k - kT, lop = kp + std::min<uint32_t>(kT, x.c - kp);
+                    const uint32_t lo2 = lop < x.c ? lop + std::min<uint32_t>(kT, x.c - lop) : lop;
+                    for (uint32_t c0 = lo2; c0 < x.m; c0 += 64)
+                        for (uint32_t r0 = c0; r0 < x.m; r0 += 64) syrk.push_back(make_uint4(s, r0, c0, 0));
+                }
             }
-            stp.npotrf = potrf.size() - stp.potrf, stp.ntrsm = trsm.size() - stp.trsm, stp.nsyrk = syrk.size() - stp.syrk;
-            lp.steps.push_back(stp);
+            stp.npfac = pfac.size() - stp.pfac, stp.nsyrk = syrk.size() - stp.syrk;
+            if (stp.npfac + stp.nsyrk) lp.steps.push_back(stp);
         }
     }
     // ord2f: order position -> free index
@@ -1282,9 +1478,12 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         (rc = f->Osys.alloc(36 * pairs.size())) || (rc = f->pairs.put(pairs)) || (rc = f->pcon.put(pcon)) ||
         (rc = f->y.alloc(6 * (size_t)nf)) || (rc = f->x.alloc(6 * (size_t)nf)) || (rc = f->loads.alloc(3 * (size_t)nn)) ||
         (rc = f->rows.put(rows)) || (rc = f->cmap.put(cmap)) || (rc = f->children.put(children)) ||
-        (rc = f->ch_off.put(ch_off)) || (rc = f->fronts.put(fr)) || (rc = f->F.alloc(off)) || (rc = f->W.alloc(woff + 1)) ||
-        (rc = f->scat.put(scat)) || (rc = f->ea_tasks.put(ea)) || (rc = f->potrf_tasks.put(potrf)) ||
-        (rc = f->trsm_tasks.put(trsm)) || (rc = f->syrk_tasks.put(syrk)) || (rc = f->flist.put(flist)) ||
+        (rc = f->ch_off.put(ch_off)) || (rc = f->fronts.put(fr)) || (rc = f->F.alloc(off)) || (rc = f->V.alloc(voff)) ||
+        (rc = f->Xs.alloc(voff)) || (rc = f->fflag.put(std::vector<uint32_t>(fb, 0u))) ||
+        (rc = f->bflag.put(std::vector<uint32_t>(fb, 0u))) || (rc = f->epoch.put(std::vector<uint32_t>(1, 0u))) ||
+        (rc = f->fs_tasks.put(fs_tasks)) || (rc = f->bs_tasks.put(bs_tasks)) ||
+        (rc = f->scat.put(scat)) || (rc = f->ea_tasks.put(ea)) || (rc = f->pfac_tasks.put(pfac)) || (rc = f->syrk_tasks.put(syrk)) ||
+        (rc = f->flist.put(flist)) || (rc = f->Lbuf.alloc(loff)) ||
         (rc = f->adj_off.put(adj_off)) || (rc = f->adj.put(adjl)) || (rc = f->fail.alloc(1)))
         return rc;
     f->nparts = (nf + 255) / 256;
@@ -1294,6 +1493,23 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(f->st));
     return BL_OK;
+}
+
+// L L^T x = b (b on the device, 6 per free index; x into f->x): forward by levels from the leaves, backward from the
+// root
+void enqueue_solve(bl_fem* f, const double* b) {
+    const hipStream_t st = f->st;
+    SolArgs so = f->sol(b);
+    hipLaunchKernelGGL(k_epoch, dim3(1), dim3(64), 0, st, f->epoch.p);
+    for (size_t L = 0; L < f->plan.size(); L++) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_fgather, dim3(lp.nlist), dim3(256), 0, st, so, f->flist.p + lp.list_off, lp.nlist);
+        hipLaunchKernelGGL(k_fsolve, dim3(lp.nfs), dim3(256), 0, st, so, f->fs_tasks.p + lp.fs_off, lp.nfs);
+    }
+    for (size_t L = f->plan.size(); L-- > 0;) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_bsolve, dim3(lp.nbs), dim3(256), 0, st, so, f->bs_tasks.p + lp.bs_off, lp.nbs);
+    }
 }
 
 int fem_factor_solve(bl_fem* f, float* ms_factor, float* ms_solve) {
@@ -1310,23 +1526,14 @@ int fem_factor_solve(bl_fem* f, float* ms_factor, float* ms_solve) {
         for (const auto& r : lp.ea)
             if (r.second) hipLaunchKernelGGL(k_extend_add, dim3(r.second), dim3(256), 0, st, fa, f->ea_tasks.p + r.first, r.second, f->cmap.p);
         for (const bl_fem::Step& s : lp.steps) {
-            if (s.npotrf) hipLaunchKernelGGL(k_potrf, dim3((uint32_t)s.npotrf), dim3(256), 0, st, fa, f->potrf_tasks.p + s.potrf, (uint32_t)s.npotrf, s.k);
-            if (s.ntrsm) hipLaunchKernelGGL(k_trsm, dim3((uint32_t)s.ntrsm), dim3(256), 0, st, fa, f->trsm_tasks.p + s.trsm, (uint32_t)s.ntrsm, s.k);
-            if (s.nsyrk) hipLaunchKernelGGL(k_syrk, dim3((uint32_t)s.nsyrk), dim3(256), 0, st, fa, f->syrk_tasks.p + s.syrk, (uint32_t)s.nsyrk, s.k);
+            hipLaunchKernelGGL(k_fstep, dim3((uint32_t)(s.npfac + s.nsyrk)), dim3(256), 0, st, fa, f->pfac_tasks.p + s.pfac,
+                               (uint32_t)s.npfac, f->syrk_tasks.p + s.syrk, (uint32_t)s.nsyrk, s.k);
         }
     }
     FCHK(hipGetLastError());
     FCHK(hipEventRecord(f->ev[2], st));
     // substitutions: forward bottom-up, backward top-down
-    SolArgs so = f->sol(f->rhs.p);
-    for (size_t L = 0; L < f->plan.size(); L++) {
-        const bl_fem::LevelPlan& lp = f->plan[L];
-        hipLaunchKernelGGL(k_forward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
-    }
-    for (size_t L = f->plan.size(); L-- > 0;) {
-        const bl_fem::LevelPlan& lp = f->plan[L];
-        hipLaunchKernelGGL(k_backward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
-    }
+    enqueue_solve(f, f->rhs.p);
     FCHK(hipGetLastError());
     FCHK(hipEventRecord(f->ev[3], st));
     FCHK(hipEventSynchronize(f->ev[3]));
@@ -1407,7 +1614,8 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
         rep->factor_nnz = f->nnz;
     }
     if (fail) {
-        blg::set_last_error("a pivot of the factorization is not positive");
+        blg::set_last_error((fail & 1) ? "a pivot of the factorization is not positive"
+                                       : "a substitution's wait for a block timed out");
         return BL_ERR_STATE;
     }
     return BL_OK;
@@ -1477,17 +1685,16 @@ int bl_fem_solve(bl_fem* f, const double* b, double* x) {
     std::vector<double> hb(b, b + 6 * (size_t)f->nf);
     int rc;
     if ((rc = bd.put(hb))) return rc;
-    SolArgs so = f->sol(bd.p);
-    for (size_t L = 0; L < f->plan.size(); L++) {
-        const bl_fem::LevelPlan& lp = f->plan[L];
-        hipLaunchKernelGGL(k_forward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
-    }
-    for (size_t L = f->plan.size(); L-- > 0;) {
-        const bl_fem::LevelPlan& lp = f->plan[L];
-        hipLaunchKernelGGL(k_backward, dim3(lp.nlist), dim3(1024), sizeof(double) * lp.max_m, st, so, f->flist.p + lp.list_off, lp.nlist);
-    }
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, f->fail.p, 1u);
+    enqueue_solve(f, bd.p);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(st));
+    int fail = 0;
+    FCHK(hipMemcpy(&fail, f->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (fail) {
+        blg::set_last_error("a substitution's wait for a block timed out");
+        return BL_ERR_STATE;
+    }
     FCHK(hipMemcpy(x, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToHost));
     return BL_OK;
 }

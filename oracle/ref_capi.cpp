@@ -6,6 +6,7 @@
 // them (copying the records as they are, so both sides start from identical bits) and runs the reference's own
 // World::step_substeps (src/phys/world.cpp:368).
 #include "phys/world.h"
+#include "phys/fem_shell.h"
 #include "phys/softbody.h"
 #include "phys/static_world.h"
 #include "core/jobs.h"
@@ -241,6 +242,40 @@ int blref_tri_stiffness(const float p[9], const float sec[6], double* K, float* 
     b.fem.tri_stiffness(0, reinterpret_cast<double(*)[18]>(K));
     for (int i = 0; i < 3; i++) X[2 * i] = b.fem.tris[0].X[i][0], X[2 * i + 1] = b.fem.tris[0].X[i][1];
     return 0;
+}
+
+// The clamped plate of the implicit-FEM workload built the reference's way (ShellMesher::grid, src/phys/fem_shell.cpp:
+// 34-46; border nodes NF_FIXED; ShellMesher::finish; FemFrame::finalize) and its implicit step as the reference's
+// fembench times it (tests/test_physics.cpp:3486-3494): clear_forces(g) plus the loads, FemFrame::compute_forces,
+// FemFrame::solve. Positions are not advanced (the world's integration does that in the reference).
+void* blref_fem_plate_new(int nu, int nv, float size, float y0, const float sec[6]) {
+    auto* b = new SoftBody();
+    ShellSection s;
+    s.E = sec[0], s.nu = sec[1], s.rho = sec[2], s.t = sec[3], s.drill = sec[4], s.damping = sec[5];
+    const uint16_t si = b->fem.add_shell_section(s);
+    ShellMesher m(*b);
+    const vec3 origin(-0.5f * size, y0, -0.5f * size), du(size / (float)nu, 0, 0), dv(0, 0, size / (float)nv);
+    const std::vector<uint32_t> id = m.grid(origin, du, dv, nu, nv, si, false);
+    for (int j = 0; j <= nv; j++)
+        for (int i = 0; i <= nu; i++)
+            if (i == 0 || j == 0 || i == nu || j == nv) b->info[id[(size_t)j * (nu + 1) + i]].flags |= NF_FIXED;
+    m.finish();
+    b->finalize();
+    b->fem.finalize(*b);
+    return b;
+}
+
+void blref_fem_plate_free(void* h) { delete static_cast<SoftBody*>(h); }
+
+// one step; returns the frame's solve failures so far
+int64_t blref_fem_plate_step(void* h, const float* loads, float dt, float theta, float dissipation, const float g[3]) {
+    SoftBody& b = *static_cast<SoftBody*>(h);
+    b.clear_forces(vec3(g[0], g[1], g[2]));
+    if (loads)
+        for (size_t i = 0; i < b.nodes.size(); i++) b.force[i] += vec3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]);
+    b.fem.compute_forces(b);
+    b.fem.solve(b, dt, dt, theta, dissipation);
+    return (int64_t)b.fem.solve_failures;
 }
 
 } // extern "C"
