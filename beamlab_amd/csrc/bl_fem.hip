@@ -540,7 +540,8 @@ __global__ void k_extend_add(FacArgs f, const EaTask* tk, uint32_t n, const uint
 //   2. factors the w x w diagonal tile in one wave (the reference's chol6 order: pivot, the column times 1 / pivot,
 //      the rank-1 update; a row per lane in registers, the finished column through LDS),
 //   3. solves its rows below the tile against it (X := X L^-T, a row per thread),
-//   4. stores its rows (the first task also the tile) as the factor's columns in L, a storage apart from the fronts.
+//   4. stores its rows (the first task also the tile) as the factor's columns in L, a storage apart from the fronts;
+//      L's diagonal holds 1 / L_jj, all the substitutions need of it.
 // Every panel task of a front computes the same tile from the same inputs with the same code, bit for bit. An update
 // task (front, 64 x 64 tile of columns >= lo2: past the next panel) takes P_r P_c^T of the previous panel off its
 // tile on the FP64 matrix cores. Panel tasks write L's columns k.. and read L's columns k - 32.., update tasks read
@@ -647,7 +648,8 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
         const uint32_t ii = e % 64, jj = e / 64, r = r0 + ii;
         if (jj >= w || r >= m || r < k) continue;
         if (r >= lo) Lf[(uint64_t)(k + jj) * m + r] = X[ii][jj];
-        else if (jj <= r - k) Lf[(uint64_t)(k + jj) * m + r] = Td[r - k][jj];
+        else if (jj < r - k) Lf[(uint64_t)(k + jj) * m + r] = Td[r - k][jj];
+        else if (jj == r - k) Lf[(uint64_t)(k + jj) * m + r] = inv[jj]; // (the diagonal stored as 1 / L_jj)
     }
 }
 
@@ -726,6 +728,13 @@ struct SolArgs {
     uint32_t spin_limit;
 };
 
+// lane l's value of v (l uniform across the wave)
+__device__ __forceinline__ double readlane_u(double v, uint32_t l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ double ld_dev(const double* p) {
     return __builtin_bit_cast(double, __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
@@ -815,7 +824,7 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
         if (R < c) {
             const uint32_t w = c - R < 64u ? c - R : 64u;
             for (uint32_t j = 0; j < w; j++) {
-                const double yj = __shfl(acc, (int)j) / Ld[j][j];
+                const double yj = readlane_u(acc, j) * Ld[j][j]; // (L's diagonal holds 1 / L_jj)
                 if (lane == j) acc = yj;
                 else if (lane > j) acc -= Ld[lane][j] * yj;
             }
@@ -895,7 +904,7 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     if (part == 0 && !sbad) {
         double acc = lane < w ? s.V[fr.voff + J + lane] - dots[lane] : 0.0;
         for (uint32_t j = w; j-- > 0;) {
-            const double xj = __shfl(acc, (int)j) / Ld[j][j];
+            const double xj = readlane_u(acc, j) * Ld[j][j]; // (L's diagonal holds 1 / L_jj)
             if (lane == j) acc = xj;
             else if (lane < j) acc -= Ld[j][lane] * xj;
         }
