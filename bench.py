@@ -12,6 +12,15 @@ per launch / its average duration, from HIP events around each launch on the ste
 reference's CPU step timed on a bounded sample of the same workload on this host's cores. The beam path alone (the
 same lattice on the ground without contact geometry, where the step runs as one persistent kernel) is reported
 beside it.
+
+The metric's second half, implicit FEM solves/s, is the object "implicit_fem": the co-rotational shell plate of
+256 x 256 cells (66 049 nodes, 65 025 free, 390 150 unknowns, 131 072 triangles; E 70 GPa, nu 0.33, 1.5 mm,
+clamped edges), 200 implicit steps of 2 ms under seeded random point loads (256 free nodes a step, N(0, 20 N) per
+component, uploaded with the step), each step the element pass, the 6x6-block assembly, the supernodal Cholesky
+(FP64 fronts on the matrix cores), the substitutions and the update, its residual measured on the device. Its
+roofline is the factorization's: FP64 flops of the numeric factorization / its device time, against the FP64
+matrix-core peak. Its CPU baseline is the compiled reference's FemFrame step on the same plate (one thread, one
+step after a warm step), run in a child process beside the GPU legs.
 """
 import argparse
 import json
@@ -23,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+FEM_CELLS = 256
 METRIC = "beam updates/s at 2 kHz substep; implicit FEM solves/s at 1e-6 residual"
 FRAME = 33
 # the forms the contact step can take: its force kernel's workgroup size and beam ends per thread
@@ -63,6 +74,74 @@ def cpu_baseline(bodies, sample_substeps, threads):
     return {"value": len(bodies[0].beams) * n / dt, "unit": "beam updates/s", "cores": threads, "kind": kind,
             "sample": f"{n} substeps of the same scene (lattice and 2000 balls) in {frames} frames of {FRAME} "
                       f"({dt:.1f} s)"}
+
+
+def fem_loads(nn, fixed, steps, seed=7, per_step=256, sigma=20.0):
+    """The seeded random point loads of the plate workload: per step `per_step` free nodes, N(0, sigma) N."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    free = np.nonzero(fixed == 0)[0]
+    out = np.zeros((steps, nn, 3), np.float32)
+    for s in range(steps):
+        idx = rng.choice(free, per_step, replace=False)
+        out[s, idx] = rng.normal(0.0, sigma, (per_step, 3)).astype(np.float32)
+    return out
+
+
+def fem_leg(steps, warmup, log):
+    """Implicit FEM solves/s on the plate (see the module docstring)."""
+    import numpy as np
+    from beamlab_amd import fem
+    G = fem.ShellBody.plate(FEM_CELLS, FEM_CELLS)
+    loads = fem_loads(G.nn, G.fixed, warmup + steps)
+    for s in range(warmup):
+        G.step(loads[s], 0.002)
+    reps = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        reps.append(G.step(loads[warmup + s], 0.002))
+        if (s + 1) % 50 == 0:
+            log(f"fem step {s + 1}/{steps}")
+    wall = time.perf_counter() - t0
+    G.close()
+    mean = lambda k: float(np.mean([r[k] for r in reps]))
+    flops = reps[0]["factor_flops"]
+    fac_s = mean("factor_ms") / 1e3
+    achieved = flops / fac_s / 1e12
+    return {
+        "value": steps / wall, "unit": "implicit FEM solves/s", "ms_per_step": wall / steps * 1e3, "steps": steps,
+        "warmup": warmup, "dtype": "f64",
+        "config": {"workload": f"co-rotational shell plate {FEM_CELLS}x{FEM_CELLS} cells, clamped edges, seeded "
+                               "random point loads", "nodes": int(G.nn), "triangles": int(G.nt),
+                   "unknowns": 6 * int((G.fixed == 0).sum()), "h_s": 0.002, "theta": 0.5, "dissipation": 0.02},
+        "residual_max": max(r["residual"] for r in reps), "failures": sum(r["failures"] for r in reps),
+        "phases_ms": {k: mean(k + "_ms") for k in ("assemble", "factor", "solve", "update")},
+        "factorization": {"supernodes": reps[0]["supernodes"], "largest_front": reps[0]["fronts_max"],
+                          "factor_nnz": reps[0]["factor_nnz"], "flops": flops},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": "k_fstep (all panel steps)",
+                     "alg_flops_per_factorization": flops, "factor_ms": fac_s * 1e3},
+    }
+
+
+def cpu_fem_worker(cells, steps):
+    """The compiled reference's FemFrame step on the plate (child process of the bench): prints one JSON line."""
+    from oracle import oracle
+    from beamlab_amd import fem
+    p, t, fx = fem.plate_mesh(cells, cells)
+    loads = fem_loads(len(p), fx, steps + 1)
+    t0 = time.perf_counter()
+    P = oracle.RefPlate(cells, cells)
+    build = time.perf_counter() - t0
+    P.step(loads[0], 0.002)  # warm step
+    t1 = time.perf_counter()
+    for s in range(steps):
+        P.step(loads[1 + s], 0.002)
+    dt = time.perf_counter() - t1
+    print(json.dumps({"value": steps / dt, "unit": "implicit FEM solves/s", "cores": 1, "kind": "reference",
+                      "sample": f"{steps} step(s) of the same {cells}x{cells} plate after one warm step: "
+                                f"FemFrame::compute_forces + FemFrame::solve, one thread ({dt:.1f} s; the "
+                                f"reference's analysis and first factorization took {build:.1f} s)"}), flush=True)
 
 
 def copy_calibration(mib):
@@ -123,7 +202,14 @@ def main():
     ap.add_argument("--no-beam-only", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner's cache")
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a plain copy of this many MiB (profiling)")
+    ap.add_argument("--fem-steps", type=int, default=200)
+    ap.add_argument("--no-fem", action="store_true")
+    ap.add_argument("--no-beams", action="store_true", help="the implicit-FEM leg only")
+    ap.add_argument("--cpu-fem-worker", type=int, nargs=2, metavar=("CELLS", "STEPS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_fem_worker:
+        cpu_fem_worker(*args.cpu_fem_worker)
+        return
     if args.copy_calib:
         copy_calibration(args.copy_calib)
         return
@@ -134,6 +220,26 @@ def main():
 
     from beamlab_amd import scenes
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    # the reference's FEM step on the CPU, beside the GPU legs (a child process, one core)
+    fem_cpu = None
+    if not args.no_cpu and not args.no_fem:
+        import subprocess
+        fem_cpu = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-fem-worker", str(FEM_CELLS), "1"],
+                                   stdout=subprocess.PIPE, text=True)
+    fem_out = None
+    if not args.no_fem:
+        fem_out = fem_leg(args.fem_steps, 3, log)
+        log(f"fem: {fem_out['value']:.1f} solves/s")
+    if args.no_beams:
+        out = {"metric": METRIC, "value": fem_out["value"], "unit": fem_out["unit"], "n_gpus": 1,
+               "steps": args.fem_steps, "warmup": 3, "ms_per_step": fem_out["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": fem_out["config"],
+               "roofline": fem_out["roofline"], "implicit_fem": fem_out}
+        if fem_cpu is not None:
+            fem_out["cpu_baseline"] = wait_cpu_fem(fem_cpu, log)
+            out["cpu_baseline"] = fem_out["cpu_baseline"]
+        print(json.dumps(out), flush=True)
+        return
     tune = tuned(lambda: build_world(scenes.config2_scene()), CONTACT_CANDIDATES, args.retune, log)
     bodies = scenes.config2_scene()
     nb, nn = len(bodies[0].beams), sum(len(b.nodes) for b in bodies)
@@ -187,9 +293,32 @@ def main():
     if not args.no_beam_only:
         out["beam_path_only"] = beam_path_only(4000, args.retune, log)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(scenes.config2_scene(), args.cpu_sample, threads)
+        threads = min(os.cpu_count() or 1, 16) - (1 if fem_cpu is not None else 0)
+        out["cpu_baseline"] = cpu_baseline(scenes.config2_scene(), args.cpu_sample, max(1, threads))
+    if fem_out is not None:
+        if fem_cpu is not None:
+            fem_out["cpu_baseline"] = wait_cpu_fem(fem_cpu, log)
+        out["implicit_fem"] = fem_out
     print(json.dumps(out), flush=True)
+
+
+def wait_cpu_fem(proc, log, limit_s=900):
+    """The reference FEM worker's line (it keeps running beside the GPU legs; progress every 30 s meanwhile)."""
+    import subprocess
+    t0 = time.perf_counter()
+    while True:
+        try:
+            line = proc.communicate(timeout=30)[0]
+            break
+        except subprocess.TimeoutExpired:
+            if time.perf_counter() - t0 > limit_s:
+                proc.kill()
+                return None
+            log(f"waiting for the reference FEM step on the CPU ({time.perf_counter() - t0:.0f} s)")
+    try:
+        return json.loads(line.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return None
 
 
 if __name__ == "__main__":

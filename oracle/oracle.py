@@ -101,6 +101,9 @@ def ref_lib():
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
             "tri_stiffness": (ctypes.c_int, [_P, _P, _P, _P]),
+            "fem_plate_new": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_plate_free": (None, [_P]),
+            "fem_plate_step": (ctypes.c_int64, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -255,6 +258,34 @@ def ref_tri_stiffness(p, section=SECTION_C3):
     if ref_lib().blref_tri_stiffness(_ptr(p), _ptr(sec), _ptr(K), _ptr(X)) != 0:
         raise ValueError("degenerate triangle")
     return K, X.reshape(3, 2)
+
+
+class RefPlate:
+    """The compiled reference's clamped plate (ShellMesher::grid) and its implicit step as its fembench times it:
+    clear_forces(g) plus the loads, FemFrame::compute_forces, FemFrame::solve (one thread)."""
+
+    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3):
+        self._lib = ref_lib()
+        sec = np.asarray(section, np.float32)
+        self._h = self._lib.blref_fem_plate_new(int(nu), int(nv), float(size), float(y0), _ptr(sec))
+        self.nn = (nu + 1) * (nv + 1)
+
+    def step(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """One step; returns the frame's solve failures so far."""
+        ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
+        gg = np.asarray(g, np.float32)
+        return self._lib.blref_fem_plate_step(self._h, _ptr(ld), h, theta, dissipation, _ptr(gg))
+
+    def close(self):
+        if self._h:
+            self._lib.blref_fem_plate_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FemPlate:
