@@ -160,6 +160,18 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 10}), flush=True)
 
 
+def measured_traffic(tun):
+    """HBM bytes per launch of the force kernel as rocprof's FETCH_SIZE / WRITE_SIZE counted them (scaled by a plain
+    copy's counts), from the committed profile of the same kernel form; None for another form."""
+    path = os.path.join(ROOT, "profiles", "k_forces_c2_scene_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t["traffic_bytes_per_launch"] if t.get("kernel") == f"k_forces<{tun.block}, {tun.incidences_per_thread}>" else None
+
+
 def tuned(make, candidates, retune, log):
     """The autotuner's choice for this workload (cached by shape), found on a scratch world."""
     from beamlab_amd import autotune
@@ -286,7 +298,7 @@ def main():
                    "wall_s": wall, "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"],
                    "narrow_tests": st["narrow_tests"], "broken_beams": st["broken_beams"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_forces",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(tun), "kernel": "k_forces",
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_launch_s * 1e6,
                      "share_of_step": ker_ms / tot_ms},
     }
