@@ -74,7 +74,7 @@ class ShellBody:
     def __del__(self):
         self.close()
 
-    def step(self, loads=None, h=0.002, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+    def step(self, loads=None, h=0.0005, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
         """One implicit step; returns the report (residual, timings, factorization figures) as a dict."""
         ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
         gg = np.asarray(g, np.float32)
