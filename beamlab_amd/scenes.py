@@ -58,8 +58,9 @@ _OFFSETS = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1
 
 def lattice_body(nx, ny, nz, spacing=0.25, node_mass=2.0, origin=(0.0, 1.0, 0.0), k=2e6, d=500.0, eps=0.05,
                  seed=1, jitter=0.01, k_range=None, weak_frac=0.0, weak_eps=0.01, deform_factor=1.0,
-                 dt=DT, faces=False, name="lattice"):
-    """A node lattice body; origin is the (x, z) centre and the bottom y."""
+                 dt=DT, faces=False, name="lattice", diag_keep=1.0):
+    """A node lattice body; origin is the (x, z) centre and the bottom y. diag_keep < 1 keeps every axis beam and each
+    diagonal beam with that probability (a random network of lower degree)."""
     rng = TRng(seed)
     n = nx * ny * nz
     iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
@@ -96,6 +97,10 @@ def lattice_body(nx, ny, nz, spacing=0.25, node_mass=2.0, origin=(0.0, 1.0, 0.0)
     bmat = np.stack(cols_b, 1)
     mmat = np.stack(cols_m, 1)
     a_idx = np.repeat(ids, len(_OFFSETS)).reshape(n, len(_OFFSETS))
+    if diag_keep < 1.0:
+        diag = np.array([dx * dx + dy * dy + dz * dz > 1 for dx, dy, dz in _OFFSETS])
+        u = rng.uniforms(n * len(_OFFSETS)).reshape(n, len(_OFFSETS))
+        mmat = mmat & (~diag[None, :] | (u < np.float32(diag_keep)))
     sel = mmat.ravel()
     beams = np.zeros(int(sel.sum()), abi.BEAM)
     beams["a"] = a_idx.ravel()[sel]
@@ -245,3 +250,96 @@ def make_world(bodies, ground=True, settings=None):
     for b in bodies:
         w.add_body(b)
     return w
+
+
+def config4_sizes(n_bodies=512, seed=4):
+    """Node counts of the batched scene: lognormal (median 2000, sigma 0.5) clipped to [500, 8000]."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.round(np.exp(np.log(2000.0) + 0.5 * rng.standard_normal(n_bodies))), 500, 8000).astype(int)
+
+
+def crate_body(nx, ny, nz, spacing=0.25, node_mass=2.0, origin=(0.0, 1.0, 0.0), k=2e6, d=500.0, eps=0.05, seed=1,
+               jitter=0.01, struts=0.5, dt=DT, name="crate"):
+    """A hollow crate: the surface nodes of an nx x ny x nz lattice, its faces triangulated (box_faces, two-sided
+    collision triangles), a beam on every triangle edge (surface degree about 6), and struts * n random internal
+    beams between surface nodes (seeded). origin is the (x, z) centre and the bottom y."""
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    surf = (ix == 0) | (iy == 0) | (iz == 0) | (ix == nx - 1) | (iy == ny - 1) | (iz == nz - 1)
+    remap = -np.ones(nx * ny * nz, np.int64)
+    remap[surf] = np.arange(int(surf.sum()))
+    n = int(surf.sum())
+    rng = TRng(seed)
+    s = np.float32(spacing)
+    x0 = np.float32(origin[0]) - np.float32((nx - 1) * spacing * 0.5)
+    y0 = np.float32(origin[1])
+    z0 = np.float32(origin[2]) - np.float32((nz - 1) * spacing * 0.5)
+    p = np.stack([x0 + ix[surf].astype(np.float32) * s, y0 + iy[surf].astype(np.float32) * s,
+                  z0 + iz[surf].astype(np.float32) * s], 1).astype(np.float32)
+    if jitter > 0:
+        u = rng.uniforms(3 * n).reshape(n, 3)
+        jt = np.float32(jitter)
+        p = (p + (-jt + (jt - (-jt)) * u)).astype(np.float32)
+    body = SoftBody(name)
+    nodes = np.zeros(n, abi.NODE)
+    nodes["p"] = p
+    nodes["mass"] = np.float32(node_mass)
+    nodes["inv_mass"] = np.float32(1.0) / np.float32(node_mass)
+    info = np.zeros(n, abi.NODE_INFO)
+    info["flags"] = abi.NF_GROUND | abi.NF_CONTACTER
+    info["friction"] = 1.0
+    info["frame"] = -1
+    info["ror_id"] = -1
+    body.nodes, body.info = nodes, info
+    tris = box_faces(nx, ny, nz)
+    for c in "abc":
+        tris[c] = remap[tris[c]]
+    body.tris = tris
+    t = np.stack([tris["a"], tris["b"], tris["c"]], 1).astype(np.int64)
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    e = np.unique(np.sort(e, 1), axis=0)
+    m = int(round(struts * n))
+    u = rng.uniforms(2 * m).reshape(m, 2)
+    sa = np.minimum((u[:, 0] * n).astype(np.int64), n - 1)
+    sb = np.minimum((u[:, 1] * n).astype(np.int64), n - 1)
+    st = np.sort(np.stack([sa, sb], 1), 1)
+    st = st[st[:, 0] != st[:, 1]]
+    allb = np.unique(np.concatenate([e, st]), axis=0)
+    beams = np.zeros(len(allb), abi.BEAM)
+    beams["a"], beams["b"] = allb[:, 0], allb[:, 1]
+    beams["k"], beams["d"] = np.float32(k), np.float32(d)
+    beams["support_limit"] = np.float32(4.0)
+    body.beams = beams
+    body.finalize()
+    body.stabilize(float(dt))
+    strength = (np.float32(eps) * body.beams["k"]) * body.beams["L0"]
+    body.beams["strength"] = strength.astype(np.float32)
+    body.beams["max_pos"] = strength
+    body.beams["max_neg"] = -strength
+    return body
+
+
+def config4_body(n_nodes, origin, seed):
+    """A body of the batched scene: a hollow crate (crate_body) of about n_nodes surface nodes, 0.2 m spacing,
+    2 kg/node, k = 2e6, d = 500, break strain 0.05, a strut per two nodes (mean degree about 7)."""
+    side = 1
+    while 6 * (side + 1) ** 2 - 12 * (side + 1) + 8 <= n_nodes:
+        side += 1
+    nx = ny = nz = max(2, side)
+    return crate_body(nx, ny, nz, 0.2, 2.0, origin, k=2e6, d=500.0, eps=0.05, seed=seed, jitter=0.005,
+                      name=f"c4_body_{seed}")
+
+
+def config4_scene(n_bodies=512, arena=200.0, seed=4):
+    """The batched scene: n_bodies random beam networks (config4_sizes) on a grid over an arena x arena ground,
+    dropped from 0.3 m, inter-body contacts on."""
+    sizes = config4_sizes(n_bodies, seed)
+    g = int(math.ceil(math.sqrt(n_bodies)))
+    pitch = arena / g
+    rng = np.random.default_rng(seed + 1)
+    bodies = []
+    for i, nb in enumerate(sizes):
+        cx = -0.5 * arena + (i % g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+        cz = -0.5 * arena + (i // g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+        bodies.append(config4_body(int(nb), (float(cx), 0.3, float(cz)), seed=1000 + i))
+    return bodies

@@ -77,6 +77,42 @@ def cpu_baseline(bodies, sample_substeps, threads):
                       f"({dt:.1f} s)"}
 
 
+def batched_leg(steps, retune, log):
+    """The batched scene (512 hollow crates, lognormal sizes, mean degree 7, inter-body contacts on, 200 x 200 m):
+    beam updates/s in frames of 32 substeps, with the autotuner's form for it."""
+    from beamlab_amd import scenes
+    t0 = time.perf_counter()
+    tune = tuned(lambda: build_world(scenes.config4_scene()), None, retune, log)
+    bodies = scenes.config4_scene()
+    nb, nn = sum(len(b.beams) for b in bodies), sum(len(b.nodes) for b in bodies)
+    w = build_world(bodies)
+    w.set_tuning(**tune)
+    log(f"batched scene ready ({time.perf_counter() - t0:.1f} s): {len(bodies)} bodies, {nn} nodes, {nb} beams")
+    w.step_substeps(64)
+    t_ms = 0.0
+    done = 0
+    while done < steps:
+        k = min(32, steps - done)
+        t_ms += w.time_substeps(k)
+        done += k
+    st = w.stats()
+    tun = w.get_tuning()
+    w.close()
+    alg = 52 * nb + 56 * nn  # the beam-path substep's algorithmic bytes (beam records + CSR, node state)
+    achieved = alg / (t_ms / 1e3 / done) / 1e9
+    return {"value": nb * done / (t_ms / 1e3), "unit": "beam updates/s", "ms_per_step": t_ms / done, "steps": done,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "the whole substep",
+                         "alg_bytes_per_substep": alg},
+            "config": {"workload": "512 hollow crates (random struts, mean degree 7), lognormal sizes (median 2000, "
+                                   "500-8000 nodes), 200 x 200 m arena, inter-body contacts", "bodies": len(bodies),
+                       "nodes": nn, "beams": nb, "frame_substeps": 32,
+                       "form": {1: "stepped", 2: "persistent", 3: "resident", 4: "halo"}.get(tun.mode, "?"),
+                       "block": tun.block, "incidences_per_thread": tun.incidences_per_thread,
+                       "substeps_per_launch": tun.substeps_per_launch, "graph": tun.use_graph,
+                       "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"]}}
+
+
 def fem_loads(nn, fixed, steps, seed=7, per_step=32, sigma=50.0):
     """The seeded random point loads of the plate workload: per step `per_step` free nodes, a force normal to the
     plate (y) of N(0, sigma) N."""
@@ -217,6 +253,8 @@ def main():
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner's cache")
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a plain copy of this many MiB (profiling)")
     ap.add_argument("--fem-steps", type=int, default=200)
+    ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fem", action="store_true")
     ap.add_argument("--no-beams", action="store_true", help="the implicit-FEM leg only")
     ap.add_argument("--cpu-fem-worker", type=int, nargs=2, metavar=("CELLS", "STEPS"), help=argparse.SUPPRESS)
@@ -306,6 +344,8 @@ def main():
     }
     if not args.no_beam_only:
         out["beam_path_only"] = beam_path_only(4000, args.retune, log)
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batched_steps, args.retune, log)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16) - (1 if fem_cpu is not None else 0)
         out["cpu_baseline"] = cpu_baseline(scenes.config2_scene(), args.cpu_sample, max(1, threads))

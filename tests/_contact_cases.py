@@ -61,5 +61,21 @@ def case_groups_islands():
     return abi.default_settings(), _boxes(GROUND), [a, b, c, d, stick], 800
 
 
+def case_c4_small():
+    """The batched scene's bodies (config4_body: random beam networks, mean degree 7) at a small size: 16 on the
+    ground on a 2 m grid, 8 more dropped onto some of them from 1.5 m above."""
+    rng = np.random.default_rng(44)
+    sizes = np.clip(np.round(np.exp(np.log(120.0) + 0.5 * rng.standard_normal(24))), 40, 400).astype(int)
+    bodies = []
+    for i in range(16):
+        bodies.append(scenes.config4_body(int(sizes[i]), (-3.0 + 2.0 * (i % 4), 0.05, -3.0 + 2.0 * (i // 4)), 500 + i))
+    for j in range(8):
+        i = int(rng.integers(0, 16))
+        x0, z0 = -3.0 + 2.0 * (i % 4), -3.0 + 2.0 * (i // 4)
+        bodies.append(scenes.config4_body(int(sizes[16 + j]), (x0 + rng.uniform(-0.3, 0.3), 1.5 + 1.2 * (j // 4),
+                                                               z0 + rng.uniform(-0.3, 0.3)), 600 + j))
+    return abi.default_settings(), _boxes(GROUND), bodies, 700
+
+
 CONTACT_CASES = {"balls_on_lattice": case_balls_on_lattice, "lattice_pile": case_lattice_pile,
-                 "groups_islands": case_groups_islands}
+                 "groups_islands": case_groups_islands, "c4_small": case_c4_small}

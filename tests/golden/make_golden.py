@@ -7,6 +7,8 @@ tests/_cases.py. Run in the container that holds the reference sources:
 Outputs (data only: inputs are regenerated from the seeds, outputs are the reference's states):
     c1_ref.npz          config-1 lattice: node and beam records after 1, 7, 666 and 2000 substeps, broken counts
     cases_ref.npz       every other case: sha256 of the final node and beam records, broken counts
+    fem_ref.npz         the shell element: the reference's FemFrame::tri_stiffness (K, 18 x 18 FP64, and the rest
+                        shape X) of 24 seeded triangles under three sections
 """
 import hashlib
 import os
@@ -47,6 +49,19 @@ def main():
         digests[f"{name}_broken"] = np.array(st[-1][-1])
     np.savez_compressed(os.path.join(HERE, "cases_ref.npz"), **digests)
     print("written", sorted(digests))
+    rng = np.random.default_rng(17)
+    sections = np.array([oracle.SECTION_C3, (2.1e11, 0.3, 7850.0, 1e-3, 0.05, 3e-4), (5e9, 0.45, 1200.0, 4e-3, 0.2, 0.0)],
+                        np.float32)
+    P, K, X = [], [], []
+    while len(P) < 24:
+        p = rng.uniform(-1, 1, (3, 3)).astype(np.float32) * np.float32(0.05 + len(P) / 10)
+        if np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0])) < 1e-3:
+            continue
+        k, x = oracle.ref_tri_stiffness(p, tuple(sections[len(P) % 3]))
+        P.append(p), K.append(k), X.append(x)
+    np.savez_compressed(os.path.join(HERE, "fem_ref.npz"), p=np.array(P), K=np.array(K), X=np.array(X),
+                        section=np.array([(i % 3) for i in range(24)]), sections=sections)
+    print("written fem_ref.npz")
 
 
 if __name__ == "__main__":

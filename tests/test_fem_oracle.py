@@ -103,3 +103,15 @@ def test_plate_mesh_matches_oracle():
         p, t, fx = fem.plate_mesh(nu, nv)
         assert np.array_equal(t, tn) and np.array_equal(fx, fixed)
         assert np.array_equal(p.astype(np.float64), xd)
+
+
+def test_tri_stiffness_matches_golden():
+    """The element against the compiled reference's FemFrame::tri_stiffness as recorded in tests/golden/fem_ref.npz
+    (runs where the reference is absent)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_ref.npz"))
+    for i in range(len(g["p"])):
+        sec = tuple(float(x) for x in g["sections"][g["section"][i]])
+        X = oracle.fem_rest_shape(g["p"][i])
+        assert X.tobytes() == g["X"][i].astype(np.float32).tobytes()
+        assert oracle.fem_tri_local_k(X, sec).tobytes() == g["K"][i].tobytes(), f"triangle {i}"
