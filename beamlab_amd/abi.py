@@ -99,6 +99,24 @@ class FemReport(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+WARM_ZERO, WARM_PREVIOUS, WARM_NETWORK = 0, 1, 2
+PRECOND_BLOCK_JACOBI, PRECOND_NETWORK = 0, 1
+
+
+class PcgOpts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("max_iter", "warm", "precond", "fallback", "check_error", "pad")] + \
+               [("tol", ctypes.c_double)]
+
+
+class PcgReport(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("iterations", "converged", "fallback_used", "pad")] + \
+               [(n, ctypes.c_double) for n in ("residual", "warm_error", "pcg_error")] + \
+               [(n, ctypes.c_float) for n in ("assemble_ms", "gnn_ms", "pcg_ms", "fallback_ms", "update_ms", "total_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
 # every entry point the header declares, with its argument types
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -136,6 +154,16 @@ SIGNATURES = {
     "bl_fem_last_system": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),
     "bl_fem_tri_k": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "bl_gnn_param_count": (ctypes.c_int, []),
+    "bl_gnn_create": (ctypes.c_int, [ctypes.POINTER(_P), _P, ctypes.c_uint32]),
+    "bl_gnn_destroy": (None, [_P]),
+    "bl_gnn_get_params": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
+    "bl_gnn_set_params": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
+    "bl_gnn_gradients": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
+    "bl_gnn_train_step": (ctypes.c_int, [_P, _P, ctypes.c_float, _P]),
+    "bl_gnn_infer": (ctypes.c_int, [_P, _P, _P, _P]),
+    "bl_fem_step_pcg": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P,
+                                       ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgReport)]),
 }
 
 _lib = None

@@ -12,8 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbeamlab_gpu.so")
-SOURCES = ["bl_step.hip", "bl_contacts.hip", "bl_capi.hip", "bl_fem.hip"]
-HEADERS = ["bl_devmath.h", "bl_world.h", "bl_contacts.h"]
+SOURCES = ["bl_step.hip", "bl_contacts.hip", "bl_capi.hip", "bl_fem.hip", "bl_gnn.hip"]
+HEADERS = ["bl_devmath.h", "bl_world.h", "bl_contacts.h", "bl_fem_api.h"]
 ARCH = os.environ.get("BL_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

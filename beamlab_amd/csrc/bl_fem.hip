@@ -19,6 +19,7 @@
 //   fronts      one dense column-major m x m FP64 matrix per supernode (its columns, then the rows below them),
 //               holding L11 / L21 after the factorization and the update matrix for its parent in the trailing part
 #include "bl_world.h"
+#include "bl_fem_api.h"
 
 #include <algorithm>
 #include <cmath>
@@ -960,6 +961,31 @@ __global__ void k_residual(FemArgs a, const double* x, const uint32_t* adj_off, 
     if (threadIdx.x == 0) part[2 * blockIdx.x] = s1[0], part[2 * blockIdx.x + 1] = s2[0];
 }
 
+// y = A x over the assembled system (the lower triangle of the diagonal blocks and each pair block both ways)
+__global__ void k_spmv(FemArgs a, const double* x, const uint32_t* adj_off, const uint2* adj, double* y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nf) return;
+    double acc[6];
+    const double* D = a.Dsys + 36 * (size_t)i;
+    for (int p = 0; p < 6; p++) {
+        double v = 0;
+        for (int q = 0; q < 6; q++) v += (p >= q ? D[p * 6 + q] : D[q * 6 + p]) * x[6 * i + q];
+        acc[p] = v;
+    }
+    for (uint32_t k = adj_off[i]; k < adj_off[i + 1]; k++) {
+        const uint2 e = adj[k];
+        const double* O = a.Osys + 36 * (size_t)e.x;
+        const uint32_t o = e.y & 0x7fffffffu;
+        const bool row_side = (e.y >> 31) != 0;
+        for (int p = 0; p < 6; p++) {
+            double v = 0;
+            for (int q = 0; q < 6; q++) v += (row_side ? O[p * 6 + q] : O[q * 6 + p]) * x[6 * o + q];
+            acc[p] += v;
+        }
+    }
+    for (int p = 0; p < 6; p++) y[6 * i + p] = acc[p];
+}
+
 __global__ void k_sum_parts(const double* part, uint32_t n, double* out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double a = 0, b = 0;
@@ -1040,7 +1066,9 @@ struct bl_fem {
     Dv<double> xd;
     Dv<float> v, w, mass, inertia, X, Kf, te, tfo, tto, tg, loads;
     Dv<float4> q, r0;
-    Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, part, sums;
+    Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, xprev, part, sums;
+    Dv<float> p0f;     // rest positions of the free nodes (3 each)
+    FemArgs step_args{}; // the last step's arguments (the update's)
     Dv<uint8_t> fixed, ton;
     Dv<uint4> tri, pairs, pcon;
     Dv<uint32_t> nc_off, nc, ford, ord2f, rows, children, ch_off, cmap, adj_off;
@@ -1466,6 +1494,9 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         }
     }
     // ord2f: order position -> free index
+    std::vector<float> p0f(3 * (size_t)nf);
+    for (uint32_t i = 0; i < nf; i++)
+        for (int k = 0; k < 3; k++) p0f[3 * i + k] = d->p[3 * freen[i] + k];
     std::vector<uint32_t> ord2f(order.begin(), order.end()), ford(nf);
     for (uint32_t i = 0; i < nf; i++) ford[i] = opos[i];
     f->hfronts = fr;
@@ -1485,7 +1516,8 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         (rc = f->nc_off.put(nc_off)) || (rc = f->nc.put(nc)) || (rc = f->fidx.put(fidx)) || (rc = f->ford.put(ford)) ||
         (rc = f->ord2f.put(ord2f)) || (rc = f->Dsys.alloc(36 * (size_t)nf)) || (rc = f->rhs.alloc(6 * (size_t)nf)) ||
         (rc = f->Osys.alloc(36 * pairs.size())) || (rc = f->pairs.put(pairs)) || (rc = f->pcon.put(pcon)) ||
-        (rc = f->y.alloc(6 * (size_t)nf)) || (rc = f->x.alloc(6 * (size_t)nf)) || (rc = f->loads.alloc(3 * (size_t)nn)) ||
+        (rc = f->y.alloc(6 * (size_t)nf)) || (rc = f->x.put(std::vector<double>(6 * (size_t)nf, 0.0))) ||
+        (rc = f->xprev.put(std::vector<double>(6 * (size_t)nf, 0.0))) || (rc = f->p0f.put(p0f)) || (rc = f->loads.alloc(3 * (size_t)nn)) ||
         (rc = f->rows.put(rows)) || (rc = f->cmap.put(cmap)) || (rc = f->children.put(children)) ||
         (rc = f->ch_off.put(ch_off)) || (rc = f->fronts.put(fr)) || (rc = f->F.alloc(off)) || (rc = f->V.alloc(voff)) ||
         (rc = f->Xs.alloc(voff)) || (rc = f->fflag.put(std::vector<uint32_t>(fb, 0u))) ||
@@ -1553,6 +1585,86 @@ int fem_factor_solve(bl_fem* f, float* ms_factor, float* ms_solve) {
 
 } // namespace
 
+namespace blf {
+
+FemView fem_view(bl_fem* f) {
+    FemView v{};
+    v.nf = f->nf, v.D = f->Dsys.p, v.O = f->Osys.p, v.pairs = f->pairs.p, v.npairs = f->npairs;
+    v.adj_off = f->adj_off.p, v.adj = f->adj.p, v.rhs = f->rhs.p, v.x = f->x.p, v.xprev = f->xprev.p;
+    v.p0 = f->p0f.p, v.st = f->st;
+    return v;
+}
+
+int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], float* ms) {
+    const hipStream_t st = f->st;
+    const float* dl = nullptr;
+    // (theta and the dissipation as FemFrame::solve_begin takes them, src/phys/frame_fem.cpp:2524-2525)
+    dissipation = std::min(1.0f, std::max(0.0f, dissipation));
+    theta = std::min(1.0f, std::max(0.25f + 0.5f * dissipation, theta));
+    if (loads) {
+        FCHK(hipMemcpyAsync(f->loads.p, loads, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice, st));
+        dl = f->loads.p;
+    }
+    FCHK(hipMemcpyAsync(f->xprev.p, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToDevice, st));
+    FemArgs a = f->args(dl, h, theta, dissipation, g);
+    f->step_args = a;
+    FCHK(hipEventRecord(f->ev[0], st));
+    hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
+    if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(f->ev[1], st));
+    f->have_system = true;
+    if (ms) {
+        FCHK(hipEventSynchronize(f->ev[1]));
+        FCHK(hipEventElapsedTime(ms, f->ev[0], f->ev[1]));
+    }
+    return BL_OK;
+}
+
+int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail) {
+    int rc = fem_factor_solve(f, ms_factor, ms_solve);
+    if (rc) return rc;
+    FCHK(hipMemcpy(fail, f->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+int fem_residual(bl_fem* f, const double* x, double* rel) {
+    const hipStream_t st = f->st;
+    hipLaunchKernelGGL(k_residual, dim3(f->nparts), dim3(256), 0, st, f->step_args, x, f->adj_off.p, f->adj.p, f->part.p);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, f->part.p, f->nparts, f->sums.p);
+    double sums[2] = {0, 0};
+    FCHK(hipMemcpyAsync(sums, f->sums.p, sizeof(sums), hipMemcpyDeviceToHost, st));
+    FCHK(hipStreamSynchronize(st));
+    *rel = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
+    return BL_OK;
+}
+
+void fem_spmv(bl_fem* f, const double* x, double* y) {
+    hipLaunchKernelGGL(k_spmv, dim3((f->nf + 255) / 256), dim3(256), 0, f->st, f->step_args, x, f->adj_off.p, f->adj.p, y);
+}
+
+int fem_update(bl_fem* f, float* ms) {
+    const hipStream_t st = f->st;
+    FCHK(hipEventRecord(f->ev[3], st));
+    hipLaunchKernelGGL(k_update, dim3((f->nn + 255) / 256), dim3(256), 0, st, f->step_args, f->x.p);
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(f->ev[4], st));
+    FCHK(hipEventSynchronize(f->ev[4]));
+    if (ms) FCHK(hipEventElapsedTime(ms, f->ev[3], f->ev[4]));
+    return BL_OK;
+}
+
+void fem_report_static(bl_fem* f, bl_fem_report* rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->factor_flops = f->flops;
+    rep->supernodes = (int32_t)f->hfronts.size();
+    rep->fronts_max = f->fronts_max;
+    rep->factor_nnz = f->nnz;
+}
+
+} // namespace blf
+
 extern "C" {
 
 int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
@@ -1583,44 +1695,18 @@ void bl_fem_destroy(bl_fem* f) { delete f; }
 int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
                 bl_fem_report* rep) {
     if (!f || !g || !(h > 0)) return BL_ERR_ARG;
-    const hipStream_t st = f->st;
-    const float* dl = nullptr;
-    if (loads) {
-        FCHK(hipMemcpyAsync(f->loads.p, loads, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice, st));
-        dl = f->loads.p;
-    }
-    FemArgs a = f->args(dl, h, theta, dissipation, g);
-    FCHK(hipEventRecord(f->ev[0], st));
-    hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
-    if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
-    FCHK(hipGetLastError());
-    float msf = 0, mss = 0, msa = 0, msu = 0;
-    int rc = fem_factor_solve(f, &msf, &mss);
-    if (rc) return rc;
-    FCHK(hipEventElapsedTime(&msa, f->ev[0], f->ev[1]));
-    int fail = 0;
-    FCHK(hipMemcpy(&fail, f->fail.p, sizeof(int), hipMemcpyDeviceToHost));
-    // the residual of the solve against the assembled system
-    hipLaunchKernelGGL(k_residual, dim3(f->nparts), dim3(256), 0, st, a, f->x.p, f->adj_off.p, f->adj.p, f->part.p);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, f->part.p, f->nparts, f->sums.p);
-    double sums[2] = {0, 0};
-    FCHK(hipMemcpyAsync(sums, f->sums.p, sizeof(sums), hipMemcpyDeviceToHost, st));
-    FCHK(hipEventRecord(f->ev[3], st));
-    if (!fail) hipLaunchKernelGGL(k_update, dim3((f->nn + 255) / 256), dim3(256), 0, st, a, f->x.p);
-    FCHK(hipEventRecord(f->ev[4], st));
-    FCHK(hipEventSynchronize(f->ev[4]));
-    FCHK(hipEventElapsedTime(&msu, f->ev[3], f->ev[4]));
-    f->have_system = true;
+    float msa = 0, msf = 0, mss = 0, msu = 0;
+    int fail = 0, rc;
+    double res = 0;
+    if ((rc = blf::fem_assemble(f, loads, h, theta, dissipation, g, &msa))) return rc;
+    if ((rc = blf::fem_cholesky(f, &msf, &mss, &fail))) return rc;
+    if ((rc = blf::fem_residual(f, f->x.p, &res))) return rc;
+    if (!fail && (rc = blf::fem_update(f, &msu))) return rc;
     if (rep) {
-        std::memset(rep, 0, sizeof(*rep));
-        rep->residual = sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
-        rep->factor_flops = f->flops;
+        blf::fem_report_static(f, rep);
+        rep->residual = res;
         rep->assemble_ms = msa, rep->factor_ms = msf, rep->solve_ms = mss, rep->update_ms = msu;
         rep->failures = fail ? 1 : 0;
-        rep->supernodes = (int32_t)f->hfronts.size();
-        rep->fronts_max = f->fronts_max;
-        rep->factor_nnz = f->nnz;
     }
     if (fail) {
         blg::set_last_error((fail & 1) ? "a pivot of the factorization is not positive"

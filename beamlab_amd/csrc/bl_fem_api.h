@@ -1,0 +1,40 @@
+// The FEM engine's phases and device arrays, for the learned warm start and the PCG solve (bl_gnn.hip). Internal to
+// the library: the C ABI is include/beamlab_gpu.h.
+#pragma once
+#include "beamlab_gpu.h"
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace blf {
+
+struct FemView {
+    uint32_t nf;             // free nodes (the system's block rows, in node order)
+    const double* D;         // 36 per free node: the diagonal block (row-major; its lower triangle is the system's)
+    const double* O;         // 36 per pair: the lower block (row node's rows, column node's columns)
+    const uint4* pairs;      // per pair: row free index, column free index, ..
+    uint32_t npairs;
+    const uint32_t* adj_off; // per free node: its pairs (adj_off[i] .. adj_off[i + 1])
+    const uint2* adj;        // (pair, other free index | 1 << 31 when this node is the pair's row node)
+    const double* rhs;       // 6 per free node
+    double* x;               // 6 per free node: the step's solution (what the update applies)
+    const double* xprev;     // 6 per free node: the previous step's solution
+    const float* p0;         // 3 per free node: rest position
+    hipStream_t st;
+};
+
+FemView fem_view(bl_fem* f);
+// the element pass and the assembly of a step (the previous solution kept as xprev)
+int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
+                 float* ms);
+// x := A^-1 rhs by the supernodal Cholesky; *fail: bit 1 a pivot not positive, bit 2 a wait timed out
+int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail);
+// || A x - rhs || / || rhs || (waits for the stream)
+int fem_residual(bl_fem* f, const double* x, double* rel);
+// y := A x (enqueued on the engine's stream)
+void fem_spmv(bl_fem* f, const double* x, double* y);
+// the update of the state from f's solution x (v, w, positions, orientations)
+int fem_update(bl_fem* f, float* ms);
+// the factorization's figures for a report
+void fem_report_static(bl_fem* f, bl_fem_report* rep);
+
+} // namespace blf

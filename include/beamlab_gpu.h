@@ -340,6 +340,54 @@ int bl_fem_solve(bl_fem* f, const double* b, double* x);
 /* The element stiffness of triangle `tri` as the device built it (18 x 18 row-major, FP64). */
 int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * The learned warm start and preconditioner of the implicit step's solve (no counterpart in the reference: the new
+ * part of the FEM path). A 4-layer message-passing network over the free nodes (hidden 64, bfloat16 products with
+ * FP32 sums on the matrix cores) predicts the step's solution and a per-dof scaling s of the block-Jacobi
+ * preconditioner (S D^-1 S). PCG starts from the prediction; the supernodal Cholesky is the fallback when PCG misses
+ * the tolerance within max_iter iterations, and the judge of the prediction's error. The network is trained on the
+ * engine's own (system, solution) pairs: a Cholesky step (bl_fem_step), then bl_gnn_train_step.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct bl_gnn bl_gnn;
+
+enum { BL_WARM_ZERO = 0, BL_WARM_PREVIOUS = 1, BL_WARM_NETWORK = 2 };
+enum { BL_PRECOND_BLOCK_JACOBI = 0, BL_PRECOND_NETWORK = 1 };
+
+typedef struct bl_pcg_opts {
+    int32_t max_iter;    /* K: PCG iterations before the fallback */
+    int32_t warm;        /* BL_WARM_* */
+    int32_t precond;     /* BL_PRECOND_* */
+    int32_t fallback;    /* 1: the Cholesky when PCG misses tol within max_iter */
+    int32_t check_error; /* 1: also the Cholesky solution, to report the warm start's and PCG's errors (not timed use) */
+    int32_t pad;
+    double tol;          /* relative residual || b - A x || / || b || */
+} bl_pcg_opts;
+
+typedef struct bl_pcg_report {
+    int32_t iterations, converged, fallback_used, pad;
+    double residual;       /* of the solution the update applied (FP64, on the device) */
+    double warm_error;     /* || x0 - x* || / || x* || (check_error) */
+    double pcg_error;      /* || x_pcg - x* || / || x* || (check_error) */
+    float assemble_ms, gnn_ms, pcg_ms, fallback_ms, update_ms, total_ms;
+} bl_pcg_report;
+
+int bl_gnn_param_count(void);
+/* The network with the given weights (bl_gnn_param_count floats, in the order of the layers). */
+int bl_gnn_create(bl_gnn** out, const float* params, uint32_t n);
+void bl_gnn_destroy(bl_gnn* g);
+int bl_gnn_get_params(bl_gnn* g, float* out, uint32_t n);
+int bl_gnn_set_params(bl_gnn* g, const float* params, uint32_t n); /* (resets the optimizer) */
+/* The loss of the engine's current system against its current solution, and its gradients (n floats, or null). */
+int bl_gnn_gradients(bl_gnn* g, bl_fem* f, float* grads, uint32_t n, double losses[2]);
+/* One Adam step on the engine's current (system, solution). */
+int bl_gnn_train_step(bl_gnn* g, bl_fem* f, float lr, double losses[2]);
+/* The warm start x0 and the scales s (6 per free node each, host memory, either may be null) for the current system. */
+int bl_gnn_infer(bl_gnn* g, bl_fem* f, double* x0, double* s);
+/* One implicit step solved by PCG (the network's warm start / preconditioner as opts choose), the Cholesky as the
+ * fallback; g holds the solve's workspace and is required. */
+int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float theta, float dissipation, const float gr[3],
+                    const bl_pcg_opts* opts, bl_pcg_report* rep);
+
 #ifdef __cplusplus
 }
 #endif

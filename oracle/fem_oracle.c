@@ -637,6 +637,12 @@ int orc_fem_block_solve(int n, double* A, double* b) {
 int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* Aout,
                  double* rhs_out, double* x_out, uint32_t* free_map, uint32_t* nfree_out) {
     const uint32_t nn = f->nn, nt = f->nt;
+    /* theta and the dissipation as FemFrame::solve_begin takes them (frame_fem.cpp:2524-2525) */
+    dissipation = dissipation < 0.0f ? 0.0f : (dissipation > 1.0f ? 1.0f : dissipation);
+    {
+        const float lo = 0.25f + 0.5f * dissipation;
+        theta = theta < lo ? lo : (theta > 1.0f ? 1.0f : theta);
+    }
     tri_out* o = (tri_out*)calloc(nt, sizeof(tri_out));
     for (uint32_t k = 0; k < nt; k++) eval_tri(f, k, &o[k]);
     /* the node forces: gravity m g, the loads, then the triangles' corners in triangle order */

@@ -3,9 +3,9 @@ checker of beamlab_amd/csrc/bl_gnn.hip). The reference has no learned part; this
 and its gradients, and is itself pinned by finite differences (tests/test_gnn_oracle.py).
 
 The network over the free nodes of a shell mesh (an edge where two free nodes share a triangle):
-    features f_i (32, of which 18 used): t_i / sigma, xprev_i / sigma, b_i / beta, with t_i = D_i^-1 b_i
-        (D_i the node's 6x6 diagonal block), sigma / beta the root mean squares of t / b over the nodes, taken apart
-        for the translations (dof 0-2) and the rotations (dof 3-5)
+    features f_i (32, of which 18 used): t_i / tau, xprev_i / sigma, b_i / beta, with t_i = D_i^-1 b_i (D_i the
+        node's 6x6 diagonal block), tau / sigma / beta the root mean squares over the nodes of t / xprev / b, taken
+        apart for the translations (dof 0-2) and the rotations (dof 3-5); sigma falls back to tau where xprev is zero
     h0 = relu(f W_e + b_e)                                           64 wide
     4 layers: P = h W_m;  agg_i = 1/deg_i sum_{j in N(i)} relu(P_j + rel_ij U + b_m)
               h <- h + relu([h, agg] W_u + b_u)
@@ -14,8 +14,9 @@ The network over the free nodes of a shell mesh (an edge where two free nodes sh
 The PCG's preconditioner is S D^-1 S (S = diag(s), D the block diagonal).
 Loss (the mean over the nodes' 6 dofs):
     L1 = mean (o[0:6] - x*/sigma)^2                                  the warm start against the Cholesky solution
-    L2 = mean (s * D^-1 (s * r0) / sigma - (x* - x0) / sigma)^2     one preconditioned step from the warm start
-         (r0 = b - A x0 and x* - x0 taken as constants)
+    L2 = mean (k z - e)^2, z = s * D^-1 (s * r0) / sigma, e = (x* - x0) / sigma, k = <z, e> / <z, z>
+         the preconditioned residual of the warm start against its error, up to the best global factor k (CG does
+         not see a preconditioner's scale); r0 = b - A x0, e and k taken as constants
 The matrix products take their operands rounded to bfloat16 (round to nearest even) and sum in FP32, as the
 kernels do on the matrix cores; `bf16=False` computes in FP64 without rounding (the finite-difference check).
 """
@@ -76,11 +77,11 @@ def features(D, b, xprev):
     t = np.einsum("nij,nj->ni", Dinv, b)
     def rms2(v):
         return np.array([np.sqrt(np.mean(v[:, :3] ** 2)), np.sqrt(np.mean(v[:, 3:] ** 2))])
-    st, sb = rms2(t) + 1e-30, rms2(b) + 1e-30
-    sigma = np.repeat(st, 3)
-    beta = np.repeat(sb, 3)
+    st, sb, sx = rms2(t), rms2(b), rms2(xprev)
+    sx = np.where(sx > 0, sx, st)
+    tau, sigma, beta = np.repeat(st + 1e-30, 3), np.repeat(sx + 1e-30, 3), np.repeat(sb + 1e-30, 3)
     f = np.zeros((len(b), FIN))
-    f[:, 0:6] = t / sigma
+    f[:, 0:6] = t / tau
     f[:, 6:12] = xprev / sigma
     f[:, 12:18] = b / beta
     return f.astype(np.float32), sigma, Dinv
@@ -133,10 +134,11 @@ def loss_and_grads(p, f, nbr_off, nbr, rel, D, Dinv, b, sigma, xstar, A_apply, u
     e = (xstar - x0) / sigma
     u = np.einsum("nij,nj->ni", Dinv, s * r0)
     z = s * u / sigma
-    L2 = np.mean((z - e) ** 2)
+    kap = np.sum(z * e) / np.sum(z * z)
+    L2 = np.mean((kap * z - e) ** 2)
     do = np.zeros((n, FOUT), np.float64)
     do[:, 0:6] = 2 * (yhat - ystar) / (6 * n)
-    dz = 2 * (z - e) / (6 * n)
+    dz = 2 * kap * (kap * z - e) / (6 * n)
     ds = dz * u / sigma + r0 * np.einsum("nk,nkm->nm", dz * s / sigma, Dinv)
     inside = (lo > -3) & (lo < 3)
     do[:, 6:12] = np.where(inside, ds * s, 0.0)
