@@ -113,6 +113,51 @@ def batched_leg(steps, retune, log):
                        "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"]}}
 
 
+def learned_leg(steps, plain_solves_per_s, log):
+    """The learned warm start on the plate workload: each step solved by PCG (block Jacobi) from the trained network's
+    prediction, K = 50 iterations to 1e-6, the supernodal Cholesky as the fallback; then, untimed, the network's error
+    per solve against the FP64 Cholesky and the iterations an uncapped PCG needs."""
+    import numpy as np
+    from beamlab_amd import abi, fem, learned
+    net = learned.Network.load()
+    G = fem.ShellBody.plate(FEM_CELLS, FEM_CELLS)
+    loads = fem_loads(G.nn, G.fixed, 3 + steps + 12, seed=21)
+    for s in range(3):
+        G.step(loads[s], FEM_H)
+    reps = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        reps.append(learned.pcg_step(G, net, loads[3 + s], h=FEM_H, max_iter=50, tol=1e-6, warm=abi.WARM_NETWORK,
+                                     precond=abi.PRECOND_BLOCK_JACOBI))
+    wall = time.perf_counter() - t0
+    log(f"learned: {steps / wall:.1f} solves/s")
+    checks = [learned.pcg_step(G, net, loads[3 + steps + k], h=FEM_H, max_iter=50, tol=1e-6, warm=abi.WARM_NETWORK,
+                               precond=abi.PRECOND_BLOCK_JACOBI, check_error=True) for k in range(8)]
+    uncapped = {}
+    for name, warm in (("network", abi.WARM_NETWORK), ("zero", abi.WARM_ZERO)):
+        state = G.state()
+        r = learned.pcg_step(G, net, loads[3 + steps + 8], h=FEM_H, max_iter=20000, tol=1e-6, warm=warm,
+                             precond=abi.PRECOND_BLOCK_JACOBI, fallback=False)
+        G.set_state(*state)
+        uncapped[name] = {"iterations": r["iterations"], "converged": r["converged"], "pcg_ms": r["pcg_ms"]}
+    G.close()
+    mean = lambda k: float(np.mean([r[k] for r in reps]))
+    value = steps / wall
+    return {"value": value, "unit": "implicit FEM solves/s", "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "dtype": "bf16 network (fp32 sums), f64 PCG and fallback",
+            "config": {"workload": f"the {FEM_CELLS}x{FEM_CELLS} plate of implicit_fem", "network": "4 layers x 64, "
+                       "trained on 20000 Cholesky-solved steps of a 64x64 plate (beamlab_amd/gnn_plate.npz)",
+                       "pcg": "block Jacobi, warm start from the network", "max_iter": 50, "tol": 1e-6,
+                       "fallback": "supernodal Cholesky"},
+            "vs_plain_cholesky": value / plain_solves_per_s if plain_solves_per_s else None,
+            "converged_rate": mean("converged"), "fallback_rate": mean("fallback_used"),
+            "iterations_mean": mean("iterations"), "residual_max": max(r["residual"] for r in reps),
+            "phases_ms": {k: mean(k + "_ms") for k in ("assemble", "gnn", "pcg", "fallback", "update")},
+            "per_solve_error_vs_fp64_cholesky": [{"warm_start": c["warm_error"], "pcg": c["pcg_error"],
+                                                  "fallback_used": c["fallback_used"]} for c in checks],
+            "uncapped_pcg_to_1e-6": uncapped}
+
+
 def fem_loads(nn, fixed, steps, seed=7, per_step=32, sigma=50.0):
     """The seeded random point loads of the plate workload: per step `per_step` free nodes, a force normal to the
     plate (y) of N(0, sigma) N."""
@@ -254,6 +299,8 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a plain copy of this many MiB (profiling)")
     ap.add_argument("--fem-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--learned-steps", type=int, default=50)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fem", action="store_true")
     ap.add_argument("--no-beams", action="store_true", help="the implicit-FEM leg only")
@@ -282,6 +329,8 @@ def main():
     if not args.no_fem:
         fem_out = fem_leg(args.fem_steps, 3, log)
         log(f"fem: {fem_out['value']:.1f} solves/s")
+        if not args.no_learned:
+            fem_out["learned"] = learned_leg(args.learned_steps, fem_out["value"], log)
     if args.no_beams:
         out = {"metric": METRIC, "value": fem_out["value"], "unit": fem_out["unit"], "n_gpus": 1,
                "steps": args.fem_steps, "warmup": 3, "ms_per_step": fem_out["ms_per_step"], "higher_is_better": True,
