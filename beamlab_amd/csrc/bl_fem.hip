@@ -22,6 +22,7 @@
 #include "bl_fem_api.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -1061,6 +1062,7 @@ struct bl_fem {
     hipStream_t st = nullptr;
     hipEvent_t ev[5] = {};
     uint32_t nn = 0, nt = 0, nf = 0, npairs = 0;
+    uint64_t serial = 0; // unique per engine in the process (an address can be reused after a destroy)
     Section sec{};
     // state and topology
     Dv<double> xd;
@@ -1591,7 +1593,7 @@ FemView fem_view(bl_fem* f) {
     FemView v{};
     v.nf = f->nf, v.D = f->Dsys.p, v.O = f->Osys.p, v.pairs = f->pairs.p, v.npairs = f->npairs;
     v.adj_off = f->adj_off.p, v.adj = f->adj.p, v.rhs = f->rhs.p, v.x = f->x.p, v.xprev = f->xprev.p;
-    v.p0 = f->p0f.p, v.st = f->st;
+    v.p0 = f->p0f.p, v.st = f->st, v.serial = f->serial;
     return v;
 }
 
@@ -1672,6 +1674,8 @@ int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
     *out = nullptr;
     if (d->n_nodes == 0 || d->n_tris == 0) return BL_ERR_ARG;
     bl_fem* f = new bl_fem();
+    static std::atomic<uint64_t> next_serial{1};
+    f->serial = next_serial.fetch_add(1);
     if (hipStreamCreate(&f->st) != hipSuccess) {
         delete f;
         return BL_ERR_HIP;

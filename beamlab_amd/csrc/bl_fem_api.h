@@ -20,6 +20,7 @@ struct FemView {
     const double* xprev;     // 6 per free node: the previous step's solution
     const float* p0;         // 3 per free node: rest position
     hipStream_t st;
+    uint64_t serial;         // unique per engine in the process
 };
 
 FemView fem_view(bl_fem* f);

@@ -625,7 +625,7 @@ struct bl_gnn {
     DB<float> params, grads, m, v2;
     uint32_t adam_t = 0;
     // the mesh's graph and the workspace (for the engine they were built for)
-    bl_fem* mesh = nullptr;
+    uint64_t mesh = 0; // the engine's serial
     uint32_t n = 0, ne = 0, nblk = 0, nchunks = 0;
     DB<uint32_t> off, nbr, rev;
     DB<float4> rel;
@@ -640,8 +640,9 @@ struct bl_gnn {
     }
 
     int setup(bl_fem* fm) {
-        if (mesh == fm) return BL_OK;
         const blf::FemView v = blf::fem_view(fm);
+        if (mesh == v.serial) return BL_OK;
+        mesh = 0;
         n = v.nf;
         // the graph from the engine's pair lists (both directions), in the node's pair order
         std::vector<uint32_t> aoff(n + 1);
@@ -704,7 +705,7 @@ struct bl_gnn {
             GCHK(hipMemcpy(rev.p, hrev.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
             GCHK(hipMemcpy(rel.p, hrel.data(), sizeof(float4) * ne, hipMemcpyHostToDevice));
         }
-        mesh = fm;
+        mesh = v.serial;
         return BL_OK;
     }
 

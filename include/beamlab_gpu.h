@@ -345,7 +345,7 @@ int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
  * part of the FEM path). A 4-layer message-passing network over the free nodes (hidden 64, bfloat16 products with
  * FP32 sums on the matrix cores) predicts the step's solution and a per-dof scaling s of the block-Jacobi
  * preconditioner (S D^-1 S). PCG starts from the prediction; the supernodal Cholesky is the fallback when PCG misses
- * the tolerance within max_iter iterations, and the judge of the prediction's error. The network is trained on the
+ * the tolerance within max_iter iterations, and the yardstick of the prediction's error. The network is trained on the
  * engine's own (system, solution) pairs: a Cholesky step (bl_fem_step), then bl_gnn_train_step.
  * ------------------------------------------------------------------------------------------------------------- */
 typedef struct bl_gnn bl_gnn;
