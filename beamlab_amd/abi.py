@@ -100,7 +100,7 @@ class FemReport(ctypes.Structure):
 
 
 WARM_ZERO, WARM_PREVIOUS, WARM_NETWORK = 0, 1, 2
-PRECOND_BLOCK_JACOBI, PRECOND_NETWORK = 0, 1
+PRECOND_BLOCK_JACOBI, PRECOND_NETWORK, PRECOND_FACTOR = 0, 1, 2
 
 
 class PcgOpts(ctypes.Structure):

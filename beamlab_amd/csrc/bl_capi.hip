@@ -109,12 +109,25 @@ struct bl_ctx {
     std::vector<uint32_t> h_noff;
     KArgs args{};
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    // captured contact segments (the substeps from one pair rebuild to the next), for the lists' current buffers
+    struct Segment {
+        int len, parity, tail;
+        hipGraphExec_t ex;
+    };
+    std::vector<Segment> segs;
+    PArgs seg_p{};
+    uint32_t seg_nblk = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;  // per-launch events of bl_time_kernels
 
+    void drop_segments() {
+        for (auto& g : segs) (void)hipGraphExecDestroy(g.ex);
+        segs.clear();
+    }
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
+        drop_segments();
     }
     void release() {
         drop_graphs();
@@ -610,10 +623,59 @@ int repair_body(HostBody& b) {
     return reset;
 }
 
+// The substeps from a pair rebuild up to the next: the first one's narrow phase, response and integration, then
+// `len - 1` whole substeps, then (`tail`) the force phase of the substep whose rebuild comes next.
+int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk, int len, int parity, int tail) {
+    const KArgs& a = c->args;
+    int rc;
+    for (int q = 0; q < len; q++) {
+        if (q > 0 && (rc = launch_forces(a, c->stream, parity, t))) return rc;
+        if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
+        if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p))) return rc;
+        parity ^= 1;
+    }
+    if (tail && (rc = launch_forces(a, c->stream, parity, t))) return rc;
+    return BL_OK;
+}
+
+// One segment as a graph launch: captured once per (length, parity, tail) for the lists' buffers of the moment;
+// the narrow phase covers the lists' capacities and reads the rebuild's counts on the device.
+int launch_segment(bl_ctx* c, const bl_tuning& t, int len, int tail) {
+    PArgs p;
+    uint32_t nblk;
+    int rc;
+    if ((rc = c->contacts.collide_fixed(p, nblk))) return rc;
+    if (nblk != c->seg_nblk || std::memcmp(&p, &c->seg_p, sizeof(PArgs)) != 0) {
+        c->drop_segments();
+        c->seg_p = p, c->seg_nblk = nblk;
+    }
+    hipGraphExec_t ex = nullptr;
+    for (const auto& g : c->segs)
+        if (g.len == len && g.parity == c->parity && g.tail == tail) ex = g.ex;
+    if (!ex) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_segment(c, t, p, nblk, len, c->parity, tail);
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        HIPCHK(ce);
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ie);
+        c->segs.push_back({len, c->parity, tail, ex});
+    }
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+    c->contacts.count_tests(len);
+    c->parity ^= (len & 1);
+    return BL_OK;
+}
+
 // A frame with inter-body contacts (World::simulate_island, src/phys/world.cpp:3211-3591): per substep the forces
 // into F, the pair lists rebuilt at the first substep and every `interval` after (from the state at that point),
 // the narrow phase and the response on F, then the static contacts and the integration. The host waits at each
-// rebuild (list sizes) - about one substep in 17 at the default 120 Hz pair rate.
+// rebuild (list sizes) - about one substep in 17 at the default 120 Hz pair rate. With the graph form the substeps
+// between two rebuilds go as one graph launch; `kev` (per-launch events around the force phase) takes the
+// launches one by one.
 int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     bl_tuning t = c->tune;
     t.block = c->built_block, t.incidences_per_thread = c->built_per;
@@ -636,33 +698,46 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     }
     HIPCHK(hipMemcpyAsync(c->cbox.p, w0.data(), sizeof(unsigned int) * w0.size(), hipMemcpyHostToDevice, c->stream));
     if ((rc = launch_frame_begin(a, c->stream, c->parity))) return rc;
-    int next = 0;
-    for (int s = 0; s < substeps; s++) {
-        if (kev) HIPCHK(hipEventRecord(kev[2 * s], c->stream));
-        if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
-        if (kev) HIPCHK(hipEventRecord(kev[2 * s + 1], c->stream));
-        if (s == 0 || s >= next) {
-            // each body's top speed so far (the frame's start included) and, after the first substep, its bounds now
-            if (s > 0 && (rc = launch_box(ab, c->stream, c->parity))) return rc;
-            if ((rc = c->contacts.rebuild(a, c->cbox.p, c->stream, c->parity))) {
-                g_last_error = "pair rebuild failed";
-                return rc;
-            }
-            if (c->contacts.last.unsupported) {
-                g_last_error = (c->contacts.last.unsupported & 1)
-                                   ? "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not "
-                                     "stepped on the GPU)"
-                                   : "a capsule meets more than 128 triangles in one pair rebuild";
-                return BL_ERR_UNSUPPORTED;
-            }
-            next = s + interval;
-        }
-        if ((rc = c->contacts.collide(a, c->stream, c->parity))) {
-            g_last_error = "contact narrow phase or response failed";
+    const bool graph = !kev && t.use_graph;
+    if (kev) HIPCHK(hipEventRecord(kev[0], c->stream));
+    if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
+    if (kev) HIPCHK(hipEventRecord(kev[1], c->stream));
+    for (int s = 0; s < substeps;) {
+        // each body's top speed so far (the frame's start included) and, after the first substep, its bounds now
+        if (s > 0 && (rc = launch_box(ab, c->stream, c->parity))) return rc;
+        if ((rc = c->contacts.rebuild(a, c->cbox.p, c->stream, c->parity))) {
+            g_last_error = "pair rebuild failed";
             return rc;
         }
-        if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p))) return rc;
-        c->parity ^= 1;
+        if (c->contacts.last.unsupported) {
+            g_last_error = (c->contacts.last.unsupported & 1)
+                               ? "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not "
+                                 "stepped on the GPU)"
+                               : "a capsule meets more than 128 triangles in one pair rebuild";
+            return BL_ERR_UNSUPPORTED;
+        }
+        const int next = std::min(s + interval, substeps);
+        if (graph) {
+            if ((rc = launch_segment(c, t, next - s, next < substeps))) {
+                g_last_error = "contact segment capture or launch failed";
+                return rc;
+            }
+            s = next;
+            continue;
+        }
+        for (; s < next; s++) {
+            if ((rc = c->contacts.collide(a, c->stream, c->parity))) {
+                g_last_error = "contact narrow phase or response failed";
+                return rc;
+            }
+            if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p))) return rc;
+            c->parity ^= 1;
+            if (s + 1 < substeps) {
+                if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1)], c->stream));
+                if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
+                if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1) + 1], c->stream));
+            }
+        }
     }
     c->substeps += substeps;
     if ((rc = launch_wstat_fold(a, c->stream, c->wstat.p)) || (rc = launch_frame_end(a, c->stream, c->parity))) return rc;

@@ -92,7 +92,8 @@ struct PArgs {
     const uint4* nt; // node body, triangle body, node, triangle
     const uint4* nc; // node body, capsule body, node, capsule
     const uint4* ct; // capsule body, triangle body, capsule, triangle
-    uint32_t nnt, nnc, nct;
+    uint32_t nnt, nnc, nct; // (the kernels take them from st, the last rebuild's counts on the device)
+    const RStat* st;
     Hit* hraw;       // per block of 256 candidates: its hits first, in order
     uint32_t* bcnt;  // per block: its hit count
     uint32_t* bpre;  // per block: the hits before it (the response fills it)
@@ -148,6 +149,12 @@ public:
     // a.body_frame; waits for the device once (and again when a list outgrew its buffer)
     int rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t st, int parity);
     int collide(const KArgs& a, hipStream_t st, int parity);
+    // The narrow phase and response sized for the lists' capacities, the counts read on the device: launches that
+    // stay valid from one rebuild to the next (for a captured graph) until a list grows. `p` and `nblk` identify
+    // them; the host's test counter is advanced by `count_tests`.
+    int collide_fixed(PArgs& p, uint32_t& nblk);
+    int launch_collide(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity);
+    void count_tests(int64_t substeps) { narrow_tests += substeps * (int64_t)(nnt + nnc + nct); }
     void release();
     ~ContactEngine() { release(); }
 

@@ -849,8 +849,10 @@ __global__ void k_rb_near(KArgs a, CArgs c, int write) {
 // pair_detect (src/phys/world.cpp:2807-2962) on the concatenated candidates (node-triangle, node-capsule,
 // capsule-triangle), 256 per block: each block's hits packed to the front of its 256 slots, in candidate order,
 // and its count
-__global__ __launch_bounds__(256) void k_narrow(KArgs a, PArgs p, int parity) {
+__global__ __launch_bounds__(256) void k_narrow(KArgs a, PArgs pin, int parity) {
     __shared__ uint32_t wcnt[4];
+    PArgs p = pin;
+    p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = p.nnt + p.nnc + p.nct;
     const float4* P = a.pm[parity];
@@ -984,63 +986,73 @@ struct ContactNodes {
     float fric;
 };
 
+// Every branch sets all five slots (the unused ones to node 0, weight 0): stores of a slot that only some branches
+// make are merged by the compiler into one store through a selected address, which puts the slots in scratch memory.
+__device__ __forceinline__ void set_slots(ContactNodes& cn, uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3,
+                                          uint32_t n4, float w0, float w1, float w2, float w3, float w4) {
+    cn.n[0] = n0, cn.n[1] = n1, cn.n[2] = n2, cn.n[3] = n3, cn.n[4] = n4;
+    cn.w[0] = w0, cn.w[1] = w1, cn.w[2] = w2, cn.w[3] = w3, cn.w[4] = w4;
+}
+
 __device__ __forceinline__ void contact_nodes(const KArgs& a, const PArgs& p, const Hit& h, ContactNodes& cn) {
     if (h.pair < p.nnt) {
         const uint4 pr = p.nt[h.pair];
         const uint4 t = a.tri_abc[pr.w];
         cn.na = 1, cn.nb = 3;
-        cn.n[0] = pr.z, cn.w[0] = 1.0f;
-        cn.n[1] = t.x, cn.n[2] = t.y, cn.n[3] = t.z;
-        cn.w[1] = h.bary.x, cn.w[2] = h.bary.y, cn.w[3] = h.bary.z;
+        set_slots(cn, pr.z, t.x, t.y, t.z, 0u, 1.0f, h.bary.x, h.bary.y, h.bary.z, 0.0f);
         // (a tyre node would take its own friction: tyre nodes are refused when a body is added)
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
     } else if (h.pair < p.nnt + p.nnc) {
         const uint4 pr = p.nc[h.pair - p.nnt];
         const uint4 cap = a.cap_data[pr.w];
         cn.na = 1, cn.nb = 2;
-        cn.n[0] = pr.z, cn.w[0] = 1.0f;
-        cn.n[1] = cap.x, cn.n[2] = cap.y;
-        cn.w[1] = 1 - h.s, cn.w[2] = h.s;
+        set_slots(cn, pr.z, cap.x, cap.y, 0u, 0u, 1.0f, 1 - h.s, h.s, 0.0f, 0.0f);
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
     } else {
         const uint4 pr = p.ct[h.pair - p.nnt - p.nnc];
         const uint4 cap = a.cap_data[pr.z];
         const uint4 t = a.tri_abc[pr.w];
         cn.na = 2, cn.nb = 3;
-        cn.n[0] = cap.x, cn.n[1] = cap.y;
-        cn.w[0] = 1 - h.s, cn.w[1] = h.s;
-        cn.n[2] = t.x, cn.n[3] = t.y, cn.n[4] = t.z;
-        cn.w[2] = h.bary.x, cn.w[3] = h.bary.y, cn.w[4] = h.bary.z;
+        set_slots(cn, cap.x, cap.y, t.x, t.y, t.z, 1 - h.s, h.s, h.bary.x, h.bary.y, h.bary.z);
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
     }
 }
 
-// contact_force<NA, NB> (src/phys/world.cpp:59-91), both sides movable, no force snapshots
+// contact_force<NA, NB> (src/phys/world.cpp:59-91), both sides movable, no force snapshots. The side a nodes
+// come first (slots 0 .. na - 1), then side b's; the loops run over the five slots unrolled, so that the slots
+// stay in registers (a runtime-indexed slot array would live in scratch memory).
 __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn, f3 nrm, float pen) {
     const float4* P = a.pm[parity];
     const float4* Vv = a.vm[parity];
+    const int na = cn.na, nn = cn.na + cn.nb;
     float W = 0;
-    for (int i = 0; i < cn.na + cn.nb; i++) W += cn.w[i] * cn.w[i] * P[cn.n[i]].w;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if (i < nn) W += cn.w[i] * cn.w[i] * P[cn.n[i]].w;
     if (W <= 1e-12f) return false;
     const float m = 1.0f / W;
     f3 va = mk(0, 0, 0), vb = mk(0, 0, 0), acc = mk(0, 0, 0);
-    for (int i = 0; i < cn.na; i++) {
-        va = add(va, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
-        acc = add(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
-    }
-    for (int i = cn.na; i < cn.na + cn.nb; i++) {
-        vb = add(vb, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
-        acc = sub(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        if (i < na) {
+            va = add(va, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
+            acc = add(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
+        } else if (i < nn) {
+            vb = add(vb, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
+            acc = sub(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
+        }
     }
     const f3 f_rel = mul(acc, m);
     const f3 f = primitive_collision(f_rel, sub(va, vb), m, nrm, a.dt, k_gm_c[BL_SURF_CONCRETE], pen, cn.fric, 2.0f, 0.0f, -1.0f);
-    for (int i = 0; i < cn.na; i++) {
-        const float4 F = a.F[cn.n[i]];
-        a.F[cn.n[i]] = make_float4(F.x + f.x * cn.w[i], F.y + f.y * cn.w[i], F.z + f.z * cn.w[i], F.w);
-    }
-    for (int i = cn.na; i < cn.na + cn.nb; i++) {
-        const float4 F = a.F[cn.n[i]];
-        a.F[cn.n[i]] = make_float4(F.x - f.x * cn.w[i], F.y - f.y * cn.w[i], F.z - f.z * cn.w[i], F.w);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        if (i < na) {
+            const float4 F = a.F[cn.n[i]];
+            a.F[cn.n[i]] = make_float4(F.x + f.x * cn.w[i], F.y + f.y * cn.w[i], F.z + f.z * cn.w[i], F.w);
+        } else if (i < nn) {
+            const float4 F = a.F[cn.n[i]];
+            a.F[cn.n[i]] = make_float4(F.x - f.x * cn.w[i], F.y - f.y * cn.w[i], F.z - f.z * cn.w[i], F.w);
+        }
     }
     return true;
 }
@@ -1050,11 +1062,13 @@ __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn
 // the window on each of its nodes), the longest chain to it its level; the levels run one after another, the hits
 // of a level (no shared node) in parallel.
 constexpr int kRespW = 1024, kRespKeys = 8192;
-__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parity, uint32_t nblk) {
+__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int parity, uint32_t nblk) {
     __shared__ uint64_t keys[kRespKeys];
     __shared__ int16_t preds[kRespW * 5];
     __shared__ uint16_t level[kRespW];
     __shared__ uint32_t sred[kRespW / 64];
+    PArgs p = pin;
+    p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // exclusive prefix of the block counts
     uint32_t carry = 0;
@@ -1101,6 +1115,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs p, int parit
         }
         for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
         __syncthreads();
+#pragma unroll
         for (int j = 0; j < 5; j++) {
             if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
             preds[tid * 5 + j] = -1;
@@ -1478,8 +1493,25 @@ int ContactEngine::collide(const KArgs& a, hipStream_t st, int parity) {
     RC(bpre.ensure((size_t)nblk + 1));
     PArgs p{};
     p.nt = nt.p, p.nc = nc.p, p.ct = ct.p;
-    p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct;
+    p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct, p.st = stat.p;
     p.hraw = hraw.p, p.bcnt = bcnt.p, p.bpre = bpre.p;
+    return launch_collide(a, p, nblk, st, parity);
+}
+
+int ContactEngine::collide_fixed(PArgs& p, uint32_t& nblk) {
+    const size_t cap = (size_t)cap_nt + cap_nc + cap_ct;
+    nblk = (uint32_t)((cap + 255) / 256);
+    if (nblk == 0) nblk = 1;
+    RC(hraw.ensure((size_t)nblk * 256));
+    RC(bcnt.ensure(nblk));
+    RC(bpre.ensure((size_t)nblk + 1));
+    p = PArgs{};
+    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.st = stat.p;
+    p.hraw = hraw.p, p.bcnt = bcnt.p, p.bpre = bpre.p;
+    return BL_OK;
+}
+
+int ContactEngine::launch_collide(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity) {
     hipLaunchKernelGGL(k_narrow, dim3(nblk), dim3(256), 0, st, a, p, parity);
     hipLaunchKernelGGL(k_response, dim3(1), dim3(kRespW), 0, st, a, p, parity, nblk);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;

@@ -728,6 +728,7 @@ struct SolArgs {
     const uint32_t* epoch;
     int* err;
     uint32_t spin_limit;
+    const int* skip;           // (or null) nonzero: every task returns at once (a converged iteration's solve)
 };
 
 // lane l's value of v (l uniform across the wave)
@@ -765,7 +766,7 @@ __global__ void k_epoch(uint32_t* e) {
 
 // the right side of every front of the level: own rows from b, zero below, plus the children's updates in order
 __global__ __launch_bounds__(256) void k_fgather(SolArgs s, const uint32_t* list, uint32_t n) {
-    if (blockIdx.x >= n) return;
+    if (blockIdx.x >= n || (s.skip && *s.skip)) return;
     const uint32_t fi = list[blockIdx.x];
     const Front fr = s.fr[fi];
     double* V = s.V + fr.voff;
@@ -780,14 +781,16 @@ __global__ __launch_bounds__(256) void k_fgather(SolArgs s, const uint32_t* list
     }
 }
 
-// task = (front, row block): rows R .. R + 64 of the front
+// task = (front, row block): rows R .. R + 64 of the front. What does not depend on the other blocks is read ahead
+// of the waits: the diagonal block into LDS first, and each column block's tile into registers before its flag is
+// awaited, so that a block's turn on the chain is the flag, the block's y, its products and the diagonal solve.
 __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, uint32_t n) {
 #pragma clang fp contract(fast)
-    __shared__ double ys[64];
+    __shared__ double ys[2][64];
     __shared__ double red[4][64];
     __shared__ double Ld[64][65];
     __shared__ int sbad;
-    if (blockIdx.x >= n) return;
+    if (blockIdx.x >= n || (s.skip && *s.skip)) return;
     const uint2 t = tasks[blockIdx.x];
     const Front fr = s.fr[t.x];
     const uint32_t m = fr.m, c = fr.c, R = t.y * 64, nrow = m - R < 64u ? m - R : 64u;
@@ -795,24 +798,7 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
     const uint32_t epoch = *s.epoch;
     const double* F = s.F + fr.off_l;
     double* V = s.V + fr.voff;
-    const double vinit = lane < nrow ? V[R + lane] : 0.0;
-    double pacc = 0;
-    const uint32_t ncol = R < c ? R : c, nblk = (ncol + 63) / 64;
     if (tid == 0) sbad = 0;
-    for (uint32_t jb = 0; jb < nblk; jb++) {
-        if (tid == 0 && !sbad && !wait_flag(s, &s.fflag[fr.fb + jb], epoch)) sbad = 1;
-        __syncthreads();
-        if (sbad) break;
-        const uint32_t J = jb * 64, w = c - J < 64u ? c - J : 64u;
-        if (tid < w) ys[tid] = ld_dev(&V[J + tid]);
-        __syncthreads();
-        if (lane < nrow) {
-            const uint32_t j0 = part * 16, j1 = j0 + 16 < w ? j0 + 16 : w;
-            for (uint32_t j = j0; j < j1; j++) pacc += F[(uint64_t)(J + j) * m + R + lane] * ys[j];
-        }
-        __syncthreads();
-    }
-    red[part][lane] = pacc;
     if (R < c) {
         const uint32_t w = c - R < 64u ? c - R : 64u;
         for (uint32_t e = tid; e < 64 * 64; e += 256) {
@@ -820,6 +806,27 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
             Ld[i][j] = (i < nrow && j < w && i >= j) ? F[(uint64_t)(R + j) * m + R + i] : 0.0;
         }
     }
+    const double vinit = lane < nrow ? V[R + lane] : 0.0;
+    double pacc = 0;
+    const uint32_t ncol = R < c ? R : c, nblk = (ncol + 63) / 64, j0 = part * 16;
+    for (uint32_t jb = 0; jb < nblk; jb++) {
+        const uint32_t J = jb * 64, w = c - J < 64u ? c - J : 64u;
+        double fv[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) fv[q] = (lane < nrow && j0 + q < w) ? F[(uint64_t)(J + j0 + q) * m + R + lane] : 0.0;
+        if (tid == 0 && !sbad && !wait_flag(s, &s.fflag[fr.fb + jb], epoch)) sbad = 1;
+        __syncthreads();
+        if (sbad) break;
+        // (two y buffers: a wave still in the last block's products reads the other one)
+        if (tid < w) ys[jb & 1][tid] = ld_dev(&V[J + tid]);
+        __syncthreads();
+        if (lane < nrow) {
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                if (j0 + q < w) pacc += fv[q] * ys[jb & 1][j0 + q];
+        }
+    }
+    red[part][lane] = pacc;
     __syncthreads();
     if (part == 0 && !sbad) {
         double acc = vinit - (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
@@ -842,14 +849,16 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
     if (tid == 0 && R < c && !sbad) __hip_atomic_store(&s.fflag[fr.fb + t.y], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// task = (front, own column block jb), the front's last block first
+// task = (front, own column block jb), the front's last block first; the diagonal block and each later block's tile
+// read ahead of the waits, as in the forward pass
 __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, uint32_t n) {
 #pragma clang fp contract(fast)
     __shared__ double xs[256];
+    __shared__ double xb[2][64];
     __shared__ double dots[64];
     __shared__ double Ld[64][65];
     __shared__ int sbad;
-    if (blockIdx.x >= n) return;
+    if (blockIdx.x >= n || (s.skip && *s.skip)) return;
     const uint2 t = tasks[blockIdx.x];
     const Front fr = s.fr[t.x];
     const uint32_t m = fr.m, c = fr.c, J = t.y * 64, w = c - J < 64u ? c - J : 64u;
@@ -858,6 +867,10 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     const double* F = s.F + fr.off_l;
     double* X = s.X + fr.voff;
     if (tid == 0) sbad = 0;
+    for (uint32_t e = tid; e < 64 * 64; e += 256) {
+        const uint32_t i = e % 64, j = e / 64;
+        Ld[i][j] = (i < w && j < w && i >= j) ? F[(uint64_t)(J + j) * m + J + i] : 0.0;
+    }
     // this wave's 16 columns: partial sums of L[r][j] x_r per lane
     double pj[16];
 #pragma unroll
@@ -878,18 +891,23 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     // the later own blocks, the last first, as their flags come up
     const uint32_t nb = (c + 63) / 64;
     for (uint32_t kb = nb; kb-- > t.y + 1;) {
+        const uint32_t K = kb * 64, wk = c - K < 64u ? c - K : 64u;
+        double fv[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint32_t j = part * 16 + q;
+            fv[q] = (j < w && lane < wk) ? F[(uint64_t)(J + j) * m + K + lane] : 0.0;
+        }
         if (tid == 0 && !sbad && !wait_flag(s, &s.bflag[fr.fb + kb], epoch)) sbad = 1;
         __syncthreads();
         if (sbad) break;
-        const uint32_t K = kb * 64, wk = c - K < 64u ? c - K : 64u;
-        if (tid < wk) xs[tid] = ld_dev(&X[K + tid]);
+        if (tid < wk) xb[kb & 1][tid] = ld_dev(&X[K + tid]);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             const uint32_t j = part * 16 + q;
-            if (j < w && lane < wk) pj[q] += F[(uint64_t)(J + j) * m + K + lane] * xs[lane];
+            if (j < w && lane < wk) pj[q] += fv[q] * xb[kb & 1][lane];
         }
-        __syncthreads();
     }
     // one sum per column (a fixed tree over the lanes)
 #pragma unroll
@@ -897,10 +915,6 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
         double v = pj[q];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
         if (lane == 0) dots[part * 16 + q] = v;
-    }
-    for (uint32_t e = tid; e < 64 * 64; e += 256) {
-        const uint32_t i = e % 64, j = e / 64;
-        Ld[i][j] = (i < w && j < w && i >= j) ? F[(uint64_t)(J + j) * m + J + i] : 0.0;
     }
     __syncthreads();
     if (part == 0 && !sbad) {
@@ -1063,6 +1077,7 @@ struct bl_fem {
     hipEvent_t ev[5] = {};
     uint32_t nn = 0, nt = 0, nf = 0, npairs = 0;
     uint64_t serial = 0; // unique per engine in the process (an address can be reused after a destroy)
+    bool factor_ok = false; // the factor buffer holds the last successful factorization (of an earlier system)
     Section sec{};
     // state and topology
     Dv<double> xd;
@@ -1079,6 +1094,8 @@ struct bl_fem {
     // the factorization
     Dv<Front> fronts;
     Dv<double> F, Lbuf, V, Xs;
+    Dv<double> Lkeep;     // a copy of the factor (kept across a factorization that must not replace it)
+    bool kept = false;
     Dv<uint32_t> fflag, bflag, epoch;
     Dv<uint2> fs_tasks, bs_tasks;
     Dv<Scat> scat;
@@ -1538,11 +1555,13 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     return BL_OK;
 }
 
-// L L^T x = b (b on the device, 6 per free index; x into f->x): forward by levels from the leaves, backward from the
-// root
-void enqueue_solve(bl_fem* f, const double* b) {
+// L L^T x = b (b on the device, 6 per free index; x into `out`, f->x when null): forward by levels from the leaves,
+// backward from the root. With `skip`, every launch returns at once when *skip is set on the device.
+void enqueue_solve(bl_fem* f, const double* b, double* out = nullptr, const int* skip = nullptr) {
     const hipStream_t st = f->st;
     SolArgs so = f->sol(b);
+    if (out) so.x = out;
+    so.skip = skip;
     hipLaunchKernelGGL(k_epoch, dim3(1), dim3(64), 0, st, f->epoch.p);
     for (size_t L = 0; L < f->plan.size(); L++) {
         const bl_fem::LevelPlan& lp = f->plan[L];
@@ -1625,9 +1644,43 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
 }
 
 int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail) {
+    f->factor_ok = false;
     int rc = fem_factor_solve(f, ms_factor, ms_solve);
     if (rc) return rc;
     FCHK(hipMemcpy(fail, f->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+    f->factor_ok = *fail == 0;
+    return BL_OK;
+}
+
+bool fem_has_factor(bl_fem* f) { return f->factor_ok; }
+
+int fem_keep_factor(bl_fem* f) {
+    f->kept = false;
+    if (!f->factor_ok || !f->Lbuf.n) return BL_OK;
+    int rc;
+    if (f->Lkeep.n != f->Lbuf.n && (rc = f->Lkeep.alloc(f->Lbuf.n))) return rc;
+    FCHK(hipMemcpyAsync(f->Lkeep.p, f->Lbuf.p, sizeof(double) * f->Lbuf.n, hipMemcpyDeviceToDevice, f->st));
+    f->kept = true;
+    return BL_OK;
+}
+
+int fem_restore_factor(bl_fem* f) {
+    if (!f->kept) return BL_OK;
+    FCHK(hipMemcpyAsync(f->Lbuf.p, f->Lkeep.p, sizeof(double) * f->Lbuf.n, hipMemcpyDeviceToDevice, f->st));
+    f->kept = false;
+    f->factor_ok = true;
+    return BL_OK;
+}
+
+int fem_precond(bl_fem* f, const double* r, double* z, const int* skip) {
+    enqueue_solve(f, r, z, skip);
+    FCHK(hipGetLastError());
+    return BL_OK;
+}
+
+int fem_solve_error(bl_fem* f, int* err) {
+    FCHK(hipMemcpyAsync(err, f->fail.p, sizeof(int), hipMemcpyDeviceToHost, f->st));
+    FCHK(hipStreamSynchronize(f->st));
     return BL_OK;
 }
 

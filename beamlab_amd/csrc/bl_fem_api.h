@@ -29,6 +29,16 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
                  float* ms);
 // x := A^-1 rhs by the supernodal Cholesky; *fail: bit 1 a pivot not positive, bit 2 a wait timed out
 int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail);
+// whether the factor buffer holds a successful factorization (of this step's system or an earlier one)
+bool fem_has_factor(bl_fem* f);
+// a copy of the factor, put back by fem_restore_factor (around a factorization that must leave it as it was)
+int fem_keep_factor(bl_fem* f);
+int fem_restore_factor(bl_fem* f);
+// z := (L L^T)^-1 r with that factor, enqueued on the engine's stream; every launch returns at once while *skip is
+// set on the device (r and z 6 per free node)
+int fem_precond(bl_fem* f, const double* r, double* z, const int* skip);
+// the substitutions' error word (nonzero: a wait timed out), after the stream drains
+int fem_solve_error(bl_fem* f, int* err);
 // || A x - rhs || / || rhs || (waits for the stream)
 int fem_residual(bl_fem* f, const double* x, double* rel);
 // y := A x (enqueued on the engine's stream)

@@ -1,6 +1,7 @@
 // The learned warm start and preconditioner of the implicit FEM solve: a 4-layer message-passing network over the
 // mesh's free nodes, its training and inference on the bfloat16 matrix cores (FP32 accumulation), and the PCG solve
-// it feeds, with the supernodal Cholesky as the fallback and the judge.
+// it feeds, with the supernodal Cholesky as the fallback and the yardstick of its error. The PCG's preconditioner is
+// block Jacobi, block Jacobi scaled by the network, or the engine's last Cholesky factor (factor reuse).
 //
 // The network (no counterpart in the reference; its math is stated once more, in NumPy, by the oracle):
 //   features f_i (32 wide, 18 used): t_i / tau, xprev_i / sigma, b_i / beta, with t_i = D_i^-1 b_i (D_i the node's
@@ -519,6 +520,27 @@ __global__ void k_pcg_scalar(const double* part, uint32_t nblk, int stride, int 
     if (b <= sc[4] * sc[3]) *done = 1;
 }
 
+// The scalars of PCG with the factor as its preconditioner (the residual tested before the preconditioner is
+// applied, so a converged iteration skips its solve). mode 0: rz0; 1: alpha = rz / p.Ap (one more iteration);
+// 2: ||r||^2 and the test; 3: beta = rz' / rz. sc as k_pcg_scalar's.
+__global__ void k_fac_scalar(const double* part, uint32_t nblk, int mode, double* sc, int* done, int* iters) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || *done) return;
+    double a = 0;
+    for (uint32_t k = 0; k < nblk; k++) a += part[k];
+    if (mode == 0) {
+        sc[0] = a;
+    } else if (mode == 1) {
+        sc[1] = a != 0 ? sc[0] / a : 0.0;
+        *iters += 1;
+    } else if (mode == 2) {
+        sc[6] = a;
+        if (a <= sc[4] * sc[3]) *done = 1;
+    } else {
+        sc[2] = sc[0] != 0 ? a / sc[0] : 0.0;
+        sc[0] = a;
+    }
+}
+
 // x += alpha p, r -= alpha Ap
 __global__ void k_pcg_xr(uint32_t n6, const double* sc, double* x, double* r, const double* p, const double* Ap, const int* done) {
     if (*done) return;
@@ -824,6 +846,54 @@ struct bl_gnn {
         *converged = hv[0], *iters = hv[1];
         return BL_OK;
     }
+
+    // PCG preconditioned by the engine's last factorization: z = (L L^T)^-1 r by the supernodal substitutions. All
+    // max_iter iterations are enqueued at once; after the residual test passes every launch returns at once.
+    int pcg_factor(bl_fem* fm, double* x, int max_iter, double tol, int* iters, int* converged) {
+        const blf::FemView v = blf::fem_view(fm);
+        const hipStream_t st = v.st;
+        const uint32_t n6 = 6 * n, nb6 = (n6 + 255) / 256;
+        double* sc = pcg_sc.p;
+        int* done = pcg_flag.p;
+        int* it = pcg_flag.p + 1;
+        int rc;
+        hipLaunchKernelGGL(k_clear_i, dim3(1), dim3(64), 0, st, pcg_flag.p, 2u);
+        hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, v.rhs, v.rhs, pcg_part.p, done);
+        hipLaunchKernelGGL(k_norm_init, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, sc, tol * tol);
+        hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)x, pcg_Ap.p, done);
+        hipLaunchKernelGGL(k_sub, grid(n6), dim3(256), 0, st, n6, v.rhs, pcg_Ap.p, pcg_r.p);
+        hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_r.p, pcg_part.p, done);
+        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 2, sc, done, it);
+        if ((rc = blf::fem_precond(fm, pcg_r.p, pcg_z.p, done))) return rc;
+        hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_z.p, pcg_part.p, done);
+        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 0, sc, done, it);
+        hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 1, done);
+        for (int k = 0; k < max_iter; k++) {
+            hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)pcg_p.p, pcg_Ap.p, done);
+            hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_p.p, pcg_Ap.p, pcg_part.p, done);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 1, sc, done, it);
+            hipLaunchKernelGGL(k_pcg_xr, grid(n6), dim3(256), 0, st, n6, sc, x, pcg_r.p, pcg_p.p, pcg_Ap.p, done);
+            hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_r.p, pcg_part.p, done);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 2, sc, done, it);
+            if (k + 1 == max_iter) break; // (the last iteration's search direction is never used)
+            if ((rc = blf::fem_precond(fm, pcg_r.p, pcg_z.p, done))) return rc;
+            hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_z.p, pcg_part.p, done);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 3, sc, done, it);
+            hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 0, done);
+        }
+        GCHK(hipGetLastError());
+        int hv[2];
+        GCHK(hipMemcpyAsync(hv, pcg_flag.p, sizeof(hv), hipMemcpyDeviceToHost, st));
+        GCHK(hipStreamSynchronize(st));
+        int err = 0;
+        if ((rc = blf::fem_solve_error(fm, &err))) return rc;
+        if (err) {
+            blg::set_last_error("a substitution's wait for a block timed out");
+            return BL_ERR_STATE;
+        }
+        *converged = hv[0], *iters = hv[1];
+        return BL_OK;
+    }
 };
 
 // ============================================================================================== C ABI
@@ -929,7 +999,10 @@ int bl_gnn_infer(bl_gnn* g, bl_fem* f, double* x0, double* s) {
 int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float theta, float dissipation, const float gr[3],
                     const bl_pcg_opts* o, bl_pcg_report* rep) {
     if (!f || !g || !gr || !o || !(h > 0) || o->max_iter < 0 || !(o->tol > 0)) return BL_ERR_ARG;
+    if (o->precond < BL_PRECOND_BLOCK_JACOBI || o->precond > BL_PRECOND_FACTOR || o->warm < BL_WARM_ZERO || o->warm > BL_WARM_NETWORK)
+        return BL_ERR_ARG;
     const bool net_warm = o->warm == BL_WARM_NETWORK, net_prec = o->precond == BL_PRECOND_NETWORK;
+    const bool fac_prec = o->precond == BL_PRECOND_FACTOR;
     int rc;
     bl_pcg_report r{};
     float ms = 0;
@@ -941,7 +1014,7 @@ int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float the
     // the warm start and the preconditioner
     GCHK(hipEventRecord(g->ev[0], st));
     if (net_warm || net_prec) g->infer(f);
-    else g->features(st, v); // (the block inverses)
+    else if (!fac_prec) g->features(st, v); // (the block inverses)
     if (o->warm == BL_WARM_ZERO) hipLaunchKernelGGL(k_zero_d, grid(n6), dim3(256), 0, st, v.x, n6);
     else if (net_warm) hipLaunchKernelGGL(k_copy_d, grid(n6), dim3(256), 0, st, (const double*)g->x0.p, v.x, n6);
     // (BL_WARM_PREVIOUS: the engine's x still holds the last step's solution)
@@ -952,19 +1025,30 @@ int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float the
     }
     GCHK(hipEventRecord(g->ev[1], st));
     int iters = 0, conv = 0;
-    if ((rc = g->pcg(f, v.x, net_prec ? g->s.p : nullptr, o->max_iter, o->tol, &iters, &conv))) return rc;
+    const bool had_factor = blf::fem_has_factor(f);
+    if (fac_prec) {
+        // (no factor yet: the step is a Cholesky step)
+        if (had_factor && (rc = g->pcg_factor(f, v.x, o->max_iter, o->tol, &iters, &conv))) return rc;
+    } else if ((rc = g->pcg(f, v.x, net_prec ? g->s.p : nullptr, o->max_iter, o->tol, &iters, &conv))) {
+        return rc;
+    }
     GCHK(hipEventRecord(g->ev[2], st));
     GCHK(hipEventSynchronize(g->ev[2]));
     GCHK(hipEventElapsedTime(&r.gnn_ms, g->ev[0], g->ev[1]));
     GCHK(hipEventElapsedTime(&r.pcg_ms, g->ev[1], g->ev[2]));
     r.iterations = iters, r.converged = conv;
     std::vector<double> hxp;
-    if (o->check_error || (!conv && o->fallback)) {
-        // the supernodal Cholesky: the fallback, and the judge of the warm start's and the PCG's error
+    // the fallback: PCG missed tol (or, preconditioned by the factor, there was none yet)
+    const bool fallback = !conv && (o->fallback || (fac_prec && !had_factor));
+    if (o->check_error || fallback) {
+        // the supernodal Cholesky: the fallback, and the yardstick of the warm start's and the PCG's error (then
+        // the factor the PCG used stays in force)
+        const bool keep = !fallback && fac_prec;
         if (o->check_error) {
             hxp.resize(n6);
             GCHK(hipMemcpy(hxp.data(), v.x, sizeof(double) * n6, hipMemcpyDeviceToHost));
         }
+        if (keep && (rc = blf::fem_keep_factor(f))) return rc;
         float fms = 0, sms = 0;
         int fail = 0;
         if ((rc = blf::fem_cholesky(f, &fms, &sms, &fail))) return rc;
@@ -973,14 +1057,15 @@ int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float the
             blg::set_last_error("a pivot of the factorization is not positive");
             return BL_ERR_STATE;
         }
+        if (keep && (rc = blf::fem_restore_factor(f))) return rc;
         if (o->check_error) {
             std::vector<double> hs(n6);
             GCHK(hipMemcpy(hs.data(), v.x, sizeof(double) * n6, hipMemcpyDeviceToHost));
             r.warm_error = rel_err(hx0, hs);
             r.pcg_error = rel_err(hxp, hs);
-            if (conv || !o->fallback) GCHK(hipMemcpy(v.x, hxp.data(), sizeof(double) * n6, hipMemcpyHostToDevice));
+            if (!fallback) GCHK(hipMemcpy(v.x, hxp.data(), sizeof(double) * n6, hipMemcpyHostToDevice));
         }
-        r.fallback_used = (!conv && o->fallback) ? 1 : 0;
+        r.fallback_used = fallback ? 1 : 0;
     }
     if ((rc = blf::fem_residual(f, v.x, &r.residual))) return rc;
     if ((rc = blf::fem_update(f, &r.update_ms))) return rc;

@@ -194,15 +194,18 @@ __device__ __forceinline__ void node_finish(const KArgs& a, uint32_t info, float
         v = add(v, mul(f, inv_mass * a.dt));
         float v2 = dot(v, v);
         const float kmax = 400.0f;
-        if (v2 > kmax * kmax) {
+        // (the counts as additions of the tests: an increment in each branch compiles to one store through a
+        // selected address, which puts the counts in scratch memory)
+        const bool clamp = v2 > kmax * kmax, nan = !clamp && !(v2 >= 0.0f);
+        if (clamp) {
             v = mul(v, kmax / fsqrt(v2));
             v2 = kmax * kmax;
-            cnt.clamps++;
-        } else if (!(v2 >= 0.0f)) {
+        } else if (nan) {
             v = mk(0, 0, 0);
             v2 = 0;
-            cnt.nans++;
         }
+        cnt.clamps += clamp ? 1u : 0u;
+        cnt.nans += nan ? 1u : 0u;
         p = add(p, mul(v, a.dt));
         cnt.max_v2 = v2 > cnt.max_v2 ? v2 : cnt.max_v2;
     }

@@ -38,9 +38,9 @@ FEM_H = 0.0005  # the reference's step (kDefaultDt, src/phys/softbody.h:25), the
 METRIC = "beam updates/s at 2 kHz substep; implicit FEM solves/s at 1e-6 residual"
 FRAME = 33
 # the forms the contact step can take: its force kernel's workgroup size and beam ends per thread
-CONTACT_CANDIDATES = [dict(mode=1, block=256, incidences_per_thread=p, substeps_per_launch=0, use_graph=0)
-                      for p in (1, 2, 4, 8)] + \
-                     [dict(mode=1, block=512, incidences_per_thread=p, substeps_per_launch=0, use_graph=0) for p in (2, 4)]
+CONTACT_CANDIDATES = [dict(mode=1, block=256, incidences_per_thread=p, substeps_per_launch=0, use_graph=g)
+                      for g in (1, 0) for p in (1, 2, 4, 8)] + \
+                     [dict(mode=1, block=512, incidences_per_thread=p, substeps_per_launch=0, use_graph=1) for p in (2, 4)]
 
 
 def build_world(bodies):
@@ -113,49 +113,82 @@ def batched_leg(steps, retune, log):
                        "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"]}}
 
 
-def learned_leg(steps, plain_solves_per_s, log):
-    """The learned warm start on the plate workload: each step solved by PCG (block Jacobi) from the trained network's
-    prediction, K = 50 iterations to 1e-6, the supernodal Cholesky as the fallback; then, untimed, the network's error
-    per solve against the FP64 Cholesky and the iterations an uncapped PCG needs."""
+LEARNED_K = 8  # PCG iterations before the Cholesky fallback, preconditioned by the last factor
+
+
+def learned_run(net, steps, warm, precond, max_iter, log, name):
+    """`steps` timed steps of the plate, each solved by PCG (warm start, preconditioner, K) with the Cholesky as the
+    fallback, after 3 Cholesky steps (the first factor); the same seeded loads for every variant."""
     import numpy as np
-    from beamlab_amd import abi, fem, learned
-    net = learned.Network.load()
+    from beamlab_amd import fem, learned
     G = fem.ShellBody.plate(FEM_CELLS, FEM_CELLS)
-    loads = fem_loads(G.nn, G.fixed, 3 + steps + 12, seed=21)
+    loads = fem_loads(G.nn, G.fixed, 3 + steps + 9, seed=21)
     for s in range(3):
         G.step(loads[s], FEM_H)
     reps = []
     t0 = time.perf_counter()
     for s in range(steps):
-        reps.append(learned.pcg_step(G, net, loads[3 + s], h=FEM_H, max_iter=50, tol=1e-6, warm=abi.WARM_NETWORK,
-                                     precond=abi.PRECOND_BLOCK_JACOBI))
+        reps.append(learned.pcg_step(G, net, loads[3 + s], h=FEM_H, max_iter=max_iter, tol=1e-6, warm=warm,
+                                     precond=precond))
     wall = time.perf_counter() - t0
-    log(f"learned: {steps / wall:.1f} solves/s")
-    checks = [learned.pcg_step(G, net, loads[3 + steps + k], h=FEM_H, max_iter=50, tol=1e-6, warm=abi.WARM_NETWORK,
-                               precond=abi.PRECOND_BLOCK_JACOBI, check_error=True) for k in range(8)]
-    uncapped = {}
-    for name, warm in (("network", abi.WARM_NETWORK), ("zero", abi.WARM_ZERO)):
-        state = G.state()
-        r = learned.pcg_step(G, net, loads[3 + steps + 8], h=FEM_H, max_iter=20000, tol=1e-6, warm=warm,
-                             precond=abi.PRECOND_BLOCK_JACOBI, fallback=False)
-        G.set_state(*state)
-        uncapped[name] = {"iterations": r["iterations"], "converged": r["converged"], "pcg_ms": r["pcg_ms"]}
-    G.close()
     mean = lambda k: float(np.mean([r[k] for r in reps]))
-    value = steps / wall
-    return {"value": value, "unit": "implicit FEM solves/s", "steps": steps, "ms_per_step": wall / steps * 1e3,
-            "dtype": "bf16 network (fp32 sums), f64 PCG and fallback",
-            "config": {"workload": f"the {FEM_CELLS}x{FEM_CELLS} plate of implicit_fem", "network": "4 layers x 64, "
-                       "trained on 20000 Cholesky-solved steps of a 64x64 plate (beamlab_amd/gnn_plate.npz)",
-                       "pcg": "block Jacobi, warm start from the network", "max_iter": 50, "tol": 1e-6,
-                       "fallback": "supernodal Cholesky"},
-            "vs_plain_cholesky": value / plain_solves_per_s if plain_solves_per_s else None,
-            "converged_rate": mean("converged"), "fallback_rate": mean("fallback_used"),
-            "iterations_mean": mean("iterations"), "residual_max": max(r["residual"] for r in reps),
-            "phases_ms": {k: mean(k + "_ms") for k in ("assemble", "gnn", "pcg", "fallback", "update")},
-            "per_solve_error_vs_fp64_cholesky": [{"warm_start": c["warm_error"], "pcg": c["pcg_error"],
-                                                  "fallback_used": c["fallback_used"]} for c in checks],
-            "uncapped_pcg_to_1e-6": uncapped}
+    out = {"value": steps / wall, "steps": steps, "ms_per_step": wall / steps * 1e3, "max_iter": max_iter,
+           "converged_rate": mean("converged"), "fallback_rate": mean("fallback_used"),
+           "iterations_mean": mean("iterations"), "residual_max": max(r["residual"] for r in reps),
+           "phases_ms": {k: mean(k + "_ms") for k in ("assemble", "gnn", "pcg", "fallback", "update")}}
+    log(f"learned [{name}]: {out['value']:.1f} solves/s, {out['iterations_mean']:.2f} iterations, "
+        f"fallback {out['fallback_rate']:.2f}")
+    return G, loads, out
+
+
+def learned_leg(steps, plain_solves_per_s, log):
+    """The learned warm start on the plate workload. The headline: each step solved by PCG from the trained network's
+    prediction, preconditioned by the engine's last Cholesky factor (factor reuse between solves), at most LEARNED_K
+    iterations to 1e-6, the supernodal Cholesky as the fallback (which refreshes the factor). Beside it, timed the
+    same way: that PCG from the previous step's solution and from zero (what the network gains), and block-Jacobi PCG
+    from the network's prediction with K = 50. Then, untimed, the network's error per solve against the FP64
+    Cholesky and the iterations an uncapped block-Jacobi PCG needs."""
+    from beamlab_amd import abi, learned
+    net = learned.Network.load()
+    G, loads, head = learned_run(net, steps, abi.WARM_NETWORK, abi.PRECOND_FACTOR, LEARNED_K, log, "network, factor")
+    checks = [learned.pcg_step(G, net, loads[3 + steps + k], h=FEM_H, max_iter=LEARNED_K, tol=1e-6,
+                               warm=abi.WARM_NETWORK, precond=abi.PRECOND_FACTOR, check_error=True) for k in range(8)]
+    G.close()
+    variants = {}
+    for name, warm, prec, k, n in (("previous_warm_factor", abi.WARM_PREVIOUS, abi.PRECOND_FACTOR, LEARNED_K, steps),
+                                   ("zero_warm_factor", abi.WARM_ZERO, abi.PRECOND_FACTOR, LEARNED_K, steps),
+                                   ("network_warm_block_jacobi", abi.WARM_NETWORK, abi.PRECOND_BLOCK_JACOBI, 50,
+                                    max(4, steps // 5))):
+        Gv, lv, out = learned_run(net, n, warm, prec, k, log, name)
+        if name == "network_warm_block_jacobi":
+            out["per_solve_error_vs_fp64_cholesky"] = [
+                {"warm_start": c["warm_error"], "pcg": c["pcg_error"], "fallback_used": c["fallback_used"]}
+                for c in (learned.pcg_step(Gv, net, lv[3 + n + q], h=FEM_H, max_iter=50, tol=1e-6,
+                                           warm=abi.WARM_NETWORK, precond=abi.PRECOND_BLOCK_JACOBI, check_error=True)
+                          for q in range(4))]
+            uncapped = {}
+            for uname, warm_u in (("network", abi.WARM_NETWORK), ("zero", abi.WARM_ZERO)):
+                state = Gv.state()
+                r = learned.pcg_step(Gv, net, lv[3 + n + 4], h=FEM_H, max_iter=20000, tol=1e-6, warm=warm_u,
+                                     precond=abi.PRECOND_BLOCK_JACOBI, fallback=False)
+                Gv.set_state(*state)
+                uncapped[uname] = {"iterations": r["iterations"], "converged": r["converged"], "pcg_ms": r["pcg_ms"]}
+            out["uncapped_pcg_to_1e-6"] = uncapped
+        Gv.close()
+        out["vs_plain_cholesky"] = out["value"] / plain_solves_per_s if plain_solves_per_s else None
+        variants[name] = out
+    head.update({
+        "unit": "implicit FEM solves/s", "dtype": "bf16 network (fp32 sums), f64 PCG and fallback",
+        "config": {"workload": f"the {FEM_CELLS}x{FEM_CELLS} plate of implicit_fem", "network": "4 layers x 64, "
+                   "trained on 20000 Cholesky-solved steps of a 64x64 plate (beamlab_amd/gnn_plate.npz)",
+                   "pcg": "warm start from the network, preconditioned by the last Cholesky factor",
+                   "max_iter": LEARNED_K, "tol": 1e-6, "fallback": "supernodal Cholesky (refreshes the factor)"},
+        "vs_plain_cholesky": head["value"] / plain_solves_per_s if plain_solves_per_s else None,
+        "per_solve_error_vs_fp64_cholesky": [{"warm_start": c["warm_error"], "pcg": c["pcg_error"],
+                                              "iterations": c["iterations"], "fallback_used": c["fallback_used"]}
+                                             for c in checks],
+        "variants": variants})
+    return head
 
 
 def fem_loads(nn, fixed, steps, seed=7, per_step=32, sigma=50.0):
@@ -299,7 +332,7 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, help="only run a plain copy of this many MiB (profiling)")
     ap.add_argument("--fem-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=2000)
-    ap.add_argument("--learned-steps", type=int, default=50)
+    ap.add_argument("--learned-steps", type=int, default=200)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fem", action="store_true")

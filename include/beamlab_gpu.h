@@ -351,7 +351,10 @@ int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
 typedef struct bl_gnn bl_gnn;
 
 enum { BL_WARM_ZERO = 0, BL_WARM_PREVIOUS = 1, BL_WARM_NETWORK = 2 };
-enum { BL_PRECOND_BLOCK_JACOBI = 0, BL_PRECOND_NETWORK = 1 };
+/* BL_PRECOND_FACTOR: the engine's last Cholesky factor, of an earlier step's system (factor reuse between solves, as
+ * the reference's held steps solve with the last factorization, src/phys/frame_fem.cpp:2342-2440). The fallback
+ * refreshes it when PCG misses tol within max_iter; with no factor yet, the step is a Cholesky step. */
+enum { BL_PRECOND_BLOCK_JACOBI = 0, BL_PRECOND_NETWORK = 1, BL_PRECOND_FACTOR = 2 };
 
 typedef struct bl_pcg_opts {
     int32_t max_iter;    /* K: PCG iterations before the fallback */

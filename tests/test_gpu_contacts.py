@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 TUNINGS = {
     "auto": dict(mode=abi.MODE_AUTO),
     "stepped_1": dict(mode=abi.MODE_STEPPED, block=256, use_graph=1, incidences_per_thread=1),
+    "stepped_1_launches": dict(mode=abi.MODE_STEPPED, block=256, use_graph=0, incidences_per_thread=1),
     "persistent": dict(mode=abi.MODE_PERSISTENT, block=1024, substeps_per_launch=50),
 }
 COUNTERS = ("contacts", "pair_rebuilds", "contact_pairs", "narrow_tests", "static_contacts", "broken_beams")

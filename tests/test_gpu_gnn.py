@@ -120,6 +120,37 @@ def test_pcg_solves_and_falls_back():
     assert rep["converged"] == 0 and rep["fallback_used"] == 1 and rep["residual"] <= 1e-8, rep
 
 
+def test_pcg_preconditioned_by_the_last_factor():
+    """PCG preconditioned by the engine's last Cholesky factor (factor reuse): the first step, with no factor yet, is
+    a Cholesky step; the later ones converge in a few iterations to the Cholesky's solution; the error check (a
+    factorization of its own) leaves the factor the PCG uses as it was, so a twin body stepped without the check
+    follows the same iterations and the same state bit for bit; a PCG that misses tol falls back and refreshes it."""
+    print("gnn pcg factor 16x16", flush=True)
+    net = learned.Network.load()
+    a, b = fem.ShellBody.plate(16, 16), fem.ShellBody.plate(16, 16)
+    rng = np.random.default_rng(8)
+    loads = [learned.plate_loads(a.nn, a.fixed, rng, 4) for _ in range(7)]
+    for k, ld in enumerate(loads[:6]):
+        ra = learned.pcg_step(a, net, ld, max_iter=30, tol=1e-8, warm=abi.WARM_NETWORK, precond=abi.PRECOND_FACTOR,
+                              check_error=True)
+        rb = learned.pcg_step(b, net, ld, max_iter=30, tol=1e-8, warm=abi.WARM_NETWORK, precond=abi.PRECOND_FACTOR)
+        print(k, {q: ra[q] for q in ("iterations", "converged", "fallback_used", "residual", "warm_error", "pcg_error")})
+        if k == 0:
+            assert ra["fallback_used"] == 1 and ra["iterations"] == 0 and rb["fallback_used"] == 1, (ra, rb)
+        else:
+            assert ra["converged"] == 1 and ra["fallback_used"] == 0 and 1 <= ra["iterations"] <= 30, ra
+            assert ra["pcg_error"] <= 1e-6, ra
+            assert rb["iterations"] == ra["iterations"] and rb["fallback_used"] == 0, (ra, rb)
+        assert ra["residual"] <= 1e-8 and rb["residual"] <= 1e-8, (ra, rb)
+        for u, v in zip(a.state(), b.state()):
+            assert np.array_equal(u, v)
+    # no iteration allowed: the residual test of the warm start fails, the fallback solves and refreshes the factor
+    rep = learned.pcg_step(a, net, loads[6], max_iter=0, tol=1e-10, warm=abi.WARM_NETWORK, precond=abi.PRECOND_FACTOR)
+    assert rep["fallback_used"] == 1 and rep["residual"] <= 1e-9, rep
+    a.close()
+    b.close()
+
+
 def test_online_training_lowers_the_loss():
     """A few hundred Adam steps on a 16 x 16 plate's own Cholesky steps lower the loss."""
     print("gnn online training 16x16", flush=True)
