@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbeamlab_gpu.so")
 SOURCES = ["bl_step.hip", "bl_contacts.hip", "bl_capi.hip", "bl_fem.hip", "bl_gnn.hip"]
-HEADERS = ["bl_devmath.h", "bl_world.h", "bl_contacts.h", "bl_fem_api.h"]
+HEADERS = ["bl_devmath.h", "bl_world.h", "bl_contacts.h", "bl_fem_api.h", "bl_islands.h"]
 ARCH = os.environ.get("BL_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -16,6 +16,7 @@
 // the latest earlier contact sharing a node with it (a sort of (node, contact) keys in LDS) and each level runs in
 // parallel - the same arithmetic, in the same per-node order, as the serial loop.
 #include "bl_contacts.h"
+#include "bl_islands.h"
 
 #include <rocprim/device/device_scan.hpp>
 
@@ -464,7 +465,7 @@ __device__ __forceinline__ uint32_t group_prefix(uint32_t v, uint32_t& total) {
     return x - v;
 }
 
-// 2) each non-torn triangle of a searching body scans its cells for nodes in reach: a group of 8 lanes per
+// 2) each non-torn triangle of a searching body scans its cells for nodes in reach: a group of kGnt lanes per
 // triangle, a lane per cell; the lanes' matches concatenate in cell order (count, then write).
 struct NtQuery {
     uint4 tr;
@@ -508,7 +509,7 @@ __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, cons
     return cnt;
 }
 
-constexpr int kGnt = 8;
+constexpr int kGnt = 16;
 __global__ __launch_bounds__(256) void k_rb_nt(KArgs a, CArgs c, int write) {
     const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
     if (t >= a.ntris) return; // (whole groups)
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(256) void k_rb_nt(KArgs a, CArgs c, int write) {
     }
 }
 
-// ... and each capsule of a searching body scans for nodes within its radius (a group of 16 lanes per capsule)
+// ... and each capsule of a searching body scans for nodes within its radius (a group of kGcap lanes per capsule)
 __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint4 cp, int isl, f3 pa, f3 pb,
                                             float margin, int x, int y, int z, uint4* out, uint32_t at, uint32_t cap,
                                             uint32_t ci) {
@@ -583,7 +584,7 @@ __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint
     return cnt;
 }
 
-constexpr int kGcap = 16;
+constexpr int kGcap = 64;
 __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c, int write) {
     const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
     const bool live = ci < a.ncaps;
@@ -688,7 +689,7 @@ __global__ void k_rb_efill(CArgs c) {
 }
 
 // each capsule of an island collects the triangles of its other bodies whose boxes meet its own, once each, in
-// (body, triangle) order: a group of 16 lanes per capsule, a lane per cell, the finds gathered in LDS, sorted, made
+// (body, triangle) order: a group of kGcap lanes per capsule, a lane per cell, the finds gathered in LDS, sorted, made
 // unique into the capsule's slot; after the scan of the counts, k_rb_ct_out moves them into the list
 constexpr int kCtCap = 256;
 __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
@@ -1319,12 +1320,8 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         const float mn[3] = {hv.mn[i].x, hv.mn[i].y, hv.mn[i].z}, mx[3] = {hv.mx[i].x, hv.mx[i].y, hv.mx[i].z};
         for (int k = 0; k < 3; k++) bx[6 * i + k] = mn[k] - r, bx[6 * i + 3 + k] = mx[k] + r;
     }
-    auto overlaps = [&](size_t i, size_t j) {
-        const float *p = &bx[6 * i], *q = &bx[6 * j];
-        return p[0] <= q[3] && p[3] >= q[0] && p[1] <= q[4] && p[4] >= q[1] && p[2] <= q[5] && p[5] >= q[2];
-    };
+    // (i < j, their boxes overlapping)
     auto consider = [&](size_t i, size_t j) {
-        if (!overlaps(i, j)) return;
         const BodyView &bi = hv.bodies[i], &bj = hv.bodies[j];
         if (bi.group != 0 && bi.group == bj.group) return;
         const bool ti = bi.ntri > 0, tj = bj.ntri > 0, ci = bi.ncap > 0, cj = bj.ncap > 0;
@@ -1333,18 +1330,7 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         const int x = find((int)i), y = find((int)j);
         if (x != y) parent[x] = y;
     };
-    if (hv.inter_body) {
-        // bodies whose boxes are not finite take the plain all-pairs test
-        std::vector<uint32_t> order, odd;
-        for (size_t i = 0; i < nb; i++) (std::isfinite(bx[6 * i]) && std::isfinite(bx[6 * i + 3]) ? order : odd).push_back((uint32_t)i);
-        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return bx[6 * a] < bx[6 * b] || (bx[6 * a] == bx[6 * b] && a < b); });
-        for (size_t u = 0; u < order.size(); u++)
-            for (size_t v = u + 1; v < order.size() && bx[6 * order[v]] <= bx[6 * order[u] + 3]; v++)
-                consider(std::min(order[u], order[v]), std::max(order[u], order[v]));
-        for (uint32_t i : odd)
-            for (size_t j = 0; j < nb; j++)
-                if (j != i) consider(std::min<size_t>(i, j), std::max<size_t>(i, j));
-    }
+    if (hv.inter_body) overlapping_pairs(bx.data(), nb, [&](uint32_t i, uint32_t j) { consider(i, j); });
     islands.clear();
     std::vector<int> island_of(nb, -1);
     for (size_t i = 0; i < nb; i++) {

@@ -569,9 +569,10 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
     double* Lf = f.L + fr.off_l;
     const bool prev = k > 0;
     const uint32_t kp = k - kT;
-    // 1. loads: the tile (lower), the rows below it, and their previous-panel factor rows
+    // 1. loads: the tile (lower), the rows below it, and their previous-panel factor rows (consecutive threads on
+    // consecutive rows of a column: the fronts and L are column-major)
     for (uint32_t e = tid; e < kT * kT; e += 256) {
-        const uint32_t ii = e / kT, jj = e % kT;
+        const uint32_t ii = e % kT, jj = e / kT;
         Td[ii][jj] = (ii < w && jj <= ii) ? A[(uint64_t)(k + jj) * m + k + ii] : 0.0;
         if (prev) Ld[ii][jj] = ii < w ? Lf[(uint64_t)(kp + jj) * m + k + ii] : 0.0;
     }
@@ -657,7 +658,8 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
 
 // C -= P_r P_c^T on the matrix cores: a 32 x 32 quarter a wave, 2 x 2 blocks of v_mfma_f64_16x16x4f64. The
 // instruction's lanes (measured on gfx950): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; item r of its
-// result is D[l / 16 + 4 r][l % 16].
+// result is D[l / 16 + 4 r][l % 16]. The product is formed transposed (A from P_c, B from P_r: D = (P_r P_c^T)^T), so
+// that a lane's items lie down a column of C and 16 lanes cover 16 consecutive rows (C is column-major).
 __device__ void update_task(const FacArgs& f, uint4 t, uint32_t kp, double* sh) {
     double (*Pa)[kT + 1] = (double (*)[kT + 1])sh;
     double (*Pb)[kT + 1] = (double (*)[kT + 1])(sh + 64 * 33);
@@ -683,13 +685,13 @@ __device__ void update_task(const FacArgs& f, uint4 t, uint32_t kp, double* sh) 
         for (int a0 = 0; a0 < 2; a0++) av[a0] = Pa[wr + a0 * 16 + li][kk + lk];
         for (int b0 = 0; b0 < 2; b0++) bv[b0] = Pb[wc + b0 * 16 + li][kk + lk];
         for (int a0 = 0; a0 < 2; a0++)
-            for (int b0 = 0; b0 < 2; b0++) acc[a0][b0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a0], bv[b0], acc[a0][b0], 0, 0, 0);
+            for (int b0 = 0; b0 < 2; b0++) acc[a0][b0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b0], av[a0], acc[a0][b0], 0, 0, 0);
     }
     double* C = f.F + fr.off;
     for (int a0 = 0; a0 < 2; a0++)
         for (int b0 = 0; b0 < 2; b0++)
             for (int r = 0; r < 4; r++) {
-                const uint32_t row = r0 + wr + a0 * 16 + lk + 4 * r, col = c0 + wc + b0 * 16 + li;
+                const uint32_t row = r0 + wr + a0 * 16 + li, col = c0 + wc + b0 * 16 + lk + 4 * r;
                 if (row < m && col < m && row >= col) C[(uint64_t)col * m + row] -= acc[a0][b0][r];
             }
 }

@@ -569,6 +569,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)
 }
 
 // ---------------------------------------------------------------------------------------------- frame kernels
+__device__ __forceinline__ unsigned int rd_word(const unsigned int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The workgroup's nodes all belong to one body when its first and last do (bodies are contiguous): then a
 // workgroup reduction and one atomic per word, else one atomic per node (small bodies: little contention).
 template <int T>
@@ -612,10 +616,16 @@ __device__ __forceinline__ void body_reduce(const KArgs& a, uint32_t i, bool val
             for (int k = 0; k < 3; k++)
                 sred[1 + k] = min(sred[1 + k], sred[w2 * 7 + 1 + k]), sred[4 + k] = max(sred[4 + k], sred[w2 * 7 + 4 + k]);
         }
+        // (the words only rise (fall) within a frame: a workgroup whose value does not pass the one it reads leaves
+        // the word alone - a stale read costs an atomic that changes nothing, never a missed one - so that the many
+        // workgroups of a large body do not queue on one address)
         unsigned int* w = a.body_frame + bfirst * BF_N;
-        if (want_v2 && sred[0]) atomicMax(&w[BF_V2], sred[0]);
+        if (want_v2 && sred[0] && sred[0] > rd_word(&w[BF_V2])) atomicMax(&w[BF_V2], sred[0]);
         if (want_box)
-            for (int k = 0; k < 3; k++) atomicMin(&w[BF_MN + k], sred[1 + k]), atomicMax(&w[BF_MX + k], sred[4 + k]);
+            for (int k = 0; k < 3; k++) {
+                if (sred[1 + k] < rd_word(&w[BF_MN + k])) atomicMin(&w[BF_MN + k], sred[1 + k]);
+                if (sred[4 + k] > rd_word(&w[BF_MX + k])) atomicMax(&w[BF_MX + k], sred[4 + k]);
+            }
     }
 }
 
