@@ -629,8 +629,13 @@ int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk
     const KArgs& a = c->args;
     int rc;
     for (int q = 0; q < len; q++) {
-        if (q > 0 && (rc = launch_forces(a, c->stream, parity, t))) return rc;
-        if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
+        if (q > 0 && t.block == 256) {
+            if ((rc = launch_forces_narrow(a, c->stream, parity, t, p, nblk))) return rc;
+            if ((rc = c->contacts.launch_response(a, p, nblk, c->stream, parity))) return rc;
+        } else {
+            if (q > 0 && (rc = launch_forces(a, c->stream, parity, t))) return rc;
+            if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
+        }
         if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p))) return rc;
         parity ^= 1;
     }
@@ -725,14 +730,28 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
             s = next;
             continue;
         }
+        // (after the segment's first substep the narrow phase runs in the force phase's launch - it reads the
+        // positions and the lists only - unless the force phase is timed alone)
+        PArgs np{};
+        uint32_t nnb = 0;
         for (; s < next; s++) {
-            if ((rc = c->contacts.collide(a, c->stream, c->parity))) {
+            rc = nnb ? c->contacts.launch_response(a, np, nnb, c->stream, c->parity) : c->contacts.collide(a, c->stream, c->parity);
+            if (rc) {
                 g_last_error = "contact narrow phase or response failed";
                 return rc;
             }
             if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p))) return rc;
             c->parity ^= 1;
+            nnb = 0;
             if (s + 1 < substeps) {
+                if (!kev && s + 1 < next && t.block == 256) {
+                    if ((rc = c->contacts.narrow_args(np, nnb))) return rc;
+                    if (nnb) {
+                        c->contacts.count_tests(1);
+                        if ((rc = launch_forces_narrow(a, c->stream, c->parity, t, np, nnb))) return rc;
+                        continue;
+                    }
+                }
                 if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1)], c->stream));
                 if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
                 if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1) + 1], c->stream));

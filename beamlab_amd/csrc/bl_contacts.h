@@ -154,6 +154,10 @@ public:
     // them; the host's test counter is advanced by `count_tests`.
     int collide_fixed(PArgs& p, uint32_t& nblk);
     int launch_collide(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity);
+    // The narrow phase's arguments for the current lists (nblk = 0: no candidates), and the response alone (after a
+    // narrow phase run by the force phase's launch, launch_forces_narrow)
+    int narrow_args(PArgs& p, uint32_t& nblk);
+    int launch_response(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity);
     void count_tests(int64_t substeps) { narrow_tests += substeps * (int64_t)(nnt + nnc + nct); }
     void release();
     ~ContactEngine() { release(); }
@@ -188,5 +192,9 @@ private:
     int ensure_lists();
     int enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st);
 };
+
+// The force phase with the narrow phase's nblk blocks after its chunks in the same launch (256-thread workgroups only:
+// BL_ERR_ARG otherwise, and the caller launches the two apart).
+int launch_forces_narrow(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t, const PArgs& p, uint32_t nblk);
 
 } // namespace blg

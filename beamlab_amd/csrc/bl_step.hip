@@ -26,6 +26,7 @@
 //                      nodes held in registers across substeps, a grid barrier between substeps (cooperative
 //                      launch, so every workgroup is resident).
 #include "bl_world.h"
+#include "bl_narrow.h"
 
 #pragma clang fp contract(off)
 
@@ -874,6 +875,34 @@ __global__ void k_wstat_fold(KArgs a, uint4* wstat, uint32_t n) {
     if (t.y) atomicAdd(&a.counters[CNT_CLAMPS], (unsigned long long)t.y);
     if (t.z) atomicAdd(&a.counters[CNT_NANS], (unsigned long long)t.z);
     wstat[w] = make_uint4(0, 0, 0, 0);
+}
+
+// The force phase and the contacts' narrow phase in one launch: the workgroups past the chunks take the narrow
+// phase's blocks (it reads the positions and the candidate lists only, and writes its own hit buffers), so that it
+// runs in the force phase's tail rather than in a launch of its own.
+template <int PER>
+__global__ __launch_bounds__(256) void k_forces_narrow(KArgs a, int parity, PArgs p) {
+    __shared__ float sfx[256 * PER], sfy[256 * PER], sfz[256 * PER];
+    __shared__ uint8_t sok[256 * PER];
+    if (blockIdx.x < a.nchunks) {
+        const uint32_t c = xcd_chunk(a, blockIdx.x);
+        NodeCounts cnt = {0, 0, 0, 0.0f};
+        chunk_substep<256, PER, true>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+    } else {
+        narrow_block(a, p, parity, blockIdx.x - a.nchunks);
+    }
+}
+
+int launch_forces_narrow(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t, const PArgs& p, uint32_t nblk) {
+    const int per = t.incidences_per_thread;
+    const dim3 g(a.nchunks + nblk);
+    if (t.block != 256) return BL_ERR_ARG;
+    if (per == 1) hipLaunchKernelGGL((k_forces_narrow<1>), g, dim3(256), 0, st, a, parity, p);
+    else if (per == 2) hipLaunchKernelGGL((k_forces_narrow<2>), g, dim3(256), 0, st, a, parity, p);
+    else if (per == 4) hipLaunchKernelGGL((k_forces_narrow<4>), g, dim3(256), 0, st, a, parity, p);
+    else if (per == 8) hipLaunchKernelGGL((k_forces_narrow<8>), g, dim3(256), 0, st, a, parity, p);
+    else return BL_ERR_ARG;
+    return BL_OK_OR_HIP();
 }
 
 int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
