@@ -555,13 +555,18 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t panel_w(const Front& fr, uint32_t k) { return fr.c - k < (uint32_t)kT ? fr.c - k : (uint32_t)kT; }
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l) << 32) |
+                                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l));
+}
+
 __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
     double (*Td)[kT + 1] = (double (*)[kT + 1])sh;               // 32 x 33: the diagonal tile
     double (*X)[kT + 1] = (double (*)[kT + 1])(sh + 32 * 33);    // 64 x 33: the task's rows
     double (*Lr)[kT + 1] = (double (*)[kT + 1])(sh + 96 * 33);   // 64 x 33: their previous-panel factor
     double (*Ld)[kT + 1] = (double (*)[kT + 1])(sh + 160 * 33);  // 32 x 33: the tile rows' previous-panel factor
     double* inv = sh + 192 * 33;
-    double* colj = inv + kT;
     const Front fr = f.fr[t.x];
     const uint32_t m = fr.m, w = panel_w(fr, k), lo = k + w, r0 = t.y;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -569,39 +574,62 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
     double* Lf = f.L + fr.off_l;
     const bool prev = k > 0;
     const uint32_t kp = k - kT;
-    // 1. loads: the tile (lower), the rows below it, and their previous-panel factor rows (consecutive threads on
-    // consecutive rows of a column: the fronts and L are column-major)
-    for (uint32_t e = tid; e < kT * kT; e += 256) {
-        const uint32_t ii = e % kT, jj = e / kT;
-        Td[ii][jj] = (ii < w && jj <= ii) ? A[(uint64_t)(k + jj) * m + k + ii] : 0.0;
-        if (prev) Ld[ii][jj] = ii < w ? Lf[(uint64_t)(kp + jj) * m + k + ii] : 0.0;
-    }
-    for (uint32_t e = tid; e < 64 * kT; e += 256) {
-        const uint32_t ii = e % 64, jj = e / 64, r = r0 + ii;
-        const bool below = r < m && r >= lo;
-        X[ii][jj] = (below && jj < w) ? A[(uint64_t)(k + jj) * m + r] : 0.0;
-        if (prev) Lr[ii][jj] = below ? Lf[(uint64_t)(kp + jj) * m + r] : 0.0;
+    // 1. loads: the tile (lower), the rows below it, and their previous-panel factor rows; every load issued before
+    // the first store to LDS (consecutive threads on consecutive rows of a column: the fronts and L are column-major)
+    {
+        double vt[kT * kT / 256], vl[kT * kT / 256], vx[64 * kT / 256], vr[64 * kT / 256];
+#pragma unroll
+        for (int q = 0; q < kT * kT / 256; q++) {
+            const uint32_t e = tid + 256 * q, ii = e % kT, jj = e / kT;
+            vt[q] = (ii < w && jj <= ii) ? A[(uint64_t)(k + jj) * m + k + ii] : 0.0;
+            vl[q] = (prev && ii < w) ? Lf[(uint64_t)(kp + jj) * m + k + ii] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 64 * kT / 256; q++) {
+            const uint32_t e = tid + 256 * q, ii = e % 64, jj = e / 64, r = r0 + ii;
+            const bool below = r < m && r >= lo;
+            vx[q] = (below && jj < w) ? A[(uint64_t)(k + jj) * m + r] : 0.0;
+            vr[q] = (prev && below) ? Lf[(uint64_t)(kp + jj) * m + r] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kT * kT / 256; q++) {
+            const uint32_t e = tid + 256 * q, ii = e % kT, jj = e / kT;
+            Td[ii][jj] = vt[q], Ld[ii][jj] = vl[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 64 * kT / 256; q++) {
+            const uint32_t e = tid + 256 * q, ii = e % 64, jj = e / 64;
+            X[ii][jj] = vx[q], Lr[ii][jj] = vr[q];
+        }
     }
     __syncthreads();
     if (prev) {
-        // the tile's lower triangle and the rows, less the previous panel's part (4 partial sums a product)
-        for (uint32_t e = tid; e < 96 * kT; e += 256) {
-            const uint32_t ii = e / kT, jj = e % kT;
-            const bool tile = ii < kT;
-            const uint32_t i2 = tile ? ii : ii - kT;
-            if (jj >= w || (tile && (jj > i2 || i2 >= w))) continue;
-            const double* u = tile ? Ld[i2] : Lr[i2];
-            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        // the tile's lower triangle and the rows, less the previous panel's part: C = U Ld^T with U = [Ld; Lr]
+        // (96 x 32) on the matrix cores, 16 x 16 output blocks (6 x 2), three a wave
+        const uint32_t li = lane & 15, lk = lane >> 4;
 #pragma unroll
-            for (int q = 0; q < kT; q += 4)
-                s0 += u[q] * Ld[jj][q], s1 += u[q + 1] * Ld[jj][q + 1], s2 += u[q + 2] * Ld[jj][q + 2], s3 += u[q + 3] * Ld[jj][q + 3];
-            const double sub = (s0 + s1) + (s2 + s3);
-            if (tile) Td[i2][jj] -= sub;
-            else X[i2][jj] -= sub;
+        for (int b3 = 0; b3 < 3; b3++) {
+            const uint32_t blk = wv + 4 * b3, i0 = (blk >> 1) * 16, j0 = (blk & 1) * 16;
+            const uint32_t iu = i0 + li;
+            const double* urow = iu < (uint32_t)kT ? Ld[iu] : Lr[iu - kT];
+            const double* vrow = Ld[j0 + li];
+            dbl4 acc = (dbl4){0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < kT; q += 4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(urow[q + lk], vrow[q + lk], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t i = i0 + lk + 4 * r, j = j0 + li;
+                if (j >= w) continue;
+                if (i < (uint32_t)kT) {
+                    if (i < w && j <= i) Td[i][j] -= acc[r];
+                } else {
+                    X[i - kT][j] -= acc[r];
+                }
+            }
         }
         __syncthreads();
     }
-    // 2. the tile's Cholesky in wave 0
+    // 2. the tile's Cholesky in wave 0: a row per lane in registers, each finished column broadcast lane to lane
     if (wv == 0) {
         const uint32_t ri = lane & 31;
         double a[kT];
@@ -611,20 +639,13 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
 #pragma unroll
         for (int j = 0; j < kT; j++) {
             if (j < (int)w) {
-                const uint64_t u = __builtin_bit_cast(uint64_t, a[j]);
-                const double piv = __builtin_bit_cast(
-                    double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j));
+                const double piv = readlane_d(a[j], j);
                 bad |= !(piv > 0);
                 const double d = sqrt(piv > 0 ? piv : 1.0), id = 1.0 / d;
                 a[j] = (int)ri == j ? d : a[j] * id;
                 if (lane == 0) inv[j] = id;
-                if (lane < kT) colj[lane] = a[j];
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int c = j + 1; c < kT; c++) a[c] -= a[j] * colj[c];
-                __builtin_amdgcn_wave_barrier();
+                for (int c = j + 1; c < kT; c++) a[c] -= a[j] * readlane_d(a[j], c);
             }
         }
         if (lane < kT)
@@ -633,17 +654,27 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
         if (bad && lane == 0 && r0 == k) atomicOr(f.fail, 1);
     }
     __syncthreads();
-    // 3. the rows: X := X L^-T, a row per thread (4 partial sums a dot product)
+    // 3. the rows: X := X L^-T, a row per lane in registers (4 partial sums a dot product)
     if (tid < 64 && r0 + tid < m && r0 + tid >= lo) {
-        double* x = X[tid];
-        for (uint32_t jj = 0; jj < w; jj++) {
-            double s0 = x[jj], s1 = 0, s2 = 0, s3 = 0;
-            uint32_t mm = 0;
-            for (; mm + 4 <= jj; mm += 4)
-                s0 -= Td[jj][mm] * x[mm], s1 -= Td[jj][mm + 1] * x[mm + 1], s2 -= Td[jj][mm + 2] * x[mm + 2], s3 -= Td[jj][mm + 3] * x[mm + 3];
-            for (; mm < jj; mm++) s0 -= Td[jj][mm] * x[mm];
-            x[jj] = ((s0 + s1) + (s2 + s3)) * inv[jj];
+        double x[kT];
+#pragma unroll
+        for (int jj = 0; jj < kT; jj++) x[jj] = X[tid][jj];
+#pragma unroll
+        for (int jj = 0; jj < kT; jj++) {
+            if (jj < (int)w) {
+                double s0 = x[jj], s1 = 0, s2 = 0, s3 = 0;
+                int mm = 0;
+#pragma unroll
+                for (; mm + 4 <= jj; mm += 4)
+                    s0 -= Td[jj][mm] * x[mm], s1 -= Td[jj][mm + 1] * x[mm + 1], s2 -= Td[jj][mm + 2] * x[mm + 2],
+                        s3 -= Td[jj][mm + 3] * x[mm + 3];
+#pragma unroll
+                for (; mm < jj; mm++) s0 -= Td[jj][mm] * x[mm];
+                x[jj] = ((s0 + s1) + (s2 + s3)) * inv[jj];
+            }
         }
+#pragma unroll
+        for (int jj = 0; jj < kT; jj++) X[tid][jj] = x[jj];
     }
     __syncthreads();
     // 4. store
