@@ -493,8 +493,17 @@ struct Scat {
     uint32_t src, kind, front, rpos, cpos;
 };
 
-__global__ void k_zero(double* p, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+// The fronts' lower triangles cleared (every kernel of the factorization reads and writes the lower triangle only):
+// task = (front, 16 columns from j0), the rows j .. m of each column.
+__global__ __launch_bounds__(256) void k_zero_lower(const Front* fr, const uint2* tasks, uint32_t n, double* F) {
+    if (blockIdx.x >= n) return;
+    const uint2 t = tasks[blockIdx.x];
+    const Front x = fr[t.x];
+    const uint32_t j1 = t.y + 16 < x.m ? t.y + 16 : x.m;
+    for (uint32_t j = t.y; j < j1; j++) {
+        double* col = F + x.off + (uint64_t)j * x.m;
+        for (uint32_t r = j + threadIdx.x; r < x.m; r += 256) col[r] = 0.0;
+    }
 }
 
 __global__ void k_clear_int(int* p, uint32_t n) {
@@ -629,52 +638,50 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
         }
         __syncthreads();
     }
-    // 2. the tile's Cholesky in wave 0: a row per lane in registers, each finished column broadcast lane to lane
-    if (wv == 0) {
+    // 2-3. the tile's Cholesky (wave 0: a row per lane in registers, each finished column broadcast lane to lane) and
+    // the rows' solve X := X L^-T (wave 1: a row per lane, in LDS, 4 partial sums a dot product) in step: column j
+    // of L and 1 / L_jj go to LDS, a barrier, then wave 1 takes column j of its rows (it needs row j of L, final once
+    // column j is) while wave 0 factors column j + 1.
+    {
         const uint32_t ri = lane & 31;
+        const bool xrow = wv == 1 && r0 + lane < m && r0 + lane >= lo;
         double a[kT];
-#pragma unroll
-        for (int j = 0; j < kT; j++) a[j] = Td[ri][j];
         bool bad = false;
+        if (wv == 0)
+#pragma unroll
+            for (int j = 0; j < kT; j++) a[j] = Td[ri][j];
+        double* x = X[lane];
 #pragma unroll
         for (int j = 0; j < kT; j++) {
             if (j < (int)w) {
-                const double piv = readlane_d(a[j], j);
-                bad |= !(piv > 0);
-                const double d = sqrt(piv > 0 ? piv : 1.0), id = 1.0 / d;
-                a[j] = (int)ri == j ? d : a[j] * id;
-                if (lane == 0) inv[j] = id;
+                if (wv == 0) {
+                    const double piv = readlane_d(a[j], j);
+                    bad |= !(piv > 0);
+                    // 1 / sqrt(piv) from the hardware estimate and a Newton step (within an ulp or so: the factor
+                    // carries its own rounding anyway), sqrt(piv) = piv / sqrt(piv)
+                    const double pv = piv > 0 ? piv : 1.0, y0 = __builtin_amdgcn_rsq(pv);
+                    const double id = y0 * (1.5 - (0.5 * pv) * (y0 * y0)), d = pv * id;
+                    a[j] = (int)ri == j ? d : a[j] * id;
+                    if (lane == 0) inv[j] = id;
 #pragma unroll
-                for (int c = j + 1; c < kT; c++) a[c] -= a[j] * readlane_d(a[j], c);
+                    for (int c = j + 1; c < kT; c++) a[c] -= a[j] * readlane_d(a[j], c);
+                    if (lane < kT) Td[ri][j] = j <= (int)ri ? a[j] : 0.0;
+                }
+                __syncthreads();
+                if (xrow) {
+                    double s0 = x[j], s1 = 0, s2 = 0, s3 = 0;
+                    int mm = 0;
+#pragma unroll
+                    for (; mm + 4 <= j; mm += 4)
+                        s0 -= Td[j][mm] * x[mm], s1 -= Td[j][mm + 1] * x[mm + 1], s2 -= Td[j][mm + 2] * x[mm + 2],
+                            s3 -= Td[j][mm + 3] * x[mm + 3];
+#pragma unroll
+                    for (; mm < j; mm++) s0 -= Td[j][mm] * x[mm];
+                    x[j] = ((s0 + s1) + (s2 + s3)) * inv[j];
+                }
             }
         }
-        if (lane < kT)
-#pragma unroll
-            for (int j = 0; j < kT; j++) Td[ri][j] = j <= (int)ri ? a[j] : 0.0;
-        if (bad && lane == 0 && r0 == k) atomicOr(f.fail, 1);
-    }
-    __syncthreads();
-    // 3. the rows: X := X L^-T, a row per lane in registers (4 partial sums a dot product)
-    if (tid < 64 && r0 + tid < m && r0 + tid >= lo) {
-        double x[kT];
-#pragma unroll
-        for (int jj = 0; jj < kT; jj++) x[jj] = X[tid][jj];
-#pragma unroll
-        for (int jj = 0; jj < kT; jj++) {
-            if (jj < (int)w) {
-                double s0 = x[jj], s1 = 0, s2 = 0, s3 = 0;
-                int mm = 0;
-#pragma unroll
-                for (; mm + 4 <= jj; mm += 4)
-                    s0 -= Td[jj][mm] * x[mm], s1 -= Td[jj][mm + 1] * x[mm + 1], s2 -= Td[jj][mm + 2] * x[mm + 2],
-                        s3 -= Td[jj][mm + 3] * x[mm + 3];
-#pragma unroll
-                for (; mm < jj; mm++) s0 -= Td[jj][mm] * x[mm];
-                x[jj] = ((s0 + s1) + (s2 + s3)) * inv[jj];
-            }
-        }
-#pragma unroll
-        for (int jj = 0; jj < kT; jj++) X[tid][jj] = x[jj];
+        if (wv == 0 && bad && lane == 0 && r0 == k) atomicOr(f.fail, 1);
     }
     __syncthreads();
     // 4. store
@@ -727,8 +734,9 @@ __device__ void update_task(const FacArgs& f, uint4 t, uint32_t kp, double* sh) 
             }
 }
 
-// launch s of a level: panel tasks of panel k = 32 s (first), then the update tasks of panel k - 32
-__global__ __launch_bounds__(256) void k_fstep(FacArgs f, const uint2* pt, uint32_t np, const uint4* ut, uint32_t nu, uint32_t k) {
+// launch s of a level: panel tasks of panel k = 32 s (first), then the update tasks of panel k - 32 (at least 3 waves a
+// SIMD: the LDS already holds a CU to 3 workgroups, and the registers must not hold it to fewer)
+__global__ __launch_bounds__(256, 3) void k_fstep(FacArgs f, const uint2* pt, uint32_t np, const uint4* ut, uint32_t nu, uint32_t k) {
 #pragma clang fp contract(fast)
     __shared__ double sh[192 * 33 + 2 * kT];
     if (blockIdx.x < np) panel_task(f, pt[blockIdx.x], k, sh);
@@ -1149,6 +1157,7 @@ struct bl_fem {
     };
     std::vector<LevelPlan> plan;
     Dv<EaTask> ea_tasks;
+    Dv<uint2> ztasks; // (front, first of 16 columns): the clearing of the fronts' lower triangles
     Dv<uint32_t> flist;
     Dv<uint2> pfac_tasks;
     Dv<uint4> syrk_tasks;
@@ -1487,6 +1496,9 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         f->levels[fr[s].level].max_c = std::max(f->levels[fr[s].level].max_c, fr[s].c);
     }
     std::vector<EaTask> ea;
+    std::vector<uint2> zt;
+    for (uint32_t s = 0; s < (uint32_t)fr.size(); s++)
+        for (uint32_t j0 = 0; j0 < fr[s].m; j0 += 16) zt.push_back(make_uint2(s, j0));
     std::vector<uint32_t> flist;
     std::vector<uint2> pfac;
     std::vector<uint4> syrk;
@@ -1575,7 +1587,7 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         (rc = f->Xs.alloc(voff)) || (rc = f->fflag.put(std::vector<uint32_t>(fb, 0u))) ||
         (rc = f->bflag.put(std::vector<uint32_t>(fb, 0u))) || (rc = f->epoch.put(std::vector<uint32_t>(1, 0u))) ||
         (rc = f->fs_tasks.put(fs_tasks)) || (rc = f->bs_tasks.put(bs_tasks)) ||
-        (rc = f->scat.put(scat)) || (rc = f->ea_tasks.put(ea)) || (rc = f->pfac_tasks.put(pfac)) || (rc = f->syrk_tasks.put(syrk)) ||
+        (rc = f->scat.put(scat)) || (rc = f->ea_tasks.put(ea)) || (rc = f->ztasks.put(zt)) || (rc = f->pfac_tasks.put(pfac)) || (rc = f->syrk_tasks.put(syrk)) ||
         (rc = f->flist.put(flist)) || (rc = f->Lbuf.alloc(loff)) ||
         (rc = f->adj_off.put(adj_off)) || (rc = f->adj.put(adjl)) || (rc = f->fail.alloc(1)))
         return rc;
@@ -1611,7 +1623,8 @@ int fem_factor_solve(bl_fem* f, float* ms_factor, float* ms_solve) {
     FacArgs fa = f->fac();
     const hipStream_t st = f->st;
     FCHK(hipEventRecord(f->ev[1], st));
-    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, f->F.p, (uint64_t)f->F.n);
+    if (f->ztasks.n)
+        hipLaunchKernelGGL(k_zero_lower, dim3((uint32_t)f->ztasks.n), dim3(256), 0, st, f->fronts.p, f->ztasks.p, (uint32_t)f->ztasks.n, f->F.p);
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, f->fail.p, 1u);
     FemArgs a = f->args(nullptr, 0, 0, 0, nullptr);
     const uint32_t nsc = (uint32_t)f->scat.n;
