@@ -83,6 +83,8 @@ struct bl_ctx {
     // device
     DevArr<float4> pm[2], vm[2], iA, iM, bcold;
     DevArr<uint32_t> ninfo, noff, ch_n0, body_c0, chunk_body;
+    DevArr<uint4> ch_rng, fch_rng;
+    DevArr<uint32_t> fwg;
     DevArr<uint4> cstat;
     DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags, wg_chunk;
     DevArr<float2> naux;
@@ -132,7 +134,7 @@ struct bl_ctx {
     void release() {
         drop_graphs();
         for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
-        iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release();
+        iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release(), ch_rng.release(), fch_rng.release(), fwg.release();
         body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
@@ -390,6 +392,37 @@ int build(bl_ctx* c) {
         for (size_t j = 0; j < xl[x].size(); j++) wg_chunk[j * 8 + x] = xl[x][j];
     ch_n0.push_back(N);
     c->h_noff = noff;
+    std::vector<uint4> ch_rng(nchunks);
+    for (uint32_t k = 0; k < nchunks; k++) ch_rng[k] = make_uint4(ch_n0[k], ch_n0[k + 1], noff[ch_n0[k]], noff[ch_n0[k + 1]]);
+    // the force phase's chunks: those with beams as they are, the beamless ones' nodes in contiguous runs of <= T
+    std::vector<uint4> fch_rng;
+    {
+        uint32_t r0 = 0, r1 = 0;
+        std::vector<uint4> packed;
+        for (uint32_t k = 0; k < nchunks; k++) {
+            const uint4 cr = ch_rng[k];
+            if (cr.z != cr.w) continue;
+            if (r1 > r0 && (cr.x != r1 || cr.y - r0 > T)) {
+                packed.push_back(make_uint4(r0, r1, noff[r0], noff[r0]));
+                r0 = r1 = 0;
+            }
+            if (r1 == r0) r0 = cr.x;
+            r1 = cr.y;
+        }
+        if (r1 > r0) packed.push_back(make_uint4(r0, r1, noff[r0], noff[r0]));
+        for (uint32_t k = 0; k < nchunks; k++)
+            if (ch_rng[k].z != ch_rng[k].w) fch_rng.push_back(ch_rng[k]);
+        fch_rng.insert(fch_rng.end(), packed.begin(), packed.end());
+    }
+    const uint32_t nf = (uint32_t)fch_rng.size();
+    std::vector<uint32_t> fwg;
+    {
+        uint32_t st[8], len[8];
+        for (uint32_t x = 0; x < 8; x++) st[x] = (uint32_t)((uint64_t)nf * x / 8), len[x] = (uint32_t)((uint64_t)nf * (x + 1) / 8) - st[x];
+        for (uint32_t j = 0; fwg.size() < nf; j++)
+            for (uint32_t x = 0; x < 8; x++)
+                if (j < len[x]) fwg.push_back(st[x] + j);
+    }
     // halos: per chunk the outside nodes its incidences read (sorted), the chunks owning them, each incidence's
     // two LDS slots
     std::vector<uint32_t> node_chunk(N);
@@ -427,7 +460,7 @@ int build(bl_ctx* c) {
     if ((rc = c->pm[0].upload(P)) || (rc = c->vm[0].upload(V)) || (rc = c->pm[1].alloc(N)) || (rc = c->vm[1].alloc(N)) ||
         (rc = c->ninfo.upload(ninfo)) || (rc = c->naux.upload(naux)) || (rc = c->noff.upload(noff)) ||
         (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iAB.upload(iAB)) || (rc = c->bcold.upload(bcold)) ||
-        (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->body_c0.upload(body_c0)) ||
+        (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->ch_rng.upload(ch_rng)) || (rc = c->fch_rng.upload(fch_rng)) || (rc = c->fwg.upload(fwg)) || (rc = c->body_c0.upload(body_c0)) ||
         (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
         (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)) || (rc = c->bar.alloc(1)) ||
@@ -454,7 +487,7 @@ int build(bl_ctx* c) {
     a.pm[0] = c->pm[0].p, a.pm[1] = c->pm[1].p, a.vm[0] = c->vm[0].p, a.vm[1] = c->vm[1].p;
     a.ninfo = c->ninfo.p, a.naux = c->naux.p, a.noff = c->noff.p;
     a.iA = c->iA.p, a.iM = c->iM.p, a.iAB = c->iAB.p, a.bcold = c->bcold.p, a.stress = c->stress.p;
-    a.ch_n0 = c->ch_n0.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
+    a.ch_n0 = c->ch_n0.p, a.ch_rng = c->ch_rng.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
     a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;
     a.halo_off = c->halo_off.p, a.halo = c->halo.p, a.nbr_off = c->nbr_off.p, a.nbr = c->nbr.p, a.iloc = c->iloc.p;
@@ -464,6 +497,7 @@ int build(bl_ctx* c) {
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
     a.wg_chunk = c->wg_chunk.p;
+    a.fch_rng = c->fch_rng.p, a.fwg = c->fwg.p, a.nfchunks = nf;
     c->parity = 0;
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;

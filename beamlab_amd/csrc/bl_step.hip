@@ -228,20 +228,29 @@ __device__ __forceinline__ f3 node_force(const KArgs& a, float mass, uint32_t b0
 // One chunk's substep with the incidence records streamed from memory: all of a thread's loads are issued before
 // its arithmetic (PER incidences per thread).
 template <int T, int PER, bool FORCES_ONLY = false>
-__device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int parity, float* sfx, float* sfy,
+__device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, int parity, float* sfx, float* sfy,
                                               float* sfz, uint8_t* sok, NodeCounts& cnt) {
     const float4* __restrict__ pin = a.pm[parity];
     const float4* __restrict__ vin = a.vm[parity];
     float4* __restrict__ pout = a.pm[parity ^ 1];
     float4* __restrict__ vout = a.vm[parity ^ 1];
-    const uint32_t n0 = a.ch_n0[c], n1 = a.ch_n0[c + 1];
-    const uint32_t e0 = a.noff[n0], e1 = a.noff[n1];
+    const uint32_t n0 = cr.x, n1 = cr.y, e0 = cr.z, e1 = cr.w;
     float4 A[PER], M[PER];
     uint2 ab[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const uint32_t e = e0 + threadIdx.x + k * T;
         if (e < e1) A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = a.iAB[e];
+    }
+    // the node phase's own loads (a chunk holds at most T nodes: one a thread), issued with the beam phase's
+    const uint32_t i = n0 + threadIdx.x;
+    const bool own = i < n1;
+    float4 P = make_float4(0, 0, 0, 0), V = make_float4(0, 0, 0, 0);
+    uint32_t nb0 = 0, nb1 = 0, info = 0;
+    float2 aux = make_float2(0, 0);
+    if (own) {
+        V = vin[i], nb0 = a.noff[i], nb1 = a.noff[i + 1];
+        if (!FORCES_ONLY) P = pin[i], info = a.ninfo[i], aux = a.naux[i];
     }
     float4 Pa[PER], Pb[PER], Va[PER], Vb[PER];
 #pragma unroll
@@ -266,16 +275,14 @@ __device__ __forceinline__ void chunk_substep(uint32_t c, const KArgs& a, int pa
         sok[s] = ok;
     }
     __syncthreads();
-    for (uint32_t i = n0 + threadIdx.x; i < n1; i += T) {
-        float4 P = pin[i], V = vin[i];
-        const uint32_t b0 = a.noff[i] - e0, b1 = a.noff[i + 1] - e0;
-        const f3 f = node_force(a, V.w, b0, b1, sfx, sfy, sfz, sok);
+    if (own) {
+        const f3 f = node_force(a, V.w, nb0 - e0, nb1 - e0, sfx, sfy, sfz, sok);
         if (FORCES_ONLY) {
             a.F[i] = make_float4(f.x, f.y, f.z, 0.0f);
-            continue;
+        } else {
+            node_finish(a, info, aux, P, V, f, cnt);
+            pout[i] = P, vout[i] = V;
         }
-        node_finish(a, a.ninfo[i], a.naux[i], P, V, f, cnt);
-        pout[i] = P, vout[i] = V;
     }
 }
 
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(T) void k_substep(KArgs a, int parity) {
     __shared__ uint4 sred[T / 64];
     const uint32_t c = xcd_chunk(a, blockIdx.x);
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    chunk_substep<T, PER>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+    chunk_substep<T, PER>(a.ch_rng[c], a, parity, sfx, sfy, sfz, sok, cnt);
     flush_chunk<T>(a, cnt, c, sred);
 }
 
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(T) void k_body_persistent(KArgs a, int parity0, int
     for (int s = 0; s < substeps; s++) {
         const int parity = parity0 ^ (s & 1);
         for (uint32_t c = c0; c < c1; c++) {
-            chunk_substep<T, PER>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+            chunk_substep<T, PER>(a.ch_rng[c], a, parity, sfx, sfy, sfz, sok, cnt);
             __syncthreads();
         }
     }
@@ -835,9 +842,8 @@ template <int T, int PER>
 __global__ __launch_bounds__(T) void k_forces(KArgs a, int parity) {
     __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
     __shared__ uint8_t sok[T * PER];
-    const uint32_t c = xcd_chunk(a, blockIdx.x);
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    chunk_substep<T, PER, true>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+    chunk_substep<T, PER, true>(a.fch_rng[a.fwg[blockIdx.x]], a, parity, sfx, sfy, sfz, sok, cnt);
 }
 
 // The static contacts and the integration from F, one thread per node (a workgroup's nodes may belong to several
@@ -884,18 +890,17 @@ template <int PER>
 __global__ __launch_bounds__(256) void k_forces_narrow(KArgs a, int parity, PArgs p) {
     __shared__ float sfx[256 * PER], sfy[256 * PER], sfz[256 * PER];
     __shared__ uint8_t sok[256 * PER];
-    if (blockIdx.x < a.nchunks) {
-        const uint32_t c = xcd_chunk(a, blockIdx.x);
+    if (blockIdx.x < a.nfchunks) {
         NodeCounts cnt = {0, 0, 0, 0.0f};
-        chunk_substep<256, PER, true>(c, a, parity, sfx, sfy, sfz, sok, cnt);
+        chunk_substep<256, PER, true>(a.fch_rng[a.fwg[blockIdx.x]], a, parity, sfx, sfy, sfz, sok, cnt);
     } else {
-        narrow_block(a, p, parity, blockIdx.x - a.nchunks);
+        narrow_block(a, p, parity, blockIdx.x - a.nfchunks);
     }
 }
 
 int launch_forces_narrow(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t, const PArgs& p, uint32_t nblk) {
     const int per = t.incidences_per_thread;
-    const dim3 g(a.nchunks + nblk);
+    const dim3 g(a.nfchunks + nblk);
     if (t.block != 256) return BL_ERR_ARG;
     if (per == 1) hipLaunchKernelGGL((k_forces_narrow<1>), g, dim3(256), 0, st, a, parity, p);
     else if (per == 2) hipLaunchKernelGGL((k_forces_narrow<2>), g, dim3(256), 0, st, a, parity, p);
@@ -907,13 +912,14 @@ int launch_forces_narrow(const KArgs& a, hipStream_t st, int parity, const bl_tu
 
 int launch_forces(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t) {
     const int per = t.incidences_per_thread, T = t.block;
-    if (T == 256 && per == 1) hipLaunchKernelGGL((k_forces<256, 1>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
-    else if (T == 256 && per == 2) hipLaunchKernelGGL((k_forces<256, 2>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
-    else if (T == 256 && per == 4) hipLaunchKernelGGL((k_forces<256, 4>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
-    else if (T == 256 && per == 8) hipLaunchKernelGGL((k_forces<256, 8>), dim3(a.nchunks), dim3(256), 0, st, a, parity);
-    else if (T == 512 && per == 2) hipLaunchKernelGGL((k_forces<512, 2>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
-    else if (T == 512 && per == 4) hipLaunchKernelGGL((k_forces<512, 4>), dim3(a.nchunks), dim3(512), 0, st, a, parity);
-    else if (T == 1024 && per == 4) hipLaunchKernelGGL((k_forces<1024, 4>), dim3(a.nchunks), dim3(1024), 0, st, a, parity);
+    if (a.nfchunks == 0) return BL_OK;
+    if (T == 256 && per == 1) hipLaunchKernelGGL((k_forces<256, 1>), dim3(a.nfchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 2) hipLaunchKernelGGL((k_forces<256, 2>), dim3(a.nfchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 4) hipLaunchKernelGGL((k_forces<256, 4>), dim3(a.nfchunks), dim3(256), 0, st, a, parity);
+    else if (T == 256 && per == 8) hipLaunchKernelGGL((k_forces<256, 8>), dim3(a.nfchunks), dim3(256), 0, st, a, parity);
+    else if (T == 512 && per == 2) hipLaunchKernelGGL((k_forces<512, 2>), dim3(a.nfchunks), dim3(512), 0, st, a, parity);
+    else if (T == 512 && per == 4) hipLaunchKernelGGL((k_forces<512, 4>), dim3(a.nfchunks), dim3(512), 0, st, a, parity);
+    else if (T == 1024 && per == 4) hipLaunchKernelGGL((k_forces<1024, 4>), dim3(a.nfchunks), dim3(1024), 0, st, a, parity);
     else return BL_ERR_ARG;
     return BL_OK_OR_HIP();
 }

@@ -49,6 +49,7 @@ struct KArgs {
     const float4* bcold;   // per beam: plastic, L0, support_limit, 0
     float* stress;         // per beam
     const uint32_t* ch_n0; // chunk node starts, [chunks + 1]
+    const uint4* ch_rng;   // per chunk: first node, end node, first incidence, end incidence (one load)
     const uint32_t* body_c0; // first chunk of each body, [bodies + 1]
     const BodyParam* bodies;
     const bl_box* boxes;
@@ -81,6 +82,11 @@ struct KArgs {
     uint32_t nchunks;
     uint32_t nbodies;
     const uint32_t* wg_chunk; // workgroup -> chunk (XCD placement), or null for the identity
+    // the force phase's own chunks (k_forces): the chunks with beams, then the nodes of the beamless chunks packed
+    // into runs of at most a workgroup (bodies without beams - balls - would each take a workgroup of their own)
+    const uint4* fch_rng;     // (first node, end node, first incidence, end incidence)
+    const uint32_t* fwg;      // workgroup -> force chunk (contiguous ranges over the XCDs)
+    uint32_t nfchunks;
 };
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
