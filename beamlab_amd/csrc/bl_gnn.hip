@@ -501,10 +501,26 @@ __global__ void k_pcg_dot(uint32_t n6, const double* a, const double* b, double*
 
 // the scalars of an iteration. mode 0: the start (rz0, ||r0||, convergence); mode 1: alpha = rz / p.Ap; mode 2: beta and
 // the convergence test after the update. sc: [rz, alpha, beta, ||b||^2, tol^2, iterations, ||r||^2]
-__global__ void k_pcg_scalar(const double* part, uint32_t nblk, int stride, int mode, double* sc, int* done, int* iters) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || *done) return;
-    double a = 0, b = 0;
-    for (uint32_t k = 0; k < nblk; k++) a += part[stride * k], b += stride > 1 ? part[stride * k + 1] : 0.0;
+// The sums of the partials (one or two interleaved series) by one workgroup of 256: each thread a strided subset
+// in order, then a fixed tree - the same order every time. Thread 0 gets the totals.
+__device__ __forceinline__ void block_sum2(const double* part, uint32_t n, int stride, double& a, double& b) {
+    __shared__ double s1[256], s2[256];
+    double x = 0, y = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += 256) x += part[stride * k], y += stride > 1 ? part[stride * k + 1] : 0.0;
+    s1[threadIdx.x] = x, s2[threadIdx.x] = y;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) s1[threadIdx.x] += s1[threadIdx.x + off], s2[threadIdx.x] += s2[threadIdx.x + off];
+        __syncthreads();
+    }
+    a = s1[0], b = s2[0];
+}
+
+__global__ __launch_bounds__(256) void k_pcg_scalar(const double* part, uint32_t nblk, int stride, int mode, double* sc, int* done, int* iters) {
+    if (*done) return;
+    double a, b;
+    block_sum2(part, nblk, stride, a, b);
+    if (threadIdx.x != 0) return;
     if (mode == 1) {
         sc[1] = a != 0 ? sc[0] / a : 0.0;
         return;
@@ -523,10 +539,11 @@ __global__ void k_pcg_scalar(const double* part, uint32_t nblk, int stride, int 
 // The scalars of PCG with the factor as its preconditioner (the residual tested before the preconditioner is
 // applied, so a converged iteration skips its solve). mode 0: rz0; 1: alpha = rz / p.Ap (one more iteration);
 // 2: ||r||^2 and the test; 3: beta = rz' / rz. sc as k_pcg_scalar's.
-__global__ void k_fac_scalar(const double* part, uint32_t nblk, int mode, double* sc, int* done, int* iters) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || *done) return;
-    double a = 0;
-    for (uint32_t k = 0; k < nblk; k++) a += part[k];
+__global__ __launch_bounds__(256) void k_fac_scalar(const double* part, uint32_t nblk, int mode, double* sc, int* done, int* iters) {
+    if (*done) return;
+    double a, b;
+    block_sum2(part, nblk, 1, a, b);
+    if (threadIdx.x != 0) return;
     if (mode == 0) {
         sc[0] = a;
     } else if (mode == 1) {
@@ -564,12 +581,11 @@ __global__ void k_clear_i(int* p, uint32_t n) {
     if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
-// ||b||^2 (the partials summed in block order) into sc[3], tol^2 into sc[4]
-__global__ void k_norm_init(const double* part, uint32_t nblk, double* sc, double tol2) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double a = 0;
-    for (uint32_t k = 0; k < nblk; k++) a += part[k];
-    sc[3] = a, sc[4] = tol2;
+// ||b||^2 (the partials summed in block_sum2's fixed order) into sc[3], tol^2 into sc[4]
+__global__ __launch_bounds__(256) void k_norm_init(const double* part, uint32_t nblk, double* sc, double tol2) {
+    double a, b;
+    block_sum2(part, nblk, 1, a, b);
+    if (threadIdx.x == 0) sc[3] = a, sc[4] = tol2;
 }
 
 // y = A x over the engine's assembled system (as its own product), skipped once the solve has converged
@@ -824,19 +840,19 @@ struct bl_gnn {
         int* it = pcg_flag.p + 1;
         hipLaunchKernelGGL(k_clear_i, dim3(1), dim3(64), 0, st, pcg_flag.p, 2u);
         hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, v.rhs, v.rhs, pcg_part.p, done);
-        hipLaunchKernelGGL(k_norm_init, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, sc, tol * tol);
+        hipLaunchKernelGGL(k_norm_init, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, sc, tol * tol);
         hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)x, pcg_Ap.p, done);
         hipLaunchKernelGGL(k_sub, grid(n6), dim3(256), 0, st, n6, v.rhs, pcg_Ap.p, pcg_r.p);
         hipLaunchKernelGGL(k_pcg_prec, dim3(nblk), dim3(256), 0, st, n, Dinv.p, sv, pcg_r.p, pcg_z.p, pcg_part.p, done);
-        hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nblk, 2, 0, sc, done, it);
+        hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nblk, 2, 0, sc, done, it);
         hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 1, done);
         for (int k = 0; k < max_iter; k++) {
             hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)pcg_p.p, pcg_Ap.p, done);
             hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_p.p, pcg_Ap.p, pcg_part.p, done);
-            hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 1, 1, sc, done, it);
+            hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 1, 1, sc, done, it);
             hipLaunchKernelGGL(k_pcg_xr, grid(n6), dim3(256), 0, st, n6, sc, x, pcg_r.p, pcg_p.p, pcg_Ap.p, done);
             hipLaunchKernelGGL(k_pcg_prec, dim3(nblk), dim3(256), 0, st, n, Dinv.p, sv, pcg_r.p, pcg_z.p, pcg_part.p, done);
-            hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nblk, 2, 2, sc, done, it);
+            hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nblk, 2, 2, sc, done, it);
             hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 0, done);
         }
         GCHK(hipGetLastError());
@@ -859,26 +875,26 @@ struct bl_gnn {
         int rc;
         hipLaunchKernelGGL(k_clear_i, dim3(1), dim3(64), 0, st, pcg_flag.p, 2u);
         hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, v.rhs, v.rhs, pcg_part.p, done);
-        hipLaunchKernelGGL(k_norm_init, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, sc, tol * tol);
+        hipLaunchKernelGGL(k_norm_init, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, sc, tol * tol);
         hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)x, pcg_Ap.p, done);
         hipLaunchKernelGGL(k_sub, grid(n6), dim3(256), 0, st, n6, v.rhs, pcg_Ap.p, pcg_r.p);
         hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_r.p, pcg_part.p, done);
-        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 2, sc, done, it);
+        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 2, sc, done, it);
         if ((rc = blf::fem_precond(fm, pcg_r.p, pcg_z.p, done))) return rc;
         hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_z.p, pcg_part.p, done);
-        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 0, sc, done, it);
+        hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 0, sc, done, it);
         hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 1, done);
         for (int k = 0; k < max_iter; k++) {
             hipLaunchKernelGGL(k_spmv_g, dim3(nblk), dim3(256), 0, st, v, (const double*)pcg_p.p, pcg_Ap.p, done);
             hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_p.p, pcg_Ap.p, pcg_part.p, done);
-            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 1, sc, done, it);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 1, sc, done, it);
             hipLaunchKernelGGL(k_pcg_xr, grid(n6), dim3(256), 0, st, n6, sc, x, pcg_r.p, pcg_p.p, pcg_Ap.p, done);
             hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_r.p, pcg_part.p, done);
-            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 2, sc, done, it);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 2, sc, done, it);
             if (k + 1 == max_iter) break; // (the last iteration's search direction is never used)
             if ((rc = blf::fem_precond(fm, pcg_r.p, pcg_z.p, done))) return rc;
             hipLaunchKernelGGL(k_pcg_dot, dim3(nb6), dim3(256), 0, st, n6, pcg_r.p, pcg_z.p, pcg_part.p, done);
-            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(64), 0, st, pcg_part.p, nb6, 3, sc, done, it);
+            hipLaunchKernelGGL(k_fac_scalar, dim3(1), dim3(256), 0, st, pcg_part.p, nb6, 3, sc, done, it);
             hipLaunchKernelGGL(k_pcg_p, grid(n6), dim3(256), 0, st, n6, sc, pcg_z.p, pcg_p.p, 0, done);
         }
         GCHK(hipGetLastError());
