@@ -807,7 +807,7 @@ bool needs_contacts(bl_ctx* c, int substeps) {
         c->hv.mn[k] = c->bodies[k].mn, c->hv.mx[k] = c->bodies[k].mx;
     }
     c->contacts.build_islands(c->hv, (float)substeps * c->s.dt);
-    return !c->contacts.pair_islands.empty();
+    return c->contacts.npair_islands() > 0;
 }
 
 // Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.

@@ -135,7 +135,10 @@ struct Buf {
 
 class ContactEngine {
 public:
-    std::vector<std::vector<uint32_t>> islands, pair_islands; // this frame's
+    // this frame's pair islands (islands of several bodies, or of one self-colliding body), by their first body,
+    // members in body order: CSR
+    std::vector<uint32_t> pair_off, pair_mem;
+    size_t npair_islands() const { return pair_off.empty() ? 0 : pair_off.size() - 1; }
     int interval = 1;
     RStat last{};
     // counters (bl_stats)
@@ -165,6 +168,13 @@ public:
 private:
     uint32_t nnodes = 0, ntris = 0, ncaps = 0, nbodies = 0, nsamples = 0;
     size_t nnt = 0, nnc = 0, nct = 0;
+    // the island building's scratch, and the tables last uploaded (a frame whose tables are the same uploads nothing)
+    std::vector<int> parent_, root_isl_, body_isl_, pair_id_, isl_first_, isl_;
+    std::vector<uint32_t> isl_size_, cursor_;
+    std::vector<float> bx_;
+    std::vector<int> up_isl_, up_mi_;
+    std::vector<uint32_t> up_off_, up_mem_;
+    bool up_valid_ = false;
     uint32_t tsize = 0; // node table buckets
     Buf<uint8_t> temp;
     Buf<int> bisl, bmem;

@@ -1022,6 +1022,7 @@ int ContactEngine::scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t
 }
 
 void ContactEngine::release() {
+    up_valid_ = false;
     temp.release();
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
@@ -1061,6 +1062,7 @@ int ContactEngine::ensure_lists() {
 
 // The per-body tables (first triangle, first size sample), the node hash and the first list sizes.
 int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C) {
+    up_valid_ = false;
     nnodes = N, ntris = T, ncaps = C, nbodies = (uint32_t)hv.bodies.size();
     std::vector<uint32_t> tri0(nbodies + 1, 0), s0(nbodies + 1, 0);
     for (uint32_t i = 0; i < nbodies; i++) {
@@ -1119,17 +1121,19 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
 // World::build_islands (src/phys/world.cpp:636-762) for awake node/beam bodies: union-find over the expanded bounds
 // of bodies that can touch (contacters against triangles or capsules, capsules against triangles); an island's
 // bodies in body order, islands by their first body. Islands of several bodies, or of one self-colliding body,
-// search for pairs. The candidate pairs of bodies are found by sorting the boxes along x (a sweep; the union order
-// does not change the islands).
+// search for pairs. The candidate pairs of bodies come from the x-z grid of bl_islands.h (the union order does not
+// change the islands). The scratch vectors live across frames.
 void ContactEngine::build_islands(const HostView& hv, float frame_time) {
     const size_t nb = hv.bodies.size();
-    std::vector<int> parent(nb);
+    std::vector<int>& parent = parent_;
+    parent.resize(nb);
     for (size_t i = 0; i < nb; i++) parent[i] = (int)i;
     auto find = [&](int x) {
         while (parent[x] != x) x = parent[x] = parent[parent[x]];
         return x;
     };
-    std::vector<float> bx(nb * 6);
+    std::vector<float>& bx = bx_;
+    bx.resize(nb * 6);
     for (size_t i = 0; i < nb; i++) {
         const float m = smin(hv.max_speed[i], 200.0f) * frame_time * 2.0f + 0.3f;
         const float r = m + 0.0f; // (no hull triangles: no hull reach)
@@ -1147,37 +1151,64 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         if (x != y) parent[x] = y;
     };
     if (hv.inter_body) overlapping_pairs(bx.data(), nb, [&](uint32_t i, uint32_t j) { consider(i, j); });
-    islands.clear();
-    std::vector<int> island_of(nb, -1);
+    // islands numbered by their first body; the pair islands' members in body order
+    root_isl_.assign(nb, -1);
+    body_isl_.resize(nb);
+    isl_size_.clear(), isl_first_.clear();
     for (size_t i = 0; i < nb; i++) {
         const int r = find((int)i);
-        if (island_of[r] < 0) island_of[r] = (int)islands.size(), islands.emplace_back();
-        islands[island_of[r]].push_back((uint32_t)i);
+        if (root_isl_[r] < 0) root_isl_[r] = (int)isl_size_.size(), isl_size_.push_back(0), isl_first_.push_back((int)i);
+        body_isl_[i] = root_isl_[r];
+        isl_size_[root_isl_[r]]++;
     }
-    pair_islands.clear();
-    for (auto& is : islands)
-        if (hv.inter_body && (is.size() > 1 || hv.bodies[is[0]].self)) pair_islands.push_back(is);
+    const size_t ni = isl_size_.size();
+    pair_id_.assign(ni, -1);
+    uint32_t np = 0;
+    for (size_t k = 0; k < ni; k++)
+        if (hv.inter_body && (isl_size_[k] > 1 || hv.bodies[isl_first_[k]].self)) pair_id_[k] = (int)np++;
+    pair_off.assign(np + 1, 0);
+    for (size_t k = 0; k < ni; k++)
+        if (pair_id_[k] >= 0) pair_off[pair_id_[k] + 1] = isl_size_[k];
+    for (uint32_t k = 0; k < np; k++) pair_off[k + 1] += pair_off[k];
+    pair_mem.resize(pair_off[np]);
+    cursor_.assign(pair_off.begin(), pair_off.end() - 1);
+    for (size_t i = 0; i < nb; i++) {
+        const int p = pair_id_[body_isl_[i]];
+        if (p >= 0) pair_mem[cursor_[p]++] = (uint32_t)i;
+    }
 }
 
 int ContactEngine::begin_frame(hipStream_t st) {
-    std::vector<int> isl(nbodies, -1), mi(nbodies, -1);
-    std::vector<uint32_t> off(pair_islands.size() + 1, 0), mem;
-    for (size_t k = 0; k < pair_islands.size(); k++) {
-        for (uint32_t g : pair_islands[k]) isl[g] = (int)k, mi[g] = (int)mem.size(), mem.push_back(g);
-        off[k + 1] = (uint32_t)mem.size();
-    }
+    const size_t np = npair_islands();
+    const std::vector<uint32_t>& off = pair_off;
+    const std::vector<uint32_t>& mem = pair_mem;
+    std::vector<int>& isl = isl_;
+    std::vector<int>& mi = root_isl_; // (the island building is done with it)
+    isl.assign(nbodies, -1), mi.assign(nbodies, -1);
+    for (size_t k = 0; k < np; k++)
+        for (uint32_t p = off[k]; p < off[k + 1]; p++) isl[mem[p]] = (int)k, mi[mem[p]] = (int)p;
     nmem = (uint32_t)mem.size();
-    RC(imem_off.ensure(off.size()));
+    const void* was[2] = {imem_off.p, imem.p};
+    RC(imem_off.ensure(np + 1));
     RC(imem.ensure(mem.size() + 1));
     RC(pcnt.ensure(nmem + 1));
     RC(poff.ensure(nmem + 1));
     cap_part = std::max<uint32_t>(cap_part, std::max<uint32_t>(4096, 8 * nmem));
     RC(plist.ensure(cap_part));
-    if (nbodies) CHK(hipMemcpyAsync(bisl.p, isl.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
-    if (nbodies) CHK(hipMemcpyAsync(bmem.p, mi.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
-    CHK(hipMemcpyAsync(imem_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
-    if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
-    CHK(hipStreamSynchronize(st)); // (the host vectors go out of scope)
+    // (the same tables as the last frame's, in the same buffers: nothing to upload)
+    const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && isl == up_isl_ && mi == up_mi_ &&
+                      off == up_off_ && mem == up_mem_;
+    if (!same) {
+        const std::vector<uint32_t> off0(1, 0u);
+        const std::vector<uint32_t>& offu = np ? off : off0;
+        if (nbodies) CHK(hipMemcpyAsync(bisl.p, isl.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
+        if (nbodies) CHK(hipMemcpyAsync(bmem.p, mi.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
+        CHK(hipMemcpyAsync(imem_off.p, offu.data(), 4 * offu.size(), hipMemcpyHostToDevice, st));
+        if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
+        CHK(hipStreamSynchronize(st)); // (off0 goes out of scope)
+        up_isl_ = isl, up_mi_ = mi, up_off_ = off, up_mem_ = mem;
+        up_valid_ = true;
+    }
     nnt = nnc = nct = 0;
     return BL_OK;
 }
@@ -1246,12 +1277,12 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
 // reads the state, so a second run finds the same lists).
 int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t st, int parity) {
     nnt = nnc = nct = 0;
-    if (pair_islands.empty()) return BL_OK;
+    if (npair_islands() == 0) return BL_OK;
     CArgs c{};
     c.parity = parity;
     c.horizon = (float)(interval > 1 ? interval : 1) * a.dt;
     c.interval = interval, c.dt = a.dt;
-    c.bisl = bisl.p, c.imem_off = imem_off.p, c.imem = imem.p, c.nisl = (uint32_t)pair_islands.size();
+    c.bisl = bisl.p, c.imem_off = imem_off.p, c.imem = imem.p, c.nisl = (uint32_t)npair_islands();
     c.nmem = nmem, c.bmem = bmem.p, c.pcnt = pcnt.p, c.poff = poff.p;
     c.ipar = ipar.p, c.bbox = bbox.p, c.bquery = bquery.p, c.body_box = body_box;
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
@@ -1279,7 +1310,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
         RC(ensure_lists());
     }
     nnt = last.nnt, nnc = last.nnc, nct = last.nct;
-    pair_rebuilds += (int64_t)pair_islands.size();
+    pair_rebuilds += (int64_t)npair_islands();
     contact_pairs = last.last_pairs;
     return BL_OK;
 }
