@@ -894,7 +894,6 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
 // read ahead of the waits, as in the forward pass
 __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, uint32_t n) {
 #pragma clang fp contract(fast)
-    __shared__ double xs[256];
     __shared__ double xb[2][64];
     __shared__ double dots[64];
     __shared__ double Ld[64][65];
@@ -908,26 +907,36 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     const double* F = s.F + fr.off_l;
     double* X = s.X + fr.voff;
     if (tid == 0) sbad = 0;
-    for (uint32_t e = tid; e < 64 * 64; e += 256) {
-        const uint32_t i = e % 64, j = e / 64;
-        Ld[i][j] = (i < w && j < w && i >= j) ? F[(uint64_t)(J + j) * m + J + i] : 0.0;
-    }
     // this wave's 16 columns: partial sums of L[r][j] x_r per lane
     double pj[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) pj[q] = 0;
-    // the rows below the own columns: x known (the ancestors'), 256 at a time
-    for (uint32_t r0 = c; r0 < m; r0 += 256) {
-        const uint32_t r = r0 + tid;
-        if (r < m) xs[tid] = s.x[6 * (size_t)s.ord2f[s.rows[fr.row0 + (r - c) / 6]] + (r - c) % 6];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const uint32_t j = part * 16 + q;
-            if (j < w)
-                for (uint32_t k = lane; k < 256 && r0 + k < m; k += 64) pj[q] += F[(uint64_t)(J + j) * m + r0 + k] * xs[k];
+    // the rows below the own columns: x known (the ancestors'), gathered 4 096 at a time (all of a front's, on this
+    // plate) into the diagonal block's LDS, then the products, lane over rows (rows in increasing order per lane)
+    for (uint32_t base = 0; base < m - c; base += 4096) {
+        double* xr = &Ld[0][0];
+        const uint32_t nb = m - c - base < 4096u ? m - c - base : 4096u;
+        for (uint32_t k = tid; k < nb; k += 256) {
+            const uint32_t kk = base + k;
+            xr[k] = s.x[6 * (size_t)s.ord2f[s.rows[fr.row0 + kk / 6]] + kk % 6];
         }
         __syncthreads();
+        for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            if (k < nb) {
+                const double xv = xr[k];
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t j = part * 16 + q;
+                    if (j < w) pj[q] += F[(uint64_t)(J + j) * m + c + base + k] * xv;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = tid; e < 64 * 64; e += 256) {
+        const uint32_t i = e % 64, j = e / 64;
+        Ld[i][j] = (i < w && j < w && i >= j) ? F[(uint64_t)(J + j) * m + J + i] : 0.0;
     }
     // the later own blocks, the last first, as their flags come up
     const uint32_t nb = (c + 63) / 64;
