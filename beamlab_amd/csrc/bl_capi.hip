@@ -358,24 +358,36 @@ int build(bl_ctx* c) {
     }
     body_c0.push_back((uint32_t)ch_n0.size());
     const uint32_t nchunks = (uint32_t)ch_n0.size();
-    // workgroup -> chunk: XCD x runs workgroups x, x + 8, ...; a body of 8 chunks or more is split into 8
-    // contiguous ranges, one per XCD (the longer ranges rotating over the XCDs from body to body), smaller bodies
-    // are dealt chunk by chunk; lists longer than their XCD's share hand their tails to the short ones
+    // workgroup -> chunk: XCD x runs workgroups x, x + 8, ...; each XCD has its own L2, and a chunk's beams read
+    // the nodes of its body. A body of more than 1/32 of all chunks (and at least 8) is split into 8 contiguous
+    // ranges, one per XCD (the longer ranges rotating over the XCDs from body to body); every other body goes whole
+    // to one XCD, the largest first onto the least loaded, so that its nodes are fetched into one L2 only. Lists
+    // longer than their XCD's share hand their tails to the short ones.
     std::vector<std::vector<uint32_t>> xl(8);
     {
         uint32_t rot = 0;
+        std::vector<uint32_t> whole;
         for (size_t k = 0; k + 1 < body_c0.size(); k++) {
             const uint32_t c0 = body_c0[k], n = body_c0[k + 1] - c0;
-            if (n >= 8) {
+            if (n >= 8 && (uint64_t)n * 32 > nchunks) {
                 uint32_t at = c0;
                 for (uint32_t q = 0; q < 8; q++) {
                     const uint32_t x = (rot + q) % 8, len = n / 8 + (q < n % 8 ? 1u : 0u);
                     for (uint32_t j = 0; j < len; j++) xl[x].push_back(at++);
                 }
                 rot = (rot + n % 8) % 8;
-            } else {
-                for (uint32_t j = 0; j < n; j++) xl[rot].push_back(c0 + j), rot = (rot + 1) % 8;
+            } else if (n) {
+                whole.push_back((uint32_t)k);
             }
+        }
+        std::stable_sort(whole.begin(), whole.end(), [&](uint32_t a, uint32_t b) {
+            return body_c0[a + 1] - body_c0[a] > body_c0[b + 1] - body_c0[b];
+        });
+        for (uint32_t k : whole) {
+            uint32_t x = 0;
+            for (uint32_t y = 1; y < 8; y++)
+                if (xl[y].size() < xl[x].size()) x = y;
+            for (uint32_t c = body_c0[k]; c < body_c0[k + 1]; c++) xl[x].push_back(c);
         }
         std::vector<uint32_t> spill;
         for (uint32_t x = 0; x < 8; x++) {

@@ -227,6 +227,17 @@ __device__ __forceinline__ f3 node_force(const KArgs& a, float mass, uint32_t b0
 // ---------------------------------------------------------------------------------------------- streamed chunk
 // One chunk's substep with the incidence records streamed from memory: all of a thread's loads are issued before
 // its arithmetic (PER incidences per thread).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+    const nt_f4 v = __builtin_nontemporal_load((const nt_f4*)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_nt(const uint2* p) {
+    const nt_u2 v = __builtin_nontemporal_load((const nt_u2*)p);
+    return make_uint2(v.x, v.y);
+}
+
 template <int T, int PER, bool FORCES_ONLY = false>
 __device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, int parity, float* sfx, float* sfy,
                                               float* sfz, uint8_t* sok, NodeCounts& cnt) {
@@ -237,10 +248,12 @@ __device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, in
     const uint32_t n0 = cr.x, n1 = cr.y, e0 = cr.z, e1 = cr.w;
     float4 A[PER], M[PER];
     uint2 ab[PER];
+    // (the incidence records stream through once a substep: loaded non-temporal, so that they do not push the
+    // nodes, read by many workgroups, out of the L2)
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const uint32_t e = e0 + threadIdx.x + k * T;
-        if (e < e1) A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = a.iAB[e];
+        if (e < e1) A[k] = ld_nt(a.iA + e), M[k] = ld_nt(a.iM + e), ab[k] = ld_nt(a.iAB + e);
     }
     // the node phase's own loads (a chunk holds at most T nodes: one a thread), issued with the beam phase's
     const uint32_t i = n0 + threadIdx.x;
