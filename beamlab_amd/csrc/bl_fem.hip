@@ -663,9 +663,36 @@ __device__ void panel_task(const FacArgs& f, uint2 t, uint32_t k, double* sh) {
                     const double id = y0 * (1.5 - (0.5 * pv) * (y0 * y0)), d = pv * id;
                     a[j] = (int)ri == j ? d : a[j] * id;
                     if (lane == 0) inv[j] = id;
+                    // (blocked in halves: the first 16 pivots update columns < 16 only; the lower right 16 x 16
+                    // block takes L21 L21^T at once after the 16th)
 #pragma unroll
-                    for (int c = j + 1; c < kT; c++) a[c] -= a[j] * readlane_d(a[j], c);
+                    for (int c = j + 1; c < (j < 16 ? 16 : kT); c++) a[c] -= a[j] * readlane_d(a[j], c);
                     if (lane < kT) Td[ri][j] = j <= (int)ri ? a[j] : 0.0;
+                    if (j == 15) {
+                        // L21 (rows 16.., columns < 16) through LDS (the previous panel's rows, done with), its
+                        // product with itself on the matrix cores, the product back into rows 16..
+                        double (*S)[17] = (double (*)[17])&Ld[0][0];
+                        double (*Dm)[17] = (double (*)[17])(&Ld[0][0] + 16 * 17);
+                        if (lane >= 16 && lane < 32)
+#pragma unroll
+                            for (int q = 0; q < 16; q++) S[lane - 16][q] = a[q];
+                        __builtin_amdgcn_s_waitcnt(0xc07f);
+                        __builtin_amdgcn_wave_barrier();
+                        const uint32_t li = lane & 15, lk = lane >> 4;
+                        dbl4 acc = (dbl4){0, 0, 0, 0};
+#pragma unroll
+                        for (int kk = 0; kk < 16; kk += 4) {
+                            const double v = S[li][kk + lk];
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; r++) Dm[lk + 4 * r][li] = acc[r];
+                        __builtin_amdgcn_s_waitcnt(0xc07f);
+                        __builtin_amdgcn_wave_barrier();
+                        if (ri >= 16)
+#pragma unroll
+                            for (int c = 16; c < kT; c++) a[c] -= Dm[ri - 16][c - 16];
+                    }
                 }
                 __syncthreads();
                 if (xrow) {
