@@ -1121,7 +1121,7 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
 // World::build_islands (src/phys/world.cpp:636-762) for awake node/beam bodies: union-find over the expanded bounds
 // of bodies that can touch (contacters against triangles or capsules, capsules against triangles); an island's
 // bodies in body order, islands by their first body. Islands of several bodies, or of one self-colliding body,
-// search for pairs. The candidate pairs of bodies come from the x-z grid of bl_islands.h (the union order does not
+// search for pairs. The candidate pairs of bodies come from the 3-D grid of bl_islands.h (the union order does not
 // change the islands). The scratch vectors live across frames.
 void ContactEngine::build_islands(const HostView& hv, float frame_time) {
     const size_t nb = hv.bodies.size();

@@ -1,5 +1,5 @@
 """The island building's broadphase (beamlab_amd/csrc/bl_islands.h, host C++): the pairs of overlapping body boxes
-from the x-z grid against the plain all-pairs test, as sets, on random scenes - small boxes in clusters, a few large
+from the 3-D grid against the plain all-pairs test, as sets, on random scenes - small boxes in clusters, a few large
 ones, boxes touching at a face, and boxes that are not finite. Compiled with g++ on the host; no GPU."""
 import os
 import shutil
@@ -81,6 +81,19 @@ def test_grid_pairs_equal_all_pairs(harness, seed, n):
     want = brute(bx.astype(np.float64))
     assert got == want
     assert len(want) > n // 4
+
+
+def test_sparse_wide_and_flat_scenes(harness):
+    """A 2 km arena of small boxes (the grid's cells enlarged to bound their count) and a flat layer (one cell in y)."""
+    rng = np.random.default_rng(5)
+    c = rng.uniform(-1000, 1000, (400, 3)).astype(np.float32)
+    c[:200] = rng.uniform(-3, 3, (200, 3)).astype(np.float32)
+    h = rng.uniform(0.1, 0.8, (400, 3)).astype(np.float32)
+    bx = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    assert run(harness, bx) == brute(bx.astype(np.float64))
+    c[:, 1] = 0.0
+    bx = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    assert run(harness, bx) == brute(bx.astype(np.float64))
 
 
 def test_empty_and_single(harness):
