@@ -611,8 +611,31 @@ __device__ __forceinline__ void body_reduce(const KArgs& a, uint32_t i, bool val
         kmx[k] = ok ? fkey(p[k]) : 0u;
     }
     if (bfirst != blast) {
+        // several bodies in the workgroup: a wave whose nodes are one body's reduces first and makes one atomic per
+        // word, any other wave one atomic per node
+        const uint32_t bd = valid ? (a.ninfo[i] >> 16) : 0xffffffffu;
+        const uint32_t b0 = (uint32_t)__shfl((int)bd, 0);
+        if (b0 != 0xffffffffu && __all(!valid || bd == b0)) {
+            for (int off = 32; off > 0; off >>= 1) {
+                kv = max(kv, (uint32_t)__shfl_xor((int)kv, off));
+                for (int k = 0; k < 3; k++) {
+                    kmn[k] = min(kmn[k], (uint32_t)__shfl_xor((int)kmn[k], off));
+                    kmx[k] = max(kmx[k], (uint32_t)__shfl_xor((int)kmx[k], off));
+                }
+            }
+            if ((threadIdx.x & 63) == 0) {
+                unsigned int* w = a.body_frame + b0 * BF_N;
+                if (want_v2 && kv && kv > rd_word(&w[BF_V2])) atomicMax(&w[BF_V2], kv);
+                if (want_box)
+                    for (int k = 0; k < 3; k++) {
+                        if (kmn[k] < rd_word(&w[BF_MN + k])) atomicMin(&w[BF_MN + k], kmn[k]);
+                        if (kmx[k] > rd_word(&w[BF_MX + k])) atomicMax(&w[BF_MX + k], kmx[k]);
+                    }
+            }
+            return;
+        }
         if (!valid) return;
-        unsigned int* w = a.body_frame + (a.ninfo[i] >> 16) * BF_N;
+        unsigned int* w = a.body_frame + bd * BF_N;
         if (kv) atomicMax(&w[BF_V2], kv);
         for (int k = 0; k < 3; k++)
             if (want_box && p[k] == p[k]) atomicMin(&w[BF_MN + k], kmn[k]), atomicMax(&w[BF_MX + k], kmx[k]);
