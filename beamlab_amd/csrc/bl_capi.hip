@@ -69,6 +69,27 @@ struct DevArr {
 
 } // namespace
 
+// Page-locked host memory (hipHostMalloc): copies to and from it run on the copy engines without staging.
+struct HostPinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t n) {
+        if (p && n <= bytes) return true;
+        release();
+        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = n;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
 struct bl_ctx {
     bl_settings s{};
     int device = 0;
@@ -106,6 +127,11 @@ struct bl_ctx {
     HostView hv;
     bool contact_geometry = false; // some body has triangles, capsules or self collision
     bool frame_pending = false;   // a frame enqueued whose post-frame check has not run
+    // pinned host staging: the frame-end readback (error word, then the body frame words), queued behind the
+    // frame's kernels so that the post-frame check only waits; the first rebuild's bounds words on their way up
+    HostPinned fin, w0;
+    bool fin_queued = false;
+    hipEvent_t w0_ev = nullptr;
     int64_t repairs = 0;
     std::vector<uint32_t> a_slot;   // global beam -> incidence of its a end
     std::vector<uint32_t> h_noff;
@@ -144,6 +170,7 @@ struct bl_ctx {
         wstat.release();
         contacts.release();
         frame_pending = false;
+        fin_queued = false;
         built = false;
     }
 };
@@ -727,6 +754,21 @@ int launch_segment(bl_ctx* c, const bl_tuning& t, int len, int tail) {
 // rebuild (list sizes) - about one substep in 17 at the default 120 Hz pair rate. With the graph form the substeps
 // between two rebuilds go as one graph launch; `kev` (per-launch events around the force phase) takes the
 // launches one by one.
+// The frame just enqueued is pending its post-frame check; its error word and body frame words are copied into
+// pinned memory behind it.
+int frame_enqueued(bl_ctx* c) {
+    c->frame_pending = true;
+    c->fin_queued = false;
+    const size_t nb = c->bodies.size();
+    if (!c->fin.ensure(256 + sizeof(unsigned int) * nb * BF_N)) return BL_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(c->fin.p, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (nb)
+        HIPCHK(hipMemcpyAsync((char*)c->fin.p + 256, c->body_frame.p, sizeof(unsigned int) * nb * BF_N,
+                              hipMemcpyDeviceToHost, c->stream));
+    c->fin_queued = true;
+    return BL_OK;
+}
+
 int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     bl_tuning t = c->tune;
     t.block = c->built_block, t.incidences_per_thread = c->built_per;
@@ -741,13 +783,18 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     int rc;
     if ((rc = c->contacts.begin_frame(c->stream))) return rc;
     // the bounds the first rebuild reads: as the last frame left them
-    std::vector<unsigned int> w0(nb * BF_N, 0);
+    if (c->w0_ev) HIPCHK(hipEventSynchronize(c->w0_ev)); // (the last upload from the staging buffer done)
+    else HIPCHK(hipEventCreateWithFlags(&c->w0_ev, hipEventDisableTiming));
+    if (!c->w0.ensure(sizeof(unsigned int) * nb * BF_N + 4)) return BL_ERR_HIP;
+    unsigned int* w0 = (unsigned int*)c->w0.p;
+    std::memset(w0, 0, sizeof(unsigned int) * nb * BF_N);
     for (size_t k = 0; k < nb; k++) {
         const HostBody& hb = c->bodies[k];
         const float mn[3] = {hb.mn.x, hb.mn.y, hb.mn.z}, mx[3] = {hb.mx.x, hb.mx.y, hb.mx.z};
         for (int q = 0; q < 3; q++) w0[k * BF_N + BF_MN + q] = fkey(mn[q]), w0[k * BF_N + BF_MX + q] = fkey(mx[q]);
     }
-    HIPCHK(hipMemcpyAsync(c->cbox.p, w0.data(), sizeof(unsigned int) * w0.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->cbox.p, w0, sizeof(unsigned int) * nb * BF_N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->w0_ev, c->stream));
     if ((rc = launch_frame_begin(a, c->stream, c->parity))) return rc;
     const bool graph = !kev && t.use_graph;
     if (kev) HIPCHK(hipEventRecord(kev[0], c->stream));
@@ -759,6 +806,10 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
         if ((rc = c->contacts.rebuild(a, c->cbox.p, c->stream, c->parity))) {
             g_last_error = "pair rebuild failed";
             return rc;
+        }
+        if (c->contacts.last.unsupported & 4) {
+            g_last_error = "a scan of the pair rebuild timed out waiting for its predecessors";
+            return BL_ERR_HIP;
         }
         if (c->contacts.last.unsupported) {
             g_last_error = (c->contacts.last.unsupported & 1)
@@ -806,8 +857,7 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     }
     c->substeps += substeps;
     if ((rc = launch_wstat_fold(a, c->stream, c->wstat.p)) || (rc = launch_frame_end(a, c->stream, c->parity))) return rc;
-    c->frame_pending = true;
-    return BL_OK;
+    return frame_enqueued(c);
 }
 
 // Whether this frame has an island that searches for pairs (World::build_islands on the bodies as the last frame
@@ -830,8 +880,7 @@ int enqueue_frame(bl_ctx* c, int substeps) {
     if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
     if ((rc = enqueue(c, substeps))) return rc;
     if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
-    c->frame_pending = true;
-    return BL_OK;
+    return frame_enqueued(c);
 }
 
 // The post-frame check of World::simulate_island (src/phys/world.cpp:3962-3979) on the frame just run: a body whose
@@ -841,17 +890,24 @@ int post_frame(bl_ctx* c) {
     if (!c->frame_pending) return BL_OK;
     c->frame_pending = false;
     HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t nb = c->bodies.size();
+    const bool queued = c->fin_queued;
+    c->fin_queued = false;
     int errw = 0;
-    HIPCHK(hipMemcpy(&errw, c->err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (queued) std::memcpy(&errw, c->fin.p, sizeof(int));
+    else HIPCHK(hipMemcpy(&errw, c->err.p, sizeof(int), hipMemcpyDeviceToHost));
     if (errw) {
         HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
     HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
         g_last_error = "a resident launch's grid barrier timed out: the state of that frame is not valid";
         return BL_ERR_HIP;
     }
-    const size_t nb = c->bodies.size();
     std::vector<unsigned int> w(nb * BF_N);
-    HIPCHK(hipMemcpy(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost));
+    if (queued) {
+        if (nb) std::memcpy(w.data(), (const char*)c->fin.p + 256, sizeof(unsigned int) * w.size());
+    } else {
+        HIPCHK(hipMemcpy(w.data(), c->body_frame.p, sizeof(unsigned int) * w.size(), hipMemcpyDeviceToHost));
+    }
     std::vector<size_t> trouble;
     const float lim = 400.0f * 0.9f;
     for (size_t k = 0; k < nb; k++) {
@@ -1057,6 +1113,8 @@ void bl_destroy(bl_ctx* c) {
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->w0_ev) (void)hipEventDestroy(c->w0_ev);
+    c->fin.release(), c->w0.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1336,7 +1394,7 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
     }
     c->substeps += substeps;
     if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
-    c->frame_pending = true;
+    if ((rc = frame_enqueued(c))) return rc;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(total_ms, c->ev0, c->ev1));

@@ -26,10 +26,18 @@ struct IslandPar {
 struct RStat {
     uint32_t npts, nnt_raw, nnc, nct, nent, nnt;
     uint32_t overflow;    // bit 0 NT, 1 NC, 2 CT, 3 entries, 4 partners
-    uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles
+    uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles, 2 a scan's wait timed out
     uint32_t last_pairs;  // candidate pairs kept in the last pair island
     uint32_t npart;       // partner entries (when they overflowed their buffer)
     uint32_t nbig;        // hash buckets of more than 16 points
+};
+
+// A one-pass launch's look-back: the workgroups' published words, the ticket counter, the tickets taken before the
+// launch and its epoch.
+struct Lookback {
+    uint64_t* words;
+    uint32_t* ticket;
+    uint32_t base, epoch;
 };
 
 // The device tables of a rebuild (all pair islands at once: one hash table, its keys salted by island).
@@ -57,7 +65,7 @@ struct CArgs {
     RStat* st;
     uint32_t* icount;         // per pair island: candidate pairs kept (NT after the filter, NC, CT)
     // hashed nodes
-    uint32_t *pflag, *pscan, *pbucket;
+    uint32_t* pbucket;
     int4* pcell;   // cell x y z, node
     float4* ppos;  // position, body (bits)
     uint32_t tmask;
@@ -66,11 +74,10 @@ struct CArgs {
     float4* spos;
     uint32_t* big; // buckets left to the wave sort
     // node-triangle, node-capsule candidates
-    uint32_t *tcnt, *tscan, *ccnt, *cscan;
     uint4 *ntraw, *nc;
     uint32_t cap_nt, cap_nc;
     // capsule-triangle: triangle cells, their table, the candidates
-    uint32_t *ecnt, *escan, *eb;
+    uint32_t* eb;
     uint4* ea;     // per entry: triangle, body, group, island
     float4* ebb;   // per entry: raw bounds (min x y z, max x)
     float2* ec;    // per entry: raw bounds (max y z)
@@ -79,12 +86,11 @@ struct CArgs {
     float2* sc;
     uint32_t cap_ent, emask;
     uint32_t *ecount, *estart, *efill;
-    uint32_t *ctcnt, *ctscan, *ctslot;
     uint4* ct;
     uint32_t cap_ct;
-    // the near-pair filter
-    uint32_t *nflag, *nscan;
+    // the near-pair list
     uint4* nt;
+    Lookback lk; // the current one-pass launch's
 };
 
 // The candidate lists of a frame (all pair islands) and the narrow phase's buffers.
@@ -176,7 +182,11 @@ private:
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;
     uint32_t tsize = 0; // node table buckets
-    Buf<uint8_t> temp;
+    // the one-pass launches' published words and workgroup tickets (tickets taken so far: lk_base_)
+    Buf<uint64_t> lk_words;
+    Buf<uint32_t> lk_ticket;
+    uint32_t lk_epoch_ = 0, lk_base_ = 0;
+    RStat* hstat = nullptr; // pinned host copy of the rebuild's status
     Buf<int> bisl, bmem;
     Buf<uint32_t> pcnt, poff, plist;
     uint32_t nmem = 0, cap_part = 0;
@@ -184,21 +194,22 @@ private:
     Buf<float> bbox, samples;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
-    Buf<uint32_t> pflag, pscan, pbucket, tcount, tstart, tfill, titems, sbody;
+    Buf<uint32_t> pbucket, tcount, tstart, tfill, titems, sbody;
     Buf<int4> pcell;
     Buf<float4> ppos;
-    Buf<uint32_t> tcnt, tscan, ccnt, cscan, ecnt, escan, eb, ecount, estart, efill, ctcnt, ctscan;
+    Buf<uint32_t> eb, ecount, estart, efill;
     Buf<int4> scell;
-    Buf<uint32_t> big, ctslot;
+    Buf<uint32_t> big;
     Buf<float4> spos, ebb, sb;
     Buf<uint4> ea, sa;
     Buf<float2> ec, sc;
-    Buf<uint32_t> nflag, nscan, bcnt, bpre;
+    Buf<uint32_t> bcnt, bpre;
     Buf<uint4> ntraw, nc, ct, nt;
     Buf<Hit> hraw;
     uint32_t cap_nt = 0, cap_nc = 0, cap_ct = 0, cap_ent = 0, esize = 0;
 
     int scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t st);
+    int lookback(Lookback& k, size_t nb, hipStream_t st);
     int ensure_lists();
     int enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st);
 };

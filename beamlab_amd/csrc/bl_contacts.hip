@@ -19,8 +19,6 @@
 #include "bl_islands.h"
 #include "bl_narrow.h"
 
-#include <rocprim/device/device_scan.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -119,6 +117,82 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------------------------- one-pass compaction
+// A count, its exclusive scan and the writes at the scanned offsets in one launch. Each workgroup takes a ticket (its
+// place in the output: workgroups are numbered in the order they start), counts its items, publishes its count,
+// and finds the count of all earlier workgroups by looking back over their published words; a word holding an
+// inclusive prefix ends the look-back. A word is (kind << 62) | (epoch << 32) | value: kind 1 a workgroup's own
+// count, 2 its inclusive prefix; the epoch tells this launch's words from an earlier launch's, so the words are never
+// cleared. A workgroup publishes its count before it waits, and waits only for workgroups with earlier tickets, which
+// have started: every wait ends. It is bounded all the same: a time-out sets bit 2 of the rebuild's unsupported word
+// (the host then fails the rebuild).
+__device__ __forceinline__ uint32_t wg_ticket(const Lookback& k, uint32_t* sh) {
+    if (threadIdx.x == 0) *sh = atomicAdd(k.ticket, 1u) - k.base;
+    __syncthreads();
+    return *sh;
+}
+
+// exclusive prefix over a workgroup of 256 threads (every thread calls it); agg: the workgroup's sum. sh: 4 words.
+__device__ __forceinline__ uint32_t wg_prefix(uint32_t v, uint32_t& agg, uint32_t* sh) {
+    uint32_t wt;
+    const uint32_t wp = wave_prefix(v, wt), wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wv] = wt;
+    __syncthreads();
+    uint32_t bp = 0;
+    agg = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) bp += w < wv ? sh[w] : 0u, agg += sh[w];
+    return bp + wp;
+}
+
+// publishes ticket t's count agg and returns the sum of the counts of the tickets before it (every thread calls
+// it; the first wave looks back, 64 predecessors at a time). sh: 1 word.
+__device__ uint32_t wg_lookback(const Lookback& k, uint32_t t, uint32_t agg, RStat* st, uint32_t* sh) {
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        constexpr uint64_t kEpoch = 0x3fffffff00000000ull;
+        const uint64_t tag = (uint64_t)(k.epoch & 0x3fffffffu) << 32;
+        uint32_t pre = 0;
+        if (t == 0) {
+            if (lane == 0) __hip_atomic_store(&k.words[0], (2ull << 62) | tag | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&k.words[t], (1ull << 62) | tag | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = (int)t - 1; // the nearest predecessor not yet summed
+            uint32_t spins = 0;
+            bool bad = false;
+            for (;;) {
+                const int idx = j - (int)lane; // lane 0 the nearest
+                uint64_t w = idx >= 0 ? 0ull : ((2ull << 62) | tag); // before the first workgroup: a zero prefix
+                for (;;) {
+                    if (idx >= 0 && (w & kEpoch) != tag)
+                        w = __hip_atomic_load(&k.words[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__all((w & kEpoch) == tag)) break;
+                    if (++spins > (1u << 22)) {
+                        bad = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (bad) break;
+                const uint64_t pm = __ballot((w >> 62) == 2);
+                const uint32_t stop = pm ? (uint32_t)__builtin_ctzll(pm) : 63u; // lanes 0..stop are summed
+                uint32_t x = lane <= stop ? (uint32_t)w : 0u;
+                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+                pre += x;
+                if (pm) break;
+                j -= 64;
+            }
+            if (bad && lane == 0) atomicOr(&st->unsupported, 4u);
+            if (lane == 0)
+                __hip_atomic_store(&k.words[t], (2ull << 62) | tag | (uint32_t)(pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) *sh = pre;
+    }
+    __syncthreads();
+    return *sh;
 }
 
 // ---------------------------------------------------------------------------------------------- rebuild stages
@@ -247,51 +321,68 @@ __global__ void k_rb_boxes(KArgs a, CArgs c) {
 
 // Each island body's partners (one wave per body, lanes over its island; count, then write in body order) and
 // whether it searches (a partner, or self collision).
-__global__ __launch_bounds__(256) void k_rb_partners(KArgs a, CArgs c, int write) {
-    const uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (m >= c.nmem) return;
-    const uint32_t g = c.imem[m];
-    const int isl = c.bisl[g];
-    const uint32_t k0 = c.imem_off[isl], k1 = c.imem_off[isl + 1];
+__global__ __launch_bounds__(256) void k_rb_partners(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t m = tk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const bool live = m < c.nmem;
+    uint32_t g = 0, k0 = 0, k1 = 0;
+    if (live) {
+        g = c.imem[m];
+        const int isl = c.bisl[g];
+        k0 = c.imem_off[isl], k1 = c.imem_off[isl + 1];
+    }
     uint32_t total = 0;
-    const uint32_t base = write ? c.poff[m] : 0;
+    for (uint32_t y0 = k0; y0 < k1; y0 += 64) {
+        const uint32_t k = y0 + lane;
+        total += (uint32_t)__popcll(__ballot(k < k1 && partners(a, c, g, c.imem[k])));
+    }
+    uint32_t agg;
+    const uint32_t wb = __shfl(wg_prefix(lane == 0 ? total : 0u, agg, sh + 1), 0);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5), base = pre + wb;
+    uint32_t done = 0;
     for (uint32_t y0 = k0; y0 < k1; y0 += 64) {
         const uint32_t k = y0 + lane;
         const bool ok = k < k1 && partners(a, c, g, c.imem[k]);
         uint32_t chunk;
         const uint32_t before = wave_prefix(ok ? 1u : 0u, chunk);
-        if (write && ok && base + total + before < c.cap_part) c.plist[base + total + before] = c.imem[k];
-        total += chunk;
+        if (ok && base + done + before < c.cap_part) c.plist[base + done + before] = c.imem[k];
+        done += chunk;
     }
-    if (lane != 0) return;
-    if (!write) {
-        c.pcnt[m] = total;
+    if (live && lane == 0) {
+        c.pcnt[m] = total, c.poff[m] = base;
         c.bquery[g] = (total > 0 || a.bodies[g].self_coll) ? 1u : 0u;
-    } else if (m == c.nmem - 1) {
-        const uint32_t tot = c.poff[m] + c.pcnt[m];
+    }
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
         if (tot > c.cap_part) atomicOr(&c.st->overflow, 16u), c.st->npart = tot;
     }
 }
 
-// 1) the contacter nodes of the searching bodies near a partner (flag; after the scan their records, in node order)
-__global__ void k_rb_pts(KArgs a, CArgs c, int write) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nnodes) return;
-    const uint32_t g = a.ninfo[i] >> 16;
-    const int isl = c.bisl[g];
-    bool ok = isl >= 0 && c.bquery[g] && (a.ninfo[i] & BL_NF_CONTACTER);
+// 1) the contacter nodes of the searching bodies near a partner: their records, in node order
+__global__ __launch_bounds__(256) void k_rb_pts(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t i = tk * 256 + threadIdx.x;
+    bool ok = false;
     f3 p = mk(0, 0, 0);
-    if (ok) {
-        p = ld3(a.pm[c.parity][i]);
-        ok = near_partner(a, c, g, p, p);
+    uint32_t g = 0;
+    int isl = -1;
+    if (i < a.nnodes) {
+        g = a.ninfo[i] >> 16;
+        isl = c.bisl[g];
+        ok = isl >= 0 && c.bquery[g] && (a.ninfo[i] & BL_NF_CONTACTER);
+        if (ok) {
+            p = ld3(a.pm[c.parity][i]);
+            ok = near_partner(a, c, g, p, p);
+        }
     }
-    if (!write) {
-        c.pflag[i] = ok;
-        return;
-    }
-    if (i == a.nnodes - 1) c.st->npts = c.pscan[i] + c.pflag[i];
+    uint32_t agg;
+    const uint32_t bp = wg_prefix(ok ? 1u : 0u, agg, sh + 1);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) c.st->npts = pre + agg;
     if (!ok) return;
-    const uint32_t k = c.pscan[i];
+    const uint32_t k = pre + bp;
     const float ic = c.ipar[isl].inv_cell;
     const int x = cell_of(p.x, ic), y = cell_of(p.y, ic), z = cell_of(p.z, ic);
     c.pcell[k] = make_int4(x, y, z, (int)i);
@@ -451,51 +542,70 @@ __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, cons
 }
 
 constexpr int kGnt = 16;
-__global__ __launch_bounds__(256) void k_rb_nt(KArgs a, CArgs c, int write) {
-    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
-    if (t >= a.ntris) return; // (whole groups)
+__global__ __launch_bounds__(256, 5) void k_rb_nt(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t t = (tk * 256 + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
     NtQuery q;
-    q.tr = a.tri_abc[t];
-    q.t = t;
-    q.isl = c.bisl[q.tr.w];
     CellRange cr;
     cr.n = 0;
-    if (q.isl >= 0 && c.bquery[q.tr.w] && !a.tri_torn[t] && c.st->npts > 0) {
-        const IslandPar ip = c.ipar[q.isl];
-        q.self_coll = a.bodies[q.tr.w].self_coll != 0;
-        q.margin = ip.margin;
-        const float rq0 = ip.rmax + ip.margin + 0.0f;
-        q.rq = rq0 + 0.0f;
-        q.pa = ld3(a.pm[c.parity][q.tr.x]), q.pb = ld3(a.pm[c.parity][q.tr.y]), q.pc = ld3(a.pm[c.parity][q.tr.z]);
-        q.tmn = vmin(q.pa, vmin(q.pb, q.pc)), q.tmx = vmax(q.pa, vmax(q.pb, q.pc));
-        const f3 r3 = mk(q.rq, q.rq, q.rq);
-        if (near_partner(a, c, q.tr.w, sub(q.tmn, r3), add(q.tmx, r3))) {
-            const f3 tn = cross(sub(q.pb, q.pa), sub(q.pc, q.pa));
-            const float tn2 = dot(tn, tn);
-            q.tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);
-            cr.set(sub(q.tmn, r3), add(q.tmx, r3), ip.inv_cell);
+    if (t < a.ntris) {
+        q.tr = a.tri_abc[t];
+        q.t = t;
+        q.isl = c.bisl[q.tr.w];
+        if (q.isl >= 0 && c.bquery[q.tr.w] && !a.tri_torn[t] && c.st->npts > 0) {
+            const IslandPar ip = c.ipar[q.isl];
+            q.self_coll = a.bodies[q.tr.w].self_coll != 0;
+            q.margin = ip.margin;
+            const float rq0 = ip.rmax + ip.margin + 0.0f;
+            q.rq = rq0 + 0.0f;
+            q.pa = ld3(a.pm[c.parity][q.tr.x]), q.pb = ld3(a.pm[c.parity][q.tr.y]), q.pc = ld3(a.pm[c.parity][q.tr.z]);
+            q.tmn = vmin(q.pa, vmin(q.pb, q.pc)), q.tmx = vmax(q.pa, vmax(q.pb, q.pc));
+            const f3 r3 = mk(q.rq, q.rq, q.rq);
+            if (near_partner(a, c, q.tr.w, sub(q.tmn, r3), add(q.tmx, r3))) {
+                const f3 tn = cross(sub(q.pb, q.pa), sub(q.pc, q.pa));
+                const float tn2 = dot(tn, tn);
+                q.tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);
+                cr.set(sub(q.tmn, r3), add(q.tmx, r3), ip.inv_cell);
+            }
         }
     }
-    const uint32_t base = write ? c.tscan[t] : 0;
-    uint32_t total = 0;
+    // count, then the group's place in the list, then write (a lane keeps the counts of its first two cells; those
+    // of further cells are counted again)
+    uint32_t total = 0, keep0 = 0, keep1 = 0;
+    for (uint32_t c0 = 0; c0 < cr.n; c0 += kGnt) {
+        const uint32_t ci = c0 + gl;
+        uint32_t cnt = 0;
+        if (ci < cr.n) {
+            int x, y, z;
+            cr.cell(ci, x, y, z);
+            cnt = nt_cell(a, c, q, x, y, z, nullptr, 0, 0);
+        }
+        if (c0 == 0) keep0 = cnt;
+        else if (c0 == kGnt) keep1 = cnt;
+        uint32_t chunk;
+        (void)group_prefix<kGnt>(cnt, chunk);
+        total += chunk;
+    }
+    uint32_t agg;
+    const uint32_t gb = __shfl(wg_prefix(gl == 0 ? total : 0u, agg, sh + 1), 0, kGnt);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5), base = pre + gb;
+    uint32_t done = 0;
     for (uint32_t c0 = 0; c0 < cr.n; c0 += kGnt) {
         const uint32_t ci = c0 + gl;
         int x = 0, y = 0, z = 0;
         uint32_t cnt = 0;
         if (ci < cr.n) {
             cr.cell(ci, x, y, z);
-            cnt = nt_cell(a, c, q, x, y, z, nullptr, 0, 0);
+            cnt = c0 == 0 ? keep0 : c0 == kGnt ? keep1 : nt_cell(a, c, q, x, y, z, nullptr, 0, 0);
         }
         uint32_t chunk;
         const uint32_t before = group_prefix<kGnt>(cnt, chunk);
-        if (write && cnt) nt_cell(a, c, q, x, y, z, c.ntraw, base + total + before, c.cap_nt);
-        total += chunk;
+        if (cnt) nt_cell(a, c, q, x, y, z, c.ntraw, base + done + before, c.cap_nt);
+        done += chunk;
     }
-    if (gl != 0) return;
-    if (!write) {
-        c.tcnt[t] = total;
-    } else if (t == a.ntris - 1) {
-        const uint32_t tot = c.tscan[t] + c.tcnt[t];
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
         c.st->nnt_raw = tot;
         if (tot > c.cap_nt) atomicOr(&c.st->overflow, 1u);
     }
@@ -526,8 +636,10 @@ __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint
 }
 
 constexpr int kGcap = 64;
-__global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c, int write) {
-    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+__global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t ci = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
     const bool live = ci < a.ncaps;
     uint4 cp = make_uint4(0, 0, 0, 0);
     int isl = -1;
@@ -547,27 +659,40 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c, int write) {
             if (near_partner(a, c, cp.z, mn, mx)) cr.set(mn, mx, ip.inv_cell);
         }
     }
-    const uint32_t base = write && live ? c.cscan[ci] : 0;
-    uint32_t total = 0;
+    uint32_t total = 0, keep0 = 0; // (a lane's first count is kept for the writes)
+    for (uint32_t c0 = 0; c0 < cr.n; c0 += kGcap) {
+        const uint32_t cc = c0 + gl;
+        uint32_t cnt = 0;
+        if (cc < cr.n) {
+            int x, y, z;
+            cr.cell(cc, x, y, z);
+            cnt = nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
+        }
+        if (c0 == 0) keep0 = cnt;
+        uint32_t chunk;
+        (void)group_prefix<kGcap>(cnt, chunk);
+        total += chunk;
+    }
+    uint32_t agg;
+    const uint32_t gb = __shfl(wg_prefix(gl == 0 ? total : 0u, agg, sh + 1), 0, kGcap);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5), base = pre + gb;
+    uint32_t done = 0;
     for (uint32_t c0 = 0; c0 < cr.n; c0 += kGcap) {
         const uint32_t cc = c0 + gl;
         int x = 0, y = 0, z = 0;
         uint32_t cnt = 0;
         if (cc < cr.n) {
             cr.cell(cc, x, y, z);
-            cnt = nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
+            cnt = c0 == 0 ? keep0 : nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
         }
         uint32_t chunk;
         const uint32_t before = group_prefix<kGcap>(cnt, chunk);
-        if (write && cnt) nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, c.nc, base + total + before, c.cap_nc, ci);
-        total += chunk;
+        if (cnt) nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, c.nc, base + done + before, c.cap_nc, ci);
+        done += chunk;
     }
-    if (write) add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
-    if (!live || gl != 0) return;
-    if (!write) {
-        c.ccnt[ci] = total;
-    } else if (ci == a.ncaps - 1) {
-        const uint32_t tot = c.cscan[ci] + c.ccnt[ci];
+    add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
         c.st->nnc = tot;
         if (tot > c.cap_nc) atomicOr(&c.st->overflow, 2u);
     }
@@ -575,38 +700,43 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c, int write) {
 
 // 3) capsules against triangles: the triangles of the searching bodies into every cell of their grown boxes, each
 // entry carrying the triangle's body, group, island and raw bounds
-__global__ void k_rb_prims(KArgs a, CArgs c, int write) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ntris) return;
-    const uint4 tr = a.tri_abc[t];
-    const int isl = c.bisl[tr.w];
+__global__ __launch_bounds__(256) void k_rb_prims(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t t = tk * 256 + threadIdx.x;
     // A prim is a triangle whose grown box spans at most 512 cells (the reference enters it into all of them). The
     // capsules keep the triangles whose raw box meets theirs, and such a triangle shares with the capsule a cell of
     // its raw box: entering those cells alone finds the same set.
     CellRange cr;
     cr.n = 0;
     f3 tmn = mk(0, 0, 0), tmx = mk(0, 0, 0);
-    if (isl >= 0 && c.bquery[tr.w] && !a.tri_torn[t]) {
-        const IslandPar ip = c.ipar[isl];
-        const float r = a.bodies[tr.w].collision_radius + ip.margin + 0.0f;
-        const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
-        tmn = vmin(pa, vmin(pb, pc)), tmx = vmax(pa, vmax(pb, pc));
-        const f3 mn = sub(tmn, mk(r, r, r)), mx = add(tmx, mk(r, r, r));
-        if (near_partner(a, c, tr.w, mn, mx)) {
-            cr.set(mn, mx, ip.inv_cell);
-            if (cr.n) cr.set(tmn, tmx, ip.inv_cell);
+    uint4 tr = make_uint4(0, 0, 0, 0);
+    int isl = -1;
+    if (t < a.ntris) {
+        tr = a.tri_abc[t];
+        isl = c.bisl[tr.w];
+        if (isl >= 0 && c.bquery[tr.w] && !a.tri_torn[t]) {
+            const IslandPar ip = c.ipar[isl];
+            const float r = a.bodies[tr.w].collision_radius + ip.margin + 0.0f;
+            const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
+            tmn = vmin(pa, vmin(pb, pc)), tmx = vmax(pa, vmax(pb, pc));
+            const f3 mn = sub(tmn, mk(r, r, r)), mx = add(tmx, mk(r, r, r));
+            if (near_partner(a, c, tr.w, mn, mx)) {
+                cr.set(mn, mx, ip.inv_cell);
+                if (cr.n) cr.set(tmn, tmx, ip.inv_cell);
+            }
         }
     }
-    if (!write) {
-        c.ecnt[t] = cr.n;
-        return;
-    }
-    if (t == a.ntris - 1) {
-        const uint32_t tot = c.escan[t] + c.ecnt[t];
+    uint32_t agg;
+    const uint32_t bp = wg_prefix(cr.n, agg, sh + 1);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
         c.st->nent = tot;
         if (tot > c.cap_ent) atomicOr(&c.st->overflow, 8u);
     }
-    const uint32_t e0 = c.escan[t];
+    if (cr.n == 0) return;
+    const uint32_t e0 = pre + bp;
     const uint4 ra = make_uint4(t, tr.w, (uint32_t)a.bodies[tr.w].group, (uint32_t)isl);
     const float4 rb = make_float4(tmn.x, tmn.y, tmn.z, tmx.x);
     const float2 rc = make_float2(tmx.y, tmx.z);
@@ -631,15 +761,19 @@ __global__ void k_rb_efill(CArgs c) {
 
 // each capsule of an island collects the triangles of its other bodies whose boxes meet its own, once each, in
 // (body, triangle) order: a group of kGcap lanes per capsule, a lane per cell, the finds gathered in LDS, sorted, made
-// unique into the capsule's slot; after the scan of the counts, k_rb_ct_out moves them into the list
+// unique, then written at the capsule's place in the list
 constexpr int kCtCap = 256;
 __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
     __shared__ uint32_t lst[256 / kGcap][kCtCap];
+    __shared__ uint32_t uni[256 / kGcap][kCtCap];
     __shared__ uint32_t lcnt[256 / kGcap];
-    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t ci = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
     const uint32_t gi = threadIdx.x / kGcap;
     const bool live = ci < a.ncaps;
     uint32_t* L = lst[gi];
+    uint32_t* U = uni[gi];
     uint4 cp = make_uint4(0, 0, 0, 0);
     int isl = -1;
     CellRange cr;
@@ -695,65 +829,44 @@ __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
             }
             wave_sync();
         }
-    // the distinct ones, in order, into the capsule's slot
+    // the distinct ones, in order
     uint32_t total = 0;
     for (uint32_t j0 = 0; j0 < n; j0 += kGcap) {
         const uint32_t j = j0 + gl;
         const bool keep = j < n && (j == 0 || L[j] != L[j - 1]);
         uint32_t chunk;
         const uint32_t before = group_prefix<kGcap>(keep ? 1u : 0u, chunk);
-        if (keep) c.ctslot[(size_t)ci * kCtCap + total + before] = L[j];
+        if (keep) U[total + before] = L[j];
         total += chunk;
     }
-    if (live && gl == 0) c.ctcnt[ci] = total;
-}
-
-__global__ __launch_bounds__(256) void k_rb_ct_out(KArgs a, CArgs c) {
-    const uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
-    const bool live = ci < a.ncaps;
-    uint32_t n = 0;
-    int isl = -1;
+    uint32_t agg;
+    const uint32_t gb = __shfl(wg_prefix(gl == 0 ? total : 0u, agg, sh + 1), 0, kGcap);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5), base = pre + gb;
     if (live) {
-        n = c.ctcnt[ci];
-        const uint4 cp = a.cap_data[ci];
-        isl = c.bisl[cp.z];
-        const uint32_t base = c.ctscan[ci];
-        for (uint32_t j = gl; j < n; j += kGcap) {
-            const uint32_t t = c.ctslot[(size_t)ci * kCtCap + j];
+        for (uint32_t j = gl; j < total; j += kGcap) {
+            const uint32_t t = U[j];
             if (base + j < c.cap_ct) c.ct[base + j] = make_uint4(cp.z, a.tri_abc[t].w, ci, t);
         }
-        if (gl == 0 && ci == a.ncaps - 1) {
-            const uint32_t tot = base + n;
-            c.st->nct = tot;
-            if (tot > c.cap_ct) atomicOr(&c.st->overflow, 4u);
-        }
     }
-    add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? n : 0u);
+    add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
+        c.st->nct = tot;
+        if (tot > c.cap_ct) atomicOr(&c.st->overflow, 4u);
+    }
 }
 
 // near_pairs (src/phys/world.cpp:2525-2596): the node-triangle candidates within reach of their relative travel
-// until the next rebuild and not receding across the plane (flag over the whole buffer; compaction by scan)
-__global__ void k_rb_near(KArgs a, CArgs c, int write) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// until the next rebuild and not receding across the plane, kept in order
+__global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t i = tk * 256 + threadIdx.x;
     const uint32_t n = c.st->nnt_raw < c.cap_nt ? c.st->nnt_raw : c.cap_nt;
-    if (write) {
-        // (every lane reaches the wave sum: no early exit)
-        uint32_t last = 0;
-        if (i < c.cap_nt) {
-            if (i == c.cap_nt - 1) c.st->nnt = c.nscan[i] + c.nflag[i];
-            if (i < n && c.nflag[i]) {
-                const uint4 pr = c.ntraw[i];
-                c.nt[c.nscan[i]] = pr;
-                last = c.bisl[pr.y] == (int)c.nisl - 1;
-            }
-        }
-        add_last(c, last);
-        return;
-    }
-    if (i >= c.cap_nt) return;
     bool keep = false;
+    uint4 pr = make_uint4(0, 0, 0, 0);
     if (i < n) {
-        const uint4 pr = c.ntraw[i];
+        pr = c.ntraw[i];
         keep = !a.tri_torn[pr.w];
         if (keep) {
             const uint4 tr = a.tri_abc[pr.w];
@@ -784,7 +897,12 @@ __global__ void k_rb_near(KArgs a, CArgs c, int write) {
             }
         }
     }
-    c.nflag[i] = keep;
+    uint32_t agg;
+    const uint32_t bp = wg_prefix(keep ? 1u : 0u, agg, sh + 1);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5);
+    if (keep) c.nt[pre + bp] = pr;
+    add_last(c, keep && c.bisl[pr.y] == (int)c.nisl - 1 ? 1u : 0u);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) c.st->nnt = pre + agg;
 }
 
 // ---------------------------------------------------------------------------------------------- narrow phase
@@ -989,6 +1107,29 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
     if (lane == 0 && contacts) atomicAdd(&a.counters[CNT_CONTACTS], (unsigned long long)contacts);
 }
 
+// ---------------------------------------------------------------------------------------------- exclusive scan
+// One launch, one pass (the look-back of the one-pass compaction), 2 048 values a workgroup.
+constexpr uint32_t kScanPer = 8, kScanTile = 256 * kScanPer;
+
+__global__ __launch_bounds__(256) void k_scan(const uint32_t* in, uint32_t* out, uint32_t n, Lookback lk, RStat* st) {
+    __shared__ uint32_t sh[8];
+    const uint32_t t = wg_ticket(lk, sh);
+    const size_t i0 = (size_t)t * kScanTile + (size_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer], run = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        const uint32_t x = i0 + k < n ? in[i0 + k] : 0u;
+        v[k] = run;
+        run += x;
+    }
+    uint32_t agg;
+    const uint32_t bp = wg_prefix(run, agg, sh + 1);
+    const uint32_t off = wg_lookback(lk, t, agg, st, sh + 5) + bp;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++)
+        if (i0 + k < n) out[i0 + k] = off + v[k];
+}
+
 // ---------------------------------------------------------------------------------------------- host side
 namespace {
 
@@ -1012,26 +1153,44 @@ inline uint32_t pow2_at_least(size_t n) {
 
 } // namespace
 
+// The look-back words and tickets of a one-pass launch of nb workgroups.
+int ContactEngine::lookback(Lookback& k, size_t nb, hipStream_t st) {
+    if (!lk_words.p || nb > lk_words.cap || lk_epoch_ >= 0x3fffffffu) {
+        // fresh words (and after the epoch wraps): cleared, so that no stale word carries the current epoch
+        RC(lk_words.ensure(nb));
+        RC(lk_ticket.ensure(1));
+        hipLaunchKernelGGL(k_rb_clear, grid_for(lk_words.cap * 2), dim3(256), 0, st, (uint32_t*)lk_words.p,
+                           (uint32_t)(lk_words.cap * 2));
+        hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, lk_ticket.p, 1u);
+        lk_epoch_ = 0, lk_base_ = 0;
+    }
+    k.words = lk_words.p, k.ticket = lk_ticket.p, k.base = lk_base_, k.epoch = ++lk_epoch_;
+    lk_base_ += (uint32_t)nb;
+    return BL_OK;
+}
+
 int ContactEngine::scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t st) {
     if (n == 0) return BL_OK;
-    size_t bytes = 0;
-    CHK(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), st));
-    RC(temp.ensure(bytes));
-    CHK(rocprim::exclusive_scan(temp.p, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), st));
-    return BL_OK;
+    const size_t nb = (n + kScanTile - 1) / kScanTile;
+    Lookback k;
+    RC(lookback(k, nb, st));
+    hipLaunchKernelGGL(k_scan, dim3((unsigned)nb), dim3(256), 0, st, in, out, (uint32_t)n, k, stat.p);
+    return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 
 void ContactEngine::release() {
     up_valid_ = false;
-    temp.release();
+    lk_words.release(), lk_ticket.release();
+    if (hstat) (void)hipHostFree(hstat);
+    hstat = nullptr;
+    lk_epoch_ = 0, lk_base_ = 0;
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
-    for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pflag, &pscan, &pbucket, &tcount,
-                             &tstart, &tfill, &titems, &sbody, &tcnt, &tscan, &ccnt, &cscan, &ecnt, &escan, &eb,
-                             &ecount, &estart, &efill, &ctcnt, &ctscan, &nflag, &nscan, &bcnt, &bpre})
+    for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
+                             &tfill, &titems, &sbody, &eb, &ecount, &estart, &efill, &bcnt, &bpre})
         b->release();
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
-    scell.release(), spos.release(), big.release(), ctslot.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
+    scell.release(), spos.release(), big.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
     for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
     hraw.release();
     nnt = nnc = nct = 0;
@@ -1042,8 +1201,6 @@ void ContactEngine::release() {
 int ContactEngine::ensure_lists() {
     RC(ntraw.ensure(cap_nt));
     RC(nt.ensure(cap_nt));
-    RC(nflag.ensure(cap_nt));
-    RC(nscan.ensure(cap_nt));
     RC(nc.ensure(cap_nc));
     RC(ct.ensure(cap_ct));
     RC(eb.ensure(cap_ent));
@@ -1087,8 +1244,6 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(icount.ensure(nbodies + 1));
     RC(ipar.ensure(nbodies + 1));
     RC(stat.ensure(1));
-    RC(pflag.ensure(N + 1));
-    RC(pscan.ensure(N + 1));
     RC(pbucket.ensure(N + 1));
     RC(pcell.ensure(N + 1));
     RC(ppos.ensure(N + 1));
@@ -1100,15 +1255,6 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(tcount.ensure((size_t)tsize + 1));
     RC(tstart.ensure((size_t)tsize + 1));
     RC(tfill.ensure((size_t)tsize + 1));
-    RC(tcnt.ensure(T + 1));
-    RC(tscan.ensure(T + 1));
-    RC(ecnt.ensure(T + 1));
-    RC(escan.ensure(T + 1));
-    RC(ccnt.ensure(C + 1));
-    RC(cscan.ensure(C + 1));
-    RC(ctcnt.ensure(C + 1));
-    RC(ctscan.ensure(C + 1));
-    RC(ctslot.ensure((size_t)C * kCtCap + 1));
     cap_nt = std::max<uint32_t>(4096, 2 * T);
     cap_nc = std::max<uint32_t>(4096, 4 * C);
     cap_ct = std::max<uint32_t>(4096, 4 * C);
@@ -1218,57 +1364,44 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     const uint32_t N = nnodes, T = ntris, C = ncaps, NB = nbodies;
     c.cap_nt = cap_nt, c.cap_nc = cap_nc, c.cap_ct = cap_ct, c.cap_ent = cap_ent, c.cap_part = cap_part;
     c.plist = plist.p;
-    c.ntraw = ntraw.p, c.nc = nc.p, c.ct = ct.p, c.nt = nt.p, c.nflag = nflag.p, c.nscan = nscan.p;
+    c.ntraw = ntraw.p, c.nc = nc.p, c.ct = ct.p, c.nt = nt.p;
     c.eb = eb.p, c.ea = ea.p, c.ebb = ebb.p, c.ec = ec.p, c.sa = sa.p, c.sb = sb.p, c.sc = sc.p;
     c.ecount = ecount.p, c.estart = estart.p, c.efill = efill.p;
     c.emask = esize - 1;
+    // the one-pass stages: a launch of nb workgroups each, with its own look-back epoch
+    auto stage = [&](auto kern, size_t nb) -> int {
+        if (nb == 0) return BL_OK;
+        RC(lookback(c.lk, nb, st));
+        hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, st, a, c);
+        return BL_OK;
+    };
     hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, (uint32_t*)stat.p, (uint32_t)(sizeof(RStat) / 4));
     if (T) hipLaunchKernelGGL(k_rb_torn, grid_for(T), dim3(256), 0, st, a, c);
     hipLaunchKernelGGL(k_rb_island, dim3(c.nisl), dim3(256), 0, st, a, c);
     hipLaunchKernelGGL(k_rb_boxes, grid_for(NB), dim3(256), 0, st, a, c);
-    hipLaunchKernelGGL(k_rb_partners, grid_for((size_t)nmem * 64), dim3(256), 0, st, a, c, 0);
-    RC(scan(pcnt.p, poff.p, nmem, st));
-    hipLaunchKernelGGL(k_rb_partners, grid_for((size_t)nmem * 64), dim3(256), 0, st, a, c, 1);
+    RC(stage(k_rb_partners, ((size_t)nmem + 3) / 4));
     // 1) the node hash
     hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tcount.p, tsize + 1);
     hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tfill.p, tsize + 1);
-    hipLaunchKernelGGL(k_rb_pts, grid_for(N), dim3(256), 0, st, a, c, 0);
-    RC(scan(pflag.p, pscan.p, N, st));
-    hipLaunchKernelGGL(k_rb_pts, grid_for(N), dim3(256), 0, st, a, c, 1);
+    RC(stage(k_rb_pts, ((size_t)N + 255) / 256));
     RC(scan(tcount.p, tstart.p, (size_t)tsize + 1, st));
     hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
     hipLaunchKernelGGL(k_rb_sortb, grid_for(tsize), dim3(256), 0, st, c, tsize);
     hipLaunchKernelGGL(k_rb_sortbig, dim3(64), dim3(256), 0, st, c);
     // 2) triangles, then capsules, against the node hash
-    if (T) {
-        hipLaunchKernelGGL(k_rb_nt, grid_for((size_t)T * kGnt), dim3(256), 0, st, a, c, 0);
-        RC(scan(tcnt.p, tscan.p, T, st));
-        hipLaunchKernelGGL(k_rb_nt, grid_for((size_t)T * kGnt), dim3(256), 0, st, a, c, 1);
-    }
-    if (C) {
-        hipLaunchKernelGGL(k_rb_nc, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c, 0);
-        RC(scan(ccnt.p, cscan.p, C, st));
-        hipLaunchKernelGGL(k_rb_nc, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c, 1);
-    }
+    if (T) RC(stage(k_rb_nt, ((size_t)T * kGnt + 255) / 256));
+    if (C) RC(stage(k_rb_nc, ((size_t)C * kGcap + 255) / 256));
     // 3) capsules against the triangles' cells
     if (C && T) {
         hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, ecount.p, esize + 1);
         hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, efill.p, esize + 1);
-        hipLaunchKernelGGL(k_rb_prims, grid_for(T), dim3(256), 0, st, a, c, 0);
-        RC(scan(ecnt.p, escan.p, T, st));
-        hipLaunchKernelGGL(k_rb_prims, grid_for(T), dim3(256), 0, st, a, c, 1);
+        RC(stage(k_rb_prims, ((size_t)T + 255) / 256));
         RC(scan(ecount.p, estart.p, (size_t)esize + 1, st));
         hipLaunchKernelGGL(k_rb_efill, grid_for(cap_ent), dim3(256), 0, st, c);
-        hipLaunchKernelGGL(k_rb_ct, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c);
-        RC(scan(ctcnt.p, ctscan.p, C, st));
-        hipLaunchKernelGGL(k_rb_ct_out, grid_for((size_t)C * kGcap), dim3(256), 0, st, a, c);
+        RC(stage(k_rb_ct, ((size_t)C * kGcap + 255) / 256));
     }
     // the near-pair filter on the node-triangle candidates
-    if (T) {
-        hipLaunchKernelGGL(k_rb_near, grid_for(cap_nt), dim3(256), 0, st, a, c, 0);
-        RC(scan(nflag.p, nscan.p, cap_nt, st));
-        hipLaunchKernelGGL(k_rb_near, grid_for(cap_nt), dim3(256), 0, st, a, c, 1);
-    }
+    if (T) RC(stage(k_rb_near, ((size_t)cap_nt + 255) / 256));
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 
@@ -1288,15 +1421,18 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
     c.sbody = sbody.p, c.nsamples = nsamples;
     c.st = stat.p, c.icount = icount.p;
-    c.pflag = pflag.p, c.pscan = pscan.p, c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
+    c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
     c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p, c.tfill = tfill.p, c.titems = titems.p;
     c.scell = scell.p, c.spos = spos.p, c.big = big.p;
-    c.tcnt = tcnt.p, c.tscan = tscan.p, c.ccnt = ccnt.p, c.cscan = cscan.p;
-    c.ecnt = ecnt.p, c.escan = escan.p, c.ctcnt = ctcnt.p, c.ctscan = ctscan.p, c.ctslot = ctslot.p;
     for (int attempt = 0;; attempt++) {
         RC(enqueue_rebuild(a, c, st));
-        CHK(hipMemcpyAsync(&last, stat.p, sizeof(RStat), hipMemcpyDeviceToHost, st));
+        if (!hstat && hipHostMalloc((void**)&hstat, sizeof(RStat), hipHostMallocDefault) != hipSuccess) {
+            hstat = nullptr;
+            return BL_ERR_HIP;
+        }
+        CHK(hipMemcpyAsync(hstat, stat.p, sizeof(RStat), hipMemcpyDeviceToHost, st));
         CHK(hipStreamSynchronize(st));
+        last = *hstat;
         if (!last.overflow) break;
         if (attempt > 4) return BL_ERR_HIP;
         if (last.overflow & 1) cap_nt = std::max<uint32_t>(cap_nt, last.nnt_raw + last.nnt_raw / 2 + 1);
