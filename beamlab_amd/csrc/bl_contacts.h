@@ -29,7 +29,6 @@ struct RStat {
     uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles, 2 a scan's wait timed out
     uint32_t last_pairs;  // candidate pairs kept in the last pair island
     uint32_t npart;       // partner entries (when they overflowed their buffer)
-    uint32_t nbig;        // hash buckets of more than 16 points
 };
 
 // A one-pass launch's look-back: the workgroups' published words, the ticket counter, the tickets taken before the
@@ -69,10 +68,9 @@ struct CArgs {
     int4* pcell;   // cell x y z, node
     float4* ppos;  // position, body (bits)
     uint32_t tmask;
-    uint32_t *tcount, *tstart, *tfill, *titems;
-    int4* scell;   // the hashed nodes' records in bucket order
+    uint32_t *tcount, *tstart, *tfill;
+    int4* scell;   // the hashed nodes' records by bucket
     float4* spos;
-    uint32_t* big; // buckets left to the wave sort
     // node-triangle, node-capsule candidates
     uint4 *ntraw, *nc;
     uint32_t cap_nt, cap_nc;
@@ -194,12 +192,11 @@ private:
     Buf<float> bbox, samples;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
-    Buf<uint32_t> pbucket, tcount, tstart, tfill, titems, sbody;
+    Buf<uint32_t> pbucket, tcount, tstart, tfill, sbody;
     Buf<int4> pcell;
     Buf<float4> ppos;
     Buf<uint32_t> eb, ecount, estart, efill;
     Buf<int4> scell;
-    Buf<uint32_t> big;
     Buf<float4> spos, ebb, sb;
     Buf<uint4> ea, sa;
     Buf<float2> ec, sc;

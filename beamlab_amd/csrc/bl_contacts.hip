@@ -9,9 +9,10 @@
 //
 // Order is part of the result: the response of a contact cancels the forces accumulated before it. Every list is
 // built in the reference's order - candidates by triangle (body, then index), then hash cell (z, y, x), then the
-// cell's nodes in insertion order (body, then node): each bucket is filled by atomics, then sorted by insertion
-// index; all compactions are order-preserving scans. A rebuild runs on the device end to end (island parameters,
-// partner boxes, hash, queries, filter) and the host waits once, for the list sizes. The response runs in one
+// cell's nodes in insertion order (body, then node): each bucket is filled by atomics, and a query puts each cell's
+// matches back into node order; all compactions keep their order (a scan, or a one-pass look-back). A rebuild runs
+// on the device end to end (island parameters, partner boxes, hash, queries, filter) and the host waits once, for
+// the list sizes. The response runs in one
 // workgroup: contacts that share no node are independent, so each contact is placed at the dependency level one past
 // the latest earlier contact sharing a node with it (a sort of (node, contact) keys in LDS) and each level runs in
 // parallel - the same arithmetic, in the same per-node order, as the serial loop.
@@ -392,77 +393,28 @@ __global__ __launch_bounds__(256) void k_rb_pts(KArgs a, CArgs c) {
     atomicAdd(&c.tcount[b], 1u);
 }
 
+// each point's record into a slot of its bucket (the slots in no particular order: a query puts each cell's
+// matches into node order itself)
 __global__ void k_rb_fill(KArgs a, CArgs c) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= c.st->npts) return;
     const uint32_t b = c.pbucket[k];
-    c.titems[c.tstart[b] + atomicAdd(&c.tfill[b], 1u)] = k;
+    const uint32_t at = c.tstart[b] + atomicAdd(&c.tfill[b], 1u);
+    c.scell[at] = c.pcell[k], c.spos[at] = c.ppos[k];
 }
 
-// each bucket in insertion order (its points by index; short buckets sorted in registers), and the points' records
-// copied into bucket order, so that a query reads them in one pass without indirection
-__global__ void k_rb_sortb(CArgs c, uint32_t nbk) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbk) return;
-    const uint32_t s = c.tstart[b], e = c.tstart[b + 1], n = e - s;
-    if (n == 0) return;
-    if (n <= 16) {
-        uint32_t v[16];
-#pragma unroll
-        for (uint32_t k = 0; k < 16; k++) v[k] = k < n ? c.titems[s + k] : 0xffffffffu;
-#pragma unroll
-        for (uint32_t i = 1; i < 16; i++)
-#pragma unroll
-            for (uint32_t j = i; j > 0; j--) {
-                const uint32_t lo = v[j - 1] < v[j] ? v[j - 1] : v[j], hi = v[j - 1] < v[j] ? v[j] : v[j - 1];
-                v[j - 1] = lo, v[j] = hi;
-            }
-#pragma unroll
-        for (uint32_t k = 0; k < 16; k++)
-            if (k < n) c.scell[s + k] = c.pcell[v[k]], c.spos[s + k] = c.ppos[v[k]];
-        return;
-    }
-    c.big[atomicAdd(&c.st->nbig, 1u)] = b; // (longer buckets: a wave each, below)
-}
-
-// the buckets of more than 16 points, one wave each: a bitonic sort in LDS (a serial one past its capacity)
-constexpr int kBig = 512;
-__global__ __launch_bounds__(256) void k_rb_sortbig(CArgs c) {
-    __shared__ uint32_t lst[4][kBig];
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t nw = gridDim.x * 4;
-    uint32_t* L = lst[wv];
-    for (uint32_t w = blockIdx.x * 4 + wv; w < c.st->nbig; w += nw) {
-        const uint32_t b = c.big[w];
-        const uint32_t s = c.tstart[b], e = c.tstart[b + 1], n = e - s;
-        if (n > (uint32_t)kBig) {
-            if (lane == 0) {
-                for (uint32_t k = s + 1; k < e; k++) {
-                    const uint32_t x = c.titems[k];
-                    uint32_t j = k;
-                    while (j > s && c.titems[j - 1] > x) c.titems[j] = c.titems[j - 1], j--;
-                    c.titems[j] = x;
-                }
-                for (uint32_t k = s; k < e; k++) c.scell[k] = c.pcell[c.titems[k]], c.spos[k] = c.ppos[c.titems[k]];
-            }
-            continue;
+// One cell's matches, written in the order its bucket holds them, put into node order (the order the points were
+// hashed in, which the lists keep): an insertion sort by one lane over the few written.
+__device__ __forceinline__ void sort_by_node(uint4* out, uint32_t at, uint32_t cnt, uint32_t cap) {
+    const uint32_t m = at >= cap ? 0u : (cnt < cap - at ? cnt : cap - at);
+    for (uint32_t i = 1; i < m; i++) {
+        const uint4 x = out[at + i];
+        uint32_t j = i;
+        while (j > 0 && out[at + j - 1].z > x.z) {
+            out[at + j] = out[at + j - 1];
+            j--;
         }
-        uint32_t np = 1;
-        while (np < n) np <<= 1;
-        for (uint32_t k = lane; k < np; k += 64) L[k] = k < n ? c.titems[s + k] : 0xffffffffu;
-        wave_sync();
-        for (uint32_t size = 2; size <= np; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t q = lane; q < np / 2; q += 64) {
-                    const uint32_t i = 2 * q - (q & (stride - 1)), j = i + stride;
-                    const bool up = (i & size) == 0;
-                    const uint32_t x = L[i], y = L[j];
-                    if ((x > y) == up) L[i] = y, L[j] = x;
-                }
-                wave_sync();
-            }
-        for (uint32_t k = lane; k < n; k += 64) c.scell[s + k] = c.pcell[L[k]], c.spos[s + k] = c.ppos[L[k]];
-        wave_sync();
+        out[at + j] = x;
     }
 }
 
@@ -538,6 +490,7 @@ __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, cons
         if (out && at + cnt < cap) out[at + cnt] = make_uint4(gbn, q.tr.w, node, q.t);
         cnt++;
     }
+    if (out) sort_by_node(out, at, cnt, cap);
     return cnt;
 }
 
@@ -632,6 +585,7 @@ __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint
         if (out && at + cnt < cap) out[at + cnt] = make_uint4(gbn, cp.z, (uint32_t)cell.w, ci);
         cnt++;
     }
+    if (out) sort_by_node(out, at, cnt, cap);
     return cnt;
 }
 
@@ -1187,10 +1141,10 @@ void ContactEngine::release() {
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
     for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
-                             &tfill, &titems, &sbody, &eb, &ecount, &estart, &efill, &bcnt, &bpre})
+                             &tfill, &sbody, &eb, &ecount, &estart, &efill, &bcnt, &bpre})
         b->release();
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
-    scell.release(), spos.release(), big.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
+    scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
     for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
     hraw.release();
     nnt = nnc = nct = 0;
@@ -1247,11 +1201,9 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(pbucket.ensure(N + 1));
     RC(pcell.ensure(N + 1));
     RC(ppos.ensure(N + 1));
-    RC(titems.ensure(N + 1));
     RC(scell.ensure(N + 1));
     RC(spos.ensure(N + 1));
     tsize = pow2_at_least(4 * (size_t)N);
-    RC(big.ensure((size_t)tsize + 1));
     RC(tcount.ensure((size_t)tsize + 1));
     RC(tstart.ensure((size_t)tsize + 1));
     RC(tfill.ensure((size_t)tsize + 1));
@@ -1386,8 +1338,6 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     RC(stage(k_rb_pts, ((size_t)N + 255) / 256));
     RC(scan(tcount.p, tstart.p, (size_t)tsize + 1, st));
     hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
-    hipLaunchKernelGGL(k_rb_sortb, grid_for(tsize), dim3(256), 0, st, c, tsize);
-    hipLaunchKernelGGL(k_rb_sortbig, dim3(64), dim3(256), 0, st, c);
     // 2) triangles, then capsules, against the node hash
     if (T) RC(stage(k_rb_nt, ((size_t)T * kGnt + 255) / 256));
     if (C) RC(stage(k_rb_nc, ((size_t)C * kGcap + 255) / 256));
@@ -1422,8 +1372,8 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.sbody = sbody.p, c.nsamples = nsamples;
     c.st = stat.p, c.icount = icount.p;
     c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
-    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p, c.tfill = tfill.p, c.titems = titems.p;
-    c.scell = scell.p, c.spos = spos.p, c.big = big.p;
+    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p, c.tfill = tfill.p;
+    c.scell = scell.p, c.spos = spos.p;
     for (int attempt = 0;; attempt++) {
         RC(enqueue_rebuild(a, c, st));
         if (!hstat && hipHostMalloc((void**)&hstat, sizeof(RStat), hipHostMallocDefault) != hipSuccess) {
