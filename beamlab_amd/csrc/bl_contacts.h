@@ -68,7 +68,7 @@ struct CArgs {
     int4* pcell;   // cell x y z, node
     float4* ppos;  // position, body (bits)
     uint32_t tmask;
-    uint32_t *tcount, *tstart, *tfill;
+    uint32_t *tcount, *tstart;
     int4* scell;   // the hashed nodes' records by bucket
     float4* spos;
     // node-triangle, node-capsule candidates
@@ -83,7 +83,7 @@ struct CArgs {
     float4* sb;
     float2* sc;
     uint32_t cap_ent, emask;
-    uint32_t *ecount, *estart, *efill;
+    uint32_t *ecount, *estart;
     uint4* ct;
     uint32_t cap_ct;
     // the near-pair list
@@ -179,6 +179,7 @@ private:
     std::vector<int> up_isl_, up_mi_;
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;
+    bool counts_dirty_ = true; // the bucket counts need clearing before the next rebuild
     uint32_t tsize = 0; // node table buckets
     // the one-pass launches' published words and workgroup tickets (tickets taken so far: lk_base_)
     Buf<uint64_t> lk_words;
@@ -192,10 +193,10 @@ private:
     Buf<float> bbox, samples;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
-    Buf<uint32_t> pbucket, tcount, tstart, tfill, sbody;
+    Buf<uint32_t> pbucket, tcount, tstart, sbody;
     Buf<int4> pcell;
     Buf<float4> ppos;
-    Buf<uint32_t> eb, ecount, estart, efill;
+    Buf<uint32_t> eb, ecount, estart;
     Buf<int4> scell;
     Buf<float4> spos, ebb, sb;
     Buf<uint4> ea, sa;

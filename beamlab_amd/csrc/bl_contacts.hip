@@ -12,10 +12,10 @@
 // cell's nodes in insertion order (body, then node): each bucket is filled by atomics, and a query puts each cell's
 // matches back into node order; all compactions keep their order (a scan, or a one-pass look-back). A rebuild runs
 // on the device end to end (island parameters, partner boxes, hash, queries, filter) and the host waits once, for
-// the list sizes. The response runs in one
-// workgroup: contacts that share no node are independent, so each contact is placed at the dependency level one past
-// the latest earlier contact sharing a node with it (a sort of (node, contact) keys in LDS) and each level runs in
-// parallel - the same arithmetic, in the same per-node order, as the serial loop.
+// the list sizes. The response runs in one workgroup: contacts that share no node are independent, so each contact
+// is placed at the dependency level one past the latest earlier contact sharing a node with it (a sort of
+// (node, contact) keys in LDS) and each level runs in parallel - the same arithmetic, in the same per-node order, as
+// the serial loop.
 #include "bl_contacts.h"
 #include "bl_islands.h"
 #include "bl_narrow.h"
@@ -393,13 +393,13 @@ __global__ __launch_bounds__(256) void k_rb_pts(KArgs a, CArgs c) {
     atomicAdd(&c.tcount[b], 1u);
 }
 
-// each point's record into a slot of its bucket (the slots in no particular order: a query puts each cell's
-// matches into node order itself)
+// each point's record into a slot of its bucket, counting the bucket down to zero (so the counts are clear for the
+// next rebuild; the slots are taken in no particular order: a query puts each cell's matches into node order itself)
 __global__ void k_rb_fill(KArgs a, CArgs c) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= c.st->npts) return;
     const uint32_t b = c.pbucket[k];
-    const uint32_t at = c.tstart[b] + atomicAdd(&c.tfill[b], 1u);
+    const uint32_t at = c.tstart[b] + atomicSub(&c.tcount[b], 1u) - 1u;
     c.scell[at] = c.pcell[k], c.spos[at] = c.ppos[k];
 }
 
@@ -709,7 +709,7 @@ __global__ void k_rb_efill(CArgs c) {
     const uint32_t n = c.st->nent < c.cap_ent ? c.st->nent : c.cap_ent;
     if (e >= n) return;
     const uint32_t b = c.eb[e];
-    const uint32_t at = c.estart[b] + atomicAdd(&c.efill[b], 1u);
+    const uint32_t at = c.estart[b] + atomicSub(&c.ecount[b], 1u) - 1u; // (down to zero, as the node hash)
     c.sa[at] = c.ea[e], c.sb[at] = c.ebb[e], c.sc[at] = c.ec[e];
 }
 
@@ -1141,7 +1141,7 @@ void ContactEngine::release() {
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
     for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
-                             &tfill, &sbody, &eb, &ecount, &estart, &efill, &bcnt, &bpre})
+                             &sbody, &eb, &ecount, &estart, &bcnt, &bpre})
         b->release();
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
     scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
@@ -1167,7 +1167,7 @@ int ContactEngine::ensure_lists() {
     esize = pow2_at_least(2 * (size_t)cap_ent);
     RC(ecount.ensure((size_t)esize + 1));
     RC(estart.ensure((size_t)esize + 1));
-    RC(efill.ensure((size_t)esize + 1));
+    counts_dirty_ = true;
     return BL_OK;
 }
 
@@ -1205,8 +1205,8 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(spos.ensure(N + 1));
     tsize = pow2_at_least(4 * (size_t)N);
     RC(tcount.ensure((size_t)tsize + 1));
+    counts_dirty_ = true;
     RC(tstart.ensure((size_t)tsize + 1));
-    RC(tfill.ensure((size_t)tsize + 1));
     cap_nt = std::max<uint32_t>(4096, 2 * T);
     cap_nc = std::max<uint32_t>(4096, 4 * C);
     cap_ct = std::max<uint32_t>(4096, 4 * C);
@@ -1318,7 +1318,7 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     c.plist = plist.p;
     c.ntraw = ntraw.p, c.nc = nc.p, c.ct = ct.p, c.nt = nt.p;
     c.eb = eb.p, c.ea = ea.p, c.ebb = ebb.p, c.ec = ec.p, c.sa = sa.p, c.sb = sb.p, c.sc = sc.p;
-    c.ecount = ecount.p, c.estart = estart.p, c.efill = efill.p;
+    c.ecount = ecount.p, c.estart = estart.p;
     c.emask = esize - 1;
     // the one-pass stages: a launch of nb workgroups each, with its own look-back epoch
     auto stage = [&](auto kern, size_t nb) -> int {
@@ -1328,13 +1328,18 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
         return BL_OK;
     };
     hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, (uint32_t*)stat.p, (uint32_t)(sizeof(RStat) / 4));
+    // the bucket counts are counted down to zero by every rebuild that runs to its end; cleared after a fresh
+    // allocation or a rebuild that did not
+    if (counts_dirty_) {
+        hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tcount.p, tsize + 1);
+        if (esize) hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, ecount.p, esize + 1);
+    }
+    counts_dirty_ = true;
     if (T) hipLaunchKernelGGL(k_rb_torn, grid_for(T), dim3(256), 0, st, a, c);
     hipLaunchKernelGGL(k_rb_island, dim3(c.nisl), dim3(256), 0, st, a, c);
     hipLaunchKernelGGL(k_rb_boxes, grid_for(NB), dim3(256), 0, st, a, c);
     RC(stage(k_rb_partners, ((size_t)nmem + 3) / 4));
     // 1) the node hash
-    hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tcount.p, tsize + 1);
-    hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)tsize + 1), dim3(256), 0, st, tfill.p, tsize + 1);
     RC(stage(k_rb_pts, ((size_t)N + 255) / 256));
     RC(scan(tcount.p, tstart.p, (size_t)tsize + 1, st));
     hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
@@ -1343,8 +1348,6 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     if (C) RC(stage(k_rb_nc, ((size_t)C * kGcap + 255) / 256));
     // 3) capsules against the triangles' cells
     if (C && T) {
-        hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, ecount.p, esize + 1);
-        hipLaunchKernelGGL(k_rb_clear, grid_for((size_t)esize + 1), dim3(256), 0, st, efill.p, esize + 1);
         RC(stage(k_rb_prims, ((size_t)T + 255) / 256));
         RC(scan(ecount.p, estart.p, (size_t)esize + 1, st));
         hipLaunchKernelGGL(k_rb_efill, grid_for(cap_ent), dim3(256), 0, st, c);
@@ -1372,7 +1375,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.sbody = sbody.p, c.nsamples = nsamples;
     c.st = stat.p, c.icount = icount.p;
     c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
-    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p, c.tfill = tfill.p;
+    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p;
     c.scell = scell.p, c.spos = spos.p;
     for (int attempt = 0;; attempt++) {
         RC(enqueue_rebuild(a, c, st));
@@ -1382,6 +1385,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
         }
         CHK(hipMemcpyAsync(hstat, stat.p, sizeof(RStat), hipMemcpyDeviceToHost, st));
         CHK(hipStreamSynchronize(st));
+        counts_dirty_ = false;
         last = *hstat;
         if (!last.overflow) break;
         if (attempt > 4) return BL_ERR_HIP;
