@@ -1002,34 +1002,56 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
             hit = p.hraw[lo * 256 + (h - p.bpre[lo])];
             contact_nodes(a, p, hit, cn);
         }
-        for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
-        __syncthreads();
+        if (5 * m <= (uint32_t)kRespW) {
+            // a small window: each (hit, node) finds its predecessor - the latest earlier hit on the same node - by
+            // reading back through the earlier hits' nodes
+            uint32_t* hn = (uint32_t*)keys;
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
-            if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
-            preds[tid * 5 + j] = -1;
-        }
-        __syncthreads();
-        // bitonic sort of the window's keys
-        for (uint32_t size = 2; size <= nk; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t q = tid; q < nk / 2; q += kRespW) {
-                    const uint32_t i = 2 * q - (q & (stride - 1));
-                    const uint32_t j = i + stride;
-                    const bool up = (i & size) == 0;
-                    const uint64_t x = keys[i], y = keys[j];
-                    if ((x > y) == up) keys[i] = y, keys[j] = x;
-                }
-                __syncthreads();
+            for (int j = 0; j < 5; j++) {
+                hn[tid * 5 + j] = j < cn.na + cn.nb ? cn.n[j] : ~0u;
+                preds[tid * 5 + j] = -1;
             }
-        // predecessors: the previous key of the same node from another (earlier) hit
-        for (uint32_t k = tid; k < nk; k += kRespW) {
-            const uint64_t key = keys[k];
-            if (key == ~0ull) continue;
-            const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
-            int q = (int)k - 1;
-            while (q >= 0 && (uint32_t)(keys[q] >> 32) == node && ((uint32_t)(keys[q] >> 3) & (kRespW - 1)) == me) q--;
-            if (q >= 0 && (uint32_t)(keys[q] >> 32) == node) preds[me * 5 + (key & 7)] = (int16_t)((keys[q] >> 3) & (kRespW - 1));
+            __syncthreads();
+            if (tid < 5 * m) {
+                const uint32_t node = hn[tid], me = tid / 5;
+                int pq = -1;
+                if (node != ~0u)
+                    for (int q = (int)me - 1; q >= 0 && pq < 0; q--)
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            if (hn[q * 5 + j] == node) pq = q;
+                preds[tid] = (int16_t)pq;
+            }
+        } else {
+            for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
+                preds[tid * 5 + j] = -1;
+            }
+            __syncthreads();
+            // bitonic sort of the window's keys
+            for (uint32_t size = 2; size <= nk; size <<= 1)
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (uint32_t q = tid; q < nk / 2; q += kRespW) {
+                        const uint32_t i = 2 * q - (q & (stride - 1));
+                        const uint32_t j = i + stride;
+                        const bool up = (i & size) == 0;
+                        const uint64_t x = keys[i], y = keys[j];
+                        if ((x > y) == up) keys[i] = y, keys[j] = x;
+                    }
+                    __syncthreads();
+                }
+            // predecessors: the previous key of the same node from another (earlier) hit
+            for (uint32_t k = tid; k < nk; k += kRespW) {
+                const uint64_t key = keys[k];
+                if (key == ~0ull) continue;
+                const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
+                int q = (int)k - 1;
+                while (q >= 0 && (uint32_t)(keys[q] >> 32) == node && ((uint32_t)(keys[q] >> 3) & (kRespW - 1)) == me) q--;
+                if (q >= 0 && (uint32_t)(keys[q] >> 32) == node) preds[me * 5 + (key & 7)] = (int16_t)((keys[q] >> 3) & (kRespW - 1));
+            }
         }
         level[tid] = 0;
         __syncthreads();
