@@ -104,7 +104,7 @@ struct bl_ctx {
     // device
     DevArr<float4> pm[2], vm[2], iA, iM, bcold;
     DevArr<uint32_t> ninfo, noff, ch_n0, body_c0, chunk_body;
-    DevArr<uint4> ch_rng, fch_rng;
+    DevArr<uint4> ch_rng, fch_rng, wg_rng, fwg_rng;
     DevArr<uint32_t> fwg;
     DevArr<uint4> cstat;
     DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags, wg_chunk;
@@ -160,7 +160,7 @@ struct bl_ctx {
     void release() {
         drop_graphs();
         for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
-        iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release(), ch_rng.release(), fch_rng.release(), fwg.release();
+        iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release(), ch_rng.release(), fch_rng.release(), fwg.release(), wg_rng.release(), fwg_rng.release();
         body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
@@ -462,6 +462,10 @@ int build(bl_ctx* c) {
             for (uint32_t x = 0; x < 8; x++)
                 if (j < len[x]) fwg.push_back(st[x] + j);
     }
+    // each workgroup's chunk range itself (one load at a workgroup's start instead of two dependent ones)
+    std::vector<uint4> wg_rng(nchunks), fwg_rng(nf);
+    for (uint32_t b = 0; b < nchunks; b++) wg_rng[b] = ch_rng[wg_chunk[b]];
+    for (uint32_t b = 0; b < nf; b++) fwg_rng[b] = fch_rng[fwg[b]];
     // halos: per chunk the outside nodes its incidences read (sorted), the chunks owning them, each incidence's
     // two LDS slots
     std::vector<uint32_t> node_chunk(N);
@@ -499,7 +503,7 @@ int build(bl_ctx* c) {
     if ((rc = c->pm[0].upload(P)) || (rc = c->vm[0].upload(V)) || (rc = c->pm[1].alloc(N)) || (rc = c->vm[1].alloc(N)) ||
         (rc = c->ninfo.upload(ninfo)) || (rc = c->naux.upload(naux)) || (rc = c->noff.upload(noff)) ||
         (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iAB.upload(iAB)) || (rc = c->bcold.upload(bcold)) ||
-        (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->ch_rng.upload(ch_rng)) || (rc = c->fch_rng.upload(fch_rng)) || (rc = c->fwg.upload(fwg)) || (rc = c->body_c0.upload(body_c0)) ||
+        (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->ch_rng.upload(ch_rng)) || (rc = c->fch_rng.upload(fch_rng)) || (rc = c->fwg.upload(fwg)) || (rc = c->wg_rng.upload(wg_rng)) || (rc = c->fwg_rng.upload(fwg_rng)) || (rc = c->body_c0.upload(body_c0)) ||
         (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
         (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)) || (rc = c->bar.alloc(1)) ||
@@ -537,6 +541,7 @@ int build(bl_ctx* c) {
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
     a.wg_chunk = c->wg_chunk.p;
     a.fch_rng = c->fch_rng.p, a.fwg = c->fwg.p, a.nfchunks = nf;
+    a.wg_rng = c->wg_rng.p, a.fwg_rng = c->fwg_rng.p;
     c->parity = 0;
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;

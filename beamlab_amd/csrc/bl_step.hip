@@ -334,10 +334,9 @@ __global__ __launch_bounds__(T) void k_substep(KArgs a, int parity) {
     __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
     __shared__ uint8_t sok[T * PER];
     __shared__ uint4 sred[T / 64];
-    const uint32_t c = xcd_chunk(a, blockIdx.x);
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    chunk_substep<T, PER>(a.ch_rng[c], a, parity, sfx, sfy, sfz, sok, cnt);
-    flush_chunk<T>(a, cnt, c, sred);
+    chunk_substep<T, PER>(a.wg_rng[blockIdx.x], a, parity, sfx, sfy, sfz, sok, cnt);
+    flush_chunk<T>(a, cnt, xcd_chunk(a, blockIdx.x), sred);
 }
 
 // Many substeps per launch, one workgroup per body: the body's chunks one after another, a workgroup barrier
@@ -879,7 +878,7 @@ __global__ __launch_bounds__(T) void k_forces(KArgs a, int parity) {
     __shared__ float sfx[T * PER], sfy[T * PER], sfz[T * PER];
     __shared__ uint8_t sok[T * PER];
     NodeCounts cnt = {0, 0, 0, 0.0f};
-    chunk_substep<T, PER, true>(a.fch_rng[a.fwg[blockIdx.x]], a, parity, sfx, sfy, sfz, sok, cnt);
+    chunk_substep<T, PER, true>(a.fwg_rng[blockIdx.x], a, parity, sfx, sfy, sfz, sok, cnt);
 }
 
 // The static contacts and the integration from F, one thread per node (a workgroup's nodes may belong to several
@@ -928,7 +927,7 @@ __global__ __launch_bounds__(256) void k_forces_narrow(KArgs a, int parity, PArg
     __shared__ uint8_t sok[256 * PER];
     if (blockIdx.x < a.nfchunks) {
         NodeCounts cnt = {0, 0, 0, 0.0f};
-        chunk_substep<256, PER, true>(a.fch_rng[a.fwg[blockIdx.x]], a, parity, sfx, sfy, sfz, sok, cnt);
+        chunk_substep<256, PER, true>(a.fwg_rng[blockIdx.x], a, parity, sfx, sfy, sfz, sok, cnt);
     } else {
         narrow_block(a, p, parity, blockIdx.x - a.nfchunks);
     }

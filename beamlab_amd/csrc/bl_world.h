@@ -87,6 +87,9 @@ struct KArgs {
     const uint4* fch_rng;     // (first node, end node, first incidence, end incidence)
     const uint32_t* fwg;      // workgroup -> force chunk (contiguous ranges over the XCDs)
     uint32_t nfchunks;
+    // the ranges in workgroup order: wg_rng[b] = ch_rng[wg_chunk[b]], fwg_rng[b] = fch_rng[fwg[b]]
+    const uint4* wg_rng;
+    const uint4* fwg_rng;
 };
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
