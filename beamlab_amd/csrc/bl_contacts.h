@@ -2,6 +2,7 @@
 // host engine that drives them once per pair rebuild and once per substep.
 #pragma once
 
+#include "bl_islands.h"
 #include "bl_world.h"
 
 #include <vector>
@@ -176,6 +177,7 @@ private:
     std::vector<int> parent_, root_isl_, body_isl_, pair_id_, isl_first_, isl_;
     std::vector<uint32_t> isl_size_, cursor_;
     std::vector<float> bx_;
+    PairScratch pair_ws_;
     std::vector<int> up_isl_, up_mi_;
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;

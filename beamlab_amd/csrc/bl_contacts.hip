@@ -1270,7 +1270,7 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
         const int x = find((int)i), y = find((int)j);
         if (x != y) parent[x] = y;
     };
-    if (hv.inter_body) overlapping_pairs(bx.data(), nb, [&](uint32_t i, uint32_t j) { consider(i, j); });
+    if (hv.inter_body) overlapping_pairs(bx.data(), nb, pair_ws_, [&](uint32_t i, uint32_t j) { consider(i, j); });
     // islands numbered by their first body; the pair islands' members in body order
     root_isl_.assign(nb, -1);
     body_isl_.resize(nb);

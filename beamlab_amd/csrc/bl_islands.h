@@ -20,12 +20,22 @@ inline bool boxes_overlap(const float* p, const float* q) {
     return p[0] <= q[3] && p[3] >= q[0] && p[1] <= q[4] && p[4] >= q[1] && p[2] <= q[5] && p[5] >= q[2];
 }
 
-template <class Emit>
-void overlapping_pairs(const float* bx, size_t n, Emit emit) {
-    constexpr size_t kMaxCellsPerBox = 4096;
-    std::vector<uint8_t> odd(n, 0);
+// The grid's working arrays, kept by the caller from one call to the next (fresh arrays of this size cost page
+// faults on every call).
+struct PairScratch {
+    std::vector<uint8_t> odd;
     std::vector<float> ext;
-    ext.reserve(n);
+    std::vector<int64_t> lo, cnt, fill;
+    std::vector<uint32_t> items;
+};
+
+template <class Emit>
+void overlapping_pairs(const float* bx, size_t n, PairScratch& ws, Emit emit) {
+    constexpr size_t kMaxCellsPerBox = 4096;
+    std::vector<uint8_t>& odd = ws.odd;
+    odd.assign(n, 0);
+    std::vector<float>& ext = ws.ext;
+    ext.clear();
     float lo3[3] = {INFINITY, INFINITY, INFINITY}, hi3[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (size_t i = 0; i < n; i++) {
         const float* b = bx + 6 * i;
@@ -54,7 +64,9 @@ void overlapping_pairs(const float* bx, size_t n, Emit emit) {
         };
         const int64_t ncell = nc[0] * nc[1] * nc[2];
         // counting sort of (cell, box) entries
-        std::vector<int64_t> lo(6 * n), cnt((size_t)ncell + 1, 0);
+        std::vector<int64_t>&lo = ws.lo, &cnt = ws.cnt;
+        lo.resize(6 * n);
+        cnt.assign((size_t)ncell + 1, 0);
         for (size_t i = 0; i < n; i++) {
             if (odd[i]) continue;
             const float* b = bx + 6 * i;
@@ -70,8 +82,10 @@ void overlapping_pairs(const float* bx, size_t n, Emit emit) {
                     for (int64_t w = r[4]; w <= r[5]; w++) cnt[(size_t)((u * nc[1] + v) * nc[2] + w) + 1]++;
         }
         for (size_t k = 1; k < cnt.size(); k++) cnt[k] += cnt[k - 1];
-        std::vector<uint32_t> items((size_t)cnt.back());
-        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        std::vector<uint32_t>& items = ws.items;
+        items.resize((size_t)cnt.back());
+        std::vector<int64_t>& fill = ws.fill;
+        fill.assign(cnt.begin(), cnt.end() - 1);
         for (size_t i = 0; i < n; i++) {
             if (odd[i]) continue;
             const int64_t* r = &lo[6 * i];

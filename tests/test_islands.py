@@ -20,7 +20,10 @@ int main() {
     if (std::fscanf(stdin, "%zu", &n) != 1) return 2;
     std::vector<float> bx(6 * n);
     for (auto& v : bx) if (std::fscanf(stdin, "%f", &v) != 1) return 2;
-    blg::overlapping_pairs(bx.data(), n, [](uint32_t i, uint32_t j) { std::printf("%u %u\n", i, j); });
+    // (a first call on half the boxes leaves its arrays behind: the second, on all of them, reuses them)
+    blg::PairScratch ws;
+    blg::overlapping_pairs(bx.data(), n / 2, ws, [](uint32_t, uint32_t) {});
+    blg::overlapping_pairs(bx.data(), n, ws, [](uint32_t i, uint32_t j) { std::printf("%u %u\n", i, j); });
     return 0;
 }
 """
