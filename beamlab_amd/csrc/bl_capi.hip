@@ -278,6 +278,7 @@ int build(bl_ctx* c) {
         BodyView v{};
         v.group = hb.opts.collision_group, v.self = hb.opts.self_collision != 0;
         v.ntri = (uint32_t)hb.tris.size(), v.ncap = (uint32_t)hb.caps.size(), v.tri0 = hb.tri0;
+        v.nnode = (uint32_t)hb.nodes.size();
         v.collision_radius = hb.opts.collision_radius;
         for (const bl_node_info& inf : hb.info) v.contacters |= (inf.flags & BL_NF_CONTACTER) ? 1u : 0u;
         c->hv.bodies.push_back(v);

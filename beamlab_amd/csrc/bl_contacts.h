@@ -58,9 +58,10 @@ struct CArgs {
     uint32_t* bquery;         // per body: has a partner or collides with itself
     const uint32_t* body_box; // per body: bounds keys (BF_MN, BF_MX words of a frame record)
     const uint32_t* body_tri0;
+    const uint32_t* itri;     // the triangles of the pair islands' bodies, in body order
+    uint32_t n_itri;
     const uint32_t* sample_base;
     float* samples;           // every 7th triangle's longest edge, in body order
-    const uint32_t* sbody;    // per sample: its body
     uint32_t nsamples;
     RStat* st;
     uint32_t* icount;         // per pair island: candidate pairs kept (NT after the filter, NC, CT)
@@ -107,7 +108,7 @@ struct PArgs {
 // What the island building reads of a body on the host.
 struct BodyView {
     int32_t group;
-    uint32_t self, contacters, ntri, ncap, tri0;
+    uint32_t self, contacters, ntri, ncap, tri0, nnode;
     float collision_radius;
 };
 
@@ -178,11 +179,16 @@ private:
     std::vector<uint32_t> isl_size_, cursor_;
     std::vector<float> bx_;
     PairScratch pair_ws_;
+    std::vector<uint32_t> h_tri0_, itri_h_; // per body: first triangle; the pair islands' triangles
+    Buf<uint32_t> itri;
+    uint32_t n_itri = 0;
     std::vector<int> up_isl_, up_mi_;
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;
     bool counts_dirty_ = true; // the bucket counts need clearing before the next rebuild
-    uint32_t tsize = 0; // node table buckets
+    uint32_t tsize = 0;   // node table buckets allocated
+    uint32_t tsize_f = 0; // this frame's buckets: about four per node of the pair islands' bodies
+    std::vector<uint32_t> h_nnode_;
     // the one-pass launches' published words and workgroup tickets (tickets taken so far: lk_base_)
     Buf<uint64_t> lk_words;
     Buf<uint32_t> lk_ticket;
@@ -195,7 +201,7 @@ private:
     Buf<float> bbox, samples;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
-    Buf<uint32_t> pbucket, tcount, tstart, sbody;
+    Buf<uint32_t> pbucket, tcount, tstart;
     Buf<int4> pcell;
     Buf<float4> ppos;
     Buf<uint32_t> eb, ecount, estart;

@@ -257,18 +257,44 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
     rmax = block_reduce(rmax, fmx, shf);
     slow = block_reduce(slow, fmx, shf);
     fast = block_reduce(fast, imx, shi);
-    // the samples (the island's, in the global order: body, then triangle): count, smallest unit in the last place,
-    // largest value, anything not finite
+    // The island's samples: its members' ranges of the samples (stored by body, then triangle). A member with few
+    // samples is swept by one thread; the members with many (up to 64 of them) by all threads, one after another.
+    __shared__ uint32_t bigm[64];
+    __shared__ uint32_t nbig;
+    if (tid == 0) nbig = 0;
+    __syncthreads();
+    for (uint32_t k = k0 + tid; k < k1; k += 256) {
+        const uint32_t g = c.imem[k];
+        if (c.sample_base[g + 1] - c.sample_base[g] > 64) {
+            const uint32_t q = atomicAdd(&nbig, 1u);
+            if (q < 64) bigm[q] = g;
+        }
+    }
+    __syncthreads();
+    const uint32_t nbm = nbig < 64u ? nbig : 64u;
+    auto sweep = [&](auto&& visit) {
+        for (uint32_t k = k0 + tid; k < k1; k += 256) {
+            const uint32_t g = c.imem[k], s0 = c.sample_base[g], s1 = c.sample_base[g + 1];
+            bool listed = false;
+            if (s1 - s0 > 64)
+                for (uint32_t q = 0; q < nbm; q++) listed = listed || bigm[q] == g;
+            if (!listed)
+                for (uint32_t j = s0; j < s1; j++) visit(c.samples[j]);
+        }
+        for (uint32_t q = 0; q < nbm; q++) {
+            const uint32_t g = bigm[q], s0 = c.sample_base[g], s1 = c.sample_base[g + 1];
+            for (uint32_t j = s0 + tid; j < s1; j += 256) visit(c.samples[j]);
+        }
+    };
+    // count, smallest unit in the last place, largest value, anything not finite
     int n = 0, emin = 1 << 20, bad = 0;
     float xmax = 0;
-    for (uint32_t j = tid; j < c.nsamples; j += 256) {
-        if (c.bisl[c.sbody[j]] != (int)i) continue;
-        const float x = c.samples[j];
+    sweep([&](float x) {
         n++;
         const uint32_t b = f2u(x), ex = (b >> 23) & 0xff;
         if (ex == 0xff || (b >> 31)) bad = 1;
         else if (b) emin = imn(emin, ex ? (int)ex - 150 : -149), xmax = fmaxf(xmax, x);
-    }
+    });
     n = block_reduce(n, iad, shi);
     emin = block_reduce(emin, imn, shi);
     bad = block_reduce(bad, imx, shi);
@@ -284,22 +310,24 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
     }
     if (exact && xmax > 0) {
         unsigned long long acc = 0;
-        for (uint32_t j = tid; j < c.nsamples; j += 256) {
-            if (c.bisl[c.sbody[j]] != (int)i) continue;
-            const uint32_t b = f2u(c.samples[j]), ex = (b >> 23) & 0xff;
-            if (!b) continue;
+        sweep([&](float x) {
+            const uint32_t b = f2u(x), ex = (b >> 23) & 0xff;
+            if (!b) return;
             const unsigned long long mant = (b & 0x7fffffu) | (ex ? 0x800000u : 0u);
             const int e = ex ? (int)ex - 150 : -149;
             acc += mant << (e - emin);
-        }
+        });
         acc = block_reduce(acc, uad, shu);
         sum = ldexp((double)acc, emin);
     }
     if (tid != 0) return;
     if (!exact) {
+        // the reference's order: the members in body order, each one's samples in order
         n = 0;
-        for (uint32_t j = 0; j < c.nsamples; j++)
-            if (c.bisl[c.sbody[j]] == (int)i) sum += (double)c.samples[j], n++;
+        for (uint32_t k = k0; k < k1; k++) {
+            const uint32_t g = c.imem[k];
+            for (uint32_t j = c.sample_base[g]; j < c.sample_base[g + 1]; j++) sum += (double)c.samples[j], n++;
+        }
     }
     if (fast) atomicOr(&c.st->unsupported, 1u);
     const float speed = slow + 0.5f;
@@ -498,11 +526,12 @@ constexpr int kGnt = 16;
 __global__ __launch_bounds__(256, 5) void k_rb_nt(KArgs a, CArgs c) {
     __shared__ uint32_t sh[8];
     const uint32_t tk = wg_ticket(c.lk, sh);
-    const uint32_t t = (tk * 256 + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
+    const uint32_t ti = (tk * 256 + threadIdx.x) / kGnt, gl = threadIdx.x & (kGnt - 1);
+    const uint32_t t = ti < c.n_itri ? c.itri[ti] : 0u;
     NtQuery q;
     CellRange cr;
     cr.n = 0;
-    if (t < a.ntris) {
+    if (ti < c.n_itri) {
         q.tr = a.tri_abc[t];
         q.t = t;
         q.isl = c.bisl[q.tr.w];
@@ -657,7 +686,8 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
 __global__ __launch_bounds__(256) void k_rb_prims(KArgs a, CArgs c) {
     __shared__ uint32_t sh[8];
     const uint32_t tk = wg_ticket(c.lk, sh);
-    const uint32_t t = tk * 256 + threadIdx.x;
+    const uint32_t ti = tk * 256 + threadIdx.x;
+    const uint32_t t = ti < c.n_itri ? c.itri[ti] : 0u;
     // A prim is a triangle whose grown box spans at most 512 cells (the reference enters it into all of them). The
     // capsules keep the triangles whose raw box meets theirs, and such a triangle shares with the capsule a cell of
     // its raw box: entering those cells alone finds the same set.
@@ -666,7 +696,7 @@ __global__ __launch_bounds__(256) void k_rb_prims(KArgs a, CArgs c) {
     f3 tmn = mk(0, 0, 0), tmx = mk(0, 0, 0);
     uint4 tr = make_uint4(0, 0, 0, 0);
     int isl = -1;
-    if (t < a.ntris) {
+    if (ti < c.n_itri) {
         tr = a.tri_abc[t];
         isl = c.bisl[tr.w];
         if (isl >= 0 && c.bquery[tr.w] && !a.tri_torn[t]) {
@@ -1163,9 +1193,10 @@ void ContactEngine::release() {
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
     for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
-                             &sbody, &eb, &ecount, &estart, &bcnt, &bpre})
+                             &eb, &ecount, &estart, &bcnt, &bpre})
         b->release();
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
+    itri.release(), n_itri = 0;
     scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
     for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
     hraw.release();
@@ -1203,11 +1234,9 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
         s0[i + 1] = s0[i] + (hv.bodies[i].ntri + 6) / 7;
     }
     nsamples = s0[nbodies];
-    std::vector<uint32_t> sb(nsamples + 1, 0);
-    for (uint32_t i = 0; i < nbodies; i++)
-        for (uint32_t j = s0[i]; j < s0[i + 1]; j++) sb[j] = i;
-    RC(sbody.ensure(nsamples + 1));
-    CHK(hipMemcpy(sbody.p, sb.data(), 4 * (nsamples + 1), hipMemcpyHostToDevice));
+    h_tri0_ = tri0;
+    h_nnode_.resize(nbodies);
+    for (uint32_t i = 0; i < nbodies; i++) h_nnode_[i] = hv.bodies[i].nnode;
     RC(body_tri0.ensure(nbodies + 1));
     RC(sample_base.ensure(nbodies + 1));
     RC(samples.ensure(nsamples + 1));
@@ -1229,7 +1258,9 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(tcount.ensure((size_t)tsize + 1));
     counts_dirty_ = true;
     RC(tstart.ensure((size_t)tsize + 1));
-    cap_nt = std::max<uint32_t>(4096, 2 * T);
+    // (first guesses: a list that outgrows its buffer grows it, and the rebuild runs again once; the filter's
+    // launch covers the node-triangle buffer, so a large guess costs every rebuild)
+    cap_nt = std::max<uint32_t>(4096, T / 8);
     cap_nc = std::max<uint32_t>(4096, 4 * C);
     cap_ct = std::max<uint32_t>(4096, 4 * C);
     cap_ent = std::max<uint32_t>(4096, 8 * T);
@@ -1308,16 +1339,28 @@ int ContactEngine::begin_frame(hipStream_t st) {
     for (size_t k = 0; k < np; k++)
         for (uint32_t p = off[k]; p < off[k + 1]; p++) isl[mem[p]] = (int)k, mi[mem[p]] = (int)p;
     nmem = (uint32_t)mem.size();
-    const void* was[2] = {imem_off.p, imem.p};
+    // the triangles the node-triangle and capsule-triangle searches run over: the pair islands' bodies', in body
+    // order (the order of the candidate lists; no other triangle has candidates)
+    itri_h_.clear();
+    for (uint32_t g = 0; g < nbodies; g++)
+        if (isl[g] >= 0)
+            for (uint32_t t = h_tri0_[g]; t < h_tri0_[g + 1]; t++) itri_h_.push_back(t);
+    n_itri = (uint32_t)itri_h_.size();
+    size_t inodes = 0;
+    for (uint32_t g = 0; g < nbodies; g++)
+        if (isl[g] >= 0) inodes += h_nnode_[g];
+    tsize_f = std::min<uint32_t>(tsize, pow2_at_least(4 * inodes));
+    const void* was[3] = {imem_off.p, imem.p, itri.p};
     RC(imem_off.ensure(np + 1));
     RC(imem.ensure(mem.size() + 1));
+    RC(itri.ensure((size_t)n_itri + 1));
     RC(pcnt.ensure(nmem + 1));
     RC(poff.ensure(nmem + 1));
     cap_part = std::max<uint32_t>(cap_part, std::max<uint32_t>(4096, 8 * nmem));
     RC(plist.ensure(cap_part));
     // (the same tables as the last frame's, in the same buffers: nothing to upload)
-    const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && isl == up_isl_ && mi == up_mi_ &&
-                      off == up_off_ && mem == up_mem_;
+    const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && was[2] == itri.p && isl == up_isl_ &&
+                      mi == up_mi_ && off == up_off_ && mem == up_mem_;
     if (!same) {
         const std::vector<uint32_t> off0(1, 0u);
         const std::vector<uint32_t>& offu = np ? off : off0;
@@ -1325,6 +1368,7 @@ int ContactEngine::begin_frame(hipStream_t st) {
         if (nbodies) CHK(hipMemcpyAsync(bmem.p, mi.data(), sizeof(int) * nbodies, hipMemcpyHostToDevice, st));
         CHK(hipMemcpyAsync(imem_off.p, offu.data(), 4 * offu.size(), hipMemcpyHostToDevice, st));
         if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
+        if (n_itri) CHK(hipMemcpyAsync(itri.p, itri_h_.data(), 4 * (size_t)n_itri, hipMemcpyHostToDevice, st));
         CHK(hipStreamSynchronize(st)); // (off0 goes out of scope)
         up_isl_ = isl, up_mi_ = mi, up_off_ = off, up_mem_ = mem;
         up_valid_ = true;
@@ -1363,14 +1407,14 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     RC(stage(k_rb_partners, ((size_t)nmem + 3) / 4));
     // 1) the node hash
     RC(stage(k_rb_pts, ((size_t)N + 255) / 256));
-    RC(scan(tcount.p, tstart.p, (size_t)tsize + 1, st));
+    RC(scan(tcount.p, tstart.p, (size_t)tsize_f + 1, st));
     hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
     // 2) triangles, then capsules, against the node hash
-    if (T) RC(stage(k_rb_nt, ((size_t)T * kGnt + 255) / 256));
+    if (n_itri) RC(stage(k_rb_nt, ((size_t)n_itri * kGnt + 255) / 256));
     if (C) RC(stage(k_rb_nc, ((size_t)C * kGcap + 255) / 256));
     // 3) capsules against the triangles' cells
     if (C && T) {
-        RC(stage(k_rb_prims, ((size_t)T + 255) / 256));
+        RC(stage(k_rb_prims, ((size_t)n_itri + 255) / 256));
         RC(scan(ecount.p, estart.p, (size_t)esize + 1, st));
         hipLaunchKernelGGL(k_rb_efill, grid_for(cap_ent), dim3(256), 0, st, c);
         RC(stage(k_rb_ct, ((size_t)C * kGcap + 255) / 256));
@@ -1394,10 +1438,11 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.nmem = nmem, c.bmem = bmem.p, c.pcnt = pcnt.p, c.poff = poff.p;
     c.ipar = ipar.p, c.bbox = bbox.p, c.bquery = bquery.p, c.body_box = body_box;
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
-    c.sbody = sbody.p, c.nsamples = nsamples;
+    c.nsamples = nsamples;
+    c.itri = itri.p, c.n_itri = n_itri;
     c.st = stat.p, c.icount = icount.p;
     c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
-    c.tmask = tsize - 1, c.tcount = tcount.p, c.tstart = tstart.p;
+    c.tmask = tsize_f - 1, c.tcount = tcount.p, c.tstart = tstart.p;
     c.scell = scell.p, c.spos = spos.p;
     for (int attempt = 0;; attempt++) {
         RC(enqueue_rebuild(a, c, st));
