@@ -56,6 +56,26 @@ def test_contacts_long_frames(oracle_mod):
     assert gw.stats()["pair_rebuilds"] == ow.stats()["pair_rebuilds"]
 
 
+def test_contact_scene_full_size(oracle_mod):
+    """The 64x32x24 lattice with 2 000 balls at full size over 660 substeps (40 pair rebuilds; the balls on the block
+    land on it after about 570), bit for bit against the oracle. Its compactions span hundreds of workgroups (node hash,
+    node-triangle search), so the look-back crosses many 64-workgroup windows; the scene's 2 001 bodies make many
+    pair islands."""
+    from _cases import GROUND, _boxes
+    print("contact case config2 full size", flush=True)
+    settings, boxes, bodies, n = abi.default_settings(), _boxes(GROUND), scenes.config2_scene(), 660
+    cps = [1, 17, 330, 660]
+    ow = oracle_mod.OracleWorld(settings, boxes)
+    so = run_world(ow, bodies, n, cps)
+    gw = gpu_world(settings, boxes, TUNINGS["stepped_1_launches"])
+    sg = run_world(gw, bodies, n, cps)
+    assert_same(sg, so, "gpu vs oracle [config2 full size]")
+    st_o, st_g = ow.stats(), gw.stats()
+    for key in COUNTERS:
+        assert st_g[key] == st_o[key], (key, st_g[key], st_o[key])
+    assert st_g["contacts"] > 0 and st_g["narrow_tests"] > 0
+
+
 def test_fast_body_refused():
     """A body faster than 8 m/s in a contact island has its own pair schedule in the reference: refused."""
     settings = abi.default_settings()
