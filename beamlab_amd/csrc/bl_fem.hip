@@ -875,7 +875,9 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
         }
     }
     const double vinit = lane < nrow ? V[R + lane] : 0.0;
-    double pacc = 0;
+    const uint32_t rown = R + lane;
+    const size_t ydst = part == 0 && lane < nrow && rown < c ? 6 * (size_t)s.ord2f[fr.own0 + rown / 6] + rown % 6 : 0;
+    double ys_own = 0, pacc = 0;
     const uint32_t ncol = R < c ? R : c, nblk = (ncol + 63) / 64, j0 = part * 16;
     for (uint32_t jb = 0; jb < nblk; jb++) {
         const uint32_t J = jb * 64, w = c - J < 64u ? c - J : 64u;
@@ -906,15 +908,14 @@ __global__ __launch_bounds__(256) void k_fsolve(SolArgs s, const uint2* tasks, u
                 else if (lane > j) acc -= Ld[lane][j] * yj;
             }
         }
-        if (lane < nrow) {
-            const uint32_t r = R + lane;
-            st_dev(&V[r], acc);
-            if (r < c) s.y[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = acc;
-        }
+        if (lane < nrow) st_dev(&V[R + lane], acc);
+        ys_own = acc;
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
     if (tid == 0 && R < c && !sbad) __hip_atomic_store(&s.fflag[fr.fb + t.y], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // (the solution in the system's order after the flag: only later launches read it)
+    if (part == 0 && !sbad && lane < nrow && R + lane < c) s.y[ydst] = ys_own;
 }
 
 // task = (front, own column block jb), the front's last block first; the diagonal block and each later block's tile
@@ -934,6 +935,10 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     const double* F = s.F + fr.off_l;
     double* X = s.X + fr.voff;
     if (tid == 0) sbad = 0;
+    // the own rows' right side (the forward pass's y), read now rather than on the chain
+    const double vown = part == 0 && lane < w ? s.V[fr.voff + J + lane] : 0.0;
+    const size_t xdst = part == 0 && lane < w ? 6 * (size_t)s.ord2f[fr.own0 + (J + lane) / 6] + (J + lane) % 6 : 0;
+    double xs_own = 0;
     // this wave's 16 columns: partial sums of L[r][j] x_r per lane
     double pj[16];
 #pragma unroll
@@ -995,21 +1000,20 @@ __global__ __launch_bounds__(256) void k_bsolve(SolArgs s, const uint2* tasks, u
     }
     __syncthreads();
     if (part == 0 && !sbad) {
-        double acc = lane < w ? s.V[fr.voff + J + lane] - dots[lane] : 0.0;
+        double acc = lane < w ? vown - dots[lane] : 0.0;
         for (uint32_t j = w; j-- > 0;) {
             const double xj = readlane_u(acc, j) * Ld[j][j]; // (L's diagonal holds 1 / L_jj)
             if (lane == j) acc = xj;
             else if (lane < j) acc -= Ld[j][lane] * xj;
         }
-        if (lane < w) {
-            const uint32_t r = J + lane;
-            st_dev(&X[r], acc);
-            s.x[6 * (size_t)s.ord2f[fr.own0 + r / 6] + r % 6] = acc;
-        }
+        if (lane < w) st_dev(&X[J + lane], acc);
+        xs_own = acc;
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
     if (tid == 0 && !sbad) __hip_atomic_store(&s.bflag[fr.fb + t.y], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // (the solution in the system's order after the flag: only later launches read it)
+    if (part == 0 && !sbad && lane < w) s.x[xdst] = xs_own;
 }
 
 // ------------------------------------------------------------------------------------------ residual and update
