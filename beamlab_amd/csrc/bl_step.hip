@@ -302,7 +302,9 @@ __device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, in
 // The workgroup's counts onto its chunk's slot (a plain read-modify-write: one workgroup owns a chunk at a time;
 // the slots are folded into the counters once a frame, so the substeps need no atomics).
 template <int T>
-__device__ __forceinline__ void flush_chunk(const KArgs& a, NodeCounts cnt, uint32_t c, uint4* sred) {
+// (pre: the slot's value, read by thread 0 ahead of time; null: read here)
+__device__ __forceinline__ void flush_chunk(const KArgs& a, NodeCounts cnt, uint32_t c, uint4* sred,
+                                            const uint4* pre = nullptr) {
     uint32_t c0 = cnt.contacts, c1 = cnt.clamps, c2 = cnt.nans;
     float v2 = cnt.max_v2;
     for (int off = 32; off > 0; off >>= 1) {
@@ -315,7 +317,7 @@ __device__ __forceinline__ void flush_chunk(const KArgs& a, NodeCounts cnt, uint
     if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = make_uint4(c0, c1, c2, f2u(v2));
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint4 t = a.cstat[c];
+        uint4 t = pre ? *pre : a.cstat[c];
         for (int w = 0; w < T / 64; w++) {
             const uint4 x = sred[w];
             t.x += x.x, t.y += x.y, t.z += x.z;
@@ -335,8 +337,12 @@ __global__ __launch_bounds__(T) void k_substep(KArgs a, int parity) {
     __shared__ uint8_t sok[T * PER];
     __shared__ uint4 sred[T / 64];
     NodeCounts cnt = {0, 0, 0, 0.0f};
+    // (the chunk's count slot read at the start, so that the workgroup's tail does not wait for it)
+    const uint32_t c = xcd_chunk(a, blockIdx.x);
+    uint4 pre = make_uint4(0, 0, 0, 0);
+    if (threadIdx.x == 0) pre = a.cstat[c];
     chunk_substep<T, PER>(a.wg_rng[blockIdx.x], a, parity, sfx, sfy, sfz, sok, cnt);
-    flush_chunk<T>(a, cnt, xcd_chunk(a, blockIdx.x), sred);
+    flush_chunk<T>(a, cnt, c, sred, &pre);
 }
 
 // Many substeps per launch, one workgroup per body: the body's chunks one after another, a workgroup barrier
@@ -890,6 +896,8 @@ __global__ __launch_bounds__(256) void k_integrate(KArgs a, int parity, uint4* w
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < a.nnodes;
     NodeCounts cnt = {0, 0, 0, 0.0f};
+    uint4 pre = make_uint4(0, 0, 0, 0); // (the workgroup's slot, read at the start)
+    if (threadIdx.x == 0) pre = wstat[blockIdx.x];
     if (valid) {
         float4 P = a.pm[parity][i], V = a.vm[parity][i];
         const float4 F = a.F[i];
@@ -902,7 +910,7 @@ __global__ __launch_bounds__(256) void k_integrate(KArgs a, int parity, uint4* w
     body_reduce<256>(a, i, valid, cnt.max_v2, 0, 0, 0, true, false, sred);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint4 t = wstat[blockIdx.x];
+        uint4 t = pre;
         for (int w = 0; w < 4; w++) t.x += scnt[w].x, t.y += scnt[w].y, t.z += scnt[w].z;
         wstat[blockIdx.x] = t;
     }
