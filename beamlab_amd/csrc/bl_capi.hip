@@ -109,7 +109,7 @@ struct bl_ctx {
     DevArr<uint4> cstat;
     DevArr<uint32_t> halo_off, halo, nbr_off, nbr, iloc, flags, wg_chunk;
     DevArr<float2> naux;
-    DevArr<uint2> iAB;
+    DevArr<uint32_t> iOth;
     DevArr<float> stress;
     DevArr<BodyParam> bparam;
     DevArr<bl_box> dboxes;
@@ -161,7 +161,7 @@ struct bl_ctx {
         drop_graphs();
         for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
         iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release(), ch_rng.release(), fch_rng.release(), fwg.release(), wg_rng.release(), fwg_rng.release();
-        body_c0.release(), naux.release(), iAB.release(), stress.release(), bparam.release(), dboxes.release();
+        body_c0.release(), naux.release(), iOth.release(), stress.release(), bparam.release(), dboxes.release();
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
         halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
@@ -432,6 +432,16 @@ int build(bl_ctx* c) {
         for (size_t j = 0; j < xl[x].size(); j++) wg_chunk[j * 8 + x] = xl[x][j];
     ch_n0.push_back(N);
     c->h_noff = noff;
+    // each incidence's own node as an offset from its chunk's first node (in the flags' high bits), and the other end
+    std::vector<uint32_t> iOth(E);
+    for (uint32_t k = 0; k + 1 < ch_n0.size(); k++)
+        for (uint32_t i = ch_n0[k]; i < ch_n0[k + 1]; i++)
+            for (uint32_t e = noff[i]; e < noff[i + 1]; e++) {
+                const uint32_t tf = f2u(iM[e].w), off = i - ch_n0[k];
+                if (off >= (1u << (32 - TF_OWN_SHIFT))) return BL_ERR_ARG;
+                iM[e].w = u2f(tf | (off << TF_OWN_SHIFT));
+                iOth[e] = (tf & TF_ISB) ? iAB[e].x : iAB[e].y;
+            }
     std::vector<uint4> ch_rng(nchunks);
     for (uint32_t k = 0; k < nchunks; k++) ch_rng[k] = make_uint4(ch_n0[k], ch_n0[k + 1], noff[ch_n0[k]], noff[ch_n0[k + 1]]);
     // the force phase's chunks: those with beams as they are, the beamless ones' nodes in contiguous runs of <= T
@@ -503,7 +513,7 @@ int build(bl_ctx* c) {
     int rc;
     if ((rc = c->pm[0].upload(P)) || (rc = c->vm[0].upload(V)) || (rc = c->pm[1].alloc(N)) || (rc = c->vm[1].alloc(N)) ||
         (rc = c->ninfo.upload(ninfo)) || (rc = c->naux.upload(naux)) || (rc = c->noff.upload(noff)) ||
-        (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iAB.upload(iAB)) || (rc = c->bcold.upload(bcold)) ||
+        (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iOth.upload(iOth)) || (rc = c->bcold.upload(bcold)) ||
         (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->ch_rng.upload(ch_rng)) || (rc = c->fch_rng.upload(fch_rng)) || (rc = c->fwg.upload(fwg)) || (rc = c->wg_rng.upload(wg_rng)) || (rc = c->fwg_rng.upload(fwg_rng)) || (rc = c->body_c0.upload(body_c0)) ||
         (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
@@ -530,7 +540,7 @@ int build(bl_ctx* c) {
     a = KArgs{};
     a.pm[0] = c->pm[0].p, a.pm[1] = c->pm[1].p, a.vm[0] = c->vm[0].p, a.vm[1] = c->vm[1].p;
     a.ninfo = c->ninfo.p, a.naux = c->naux.p, a.noff = c->noff.p;
-    a.iA = c->iA.p, a.iM = c->iM.p, a.iAB = c->iAB.p, a.bcold = c->bcold.p, a.stress = c->stress.p;
+    a.iA = c->iA.p, a.iM = c->iM.p, a.iOth = c->iOth.p, a.bcold = c->bcold.p, a.stress = c->stress.p;
     a.ch_n0 = c->ch_n0.p, a.ch_rng = c->ch_rng.p, a.body_c0 = c->body_c0.p, a.bodies = c->bparam.p, a.boxes = c->dboxes.p;
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
     a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;

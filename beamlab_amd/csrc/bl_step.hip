@@ -10,7 +10,7 @@
 //   nodes      pm[2][N] float4 {p, inv_mass}, vm[2][N] float4 {v, mass}  (double-buffered: a substep reads one
 //              buffer and writes the other, so no node is read after it was integrated)
 //   incidences one record per (beam, end node), node-major and in beam-index order within a node (CSR noff[]):
-//              iA float4 {L, k, d, strength}, iM float4 {max_pos, max_neg, beam index, flags}, iAB uint2 {a, b}.
+//              iA float4 {L, k, d, strength}, iM float4 {max_pos, max_neg, beam index, flags}, iOth the other end.
 //              Each end keeps its own copy of the beam's mutable state; both copies see the same inputs and run
 //              the same arithmetic, so they stay identical and no end ever reads what the other writes.
 //   chunks     contiguous node ranges of one body whose incidences fit in LDS (ch_n0[]): one workgroup each.
@@ -228,14 +228,16 @@ __device__ __forceinline__ f3 node_force(const KArgs& a, float mass, uint32_t b0
 // One chunk's substep with the incidence records streamed from memory: all of a thread's loads are issued before
 // its arithmetic (PER incidences per thread).
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
-typedef uint32_t nt_u2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float4 ld_nt(const float4* p) {
     const nt_f4 v = __builtin_nontemporal_load((const nt_f4*)p);
     return make_float4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ uint2 ld_nt(const uint2* p) {
-    const nt_u2 v = __builtin_nontemporal_load((const nt_u2*)p);
-    return make_uint2(v.x, v.y);
+__device__ __forceinline__ uint32_t ld_nt(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+// an incidence's beam ends (a, b) from its own node (the chunk's first node n0 plus the offset in its flags tf) and
+// the other end
+__device__ __forceinline__ uint2 inc_ends(uint32_t n0, uint32_t tf, uint32_t other) {
+    const uint32_t own = n0 + (tf >> TF_OWN_SHIFT);
+    return (tf & TF_ISB) ? make_uint2(other, own) : make_uint2(own, other);
 }
 
 template <int T, int PER, bool FORCES_ONLY = false>
@@ -247,13 +249,14 @@ __device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, in
     float4* __restrict__ vout = a.vm[parity ^ 1];
     const uint32_t n0 = cr.x, n1 = cr.y, e0 = cr.z, e1 = cr.w;
     float4 A[PER], M[PER];
+    uint32_t oth[PER];
     uint2 ab[PER];
     // (the incidence records stream through once a substep: loaded non-temporal, so that they do not push the
     // nodes, read by many workgroups, out of the L2)
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const uint32_t e = e0 + threadIdx.x + k * T;
-        if (e < e1) A[k] = ld_nt(a.iA + e), M[k] = ld_nt(a.iM + e), ab[k] = ld_nt(a.iAB + e);
+        if (e < e1) A[k] = ld_nt(a.iA + e), M[k] = ld_nt(a.iM + e), oth[k] = ld_nt(a.iOth + e);
     }
     // the node phase's own loads (a chunk holds at most T nodes: one a thread), issued with the beam phase's
     const uint32_t i = n0 + threadIdx.x;
@@ -269,7 +272,10 @@ __device__ __forceinline__ void chunk_substep(const uint4 cr, const KArgs& a, in
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const uint32_t e = e0 + threadIdx.x + k * T;
-        if (e < e1) Pa[k] = pin[ab[k].x], Pb[k] = pin[ab[k].y], Va[k] = vin[ab[k].x], Vb[k] = vin[ab[k].y];
+        if (e < e1) {
+            ab[k] = inc_ends(n0, f2u(M[k].w), oth[k]);
+            Pa[k] = pin[ab[k].x], Pb[k] = pin[ab[k].y], Va[k] = vin[ab[k].x], Vb[k] = vin[ab[k].y];
+        }
     }
 #pragma unroll
     for (int k = 0; k < PER; k++) {
@@ -416,7 +422,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)
         A[k] = make_float4(0, 0, 0, 0), M[k] = make_float4(0, 0, 0, u2f(TF_SKIP)), ab[k] = make_uint2(0, 0);
         stress[k] = 0;
         if (e < e1) {
-            A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = a.iAB[e];
+            A[k] = a.iA[e], M[k] = a.iM[e], ab[k] = inc_ends(n0, f2u(M[k].w), a.iOth[e]);
             stress[k] = (f2u(M[k].w) & TF_ISB) ? 0.0f : a.stress[f2u(M[k].z)];
         }
     }

@@ -19,6 +19,8 @@ enum : uint32_t {
     TF_ISB = 1u << 8,      // this incidence is the beam's b end
 };
 constexpr int TF_TYPE_SHIFT = 6;
+// bits 16 and up: the incidence's own node less its chunk's first node (the other end is in iOth)
+constexpr int TF_OWN_SHIFT = 16;
 
 enum { CNT_STATIC = 0, CNT_CLAMPS = 1, CNT_NANS = 2, CNT_CONTACTS = 3, CNT_N = 4 };
 // per-body frame words: [0] top speed squared (float bits), [1..3] min key x y z, [4..6] max key x y z
@@ -45,7 +47,7 @@ struct KArgs {
     const uint32_t* noff;  // CSR offsets of the incidences, [N + 1]
     float4* iA;            // L, k, d, strength
     float4* iM;            // max_pos, max_neg, beam index (bits), flags (bits)
-    const uint2* iAB;      // a, b
+    const uint32_t* iOth;  // the beam's other end (its a end when this is its b end, else its b end)
     const float4* bcold;   // per beam: plastic, L0, support_limit, 0
     float* stress;         // per beam
     const uint32_t* ch_n0; // chunk node starts, [chunks + 1]
