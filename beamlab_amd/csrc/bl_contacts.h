@@ -32,8 +32,8 @@ struct RStat {
     uint32_t npart;       // partner entries (when they overflowed their buffer)
 };
 
-// A one-pass launch's look-back: the workgroups' published words, the ticket counter, the tickets taken before the
-// launch and its epoch.
+// A one-pass launch's look-back: the workgroups' published words, the ticket counter (null: workgroups in index
+// order), the tickets taken before the launch and its epoch.
 struct Lookback {
     uint64_t* words;
     uint32_t* ticket;
@@ -193,6 +193,7 @@ private:
     Buf<uint64_t> lk_words;
     Buf<uint32_t> lk_ticket;
     uint32_t lk_epoch_ = 0, lk_base_ = 0;
+    bool lk_tickets_ = false; // order the one-pass launches' workgroups by tickets (else by index)
     RStat* hstat = nullptr; // pinned host copy of the rebuild's status
     Buf<int> bisl, bmem;
     Buf<uint32_t> pcnt, poff, plist;

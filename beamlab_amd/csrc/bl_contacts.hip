@@ -130,6 +130,10 @@ __device__ __forceinline__ void wave_sync() {
 // have started: every wait ends. It is bounded all the same: a time-out sets bit 2 of the rebuild's unsupported word
 // (the host then fails the rebuild).
 __device__ __forceinline__ uint32_t wg_ticket(const Lookback& k, uint32_t* sh) {
+    // (without a ticket counter: the workgroup's index. The hardware starts the workgroups of an XCD in index
+    // order and deals them to the XCDs in turn, so the lowest unfinished workgroup has started and every wait
+    // ends; should a wait time out all the same, the host runs the rebuild again with tickets)
+    if (!k.ticket) return blockIdx.x;
     if (threadIdx.x == 0) *sh = atomicAdd(k.ticket, 1u) - k.base;
     __syncthreads();
     return *sh;
@@ -1170,8 +1174,8 @@ int ContactEngine::lookback(Lookback& k, size_t nb, hipStream_t st) {
         hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, lk_ticket.p, 1u);
         lk_epoch_ = 0, lk_base_ = 0;
     }
-    k.words = lk_words.p, k.ticket = lk_ticket.p, k.base = lk_base_, k.epoch = ++lk_epoch_;
-    lk_base_ += (uint32_t)nb;
+    k.words = lk_words.p, k.ticket = lk_tickets_ ? lk_ticket.p : nullptr, k.base = lk_base_, k.epoch = ++lk_epoch_;
+    if (lk_tickets_) lk_base_ += (uint32_t)nb;
     return BL_OK;
 }
 
@@ -1454,6 +1458,12 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
         CHK(hipStreamSynchronize(st));
         counts_dirty_ = false;
         last = *hstat;
+        if ((last.unsupported & 4) && !lk_tickets_) {
+            // a look-back wait timed out with workgroups ordered by index: tickets from now on, and again
+            lk_tickets_ = true;
+            counts_dirty_ = true;
+            continue;
+        }
         if (!last.overflow) break;
         if (attempt > 4) return BL_ERR_HIP;
         if (last.overflow & 1) cap_nt = std::max<uint32_t>(cap_nt, last.nnt_raw + last.nnt_raw / 2 + 1);
