@@ -1,5 +1,5 @@
 // Host-side broadphase of the island building: every pair of body boxes that overlap, each pair once. A uniform
-// 3-D grid (the cell about the median box extent, larger where the scene is sparse: at most about 8 cells a box)
+// 3-D grid (the cell about the median box extent, larger where the scene is sparse: at most about 2 cells a box)
 // holds each box in every cell it covers; a pair is reported by the first cell (lowest x, then y, then z) that both
 // boxes cover, so it comes out once. Boxes that are not finite, or cover more than kMaxCellsPerBox cells, are tested
 // against every other box instead. The set of pairs is
@@ -51,10 +51,11 @@ void overlapping_pairs(const float* bx, size_t n, PairScratch& ws, Emit emit) {
     if (!ext.empty()) {
         std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
         double cell = std::max<double>(ext[ext.size() / 2], 1e-3);
-        // at most about 8 cells a box over the whole extent
+        // at most about 2 cells a box over the whole extent (a sparse scene - bodies spread on the ground around a
+        // tall one - would otherwise spend its time on empty cells)
         double vol = 1;
         for (int k = 0; k < 3; k++) vol *= std::max<double>(hi3[k] - lo3[k], cell);
-        const double cap = 8.0 * (double)n + 64;
+        const double cap = 2.0 * (double)n + 64;
         if (vol / (cell * cell * cell) > cap) cell = std::cbrt(vol / cap);
         const double inv = 1.0 / cell;
         int64_t nc[3];
