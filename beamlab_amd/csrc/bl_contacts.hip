@@ -1344,16 +1344,19 @@ int ContactEngine::begin_frame(hipStream_t st) {
         for (uint32_t p = off[k]; p < off[k + 1]; p++) isl[mem[p]] = (int)k, mi[mem[p]] = (int)p;
     nmem = (uint32_t)mem.size();
     // the triangles the node-triangle and capsule-triangle searches run over: the pair islands' bodies', in body
-    // order (the order of the candidate lists; no other triangle has candidates)
-    itri_h_.clear();
-    for (uint32_t g = 0; g < nbodies; g++)
-        if (isl[g] >= 0)
-            for (uint32_t t = h_tri0_[g]; t < h_tri0_[g + 1]; t++) itri_h_.push_back(t);
-    n_itri = (uint32_t)itri_h_.size();
-    size_t inodes = 0;
-    for (uint32_t g = 0; g < nbodies; g++)
-        if (isl[g] >= 0) inodes += h_nnode_[g];
-    tsize_f = std::min<uint32_t>(tsize, pow2_at_least(4 * inodes));
+    // order (the order of the candidate lists; no other triangle has candidates); and the node hash's size (only
+    // when the island bodies changed)
+    if (!up_valid_ || isl != up_isl_) {
+        itri_h_.clear();
+        size_t inodes = 0;
+        for (uint32_t g = 0; g < nbodies; g++)
+            if (isl[g] >= 0) {
+                for (uint32_t t = h_tri0_[g]; t < h_tri0_[g + 1]; t++) itri_h_.push_back(t);
+                inodes += h_nnode_[g];
+            }
+        n_itri = (uint32_t)itri_h_.size();
+        tsize_f = std::min<uint32_t>(tsize, pow2_at_least(4 * inodes));
+    }
     const void* was[3] = {imem_off.p, imem.p, itri.p};
     RC(imem_off.ensure(np + 1));
     RC(imem.ensure(mem.size() + 1));
