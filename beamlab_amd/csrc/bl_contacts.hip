@@ -496,9 +496,15 @@ __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, cons
                                             uint4* out, uint32_t at, uint32_t cap) {
     uint32_t cnt = 0;
     const uint32_t bk = bucket_of(x, y, z, q.isl, c.tmask);
-    for (uint32_t k = c.tstart[bk]; k < c.tstart[bk + 1]; k++) {
-        const int4 cell = c.scell[k];
-        const float4 pp = c.spos[k];
+    // (the next entry's records read while this one is tested)
+    const uint32_t k0 = c.tstart[bk], k1 = c.tstart[bk + 1];
+    int4 ncell = make_int4(0, 0, 0, 0);
+    float4 npp = make_float4(0, 0, 0, 0);
+    if (k0 < k1) ncell = c.scell[k0], npp = c.spos[k0];
+    for (uint32_t k = k0; k < k1; k++) {
+        const int4 cell = ncell;
+        const float4 pp = npp;
+        if (k + 1 < k1) ncell = c.scell[k + 1], npp = c.spos[k + 1];
         if (cell.x != x || cell.y != y || cell.z != z) continue;
         const uint32_t gbn = f2u(pp.w);
         if (gbn == q.tr.w && !q.self_coll) continue;
@@ -603,9 +609,15 @@ __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint
                                             uint32_t ci) {
     uint32_t cnt = 0;
     const uint32_t bk = bucket_of(x, y, z, isl, c.tmask);
-    for (uint32_t k = c.tstart[bk]; k < c.tstart[bk + 1]; k++) {
-        const int4 cell = c.scell[k];
-        const float4 pp = c.spos[k];
+    // (the next entry's records read while this one is tested)
+    const uint32_t k0 = c.tstart[bk], k1 = c.tstart[bk + 1];
+    int4 ncell = make_int4(0, 0, 0, 0);
+    float4 npp = make_float4(0, 0, 0, 0);
+    if (k0 < k1) ncell = c.scell[k0], npp = c.spos[k0];
+    for (uint32_t k = k0; k < k1; k++) {
+        const int4 cell = ncell;
+        const float4 pp = npp;
+        if (k + 1 < k1) ncell = c.scell[k + 1], npp = c.spos[k + 1];
         if (cell.x != x || cell.y != y || cell.z != z) continue;
         const uint32_t gbn = f2u(pp.w);
         if (gbn == cp.z || c.bisl[gbn] != isl) continue;
@@ -787,11 +799,18 @@ __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
             int x, y, z;
             cr.cell(cc, x, y, z);
             const uint32_t bk = bucket_of(x, y, z, isl, c.emask);
-            for (uint32_t k = c.estart[bk]; k < c.estart[bk + 1]; k++) {
-                const uint4 ra = c.sa[k];
+            // (an entry's three records are read together, and the next entry's while this one is tested)
+            const uint32_t k0 = c.estart[bk], k1 = c.estart[bk + 1];
+            uint4 na = make_uint4(0, 0, 0, 0);
+            float4 nb = make_float4(0, 0, 0, 0);
+            float2 nc2 = make_float2(0, 0);
+            if (k0 < k1) na = c.sa[k0], nb = c.sb[k0], nc2 = c.sc[k0];
+            for (uint32_t k = k0; k < k1; k++) {
+                const uint4 ra = na;
+                const float4 rb = nb;
+                const float2 rc = nc2;
+                if (k + 1 < k1) na = c.sa[k + 1], nb = c.sb[k + 1], nc2 = c.sc[k + 1];
                 if ((int)ra.w != isl || ra.y == cp.z || (cg != 0 && cg == ra.z)) continue;
-                const float4 rb = c.sb[k];
-                const float2 rc = c.sc[k];
                 if (rb.w < mn.x || rc.x < mn.y || rc.y < mn.z || rb.x > mx.x || rb.y > mx.y || rb.z > mx.z) continue;
                 const uint32_t at = atomicAdd(&lcnt[gi], 1u);
                 if (at < (uint32_t)kCtCap) L[at] = ra.x;
