@@ -1009,7 +1009,11 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
     __shared__ int16_t preds[kRespW * 5];
     __shared__ uint16_t level[kRespW];
     __shared__ uint32_t sred[kRespW / 64];
+    // the blocks' hit prefix in LDS when it fits (each hit's block is found by a binary search over it)
+    constexpr uint32_t kPreLds = 2048;
+    __shared__ uint32_t spre[kPreLds];
     PArgs p = pin;
+    uint32_t* pre = nblk <= kPreLds ? spre : p.bpre;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // exclusive prefix of the block counts
@@ -1029,7 +1033,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
             before += w < wv ? sred[w] : 0u;
             total += sred[w];
         }
-        if (i < nblk) p.bpre[i] = carry + before + x - v;
+        if (i < nblk) pre[i] = carry + before + x - v;
         carry += total;
         __syncthreads();
     }
@@ -1049,10 +1053,10 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
             uint32_t lo = 0, hi = nblk - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) >> 1;
-                if (p.bpre[mid] <= h) lo = mid;
+                if (pre[mid] <= h) lo = mid;
                 else hi = mid - 1;
             }
-            hit = p.hraw[lo * 256 + (h - p.bpre[lo])];
+            hit = p.hraw[lo * 256 + (h - pre[lo])];
             contact_nodes(a, p, hit, cn);
         }
         if (5 * m <= (uint32_t)kRespW) {
