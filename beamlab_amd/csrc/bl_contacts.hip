@@ -86,9 +86,14 @@ __device__ __forceinline__ bool partners(const KArgs& a, const CArgs& c, uint32_
 __device__ bool near_partner(const KArgs& a, const CArgs& c, uint32_t g, f3 mn, f3 mx) {
     if (a.bodies[g].self_coll) return true;
     const int m = c.bmem[g];
-    const uint32_t k1 = c.poff[m] + c.pcnt[m] < c.cap_part ? c.poff[m] + c.pcnt[m] : c.cap_part;
-    for (uint32_t k = c.poff[m]; k < k1; k++)
-        if (box_meets(c.bbox + 6 * c.plist[k], mn, mx)) return true;
+    const uint32_t k0 = c.poff[m], k1 = k0 + c.pcnt[m] < c.cap_part ? k0 + c.pcnt[m] : c.cap_part;
+    // (the next partner's index read while this one's box is tested)
+    uint32_t y = k0 < k1 ? c.plist[k0] : 0u;
+    for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t yk = y;
+        if (k + 1 < k1) y = c.plist[k + 1];
+        if (box_meets(c.bbox + 6 * yk, mn, mx)) return true;
+    }
     return false;
 }
 
