@@ -932,29 +932,31 @@ __global__ void k_wstat_fold(KArgs a, uint4* wstat, uint32_t n) {
     wstat[w] = make_uint4(0, 0, 0, 0);
 }
 
-// The force phase and the contacts' narrow phase in one launch: the workgroups past the chunks take the narrow
-// phase's blocks (it reads the positions and the candidate lists only, and writes its own hit buffers), so that it
-// runs in the force phase's tail rather than in a launch of its own.
+// The force phase and the contacts' narrow phase in one launch (the narrow phase reads the positions and the
+// candidate lists only, and writes its own hit buffers). The narrow phase's blocks come first, so that their chains
+// of dependent reads start with the launch, under the force chunks; padded to a multiple of 8, so that the force
+// chunks keep the XCDs their table gives them.
 template <int PER>
-__global__ __launch_bounds__(256) void k_forces_narrow(KArgs a, int parity, PArgs p) {
+__global__ __launch_bounds__(256) void k_forces_narrow(KArgs a, int parity, PArgs p, uint32_t nblk) {
     __shared__ float sfx[256 * PER], sfy[256 * PER], sfz[256 * PER];
     __shared__ uint8_t sok[256 * PER];
-    if (blockIdx.x < a.nfchunks) {
+    const uint32_t nb8 = (nblk + 7) & ~7u;
+    if (blockIdx.x >= nb8) {
         NodeCounts cnt = {0, 0, 0, 0.0f};
-        chunk_substep<256, PER, true>(a.fwg_rng[blockIdx.x], a, parity, sfx, sfy, sfz, sok, cnt);
-    } else {
-        narrow_block(a, p, parity, blockIdx.x - a.nfchunks);
+        chunk_substep<256, PER, true>(a.fwg_rng[blockIdx.x - nb8], a, parity, sfx, sfy, sfz, sok, cnt);
+    } else if (blockIdx.x < nblk) {
+        narrow_block(a, p, parity, blockIdx.x);
     }
 }
 
 int launch_forces_narrow(const KArgs& a, hipStream_t st, int parity, const bl_tuning& t, const PArgs& p, uint32_t nblk) {
     const int per = t.incidences_per_thread;
-    const dim3 g(a.nfchunks + nblk);
+    const dim3 g(a.nfchunks + ((nblk + 7) & ~7u));
     if (t.block != 256) return BL_ERR_ARG;
-    if (per == 1) hipLaunchKernelGGL((k_forces_narrow<1>), g, dim3(256), 0, st, a, parity, p);
-    else if (per == 2) hipLaunchKernelGGL((k_forces_narrow<2>), g, dim3(256), 0, st, a, parity, p);
-    else if (per == 4) hipLaunchKernelGGL((k_forces_narrow<4>), g, dim3(256), 0, st, a, parity, p);
-    else if (per == 8) hipLaunchKernelGGL((k_forces_narrow<8>), g, dim3(256), 0, st, a, parity, p);
+    if (per == 1) hipLaunchKernelGGL((k_forces_narrow<1>), g, dim3(256), 0, st, a, parity, p, nblk);
+    else if (per == 2) hipLaunchKernelGGL((k_forces_narrow<2>), g, dim3(256), 0, st, a, parity, p, nblk);
+    else if (per == 4) hipLaunchKernelGGL((k_forces_narrow<4>), g, dim3(256), 0, st, a, parity, p, nblk);
+    else if (per == 8) hipLaunchKernelGGL((k_forces_narrow<8>), g, dim3(256), 0, st, a, parity, p, nblk);
     else return BL_ERR_ARG;
     return BL_OK_OR_HIP();
 }
