@@ -111,6 +111,202 @@ BL_HD float bl_expf(float x) {
     return (float)y;
 }
 
+// ---------------------------------------------------------------------------------------------- sinf and cosf
+// glibc's sinf / cosf method (Szabolcs Nagy's, from ARM's optimized-routines, in glibc 2.28+): the argument reduced to
+// [-pi/4, pi/4] in double (one multiply-add below 120, a 96-bit product with the bits of 2/pi above), then odd and even
+// polynomials in double, rounded once to float; glibc's x86-64 FMA build forms the multiply-adds fused, as here. The
+// coefficients are the method's published ones. Equal to the host libm's sinf / cosf on every float input (checked
+// exhaustively by tests/test_devmath.py).
+#define BL_SC_S1 (-0x1.555545995a603p-3)
+#define BL_SC_S2 (0x1.1107605230bc4p-7)
+#define BL_SC_S3 (-0x1.994eb3774cf24p-13)
+#define BL_SC_C1 (-0x1.ffffffd0c621cp-2)
+#define BL_SC_C2 (0x1.55553e1068f19p-5)
+#define BL_SC_C3 (-0x1.6c087e89a359dp-10)
+#define BL_SC_C4 (0x1.99343027bf8c3p-16)
+// The bits of 2/pi (0.a2f9836e 4e441529 ...), a 32-bit window every 8 bits, three leading partial words.
+#define BL_INV_PIO4                                                                                                   \
+    {0xa2u,        0xa2f9u,      0xa2f983u,    0xa2f9836eu,  0xf9836e4eu,  0x836e4e44u,  0x6e4e4415u,  0x4e441529u,   \
+     0x441529fcu,  0x1529fc27u,  0x29fc2757u,  0xfc2757d1u,  0x2757d1f5u,  0x57d1f534u,  0xd1f534ddu,  0xf534ddc0u,   \
+     0x34ddc0dbu,  0xddc0db62u,  0xc0db6295u,  0xdb629599u,  0x6295993cu,  0x95993c43u,  0x993c4390u,  0x3c439041u}
+#if defined(__HIPCC__)
+__constant__ const uint32_t k_inv_pio4[24] = BL_INV_PIO4;
+#define BL_INV_PIO4_TABLE k_inv_pio4
+#else
+static const uint32_t k_inv_pio4_host[24] = BL_INV_PIO4;
+#define BL_INV_PIO4_TABLE k_inv_pio4_host
+#endif
+
+// The polynomials on the reduced argument: odd (sine) when n is even, even (cosine, times cs = +-1) when n is odd.
+BL_HD float sc_poly(double x, double x2, int n, double cs) {
+    if ((n & 1) == 0) {
+        const double x3 = x * x2;
+        const double s1 = dfma(x2, BL_SC_S3, BL_SC_S2);
+        const double x7 = x3 * x2;
+        const double s = dfma(x3, BL_SC_S1, x);
+        return (float)dfma(x7, s1, s);
+    }
+    const double x4 = x2 * x2;
+    const double c2 = dfma(x2, cs * BL_SC_C4, cs * BL_SC_C3);
+    const double c1 = dfma(x2, cs * BL_SC_C1, cs);
+    const double x6 = x4 * x2;
+    const double c = dfma(x4, cs * BL_SC_C2, c1);
+    return (float)dfma(x6, c2, c);
+}
+// |y| < 120: y = x + n pi/2, x in [-pi/4, pi/4] (2/pi scaled by 2^24, so the quadrant is bits 24..31 of the product)
+BL_HD double sc_reduce_fast(double x, int* np) {
+    const double r = x * 0x1.45f306dc9c883p+23;
+    const int n = ((int32_t)r + 0x800000) >> 24;
+    *np = n;
+    return dfma(-(double)n, 0x1.921fb54442d18p0, x);
+}
+// |y| >= 120: the mantissa times 96 bits of 2/pi (from the window the exponent selects)
+BL_HD double sc_reduce_large(uint32_t xi, int* np) {
+    const uint32_t* arr = &BL_INV_PIO4_TABLE[(xi >> 26) & 15];
+    const int shift = (xi >> 23) & 7;
+    xi = (xi & 0xffffffu) | 0x800000u;
+    xi <<= shift;
+    uint64_t res0 = (uint64_t)(uint32_t)(xi * arr[0]);
+    const uint64_t res1 = (uint64_t)xi * arr[4];
+    const uint64_t res2 = (uint64_t)xi * arr[8];
+    res0 = (res2 >> 32) | (res0 << 32);
+    res0 += res1;
+    const uint64_t n = (res0 + (1ull << 61)) >> 62;
+    res0 -= n << 62;
+    *np = (int)n;
+    return (double)(int64_t)res0 * 0x1.921fb54442d18p-62;
+}
+BL_HD float bl_sinf(float y) {
+    const uint32_t top = (f2u(y) >> 20) & 0x7ff;
+    double x = (double)y;
+    int n;
+    if (top < (0x3f490fdbu >> 20)) { // |y| < 0.75 (the top 12 bits of pi/4's)
+        if (top < (0x39800000u >> 20)) return y;  // |y| < 2^-12
+        return sc_poly(x, x * x, 0, 1.0);
+    }
+    if (top < (0x42f00000u >> 20)) { // |y| < 120
+        x = sc_reduce_fast(x, &n);
+        const double s = (n & 3) == 1 || (n & 3) == 2 ? -1.0 : 1.0;
+        return sc_poly(x * s, x * x, n, (n & 2) ? -1.0 : 1.0);
+    }
+    if (top < (0x7f800000u >> 20)) {
+        const uint32_t xi = f2u(y);
+        const int sign = (int)(xi >> 31);
+        x = sc_reduce_large(xi, &n);
+        const int m = (n + sign) & 3;
+        const double s = m == 1 || m == 2 ? -1.0 : 1.0;
+        return sc_poly(x * s, x * x, n, (m & 2) ? -1.0 : 1.0);
+    }
+    return (y - y) / (y - y); // inf or NaN: NaN
+}
+BL_HD float bl_cosf(float y) {
+    const uint32_t top = (f2u(y) >> 20) & 0x7ff;
+    double x = (double)y;
+    int n;
+    if (top < (0x3f490fdbu >> 20)) {
+        if (top < (0x39800000u >> 20)) return 1.0f;
+        return sc_poly(x, x * x, 1, 1.0);
+    }
+    if (top < (0x42f00000u >> 20)) {
+        x = sc_reduce_fast(x, &n);
+        const double s = (n & 3) == 1 || (n & 3) == 2 ? -1.0 : 1.0;
+        return sc_poly(x * s, x * x, n ^ 1, (n & 2) ? -1.0 : 1.0);
+    }
+    if (top < (0x7f800000u >> 20)) {
+        const uint32_t xi = f2u(y);
+        const int sign = (int)(xi >> 31);
+        x = sc_reduce_large(xi, &n);
+        const int m = (n + sign) & 3;
+        const double s = m == 1 || m == 2 ? -1.0 : 1.0;
+        return sc_poly(x * s, x * x, n ^ 1, (m & 2) ? -1.0 : 1.0);
+    }
+    return (y - y) / (y - y);
+}
+
+// ---------------------------------------------------------------------------------------------- atanf and atan2f
+// glibc's float atanf / atan2f (sysdeps/ieee754/flt-32 s_atanf.c, e_atan2f.c: fdlibm's, Sun Microsystems 1993,
+// float version by Cygnus): the argument reduced about 0, 1/2, 1, 3/2 or infinity, an 11-term polynomial in float
+// split into odd and even parts, atan(c) in two parts added back. Every operation in float, unfused, as glibc's
+// generic build performs it. Equal to the host libm's on every float input (atanf) and on the pairs
+// tests/test_devmath.py draws (atan2f).
+BL_HD float bl_atanf(float x) {
+    const float atanhi[4] = {4.6364760399e-01f, 7.8539812565e-01f, 9.8279368877e-01f, 1.5707962513e+00f};
+    const float atanlo[4] = {5.0121582440e-09f, 3.7748947079e-08f, 3.4473217170e-08f, 7.5497894159e-08f};
+    const float aT[11] = {3.3333334327e-01f, -2.0000000298e-01f, 1.4285714924e-01f, -1.1111110449e-01f,
+                          9.0908870101e-02f, -7.6918758452e-02f, 6.6610731184e-02f, -5.8335702866e-02f,
+                          4.9768779427e-02f, -3.6531571299e-02f, 1.6285819933e-02f};
+    const uint32_t hx = f2u(x), ix = hx & 0x7fffffffu;
+    int id;
+    if (ix >= 0x4c000000u) { // |x| >= 2^25
+        if (ix > 0x7f800000u) return x + x;
+        return (hx >> 31) ? -atanhi[3] - atanlo[3] : atanhi[3] + atanlo[3];
+    }
+    if (ix < 0x3ee00000u) { // |x| < 0.4375
+        if (ix < 0x31000000u) return x; // |x| < 2^-29
+        id = -1;
+    } else {
+        x = ffabs(x);
+        if (ix < 0x3f980000u) {      // |x| < 1.1875
+            if (ix < 0x3f300000u) {  // 7/16 <= |x| < 11/16
+                id = 0, x = (2.0f * x - 1.0f) / (2.0f + x);
+            } else {                 // 11/16 <= |x| < 19/16
+                id = 1, x = (x - 1.0f) / (x + 1.0f);
+            }
+        } else if (ix < 0x401c0000u) { // |x| < 2.4375
+            id = 2, x = (x - 1.5f) / (1.0f + 1.5f * x);
+        } else {                       // 2.4375 <= |x| < 2^25
+            id = 3, x = -1.0f / x;
+        }
+    }
+    const float z = x * x, w = z * z;
+    const float s1 = z * (aT[0] + w * (aT[2] + w * (aT[4] + w * (aT[6] + w * (aT[8] + w * aT[10])))));
+    const float s2 = w * (aT[1] + w * (aT[3] + w * (aT[5] + w * (aT[7] + w * aT[9]))));
+    if (id < 0) return x - x * (s1 + s2);
+    const float r = atanhi[id] - ((x * (s1 + s2) - atanlo[id]) - x);
+    return (hx >> 31) ? -r : r;
+}
+BL_HD float bl_atan2f(float y, float x) {
+    const float pi_o_2 = 1.5707963705e+00f, pi = 3.1415927410e+00f, pi_lo = -8.7422776573e-08f, tiny = 1.0e-30f;
+    const uint32_t hx = f2u(x), hy = f2u(y), ix = hx & 0x7fffffffu, iy = hy & 0x7fffffffu;
+    if (ix > 0x7f800000u || iy > 0x7f800000u) return x + y; // NaN
+    if (hx == 0x3f800000u) return bl_atanf(y);               // x = 1
+    const int m = (int)((hy >> 31) & 1) | (int)((hx >> 30) & 2);
+    if (iy == 0) {
+        if (m <= 1) return y;
+        return m == 2 ? pi + tiny : -pi - tiny;
+    }
+    if (ix == 0) return (hy >> 31) ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    if (ix == 0x7f800000u) {
+        const float pi_o_4 = 7.8539818525e-01f;
+        if (iy == 0x7f800000u) {
+            switch (m) {
+            case 0: return pi_o_4 + tiny;
+            case 1: return -pi_o_4 - tiny;
+            case 2: return 3.0f * pi_o_4 + tiny;
+            default: return -3.0f * pi_o_4 - tiny;
+            }
+        }
+        switch (m) {
+        case 0: return 0.0f;
+        case 1: return -0.0f;
+        case 2: return pi + tiny;
+        default: return -pi - tiny;
+        }
+    }
+    if (iy == 0x7f800000u) return (hy >> 31) ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    const int k = ((int)iy - (int)ix) >> 23;
+    float z;
+    if (k > 60) z = pi_o_2 + 0.5f * pi_lo;          // |y/x| > 2^60
+    else if ((hx >> 31) && k < -60) z = 0.0f;       // |y|/x < -2^60
+    else z = bl_atanf(ffabs(y / x));
+    switch (m) {
+    case 0: return z;
+    case 1: return -z;
+    case 2: return pi - (z - pi_lo);
+    default: return (z - pi_lo) - pi;
+    }
+}
+
 // The ground models (src/phys/static_world.cpp:85-115: RoR's ground_models.cfg values; alpha 2, strength 1 for all).
 struct ground_model {
     float va, ms, mc, t2, vs, alpha, strength, soft_depth, fluid_drag, fluid_density;

@@ -13,7 +13,10 @@
  *   gather_part (triangles, no contacts)    frame_fem.cpp:916-1086 (6x6 blocks, FP64)
  *   chol6, right_solve_lt, sub_abt, sub_mv, sub_mtv, forward6, backward6   frame_fem.cpp:2193-2317
  *   the factorization and substitution loops   frame_fem.cpp:2899-2943 (here over the nodes' natural order, dense)
- *   the update of v, w, positions, rotations   frame_fem.cpp:3003-3027, 3407-3416
+ *   FemFrame::begin_forces's resync of xd     frame_fem.cpp:1867-1872
+ *   the update (dv as force, clamps, skips)   frame_fem.cpp:2904-2908, 3000-3027
+ *   SoftBody::integrate_nodes                 softbody.cpp:769-795
+ *   FemFrame::sync_positions                  frame_fem.cpp:3407-3422
  * The element stiffness is pinned bit for bit against the compiled reference's FemFrame::tri_stiffness; the rest by
  * closed forms (rigid-body modes, plate theory) and by residuals.
  */
@@ -203,7 +206,10 @@ void orc_fem_tri_local_k(const float Xf[6], float E, float nuf, float tf, float 
 struct orc_fem {
     uint32_t nn, nt;
     double* xd;
+    float *p, *inv_mass, *force; /* the body's Node p and inv_mass; the forces the update hands the integrator */
+    float* impulse;              /* gather_part's held impulse J (3 per node): f (-(h - step)), 0 for a step of h */
     float *v, *w, *q, *mass, *inertia, *pos0;
+    int64_t failures, clamps, nan_resets;
     uint8_t* fixed;
     uint32_t* tn;
     float *X, *r0, *Kf, *tmass, *area0;
@@ -229,6 +235,10 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
     f->nn = nn, f->nt = nt;
     f->xd = (double*)calloc(3 * nn, sizeof(double));
     f->pos0 = (float*)calloc(3 * nn, sizeof(float));
+    f->p = (float*)calloc(3 * nn, sizeof(float));
+    f->inv_mass = (float*)calloc(nn, sizeof(float));
+    f->force = (float*)calloc(3 * nn, sizeof(float));
+    f->impulse = (float*)calloc(3 * nn, sizeof(float));
     f->v = (float*)calloc(3 * nn, sizeof(float));
     f->w = (float*)calloc(3 * nn, sizeof(float));
     f->q = (float*)calloc(4 * nn, sizeof(float));
@@ -248,6 +258,7 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
             const uint32_t id = (uint32_t)(j * (nu + 1) + i);
             const f3 p = add3(add3(origin, mul3(du, (float)i)), mul3(dv, (float)j));
             f->pos0[3 * id] = p.x, f->pos0[3 * id + 1] = p.y, f->pos0[3 * id + 2] = p.z;
+            f->p[3 * id] = p.x, f->p[3 * id + 1] = p.y, f->p[3 * id + 2] = p.z;
             f->xd[3 * id] = p.x, f->xd[3 * id + 1] = p.y, f->xd[3 * id + 2] = p.z;
             f->q[4 * id + 3] = 1.0f;
             f->fixed[id] = (i == 0 || j == 0 || i == nu || j == nv);
@@ -277,6 +288,8 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
                 for (int c = 0; c < 3; c++) f->mass[tri2[k][c]] += m / 3.0f;
             }
         }
+    /* ShellMesher::finish (fem_shell.cpp:48-53): inverse masses, 0 for the clamped nodes */
+    for (uint32_t i = 0; i < nn; i++) f->inv_mass[i] = f->fixed[i] || f->mass[i] <= 0 ? 0.0f : 1.0f / f->mass[i];
     /* finalize: rest shapes, rest frames, stored stiffness (tri_build_k) */
     for (uint32_t k = 0; k < nt; k++) {
         float p[9];
@@ -317,7 +330,7 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
 
 void orc_fem_destroy(orc_fem* f) {
     if (!f) return;
-    free(f->xd), free(f->pos0), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
+    free(f->xd), free(f->pos0), free(f->p), free(f->inv_mass), free(f->force), free(f->impulse), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
     free(f->tn), free(f->X), free(f->r0), free(f->Kf), free(f->tmass), free(f->area0);
     free(f);
 }
@@ -337,10 +350,33 @@ void orc_fem_get_state(const orc_fem* f, double* xd, float* v, float* w, float* 
     if (q) memcpy(q, f->q, 4 * sizeof(float) * f->nn);
 }
 void orc_fem_set_state(orc_fem* f, const double* xd, const float* v, const float* w, const float* q) {
-    if (xd) memcpy(f->xd, xd, 3 * sizeof(double) * f->nn);
+    if (xd) {
+        memcpy(f->xd, xd, 3 * sizeof(double) * f->nn);
+        for (uint32_t i = 0; i < 3 * f->nn; i++) f->p[i] = (float)xd[i]; /* the body's float positions follow */
+    }
     if (v) memcpy(f->v, v, 3 * sizeof(float) * f->nn);
     if (w) memcpy(f->w, w, 3 * sizeof(float) * f->nn);
     if (q) memcpy(f->q, q, 4 * sizeof(float) * f->nn);
+}
+
+/* the nodes pinned (1) or free (0): ShellMesher::finish's inverse masses again (fem_shell.cpp:48-53) */
+void orc_fem_set_fixed(orc_fem* f, const uint8_t* fixed) {
+    for (uint32_t i = 0; i < f->nn; i++) {
+        f->fixed[i] = fixed[i] ? 1 : 0;
+        f->inv_mass[i] = f->fixed[i] || f->mass[i] <= 0 ? 0.0f : 1.0f / f->mass[i];
+    }
+}
+
+/* the body's node records {p, inv_mass, v, mass} (as bl_node) and the frame's counters */
+void orc_fem_get_nodes(const orc_fem* f, float* nodes, float* force, int64_t counters[3]) {
+    if (nodes)
+        for (uint32_t i = 0; i < f->nn; i++) {
+            float* n = &nodes[8 * (size_t)i];
+            n[0] = f->p[3 * i], n[1] = f->p[3 * i + 1], n[2] = f->p[3 * i + 2], n[3] = f->inv_mass[i];
+            n[4] = f->v[3 * i], n[5] = f->v[3 * i + 1], n[6] = f->v[3 * i + 2], n[7] = f->mass[i];
+        }
+    if (force) memcpy(force, f->force, 3 * sizeof(float) * f->nn);
+    if (counters) counters[0] = f->failures, counters[1] = f->clamps, counters[2] = f->nan_resets;
 }
 
 /* ------------------------------------------------------------------ the element pass (eval_tris, elastic) */
@@ -601,36 +637,66 @@ static void put_block(double* A, int n, int r, int c, const double* B) {
         for (int q = 0; q < 6; q++) A[(size_t)(6 * r + p) * 6 * n + 6 * c + q] = B[p * 6 + q];
 }
 
-int orc_fem_block_solve(int n, double* A, double* b) {
-    /* blocks of the lower triangle, right-looking over the columns (frame_fem.cpp:2899-2919), then L L^T x = b
-     * (:2933-2942) */
-    double* L = (double*)malloc(sizeof(double) * 36 * (size_t)n * n);
-    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)n);
-    for (int r = 0; r < n; r++)
-        for (int c = 0; c <= r; c++) get_block(A, n, r, c, &L[((size_t)r * n + c) * 36]);
+/* The block lower triangle in envelope storage: block row r holds its blocks of columns fc[r] .. r (36 doubles each,
+ * from roff[r]); the right-looking factorization keeps its fill inside that envelope. The reference's loops
+ * (frame_fem.cpp:2899-2919, then L L^T x = b, :2933-2942) over the blocks of the natural order; a block outside the
+ * envelope is zero, and its updates (T -= 0) would leave every value as it is, so they are skipped. */
+#define ENV(r, c) (&L[(roff[r] + (size_t)((c) - fc[r])) * 36])
+static int env_solve(int n, const int* fc, const size_t* roff, double* L, double* b) {
+    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)(n ? n : 1));
     int ok = 1;
     for (int k = 0; k < n && ok; k++) {
-        double* Lkk = &L[((size_t)k * n + k) * 36];
+        double* Lkk = ENV(k, k);
         if (!chol6(Lkk, &dinv[6 * k])) { ok = 0; break; }
-        for (int r = k + 1; r < n; r++) right_solve_lt(Lkk, &dinv[6 * k], &L[((size_t)r * n + k) * 36]);
         for (int r = k + 1; r < n; r++)
+            if (fc[r] <= k) right_solve_lt(Lkk, &dinv[6 * k], ENV(r, k));
+        for (int r = k + 1; r < n; r++) {
+            if (fc[r] > k) continue;
             for (int c = k + 1; c <= r; c++)
-                sub_abt(&L[((size_t)r * n + c) * 36], &L[((size_t)r * n + k) * 36], &L[((size_t)c * n + k) * 36]);
+                if (fc[c] <= k) sub_abt(ENV(r, c), ENV(r, k), ENV(c, k));
+        }
     }
     if (ok) {
         for (int k = 0; k < n; k++) {
-            forward6(&L[((size_t)k * n + k) * 36], &dinv[6 * k], &b[6 * k]);
-            for (int r = k + 1; r < n; r++) sub_mv(&L[((size_t)r * n + k) * 36], &b[6 * k], &b[6 * r]);
+            forward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
+            for (int r = k + 1; r < n; r++)
+                if (fc[r] <= k) sub_mv(ENV(r, k), &b[6 * k], &b[6 * r]);
         }
         for (int k = n - 1; k >= 0; k--) {
-            for (int r = k + 1; r < n; r++) sub_mtv(&L[((size_t)r * n + k) * 36], &b[6 * r], &b[6 * k]);
-            backward6(&L[((size_t)k * n + k) * 36], &dinv[6 * k], &b[6 * k]);
+            for (int r = k + 1; r < n; r++)
+                if (fc[r] <= k) sub_mtv(ENV(r, k), &b[6 * r], &b[6 * k]);
+            backward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
         }
-        for (int r = 0; r < n; r++)
-            for (int c = 0; c <= r; c++) put_block(A, n, r, c, &L[((size_t)r * n + c) * 36]);
     }
-    free(L), free(dinv);
+    free(dinv);
     return ok ? 0 : -1;
+}
+
+int orc_fem_block_solve(int n, double* A, double* b) {
+    /* the envelope of A's nonzero lower blocks; the factor written back over A's lower blocks */
+    int* fc = (int*)malloc(sizeof(int) * (size_t)(n ? n : 1));
+    size_t* roff = (size_t*)malloc(sizeof(size_t) * (size_t)(n + 1));
+    size_t tot = 0;
+    for (int r = 0; r < n; r++) {
+        int first = r;
+        for (int c = 0; c < r && first == r; c++) {
+            double B[36];
+            get_block(A, n, r, c, B);
+            for (int q = 0; q < 36; q++)
+                if (B[q] != 0) { first = c; break; }
+        }
+        fc[r] = first, roff[r] = tot, tot += (size_t)(r - first + 1);
+    }
+    roff[n] = tot;
+    double* L = (double*)malloc(sizeof(double) * 36 * (tot ? tot : 1));
+    for (int r = 0; r < n; r++)
+        for (int c = fc[r]; c <= r; c++) get_block(A, n, r, c, ENV(r, c));
+    const int rc = env_solve(n, fc, roff, L, b);
+    if (rc == 0)
+        for (int r = 0; r < n; r++)
+            for (int c = fc[r]; c <= r; c++) put_block(A, n, r, c, ENV(r, c));
+    free(fc), free(roff), free(L);
+    return rc;
 }
 
 /* ------------------------------------------------------------------ the step */
@@ -642,6 +708,13 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
     {
         const float lo = 0.25f + 0.5f * dissipation;
         theta = theta < lo ? lo : (theta > 1.0f ? 1.0f : theta);
+    }
+    /* FemFrame::begin_forces (frame_fem.cpp:1867-1872): a node not where its double position rounds to was moved by
+     * something else; its double position is taken from it */
+    for (uint32_t i = 0; i < nn; i++) {
+        double* x = &f->xd[3 * i];
+        const float* pp = &f->p[3 * i];
+        if ((float)x[0] != pp[0] || (float)x[1] != pp[1] || (float)x[2] != pp[2]) x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
     }
     tri_out* o = (tri_out*)calloc(nt, sizeof(tri_out));
     for (uint32_t k = 0; k < nt; k++) eval_tri(f, k, &o[k]);
@@ -668,68 +741,148 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
             if (fi[i] >= 0) free_map[fi[i]] = i;
     if (nfree_out) *nfree_out = (uint32_t)nf;
     const size_t N6 = 6 * (size_t)nf;
-    double* A = (double*)calloc(N6 * N6, sizeof(double));
-    double* r = (double*)calloc(N6, sizeof(double));
+    /* the envelope: a free node's row from the lowest free node it shares a triangle with */
+    int* fc = (int*)malloc(sizeof(int) * (size_t)(nf ? nf : 1));
+    size_t* roff = (size_t*)malloc(sizeof(size_t) * (size_t)(nf + 1));
+    for (int q = 0; q < nf; q++) fc[q] = q;
+    for (uint32_t k = 0; k < nt; k++)
+        for (int ci = 0; ci < 3; ci++)
+            for (int cj = 0; cj < 3; cj++) {
+                const int a = fi[f->tn[3 * k + ci]], b = fi[f->tn[3 * k + cj]];
+                if (a >= 0 && b >= 0 && b < fc[a]) fc[a] = b;
+            }
+    size_t tot = 0;
+    for (int q = 0; q < nf; q++) roff[q] = tot, tot += (size_t)(q - fc[q] + 1);
+    roff[nf] = tot;
+    double* L = (double*)calloc(36 * (tot ? tot : 1), sizeof(double));
+    double* r = (double*)calloc(N6 ? N6 : 1, sizeof(double));
     const double h2 = (double)theta * h * h, hd = (double)dissipation * h * h;
     const double c = h2 + (double)f->damping * h, cr = hd + (double)f->damping * h;
+    /* node -> (triangle, corner) in triangle order */
+    uint32_t* ncnt = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
+    for (uint32_t k = 0; k < 3 * nt; k++) ncnt[f->tn[k] + 1]++;
+    for (uint32_t i = 0; i < nn; i++) ncnt[i + 1] += ncnt[i];
+    uint32_t* nc = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (nt ? nt : 1));
+    {
+        uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * (nn ? nn : 1));
+        memcpy(fill, ncnt, sizeof(uint32_t) * nn);
+        for (uint32_t k = 0; k < 3 * nt; k++) nc[fill[f->tn[k]]++] = k;
+        free(fill);
+    }
     for (uint32_t i = 0; i < nn; i++) {
         if (fi[i] < 0) continue;
-        double D[36] = {0}, rd[6] = {0};
+        double* D = ENV(fi[i], fi[i]);
+        double rd[6] = {0};
         D[0] = D[7] = D[14] = f->mass[i];
         D[21] = D[28] = D[35] = f->inertia[i];
-        for (uint32_t k = 0; k < nt; k++) {
+        for (uint32_t e = ncnt[i]; e < ncnt[i + 1]; e++) {
+            const uint32_t k = nc[e] / 3, ci = nc[e] % 3;
             if (!o[k].on) continue;
-            for (int ci = 0; ci < 3; ci++)
-                if (f->tn[3 * k + ci] == i) {
-                    tri_block(f, k, &o[k], ci, ci, c, h2, D);
-                    if (cr != 0)
-                        for (int j = 0; j < 6; j++) rd[j] -= cr * o[k].kv[6 * ci + j];
-                }
+            tri_block(f, k, &o[k], (int)ci, (int)ci, c, h2, D);
+            if (cr != 0)
+                for (int j = 0; j < 6; j++) rd[j] -= cr * o[k].kv[6 * ci + j];
         }
-        put_block(A, nf, fi[i], fi[i], D);
         double* rr = &r[6 * (size_t)fi[i]];
-        rr[0] = h * (double)F[i].x, rr[1] = h * (double)F[i].y, rr[2] = h * (double)F[i].z;
+        /* h f - (h - step) f_c + J with no contact forces and step = h; J, then the held impulse f (-(h - step)): +-0,
+         * or NaN after a non-finite force (frame_fem.cpp:1030-1042) */
+        float* J = &f->impulse[3 * i];
+        rr[0] = h * (double)F[i].x - 0.0 * 0.0f + J[0], rr[1] = h * (double)F[i].y - 0.0 * 0.0f + J[1];
+        rr[2] = h * (double)F[i].z - 0.0 * 0.0f + J[2];
+        J[0] = F[i].x * -0.0f, J[1] = F[i].y * -0.0f, J[2] = F[i].z * -0.0f;
         rr[3] = h * (double)T[i].x, rr[4] = h * (double)T[i].y, rr[5] = h * (double)T[i].z;
         for (int j = 0; j < 6; j++) rr[j] += rd[j];
     }
-    /* the off-diagonal blocks: corners (i, j) of every triangle, both free */
+    free(ncnt), free(nc);
+    /* the lower off-diagonal blocks: corners (i, j) of every triangle, both free, the row node later */
     for (uint32_t k = 0; k < nt; k++) {
         if (!o[k].on) continue;
         for (int ci = 0; ci < 3; ci++)
             for (int cj = 0; cj < 3; cj++) {
                 if (ci == cj) continue;
                 const int a = fi[f->tn[3 * k + ci]], b = fi[f->tn[3 * k + cj]];
-                if (a < 0 || b < 0) continue;
-                double B[36];
-                get_block(A, nf, a, b, B);
-                tri_block(f, k, &o[k], ci, cj, c, h2, B);
-                put_block(A, nf, a, b, B);
+                if (a < 0 || b < 0 || a < b) continue;
+                tri_block(f, k, &o[k], ci, cj, c, h2, ENV(a, b));
             }
     }
-    if (Aout) memcpy(Aout, A, sizeof(double) * N6 * N6);
+    if (Aout) { /* dense, the lower blocks and their mirror */
+        memset(Aout, 0, sizeof(double) * N6 * N6);
+        for (int q = 0; q < nf; q++)
+            for (int cc = fc[q]; cc <= q; cc++) {
+                const double* B = ENV(q, cc);
+                put_block(Aout, nf, q, cc, B);
+                if (cc != q)
+                    for (int p2 = 0; p2 < 6; p2++)
+                        for (int q2 = 0; q2 < 6; q2++) Aout[(size_t)(6 * cc + q2) * N6 + 6 * q + p2] = B[p2 * 6 + q2];
+            }
+    }
     if (rhs_out) memcpy(rhs_out, r, sizeof(double) * N6);
-    int rc = orc_fem_block_solve(nf, A, r);
+    int rc = env_solve(nf, fc, roff, L, r);
     if (rc == 0) {
         if (x_out) memcpy(x_out, r, sizeof(double) * N6);
-        /* the new velocities and rotations, then the positions and orientations */
+        /* solve_impl's update (frame_fem.cpp:3000-3027): a clamped node's spin zeroed; a free node's new velocity
+         * handed to the integrator as the force dv m / step, unless the solution is not finite (counted; the node's
+         * forces then act explicitly); |dv| <= 40 m/s and |w| <= 3000 rad/s (counted) */
         for (uint32_t i = 0; i < nn; i++) {
             if (fi[i] < 0) {
                 f->w[3 * i] = f->w[3 * i + 1] = f->w[3 * i + 2] = 0;
                 continue;
             }
             const double* x = &r[6 * (size_t)fi[i]];
-            for (int j = 0; j < 3; j++) f->v[3 * i + j] += (float)x[j], f->w[3 * i + j] += (float)x[3 + j];
-        }
-        for (uint32_t i = 0; i < nn; i++) {
-            if (f->fixed[i]) continue;
-            for (int j = 0; j < 3; j++) f->xd[3 * i + j] += (double)h * f->v[3 * i + j];
-            const f3 wv = F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]);
-            if (dot3(wv, wv) > 0) {
-                const qf qn = qnorm(qmul(qexp(mul3(wv, h)), Q(f->q[4 * i], f->q[4 * i + 1], f->q[4 * i + 2], f->q[4 * i + 3])));
-                f->q[4 * i] = qn.x, f->q[4 * i + 1] = qn.y, f->q[4 * i + 2] = qn.z, f->q[4 * i + 3] = qn.w;
+            f3 dv = F3((float)x[0], (float)x[1], (float)x[2]);
+            const f3 dw = F3((float)x[3], (float)x[4], (float)x[5]);
+            if (!isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
+                f->failures++;
+                continue;
             }
+            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
+            float l = len3(dv);
+            if (l > kMaxDv) dv = mul3(dv, kMaxDv / l), f->clamps++;
+            F[i] = mul3(dv, f->mass[i] / h);
+            f3 wv = add3(F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]), dw);
+            l = len3(wv);
+            if (l > kMaxW) wv = mul3(wv, kMaxW / l), f->clamps++;
+            f->w[3 * i] = wv.x, f->w[3 * i + 1] = wv.y, f->w[3 * i + 2] = wv.z;
         }
+    } else {
+        /* a pivot block not positive definite: the held impulses cleared, the forces act explicitly this once
+         * (:2904-2908) */
+        f->failures++;
+        for (uint32_t i = 0; i < nn; i++)
+            if (fi[i] >= 0) f->impulse[3 * i] = f->impulse[3 * i + 1] = f->impulse[3 * i + 2] = 0;
     }
-    free(o), free(F), free(T), free(fi), free(A), free(r);
+    /* SoftBody::integrate_nodes (softbody.cpp:769-795) over the step */
+    for (uint32_t i = 0; i < nn; i++) {
+        f->force[3 * i] = F[i].x, f->force[3 * i + 1] = F[i].y, f->force[3 * i + 2] = F[i].z;
+        if (!(f->inv_mass[i] > 0)) continue;
+        f3 v = add3(F3(f->v[3 * i], f->v[3 * i + 1], f->v[3 * i + 2]), mul3(F[i], f->inv_mass[i] * h));
+        const float v2 = dot3(v, v);
+        if (v2 > 400.0f * 400.0f) {
+            v = mul3(v, 400.0f / sqrtf(v2));
+        } else if (!(v2 >= 0.0f)) {
+            v = F3(0, 0, 0);
+            f->nan_resets++;
+        }
+        f->v[3 * i] = v.x, f->v[3 * i + 1] = v.y, f->v[3 * i + 2] = v.z;
+        const f3 pn = add3(F3(f->p[3 * i], f->p[3 * i + 1], f->p[3 * i + 2]), mul3(v, h));
+        f->p[3 * i] = pn.x, f->p[3 * i + 1] = pn.y, f->p[3 * i + 2] = pn.z;
+    }
+    /* FemFrame::sync_positions (frame_fem.cpp:3407-3422): x += h v in double, the orientation turned by w h, the
+     * float position taken from the double one where within the float's rounding, else the other way */
+    for (uint32_t i = 0; i < nn; i++) {
+        double* x = &f->xd[3 * i];
+        float* pp = &f->p[3 * i];
+        x[0] += (double)h * f->v[3 * i], x[1] += (double)h * f->v[3 * i + 1], x[2] += (double)h * f->v[3 * i + 2];
+        const f3 wv = F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]);
+        if (dot3(wv, wv) > 0) {
+            const qf qn = qnorm(qmul(qexp(mul3(wv, h)), Q(f->q[4 * i], f->q[4 * i + 1], f->q[4 * i + 2], f->q[4 * i + 3])));
+            f->q[4 * i] = qn.x, f->q[4 * i + 1] = qn.y, f->q[4 * i + 2] = qn.z, f->q[4 * i + 3] = qn.w;
+        }
+        const double tol = 1e-4 * (1.0 + fabsf(pp[0]) + fabsf(pp[1]) + fabsf(pp[2]));
+        if (f->inv_mass[i] > 0 && fabs(x[0] - pp[0]) <= tol && fabs(x[1] - pp[1]) <= tol && fabs(x[2] - pp[2]) <= tol)
+            pp[0] = (float)x[0], pp[1] = (float)x[1], pp[2] = (float)x[2];
+        else
+            x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
+    }
+    free(o), free(F), free(T), free(fi), free(L), free(fc), free(roff), free(r);
     return rc;
 }

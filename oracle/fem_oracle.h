@@ -27,9 +27,15 @@ void orc_fem_topology(const orc_fem* f, uint32_t* tri_nodes, double* xd, uint8_t
 /* the state: v, w (3 per node), q (4 per node: x y z w) */
 void orc_fem_get_state(const orc_fem* f, double* xd, float* v, float* w, float* q);
 void orc_fem_set_state(orc_fem* f, const double* xd, const float* v, const float* w, const float* q);
+/* the nodes pinned (1) or free (0), their inverse masses again (ShellMesher::finish) */
+void orc_fem_set_fixed(orc_fem* f, const uint8_t* fixed);
+/* the body's node records {p, inv_mass, v, mass} (8 floats each, as bl_node), the forces the last step handed the
+ * integrator (3 per node), counters {solve failures, clamps, NaN resets}; any pointer may be null */
+void orc_fem_get_nodes(const orc_fem* f, float* nodes, float* force, int64_t counters[3]);
 
-/* One implicit step of h: element pass, gather, assembly, block Cholesky (chol6 / right_solve_lt / sub_abt in the
- * reference's right-looking order over the nodes' natural order, dense), substitution, update. loads: 3 per node
+/* One substep of h of the plate body: element pass, gather, assembly, block Cholesky (chol6 / right_solve_lt /
+ * sub_abt in the reference's right-looking order over the nodes' natural order, dense), substitution, the update
+ * (velocities as forces), the body's integration and the frame's position sync. loads: 3 per node
  * (N), added to gravity m g. If `A` is non-null it receives the assembled system of the free nodes in the natural
  * order: dense (nf*6)^2 row-major, `rhs` nf*6, `x` the solution; `free_map` the free nodes. Returns 0, or -1 when a
  * pivot block is not positive definite. */

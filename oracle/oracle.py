@@ -73,6 +73,8 @@ def oracle_lib():
             "fem_topology": (None, [_P, _P, _P, _P, _P, _P]),
             "fem_get_state": (None, [_P, _P, _P, _P, _P]),
             "fem_set_state": (None, [_P, _P, _P, _P, _P]),
+            "fem_get_nodes": (None, [_P, _P, _P, _P]),
+            "fem_set_fixed": (None, [_P, _P]),
             "fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P,
                                         ctypes.POINTER(ctypes.c_uint32)]),
             "fem_block_solve": (ctypes.c_int, [ctypes.c_int, _P, _P]),
@@ -101,9 +103,12 @@ def ref_lib():
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
             "tri_stiffness": (ctypes.c_int, [_P, _P, _P, _P]),
-            "fem_plate_new": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_plate_new": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P, ctypes.c_int]),
             "fem_plate_free": (None, [_P]),
             "fem_plate_step": (ctypes.c_int64, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_plate_substep": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+            "fem_plate_read": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+            "fem_plate_write": (ctypes.c_int, [_P, _P, _P, _P, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -264,10 +269,10 @@ class RefPlate:
     """The compiled reference's clamped plate (ShellMesher::grid) and its implicit step as its fembench times it:
     clear_forces(g) plus the loads, FemFrame::compute_forces, FemFrame::solve (one thread)."""
 
-    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3):
+    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3, clamped=True):
         self._lib = ref_lib()
         sec = np.asarray(section, np.float32)
-        self._h = self._lib.blref_fem_plate_new(int(nu), int(nv), float(size), float(y0), _ptr(sec))
+        self._h = self._lib.blref_fem_plate_new(int(nu), int(nv), float(size), float(y0), _ptr(sec), int(bool(clamped)))
         self.nn = (nu + 1) * (nv + 1)
 
     def step(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
@@ -275,6 +280,40 @@ class RefPlate:
         ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
         gg = np.asarray(g, np.float32)
         return self._lib.blref_fem_plate_step(self._h, _ptr(ld), h, theta, dissipation, _ptr(gg))
+
+    def substep(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """The plate body's whole substep as the reference's world runs it: the step, then integrate_nodes and
+        sync_positions."""
+        ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
+        gg = np.asarray(g, np.float32)
+        self._lib.blref_fem_plate_substep(self._h, _ptr(ld), h, theta, dissipation, _ptr(gg))
+
+    def read(self):
+        """{nodes (bl_node records: p, inv_mass, v, mass), xd, w, q, force, counters (failures, clamps)}."""
+        nodes = np.zeros((self.nn, 8), np.float32)
+        xd = np.zeros((self.nn, 3), np.float64)
+        w = np.zeros((self.nn, 3), np.float32)
+        q = np.zeros((self.nn, 4), np.float32)
+        force = np.zeros((self.nn, 3), np.float32)
+        cnt = np.zeros(2, np.int64)
+        if self._lib.blref_fem_plate_read(self._h, _ptr(nodes), _ptr(xd), _ptr(w), _ptr(q), _ptr(force), _ptr(cnt)) != self.nn:
+            raise RuntimeError("the reference plate's frame nodes are not the body's")
+        return dict(p=nodes[:, 0:3].copy(), inv_mass=nodes[:, 3].copy(), v=nodes[:, 4:7].copy(), mass=nodes[:, 7].copy(),
+                    xd=xd, w=w, q=q, force=force, failures=int(cnt[0]), clamps=int(cnt[1]))
+
+    def write(self, p=None, v=None, xd=None, w=None, q=None):
+        """Overwrites parts of the state (the nodes' p and v, the frame's xd, w, q)."""
+        nodes = None
+        if p is not None or v is not None:
+            nodes = np.zeros((self.nn, 8), np.float32)
+            self._lib.blref_fem_plate_read(self._h, _ptr(nodes), None, None, None, None, None)
+            if p is not None:
+                nodes[:, 0:3] = p
+            if v is not None:
+                nodes[:, 4:7] = v
+        cv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        self._lib.blref_fem_plate_write(self._h, _ptr(nodes), _ptr(cv(xd, np.float64)), _ptr(cv(w, np.float32)),
+                                        _ptr(cv(q, np.float32)))
 
     def close(self):
         if self._h:
@@ -291,12 +330,14 @@ class RefPlate:
 class FemPlate:
     """The oracle's plate (ShellMesher::grid) and its implicit step (see fem_oracle.c)."""
 
-    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3):
+    def __init__(self, nu, nv, size=1.0, y0=0.0, section=SECTION_C3, clamped=True):
         self._lib = oracle_lib()
         sec = np.asarray(section, np.float32)
         self._h = self._lib.orc_fem_plate(int(nu), int(nv), float(size), float(y0), _ptr(sec))
         self.nn = self._lib.orc_fem_nodes(self._h)
         self.nt = self._lib.orc_fem_tris(self._h)
+        if not clamped:
+            self._lib.orc_fem_set_fixed(self._h, _ptr(np.zeros(self.nn, np.uint8)))
 
     def topology(self):
         tn = np.zeros((self.nt, 3), np.uint32)
@@ -315,12 +356,24 @@ class FemPlate:
         self._lib.orc_fem_get_state(self._h, _ptr(xd), _ptr(v), _ptr(w), _ptr(q))
         return xd, v, w, q
 
+    def read(self):
+        """The same record as RefPlate.read: the body's nodes, the frame's xd, w, q, the forces, the counters."""
+        nodes = np.zeros((self.nn, 8), np.float32)
+        force = np.zeros((self.nn, 3), np.float32)
+        cnt = np.zeros(3, np.int64)
+        self._lib.orc_fem_get_nodes(self._h, _ptr(nodes), _ptr(force), _ptr(cnt))
+        xd, v, w, q = self.state()
+        return dict(p=nodes[:, 0:3].copy(), inv_mass=nodes[:, 3].copy(), v=nodes[:, 4:7].copy(), mass=nodes[:, 7].copy(),
+                    xd=xd, w=w, q=q, force=force, failures=int(cnt[0]), clamps=int(cnt[1]), nan_resets=int(cnt[2]))
+
     def set_state(self, xd, v, w, q):
         self._lib.orc_fem_set_state(self._h, *[_ptr(np.ascontiguousarray(a, t)) for a, t in
                                               ((xd, np.float64), (v, np.float32), (w, np.float32), (q, np.float32))])
 
     def step(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0), system=False):
         """One implicit step; with system=True also (A, rhs, x, free nodes) of the step's linear system."""
+        if loads is None:
+            loads = np.zeros((self.nn, 3), np.float32)
         loads = np.ascontiguousarray(loads, np.float32).reshape(self.nn, 3)
         gg = np.asarray(g, np.float32)
         nf = ctypes.c_uint32(0)

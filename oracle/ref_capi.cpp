@@ -244,11 +244,11 @@ int blref_tri_stiffness(const float p[9], const float sec[6], double* K, float* 
     return 0;
 }
 
-// The clamped plate of the implicit-FEM workload built the reference's way (ShellMesher::grid, src/phys/fem_shell.cpp:
+// The clamped plate of the implicit-FEM workload (clamped = 0: its border free) built the reference's way (ShellMesher::grid, src/phys/fem_shell.cpp:
 // 34-46; border nodes NF_FIXED; ShellMesher::finish; FemFrame::finalize) and its implicit step as the reference's
 // fembench times it (tests/test_physics.cpp:3486-3494): clear_forces(g) plus the loads, FemFrame::compute_forces,
 // FemFrame::solve. Positions are not advanced (the world's integration does that in the reference).
-void* blref_fem_plate_new(int nu, int nv, float size, float y0, const float sec[6]) {
+void* blref_fem_plate_new(int nu, int nv, float size, float y0, const float sec[6], int clamped) {
     auto* b = new SoftBody();
     ShellSection s;
     s.E = sec[0], s.nu = sec[1], s.rho = sec[2], s.t = sec[3], s.drill = sec[4], s.damping = sec[5];
@@ -258,7 +258,7 @@ void* blref_fem_plate_new(int nu, int nv, float size, float y0, const float sec[
     const std::vector<uint32_t> id = m.grid(origin, du, dv, nu, nv, si, false);
     for (int j = 0; j <= nv; j++)
         for (int i = 0; i <= nu; i++)
-            if (i == 0 || j == 0 || i == nu || j == nv) b->info[id[(size_t)j * (nu + 1) + i]].flags |= NF_FIXED;
+            if (clamped && (i == 0 || j == 0 || i == nu || j == nv)) b->info[id[(size_t)j * (nu + 1) + i]].flags |= NF_FIXED;
     m.finish();
     b->finalize();
     b->fem.finalize(*b);
@@ -276,6 +276,59 @@ int64_t blref_fem_plate_step(void* h, const float* loads, float dt, float theta,
     b.fem.compute_forces(b);
     b.fem.solve(b, dt, dt, theta, dissipation);
     return (int64_t)b.fem.solve_failures;
+}
+
+// The plate body's whole substep as World::simulate_island runs a frame body stepping every substep
+// (src/phys/world.cpp:3265-3269, 3573-3580, 3812-3822, 3876-3886): clear_forces(g) plus the loads,
+// FemFrame::compute_forces, FemFrame::solve (its update writes the new velocities as forces, :3000-3027), then
+// SoftBody::integrate_nodes (src/phys/softbody.cpp:769-795) and FemFrame::sync_positions (:3407-3422).
+int blref_fem_plate_substep(void* h, const float* loads, float dt, float theta, float dissipation, const float g[3]) {
+    SoftBody& b = *static_cast<SoftBody*>(h);
+    b.clear_forces(vec3(g[0], g[1], g[2]));
+    if (loads)
+        for (size_t i = 0; i < b.nodes.size(); i++) b.force[i] += vec3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]);
+    b.fem.compute_forces(b);
+    b.fem.solve(b, dt, dt, theta, dissipation);
+    vec3 mn(1e30f), mx(-1e30f);
+    float max_v2 = 0;
+    b.integrate_nodes(0, b.nodes.size(), dt, mn, mx, max_v2);
+    b.fem.sync_positions(b, dt);
+    return 0;
+}
+
+// The plate's state: the body's nodes (p, inv_mass, v, mass), the frame's double positions, spins and orientations
+// (scattered to the body's node order: the frame numbers its nodes as the triangles first name them), the forces the
+// update left (3 per node), and the counters {solve_failures, clamps}. Any pointer may be null. Returns the frame's
+// node count.
+int blref_fem_plate_read(void* h, bl_node* nodes, double* xd, float* w, float* q, float* force, int64_t* counters) {
+    const SoftBody& b = *static_cast<SoftBody*>(h);
+    const FemFrame& f = b.fem;
+    if (nodes) std::memcpy(nodes, b.nodes.data(), sizeof(Node) * b.nodes.size());
+    for (size_t i = 0; i < f.node.size(); i++) {
+        const size_t n = f.node[i];
+        if (xd) xd[3 * n] = f.xd[3 * i], xd[3 * n + 1] = f.xd[3 * i + 1], xd[3 * n + 2] = f.xd[3 * i + 2];
+        if (w) w[3 * n] = f.w[i].x, w[3 * n + 1] = f.w[i].y, w[3 * n + 2] = f.w[i].z;
+        if (q) q[4 * n] = f.q[i].x, q[4 * n + 1] = f.q[i].y, q[4 * n + 2] = f.q[i].z, q[4 * n + 3] = f.q[i].w;
+    }
+    if (force)
+        for (size_t i = 0; i < b.force.size(); i++) force[3 * i] = b.force[i].x, force[3 * i + 1] = b.force[i].y, force[3 * i + 2] = b.force[i].z;
+    if (counters) counters[0] = f.solve_failures, counters[1] = f.clamps;
+    return (int)f.node.size();
+}
+
+// The plate's state written (nodes as bl_node; the frame's xd, w, q in the body's node order); null pointers leave a
+// part as it is.
+int blref_fem_plate_write(void* h, const bl_node* nodes, const double* xd, const float* w, const float* q) {
+    SoftBody& b = *static_cast<SoftBody*>(h);
+    FemFrame& f = b.fem;
+    if (nodes) std::memcpy(b.nodes.data(), nodes, sizeof(Node) * b.nodes.size());
+    for (size_t i = 0; i < f.node.size(); i++) {
+        const size_t n = f.node[i];
+        if (xd) f.xd[3 * i] = xd[3 * n], f.xd[3 * i + 1] = xd[3 * n + 1], f.xd[3 * i + 2] = xd[3 * n + 2];
+        if (w) f.w[i] = vec3(w[3 * n], w[3 * n + 1], w[3 * n + 2]);
+        if (q) f.q[i] = quat(q[4 * n], q[4 * n + 1], q[4 * n + 2], q[4 * n + 3]);
+    }
+    return 0;
 }
 
 } // extern "C"

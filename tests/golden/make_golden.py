@@ -9,6 +9,9 @@ Outputs (data only: inputs are regenerated from the seeds, outputs are the refer
     cases_ref.npz       every other case: sha256 of the final node and beam records, broken counts
     fem_ref.npz         the shell element: the reference's FemFrame::tri_stiffness (K, 18 x 18 FP64, and the rest
                         shape X) of 24 seeded triangles under three sections
+    fem_step_ref.npz    the plate body's whole substep (compute_forces, solve, integrate_nodes, sync_positions) of the
+                        scenarios of tests/_fem_cases.py: the update-path scenario after every step, the spun plate
+                        every 100 steps (nodes' p and v, xd, w, q, forces, counters)
 """
 import hashlib
 import os
@@ -20,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
+import _fem_cases as fc  # noqa: E402
 from _cases import CPU_CASES, run_world  # noqa: E402
 from _contact_cases import CONTACT_CASES  # noqa: E402
 from oracle import oracle  # noqa: E402
@@ -62,6 +66,17 @@ def main():
     np.savez_compressed(os.path.join(HERE, "fem_ref.npz"), p=np.array(P), K=np.array(K), X=np.array(X),
                         section=np.array([(i % 3) for i in range(24)]), sections=sections)
     print("written fem_ref.npz")
+    out = {"spun_steps": np.array(400)}
+    for name, sc, every in (("events", fc.events(), 1), ("spun", fc.spinning(400), 100)):
+        R = fc.make(oracle, sc, ref=True)
+        for s, st in fc.apply(R, sc, ref=True):
+            if s % every == every - 1:
+                for k in fc.STATE_KEYS:
+                    out[f"{name}_{s}_{k}"] = st[k]
+                out[f"{name}_{s}_counters"] = np.array([st["failures"], st["clamps"]])
+        R.close()
+    np.savez_compressed(os.path.join(HERE, "fem_step_ref.npz"), **out)
+    print("written fem_step_ref.npz")
 
 
 if __name__ == "__main__":

@@ -115,3 +115,84 @@ def test_tri_stiffness_matches_golden():
         X = oracle.fem_rest_shape(g["p"][i])
         assert X.tobytes() == g["X"][i].astype(np.float32).tobytes()
         assert oracle.fem_tri_local_k(X, sec).tobytes() == g["K"][i].tobytes(), f"triangle {i}"
+
+
+# ---------------------------------------------------------------------------------------------- the whole substep
+import _fem_cases as fc  # noqa: E402
+
+
+def _same(a, b, what):
+    for k in fc.STATE_KEYS:
+        assert a[k].tobytes() == b[k].tobytes(), f"{what}: {k} differs from the reference"
+    assert (a["failures"], a["clamps"]) == (b["failures"], b["clamps"]), f"{what}: counters"
+
+
+@pytest.mark.parametrize("n,steps", [(8, 24), (16, 20), (32, 20)])
+def test_fem_substep_matches_reference_bitwise(ref_mod, n, steps):
+    """The oracle's plate substep - element pass, assembly, factorization and substitution, the update (the new
+    velocities as forces), integrate_nodes, sync_positions - against the compiled reference's, bit for bit: the
+    nodes' p and v, the frame's double positions, spins and orientations, the forces, the counters. theta 0.5,
+    dissipation 0.02, the section's damping, gravity and random loads."""
+    sc = fc.loaded(n, steps, 20.0, n)
+    P, R = fc.make(ref_mod, sc), fc.make(ref_mod, sc, ref=True)
+    moved = 0.0
+    for (s, a), (_, b) in zip(fc.apply(R, sc, ref=True), fc.apply(P, sc)):
+        _same(a, b, f"{n}x{n} step {s}")
+        moved = max(moved, float(np.abs(b["v"]).max()))
+    assert moved > 1e-3
+
+
+def test_fem_update_paths_match_reference_bitwise(ref_mod):
+    """The update's safety paths bit for bit against the reference: velocity changes beyond 40 m/s clamped and
+    counted, a non-finite solution skipped and counted with the forces then applied explicitly, the NaN velocity reset,
+    and the held impulse carrying the NaN into the next step."""
+    sc = fc.events()
+    P, R = fc.make(ref_mod, sc), fc.make(ref_mod, sc, ref=True)
+    for (s, a), (_, b) in zip(fc.apply(R, sc, ref=True), fc.apply(P, sc)):
+        _same(a, b, f"step {s}")
+    assert b["clamps"] > 0 and b["failures"] == 2 * 49 and b["nan_resets"] == 1
+
+
+def test_spun_plate_keeps_energy_and_matches_reference(ref_mod):
+    """The reference's spun free plate (tests/test_physics.cpp:1615-1640): after a second at 2 kHz its kinetic energy
+    is within 2 % of the start, with no failures or clamps; and the oracle follows the reference bit for bit."""
+    sc = fc.spinning(2000)
+    P, R = fc.make(ref_mod, sc), fc.make(ref_mod, sc, ref=True)
+    inertia = P.topology()[4]
+    k0 = fc.kinetic(P.read(), inertia)
+    for (s, a), (_, b) in zip(fc.apply(R, sc, ref=True), fc.apply(P, sc)):
+        if s % 250 == 249:
+            _same(a, b, f"spun plate step {s}")
+    k1 = fc.kinetic(b, inertia)
+    assert abs(k1 / k0 - 1) < 0.02 and b["failures"] == 0 and b["clamps"] == 0, (k0, k1)
+
+
+def test_spun_plate_keeps_energy():
+    """The same closed form on the oracle alone (where the reference is absent)."""
+    sc = fc.spinning(2000)
+    P = fc.make(oracle, sc)
+    inertia = P.topology()[4]
+    k0 = fc.kinetic(P.read(), inertia)
+    for s, b in fc.apply(P, sc):
+        pass
+    k1 = fc.kinetic(b, inertia)
+    assert abs(k1 / k0 - 1) < 0.02 and b["failures"] == 0 and b["clamps"] == 0, (k0, k1)
+    # the plate kept its shape: every node's distance from the centre of mass as at the start, within 1 mm
+    p0 = P.topology()[1]
+    d0 = np.linalg.norm(p0 - p0.mean(0), axis=1)
+    d1 = np.linalg.norm(b["xd"] - b["xd"].mean(0), axis=1)
+    assert np.abs(d1 - d0).max() < 1e-3
+
+
+def test_fem_substeps_match_golden():
+    """The oracle against the reference's states recorded in tests/golden/fem_step_ref.npz (the update-path scenario
+    and the spun plate), bit for bit: runs where the reference is absent."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_step_ref.npz"))
+    for name, sc in (("events", fc.events()), ("spun", fc.spinning(int(g["spun_steps"])))):
+        P = fc.make(oracle, sc)
+        for s, b in fc.apply(P, sc):
+            if f"{name}_{s}_p" in g.files:
+                for k in fc.STATE_KEYS:
+                    assert b[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
+                assert [b["failures"], b["clamps"]] == list(g[f"{name}_{s}_counters"]), f"{name} step {s}: counters"
