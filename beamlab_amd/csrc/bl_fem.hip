@@ -81,7 +81,7 @@ __device__ f3 qlog(qf qin) {
     const f3 v = mk(q.x, q.y, q.z);
     const float s = blg::length(v);
     if (s < 1e-8f) return blg::mul(v, 2.0f);
-    const float angle = 2.0f * atan2f(s, q.w);
+    const float angle = 2.0f * blg::bl_atan2f(s, q.w);
     return blg::mul(v, angle / s);
 }
 __device__ qf qexp(f3 rv) {
@@ -90,8 +90,8 @@ __device__ qf qexp(f3 rv) {
     const f3 d = blg::divs(rv, a);
     const float l = blg::length(d);
     const f3 axis = l > 1e-12f ? blg::mul(d, 1.0f / l) : mk(0, 0, 0);
-    const float hh = a * 0.5f, s = sinf(hh);
-    return Q(axis.x * s, axis.y * s, axis.z * s, cosf(hh));
+    const float hh = a * 0.5f, s = blg::bl_sinf(hh);
+    return Q(axis.x * s, axis.y * s, axis.z * s, blg::bl_cosf(hh));
 }
 
 // ------------------------------------------------------------------------------------------ the element, FP64
@@ -203,6 +203,11 @@ struct FemArgs {
     float *v, *w;
     float4* q;
     const float *mass, *inertia;
+    float* pf;           // the body's float positions (Node::p), 3 per node
+    const float* invm;   // Node::inv_mass: 0 for a pinned node
+    float* fnode;        // the node forces the step gathered (3 per node); the update's forces for the integrator
+    float* imp;          // gather_part's held impulse J (3 per node)
+    unsigned long long* cnt; // solve failures (not finite), clamps, NaN velocity resets
     const uint8_t* fixed;
     const uint4* tri;   // corners, 0
     const float* X;     // 6 per triangle
@@ -419,10 +424,17 @@ __global__ void k_nodes(FemArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nn) return;
     const int fi = a.fidx[i];
-    if (fi < 0) return;
     f3 F = blg::mul(mk(a.g[0], a.g[1], a.g[2]), a.mass[i]);
     if (a.loads) F = blg::add(F, mk(a.loads[3 * i], a.loads[3 * i + 1], a.loads[3 * i + 2]));
     f3 T = mk(0, 0, 0);
+    if (fi < 0) { // (a pinned node: its forces only, for the record)
+        for (uint32_t k = a.nc_off[i]; k < a.nc_off[i + 1]; k++) {
+            const uint32_t t = a.nc[k] / 3, corner = a.nc[k] % 3;
+            if (a.ton[t]) F = blg::add(F, mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]));
+        }
+        a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
+        return;
+    }
     const double h = a.h;
     const double h2 = (double)a.theta * a.h * a.h, hd = (double)a.dissipation * a.h * a.h;
     const double c = h2 + (double)a.sec.damping * h, cr = hd + (double)a.sec.damping * h;
@@ -443,9 +455,15 @@ __global__ void k_nodes(FemArgs a) {
     double* Dd = a.Dsys + 36 * (size_t)fi;
     for (int k = 0; k < 36; k++) Dd[k] = D[k];
     double* r = a.rhs + 6 * (size_t)fi;
-    r[0] = h * (double)F.x, r[1] = h * (double)F.y, r[2] = h * (double)F.z;
+    // h f - (h - step) f_c + J, no contact forces, step = h; then the held impulse J = f (-(h - step)): +-0, or NaN
+    // after a non-finite force, carried into the next step (src/phys/frame_fem.cpp:1030-1042)
+    float* J = a.imp + 3 * i;
+    r[0] = h * (double)F.x - 0.0 * 0.0f + J[0], r[1] = h * (double)F.y - 0.0 * 0.0f + J[1];
+    r[2] = h * (double)F.z - 0.0 * 0.0f + J[2];
+    J[0] = F.x * -0.0f, J[1] = F.y * -0.0f, J[2] = F.z * -0.0f;
     r[3] = h * (double)T.x, r[4] = h * (double)T.y, r[5] = h * (double)T.z;
     for (int j = 0; j < 6; j++) r[j] += rd[j];
+    a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
 }
 
 // Per pair of free nodes sharing a triangle: the lower block (later node's row, earlier node's column), its
@@ -1089,23 +1107,74 @@ __global__ void k_sum_parts(const double* part, uint32_t n, double* out) {
     out[0] = a, out[1] = b;
 }
 
-// the new velocities and rotations, then positions and orientations (src/phys/frame_fem.cpp:3003-3027, 3407-3416)
-__global__ void k_update(FemArgs a, const double* x) {
+// The end of the substep, per node:
+//  - solve_impl's update (src/phys/frame_fem.cpp:3000-3027): a pinned node's spin zeroed; a free node's new velocity
+//    handed to the integrator as the force dv m / step, unless the solution is not finite (counted; the node's gathered
+//    forces then act explicitly), |dv| <= 40 m/s and |w| <= 3000 rad/s (counted). After a factorization that failed
+//    (explicit_step) every node's gathered forces act and the held impulses are cleared (:2904-2908);
+//  - SoftBody::integrate_nodes (src/phys/softbody.cpp:769-795): v += f inv_mass dt, the 400 m/s clamp, a NaN velocity
+//    reset (counted), p += v dt;
+//  - FemFrame::sync_positions (:3407-3422): x += h v in double, the orientation turned by w h, the float position
+//    taken from the double one when within 1e-4 (1 + |p|), else the double one from it. (So (float)x == p after every
+//    step, and FemFrame::begin_forces's resync of x from p, :1867-1872, changes nothing: the host's set_state keeps
+//    the two in step.)
+__global__ void k_update(FemArgs a, const double* x, int explicit_step) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (explicit_step && i == 0) atomicAdd(&a.cnt[0], 1ull);
     if (i >= a.nn) return;
     const int fi = a.fidx[i];
+    f3 v = mk(a.v[3 * i], a.v[3 * i + 1], a.v[3 * i + 2]);
+    f3 w = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
+    f3 F = mk(a.fnode[3 * i], a.fnode[3 * i + 1], a.fnode[3 * i + 2]);
     if (fi < 0) {
-        a.w[3 * i] = a.w[3 * i + 1] = a.w[3 * i + 2] = 0;
-        return;
+        w = mk(0, 0, 0);
+    } else if (explicit_step) {
+        a.imp[3 * i] = a.imp[3 * i + 1] = a.imp[3 * i + 2] = 0;
+    } else {
+        const double* xs = x + 6 * (size_t)fi;
+        f3 dv = mk((float)xs[0], (float)xs[1], (float)xs[2]);
+        const f3 dw = mk((float)xs[3], (float)xs[4], (float)xs[5]);
+        if (!__builtin_isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
+            atomicAdd(&a.cnt[0], 1ull);
+        } else {
+            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
+            float l = blg::length(dv);
+            if (l > kMaxDv) dv = blg::mul(dv, kMaxDv / l), atomicAdd(&a.cnt[1], 1ull);
+            F = blg::mul(dv, a.mass[i] / a.h);
+            w = blg::add(w, dw);
+            l = blg::length(w);
+            if (l > kMaxW) w = blg::mul(w, kMaxW / l), atomicAdd(&a.cnt[1], 1ull);
+        }
+        a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
     }
-    for (int j = 0; j < 3; j++) a.v[3 * i + j] += (float)x[6 * fi + j], a.w[3 * i + j] += (float)x[6 * fi + 3 + j];
-    for (int j = 0; j < 3; j++) a.xd[3 * i + j] += (double)a.h * a.v[3 * i + j];
-    const f3 wv = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
-    if (blg::dot(wv, wv) > 0) {
+    a.w[3 * i] = w.x, a.w[3 * i + 1] = w.y, a.w[3 * i + 2] = w.z;
+    const float im = a.invm[i], h = a.h;
+    f3 p = mk(a.pf[3 * i], a.pf[3 * i + 1], a.pf[3 * i + 2]);
+    if (im > 0) {
+        v = blg::add(v, blg::mul(F, im * h));
+        const float v2 = blg::dot(v, v);
+        if (v2 > 400.0f * 400.0f) {
+            v = blg::mul(v, 400.0f / sqrtf(v2));
+        } else if (!(v2 >= 0.0f)) {
+            v = mk(0, 0, 0);
+            atomicAdd(&a.cnt[2], 1ull);
+        }
+        p = blg::add(p, blg::mul(v, h));
+        a.v[3 * i] = v.x, a.v[3 * i + 1] = v.y, a.v[3 * i + 2] = v.z;
+    }
+    double* xd = a.xd + 3 * (size_t)i;
+    xd[0] += (double)h * v.x, xd[1] += (double)h * v.y, xd[2] += (double)h * v.z;
+    if (blg::dot(w, w) > 0) {
         const float4 q0 = a.q[i];
-        const qf qn = qnorm(qmul(qexp(blg::mul(wv, a.h)), Q(q0.x, q0.y, q0.z, q0.w)));
+        const qf qn = qnorm(qmul(qexp(blg::mul(w, h)), Q(q0.x, q0.y, q0.z, q0.w)));
         a.q[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
     }
+    const double tol = 1e-4 * (1.0 + fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
+    if (im > 0 && fabs(xd[0] - p.x) <= tol && fabs(xd[1] - p.y) <= tol && fabs(xd[2] - p.z) <= tol)
+        p = mk((float)xd[0], (float)xd[1], (float)xd[2]);
+    else
+        xd[0] = p.x, xd[1] = p.y, xd[2] = p.z;
+    a.pf[3 * i] = p.x, a.pf[3 * i + 1] = p.y, a.pf[3 * i + 2] = p.z;
 }
 
 } // namespace blf
@@ -1163,6 +1232,8 @@ struct bl_fem {
     // state and topology
     Dv<double> xd;
     Dv<float> v, w, mass, inertia, X, Kf, te, tfo, tto, tg, loads;
+    Dv<float> pf, invm, fnode, imp; // the body's node positions and inverse masses, the forces, the held impulses
+    Dv<unsigned long long> cnt;     // solve failures, clamps, NaN resets
     Dv<float4> q, r0;
     Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, xprev, part, sums;
     Dv<float> p0f;     // rest positions of the free nodes (3 each)
@@ -1218,6 +1289,7 @@ struct bl_fem {
         FemArgs a{};
         a.nn = nn, a.nt = nt, a.nf = nf, a.sec = sec;
         a.xd = xd.p, a.v = v.p, a.w = w.p, a.q = q.p, a.mass = mass.p, a.inertia = inertia.p, a.fixed = fixed.p;
+        a.pf = pf.p, a.invm = invm.p, a.fnode = fnode.p, a.imp = imp.p, a.cnt = cnt.p;
         a.tri = tri.p, a.X = X.p, a.r0 = r0.p, a.Kf = Kf.p, a.Kd = Kd.p;
         a.te = te.p, a.tfo = tfo.p, a.tto = tto.p, a.tg = tg.p, a.tkv = tkv.p, a.ton = ton.p;
         a.nc_off = nc_off.p, a.nc = nc.p, a.fidx = fidx.p, a.ford = ford.p;
@@ -1610,7 +1682,14 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     std::vector<float4> q(nn, make_float4(0, 0, 0, 1));
     std::vector<uint8_t> fixed(nn, 0);
     if (d->fixed) std::memcpy(fixed.data(), d->fixed, nn);
+    // the body's Node::p and inv_mass (ShellMesher::finish, src/phys/fem_shell.cpp:48-53: 0 for a pinned node)
+    std::vector<float> pf(d->p, d->p + 3 * (size_t)nn), invm(nn);
+    for (uint32_t i = 0; i < nn; i++) invm[i] = fixed[i] || mass[i] <= 0 ? 0.0f : 1.0f / mass[i];
     int rc;
+    if ((rc = f->pf.put(pf)) || (rc = f->invm.put(invm)) || (rc = f->fnode.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
+        (rc = f->imp.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
+        (rc = f->cnt.put(std::vector<unsigned long long>(3, 0ull))))
+        return rc;
     if ((rc = f->xd.put(xd)) || (rc = f->v.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
         (rc = f->w.put(std::vector<float>(3 * (size_t)nn, 0.0f))) || (rc = f->q.put(q)) || (rc = f->mass.put(mass)) ||
         (rc = f->inertia.put(inertia)) || (rc = f->fixed.put(fixed)) || (rc = f->tri.put(tri)) || (rc = f->X.put(X)) ||
@@ -1785,10 +1864,10 @@ void fem_spmv(bl_fem* f, const double* x, double* y) {
     hipLaunchKernelGGL(k_spmv, dim3((f->nf + 255) / 256), dim3(256), 0, f->st, f->step_args, x, f->adj_off.p, f->adj.p, y);
 }
 
-int fem_update(bl_fem* f, float* ms) {
+int fem_update(bl_fem* f, float* ms, bool explicit_step) {
     const hipStream_t st = f->st;
     FCHK(hipEventRecord(f->ev[3], st));
-    hipLaunchKernelGGL(k_update, dim3((f->nn + 255) / 256), dim3(256), 0, st, f->step_args, f->x.p);
+    hipLaunchKernelGGL(k_update, dim3((f->nn + 255) / 256), dim3(256), 0, st, f->step_args, f->x.p, explicit_step ? 1 : 0);
     FCHK(hipGetLastError());
     FCHK(hipEventRecord(f->ev[4], st));
     FCHK(hipEventSynchronize(f->ev[4]));
@@ -1844,17 +1923,17 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
     if ((rc = blf::fem_assemble(f, loads, h, theta, dissipation, g, &msa))) return rc;
     if ((rc = blf::fem_cholesky(f, &msf, &mss, &fail))) return rc;
     if ((rc = blf::fem_residual(f, f->x.p, &res))) return rc;
-    if (!fail && (rc = blf::fem_update(f, &msu))) return rc;
+    if (fail & 2) { // (the device's own trouble, not the system's: the state is left as it was)
+        blg::set_last_error("a substitution's wait for a block timed out");
+        return BL_ERR_STATE;
+    }
+    // a pivot not positive: the gathered forces act explicitly this once, counted (the reference's solve_failures)
+    if ((rc = blf::fem_update(f, &msu, (fail & 1) != 0))) return rc;
     if (rep) {
         blf::fem_report_static(f, rep);
         rep->residual = res;
         rep->assemble_ms = msa, rep->factor_ms = msf, rep->solve_ms = mss, rep->update_ms = msu;
         rep->failures = fail ? 1 : 0;
-    }
-    if (fail) {
-        blg::set_last_error((fail & 1) ? "a pivot of the factorization is not positive"
-                                       : "a substitution's wait for a block timed out");
-        return BL_ERR_STATE;
     }
     return BL_OK;
 }
@@ -1872,10 +1951,39 @@ int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q) {
 int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q) {
     if (!f) return BL_ERR_ARG;
     FCHK(hipStreamSynchronize(f->st));
-    if (x) FCHK(hipMemcpy(f->xd.p, x, sizeof(double) * 3 * f->nn, hipMemcpyHostToDevice));
+    if (x) {
+        FCHK(hipMemcpy(f->xd.p, x, sizeof(double) * 3 * f->nn, hipMemcpyHostToDevice));
+        std::vector<float> pf(3 * (size_t)f->nn); // (the body's float positions follow: (float)x == p)
+        for (size_t i = 0; i < pf.size(); i++) pf[i] = (float)x[i];
+        FCHK(hipMemcpy(f->pf.p, pf.data(), sizeof(float) * pf.size(), hipMemcpyHostToDevice));
+    }
     if (v) FCHK(hipMemcpy(f->v.p, v, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
     if (w) FCHK(hipMemcpy(f->w.p, w, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
     if (q) FCHK(hipMemcpy(f->q.p, q, sizeof(float) * 4 * f->nn, hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_fem_get_nodes(bl_fem* f, bl_node* nodes, float* force, int64_t counters[3]) {
+    if (!f) return BL_ERR_ARG;
+    FCHK(hipStreamSynchronize(f->st));
+    if (nodes) {
+        std::vector<float> p(3 * (size_t)f->nn), v(3 * (size_t)f->nn), im(f->nn), m(f->nn);
+        FCHK(hipMemcpy(p.data(), f->pf.p, sizeof(float) * p.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(v.data(), f->v.p, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(im.data(), f->invm.p, sizeof(float) * im.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(m.data(), f->mass.p, sizeof(float) * m.size(), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < f->nn; i++) {
+            bl_node& n = nodes[i];
+            n.p[0] = p[3 * i], n.p[1] = p[3 * i + 1], n.p[2] = p[3 * i + 2], n.inv_mass = im[i];
+            n.v[0] = v[3 * i], n.v[1] = v[3 * i + 1], n.v[2] = v[3 * i + 2], n.mass = m[i];
+        }
+    }
+    if (force) FCHK(hipMemcpy(force, f->fnode.p, sizeof(float) * 3 * f->nn, hipMemcpyDeviceToHost));
+    if (counters) {
+        unsigned long long c[3];
+        FCHK(hipMemcpy(c, f->cnt.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 3; k++) counters[k] = (int64_t)c[k];
+    }
     return BL_OK;
 }
 

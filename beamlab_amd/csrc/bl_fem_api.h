@@ -43,8 +43,9 @@ int fem_solve_error(bl_fem* f, int* err);
 int fem_residual(bl_fem* f, const double* x, double* rel);
 // y := A x (enqueued on the engine's stream)
 void fem_spmv(bl_fem* f, const double* x, double* y);
-// the update of the state from f's solution x (v, w, positions, orientations)
-int fem_update(bl_fem* f, float* ms);
+// the end of the substep from f's solution x: the update (velocities as forces, clamps), the integration, the position
+// sync; explicit_step: the factorization failed, the gathered forces act instead
+int fem_update(bl_fem* f, float* ms, bool explicit_step = false);
 // the factorization's figures for a report
 void fem_report_static(bl_fem* f, bl_fem_report* rep);
 

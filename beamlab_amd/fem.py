@@ -62,9 +62,9 @@ class ShellBody:
         self.report = abi.FemReport()
 
     @classmethod
-    def plate(cls, nu, nv, size=1.0, y0=0.0, section=SECTION_PLATE):
+    def plate(cls, nu, nv, size=1.0, y0=0.0, section=SECTION_PLATE, clamped=True):
         p, t, fx = plate_mesh(nu, nv, size, y0)
-        return cls(p, t, fx, section)
+        return cls(p, t, fx if clamped else np.zeros_like(fx), section)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -90,6 +90,18 @@ class ShellBody:
         abi.check(self._lib.bl_fem_get_state(self._h, abi.ptr(x.reshape(-1)), abi.ptr(v.reshape(-1)),
                                              abi.ptr(w.reshape(-1)), abi.ptr(q.reshape(-1))), "bl_fem_get_state")
         return x, v, w, q
+
+    def read(self):
+        """The body after the last substep: its nodes' p, inv_mass, v, mass (SoftBody::nodes), the frame's xd, w, q,
+        the forces handed to the integrator, and the counters (solve failures, clamps, NaN resets)."""
+        nodes = np.zeros((self.nn, 8), np.float32)
+        force = np.zeros((self.nn, 3), np.float32)
+        cnt = np.zeros(3, np.int64)
+        abi.check(self._lib.bl_fem_get_nodes(self._h, abi.ptr(nodes.reshape(-1)), abi.ptr(force.reshape(-1)),
+                                             abi.ptr(cnt)), "bl_fem_get_nodes")
+        xd, v, w, q = self.state()
+        return dict(p=nodes[:, 0:3].copy(), inv_mass=nodes[:, 3].copy(), v=nodes[:, 4:7].copy(), mass=nodes[:, 7].copy(),
+                    xd=xd, w=w, q=q, force=force, failures=int(cnt[0]), clamps=int(cnt[1]), nan_resets=int(cnt[2]))
 
     def set_state(self, x=None, v=None, w=None, q=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dt).reshape(-1)

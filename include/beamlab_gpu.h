@@ -325,13 +325,22 @@ typedef struct bl_fem_report {
 /* FemFrame::finalize for the triangles (src/phys/frame_fem.cpp:259-276, 3075-3086) and the symbolic analysis. */
 int bl_fem_create(bl_fem** out, const bl_fem_desc* d);
 void bl_fem_destroy(bl_fem* f);
-/* One implicit step of h (theta, numerical dissipation, gravity g): element pass, assembly, factorization,
- * substitution, update of v, w, positions and orientations. loads: 3 per node (N, host memory) or null. */
+/* One substep of h of the shell body (theta, numerical dissipation, gravity g) as World::simulate_island runs a frame
+ * body stepping every substep (src/phys/world.cpp:3265-3269, 3573-3580, 3812-3822, 3876-3886): element pass,
+ * assembly, factorization, substitution, the update (the new velocities as forces, src/phys/frame_fem.cpp:3000-3027),
+ * SoftBody::integrate_nodes and FemFrame::sync_positions. loads: 3 per node (N, host memory) or null. A factorization
+ * that fails is counted (rep->failures, bl_fem_get_nodes) and the gathered forces act explicitly, as the reference
+ * does; BL_ERR_STATE only for the device's own trouble (a substitution's wait timed out). */
 int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
                 bl_fem_report* rep);
 /* The state: positions (double, 3 per node), v, w (3 per node), q (4 per node, x y z w); any pointer may be null. */
 int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q);
 int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q);
+/* The body's node records after the last step (SoftBody::nodes: p, inv_mass, v, mass), the forces the step handed the
+ * integrator (SoftBody::force: dv m / step of a free node, the gathered forces where the solve was skipped; 3 per
+ * node) and the counters {FemFrame::solve_failures, FemFrame::clamps, NaN velocity resets}
+ * (src/phys/frame_fem.h:342-343, frame_fem.cpp:3000-3027, softbody.cpp:769-795). Any pointer may be null. */
+int bl_fem_get_nodes(bl_fem* f, bl_node* nodes, float* force, int64_t counters[3]);
 /* The last step's linear system over the free nodes in node order: dense (6 nf)^2 row-major A (lower triangle and
  * its mirror), rhs and solution (6 nf each); *nf the free nodes (testing; small meshes). */
 int bl_fem_last_system(bl_fem* f, double* A, double* rhs, double* x, uint32_t* nf);
