@@ -207,17 +207,20 @@ def config2_body(seed=2, faces=False):
 
 def config2_scene(seed=2, n_balls=2000):
     """The 64x32x24 lattice with its faces as triangles and self collision on (collision radius 0.04, as
-    build_soft_box sets), resting 5 cm above the ground, and `n_balls` rigid balls (radius uniform in [0.1, 0.3] m,
-    5 kg): one layer on the block's top, 1 m in from its edges, the rest on the ground around it, nearest first,
-    the first ring 5 cm from its sides (0.7 m grid, +-2 cm jitter). Every body stays below 8 m/s (the slow bodies'
-    pair schedule) over 4000 substeps."""
-    body = lattice_body(64, 32, 24, 0.25, 2.0, (0.0, 0.05, 0.0), k=None, d=500.0, eps=0.05, seed=seed, jitter=0.01,
+    build_soft_box sets), standing on the ground (its bottom 1 cm up, the nodes' jitter +-1 cm), and `n_balls` rigid
+    balls (radius uniform in [0.1, 0.3] m, 5 kg): one layer resting on the block's top (each ball 5 mm above its
+    radius from the top face, inside the triangles' contact reach: ball-lattice contacts from the first substep), 1 m
+    in from its edges, the rest on the ground around it (1 mm into the ground), nearest first, the first ring 5 cm
+    from its sides (0.7 m grid, +-2 cm jitter). Every body stays below 8 m/s (the slow bodies' pair schedule) over
+    4000 substeps."""
+    y0 = 0.01
+    body = lattice_body(64, 32, 24, 0.25, 2.0, (0.0, y0, 0.0), k=None, d=500.0, eps=0.05, seed=seed, jitter=0.01,
                         k_range=(1e5, 5e6), weak_frac=0.02, weak_eps=0.01, faces=True, name="c2_lattice")
     body.opts.self_collision = 1
     body.opts.collision_radius = np.float32(0.04)
     rng = TRng(seed + 1000)
-    hx, hz, top = 7.875, 2.875, 0.05 + 31 * 0.25
-    spots = [(-6.65 + 0.7 * ix, top + 0.4, -1.75 + 0.7 * iz) for iz in range(6) for ix in range(20)]
+    hx, hz, top = 7.875, 2.875, y0 + 31 * 0.25
+    spots = [(-6.65 + 0.7 * ix, None, -1.75 + 0.7 * iz) for iz in range(6) for ix in range(20)]
     xs = sorted({round(s * (hx + 0.35 + 0.7 * i), 4) for s in (-1, 1) for i in range(40)} |
                 {round(-7.7 + 0.7 * j, 4) for j in range(23)})
     zs = sorted({round(s * (hz + 0.35 + 0.7 * i), 4) for s in (-1, 1) for i in range(40)} |
@@ -235,10 +238,11 @@ def config2_scene(seed=2, n_balls=2000):
         r = rng.range(0.1, 0.3)
         jx, jz = rng.range(-0.02, 0.02), rng.range(-0.02, 0.02)
         if k < len(spots):
-            x, y, z = spots[k]
+            x, _, z = spots[k]
+            y = top + float(r) + 0.005
         else:
             x, z = ring[k - len(spots)][3:]
-            y = float(r) + 0.02
+            y = float(r) - 0.001
         balls.append(ball_body((np.float32(x) + jx, np.float32(y), np.float32(z) + jz), r, 5.0, name=f"ball{k}"))
     return [body] + balls
 
@@ -331,15 +335,24 @@ def config4_body(n_nodes, origin, seed):
 
 
 def config4_scene(n_bodies=512, arena=200.0, seed=4):
-    """The batched scene: n_bodies random beam networks (config4_sizes) on a grid over an arena x arena ground,
-    dropped from 0.3 m, inter-body contacts on."""
+    """The batched scene: n_bodies random beam networks (config4_sizes) in stacks of two on a grid over an
+    arena x arena ground, inter-body contacts on. Each stack's larger crate stands on the ground (its bottom 1 cm up,
+    the nodes' jitter 5 mm), the smaller one on it, its bottom 4 cm above the lower one's top: inside the triangles'
+    contact reach (the bodies' 5 cm collision radius), so every stack has inter-body contacts from the first
+    substep."""
     sizes = config4_sizes(n_bodies, seed)
-    g = int(math.ceil(math.sqrt(n_bodies)))
+    stacks = (n_bodies + 1) // 2
+    g = int(math.ceil(math.sqrt(stacks)))
     pitch = arena / g
     rng = np.random.default_rng(seed + 1)
     bodies = []
-    for i, nb in enumerate(sizes):
-        cx = -0.5 * arena + (i % g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
-        cz = -0.5 * arena + (i // g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
-        bodies.append(config4_body(int(nb), (float(cx), 0.3, float(cz)), seed=1000 + i))
+    for k in range(stacks):
+        cx = -0.5 * arena + (k % g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+        cz = -0.5 * arena + (k // g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+        pair = sorted(range(2 * k, min(2 * k + 2, n_bodies)), key=lambda i: -int(sizes[i]))
+        y = 0.01
+        for i in pair:
+            b = config4_body(int(sizes[i]), (float(cx), y, float(cz)), seed=1000 + i)
+            bodies.append(b)
+            y = float(b.nodes["p"][:, 1].max()) + 0.04
     return bodies

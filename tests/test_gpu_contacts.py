@@ -58,7 +58,7 @@ def test_contacts_long_frames(oracle_mod):
 
 def test_contact_scene_full_size(oracle_mod):
     """The 64x32x24 lattice with 2 000 balls at full size over 660 substeps (40 pair rebuilds; the balls on the block
-    land on it after about 570), bit for bit against the oracle. Its compactions span hundreds of workgroups (node hash,
+    rest on it from the first substep), bit for bit against the oracle. Its compactions span hundreds of workgroups (node hash,
     node-triangle search), so the look-back crosses many 64-workgroup windows; the scene's 2 001 bodies make many
     pair islands."""
     from _cases import GROUND, _boxes
