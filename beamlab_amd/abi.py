@@ -152,6 +152,7 @@ SIGNATURES = {
     "bl_fem_get_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "bl_fem_set_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "bl_fem_get_nodes": (ctypes.c_int, [_P, _P, _P, _P]),
+    "bl_fem_solve_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "bl_fem_last_system": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),
     "bl_fem_tri_k": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),

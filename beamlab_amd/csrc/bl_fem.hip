@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1885,6 +1886,51 @@ void fem_report_static(bl_fem* f, bl_fem_report* rep) {
 
 } // namespace blf
 
+// ---------------------------------------------------------------------------------------------- the no-allocation solve
+// The substitutions' scratch of bl_fem_solve_noalloc: a file-scope device array, one copy per GPU, held until the
+// process ends. Layout: [epoch u32, error word i32, pad to 256 B][forward flags u32 x nflag][backward flags u32 x
+// nflag][pad to 256 B][y f64 x 6 nf][V f64 x nv][X f64 x nv]. The flags hold the epoch of the solve that published
+// them, so stale ones of earlier calls (of any engine) never read as published and need no clearing; the error word
+// is cleared by a kernel at each call.
+constexpr size_t kNoAllocScratchBytes = 32u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (hipGetSymbolAddress once per device, with it current).
+int scratch_address(void** out) {
+    int dev = 0;
+    FCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        blg::set_last_error("device index out of range for the no-allocation scratch");
+        return BL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        FCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) {
+            blg::set_last_error("hipGetSymbolAddress returned no address for the scratch array");
+            return BL_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return BL_OK;
+}
+
+// the scratch bytes the engine's substitutions need
+size_t noalloc_bytes(const bl_fem* f) {
+    const size_t head = 256, flags = ((2 * f->fflag.n * sizeof(uint32_t) + 255) / 256) * 256;
+    return head + flags + sizeof(double) * (6 * (size_t)f->nf + 2 * f->V.n);
+}
+
+} // namespace
+
 extern "C" {
 
 int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
@@ -1903,7 +1949,9 @@ int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
             delete f;
             return BL_ERR_HIP;
         }
-    const int rc = fem_build(f, d);
+    int rc = fem_build(f, d);
+    void* sp = nullptr;
+    if (!rc) rc = scratch_address(&sp); // (the no-allocation solve's scratch on this device, once)
     if (rc) {
         delete f;
         return rc;
@@ -2042,6 +2090,57 @@ int bl_fem_solve(bl_fem* f, const double* b, double* x) {
         return BL_ERR_STATE;
     }
     FCHK(hipMemcpy(x, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToHost));
+    return BL_OK;
+}
+
+int bl_fem_solve_noalloc(bl_fem* f, const double* b, double* x, uint64_t* scratch_bytes) {
+    if (!f) return BL_ERR_ARG;
+    const size_t need = noalloc_bytes(f);
+    if (scratch_bytes) *scratch_bytes = need;
+    if (!b || !x) return scratch_bytes ? BL_OK : BL_ERR_ARG;
+    if (need > kNoAllocScratchBytes) {
+        blg::set_last_error(("the system needs " + std::to_string(need) + " bytes of substitution scratch; the "
+                             "no-allocation solve holds " + std::to_string(kNoAllocScratchBytes)).c_str());
+        return BL_ERR_ARG;
+    }
+    if (!f->have_system) return BL_ERR_STATE;
+    void* base = nullptr;
+    int rc;
+    if ((rc = scratch_address(&base))) return rc;
+    unsigned char* sp = static_cast<unsigned char*>(base);
+    const size_t nflag = f->fflag.n;
+    const hipStream_t st = f->st;
+    SolArgs so = f->sol(b);
+    uint32_t* epoch = reinterpret_cast<uint32_t*>(sp);
+    so.epoch = epoch;
+    so.err = reinterpret_cast<int*>(sp + 4);
+    so.fflag = reinterpret_cast<uint32_t*>(sp + 256);
+    so.bflag = so.fflag + nflag;
+    double* dp = reinterpret_cast<double*>(sp + 256 + ((2 * nflag * sizeof(uint32_t) + 255) / 256) * 256);
+    so.y = dp;
+    so.V = dp + 6 * (size_t)f->nf;
+    so.X = so.V + f->V.n;
+    so.x = x;
+    so.skip = nullptr;
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, so.err, 1u);
+    hipLaunchKernelGGL(k_epoch, dim3(1), dim3(64), 0, st, epoch);
+    for (size_t L = 0; L < f->plan.size(); L++) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_fgather, dim3(lp.nlist), dim3(256), 0, st, so, f->flist.p + lp.list_off, lp.nlist);
+        hipLaunchKernelGGL(k_fsolve, dim3(lp.nfs), dim3(256), 0, st, so, f->fs_tasks.p + lp.fs_off, lp.nfs);
+    }
+    for (size_t L = f->plan.size(); L-- > 0;) {
+        const bl_fem::LevelPlan& lp = f->plan[L];
+        hipLaunchKernelGGL(k_bsolve, dim3(lp.nbs), dim3(256), 0, st, so, f->bs_tasks.p + lp.bs_off, lp.nbs);
+    }
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(st));
+    int fail = 0;
+    FCHK(hipMemcpy(&fail, so.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (fail) {
+        blg::set_last_error("a substitution's wait for a block timed out");
+        return BL_ERR_STATE;
+    }
     return BL_OK;
 }
 

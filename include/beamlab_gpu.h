@@ -346,6 +346,15 @@ int bl_fem_get_nodes(bl_fem* f, bl_node* nodes, float* force, int64_t counters[3
 int bl_fem_last_system(bl_fem* f, double* A, double* rhs, double* x, uint32_t* nf);
 /* Solves A x = b with the last step's factorization (b, x: 6 per free node, node order, host memory). */
 int bl_fem_solve(bl_fem* f, const double* b, double* x);
+/* Solves A x = b with the last step's factorization, allocating nothing: b and x are DEVICE pointers (6 per free
+ * node, node order, FP64), and the substitutions' scratch (per-front vectors, block flags, y) is a file-scope device
+ * array of 32 MiB per GPU, held until the process ends (fetched once per device with hipGetSymbolAddress at
+ * bl_fem_create). No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Runs on the engine's stream and waits for
+ * it. Every call on one GPU shares that GPU's array: calls that overlap on one GPU race. A system whose scratch
+ * exceeds 32 MiB is rejected (BL_ERR_ARG); *scratch_bytes (may be null) receives what it needs, and with b or x
+ * null the call only reports that. Results are those of bl_fem_solve, bit for bit. */
+int bl_fem_solve_noalloc(bl_fem* f, const double* b, double* x, uint64_t* scratch_bytes);
 /* The element stiffness of triangle `tri` as the device built it (18 x 18 row-major, FP64). */
 int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
 
