@@ -30,6 +30,7 @@ struct RStat {
     uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles, 2 a scan's wait timed out
     uint32_t last_pairs;  // candidate pairs kept in the last pair island
     uint32_t npart;       // partner entries (when they overflowed their buffer)
+    uint32_t nisl;        // pair islands of the rebuild (the response's workgroups read it)
 };
 
 // A one-pass launch's look-back: the workgroups' published words, the ticket counter (null: workgroups in index
@@ -58,8 +59,11 @@ struct CArgs {
     uint32_t* bquery;         // per body: has a partner or collides with itself
     const uint32_t* body_box; // per body: bounds keys (BF_MN, BF_MX words of a frame record)
     const uint32_t* body_tri0;
-    const uint32_t* itri;     // the triangles of the pair islands' bodies, in body order
+    const uint32_t* itri;     // the triangles of the pair islands' bodies, island by island, each in body order
     uint32_t n_itri;
+    const uint32_t* icap;     // the capsules of the pair islands' bodies, in the same order
+    uint32_t n_icap;
+    uint32_t* irange;         // per pair island: its runs [start, end) in the NT, NC and CT lists (6 words)
     const uint32_t* sample_base;
     float* samples;           // every 7th triangle's longest edge, in body order
     uint32_t nsamples;
@@ -102,7 +106,8 @@ struct PArgs {
     const RStat* st;
     Hit* hraw;       // per block of 256 candidates: its hits first, in order
     uint32_t* bcnt;  // per block: its hit count
-    uint32_t* bpre;  // per block: the hits before it (the response fills it)
+    const uint32_t* irange; // per pair island: its runs [start, end) in the NT, NC and CT lists
+    uint32_t gresp;  // the response's workgroups (at least the pair islands: st->nisl)
 };
 
 // What the island building reads of a body on the host.
@@ -180,8 +185,9 @@ private:
     std::vector<float> bx_;
     PairScratch pair_ws_;
     std::vector<uint32_t> h_tri0_, itri_h_; // per body: first triangle; the pair islands' triangles
-    Buf<uint32_t> itri;
-    uint32_t n_itri = 0;
+    std::vector<uint32_t> h_cap0_, icap_h_; // per body: first capsule; the pair islands' capsules
+    Buf<uint32_t> itri, icap, irange;
+    uint32_t n_itri = 0, n_icap = 0, gresp_ = 0;
     std::vector<int> up_isl_, up_mi_;
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;
@@ -210,7 +216,7 @@ private:
     Buf<float4> spos, ebb, sb;
     Buf<uint4> ea, sa;
     Buf<float2> ec, sc;
-    Buf<uint32_t> bcnt, bpre;
+    Buf<uint32_t> bcnt;
     Buf<uint4> ntraw, nc, ct, nt;
     Buf<Hit> hraw;
     uint32_t cap_nt = 0, cap_nc = 0, cap_ct = 0, cap_ent = 0, esize = 0;

@@ -643,8 +643,9 @@ constexpr int kGcap = 64;
 __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
     __shared__ uint32_t sh[8];
     const uint32_t tk = wg_ticket(c.lk, sh);
-    const uint32_t ci = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
-    const bool live = ci < a.ncaps;
+    const uint32_t qi = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    const bool live = qi < c.n_icap;
+    const uint32_t ci = live ? c.icap[qi] : 0u; // (the islands' capsules island by island: the list is island-major)
     uint4 cp = make_uint4(0, 0, 0, 0);
     int isl = -1;
     CellRange cr;
@@ -774,9 +775,10 @@ __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
     __shared__ uint32_t lcnt[256 / kGcap];
     __shared__ uint32_t sh[8];
     const uint32_t tk = wg_ticket(c.lk, sh);
-    const uint32_t ci = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
+    const uint32_t qi = (tk * 256 + threadIdx.x) / kGcap, gl = threadIdx.x & (kGcap - 1);
     const uint32_t gi = threadIdx.x / kGcap;
-    const bool live = ci < a.ncaps;
+    const bool live = qi < c.n_icap;
+    const uint32_t ci = live ? c.icap[qi] : 0u; // (the islands' capsules island by island: the list is island-major)
     uint32_t* L = lst[gi];
     uint32_t* U = uni[gi];
     uint4 cp = make_uint4(0, 0, 0, 0);
@@ -917,6 +919,31 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
     if (tk == gridDim.x - 1 && threadIdx.x == 0) c.st->nnt = pre + agg;
 }
 
+// Each pair island's run in the three final lists (NT after the filter, NC, CT): the searches go over the islands'
+// triangles and capsules island by island, so an island's candidates are one run in each list; its bounds are where
+// the island changes. (A list whose islands are not in order would break that: flagged, the rebuild fails.)
+__device__ __forceinline__ void island_run(const int* bisl, const uint4* L, uint32_t n, uint32_t i, uint32_t* irange,
+                                           int which, RStat* st) {
+    if (i >= n) return;
+    const int isl = bisl[L[i].x];
+    const int prev = i > 0 ? bisl[L[i - 1].x] : -1;
+    const int next = i + 1 < n ? bisl[L[i + 1].x] : -1;
+    if (isl < 0 || prev > isl) atomicOr(&st->unsupported, 8u);
+    if (isl < 0) return;
+    if (isl != prev) irange[6 * isl + 2 * which] = i;
+    if (isl != next) irange[6 * isl + 2 * which + 1] = i + 1;
+}
+
+__global__ void k_rb_ranges(KArgs a, CArgs c) {
+    (void)a;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) c.st->nisl = c.nisl;
+    const uint32_t nnt = min(c.st->nnt, c.cap_nt), nnc = min(c.st->nnc, c.cap_nc), nct = min(c.st->nct, c.cap_ct);
+    island_run(c.bisl, c.nt, nnt, i, c.irange, 0, c.st);
+    island_run(c.bisl, c.nc, nnc, i, c.irange, 1, c.st);
+    island_run(c.bisl, c.ct, nct, i, c.irange, 2, c.st);
+}
+
 // ---------------------------------------------------------------------------------------------- narrow phase
 // pair_detect (src/phys/world.cpp:2807-2962) on the concatenated candidates (node-triangle, node-capsule,
 // capsule-triangle), 256 per block: each block's hits packed to the front of its 256 slots, in candidate order,
@@ -1004,141 +1031,164 @@ __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn
     return true;
 }
 
-// One workgroup. The hits' positions first (a prefix over the narrow phase's per-block counts), then windows of
-// 1024 hits: the window's (node, hit) keys sorted in LDS give each hit its predecessors (the latest earlier hit of
-// the window on each of its nodes), the longest chain to it its level; the levels run one after another, the hits
-// of a level (no shared node) in parallel.
-constexpr int kRespW = 1024, kRespKeys = 8192;
-__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int parity, uint32_t nblk) {
+// One workgroup per pair island (a grid-stride loop over them; islands share no node, so they run side by side and
+// the result is the serial one). An island's hits are those of its runs in the three lists, in list order; they go
+// in windows of at most 1024: the consecutive narrow-phase blocks of a run whose hits in the run fit. In a window each
+// hit's predecessors (the latest earlier hit of the window on each of its nodes) give its level, the longest chain to
+// it; the levels run one after another, the hits of a level (no shared node) in parallel.
+constexpr int kRespW = 1024, kRespKeys = 8192, kRespBlk = 16;
+__global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int parity) {
     __shared__ uint64_t keys[kRespKeys];
     __shared__ int16_t preds[kRespW * 5];
     __shared__ uint16_t level[kRespW];
     __shared__ uint32_t sred[kRespW / 64];
-    // the blocks' hit prefix in LDS when it fits (each hit's block is found by a binary search over it)
-    constexpr uint32_t kPreLds = 2048;
-    __shared__ uint32_t spre[kPreLds];
+    __shared__ uint32_t wfirst[kRespBlk], wcnt[kRespBlk], wpre[kRespBlk + 1], wnb;
     PArgs p = pin;
-    uint32_t* pre = nblk <= kPreLds ? spre : p.bpre;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
+    const uint32_t nisl = pin.st->nisl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // exclusive prefix of the block counts
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < nblk; c0 += kRespW) {
-        const uint32_t i = c0 + tid;
-        const uint32_t v = i < nblk ? p.bcnt[i] : 0u;
-        uint32_t x = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= (uint32_t)off) x += y;
-        }
-        if (lane == 63) sred[wv] = x;
-        __syncthreads();
-        uint32_t before = 0, total = 0;
-        for (uint32_t w = 0; w < kRespW / 64; w++) {
-            before += w < wv ? sred[w] : 0u;
-            total += sred[w];
-        }
-        if (i < nblk) pre[i] = carry + before + x - v;
-        carry += total;
-        __syncthreads();
-    }
-    const uint32_t nh = carry;
     uint32_t contacts = 0;
-    for (uint32_t w0 = 0; w0 < nh; w0 += kRespW) {
-        const uint32_t m = nh - w0 < (uint32_t)kRespW ? nh - w0 : (uint32_t)kRespW;
-        uint32_t nk = 64;
-        while (nk < 5 * m) nk <<= 1;
-        const uint32_t h = w0 + tid;
-        const bool valid = tid < m;
-        Hit hit;
-        ContactNodes cn;
-        cn.na = cn.nb = 0;
-        if (valid) {
-            // the block holding hit h: the last one whose prefix is at most h
-            uint32_t lo = 0, hi = nblk - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (pre[mid] <= h) lo = mid;
-                else hi = mid - 1;
-            }
-            hit = p.hraw[lo * 256 + (h - pre[lo])];
-            contact_nodes(a, p, hit, cn);
-        }
-        if (5 * m <= (uint32_t)kRespW) {
-            // a small window: each (hit, node) finds its predecessor - the latest earlier hit on the same node - by
-            // reading back through the earlier hits' nodes
-            uint32_t* hn = (uint32_t*)keys;
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-                hn[tid * 5 + j] = j < cn.na + cn.nb ? cn.n[j] : ~0u;
-                preds[tid * 5 + j] = -1;
-            }
-            __syncthreads();
-            if (tid < 5 * m) {
-                const uint32_t node = hn[tid], me = tid / 5;
-                int pq = -1;
-                if (node != ~0u)
-                    for (int q = (int)me - 1; q >= 0 && pq < 0; q--)
-#pragma unroll
-                        for (int j = 0; j < 5; j++)
-                            if (hn[q * 5 + j] == node) pq = q;
-                preds[tid] = (int16_t)pq;
-            }
-        } else {
-            for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-                if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
-                preds[tid * 5 + j] = -1;
-            }
-            __syncthreads();
-            // bitonic sort of the window's keys
-            for (uint32_t size = 2; size <= nk; size <<= 1)
-                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (uint32_t q = tid; q < nk / 2; q += kRespW) {
-                        const uint32_t i = 2 * q - (q & (stride - 1));
-                        const uint32_t j = i + stride;
-                        const bool up = (i & size) == 0;
-                        const uint64_t x = keys[i], y = keys[j];
-                        if ((x > y) == up) keys[i] = y, keys[j] = x;
+    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {
+        for (int L = 0; L < 3; L++) {
+            const uint32_t base = L == 0 ? 0u : (L == 1 ? p.nnt : p.nnt + p.nnc);
+            const uint32_t lo = base + p.irange[6 * isl + 2 * L], hi = base + p.irange[6 * isl + 2 * L + 1];
+            if (lo >= hi) continue;
+            const uint32_t b1 = (hi - 1) / 256;
+            for (uint32_t wb = lo / 256; wb <= b1;) {
+                // the next blocks' hits within [lo, hi) (hits are in candidate order in a block), and how many of the
+                // blocks fit one window
+                if (tid < (uint32_t)kRespBlk) {
+                    const uint32_t b = wb + tid;
+                    uint32_t f = 0, e = 0;
+                    if (b <= b1) {
+                        const Hit* H = p.hraw + (size_t)b * 256;
+                        e = p.bcnt[b];
+                        if (b * 256 < lo) {
+                            uint32_t l2 = 0, h2 = e;
+                            while (l2 < h2) {
+                                const uint32_t mid = (l2 + h2) >> 1;
+                                if (H[mid].pair < lo) l2 = mid + 1;
+                                else h2 = mid;
+                            }
+                            f = l2;
+                        }
+                        if (b * 256 + 256 > hi) {
+                            uint32_t l2 = f, h2 = e;
+                            while (l2 < h2) {
+                                const uint32_t mid = (l2 + h2) >> 1;
+                                if (H[mid].pair < hi) l2 = mid + 1;
+                                else h2 = mid;
+                            }
+                            e = l2;
+                        }
                     }
-                    __syncthreads();
+                    wfirst[tid] = f, wcnt[tid] = e - f;
                 }
-            // predecessors: the previous key of the same node from another (earlier) hit
-            for (uint32_t k = tid; k < nk; k += kRespW) {
-                const uint64_t key = keys[k];
-                if (key == ~0ull) continue;
-                const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
-                int q = (int)k - 1;
-                while (q >= 0 && (uint32_t)(keys[q] >> 32) == node && ((uint32_t)(keys[q] >> 3) & (kRespW - 1)) == me) q--;
-                if (q >= 0 && (uint32_t)(keys[q] >> 32) == node) preds[me * 5 + (key & 7)] = (int16_t)((keys[q] >> 3) & (kRespW - 1));
+                __syncthreads();
+                if (tid == 0) {
+                    uint32_t acc = 0, nb = 0;
+                    wpre[0] = 0;
+                    while (nb < (uint32_t)kRespBlk && wb + nb <= b1 && (nb == 0 || acc + wcnt[nb] <= (uint32_t)kRespW)) {
+                        acc += wcnt[nb];
+                        wpre[++nb] = acc;
+                    }
+                    wnb = nb;
+                }
+                __syncthreads();
+                const uint32_t nbw = wnb, m = wpre[nbw];
+                const bool valid = tid < m;
+                Hit hit;
+                ContactNodes cn;
+                cn.na = cn.nb = 0;
+                if (valid) {
+                    uint32_t k = 0;
+                    while (k + 1 < nbw && wpre[k + 1] <= tid) k++;
+                    hit = p.hraw[(size_t)(wb + k) * 256 + wfirst[k] + (tid - wpre[k])];
+                    contact_nodes(a, p, hit, cn);
+                }
+                if (m > 0) {
+                    uint32_t nk = 64;
+                    while (nk < 5 * m) nk <<= 1;
+                    if (5 * m <= (uint32_t)kRespW) {
+                        // a small window: each (hit, node) finds its predecessor - the latest earlier hit on the same
+                        // node - by reading back through the earlier hits' nodes
+                        uint32_t* hn = (uint32_t*)keys;
+#pragma unroll
+                        for (int j = 0; j < 5; j++) {
+                            hn[tid * 5 + j] = j < cn.na + cn.nb ? cn.n[j] : ~0u;
+                            preds[tid * 5 + j] = -1;
+                        }
+                        __syncthreads();
+                        if (tid < 5 * m) {
+                            const uint32_t node = hn[tid], me = tid / 5;
+                            int pq = -1;
+                            if (node != ~0u)
+                                for (int q = (int)me - 1; q >= 0 && pq < 0; q--)
+#pragma unroll
+                                    for (int j = 0; j < 5; j++)
+                                        if (hn[q * 5 + j] == node) pq = q;
+                            preds[tid] = (int16_t)pq;
+                        }
+                    } else {
+                        for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
+                        __syncthreads();
+#pragma unroll
+                        for (int j = 0; j < 5; j++) {
+                            if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
+                            preds[tid * 5 + j] = -1;
+                        }
+                        __syncthreads();
+                        // bitonic sort of the window's keys
+                        for (uint32_t size = 2; size <= nk; size <<= 1)
+                            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                                for (uint32_t q = tid; q < nk / 2; q += kRespW) {
+                                    const uint32_t i = 2 * q - (q & (stride - 1));
+                                    const uint32_t j = i + stride;
+                                    const bool up = (i & size) == 0;
+                                    const uint64_t x = keys[i], y = keys[j];
+                                    if ((x > y) == up) keys[i] = y, keys[j] = x;
+                                }
+                                __syncthreads();
+                            }
+                        // predecessors: the previous key of the same node from another (earlier) hit
+                        for (uint32_t k = tid; k < nk; k += kRespW) {
+                            const uint64_t key = keys[k];
+                            if (key == ~0ull) continue;
+                            const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
+                            int q = (int)k - 1;
+                            while (q >= 0 && (uint32_t)(keys[q] >> 32) == node && ((uint32_t)(keys[q] >> 3) & (kRespW - 1)) == me) q--;
+                            if (q >= 0 && (uint32_t)(keys[q] >> 32) == node) preds[me * 5 + (key & 7)] = (int16_t)((keys[q] >> 3) & (kRespW - 1));
+                        }
+                    }
+                    level[tid] = 0;
+                    __syncthreads();
+                    // levels: one past the deepest predecessor (as many rounds as the longest chain)
+                    int changed = 1;
+                    while (__syncthreads_or(changed)) {
+                        changed = 0;
+                        uint32_t Lv = level[tid];
+                        for (int j = 0; j < 5; j++) {
+                            const int pq = preds[tid * 5 + j];
+                            if (pq >= 0 && level[pq] + 1u > Lv) Lv = level[pq] + 1u;
+                        }
+                        __syncthreads();
+                        if (Lv != level[tid]) level[tid] = (uint16_t)Lv, changed = 1;
+                    }
+                    const uint32_t my = valid ? level[tid] : 0u;
+                    uint32_t top = my;
+                    for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
+                    if (lane == 0) sred[wv] = top;
+                    __syncthreads();
+                    uint32_t maxl = 0;
+                    for (uint32_t w = 0; w < kRespW / 64; w++) maxl = max(maxl, sred[w]);
+                    for (uint32_t Lv = 0; Lv <= maxl; Lv++) {
+                        if (valid && my == Lv && contact_force(a, parity, cn, hit.nrm, hit.pen)) contacts++;
+                        __syncthreads();
+                    }
+                }
+                __syncthreads();
+                wb += nbw;
             }
-        }
-        level[tid] = 0;
-        __syncthreads();
-        // levels: one past the deepest predecessor (as many rounds as the longest chain)
-        int changed = 1;
-        while (__syncthreads_or(changed)) {
-            changed = 0;
-            uint32_t L = level[tid];
-            for (int j = 0; j < 5; j++) {
-                const int pq = preds[tid * 5 + j];
-                if (pq >= 0 && level[pq] + 1u > L) L = level[pq] + 1u;
-            }
-            __syncthreads();
-            if (L != level[tid]) level[tid] = (uint16_t)L, changed = 1;
-        }
-        const uint32_t my = valid ? level[tid] : 0u;
-        uint32_t top = my;
-        for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
-        if (lane == 0) sred[wv] = top;
-        __syncthreads();
-        uint32_t maxl = 0;
-        for (uint32_t w = 0; w < kRespW / 64; w++) maxl = max(maxl, sred[w]);
-        for (uint32_t L = 0; L <= maxl; L++) {
-            if (valid && my == L && contact_force(a, parity, cn, hit.nrm, hit.pen)) contacts++;
-            __syncthreads();
         }
     }
     for (int off = 32; off > 0; off >>= 1) contacts += __shfl_xor(contacts, off);
@@ -1225,10 +1275,10 @@ void ContactEngine::release() {
     bisl.release(), bmem.release(), pcnt.release(), poff.release(), plist.release();
     nmem = cap_part = 0;
     for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
-                             &eb, &ecount, &estart, &bcnt, &bpre})
+                             &eb, &ecount, &estart, &bcnt, &icap, &irange})
         b->release();
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
-    itri.release(), n_itri = 0;
+    itri.release(), n_itri = 0, n_icap = 0, gresp_ = 0;
     scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
     for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
     hraw.release();
@@ -1267,6 +1317,8 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     }
     nsamples = s0[nbodies];
     h_tri0_ = tri0;
+    h_cap0_.assign(nbodies + 1, 0);
+    for (uint32_t i = 0; i < nbodies; i++) h_cap0_[i + 1] = h_cap0_[i] + hv.bodies[i].ncap;
     h_nnode_.resize(nbodies);
     for (uint32_t i = 0; i < nbodies; i++) h_nnode_[i] = hv.bodies[i].nnode;
     RC(body_tri0.ensure(nbodies + 1));
@@ -1371,30 +1423,39 @@ int ContactEngine::begin_frame(hipStream_t st) {
     for (size_t k = 0; k < np; k++)
         for (uint32_t p = off[k]; p < off[k + 1]; p++) isl[mem[p]] = (int)k, mi[mem[p]] = (int)p;
     nmem = (uint32_t)mem.size();
-    // the triangles the node-triangle and capsule-triangle searches run over: the pair islands' bodies', in body
-    // order (the order of the candidate lists; no other triangle has candidates); and the node hash's size (only
-    // when the island bodies changed)
+    // the triangles and capsules the searches run over: the pair islands' bodies', island by island, each island's
+    // in body order. The candidate lists follow this order, so each island's candidates are one run in each list (an
+    // island's own order is the reference's; islands share no node, so their relative order changes nothing); no
+    // other triangle or capsule has candidates. And the node hash's size (only when the island bodies changed).
     if (!up_valid_ || isl != up_isl_) {
-        itri_h_.clear();
+        itri_h_.clear(), icap_h_.clear();
         size_t inodes = 0;
-        for (uint32_t g = 0; g < nbodies; g++)
-            if (isl[g] >= 0) {
+        for (size_t k = 0; k < np; k++)
+            for (uint32_t q = off[k]; q < off[k + 1]; q++) {
+                const uint32_t g = mem[q];
                 for (uint32_t t = h_tri0_[g]; t < h_tri0_[g + 1]; t++) itri_h_.push_back(t);
+                for (uint32_t t = h_cap0_[g]; t < h_cap0_[g + 1]; t++) icap_h_.push_back(t);
                 inodes += h_nnode_[g];
             }
-        n_itri = (uint32_t)itri_h_.size();
+        n_itri = (uint32_t)itri_h_.size(), n_icap = (uint32_t)icap_h_.size();
         tsize_f = std::min<uint32_t>(tsize, pow2_at_least(4 * inodes));
     }
-    const void* was[3] = {imem_off.p, imem.p, itri.p};
+    const void* was[4] = {imem_off.p, imem.p, itri.p, icap.p};
     RC(imem_off.ensure(np + 1));
     RC(imem.ensure(mem.size() + 1));
     RC(itri.ensure((size_t)n_itri + 1));
+    RC(icap.ensure((size_t)n_icap + 1));
+    // the response's workgroups: one per pair island, in powers of two (a captured segment keeps its launch until
+    // the count outgrows it)
+    if (np > gresp_) gresp_ = pow2_at_least(np);
+    RC(irange.ensure(6 * (size_t)std::max<uint32_t>(gresp_, 1)));
     RC(pcnt.ensure(nmem + 1));
     RC(poff.ensure(nmem + 1));
     cap_part = std::max<uint32_t>(cap_part, std::max<uint32_t>(4096, 8 * nmem));
     RC(plist.ensure(cap_part));
     // (the same tables as the last frame's, in the same buffers: nothing to upload)
-    const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && was[2] == itri.p && isl == up_isl_ &&
+    const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && was[2] == itri.p && was[3] == icap.p &&
+                      isl == up_isl_ &&
                       mi == up_mi_ && off == up_off_ && mem == up_mem_;
     if (!same) {
         const std::vector<uint32_t> off0(1, 0u);
@@ -1404,6 +1465,7 @@ int ContactEngine::begin_frame(hipStream_t st) {
         CHK(hipMemcpyAsync(imem_off.p, offu.data(), 4 * offu.size(), hipMemcpyHostToDevice, st));
         if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
         if (n_itri) CHK(hipMemcpyAsync(itri.p, itri_h_.data(), 4 * (size_t)n_itri, hipMemcpyHostToDevice, st));
+        if (n_icap) CHK(hipMemcpyAsync(icap.p, icap_h_.data(), 4 * (size_t)n_icap, hipMemcpyHostToDevice, st));
         CHK(hipStreamSynchronize(st)); // (off0 goes out of scope)
         up_isl_ = isl, up_mi_ = mi, up_off_ = off, up_mem_ = mem;
         up_valid_ = true;
@@ -1446,16 +1508,20 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     hipLaunchKernelGGL(k_rb_fill, grid_for(N), dim3(256), 0, st, a, c);
     // 2) triangles, then capsules, against the node hash
     if (n_itri) RC(stage(k_rb_nt, ((size_t)n_itri * kGnt + 255) / 256));
-    if (C) RC(stage(k_rb_nc, ((size_t)C * kGcap + 255) / 256));
+    if (n_icap) RC(stage(k_rb_nc, ((size_t)n_icap * kGcap + 255) / 256));
     // 3) capsules against the triangles' cells
     if (C && T) {
         RC(stage(k_rb_prims, ((size_t)n_itri + 255) / 256));
         RC(scan(ecount.p, estart.p, (size_t)esize + 1, st));
         hipLaunchKernelGGL(k_rb_efill, grid_for(cap_ent), dim3(256), 0, st, c);
-        RC(stage(k_rb_ct, ((size_t)C * kGcap + 255) / 256));
+        if (n_icap) RC(stage(k_rb_ct, ((size_t)n_icap * kGcap + 255) / 256));
     }
     // the near-pair filter on the node-triangle candidates
     if (T) RC(stage(k_rb_near, ((size_t)cap_nt + 255) / 256));
+    // each island's runs in the three lists
+    hipLaunchKernelGGL(k_rb_clear, grid_for(6 * (size_t)c.nisl), dim3(256), 0, st, c.irange, 6 * c.nisl);
+    const size_t nr = std::max<size_t>(1, std::max<size_t>(cap_nt, std::max(cap_nc, cap_ct)));
+    hipLaunchKernelGGL(k_rb_ranges, grid_for(nr), dim3(256), 0, st, a, c);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 
@@ -1474,7 +1540,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.ipar = ipar.p, c.bbox = bbox.p, c.bquery = bquery.p, c.body_box = body_box;
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
     c.nsamples = nsamples;
-    c.itri = itri.p, c.n_itri = n_itri;
+    c.itri = itri.p, c.n_itri = n_itri, c.icap = icap.p, c.n_icap = n_icap, c.irange = irange.p;
     c.st = stat.p, c.icount = icount.p;
     c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
     c.tmask = tsize_f - 1, c.tcount = tcount.p, c.tstart = tstart.p;
@@ -1487,8 +1553,14 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
         }
         CHK(hipMemcpyAsync(hstat, stat.p, sizeof(RStat), hipMemcpyDeviceToHost, st));
         CHK(hipStreamSynchronize(st));
-        counts_dirty_ = false;
         last = *hstat;
+        // (a rebuild that timed out or overflowed may not have counted the buckets back down: cleared next time)
+        counts_dirty_ = (last.unsupported & 4) != 0 || last.overflow != 0;
+        if (last.unsupported & 8) {
+            blg::set_last_error("internal: a pair island's candidates are not one run in a list");
+            counts_dirty_ = true;
+            return BL_ERR_STATE;
+        }
         if ((last.unsupported & 4) && !lk_tickets_) {
             // a look-back wait timed out with workgroups ordered by index: tickets from now on, and again
             lk_tickets_ = true;
@@ -1521,10 +1593,9 @@ int ContactEngine::narrow_args(PArgs& p, uint32_t& nblk) {
     if (n == 0) return BL_OK;
     RC(hraw.ensure((size_t)nblk * 256));
     RC(bcnt.ensure(nblk));
-    RC(bpre.ensure((size_t)nblk + 1));
     p.nt = nt.p, p.nc = nc.p, p.ct = ct.p;
     p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct, p.st = stat.p;
-    p.hraw = hraw.p, p.bcnt = bcnt.p, p.bpre = bpre.p;
+    p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
     return BL_OK;
 }
 
@@ -1538,7 +1609,8 @@ int ContactEngine::collide(const KArgs& a, hipStream_t st, int parity) {
 }
 
 int ContactEngine::launch_response(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity) {
-    hipLaunchKernelGGL(k_response, dim3(1), dim3(kRespW), 0, st, a, p, parity, nblk);
+    (void)nblk;
+    hipLaunchKernelGGL(k_response, dim3(p.gresp), dim3(kRespW), 0, st, a, p, parity);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 
@@ -1548,16 +1620,15 @@ int ContactEngine::collide_fixed(PArgs& p, uint32_t& nblk) {
     if (nblk == 0) nblk = 1;
     RC(hraw.ensure((size_t)nblk * 256));
     RC(bcnt.ensure(nblk));
-    RC(bpre.ensure((size_t)nblk + 1));
     p = PArgs{};
     p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.st = stat.p;
-    p.hraw = hraw.p, p.bcnt = bcnt.p, p.bpre = bpre.p;
+    p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
     return BL_OK;
 }
 
 int ContactEngine::launch_collide(const KArgs& a, const PArgs& p, uint32_t nblk, hipStream_t st, int parity) {
     hipLaunchKernelGGL(k_narrow, dim3(nblk), dim3(256), 0, st, a, p, parity);
-    hipLaunchKernelGGL(k_response, dim3(1), dim3(kRespW), 0, st, a, p, parity, nblk);
+    hipLaunchKernelGGL(k_response, dim3(p.gresp), dim3(kRespW), 0, st, a, p, parity);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
 

@@ -334,14 +334,13 @@ def config4_body(n_nodes, origin, seed):
                       name=f"c4_body_{seed}")
 
 
-def config4_scene(n_bodies=512, arena=200.0, seed=4):
-    """The batched scene: n_bodies random beam networks (config4_sizes) in stacks of two on a grid over an
-    arena x arena ground, inter-body contacts on. Each stack's larger crate stands on the ground (its bottom 1 cm up,
-    the nodes' jitter 5 mm), the smaller one on it, its bottom 4 cm above the lower one's top: inside the triangles'
-    contact reach (the bodies' 5 cm collision radius), so every stack has inter-body contacts from the first
-    substep."""
-    sizes = config4_sizes(n_bodies, seed)
-    stacks = (n_bodies + 1) // 2
+def stacked_crates(sizes, arena, seed):
+    """Crates (config4_body, one per size) in stacks of two on a grid over an arena x arena ground: each stack's larger
+    crate stands on the ground (its bottom 1 cm up, the nodes' jitter 5 mm), the smaller one on it, its bottom 4 cm
+    above the lower one's top - inside the triangles' contact reach (the bodies' 5 cm collision radius), so every
+    stack has inter-body contacts from the first substep."""
+    n = len(sizes)
+    stacks = (n + 1) // 2
     g = int(math.ceil(math.sqrt(stacks)))
     pitch = arena / g
     rng = np.random.default_rng(seed + 1)
@@ -349,10 +348,16 @@ def config4_scene(n_bodies=512, arena=200.0, seed=4):
     for k in range(stacks):
         cx = -0.5 * arena + (k % g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
         cz = -0.5 * arena + (k // g + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
-        pair = sorted(range(2 * k, min(2 * k + 2, n_bodies)), key=lambda i: -int(sizes[i]))
+        pair = sorted(range(2 * k, min(2 * k + 2, n)), key=lambda i: -int(sizes[i]))
         y = 0.01
         for i in pair:
             b = config4_body(int(sizes[i]), (float(cx), y, float(cz)), seed=1000 + i)
             bodies.append(b)
             y = float(b.nodes["p"][:, 1].max()) + 0.04
     return bodies
+
+
+def config4_scene(n_bodies=512, arena=200.0, seed=4):
+    """The batched scene: n_bodies random beam networks (config4_sizes) in stacks of two (stacked_crates) over a
+    200 x 200 m arena, inter-body contacts on."""
+    return stacked_crates(config4_sizes(n_bodies, seed), arena, seed)

@@ -77,5 +77,13 @@ def case_c4_small():
     return abi.default_settings(), _boxes(GROUND), bodies, 700
 
 
+def case_crate_stacks():
+    """The batched scene's layout at a reduced size: 16 stacks of two crates (700-1100 nodes each) resting on each
+    other over a 40 x 40 m ground, about 10^4 contact hits a substep from the first, in 16 pair islands."""
+    rng = np.random.default_rng(45)
+    sizes = rng.integers(700, 1100, 32)
+    return abi.default_settings(), _boxes(GROUND), scenes.stacked_crates(sizes, 40.0, 45), 150
+
+
 CONTACT_CASES = {"balls_on_lattice": case_balls_on_lattice, "lattice_pile": case_lattice_pile,
-                 "groups_islands": case_groups_islands, "c4_small": case_c4_small}
+                 "groups_islands": case_groups_islands, "c4_small": case_c4_small, "crate_stacks": case_crate_stacks}
