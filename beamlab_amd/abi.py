@@ -3,8 +3,10 @@
 The record dtypes are layout-identical to the reference's Node / NodeInfo / Beam / Triangle / Capsule
 (src/phys/softbody.h:43-196), as the C header is.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -179,6 +181,26 @@ class BeamlabError(RuntimeError):
         self.status = status
 
 
+_live = {}  # objects holding library handles (World, ShellBody, Network), in creation order: closed at exit
+
+
+def track(obj):
+    """Registers an object whose close() releases library handles: the interpreter's exit closes it, newest first,
+    before the HIP runtime (and a profiler attached to it) tear down."""
+    _live[id(obj)] = weakref.ref(obj)
+
+
+def _close_all():
+    for ref in reversed(list(_live.values())):
+        obj = ref()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _live.clear()
+
+
 def load():
     """Loads libbeamlab_gpu.so (built by beamlab_amd.build); raises if it is missing - there is no fallback."""
     global _lib
@@ -192,6 +214,7 @@ def load():
         fn.restype = res
         fn.argtypes = args
     _lib = lib
+    atexit.register(_close_all)
     return lib
 
 

@@ -60,6 +60,7 @@ class ShellBody:
         abi.check(self._lib.bl_fem_create(ctypes.byref(h), ctypes.byref(d)), "bl_fem_create")
         self._h = h
         self.report = abi.FemReport()
+        abi.track(self)
 
     @classmethod
     def plate(cls, nu, nv, size=1.0, y0=0.0, section=SECTION_PLATE, clamped=True):

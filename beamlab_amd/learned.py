@@ -48,6 +48,7 @@ class Network:
         h = ctypes.c_void_p()
         abi.check(self._lib.bl_gnn_create(ctypes.byref(h), abi.ptr(w), self.n), "bl_gnn_create")
         self._h = h
+        abi.track(self)
 
     @classmethod
     def load(cls, path=WEIGHTS):

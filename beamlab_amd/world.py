@@ -127,6 +127,7 @@ class World:
         self.bodies = []
         self._ctx = None
         self._statics_sent = None
+        abi.track(self)
 
     def _lib(self):
         return abi.load()

@@ -2,10 +2,11 @@
 
 Workload (the single-large-body configuration): the 64x32x24 lattice (49 152 nodes, 420 848 beams, per-beam k
 log-uniform in [1e5, 5e6] N/m, 2 % of the beams weakened to break strain 0.01) with its faces as collision
-triangles and self contact on, resting on a concrete ground box, and 2 000 rigid balls (radius 0.1-0.3 m, 5 kg) on
-its top and around it; inter-body contacts through the spatial hash (pair rebuild every 17 substeps, narrow phase
-and ordered response every substep). 4000 substeps of 0.5 ms in frames of 33 substeps (the app's 60 Hz frame). One
-step = one substep of every beam, node and contact. Inputs are resident in HBM before the timed region.
+triangles and self contact on, standing on a concrete ground box, and 2 000 rigid balls (radius 0.1-0.3 m, 5 kg) resting on its top and around
+it; inter-body contacts through the spatial hash (pair rebuild every 17 substeps, narrow phase and ordered response
+every substep), ball-lattice contacts in every substep from the first. 4000 substeps of 0.5 ms in frames of 33
+substeps (the app's 60 Hz frame). One step = one substep of every beam, node and contact. Inputs are resident in HBM
+before the timed region; the rate is on the wall clock around the timed frames.
 
 Prints one JSON line with the throughput, the dominant kernel's HBM roofline figures (achieved = algorithmic bytes
 per launch / its average duration, from HIP events around each launch on the step's stream) and the compiled
@@ -52,9 +53,23 @@ def build_world(bodies):
     return w
 
 
-def cpu_baseline(bodies, sample_substeps, threads):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(bodies, sample_substeps, threads, runs=5):
     """The compiled reference's World::step_substeps (oracle/_ref) on the same scene; the C restatement when the
-    reference library is absent."""
+    reference library is absent. A warm-up of 2 frames, then `runs` timed samples of sample_substeps / runs substeps
+    each, in frames of 33; the median sample's rate."""
+    import statistics
     from beamlab_amd import abi, scenes
     from oracle import oracle
     w = scenes.World()
@@ -65,16 +80,21 @@ def cpu_baseline(bodies, sample_substeps, threads):
         cw, kind, threads = oracle.OracleWorld(abi.default_settings(), w.statics.boxes), "port", 1
     for b in bodies:
         cw.add_body(b)
-    cw.step_substeps(FRAME)  # warm-up
-    frames = sample_substeps // FRAME
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        cw.step_substeps(FRAME)
-    dt = time.perf_counter() - t0
+    cw.step_substeps(2 * FRAME)  # warm-up
+    frames = max(1, sample_substeps // FRAME // runs)
+    rates, total = [], 0.0
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            cw.step_substeps(FRAME)
+        dt = time.perf_counter() - t0
+        total += dt
+        rates.append(len(bodies[0].beams) * frames * FRAME / dt)
     n = frames * FRAME
-    return {"value": len(bodies[0].beams) * n / dt, "unit": "beam updates/s", "cores": threads, "kind": kind,
-            "sample": f"{n} substeps of the same scene (lattice and 2000 balls) in {frames} frames of {FRAME} "
-                      f"({dt:.1f} s)"}
+    return {"value": statistics.median(rates), "unit": "beam updates/s", "cores": threads, "kind": kind,
+            "cpu_model": cpu_model(), "runs": [round(r, 1) for r in rates],
+            "sample": f"median of {runs} samples of {n} substeps each ({frames} frames of {FRAME}) of the same scene "
+                      f"(lattice and 2000 balls), after a 2-frame warm-up; {total:.1f} s timed"}
 
 
 def batched_leg(steps, retune, log):
@@ -91,21 +111,30 @@ def batched_leg(steps, retune, log):
     w.step_substeps(64)
     t_ms = 0.0
     done = 0
+    st0 = w.stats()
+    t0 = time.perf_counter()
     while done < steps:
         k = min(32, steps - done)
         t_ms += w.time_substeps(k)
         done += k
+    wall = time.perf_counter() - t0
     st = w.stats()
     tun = w.get_tuning()
     w.close()
+    contacts = (st["contacts"] - st0["contacts"]) / done
+    if contacts <= 0:
+        raise RuntimeError("the batched scene's timed substeps had no inter-body contacts")
     alg = 52 * nb + 56 * nn  # the beam-path substep's algorithmic bytes (beam records + CSR, node state)
     achieved = alg / (t_ms / 1e3 / done) / 1e9
-    return {"value": nb * done / (t_ms / 1e3), "unit": "beam updates/s", "ms_per_step": t_ms / done, "steps": done,
+    return {"value": nb * done / wall, "unit": "beam updates/s", "ms_per_step": wall / done * 1e3,
+            "stream_ms_per_step": t_ms / done, "steps": done,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "the whole substep",
                          "alg_bytes_per_substep": alg},
             "config": {"workload": "512 hollow crates (random struts, mean degree 7), lognormal sizes (median 2000, "
-                                   "500-8000 nodes), 200 x 200 m arena, inter-body contacts", "bodies": len(bodies),
+                                   "500-8000 nodes), in stacks of two (the smaller resting on the larger) over a "
+                                   "200 x 200 m arena, inter-body contacts", "bodies": len(bodies),
+                       "contacts_per_substep": contacts,
                        "nodes": nn, "beams": nb, "frame_substeps": 32,
                        "form": {1: "stepped", 2: "persistent", 3: "resident", 4: "halo"}.get(tun.mode, "?"),
                        "block": tun.block, "incidences_per_thread": tun.incidences_per_thread,
@@ -384,9 +413,11 @@ def main():
         k = min(FRAME, args.warmup - done)
         w.step_substeps(k)
         done += k
-    # timed region: frames of 33 substeps, inputs resident (each frame's wall time on the stream)
+    # timed region: frames of 33 substeps, inputs resident; the wall clock around all of it (the host's frame work
+    # included), each frame's time on the stream beside it
     t_ms = 0.0
     done = 0
+    st0 = w.stats()
     t0 = time.perf_counter()
     while done < args.steps:
         k = min(FRAME, args.steps - done)
@@ -394,6 +425,9 @@ def main():
         done += k
     wall = time.perf_counter() - t0
     st = w.stats()
+    contacts = (st["contacts"] - st0["contacts"]) / args.steps
+    if contacts <= 0:
+        raise RuntimeError("the contact scene's timed substeps had no contacts")
     # the dominant kernel: the force phase (beams, yield and break, gather into F), timed launch by launch
     tot_ms, ker_ms, launches = w.time_kernels(FRAME)
     avg_launch_s = ker_ms / launches / 1e3
@@ -403,12 +437,12 @@ def main():
     w.close()
     out = {
         "metric": METRIC,
-        "value": nb * args.steps / (t_ms / 1e3),
+        "value": nb * args.steps / wall,
         "unit": "beam updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_ms / args.steps,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -417,7 +451,8 @@ def main():
         "config": {"workload": "64x32x24 beam lattice, faces as triangles, self contact, 2000 rigid balls, ground box",
                    "nodes": nn, "beams": nb, "bodies": len(bodies), "substep_s": 0.0005, "substeps": args.steps,
                    "frame_substeps": FRAME, "block": tun.block, "incidences_per_thread": tun.incidences_per_thread,
-                   "wall_s": wall, "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"],
+                   "wall_s": wall, "stream_ms_per_step": t_ms / args.steps, "contacts_per_substep": contacts,
+                   "contacts": st["contacts"], "pair_rebuilds": st["pair_rebuilds"],
                    "narrow_tests": st["narrow_tests"], "broken_beams": st["broken_beams"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(tun), "kernel": "k_forces",
