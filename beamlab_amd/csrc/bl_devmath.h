@@ -307,6 +307,47 @@ BL_HD float bl_atan2f(float y, float x) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------- acosf
+// glibc's float acosf (sysdeps/ieee754/flt-32 e_acosf.c: fdlibm's, float version by Cygnus): acos(x) = pi/2 - x - x r(x^2)
+// below |x| 0.5, 2 asin(sqrt((1 - |x|) / 2)) above, r a rational approximation (degree 6 over 4) in float; every
+// operation in float, unfused. Equal to the host libm's on every float input (tests/test_devmath.py).
+BL_HD float bl_acosf(float x) {
+    const float one = 1.0f, pi = 3.1415925026e+00f, pio2_hi = 1.5707962513e+00f, pio2_lo = 7.5497894159e-08f;
+    const float pS0 = 1.6666667163e-01f, pS1 = -3.2556581497e-01f, pS2 = 2.0121252537e-01f, pS3 = -4.0055535734e-02f,
+                pS4 = 7.9153501429e-04f, pS5 = 3.4793309169e-05f, qS1 = -2.4033949375e+00f, qS2 = 2.0209457874e+00f,
+                qS3 = -6.8828397989e-01f, qS4 = 7.7038154006e-02f;
+    const uint32_t hx = f2u(x), ix = hx & 0x7fffffffu;
+    if (ix == 0x3f800000u) return (hx >> 31) ? pi + 2.0f * pio2_lo : 0.0f; // |x| = 1
+    if (ix > 0x3f800000u) return (x - x) / (x - x);                         // |x| > 1: NaN
+    if (ix < 0x3f000000u) {                                                 // |x| < 0.5
+        if (ix <= 0x23000000u) return pio2_hi + pio2_lo;                   // |x| < 2^-57
+        const float z = x * x;
+        const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+        const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+        const float r = p / q;
+        return pio2_hi - (x - (pio2_lo - x * r));
+    }
+    if (hx >> 31) { // x < -0.5
+        const float z = (one + x) * 0.5f;
+        const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+        const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+        const float s = fsqrt(z);
+        const float r = p / q;
+        const float w = r * s - pio2_lo;
+        return pi - 2.0f * (s + w);
+    }
+    // x > 0.5
+    const float z = (one - x) * 0.5f;
+    const float s = fsqrt(z);
+    const float df = u2f(f2u(s) & 0xfffff000u);
+    const float c = (z - df * df) / (s + df);
+    const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    const float r = p / q;
+    const float w = r * s + c;
+    return 2.0f * (df + w);
+}
+
 // The ground models (src/phys/static_world.cpp:85-115: RoR's ground_models.cfg values; alpha 2, strength 1 for all).
 struct ground_model {
     float va, ms, mc, t2, vs, alpha, strength, soft_depth, fluid_drag, fluid_density;

@@ -208,12 +208,20 @@ struct orc_fem {
     double* xd;
     float *p, *inv_mass, *force; /* the body's Node p and inv_mass; the forces the update hands the integrator */
     float* impulse;              /* gather_part's held impulse J (3 per node): f (-(h - step)), 0 for a step of h */
+    float* torque;               /* FemFrame::torque: moments from outside for the next step (3 per node) */
     float *v, *w, *q, *mass, *inertia, *pos0;
     int64_t failures, clamps, nan_resets;
     uint8_t* fixed;
     uint32_t* tn;
     float *X, *r0, *Kf, *tmass, *area0;
     float E, nu, rho, t, drill, damping;
+    /* members (FrameElement): ends, section, joints; rest length, rest frames in the end nodes' frames, the elastic
+     * stiffness (ka, kt, cy[3], cz[3]) and where the co-rotated frame takes its twist from */
+    uint32_t nm, nsec;
+    uint32_t* mem;               /* 4 per member: a, b, section, end_a | end_b << 8 */
+    float *mL0, *mqa, *mqb, *mst, *mtw;
+    uint8_t* mbroken;
+    float* sec;                  /* ORC_FRAME_SECTION_FIELDS per section */
 };
 
 int orc_fem_rest_shape(const float p[9], float X[6]) {
@@ -228,78 +236,145 @@ int orc_fem_rest_shape(const float p[9], float X[6]) {
     return 0;
 }
 
-orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6]) {
+/* end_flex, condense, ends2, stiffness (frame_fem.cpp:1412-1476): the member's elastic stiffness in float */
+static void mem_condense(float c[3], int pin_a, int pin_b) {
+    if (pin_a && pin_b) c[0] = c[1] = c[2] = 0;
+    else if (pin_a) c[2] = c[0] > 0 ? c[2] - c[1] * c[1] / c[0] : 0, c[0] = c[1] = 0;
+    else if (pin_b) c[0] = c[2] > 0 ? c[0] - c[1] * c[1] / c[2] : 0, c[1] = c[2] = 0;
+}
+typedef struct { int rel[3]; float f; } end_flex_t;
+static end_flex_t mem_end_flex(uint32_t j, const float* s) {
+    end_flex_t e = {{0, 0, 0}, 0.0f};
+    switch (j) {
+    case 1: e.rel[0] = e.rel[1] = e.rel[2] = 1; break; /* FJ_BALL */
+    case 2: e.rel[2] = 1; break;                       /* FJ_HINGE_V */
+    case 3: e.rel[1] = 1; break;                       /* FJ_HINGE_H */
+    case 4: e.rel[0] = 1; break;                       /* FJ_SWIVEL */
+    case 5: e.f = s[ORC_SEC_JOINT_K] > 0 ? 1.0f / s[ORC_SEC_JOINT_K] : 0.0f; break; /* FJ_ELASTIC */
+    default: break;
+    }
+    return e;
+}
+static void mem_ends2(float c[3], int ra, int rb, float fa, float fb) {
+    if (ra || rb) {
+        mem_condense(c, ra, rb);
+        if (ra != rb) {
+            float* k = ra ? &c[2] : &c[0];
+            const float fl = ra ? fb : fa;
+            if (fl > 0 && *k > 0) *k = 1.0f / (1.0f / *k + fl);
+        }
+        return;
+    }
+    if (fa <= 0 && fb <= 0) return;
+    const double det = (double)c[0] * c[2] - (double)c[1] * c[1];
+    if (!(det > 0)) return;
+    const double F00 = c[2] / det + fa, F01 = -c[1] / det, F11 = c[0] / det + fb;
+    const double d2 = F00 * F11 - F01 * F01;
+    c[0] = (float)(F11 / d2), c[1] = (float)(-F01 / d2), c[2] = (float)(F00 / d2);
+}
+static void mem_bend(const float* s, float L, float I, float As, float c[3], int axis, end_flex_t fa, end_flex_t fb) {
+    const float E = s[ORC_SEC_E], G = s[ORC_SEC_G];
+    const float phi = As > 0 ? 12.0f * E * I / (G * As * L * L) : 0.0f;
+    const float k0 = E * I / (L * (1.0f + phi));
+    c[0] = c[2] = (4.0f + phi) * k0;
+    c[1] = (2.0f - phi) * k0;
+    mem_ends2(c, fa.rel[axis], fb.rel[axis], fa.f, fb.f);
+}
+static void mem_stiffness(const float* s, float L, uint32_t end_a, uint32_t end_b, float st[8]) {
+    const end_flex_t fa = mem_end_flex(end_a, s), fb = mem_end_flex(end_b, s);
+    st[0] = s[ORC_SEC_AXIAL] * s[ORC_SEC_E] * s[ORC_SEC_A] / L;
+    st[1] = s[ORC_SEC_G] * s[ORC_SEC_J] / L;
+    if (fa.rel[0] || fb.rel[0]) st[1] = 0;
+    else if (fa.f + fb.f > 0) st[1] = 1.0f / (1.0f / st[1] + fa.f + fb.f);
+    mem_bend(s, L, s[ORC_SEC_IY], s[ORC_SEC_ASZ], &st[2], 1, fa, fb);
+    mem_bend(s, L, s[ORC_SEC_IZ], s[ORC_SEC_ASY], &st[5], 2, fa, fb);
+}
+
+/* A frame body (FemFrame with its nodes in the body's order): nodes p, mass (as the body has them), pinned flags;
+ * shell triangles of one section (E, nu, rho, t, drill, damping); members (a, b, section, end_a | end_b << 8) of
+ * elastic sections (ORC_FRAME_SECTION_FIELDS floats each). FemFrame::finalize (frame_fem.cpp:240-290) and
+ * node_inertia (:3075-3086). */
+orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
+                        const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
+                        const float* sections) {
     orc_fem* f = (orc_fem*)calloc(1, sizeof(orc_fem));
-    f->E = sec[0], f->nu = sec[1], f->rho = sec[2], f->t = sec[3], f->drill = sec[4], f->damping = sec[5];
-    const uint32_t nn = (uint32_t)((nu + 1) * (nv + 1)), nt = (uint32_t)(2 * nu * nv);
-    f->nn = nn, f->nt = nt;
-    f->xd = (double*)calloc(3 * nn, sizeof(double));
-    f->pos0 = (float*)calloc(3 * nn, sizeof(float));
-    f->p = (float*)calloc(3 * nn, sizeof(float));
-    f->inv_mass = (float*)calloc(nn, sizeof(float));
-    f->force = (float*)calloc(3 * nn, sizeof(float));
-    f->impulse = (float*)calloc(3 * nn, sizeof(float));
-    f->v = (float*)calloc(3 * nn, sizeof(float));
-    f->w = (float*)calloc(3 * nn, sizeof(float));
-    f->q = (float*)calloc(4 * nn, sizeof(float));
-    f->mass = (float*)calloc(nn, sizeof(float));
-    f->inertia = (float*)calloc(nn, sizeof(float));
-    f->fixed = (uint8_t*)calloc(nn, 1);
-    f->tn = (uint32_t*)calloc(3 * nt, sizeof(uint32_t));
-    f->X = (float*)calloc(6 * nt, sizeof(float));
-    f->r0 = (float*)calloc(12 * nt, sizeof(float));
-    f->Kf = (float*)calloc(162 * (size_t)nt, sizeof(float));
-    f->tmass = (float*)calloc(nt, sizeof(float));
-    f->area0 = (float*)calloc(nt, sizeof(float));
-    /* ShellMesher::grid: origin + du i + dv j (float vector arithmetic) */
-    const f3 origin = F3(-0.5f * size, y0, -0.5f * size), du = F3(size / (float)nu, 0, 0), dv = F3(0, 0, size / (float)nv);
-    for (int j = 0; j <= nv; j++)
-        for (int i = 0; i <= nu; i++) {
-            const uint32_t id = (uint32_t)(j * (nu + 1) + i);
-            const f3 p = add3(add3(origin, mul3(du, (float)i)), mul3(dv, (float)j));
-            f->pos0[3 * id] = p.x, f->pos0[3 * id + 1] = p.y, f->pos0[3 * id + 2] = p.z;
-            f->p[3 * id] = p.x, f->p[3 * id + 1] = p.y, f->p[3 * id + 2] = p.z;
-            f->xd[3 * id] = p.x, f->xd[3 * id + 1] = p.y, f->xd[3 * id + 2] = p.z;
-            f->q[4 * id + 3] = 1.0f;
-            f->fixed[id] = (i == 0 || j == 0 || i == nu || j == nv);
+    f->E = shell[0], f->nu = shell[1], f->rho = shell[2], f->t = shell[3], f->drill = shell[4], f->damping = shell[5];
+    f->nn = nn, f->nt = nt, f->nm = nm, f->nsec = nsec;
+    f->xd = (double*)calloc(3 * nn + 1, sizeof(double));
+    f->pos0 = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->p = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->inv_mass = (float*)calloc(nn + 1, sizeof(float));
+    f->force = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->impulse = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->torque = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->v = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->w = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->q = (float*)calloc(4 * nn + 1, sizeof(float));
+    f->mass = (float*)calloc(nn + 1, sizeof(float));
+    f->inertia = (float*)calloc(nn + 1, sizeof(float));
+    f->fixed = (uint8_t*)calloc(nn + 1, 1);
+    f->tn = (uint32_t*)calloc(3 * nt + 1, sizeof(uint32_t));
+    f->X = (float*)calloc(6 * nt + 1, sizeof(float));
+    f->r0 = (float*)calloc(12 * nt + 1, sizeof(float));
+    f->Kf = (float*)calloc(162 * (size_t)nt + 1, sizeof(float));
+    f->tmass = (float*)calloc(nt + 1, sizeof(float));
+    f->area0 = (float*)calloc(nt + 1, sizeof(float));
+    f->mem = (uint32_t*)calloc(4 * nm + 1, sizeof(uint32_t));
+    f->mL0 = (float*)calloc(nm + 1, sizeof(float));
+    f->mqa = (float*)calloc(4 * nm + 1, sizeof(float));
+    f->mqb = (float*)calloc(4 * nm + 1, sizeof(float));
+    f->mst = (float*)calloc(8 * nm + 1, sizeof(float));
+    f->mtw = (float*)calloc(nm + 1, sizeof(float));
+    f->mbroken = (uint8_t*)calloc(nm + 1, 1);
+    f->sec = (float*)calloc(ORC_FRAME_SECTION_FIELDS * nsec + 1, sizeof(float));
+    memcpy(f->pos0, p, 3 * sizeof(float) * nn);
+    memcpy(f->p, p, 3 * sizeof(float) * nn);
+    for (uint32_t i = 0; i < 3 * nn; i++) f->xd[i] = p[i];
+    memcpy(f->mass, mass, sizeof(float) * nn);
+    if (fixed) memcpy(f->fixed, fixed, nn);
+    for (uint32_t i = 0; i < nn; i++) {
+        f->q[4 * i + 3] = 1.0f;
+        f->inv_mass[i] = f->fixed[i] || f->mass[i] <= 0 ? 0.0f : 1.0f / f->mass[i];
+    }
+    if (nt) memcpy(f->tn, tris, 3 * sizeof(uint32_t) * nt);
+    if (nm) memcpy(f->mem, members, 4 * sizeof(uint32_t) * nm);
+    if (nsec) memcpy(f->sec, sections, ORC_FRAME_SECTION_FIELDS * sizeof(float) * nsec);
+    /* the members' rest length and frames (x along the member, y towards the world's up; a vertical member's
+     * towards x), their stiffness and twist weight (twist_weight, :1372-1375) */
+    for (uint32_t k = 0; k < nm; k++) {
+        const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1], ea = f->mem[4 * k + 3] & 0xff, eb = f->mem[4 * k + 3] >> 8;
+        const float* s = &f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2]];
+        const f3 xa = F3(p[3 * a], p[3 * a + 1], p[3 * a + 2]), xb = F3(p[3 * b], p[3 * b + 1], p[3 * b + 2]);
+        const f3 d = sub3(xb, xa);
+        const float L0 = len3(d);
+        f->mL0[k] = L0;
+        if (L0 < 1e-5f || a == b) {
+            f->mbroken[k] = 1;
+            continue;
         }
-    uint32_t t = 0;
+        const f3 e1 = div3(d, L0);
+        const f3 ref = fabsf(e1.y) < 0.9f ? F3(0, 1, 0) : F3(1, 0, 0);
+        const f3 e2 = norm3(sub3(ref, mul3(e1, dot3(e1, ref)))), e3 = cross3(e1, e2);
+        const qf E0 = from_cols(e1, e2, e3);
+        const qf qa = qnorm(qmul(qconj(Q(0, 0, 0, 1)), E0)), qb = qnorm(qmul(qconj(Q(0, 0, 0, 1)), E0));
+        f->mqa[4 * k] = qa.x, f->mqa[4 * k + 1] = qa.y, f->mqa[4 * k + 2] = qa.z, f->mqa[4 * k + 3] = qa.w;
+        f->mqb[4 * k] = qb.x, f->mqb[4 * k + 1] = qb.y, f->mqb[4 * k + 2] = qb.z, f->mqb[4 * k + 3] = qb.w;
+        mem_stiffness(s, L0, ea, eb, &f->mst[8 * k]);
+        const int fa = ea == 1 || ea == 4, fb = eb == 1 || eb == 4;
+        f->mtw[k] = fa == fb ? 0.5f : fa ? 1.0f : 0.0f;
+    }
+    /* the triangles: rest shapes, rest frames, stored stiffness (tri_build_k), masses (tri_mass) */
     const float mpa = f->rho * f->t; /* ShellSection::mass_per_m2 */
-    for (int j = 0; j < nv; j++)
-        for (int i = 0; i < nu; i++) {
-            const uint32_t n00 = (uint32_t)(j * (nu + 1) + i), n10 = n00 + 1, n01 = n00 + (uint32_t)(nu + 1), n11 = n01 + 1;
-            uint32_t tri2[2][3];
-            if ((i + j) & 1) {
-                uint32_t a[2][3] = {{n00, n10, n11}, {n00, n11, n01}};
-                memcpy(tri2, a, sizeof(a));
-            } else {
-                uint32_t a[2][3] = {{n00, n10, n01}, {n10, n11, n01}};
-                memcpy(tri2, a, sizeof(a));
-            }
-            for (int k = 0; k < 2; k++, t++) {
-                memcpy(&f->tn[3 * t], tri2[k], sizeof(tri2[k]));
-                const float* pa = &f->pos0[3 * tri2[k][0]];
-                const float* pb = &f->pos0[3 * tri2[k][1]];
-                const float* pc = &f->pos0[3 * tri2[k][2]];
-                const f3 A = F3(pa[0], pa[1], pa[2]), B = F3(pb[0], pb[1], pb[2]), C = F3(pc[0], pc[1], pc[2]);
-                const float area = 0.5f * len3(cross3(sub3(B, A), sub3(C, A)));
-                const float m = mpa * area;
-                f->tmass[t] = m;
-                for (int c = 0; c < 3; c++) f->mass[tri2[k][c]] += m / 3.0f;
-            }
-        }
-    /* ShellMesher::finish (fem_shell.cpp:48-53): inverse masses, 0 for the clamped nodes */
-    for (uint32_t i = 0; i < nn; i++) f->inv_mass[i] = f->fixed[i] || f->mass[i] <= 0 ? 0.0f : 1.0f / f->mass[i];
-    /* finalize: rest shapes, rest frames, stored stiffness (tri_build_k) */
     for (uint32_t k = 0; k < nt; k++) {
-        float p[9];
-        for (int c = 0; c < 3; c++) memcpy(&p[3 * c], &f->pos0[3 * f->tn[3 * k + c]], 3 * sizeof(float));
-        orc_fem_rest_shape(p, &f->X[6 * k]);
-        const f3 x0 = F3(p[0], p[1], p[2]), x1 = F3(p[3], p[4], p[5]), x2 = F3(p[6], p[7], p[8]);
+        float pp[9];
+        for (int c = 0; c < 3; c++) memcpy(&pp[3 * c], &f->pos0[3 * f->tn[3 * k + c]], 3 * sizeof(float));
+        orc_fem_rest_shape(pp, &f->X[6 * k]);
+        const f3 x0 = F3(pp[0], pp[1], pp[2]), x1 = F3(pp[3], pp[4], pp[5]), x2 = F3(pp[6], pp[7], pp[8]);
         const f3 nn3 = cross3(sub3(x1, x0), sub3(x2, x0));
         const float a2 = len3(nn3);
         const f3 e3 = div3(nn3, a2), e1 = norm3(sub3(x1, x0)), e2 = cross3(e3, e1);
         f->area0[k] = 0.5f * a2;
+        f->tmass[k] = mpa * f->area0[k];
         const qf E0 = from_cols(e1, e2, e3);
         for (int c = 0; c < 3; c++) {
             const float* qn = &f->q[4 * f->tn[3 * k + c]];
@@ -317,8 +392,15 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
                 P[81 + a * 9 + c] = (float)K[ip < jp ? ip : jp][ip < jp ? jp : ip];
             }
     }
-    /* node_inertia: a corner's share of its triangles, with the node's floor */
+    /* node_inertia: the members' halves (m L^2 / 78 per end, element order), then a corner's share of its triangles,
+     * with the node's floor */
     for (uint32_t i = 0; i < nn; i++) f->inertia[i] = 0;
+    for (uint32_t k = 0; k < nm; k++) {
+        if (f->mbroken[k]) continue;
+        const float* s = &f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2]];
+        const float em = s[ORC_SEC_RHO] * s[ORC_SEC_A] * f->mL0[k]; /* FrameSection::mass_per_m() L0 */
+        for (int c = 0; c < 2; c++) f->inertia[f->mem[4 * k + c]] += em * f->mL0[k] * f->mL0[k] / 78.0f;
+    }
     for (uint32_t k = 0; k < nt; k++)
         for (int c = 0; c < 3; c++) f->inertia[f->tn[3 * k + c]] += f->tmass[k] * f->area0[k] / 108.0f;
     for (uint32_t i = 0; i < nn; i++) {
@@ -328,8 +410,54 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
     return f;
 }
 
+orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6]) {
+    const uint32_t nn = (uint32_t)((nu + 1) * (nv + 1)), nt = (uint32_t)(2 * nu * nv);
+    float* p = (float*)calloc(3 * nn, sizeof(float));
+    float* mass = (float*)calloc(nn, sizeof(float));
+    uint8_t* fixed = (uint8_t*)calloc(nn, 1);
+    uint32_t* tn = (uint32_t*)calloc(3 * nt, sizeof(uint32_t));
+    /* ShellMesher::grid: origin + du i + dv j (float vector arithmetic) */
+    const f3 origin = F3(-0.5f * size, y0, -0.5f * size), du = F3(size / (float)nu, 0, 0), dv = F3(0, 0, size / (float)nv);
+    for (int j = 0; j <= nv; j++)
+        for (int i = 0; i <= nu; i++) {
+            const uint32_t id = (uint32_t)(j * (nu + 1) + i);
+            const f3 q = add3(add3(origin, mul3(du, (float)i)), mul3(dv, (float)j));
+            p[3 * id] = q.x, p[3 * id + 1] = q.y, p[3 * id + 2] = q.z;
+            fixed[id] = (i == 0 || j == 0 || i == nu || j == nv);
+        }
+    uint32_t t = 0;
+    const float mpa = sec[2] * sec[3]; /* ShellSection::mass_per_m2 */
+    for (int j = 0; j < nv; j++)
+        for (int i = 0; i < nu; i++) {
+            const uint32_t n00 = (uint32_t)(j * (nu + 1) + i), n10 = n00 + 1, n01 = n00 + (uint32_t)(nu + 1), n11 = n01 + 1;
+            uint32_t tri2[2][3];
+            if ((i + j) & 1) {
+                uint32_t a[2][3] = {{n00, n10, n11}, {n00, n11, n01}};
+                memcpy(tri2, a, sizeof(a));
+            } else {
+                uint32_t a[2][3] = {{n00, n10, n01}, {n10, n11, n01}};
+                memcpy(tri2, a, sizeof(a));
+            }
+            for (int k = 0; k < 2; k++, t++) {
+                memcpy(&tn[3 * t], tri2[k], sizeof(tri2[k]));
+                const float* pa = &p[3 * tri2[k][0]];
+                const float* pb = &p[3 * tri2[k][1]];
+                const float* pc = &p[3 * tri2[k][2]];
+                const f3 A = F3(pa[0], pa[1], pa[2]), B = F3(pb[0], pb[1], pb[2]), C = F3(pc[0], pc[1], pc[2]);
+                const float area = 0.5f * len3(cross3(sub3(B, A), sub3(C, A)));
+                const float m = mpa * area; /* ShellMesher::tri: a third on each corner */
+                for (int c = 0; c < 3; c++) mass[tri2[k][c]] += m / 3.0f;
+            }
+        }
+    orc_fem* f = orc_fem_create(nn, p, mass, fixed, nt, tn, sec, 0, NULL, 0, NULL);
+    free(p), free(mass), free(fixed), free(tn);
+    return f;
+}
+
 void orc_fem_destroy(orc_fem* f) {
     if (!f) return;
+    free(f->torque), free(f->mem), free(f->mL0), free(f->mqa), free(f->mqb), free(f->mst), free(f->mtw), free(f->mbroken);
+    free(f->sec);
     free(f->xd), free(f->pos0), free(f->p), free(f->inv_mass), free(f->force), free(f->impulse), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
     free(f->tn), free(f->X), free(f->r0), free(f->Kf), free(f->tmass), free(f->area0);
     free(f);
@@ -515,6 +643,175 @@ static void eval_tri(const orc_fem* f, uint32_t k, tri_out* o) {
                 o->kv[6 * i + 3 + p] = E[p][0] * fl[6 * i + 3] + E[p][1] * fl[6 * i + 4] + E[p][2] * fl[6 * i + 5];
             }
     }
+}
+
+/* ------------------------------------------------------------------ members (frame_fem.cpp:1384-1405, 1896-2070) */
+static inline qf qadd(qf a, qf b) { return Q(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline float qdot(qf a, qf b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+/* quat::rotate (core/math.h:150-154) */
+static f3 qrot(qf q, f3 p) {
+    const f3 u = F3(q.x, q.y, q.z);
+    const f3 t = mul3(cross3(u, p), 2.0f);
+    return add3(add3(p, mul3(t, q.w)), cross3(u, t));
+}
+/* quat::axis_angle, any_perpendicular, quat_from_to, slerp (core/math.h:82-85, 136-140, 203-212, 227-233) */
+static qf axis_angle(f3 axis, float angle) {
+    const float hh = angle * 0.5f, s = sinf(hh);
+    const f3 a = norm3(axis);
+    return Q(a.x * s, a.y * s, a.z * s, cosf(hh));
+}
+static qf quat_from_to(f3 a, f3 b) {
+    const float d = dot3(a, b);
+    if (d < -0.999999f) {
+        const f3 t = fabsf(a.x) < 0.7f ? F3(1, 0, 0) : F3(0, 1, 0);
+        return axis_angle(norm3(cross3(a, t)), 3.14159265358979323846f);
+    }
+    const f3 c = cross3(a, b);
+    return qnorm(Q(c.x, c.y, c.z, 1.0f + d));
+}
+static qf qslerp(qf a, qf b, float t) {
+    float d = qdot(a, b);
+    if (d < 0) b = qscale(b, -1.0f), d = -d;
+    if (d > 0.9995f) return qnorm(qadd(a, qscale(qadd(b, qscale(a, -1.0f)), t)));
+    const float th = acosf(d), s = sinf(th);
+    return qadd(qscale(a, sinf((1 - t) * th) / s), qscale(b, sinf(t * th) / s));
+}
+
+typedef struct {
+    f3 e1, e2, e3;
+    float L;
+    double dL;
+    f3 ra, rb;
+} mem_kin_t;
+
+/* kinematics (:1384-1405): the chord, the co-rotated frame (the ends' frames, their mean turned onto the chord), the
+ * end rotations against it */
+static int mem_kin(const orc_fem* f, uint32_t k, mem_kin_t* K) {
+    const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
+    const double* xa = &f->xd[3 * a];
+    const double* xb = &f->xd[3 * b];
+    const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+    const double L = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(L > 1e-6) || !isfinite(L)) return 0;
+    K->L = (float)L;
+    K->dL = L - (double)f->mL0[k];
+    K->e1 = F3((float)(dx / L), (float)(dy / L), (float)(dz / L));
+    const float* qa = &f->mqa[4 * k];
+    const float* qb = &f->mqb[4 * k];
+    const float* na = &f->q[4 * a];
+    const float* nb = &f->q[4 * b];
+    const qf Ea = qnorm(qmul(Q(na[0], na[1], na[2], na[3]), Q(qa[0], qa[1], qa[2], qa[3])));
+    const qf Eb = qnorm(qmul(Q(nb[0], nb[1], nb[2], nb[3]), Q(qb[0], qb[1], qb[2], qb[3])));
+    const float t = f->mtw[k];
+    const qf Em = t <= 0.0f ? Ea : t >= 1.0f ? Eb : qslerp(Ea, Eb, t);
+    const qf E = qnorm(qmul(quat_from_to(norm3(qrot(Em, F3(1, 0, 0))), K->e1), Em));
+    const qf Ec = qconj(E);
+    K->ra = qlog(qmul(Ec, Ea));
+    K->rb = qlog(qmul(Ec, Eb));
+    K->e2 = qrot(E, F3(0, 1, 0));
+    K->e3 = qrot(E, F3(0, 0, 1));
+    return 1;
+}
+
+typedef struct {
+    int on;
+    f3 fa, ta, tb;     /* the force on end a (end b takes -fa), the moments on the ends */
+    f3 e1;             /* the tangent's chord */
+    float ngeo;        /* the tension's geometric stiffness N / L (tension only) */
+    double W[144];     /* the 12 x 12 matrix's blocks aa, bb, ba, ab (mem_kw_) */
+    double kv[12];     /* K u at the ends' velocities and spins (mem_kv_) */
+} mem_out;
+
+/* element_matrix (:2157-2186) */
+static void mem_matrix(f3 te1, f3 te2, f3 te3, float tL, float ka_f, float kt_f, const float* cy, const float* cz,
+                       double K[12][12]) {
+    const double iL = 1.0 / tL;
+    const double e1[3] = {te1.x, te1.y, te1.z}, e2[3] = {te2.x, te2.y, te2.z}, e3[3] = {te3.x, te3.y, te3.z};
+    const double g[3] = {e3[0] * iL, e3[1] * iL, e3[2] * iL}, h[3] = {e2[0] * iL, e2[1] * iL, e2[2] * iL};
+    const double cy0 = cy[0], cy1 = cy[1], cy2 = cy[2], cz0 = cz[0], cz1 = cz[1], cz2 = cz[2];
+    const double Sy = cy0 + 2 * cy1 + cy2, Sz = cz0 + 2 * cz1 + cz2, ay = cy0 + cy1, by = cy1 + cy2, az = cz0 + cz1, bz = cz1 + cz2;
+    const double ka = ka_f, kt = kt_f;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const double uu = ka * e1[i] * e1[j] + Sy * g[i] * g[j] + Sz * h[i] * h[j];
+            const double up = -ay * g[i] * e2[j] + az * h[i] * e3[j];
+            const double ub = -by * g[i] * e2[j] + bz * h[i] * e3[j];
+            const double tt = kt * e1[i] * e1[j];
+            const double pa = tt + cy0 * e2[i] * e2[j] + cz0 * e3[i] * e3[j];
+            const double pb = -tt + cy1 * e2[i] * e2[j] + cz1 * e3[i] * e3[j];
+            const double qb = tt + cy2 * e2[i] * e2[j] + cz2 * e3[i] * e3[j];
+            K[i][j] = uu, K[i][3 + j] = up, K[i][6 + j] = -uu, K[i][9 + j] = ub;
+            K[3 + i][3 + j] = pa, K[3 + i][9 + j] = pb;
+            K[6 + i][6 + j] = uu, K[6 + i][9 + j] = -ub;
+            K[9 + i][9 + j] = qb;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) K[3 + i][6 + j] = -K[j][3 + i];
+    for (int r = 0; r < 12; r++)
+        for (int c = r + 1; c < 12; c++) K[c][r] = K[r][c];
+}
+
+/* eval_forces for an elastic member (:1905-2063 with no plastic limits): the axial force, the torque, the end moments,
+ * the ends' forces and moments; the tangent and member_world_k (:1834-1850) */
+static void eval_member(const orc_fem* f, uint32_t k, mem_out* o) {
+    o->on = 0;
+    if (f->mbroken[k]) return;
+    mem_kin_t K;
+    if (!mem_kin(f, k, &K)) return;
+    const float* st = &f->mst[8 * k];
+    const float ka = st[0], kt = st[1];
+    const float* cy = &st[2];
+    const float* cz = &st[5];
+    const float delta = (float)K.dL;
+    const float N = ka * (delta - 0.0f); /* (no plastic elongation) */
+    const float tw = K.rb.x - K.ra.x;
+    const float T = kt * (tw - 0.0f);
+    const float dax = K.ra.y - 0.0f, day = K.ra.z - 0.0f, dbx = K.rb.y - 0.0f, dby = K.rb.z - 0.0f;
+    const float Max = cy[0] * dax + cy[1] * dbx, May = cz[0] * day + cz[1] * dby;
+    const float Mbx = cy[1] * dax + cy[2] * dbx, Mby = cz[1] * day + cz[2] * dby;
+    const float Sy = Max + Mbx, Sz = May + Mby;
+    o->fa = sub3(add3(mul3(K.e1, N), mul3(K.e3, Sy / K.L)), mul3(K.e2, Sz / K.L));
+    o->ta = sub3(sub3(mul3(K.e1, T), mul3(K.e2, Max)), mul3(K.e3, May));
+    o->tb = sub3(sub3(mul3(K.e1, -T), mul3(K.e2, Mbx)), mul3(K.e3, Mby));
+    o->on = 1;
+    o->e1 = K.e1;
+    o->ngeo = N > 0 ? N / K.L : 0.0f;
+    double M[12][12];
+    mem_matrix(K.e1, K.e2, K.e3, K.L, ka, kt, cy, cz, M);
+    for (int r = 0; r < 6; r++)
+        for (int q = 0; q < 6; q++)
+            o->W[r * 6 + q] = M[r][q], o->W[36 + r * 6 + q] = M[6 + r][6 + q], o->W[72 + r * 6 + q] = M[6 + r][q],
+            o->W[108 + r * 6 + q] = M[r][6 + q];
+    const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
+    const double u[12] = {f->v[3 * a], f->v[3 * a + 1], f->v[3 * a + 2], f->w[3 * a], f->w[3 * a + 1], f->w[3 * a + 2],
+                          f->v[3 * b], f->v[3 * b + 1], f->v[3 * b + 2], f->w[3 * b], f->w[3 * b + 1], f->w[3 * b + 2]};
+    for (int r = 0; r < 12; r++) {
+        double s2 = 0;
+        for (int q = 0; q < 12; q++) s2 += M[r][q] * u[q];
+        o->kv[r] = s2;
+    }
+}
+
+/* gather_part's member_block (:926-941): block blk (0 aa, 1 bb, 2 ba, 3 ab) times c, the tension's geometric stiffness
+ * with sign gs */
+static void mem_block(const orc_fem* f, uint32_t k, const mem_out* o, int blk, double gs, double h2, double h, double* D) {
+    const float* s = &f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2]];
+    const double c = h2 + (double)s[ORC_SEC_DAMPING] * h;
+    const double* B = &o->W[blk * 36];
+    for (int x = 0; x < 36; x++) D[x] += c * B[x];
+    const double gg = h2 * o->ngeo;
+    if (gg != 0) {
+        double P[9];
+        const double ee[3] = {o->e1.x, o->e1.y, o->e1.z};
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) P[r * 3 + q] = gg * ((r == q ? 1.0 : 0.0) - ee[r] * ee[q]);
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) D[r * 6 + q] += gs * P[r * 3 + q];
+    }
+}
+
+void orc_fem_add_torques(orc_fem* f, const float* torques) {
+    for (uint32_t i = 0; i < 3 * f->nn; i++) f->torque[i] += torques[i];
 }
 
 /* gather_part's tri_block (frame_fem.cpp:948-985): corners (i, j) in world axes, times c, tension on the diagonal */
@@ -716,15 +1013,26 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         const float* pp = &f->p[3 * i];
         if ((float)x[0] != pp[0] || (float)x[1] != pp[1] || (float)x[2] != pp[2]) x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
     }
-    tri_out* o = (tri_out*)calloc(nt, sizeof(tri_out));
+    const uint32_t nm = f->nm;
+    mem_out* mo = (mem_out*)calloc(nm + 1, sizeof(mem_out));
+    for (uint32_t k = 0; k < nm; k++) eval_member(f, k, &mo[k]);
+    tri_out* o = (tri_out*)calloc(nt + 1, sizeof(tri_out));
     for (uint32_t k = 0; k < nt; k++) eval_tri(f, k, &o[k]);
-    /* the node forces: gravity m g, the loads, then the triangles' corners in triangle order */
-    f3* F = (f3*)malloc(sizeof(f3) * nn);
-    f3* T = (f3*)calloc(nn, sizeof(f3));
+    /* the node forces: gravity m g, the loads, then the members' ends in member order, then the triangles' corners
+     * in triangle order (gather_lists, :862-898); the moments: those from outside, then the same */
+    f3* F = (f3*)malloc(sizeof(f3) * (nn + 1));
+    f3* T = (f3*)calloc(nn + 1, sizeof(f3));
     for (uint32_t i = 0; i < nn; i++) {
         F[i] = mul3(F3(g[0], g[1], g[2]), f->mass[i]);
         if (loads) F[i] = add3(F[i], F3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]));
+        T[i] = F3(f->torque[3 * i], f->torque[3 * i + 1], f->torque[3 * i + 2]);
     }
+    for (uint32_t k = 0; k < nm; k++)
+        if (mo[k].on) {
+            const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
+            F[a] = add3(F[a], mo[k].fa), T[a] = add3(T[a], mo[k].ta);
+            F[b] = sub3(F[b], mo[k].fa), T[b] = add3(T[b], mo[k].tb);
+        }
     for (uint32_t k = 0; k < nt; k++)
         if (o[k].on)
             for (int c = 0; c < 3; c++) {
@@ -745,6 +1053,13 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
     int* fc = (int*)malloc(sizeof(int) * (size_t)(nf ? nf : 1));
     size_t* roff = (size_t*)malloc(sizeof(size_t) * (size_t)(nf + 1));
     for (int q = 0; q < nf; q++) fc[q] = q;
+    for (uint32_t k = 0; k < nm; k++) {
+        const int a = fi[f->mem[4 * k]], b = fi[f->mem[4 * k + 1]];
+        if (a >= 0 && b >= 0 && !f->mbroken[k]) {
+            if (b < fc[a]) fc[a] = b;
+            if (a < fc[b]) fc[b] = a;
+        }
+    }
     for (uint32_t k = 0; k < nt; k++)
         for (int ci = 0; ci < 3; ci++)
             for (int cj = 0; cj < 3; cj++) {
@@ -769,12 +1084,32 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         for (uint32_t k = 0; k < 3 * nt; k++) nc[fill[f->tn[k]]++] = k;
         free(fill);
     }
+    /* node -> (member, end) in member order */
+    uint32_t* mcnt = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
+    for (uint32_t k = 0; k < 2 * nm; k++) mcnt[f->mem[4 * (k / 2) + k % 2] + 1]++;
+    for (uint32_t i = 0; i < nn; i++) mcnt[i + 1] += mcnt[i];
+    uint32_t* mnc = (uint32_t*)malloc(sizeof(uint32_t) * (2 * nm + 1));
+    {
+        uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * (nn + 1));
+        memcpy(fill, mcnt, sizeof(uint32_t) * nn);
+        for (uint32_t k = 0; k < 2 * nm; k++) mnc[fill[f->mem[4 * (k / 2) + k % 2]]++] = k;
+        free(fill);
+    }
     for (uint32_t i = 0; i < nn; i++) {
         if (fi[i] < 0) continue;
         double* D = ENV(fi[i], fi[i]);
         double rd[6] = {0};
         D[0] = D[7] = D[14] = f->mass[i];
         D[21] = D[28] = D[35] = f->inertia[i];
+        /* its members' blocks first, then its triangles' (gather_part, :1004-1024) */
+        for (uint32_t e = mcnt[i]; e < mcnt[i + 1]; e++) {
+            const uint32_t k = mnc[e] / 2, end = mnc[e] % 2;
+            if (!mo[k].on) continue;
+            mem_block(f, k, &mo[k], (int)end, 1.0, h2, h, D);
+            const double crm = hd + (double)f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2] + ORC_SEC_DAMPING] * h;
+            if (crm != 0)
+                for (int j = 0; j < 6; j++) rd[j] -= crm * mo[k].kv[6 * end + j];
+        }
         for (uint32_t e = ncnt[i]; e < ncnt[i + 1]; e++) {
             const uint32_t k = nc[e] / 3, ci = nc[e] % 3;
             if (!o[k].on) continue;
@@ -792,8 +1127,16 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         rr[3] = h * (double)T[i].x, rr[4] = h * (double)T[i].y, rr[5] = h * (double)T[i].z;
         for (int j = 0; j < 6; j++) rr[j] += rd[j];
     }
-    free(ncnt), free(nc);
-    /* the lower off-diagonal blocks: corners (i, j) of every triangle, both free, the row node later */
+    free(ncnt), free(nc), free(mcnt), free(mnc);
+    /* the lower off-diagonal blocks: the members' (the row node's block: ba when b comes later, else ab), then the
+     * triangles' corners (i, j), both free, the row node later */
+    for (uint32_t k = 0; k < nm; k++) {
+        if (!mo[k].on) continue;
+        const int a = fi[f->mem[4 * k]], b = fi[f->mem[4 * k + 1]];
+        if (a < 0 || b < 0 || a == b) continue;
+        if (b > a) mem_block(f, k, &mo[k], 2, -1.0, h2, h, ENV(b, a));
+        else mem_block(f, k, &mo[k], 3, -1.0, h2, h, ENV(a, b));
+    }
     for (uint32_t k = 0; k < nt; k++) {
         if (!o[k].on) continue;
         for (int ci = 0; ci < 3; ci++)
@@ -883,6 +1226,7 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         else
             x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
     }
-    free(o), free(F), free(T), free(fi), free(L), free(fc), free(roff), free(r);
+    memset(f->torque, 0, 3 * sizeof(float) * nn); /* (FemFrame::solve_end: the moments from outside used up) */
+    free(mo), free(o), free(F), free(T), free(fi), free(L), free(fc), free(roff), free(r);
     return rc;
 }

@@ -11,6 +11,12 @@ extern "C" {
 
 typedef struct orc_fem orc_fem;
 
+/* A member section's fields (FrameSection, src/phys/frame_fem.h:90-112, the elastic step's): their order in a section */
+enum {
+    ORC_SEC_E, ORC_SEC_G, ORC_SEC_RHO, ORC_SEC_A, ORC_SEC_IY, ORC_SEC_IZ, ORC_SEC_J, ORC_SEC_ASY, ORC_SEC_ASZ,
+    ORC_SEC_DAMPING, ORC_SEC_AXIAL, ORC_SEC_JOINT_K, ORC_FRAME_SECTION_FIELDS
+};
+
 /* the element stiffness of one triangle (tri_local_k, src/phys/frame_fem.cpp:1524-1579), 18 x 18 row-major */
 void orc_fem_tri_local_k(const float X[6], float E, float nu, float t, float drill, double* K);
 /* the rest shape of a triangle from its corners (FemFrame::finalize, src/phys/frame_fem.cpp:259-276): X[6] */
@@ -19,6 +25,14 @@ int orc_fem_rest_shape(const float p[9], float X[6]);
 /* A plate of nu x nv cells (ShellMesher::grid, src/phys/fem_shell.cpp:34-46) of `size` metres in the x-z plane,
  * centred on the origin at height y0, its border nodes clamped. Section: E, nu, rho, t, drill, damping. */
 orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float section[6]);
+/* A frame body: nodes (p, mass as the body has it, pinned flags), shell triangles of one section (E, nu, rho, t, drill,
+ * damping), members (4 words each: a, b, section, end_a | end_b << 8, the ends' FrameJoint) of elastic sections
+ * (ORC_FRAME_SECTION_FIELDS floats each). FemFrame::finalize and node_inertia. */
+orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
+                        const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
+                        const float* sections);
+/* moments from outside on the nodes for the next step (FemFrame::torque, 3 per node), added to any set before */
+void orc_fem_add_torques(orc_fem* f, const float* torques);
 void orc_fem_destroy(orc_fem* f);
 uint32_t orc_fem_nodes(const orc_fem* f);
 uint32_t orc_fem_tris(const orc_fem* f);

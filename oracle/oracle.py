@@ -75,6 +75,9 @@ def oracle_lib():
             "fem_set_state": (None, [_P, _P, _P, _P, _P]),
             "fem_get_nodes": (None, [_P, _P, _P, _P]),
             "fem_set_fixed": (None, [_P, _P]),
+            "fem_create": (_P, [ctypes.c_uint32, _P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P,
+                                ctypes.c_uint32, _P]),
+            "fem_add_torques": (None, [_P, _P]),
             "fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P,
                                         ctypes.POINTER(ctypes.c_uint32)]),
             "fem_block_solve": (ctypes.c_int, [ctypes.c_int, _P, _P]),
@@ -109,6 +112,9 @@ def ref_lib():
             "fem_plate_substep": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
             "fem_plate_read": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
             "fem_plate_write": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+            "frame_new": (_P, [ctypes.c_uint32, _P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P,
+                               ctypes.c_uint32, _P]),
+            "frame_add_torques": (None, [_P, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -265,6 +271,24 @@ def ref_tri_stiffness(p, section=SECTION_C3):
     return K, X.reshape(3, 2)
 
 
+def _frame_arrays(p, mass, fixed, tris, shell, members, sections):
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    nn = len(p)
+    mass = np.ascontiguousarray(mass, np.float32).reshape(nn)
+    fixed = np.zeros(nn, np.uint8) if fixed is None else np.ascontiguousarray(fixed, np.uint8).reshape(nn)
+    tris = np.zeros((0, 3), np.uint32) if tris is None else np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    shell = np.asarray(SECTION_C3 if shell is None else shell, np.float32)
+    members = np.zeros((0, 4), np.uint32) if members is None else np.ascontiguousarray(members, np.uint32).reshape(-1, 4)
+    sections = np.zeros((0, 12), np.float32) if sections is None else np.ascontiguousarray(sections, np.float32).reshape(-1, 12)
+    return p, mass, fixed, tris, shell, members, sections
+
+
+def _frame_args(arrs):
+    p, mass, fixed, tris, shell, members, sections = arrs
+    return (len(p), _ptr(p), _ptr(mass), _ptr(fixed), len(tris), _ptr(tris), _ptr(shell), len(members), _ptr(members),
+            len(sections), _ptr(sections))
+
+
 class RefPlate:
     """The compiled reference's clamped plate (ShellMesher::grid) and its implicit step as its fembench times it:
     clear_forces(g) plus the loads, FemFrame::compute_forces, FemFrame::solve (one thread)."""
@@ -280,6 +304,20 @@ class RefPlate:
         ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
         gg = np.asarray(g, np.float32)
         return self._lib.blref_fem_plate_step(self._h, _ptr(ld), h, theta, dissipation, _ptr(gg))
+
+    @classmethod
+    def frame(cls, p, mass, fixed=None, tris=None, shell=None, members=None, sections=None):
+        """A frame body (blref_frame_new): nodes, shell triangles of one section, members (a, b, section,
+        end_a | end_b << 8) of elastic sections (12 floats each)."""
+        self = cls.__new__(cls)
+        self._lib = ref_lib()
+        self._arrs = _frame_arrays(p, mass, fixed, tris, shell, members, sections)
+        self._h = self._lib.blref_frame_new(*_frame_args(self._arrs))
+        self.nn = len(self._arrs[0])
+        return self
+
+    def add_torques(self, t):
+        self._lib.blref_frame_add_torques(self._h, _ptr(np.ascontiguousarray(t, np.float32).reshape(-1)))
 
     def substep(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
         """The plate body's whole substep as the reference's world runs it: the step, then integrate_nodes and
@@ -338,6 +376,21 @@ class FemPlate:
         self.nt = self._lib.orc_fem_tris(self._h)
         if not clamped:
             self._lib.orc_fem_set_fixed(self._h, _ptr(np.zeros(self.nn, np.uint8)))
+
+    @classmethod
+    def frame(cls, p, mass, fixed=None, tris=None, shell=None, members=None, sections=None):
+        """A frame body (orc_fem_create): nodes, shell triangles of one section, members (a, b, section,
+        end_a | end_b << 8) of elastic sections (12 floats each)."""
+        self = cls.__new__(cls)
+        self._lib = oracle_lib()
+        self._arrs = _frame_arrays(p, mass, fixed, tris, shell, members, sections)
+        self._h = self._lib.orc_fem_create(*_frame_args(self._arrs))
+        self.nn = self._lib.orc_fem_nodes(self._h)
+        self.nt = self._lib.orc_fem_tris(self._h)
+        return self
+
+    def add_torques(self, t):
+        self._lib.orc_fem_add_torques(self._h, _ptr(np.ascontiguousarray(t, np.float32).reshape(-1)))
 
     def topology(self):
         tn = np.zeros((self.nt, 3), np.uint32)

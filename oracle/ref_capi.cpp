@@ -278,6 +278,47 @@ int64_t blref_fem_plate_step(void* h, const float* loads, float dt, float theta,
     return (int64_t)b.fem.solve_failures;
 }
 
+// A frame body built the reference's way: nodes (SoftBody::add_node with their masses, NF_FIXED where pinned), shell
+// triangles of one section (FemFrame::add_shell_section, add_tri), members (FemFrame::add_section, add_element: a, b,
+// section, end_a | end_b << 8) of elastic sections (12 floats each: E, G, rho, A, Iy, Iz, J, As_y, As_z, damping,
+// axial, joint_k; no plastic limits), then SoftBody::finalize and FemFrame::finalize.
+void* blref_frame_new(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
+                      const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
+                      const float* sections) {
+    auto* b = new SoftBody();
+    for (uint32_t i = 0; i < nn; i++)
+        b->add_node(vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), mass[i], fixed && fixed[i] ? NF_FIXED : NF_NONE);
+    if (nt) {
+        ShellSection s;
+        s.E = shell[0], s.nu = shell[1], s.rho = shell[2], s.t = shell[3], s.drill = shell[4], s.damping = shell[5];
+        const uint16_t si = b->fem.add_shell_section(s);
+        for (uint32_t k = 0; k < nt; k++) b->fem.add_tri(tris[3 * k], tris[3 * k + 1], tris[3 * k + 2], si);
+    }
+    for (uint32_t k = 0; k < nsec; k++) {
+        const float* x = &sections[12 * k];
+        FrameSection s;
+        s.E = x[0], s.G = x[1], s.rho = x[2], s.A = x[3], s.Iy = x[4], s.Iz = x[5], s.J = x[6], s.As_y = x[7], s.As_z = x[8];
+        s.damping = x[9], s.axial = x[10], s.joint_k = x[11];
+        s.Np = s.Mp = s.Tp = 0;
+        b->fem.add_section(s);
+    }
+    for (uint32_t k = 0; k < nm; k++)
+        b->fem.add_element(members[4 * k], members[4 * k + 1], (uint16_t)members[4 * k + 2], (uint8_t)(members[4 * k + 3] & 0xff),
+                           (uint8_t)(members[4 * k + 3] >> 8));
+    b->finalize();
+    b->fem.finalize(*b);
+    return b;
+}
+
+// moments from outside on the frame's nodes for the next step (FemFrame::torque, added; 3 per body node)
+void blref_frame_add_torques(void* h, const float* t) {
+    SoftBody& b = *static_cast<SoftBody*>(h);
+    for (size_t i = 0; i < b.nodes.size(); i++) {
+        const int sl = b.fem.slot((uint32_t)i);
+        if (sl >= 0) b.fem.torque[sl] += vec3(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    }
+}
+
 // The plate body's whole substep as World::simulate_island runs a frame body stepping every substep
 // (src/phys/world.cpp:3265-3269, 3573-3580, 3812-3822, 3876-3886): clear_forces(g) plus the loads,
 // FemFrame::compute_forces, FemFrame::solve (its update writes the new velocities as forces, :3000-3027), then

@@ -82,3 +82,92 @@ def kinetic(st, inertia):
 
 
 STATE_KEYS = ("p", "v", "xd", "w", "q", "force")
+
+
+# ---------------------------------------------------------------------------------------------- frames (members)
+FJ_RIGID, FJ_BALL, FJ_HINGE_V, FJ_HINGE_H, FJ_SWIVEL, FJ_ELASTIC = range(6)
+
+
+def frame_section(E, G, rho, A, Iy, Iz, J, As_y, As_z, damping=2e-5, axial=1.0, joint_k=2.0e4):
+    """A member section's 12 floats (FrameSection's elastic fields, the oracle's and the reference shim's order)."""
+    return np.array([E, G, rho, A, Iy, Iz, J, As_y, As_z, damping, axial, joint_k], np.float32)
+
+
+def steel_tube(outer=0.04, wall=0.002, damping=2e-4, joint_k=2.0e4):
+    """make_frame_section("Steel", Tube, outer, wall) (src/phys/frame_fem.cpp:95-135) without its plastic limits:
+    A, I, J and the shear areas (0.53 A) in double, stored as floats."""
+    D, t = outer, min(max(wall, 1e-4), outer * 0.5)
+    d = D - 2 * t
+    A = np.pi / 4 * (D * D - d * d)
+    I = np.pi / 64 * (D ** 4 - d ** 4)
+    return frame_section(2.1e11, 8.1e10, 7850.0, A, I, I, 2 * I, 0.53 * A, 0.53 * A, damping, 1.0, joint_k)
+
+
+def frame_line(n, p0, p1, sec, root=FJ_RIGID):
+    """The reference's frame_line (tests/test_physics.cpp:1383-1394): n members from p0 to p1, the first node
+    clamped, lumped masses (a member's mass split over its ends), the root member's joint at its first node."""
+    p0, p1 = np.asarray(p0, np.float32), np.asarray(p1, np.float32)
+    L = np.float32(np.linalg.norm((p1 - p0).astype(np.float32)))
+    m = np.float32(sec[2] * sec[3]) * L / np.float32(n)
+    p = np.array([p0 + (p1 - p0) * (np.float32(i) / np.float32(n)) for i in range(n + 1)], np.float32)
+    mass = np.array([m * np.float32(0.5) if i in (0, n) else m for i in range(n + 1)], np.float32)
+    fixed = np.zeros(n + 1, np.uint8)
+    fixed[0] = 1
+    members = np.array([(i, i + 1, 0, (root if i == 0 else FJ_RIGID)) for i in range(n)], np.uint32)
+    return dict(p=p, mass=mass, fixed=fixed, members=members, sections=sec[None, :])
+
+
+def portal(sec, n=4, feet=FJ_RIGID):
+    """The reference's portal frame (tests/test_physics.cpp:2109-2128): clamped feet at (0,0,0), (1,0,0), corners at
+    height 1, three chains of n members (frame_chain), the feet's joints `feet`."""
+    p = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0)]
+    mass = [0.0, 0.0, 0.0, 0.0]
+    members = []
+    fixed = [1, 1, 0, 0]
+    for a, c in ((0, 2), (1, 3), (2, 3)):
+        p0, p1 = np.asarray(p[a], np.float32), np.asarray(p[c], np.float32)
+        m = np.float32(sec[2] * sec[3]) * np.float32(np.linalg.norm((p1 - p0).astype(np.float32))) / np.float32(n)
+        mass[a] = np.float32(mass[a]) + m * np.float32(0.5)
+        mass[c] = np.float32(mass[c]) + m * np.float32(0.5)
+        prev = a
+        for i in range(1, n + 1):
+            cur = c
+            if i < n:
+                p.append(tuple(p0 + (p1 - p0) * (np.float32(i) / np.float32(n))))
+                mass.append(m)
+                fixed.append(0)
+                cur = len(p) - 1
+            ends = (feet if prev in (0, 1) else FJ_RIGID)
+            members.append((prev, cur, 0, ends))
+            prev = cur
+    return dict(p=np.array(p, np.float32), mass=np.array(mass, np.float32), fixed=np.array(fixed, np.uint8),
+                members=np.array(members, np.uint32), sections=sec[None, :])
+
+
+def stiffened_plate(n=6, sec=None):
+    """A clamped n x n plate (C3's section) with a steel tube along its middle line under the shell: members on the
+    plate's own nodes (member and triangle blocks share node pairs)."""
+    sec = steel_tube() if sec is None else sec
+    from beamlab_amd import fem
+    p, t, fx = fem.plate_mesh(n, n)
+    mass = np.zeros(len(p), np.float32)
+    mpa = np.float32(SECTION_C3[2]) * np.float32(SECTION_C3[3])
+    for tri in t:
+        a, b, c = p[tri[0]], p[tri[1]], p[tri[2]]
+        area = np.float32(0.5) * np.float32(np.sqrt(np.float32(np.dot(np.cross(b - a, c - a), np.cross(b - a, c - a)))))
+        for v in tri:
+            mass[v] = mass[v] + (mpa * area) / np.float32(3.0)
+    j = n // 2
+    row = [j * (n + 1) + i for i in range(n + 1)]
+    members = [(row[i], row[i + 1], 0, FJ_RIGID) for i in range(n)]
+    for a, b, _, _ in members:
+        m = np.float32(sec[2] * sec[3]) * np.float32(np.linalg.norm(p[b] - p[a]))
+        mass[a] += m * np.float32(0.5)
+        mass[b] += m * np.float32(0.5)
+    return dict(p=p, mass=mass, fixed=fx, tris=t, members=np.array(members, np.uint32), sections=sec[None, :])
+
+
+def make_frame(mod, fr, ref=False):
+    cls = mod.RefPlate if ref else mod.FemPlate
+    return cls.frame(fr["p"], fr["mass"], fr.get("fixed"), fr.get("tris"), SECTION_C3, fr.get("members"),
+                     fr.get("sections"))

@@ -105,6 +105,7 @@ int main(int argc, char** argv) {
     if (all || !std::strcmp(which, "sinf")) scan_all("sinf", [](float x) { return bl_sinf(x); }, [](float x) { return sinf(x); });
     if (all || !std::strcmp(which, "cosf")) scan_all("cosf", [](float x) { return bl_cosf(x); }, [](float x) { return cosf(x); });
     if (all || !std::strcmp(which, "atanf")) scan_all("atanf", [](float x) { return bl_atanf(x); }, [](float x) { return atanf(x); });
+    if (all || !std::strcmp(which, "acosf")) scan_all("acosf", [](float x) { return bl_acosf(x); }, [](float x) { return acosf(x); });
     if (all || !std::strcmp(which, "atan2f")) scan_atan2(argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 300000000ull);
     return 0;
 }

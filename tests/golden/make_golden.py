@@ -12,6 +12,8 @@ Outputs (data only: inputs are regenerated from the seeds, outputs are the refer
     fem_step_ref.npz    the plate body's whole substep (compute_forces, solve, integrate_nodes, sync_positions) of the
                         scenarios of tests/_fem_cases.py: the update-path scenario after every step, the spun plate
                         every 100 steps (nodes' p and v, xd, w, q, forces, counters)
+    fem_frame_ref.npz   frame bodies with members (tests/test_fem_oracle.py's frame cases: cantilevers on every joint,
+                        portals, a stiffened plate) under seeded loads and moments: the states after 10, 20, 30 steps
 """
 import hashlib
 import os
@@ -77,6 +79,21 @@ def main():
         R.close()
     np.savez_compressed(os.path.join(HERE, "fem_step_ref.npz"), **out)
     print("written fem_step_ref.npz")
+    from test_fem_oracle import _frames, frame_loads
+    out = {}
+    for name, fr in sorted(_frames().items()):
+        R = fc.make_frame(oracle, fr, ref=True)
+        for s, (ld, tq) in enumerate(frame_loads(name, R.nn, 30, 3)):
+            R.add_torques(tq)
+            R.substep(ld, fc.H)
+            if s % 10 == 9:
+                st = R.read()
+                for k in fc.STATE_KEYS:
+                    out[f"{name}_{s}_{k}"] = st[k]
+                out[f"{name}_{s}_counters"] = np.array([st["failures"], st["clamps"]])
+        R.close()
+    np.savez_compressed(os.path.join(HERE, "fem_frame_ref.npz"), **out)
+    print("written fem_frame_ref.npz")
 
 
 if __name__ == "__main__":

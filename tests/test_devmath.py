@@ -61,7 +61,7 @@ def _run(checker, *args):
     return rows
 
 
-@pytest.mark.parametrize("fn", ["table", "expf", "pow2", "sinf", "cosf", "atanf"])
+@pytest.mark.parametrize("fn", ["table", "expf", "pow2", "sinf", "cosf", "atanf", "acosf"])
 def test_exhaustive(checker, fn):
     rows = _run(checker, fn)
     assert len(rows) == 1

@@ -196,3 +196,149 @@ def test_fem_substeps_match_golden():
                 for k in fc.STATE_KEYS:
                     assert b[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
                 assert [b["failures"], b["clamps"]] == list(g[f"{name}_{s}_counters"]), f"{name} step {s}: counters"
+
+
+# ---------------------------------------------------------------------------------------------- members
+def _frames():
+    sec = fc.steel_tube()
+    return {
+        "cantilever": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec),
+        "two_members": fc.frame_line(2, (0, 1, 0), (1, 1, 0), sec),
+        "portal": fc.portal(sec),
+        "portal_ball_feet": fc.portal(sec, feet=fc.FJ_BALL),
+        "hinge_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_HINGE_V),
+        "swivel_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_SWIVEL),
+        "elastic_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), fc.steel_tube(joint_k=5.0e4), fc.FJ_ELASTIC),
+        "stiffened_plate": fc.stiffened_plate(6),
+    }
+
+
+def frame_loads(name, nn, steps, seed):
+    """Random loads and moments on every node; the two-member cantilever also gets a heavy tip load, bending its
+    members far enough that their co-rotated frames take the slerp's arc (end frames more than 3.6 degrees apart)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(steps):
+        ld = rng.normal(0, 20, (nn, 3)).astype(np.float32)
+        tq = rng.normal(0, 2, (nn, 3)).astype(np.float32)
+        if name == "two_members":
+            ld[-1, 1] -= 3000.0
+        out.append((ld, tq))
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(_frames()))
+def test_frame_substeps_match_reference_bitwise(ref_mod, name):
+    """Co-rotational Timoshenko members (kinematics, stiffness with joints and shear, the ends' forces and moments,
+    element_matrix, member_world_k, their blocks and damping in the assembly, the moments from outside) against the
+    compiled reference's FemFrame, bit for bit over 30 substeps of random loads and moments."""
+    fr = _frames()[name]
+    P, R = fc.make_frame(oracle, fr), fc.make_frame(oracle, fr, ref=True)
+    a, b = R.read(), P.read()
+    for k in ("p", "inv_mass", "v", "mass", "xd", "w", "q"):
+        assert a[k].tobytes() == b[k].tobytes(), k
+    for s, (ld, tq) in enumerate(frame_loads(name, P.nn, 30, 3)):
+        P.add_torques(tq), R.add_torques(tq)
+        R.substep(ld, fc.H)
+        P.step(ld, fc.H)
+        _same(R.read(), P.read(), f"{name} step {s}")
+
+
+def _settle(fr, steps, force=None, torque=None, at=-1):
+    """The reference's quasi-static frame cases (tests/test_physics.cpp:2062-2067): no gravity, backward Euler with
+    full dissipation (theta 1, dissipation 1), a constant load at one node; the node's displacement and rotation."""
+    P = fc.make_frame(oracle, fr)
+    x0 = P.read()["xd"][at].copy()
+    ld = np.zeros((P.nn, 3), np.float32)
+    tq = np.zeros((P.nn, 3), np.float32)
+    if force is not None:
+        ld[at] = force
+    if torque is not None:
+        tq[at] = torque
+    for _ in range(steps):
+        P.add_torques(tq)
+        P.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    st = P.read()
+    return st["xd"][at] - x0, _qlog(st["q"][at]), st
+
+
+def _qlog(q):
+    """quat_log (src/core/math.h:214-221), in double"""
+    q = q.astype(np.float64) * (-1 if q[3] < 0 else 1)
+    s = np.linalg.norm(q[:3])
+    return q[:3] * 2.0 if s < 1e-8 else q[:3] * (2 * np.arctan2(s, q[3]) / s)
+
+
+def _tube_props(sec):
+    E, G, A, Iz, J, As = (float(sec[i]) for i in (0, 1, 3, 5, 6, 7))
+    return E * Iz, G * J, E * A, G * As
+
+
+def test_cantilever_closed_forms():
+    """tests/test_physics.cpp:2068-2105: a 1 m cantilever of 10 members - tip load PL^3/3EI + PL/kGA and rotation
+    PL^2/2EI (1 %), tip torque TL/GJ (1 %), pull PL/EA (2 %)."""
+    sec = fc.steel_tube()
+    EI, GJ, EA, kGA = _tube_props(sec)
+    fr = fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec)
+    d, r, st = _settle(fr, 4000, force=(0, -50, 0))
+    want, want_r = 50 / (3 * EI) + 50 / kGA, 50 / (2 * EI)
+    assert abs(-d[1] / want - 1) < 0.01 and abs(-r[2] / want_r - 1) < 0.01, (d, r, want, want_r)
+    assert st["failures"] == 0
+    d, r, _ = _settle(fr, 4000, torque=(20, 0, 0))
+    assert abs(r[0] / (20 / GJ) - 1) < 0.01, (r, 20 / GJ)
+    d, r, _ = _settle(fr, 4000, force=(1e4, 0, 0))
+    assert abs(d[0] / (1e4 / EA) - 1) < 0.02, (d, 1e4 / EA)
+
+
+def test_portal_frames_closed_forms():
+    """tests/test_physics.cpp:2106-2160: a portal with clamped feet sways P / (24 EI (6k + 1)/(6k + 4)), k = 1 (2 %); on
+    ball feet P H^3 (2k + 1) / (12 EI k) (2 %)."""
+    sec = fc.steel_tube()
+    EI = _tube_props(sec)[0]
+    for feet, P, steps, want in ((fc.FJ_RIGID, 200.0, 4000, 200.0 / (24 * EI * 7 / 10)),
+                                 (fc.FJ_BALL, 100.0, 8000, 100.0 * 3.0 / (12 * EI))):
+        fr = fc.portal(sec, feet=feet)
+        Pf = fc.make_frame(oracle, fr)
+        x0 = Pf.read()["xd"].copy()
+        ld = np.zeros((Pf.nn, 3), np.float32)
+        ld[2, 0] = P
+        for _ in range(steps):
+            Pf.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+        xd = Pf.read()["xd"]
+        got = 0.5 * ((xd[2] - x0[2])[0] + (xd[3] - x0[3])[0])
+        assert abs(got / want - 1) < 0.02, (feet, got, want)
+
+
+def test_cantilever_joints_closed_forms():
+    """tests/test_physics.cpp:2162-2200: on a hinge loaded across its swing and on a swivel bent, the cantilever's
+    deflection (1 %); on an elastic joint (k_j 5e4 N m/rad) PL^3/3EI + PL/kGA + PL^2/k_j (1 %); on a hinge loaded in
+    its swing it swings far; on a swivel twisted it spins about its axis."""
+    sec = fc.steel_tube(joint_k=5.0e4)
+    EI, GJ, EA, kGA = _tube_props(sec)
+    base = 50 / (3 * EI) + 50 / kGA
+    for root, force, want in ((fc.FJ_HINGE_V, (0, 0, 50), base), (fc.FJ_SWIVEL, (0, -50, 0), base),
+                              (fc.FJ_ELASTIC, (0, -50, 0), base + 50 / 5.0e4)):
+        d, _, _ = _settle(fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, root), 4000, force=force)
+        got = abs(float(np.dot(d, np.asarray(force) / np.linalg.norm(force))))
+        assert abs(got / want - 1) < 0.01, (root, got, want)
+    d, _, _ = _settle(fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_HINGE_V), 4000, force=(0, -20, 0))
+    assert np.linalg.norm(d) > 0.3
+    _, _, st = _settle(fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_SWIVEL), 4000, torque=(5, 0, 0))
+    wt = st["w"][-1]
+    assert abs(wt[0]) > 10.0 and np.hypot(wt[1], wt[2]) < 0.01 * abs(wt[0])
+
+
+def test_frame_substeps_match_golden():
+    """The oracle's members against the reference's recorded states (tests/golden/fem_frame_ref.npz), bit for bit:
+    runs where the reference is absent."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_frame_ref.npz"))
+    for name, fr in sorted(_frames().items()):
+        P = fc.make_frame(oracle, fr)
+        for s, (ld, tq) in enumerate(frame_loads(name, P.nn, 30, 3)):
+            P.add_torques(tq)
+            P.step(ld, fc.H)
+            if s % 10 == 9:
+                st = P.read()
+                for k in fc.STATE_KEYS:
+                    assert st[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
