@@ -86,10 +86,20 @@ class ShellSection(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("E", "nu", "rho", "t", "drill", "damping")]
 
 
+class FrameSection(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("E", "G", "rho", "A", "Iy", "Iz", "J", "As_y", "As_z", "damping",
+                                               "axial", "joint_k")]
+
+
+FRAME_MEMBER = np.dtype([("a", np.uint32), ("b", np.uint32), ("section", np.uint16), ("end_a", np.uint8),
+                         ("end_b", np.uint8)])
+
+
 class FemDesc(ctypes.Structure):
     _fields_ = [("n_nodes", ctypes.c_uint32), ("n_tris", ctypes.c_uint32), ("p", ctypes.c_void_p),
                 ("tris", ctypes.c_void_p), ("fixed", ctypes.c_void_p), ("mass", ctypes.c_void_p),
-                ("section", ShellSection)]
+                ("section", ShellSection), ("n_members", ctypes.c_uint32), ("n_sections", ctypes.c_uint32),
+                ("members", ctypes.c_void_p), ("sections", ctypes.c_void_p)]
 
 
 class FemReport(ctypes.Structure):
@@ -154,6 +164,7 @@ SIGNATURES = {
     "bl_fem_get_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "bl_fem_set_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "bl_fem_get_nodes": (ctypes.c_int, [_P, _P, _P, _P]),
+    "bl_fem_add_torques": (ctypes.c_int, [_P, _P]),
     "bl_fem_solve_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "bl_fem_last_system": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),

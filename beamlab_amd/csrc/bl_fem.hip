@@ -209,6 +209,23 @@ struct FemArgs {
     float* fnode;        // the node forces the step gathered (3 per node); the update's forces for the integrator
     float* imp;          // gather_part's held impulse J (3 per node)
     unsigned long long* cnt; // solve failures (not finite), clamps, NaN velocity resets
+    float* torq;         // FemFrame::torque: moments from outside for this step (3 per node; the update clears them)
+    // members: (a, b, -, broken), rest length, rest frames in the end nodes' frames, elastic stiffness (ka, kt, cy[3],
+    // cz[3]), twist weight, damping; the element pass's outputs: the ends' force and moments (fa, ta, tb), the chord and
+    // the tension's geometric stiffness (e1, N / L), the 12 x 12 matrix's blocks (aa, bb, ba, ab) and K u
+    uint32_t nm;
+    const uint4* mem;
+    const float* mL0;
+    const float4* mqa;
+    const float4* mqb;
+    const float* mst;
+    const float* mtw;
+    const float* mdamp;
+    uint8_t* mon;
+    float* mfo;
+    float4* me1;
+    double* mW;
+    double* mkv;
     const uint8_t* fixed;
     const uint4* tri;   // corners, 0
     const float* X;     // 6 per triangle
@@ -254,6 +271,144 @@ __global__ void k_tri_k(FemArgs a) {
             P[i * 9 + j] = (float)K[(im < jm ? im : jm) * 18 + (im < jm ? jm : im)];
             P[81 + i * 9 + j] = (float)K[(ip < jp ? ip : jp) * 18 + (ip < jp ? jp : ip)];
         }
+}
+
+// ------------------------------------------------------------------------------------------ members
+// (src/core/math.h:82-85, 136-154, 203-212, 227-233: quat::rotate, axis_angle, quat_from_to, slerp; the same operations
+// in the same order, sinf / cosf / acosf glibc's)
+__device__ f3 qrot(qf q, f3 p) {
+    const f3 u = mk(q.x, q.y, q.z);
+    const f3 c1 = mk(u.y * p.z - u.z * p.y, u.z * p.x - u.x * p.z, u.x * p.y - u.y * p.x);
+    const f3 t = blg::mul(c1, 2.0f);
+    const f3 c2 = mk(u.y * t.z - u.z * t.y, u.z * t.x - u.x * t.z, u.x * t.y - u.y * t.x);
+    return blg::add(blg::add(p, blg::mul(t, q.w)), c2);
+}
+__device__ f3 nrm3(f3 a) {
+    const float l = blg::length(a);
+    return l > 1e-12f ? blg::mul(a, 1.0f / l) : mk(0, 0, 0);
+}
+__device__ qf qadd(qf a, qf b) { return Q(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ qf quat_from_to(f3 a, f3 b) {
+    const float d = blg::dot(a, b);
+    if (d < -0.999999f) {
+        const f3 t = fabsf(a.x) < 0.7f ? mk(1, 0, 0) : mk(0, 1, 0);
+        const f3 ax = nrm3(nrm3(mk(a.y * t.z - a.z * t.y, a.z * t.x - a.x * t.z, a.x * t.y - a.y * t.x)));
+        const float hh = 3.14159265358979323846f * 0.5f, sn = blg::bl_sinf(hh);
+        return Q(ax.x * sn, ax.y * sn, ax.z * sn, blg::bl_cosf(hh));
+    }
+    const f3 c = mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+    return qnorm(Q(c.x, c.y, c.z, 1.0f + d));
+}
+__device__ qf qslerp(qf a, qf b, float t) {
+    float d = a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+    if (d < 0) b = qscale(b, -1.0f), d = -d;
+    if (d > 0.9995f) return qnorm(qadd(a, qscale(qadd(b, qscale(a, -1.0f)), t)));
+    const float th = blg::bl_acosf(d), sn = blg::bl_sinf(th);
+    return qadd(qscale(a, blg::bl_sinf((1 - t) * th) / sn), qscale(b, blg::bl_sinf(t * th) / sn));
+}
+
+// K[r][c] of element_matrix (src/phys/frame_fem.cpp:2157-2186) at its place among the stored blocks: aa, bb, ba, ab
+__device__ __forceinline__ uint32_t kidx(int r, int c) {
+    return r < 6 ? (c < 6 ? r * 6 + c : 108 + r * 6 + (c - 6)) : (c < 6 ? 72 + (r - 6) * 6 + c : 36 + (r - 6) * 6 + (c - 6));
+}
+
+// eval_forces for elastic members (src/phys/frame_fem.cpp:1896-2063 with no plastic limits): kinematics (:1384-1405)
+// - the chord, the co-rotated frame (the ends' frames, their mean turned onto the chord), the end rotations against
+// it -, the axial force, torque and end moments, the ends' force and moments; the tangent, element_matrix and
+// member_world_k (:1834-1850: the blocks and K u at the ends' velocities and spins). One thread per member.
+__global__ void k_members(FemArgs a) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.nm) return;
+    a.mon[m] = 0;
+    const uint4 me = a.mem[m];
+    if (me.w) return; // (broken: no length)
+    const uint32_t na = me.x, nb = me.y;
+    const double* xa = a.xd + 3 * (size_t)na;
+    const double* xb = a.xd + 3 * (size_t)nb;
+    const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+    const double L = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(L > 1e-6) || !__builtin_isfinite(L)) return;
+    const float Lf = (float)L;
+    const double dL = L - (double)a.mL0[m];
+    const f3 e1 = mk((float)(dx / L), (float)(dy / L), (float)(dz / L));
+    const float4 qna = a.q[na], qnb = a.q[nb], qa = a.mqa[m], qb = a.mqb[m];
+    const qf Ea = qnorm(qmul(Q(qna.x, qna.y, qna.z, qna.w), Q(qa.x, qa.y, qa.z, qa.w)));
+    const qf Eb = qnorm(qmul(Q(qnb.x, qnb.y, qnb.z, qnb.w), Q(qb.x, qb.y, qb.z, qb.w)));
+    const float t = a.mtw[m];
+    const qf Em = t <= 0.0f ? Ea : t >= 1.0f ? Eb : qslerp(Ea, Eb, t);
+    const qf E = qnorm(qmul(quat_from_to(nrm3(qrot(Em, mk(1, 0, 0))), e1), Em));
+    const qf Ec = qconj(E);
+    const f3 ra = qlog(qmul(Ec, Ea)), rb = qlog(qmul(Ec, Eb));
+    const f3 e2 = qrot(E, mk(0, 1, 0)), e3 = qrot(E, mk(0, 0, 1));
+    const float* st = a.mst + 8 * (size_t)m;
+    const float ka = st[0], kt = st[1], cy0 = st[2], cy1 = st[3], cy2 = st[4], cz0 = st[5], cz1 = st[6], cz2 = st[7];
+    const float delta = (float)dL;
+    const float N = ka * (delta - 0.0f); // (no plastic elongation, twist or hinge rotations)
+    const float tw = rb.x - ra.x;
+    const float T = kt * (tw - 0.0f);
+    const float dax = ra.y - 0.0f, day = ra.z - 0.0f, dbx = rb.y - 0.0f, dby = rb.z - 0.0f;
+    const float Max = cy0 * dax + cy1 * dbx, May = cz0 * day + cz1 * dby;
+    const float Mbx = cy1 * dax + cy2 * dbx, Mby = cz1 * day + cz2 * dby;
+    const float Sy = Max + Mbx, Sz = May + Mby;
+    const f3 fa = blg::sub(blg::add(blg::mul(e1, N), blg::mul(e3, Sy / Lf)), blg::mul(e2, Sz / Lf));
+    const f3 ta = blg::sub(blg::sub(blg::mul(e1, T), blg::mul(e2, Max)), blg::mul(e3, May));
+    const f3 tb = blg::sub(blg::sub(blg::mul(e1, -T), blg::mul(e2, Mbx)), blg::mul(e3, Mby));
+    float* fo = a.mfo + 9 * (size_t)m;
+    fo[0] = fa.x, fo[1] = fa.y, fo[2] = fa.z, fo[3] = ta.x, fo[4] = ta.y, fo[5] = ta.z, fo[6] = tb.x, fo[7] = tb.y, fo[8] = tb.z;
+    a.me1[m] = make_float4(e1.x, e1.y, e1.z, N > 0 ? N / Lf : 0.0f);
+    // element_matrix at the tangent: written to its blocks as the reference's loop sets K, then the lower triangle
+    // taken from the upper one
+    double* W = a.mW + 144 * (size_t)m;
+    {
+        const double iL = 1.0 / Lf;
+        const double E1[3] = {e1.x, e1.y, e1.z}, E2[3] = {e2.x, e2.y, e2.z}, E3[3] = {e3.x, e3.y, e3.z};
+        const double g[3] = {E3[0] * iL, E3[1] * iL, E3[2] * iL}, h[3] = {E2[0] * iL, E2[1] * iL, E2[2] * iL};
+        const double c0 = cy0, c1 = cy1, c2 = cy2, d0 = cz0, d1 = cz1, d2 = cz2;
+        const double Syy = c0 + 2 * c1 + c2, Szz = d0 + 2 * d1 + d2, ay = c0 + c1, by = c1 + c2, az = d0 + d1, bz = d1 + d2;
+        const double kad = ka, ktd = kt;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double uu = kad * E1[i] * E1[j] + Syy * g[i] * g[j] + Szz * h[i] * h[j];
+                const double up = -ay * g[i] * E2[j] + az * h[i] * E3[j];
+                const double ub = -by * g[i] * E2[j] + bz * h[i] * E3[j];
+                const double tt = ktd * E1[i] * E1[j];
+                const double pa = tt + c0 * E2[i] * E2[j] + d0 * E3[i] * E3[j];
+                const double pb = -tt + c1 * E2[i] * E2[j] + d1 * E3[i] * E3[j];
+                const double qb2 = tt + c2 * E2[i] * E2[j] + d2 * E3[i] * E3[j];
+                W[kidx(i, j)] = uu, W[kidx(i, 3 + j)] = up, W[kidx(i, 6 + j)] = -uu, W[kidx(i, 9 + j)] = ub;
+                W[kidx(3 + i, 3 + j)] = pa, W[kidx(3 + i, 9 + j)] = pb;
+                W[kidx(6 + i, 6 + j)] = uu, W[kidx(6 + i, 9 + j)] = -ub;
+                W[kidx(9 + i, 9 + j)] = qb2;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) W[kidx(3 + i, 6 + j)] = -W[kidx(j, 3 + i)];
+        for (int r = 0; r < 12; r++)
+            for (int c = r + 1; c < 12; c++) W[kidx(c, r)] = W[kidx(r, c)];
+    }
+    const double u[12] = {a.v[3 * na], a.v[3 * na + 1], a.v[3 * na + 2], a.w[3 * na], a.w[3 * na + 1], a.w[3 * na + 2],
+                          a.v[3 * nb], a.v[3 * nb + 1], a.v[3 * nb + 2], a.w[3 * nb], a.w[3 * nb + 1], a.w[3 * nb + 2]};
+    double* kv = a.mkv + 12 * (size_t)m;
+    for (int r = 0; r < 12; r++) {
+        double s2 = 0;
+        for (int c = 0; c < 12; c++) s2 += W[kidx(r, c)] * u[c];
+        kv[r] = s2;
+    }
+    a.mon[m] = 1;
+}
+
+// gather_part's member_block (src/phys/frame_fem.cpp:926-941): block blk (0 aa, 1 bb, 2 ba, 3 ab) times c, the tension's
+// geometric stiffness with sign gs
+__device__ void member_block(const FemArgs& a, uint32_t m, int blk, double gs, double h2, double* D) {
+    const double c = h2 + (double)a.mdamp[m] * a.h;
+    const double* B = a.mW + 144 * (size_t)m + 36 * blk;
+    for (int x = 0; x < 36; x++) D[x] += c * B[x];
+    const float4 e = a.me1[m];
+    const double gg = h2 * e.w;
+    if (gg != 0) {
+        const double ee[3] = {e.x, e.y, e.z};
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) D[r * 6 + q] += gs * (gg * ((r == q ? 1.0 : 0.0) - ee[r] * ee[q]));
+    }
 }
 
 // tri_kinematics + eval_tris (elastic): the co-rotated frame, the corners' forces and moments, the tangent
@@ -427,10 +582,18 @@ __global__ void k_nodes(FemArgs a) {
     const int fi = a.fidx[i];
     f3 F = blg::mul(mk(a.g[0], a.g[1], a.g[2]), a.mass[i]);
     if (a.loads) F = blg::add(F, mk(a.loads[3 * i], a.loads[3 * i + 1], a.loads[3 * i + 2]));
-    f3 T = mk(0, 0, 0);
+    f3 T = mk(a.torq[3 * i], a.torq[3 * i + 1], a.torq[3 * i + 2]);
     if (fi < 0) { // (a pinned node: its forces only, for the record)
         for (uint32_t k = a.nc_off[i]; k < a.nc_off[i + 1]; k++) {
-            const uint32_t t = a.nc[k] / 3, corner = a.nc[k] % 3;
+            const uint32_t e = a.nc[k];
+            if (e >> 31) {
+                const uint32_t m = (e & 0x7fffffffu) >> 1, end = e & 1;
+                if (!a.mon[m]) continue;
+                const float* fo = a.mfo + 9 * (size_t)m;
+                F = end ? blg::sub(F, mk(fo[0], fo[1], fo[2])) : blg::add(F, mk(fo[0], fo[1], fo[2]));
+                continue;
+            }
+            const uint32_t t = e / 3, corner = e % 3;
             if (a.ton[t]) F = blg::add(F, mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]));
         }
         a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
@@ -444,8 +607,22 @@ __global__ void k_nodes(FemArgs a) {
     double rd[6] = {0, 0, 0, 0, 0, 0};
     D[0] = D[7] = D[14] = a.mass[i];
     D[21] = D[28] = D[35] = a.inertia[i];
+    // its members' ends first (member order), then its triangles' corners (gather_lists, src/phys/frame_fem.cpp:862-898)
     for (uint32_t k = a.nc_off[i]; k < a.nc_off[i + 1]; k++) {
-        const uint32_t t = a.nc[k] / 3, corner = a.nc[k] % 3;
+        const uint32_t e = a.nc[k];
+        if (e >> 31) {
+            const uint32_t m = (e & 0x7fffffffu) >> 1, end = e & 1;
+            if (!a.mon[m]) continue;
+            const float* fo = a.mfo + 9 * (size_t)m;
+            F = end ? blg::sub(F, mk(fo[0], fo[1], fo[2])) : blg::add(F, mk(fo[0], fo[1], fo[2]));
+            T = blg::add(T, end ? mk(fo[6], fo[7], fo[8]) : mk(fo[3], fo[4], fo[5]));
+            member_block(a, m, (int)end, 1.0, h2, D);
+            const double crm = hd + (double)a.mdamp[m] * h;
+            if (crm != 0)
+                for (int j = 0; j < 6; j++) rd[j] -= crm * a.mkv[12 * (size_t)m + 6 * end + j];
+            continue;
+        }
+        const uint32_t t = e / 3, corner = e % 3;
         if (!a.ton[t]) continue;
         F = blg::add(F, mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]));
         T = blg::add(T, mk(a.tto[9 * t + 3 * corner], a.tto[9 * t + 3 * corner + 1], a.tto[9 * t + 3 * corner + 2]));
@@ -477,9 +654,14 @@ __global__ void k_pairs(FemArgs a) {
     const double c = h2 + (double)a.sec.damping * a.h;
     double O[36];
     for (int k = 0; k < 36; k++) O[k] = 0;
-    for (uint32_t k = pr.z; k < pr.w; k++) {
+    for (uint32_t k = pr.z; k < pr.w; k++) { // (the members' blocks first, then the triangles')
         const uint4 cn = a.pcon[k];
-        if (a.ton[cn.x]) tri_block(a, cn.x, (int)cn.y, (int)cn.z, c, h2, O);
+        if (cn.x >> 31) {
+            const uint32_t m = cn.x & 0x7fffffffu;
+            if (a.mon[m]) member_block(a, m, (int)cn.y, -1.0, h2, O);
+        } else if (a.ton[cn.x]) {
+            tri_block(a, cn.x, (int)cn.y, (int)cn.z, c, h2, O);
+        }
     }
     double* Od = a.Osys + 36 * (size_t)pi;
     for (int k = 0; k < 36; k++) Od[k] = O[k];
@@ -1123,6 +1305,7 @@ __global__ void k_update(FemArgs a, const double* x, int explicit_step) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (explicit_step && i == 0) atomicAdd(&a.cnt[0], 1ull);
     if (i >= a.nn) return;
+    a.torq[3 * i] = a.torq[3 * i + 1] = a.torq[3 * i + 2] = 0; // (FemFrame::solve_end: the moments from outside used up)
     const int fi = a.fidx[i];
     f3 v = mk(a.v[3 * i], a.v[3 * i + 1], a.v[3 * i + 2]);
     f3 w = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
@@ -1235,6 +1418,13 @@ struct bl_fem {
     Dv<float> v, w, mass, inertia, X, Kf, te, tfo, tto, tg, loads;
     Dv<float> pf, invm, fnode, imp; // the body's node positions and inverse masses, the forces, the held impulses
     Dv<unsigned long long> cnt;     // solve failures, clamps, NaN resets
+    Dv<float> torq;                 // moments from outside for the next step
+    uint32_t nm = 0;                // members and their arrays (FemArgs)
+    Dv<uint4> mem;
+    Dv<float> mL0, mst, mtw, mdamp, mfo;
+    Dv<float4> mqa, mqb, me1;
+    Dv<uint8_t> mon;
+    Dv<double> mW, mkv;
     Dv<float4> q, r0;
     Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, xprev, part, sums;
     Dv<float> p0f;     // rest positions of the free nodes (3 each)
@@ -1290,7 +1480,9 @@ struct bl_fem {
         FemArgs a{};
         a.nn = nn, a.nt = nt, a.nf = nf, a.sec = sec;
         a.xd = xd.p, a.v = v.p, a.w = w.p, a.q = q.p, a.mass = mass.p, a.inertia = inertia.p, a.fixed = fixed.p;
-        a.pf = pf.p, a.invm = invm.p, a.fnode = fnode.p, a.imp = imp.p, a.cnt = cnt.p;
+        a.pf = pf.p, a.invm = invm.p, a.fnode = fnode.p, a.imp = imp.p, a.cnt = cnt.p, a.torq = torq.p;
+        a.nm = nm, a.mem = mem.p, a.mL0 = mL0.p, a.mqa = mqa.p, a.mqb = mqb.p, a.mst = mst.p, a.mtw = mtw.p;
+        a.mdamp = mdamp.p, a.mon = mon.p, a.mfo = mfo.p, a.me1 = me1.p, a.mW = mW.p, a.mkv = mkv.p;
         a.tri = tri.p, a.X = X.p, a.r0 = r0.p, a.Kf = Kf.p, a.Kd = Kd.p;
         a.te = te.p, a.tfo = tfo.p, a.tto = tto.p, a.tg = tg.p, a.tkv = tkv.p, a.ton = ton.p;
         a.nc_off = nc_off.p, a.nc = nc.p, a.fidx = fidx.p, a.ford = ford.p;
@@ -1426,16 +1618,112 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         }
     }
     if (d->mass) std::memcpy(mass.data(), d->mass, sizeof(float) * nn);
+    // the members (FemFrame::finalize, src/phys/frame_fem.cpp:244-256): rest length and frames (x along the member, y
+    // towards the world's up, a vertical member's towards x; the nodes' orientations start at identity); their elastic
+    // stiffness (stiffness, :1408-1476) and twist weight (twist_weight, :1372-1375), in float as the reference has them
+    const uint32_t nm = d->n_members;
+    f->nm = nm;
+    std::vector<uint4> mem(nm);
+    std::vector<float> mL0(nm), mst(8 * (size_t)nm), mtw(nm), mdamp(nm);
+    std::vector<float4> mqa(nm), mqb(nm);
+    for (uint32_t k = 0; k < nm; k++) {
+        const bl_frame_member& e = d->members[k];
+        if (e.a >= nn || e.b >= nn || e.section >= d->n_sections || e.end_a > BL_FJ_ELASTIC || e.end_b > BL_FJ_ELASTIC) {
+            blg::set_last_error("a member's nodes, section or joints are out of range");
+            return BL_ERR_ARG;
+        }
+        const bl_frame_section& sc = d->sections[e.section];
+        const f3 dd = blg::sub(P(e.b), P(e.a));
+        const float L0 = blg::length(dd);
+        mL0[k] = L0;
+        const bool broken = L0 < 1e-5f || e.a == e.b;
+        mem[k] = make_uint4(e.a, e.b, 0, broken ? 1u : 0u);
+        mdamp[k] = sc.damping;
+        if (broken) continue;
+        const f3 e1 = blg::divs(dd, L0);
+        const f3 ref = std::fabs(e1.y) < 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+        const f3 rr = blg::sub(ref, blg::mul(e1, blg::dot(e1, ref)));
+        const float lr = blg::length(rr);
+        const f3 e2 = lr > 1e-12f ? blg::mul(rr, 1.0f / lr) : mk(0, 0, 0);
+        const f3 e3 = mk(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+        const qf E0 = from_cols(e1, e2, e3);
+        const qf qa = qnorm(qmul(qconj(Q(0, 0, 0, 1)), E0)), qb = qnorm(qmul(qconj(Q(0, 0, 0, 1)), E0));
+        mqa[k] = make_float4(qa.x, qa.y, qa.z, qa.w), mqb[k] = make_float4(qb.x, qb.y, qb.z, qb.w);
+        // end_flex, condense, ends2, stiffness
+        struct Flex {
+            bool rel[3];
+            float f;
+        };
+        auto flex = [&](uint32_t j) {
+            Flex x{{false, false, false}, 0.0f};
+            if (j == BL_FJ_BALL) x.rel[0] = x.rel[1] = x.rel[2] = true;
+            else if (j == BL_FJ_HINGE_V) x.rel[2] = true;
+            else if (j == BL_FJ_HINGE_H) x.rel[1] = true;
+            else if (j == BL_FJ_SWIVEL) x.rel[0] = true;
+            else if (j == BL_FJ_ELASTIC) x.f = sc.joint_k > 0 ? 1.0f / sc.joint_k : 0.0f;
+            return x;
+        };
+        auto condense = [](float c[3], bool pa, bool pb) {
+            if (pa && pb) c[0] = c[1] = c[2] = 0;
+            else if (pa) c[2] = c[0] > 0 ? c[2] - c[1] * c[1] / c[0] : 0, c[0] = c[1] = 0;
+            else if (pb) c[0] = c[2] > 0 ? c[0] - c[1] * c[1] / c[2] : 0, c[1] = c[2] = 0;
+        };
+        const Flex fa = flex(e.end_a), fb = flex(e.end_b);
+        float* st = &mst[8 * (size_t)k];
+        st[0] = sc.axial * sc.E * sc.A / L0;
+        st[1] = sc.G * sc.J / L0;
+        if (fa.rel[0] || fb.rel[0]) st[1] = 0;
+        else if (fa.f + fb.f > 0) st[1] = 1.0f / (1.0f / st[1] + fa.f + fb.f);
+        auto bend = [&](float I, float As, float* c, int axis) {
+            const float phi = As > 0 ? 12.0f * sc.E * I / (sc.G * As * L0 * L0) : 0.0f;
+            const float k0 = sc.E * I / (L0 * (1.0f + phi));
+            c[0] = c[2] = (4.0f + phi) * k0;
+            c[1] = (2.0f - phi) * k0;
+            const bool ra = fa.rel[axis], rb = fb.rel[axis];
+            if (ra || rb) {
+                condense(c, ra, rb);
+                if (ra != rb) {
+                    float& kk = ra ? c[2] : c[0];
+                    const float fl = ra ? fb.f : fa.f;
+                    if (fl > 0 && kk > 0) kk = 1.0f / (1.0f / kk + fl);
+                }
+                return;
+            }
+            if (fa.f <= 0 && fb.f <= 0) return;
+            const double det = (double)c[0] * c[2] - (double)c[1] * c[1];
+            if (!(det > 0)) return;
+            const double F00 = c[2] / det + fa.f, F01 = -c[1] / det, F11 = c[0] / det + fb.f;
+            const double d2 = F00 * F11 - F01 * F01;
+            c[0] = (float)(F11 / d2), c[1] = (float)(-F01 / d2), c[2] = (float)(F00 / d2);
+        };
+        bend(sc.Iy, sc.As_z, st + 2, 1);
+        bend(sc.Iz, sc.As_y, st + 5, 2);
+        const bool ta = e.end_a == BL_FJ_BALL || e.end_a == BL_FJ_SWIVEL, tb = e.end_b == BL_FJ_BALL || e.end_b == BL_FJ_SWIVEL;
+        mtw[k] = ta == tb ? 0.5f : ta ? 1.0f : 0.0f;
+    }
+    // node_inertia (:3075-3086): the members' halves (m L^2 / 78 per end), then a corner's share of its triangles, the
+    // node's floor
+    for (uint32_t k = 0; k < nm; k++) {
+        if (mem[k].w) continue;
+        const bl_frame_section& sc = d->sections[d->members[k].section];
+        const float em = sc.rho * sc.A * mL0[k];
+        inertia[mem[k].x] += em * mL0[k] * mL0[k] / 78.0f;
+        inertia[mem[k].y] += em * mL0[k] * mL0[k] / 78.0f;
+    }
     for (uint32_t k = 0; k < nt; k++)
         for (int c = 0; c < 3; c++) inertia[d->tris[3 * k + c]] += tmass[k] * area0[k] / 108.0f;
     for (uint32_t i = 0; i < nn; i++) inertia[i] = std::max(inertia[i], std::max(1e-6f, mass[i] * 1e-4f));
-    // node -> incidences in triangle order
-    std::vector<uint32_t> nc_off(nn + 1, 0), nc(3 * (size_t)nt);
+    // node -> incidences: its members' ends in member order (1 << 31 | 2 member + end), then its triangles' corners
+    // in triangle order (3 triangle + corner)
+    std::vector<uint32_t> nc_off(nn + 1, 0), nc(2 * (size_t)nm + 3 * (size_t)nt);
+    for (uint32_t k = 0; k < nm; k++) nc_off[mem[k].x + 1]++, nc_off[mem[k].y + 1]++;
     for (uint32_t k = 0; k < nt; k++)
         for (int c = 0; c < 3; c++) nc_off[d->tris[3 * k + c] + 1]++;
     for (uint32_t i = 0; i < nn; i++) nc_off[i + 1] += nc_off[i];
     {
         std::vector<uint32_t> fill(nc_off.begin(), nc_off.end() - 1);
+        for (uint32_t k = 0; k < nm; k++)
+            for (uint32_t end = 0; end < 2; end++) nc[fill[end ? mem[k].y : mem[k].x]++] = 0x80000000u | (2 * k + end);
         for (uint32_t k = 0; k < nt; k++)
             for (int c = 0; c < 3; c++) nc[fill[d->tris[3 * k + c]]++] = 3 * k + (uint32_t)c;
     }
@@ -1447,6 +1735,10 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     const uint32_t nf = (uint32_t)freen.size();
     f->nf = nf;
     std::vector<std::vector<uint32_t>> adj(nf);
+    for (uint32_t k = 0; k < nm; k++) {
+        const int a = fidx[mem[k].x], b = fidx[mem[k].y];
+        if (!mem[k].w && a >= 0 && b >= 0 && a != b) adj[a].push_back((uint32_t)b), adj[b].push_back((uint32_t)a);
+    }
     for (uint32_t k = 0; k < nt; k++)
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
@@ -1542,7 +1834,16 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     {
         std::vector<std::vector<uint4>> con; // per pair
         std::vector<uint64_t> keys;
-        std::vector<uint4> raw; // (row free, col free, tri, corners)
+        // (row free, col free, member m or nm + triangle, corners / the member's block): a pair's members first, then
+        // its triangles (gather_lists, src/phys/frame_fem.cpp:883-896)
+        std::vector<uint4> raw;
+        for (uint32_t k = 0; k < nm; k++) {
+            const int a = fidx[mem[k].x], b = fidx[mem[k].y];
+            if (mem[k].w || a < 0 || b < 0 || a == b) continue;
+            // (the row is the later node: its block of the member's matrix, ba when b is later, else ab)
+            if (opos[b] > opos[a]) raw.push_back(make_uint4((uint32_t)b, (uint32_t)a, k, 2u));
+            else raw.push_back(make_uint4((uint32_t)a, (uint32_t)b, k, 3u));
+        }
         for (uint32_t k = 0; k < nt; k++)
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) {
@@ -1550,7 +1851,7 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
                     const int a = fidx[d->tris[3 * k + i]], b = fidx[d->tris[3 * k + j]];
                     if (a < 0 || b < 0) continue;
                     if (opos[a] <= opos[b]) continue; // (row: the later node in the order)
-                    raw.push_back(make_uint4((uint32_t)a, (uint32_t)b, k, (uint32_t)(i | (j << 8))));
+                    raw.push_back(make_uint4((uint32_t)a, (uint32_t)b, nm + k, (uint32_t)(i | (j << 8))));
                 }
         std::stable_sort(raw.begin(), raw.end(), [](const uint4& x, const uint4& y) {
             return x.x != y.x ? x.x < y.x : x.y != y.y ? x.y < y.y : x.z < y.z;
@@ -1559,7 +1860,9 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
             size_t e2 = e;
             while (e2 < raw.size() && raw[e2].x == raw[e].x && raw[e2].y == raw[e].y) e2++;
             pairs.push_back(make_uint4(raw[e].x, raw[e].y, (uint32_t)pcon.size(), (uint32_t)(pcon.size() + (e2 - e))));
-            for (size_t q2 = e; q2 < e2; q2++) pcon.push_back(make_uint4(raw[q2].z, raw[q2].w & 0xff, raw[q2].w >> 8, 0));
+            for (size_t q2 = e; q2 < e2; q2++)
+                pcon.push_back(raw[q2].z < nm ? make_uint4(0x80000000u | raw[q2].z, raw[q2].w, 0, 0)
+                                              : make_uint4(raw[q2].z - nm, raw[q2].w & 0xff, raw[q2].w >> 8, 0));
             e = e2;
         }
     }
@@ -1687,6 +1990,11 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     std::vector<float> pf(d->p, d->p + 3 * (size_t)nn), invm(nn);
     for (uint32_t i = 0; i < nn; i++) invm[i] = fixed[i] || mass[i] <= 0 ? 0.0f : 1.0f / mass[i];
     int rc;
+    if ((rc = f->mem.put(mem)) || (rc = f->mL0.put(mL0)) || (rc = f->mqa.put(mqa)) || (rc = f->mqb.put(mqb)) ||
+        (rc = f->mst.put(mst)) || (rc = f->mtw.put(mtw)) || (rc = f->mdamp.put(mdamp)) || (rc = f->mon.alloc(nm)) ||
+        (rc = f->mfo.alloc(9 * (size_t)nm)) || (rc = f->me1.alloc(nm)) || (rc = f->mW.alloc(144 * (size_t)nm)) ||
+        (rc = f->mkv.alloc(12 * (size_t)nm)) || (rc = f->torq.put(std::vector<float>(3 * (size_t)nn, 0.0f))))
+        return rc;
     if ((rc = f->pf.put(pf)) || (rc = f->invm.put(invm)) || (rc = f->fnode.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
         (rc = f->imp.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
         (rc = f->cnt.put(std::vector<unsigned long long>(3, 0ull))))
@@ -1796,7 +2104,8 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
     FemArgs a = f->args(dl, h, theta, dissipation, g);
     f->step_args = a;
     FCHK(hipEventRecord(f->ev[0], st));
-    hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
+    if (f->nm) hipLaunchKernelGGL(k_members, dim3((f->nm + 63) / 64), dim3(64), 0, st, a);
+    if (f->nt) hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
     hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
     if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
     FCHK(hipGetLastError());
@@ -1934,9 +2243,14 @@ size_t noalloc_bytes(const bl_fem* f) {
 extern "C" {
 
 int bl_fem_create(bl_fem** out, const bl_fem_desc* d) {
-    if (!out || !d || (d->n_nodes && !d->p) || (d->n_tris && !d->tris)) return BL_ERR_ARG;
+    if (!out || !d || (d->n_nodes && !d->p) || (d->n_tris && !d->tris) || (d->n_members && (!d->members || !d->sections)))
+        return BL_ERR_ARG;
     *out = nullptr;
-    if (d->n_nodes == 0 || d->n_tris == 0) return BL_ERR_ARG;
+    if (d->n_nodes == 0 || (d->n_tris == 0 && d->n_members == 0)) return BL_ERR_ARG;
+    if (d->n_members && !d->mass) {
+        blg::set_last_error("a body with members passes its node masses");
+        return BL_ERR_ARG;
+    }
     bl_fem* f = new bl_fem();
     static std::atomic<uint64_t> next_serial{1};
     f->serial = next_serial.fetch_add(1);
@@ -2008,6 +2322,16 @@ int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w,
     if (v) FCHK(hipMemcpy(f->v.p, v, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
     if (w) FCHK(hipMemcpy(f->w.p, w, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice));
     if (q) FCHK(hipMemcpy(f->q.p, q, sizeof(float) * 4 * f->nn, hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_fem_add_torques(bl_fem* f, const float* torques) {
+    if (!f || !torques) return BL_ERR_ARG;
+    std::vector<float> t(3 * (size_t)f->nn);
+    FCHK(hipStreamSynchronize(f->st));
+    FCHK(hipMemcpy(t.data(), f->torq.p, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < t.size(); i++) t[i] += torques[i];
+    FCHK(hipMemcpy(f->torq.p, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
     return BL_OK;
 }
 

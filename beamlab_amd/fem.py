@@ -43,19 +43,36 @@ def plate_mesh(nu, nv, size=1.0, y0=0.0):
 class ShellBody:
     """A shell body stepped implicitly on the GPU: bl_fem_create / bl_fem_step / bl_fem_get_state."""
 
-    def __init__(self, p, tris, fixed=None, section=SECTION_PLATE, mass=None):
+    def __init__(self, p, tris, fixed=None, section=SECTION_PLATE, mass=None, members=None, sections=None):
+        """members: (a, b, section, end_a | end_b << 8) rows or an abi.FRAME_MEMBER array; sections: rows of the 12
+        abi.FrameSection fields."""
         self._lib = abi.load()
         self.p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
-        self.tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        self.tris = np.zeros((0, 3), np.uint32) if tris is None else np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
         self.nn, self.nt = len(self.p), len(self.tris)
         self.fixed = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
         self.mass = None if mass is None else np.ascontiguousarray(mass, np.float32)
+        self.members = np.zeros(0, abi.FRAME_MEMBER)
+        if members is not None and len(members):
+            m = np.asarray(members)
+            if m.dtype != abi.FRAME_MEMBER:
+                m = m.astype(np.uint32).reshape(-1, 4)
+                mm = np.zeros(len(m), abi.FRAME_MEMBER)
+                mm["a"], mm["b"], mm["section"] = m[:, 0], m[:, 1], m[:, 2]
+                mm["end_a"], mm["end_b"] = m[:, 3] & 0xff, m[:, 3] >> 8
+                m = mm
+            self.members = np.ascontiguousarray(m)
+        self.sections = np.zeros((0, 12), np.float32) if sections is None else \
+            np.ascontiguousarray(sections, np.float32).reshape(-1, 12)
         d = abi.FemDesc()
         d.n_nodes, d.n_tris = self.nn, self.nt
         d.p, d.tris = abi.ptr(self.p.reshape(-1)), abi.ptr(self.tris.reshape(-1))
         d.fixed = abi.ptr(self.fixed)
         d.mass = abi.ptr(self.mass)
         d.section = abi.ShellSection(*[float(x) for x in section])
+        d.n_members, d.n_sections = len(self.members), len(self.sections)
+        d.members = abi.ptr(self.members) if len(self.members) else None
+        d.sections = abi.ptr(self.sections.reshape(-1)) if len(self.sections) else None
         h = ctypes.c_void_p()
         abi.check(self._lib.bl_fem_create(ctypes.byref(h), ctypes.byref(d)), "bl_fem_create")
         self._h = h
@@ -66,6 +83,16 @@ class ShellBody:
     def plate(cls, nu, nv, size=1.0, y0=0.0, section=SECTION_PLATE, clamped=True):
         p, t, fx = plate_mesh(nu, nv, size, y0)
         return cls(p, t, fx if clamped else np.zeros_like(fx), section)
+
+    @classmethod
+    def frame(cls, p, mass, fixed=None, tris=None, section=SECTION_PLATE, members=None, sections=None):
+        """A frame body: nodes with their masses, shell triangles of one section, members of elastic sections."""
+        return cls(p, tris, fixed, section, mass, members, sections)
+
+    def add_torques(self, t):
+        """Moments from outside on the nodes for the next step (N m, 3 per node)."""
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        abi.check(self._lib.bl_fem_add_torques(self._h, abi.ptr(t)), "bl_fem_add_torques")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -289,9 +289,10 @@ int bl_time_step(bl_ctx* ctx, int32_t substeps, float* ms);
 int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kernel_ms, int32_t* launches);
 
 /* ---------------------------------------------------------------------------------------------------------------
- * Shell FEM: the implicit co-rotational step of a triangle shell (FemFrame's triangles: src/phys/frame_fem.h:154-181,
- * FemFrame::solve src/phys/frame_fem.cpp:2514-3027). Replaces SoftBody::fem for a body made of shell triangles
- * (elastic sections; no members, mounts, contacts or held steps). The system - 6x6 blocks per node pair, FP64 - is
+ * FEM frames: the implicit co-rotational step of a frame body - shell triangles (FemFrame's triangles:
+ * src/phys/frame_fem.h:154-181) and Timoshenko members (FemFrame::elems: src/phys/frame_fem.h:197-224, eval_forces
+ * src/phys/frame_fem.cpp:1896-2070) - FemFrame::solve (src/phys/frame_fem.cpp:2514-3027). Replaces SoftBody::fem for a
+ * body of elastic sections (no plastic limits, mounts, contacts or held steps). The system - 6x6 blocks per node pair, FP64 - is
  * factored each step by a supernodal (multifrontal) Cholesky over a nested-dissection order, its dense fronts on
  * FP64 matrix cores.
  * ------------------------------------------------------------------------------------------------------------- */
@@ -304,13 +305,37 @@ typedef struct bl_shell_section {
     float damping; /* stiffness-proportional damping (s, default 3e-4) */
 } bl_shell_section;
 
+/* == FrameSection (src/phys/frame_fem.h:90-112), the fields an elastic member reads (no plastic limits) */
+typedef struct bl_frame_section {
+    float E, G, rho;
+    float A, Iy, Iz, J;     /* area, second moments about the member's y and z, torsion constant */
+    float As_y, As_z;       /* shear areas (Timoshenko; 0: no shear deformation) */
+    float damping;          /* stiffness-proportional damping (s) */
+    float axial;            /* factor on the axial stiffness */
+    float joint_k;          /* BL_FJ_ELASTIC ends: rotational stiffness (N m/rad) */
+} bl_frame_section;
+
+/* == FrameJoint (src/phys/frame_fem.h:185-193): what a member's end frees at its node */
+enum { BL_FJ_RIGID = 0, BL_FJ_BALL = 1, BL_FJ_HINGE_V = 2, BL_FJ_HINGE_H = 3, BL_FJ_SWIVEL = 4, BL_FJ_ELASTIC = 5 };
+
+/* == FrameElement (src/phys/frame_fem.h:197-224), as FemFrame::add_element takes it */
+typedef struct bl_frame_member {
+    uint32_t a, b;          /* its end nodes */
+    uint16_t section;       /* index into bl_fem_desc::sections */
+    uint8_t end_a, end_b;   /* BL_FJ_* of each end */
+} bl_frame_member;
+
 typedef struct bl_fem_desc {
     uint32_t n_nodes, n_tris;
     const float* p;         /* 3 per node: rest positions (the body's Node::p) */
     const uint32_t* tris;   /* 3 per triangle (FrameTri::n), counter-clockwise about the rest normal */
     const uint8_t* fixed;   /* per node: 1 = pinned (inv_mass 0) */
-    const float* mass;      /* per node (ShellMesher::tri's thirds), or null: from the section's rho t per area */
+    const float* mass;      /* per node (the body's Node::mass), or null: the triangles' thirds of their section's rho t
+                               per area (ShellMesher::tri; a body with members passes its masses) */
     bl_shell_section section;
+    uint32_t n_members, n_sections;
+    const bl_frame_member* members;   /* co-rotational Timoshenko members (FemFrame::elems), may be null */
+    const bl_frame_section* sections;
 } bl_fem_desc;
 
 typedef struct bl_fem_report {
@@ -336,6 +361,9 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
 /* The state: positions (double, 3 per node), v, w (3 per node), q (4 per node, x y z w); any pointer may be null. */
 int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q);
 int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q);
+/* Moments from outside on the nodes for the next step (FemFrame::torque: N m, 3 per node, host memory), added to any
+ * given before; the step uses them up. */
+int bl_fem_add_torques(bl_fem* f, const float* torques);
 /* The body's node records after the last step (SoftBody::nodes: p, inv_mass, v, mass), the forces the step handed the
  * integrator (SoftBody::force: dv m / step of a free node, the gathered forces where the solve was skipped; 3 per
  * node) and the counters {FemFrame::solve_failures, FemFrame::clamps, NaN velocity resets}

@@ -171,3 +171,31 @@ def make_frame(mod, fr, ref=False):
     cls = mod.RefPlate if ref else mod.FemPlate
     return cls.frame(fr["p"], fr["mass"], fr.get("fixed"), fr.get("tris"), SECTION_C3, fr.get("members"),
                      fr.get("sections"))
+
+
+def frames():
+    sec = steel_tube()
+    return {
+        "cantilever": frame_line(10, (0, 1, 0), (1, 1, 0), sec),
+        "two_members": frame_line(2, (0, 1, 0), (1, 1, 0), sec),
+        "portal": portal(sec),
+        "portal_ball_feet": portal(sec, feet=FJ_BALL),
+        "hinge_root": frame_line(10, (0, 1, 0), (1, 1, 0), sec, FJ_HINGE_V),
+        "swivel_root": frame_line(10, (0, 1, 0), (1, 1, 0), sec, FJ_SWIVEL),
+        "elastic_root": frame_line(10, (0, 1, 0), (1, 1, 0), steel_tube(joint_k=5.0e4), FJ_ELASTIC),
+        "stiffened_plate": stiffened_plate(6),
+    }
+
+
+def frame_loads(name, nn, steps, seed):
+    """Random loads and moments on every node; the two-member cantilever also gets a heavy tip load, bending its
+    members far enough that their co-rotated frames take the slerp's arc (end frames more than 3.6 degrees apart)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(steps):
+        ld = rng.normal(0, 20, (nn, 3)).astype(np.float32)
+        tq = rng.normal(0, 2, (nn, 3)).astype(np.float32)
+        if name == "two_members":
+            ld[-1, 1] -= 3000.0
+        out.append((ld, tq))
+    return out

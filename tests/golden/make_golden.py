@@ -79,11 +79,10 @@ def main():
         R.close()
     np.savez_compressed(os.path.join(HERE, "fem_step_ref.npz"), **out)
     print("written fem_step_ref.npz")
-    from test_fem_oracle import _frames, frame_loads
     out = {}
-    for name, fr in sorted(_frames().items()):
+    for name, fr in sorted(fc.frames().items()):
         R = fc.make_frame(oracle, fr, ref=True)
-        for s, (ld, tq) in enumerate(frame_loads(name, R.nn, 30, 3)):
+        for s, (ld, tq) in enumerate(fc.frame_loads(name, R.nn, 30, 3)):
             R.add_torques(tq)
             R.substep(ld, fc.H)
             if s % 10 == 9:

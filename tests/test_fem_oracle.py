@@ -199,45 +199,17 @@ def test_fem_substeps_match_golden():
 
 
 # ---------------------------------------------------------------------------------------------- members
-def _frames():
-    sec = fc.steel_tube()
-    return {
-        "cantilever": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec),
-        "two_members": fc.frame_line(2, (0, 1, 0), (1, 1, 0), sec),
-        "portal": fc.portal(sec),
-        "portal_ball_feet": fc.portal(sec, feet=fc.FJ_BALL),
-        "hinge_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_HINGE_V),
-        "swivel_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec, fc.FJ_SWIVEL),
-        "elastic_root": fc.frame_line(10, (0, 1, 0), (1, 1, 0), fc.steel_tube(joint_k=5.0e4), fc.FJ_ELASTIC),
-        "stiffened_plate": fc.stiffened_plate(6),
-    }
-
-
-def frame_loads(name, nn, steps, seed):
-    """Random loads and moments on every node; the two-member cantilever also gets a heavy tip load, bending its
-    members far enough that their co-rotated frames take the slerp's arc (end frames more than 3.6 degrees apart)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for s in range(steps):
-        ld = rng.normal(0, 20, (nn, 3)).astype(np.float32)
-        tq = rng.normal(0, 2, (nn, 3)).astype(np.float32)
-        if name == "two_members":
-            ld[-1, 1] -= 3000.0
-        out.append((ld, tq))
-    return out
-
-
-@pytest.mark.parametrize("name", sorted(_frames()))
+@pytest.mark.parametrize("name", sorted(fc.frames()))
 def test_frame_substeps_match_reference_bitwise(ref_mod, name):
     """Co-rotational Timoshenko members (kinematics, stiffness with joints and shear, the ends' forces and moments,
     element_matrix, member_world_k, their blocks and damping in the assembly, the moments from outside) against the
     compiled reference's FemFrame, bit for bit over 30 substeps of random loads and moments."""
-    fr = _frames()[name]
+    fr = fc.frames()[name]
     P, R = fc.make_frame(oracle, fr), fc.make_frame(oracle, fr, ref=True)
     a, b = R.read(), P.read()
     for k in ("p", "inv_mass", "v", "mass", "xd", "w", "q"):
         assert a[k].tobytes() == b[k].tobytes(), k
-    for s, (ld, tq) in enumerate(frame_loads(name, P.nn, 30, 3)):
+    for s, (ld, tq) in enumerate(fc.frame_loads(name, P.nn, 30, 3)):
         P.add_torques(tq), R.add_torques(tq)
         R.substep(ld, fc.H)
         P.step(ld, fc.H)
@@ -333,9 +305,9 @@ def test_frame_substeps_match_golden():
     runs where the reference is absent."""
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_frame_ref.npz"))
-    for name, fr in sorted(_frames().items()):
+    for name, fr in sorted(fc.frames().items()):
         P = fc.make_frame(oracle, fr)
-        for s, (ld, tq) in enumerate(frame_loads(name, P.nn, 30, 3)):
+        for s, (ld, tq) in enumerate(fc.frame_loads(name, P.nn, 30, 3)):
             P.add_torques(tq)
             P.step(ld, fc.H)
             if s % 10 == 9:

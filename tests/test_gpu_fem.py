@@ -203,3 +203,89 @@ def test_fem_large_plate_residual():
         rep = G.step(rng.normal(0, 5, (G.nn, 3)).astype(np.float32), 0.002)
         assert rep["residual"] <= 1e-8 and rep["failures"] == 0, rep
     print(rep, flush=True)
+
+
+# ---------------------------------------------------------------------------------------------- members
+def _gpu_frame(fr):
+    return fem.ShellBody.frame(fr["p"], fr["mass"], fr.get("fixed"), fr.get("tris"), fc.SECTION_C3, fr.get("members"),
+                               fr.get("sections"))
+
+
+@pytest.mark.parametrize("name", sorted(fc.frames()))
+def test_frame_substeps_track_oracle(oracle_mod, name):
+    """Frame bodies with co-rotational Timoshenko members (cantilevers on every joint kind, portals, a two-member
+    cantilever bent through the slerp's arc, a plate with a stiffener) on the GPU against the oracle (itself equal
+    bit for bit to the compiled reference): 30 substeps of random loads and moments, positions, velocities and spins
+    within 1e-5 relative every step, the counters equal."""
+    print(f"frame {name}", flush=True)
+    fr = fc.frames()[name]
+    P, G = fc.make_frame(oracle_mod, fr), _gpu_frame(fr)
+    a, b = G.read(), P.read()
+    for k in ("p", "inv_mass", "v", "mass", "xd", "w", "q"):
+        assert a[k].tobytes() == b[k].tobytes(), k
+    x0 = b["xd"].copy()
+    scale = dict(disp=0.0, v=0.0, w=0.0)
+    for s, (ld, tq) in enumerate(fc.frame_loads(name, P.nn, 30, 3)):
+        P.add_torques(tq), G.add_torques(tq)
+        P.step(ld, fc.H)
+        rep = G.step(ld, fc.H)
+        assert rep["residual"] <= 1e-8, rep
+        a, b = G.read(), P.read()
+        scale["disp"] = max(scale["disp"], float(np.abs(b["xd"] - x0).max()))
+        scale["v"] = max(scale["v"], float(np.abs(b["v"]).max()))
+        scale["w"] = max(scale["w"], float(np.abs(b["w"]).max()))
+        _close(a, b, x0, scale, f"{name} step {s}")
+        assert (a["failures"], a["clamps"]) == (b["failures"], b["clamps"])
+
+
+def test_frame_substeps_match_reference_golden():
+    """The GPU's members against the compiled reference's recorded states (tests/golden/fem_frame_ref.npz), within
+    1e-5 relative."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_frame_ref.npz"))
+    for name, fr in sorted(fc.frames().items()):
+        G = _gpu_frame(fr)
+        x0 = G.read()["xd"].copy()
+        scale = dict(disp=0.0, v=0.0, w=0.0)
+        for s, (ld, tq) in enumerate(fc.frame_loads(name, G.nn, 30, 3)):
+            G.add_torques(tq)
+            G.step(ld, fc.H)
+            a = G.read()
+            scale["disp"] = max(scale["disp"], float(np.abs(a["xd"] - x0).max()))
+            scale["v"] = max(scale["v"], float(np.abs(a["v"]).max()))
+            scale["w"] = max(scale["w"], float(np.abs(a["w"]).max()))
+            if s % 10 == 9:
+                b = {k: g[f"{name}_{s}_{k}"] for k in fc.STATE_KEYS}
+                _close(a, b, x0, scale, f"{name} step {s}", rel=2e-5)
+        G.close()
+
+
+def test_frame_closed_forms_on_gpu():
+    """The reference's quasi-static frame cases on the GPU (tests/test_physics.cpp:2068-2128): the cantilever's tip
+    under 50 N (PL^3/3EI + PL/kGA, 1 %) and its rotation (PL^2/2EI, 1 %), and the portal's sway under 200 N
+    (P / (24 EI 7/10), 2 %); no gravity, theta 1, dissipation 1."""
+    sec = fc.steel_tube()
+    E, G_, A, Iz, J, As = (float(sec[i]) for i in (0, 1, 3, 5, 6, 7))
+    EI, kGA = E * Iz, G_ * As
+    Gb = _gpu_frame(fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec))
+    x0 = Gb.read()["xd"][-1].copy()
+    ld = np.zeros((Gb.nn, 3), np.float32)
+    ld[-1, 1] = -50.0
+    for _ in range(4000):
+        Gb.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    st = Gb.read()
+    q = st["q"][-1].astype(np.float64)
+    sn = np.linalg.norm(q[:3])
+    rot = q[:3] * (2 * np.arctan2(sn, q[3]) / sn)
+    want, want_r = 50 / (3 * EI) + 50 / kGA, 50 / (2 * EI)
+    assert abs(-(st["xd"][-1] - x0)[1] / want - 1) < 0.01 and abs(-rot[2] / want_r - 1) < 0.01
+    Gp = _gpu_frame(fc.portal(sec))
+    x0 = Gp.read()["xd"].copy()
+    ld = np.zeros((Gp.nn, 3), np.float32)
+    ld[2, 0] = 200.0
+    for _ in range(4000):
+        Gp.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    xd = Gp.read()["xd"]
+    got = 0.5 * ((xd[2] - x0[2])[0] + (xd[3] - x0[3])[0])
+    want = 200.0 / (24 * EI * 7 / 10)
+    assert abs(got / want - 1) < 0.02, (got, want)
