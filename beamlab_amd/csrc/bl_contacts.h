@@ -30,7 +30,7 @@ struct RStat {
     uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles, 2 a scan's wait timed out
     uint32_t last_pairs;  // candidate pairs kept in the last pair island
     uint32_t npart;       // partner entries (when they overflowed their buffer)
-    uint32_t nisl;        // pair islands of the rebuild (the response's workgroups read it)
+    uint32_t nisl;        // response groups of the rebuild (the response's workgroups read it)
 };
 
 // A one-pass launch's look-back: the workgroups' published words, the ticket counter (null: workgroups in index
@@ -63,7 +63,9 @@ struct CArgs {
     uint32_t n_itri;
     const uint32_t* icap;     // the capsules of the pair islands' bodies, in the same order
     uint32_t n_icap;
-    uint32_t* irange;         // per pair island: its runs [start, end) in the NT, NC and CT lists (6 words)
+    const int* igroup;        // per pair island: its response group (consecutive islands, balanced by size)
+    uint32_t ngroup;
+    uint32_t* irange;         // per response group: its runs [start, end) in the NT, NC and CT lists (6 words)
     const uint32_t* sample_base;
     float* samples;           // every 7th triangle's longest edge, in body order
     uint32_t nsamples;
@@ -187,8 +189,10 @@ private:
     std::vector<uint32_t> h_tri0_, itri_h_; // per body: first triangle; the pair islands' triangles
     std::vector<uint32_t> h_cap0_, icap_h_; // per body: first capsule; the pair islands' capsules
     Buf<uint32_t> itri, icap, irange;
-    uint32_t n_itri = 0, n_icap = 0, gresp_ = 0;
-    std::vector<int> up_isl_, up_mi_;
+    Buf<int> igroup;
+    std::vector<int> igroup_h_;
+    uint32_t n_itri = 0, n_icap = 0, gresp_ = 0, ngroup_ = 0;
+    std::vector<int> up_isl_, up_mi_, up_group_;
     std::vector<uint32_t> up_off_, up_mem_;
     bool up_valid_ = false;
     bool counts_dirty_ = true; // the bucket counts need clearing before the next rebuild

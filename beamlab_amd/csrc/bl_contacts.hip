@@ -919,15 +919,17 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
     if (tk == gridDim.x - 1 && threadIdx.x == 0) c.st->nnt = pre + agg;
 }
 
-// Each pair island's run in the three final lists (NT after the filter, NC, CT): the searches go over the islands'
-// triangles and capsules island by island, so an island's candidates are one run in each list; its bounds are where
-// the island changes. (A list whose islands are not in order would break that: flagged, the rebuild fails.)
-__device__ __forceinline__ void island_run(const int* bisl, const uint4* L, uint32_t n, uint32_t i, uint32_t* irange,
-                                           int which, RStat* st) {
+// Each response group's run in the three final lists (NT after the filter, NC, CT): the searches go over the islands'
+// triangles and capsules island by island, so an island's candidates - and a group's, its islands being consecutive -
+// are one run in each list; its bounds are where the group changes. (A list whose islands are not in order would break
+// that: flagged, the rebuild fails.)
+__device__ __forceinline__ void island_run(const int* bisl, const int* igroup, const uint4* L, uint32_t n, uint32_t i,
+                                           uint32_t* irange, int which, RStat* st) {
     if (i >= n) return;
-    const int isl = bisl[L[i].x];
-    const int prev = i > 0 ? bisl[L[i - 1].x] : -1;
-    const int next = i + 1 < n ? bisl[L[i + 1].x] : -1;
+    auto group = [&](uint32_t k) { const int b = bisl[L[k].x]; return b < 0 ? -1 : igroup[b]; };
+    const int isl = group(i);
+    const int prev = i > 0 ? group(i - 1) : -1;
+    const int next = i + 1 < n ? group(i + 1) : -1;
     if (isl < 0 || prev > isl) atomicOr(&st->unsupported, 8u);
     if (isl < 0) return;
     if (isl != prev) irange[6 * isl + 2 * which] = i;
@@ -937,11 +939,11 @@ __device__ __forceinline__ void island_run(const int* bisl, const uint4* L, uint
 __global__ void k_rb_ranges(KArgs a, CArgs c) {
     (void)a;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) c.st->nisl = c.nisl;
+    if (i == 0) c.st->nisl = c.ngroup;
     const uint32_t nnt = min(c.st->nnt, c.cap_nt), nnc = min(c.st->nnc, c.cap_nc), nct = min(c.st->nct, c.cap_ct);
-    island_run(c.bisl, c.nt, nnt, i, c.irange, 0, c.st);
-    island_run(c.bisl, c.nc, nnc, i, c.irange, 1, c.st);
-    island_run(c.bisl, c.ct, nct, i, c.irange, 2, c.st);
+    island_run(c.bisl, c.igroup, c.nt, nnt, i, c.irange, 0, c.st);
+    island_run(c.bisl, c.igroup, c.nc, nnc, i, c.irange, 1, c.st);
+    island_run(c.bisl, c.igroup, c.ct, nct, i, c.irange, 2, c.st);
 }
 
 // ---------------------------------------------------------------------------------------------- narrow phase
@@ -1031,36 +1033,47 @@ __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn
     return true;
 }
 
-// One workgroup per pair island (a grid-stride loop over them; islands share no node, so they run side by side and
-// the result is the serial one). An island's hits are those of its runs in the three lists, in list order; they go
-// in windows of at most 1024: the consecutive narrow-phase blocks of a run whose hits in the run fit. In a window each
+// One workgroup per response group (consecutive pair islands; a grid-stride loop over them; groups share no node, so
+// they run side by side and the result is the serial one). A group's hits are those of its runs in the three lists, in
+// list order (within an island the reference's order; its islands' hits are independent); they go in windows of at
+// most 1024: the most consecutive narrow-phase blocks of a run (up to 256) whose hits in the run fit. In a window each
 // hit's predecessors (the latest earlier hit of the window on each of its nodes) give its level, the longest chain to
 // it; the levels run one after another, the hits of a level (no shared node) in parallel.
-constexpr int kRespW = 1024, kRespKeys = 8192, kRespBlk = 16;
+constexpr int kRespW = 1024, kRespKeys = 8192, kRespBlk = 256;
 __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int parity) {
     __shared__ uint64_t keys[kRespKeys];
     __shared__ int16_t preds[kRespW * 5];
     __shared__ uint16_t level[kRespW];
     __shared__ uint32_t sred[kRespW / 64];
-    __shared__ uint32_t wfirst[kRespBlk], wcnt[kRespBlk], wpre[kRespBlk + 1], wnb;
+    __shared__ uint16_t wfirst[kRespBlk];
+    __shared__ uint32_t wblk[kRespBlk];
+    __shared__ uint32_t wpre[kRespBlk + 1], wtot[kRespBlk / 64], wnb;
     PArgs p = pin;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
     const uint32_t nisl = pin.st->nisl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t contacts = 0;
-    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {
+    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) { // (a group)
+        // its runs in the concatenated candidates (node-triangle, node-capsule, capsule-triangle) as one sequence of
+        // (virtual) blocks: each run's blocks, a block shared by two runs once for each
+        uint32_t slo[3], shi[3], sb0[3], sv0[3], nv = 0;
         for (int L = 0; L < 3; L++) {
             const uint32_t base = L == 0 ? 0u : (L == 1 ? p.nnt : p.nnt + p.nnc);
-            const uint32_t lo = base + p.irange[6 * isl + 2 * L], hi = base + p.irange[6 * isl + 2 * L + 1];
-            if (lo >= hi) continue;
-            const uint32_t b1 = (hi - 1) / 256;
-            for (uint32_t wb = lo / 256; wb <= b1;) {
-                // the next blocks' hits within [lo, hi) (hits are in candidate order in a block), and how many of the
-                // blocks fit one window
+            slo[L] = base + p.irange[6 * isl + 2 * L], shi[L] = base + p.irange[6 * isl + 2 * L + 1];
+            sb0[L] = slo[L] / 256, sv0[L] = nv;
+            if (slo[L] < shi[L]) nv += (shi[L] - 1) / 256 - sb0[L] + 1;
+        }
+        {
+            for (uint32_t wb = 0; wb < nv;) {
+                // the next blocks' hits within [lo, hi) (hits are in candidate order in a block), their prefix, and how
+                // many of the blocks fit one window (at least one)
+                uint32_t cntv = 0;
                 if (tid < (uint32_t)kRespBlk) {
-                    const uint32_t b = wb + tid;
+                    const uint32_t v = wb + tid;
+                    const int L = v < sv0[1] ? 0 : (v < sv0[2] ? 1 : 2);
+                    const uint32_t b = sb0[L] + (v - sv0[L]), lo = slo[L], hi = shi[L];
                     uint32_t f = 0, e = 0;
-                    if (b <= b1) {
+                    if (v < nv) {
                         const Hit* H = p.hraw + (size_t)b * 256;
                         e = p.bcnt[b];
                         if (b * 256 < lo) {
@@ -1082,18 +1095,23 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                             e = l2;
                         }
                     }
-                    wfirst[tid] = f, wcnt[tid] = e - f;
+                    wfirst[tid] = (uint16_t)f, wblk[tid] = b, cntv = e - f;
+                }
+                uint32_t x = cntv;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t y = __shfl_up(x, off);
+                    if (lane >= (uint32_t)off) x += y;
+                }
+                if (tid < (uint32_t)kRespBlk && lane == 63) wtot[wv] = x;
+                if (tid == 0) wpre[0] = 0, wnb = 0;
+                __syncthreads();
+                if (tid < (uint32_t)kRespBlk) {
+                    for (uint32_t w = 0; w < wv; w++) x += wtot[w];
+                    wpre[tid + 1] = x;
                 }
                 __syncthreads();
-                if (tid == 0) {
-                    uint32_t acc = 0, nb = 0;
-                    wpre[0] = 0;
-                    while (nb < (uint32_t)kRespBlk && wb + nb <= b1 && (nb == 0 || acc + wcnt[nb] <= (uint32_t)kRespW)) {
-                        acc += wcnt[nb];
-                        wpre[++nb] = acc;
-                    }
-                    wnb = nb;
-                }
+                if (tid < (uint32_t)kRespBlk && wb + tid < nv && (tid == 0 || wpre[tid + 1] <= (uint32_t)kRespW))
+                    atomicMax(&wnb, tid + 1);
                 __syncthreads();
                 const uint32_t nbw = wnb, m = wpre[nbw];
                 const bool valid = tid < m;
@@ -1101,9 +1119,14 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                 ContactNodes cn;
                 cn.na = cn.nb = 0;
                 if (valid) {
-                    uint32_t k = 0;
-                    while (k + 1 < nbw && wpre[k + 1] <= tid) k++;
-                    hit = p.hraw[(size_t)(wb + k) * 256 + wfirst[k] + (tid - wpre[k])];
+                    // the block holding the window's hit tid: the last one whose prefix is at most tid
+                    uint32_t l2 = 0, h2 = nbw - 1;
+                    while (l2 < h2) {
+                        const uint32_t mid = (l2 + h2 + 1) >> 1;
+                        if (wpre[mid] <= tid) l2 = mid;
+                        else h2 = mid - 1;
+                    }
+                    hit = p.hraw[(size_t)wblk[l2] * 256 + wfirst[l2] + (tid - wpre[l2])];
                     contact_nodes(a, p, hit, cn);
                 }
                 if (m > 0) {
@@ -1277,6 +1300,7 @@ void ContactEngine::release() {
     for (Buf<uint32_t>* b : {&imem_off, &imem, &bquery, &body_tri0, &sample_base, &icount, &pbucket, &tcount, &tstart,
                              &eb, &ecount, &estart, &bcnt, &icap, &irange})
         b->release();
+    igroup.release(), ngroup_ = 0;
     bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
     itri.release(), n_itri = 0, n_icap = 0, gresp_ = 0;
     scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
@@ -1445,17 +1469,43 @@ int ContactEngine::begin_frame(hipStream_t st) {
     RC(imem.ensure(mem.size() + 1));
     RC(itri.ensure((size_t)n_itri + 1));
     RC(icap.ensure((size_t)n_icap + 1));
-    // the response's workgroups: one per pair island, in powers of two (a captured segment keeps its launch until
-    // the count outgrows it)
-    if (np > gresp_) gresp_ = pow2_at_least(np);
+    // the response's groups: consecutive pair islands balanced by size (nodes and triangles), at most kRespGroups, so
+    // that a group's hits are one run in each list; a workgroup per group, in powers of two (a captured segment keeps
+    // its launch until the count outgrows it)
+    constexpr uint32_t kRespGroups = 256;
+    {
+        igroup_h_.assign(np + 1, 0);
+        double total = 0;
+        std::vector<double> wk(np, 0.0);
+        for (size_t k = 0; k < np; k++) {
+            for (uint32_t q = off[k]; q < off[k + 1]; q++) {
+                const uint32_t g = mem[q];
+                wk[k] += (double)h_nnode_[g] + (double)(h_tri0_[g + 1] - h_tri0_[g]);
+            }
+            total += wk[k];
+        }
+        const uint32_t G = (uint32_t)std::min<size_t>(np, kRespGroups);
+        uint32_t grp = 0;
+        double acc = 0;
+        for (size_t k = 0; k < np; k++) {
+            igroup_h_[k] = (int)grp;
+            acc += wk[k];
+            if (grp + 1 < G && acc >= total * (double)(grp + 1) / (double)G) grp++;
+        }
+        ngroup_ = np ? std::min<uint32_t>(grp + 1, G) : 0;
+        if (np && igroup_h_[np - 1] + 1 < (int)ngroup_) ngroup_ = (uint32_t)igroup_h_[np - 1] + 1;
+    }
+    if (ngroup_ > gresp_) gresp_ = pow2_at_least(ngroup_);
     RC(irange.ensure(6 * (size_t)std::max<uint32_t>(gresp_, 1)));
+    const void* was_g = igroup.p;
+    RC(igroup.ensure(np + 1));
     RC(pcnt.ensure(nmem + 1));
     RC(poff.ensure(nmem + 1));
     cap_part = std::max<uint32_t>(cap_part, std::max<uint32_t>(4096, 8 * nmem));
     RC(plist.ensure(cap_part));
     // (the same tables as the last frame's, in the same buffers: nothing to upload)
     const bool same = up_valid_ && was[0] == imem_off.p && was[1] == imem.p && was[2] == itri.p && was[3] == icap.p &&
-                      isl == up_isl_ &&
+                      was_g == igroup.p && igroup_h_ == up_group_ && isl == up_isl_ &&
                       mi == up_mi_ && off == up_off_ && mem == up_mem_;
     if (!same) {
         const std::vector<uint32_t> off0(1, 0u);
@@ -1466,8 +1516,9 @@ int ContactEngine::begin_frame(hipStream_t st) {
         if (!mem.empty()) CHK(hipMemcpyAsync(imem.p, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, st));
         if (n_itri) CHK(hipMemcpyAsync(itri.p, itri_h_.data(), 4 * (size_t)n_itri, hipMemcpyHostToDevice, st));
         if (n_icap) CHK(hipMemcpyAsync(icap.p, icap_h_.data(), 4 * (size_t)n_icap, hipMemcpyHostToDevice, st));
+        if (np) CHK(hipMemcpyAsync(igroup.p, igroup_h_.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
         CHK(hipStreamSynchronize(st)); // (off0 goes out of scope)
-        up_isl_ = isl, up_mi_ = mi, up_off_ = off, up_mem_ = mem;
+        up_isl_ = isl, up_mi_ = mi, up_off_ = off, up_mem_ = mem, up_group_ = igroup_h_;
         up_valid_ = true;
     }
     nnt = nnc = nct = 0;
@@ -1519,7 +1570,7 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     // the near-pair filter on the node-triangle candidates
     if (T) RC(stage(k_rb_near, ((size_t)cap_nt + 255) / 256));
     // each island's runs in the three lists
-    hipLaunchKernelGGL(k_rb_clear, grid_for(6 * (size_t)c.nisl), dim3(256), 0, st, c.irange, 6 * c.nisl);
+    hipLaunchKernelGGL(k_rb_clear, grid_for(6 * (size_t)c.ngroup), dim3(256), 0, st, c.irange, 6 * c.ngroup);
     const size_t nr = std::max<size_t>(1, std::max<size_t>(cap_nt, std::max(cap_nc, cap_ct)));
     hipLaunchKernelGGL(k_rb_ranges, grid_for(nr), dim3(256), 0, st, a, c);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
@@ -1541,6 +1592,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
     c.nsamples = nsamples;
     c.itri = itri.p, c.n_itri = n_itri, c.icap = icap.p, c.n_icap = n_icap, c.irange = irange.p;
+    c.igroup = igroup.p, c.ngroup = ngroup_;
     c.st = stat.p, c.icount = icount.p;
     c.pbucket = pbucket.p, c.pcell = pcell.p, c.ppos = ppos.p;
     c.tmask = tsize_f - 1, c.tcount = tcount.p, c.tstart = tstart.p;
