@@ -1033,17 +1033,123 @@ __device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn
     return true;
 }
 
+// contact_force and primitive_collision split at the forces: the terms that read only velocities, inverse masses and
+// the geometry (which the response does not change) are computed once per hit, before the window's rounds; each round
+// the concrete model of the inter-body contacts, as compile-time constants
+constexpr ground_model kGmAll[BL_SURF_N] = BL_GROUND_MODELS;
+constexpr ground_model kGmContact = kGmAll[BL_SURF_CONCRETE];
+static_assert(kGmContact.soft_depth == 0, "the contact model has no fluid layer");
+
+// then does what reads the forces accumulated so far. The same operations in the same order as contact_force above.
+struct CfPre {
+    float m, Vn, roff, sv, stop, e_static, g_slide;
+    f3 vel, slip;
+    float wim[5]; // w_i x inverse mass_i
+    bool ok;
+};
+
+__device__ __forceinline__ void contact_force_pre(const ContactNodes& cn, const float* im, const f3* vel, f3 nrm,
+                                                  float pen, float dt, CfPre& q) {
+    constexpr ground_model gm = kGmContact;
+    const int na = cn.na, nn = cn.na + cn.nb;
+    float W = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if (i < nn) W += cn.w[i] * cn.w[i] * im[i];
+    q.ok = !(W <= 1e-12f);
+    q.m = 1.0f / W;
+    f3 va = mk(0, 0, 0), vb = mk(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        q.wim[i] = cn.w[i] * im[i];
+        if (i < na) va = add(va, mul(vel[i], cn.w[i]));
+        else if (i < nn) vb = add(vb, mul(vel[i], cn.w[i]));
+    }
+    // primitive_collision(f_rel, va - vb, m, nrm, dt, concrete, pen, fric, push_max 2, slop 0, restitution -1)
+    const f3 v = sub(va, vb);
+    q.vel = v;
+    q.Vn = dot(v, nrm);
+    const float sgl = gm.soft_depth, take = 0.8f;
+    q.roff = (take * q.Vn - smin(0.2f * smax(0.0f, pen - sgl - 0.0f) / dt, 2.0f)) * q.m / dt;
+    f3 slip = sub(v, mul(nrm, q.Vn));
+    q.sv = length(slip);
+    q.slip = q.sv > 1e-9f ? divs(slip, q.sv) : mk(0, 0, 0);
+    q.stop = q.m * q.sv / dt;
+    q.e_static = 1.0f - bl_expf(-q.sv / gm.va);
+    const float g = gm.mc + (gm.ms - gm.mc) * bl_expf(-pow_alpha(q.sv / gm.vs, gm.alpha));
+    q.g_slide = g + smin(gm.t2 * q.sv, 5.0f);
+}
+
+// the LDS cell of the slots a contact does not use (slots of other contacts are 0 .. 5 kRespW - 1)
+constexpr uint32_t kRespDummy = 5 * 512;
+
+__device__ __forceinline__ float selp(bool c, float x, float y) { return c ? x : y; }
+__device__ __forceinline__ f3 selp(bool c, f3 x, f3 y) { return mk(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z); }
+
+// The rest of contact_force on the forces in LDS, without branches: slot i of the contact reads and writes
+// fs[3 root[i] ..], the one LDS copy of its node's force (an unused slot the dummy cell, its result not taken). The
+// forces are all read first, as contact_force reads them all before it writes; a slot after another of the same node
+// in the contact (a ball's capsule of no length: dup[i], else -1) starts its write from that one's result.
+__device__ __forceinline__ void contact_force_apply(const ContactNodes& cn, const CfPre& q, const uint16_t* root,
+                                                    const int* dup, float* fs, f3 nrm, float pen) {
+    constexpr ground_model gm = kGmContact;
+    const int na = cn.na, nn = cn.na + cn.nb;
+    f3 Fi[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const float* F = fs + 3 * root[i];
+        Fi[i] = mk(F[0], F[1], F[2]);
+    }
+    f3 acc = mk(0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const f3 x = mul(Fi[i], q.wim[i]);
+        acc = selp(i < na, add(acc, x), selp(i < nn, sub(acc, x), acc));
+    }
+    const f3 F = mul(acc, q.m);
+    const float Fn = dot(F, nrm);
+    float R = -Fn;
+    R = selp(q.Vn < 0, R - q.roff, R);
+    const f3 slipF = sub(F, mul(nrm, Fn));
+    const float G = R * gm.strength * cn.fric;
+    const bool stat = q.sv < gm.va && G > 0 && length(slipF) <= gm.ms * G;
+    const f3 o_stat = sub(sub(mul(nrm, R), mul(q.slip, smin(gm.ms * G * q.e_static, q.stop))), slipF);
+    const f3 o_slide = sub(mul(nrm, R), mul(q.slip, smin(q.g_slide * G, smax(0.0f, q.stop + dot(slipF, q.slip)))));
+    const f3 zero = mk(0, 0, 0);
+    const f3 out = selp(pen >= gm.soft_depth && R > 0, add(zero, selp(stat, o_stat, o_slide)), zero);
+    f3 nw[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        f3 base = Fi[i];
+#pragma unroll
+        for (int j = 0; j < i; j++) base = selp(dup[i] == j, nw[j], base);
+        const f3 d = mul(out, cn.w[i]);
+        nw[i] = selp(i < na, add(base, d), sub(base, d));
+        float* Fs = fs + 3 * root[i];
+        Fs[0] = nw[i].x, Fs[1] = nw[i].y, Fs[2] = nw[i].z;
+    }
+}
+
 // One workgroup per response group (consecutive pair islands; a grid-stride loop over them; groups share no node, so
 // they run side by side and the result is the serial one). A group's hits are those of its runs in the three lists, in
 // list order (within an island the reference's order; its islands' hits are independent); they go in windows of at
-// most 1024: the most consecutive narrow-phase blocks of a run (up to 256) whose hits in the run fit. In a window each
+// most 512: the most consecutive narrow-phase blocks of a run (up to 256) whose hits in the run fit. In a window each
 // hit's predecessors (the latest earlier hit of the window on each of its nodes) give its level, the longest chain to
-// it; the levels run one after another, the hits of a level (no shared node) in parallel.
-constexpr int kRespW = 1024, kRespKeys = 8192, kRespBlk = 256;
+// it; the levels run one after another, the hits of a level (no shared node) in parallel: in round r the hits whose
+// predecessors all ran in earlier rounds. The window's node forces are held in LDS meanwhile, one copy per node at the
+// node's first slot in the window (its root), read from F before the first round and written back after the last.
+constexpr int kRespW = 512, kRespKeys = 4096, kRespBlk = 256, kRespH = 4096;
+// a node's bucket among the window's kRespH (a multiplicative hash: consecutive nodes spread)
+__device__ __forceinline__ uint32_t resp_bucket(uint32_t n) { return (n * 0x9E3779B1u) >> (32 - 12); }
+static_assert(kRespH == 1 << 12 && kRespH % kRespW == 0, "resp_bucket takes the hash's top 12 bits");
+static_assert(kRespDummy == 5 * kRespW && 2 * kRespKeys >= 3 * (kRespDummy + 1), "the dummy slot follows the window's slots");
 __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int parity) {
     __shared__ uint64_t keys[kRespKeys];
-    __shared__ int16_t preds[kRespW * 5];
-    __shared__ uint16_t level[kRespW];
+    __shared__ uint32_t bstart[kRespH + 1];
+    __shared__ uint16_t sroot[kRespW * 5]; // per slot: the first slot of its node in the window
+    __shared__ int16_t spred[kRespW * 5];  // per slot: the latest earlier hit with a slot of its node, or -1
+    __shared__ uint32_t left[3];           // the hits run by a round (three rounds' counts in turn)
+    __shared__ uint16_t level[kRespW + 1]; // per hit: the round it ran in (0xffff: not yet); [kRespW]: always 0
     __shared__ uint32_t sred[kRespW / 64];
     __shared__ uint16_t wfirst[kRespBlk];
     __shared__ uint32_t wblk[kRespBlk];
@@ -1130,84 +1236,137 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                     contact_nodes(a, p, hit, cn);
                 }
                 if (m > 0) {
-                    uint32_t nk = 64;
-                    while (nk < 5 * m) nk <<= 1;
-                    if (5 * m <= (uint32_t)kRespW) {
-                        // a small window: each (hit, node) finds its predecessor - the latest earlier hit on the same
-                        // node - by reading back through the earlier hits' nodes
-                        uint32_t* hn = (uint32_t*)keys;
+                    // the window's slots hashed by node into buckets (count, scan, scatter): a slot finds the other
+                    // slots of its node in its bucket
+                    const int nn = cn.na + cn.nb;
+                    // the nodes' inverse masses, velocities and forces (read now, used after the bucketing)
+                    float im[5];
+                    f3 vel[5], f0[5];
+                    {
+                        const float4* P = a.pm[parity];
+                        const float4* Vv = a.vm[parity];
 #pragma unroll
                         for (int j = 0; j < 5; j++) {
-                            hn[tid * 5 + j] = j < cn.na + cn.nb ? cn.n[j] : ~0u;
-                            preds[tid * 5 + j] = -1;
-                        }
-                        __syncthreads();
-                        if (tid < 5 * m) {
-                            const uint32_t node = hn[tid], me = tid / 5;
-                            int pq = -1;
-                            if (node != ~0u)
-                                for (int q = (int)me - 1; q >= 0 && pq < 0; q--)
-#pragma unroll
-                                    for (int j = 0; j < 5; j++)
-                                        if (hn[q * 5 + j] == node) pq = q;
-                            preds[tid] = (int16_t)pq;
-                        }
-                    } else {
-                        for (uint32_t k = tid; k < nk; k += kRespW) keys[k] = ~0ull;
-                        __syncthreads();
-#pragma unroll
-                        for (int j = 0; j < 5; j++) {
-                            if (j < cn.na + cn.nb) keys[tid * 5 + j] = ((uint64_t)cn.n[j] << 32) | (tid << 3) | (uint32_t)j;
-                            preds[tid * 5 + j] = -1;
-                        }
-                        __syncthreads();
-                        // bitonic sort of the window's keys
-                        for (uint32_t size = 2; size <= nk; size <<= 1)
-                            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                                for (uint32_t q = tid; q < nk / 2; q += kRespW) {
-                                    const uint32_t i = 2 * q - (q & (stride - 1));
-                                    const uint32_t j = i + stride;
-                                    const bool up = (i & size) == 0;
-                                    const uint64_t x = keys[i], y = keys[j];
-                                    if ((x > y) == up) keys[i] = y, keys[j] = x;
-                                }
-                                __syncthreads();
+                            im[j] = 0, vel[j] = mk(0, 0, 0), f0[j] = mk(0, 0, 0);
+                            if (j < nn) {
+                                const uint32_t n = cn.n[j];
+                                im[j] = P[n].w, vel[j] = ld3(Vv[n]), f0[j] = ld3(a.F[n]);
                             }
-                        // predecessors: the previous key of the same node from another (earlier) hit
-                        for (uint32_t k = tid; k < nk; k += kRespW) {
-                            const uint64_t key = keys[k];
-                            if (key == ~0ull) continue;
-                            const uint32_t node = (uint32_t)(key >> 32), me = (uint32_t)(key >> 3) & (kRespW - 1);
-                            int q = (int)k - 1;
-                            while (q >= 0 && (uint32_t)(keys[q] >> 32) == node && ((uint32_t)(keys[q] >> 3) & (kRespW - 1)) == me) q--;
-                            if (q >= 0 && (uint32_t)(keys[q] >> 32) == node) preds[me * 5 + (key & 7)] = (int16_t)((keys[q] >> 3) & (kRespW - 1));
                         }
                     }
-                    level[tid] = 0;
+                    for (uint32_t k = tid; k < (uint32_t)kRespH; k += kRespW) bstart[k] = 0;
                     __syncthreads();
-                    // levels: one past the deepest predecessor (as many rounds as the longest chain)
-                    int changed = 1;
-                    while (__syncthreads_or(changed)) {
-                        changed = 0;
-                        uint32_t Lv = level[tid];
-                        for (int j = 0; j < 5; j++) {
-                            const int pq = preds[tid * 5 + j];
-                            if (pq >= 0 && level[pq] + 1u > Lv) Lv = level[pq] + 1u;
+                    uint32_t pos[5];
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (j < nn) pos[j] = atomicAdd(&bstart[resp_bucket(cn.n[j])], 1u);
+                    __syncthreads();
+                    {
+                        constexpr int per = kRespH / kRespW;
+                        uint32_t v[per], tot = 0;
+#pragma unroll
+                        for (int i = 0; i < per; i++) v[i] = bstart[tid * per + i], tot += v[i];
+                        uint32_t x = tot;
+                        for (int off = 1; off < 64; off <<= 1) {
+                            const uint32_t y = __shfl_up(x, off);
+                            if (lane >= (uint32_t)off) x += y;
                         }
+                        if (lane == 63) sred[wv] = x;
                         __syncthreads();
-                        if (Lv != level[tid]) level[tid] = (uint16_t)Lv, changed = 1;
+                        uint32_t run = x - tot;
+                        for (uint32_t w = 0; w < wv; w++) run += sred[w];
+#pragma unroll
+                        for (int i = 0; i < per; i++) bstart[tid * per + i] = run, run += v[i];
+                        if (tid == kRespW - 1) bstart[kRespH] = run;
                     }
-                    const uint32_t my = valid ? level[tid] : 0u;
-                    uint32_t top = my;
-                    for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
-                    if (lane == 0) sred[wv] = top;
                     __syncthreads();
-                    uint32_t maxl = 0;
-                    for (uint32_t w = 0; w < kRespW / 64; w++) maxl = max(maxl, sred[w]);
-                    for (uint32_t Lv = 0; Lv <= maxl; Lv++) {
-                        if (valid && my == Lv && contact_force(a, parity, cn, hit.nrm, hit.pen)) contacts++;
-                        __syncthreads();
+                    uint64_t* ent = keys;
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (j < nn) ent[bstart[resp_bucket(cn.n[j])] + pos[j]] = ((uint64_t)cn.n[j] << 32) | (tid * 5 + j);
+                    __syncthreads();
+                    // each bucket record's root (the least slot of its node in the bucket) and predecessor (the
+                    // greatest earlier hit with a slot of the node), a thread per record, into LDS by slot
+                    const uint32_t nent = bstart[kRespH];
+                    for (uint32_t k = tid; k < nent; k += kRespW) {
+                        const uint64_t me = ent[k];
+                        const uint32_t node = (uint32_t)(me >> 32), t0 = (uint32_t)me, h0 = t0 / 5;
+                        const uint32_t b = resp_bucket(node), lo = bstart[b], hi = bstart[b + 1];
+                        uint32_t root = t0;
+                        int pq = -1;
+                        for (uint32_t i = lo; i < hi; i += 4) {
+                            uint64_t e[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) e[u] = i + u < hi ? ent[i + u] : ~0ull;
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if ((uint32_t)(e[u] >> 32) != node) continue;
+                                const uint32_t t = (uint32_t)e[u], h = t / 5;
+                                root = min(root, t);
+                                if (h < h0) pq = max(pq, (int)h);
+                            }
+                        }
+                        sroot[t0] = (uint16_t)root, spred[t0] = (int16_t)pq;
                     }
+                    __syncthreads();
+                    // (a slot the contact does not use: the dummy cell, no predecessor - the always-zero level)
+                    uint16_t rt[5], px[5];
+                    int dup[5];
+#pragma unroll
+                    for (int j = 0; j < 5; j++) {
+                        rt[j] = (uint16_t)kRespDummy, px[j] = (uint16_t)kRespW, dup[j] = -1;
+                        if (j < nn) {
+                            const int pq = spred[tid * 5 + j];
+                            rt[j] = sroot[tid * 5 + j], px[j] = (uint16_t)(pq >= 0 ? pq : kRespW);
+                        }
+#pragma unroll
+                        for (int i = 0; i < j; i++)
+                            if (j < nn && rt[i] == rt[j]) dup[j] = i;
+                    }
+                    // the nodes' forces into LDS at their roots (the bucket records are read by now), the
+                    // velocity-only terms of each hit
+                    __syncthreads();
+                    float* fs = (float*)keys;
+                    CfPre q;
+                    q.ok = false;
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (j < nn && rt[j] == tid * 5 + j)
+                            fs[3 * rt[j]] = f0[j].x, fs[3 * rt[j] + 1] = f0[j].y, fs[3 * rt[j] + 2] = f0[j].z;
+                    if (valid) contact_force_pre(cn, im, vel, hit.nrm, hit.pen, a.dt, q);
+                    // rounds: a hit runs once every predecessor ran in an earlier round (rounds = the longest chain).
+                    // One barrier a round: round r counts the hits it ran into left[r % 3] (one atomic per wave),
+                    // which every thread adds to its own total after the barrier; left[(r + 1) % 3], last read in
+                    // round r - 1, is cleared for the next round meanwhile.
+                    level[tid] = valid ? 0xffffu : 0u;
+                    if (tid < 3) left[tid] = 0;
+                    if (tid == 0) level[kRespW] = 0;
+                    bool done = !valid;
+                    __syncthreads();
+                    for (uint32_t r = 1, total = 0; total < m; r++) {
+                        bool ran = false;
+                        if (!done) {
+                            bool ready = true;
+#pragma unroll
+                            for (int j = 0; j < 5; j++) ready = ready && level[px[j]] < r;
+                            if (ready) {
+                                if (q.ok) contact_force_apply(cn, q, rt, dup, fs, hit.nrm, hit.pen), contacts++;
+                                level[tid] = (uint16_t)r, done = true, ran = true;
+                            }
+                        }
+                        const uint32_t nran = (uint32_t)__popcll(__ballot(ran));
+                        if (lane == 0 && nran) atomicAdd(&left[r % 3], nran);
+                        if (tid == 0) left[(r + 1) % 3] = 0;
+                        __syncthreads();
+                        total += left[r % 3];
+                    }
+                    // the forces back to F
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (j < nn && rt[j] == tid * 5 + j) {
+                            float* F = (float*)(a.F + cn.n[j]);
+                            F[0] = fs[3 * rt[j]], F[1] = fs[3 * rt[j] + 1], F[2] = fs[3 * rt[j] + 2];
+                        }
                 }
                 __syncthreads();
                 wb += nbw;
