@@ -52,7 +52,7 @@ class BodyOpts(ctypes.Structure):
                                                "contact_slop", "hull_depth", "sphere_ball", "contact_friction")] + \
                [(n, ctypes.c_int32) for n in ("allow_break", "allow_deform", "ductile", "self_collision",
                                                "collision_group", "faces_only")] + \
-               [("air_drag", ctypes.c_float), ("aero_cda", ctypes.c_float), ("rest_damp", ctypes.c_float),
+               [("face_skin", ctypes.c_float), ("air_drag", ctypes.c_float), ("aero_cda", ctypes.c_float), ("rest_damp", ctypes.c_float),
                 ("energy_guard", ctypes.c_int32)]
 
 
@@ -261,6 +261,7 @@ def default_body_opts():
     o.contact_push_max = 2.0
     o.contact_slop = 0.0
     o.hull_depth = 0.3
+    o.face_skin = 0.002
     o.contact_friction = 1.0
     o.allow_break = 1
     o.allow_deform = 1

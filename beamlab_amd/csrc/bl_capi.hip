@@ -1013,6 +1013,7 @@ void bl_default_body_opts(bl_body_opts* o) {
     o->contact_push_max = 2.0f;
     o->contact_slop = 0.0f;
     o->hull_depth = 0.3f;
+    o->face_skin = 0.002f;
     o->contact_friction = 1.0f;
     o->allow_break = 1;
     o->allow_deform = 1;

@@ -160,6 +160,7 @@ typedef struct bl_body_opts {
     int32_t self_collision;    /* default 0 */
     int32_t collision_group;   /* default 0 */
     int32_t faces_only;        /* default 0 */
+    float face_skin;           /* default 0.002: a faces-only hull triangle's reach (m) */
     /* features the GPU path does not step; non-zero values are rejected with BL_ERR_UNSUPPORTED */
     float air_drag;
     float aero_cda;

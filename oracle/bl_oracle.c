@@ -114,6 +114,7 @@ typedef struct {
     uint32_t ntri;
     bl_capsule* caps;
     uint32_t ncap;
+    float hull_reach; /* SoftBody::hull_reach (src/phys/softbody.h:751-758): hull_depth when any triangle is one-sided */
 } orc_body;
 
 struct orc_world {
@@ -123,7 +124,7 @@ struct orc_world {
     orc_body* bodies;
     int nbodies;
     bl_stats st;
-    int unsupported; /* a feature met that the restatement leaves out (fast bodies' pairs) */
+    int64_t fast_pairs; /* node-triangle and node-capsule candidates of fast bodies, summed over the rebuilds */
 };
 
 orc_world* orc_create(const bl_settings* s) {
@@ -172,6 +173,9 @@ int orc_add_body(orc_world* w, const bl_body_desc* d) {
     b->caps = (bl_capsule*)malloc(sizeof(bl_capsule) * (b->ncap ? b->ncap : 1));
     if (b->ntri) memcpy(b->tris, d->tris, sizeof(bl_triangle) * b->ntri);
     if (b->ncap) memcpy(b->caps, d->capsules, sizeof(bl_capsule) * b->ncap);
+    int hulls = 0;
+    for (uint32_t t = 0; t < b->ntri; t++) hulls += !b->tris[t].two_sided;
+    b->hull_reach = hulls > 0 ? b->opts.hull_depth : 0.0f;
     /* SoftBody::compute_aabb at World::add_body (src/phys/world.cpp:283-289, src/phys/softbody.cpp:271-275) */
     b->mn = V(1e30f, 1e30f, 1e30f), b->mx = V(-1e30f, -1e30f, -1e30f);
     for (uint32_t i = 0; i < b->n; i++) b->mn = vmin3(b->mn, ld3(b->nodes[i].p)), b->mx = vmax3(b->mx, ld3(b->nodes[i].p));
@@ -534,13 +538,20 @@ static void closest_seg_seg(v3 p1, v3 q1, v3 p2, v3 q2, float* s, float* t) {
     else if (*t > 1) { *t = 1; *s = clampf01((b - c) / a); }
 }
 
-/* near_triangle (src/phys/world.cpp:1135-1142), two-sided triangles */
-static int near_triangle(v3 p, v3 a, v3 b, v3 c, v3 tu, float r) {
+/* near_triangle (src/phys/world.cpp:1135-1142): a two-sided triangle within r of p; a hull (one-sided) triangle also
+ * takes p up to `depth` behind it, over its face */
+static int near_triangle(v3 p, v3 a, v3 b, v3 c, v3 tu, float r, int two_sided, float depth) {
     const float s = vdot(vsub(p, a), tu);
     v3 bary;
-    v3 q = closest_on_triangle(p, a, b, c, &bary);
-    v3 d = vsub(q, p);
-    return fabsf(s) <= r && vdot(d, d) <= r * r;
+    if (two_sided) {
+        if (!(fabsf(s) <= r)) return 0;
+        v3 d = vsub(closest_on_triangle(p, a, b, c, &bary), p);
+        return vdot(d, d) <= r * r;
+    }
+    if (s > r || s < -r - depth) return 0;
+    const v3 q = vsub(p, vmul(tu, s));
+    v3 d = vsub(closest_on_triangle(q, a, b, c, &bary), q);
+    return vdot(d, d) <= r * r;
 }
 
 static inline int cell_of(float v, float inv_cell) { return (int)floorf(v * inv_cell); }
@@ -577,7 +588,7 @@ static int build_islands(orc_world* w, float frame_time, orc_island** out) {
         parent[i] = i;
         aabb3 a = {B->mn, B->mx};
         float m = smin(B->max_speed, 200.0f) * frame_time * 2.0f + 0.3f;
-        boxes[i] = aabb_expand(a, m + 0.0f /* hull_reach: two-sided triangles only */);
+        boxes[i] = aabb_expand(a, m + B->hull_reach);
         hc[i] = has_contacters(B);
     }
     if (w->s.inter_body_collisions)
@@ -660,7 +671,12 @@ static int cmp_ct(const void* x, const void* y) {
     return a->tri < b->tri ? -1 : a->tri > b->tri ? 1 : 0;
 }
 
-/* World::rebuild_pairs (src/phys/world.cpp:1851-2308) without fast bodies, hull triangles and FEM mid points */
+static void fast_pairs(orc_world* w, orc_island* I, const bp_ctx* c, const float* extra, const float* bulk,
+                       float slow_speed);
+
+/* World::rebuild_pairs (src/phys/world.cpp:1851-2308) without FEM mid points: the slow bodies' node hash, the
+ * triangles' and capsules' queries of it (margins grown by a fast body's own travel, a hull triangle's depth behind
+ * it), the fast bodies' own pairs after them, then capsules against triangles */
 static void rebuild_pairs(orc_world* w, orc_island* I) {
     w->st.pair_rebuilds++;
     I->nnt = I->nnc = I->nct = 0;
@@ -683,15 +699,30 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
     for (int k = 0; k < nb; k++) {
         orc_body* B = &w->bodies[I->bodies[k]];
         if (B->max_speed <= 8.0f) slow_speed = smax(slow_speed, B->max_speed);
-        else w->unsupported = 1; /* a fast body: its own pair schedule (fast_pairs) is not restated */
     }
     const float speed = slow_speed + 0.5f;
     const double hz = (double)smax(1.0f, w->s.collision_hz);
     int interval = (int)ceil(1.0 / (hz * (double)w->s.dt) - 1e-6);
     if (interval < 1) interval = 1;
     I->interval = interval;
+    /* a fast body's bulk speed (its nodes' mass-weighted RMS speed, +25 %) and its extra margin (:1893-1915) */
+    float* bulk = calloc(nb, sizeof(float));
+    float* extra = calloc(nb, sizeof(float));
+    for (int k = 0; k < nb; k++) {
+        orc_body* B = &w->bodies[I->bodies[k]];
+        if (B->max_speed <= 8.0f) continue;
+        double mv2 = 0, m = 0;
+        for (uint32_t i = 0; i < B->n; i++) {
+            const v3 v = ld3(B->nodes[i].v);
+            mv2 += (double)B->nodes[i].mass * vdot(v, v);
+            m += B->nodes[i].mass;
+        }
+        bulk[k] = clampf_(m > 0 ? 1.25f * (float)sqrt(mv2 / m) : B->max_speed, 8.0f, 400.0f);
+    }
     const float margin = 2.0f * speed * (float)interval * w->s.dt + 0.02f;
     I->margin = margin;
+    for (int k = 0; k < nb; k++)
+        if (w->bodies[I->bodies[k]].max_speed > 8.0f) extra[k] = bulk[k] * (float)interval * w->s.dt;
     {
         double tri_size = 0;
         int samples = 0;
@@ -716,7 +747,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
     for (int k = 0; k < nb; k++) {
         orc_body* B = &w->bodies[I->bodies[k]];
         aabb3 a = {B->mn, B->mx};
-        c.boxes[k] = aabb_expand(a, reach + 0.0f + 0.0f);
+        c.boxes[k] = aabb_expand(a, reach + extra[k] + B->hull_reach);
         c.partners[k] = malloc(sizeof(int) * nb);
     }
     for (int a = 0; a < nb; a++)
@@ -732,7 +763,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
     size_t nent = 0, cent = 0;
     for (int bi = 0; bi < nb; bi++) {
         orc_body* B = &w->bodies[I->bodies[bi]];
-        if (c.npart[bi] == 0 && !B->opts.self_collision) continue;
+        if ((c.npart[bi] == 0 && !B->opts.self_collision) || extra[bi] > 0) continue; /* (fast bodies: fast_pairs) */
         for (uint32_t ni = 0; ni < B->n; ni++) {
             if (!(B->info[ni].flags & BL_NF_CONTACTER)) continue;
             v3 p = ld3(B->nodes[ni].p);
@@ -758,11 +789,11 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
         for (int bt = 0; bt < nb; bt++) {
             orc_body* O = &w->bodies[I->bodies[bt]];
             if (c.npart[bt] == 0 && !O->opts.self_collision) continue;
-            const float rq0 = rmax + margin + 0.0f;
+            const float rq0 = rmax + margin + extra[bt];
             for (uint32_t t = 0; t < O->ntri; t++) {
                 const bl_triangle* tr = &O->tris[t];
                 if (tr->torn) continue;
-                const float depth = 0.0f, rq = rq0 + depth;
+                const float depth = tr->two_sided ? 0.0f : O->opts.hull_depth, rq = rq0 + depth;
                 v3 a = ld3(O->nodes[tr->a].p), bb = ld3(O->nodes[tr->b].p), cc = ld3(O->nodes[tr->c].p);
                 v3 tmn = vmin3(a, vmin3(bb, cc)), tmx = vmax3(a, vmax3(bb, cc));
                 if (!near_partner(&c, bt, vsub(tmn, V(rq, rq, rq)), vadd(tmx, V(rq, rq, rq)))) continue;
@@ -790,10 +821,10 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
                                 if (self && (!Bn->opts.self_collision || tr->a == pt->node || tr->b == pt->node || tr->c == pt->node)) continue;
                                 if (!self && same_group(w, I, bn, bt)) continue;
                                 v3 p = ld3(Bn->nodes[pt->node].p);
-                                float r = Bn->opts.collision_radius + margin + 0.0f;
+                                float r = Bn->opts.collision_radius + margin + extra[bt];
                                 const float rb = r + depth;
                                 if (p.x < tmn.x - rb || p.y < tmn.y - rb || p.z < tmn.z - rb || p.x > tmx.x + rb || p.y > tmx.y + rb || p.z > tmx.z + rb) continue;
-                                if (!near_triangle(p, a, bb, cc, tu, r)) continue;
+                                if (!near_triangle(p, a, bb, cc, tu, r, tr->two_sided, depth)) continue;
                                 orc_nt q = {(uint16_t)bn, (uint16_t)bt, pt->node, t};
                                 VEC_PUSH(I->nt, I->nnt, I->cnt, q);
                             }
@@ -802,7 +833,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
             for (uint32_t ci = 0; ci < O->ncap; ci++) {
                 const bl_capsule* cp = &O->caps[ci];
                 v3 a = ld3(O->nodes[cp->a].p), bb = ld3(O->nodes[cp->b].p);
-                float rq2 = cp->radius + rmax + margin + 0.0f;
+                float rq2 = cp->radius + rmax + margin + extra[bt];
                 v3 mn = vsub(vmin3(a, bb), V(rq2, rq2, rq2)), mx = vadd(vmax3(a, bb), V(rq2, rq2, rq2));
                 if (!near_partner(&c, bt, mn, mx)) continue;
                 int x0 = cell_of(mn.x, inv_cell), x1 = cell_of(mx.x, inv_cell), y0 = cell_of(mn.y, inv_cell), y1 = cell_of(mx.y, inv_cell);
@@ -820,7 +851,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
                                 if (same_group(w, I, bn, bt)) continue;
                                 orc_body* Bn = &w->bodies[I->bodies[bn]];
                                 v3 p = ld3(Bn->nodes[pt->node].p);
-                                float r = cp->radius + Bn->opts.collision_radius + margin + 0.0f;
+                                float r = cp->radius + Bn->opts.collision_radius + margin + extra[bt];
                                 float sg = closest_on_segment(p, a, bb);
                                 v3 d = vsub(vadd(a, vmul(vsub(bb, a), sg)), p);
                                 if (vdot(d, d) > r * r) continue;
@@ -832,6 +863,16 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
         }
         free(bstart), free(bitems), free(run_end);
     }
+    /* 2b) the fast bodies' nodes against their partners' triangles and capsules */
+    {
+        int any_fast = 0;
+        for (int k = 0; k < nb; k++) any_fast |= extra[k] > 0;
+        if (any_fast) {
+            const size_t n0 = I->nnt + I->nnc;
+            fast_pairs(w, I, &c, extra, bulk, speed);
+            w->fast_pairs += (int64_t)(I->nnt + I->nnc - n0);
+        }
+    }
     /* 3) capsules against triangles: the triangles hashed into every cell of their boxes */
     if (any_caps && any_tris) {
         typedef struct { uint16_t body; uint32_t idx; } prim;
@@ -841,7 +882,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
         for (int bi = 0; bi < nb; bi++) {
             orc_body* B = &w->bodies[I->bodies[bi]];
             if (c.npart[bi] == 0 && !B->opts.self_collision) continue;
-            const float r = B->opts.collision_radius + margin + 0.0f;
+            const float r = B->opts.collision_radius + margin + extra[bi];
             for (uint32_t t = 0; t < B->ntri; t++) {
                 const bl_triangle* tr = &B->tris[t];
                 if (tr->torn) continue;
@@ -869,7 +910,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
             for (uint32_t ci = 0; ci < B->ncap; ci++) {
                 const bl_capsule* cp = &B->caps[ci];
                 v3 a = ld3(B->nodes[cp->a].p), bb = ld3(B->nodes[cp->b].p);
-                float rr = cp->radius + B->opts.collision_radius + margin + 0.0f;
+                float rr = cp->radius + B->opts.collision_radius + margin + extra[bc];
                 v3 mn = vsub(vmin3(a, bb), V(rr, rr, rr)), mx = vadd(vmax3(a, bb), V(rr, rr, rr));
                 int x0 = cell_of(mn.x, inv_cell), x1 = cell_of(mx.x, inv_cell), y0 = cell_of(mn.y, inv_cell), y1 = cell_of(mx.y, inv_cell);
                 int z0 = cell_of(mn.z, inv_cell), z1 = cell_of(mx.z, inv_cell);
@@ -907,7 +948,184 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
     free(pts), free(ent);
     for (int k = 0; k < nb; k++) free(c.partners[k]);
     free(c.partners), free(c.npart), free(c.boxes);
+    free(bulk), free(extra);
     w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
+}
+
+/* World::fast_pairs (src/phys/world.cpp:2310-2522): each fast body's contacter nodes in a grid of its own (cells about
+ * the reach, at most 48 an axis; a node in every cell within its own travel until the next rebuild), each partner
+ * triangle (then capsule) scanning the cells of its reach box: the nodes found there that are within reach, in
+ * ascending order. (The fast pairs are refreshed with the rest, at the slow schedule's interval.) */
+typedef struct {
+    float inv_cell;
+    v3 origin;
+    int dim[3];
+    uint32_t *start, *items;
+} fast_grid;
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
+static inline float v3c(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+static uint32_t fg_cell(const fast_grid* G, v3 p) {
+    const int x = clampi((int)((p.x - G->origin.x) * G->inv_cell), 0, G->dim[0] - 1);
+    const int y = clampi((int)((p.y - G->origin.y) * G->inv_cell), 0, G->dim[1] - 1);
+    const int z = clampi((int)((p.z - G->origin.z) * G->inv_cell), 0, G->dim[2] - 1);
+    return (uint32_t)((z * G->dim[1] + y) * G->dim[0] + x);
+}
+
+/* the cells of node ni (the box of cells between its position -+ its travel), visited in z, y, x order */
+static uint32_t fg_cells(const fast_grid* G, const orc_body* F, uint32_t ni, float travel, uint32_t* out) {
+    const v3 p = ld3(F->nodes[ni].p);
+    const float m = vlen(ld3(F->nodes[ni].v)) * travel;
+    const uint32_t lo = fg_cell(G, vsub(p, V(m, m, m))), hi = fg_cell(G, vadd(p, V(m, m, m)));
+    if (lo == hi) {
+        out[0] = lo;
+        return 1;
+    }
+    const int x0 = (int)(lo % G->dim[0]), y0 = (int)(lo / G->dim[0] % G->dim[1]), z0 = (int)(lo / (G->dim[0] * G->dim[1]));
+    const int x1 = (int)(hi % G->dim[0]), y1 = (int)(hi / G->dim[0] % G->dim[1]), z1 = (int)(hi / (G->dim[0] * G->dim[1]));
+    uint32_t n = 0;
+    for (int z = z0; z <= z1; z++)
+        for (int y = y0; y <= y1; y++)
+            for (int x = x0; x <= x1; x++) out[n++] = (uint32_t)((z * G->dim[1] + y) * G->dim[0] + x);
+    return n;
+}
+
+static int cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* the nodes listed in the cells of [mn, mx], ascending, each once */
+static size_t fg_scan(const fast_grid* G, v3 mn, v3 mx, uint32_t** buf, size_t* cap) {
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        const int a = (int)floorf((v3c(mn, k) - v3c(G->origin, k)) * G->inv_cell);
+        const int b = (int)((v3c(mx, k) - v3c(G->origin, k)) * G->inv_cell);
+        lo[k] = a < 0 ? 0 : a;
+        hi[k] = b < G->dim[k] - 1 ? b : G->dim[k] - 1;
+        if (lo[k] > hi[k]) return 0;
+    }
+    size_t n = 0;
+    for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++) {
+            const uint32_t row = (uint32_t)((z * G->dim[1] + y) * G->dim[0]);
+            const uint32_t k0 = G->start[row + lo[0]], k1 = G->start[row + hi[0] + 1];
+            for (uint32_t k = k0; k < k1; k++) VEC_PUSH(*buf, n, *cap, G->items[k]);
+        }
+    qsort(*buf, n, sizeof(uint32_t), cmp_u32);
+    size_t u = 0;
+    for (size_t i = 0; i < n; i++)
+        if (i == 0 || (*buf)[i] != (*buf)[i - 1]) (*buf)[u++] = (*buf)[i];
+    return u;
+}
+
+static void fast_pairs(orc_world* w, orc_island* I, const bp_ctx* c, const float* extra, const float* bulk,
+                       float slow_speed) {
+    const int nb = I->nb;
+    const float dt = w->s.dt;
+    const int fi = I->interval; /* fast_interval: the slow schedule's */
+#define FAST_EXTRA(b) (extra[(b)] <= 0 ? 0.0f : bulk[(b)] * (float)fi * dt)
+    const float travel = 1.25f * (float)fi * dt;
+    const float slow_reach = 2.0f * slow_speed * (float)fi * dt + 0.01f;
+    uint32_t* found = NULL;
+    size_t fcap = 0;
+    for (int bf = 0; bf < nb; bf++) {
+        if (FAST_EXTRA(bf) <= 0) continue;
+        orc_body* F = &w->bodies[I->bodies[bf]];
+        float rmax = 0;
+        for (int k = 0; k < c->npart[bf]; k++)
+            rmax = smax(rmax, F->opts.collision_radius + slow_reach + FAST_EXTRA(c->partners[bf][k]));
+        fast_grid G;
+        const v3 ext = vadd(vsub(F->mx, F->mn), V(2e-3f, 2e-3f, 2e-3f));
+        float cellsz = clampf_(2.0f * rmax, 0.15f, 2.0f);
+        cellsz = smax(cellsz, smax(ext.x, smax(ext.y, ext.z)) / 48.0f);
+        G.inv_cell = 1.0f / cellsz;
+        G.origin = vsub(F->mn, V(1e-3f, 1e-3f, 1e-3f));
+        for (int k = 0; k < 3; k++) {
+            int d = (int)(v3c(ext, k) * G.inv_cell) + 1;
+            G.dim[k] = d > 48 ? 48 : d;
+            if (G.dim[k] < 1) G.dim[k] = 1;
+        }
+        const uint32_t ncell = (uint32_t)(G.dim[0] * G.dim[1] * G.dim[2]);
+        G.start = calloc(ncell + 1, sizeof(uint32_t));
+        uint32_t count = 0;
+        uint32_t* big = malloc(sizeof(uint32_t) * ncell); /* (a node's cells: at most the whole grid) */
+        for (uint32_t ni = 0; ni < F->n; ni++) {
+            if (!(F->info[ni].flags & BL_NF_CONTACTER)) continue;
+            const uint32_t nc = fg_cells(&G, F, ni, travel, big);
+            for (uint32_t k = 0; k < nc; k++) G.start[big[k] + 1]++, count++;
+        }
+        for (uint32_t i = 0; i < ncell; i++) G.start[i + 1] += G.start[i];
+        G.items = malloc(sizeof(uint32_t) * (count ? count : 1));
+        uint32_t* fill = malloc(sizeof(uint32_t) * (ncell ? ncell : 1));
+        memcpy(fill, G.start, sizeof(uint32_t) * ncell);
+        for (uint32_t ni = 0; ni < F->n; ni++) {
+            if (!(F->info[ni].flags & BL_NF_CONTACTER)) continue;
+            const uint32_t nc = fg_cells(&G, F, ni, travel, big);
+            for (uint32_t k = 0; k < nc; k++) G.items[fill[big[k]]++] = ni;
+        }
+        free(fill), free(big);
+        for (int pk = 0; pk < c->npart[bf]; pk++) {
+            const int bo = c->partners[bf][pk];
+            orc_body* O = &w->bodies[I->bodies[bo]];
+            const int o_fast = extra[bo] > 0;
+            const float r0 = F->opts.collision_radius + (o_fast ? FAST_EXTRA(bo) + 0.01f : slow_reach);
+            const float r = F->opts.collision_radius + slow_reach + travel * (F->max_speed + (o_fast ? O->max_speed : 0.0f)) + FAST_EXTRA(bo);
+            const aabb3 fb = aabb_expand((aabb3){F->mn, F->mx}, r);
+            const float r00 = F->opts.collision_radius + (o_fast ? 0.01f : slow_reach);
+            for (uint32_t t = 0; t < O->ntri; t++) {
+                const bl_triangle* tr = &O->tris[t];
+                if (tr->torn) continue;
+                const v3 a = ld3(O->nodes[tr->a].p), bb = ld3(O->nodes[tr->b].p), cc = ld3(O->nodes[tr->c].p);
+                const v3 tmn = vmin3(a, vmin3(bb, cc)), tmx = vmax3(a, vmax3(bb, cc));
+                if (tmx.x < fb.mn.x || tmx.y < fb.mn.y || tmx.z < fb.mn.z || tmn.x > fb.mx.x || tmn.y > fb.mx.y || tmn.z > fb.mx.z) continue;
+                float rt = r00;
+                if (o_fast) {
+                    const v3 va = ld3(O->nodes[tr->a].v), vb = ld3(O->nodes[tr->b].v), vc = ld3(O->nodes[tr->c].v);
+                    rt = r00 + travel * sqrtf(smax(vdot(va, va), smax(vdot(vb, vb), vdot(vc, vc))));
+                }
+                const float depth = tr->two_sided ? 0.0f : O->opts.hull_depth;
+                const v3 tn = vcross(vsub(bb, a), vsub(cc, a));
+                const float tn2 = vdot(tn, tn);
+                const v3 tu = tn2 > 1e-20f ? vmul(tn, 1.0f / sqrtf(tn2)) : V(0, 0, 0);
+                const float e = rt + depth;
+                const size_t n = fg_scan(&G, vsub(tmn, V(e, e, e)), vadd(tmx, V(e, e, e)), &found, &fcap);
+                for (size_t i = 0; i < n; i++) {
+                    const uint32_t ni = found[i];
+                    const v3 p = ld3(F->nodes[ni].p), v = ld3(F->nodes[ni].v);
+                    const float rn = rt + vlen(v) * travel, rb = rn + depth;
+                    if (p.x < tmn.x - rb || p.y < tmn.y - rb || p.z < tmn.z - rb || p.x > tmx.x + rb || p.y > tmx.y + rb || p.z > tmx.z + rb) continue;
+                    if (!near_triangle(p, a, bb, cc, tu, rn, tr->two_sided, depth)) continue;
+                    orc_nt q = {(uint16_t)bf, (uint16_t)bo, ni, t};
+                    VEC_PUSH(I->nt, I->nnt, I->cnt, q);
+                }
+            }
+            for (uint32_t ci = 0; ci < O->ncap; ci++) {
+                const bl_capsule* cp = &O->caps[ci];
+                const v3 a = ld3(O->nodes[cp->a].p), bb = ld3(O->nodes[cp->b].p);
+                const float rc = cp->radius + r;
+                v3 cmn = vsub(vmin3(a, bb), V(rc, rc, rc)), cmx = vadd(vmax3(a, bb), V(rc, rc, rc));
+                if (cmx.x < F->mn.x || cmx.y < F->mn.y || cmx.z < F->mn.z || cmn.x > F->mx.x || cmn.y > F->mx.y || cmn.z > F->mx.z) continue;
+                const float rc0 = cp->radius + r0;
+                cmn = vsub(vmin3(a, bb), V(rc0, rc0, rc0)), cmx = vadd(vmax3(a, bb), V(rc0, rc0, rc0));
+                const size_t n = fg_scan(&G, cmn, cmx, &found, &fcap);
+                for (size_t i = 0; i < n; i++) {
+                    const uint32_t ni = found[i];
+                    const v3 p = ld3(F->nodes[ni].p), v = ld3(F->nodes[ni].v);
+                    const float rn = rc0 + vlen(v) * travel;
+                    const float sg = closest_on_segment(p, a, bb);
+                    const v3 d = vsub(vadd(a, vmul(vsub(bb, a), sg)), p);
+                    if (!(vdot(d, d) <= rn * rn)) continue;
+                    orc_nc q = {(uint16_t)bf, (uint16_t)bo, ni, ci};
+                    VEC_PUSH(I->nc, I->nnc, I->cnc, q);
+                }
+            }
+        }
+        free(G.start), free(G.items);
+    }
+    free(found);
+#undef FAST_EXTRA
 }
 
 /* World::near_pairs (src/phys/world.cpp:2525-2596): the node-triangle candidates within reach of their relative
@@ -927,7 +1145,7 @@ static void near_pairs(orc_world* w, orc_island* I) {
         const bl_node *a = &Bt->nodes[t->a], *b = &Bt->nodes[t->b], *c3 = &Bt->nodes[t->c];
         v3 bary;
         const v3 q = closest_on_triangle(ld3(x->p), ld3(a->p), ld3(b->p), ld3(c3->p), &bary);
-        const float r0 = Bn->opts.collision_radius + 0.0f;
+        const float r0 = Bn->opts.collision_radius + (t->two_sided ? 0.0f : Bt->opts.hull_depth);
         const v3 da = vsub(ld3(x->v), ld3(a->v)), db = vsub(ld3(x->v), ld3(b->v)), dc = vsub(ld3(x->v), ld3(c3->v));
         const float vrel = sqrtf(smax(vdot(da, da), smax(vdot(db, db), vdot(dc, dc))));
         const float reach = r0 + 1.25f * vrel * T + 0.02f;
@@ -938,9 +1156,12 @@ static void near_pairs(orc_world* w, orc_island* I) {
         if (nl > 1e-12f) {
             const v3 n0 = vdivs(nn, nl);
             const float s0 = vdot(n0, vsub(ld3(x->p), ld3(a->p)));
-            const float sg = s0 >= 0 ? 1.0f : -1.0f;
-            const float close = smax(0.0f, smax(-sg * vdot(n0, da), smax(-sg * vdot(n0, db), -sg * vdot(n0, dc))));
-            if (fabsf(s0) - 1.25f * close * T - 0.02f >= Bn->opts.collision_radius) continue;
+            /* (a hull's triangle only from in front) */
+            if (t->two_sided || s0 > Bn->opts.collision_radius) {
+                const float sg = s0 >= 0 ? 1.0f : -1.0f;
+                const float close = smax(0.0f, smax(-sg * vdot(n0, da), smax(-sg * vdot(n0, db), -sg * vdot(n0, dc))));
+                if (fabsf(s0) - 1.25f * close * T - 0.02f >= Bn->opts.collision_radius) continue;
+            }
         }
         I->nt[wr++] = pr;
     }
@@ -948,7 +1169,7 @@ static void near_pairs(orc_world* w, orc_island* I) {
     w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
 }
 
-/* World::pair_detect (src/phys/world.cpp:2807-2962), two-sided triangles */
+/* World::pair_detect (src/phys/world.cpp:2807-2962) */
 static void pair_detect(orc_world* w, orc_island* I) {
     I->nhit = 0;
     for (size_t i = 0; i < I->nnt; i++) {
@@ -959,11 +1180,27 @@ static void pair_detect(orc_world* w, orc_island* I) {
         const bl_triangle* t = &Bt->tris[pr.tri];
         if (t->torn) continue;
         const bl_node *a = &Bt->nodes[t->a], *b = &Bt->nodes[t->b], *cc = &Bt->nodes[t->c];
-        const float r = Bn->opts.collision_radius;
+        const float r = Bt->opts.faces_only && !t->two_sided ? Bt->opts.face_skin : Bn->opts.collision_radius;
         v3 np = ld3(n->p), ap = ld3(a->p), bp = ld3(b->p), cp3 = ld3(cc->p);
-        const v3 lo = vsub(vmin3(ap, vmin3(bp, cp3)), V(r, r, r)), hi = vadd(vmax3(ap, vmax3(bp, cp3)), V(r, r, r));
-        if (np.x < lo.x || np.y < lo.y || np.z < lo.z || np.x > hi.x || np.y > hi.y || np.z > hi.z) continue;
         v3 bary;
+        if (!t->two_sided) {
+            /* a hull triangle: over its face the node is pushed out along the face's normal, from as deep as the
+             * hull's depth behind it; off the face, outside, the ordinary contact with its edges */
+            const v3 fn = vnormalize(vcross(vsub(bp, ap), vsub(cp3, ap)));
+            const float sf = vdot(vsub(np, ap), fn);
+            if (sf >= r || sf < -Bt->opts.hull_depth) continue;
+            const v3 q = vsub(np, vmul(fn, sf));
+            const v3 cq = vsub(closest_on_triangle(q, ap, bp, cp3, &bary), q);
+            if (vdot(cq, cq) <= 1e-6f) {
+                orc_hit h = {(uint32_t)i, 0.0f, bary, fn, r - sf};
+                VEC_PUSH(I->hits, I->nhit, I->chit, h);
+                continue;
+            }
+            if (sf < 0 || Bt->opts.faces_only) continue;
+        } else {
+            const v3 lo = vsub(vmin3(ap, vmin3(bp, cp3)), V(r, r, r)), hi = vadd(vmax3(ap, vmax3(bp, cp3)), V(r, r, r));
+            if (np.x < lo.x || np.y < lo.y || np.z < lo.z || np.x > hi.x || np.y > hi.y || np.z > hi.z) continue;
+        }
         v3 cp = closest_on_triangle(np, ap, bp, cp3, &bary);
         v3 d = vsub(np, cp);
         float dist2 = vdot(d, d);
@@ -1087,6 +1324,28 @@ static int contact_force(orc_world* w, bl_node** A, v3** FA, const float* wa, in
     return 1;
 }
 
+/* hull_push_out<3> (src/phys/world.cpp:93-120), both sides movable, no force snapshots: the normal force that makes
+ * the node's velocity relative to the face after the step the push-out speed (a fifth of the depth a step, at most
+ * 2 m/s), counting the forces on both sides so far, only when it would come out slower */
+static int hull_push_out(bl_node* A, v3* FA, bl_node** B, v3** FB, const float* wb, v3 n, float pen, float dt) {
+    float W = A->inv_mass;
+    for (int i = 0; i < 3; i++) W += wb[i] * wb[i] * B[i]->inv_mass;
+    if (W <= 1e-12f) return 0;
+    v3 vb = V(0, 0, 0), acc = V(0, 0, 0);
+    acc = vadd(acc, vmul(*FA, A->inv_mass));
+    for (int i = 0; i < 3; i++) {
+        vb = vadd(vb, vmul(ld3(B[i]->v), wb[i]));
+        acc = vsub(acc, vmul(*FB[i], wb[i] * B[i]->inv_mass));
+    }
+    const float m = 1.0f / W, vr = vdot(vsub(ld3(A->v), vb), n), target = smin(0.2f * pen / dt, 2.0f);
+    const float fn = (target - vr) * m / dt - vdot(acc, n) * m;
+    if (fn <= 0) return 1;
+    const v3 f = vmul(n, fn);
+    *FA = vadd(*FA, f);
+    for (int i = 0; i < 3; i++) *FB[i] = vsub(*FB[i], vmul(f, wb[i]));
+    return 1;
+}
+
 /* World::collide_pairs (src/phys/world.cpp:2964-3079): the narrow phase, then the response in pair order */
 static void collide_pairs(orc_world* w, orc_island* I) {
     w->st.narrow_tests += (int64_t)(I->nnt + I->nnc + I->nct);
@@ -1107,6 +1366,10 @@ static void collide_pairs(orc_world* w, orc_island* I) {
             float wb[3] = {h->bary.x, h->bary.y, h->bary.z};
             float fric = (Bn->info[pr.node].flags & BL_NF_TYRE) ? Bn->info[pr.node].friction
                                                                  : 0.8f * smin(Bn->opts.contact_friction, Bt->opts.contact_friction);
+            if (!t->two_sided && h->pen > Bn->opts.collision_radius) { /* (behind a hull's face) */
+                if (hull_push_out(A[0], FA[0], B, FB, wb, h->nrm, h->pen, dt)) w->st.contacts++;
+                continue;
+            }
             if (contact_force(w, A, FA, wa, 1, B, FB, wb, 3, h->nrm, h->pen, dt, fric)) w->st.contacts++;
         } else if (h->pair < I->nnt + I->nnc) {
             const orc_nc pr = I->nc[h->pair - I->nnt];
@@ -1210,6 +1473,8 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
     if (beams && B->nb) memcpy(B->beams, beams, sizeof(bl_beam) * B->nb);
     return 0;
 }
+
+int64_t orc_fast_pairs(const orc_world* w) { return w->fast_pairs; }
 
 int orc_stats(orc_world* w, bl_stats* out) {
     *out = w->st;

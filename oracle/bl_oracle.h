@@ -28,6 +28,8 @@ int orc_step(orc_world* w, int substeps);
 int orc_read_body(orc_world* w, uint32_t id, bl_node* nodes, bl_beam* beams, int64_t* broken);
 int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_beam* beams);
 int orc_stats(orc_world* w, bl_stats* out);
+/* the fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds */
+int64_t orc_fast_pairs(const orc_world* w);
 
 int orc_finalize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb);
 int orc_stabilize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb, float dt, float kb, float db,

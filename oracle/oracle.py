@@ -58,6 +58,7 @@ def oracle_lib():
             "read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "stats": (ctypes.c_int, [_P, _P]),
+            "fast_pairs": (ctypes.c_int64, [_P]),
             "finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
             "stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, ctypes.POINTER(ctypes.c_int)]),
@@ -186,6 +187,10 @@ class OracleWorld(_CpuWorld):
         s = abi.Stats()
         self._lib.orc_stats(self._h, ctypes.byref(s))
         return s.as_dict()
+
+    def fast_pairs(self):
+        """The fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds."""
+        return int(self._lib.orc_fast_pairs(self._h))
 
 
 class RefWorld(_CpuWorld):

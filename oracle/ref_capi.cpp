@@ -69,6 +69,7 @@ void apply_opts(SoftBody& b, const bl_body_opts& o) {
     b.self_collision = o.self_collision != 0;
     b.collision_group = o.collision_group;
     b.faces_only = o.faces_only != 0;
+    b.face_skin = o.face_skin;
     b.air_drag = o.air_drag;
     b.aero_cda = o.aero_cda;
     b.rest_damp = o.rest_damp;

@@ -27,7 +27,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import _fem_cases as fc  # noqa: E402
 from _cases import CPU_CASES, run_world  # noqa: E402
-from _contact_cases import CONTACT_CASES  # noqa: E402
+from _contact_cases import CONTACT_CASES, golden_checkpoints  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 
@@ -46,7 +46,7 @@ def main():
     digests = {}
     for name in sorted(CPU_CASES) + sorted(CONTACT_CASES):
         settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
-        st = run_world(oracle.RefWorld(settings, boxes), bodies, n, sorted({1, 7, n // 3, n}))
+        st = run_world(oracle.RefWorld(settings, boxes), bodies, n, golden_checkpoints(name, n))
         h = hashlib.sha256()
         for nodes, beams in st[-1][:-1]:
             h.update(nodes.tobytes())

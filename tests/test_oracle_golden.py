@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from _cases import CPU_CASES, run_world
-from _contact_cases import CONTACT_CASES
+from _contact_cases import CONTACT_CASES, golden_checkpoints
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -29,7 +29,7 @@ def test_c1_states(oracle_mod):
 def test_case_digests(name, oracle_mod):
     g = np.load(os.path.join(GOLD, "cases_ref.npz"))
     settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
-    st = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, sorted({1, 7, n // 3, n}))
+    st = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, golden_checkpoints(name, n))
     h = hashlib.sha256()
     for nodes, beams in st[-1][:-1]:
         h.update(nodes.tobytes())

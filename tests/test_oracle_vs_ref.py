@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from _cases import CPU_CASES, assert_same, run_world
-from _contact_cases import CONTACT_CASES
+from _contact_cases import CONTACT_CASES, checkpoints
 
 
 @pytest.mark.parametrize("name", sorted(CPU_CASES))
@@ -24,13 +24,17 @@ def test_oracle_matches_reference(name, ref_mod):
 @pytest.mark.parametrize("name", sorted(CONTACT_CASES))
 def test_oracle_contacts_match_reference(name, ref_mod):
     settings, boxes, bodies, n = CONTACT_CASES[name]()
-    cps = sorted({1, 17, 40, n // 2, n})
+    cps = checkpoints(name, n)
     ow = ref_mod.OracleWorld(settings, boxes)
     so = run_world(ow, bodies, n, cps)
     sr = run_world(ref_mod.RefWorld(settings, boxes, threads=1), bodies, n, cps)
     assert_same(so, sr, f"oracle vs reference [{name}]")
     st = ow.stats()
-    assert st["contacts"] > 100 and st["pair_rebuilds"] > 0, st
+    assert st["pair_rebuilds"] > 0, st
+    # (behind a two-sided triangle the ball is out of the thin surface's reach: no contact at all)
+    assert (st["contacts"] == 0) if name == "hull_through" else (st["contacts"] > 100), st
+    if name == "fast_bodies":
+        assert ow.fast_pairs() > 100
 
 
 def test_reference_threads_bitwise(ref_mod):
