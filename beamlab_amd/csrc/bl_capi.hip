@@ -43,6 +43,14 @@ struct HostBody {
     }
 };
 
+// SoftBody::hull_reach (src/phys/softbody.h:751-758): how far past its bounds the body reaches for other bodies'
+// nodes - its hull depth, when it has a hull (one-sided) triangle
+inline float hull_reach(const HostBody& hb) {
+    for (const bl_triangle& t : hb.tris)
+        if (!t.two_sided) return hb.opts.hull_depth;
+    return 0.0f;
+}
+
 template <class T>
 struct DevArr {
     T* p = nullptr;
@@ -121,6 +129,7 @@ struct bl_ctx {
     DevArr<uint4> tri_abc, cap_data;
     DevArr<float> tri_rest2;
     DevArr<uint32_t> tri_torn;
+    DevArr<uint8_t> tri_hull;
     DevArr<unsigned int> cbox;
     DevArr<uint4> wstat; // per workgroup of the node-parallel integration: its counts of the frame
     ContactEngine contacts;
@@ -166,7 +175,7 @@ struct bl_ctx {
         chunk_body.release(), cstat.release();
         halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
         wg_chunk.release();
-        F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), cbox.release();
+        F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), tri_hull.release(), cbox.release();
         wstat.release();
         contacts.release();
         frame_pending = false;
@@ -260,6 +269,7 @@ int build(bl_ctx* c) {
     std::vector<uint4> tri_abc(NT), cap_data(NC);
     std::vector<float> tri_rest2(NT);
     std::vector<uint32_t> tri_torn(NT);
+    std::vector<uint8_t> tri_hull(NT);
     c->hv = HostView{};
     c->hv.dt = c->s.dt, c->hv.inter_body = c->s.inter_body_collisions;
     c->contact_geometry = false;
@@ -270,6 +280,7 @@ int build(bl_ctx* c) {
             tri_abc[hb.tri0 + t] = make_uint4(hb.node0 + tr.a, hb.node0 + tr.b, hb.node0 + tr.c, (uint32_t)k);
             tri_rest2[hb.tri0 + t] = tr.rest_edge2;
             tri_torn[hb.tri0 + t] = tr.torn;
+            tri_hull[hb.tri0 + t] = tr.two_sided ? 0 : 1;
         }
         for (size_t q = 0; q < hb.caps.size(); q++) {
             const bl_capsule& cp = hb.caps[q];
@@ -280,6 +291,7 @@ int build(bl_ctx* c) {
         v.ntri = (uint32_t)hb.tris.size(), v.ncap = (uint32_t)hb.caps.size(), v.tri0 = hb.tri0;
         v.nnode = (uint32_t)hb.nodes.size();
         v.collision_radius = hb.opts.collision_radius;
+        v.hull_reach = hull_reach(hb);
         for (const bl_node_info& inf : hb.info) v.contacters |= (inf.flags & BL_NF_CONTACTER) ? 1u : 0u;
         c->hv.bodies.push_back(v);
         c->contact_geometry |= v.ntri > 0 || v.ncap > 0 || v.self;
@@ -294,7 +306,8 @@ int build(bl_ctx* c) {
     for (size_t k = 0; k < c->bodies.size(); k++) {
         const HostBody& hb = c->bodies[k];
         bp[k] = BodyParam{hb.opts.bounce, hb.opts.contact_push_max, hb.opts.contact_slop, hb.opts.collision_radius,
-                          hb.opts.contact_friction, hb.opts.collision_group, hb.opts.self_collision, 0};
+                          hb.opts.contact_friction, hb.opts.hull_depth, hb.opts.face_skin, hull_reach(hb),
+                          hb.opts.collision_group, hb.opts.self_collision, hb.opts.faces_only, 0};
         const float fric_body = hb.opts.ground_friction * 1.0f; // (resting is off: src/phys/world.cpp:792)
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const bl_node& n = hb.nodes[i];
@@ -522,7 +535,7 @@ int build(bl_ctx* c) {
         (rc = c->cstat.alloc(std::max<size_t>(1, nchunks))) || (rc = c->halo_off.upload(halo_off)) ||
         (rc = c->halo.upload(halo)) || (rc = c->nbr_off.upload(nbr_off)) || (rc = c->nbr.upload(nbr)) ||
         (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))) ||
-        (rc = c->tri_abc.upload(tri_abc)) || (rc = c->tri_rest2.upload(tri_rest2)) || (rc = c->tri_torn.upload(tri_torn)) ||
+        (rc = c->tri_abc.upload(tri_abc)) || (rc = c->tri_rest2.upload(tri_rest2)) || (rc = c->tri_torn.upload(tri_torn)) || (rc = c->tri_hull.upload(tri_hull)) ||
         (rc = c->cap_data.upload(cap_data)) || (rc = c->wg_chunk.upload(wg_chunk)) ||
         (rc = c->F.alloc(c->contact_geometry ? N : 0)) || (rc = c->wstat.alloc((N + 255) / 256 + 1)) ||
         (rc = c->cbox.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
@@ -545,7 +558,7 @@ int build(bl_ctx* c) {
     a.counters = c->counters.p, a.broken = c->broken.p, a.body_frame = c->body_frame.p;
     a.cstat = c->cstat.p, a.chunk_body = c->chunk_body.p;
     a.halo_off = c->halo_off.p, a.halo = c->halo.p, a.nbr_off = c->nbr_off.p, a.nbr = c->nbr.p, a.iloc = c->iloc.p;
-    a.F = c->F.p, a.tri_abc = c->tri_abc.p, a.tri_rest2 = c->tri_rest2.p, a.tri_torn = c->tri_torn.p;
+    a.F = c->F.p, a.tri_abc = c->tri_abc.p, a.tri_rest2 = c->tri_rest2.p, a.tri_torn = c->tri_torn.p, a.tri_hull = c->tri_hull.p;
     a.cap_data = c->cap_data.p, a.ntris = NT, a.ncaps = NC;
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
@@ -1169,10 +1182,6 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     for (uint32_t t = 0; t < d->n_tris; t++) {
         const bl_triangle& tr = d->tris[t];
         if (tr.a >= d->n_nodes || tr.b >= d->n_nodes || tr.c >= d->n_nodes) return BL_ERR_ARG;
-        if (!tr.two_sided) {
-            g_last_error = "hull (one-sided) triangles are not stepped on the GPU";
-            return BL_ERR_UNSUPPORTED;
-        }
     }
     for (uint32_t q = 0; q < d->n_capsules; q++) {
         const bl_capsule& cp = d->capsules[q];

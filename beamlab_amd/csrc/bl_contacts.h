@@ -21,6 +21,7 @@ struct Hit {
 // A pair island's search parameters of the current rebuild (rebuild_pairs, src/phys/world.cpp:1879-1935).
 struct IslandPar {
     float margin, inv_cell, rmax, reach;
+    float speed; // the slow bodies' top speed + 0.5 m/s (the margins' speed)
 };
 
 // What a rebuild leaves for the host: list sizes (needed sizes when a list overflowed its buffer) and flags.
@@ -56,6 +57,7 @@ struct CArgs {
     float dt;
     IslandPar* ipar;
     float* bbox;              // per body: the partner box (bounds +- the island's reach)
+    float* extra;             // per body: a fast body's extra margin (its bulk speed x the interval), else 0
     uint32_t* bquery;         // per body: has a partner or collides with itself
     const uint32_t* body_box; // per body: bounds keys (BF_MN, BF_MX words of a frame record)
     const uint32_t* body_tri0;
@@ -117,6 +119,7 @@ struct BodyView {
     int32_t group;
     uint32_t self, contacters, ntri, ncap, tri0, nnode;
     float collision_radius;
+    float hull_reach; // SoftBody::hull_reach: the hull depth when it has a hull triangle
 };
 
 struct HostView {
@@ -209,7 +212,7 @@ private:
     Buf<uint32_t> pcnt, poff, plist;
     uint32_t nmem = 0, cap_part = 0;
     Buf<uint32_t> imem_off, imem, bquery, body_tri0, sample_base, icount;
-    Buf<float> bbox, samples;
+    Buf<float> bbox, samples, extra;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
     Buf<uint32_t> pbucket, tcount, tstart;

@@ -44,13 +44,20 @@ __device__ __forceinline__ int cell_of(float v, float inv_cell) { return (int)fl
 
 
 
-// near_triangle (src/phys/world.cpp:1135-1142) for a two-sided triangle
-__device__ __forceinline__ bool near_triangle(f3 p, f3 a, f3 b, f3 c, f3 tu, float r) {
+// near_triangle (src/phys/world.cpp:1135-1142): a two-sided triangle within r of p; a hull (one-sided) triangle also
+// takes p up to `depth` behind it, over its face
+__device__ __forceinline__ bool near_triangle(f3 p, f3 a, f3 b, f3 c, f3 tu, float r, bool two_sided, float depth) {
     const float s = dot(sub(p, a), tu);
     f3 bary;
-    const f3 q = closest_on_triangle(p, a, b, c, bary);
-    const f3 d = sub(q, p);
-    return ffabs(s) <= r && dot(d, d) <= r * r;
+    if (two_sided) {
+        if (!(ffabs(s) <= r)) return false;
+        const f3 d = sub(closest_on_triangle(p, a, b, c, bary), p);
+        return dot(d, d) <= r * r;
+    }
+    if (s > r || s < -r - depth) return false;
+    const f3 q = sub(p, mul(tu, s));
+    const f3 d = sub(closest_on_triangle(q, a, b, c, bary), q);
+    return dot(d, d) <= r * r;
 }
 
 __device__ __forceinline__ bool same_group(const KArgs& a, uint32_t ga, uint32_t gb) {
@@ -329,6 +336,26 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
         acc = block_reduce(acc, uad, shu);
         sum = ldexp((double)acc, emin);
     }
+    // a fast body's extra margin (src/phys/world.cpp:1893-1915): its bulk speed - the mass-weighted RMS of its nodes'
+    // speeds, +25 %, within [8, 400] m/s - times the interval; the sums in double in node order, one thread a body
+    for (uint32_t k = k0 + tid; k < k1; k += 256) {
+        const uint32_t g = c.imem[k];
+        const float ms = fsqrt(u2f(a.body_frame[g * BF_N + BF_V2]));
+        float extra = 0;
+        if (ms > 8.0f) {
+            const uint32_t n0 = a.ch_n0[a.body_c0[g]], n1 = a.ch_n0[a.body_c0[g + 1]];
+            const float4* Vv = a.vm[c.parity];
+            double mv2 = 0, m = 0;
+            for (uint32_t i = n0; i < n1; i++) {
+                const float4 v = Vv[i];
+                mv2 += (double)v.w * (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+                m += (double)v.w;
+            }
+            const float bulk = clampf(m > 0 ? 1.25f * (float)sqrt_cr(mv2 / m) : ms, 8.0f, 400.0f);
+            extra = bulk * (float)c.interval * c.dt;
+        }
+        c.extra[g] = extra;
+    }
     if (tid != 0) return;
     if (!exact) {
         // the reference's order: the members in body order, each one's samples in order
@@ -343,7 +370,7 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
     const float margin = 2.0f * speed * (float)c.interval * c.dt + 0.02f;
     const float mean = n ? (float)(sum / (double)n) : 1.0f;
     IslandPar ip;
-    ip.margin = margin, ip.rmax = rmax, ip.reach = rmax + margin;
+    ip.margin = margin, ip.rmax = rmax, ip.reach = rmax + margin, ip.speed = speed;
     ip.inv_cell = 1.0f / clampf(mean + 2.0f * (rmax + margin), 0.25f, 1.0f);
     c.ipar[i] = ip;
 }
@@ -352,7 +379,7 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
 __global__ void k_rb_boxes(KArgs a, CArgs c) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nbodies || c.bisl[g] < 0) return;
-    const float r = c.ipar[c.bisl[g]].reach + 0.0f + 0.0f;
+    const float r = c.ipar[c.bisl[g]].reach + c.extra[g] + a.bodies[g].hull_reach;
     const uint32_t* w = c.body_box + g * BF_N;
     for (int k = 0; k < 3; k++) c.bbox[6 * g + k] = fkey_inv(w[BF_MN + k]) - r, c.bbox[6 * g + 3 + k] = fkey_inv(w[BF_MX + k]) + r;
 }
@@ -409,7 +436,8 @@ __global__ __launch_bounds__(256) void k_rb_pts(KArgs a, CArgs c) {
     if (i < a.nnodes) {
         g = a.ninfo[i] >> 16;
         isl = c.bisl[g];
-        ok = isl >= 0 && c.bquery[g] && (a.ninfo[i] & BL_NF_CONTACTER);
+        // (a fast body's nodes are not hashed: they are paired by the fast bodies' own search)
+        ok = isl >= 0 && c.bquery[g] && !(c.extra[g] > 0) && (a.ninfo[i] & BL_NF_CONTACTER);
         if (ok) {
             p = ld3(a.pm[c.parity][i]);
             ok = near_partner(a, c, g, p, p);
@@ -493,8 +521,8 @@ struct NtQuery {
     uint32_t t;
     int isl;
     f3 pa, pb, pc, tmn, tmx, tu;
-    float rq, margin;
-    bool self_coll;
+    float rq, margin, extra, depth; // extra: the triangle body's fast margin; depth: a hull triangle's, behind it
+    bool self_coll, two_sided;
 };
 
 __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, const NtQuery& q, int x, int y, int z,
@@ -524,12 +552,12 @@ __device__ __forceinline__ uint32_t nt_cell(const KArgs& a, const CArgs& c, cons
         if (self && (!B.self_coll || q.tr.x == node || q.tr.y == node || q.tr.z == node)) continue;
         if (!self && same_group(a, gbn, q.tr.w)) continue;
         const f3 p = mk(pp.x, pp.y, pp.z);
-        const float r = B.collision_radius + q.margin + 0.0f;
-        const float rb = r + 0.0f;
+        const float r = B.collision_radius + q.margin + q.extra;
+        const float rb = r + q.depth;
         if (p.x < q.tmn.x - rb || p.y < q.tmn.y - rb || p.z < q.tmn.z - rb || p.x > q.tmx.x + rb ||
             p.y > q.tmx.y + rb || p.z > q.tmx.z + rb)
             continue;
-        if (!near_triangle(p, q.pa, q.pb, q.pc, q.tu, r)) continue;
+        if (!near_triangle(p, q.pa, q.pb, q.pc, q.tu, r, q.two_sided, q.depth)) continue;
         if (out && at + cnt < cap) out[at + cnt] = make_uint4(gbn, q.tr.w, node, q.t);
         cnt++;
     }
@@ -553,9 +581,11 @@ __global__ __launch_bounds__(256, 5) void k_rb_nt(KArgs a, CArgs c) {
         if (q.isl >= 0 && c.bquery[q.tr.w] && !a.tri_torn[t] && c.st->npts > 0) {
             const IslandPar ip = c.ipar[q.isl];
             q.self_coll = a.bodies[q.tr.w].self_coll != 0;
-            q.margin = ip.margin;
-            const float rq0 = ip.rmax + ip.margin + 0.0f;
-            q.rq = rq0 + 0.0f;
+            q.margin = ip.margin, q.extra = c.extra[q.tr.w];
+            q.two_sided = !a.tri_hull[t];
+            q.depth = q.two_sided ? 0.0f : a.bodies[q.tr.w].hull_depth;
+            const float rq0 = ip.rmax + ip.margin + q.extra;
+            q.rq = rq0 + q.depth;
             q.pa = ld3(a.pm[c.parity][q.tr.x]), q.pb = ld3(a.pm[c.parity][q.tr.y]), q.pc = ld3(a.pm[c.parity][q.tr.z]);
             q.tmn = vmin(q.pa, vmin(q.pb, q.pc)), q.tmx = vmax(q.pa, vmax(q.pb, q.pc));
             const f3 r3 = mk(q.rq, q.rq, q.rq);
@@ -610,8 +640,8 @@ __global__ __launch_bounds__(256, 5) void k_rb_nt(KArgs a, CArgs c) {
 
 // ... and each capsule of a searching body scans for nodes within its radius (a group of kGcap lanes per capsule)
 __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint4 cp, int isl, f3 pa, f3 pb,
-                                            float margin, int x, int y, int z, uint4* out, uint32_t at, uint32_t cap,
-                                            uint32_t ci) {
+                                            float margin, float extra, int x, int y, int z, uint4* out, uint32_t at,
+                                            uint32_t cap, uint32_t ci) {
     uint32_t cnt = 0;
     const uint32_t bk = bucket_of(x, y, z, isl, c.tmask);
     // (the next entry's records read while this one is tested)
@@ -628,7 +658,7 @@ __device__ __forceinline__ uint32_t nc_cell(const KArgs& a, const CArgs& c, uint
         if (gbn == cp.z || c.bisl[gbn] != isl) continue;
         if (same_group(a, gbn, cp.z)) continue;
         const f3 p = mk(pp.x, pp.y, pp.z);
-        const float r = u2f(cp.w) + a.bodies[gbn].collision_radius + margin + 0.0f;
+        const float r = u2f(cp.w) + a.bodies[gbn].collision_radius + margin + extra;
         const float s = closest_on_segment(p, pa, pb);
         const f3 d = sub(add(pa, mul(sub(pb, pa), s)), p);
         if (dot(d, d) > r * r) continue;
@@ -651,15 +681,15 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
     CellRange cr;
     cr.n = 0;
     f3 pa = mk(0, 0, 0), pb = mk(0, 0, 0);
-    float margin = 0;
+    float margin = 0, extra = 0;
     if (live) {
         cp = a.cap_data[ci];
         isl = c.bisl[cp.z];
         if (isl >= 0 && c.bquery[cp.z] && c.st->npts > 0) {
             const IslandPar ip = c.ipar[isl];
-            margin = ip.margin;
+            margin = ip.margin, extra = c.extra[cp.z];
             pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
-            const float rq2 = u2f(cp.w) + ip.rmax + ip.margin + 0.0f;
+            const float rq2 = u2f(cp.w) + ip.rmax + ip.margin + extra;
             const f3 mn = sub(vmin(pa, pb), mk(rq2, rq2, rq2)), mx = add(vmax(pa, pb), mk(rq2, rq2, rq2));
             if (near_partner(a, c, cp.z, mn, mx)) cr.set(mn, mx, ip.inv_cell);
         }
@@ -671,7 +701,7 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
         if (cc < cr.n) {
             int x, y, z;
             cr.cell(cc, x, y, z);
-            cnt = nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
+            cnt = nc_cell(a, c, cp, isl, pa, pb, margin, extra, x, y, z, nullptr, 0, 0, ci);
         }
         if (c0 == 0) keep0 = cnt;
         uint32_t chunk;
@@ -688,11 +718,11 @@ __global__ __launch_bounds__(256) void k_rb_nc(KArgs a, CArgs c) {
         uint32_t cnt = 0;
         if (cc < cr.n) {
             cr.cell(cc, x, y, z);
-            cnt = c0 == 0 ? keep0 : nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, nullptr, 0, 0, ci);
+            cnt = c0 == 0 ? keep0 : nc_cell(a, c, cp, isl, pa, pb, margin, extra, x, y, z, nullptr, 0, 0, ci);
         }
         uint32_t chunk;
         const uint32_t before = group_prefix<kGcap>(cnt, chunk);
-        if (cnt) nc_cell(a, c, cp, isl, pa, pb, margin, x, y, z, c.nc, base + done + before, c.cap_nc, ci);
+        if (cnt) nc_cell(a, c, cp, isl, pa, pb, margin, extra, x, y, z, c.nc, base + done + before, c.cap_nc, ci);
         done += chunk;
     }
     add_last(c, (live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
@@ -723,7 +753,7 @@ __global__ __launch_bounds__(256) void k_rb_prims(KArgs a, CArgs c) {
         isl = c.bisl[tr.w];
         if (isl >= 0 && c.bquery[tr.w] && !a.tri_torn[t]) {
             const IslandPar ip = c.ipar[isl];
-            const float r = a.bodies[tr.w].collision_radius + ip.margin + 0.0f;
+            const float r = a.bodies[tr.w].collision_radius + ip.margin + c.extra[tr.w];
             const f3 pa = ld3(a.pm[c.parity][tr.x]), pb = ld3(a.pm[c.parity][tr.y]), pc = ld3(a.pm[c.parity][tr.z]);
             tmn = vmin(pa, vmin(pb, pc)), tmx = vmax(pa, vmax(pb, pc));
             const f3 mn = sub(tmn, mk(r, r, r)), mx = add(tmx, mk(r, r, r));
@@ -792,7 +822,7 @@ __global__ __launch_bounds__(256) void k_rb_ct(KArgs a, CArgs c) {
         if (isl >= 0) {
             const IslandPar ip = c.ipar[isl];
             const f3 pa = ld3(a.pm[c.parity][cp.x]), pb = ld3(a.pm[c.parity][cp.y]);
-            const float rr = u2f(cp.w) + a.bodies[cp.z].collision_radius + ip.margin + 0.0f;
+            const float rr = u2f(cp.w) + a.bodies[cp.z].collision_radius + ip.margin + c.extra[cp.z];
             mn = sub(vmin(pa, pb), mk(rr, rr, rr)), mx = add(vmax(pa, pb), mk(rr, rr, rr));
             cr.set(mn, mx, ip.inv_cell);
         }
@@ -892,7 +922,8 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
             f3 bary;
             const f3 q = closest_on_triangle(xp, ap, bp, cp, bary);
             const float cr = a.bodies[pr.x].collision_radius;
-            const float r0 = cr + 0.0f;
+            const bool two_sided = !a.tri_hull[pr.w];
+            const float r0 = cr + (two_sided ? 0.0f : a.bodies[pr.y].hull_depth);
             const f3 da = sub(xv, av), db = sub(xv, bv), dc = sub(xv, cv);
             const float vrel = fsqrt(smax(dot(da, da), smax(dot(db, db), dot(dc, dc))));
             const float reach = r0 + 1.25f * vrel * c.horizon + 0.02f;
@@ -904,9 +935,12 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
                 if (nl > 1e-12f) {
                     const f3 n0 = divs(nn, nl);
                     const float s0 = dot(n0, sub(xp, ap));
-                    const float sg = s0 >= 0 ? 1.0f : -1.0f;
-                    const float close = smax(0.0f, smax(-sg * dot(n0, da), smax(-sg * dot(n0, db), -sg * dot(n0, dc))));
-                    if (ffabs(s0) - 1.25f * close * c.horizon - 0.02f >= cr) keep = false;
+                    // (a hull's triangle only from in front)
+                    if (two_sided || s0 > cr) {
+                        const float sg = s0 >= 0 ? 1.0f : -1.0f;
+                        const float close = smax(0.0f, smax(-sg * dot(n0, da), smax(-sg * dot(n0, db), -sg * dot(n0, dc))));
+                        if (ffabs(s0) - 1.25f * close * c.horizon - 0.02f >= cr) keep = false;
+                    }
                 }
             }
         }
@@ -960,6 +994,7 @@ struct ContactNodes {
     float w[5];
     int na, nb;
     float fric;
+    bool hull; // a node behind a hull triangle's face, deeper than its contact radius: hull_push_out
 };
 
 // Every branch sets all five slots (the unused ones to node 0, weight 0): stores of a slot that only some branches
@@ -978,71 +1013,36 @@ __device__ __forceinline__ void contact_nodes(const KArgs& a, const PArgs& p, co
         set_slots(cn, pr.z, t.x, t.y, t.z, 0u, 1.0f, h.bary.x, h.bary.y, h.bary.z, 0.0f);
         // (a tyre node would take its own friction: tyre nodes are refused when a body is added)
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
+        cn.hull = a.tri_hull[pr.w] && h.pen > a.bodies[pr.x].collision_radius;
     } else if (h.pair < p.nnt + p.nnc) {
         const uint4 pr = p.nc[h.pair - p.nnt];
         const uint4 cap = a.cap_data[pr.w];
-        cn.na = 1, cn.nb = 2;
+        cn.na = 1, cn.nb = 2, cn.hull = false;
         set_slots(cn, pr.z, cap.x, cap.y, 0u, 0u, 1.0f, 1 - h.s, h.s, 0.0f, 0.0f);
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
     } else {
         const uint4 pr = p.ct[h.pair - p.nnt - p.nnc];
         const uint4 cap = a.cap_data[pr.z];
         const uint4 t = a.tri_abc[pr.w];
-        cn.na = 2, cn.nb = 3;
+        cn.na = 2, cn.nb = 3, cn.hull = false;
         set_slots(cn, cap.x, cap.y, t.x, t.y, t.z, 1 - h.s, h.s, h.bary.x, h.bary.y, h.bary.z);
         cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
     }
 }
 
-// contact_force<NA, NB> (src/phys/world.cpp:59-91), both sides movable, no force snapshots. The side a nodes
-// come first (slots 0 .. na - 1), then side b's; the loops run over the five slots unrolled, so that the slots
-// stay in registers (a runtime-indexed slot array would live in scratch memory).
-__device__ bool contact_force(const KArgs& a, int parity, const ContactNodes& cn, f3 nrm, float pen) {
-    const float4* P = a.pm[parity];
-    const float4* Vv = a.vm[parity];
-    const int na = cn.na, nn = cn.na + cn.nb;
-    float W = 0;
-#pragma unroll
-    for (int i = 0; i < 5; i++)
-        if (i < nn) W += cn.w[i] * cn.w[i] * P[cn.n[i]].w;
-    if (W <= 1e-12f) return false;
-    const float m = 1.0f / W;
-    f3 va = mk(0, 0, 0), vb = mk(0, 0, 0), acc = mk(0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        if (i < na) {
-            va = add(va, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
-            acc = add(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
-        } else if (i < nn) {
-            vb = add(vb, mul(ld3(Vv[cn.n[i]]), cn.w[i]));
-            acc = sub(acc, mul(ld3(a.F[cn.n[i]]), cn.w[i] * P[cn.n[i]].w));
-        }
-    }
-    const f3 f_rel = mul(acc, m);
-    const f3 f = primitive_collision(f_rel, sub(va, vb), m, nrm, a.dt, k_gm_c[BL_SURF_CONCRETE], pen, cn.fric, 2.0f, 0.0f, -1.0f);
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        if (i < na) {
-            const float4 F = a.F[cn.n[i]];
-            a.F[cn.n[i]] = make_float4(F.x + f.x * cn.w[i], F.y + f.y * cn.w[i], F.z + f.z * cn.w[i], F.w);
-        } else if (i < nn) {
-            const float4 F = a.F[cn.n[i]];
-            a.F[cn.n[i]] = make_float4(F.x - f.x * cn.w[i], F.y - f.y * cn.w[i], F.z - f.z * cn.w[i], F.w);
-        }
-    }
-    return true;
-}
-
-// contact_force and primitive_collision split at the forces: the terms that read only velocities, inverse masses and
-// the geometry (which the response does not change) are computed once per hit, before the window's rounds; each round
 // the concrete model of the inter-body contacts, as compile-time constants
 constexpr ground_model kGmAll[BL_SURF_N] = BL_GROUND_MODELS;
 constexpr ground_model kGmContact = kGmAll[BL_SURF_CONCRETE];
 static_assert(kGmContact.soft_depth == 0, "the contact model has no fluid layer");
 
-// then does what reads the forces accumulated so far. The same operations in the same order as contact_force above.
+// contact_force<NA, NB> (src/phys/world.cpp:59-91, primitive_collision src/phys/static_world.cpp:8-51) and
+// hull_push_out<3> (src/phys/world.cpp:93-120), both sides movable, no force snapshots, split at the forces: the terms
+// that read only velocities, inverse masses and the geometry (which the response does not change) are computed once
+// per hit, before the window's rounds; each round then does what reads the forces accumulated so far, the same
+// operations in the same order. The side a nodes come first (slots 0 .. na - 1), then side b's.
 struct CfPre {
     float m, Vn, roff, sv, stop, e_static, g_slide;
+    float t_push; // hull_push_out: (target - vr) m / dt
     f3 vel, slip;
     float wim[5]; // w_i x inverse mass_i
     bool ok;
@@ -1052,6 +1052,7 @@ __device__ __forceinline__ void contact_force_pre(const ContactNodes& cn, const 
                                                   float pen, float dt, CfPre& q) {
     constexpr ground_model gm = kGmContact;
     const int na = cn.na, nn = cn.na + cn.nb;
+    // (hull_push_out's W - its node's inverse mass, then the triangle's weighted - is this sum with w_0 = 1)
     float W = 0;
 #pragma unroll
     for (int i = 0; i < 5; i++)
@@ -1065,6 +1066,9 @@ __device__ __forceinline__ void contact_force_pre(const ContactNodes& cn, const 
         if (i < na) va = add(va, mul(vel[i], cn.w[i]));
         else if (i < nn) vb = add(vb, mul(vel[i], cn.w[i]));
     }
+    // hull_push_out: the push-out speed (a fifth of the depth a step, at most 2 m/s) against the relative speed
+    const float vr = dot(sub(vel[0], vb), nrm), target = smin(0.2f * pen / dt, 2.0f);
+    q.t_push = (target - vr) * q.m / dt;
     // primitive_collision(f_rel, va - vb, m, nrm, dt, concrete, pen, fric, push_max 2, slop 0, restitution -1)
     const f3 v = sub(va, vb);
     q.vel = v;
@@ -1086,10 +1090,11 @@ constexpr uint32_t kRespDummy = 5 * 512;
 __device__ __forceinline__ float selp(bool c, float x, float y) { return c ? x : y; }
 __device__ __forceinline__ f3 selp(bool c, f3 x, f3 y) { return mk(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z); }
 
-// The rest of contact_force on the forces in LDS, without branches: slot i of the contact reads and writes
-// fs[3 root[i] ..], the one LDS copy of its node's force (an unused slot the dummy cell, its result not taken). The
-// forces are all read first, as contact_force reads them all before it writes; a slot after another of the same node
-// in the contact (a ball's capsule of no length: dup[i], else -1) starts its write from that one's result.
+// The rest of contact_force (or hull_push_out) on the forces in LDS, without branches: slot i of the contact reads
+// and writes fs[3 root[i] ..], the one LDS copy of its node's force (an unused slot the dummy cell, its result not
+// taken). The forces are all read first, as contact_force reads them all before it writes; a slot after another of
+// the same node in the contact (a ball's capsule of no length: dup[i], else -1) starts its write from that one's
+// result.
 __device__ __forceinline__ void contact_force_apply(const ContactNodes& cn, const CfPre& q, const uint16_t* root,
                                                     const int* dup, float* fs, f3 nrm, float pen) {
     constexpr ground_model gm = kGmContact;
@@ -1116,7 +1121,12 @@ __device__ __forceinline__ void contact_force_apply(const ContactNodes& cn, cons
     const f3 o_stat = sub(sub(mul(nrm, R), mul(q.slip, smin(gm.ms * G * q.e_static, q.stop))), slipF);
     const f3 o_slide = sub(mul(nrm, R), mul(q.slip, smin(q.g_slide * G, smax(0.0f, q.stop + dot(slipF, q.slip)))));
     const f3 zero = mk(0, 0, 0);
-    const f3 out = selp(pen >= gm.soft_depth && R > 0, add(zero, selp(stat, o_stat, o_slide)), zero);
+    f3 out = selp(pen >= gm.soft_depth && R > 0, add(zero, selp(stat, o_stat, o_slide)), zero);
+    // hull_push_out: the normal force the push-out needs besides what the node has, applied when positive (its
+    // force on its node is f, f x the weight 1 here)
+    const float fn = q.t_push - dot(acc, nrm) * q.m;
+    out = selp(cn.hull, mul(nrm, fn), out);
+    const bool write = !cn.hull || fn > 0;
     f3 nw[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -1124,7 +1134,7 @@ __device__ __forceinline__ void contact_force_apply(const ContactNodes& cn, cons
 #pragma unroll
         for (int j = 0; j < i; j++) base = selp(dup[i] == j, nw[j], base);
         const f3 d = mul(out, cn.w[i]);
-        nw[i] = selp(i < na, add(base, d), sub(base, d));
+        nw[i] = selp(write, selp(i < na, add(base, d), sub(base, d)), base);
         float* Fs = fs + 3 * root[i];
         Fs[0] = nw[i].x, Fs[1] = nw[i].y, Fs[2] = nw[i].z;
     }
@@ -1460,7 +1470,7 @@ void ContactEngine::release() {
                              &eb, &ecount, &estart, &bcnt, &icap, &irange})
         b->release();
     igroup.release(), ngroup_ = 0;
-    bbox.release(), samples.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
+    bbox.release(), samples.release(), extra.release(), ipar.release(), stat.release(), pcell.release(), ppos.release();
     itri.release(), n_itri = 0, n_icap = 0, gresp_ = 0;
     scell.release(), spos.release(), ea.release(), ebb.release(), ec.release(), sa.release(), sb.release(), sc.release();
     for (Buf<uint4>* b : {&ntraw, &nc, &ct, &nt}) b->release();
@@ -1512,6 +1522,7 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(bisl.ensure(nbodies + 1));
     RC(bmem.ensure(nbodies + 1));
     RC(bbox.ensure(6 * (size_t)nbodies + 6));
+    RC(extra.ensure((size_t)nbodies + 1));
     RC(bquery.ensure(nbodies + 1));
     RC(icount.ensure(nbodies + 1));
     RC(ipar.ensure(nbodies + 1));
@@ -1554,7 +1565,7 @@ void ContactEngine::build_islands(const HostView& hv, float frame_time) {
     bx.resize(nb * 6);
     for (size_t i = 0; i < nb; i++) {
         const float m = smin(hv.max_speed[i], 200.0f) * frame_time * 2.0f + 0.3f;
-        const float r = m + 0.0f; // (no hull triangles: no hull reach)
+        const float r = m + hv.bodies[i].hull_reach;
         const float mn[3] = {hv.mn[i].x, hv.mn[i].y, hv.mn[i].z}, mx[3] = {hv.mx[i].x, hv.mx[i].y, hv.mx[i].z};
         for (int k = 0; k < 3; k++) bx[6 * i + k] = mn[k] - r, bx[6 * i + 3 + k] = mx[k] + r;
     }
@@ -1747,7 +1758,7 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.interval = interval, c.dt = a.dt;
     c.bisl = bisl.p, c.imem_off = imem_off.p, c.imem = imem.p, c.nisl = (uint32_t)npair_islands();
     c.nmem = nmem, c.bmem = bmem.p, c.pcnt = pcnt.p, c.poff = poff.p;
-    c.ipar = ipar.p, c.bbox = bbox.p, c.bquery = bquery.p, c.body_box = body_box;
+    c.ipar = ipar.p, c.bbox = bbox.p, c.extra = extra.p, c.bquery = bquery.p, c.body_box = body_box;
     c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
     c.nsamples = nsamples;
     c.itri = itri.p, c.n_itri = n_itri, c.icap = icap.p, c.n_icap = n_icap, c.irange = irange.p;

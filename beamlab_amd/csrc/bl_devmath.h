@@ -60,6 +60,16 @@ BL_HD double u2d(uint64_t u) { double f; memcpy(&f, &u, 8); return f; }
 
 BL_HD float length(f3 a) { return fsqrt(dot(a, a)); }
 
+// sqrt of a double, rounded to nearest as the host's: the hardware result, then of it and its two neighbours the one
+// whose square is nearest x (exact residuals by fma)
+BL_HD double sqrt_cr(double x) {
+    const double s = sqrt(x);
+    if (!(s > 0) || !(s < 1.7976931348623157e308)) return s;
+    const double dn = u2d(d2u(s) - 1), up = u2d(d2u(s) + 1);
+    const double rs = fabs(dfma(-s, s, x)), rd = fabs(dfma(-dn, dn, x)), ru = fabs(dfma(-up, up, x));
+    return rd < rs ? dn : (ru < rs ? up : s);
+}
+
 // 2^(i/32) rounded to double, less i << 47 (the exponent part the integer k adds back): the table of the method.
 // Generated by hand from the definition (exp2l in extended precision, rounded to double); tests/test_devmath.py
 // recomputes it.

@@ -95,10 +95,33 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
         const uint4 pr = p.nt[i];
         if (!a.tri_torn[pr.w]) {
             const uint4 tr = a.tri_abc[pr.w];
-            const float r = a.bodies[pr.x].collision_radius;
+            const BodyParam& Bt = a.bodies[pr.y];
+            const bool two_sided = !a.tri_hull[pr.w];
+            const float r = Bt.faces_only && !two_sided ? Bt.face_skin : a.bodies[pr.x].collision_radius;
             const f3 np = ld3(P[pr.z]), ap = ld3(P[tr.x]), bp = ld3(P[tr.y]), cp3 = ld3(P[tr.z]);
-            const f3 lo = sub(vmin(ap, vmin(bp, cp3)), mk(r, r, r)), hi = add(vmax(ap, vmax(bp, cp3)), mk(r, r, r));
-            if (!(np.x < lo.x || np.y < lo.y || np.z < lo.z || np.x > hi.x || np.y > hi.y || np.z > hi.z)) {
+            bool go;
+            if (!two_sided) {
+                // a hull triangle: over its face the node is pushed out along the face's normal, from as deep as
+                // the hull's depth behind it; off the face, outside, the ordinary contact with its edges
+                const f3 fn = normalize(cross(sub(bp, ap), sub(cp3, ap)));
+                const float sf = dot(sub(np, ap), fn);
+                go = !(sf >= r || sf < -Bt.hull_depth);
+                if (go) {
+                    f3 bary;
+                    const f3 q = sub(np, mul(fn, sf));
+                    const f3 cq = sub(closest_on_triangle(q, ap, bp, cp3, bary), q);
+                    if (dot(cq, cq) <= 1e-6f) {
+                        h.bary = bary, h.nrm = fn, h.pen = r - sf;
+                        hit = true, go = false;
+                    } else if (sf < 0 || Bt.faces_only) {
+                        go = false;
+                    }
+                }
+            } else {
+                const f3 lo = sub(vmin(ap, vmin(bp, cp3)), mk(r, r, r)), hi = add(vmax(ap, vmax(bp, cp3)), mk(r, r, r));
+                go = !(np.x < lo.x || np.y < lo.y || np.z < lo.z || np.x > hi.x || np.y > hi.y || np.z > hi.z);
+            }
+            if (go) {
                 f3 bary;
                 const f3 cp = closest_on_triangle(np, ap, bp, cp3, bary);
                 const f3 d = sub(np, cp);

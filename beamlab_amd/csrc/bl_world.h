@@ -36,7 +36,9 @@ BL_HD float fkey_inv(uint32_t k) { return u2f((k & 0x80000000u) ? (k & 0x7ffffff
 struct BodyParam {
     float bounce, push_max, slop;
     float collision_radius, contact_friction; // inter-body contacts
-    int32_t group, self_coll, pad;
+    float hull_depth, face_skin;              // its hull (one-sided) triangles: depth behind, faces-only skin
+    float hull_reach;                         // hull_depth when it has a hull triangle, else 0
+    int32_t group, self_coll, faces_only, pad;
 };
 
 struct KArgs {
@@ -72,6 +74,7 @@ struct KArgs {
     // and capsules with global node indices
     float4* F;
     const uint4* tri_abc;  // a, b, c, body
+    const uint8_t* tri_hull; // 1: a hull (one-sided) triangle
     const float* tri_rest2;
     uint32_t* tri_torn;
     const uint4* cap_data; // a, b, body, radius (bits)

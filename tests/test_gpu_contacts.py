@@ -1,11 +1,12 @@
 """Inter-body and self contacts on the GPU against the oracle, bit for bit (node p and v, beam state, break
 counts) at several checkpoints, with the contact counters (pair rebuilds, candidate pairs, narrow-phase tests,
-contacts) equal too; and the refusals of what the GPU path does not step."""
+contacts) equal too - hull (one-sided) triangles and fast bodies among the cases; and the refusals of what the
+GPU path does not step."""
 import numpy as np
 import pytest
 
 from _cases import assert_same, run_world
-from _contact_cases import CONTACT_CASES
+from _contact_cases import CONTACT_CASES, checkpoints
 from beamlab_amd import abi, scenes
 from beamlab_amd.world import World
 
@@ -32,7 +33,7 @@ def gpu_world(settings, boxes, tuning):
 def test_gpu_contacts_match_oracle(name, tuning, oracle_mod):
     print(f"contact case {name} tuning {tuning}", flush=True)
     settings, boxes, bodies, n = CONTACT_CASES[name]()
-    cps = sorted({1, 17, 40, n // 2, n})
+    cps = checkpoints(name, n)
     ow = oracle_mod.OracleWorld(settings, boxes)
     so = run_world(ow, bodies, n, cps)
     gw = gpu_world(settings, boxes, TUNINGS[tuning])
@@ -41,7 +42,8 @@ def test_gpu_contacts_match_oracle(name, tuning, oracle_mod):
     st_o, st_g = ow.stats(), gw.stats()
     for key in COUNTERS:
         assert st_g[key] == st_o[key], (key, st_g[key], st_o[key])
-    assert st_g["contacts"] > 100
+    # (behind a two-sided triangle the ball is out of the thin surface's reach: no contact at all)
+    assert (st_g["contacts"] == 0) if name == "hull_through" else (st_g["contacts"] > 100)
 
 
 def test_contacts_long_frames(oracle_mod):
@@ -92,13 +94,8 @@ def test_fast_body_refused():
     assert e.value.status == abi.BL_ERR_UNSUPPORTED
 
 
-def test_hull_triangles_and_jointed_capsules_refused():
+def test_jointed_capsules_refused():
     w = World(abi.default_settings())
-    lat = scenes.lattice_body(3, 2, 2, 0.25, 1.0, (0.0, 0.3, 0.0), seed=5, faces=True)
-    lat.tris["two_sided"] = 0
-    with pytest.raises(abi.BeamlabError) as e:
-        w.add_body(lat)
-    assert e.value.status == abi.BL_ERR_UNSUPPORTED
     ball = scenes.ball_body((0.0, 1.5, 0.0), 0.1, 2.0)
     ball.capsules["joint"] = 0
     with pytest.raises(abi.BeamlabError) as e:
