@@ -841,10 +841,7 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
             return BL_ERR_HIP;
         }
         if (c->contacts.last.unsupported) {
-            g_last_error = (c->contacts.last.unsupported & 1)
-                               ? "a body in a contact island moves faster than 8 m/s (fast bodies' pairs are not "
-                                 "stepped on the GPU)"
-                               : "a capsule meets more than 256 triangles in one pair rebuild";
+            g_last_error = "a capsule meets more than 256 triangles in one pair rebuild";
             return BL_ERR_UNSUPPORTED;
         }
         const int next = std::min(s + interval, substeps);

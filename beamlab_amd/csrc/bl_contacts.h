@@ -24,14 +24,23 @@ struct IslandPar {
     float speed; // the slow bodies' top speed + 0.5 m/s (the margins' speed)
 };
 
+// A fast body's own grid (World::fast_pairs, src/phys/world.cpp:2330-2344): origin, inverse cell size, cells an axis.
+struct FastGrid {
+    float ox, oy, oz, inv;
+    int dx, dy, dz, pad;
+};
+
 // What a rebuild leaves for the host: list sizes (needed sizes when a list overflowed its buffer) and flags.
 struct RStat {
     uint32_t npts, nnt_raw, nnc, nct, nent, nnt;
-    uint32_t overflow;    // bit 0 NT, 1 NC, 2 CT, 3 entries, 4 partners
-    uint32_t unsupported; // bit 0 a fast body, 1 a capsule meeting too many triangles, 2 a scan's wait timed out
+    uint32_t overflow;    // bit 0 NT, 1 NC, 2 CT, 3 entries, 4 partners, 5 fast queries, 6 fast NT, 7 fast NC
+    uint32_t unsupported; // bit 1 a capsule meeting too many triangles, 2 a scan's wait timed out, 3 lists out of order
     uint32_t last_pairs;  // candidate pairs kept in the last pair island
     uint32_t npart;       // partner entries (when they overflowed their buffer)
     uint32_t nisl;        // response groups of the rebuild (the response's workgroups read it)
+    uint32_t nfq;         // the fast bodies' queries (their partners' triangles and capsules, summed)
+    uint32_t nfnt, nfnc;  // the fast bodies' node-triangle and node-capsule candidates
+    uint32_t nnc_slow;    // the node-capsule candidates of the node hash (nnc counts the fast ones too)
 };
 
 // A one-pass launch's look-back: the workgroups' published words, the ticket counter (null: workgroups in index
@@ -61,6 +70,7 @@ struct CArgs {
     uint32_t* bquery;         // per body: has a partner or collides with itself
     const uint32_t* body_box; // per body: bounds keys (BF_MN, BF_MX words of a frame record)
     const uint32_t* body_tri0;
+    const uint32_t* body_cap0;
     const uint32_t* itri;     // the triangles of the pair islands' bodies, island by island, each in body order
     uint32_t n_itri;
     const uint32_t* icap;     // the capsules of the pair islands' bodies, in the same order
@@ -96,6 +106,13 @@ struct CArgs {
     uint32_t *ecount, *estart;
     uint4* ct;
     uint32_t cap_ct;
+    // the fast bodies' pairs: per island member its grid and its first query (its partners' triangles, then their
+    // capsules, in partner order), the fast candidates, and the lists merged island by island (the slow candidates of
+    // an island, then its fast ones) when there are fast candidates
+    FastGrid* fgrid;
+    uint32_t* fqoff; // [nmem + 1]
+    uint4 *fnt, *fnc, *ntm, *ncm;
+    uint32_t cap_fq, cap_fnt, cap_fnc;
     // the near-pair list
     uint4* nt;
     Lookback lk; // the current one-pass launch's
@@ -106,6 +123,7 @@ struct PArgs {
     const uint4* nt; // node body, triangle body, node, triangle
     const uint4* nc; // node body, capsule body, node, capsule
     const uint4* ct; // capsule body, triangle body, capsule, triangle
+    const uint4* ncm; // the node-capsule list merged with the fast bodies' (the list when st->nfnc is not 0)
     uint32_t nnt, nnc, nct; // (the kernels take them from st, the last rebuild's counts on the device)
     const RStat* st;
     Hit* hraw;       // per block of 256 candidates: its hits first, in order
@@ -213,6 +231,10 @@ private:
     uint32_t nmem = 0, cap_part = 0;
     Buf<uint32_t> imem_off, imem, bquery, body_tri0, sample_base, icount;
     Buf<float> bbox, samples, extra;
+    Buf<uint32_t> body_cap0, fqoff;
+    Buf<FastGrid> fgrid;
+    Buf<uint4> fnt, fnc, ntm, ncm;
+    uint32_t cap_fq = 0, cap_fnt = 0, cap_fnc = 0;
     Buf<IslandPar> ipar;
     Buf<RStat> stat;
     Buf<uint32_t> pbucket, tcount, tstart;

@@ -257,13 +257,11 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
     const uint32_t i = blockIdx.x, tid = threadIdx.x;
     const uint32_t k0 = c.imem_off[i], k1 = c.imem_off[i + 1];
     float rmax = 0, slow = 0;
-    int fast = 0;
     for (uint32_t k = k0 + tid; k < k1; k += 256) {
         const uint32_t g = c.imem[k];
         rmax = fmaxf(rmax, a.bodies[g].collision_radius);
         const float ms = fsqrt(u2f(a.body_frame[g * BF_N + BF_V2]));
         if (ms <= 8.0f) slow = fmaxf(slow, ms);
-        else fast = 1;
     }
     auto fmx = [](float x, float y) { return fmaxf(x, y); };
     auto imx = [](int x, int y) { return x > y ? x : y; };
@@ -272,7 +270,6 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
     auto uad = [](unsigned long long x, unsigned long long y) { return x + y; };
     rmax = block_reduce(rmax, fmx, shf);
     slow = block_reduce(slow, fmx, shf);
-    fast = block_reduce(fast, imx, shi);
     // The island's samples: its members' ranges of the samples (stored by body, then triangle). A member with few
     // samples is swept by one thread; the members with many (up to 64 of them) by all threads, one after another.
     __shared__ uint32_t bigm[64];
@@ -365,7 +362,6 @@ __global__ __launch_bounds__(256) void k_rb_island(KArgs a, CArgs c) {
             for (uint32_t j = c.sample_base[g]; j < c.sample_base[g + 1]; j++) sum += (double)c.samples[j], n++;
         }
     }
-    if (fast) atomicOr(&c.st->unsupported, 1u);
     const float speed = slow + 0.5f;
     const float margin = 2.0f * speed * (float)c.interval * c.dt + 0.02f;
     const float mean = n ? (float)(sum / (double)n) : 1.0f;
@@ -906,11 +902,13 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
     __shared__ uint32_t sh[8];
     const uint32_t tk = wg_ticket(c.lk, sh);
     const uint32_t i = tk * 256 + threadIdx.x;
-    const uint32_t n = c.st->nnt_raw < c.cap_nt ? c.st->nnt_raw : c.cap_nt;
+    // (with fast candidates, the list merged island by island)
+    const uint32_t nf = min(c.st->nfnt, c.cap_fnt), ns = min(c.st->nnt_raw, c.cap_nt), n = ns + nf;
+    const uint4* src = nf ? c.ntm : c.ntraw;
     bool keep = false;
     uint4 pr = make_uint4(0, 0, 0, 0);
     if (i < n) {
-        pr = c.ntraw[i];
+        pr = src[i];
         keep = !a.tri_torn[pr.w];
         if (keep) {
             const uint4 tr = a.tri_abc[pr.w];
@@ -953,6 +951,317 @@ __global__ __launch_bounds__(256) void k_rb_near(KArgs a, CArgs c) {
     if (tk == gridDim.x - 1 && threadIdx.x == 0) c.st->nnt = pre + agg;
 }
 
+// ---------------------------------------------------------------------------------------------- fast bodies
+// World::fast_pairs (src/phys/world.cpp:2310-2522). The margins: a node's own travel until the next refresh (its
+// speed x 1.25 intervals), a slow partner's reach (twice the slow bodies' speed x the interval + 1 cm), a fast
+// partner's own extra margin (+ 1 cm, + its triangle's corners' travel). (The fast pairs are refreshed with the
+// rest, every interval: the fast interval is the slow one.)
+struct FastMargins {
+    float travel, slow_reach;
+};
+__device__ __forceinline__ FastMargins fast_margins(const CArgs& c, int isl) {
+    FastMargins f;
+    f.travel = 1.25f * (float)c.interval * c.dt;
+    f.slow_reach = 2.0f * c.ipar[isl].speed * (float)c.interval * c.dt + 0.01f;
+    return f;
+}
+__device__ __forceinline__ f3 body_mn(const KArgs& a, const CArgs& c, uint32_t g) {
+    const uint32_t* w = c.body_box + g * BF_N;
+    return mk(fkey_inv(w[BF_MN]), fkey_inv(w[BF_MN + 1]), fkey_inv(w[BF_MN + 2]));
+}
+__device__ __forceinline__ f3 body_mx(const KArgs& a, const CArgs& c, uint32_t g) {
+    const uint32_t* w = c.body_box + g * BF_N;
+    return mk(fkey_inv(w[BF_MX]), fkey_inv(w[BF_MX + 1]), fkey_inv(w[BF_MX + 2]));
+}
+__device__ __forceinline__ float body_speed(const KArgs& a, uint32_t g) { return fsqrt(u2f(a.body_frame[g * BF_N + BF_V2])); }
+
+// Each island member's fast queries (a fast body: its partners' triangles, then their capsules; else none) and, for a
+// fast body, its grid: cells about its widest reach (at least 15 cm, at most 2 m, at most 48 an axis) over its
+// bounds. One workgroup; the members in chunks, the query counts scanned into fqoff.
+__global__ __launch_bounds__(1024) void k_rb_fgrid(KArgs a, CArgs c) {
+    __shared__ uint32_t wsum[16], carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t m0 = 0; m0 < c.nmem; m0 += 1024) {
+        const uint32_t m = m0 + tid;
+        uint32_t nq = 0;
+        if (m < c.nmem) {
+            const uint32_t g = c.imem[m];
+            if (c.extra[g] > 0) {
+                const int isl = c.bisl[g];
+                const FastMargins fm = fast_margins(c, isl);
+                const float cr = a.bodies[g].collision_radius;
+                const uint32_t k0 = c.poff[m], k1 = min(k0 + c.pcnt[m], c.cap_part);
+                float rmax = 0;
+                for (uint32_t k = k0; k < k1; k++) {
+                    const uint32_t bo = c.plist[k];
+                    rmax = smax(rmax, cr + fm.slow_reach + c.extra[bo]);
+                    nq += (c.body_tri0[bo + 1] - c.body_tri0[bo]) + (c.body_cap0[bo + 1] - c.body_cap0[bo]);
+                }
+                const f3 mn = body_mn(a, c, g), mx = body_mx(a, c, g);
+                const f3 ext = add(sub(mx, mn), mk(2e-3f, 2e-3f, 2e-3f));
+                float cellsz = clampf(2.0f * rmax, 0.15f, 2.0f);
+                cellsz = smax(cellsz, smax(ext.x, smax(ext.y, ext.z)) / 48.0f);
+                FastGrid G;
+                G.inv = 1.0f / cellsz;
+                G.ox = mn.x - 1e-3f, G.oy = mn.y - 1e-3f, G.oz = mn.z - 1e-3f;
+                G.dx = max(1, min(48, (int)(ext.x * G.inv) + 1));
+                G.dy = max(1, min(48, (int)(ext.y * G.inv) + 1));
+                G.dz = max(1, min(48, (int)(ext.z * G.inv) + 1));
+                G.pad = 0;
+                c.fgrid[m] = G;
+            }
+        }
+        // exclusive scan of the counts
+        uint32_t x = nq;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        uint32_t pre = carry + x - nq;
+        for (uint32_t w = 0; w < wv; w++) pre += wsum[w];
+        if (m < c.nmem) c.fqoff[m] = pre;
+        __syncthreads();
+        if (tid == 1023) carry = pre + nq;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        c.fqoff[c.nmem] = carry;
+        c.st->nfq = carry;
+        if (carry > c.cap_fq) atomicOr(&c.st->overflow, 32u);
+    }
+}
+
+// The cell of a coordinate along one axis of a fast body's grid, clamped (Grid::cell_index's per axis)
+__device__ __forceinline__ int fg_axis(float v, float o, float inv, int d) { return min(max((int)((v - o) * inv), 0), d - 1); }
+
+// A fast body's node test against one query: its cells (between its position -+ its own travel) meet the query's
+// cells [lo, hi], and it passes the query's test
+struct FastQuery {
+    int lo[3], hi[3];
+    bool tri, live;
+    uint32_t g, bo, prim;   // the fast body, the partner, the triangle or capsule (global index)
+    f3 a, b, cc, tmn, tmx, tu;
+    float rt, depth, rc0;   // a triangle's reach and hull depth; a capsule's
+    bool two_sided;
+};
+
+__device__ __forceinline__ bool fast_node(const KArgs& a, const CArgs& c, const FastGrid& G, const FastQuery& q,
+                                          float travel, uint32_t ni) {
+    if (!(a.ninfo[ni] & BL_NF_CONTACTER)) return false;
+    const f3 p = ld3(a.pm[c.parity][ni]), v = ld3(a.vm[c.parity][ni]);
+    const float mv = length(v) * travel;
+    const float o[3] = {G.ox, G.oy, G.oz};
+    const int d[3] = {G.dx, G.dy, G.dz};
+    const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int l = fg_axis(pc[k] - mv, o[k], G.inv, d[k]), h = fg_axis(pc[k] + mv, o[k], G.inv, d[k]);
+        if (h < q.lo[k] || l > q.hi[k]) return false;
+    }
+    if (q.tri) {
+        const float rn = q.rt + length(v) * travel, rb = rn + q.depth;
+        if (p.x < q.tmn.x - rb || p.y < q.tmn.y - rb || p.z < q.tmn.z - rb || p.x > q.tmx.x + rb || p.y > q.tmx.y + rb ||
+            p.z > q.tmx.z + rb)
+            return false;
+        return near_triangle(p, q.a, q.b, q.cc, q.tu, rn, q.two_sided, q.depth);
+    }
+    const float rn = q.rc0 + length(v) * travel;
+    const float sg = closest_on_segment(p, q.a, q.b);
+    const f3 dd = sub(add(q.a, mul(sub(q.b, q.a), sg)), p);
+    return dot(dd, dd) <= rn * rn;
+}
+
+// Each fast query scans its fast body's nodes, in order, for the ones found in its cells within reach: a group of
+// kGf lanes per query, a lane per node (count, then the group's place in the list, then write). CAPS: the capsule
+// queries (node-capsule candidates) instead of the triangle ones.
+constexpr int kGf = 16;
+template <bool CAPS>
+__global__ __launch_bounds__(256) void k_rb_fast(KArgs a, CArgs c) {
+    __shared__ uint32_t sh[8];
+    const uint32_t tk = wg_ticket(c.lk, sh);
+    const uint32_t qi = (tk * 256 + threadIdx.x) / kGf, gl = threadIdx.x & (kGf - 1);
+    const uint32_t nq = min(c.st->nfq, c.cap_fq);
+    FastQuery q;
+    q.live = false;
+    FastGrid G{};
+    uint32_t n0 = 0, n1 = 0;
+    float travel = 0;
+    int isl = -1;
+    if (qi < nq) {
+        // the member: the last whose first query is at most qi; then the partner and the primitive
+        uint32_t lo = 0, hi = c.nmem;
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (c.fqoff[mid] <= qi) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t m = lo, g = c.imem[m];
+        uint32_t l = qi - c.fqoff[m];
+        const uint32_t k0 = c.poff[m], k1 = min(k0 + c.pcnt[m], c.cap_part);
+        uint32_t bo = 0;
+        bool tri = false, found = false;
+        for (uint32_t k = k0; k < k1 && !found; k++) {
+            bo = c.plist[k];
+            const uint32_t nt = c.body_tri0[bo + 1] - c.body_tri0[bo], ncp = c.body_cap0[bo + 1] - c.body_cap0[bo];
+            if (l < nt) tri = true, found = true;
+            else if (l < nt + ncp) l -= nt, found = true;
+            else l -= nt + ncp;
+        }
+        isl = c.bisl[g];
+        if (found && tri != CAPS) {
+            G = c.fgrid[m];
+            n0 = a.ch_n0[a.body_c0[g]], n1 = a.ch_n0[a.body_c0[g + 1]];
+            const FastMargins fm = fast_margins(c, isl);
+            travel = fm.travel;
+            const float cr = a.bodies[g].collision_radius;
+            const bool o_fast = c.extra[bo] > 0;
+            const float fe = c.extra[bo];
+            const float r0 = cr + (o_fast ? fe + 0.01f : fm.slow_reach);
+            const float r = cr + fm.slow_reach + travel * (body_speed(a, g) + (o_fast ? body_speed(a, bo) : 0.0f)) + (o_fast ? fe : 0.0f);
+            const f3 fmn = body_mn(a, c, g), fmx = body_mx(a, c, g);
+            q.g = g, q.bo = bo, q.tri = tri;
+            f3 smn, smx;
+            if (tri) {
+                const uint32_t t = c.body_tri0[bo] + l;
+                q.prim = t;
+                const uint4 tr = a.tri_abc[t];
+                const f3 fbmn = sub(fmn, mk(r, r, r)), fbmx = add(fmx, mk(r, r, r));
+                if (!a.tri_torn[t]) {
+                    q.a = ld3(a.pm[c.parity][tr.x]), q.b = ld3(a.pm[c.parity][tr.y]), q.cc = ld3(a.pm[c.parity][tr.z]);
+                    q.tmn = vmin(q.a, vmin(q.b, q.cc)), q.tmx = vmax(q.a, vmax(q.b, q.cc));
+                    const f3 &tmn = q.tmn, &tmx = q.tmx;
+                    if (!(tmx.x < fbmn.x || tmx.y < fbmn.y || tmx.z < fbmn.z || tmn.x > fbmx.x || tmn.y > fbmx.y || tmn.z > fbmx.z)) {
+                        const float r00 = cr + (o_fast ? 0.01f : fm.slow_reach);
+                        float rt = r00;
+                        if (o_fast) {
+                            const f3 va = ld3(a.vm[c.parity][tr.x]), vb = ld3(a.vm[c.parity][tr.y]), vc = ld3(a.vm[c.parity][tr.z]);
+                            rt = r00 + travel * fsqrt(smax(dot(va, va), smax(dot(vb, vb), dot(vc, vc))));
+                        }
+                        q.rt = rt;
+                        q.two_sided = !a.tri_hull[t];
+                        q.depth = q.two_sided ? 0.0f : a.bodies[bo].hull_depth;
+                        const f3 tn = cross(sub(q.b, q.a), sub(q.cc, q.a));
+                        const float tn2 = dot(tn, tn);
+                        q.tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);
+                        const float e = rt + q.depth;
+                        smn = sub(tmn, mk(e, e, e)), smx = add(tmx, mk(e, e, e));
+                        q.live = true;
+                    }
+                }
+            } else {
+                const uint32_t ci = c.body_cap0[bo] + l;
+                q.prim = ci;
+                const uint4 cp = a.cap_data[ci];
+                q.a = ld3(a.pm[c.parity][cp.x]), q.b = ld3(a.pm[c.parity][cp.y]);
+                const float rc = u2f(cp.w) + r;
+                f3 cmn = sub(vmin(q.a, q.b), mk(rc, rc, rc)), cmx = add(vmax(q.a, q.b), mk(rc, rc, rc));
+                if (!(cmx.x < fmn.x || cmx.y < fmn.y || cmx.z < fmn.z || cmn.x > fmx.x || cmn.y > fmx.y || cmn.z > fmx.z)) {
+                    q.rc0 = u2f(cp.w) + r0;
+                    smn = sub(vmin(q.a, q.b), mk(q.rc0, q.rc0, q.rc0)), smx = add(vmax(q.a, q.b), mk(q.rc0, q.rc0, q.rc0));
+                    q.live = true;
+                }
+            }
+            if (q.live) {
+                // the query's cells (Grid scan: floor below, truncation above, clamped; none when it misses the grid)
+                const float mnv[3] = {smn.x, smn.y, smn.z}, mxv[3] = {smx.x, smx.y, smx.z}, o[3] = {G.ox, G.oy, G.oz};
+                const int d[3] = {G.dx, G.dy, G.dz};
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    q.lo[k] = max(0, (int)floorf((mnv[k] - o[k]) * G.inv));
+                    q.hi[k] = min(d[k] - 1, (int)((mxv[k] - o[k]) * G.inv));
+                    if (q.lo[k] > q.hi[k]) q.live = false;
+                }
+            }
+        }
+    }
+    uint32_t total = 0;
+    if (q.live)
+        for (uint32_t i0 = n0; i0 < n1; i0 += kGf) {
+            const uint32_t ni = i0 + gl;
+            const bool ok = ni < n1 && fast_node(a, c, G, q, travel, ni);
+            uint32_t chunk;
+            (void)group_prefix<kGf>(ok ? 1u : 0u, chunk);
+            total += chunk;
+        }
+    uint32_t agg;
+    const uint32_t gb = __shfl(wg_prefix(gl == 0 ? total : 0u, agg, sh + 1), 0, kGf);
+    const uint32_t pre = wg_lookback(c.lk, tk, agg, c.st, sh + 5), base = pre + gb;
+    uint4* out = CAPS ? c.fnc : c.fnt;
+    const uint32_t cap = CAPS ? c.cap_fnc : c.cap_fnt;
+    uint32_t done = 0;
+    if (q.live && total)
+        for (uint32_t i0 = n0; i0 < n1; i0 += kGf) {
+            const uint32_t ni = i0 + gl;
+            const bool ok = ni < n1 && fast_node(a, c, G, q, travel, ni);
+            uint32_t chunk;
+            const uint32_t before = group_prefix<kGf>(ok ? 1u : 0u, chunk);
+            if (ok && base + done + before < cap) out[base + done + before] = make_uint4(q.g, q.bo, ni, q.prim);
+            done += chunk;
+        }
+    if (CAPS) add_last(c, (q.live && gl == 0 && isl == (int)c.nisl - 1) ? total : 0u);
+    if (tk == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t tot = pre + agg;
+        if (CAPS) {
+            c.st->nfnc = tot;
+            c.st->nnc_slow = c.st->nnc;
+            c.st->nnc = c.st->nnc + tot;
+            if (tot > c.cap_fnc) atomicOr(&c.st->overflow, 128u);
+        } else {
+            c.st->nfnt = tot;
+            if (tot > c.cap_fnt) atomicOr(&c.st->overflow, 64u);
+        }
+    }
+}
+
+// The first entry of a list (island-major) of pair island isl or later
+__device__ __forceinline__ uint32_t isl_lower(const int* bisl, const uint4* L, uint32_t n, int isl) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bisl[L[mid].x] < isl) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// With fast candidates: the node-triangle (raw) and node-capsule lists merged island by island, each island's
+// slow candidates, then its fast ones. Entry j of an island's slow run moves past the fast candidates of the earlier
+// islands; fast entry j past the slow candidates of the islands up to its own.
+__global__ __launch_bounds__(256) void k_rb_splice(KArgs a, CArgs c) {
+    (void)a;
+    const uint32_t nft = min(c.st->nfnt, c.cap_fnt), nfc = min(c.st->nfnc, c.cap_fnc);
+    if (nft == 0 && nfc == 0) return;
+    const uint32_t nst = min(c.st->nnt_raw, c.cap_nt), nsc = min(c.st->nnc_slow, c.cap_nc);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t segs[4] = {nft ? nst : 0u, nft, nfc ? nsc : 0u, nfc};
+    if (i < segs[0]) {
+        const uint4 e = c.ntraw[i];
+        c.ntm[i + isl_lower(c.bisl, c.fnt, nft, c.bisl[e.x])] = e;
+        return;
+    }
+    i -= segs[0];
+    if (i < segs[1]) {
+        const uint4 e = c.fnt[i];
+        c.ntm[i + isl_lower(c.bisl, c.ntraw, nst, c.bisl[e.x] + 1)] = e;
+        return;
+    }
+    i -= segs[1];
+    if (i < segs[2]) {
+        const uint4 e = c.nc[i];
+        c.ncm[i + isl_lower(c.bisl, c.fnc, nfc, c.bisl[e.x])] = e;
+        return;
+    }
+    i -= segs[2];
+    if (i < segs[3]) {
+        const uint4 e = c.fnc[i];
+        c.ncm[i + isl_lower(c.bisl, c.nc, nsc, c.bisl[e.x] + 1)] = e;
+    }
+}
+
 // Each response group's run in the three final lists (NT after the filter, NC, CT): the searches go over the islands'
 // triangles and capsules island by island, so an island's candidates - and a group's, its islands being consecutive -
 // are one run in each list; its bounds are where the group changes. (A list whose islands are not in order would break
@@ -974,9 +1283,11 @@ __global__ void k_rb_ranges(KArgs a, CArgs c) {
     (void)a;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) c.st->nisl = c.ngroup;
-    const uint32_t nnt = min(c.st->nnt, c.cap_nt), nnc = min(c.st->nnc, c.cap_nc), nct = min(c.st->nct, c.cap_ct);
+    const uint32_t nnt = min(c.st->nnt, c.cap_nt + c.cap_fnt);
+    const uint32_t nnc = min(c.st->nnc, c.st->nfnc ? c.cap_nc + c.cap_fnc : c.cap_nc);
+    const uint32_t nct = min(c.st->nct, c.cap_ct);
     island_run(c.bisl, c.igroup, c.nt, nnt, i, c.irange, 0, c.st);
-    island_run(c.bisl, c.igroup, c.nc, nnc, i, c.irange, 1, c.st);
+    island_run(c.bisl, c.igroup, c.st->nfnc ? c.ncm : c.nc, nnc, i, c.irange, 1, c.st);
     island_run(c.bisl, c.igroup, c.ct, nct, i, c.irange, 2, c.st);
 }
 
@@ -1166,6 +1477,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
     __shared__ uint32_t wpre[kRespBlk + 1], wtot[kRespBlk / 64], wnb;
     PArgs p = pin;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
+    if (pin.st->nfnc) p.nc = pin.ncm; // (the node-capsule list merged with the fast bodies')
     const uint32_t nisl = pin.st->nisl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t contacts = 0;
@@ -1477,13 +1789,21 @@ void ContactEngine::release() {
     hraw.release();
     nnt = nnc = nct = 0;
     cap_nt = cap_nc = cap_ct = cap_ent = esize = 0;
+    cap_fq = cap_fnt = cap_fnc = 0;
+    for (Buf<uint4>* b : {&fnt, &fnc, &ntm, &ncm}) b->release();
+    body_cap0.release(), fqoff.release(), fgrid.release();
 }
 
 // The buffers whose sizes follow the candidate counts (grown when a rebuild reports an overflow).
 int ContactEngine::ensure_lists() {
+    if (!cap_fq) cap_fq = 256, cap_fnt = 1024, cap_fnc = 1024;
     RC(ntraw.ensure(cap_nt));
-    RC(nt.ensure(cap_nt));
+    RC(nt.ensure((size_t)cap_nt + cap_fnt));
     RC(nc.ensure(cap_nc));
+    RC(fnt.ensure(cap_fnt));
+    RC(fnc.ensure(cap_fnc));
+    RC(ntm.ensure((size_t)cap_nt + cap_fnt));
+    RC(ncm.ensure((size_t)cap_nc + cap_fnc));
     RC(ct.ensure(cap_ct));
     RC(eb.ensure(cap_ent));
     RC(ea.ensure(cap_ent));
@@ -1518,6 +1838,8 @@ int ContactEngine::setup(const HostView& hv, uint32_t N, uint32_t T, uint32_t C)
     RC(sample_base.ensure(nbodies + 1));
     RC(samples.ensure(nsamples + 1));
     CHK(hipMemcpy(body_tri0.p, tri0.data(), 4 * (nbodies + 1), hipMemcpyHostToDevice));
+    RC(body_cap0.ensure(nbodies + 1));
+    CHK(hipMemcpy(body_cap0.p, h_cap0_.data(), 4 * (nbodies + 1), hipMemcpyHostToDevice));
     CHK(hipMemcpy(sample_base.p, s0.data(), 4 * (nbodies + 1), hipMemcpyHostToDevice));
     RC(bisl.ensure(nbodies + 1));
     RC(bmem.ensure(nbodies + 1));
@@ -1637,6 +1959,8 @@ int ContactEngine::begin_frame(hipStream_t st) {
     const void* was[4] = {imem_off.p, imem.p, itri.p, icap.p};
     RC(imem_off.ensure(np + 1));
     RC(imem.ensure(mem.size() + 1));
+    RC(fqoff.ensure(mem.size() + 1));
+    RC(fgrid.ensure(mem.size() + 1));
     RC(itri.ensure((size_t)n_itri + 1));
     RC(icap.ensure((size_t)n_icap + 1));
     // the response's groups: consecutive pair islands balanced by size (nodes and triangles), at most kRespGroups, so
@@ -1699,6 +2023,8 @@ int ContactEngine::begin_frame(hipStream_t st) {
 int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     const uint32_t N = nnodes, T = ntris, C = ncaps, NB = nbodies;
     c.cap_nt = cap_nt, c.cap_nc = cap_nc, c.cap_ct = cap_ct, c.cap_ent = cap_ent, c.cap_part = cap_part;
+    c.cap_fq = cap_fq, c.cap_fnt = cap_fnt, c.cap_fnc = cap_fnc;
+    c.fnt = fnt.p, c.fnc = fnc.p, c.ntm = ntm.p, c.ncm = ncm.p;
     c.plist = plist.p;
     c.ntraw = ntraw.p, c.nc = nc.p, c.ct = ct.p, c.nt = nt.p;
     c.eb = eb.p, c.ea = ea.p, c.ebb = ebb.p, c.ec = ec.p, c.sa = sa.p, c.sb = sb.p, c.sc = sc.p;
@@ -1730,6 +2056,12 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
     // 2) triangles, then capsules, against the node hash
     if (n_itri) RC(stage(k_rb_nt, ((size_t)n_itri * kGnt + 255) / 256));
     if (n_icap) RC(stage(k_rb_nc, ((size_t)n_icap * kGcap + 255) / 256));
+    // 2b) the fast bodies' nodes against their partners' triangles and capsules (their grids, queries, candidates),
+    // then the lists merged island by island where there are any
+    hipLaunchKernelGGL(k_rb_fgrid, dim3(1), dim3(1024), 0, st, a, c);
+    RC(stage(k_rb_fast<false>, ((size_t)cap_fq * kGf + 255) / 256));
+    RC(stage(k_rb_fast<true>, ((size_t)cap_fq * kGf + 255) / 256));
+    hipLaunchKernelGGL(k_rb_splice, grid_for((size_t)cap_nt + cap_fnt + cap_nc + cap_fnc), dim3(256), 0, st, a, c);
     // 3) capsules against the triangles' cells
     if (C && T) {
         RC(stage(k_rb_prims, ((size_t)n_itri + 255) / 256));
@@ -1738,10 +2070,10 @@ int ContactEngine::enqueue_rebuild(const KArgs& a, CArgs& c, hipStream_t st) {
         if (n_icap) RC(stage(k_rb_ct, ((size_t)n_icap * kGcap + 255) / 256));
     }
     // the near-pair filter on the node-triangle candidates
-    if (T) RC(stage(k_rb_near, ((size_t)cap_nt + 255) / 256));
+    if (T) RC(stage(k_rb_near, ((size_t)cap_nt + cap_fnt + 255) / 256));
     // each island's runs in the three lists
     hipLaunchKernelGGL(k_rb_clear, grid_for(6 * (size_t)c.ngroup), dim3(256), 0, st, c.irange, 6 * c.ngroup);
-    const size_t nr = std::max<size_t>(1, std::max<size_t>(cap_nt, std::max(cap_nc, cap_ct)));
+    const size_t nr = std::max<size_t>(1, std::max<size_t>((size_t)cap_nt + cap_fnt, std::max<size_t>((size_t)cap_nc + cap_fnc, cap_ct)));
     hipLaunchKernelGGL(k_rb_ranges, grid_for(nr), dim3(256), 0, st, a, c);
     return hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP;
 }
@@ -1759,7 +2091,8 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
     c.bisl = bisl.p, c.imem_off = imem_off.p, c.imem = imem.p, c.nisl = (uint32_t)npair_islands();
     c.nmem = nmem, c.bmem = bmem.p, c.pcnt = pcnt.p, c.poff = poff.p;
     c.ipar = ipar.p, c.bbox = bbox.p, c.extra = extra.p, c.bquery = bquery.p, c.body_box = body_box;
-    c.body_tri0 = body_tri0.p, c.sample_base = sample_base.p, c.samples = samples.p;
+    c.body_tri0 = body_tri0.p, c.body_cap0 = body_cap0.p, c.sample_base = sample_base.p, c.samples = samples.p;
+    c.fgrid = fgrid.p, c.fqoff = fqoff.p;
     c.nsamples = nsamples;
     c.itri = itri.p, c.n_itri = n_itri, c.icap = icap.p, c.n_icap = n_icap, c.irange = irange.p;
     c.igroup = igroup.p, c.ngroup = ngroup_;
@@ -1792,7 +2125,10 @@ int ContactEngine::rebuild(const KArgs& a, const uint32_t* body_box, hipStream_t
         if (!last.overflow) break;
         if (attempt > 4) return BL_ERR_HIP;
         if (last.overflow & 1) cap_nt = std::max<uint32_t>(cap_nt, last.nnt_raw + last.nnt_raw / 2 + 1);
-        if (last.overflow & 2) cap_nc = std::max<uint32_t>(cap_nc, last.nnc + last.nnc / 2 + 1);
+        if (last.overflow & 2) cap_nc = std::max<uint32_t>(cap_nc, last.nnc_slow + last.nnc_slow / 2 + 1);
+        if (last.overflow & 32) cap_fq = std::max<uint32_t>(cap_fq, last.nfq + last.nfq / 2 + 1);
+        if (last.overflow & 64) cap_fnt = std::max<uint32_t>(cap_fnt, last.nfnt + last.nfnt / 2 + 1);
+        if (last.overflow & 128) cap_fnc = std::max<uint32_t>(cap_fnc, last.nfnc + last.nfnc / 2 + 1);
         if (last.overflow & 4) cap_ct = std::max<uint32_t>(cap_ct, last.nct + last.nct / 2 + 1);
         if (last.overflow & 8) cap_ent = std::max<uint32_t>(cap_ent, last.nent + last.nent / 2 + 1);
         if (last.overflow & 16) {
@@ -1815,7 +2151,7 @@ int ContactEngine::narrow_args(PArgs& p, uint32_t& nblk) {
     if (n == 0) return BL_OK;
     RC(hraw.ensure((size_t)nblk * 256));
     RC(bcnt.ensure(nblk));
-    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p;
+    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.ncm = ncm.p;
     p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct, p.st = stat.p;
     p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
     return BL_OK;
@@ -1837,13 +2173,13 @@ int ContactEngine::launch_response(const KArgs& a, const PArgs& p, uint32_t nblk
 }
 
 int ContactEngine::collide_fixed(PArgs& p, uint32_t& nblk) {
-    const size_t cap = (size_t)cap_nt + cap_nc + cap_ct;
+    const size_t cap = (size_t)cap_nt + cap_fnt + cap_nc + cap_fnc + cap_ct;
     nblk = (uint32_t)((cap + 255) / 256);
     if (nblk == 0) nblk = 1;
     RC(hraw.ensure((size_t)nblk * 256));
     RC(bcnt.ensure(nblk));
     p = PArgs{};
-    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.st = stat.p;
+    p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.ncm = ncm.p, p.st = stat.p;
     p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
     return BL_OK;
 }

@@ -82,6 +82,7 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
     __shared__ uint32_t wcnt[4];
     PArgs p = pin;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
+    if (pin.st->nfnc) p.nc = pin.ncm; // (the node-capsule list merged with the fast bodies')
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t n = p.nnt + p.nnc + p.nct;
     const float4* P = a.pm[parity];

@@ -78,22 +78,6 @@ def test_contact_scene_full_size(oracle_mod):
     assert st_g["contacts"] > 0 and st_g["narrow_tests"] > 0
 
 
-def test_fast_body_refused():
-    """A body faster than 8 m/s in a contact island has its own pair schedule in the reference: refused."""
-    settings = abi.default_settings()
-    w = World(settings)
-    scenes.ground_box(w)
-    lat = scenes.lattice_body(4, 3, 3, 0.25, 1.0, (0.0, 0.3, 0.0), seed=5, faces=True)
-    ball = scenes.ball_body((0.0, 1.5, 0.0), 0.1, 2.0)
-    ball.nodes["v"][0] = (0.0, -20.0, 0.0)
-    w.add_body(lat)
-    w.add_body(ball)
-    with pytest.raises(abi.BeamlabError) as e:
-        for _ in range(3):  # (the first frame's islands see the bodies apart: top speed 0 at add_body)
-            w.step_substeps(40)
-    assert e.value.status == abi.BL_ERR_UNSUPPORTED
-
-
 def test_jointed_capsules_refused():
     w = World(abi.default_settings())
     ball = scenes.ball_body((0.0, 1.5, 0.0), 0.1, 2.0)
