@@ -547,6 +547,7 @@ int build(bl_ctx* c) {
     HIPCHK(hipMemset(c->counters.p, 0, sizeof(unsigned long long) * CNT_N));
     HIPCHK(hipMemset(c->broken.p, 0, sizeof(unsigned long long) * c->broken.n));
     HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
+    c->contacts.err_word = c->err.p;
     HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
     HIPCHK(hipMemset(c->wstat.p, 0, sizeof(uint4) * c->wstat.n));
     KArgs& a = c->args;
@@ -925,7 +926,8 @@ int post_frame(bl_ctx* c) {
     if (errw) {
         HIPCHK(hipMemset(c->err.p, 0, sizeof(int)));
     HIPCHK(hipMemset(c->cstat.p, 0, sizeof(uint4) * c->cstat.n));
-        g_last_error = "a resident launch's grid barrier timed out: the state of that frame is not valid";
+        g_last_error = (errw & 2) ? "a contact response's dependency wait gave up: the state of that frame is not valid"
+                                  : "a resident launch's grid barrier timed out: the state of that frame is not valid";
         return BL_ERR_HIP;
     }
     std::vector<unsigned int> w(nb * BF_N);

@@ -130,6 +130,7 @@ struct PArgs {
     uint32_t* bcnt;  // per block: its hit count
     const uint32_t* irange; // per pair island: its runs [start, end) in the NT, NC and CT lists
     uint32_t gresp;  // the response's workgroups (at least the pair islands: st->nisl)
+    int* err;        // the context's error word (bit 1: a response's dependency wait gave up)
 };
 
 // What the island building reads of a body on the host.
@@ -174,6 +175,7 @@ public:
     std::vector<uint32_t> pair_off, pair_mem;
     size_t npair_islands() const { return pair_off.empty() ? 0 : pair_off.size() - 1; }
     int interval = 1;
+    int* err_word = nullptr; // the context's device error word (the response raises bit 1 on a wait that gives up)
     RStat last{};
     // counters (bl_stats)
     int64_t pair_rebuilds = 0, contact_pairs = 0, narrow_tests = 0;

@@ -1460,6 +1460,7 @@ __device__ __forceinline__ void contact_force_apply(const ContactNodes& cn, cons
 // predecessors all ran in earlier rounds. The window's node forces are held in LDS meanwhile, one copy per node at the
 // node's first slot in the window (its root), read from F before the first round and written back after the last.
 constexpr int kRespW = 512, kRespKeys = 4096, kRespBlk = 256, kRespH = 4096;
+constexpr uint32_t kRespSpin = 1u << 22; // a wave's polls for its hits' predecessors before the response gives up
 // a node's bucket among the window's kRespH (a multiplicative hash: consecutive nodes spread)
 __device__ __forceinline__ uint32_t resp_bucket(uint32_t n) { return (n * 0x9E3779B1u) >> (32 - 12); }
 static_assert(kRespH == 1 << 12 && kRespH % kRespW == 0, "resp_bucket takes the hash's top 12 bits");
@@ -1469,8 +1470,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
     __shared__ uint32_t bstart[kRespH + 1];
     __shared__ uint16_t sroot[kRespW * 5]; // per slot: the first slot of its node in the window
     __shared__ int16_t spred[kRespW * 5];  // per slot: the latest earlier hit with a slot of its node, or -1
-    __shared__ uint32_t left[3];           // the hits run by a round (three rounds' counts in turn)
-    __shared__ uint16_t level[kRespW + 1]; // per hit: the round it ran in (0xffff: not yet); [kRespW]: always 0
+    __shared__ uint32_t dn[kRespW + 1];    // per hit: 1 once it has run; [kRespW]: always 1 (no predecessor)
     __shared__ uint32_t sred[kRespW / 64];
     __shared__ uint16_t wfirst[kRespBlk];
     __shared__ uint32_t wblk[kRespBlk];
@@ -1631,7 +1631,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                         sroot[t0] = (uint16_t)root, spred[t0] = (int16_t)pq;
                     }
                     __syncthreads();
-                    // (a slot the contact does not use: the dummy cell, no predecessor - the always-zero level)
+                    // (a slot the contact does not use: the dummy cell, no predecessor - the always-set flag)
                     uint16_t rt[5], px[5];
                     int dup[5];
 #pragma unroll
@@ -1656,32 +1656,32 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                         if (j < nn && rt[j] == tid * 5 + j)
                             fs[3 * rt[j]] = f0[j].x, fs[3 * rt[j] + 1] = f0[j].y, fs[3 * rt[j] + 2] = f0[j].z;
                     if (valid) contact_force_pre(cn, im, vel, hit.nrm, hit.pen, a.dt, q);
-                    // rounds: a hit runs once every predecessor ran in an earlier round (rounds = the longest chain).
-                    // One barrier a round: round r counts the hits it ran into left[r % 3] (one atomic per wave),
-                    // which every thread adds to its own total after the barrier; left[(r + 1) % 3], last read in
-                    // round r - 1, is cleared for the next round meanwhile.
-                    level[tid] = valid ? 0xffffu : 0u;
-                    if (tid < 3) left[tid] = 0;
-                    if (tid == 0) level[kRespW] = 0;
+                    // dataflow: a hit runs as soon as every predecessor has run (a flag per hit in LDS, written
+                    // with release and read with acquire at workgroup scope), so a chain's link costs one contact's
+                    // own latency, not a barrier round. Each wave loops until all its hits have run: a hit's
+                    // predecessors are earlier hits, so the earliest hit not run is always ready. The polls are
+                    // bounded all the same: a wait that outlasts kRespSpin passes raises the error word and ends.
+                    if (tid == 0) dn[kRespW] = 1u;
+                    dn[tid] = valid ? 0u : 1u;
                     bool done = !valid;
                     __syncthreads();
-                    for (uint32_t r = 1, total = 0; total < m; r++) {
-                        bool ran = false;
+                    for (uint32_t spins = 0; !__all(done); spins++) {
                         if (!done) {
-                            bool ready = true;
+                            uint32_t f[5];
 #pragma unroll
-                            for (int j = 0; j < 5; j++) ready = ready && level[px[j]] < r;
-                            if (ready) {
+                            for (int j = 0; j < 5; j++)
+                                f[j] = __hip_atomic_load(&dn[px[j]], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (f[0] & f[1] & f[2] & f[3] & f[4]) {
                                 if (q.ok) contact_force_apply(cn, q, rt, dup, fs, hit.nrm, hit.pen), contacts++;
-                                level[tid] = (uint16_t)r, done = true, ran = true;
+                                __hip_atomic_store(&dn[tid], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                done = true;
+                            } else if (spins >= kRespSpin) {
+                                if (p.err) atomicOr(p.err, 2);
+                                done = true;
                             }
                         }
-                        const uint32_t nran = (uint32_t)__popcll(__ballot(ran));
-                        if (lane == 0 && nran) atomicAdd(&left[r % 3], nran);
-                        if (tid == 0) left[(r + 1) % 3] = 0;
-                        __syncthreads();
-                        total += left[r % 3];
                     }
+                    __syncthreads();
                     // the forces back to F
 #pragma unroll
                     for (int j = 0; j < 5; j++)
@@ -2154,6 +2154,7 @@ int ContactEngine::narrow_args(PArgs& p, uint32_t& nblk) {
     p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.ncm = ncm.p;
     p.nnt = (uint32_t)nnt, p.nnc = (uint32_t)nnc, p.nct = (uint32_t)nct, p.st = stat.p;
     p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
+    p.err = err_word;
     return BL_OK;
 }
 
@@ -2181,6 +2182,7 @@ int ContactEngine::collide_fixed(PArgs& p, uint32_t& nblk) {
     p = PArgs{};
     p.nt = nt.p, p.nc = nc.p, p.ct = ct.p, p.ncm = ncm.p, p.st = stat.p;
     p.hraw = hraw.p, p.bcnt = bcnt.p, p.irange = irange.p, p.gresp = std::max<uint32_t>(gresp_, 1);
+    p.err = err_word;
     return BL_OK;
 }
 

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BL_ABI_VERSION 1
+#define BL_ABI_VERSION 2
 
 enum {
     BL_OK = 0,

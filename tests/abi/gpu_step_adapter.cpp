@@ -1,0 +1,119 @@
+// The reference-side adapter of INTEGRATION.md, as a maintainer would add it to the reference tree (e.g.
+// src/phys/gpu_step.h): it hands the bodies the GPU can step to the drop-in through the C ABI and keeps the rest on
+// the CPU. tests/test_abi_client.py compiles it against the reference's own headers where they lie (syntax and
+// types: the records' layouts are pinned by oracle/ref_capi.cpp's static asserts).
+#include "beamlab_gpu.h"
+#include "phys/world.h"
+
+#include <stdexcept>
+#include <vector>
+
+namespace bl::phys {
+
+// Steps the eligible bodies of a World on the GPU; the others stay with World::step_substeps.
+class GpuStep {
+public:
+    explicit GpuStep(const World& w) {
+        bl_settings s;
+        bl_default_settings(&s);
+        s.gravity[0] = w.settings.gravity.x, s.gravity[1] = w.settings.gravity.y, s.gravity[2] = w.settings.gravity.z;
+        s.dt = w.settings.dt; // WorldSettings (src/phys/world.h:24-60)
+        s.frame_theta = w.settings.frame_theta;
+        s.frame_dissipation = w.settings.frame_dissipation;
+        s.sleeping = 0; // sleeping bodies are not stepped on the GPU
+        check(bl_create(&ctx_, &s));
+    }
+    ~GpuStep() { bl_destroy(ctx_); }
+    GpuStep(const GpuStep&) = delete;
+    GpuStep& operator=(const GpuStep&) = delete;
+
+    void set_statics(const std::vector<bl_box>& boxes) { // StaticWorld boxes (src/phys/static_world.cpp:298-306)
+        check(bl_set_statics(ctx_, boxes.data(), (int32_t)boxes.size()));
+    }
+    // World::add_body (src/phys/world.cpp:280-300): the body's arrays as they are, its options from its fields
+    uint32_t add(SoftBody& b) {
+        static_assert(sizeof(Node) == sizeof(bl_node) && sizeof(Beam) == sizeof(bl_beam) &&
+                          sizeof(Triangle) == sizeof(bl_triangle) && sizeof(Capsule) == sizeof(bl_capsule),
+                      "the records pass unchanged");
+        bl_body_desc d{};
+        d.nodes = reinterpret_cast<const bl_node*>(b.nodes.data());
+        d.info = reinterpret_cast<const bl_node_info*>(b.info.data());
+        d.n_nodes = (uint32_t)b.nodes.size();
+        d.beams = reinterpret_cast<const bl_beam*>(b.beams.data());
+        d.n_beams = (uint32_t)b.beams.size();
+        d.tris = reinterpret_cast<const bl_triangle*>(b.tris.data());
+        d.n_tris = (uint32_t)b.tris.size();
+        d.capsules = reinterpret_cast<const bl_capsule*>(b.capsules.data());
+        d.n_capsules = (uint32_t)b.capsules.size();
+        bl_default_body_opts(&d.opts);
+        d.opts.collision_radius = b.collision_radius, d.opts.ground_friction = b.ground_friction;
+        d.opts.bounce = b.bounce, d.opts.contact_push_max = b.contact_push_max, d.opts.contact_slop = b.contact_slop;
+        d.opts.hull_depth = b.hull_depth, d.opts.sphere_ball = b.sphere_ball, d.opts.contact_friction = b.contact_friction;
+        d.opts.allow_break = b.allow_break, d.opts.allow_deform = b.allow_deform, d.opts.ductile = b.ductile;
+        d.opts.self_collision = b.self_collision, d.opts.collision_group = b.collision_group;
+        d.opts.faces_only = b.faces_only, d.opts.face_skin = b.face_skin;
+        d.opts.air_drag = b.air_drag, d.opts.aero_cda = b.aero_cda, d.opts.rest_damp = b.rest_damp;
+        d.opts.energy_guard = b.energy_guard;
+        uint32_t id;
+        const int rc = bl_add_body(ctx_, &d, &id);
+        if (rc == BL_ERR_UNSUPPORTED) throw Unsupported(bl_error_string(rc)); // keep this body on the CPU
+        check(rc);
+        bodies_.push_back(&b);
+        return id;
+    }
+    // World::step_substeps for the GPU bodies; their arrays read back after the frame
+    void step_substeps(int n) {
+        check(bl_step(ctx_, n));
+        for (uint32_t i = 0; i < bodies_.size(); i++)
+            check(bl_read_body(ctx_, i, reinterpret_cast<bl_node*>(bodies_[i]->nodes.data()),
+                               reinterpret_cast<bl_beam*>(bodies_[i]->beams.data())));
+    }
+
+    struct Unsupported : std::runtime_error {
+        using std::runtime_error::runtime_error;
+    };
+
+private:
+    static void check(int rc) {
+        if (rc != BL_OK) throw std::runtime_error(bl_error_string(rc));
+    }
+    bl_ctx* ctx_ = nullptr;
+    std::vector<SoftBody*> bodies_;
+};
+
+// A frame body's implicit step (FemFrame::finalize -> bl_fem_create, compute_forces + solve -> bl_fem_step): its
+// triangles and members, its masses; its node records back after the step.
+inline bl_fem* make_fem(const SoftBody& b, const std::vector<uint32_t>& tri_corners, const std::vector<uint8_t>& fixed,
+                        const std::vector<float>& masses, const bl_shell_section& shell,
+                        const std::vector<bl_frame_member>& members, const std::vector<bl_frame_section>& sections) {
+    std::vector<float> p(3 * b.nodes.size());
+    for (size_t i = 0; i < b.nodes.size(); i++) p[3 * i] = b.nodes[i].p.x, p[3 * i + 1] = b.nodes[i].p.y, p[3 * i + 2] = b.nodes[i].p.z;
+    bl_fem_desc d{};
+    d.n_nodes = (uint32_t)b.nodes.size();
+    d.n_tris = (uint32_t)(tri_corners.size() / 3);
+    d.p = p.data();
+    d.tris = tri_corners.data();
+    d.fixed = fixed.data();
+    d.mass = masses.data();
+    d.section = shell;
+    d.n_members = (uint32_t)members.size(), d.members = members.data();
+    d.n_sections = (uint32_t)sections.size(), d.sections = sections.data();
+    bl_fem* f = nullptr;
+    if (bl_fem_create(&f, &d) != BL_OK) return nullptr;
+    return f;
+}
+
+inline int fem_substep(bl_fem* f, SoftBody& b, const World& w, const std::vector<float>& loads,
+                       const std::vector<float>& moments) {
+    if (!moments.empty() && bl_fem_add_torques(f, moments.data()) != BL_OK) return -1;
+    const float g[3] = {w.settings.gravity.x, w.settings.gravity.y, w.settings.gravity.z};
+    bl_fem_report rep;
+    int rc = bl_fem_step(f, loads.empty() ? nullptr : loads.data(), w.settings.dt, w.settings.frame_theta,
+                         w.settings.frame_dissipation, g, &rep);
+    if (rc != BL_OK) return rc;
+    std::vector<float> force(3 * b.nodes.size());
+    int64_t counters[3];
+    return bl_fem_get_nodes(f, reinterpret_cast<bl_node*>(b.nodes.data()), force.data(), counters);
+}
+
+} // namespace bl::phys
