@@ -136,6 +136,7 @@ struct bl_ctx {
     HostView hv;
     bool contact_geometry = false; // some body has triangles, capsules or self collision
     bool frame_pending = false;   // a frame enqueued whose post-frame check has not run
+    bool no_coop = false;         // the runtime refused a cooperative launch: the automatic choice steps
     // pinned host staging: the frame-end readback (error word, then the body frame words), queued behind the
     // frame's kernels so that the post-frame check only waits; the first rebuild's bounds words on their way up
     HostPinned fin, w0;
@@ -582,9 +583,11 @@ int ensure_built(bl_ctx* c) {
     return build(c);
 }
 
-// AUTO: the resident form when the chip holds every chunk's workgroup at once, else one launch per substep.
+// AUTO: the halo or resident form when the chip holds every chunk's workgroup at once (and the runtime has not
+// refused a cooperative launch), else one launch per substep.
 int resolved_mode(const bl_ctx* c) {
     if (c->tune.mode != BL_MODE_AUTO) return c->tune.mode;
+    if (c->no_coop) return BL_MODE_STEPPED;
     const int T = effective_block(c), per = effective_per(c);
     bl_tuning h{BL_MODE_HALO, T, 0, 0, per};
     if (tuning_valid(h) && halo_fits(T, per, c->args.nchunks)) return BL_MODE_HALO;
@@ -613,8 +616,13 @@ int enqueue(bl_ctx* c, int substeps) {
         for (int s = 0; s < substeps; s += per) {
             const int k = std::min(per, substeps - s);
             if ((rc = launch_halo(c->args, c->stream, c->parity, k, t, c->flags.p, c->err.p, kSpinLimit))) {
-                g_last_error = "cooperative launch of the halo form failed";
-                return rc;
+                g_last_error = std::string("cooperative launch of the halo form failed: ") + coop_error_name();
+                if (c->tune.mode != BL_MODE_AUTO) return rc;
+                // (the runtime may refuse a cooperative launch, e.g. beside another process on the device: the
+                // automatic choice then takes the stepped form, the same arithmetic, for the rest)
+                (void)hipGetLastError();
+                c->no_coop = true;
+                return enqueue(c, substeps - s);
             }
             c->parity ^= (k & 1);
         }
@@ -627,8 +635,11 @@ int enqueue(bl_ctx* c, int substeps) {
         for (int s = 0; s < substeps; s += per) {
             const int k = std::min(per, substeps - s);
             if ((rc = launch_resident(c->args, c->stream, c->parity, k, t, c->bar.p, c->err.p, kSpinLimit))) {
-                g_last_error = "cooperative launch of the resident form failed";
-                return rc;
+                g_last_error = std::string("cooperative launch of the resident form failed: ") + coop_error_name();
+                if (c->tune.mode != BL_MODE_AUTO) return rc;
+                (void)hipGetLastError();
+                c->no_coop = true;
+                return enqueue(c, substeps - s);
             }
             c->parity ^= (k & 1);
         }

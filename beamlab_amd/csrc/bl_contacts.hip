@@ -104,6 +104,23 @@ __device__ bool near_partner(const KArgs& a, const CArgs& c, uint32_t g, f3 mn, 
     return false;
 }
 
+// near_partner by a group of G lanes with the same query (G divides 64; every lane of the group calls it): the
+// partners split over the lanes, the group stopping at the first pass in which a lane finds one
+template <int G>
+__device__ __forceinline__ bool near_partner_group(const KArgs& a, const CArgs& c, uint32_t g, f3 mn, f3 mx) {
+    if (a.bodies[g].self_coll) return true;
+    const int m = c.bmem[g];
+    const uint32_t k0 = c.poff[m], k1 = k0 + c.pcnt[m] < c.cap_part ? k0 + c.pcnt[m] : c.cap_part;
+    const uint32_t gl = threadIdx.x & (G - 1), shift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    for (uint32_t kb = k0; kb < k1; kb += G) {
+        const uint32_t k = kb + gl;
+        const bool hit = k < k1 && box_meets(c.bbox + 6 * c.plist[k], mn, mx);
+        if ((__ballot(hit) >> shift) & gmask) return true;
+    }
+    return false;
+}
+
 // wave-aggregated add of the counts of the last pair island (the reference's pair counter keeps the last island's);
 // every lane of the wave calls it
 __device__ __forceinline__ void add_last(const CArgs& c, uint32_t n) {
@@ -585,7 +602,7 @@ __global__ __launch_bounds__(256, 5) void k_rb_nt(KArgs a, CArgs c) {
             q.pa = ld3(a.pm[c.parity][q.tr.x]), q.pb = ld3(a.pm[c.parity][q.tr.y]), q.pc = ld3(a.pm[c.parity][q.tr.z]);
             q.tmn = vmin(q.pa, vmin(q.pb, q.pc)), q.tmx = vmax(q.pa, vmax(q.pb, q.pc));
             const f3 r3 = mk(q.rq, q.rq, q.rq);
-            if (near_partner(a, c, q.tr.w, sub(q.tmn, r3), add(q.tmx, r3))) {
+            if (near_partner_group<kGnt>(a, c, q.tr.w, sub(q.tmn, r3), add(q.tmx, r3))) {
                 const f3 tn = cross(sub(q.pb, q.pa), sub(q.pc, q.pa));
                 const float tn2 = dot(tn, tn);
                 q.tu = tn2 > 1e-20f ? mul(tn, 1.0f / fsqrt(tn2)) : mk(0, 0, 0);

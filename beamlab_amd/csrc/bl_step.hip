@@ -759,6 +759,10 @@ __global__ void k_clear_words(unsigned int* p, uint32_t n) {
 // ---------------------------------------------------------------------------------------------- launches
 #define BL_OK_OR_HIP() (hipGetLastError() == hipSuccess ? BL_OK : BL_ERR_HIP)
 
+// the last cooperative launch's status (its name goes into the error message)
+thread_local hipError_t g_coop_error = hipSuccess;
+const char* coop_error_name() { return hipGetErrorName(g_coop_error); }
+
 template <int T, int PER, int WAVES>
 static int launch_resident_t(const KArgs& a, hipStream_t st, int parity, int substeps, unsigned int* bar, int* err,
                              uint32_t spin_limit) {
@@ -766,7 +770,9 @@ static int launch_resident_t(const KArgs& a, hipStream_t st, int parity, int sub
     KArgs aa = a;
     int p = parity, s = substeps;
     void* args[] = {&aa, &p, &s, &bar, &err, &spin_limit};
-    if (hipLaunchCooperativeKernel((const void*)k_resident<T, PER, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st) != hipSuccess)
+    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_resident<T, PER, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st);
+    g_coop_error = e;
+    if (e != hipSuccess)
         return BL_ERR_HIP;
     return BL_OK_OR_HIP();
 }
@@ -855,8 +861,9 @@ static int launch_halo_t(const KArgs& a, hipStream_t st, int parity, int substep
     KArgs aa = a;
     int p = parity, s = substeps;
     void* args[] = {&aa, &p, &s, &flags, &err, &spin_limit};
-    if (hipLaunchCooperativeKernel((const void*)k_halo<T, PER, HCAP, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st) !=
-        hipSuccess)
+    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_halo<T, PER, HCAP, WAVES>, dim3(a.nchunks), dim3(T), args, 0, st);
+    g_coop_error = e;
+    if (e != hipSuccess)
         return BL_ERR_HIP;
     return BL_OK_OR_HIP();
 }

@@ -105,6 +105,8 @@ int launch_halo(const KArgs& a, hipStream_t st, int parity, int substeps, const 
                 int* err, uint32_t spin_limit);
 int halo_fits(int block, int per, uint32_t nchunks);
 int halo_capacity(int block, int per);
+// the name of the last failed cooperative launch's status (resident and halo forms)
+const char* coop_error_name();
 int launch_bounds(const KArgs& a, hipStream_t st, int parity, unsigned int* out);
 int launch_frame_begin(const KArgs& a, hipStream_t st, int parity);
 int launch_frame_end(const KArgs& a, hipStream_t st, int parity);

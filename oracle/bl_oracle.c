@@ -125,6 +125,8 @@ struct orc_world {
     int nbodies;
     bl_stats st;
     int64_t fast_pairs; /* node-triangle and node-capsule candidates of fast bodies, summed over the rebuilds */
+    int64_t last_rb[6]; /* summed over every rebuild: hashed nodes, triangles queried, NT before and after the near
+                           filter, NC, CT (the measurement's algorithmic bytes) */
 };
 
 orc_world* orc_create(const bl_settings* s) {
@@ -774,6 +776,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
             VEC_PUSH(pts, npts, cpts, pt);
         }
     }
+    w->last_rb[0] += (int64_t)npts;
     if (npts && (any_tris || any_caps)) {
         uint32_t *bstart, *bitems;
         const uint32_t mask = build_buckets(ent, nent, &bstart, &bitems);
@@ -797,6 +800,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
                 v3 a = ld3(O->nodes[tr->a].p), bb = ld3(O->nodes[tr->b].p), cc = ld3(O->nodes[tr->c].p);
                 v3 tmn = vmin3(a, vmin3(bb, cc)), tmx = vmax3(a, vmax3(bb, cc));
                 if (!near_partner(&c, bt, vsub(tmn, V(rq, rq, rq)), vadd(tmx, V(rq, rq, rq)))) continue;
+                w->last_rb[1]++;
                 const v3 tn = vcross(vsub(bb, a), vsub(cc, a));
                 const float tn2 = vdot(tn, tn);
                 const v3 tu = tn2 > 1e-20f ? vmul(tn, 1.0f / sqrtf(tn2)) : V(0, 0, 0);
@@ -949,6 +953,7 @@ static void rebuild_pairs(orc_world* w, orc_island* I) {
     for (int k = 0; k < nb; k++) free(c.partners[k]);
     free(c.partners), free(c.npart), free(c.boxes);
     free(bulk), free(extra);
+    w->last_rb[2] += (int64_t)I->nnt, w->last_rb[4] += (int64_t)I->nnc, w->last_rb[5] += (int64_t)I->nct;
     w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
 }
 
@@ -1166,6 +1171,7 @@ static void near_pairs(orc_world* w, orc_island* I) {
         I->nt[wr++] = pr;
     }
     I->nnt = wr;
+    w->last_rb[3] += (int64_t)wr;
     w->st.contact_pairs = (int64_t)(I->nnt + I->nnc + I->nct);
 }
 
@@ -1475,6 +1481,10 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
 }
 
 int64_t orc_fast_pairs(const orc_world* w) { return w->fast_pairs; }
+
+void orc_rebuild_counts(const orc_world* w, int64_t* out) {
+    for (int k = 0; k < 6; k++) out[k] = w->last_rb[k];
+}
 
 int orc_stats(orc_world* w, bl_stats* out) {
     *out = w->st;

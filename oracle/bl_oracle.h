@@ -30,6 +30,9 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
 int orc_stats(orc_world* w, bl_stats* out);
 /* the fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds */
 int64_t orc_fast_pairs(const orc_world* w);
+/* summed over every rebuild (every island): hashed nodes, triangles queried, NT before and after the near filter,
+ * NC, CT */
+void orc_rebuild_counts(const orc_world* w, int64_t* out);
 
 int orc_finalize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb);
 int orc_stabilize(const bl_node* nodes, uint32_t n, bl_beam* beams, uint32_t nb, float dt, float kb, float db,

@@ -59,6 +59,7 @@ def oracle_lib():
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "stats": (ctypes.c_int, [_P, _P]),
             "fast_pairs": (ctypes.c_int64, [_P]),
+            "rebuild_counts": (None, [_P, _P]),
             "finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
             "stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, ctypes.POINTER(ctypes.c_int)]),
@@ -187,6 +188,13 @@ class OracleWorld(_CpuWorld):
         s = abi.Stats()
         self._lib.orc_stats(self._h, ctypes.byref(s))
         return s.as_dict()
+
+    def rebuild_counts(self):
+        """The rebuilds' sizes, summed over every rebuild: hashed nodes, triangles queried, node-triangle candidates
+        before and after the near filter, node-capsule and capsule-triangle candidates."""
+        out = (ctypes.c_int64 * 6)()
+        self._lib.orc_rebuild_counts(self._h, out)
+        return dict(zip(("hashed", "tris_queried", "nt_raw", "nt", "nc", "ct"), list(out)))
 
     def fast_pairs(self):
         """The fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds."""
