@@ -1,15 +1,18 @@
 /* Test infrastructure only: the checker of the HIP shell-FEM path. Only tests/, the smoke check and bench.py's
  * CPU-baseline leg load it; the product never does.
  *
- * A restatement in C of the reference's implicit shell step on a plate (zenkovich/beamlab src/phys/frame_fem.cpp,
- * fem_shell.cpp, core/math.h), elastic sections (no yield, tears or refinement), no members, mounts, contacts or
- * held steps:
+ * A restatement in C of the reference's implicit step of a frame body of shell triangles and Timoshenko members
+ * (zenkovich/beamlab src/phys/frame_fem.cpp, fem_shell.cpp, core/math.h), with plasticity but no topology events (the
+ * body's allow_break off: no tears, splits or letting go), no mounts, contacts or held steps:
  *   ShellMesher::grid / tri                 fem_shell.cpp:17-46   (the plate, its masses)
  *   FemFrame::finalize (triangles)          frame_fem.cpp:259-276 (rest shape, rest frames)
  *   FemFrame::node_inertia                  frame_fem.cpp:3075-3086
  *   tri_grad, dkt_b, tri_local_k            frame_fem.cpp:1482-1579 (the element stiffness, FP64)
  *   tri_build_k                             frame_fem.cpp:1625-1647 (stored as floats)
- *   tri_kinematics, eval_tris (elastic)     frame_fem.cpp:1584-1622, 1661-1831
+ *   tri_kinematics, eval_tris               frame_fem.cpp:1584-1622, 1661-1831 (membrane flow, bending hinges)
+ *   kinematics, stiffness, eval_forces      frame_fem.cpp:1384-1476, 1896-2070 (members: axial, torsion, hinges)
+ *   effective, element_matrix, member_world_k  frame_fem.cpp:1834-1850, 2147-2186
+ *   the unloading re-pass and its budget    frame_fem.cpp:2745-2760, 2959-2996, 1863-1864, 2591
  *   gather_part (triangles, no contacts)    frame_fem.cpp:916-1086 (6x6 blocks, FP64)
  *   chol6, right_solve_lt, sub_abt, sub_mv, sub_mtv, forward6, backward6   frame_fem.cpp:2193-2317
  *   the factorization and substitution loops   frame_fem.cpp:2899-2943 (here over the nodes' natural order, dense)
@@ -17,8 +20,9 @@
  *   the update (dv as force, clamps, skips)   frame_fem.cpp:2904-2908, 3000-3027
  *   SoftBody::integrate_nodes                 softbody.cpp:769-795
  *   FemFrame::sync_positions                  frame_fem.cpp:3407-3422
- * The element stiffness is pinned bit for bit against the compiled reference's FemFrame::tri_stiffness; the rest by
- * closed forms (rigid-body modes, plate theory) and by residuals.
+ * The element stiffness is pinned bit for bit against the compiled reference's FemFrame::tri_stiffness, the whole
+ * substep (plastic state included) against the compiled reference's FemFrame, and by closed forms (rigid-body modes,
+ * plate theory, the frames' deflections, plastic collapse loads) and by residuals.
  */
 #include "fem_oracle.h"
 
@@ -203,6 +207,18 @@ void orc_fem_tri_local_k(const float Xf[6], float E, float nuf, float tf, float 
 }
 
 /* ------------------------------------------------------------------ the plate */
+/* FemFrame::Tangent (frame_fem.h:657-674): a member's tangent at the step's start */
+typedef struct {
+    f3 e1, e2, e3;
+    float L, ka, kt, cy[3], cz[3], ngeo;
+    int on;
+    uint8_t yield, unload, hold; /* yield / unload bits: 1 hinge a, 2 hinge b, 4 axial, 8 torsion; elastic_hold */
+    float ma[2], mb[2], N, T;
+} mem_tan;
+
+/* a member's plastic state (FrameElement, frame_fem.h:209-223): its order in orc_fem::mps */
+enum { PS_UP, PS_TP, PS_PA, PS_PB = 4, PS_BA = 6, PS_BB = 8, PS_BT = 10, PS_DAMAGE, PS_DMG_A, PS_DMG_B, PS_UTIL, PS_N, PS_N_ };
+
 struct orc_fem {
     uint32_t nn, nt;
     double* xd;
@@ -214,14 +230,27 @@ struct orc_fem {
     uint8_t* fixed;
     uint32_t* tn;
     float *X, *r0, *Kf, *tmass, *area0;
-    float E, nu, rho, t, drill, damping;
+    float E, nu, rho, t, drill, damping, yield;
+    /* the triangles' plastic state: the authored rest shape, the corners' plastic rest rotations (element axes, not
+     * yet baked), the plastic stretch reached, the utilisation; the bending's curvature rows at the three points
+     * (tri_b_, 54 floats) and whether the stored stiffness is that of the current rest shape (tri_k_ok_) */
+    float *X0, *th0, *tdmg, *tutil, *Bc;
+    uint8_t* tk_ok;
     /* members (FrameElement): ends, section, joints; rest length, rest frames in the end nodes' frames, the elastic
-     * stiffness (ka, kt, cy[3], cz[3]) and where the co-rotated frame takes its twist from */
+     * stiffness (ka, kt, cy[3], cz[3]) and where the co-rotated frame takes its twist from; the plastic state and the
+     * tangents */
     uint32_t nm, nsec;
     uint32_t* mem;               /* 4 per member: a, b, section, end_a | end_b << 8 */
     float *mL0, *mqa, *mqb, *mst, *mtw;
     uint8_t* mbroken;
     float* sec;                  /* ORC_FRAME_SECTION_FIELDS per section */
+    float* mps;                  /* PS_N_ per member */
+    mem_tan* tan;
+    /* the components (FemFrame::analyse, :557-566: the frame's nodes joined by members and triangles) and their node
+     * counts; the re-pass budget (:1863-1864, 2591) and the statistics */
+    uint32_t *comp, *comp_size, ncomp;
+    float repass_credit;
+    int64_t repasses, passes;
 };
 
 int orc_fem_rest_shape(const float p[9], float X[6]) {
@@ -294,11 +323,53 @@ static void mem_stiffness(const float* s, float L, uint32_t end_a, uint32_t end_
  * shell triangles of one section (E, nu, rho, t, drill, damping); members (a, b, section, end_a | end_b << 8) of
  * elastic sections (ORC_FRAME_SECTION_FIELDS floats each). FemFrame::finalize (frame_fem.cpp:240-290) and
  * node_inertia (:3075-3086). */
+/* tri_build_k (:1625-1647): the stored stiffness (float: the membrane's and the plate's 9 x 9) and the bending's
+ * curvature rows at the three points from the corners' rotations alone, of the current rest shape */
+static void tri_build_k(orc_fem* f, uint32_t k) {
+    double K[18][18];
+    const float* X = &f->X[6 * k];
+    orc_fem_tri_local_k(X, f->E, f->nu, f->t, f->drill, &K[0][0]);
+    float* P = &f->Kf[162 * (size_t)k];
+    for (int a = 0; a < 9; a++)
+        for (int c = 0; c < 9; c++) {
+            const int im = 6 * (a / 3) + kMemDof[a % 3], jm = 6 * (c / 3) + kMemDof[c % 3];
+            const int ip = 6 * (a / 3) + kPlateDof[a % 3], jp = 6 * (c / 3) + kPlateDof[c % 3];
+            P[a * 9 + c] = (float)K[im < jm ? im : jm][im < jm ? jm : im];
+            P[81 + a * 9 + c] = (float)K[ip < jp ? ip : jp][ip < jp ? jp : ip];
+        }
+    const double x[3] = {X[0], X[2], X[4]}, y[3] = {X[1], X[3], X[5]};
+    float* Bc = &f->Bc[54 * (size_t)k];
+    for (int g = 0; g < 3; g++) {
+        double B[3][9];
+        dkt_b(x, y, kGauss[g][0], kGauss[g][1], B);
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < 3; i++) Bc[g * 18 + r * 6 + 2 * i] = (float)B[r][3 * i + 1], Bc[g * 18 + r * 6 + 2 * i + 1] = (float)B[r][3 * i + 2];
+    }
+    f->tk_ok[k] = 1;
+}
+
+static uint32_t uf_find(uint32_t* up, uint32_t x) {
+    while (up[x] != x) x = up[x] = up[up[x]];
+    return x;
+}
+
 orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
-                        const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
-                        const float* sections) {
+                        const uint32_t* tris, const float shell[ORC_SHELL_FIELDS], uint32_t nm,
+                        const uint32_t* members, uint32_t nsec, const float* sections) {
     orc_fem* f = (orc_fem*)calloc(1, sizeof(orc_fem));
     f->E = shell[0], f->nu = shell[1], f->rho = shell[2], f->t = shell[3], f->drill = shell[4], f->damping = shell[5];
+    f->yield = shell[6];
+    f->X0 = (float*)calloc(6 * nt + 1, sizeof(float));
+    f->th0 = (float*)calloc(9 * nt + 1, sizeof(float));
+    f->tdmg = (float*)calloc(nt + 1, sizeof(float));
+    f->tutil = (float*)calloc(nt + 1, sizeof(float));
+    f->Bc = (float*)calloc(54 * (size_t)nt + 1, sizeof(float));
+    f->tk_ok = (uint8_t*)calloc(nt + 1, 1);
+    f->mps = (float*)calloc(PS_N_ * (size_t)nm + 1, sizeof(float));
+    f->tan = (mem_tan*)calloc(nm + 1, sizeof(mem_tan));
+    f->comp = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
+    f->comp_size = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
+    f->repass_credit = 8.0f; /* kRepassBurst */
     f->nn = nn, f->nt = nt, f->nm = nm, f->nsec = nsec;
     f->xd = (double*)calloc(3 * nn + 1, sizeof(double));
     f->pos0 = (float*)calloc(3 * nn + 1, sizeof(float));
@@ -381,16 +452,27 @@ orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const ui
             const qf r = qnorm(qmul(qconj(Q(qn[0], qn[1], qn[2], qn[3])), E0));
             f->r0[12 * k + 4 * c] = r.x, f->r0[12 * k + 4 * c + 1] = r.y, f->r0[12 * k + 4 * c + 2] = r.z, f->r0[12 * k + 4 * c + 3] = r.w;
         }
-        double K[18][18];
-        orc_fem_tri_local_k(&f->X[6 * k], f->E, f->nu, f->t, f->drill, &K[0][0]);
-        float* P = &f->Kf[162 * (size_t)k];
-        for (int a = 0; a < 9; a++)
-            for (int c = 0; c < 9; c++) {
-                const int im = 6 * (a / 3) + kMemDof[a % 3], jm = 6 * (c / 3) + kMemDof[c % 3];
-                const int ip = 6 * (a / 3) + kPlateDof[a % 3], jp = 6 * (c / 3) + kPlateDof[c % 3];
-                P[a * 9 + c] = (float)K[im < jm ? im : jm][im < jm ? jm : im];
-                P[81 + a * 9 + c] = (float)K[ip < jp ? ip : jp][ip < jp ? jp : ip];
-            }
+        memcpy(&f->X0[6 * k], &f->X[6 * k], 6 * sizeof(float));
+        tri_build_k(f, k);
+    }
+    /* the components in the order of their first node (:557-566, 594-599) */
+    {
+        uint32_t* up = (uint32_t*)malloc(sizeof(uint32_t) * (nn + 1));
+        for (uint32_t i = 0; i < nn; i++) up[i] = i;
+        for (uint32_t k = 0; k < nm; k++) up[uf_find(up, f->mem[4 * k])] = uf_find(up, f->mem[4 * k + 1]);
+        for (uint32_t k = 0; k < nt; k++) {
+            const uint32_t* n = &f->tn[3 * k];
+            up[uf_find(up, n[0])] = uf_find(up, n[1]), up[uf_find(up, n[1])] = uf_find(up, n[2]);
+        }
+        uint32_t* id = (uint32_t*)malloc(sizeof(uint32_t) * (nn + 1));
+        for (uint32_t i = 0; i < nn; i++) id[i] = UINT32_MAX;
+        for (uint32_t i = 0; i < nn; i++) {
+            const uint32_t r = uf_find(up, i);
+            if (id[r] == UINT32_MAX) id[r] = f->ncomp++;
+            f->comp[i] = id[r];
+            f->comp_size[id[r]]++;
+        }
+        free(up), free(id);
     }
     /* node_inertia: the members' halves (m L^2 / 78 per end, element order), then a corner's share of its triangles,
      * with the node's floor */
@@ -449,7 +531,8 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
                 for (int c = 0; c < 3; c++) mass[tri2[k][c]] += m / 3.0f;
             }
         }
-    orc_fem* f = orc_fem_create(nn, p, mass, fixed, nt, tn, sec, 0, NULL, 0, NULL);
+    const float shell[ORC_SHELL_FIELDS] = {sec[0], sec[1], sec[2], sec[3], sec[4], sec[5], 0.0f};
+    orc_fem* f = orc_fem_create(nn, p, mass, fixed, nt, tn, shell, 0, NULL, 0, NULL);
     free(p), free(mass), free(fixed), free(tn);
     return f;
 }
@@ -457,7 +540,8 @@ orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float sec[6])
 void orc_fem_destroy(orc_fem* f) {
     if (!f) return;
     free(f->torque), free(f->mem), free(f->mL0), free(f->mqa), free(f->mqb), free(f->mst), free(f->mtw), free(f->mbroken);
-    free(f->sec);
+    free(f->sec), free(f->X0), free(f->th0), free(f->tdmg), free(f->tutil), free(f->Bc), free(f->tk_ok), free(f->mps);
+    free(f->tan), free(f->comp), free(f->comp_size);
     free(f->xd), free(f->pos0), free(f->p), free(f->inv_mass), free(f->force), free(f->impulse), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
     free(f->tn), free(f->X), free(f->r0), free(f->Kf), free(f->tmass), free(f->area0);
     free(f);
@@ -553,13 +637,110 @@ static int tri_kin(const orc_fem* f, uint32_t k, double d[18], f3* E1, f3* E2, f
         d[6 * i + 2] = 0;
         const float* qn = &f->q[4 * n[i]];
         const float* r0 = &f->r0[12 * k + 4 * i];
-        const f3 th = qlog(qmul(Ec, qnorm(qmul(Q(qn[0], qn[1], qn[2], qn[3]), Q(r0[0], r0[1], r0[2], r0[3])))));
+        const float* t0 = &f->th0[9 * k + 3 * i];
+        const f3 th = sub3(qlog(qmul(Ec, qnorm(qmul(Q(qn[0], qn[1], qn[2], qn[3]), Q(r0[0], r0[1], r0[2], r0[3]))))),
+                           F3(t0[0], t0[1], t0[2]));
         d[6 * i + 3] = th.x, d[6 * i + 4] = th.y, d[6 * i + 5] = th.z;
     }
     return 1;
 }
 
-static void eval_tri(const orc_fem* f, uint32_t k, tri_out* o) {
+/* std::max / std::min as the reference calls them (the first argument unless the second compares greater / less) */
+#define SMAX(a, b) ((a) < (b) ? (b) : (a))
+#define SMIN(a, b) ((b) < (a) ? (b) : (a))
+
+/* eval_tris's membrane strain and stress (Pa) of the corners' in-plane displacements */
+static void tri_membrane(const double b3[3], const double c3[3], double A2, const double d[18], double nu, double Dm,
+                         double eps[3], double sig[3]) {
+    eps[0] = eps[1] = eps[2] = 0;
+    for (int i = 0; i < 3; i++) {
+        eps[0] += b3[i] * d[6 * i] / A2;
+        eps[1] += c3[i] * d[6 * i + 1] / A2;
+        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
+    }
+    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+}
+
+static double von_mises(const double m[3]) {
+    const double v = m[0] * m[0] - m[0] * m[1] + m[1] * m[1] + 3 * m[2] * m[2];
+    return sqrt(SMAX(0.0, v));
+}
+
+/* eval_tris's plasticity (:1690-1742, the body's allow_break off): the membrane flows radially back to the yield
+ * stress, the rest shape following the stretch (the plastic stretch against the authored shape: the damage; the
+ * stiffness rebuilt); the bending's moments at the three points against the plate's plastic moment, past it the
+ * corners' plastic rest rotations following theirs; plastic rotations past 0.02 rad baked into the rest frames.
+ * Returns the utilisation. */
+static float tri_plastic(orc_fem* f, uint32_t k, double d[18], double b3[3], double c3[3], double* A2, double eps[3],
+                         double sig[3]) {
+    float* X = &f->X[6 * k];
+    const double nu = f->nu, Dm = (double)f->E / (1 - nu * nu);
+    const double seq = von_mises(sig);
+    float util = (float)(seq / f->yield);
+    if (seq > f->yield) {
+        const double r = 1.0 - f->yield / seq;
+        for (int i = 0; i < 3; i++) {
+            X[2 * i] += (float)(r * d[6 * i]), X[2 * i + 1] += (float)(r * d[6 * i + 1]);
+            d[6 * i] *= 1 - r, d[6 * i + 1] *= 1 - r;
+        }
+        const float mx = (X[0] + X[2] + X[4]) / 3.0f, my = (X[1] + X[3] + X[5]) / 3.0f;
+        for (int i = 0; i < 3; i++) X[2 * i] -= mx, X[2 * i + 1] -= my;
+        /* the plastic stretch: F = [dX] [dX0]^-1, its largest principal stretch */
+        const float* X0 = &f->X0[6 * k];
+        const double a0x = X0[2] - X0[0], a0y = X0[3] - X0[1], b0x = X0[4] - X0[0], b0y = X0[5] - X0[1];
+        const double ax = X[2] - X[0], ay = X[3] - X[1], bx = X[4] - X[0], by = X[5] - X[1];
+        const double det = a0x * b0y - b0x * a0y;
+        if (fabs(det) > 1e-12) {
+            const double i00 = b0y / det, i01 = -b0x / det, i10 = -a0y / det, i11 = a0x / det;
+            const double F00 = ax * i00 + bx * i10, F01 = ax * i01 + bx * i11, F10 = ay * i00 + by * i10, F11 = ay * i01 + by * i11;
+            const double C00 = F00 * F00 + F10 * F10, C01 = F00 * F01 + F10 * F11, C11 = F01 * F01 + F11 * F11;
+            const double lmax = 0.5 * (C00 + C11) + sqrt(0.25 * (C00 - C11) * (C00 - C11) + C01 * C01);
+            const float st = (float)(sqrt(lmax) - 1.0);
+            f->tdmg[k] = SMAX(f->tdmg[k], st);
+        }
+        f->tk_ok[k] = 0;
+        const float Xa[3][2] = {{X[0], X[1]}, {X[2], X[3]}, {X[4], X[5]}};
+        tri_grad(Xa, b3, c3, A2);
+        tri_membrane(b3, c3, *A2, d, nu, Dm, eps, sig);
+    }
+    if (!f->tk_ok[k]) tri_build_k(f, k);
+    const float* Bc = &f->Bc[54 * (size_t)k];
+    const double D = f->E * f->t * f->t * f->t / (12.0f * (1.0f - f->nu * f->nu)); /* ShellSection::D(), in float */
+    const double Mp = 0.25 * f->yield * (double)f->t * f->t;
+    double meq = 0;
+    for (int g = 0; g < 3; g++) {
+        double kap[3] = {0, 0, 0};
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < 3; i++)
+                kap[r] += Bc[g * 18 + r * 6 + 2 * i] * d[6 * i + 3] + Bc[g * 18 + r * 6 + 2 * i + 1] * d[6 * i + 4];
+        const double m[3] = {D * (kap[0] + nu * kap[1]), D * (nu * kap[0] + kap[1]), D * 0.5 * (1 - nu) * kap[2]};
+        const double vm = von_mises(m);
+        meq = SMAX(meq, vm);
+    }
+    const float ub = (float)(meq / Mp);
+    util = SMAX(util, ub);
+    float* th0 = &f->th0[9 * k];
+    if (meq > Mp) {
+        const double r = 1.0 - Mp / meq;
+        for (int i = 0; i < 3; i++) {
+            th0[3 * i] += (float)(r * d[6 * i + 3]), th0[3 * i + 1] += (float)(r * d[6 * i + 4]);
+            d[6 * i + 3] *= 1 - r, d[6 * i + 4] *= 1 - r;
+        }
+        f->tdmg[k] = SMAX(f->tdmg[k], 1e-5f);
+    }
+    for (int i = 0; i < 3; i++) {
+        float* t0 = &th0[3 * i];
+        if (t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2] > 0.02f * 0.02f) {
+            float* r0 = &f->r0[12 * k + 4 * i];
+            const qf q = qnorm(qmul(Q(r0[0], r0[1], r0[2], r0[3]), qexp(F3(-t0[0], -t0[1], -t0[2]))));
+            r0[0] = q.x, r0[1] = q.y, r0[2] = q.z, r0[3] = q.w;
+            t0[0] = t0[1] = t0[2] = 0;
+        }
+    }
+    return util;
+}
+
+static void eval_tri(orc_fem* f, uint32_t k, tri_out* o) {
     o->on = 0;
     double d[18];
     f3 e1, e2, e3;
@@ -569,13 +750,12 @@ static void eval_tri(const orc_fem* f, uint32_t k, tri_out* o) {
     tri_grad(Xa, b3, c3, &A2);
     if (!(A2 > 0)) return;
     const double nu = f->nu, Dm = (double)f->E / (1 - nu * nu);
-    double eps[3] = {0, 0, 0}, sig[3];
-    for (int i = 0; i < 3; i++) {
-        eps[0] += b3[i] * d[6 * i] / A2;
-        eps[1] += c3[i] * d[6 * i + 1] / A2;
-        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
-    }
-    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+    double eps[3], sig[3];
+    tri_membrane(b3, c3, A2, d, nu, Dm, eps, sig);
+    float util = 0;
+    if (f->yield > 0) util = tri_plastic(f, k, d, b3, c3, &A2, eps, sig);
+    f->tutil[k] = util;
+    if (!f->tk_ok[k]) tri_build_k(f, k);
     const float* Km = &f->Kf[162 * (size_t)k];
     const float* Kp = Km + 81;
     double vl[18];
@@ -751,37 +931,165 @@ static void mem_matrix(f3 te1, f3 te2, f3 te3, float tL, float ka_f, float kt_f,
         for (int c = r + 1; c < 12; c++) K[c][r] = K[r][c];
 }
 
-/* eval_forces for an elastic member (:1905-2063 with no plastic limits): the axial force, the torque, the end moments,
- * the ends' forces and moments; the tangent and member_world_k (:1834-1850) */
-static void eval_member(const orc_fem* f, uint32_t k, mem_out* o) {
+/* FemFrame::effective (:2147-2155): the tangent's stiffness where the member yields and the step does not unload it -
+ * a flowing hinge turns like a pin, a flowing bar or twist keeps a hundredth */
+static void mem_effective(const mem_tan* t, float* ka, float* kt, float cy[3], float cz[3]) {
+    const uint8_t y = (uint8_t)(t->yield & ~t->unload);
+    *ka = t->ka, *kt = t->kt;
+    for (int i = 0; i < 3; i++) cy[i] = t->cy[i], cz[i] = t->cz[i];
+    if (y & 3) mem_condense(cy, y & 1, (y & 2) != 0), mem_condense(cz, y & 1, (y & 2) != 0);
+    if (y & 4) *ka *= 0.01f;
+    if (y & 8) *kt *= 0.01f;
+}
+
+/* element_matrix at a member's effective tangent (member_world_k, :1834-1850): the blocks aa, bb, ba, ab */
+static void mem_world_k(const mem_tan* t, double W[144], double M[12][12]) {
+    float ka, kt, cy[3], cz[3];
+    mem_effective(t, &ka, &kt, cy, cz);
+    mem_matrix(t->e1, t->e2, t->e3, t->L, ka, kt, cy, cz, M);
+    for (int r = 0; r < 6; r++)
+        for (int q = 0; q < 6; q++)
+            W[r * 6 + q] = M[r][q], W[36 + r * 6 + q] = M[6 + r][6 + q], W[72 + r * 6 + q] = M[6 + r][q],
+            W[108 + r * 6 + q] = M[r][6 + q];
+}
+
+/* the end moments (y and z components) at the end rotations less the hinges' plastic ones (:1946-1950) */
+static void mem_moments(const mem_kin_t* K, const float* st, const float* ps, float Ma[2], float Mb[2]) {
+    const float dax = K->ra.y - ps[PS_PA], day = K->ra.z - ps[PS_PA + 1];
+    const float dbx = K->rb.y - ps[PS_PB], dby = K->rb.z - ps[PS_PB + 1];
+    Ma[0] = st[2] * dax + st[3] * dbx, Ma[1] = st[5] * day + st[6] * dby;
+    Mb[0] = st[3] * dax + st[4] * dbx, Mb[1] = st[6] * day + st[7] * dby;
+}
+
+static inline float len2f(float x, float y) { return sqrtf(x * x + y * y); }
+
+/* a hinge's radial return (:1960-1972): the yield moment hardening with the hinge's largest rotation */
+static int mem_yield_end(const float* s, const float M[2], float cyy, float czz, float* p, const float* base, float* dmg,
+                         float* flow) {
+    const float Mp = s[ORC_SEC_MP] * (1.0f + s[ORC_SEC_HARDENING] * SMIN(1.0f, *dmg));
+    const float l = len2f(M[0], M[1]);
+    if (l <= Mp) return 0;
+    const float r = 1.0f - Mp / l;
+    const float dp0 = cyy > 0 ? M[0] * r / cyy : 0.0f, dp1 = czz > 0 ? M[1] * r / czz : 0.0f;
+    p[0] += dp0, p[1] += dp1;
+    *flow += len2f(dp0, dp1);
+    const float lp = len2f(p[0] + base[0], p[1] + base[1]);
+    *dmg = SMAX(*dmg, lp);
+    return 1;
+}
+
+/* eval_forces for a member (:1905-2069, the body's allow_break off; no joint damping): the axial force (yielding in
+ * tension at Np, in compression at the lower of Np and the Euler load), the torque (at Tp), the end moments (the
+ * hinges' radial return, twice), the ends' forces and moments; the plastic rotations past 0.02 rad baked into the
+ * rest frames; the tangent (its yield bits where it flows this step, elastic where held or the budget is spent) and
+ * member_world_k */
+static void eval_member(orc_fem* f, uint32_t k, mem_out* o, int elastic_step) {
     o->on = 0;
+    mem_tan* t = &f->tan[k];
+    t->on = 0;
     if (f->mbroken[k]) return;
     mem_kin_t K;
     if (!mem_kin(f, k, &K)) return;
+    const float* s = &f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2]];
     const float* st = &f->mst[8 * k];
     const float ka = st[0], kt = st[1];
-    const float* cy = &st[2];
-    const float* cz = &st[5];
+    float* ps = &f->mps[PS_N_ * (size_t)k];
+    const float L0 = f->mL0[k];
+    const int plastic = s[ORC_SEC_NP] > 0;
     const float delta = (float)K.dL;
-    const float N = ka * (delta - 0.0f); /* (no plastic elongation) */
+    float N = ka * (delta - ps[PS_UP]);
+    float Nt = s[ORC_SEC_NP], Nc = s[ORC_SEC_NP];
+    float flow_ax = 0, flow_t = 0;
+    if (plastic) {
+        const float Le = 0.7f * L0, kPi = 3.14159265358979323846f;
+        const float eul = kPi * kPi * s[ORC_SEC_E] * SMIN(s[ORC_SEC_IY], s[ORC_SEC_IZ]) / (Le * Le);
+        Nc = SMIN(s[ORC_SEC_NP], eul);
+        if (N > Nt || N < -Nc) {
+            const float lim = N > 0 ? Nt : -Nc;
+            const float dup = (N - lim) / ka;
+            ps[PS_UP] += dup;
+            flow_ax = fabsf(dup) / L0;
+            const float dm = ps[PS_UP] / L0;
+            ps[PS_DAMAGE] = SMAX(ps[PS_DAMAGE], dm);
+            N = lim;
+        }
+    }
     const float tw = K.rb.x - K.ra.x;
-    const float T = kt * (tw - 0.0f);
-    const float dax = K.ra.y - 0.0f, day = K.ra.z - 0.0f, dbx = K.rb.y - 0.0f, dby = K.rb.z - 0.0f;
-    const float Max = cy[0] * dax + cy[1] * dbx, May = cz[0] * day + cz[1] * dby;
-    const float Mbx = cy[1] * dax + cy[2] * dbx, Mby = cz[1] * day + cz[2] * dby;
-    const float Sy = Max + Mbx, Sz = May + Mby;
+    float T = kt * (tw - ps[PS_TP]);
+    if (plastic && s[ORC_SEC_TP] > 0 && kt > 0 && fabsf(T) > s[ORC_SEC_TP]) {
+        const float lim = T > 0 ? s[ORC_SEC_TP] : -s[ORC_SEC_TP];
+        const float dtp = (T - lim) / kt;
+        ps[PS_TP] += dtp;
+        flow_t = fabsf(dtp);
+        const float dm = fabsf(ps[PS_TP] + ps[PS_BT]) * s[ORC_SEC_HALF] / L0;
+        ps[PS_DAMAGE] = SMAX(ps[PS_DAMAGE], dm);
+        T = lim;
+    }
+    float Ma[2], Mb[2];
+    mem_moments(&K, st, ps, Ma, Mb);
+    float flow_a = 0, flow_b = 0;
+    if (plastic && s[ORC_SEC_MP] > 0)
+        for (int it = 0; it < 2; it++) {
+            if (mem_yield_end(s, Ma, st[2], st[5], &ps[PS_PA], &ps[PS_BA], &ps[PS_DMG_A], &flow_a)) mem_moments(&K, st, ps, Ma, Mb);
+            if (mem_yield_end(s, Mb, st[4], st[7], &ps[PS_PB], &ps[PS_BB], &ps[PS_DMG_B], &flow_b)) mem_moments(&K, st, ps, Ma, Mb);
+        }
+    float util = 0;
+    if (s[ORC_SEC_NP] > 0) {
+        const float u = N > 0 ? N / Nt : -N / Nc;
+        util = SMAX(util, u);
+    }
+    if (s[ORC_SEC_MP] > 0) {
+        const float ua = len2f(Ma[0], Ma[1]) / (1.0f + s[ORC_SEC_HARDENING] * SMIN(1.0f, ps[PS_DMG_A]));
+        const float ub = len2f(Mb[0], Mb[1]) / (1.0f + s[ORC_SEC_HARDENING] * SMIN(1.0f, ps[PS_DMG_B]));
+        const float u = SMAX(ua, ub) / s[ORC_SEC_MP];
+        util = SMAX(util, u);
+    }
+    if (s[ORC_SEC_TP] > 0) {
+        const float u = fabsf(T) / s[ORC_SEC_TP];
+        util = SMAX(util, u);
+    }
+    ps[PS_UTIL] = util, ps[PS_N] = N;
+    const float Sy = Ma[0] + Mb[0], Sz = Ma[1] + Mb[1];
     o->fa = sub3(add3(mul3(K.e1, N), mul3(K.e3, Sy / K.L)), mul3(K.e2, Sz / K.L));
-    o->ta = sub3(sub3(mul3(K.e1, T), mul3(K.e2, Max)), mul3(K.e3, May));
-    o->tb = sub3(sub3(mul3(K.e1, -T), mul3(K.e2, Mbx)), mul3(K.e3, Mby));
+    o->ta = sub3(sub3(mul3(K.e1, T), mul3(K.e2, Ma[0])), mul3(K.e3, Ma[1]));
+    o->tb = sub3(sub3(mul3(K.e1, -T), mul3(K.e2, Mb[0])), mul3(K.e3, Mb[1]));
     o->on = 1;
+    /* the plastic rotations into the rest frames (:2037-2045) */
+    const float kBake = 0.02f;
+    float* qa = &f->mqa[4 * k];
+    float* qb = &f->mqb[4 * k];
+    if (ps[PS_PA] * ps[PS_PA] + ps[PS_PA + 1] * ps[PS_PA + 1] > kBake * kBake) {
+        const qf q = qnorm(qmul(Q(qa[0], qa[1], qa[2], qa[3]), qexp(F3(0, -ps[PS_PA], -ps[PS_PA + 1]))));
+        qa[0] = q.x, qa[1] = q.y, qa[2] = q.z, qa[3] = q.w;
+        ps[PS_BA] += ps[PS_PA], ps[PS_BA + 1] += ps[PS_PA + 1], ps[PS_PA] = ps[PS_PA + 1] = 0;
+    }
+    if (ps[PS_PB] * ps[PS_PB] + ps[PS_PB + 1] * ps[PS_PB + 1] > kBake * kBake) {
+        const qf q = qnorm(qmul(Q(qb[0], qb[1], qb[2], qb[3]), qexp(F3(0, -ps[PS_PB], -ps[PS_PB + 1]))));
+        qb[0] = q.x, qb[1] = q.y, qb[2] = q.z, qb[3] = q.w;
+        ps[PS_BB] += ps[PS_PB], ps[PS_BB + 1] += ps[PS_PB + 1], ps[PS_PB] = ps[PS_PB + 1] = 0;
+    }
+    if (fabsf(ps[PS_TP]) > kBake) {
+        const qf q1 = qnorm(qmul(Q(qa[0], qa[1], qa[2], qa[3]), qexp(F3(0.5f * ps[PS_TP], 0, 0))));
+        const qf q2 = qnorm(qmul(Q(qb[0], qb[1], qb[2], qb[3]), qexp(F3(-0.5f * ps[PS_TP], 0, 0))));
+        qa[0] = q1.x, qa[1] = q1.y, qa[2] = q1.z, qa[3] = q1.w;
+        qb[0] = q2.x, qb[1] = q2.y, qb[2] = q2.z, qb[3] = q2.w;
+        ps[PS_BT] += ps[PS_TP], ps[PS_TP] = 0;
+    }
+    /* the tangent (:2051-2068) */
+    t->on = 1;
+    t->e1 = K.e1, t->e2 = K.e2, t->e3 = K.e3, t->L = K.L, t->ka = ka, t->kt = kt;
+    for (int i = 0; i < 3; i++) t->cy[i] = st[2 + i], t->cz[i] = st[5 + i];
+    t->ngeo = N > 0 ? N / K.L : 0.0f;
+    const float kFlow = 1e-4f;
+    t->yield = (uint8_t)((flow_a > kFlow ? 1 : 0) | (flow_b > kFlow ? 2 : 0) | (flow_ax > 0.1f * kFlow ? 4 : 0) | (flow_t > kFlow ? 8 : 0));
+    t->unload = 0;
+    if (t->hold > 0) t->hold--, t->unload = t->yield;
+    if (elastic_step) t->unload = t->yield;
+    t->ma[0] = Ma[0], t->ma[1] = Ma[1], t->mb[0] = Mb[0], t->mb[1] = Mb[1], t->N = N, t->T = T;
     o->e1 = K.e1;
-    o->ngeo = N > 0 ? N / K.L : 0.0f;
+    o->ngeo = t->ngeo;
     double M[12][12];
-    mem_matrix(K.e1, K.e2, K.e3, K.L, ka, kt, cy, cz, M);
-    for (int r = 0; r < 6; r++)
-        for (int q = 0; q < 6; q++)
-            o->W[r * 6 + q] = M[r][q], o->W[36 + r * 6 + q] = M[6 + r][6 + q], o->W[72 + r * 6 + q] = M[6 + r][q],
-            o->W[108 + r * 6 + q] = M[r][6 + q];
+    mem_world_k(t, o->W, M);
     const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
     const double u[12] = {f->v[3 * a], f->v[3 * a + 1], f->v[3 * a + 2], f->w[3 * a], f->w[3 * a + 1], f->w[3 * a + 2],
                           f->v[3 * b], f->v[3 * b + 1], f->v[3 * b + 2], f->w[3 * b], f->w[3 * b + 1], f->w[3 * b + 2]};
@@ -969,6 +1277,26 @@ static int env_solve(int n, const int* fc, const size_t* roff, double* L, double
     return ok ? 0 : -1;
 }
 
+void orc_fem_plastic(const orc_fem* f, float* tris, float* members, int64_t counters[2]) {
+    if (tris)
+        for (uint32_t k = 0; k < f->nt; k++) {
+            float* o = &tris[ORC_TRI_STATE * (size_t)k];
+            memcpy(o, &f->X[6 * k], 6 * sizeof(float));
+            memcpy(o + 6, &f->th0[9 * k], 9 * sizeof(float));
+            memcpy(o + 15, &f->r0[12 * k], 12 * sizeof(float));
+            o[27] = f->tdmg[k], o[28] = f->tutil[k];
+        }
+    if (members)
+        for (uint32_t k = 0; k < f->nm; k++) {
+            float* o = &members[ORC_MEM_STATE * (size_t)k];
+            memcpy(o, &f->mps[PS_N_ * (size_t)k], PS_N_ * sizeof(float));
+            memcpy(o + 16, &f->mqa[4 * k], 4 * sizeof(float));
+            memcpy(o + 20, &f->mqb[4 * k], 4 * sizeof(float));
+            o[24] = f->tan[k].yield, o[25] = f->tan[k].unload, o[26] = f->tan[k].hold;
+        }
+    if (counters) counters[0] = f->repasses, counters[1] = f->passes;
+}
+
 int orc_fem_block_solve(int n, double* A, double* b) {
     /* the envelope of A's nonzero lower blocks; the factor written back over A's lower blocks */
     int* fc = (int*)malloc(sizeof(int) * (size_t)(n ? n : 1));
@@ -996,6 +1324,117 @@ int orc_fem_block_solve(int n, double* A, double* b) {
     return rc;
 }
 
+/* solve_impl's add_member (:2684-2730) for a pass again: a member's effective-tangent matrix times sign into the
+ * assembled system (its ends' diagonal blocks, the block of the pair, the right side's damping term); the geometric
+ * stiffness is left as it was */
+static void add_member(const orc_fem* f, uint32_t k, const mem_tan* t, double sign, const int* fi, const int* fc,
+                       const size_t* roff, double* L, double* R, double h, double h2, double hd) {
+    double K[12][12], W[144];
+    mem_world_k(t, W, K);
+    const float* s = &f->sec[ORC_FRAME_SECTION_FIELDS * f->mem[4 * k + 2]];
+    const double beta = (double)s[ORC_SEC_DAMPING] * h;
+    const double c = sign * (h2 + beta), cr = sign * (hd + beta);
+    const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
+    const int ka = fi[a], kb = fi[b], fa = ka < 0, fb = kb < 0;
+    if (cr != 0) {
+        const double u[12] = {f->v[3 * a], f->v[3 * a + 1], f->v[3 * a + 2], f->w[3 * a], f->w[3 * a + 1], f->w[3 * a + 2],
+                              f->v[3 * b], f->v[3 * b + 1], f->v[3 * b + 2], f->w[3 * b], f->w[3 * b + 1], f->w[3 * b + 2]};
+        for (int r = 0; r < 12; r++) {
+            double s2 = 0;
+            for (int q = 0; q < 12; q++) s2 += K[r][q] * u[q];
+            if (r < 6) {
+                if (!fa) R[6 * (size_t)ka + r] -= cr * s2;
+            } else if (!fb) {
+                R[6 * (size_t)kb + r - 6] -= cr * s2;
+            }
+        }
+    }
+#define ADD_BLOCK(D, ro, co)                                                                                          \
+    for (int r = 0; r < 6; r++)                                                                                       \
+        for (int q = 0; q < 6; q++) (D)[r * 6 + q] += c * K[(ro) + r][(co) + q];
+    if (!fa) { ADD_BLOCK(ENV(ka, ka), 0, 0) }
+    if (!fb) { ADD_BLOCK(ENV(kb, kb), 6, 6) }
+    if (!fa && !fb && ka != kb) {
+        if (kb > ka) { ADD_BLOCK(ENV(kb, ka), 6, 0) }
+        else { ADD_BLOCK(ENV(ka, kb), 0, 6) }
+    }
+#undef ADD_BLOCK
+}
+
+/* solve_impl's passes (:2741-2760, 2899-2996): factor and solve; if a member yields, the rates of its deformations at
+ * the solution's velocities - a hinge turning against its moment, a bar against its force, a twist against its
+ * torque - unload it: its elastic tangent (held 16 steps), the system corrected and solved again, at most three
+ * passes. A pass again of a component of 60 nodes or more is counted and spends the budget. Returns env_solve's. */
+static int solve_passes(orc_fem* f, const mem_out* mo, const int* fi, int nf, const int* fc, const size_t* roff,
+                        size_t tot, double* L, double* r, double h, double h2, double hd) {
+    const uint32_t nm = f->nm;
+    int any_yield = 0;
+    for (uint32_t k = 0; k < nm; k++) any_yield |= mo[k].on && (f->tan[k].yield & ~f->tan[k].unload);
+    double *La = NULL, *ra = NULL;
+    uint32_t* chg = NULL;
+    uint8_t *was = NULL, *again = NULL;
+    if (any_yield) {
+        La = (double*)malloc(sizeof(double) * 36 * (tot ? tot : 1));
+        ra = (double*)malloc(sizeof(double) * 6 * (size_t)(nf ? nf : 1));
+        memcpy(La, L, sizeof(double) * 36 * tot), memcpy(ra, r, sizeof(double) * 6 * (size_t)nf);
+        chg = (uint32_t*)malloc(sizeof(uint32_t) * (nm + 1));
+        was = (uint8_t*)malloc(nm + 1);
+        again = (uint8_t*)malloc(f->ncomp + 1);
+    }
+    int rc = 0, nchg = 0;
+    for (int pass = 0; pass < 3; pass++) {
+        f->passes++;
+        if (pass > 0) {
+            for (int c = 0; c < nchg; c++) {
+                mem_tan t0 = f->tan[chg[c]];
+                t0.unload = was[c];
+                add_member(f, chg[c], &t0, -1.0, fi, fc, roff, La, ra, h, h2, hd);
+                add_member(f, chg[c], &f->tan[chg[c]], 1.0, fi, fc, roff, La, ra, h, h2, hd);
+            }
+            nchg = 0;
+            memcpy(L, La, sizeof(double) * 36 * tot), memcpy(r, ra, sizeof(double) * 6 * (size_t)nf);
+        }
+        rc = env_solve(nf, fc, roff, L, r);
+        if (rc || !any_yield || pass == 2) break;
+        memset(again, 0, f->ncomp + 1);
+        for (uint32_t k = 0; k < nm; k++) {
+            mem_tan* t = &f->tan[k];
+            const uint8_t y = (uint8_t)(t->yield & ~t->unload);
+            if (!t->on || !y || f->mbroken[k]) continue;
+            f3 vv[2], ww[2];
+            for (int e = 0; e < 2; e++) {
+                const uint32_t n = f->mem[4 * k + e];
+                if (fi[n] < 0) {
+                    vv[e] = ww[e] = F3(0, 0, 0);
+                    continue;
+                }
+                const double* x = &r[6 * (size_t)fi[n]];
+                vv[e] = add3(F3(f->v[3 * n], f->v[3 * n + 1], f->v[3 * n + 2]), F3((float)x[0], (float)x[1], (float)x[2]));
+                ww[e] = add3(F3(f->w[3 * n], f->w[3 * n + 1], f->w[3 * n + 2]), F3((float)x[3], (float)x[4], (float)x[5]));
+            }
+            const float iL = 1.0f / t->L;
+            const f3 dv = sub3(vv[1], vv[0]);
+            const float rax = dot3(t->e3, dv) * iL + dot3(t->e2, ww[0]), ray = -dot3(t->e2, dv) * iL + dot3(t->e3, ww[0]);
+            const float rbx = dot3(t->e3, dv) * iL + dot3(t->e2, ww[1]), rby = -dot3(t->e2, dv) * iL + dot3(t->e3, ww[1]);
+            uint8_t u = 0;
+            if ((y & 1) && rax * t->ma[0] + ray * t->ma[1] < 0) u |= 1;
+            if ((y & 2) && rbx * t->mb[0] + rby * t->mb[1] < 0) u |= 2;
+            if ((y & 4) && dot3(t->e1, dv) * t->N < 0) u |= 4;
+            if ((y & 8) && dot3(t->e1, sub3(ww[1], ww[0])) * t->T < 0) u |= 8;
+            if (u) {
+                chg[nchg] = k, was[nchg++] = t->unload;
+                t->unload |= u, t->hold = 16;
+                again[f->comp[f->mem[4 * k]]] = 1;
+            }
+        }
+        for (uint32_t c = 0; c < f->ncomp; c++)
+            if (again[c] && f->comp_size[c] >= 60) f->repasses++, f->repass_credit -= 1.0f;
+        if (!nchg) break;
+    }
+    free(La), free(ra), free(chg), free(was), free(again);
+    return rc;
+}
+
 /* ------------------------------------------------------------------ the step */
 int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* Aout,
                  double* rhs_out, double* x_out, uint32_t* free_map, uint32_t* nfree_out) {
@@ -1013,9 +1452,16 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         const float* pp = &f->p[3 * i];
         if ((float)x[0] != pp[0] || (float)x[1] != pp[1] || (float)x[2] != pp[2]) x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
     }
+    /* the re-pass budget (begin_forces, :1863-1864): a pass again costs one credit, one more every fourth step, at
+     * most 8; with less than one left the yielding members start from the elastic tangent */
+    {
+        const float cr = f->repass_credit + 0.25f;
+        f->repass_credit = SMIN(8.0f, cr);
+    }
+    const int elastic_step = f->repass_credit < 1.0f;
     const uint32_t nm = f->nm;
     mem_out* mo = (mem_out*)calloc(nm + 1, sizeof(mem_out));
-    for (uint32_t k = 0; k < nm; k++) eval_member(f, k, &mo[k]);
+    for (uint32_t k = 0; k < nm; k++) eval_member(f, k, &mo[k], elastic_step);
     tri_out* o = (tri_out*)calloc(nt + 1, sizeof(tri_out));
     for (uint32_t k = 0; k < nt; k++) eval_tri(f, k, &o[k]);
     /* the node forces: gravity m g, the loads, then the members' ends in member order, then the triangles' corners
@@ -1159,7 +1605,7 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
             }
     }
     if (rhs_out) memcpy(rhs_out, r, sizeof(double) * N6);
-    int rc = env_solve(nf, fc, roff, L, r);
+    int rc = solve_passes(f, mo, fi, nf, fc, roff, tot, L, r, h, h2, hd);
     if (rc == 0) {
         if (x_out) memcpy(x_out, r, sizeof(double) * N6);
         /* solve_impl's update (frame_fem.cpp:3000-3027): a clamped node's spin zeroed; a free node's new velocity

@@ -11,11 +11,20 @@ extern "C" {
 
 typedef struct orc_fem orc_fem;
 
-/* A member section's fields (FrameSection, src/phys/frame_fem.h:90-112, the elastic step's): their order in a section */
+/* A member section's fields (FrameSection, src/phys/frame_fem.h:90-112, the ones the step reads): their order in a
+ * section. Np, Mp, Tp 0: the member stays elastic. */
 enum {
     ORC_SEC_E, ORC_SEC_G, ORC_SEC_RHO, ORC_SEC_A, ORC_SEC_IY, ORC_SEC_IZ, ORC_SEC_J, ORC_SEC_ASY, ORC_SEC_ASZ,
-    ORC_SEC_DAMPING, ORC_SEC_AXIAL, ORC_SEC_JOINT_K, ORC_FRAME_SECTION_FIELDS
+    ORC_SEC_DAMPING, ORC_SEC_AXIAL, ORC_SEC_JOINT_K, ORC_SEC_NP, ORC_SEC_MP, ORC_SEC_TP, ORC_SEC_HARDENING,
+    ORC_SEC_HALF, ORC_FRAME_SECTION_FIELDS
 };
+/* A shell section's fields (ShellSection, src/phys/frame_fem.h:117-149): E, nu, rho, t, drill, damping, yield (Pa; 0:
+ * elastic) */
+enum { ORC_SHELL_FIELDS = 7 };
+/* The plastic state read back per triangle (FrameTri: X[6], th0[9], r0[12] (x y z w), dmg, util) and per member
+ * (FrameElement: up, tp, pa[2], pb[2], ba[2], bb[2], bt, damage, dmg_a, dmg_b, util, N, qa[4], qb[4]; then the
+ * tangent's yield, unload and elastic_hold bits, FemFrame::Tangent) */
+enum { ORC_TRI_STATE = 29, ORC_MEM_STATE = 27 };
 
 /* the element stiffness of one triangle (tri_local_k, src/phys/frame_fem.cpp:1524-1579), 18 x 18 row-major */
 void orc_fem_tri_local_k(const float X[6], float E, float nu, float t, float drill, double* K);
@@ -23,14 +32,18 @@ void orc_fem_tri_local_k(const float X[6], float E, float nu, float t, float dri
 int orc_fem_rest_shape(const float p[9], float X[6]);
 
 /* A plate of nu x nv cells (ShellMesher::grid, src/phys/fem_shell.cpp:34-46) of `size` metres in the x-z plane,
- * centred on the origin at height y0, its border nodes clamped. Section: E, nu, rho, t, drill, damping. */
+ * centred on the origin at height y0, its border nodes clamped. Section: E, nu, rho, t, drill, damping (elastic). */
 orc_fem* orc_fem_plate(int nu, int nv, float size, float y0, const float section[6]);
-/* A frame body: nodes (p, mass as the body has it, pinned flags), shell triangles of one section (E, nu, rho, t, drill,
- * damping), members (4 words each: a, b, section, end_a | end_b << 8, the ends' FrameJoint) of elastic sections
- * (ORC_FRAME_SECTION_FIELDS floats each). FemFrame::finalize and node_inertia. */
+/* A frame body: nodes (p, mass as the body has it, pinned flags), shell triangles of one section (ORC_SHELL_FIELDS
+ * floats), members (4 words each: a, b, section, end_a | end_b << 8, the ends' FrameJoint) of sections of
+ * ORC_FRAME_SECTION_FIELDS floats. FemFrame::finalize and node_inertia. The body may deform (SoftBody::allow_deform)
+ * and does not break (allow_break off: a yielding member or triangle is never split, torn or let go). */
 orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
-                        const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
-                        const float* sections);
+                        const uint32_t* tris, const float shell[ORC_SHELL_FIELDS], uint32_t nm,
+                        const uint32_t* members, uint32_t nsec, const float* sections);
+/* the plastic state (ORC_TRI_STATE floats per triangle, ORC_MEM_STATE per member; either may be null) and the
+ * counters {FemFrame::repasses, passes_} */
+void orc_fem_plastic(const orc_fem* f, float* tris, float* members, int64_t counters[2]);
 /* moments from outside on the nodes for the next step (FemFrame::torque, 3 per node), added to any set before */
 void orc_fem_add_torques(orc_fem* f, const float* torques);
 void orc_fem_destroy(orc_fem* f);

@@ -80,6 +80,7 @@ def oracle_lib():
             "fem_create": (_P, [ctypes.c_uint32, _P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P,
                                 ctypes.c_uint32, _P]),
             "fem_add_torques": (None, [_P, _P]),
+            "fem_plastic": (None, [_P, _P, _P, _P]),
             "fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P,
                                         ctypes.POINTER(ctypes.c_uint32)]),
             "fem_block_solve": (ctypes.c_int, [ctypes.c_int, _P, _P]),
@@ -117,6 +118,7 @@ def ref_lib():
             "frame_new": (_P, [ctypes.c_uint32, _P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P,
                                ctypes.c_uint32, _P]),
             "frame_add_torques": (None, [_P, _P]),
+            "frame_plastic": (None, [_P, _P, _P, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -290,10 +292,27 @@ def _frame_arrays(p, mass, fixed, tris, shell, members, sections):
     mass = np.ascontiguousarray(mass, np.float32).reshape(nn)
     fixed = np.zeros(nn, np.uint8) if fixed is None else np.ascontiguousarray(fixed, np.uint8).reshape(nn)
     tris = np.zeros((0, 3), np.uint32) if tris is None else np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
-    shell = np.asarray(SECTION_C3 if shell is None else shell, np.float32)
+    shell = np.asarray(SECTION_C3 if shell is None else shell, np.float32).reshape(-1)
+    if len(shell) == 6:  # (no yield stress: elastic)
+        shell = np.append(shell, np.float32(0.0)).astype(np.float32)
     members = np.zeros((0, 4), np.uint32) if members is None else np.ascontiguousarray(members, np.uint32).reshape(-1, 4)
-    sections = np.zeros((0, 12), np.float32) if sections is None else np.ascontiguousarray(sections, np.float32).reshape(-1, 12)
+    sections = np.zeros((0, 17), np.float32) if sections is None else np.asarray(sections, np.float32)
+    if sections.ndim == 2 and sections.shape[1] == 12:  # (no plastic limits; FrameSection's hardening and half)
+        pad = np.tile(np.array([0, 0, 0, 0.25, 0.02], np.float32), (len(sections), 1))
+        sections = np.concatenate([sections, pad], axis=1)
+    sections = np.ascontiguousarray(sections, np.float32).reshape(-1, 17)
     return p, mass, fixed, tris, shell, members, sections
+
+
+TRI_STATE, MEM_STATE = 29, 27
+
+
+def _plastic_record(tris, members, cnt):
+    """The plastic state as arrays: triangles' X (3 x 2), th0 (3 x 3), r0 (3 x 4), dmg, util; members' up, tp, pa, pb,
+    ba, bb, bt, damage, dmg_a, dmg_b, util, N, qa, qb (and the oracle's tangent bits); {repasses, passes}."""
+    return dict(X=tris[:, 0:6].reshape(-1, 3, 2), th0=tris[:, 6:15].reshape(-1, 3, 3), r0=tris[:, 15:27].reshape(-1, 3, 4),
+                tdmg=tris[:, 27], tutil=tris[:, 28], mstate=members[:, 0:16], qa=members[:, 16:20], qb=members[:, 20:24],
+                mbits=members[:, 24:27], repasses=int(cnt[0]), passes=int(cnt[1]))
 
 
 def _frame_args(arrs):
@@ -331,6 +350,15 @@ class RefPlate:
 
     def add_torques(self, t):
         self._lib.blref_frame_add_torques(self._h, _ptr(np.ascontiguousarray(t, np.float32).reshape(-1)))
+
+    def plastic(self):
+        """The frame's plastic state (blref_frame_plastic; the tangent's bits are private there: zeros)."""
+        nt, nm = len(self._arrs[3]), len(self._arrs[5])
+        tr = np.zeros((nt, TRI_STATE), np.float32)
+        me = np.zeros((nm, MEM_STATE), np.float32)
+        cnt = np.zeros(2, np.int64)
+        self._lib.blref_frame_plastic(self._h, _ptr(tr.reshape(-1)), _ptr(me.reshape(-1)), _ptr(cnt))
+        return _plastic_record(tr, me, cnt)
 
     def substep(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
         """The plate body's whole substep as the reference's world runs it: the step, then integrate_nodes and
@@ -404,6 +432,15 @@ class FemPlate:
 
     def add_torques(self, t):
         self._lib.orc_fem_add_torques(self._h, _ptr(np.ascontiguousarray(t, np.float32).reshape(-1)))
+
+    def plastic(self):
+        """The frame's plastic state (orc_fem_plastic)."""
+        nm = len(self._arrs[5]) if hasattr(self, "_arrs") else 0
+        tr = np.zeros((self.nt, TRI_STATE), np.float32)
+        me = np.zeros((nm, MEM_STATE), np.float32)
+        cnt = np.zeros(2, np.int64)
+        self._lib.orc_fem_plastic(self._h, _ptr(tr.reshape(-1)), _ptr(me.reshape(-1)), _ptr(cnt))
+        return _plastic_record(tr, me, cnt)
 
     def topology(self):
         tn = np.zeros((self.nt, 3), np.uint32)

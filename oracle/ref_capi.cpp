@@ -280,27 +280,31 @@ int64_t blref_fem_plate_step(void* h, const float* loads, float dt, float theta,
 }
 
 // A frame body built the reference's way: nodes (SoftBody::add_node with their masses, NF_FIXED where pinned), shell
-// triangles of one section (FemFrame::add_shell_section, add_tri), members (FemFrame::add_section, add_element: a, b,
-// section, end_a | end_b << 8) of elastic sections (12 floats each: E, G, rho, A, Iy, Iz, J, As_y, As_z, damping,
-// axial, joint_k; no plastic limits), then SoftBody::finalize and FemFrame::finalize.
+// triangles of one section (FemFrame::add_shell_section, add_tri; 7 floats: E, nu, rho, t, drill, damping, yield),
+// members (FemFrame::add_section, add_element: a, b, section, end_a | end_b << 8) of sections of 17 floats each (E, G,
+// rho, A, Iy, Iz, J, As_y, As_z, damping, axial, joint_k, Np, Mp, Tp, hardening, half), then SoftBody::finalize and
+// FemFrame::finalize. The body may deform and does not break (SoftBody::allow_break off: no tears, splits or letting
+// go).
 void* blref_frame_new(uint32_t nn, const float* p, const float* mass, const uint8_t* fixed, uint32_t nt,
-                      const uint32_t* tris, const float shell[6], uint32_t nm, const uint32_t* members, uint32_t nsec,
+                      const uint32_t* tris, const float shell[7], uint32_t nm, const uint32_t* members, uint32_t nsec,
                       const float* sections) {
     auto* b = new SoftBody();
+    b->allow_break = false;
     for (uint32_t i = 0; i < nn; i++)
         b->add_node(vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), mass[i], fixed && fixed[i] ? NF_FIXED : NF_NONE);
     if (nt) {
         ShellSection s;
         s.E = shell[0], s.nu = shell[1], s.rho = shell[2], s.t = shell[3], s.drill = shell[4], s.damping = shell[5];
+        s.yield = shell[6];
         const uint16_t si = b->fem.add_shell_section(s);
         for (uint32_t k = 0; k < nt; k++) b->fem.add_tri(tris[3 * k], tris[3 * k + 1], tris[3 * k + 2], si);
     }
     for (uint32_t k = 0; k < nsec; k++) {
-        const float* x = &sections[12 * k];
+        const float* x = &sections[17 * k];
         FrameSection s;
         s.E = x[0], s.G = x[1], s.rho = x[2], s.A = x[3], s.Iy = x[4], s.Iz = x[5], s.J = x[6], s.As_y = x[7], s.As_z = x[8];
         s.damping = x[9], s.axial = x[10], s.joint_k = x[11];
-        s.Np = s.Mp = s.Tp = 0;
+        s.Np = x[12], s.Mp = x[13], s.Tp = x[14], s.hardening = x[15], s.half = x[16];
         b->fem.add_section(s);
     }
     for (uint32_t k = 0; k < nm; k++)
@@ -318,6 +322,34 @@ void blref_frame_add_torques(void* h, const float* t) {
         const int sl = b.fem.slot((uint32_t)i);
         if (sl >= 0) b.fem.torque[sl] += vec3(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
     }
+}
+
+// The plastic state: per triangle 29 floats (FrameTri: X[6], th0[9], r0[12] x y z w, dmg, util), per member 27
+// (FrameElement: up, tp, pa, pb, ba, bb, bt, damage, dmg_a, dmg_b, util, N, qa, qb; three zeros where the oracle puts
+// the private tangent's bits); counters {FemFrame::repasses, passes_}. Any pointer may be null.
+void blref_frame_plastic(void* h, float* tris, float* members, int64_t* counters) {
+    const FemFrame& f = static_cast<SoftBody*>(h)->fem;
+    if (tris)
+        for (size_t k = 0; k < f.tris.size(); k++) {
+            const FrameTri& t = f.tris[k];
+            float* o = &tris[29 * k];
+            for (int i = 0; i < 3; i++) {
+                o[2 * i] = t.X[i][0], o[2 * i + 1] = t.X[i][1];
+                o[6 + 3 * i] = t.th0[i].x, o[7 + 3 * i] = t.th0[i].y, o[8 + 3 * i] = t.th0[i].z;
+                o[15 + 4 * i] = t.r0[i].x, o[16 + 4 * i] = t.r0[i].y, o[17 + 4 * i] = t.r0[i].z, o[18 + 4 * i] = t.r0[i].w;
+            }
+            o[27] = t.dmg, o[28] = t.util;
+        }
+    if (members)
+        for (size_t k = 0; k < f.elems.size(); k++) {
+            const FrameElement& e = f.elems[k];
+            float* o = &members[27 * k];
+            const float v[27] = {e.up, e.tp, e.pa.x, e.pa.y, e.pb.x, e.pb.y, e.ba.x, e.ba.y, e.bb.x, e.bb.y, e.bt, e.damage,
+                                 e.dmg_a, e.dmg_b, e.util, e.N, e.qa.x, e.qa.y, e.qa.z, e.qa.w, e.qb.x, e.qb.y, e.qb.z,
+                                 e.qb.w, 0, 0, 0};
+            std::memcpy(o, v, sizeof(v));
+        }
+    if (counters) counters[0] = f.repasses, counters[1] = f.passes_;
 }
 
 // The plate body's whole substep as World::simulate_island runs a frame body stepping every substep

@@ -199,3 +199,93 @@ def frame_loads(name, nn, steps, seed):
             ld[-1, 1] -= 3000.0
         out.append((ld, tq))
     return out
+
+
+# ---------------------------------------------------------------------------------------------- plasticity
+STEEL_YIELD = 3.5e8  # frame_material("Steel").yield (src/phys/frame_fem.cpp:35)
+
+
+def plastic_tube(outer=0.04, wall=0.002, fy=STEEL_YIELD, hardening=0.25, damping=2e-4):
+    """make_frame_section("Steel", Tube, outer, wall) (src/phys/frame_fem.cpp:95-135) with its plastic limits, in
+    double and stored as floats: Np = fy A, Mp = fy Zp with Zp = (D^3 - d^3) / 6, Tp = 2 pi / 3 tau (R^3 - r^3) with
+    tau = fy / sqrt(3); the hinges' hardening and half the outer size. 17 floats."""
+    base = steel_tube(outer, wall, damping)
+    D = float(outer)
+    t = min(max(float(wall), 1e-4), D * 0.5)
+    d, R = D - 2 * t, D * 0.5
+    r = d * 0.5
+    fy = float(np.float32(fy))
+    A = np.pi / 4 * (D * D - d * d)
+    Zp = (D * D * D - d * d * d) / 6
+    Tp = 2 * np.pi / 3 * (fy / np.sqrt(3.0)) * (R * R * R - r * r * r)
+    return np.concatenate([base, np.array([fy * A, fy * Zp, Tp, hardening, D * 0.5], np.float32)]).astype(np.float32)
+
+
+def plastic_shell(fy=1.0e8):
+    """C3's section (aluminium, 1.5 mm) with a yield stress: 7 floats."""
+    return tuple(SECTION_C3) + (float(fy),)
+
+
+def plastic_plate(n=6, fy=1.0e8):
+    """A clamped n x n plate of plastic_shell(fy) as a frame body (the plate mesher's nodes, triangles and masses)."""
+    from beamlab_amd import fem
+    p, t, fx = fem.plate_mesh(n, n)
+    mass = np.zeros(len(p), np.float32)
+    mpa = np.float32(SECTION_C3[2]) * np.float32(SECTION_C3[3])
+    for tri in t:
+        a, b, c = p[tri[0]], p[tri[1]], p[tri[2]]
+        area = np.float32(0.5) * np.float32(np.sqrt(np.float32(np.dot(np.cross(b - a, c - a), np.cross(b - a, c - a)))))
+        for v in tri:
+            mass[v] = mass[v] + (mpa * area) / np.float32(3.0)
+    return dict(p=p, mass=mass, fixed=fx, tris=t, shell=plastic_shell(fy))
+
+
+def plastic_frames():
+    """Frame bodies whose members and triangles yield under plastic_loads."""
+    sec = plastic_tube()
+    stiff = stiffened_plate(6, plastic_tube(outer=0.02, wall=0.001))
+    stiff["shell"] = plastic_shell()
+    return {
+        "cantilever": frame_line(10, (0, 1, 0), (1, 1, 0), sec),
+        "bar": frame_line(6, (0, 1, 0), (0.6, 1, 0), sec),
+        "portal": portal(sec),
+        "long": frame_line(70, (0, 1, 0), (1.4, 1, 0), plastic_tube(outer=0.03, wall=0.002)),
+        "plate": plastic_plate(6),
+        "stiffened_plate": stiff,
+    }
+
+
+def make_plastic(mod, fr, ref=False):
+    cls = mod.RefPlate if ref else mod.FemPlate
+    return cls.frame(fr["p"], fr["mass"], fr.get("fixed"), fr.get("tris"), fr.get("shell", SECTION_C3),
+                     fr.get("members"), fr.get("sections"))
+
+
+def plastic_loads(name, nn, steps, seed):
+    """Loads past the sections' limits, then reversed (the hinges unload: passes again): on the cantilever a tip load
+    of about 1.5 Mp / L down then up and a tip torque past Tp; on the bar a pull past Np then a push; on the portal a
+    sway load past its collapse load (4 Mp / H); on the long chain a tip load alternating every other step (more
+    passes again than the budget holds); on the plates a load at the middle node past their collapse. Random loads and
+    moments of 20 N and 2 N m on every node."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(steps):
+        ld = rng.normal(0, 20, (nn, 3)).astype(np.float32)
+        tq = rng.normal(0, 2, (nn, 3)).astype(np.float32)
+        half = s < steps // 2
+        if name == "cantilever":
+            ld[-1, 1] += -1500.0 if half else 1500.0
+            if steps // 6 <= s < steps // 3:
+                tq[-1, 0] += 1300.0
+        elif name == "bar":
+            ld[-1, 0] += 1.0e5 if half else -1.2e5
+        elif name == "portal":
+            ld[2, 0] += 6000.0 if half else -6000.0
+        elif name == "long":
+            ld[-1, 1] += -1500.0 if s < steps // 3 else (3000.0 if (s // 2) % 2 == 0 else -3000.0)
+        elif name == "plate":
+            ld[nn // 2, 1] += -2500.0 if half else 1500.0
+        elif name == "stiffened_plate":
+            ld[nn // 2, 1] += -6000.0 if half else 4000.0
+        out.append((ld, tq))
+    return out

@@ -14,6 +14,11 @@ Outputs (data only: inputs are regenerated from the seeds, outputs are the refer
                         every 100 steps (nodes' p and v, xd, w, q, forces, counters)
     fem_frame_ref.npz   frame bodies with members (tests/test_fem_oracle.py's frame cases: cantilevers on every joint,
                         portals, a stiffened plate) under seeded loads and moments: the states after 10, 20, 30 steps
+    fem_plastic_ref.npz frame bodies that yield (tests/_fem_cases.py's plastic frames: members in tension, torsion and
+                        at their hinges, a chain that spends the passes-again budget, plates whose shells flow) under
+                        loads past their limits: the states and the plastic state after 20, 40, 60 steps
+
+`python tests/golden/make_golden.py plastic` writes fem_plastic_ref.npz alone.
 """
 import hashlib
 import os
@@ -93,7 +98,33 @@ def main():
         R.close()
     np.savez_compressed(os.path.join(HERE, "fem_frame_ref.npz"), **out)
     print("written fem_frame_ref.npz")
+    plastic()
+
+
+PLASTIC_KEYS = ("X", "th0", "r0", "tdmg", "tutil", "mstate", "qa", "qb")
+
+
+def plastic():
+    out = {}
+    for name, fr in sorted(fc.plastic_frames().items()):
+        R = fc.make_plastic(oracle, fr, ref=True)
+        for s, (ld, tq) in enumerate(fc.plastic_loads(name, R.nn, 60, 5)):
+            R.add_torques(tq)
+            R.substep(ld, fc.H)
+            if s % 20 == 19:
+                st, pl = R.read(), R.plastic()
+                for k in fc.STATE_KEYS:
+                    out[f"{name}_{s}_{k}"] = st[k]
+                for k in PLASTIC_KEYS:
+                    out[f"{name}_{s}_{k}"] = pl[k]
+                out[f"{name}_{s}_counters"] = np.array([st["failures"], st["clamps"], pl["repasses"], pl["passes"]])
+        R.close()
+    np.savez_compressed(os.path.join(HERE, "fem_plastic_ref.npz"), **out)
+    print("written fem_plastic_ref.npz")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["plastic"]:
+        plastic()
+    else:
+        main()

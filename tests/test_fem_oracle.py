@@ -314,3 +314,110 @@ def test_frame_substeps_match_golden():
                 st = P.read()
                 for k in fc.STATE_KEYS:
                     assert st[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
+
+
+# ---------------------------------------------------------------------------------------------- plasticity
+def _same_plastic(a, b, what):
+    for k in ("X", "th0", "r0", "tdmg", "tutil", "mstate", "qa", "qb"):
+        assert a[k].tobytes() == b[k].tobytes(), f"{what}: {k} differs from the reference"
+    assert (a["repasses"], a["passes"]) == (b["repasses"], b["passes"]), f"{what}: repasses, passes"
+
+
+@pytest.mark.parametrize("name", sorted(fc.plastic_frames()))
+def test_plastic_frames_match_reference_bitwise(ref_mod, name):
+    """Plasticity against the compiled reference's FemFrame, bit for bit over 60 substeps (the body's allow_break off):
+    members yielding in tension, compression, torsion and at their hinges (radial return, hardening, the plastic
+    rotations baked into the rest frames, the damage), the plastic tangent and the unloading passes again with their
+    budget (the 71-node chain spends it); shell triangles whose membrane flows (the rest shape, the plastic stretch,
+    the stiffness rebuilt) and whose bending yields (the corners' plastic rotations, baked). The nodes' state, the
+    plastic state and the repasses / passes counters."""
+    fr = fc.plastic_frames()[name]
+    P, R = fc.make_plastic(oracle, fr), fc.make_plastic(oracle, fr, ref=True)
+    for s, (ld, tq) in enumerate(fc.plastic_loads(name, P.nn, 60, 5)):
+        P.add_torques(tq), R.add_torques(tq)
+        R.substep(ld, fc.H)
+        P.step(ld, fc.H)
+        _same(R.read(), P.read(), f"{name} step {s}")
+        _same_plastic(R.plastic(), P.plastic(), f"{name} step {s}")
+    pl = P.plastic()
+    if name in ("cantilever", "bar", "portal", "long"):
+        assert pl["passes"] > 60 and np.abs(pl["mstate"][:, 6:10]).max() + np.abs(pl["mstate"][:, 2:6]).max() > 0
+    if name == "long":
+        assert pl["repasses"] > 8
+    if name in ("plate", "stiffened_plate"):
+        assert pl["tdmg"].max() > 0
+
+
+def _plastic_cantilever(ratio, steps=6000):
+    """A 1 m cantilever of 10 plastic steel tubes (no hardening), its tip loaded by ratio Mp / L across it, ramped over
+    the first half of the steps; quasi-static (theta 1, dissipation 1, no gravity)."""
+    sec = fc.plastic_tube(hardening=0.0)
+    fr = fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec)
+    P = fc.make_plastic(oracle, fr)
+    F = ratio * float(sec[13])
+    ld = np.zeros((P.nn, 3), np.float32)
+    for s in range(steps):
+        ld[-1, 1] = -F * min(1.0, (s + 1) / (steps // 2))
+        P.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    return sec, F, P.read(), P.plastic()
+
+
+def test_plastic_cantilever_collapse_closed_forms():
+    """The plastic collapse load of a cantilever is Mp / L. Below it (0.9 Mp / L) no hinge turns plastic and the tip
+    deflects PL^3/3EI + PL/kGA (1 %); above it (1.2 and 1.5 Mp / L) the root's hinge turns (a mechanism: the tip drops
+    by more than a third of the span) and no other; at rest, in both, the root member's moment (its utilisation times
+    Mp) equals the load times the tip's lever arm (0.5 %) and stays within Mp."""
+    sec, F, st, pl = _plastic_cantilever(0.9)
+    EI, _, _, kGA = _tube_props(sec)
+    want = F / (3 * EI) + F / kGA
+    assert abs(-(st["xd"][-1][1] - 1.0) / want - 1) < 0.01
+    assert np.abs(pl["mstate"][:, 2:10]).max() == 0 and pl["mstate"][:, 14].max() < 1
+    assert abs(pl["mstate"][0, 14] * sec[13] / (F * st["xd"][-1][0]) - 1) < 0.005
+    for ratio in (1.2, 1.5):
+        sec, F, st, pl = _plastic_cantilever(ratio)
+        rot = np.hypot(*(pl["mstate"][:, 2:4] + pl["mstate"][:, 6:8]).T)
+        assert rot[0] > 0.5 and np.abs(pl["mstate"][1:, 2:10]).max() == 0, rot
+        assert -(st["xd"][-1][1] - 1.0) > 1.0 / 3
+        x_tip = st["xd"][-1][0]
+        assert abs(pl["mstate"][0, 14] * sec[13] / (F * x_tip) - 1) < 0.005 and pl["mstate"][0, 14] <= 1.0
+
+
+def test_plastic_bar_closed_forms():
+    """A 0.6 m bar of 6 plastic tubes pulled at its tip: at 0.9 Np it stretches FL/EA (1 %) and stays elastic; at 1.1
+    Np every member carries exactly Np and stretches plastically (its elongation and damage grow)."""
+    sec = fc.plastic_tube(hardening=0.0)
+    Np, EA = float(sec[12]), float(sec[0]) * float(sec[3])
+    for ratio in (0.9, 1.1):
+        fr = fc.frame_line(6, (0, 1, 0), (0.6, 1, 0), sec)
+        P = fc.make_plastic(oracle, fr)
+        x0 = P.read()["xd"][-1][0]
+        ld = np.zeros((P.nn, 3), np.float32)
+        ld[-1, 0] = ratio * Np
+        for s in range(400 if ratio > 1 else 3000):
+            P.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+        st, pl = P.read(), P.plastic()
+        if ratio < 1:
+            assert abs((st["xd"][-1][0] - x0) / (ratio * Np * 0.6 / EA) - 1) < 0.01
+            assert pl["mstate"][:, 0].max() == 0
+        else:
+            assert np.all(pl["mstate"][:, 15] == np.float32(Np)) and pl["mstate"][:, 0].min() > 0
+            assert np.all(pl["mstate"][:, 11] > 0)
+
+
+def test_plastic_frames_match_golden():
+    """The oracle's plastic frames against the reference's recorded states (tests/golden/fem_plastic_ref.npz), bit for
+    bit: runs where the reference is absent."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_plastic_ref.npz"))
+    for name, fr in sorted(fc.plastic_frames().items()):
+        P = fc.make_plastic(oracle, fr)
+        for s, (ld, tq) in enumerate(fc.plastic_loads(name, P.nn, 60, 5)):
+            P.add_torques(tq)
+            P.step(ld, fc.H)
+            if s % 20 == 19:
+                st, pl = P.read(), P.plastic()
+                for k in fc.STATE_KEYS:
+                    assert st[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
+                for k in ("X", "th0", "r0", "tdmg", "tutil", "mstate", "qa", "qb"):
+                    assert pl[k].tobytes() == g[f"{name}_{s}_{k}"].tobytes(), f"{name} step {s}: {k}"
+                assert [st["failures"], st["clamps"], pl["repasses"], pl["passes"]] == list(g[f"{name}_{s}_counters"])
