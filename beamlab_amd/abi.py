@@ -83,12 +83,16 @@ class Tuning(ctypes.Structure):
 
 
 class ShellSection(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_float) for n in ("E", "nu", "rho", "t", "drill", "damping")]
+    _fields_ = [(n, ctypes.c_float) for n in ("E", "nu", "rho", "t", "drill", "damping", "yield_")]
 
 
 class FrameSection(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("E", "G", "rho", "A", "Iy", "Iz", "J", "As_y", "As_z", "damping",
-                                               "axial", "joint_k")]
+                                               "axial", "joint_k", "Np", "Mp", "Tp", "hardening", "half")]
+
+
+FRAME_SECTION_FIELDS = 17
+FEM_TRI_STATE, FEM_MEM_STATE = 29, 27
 
 
 FRAME_MEMBER = np.dtype([("a", np.uint32), ("b", np.uint32), ("section", np.uint16), ("end_a", np.uint8),
@@ -105,7 +109,8 @@ class FemDesc(ctypes.Structure):
 class FemReport(ctypes.Structure):
     _fields_ = [("residual", ctypes.c_double), ("factor_flops", ctypes.c_double)] + \
                [(n, ctypes.c_float) for n in ("assemble_ms", "factor_ms", "solve_ms", "update_ms")] + \
-               [(n, ctypes.c_int32) for n in ("failures", "supernodes", "fronts_max")] + [("factor_nnz", ctypes.c_int64)]
+               [(n, ctypes.c_int32) for n in ("failures", "supernodes", "fronts_max")] + [("factor_nnz", ctypes.c_int64)] + \
+               [(n, ctypes.c_int32) for n in ("passes", "repasses")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -169,6 +174,7 @@ SIGNATURES = {
     "bl_fem_last_system": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),
     "bl_fem_tri_k": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "bl_fem_get_plastic": (ctypes.c_int, [_P, _P, _P, _P]),
     "bl_gnn_param_count": (ctypes.c_int, []),
     "bl_gnn_create": (ctypes.c_int, [ctypes.POINTER(_P), _P, ctypes.c_uint32]),
     "bl_gnn_destroy": (None, [_P]),

@@ -144,7 +144,7 @@ __device__ __forceinline__ void tri_grad(const float* X, double b[3], double c[3
 }
 
 struct Section {
-    float E, nu, rho, t, drill, damping;
+    float E, nu, rho, t, drill, damping, yield;
 };
 
 // tri_local_k (src/phys/frame_fem.cpp:1524-1579): K (18 x 18, row-major) of the rest shape X
@@ -216,8 +216,8 @@ struct FemArgs {
     uint32_t nm;
     const uint4* mem;
     const float* mL0;
-    const float4* mqa;
-    const float4* mqb;
+    float4* mqa;
+    float4* mqb;
     const float* mst;
     const float* mtw;
     const float* mdamp;
@@ -226,12 +226,29 @@ struct FemArgs {
     float4* me1;
     double* mW;
     double* mkv;
+    // members' plasticity: the sections (the 17 floats of bl_frame_section; mem.z the member's), the plastic state
+    // (16 per member: up, tp, pa, pb, ba, bb, bt, damage, dmg_a, dmg_b, util, N), the tangent's kinematic part (17
+    // per member: e1, e2, e3, L, ngeo, ma, mb, N, T) and its bits (yield | unload << 8 | elastic_hold << 16), whether
+    // this step starts every yielding member from its elastic tangent (the passes-again budget spent)
+    const float* msec;
+    float* mps;
+    float* mtan;
+    uint32_t* mbits;
+    int elastic_step;
+    uint32_t* nyield;   // [0] members yielding at the step's start, [1] members a pass unloads
     const uint8_t* fixed;
     const uint4* tri;   // corners, 0
-    const float* X;     // 6 per triangle
-    const float4* r0;   // 3 per triangle
+    float* X;           // 6 per triangle: the rest shape (the plastic one)
+    float4* r0;         // 3 per triangle
     float* Kf;          // 162 per triangle
     double* Kd;         // 324 per triangle (kept for export)
+    // triangles' plasticity: the authored rest shape (6), the corners' plastic rest rotations (9), the plastic
+    // stretch and utilisation, the bending's curvature rows at the three points (54)
+    const float* X0;
+    float* th0;
+    float* tdmg;
+    float* tutil;
+    float* Bc;
     // element pass outputs
     float* te;          // 9 per triangle: e1 e2 e3
     float* tfo;         // 9: the corners' forces
@@ -255,14 +272,17 @@ struct FemArgs {
     const float* loads;  // 3 per node
     float g[3];
     float h, theta, dissipation;
+    // a pass again: the held impulses the first pass used (3 per node), the pass, the members' components and the
+    // components a pass unloaded
+    float* jsave;
+    int pass;
+    const uint32_t* mcomp;
+    uint32_t* again;
 };
 
-// FemFrame::tri_build_k on the device: the element stiffness (FP64) and its float copy
-__global__ void k_tri_k(FemArgs a) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nt) return;
-    double* K = a.Kd + (size_t)t * 324;
-    tri_local_k(a.X + 6 * (size_t)t, a.sec, K);
+// tri_build_k's stored parts (src/phys/frame_fem.cpp:1628-1646) from the FP64 stiffness K of the rest shape: the float
+// membrane and plate 9 x 9 and, for a section that yields, the bending's curvature rows at the three points
+__device__ void tri_store_k(const FemArgs& a, uint32_t t, const double* K) {
     float* P = a.Kf + (size_t)t * 162;
     for (int i = 0; i < 9; i++)
         for (int j = 0; j < 9; j++) {
@@ -271,6 +291,26 @@ __global__ void k_tri_k(FemArgs a) {
             P[i * 9 + j] = (float)K[(im < jm ? im : jm) * 18 + (im < jm ? jm : im)];
             P[81 + i * 9 + j] = (float)K[(ip < jp ? ip : jp) * 18 + (ip < jp ? jp : ip)];
         }
+    if (!(a.sec.yield > 0)) return;
+    const float* X = a.X + 6 * (size_t)t;
+    const double x[3] = {X[0], X[2], X[4]}, y[3] = {X[1], X[3], X[5]};
+    const double gauss[3][2] = {{1.0 / 6, 1.0 / 6}, {2.0 / 3, 1.0 / 6}, {1.0 / 6, 2.0 / 3}};
+    float* Bc = a.Bc + 54 * (size_t)t;
+    for (int g = 0; g < 3; g++) {
+        double B[3][9];
+        dkt_b(x, y, gauss[g][0], gauss[g][1], B);
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < 3; i++) Bc[g * 18 + r * 6 + 2 * i] = (float)B[r][3 * i + 1], Bc[g * 18 + r * 6 + 2 * i + 1] = (float)B[r][3 * i + 2];
+    }
+}
+
+// FemFrame::tri_build_k on the device: the element stiffness (FP64) and its float copy
+__global__ void k_tri_k(FemArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nt) return;
+    double* K = a.Kd + (size_t)t * 324;
+    tri_local_k(a.X + 6 * (size_t)t, a.sec, K);
+    tri_store_k(a, t, K);
 }
 
 // ------------------------------------------------------------------------------------------ members
@@ -312,10 +352,96 @@ __device__ __forceinline__ uint32_t kidx(int r, int c) {
     return r < 6 ? (c < 6 ? r * 6 + c : 108 + r * 6 + (c - 6)) : (c < 6 ? 72 + (r - 6) * 6 + c : 36 + (r - 6) * 6 + (c - 6));
 }
 
-// eval_forces for elastic members (src/phys/frame_fem.cpp:1896-2063 with no plastic limits): kinematics (:1384-1405)
-// - the chord, the co-rotated frame (the ends' frames, their mean turned onto the chord), the end rotations against
-// it -, the axial force, torque and end moments, the ends' force and moments; the tangent, element_matrix and
-// member_world_k (:1834-1850: the blocks and K u at the ends' velocities and spins). One thread per member.
+// condense (src/phys/frame_fem.cpp:1414-1418): a pinned end's bending stiffness condensed out
+__device__ void mem_condense(float* c, bool pa, bool pb) {
+    if (pa && pb) c[0] = c[1] = c[2] = 0;
+    else if (pa) c[2] = c[0] > 0 ? c[2] - c[1] * c[1] / c[0] : 0, c[0] = c[1] = 0;
+    else if (pb) c[0] = c[2] > 0 ? c[0] - c[1] * c[1] / c[2] : 0, c[1] = c[2] = 0;
+}
+
+// member_world_k (:1834-1850): element_matrix (:2157-2186) at the member's effective tangent (effective, :2147-2155:
+// a flowing hinge turns like a pin, a flowing bar or twist keeps a hundredth of its stiffness unless the step unloads
+// it), written to its blocks aa, bb, ba, ab as the reference's loop sets K, the lower triangle from the upper one;
+// and K u at the ends' velocities and spins
+__device__ void member_world_k(const FemArgs& a, uint32_t m) {
+    const float* st = a.mst + 8 * (size_t)m;
+    const float* tn = a.mtan + 17 * (size_t)m;
+    const uint32_t bits = a.mbits[m];
+    const uint32_t y = (bits & 0xffu) & ~((bits >> 8) & 0xffu);
+    float ka = st[0], kt = st[1];
+    float cy[3] = {st[2], st[3], st[4]}, cz[3] = {st[5], st[6], st[7]};
+    if (y & 3) mem_condense(cy, (y & 1) != 0, (y & 2) != 0), mem_condense(cz, (y & 1) != 0, (y & 2) != 0);
+    if (y & 4) ka *= 0.01f;
+    if (y & 8) kt *= 0.01f;
+    double* W = a.mW + 144 * (size_t)m;
+    const double iL = 1.0 / tn[9];
+    const double E1[3] = {tn[0], tn[1], tn[2]}, E2[3] = {tn[3], tn[4], tn[5]}, E3[3] = {tn[6], tn[7], tn[8]};
+    const double g[3] = {E3[0] * iL, E3[1] * iL, E3[2] * iL}, h[3] = {E2[0] * iL, E2[1] * iL, E2[2] * iL};
+    const double c0 = cy[0], c1 = cy[1], c2 = cy[2], d0 = cz[0], d1 = cz[1], d2 = cz[2];
+    const double Syy = c0 + 2 * c1 + c2, Szz = d0 + 2 * d1 + d2, ay = c0 + c1, by = c1 + c2, az = d0 + d1, bz = d1 + d2;
+    const double kad = ka, ktd = kt;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const double uu = kad * E1[i] * E1[j] + Syy * g[i] * g[j] + Szz * h[i] * h[j];
+            const double up = -ay * g[i] * E2[j] + az * h[i] * E3[j];
+            const double ub = -by * g[i] * E2[j] + bz * h[i] * E3[j];
+            const double tt = ktd * E1[i] * E1[j];
+            const double pa = tt + c0 * E2[i] * E2[j] + d0 * E3[i] * E3[j];
+            const double pb = -tt + c1 * E2[i] * E2[j] + d1 * E3[i] * E3[j];
+            const double qb2 = tt + c2 * E2[i] * E2[j] + d2 * E3[i] * E3[j];
+            W[kidx(i, j)] = uu, W[kidx(i, 3 + j)] = up, W[kidx(i, 6 + j)] = -uu, W[kidx(i, 9 + j)] = ub;
+            W[kidx(3 + i, 3 + j)] = pa, W[kidx(3 + i, 9 + j)] = pb;
+            W[kidx(6 + i, 6 + j)] = uu, W[kidx(6 + i, 9 + j)] = -ub;
+            W[kidx(9 + i, 9 + j)] = qb2;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) W[kidx(3 + i, 6 + j)] = -W[kidx(j, 3 + i)];
+    for (int r = 0; r < 12; r++)
+        for (int c = r + 1; c < 12; c++) W[kidx(c, r)] = W[kidx(r, c)];
+    const uint4 me = a.mem[m];
+    const uint32_t na = me.x, nb = me.y;
+    const double u[12] = {a.v[3 * na], a.v[3 * na + 1], a.v[3 * na + 2], a.w[3 * na], a.w[3 * na + 1], a.w[3 * na + 2],
+                          a.v[3 * nb], a.v[3 * nb + 1], a.v[3 * nb + 2], a.w[3 * nb], a.w[3 * nb + 1], a.w[3 * nb + 2]};
+    double* kv = a.mkv + 12 * (size_t)m;
+    for (int r = 0; r < 12; r++) {
+        double s2 = 0;
+        for (int c = 0; c < 12; c++) s2 += W[kidx(r, c)] * u[c];
+        kv[r] = s2;
+    }
+}
+
+// (std::max / std::min as the reference calls them: the first argument unless the second compares greater / less)
+__device__ __forceinline__ float smax(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ float smin(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float len2f(float x, float y) { return sqrtf(x * x + y * y); }
+
+// the end moments at the end rotations less the hinges' plastic rotations (:1946-1950)
+__device__ void mem_moments(f3 ra, f3 rb, const float* st, const float* ps, float* Ma, float* Mb) {
+    const float dax = ra.y - ps[2], day = ra.z - ps[3], dbx = rb.y - ps[4], dby = rb.z - ps[5];
+    Ma[0] = st[2] * dax + st[3] * dbx, Ma[1] = st[5] * day + st[6] * dby;
+    Mb[0] = st[3] * dax + st[4] * dbx, Mb[1] = st[6] * day + st[7] * dby;
+}
+
+// a hinge's radial return (:1960-1972): the yield moment hardening with the hinge's largest rotation
+__device__ bool mem_yield_end(const float* sc, const float* M, float cyy, float czz, float* p, const float* base, float* dmg,
+                              float* flow) {
+    const float Mp = sc[13] * (1.0f + sc[15] * smin(1.0f, *dmg));
+    const float l = len2f(M[0], M[1]);
+    if (l <= Mp) return false;
+    const float r = 1.0f - Mp / l;
+    const float dp0 = cyy > 0 ? M[0] * r / cyy : 0.0f, dp1 = czz > 0 ? M[1] * r / czz : 0.0f;
+    p[0] += dp0, p[1] += dp1;
+    *flow += len2f(dp0, dp1);
+    *dmg = smax(*dmg, len2f(p[0] + base[0], p[1] + base[1]));
+    return true;
+}
+
+// eval_forces (src/phys/frame_fem.cpp:1896-2069, the body's allow_break off, no joint damping), one thread per member:
+// kinematics (:1384-1405) - the chord, the co-rotated frame (the ends' frames, their mean turned onto the chord), the
+// end rotations against it -; the axial force (yielding in tension at Np, in compression at the lower of Np and the
+// Euler load), the torque (at Tp), the end moments (each hinge's radial return, twice); the ends' force and moments;
+// the plastic rotations past 0.02 rad baked into the rest frames; the tangent (its yield bits where it flows this
+// step; elastic while held or with the budget spent) and member_world_k. The plastic state is the thread's own.
 __global__ void k_members(FemArgs a) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.nm) return;
@@ -329,9 +455,11 @@ __global__ void k_members(FemArgs a) {
     const double L = sqrt(dx * dx + dy * dy + dz * dz);
     if (!(L > 1e-6) || !__builtin_isfinite(L)) return;
     const float Lf = (float)L;
-    const double dL = L - (double)a.mL0[m];
+    const float L0 = a.mL0[m];
+    const double dL = L - (double)L0;
     const f3 e1 = mk((float)(dx / L), (float)(dy / L), (float)(dz / L));
-    const float4 qna = a.q[na], qnb = a.q[nb], qa = a.mqa[m], qb = a.mqb[m];
+    const float4 qna = a.q[na], qnb = a.q[nb];
+    float4 qa = a.mqa[m], qb = a.mqb[m];
     const qf Ea = qnorm(qmul(Q(qna.x, qna.y, qna.z, qna.w), Q(qa.x, qa.y, qa.z, qa.w)));
     const qf Eb = qnorm(qmul(Q(qnb.x, qnb.y, qnb.z, qnb.w), Q(qb.x, qb.y, qb.z, qb.w)));
     const float t = a.mtw[m];
@@ -341,59 +469,141 @@ __global__ void k_members(FemArgs a) {
     const f3 ra = qlog(qmul(Ec, Ea)), rb = qlog(qmul(Ec, Eb));
     const f3 e2 = qrot(E, mk(0, 1, 0)), e3 = qrot(E, mk(0, 0, 1));
     const float* st = a.mst + 8 * (size_t)m;
-    const float ka = st[0], kt = st[1], cy0 = st[2], cy1 = st[3], cy2 = st[4], cz0 = st[5], cz1 = st[6], cz2 = st[7];
+    const float* sc = a.msec + 17 * (size_t)me.z;
+    const float ka = st[0], kt = st[1];
+    float ps[16];
+    for (int k = 0; k < 16; k++) ps[k] = a.mps[16 * (size_t)m + k];
+    const bool plastic = sc[12] > 0;
     const float delta = (float)dL;
-    const float N = ka * (delta - 0.0f); // (no plastic elongation, twist or hinge rotations)
+    float N = ka * (delta - ps[0]);
+    float Nt = sc[12], Nc = sc[12];
+    float flow_ax = 0, flow_t = 0;
+    if (plastic) {
+        const float Le = 0.7f * L0, kPi = 3.14159265358979323846f;
+        const float eul = kPi * kPi * sc[0] * smin(sc[4], sc[5]) / (Le * Le);
+        Nc = smin(sc[12], eul);
+        if (N > Nt || N < -Nc) {
+            const float lim = N > 0 ? Nt : -Nc;
+            const float dup = (N - lim) / ka;
+            ps[0] += dup;
+            flow_ax = fabsf(dup) / L0;
+            ps[11] = smax(ps[11], ps[0] / L0);
+            N = lim;
+        }
+    }
     const float tw = rb.x - ra.x;
-    const float T = kt * (tw - 0.0f);
-    const float dax = ra.y - 0.0f, day = ra.z - 0.0f, dbx = rb.y - 0.0f, dby = rb.z - 0.0f;
-    const float Max = cy0 * dax + cy1 * dbx, May = cz0 * day + cz1 * dby;
-    const float Mbx = cy1 * dax + cy2 * dbx, Mby = cz1 * day + cz2 * dby;
-    const float Sy = Max + Mbx, Sz = May + Mby;
+    float T = kt * (tw - ps[1]);
+    if (plastic && sc[14] > 0 && kt > 0 && fabsf(T) > sc[14]) {
+        const float lim = T > 0 ? sc[14] : -sc[14];
+        const float dtp = (T - lim) / kt;
+        ps[1] += dtp;
+        flow_t = fabsf(dtp);
+        ps[11] = smax(ps[11], fabsf(ps[1] + ps[10]) * sc[16] / L0);
+        T = lim;
+    }
+    float Ma[2], Mb[2];
+    mem_moments(ra, rb, st, ps, Ma, Mb);
+    float flow_a = 0, flow_b = 0;
+    if (plastic && sc[13] > 0)
+        for (int it = 0; it < 2; it++) {
+            if (mem_yield_end(sc, Ma, st[2], st[5], &ps[2], &ps[6], &ps[12], &flow_a)) mem_moments(ra, rb, st, ps, Ma, Mb);
+            if (mem_yield_end(sc, Mb, st[4], st[7], &ps[4], &ps[8], &ps[13], &flow_b)) mem_moments(ra, rb, st, ps, Ma, Mb);
+        }
+    float util = 0;
+    if (sc[12] > 0) util = smax(util, N > 0 ? N / Nt : -N / Nc);
+    if (sc[13] > 0) {
+        const float ua = len2f(Ma[0], Ma[1]) / (1.0f + sc[15] * smin(1.0f, ps[12]));
+        const float ub = len2f(Mb[0], Mb[1]) / (1.0f + sc[15] * smin(1.0f, ps[13]));
+        util = smax(util, smax(ua, ub) / sc[13]);
+    }
+    if (sc[14] > 0) util = smax(util, fabsf(T) / sc[14]);
+    ps[14] = util, ps[15] = N;
+    const float Sy = Ma[0] + Mb[0], Sz = Ma[1] + Mb[1];
     const f3 fa = blg::sub(blg::add(blg::mul(e1, N), blg::mul(e3, Sy / Lf)), blg::mul(e2, Sz / Lf));
-    const f3 ta = blg::sub(blg::sub(blg::mul(e1, T), blg::mul(e2, Max)), blg::mul(e3, May));
-    const f3 tb = blg::sub(blg::sub(blg::mul(e1, -T), blg::mul(e2, Mbx)), blg::mul(e3, Mby));
+    const f3 ta = blg::sub(blg::sub(blg::mul(e1, T), blg::mul(e2, Ma[0])), blg::mul(e3, Ma[1]));
+    const f3 tb = blg::sub(blg::sub(blg::mul(e1, -T), blg::mul(e2, Mb[0])), blg::mul(e3, Mb[1]));
     float* fo = a.mfo + 9 * (size_t)m;
     fo[0] = fa.x, fo[1] = fa.y, fo[2] = fa.z, fo[3] = ta.x, fo[4] = ta.y, fo[5] = ta.z, fo[6] = tb.x, fo[7] = tb.y, fo[8] = tb.z;
-    a.me1[m] = make_float4(e1.x, e1.y, e1.z, N > 0 ? N / Lf : 0.0f);
-    // element_matrix at the tangent: written to its blocks as the reference's loop sets K, then the lower triangle
-    // taken from the upper one
-    double* W = a.mW + 144 * (size_t)m;
-    {
-        const double iL = 1.0 / Lf;
-        const double E1[3] = {e1.x, e1.y, e1.z}, E2[3] = {e2.x, e2.y, e2.z}, E3[3] = {e3.x, e3.y, e3.z};
-        const double g[3] = {E3[0] * iL, E3[1] * iL, E3[2] * iL}, h[3] = {E2[0] * iL, E2[1] * iL, E2[2] * iL};
-        const double c0 = cy0, c1 = cy1, c2 = cy2, d0 = cz0, d1 = cz1, d2 = cz2;
-        const double Syy = c0 + 2 * c1 + c2, Szz = d0 + 2 * d1 + d2, ay = c0 + c1, by = c1 + c2, az = d0 + d1, bz = d1 + d2;
-        const double kad = ka, ktd = kt;
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                const double uu = kad * E1[i] * E1[j] + Syy * g[i] * g[j] + Szz * h[i] * h[j];
-                const double up = -ay * g[i] * E2[j] + az * h[i] * E3[j];
-                const double ub = -by * g[i] * E2[j] + bz * h[i] * E3[j];
-                const double tt = ktd * E1[i] * E1[j];
-                const double pa = tt + c0 * E2[i] * E2[j] + d0 * E3[i] * E3[j];
-                const double pb = -tt + c1 * E2[i] * E2[j] + d1 * E3[i] * E3[j];
-                const double qb2 = tt + c2 * E2[i] * E2[j] + d2 * E3[i] * E3[j];
-                W[kidx(i, j)] = uu, W[kidx(i, 3 + j)] = up, W[kidx(i, 6 + j)] = -uu, W[kidx(i, 9 + j)] = ub;
-                W[kidx(3 + i, 3 + j)] = pa, W[kidx(3 + i, 9 + j)] = pb;
-                W[kidx(6 + i, 6 + j)] = uu, W[kidx(6 + i, 9 + j)] = -ub;
-                W[kidx(9 + i, 9 + j)] = qb2;
-            }
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) W[kidx(3 + i, 6 + j)] = -W[kidx(j, 3 + i)];
-        for (int r = 0; r < 12; r++)
-            for (int c = r + 1; c < 12; c++) W[kidx(c, r)] = W[kidx(r, c)];
+    // the plastic rotations into the rest frames (:2037-2045)
+    const float kBake = 0.02f;
+    if (ps[2] * ps[2] + ps[3] * ps[3] > kBake * kBake) {
+        const qf q2 = qnorm(qmul(Q(qa.x, qa.y, qa.z, qa.w), qexp(mk(0, -ps[2], -ps[3]))));
+        qa = make_float4(q2.x, q2.y, q2.z, q2.w);
+        ps[6] += ps[2], ps[7] += ps[3], ps[2] = ps[3] = 0;
     }
-    const double u[12] = {a.v[3 * na], a.v[3 * na + 1], a.v[3 * na + 2], a.w[3 * na], a.w[3 * na + 1], a.w[3 * na + 2],
-                          a.v[3 * nb], a.v[3 * nb + 1], a.v[3 * nb + 2], a.w[3 * nb], a.w[3 * nb + 1], a.w[3 * nb + 2]};
-    double* kv = a.mkv + 12 * (size_t)m;
-    for (int r = 0; r < 12; r++) {
-        double s2 = 0;
-        for (int c = 0; c < 12; c++) s2 += W[kidx(r, c)] * u[c];
-        kv[r] = s2;
+    if (ps[4] * ps[4] + ps[5] * ps[5] > kBake * kBake) {
+        const qf q2 = qnorm(qmul(Q(qb.x, qb.y, qb.z, qb.w), qexp(mk(0, -ps[4], -ps[5]))));
+        qb = make_float4(q2.x, q2.y, q2.z, q2.w);
+        ps[8] += ps[4], ps[9] += ps[5], ps[4] = ps[5] = 0;
     }
+    if (fabsf(ps[1]) > kBake) {
+        const qf q1 = qnorm(qmul(Q(qa.x, qa.y, qa.z, qa.w), qexp(mk(0.5f * ps[1], 0, 0))));
+        const qf q2 = qnorm(qmul(Q(qb.x, qb.y, qb.z, qb.w), qexp(mk(-0.5f * ps[1], 0, 0))));
+        qa = make_float4(q1.x, q1.y, q1.z, q1.w), qb = make_float4(q2.x, q2.y, q2.z, q2.w);
+        ps[10] += ps[1], ps[1] = 0;
+    }
+    if (plastic) {
+        a.mqa[m] = qa, a.mqb[m] = qb;
+        for (int k = 0; k < 16; k++) a.mps[16 * (size_t)m + k] = ps[k];
+    } else {
+        a.mps[16 * (size_t)m + 14] = util, a.mps[16 * (size_t)m + 15] = N;
+    }
+    // the tangent (:2051-2068)
+    const float ngeo = N > 0 ? N / Lf : 0.0f;
+    a.me1[m] = make_float4(e1.x, e1.y, e1.z, ngeo);
+    float* tn = a.mtan + 17 * (size_t)m;
+    tn[0] = e1.x, tn[1] = e1.y, tn[2] = e1.z, tn[3] = e2.x, tn[4] = e2.y, tn[5] = e2.z, tn[6] = e3.x, tn[7] = e3.y, tn[8] = e3.z;
+    tn[9] = Lf, tn[10] = ngeo, tn[11] = Ma[0], tn[12] = Ma[1], tn[13] = Mb[0], tn[14] = Mb[1], tn[15] = N, tn[16] = T;
+    const float kFlow = 1e-4f;
+    const uint32_t yb = (flow_a > kFlow ? 1u : 0u) | (flow_b > kFlow ? 2u : 0u) | (flow_ax > 0.1f * kFlow ? 4u : 0u) | (flow_t > kFlow ? 8u : 0u);
+    uint32_t hold = (a.mbits[m] >> 16) & 0xffu, unload = 0;
+    if (hold > 0) hold--, unload = yb;
+    if (a.elastic_step) unload = yb;
+    a.mbits[m] = yb | unload << 8 | hold << 16;
+    if (yb & ~unload) atomicAdd(&a.nyield[0], 1u);
+    member_world_k(a, m);
     a.mon[m] = 1;
+}
+
+// solve_impl's check after a pass (src/phys/frame_fem.cpp:2965-2988): a yielding member's deformation rates at the
+// pass's solution (the start's velocities plus the solution's change) - a hinge turning against its moment, the bar
+// stretching against its force, the twist against its torque - unload it: its elastic tangent from now (held 16
+// steps), its matrix and K u again, its component marked for a pass again
+__global__ void k_mem_repass(FemArgs a, const double* x) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.nm || !a.mon[m]) return;
+    const uint32_t bits = a.mbits[m];
+    const uint32_t y = (bits & 0xffu) & ~((bits >> 8) & 0xffu);
+    if (!y) return;
+    const uint4 me = a.mem[m];
+    f3 vv[2], ww[2];
+    for (int e = 0; e < 2; e++) {
+        const uint32_t n = e ? me.y : me.x;
+        const int fi = a.fidx[n];
+        if (fi < 0) {
+            vv[e] = ww[e] = mk(0, 0, 0);
+            continue;
+        }
+        const double* xs = x + 6 * (size_t)fi;
+        vv[e] = blg::add(mk(a.v[3 * n], a.v[3 * n + 1], a.v[3 * n + 2]), mk((float)xs[0], (float)xs[1], (float)xs[2]));
+        ww[e] = blg::add(mk(a.w[3 * n], a.w[3 * n + 1], a.w[3 * n + 2]), mk((float)xs[3], (float)xs[4], (float)xs[5]));
+    }
+    const float* tn = a.mtan + 17 * (size_t)m;
+    const f3 e1 = mk(tn[0], tn[1], tn[2]), e2 = mk(tn[3], tn[4], tn[5]), e3 = mk(tn[6], tn[7], tn[8]);
+    const float iL = 1.0f / tn[9];
+    const f3 dv = blg::sub(vv[1], vv[0]);
+    const float rax = blg::dot(e3, dv) * iL + blg::dot(e2, ww[0]), ray = -blg::dot(e2, dv) * iL + blg::dot(e3, ww[0]);
+    const float rbx = blg::dot(e3, dv) * iL + blg::dot(e2, ww[1]), rby = -blg::dot(e2, dv) * iL + blg::dot(e3, ww[1]);
+    uint32_t u = 0;
+    if ((y & 1) && rax * tn[11] + ray * tn[12] < 0) u |= 1;
+    if ((y & 2) && rbx * tn[13] + rby * tn[14] < 0) u |= 2;
+    if ((y & 4) && blg::dot(e1, dv) * tn[15] < 0) u |= 4;
+    if ((y & 8) && blg::dot(e1, blg::sub(ww[1], ww[0])) * tn[16] < 0) u |= 8;
+    if (!u) return;
+    a.mbits[m] = (bits & 0xffu) | ((((bits >> 8) & 0xffu) | u) << 8) | 16u << 16;
+    member_world_k(a, m);
+    a.again[a.mcomp[m]] = 1u;
+    atomicAdd(&a.nyield[1], 1u);
 }
 
 // gather_part's member_block (src/phys/frame_fem.cpp:926-941): block blk (0 aa, 1 bb, 2 ba, 3 ab) times c, the tension's
@@ -411,7 +621,95 @@ __device__ void member_block(const FemArgs& a, uint32_t m, int blk, double gs, d
     }
 }
 
-// tri_kinematics + eval_tris (elastic): the co-rotated frame, the corners' forces and moments, the tangent
+__device__ double von_mises(double m0, double m1, double m2) {
+    const double v = m0 * m0 - m0 * m1 + m1 * m1 + 3 * m2 * m2;
+    return sqrt(0.0 < v ? v : 0.0);
+}
+
+__device__ void tri_membrane(const double* b3, const double* c3, double A2, const double* d, double nu, double Dm,
+                             double* eps, double* sig) {
+    eps[0] = eps[1] = eps[2] = 0;
+    for (int i = 0; i < 3; i++) {
+        eps[0] += b3[i] * d[6 * i] / A2;
+        eps[1] += c3[i] * d[6 * i + 1] / A2;
+        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
+    }
+    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+}
+
+// eval_tris's plasticity (src/phys/frame_fem.cpp:1690-1742, the body's allow_break off): the membrane flows radially
+// back to the yield stress, the rest shape following the stretch (the plastic stretch against the authored shape: the
+// damage; the stiffness rebuilt, tri_build_k, into this triangle's own FP64 and float copies); the bending's moments
+// at the three points against the plate's plastic moment, past it the corners' plastic rest rotations following;
+// plastic rotations past 0.02 rad baked into the rest frames. Returns the utilisation.
+__device__ float tri_plastic(const FemArgs& a, uint32_t t, double* d, double* b3, double* c3, double* A2, double* eps,
+                             double* sig) {
+    float* X = a.X + 6 * (size_t)t;
+    const double nu = a.sec.nu, Dm = (double)a.sec.E / (1 - nu * nu);
+    const double seq = von_mises(sig[0], sig[1], sig[2]);
+    float util = (float)(seq / a.sec.yield);
+    if (seq > a.sec.yield) {
+        const double r = 1.0 - a.sec.yield / seq;
+        for (int i = 0; i < 3; i++) {
+            X[2 * i] += (float)(r * d[6 * i]), X[2 * i + 1] += (float)(r * d[6 * i + 1]);
+            d[6 * i] *= 1 - r, d[6 * i + 1] *= 1 - r;
+        }
+        const float mx = (X[0] + X[2] + X[4]) / 3.0f, my = (X[1] + X[3] + X[5]) / 3.0f;
+        for (int i = 0; i < 3; i++) X[2 * i] -= mx, X[2 * i + 1] -= my;
+        const float* X0 = a.X0 + 6 * (size_t)t;
+        const double a0x = X0[2] - X0[0], a0y = X0[3] - X0[1], b0x = X0[4] - X0[0], b0y = X0[5] - X0[1];
+        const double ax = X[2] - X[0], ay = X[3] - X[1], bx = X[4] - X[0], by = X[5] - X[1];
+        const double det = a0x * b0y - b0x * a0y;
+        if (fabs(det) > 1e-12) {
+            const double i00 = b0y / det, i01 = -b0x / det, i10 = -a0y / det, i11 = a0x / det;
+            const double F00 = ax * i00 + bx * i10, F01 = ax * i01 + bx * i11, F10 = ay * i00 + by * i10, F11 = ay * i01 + by * i11;
+            const double C00 = F00 * F00 + F10 * F10, C01 = F00 * F01 + F10 * F11, C11 = F01 * F01 + F11 * F11;
+            const double lmax = 0.5 * (C00 + C11) + sqrt(0.25 * (C00 - C11) * (C00 - C11) + C01 * C01);
+            a.tdmg[t] = smax(a.tdmg[t], (float)(sqrt(lmax) - 1.0));
+        }
+        double* K = a.Kd + (size_t)t * 324;
+        tri_local_k(X, a.sec, K);
+        tri_store_k(a, t, K);
+        tri_grad(X, b3, c3, *A2);
+        tri_membrane(b3, c3, *A2, d, nu, Dm, eps, sig);
+    }
+    const float* Bc = a.Bc + 54 * (size_t)t;
+    const double D = a.sec.E * a.sec.t * a.sec.t * a.sec.t / (12.0f * (1.0f - a.sec.nu * a.sec.nu)); // (float)
+    const double Mp = 0.25 * a.sec.yield * (double)a.sec.t * a.sec.t;
+    double meq = 0;
+    for (int g = 0; g < 3; g++) {
+        double kap[3] = {0, 0, 0};
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < 3; i++)
+                kap[r] += Bc[g * 18 + r * 6 + 2 * i] * d[6 * i + 3] + Bc[g * 18 + r * 6 + 2 * i + 1] * d[6 * i + 4];
+        const double vm = von_mises(D * (kap[0] + nu * kap[1]), D * (nu * kap[0] + kap[1]), D * 0.5 * (1 - nu) * kap[2]);
+        meq = meq < vm ? vm : meq;
+    }
+    util = smax(util, (float)(meq / Mp));
+    float* th0 = a.th0 + 9 * (size_t)t;
+    if (meq > Mp) {
+        const double r = 1.0 - Mp / meq;
+        for (int i = 0; i < 3; i++) {
+            th0[3 * i] += (float)(r * d[6 * i + 3]), th0[3 * i + 1] += (float)(r * d[6 * i + 4]);
+            d[6 * i + 3] *= 1 - r, d[6 * i + 4] *= 1 - r;
+        }
+        a.tdmg[t] = smax(a.tdmg[t], 1e-5f);
+    }
+    for (int i = 0; i < 3; i++) {
+        float* t0 = th0 + 3 * i;
+        if (t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2] > 0.02f * 0.02f) {
+            const float4 r0 = a.r0[3 * (size_t)t + i];
+            const qf q = qnorm(qmul(Q(r0.x, r0.y, r0.z, r0.w), qexp(mk(-t0[0], -t0[1], -t0[2]))));
+            a.r0[3 * (size_t)t + i] = make_float4(q.x, q.y, q.z, q.w);
+            t0[0] = t0[1] = t0[2] = 0;
+        }
+    }
+    return util;
+}
+
+// tri_kinematics + eval_tris: the co-rotated frame, the corners' forces and moments, the tangent; PLASTIC: a section
+// that yields (the corners' plastic rest rotations off the measured ones, then tri_plastic)
+template <bool PLASTIC>
 __global__ void k_tris(FemArgs a) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nt) return;
@@ -452,20 +750,20 @@ __global__ void k_tris(FemArgs a) {
         d[6 * i + 1] = -sp * p[i][0] + cp * p[i][1] - X[2 * i + 1];
         d[6 * i + 2] = 0;
         const float4 qn = a.q[n[i]], r0 = a.r0[3 * (size_t)t + i];
-        const f3 th = qlog(qmul(Ec, qnorm(qmul(Q(qn.x, qn.y, qn.z, qn.w), Q(r0.x, r0.y, r0.z, r0.w)))));
+        f3 th = qlog(qmul(Ec, qnorm(qmul(Q(qn.x, qn.y, qn.z, qn.w), Q(r0.x, r0.y, r0.z, r0.w)))));
+        if (PLASTIC) {
+            const float* t0 = a.th0 + 9 * (size_t)t + 3 * i;
+            th = blg::sub(th, mk(t0[0], t0[1], t0[2]));
+        }
         d[6 * i + 3] = th.x, d[6 * i + 4] = th.y, d[6 * i + 5] = th.z;
     }
     double b3[3], c3[3], A2;
     tri_grad(X, b3, c3, A2);
     if (!(A2 > 0)) return;
     const double nu = a.sec.nu, Dm = (double)a.sec.E / (1 - nu * nu);
-    double eps[3] = {0, 0, 0}, sig[3];
-    for (int i = 0; i < 3; i++) {
-        eps[0] += b3[i] * d[6 * i] / A2;
-        eps[1] += c3[i] * d[6 * i + 1] / A2;
-        eps[2] += (c3[i] * d[6 * i] + b3[i] * d[6 * i + 1]) / A2;
-    }
-    sig[0] = Dm * (eps[0] + nu * eps[1]), sig[1] = Dm * (nu * eps[0] + eps[1]), sig[2] = Dm * 0.5 * (1 - nu) * eps[2];
+    double eps[3], sig[3];
+    tri_membrane(b3, c3, A2, d, nu, Dm, eps, sig);
+    if (PLASTIC) a.tutil[t] = tri_plastic(a, t, d, b3, c3, &A2, eps, sig);
     const float* Km = a.Kf + (size_t)t * 162;
     const float* Kp = Km + 81;
     const double E[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
@@ -635,10 +933,14 @@ __global__ void k_nodes(FemArgs a) {
     double* r = a.rhs + 6 * (size_t)fi;
     // h f - (h - step) f_c + J, no contact forces, step = h; then the held impulse J = f (-(h - step)): +-0, or NaN
     // after a non-finite force, carried into the next step (src/phys/frame_fem.cpp:1030-1042)
-    float* J = a.imp + 3 * i;
+    // (a pass again takes the impulses its first pass took: held aside in jsave)
+    float* J = a.pass ? a.jsave + 3 * i : a.imp + 3 * i;
     r[0] = h * (double)F.x - 0.0 * 0.0f + J[0], r[1] = h * (double)F.y - 0.0 * 0.0f + J[1];
     r[2] = h * (double)F.z - 0.0 * 0.0f + J[2];
-    J[0] = F.x * -0.0f, J[1] = F.y * -0.0f, J[2] = F.z * -0.0f;
+    if (!a.pass) {
+        if (a.jsave) a.jsave[3 * i] = J[0], a.jsave[3 * i + 1] = J[1], a.jsave[3 * i + 2] = J[2];
+        J[0] = F.x * -0.0f, J[1] = F.y * -0.0f, J[2] = F.z * -0.0f;
+    }
     r[3] = h * (double)T.x, r[4] = h * (double)T.y, r[5] = h * (double)T.z;
     for (int j = 0; j < 6; j++) r[j] += rd[j];
     a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
@@ -1425,6 +1727,16 @@ struct bl_fem {
     Dv<float4> mqa, mqb, me1;
     Dv<uint8_t> mon;
     Dv<double> mW, mkv;
+    // plasticity: the members' sections, plastic state, tangents and their bits; the triangles' authored rest shapes,
+    // plastic rest rotations, damage, utilisation and curvature rows; the held impulses of a step's first pass; the
+    // components (the members' own, their node counts); the passes-again budget and the statistics
+    Dv<float> msec, mps, mtan, X0, th0, tdmg, tutil, Bc, jsave;
+    Dv<uint32_t> mbits, nyield, mcomp, again;
+    std::vector<uint32_t> comp_size;
+    uint32_t ncomp = 0;
+    bool plastic_members = false;
+    float repass_credit = 8.0f;
+    int64_t repasses = 0, passes = 0;
     Dv<float4> q, r0;
     Dv<double> Kd, tkv, Dsys, rhs, Osys, y, x, xprev, part, sums;
     Dv<float> p0f;     // rest positions of the free nodes (3 each)
@@ -1483,7 +1795,10 @@ struct bl_fem {
         a.pf = pf.p, a.invm = invm.p, a.fnode = fnode.p, a.imp = imp.p, a.cnt = cnt.p, a.torq = torq.p;
         a.nm = nm, a.mem = mem.p, a.mL0 = mL0.p, a.mqa = mqa.p, a.mqb = mqb.p, a.mst = mst.p, a.mtw = mtw.p;
         a.mdamp = mdamp.p, a.mon = mon.p, a.mfo = mfo.p, a.me1 = me1.p, a.mW = mW.p, a.mkv = mkv.p;
+        a.msec = msec.p, a.mps = mps.p, a.mtan = mtan.p, a.mbits = mbits.p, a.nyield = nyield.p, a.mcomp = mcomp.p;
+        a.again = again.p, a.jsave = jsave.p;
         a.tri = tri.p, a.X = X.p, a.r0 = r0.p, a.Kf = Kf.p, a.Kd = Kd.p;
+        a.X0 = X0.p, a.th0 = th0.p, a.tdmg = tdmg.p, a.tutil = tutil.p, a.Bc = Bc.p;
         a.te = te.p, a.tfo = tfo.p, a.tto = tto.p, a.tg = tg.p, a.tkv = tkv.p, a.ton = ton.p;
         a.nc_off = nc_off.p, a.nc = nc.p, a.fidx = fidx.p, a.ford = ford.p;
         a.Dsys = Dsys.p, a.rhs = rhs.p, a.Osys = Osys.p, a.pairs = pairs.p, a.pcon = pcon.p, a.npairs = npairs;
@@ -1577,7 +1892,8 @@ void dissect(const std::vector<uint32_t>& S, const std::vector<float>& pos, cons
 int fem_build(bl_fem* f, const bl_fem_desc* d) {
     const uint32_t nn = d->n_nodes, nt = d->n_tris;
     f->nn = nn, f->nt = nt;
-    f->sec = Section{d->section.E, d->section.nu, d->section.rho, d->section.t, d->section.drill, d->section.damping};
+    f->sec = Section{d->section.E, d->section.nu, d->section.rho, d->section.t, d->section.drill, d->section.damping,
+                     d->section.yield};
     // masses (ShellMesher::tri), rest shapes and frames (FemFrame::finalize), inertia (node_inertia)
     std::vector<float> mass(nn, 0.0f), inertia(nn, 0.0f), X(6 * (size_t)nt, 0.0f), tmass(nt), area0(nt);
     std::vector<float4> r0(3 * (size_t)nt);
@@ -1637,7 +1953,7 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         const float L0 = blg::length(dd);
         mL0[k] = L0;
         const bool broken = L0 < 1e-5f || e.a == e.b;
-        mem[k] = make_uint4(e.a, e.b, 0, broken ? 1u : 0u);
+        mem[k] = make_uint4(e.a, e.b, e.section, broken ? 1u : 0u);
         mdamp[k] = sc.damping;
         if (broken) continue;
         const f3 e1 = blg::divs(dd, L0);
@@ -1989,7 +2305,42 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
     // the body's Node::p and inv_mass (ShellMesher::finish, src/phys/fem_shell.cpp:48-53: 0 for a pinned node)
     std::vector<float> pf(d->p, d->p + 3 * (size_t)nn), invm(nn);
     for (uint32_t i = 0; i < nn; i++) invm[i] = fixed[i] || mass[i] <= 0 ? 0.0f : 1.0f / mass[i];
+    // the components (FemFrame::analyse, src/phys/frame_fem.cpp:557-566, 594-599: the frame's nodes joined by members
+    // and triangles, in the order of their first node), each member's, their node counts
+    std::vector<uint32_t> up(nn), cid(nn, UINT32_MAX), mcomp(nm);
+    for (uint32_t i = 0; i < nn; i++) up[i] = i;
+    auto find = [&](uint32_t x) {
+        while (up[x] != x) x = up[x] = up[up[x]];
+        return x;
+    };
+    for (uint32_t k = 0; k < nm; k++) up[find(mem[k].x)] = find(mem[k].y);
+    for (uint32_t k = 0; k < nt; k++) {
+        const uint32_t* n = &d->tris[3 * k];
+        up[find(n[0])] = find(n[1]), up[find(n[1])] = find(n[2]);
+    }
+    f->comp_size.clear();
+    for (uint32_t i = 0; i < nn; i++) {
+        const uint32_t r = find(i);
+        if (cid[r] == UINT32_MAX) cid[r] = (uint32_t)f->comp_size.size(), f->comp_size.push_back(0);
+        f->comp_size[cid[r]]++;
+    }
+    f->ncomp = (uint32_t)f->comp_size.size();
+    for (uint32_t k = 0; k < nm; k++) mcomp[k] = cid[find(mem[k].x)];
+    static_assert(sizeof(bl_frame_section) == 17 * sizeof(float), "bl_frame_section is 17 floats");
+    std::vector<float> msec(17 * (size_t)d->n_sections);
+    if (d->n_sections) std::memcpy(msec.data(), d->sections, sizeof(float) * msec.size());
+    f->plastic_members = false;
+    for (uint32_t k = 0; k < nm; k++) f->plastic_members |= d->sections[d->members[k].section].Np > 0;
+    const bool shell_yield = d->section.yield > 0;
     int rc;
+    if ((rc = f->msec.put(msec)) || (rc = f->mps.put(std::vector<float>(16 * (size_t)nm, 0.0f))) ||
+        (rc = f->mtan.alloc(17 * (size_t)nm)) || (rc = f->mbits.put(std::vector<uint32_t>(nm, 0u))) ||
+        (rc = f->nyield.put(std::vector<uint32_t>(2, 0u))) || (rc = f->mcomp.put(mcomp)) ||
+        (rc = f->again.put(std::vector<uint32_t>(f->ncomp, 0u))) || (rc = f->X0.put(X)) ||
+        (rc = f->th0.put(std::vector<float>(9 * (size_t)nt, 0.0f))) || (rc = f->tdmg.put(std::vector<float>(nt, 0.0f))) ||
+        (rc = f->tutil.put(std::vector<float>(nt, 0.0f))) || (rc = f->Bc.alloc(shell_yield ? 54 * (size_t)nt : 0)) ||
+        (rc = f->jsave.alloc(3 * (size_t)nn)))
+        return rc;
     if ((rc = f->mem.put(mem)) || (rc = f->mL0.put(mL0)) || (rc = f->mqa.put(mqa)) || (rc = f->mqb.put(mqb)) ||
         (rc = f->mst.put(mst)) || (rc = f->mtw.put(mtw)) || (rc = f->mdamp.put(mdamp)) || (rc = f->mon.alloc(nm)) ||
         (rc = f->mfo.alloc(9 * (size_t)nm)) || (rc = f->me1.alloc(nm)) || (rc = f->mW.alloc(144 * (size_t)nm)) ||
@@ -2102,10 +2453,17 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
     }
     FCHK(hipMemcpyAsync(f->xprev.p, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToDevice, st));
     FemArgs a = f->args(dl, h, theta, dissipation, g);
+    // the passes-again budget (FemFrame::begin_forces, src/phys/frame_fem.cpp:1863-1864): one more every fourth step,
+    // at most 8; with less than one left the yielding members start from their elastic tangents
+    f->repass_credit = std::min(8.0f, f->repass_credit + 0.25f);
+    a.elastic_step = f->repass_credit < 1.0f ? 1 : 0;
+    a.pass = 0;
     f->step_args = a;
     FCHK(hipEventRecord(f->ev[0], st));
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, (int*)f->nyield.p, 2u);
     if (f->nm) hipLaunchKernelGGL(k_members, dim3((f->nm + 63) / 64), dim3(64), 0, st, a);
-    if (f->nt) hipLaunchKernelGGL(k_tris, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
+    if (f->nt && f->sec.yield > 0) hipLaunchKernelGGL(k_tris<true>, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
+    else if (f->nt) hipLaunchKernelGGL(k_tris<false>, dim3((f->nt + 63) / 64), dim3(64), 0, st, a);
     hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
     if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
     FCHK(hipGetLastError());
@@ -2117,6 +2475,45 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
     }
     return BL_OK;
 }
+
+// solve_impl's passes again (src/phys/frame_fem.cpp:2959-2996) after the first pass's solve: while a member that
+// yielded at the step's start is unloaded by the pass's solution (k_mem_repass: its elastic tangent, its matrix
+// again), the system is assembled, factored and solved again, at most three passes; a pass again of a component of
+// 60 nodes or more spends the budget. *passes counts the systems solved, *repasses the components' passes again.
+int fem_repasses(bl_fem* f, float* ms_factor, float* ms_solve, int* fail, int* passes, int* repasses) {
+    const hipStream_t st = f->st;
+    uint32_t ny[2];
+    FCHK(hipMemcpy(ny, f->nyield.p, sizeof(ny), hipMemcpyDeviceToHost));
+    if (!ny[0]) return BL_OK;
+    FemArgs a = f->step_args;
+    std::vector<uint32_t> again(f->ncomp);
+    for (int pass = 1; pass <= 2; pass++) {
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(256), 0, st, (int*)f->again.p, f->ncomp);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, (int*)f->nyield.p + 1, 1u);
+        hipLaunchKernelGGL(k_mem_repass, dim3((f->nm + 63) / 64), dim3(64), 0, st, a, (const double*)f->x.p);
+        FCHK(hipGetLastError());
+        FCHK(hipMemcpyAsync(ny, f->nyield.p, sizeof(ny), hipMemcpyDeviceToHost, st));
+        FCHK(hipMemcpyAsync(again.data(), f->again.p, sizeof(uint32_t) * f->ncomp, hipMemcpyDeviceToHost, st));
+        FCHK(hipStreamSynchronize(st));
+        for (uint32_t c = 0; c < f->ncomp; c++)
+            if (again[c] && f->comp_size[c] >= 60) (*repasses)++, f->repass_credit -= 1.0f;
+        if (!ny[1]) break;
+        a.pass = pass;
+        hipLaunchKernelGGL(k_nodes, dim3((f->nn + 63) / 64), dim3(64), 0, st, a);
+        if (f->npairs) hipLaunchKernelGGL(k_pairs, dim3((f->npairs + 63) / 64), dim3(64), 0, st, a);
+        FCHK(hipGetLastError());
+        (*passes)++;
+        float mf = 0, ms = 0;
+        int rc = fem_cholesky(f, &mf, &ms, fail);
+        if (rc) return rc;
+        if (ms_factor) *ms_factor += mf;
+        if (ms_solve) *ms_solve += ms;
+        if (*fail) break;
+    }
+    return BL_OK;
+}
+
+bool fem_plastic(const bl_fem* f) { return f->plastic_members || f->sec.yield > 0; }
 
 int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail) {
     f->factor_ok = false;
@@ -2284,6 +2681,9 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
     double res = 0;
     if ((rc = blf::fem_assemble(f, loads, h, theta, dissipation, g, &msa))) return rc;
     if ((rc = blf::fem_cholesky(f, &msf, &mss, &fail))) return rc;
+    int passes = 1, repasses = 0;
+    if (!fail && f->plastic_members && (rc = blf::fem_repasses(f, &msf, &mss, &fail, &passes, &repasses))) return rc;
+    f->passes += passes, f->repasses += repasses;
     if ((rc = blf::fem_residual(f, f->x.p, &res))) return rc;
     if (fail & 2) { // (the device's own trouble, not the system's: the state is left as it was)
         blg::set_last_error("a substitution's wait for a block timed out");
@@ -2296,7 +2696,51 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
         rep->residual = res;
         rep->assemble_ms = msa, rep->factor_ms = msf, rep->solve_ms = mss, rep->update_ms = msu;
         rep->failures = fail ? 1 : 0;
+        rep->passes = passes, rep->repasses = repasses;
     }
+    return BL_OK;
+}
+
+int bl_fem_get_plastic(bl_fem* f, float* tris, float* members, int64_t counters[2]) {
+    if (!f) return BL_ERR_ARG;
+    FCHK(hipStreamSynchronize(f->st));
+    const size_t nt = f->nt, nm = f->nm;
+    if (tris && nt) {
+        std::vector<float> X(6 * nt), th0(9 * nt), dmg(nt), util(nt);
+        std::vector<float4> r0(3 * nt);
+        FCHK(hipMemcpy(X.data(), f->X.p, sizeof(float) * X.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(th0.data(), f->th0.p, sizeof(float) * th0.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(dmg.data(), f->tdmg.p, sizeof(float) * nt, hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(util.data(), f->tutil.p, sizeof(float) * nt, hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(r0.data(), f->r0.p, sizeof(float4) * r0.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nt; k++) {
+            float* o = tris + BL_FEM_TRI_STATE * k;
+            std::memcpy(o, &X[6 * k], 6 * sizeof(float));
+            std::memcpy(o + 6, &th0[9 * k], 9 * sizeof(float));
+            for (int i = 0; i < 3; i++) {
+                const float4 r = r0[3 * k + i];
+                o[15 + 4 * i] = r.x, o[16 + 4 * i] = r.y, o[17 + 4 * i] = r.z, o[18 + 4 * i] = r.w;
+            }
+            o[27] = dmg[k], o[28] = util[k];
+        }
+    }
+    if (members && nm) {
+        std::vector<float> ps(16 * nm);
+        std::vector<float4> qa(nm), qb(nm);
+        std::vector<uint32_t> bits(nm);
+        FCHK(hipMemcpy(ps.data(), f->mps.p, sizeof(float) * ps.size(), hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(qa.data(), f->mqa.p, sizeof(float4) * nm, hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(qb.data(), f->mqb.p, sizeof(float4) * nm, hipMemcpyDeviceToHost));
+        FCHK(hipMemcpy(bits.data(), f->mbits.p, sizeof(uint32_t) * nm, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nm; k++) {
+            float* o = members + BL_FEM_MEM_STATE * k;
+            std::memcpy(o, &ps[16 * k], 16 * sizeof(float));
+            o[16] = qa[k].x, o[17] = qa[k].y, o[18] = qa[k].z, o[19] = qa[k].w;
+            o[20] = qb[k].x, o[21] = qb[k].y, o[22] = qb[k].z, o[23] = qb[k].w;
+            o[24] = (float)(bits[k] & 0xffu), o[25] = (float)((bits[k] >> 8) & 0xffu), o[26] = (float)((bits[k] >> 16) & 0xffu);
+        }
+    }
+    if (counters) counters[0] = f->repasses, counters[1] = f->passes;
     return BL_OK;
 }
 

@@ -48,5 +48,7 @@ void fem_spmv(bl_fem* f, const double* x, double* y);
 int fem_update(bl_fem* f, float* ms, bool explicit_step = false);
 // the factorization's figures for a report
 void fem_report_static(bl_fem* f, bl_fem_report* rep);
+// whether a section of the body yields (a shell yield stress or a member's plastic limits)
+bool fem_plastic(const bl_fem* f);
 
 } // namespace blf

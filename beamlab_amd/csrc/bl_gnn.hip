@@ -1019,6 +1019,10 @@ int bl_fem_step_pcg(bl_fem* f, bl_gnn* g, const float* loads, float h, float the
         return BL_ERR_ARG;
     const bool net_warm = o->warm == BL_WARM_NETWORK, net_prec = o->precond == BL_PRECOND_NETWORK;
     const bool fac_prec = o->precond == BL_PRECOND_FACTOR;
+    if (blf::fem_plastic(f)) { // (its passes again need the direct solve: bl_fem_step)
+        blg::set_last_error("the PCG step takes bodies of elastic sections only");
+        return BL_ERR_UNSUPPORTED;
+    }
     int rc;
     bl_pcg_report r{};
     float ms = 0;

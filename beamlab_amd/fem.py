@@ -44,8 +44,9 @@ class ShellBody:
     """A shell body stepped implicitly on the GPU: bl_fem_create / bl_fem_step / bl_fem_get_state."""
 
     def __init__(self, p, tris, fixed=None, section=SECTION_PLATE, mass=None, members=None, sections=None):
-        """members: (a, b, section, end_a | end_b << 8) rows or an abi.FRAME_MEMBER array; sections: rows of the 12
-        abi.FrameSection fields."""
+        """members: (a, b, section, end_a | end_b << 8) rows or an abi.FRAME_MEMBER array; sections: rows of the 17
+        abi.FrameSection fields (rows of the first 12 - no plastic limits - take FrameSection's defaults for the rest);
+        section: the shell's 6 or 7 abi.ShellSection fields (a 7th, the yield stress, 0 when left out)."""
         self._lib = abi.load()
         self.p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
         self.tris = np.zeros((0, 3), np.uint32) if tris is None else np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
@@ -62,8 +63,10 @@ class ShellBody:
                 mm["end_a"], mm["end_b"] = m[:, 3] & 0xff, m[:, 3] >> 8
                 m = mm
             self.members = np.ascontiguousarray(m)
-        self.sections = np.zeros((0, 12), np.float32) if sections is None else \
-            np.ascontiguousarray(sections, np.float32).reshape(-1, 12)
+        secs = np.zeros((0, abi.FRAME_SECTION_FIELDS), np.float32) if sections is None else np.asarray(sections, np.float32)
+        if secs.ndim == 2 and secs.shape[1] == 12:  # (no plastic limits; FrameSection's hardening and half)
+            secs = np.concatenate([secs, np.tile(np.array([0, 0, 0, 0.25, 0.02], np.float32), (len(secs), 1))], axis=1)
+        self.sections = np.ascontiguousarray(secs, np.float32).reshape(-1, abi.FRAME_SECTION_FIELDS)
         d = abi.FemDesc()
         d.n_nodes, d.n_tris = self.nn, self.nt
         d.p, d.tris = abi.ptr(self.p.reshape(-1)), abi.ptr(self.tris.reshape(-1))
@@ -152,6 +155,19 @@ class ShellBody:
         x = np.zeros_like(b)
         abi.check(self._lib.bl_fem_solve(self._h, abi.ptr(b), abi.ptr(x)), "bl_fem_solve")
         return x
+
+    def plastic(self):
+        """The plastic state (bl_fem_get_plastic): the triangles' X (3 x 2), th0 (3 x 3), r0 (3 x 4), dmg, util; the
+        members' up, tp, pa, pb, ba, bb, bt, damage, dmg_a, dmg_b, util, N (mstate), qa, qb and the tangent's yield,
+        unload and elastic_hold bits; the counters repasses and passes."""
+        tr = np.zeros((self.nt, abi.FEM_TRI_STATE), np.float32)
+        me = np.zeros((len(self.members), abi.FEM_MEM_STATE), np.float32)
+        cnt = np.zeros(2, np.int64)
+        abi.check(self._lib.bl_fem_get_plastic(self._h, abi.ptr(tr.reshape(-1)), abi.ptr(me.reshape(-1)), abi.ptr(cnt)),
+                  "bl_fem_get_plastic")
+        return dict(X=tr[:, 0:6].reshape(-1, 3, 2), th0=tr[:, 6:15].reshape(-1, 3, 3), r0=tr[:, 15:27].reshape(-1, 3, 4),
+                    tdmg=tr[:, 27], tutil=tr[:, 28], mstate=me[:, 0:16], qa=me[:, 16:20], qb=me[:, 20:24],
+                    mbits=me[:, 24:27], repasses=int(cnt[0]), passes=int(cnt[1]))
 
     def tri_k(self, k):
         K = np.zeros((18, 18), np.float64)

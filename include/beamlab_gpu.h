@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BL_ABI_VERSION 2
+#define BL_ABI_VERSION 3
 
 enum {
     BL_OK = 0,
@@ -293,20 +293,24 @@ int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kerne
  * FEM frames: the implicit co-rotational step of a frame body - shell triangles (FemFrame's triangles:
  * src/phys/frame_fem.h:154-181) and Timoshenko members (FemFrame::elems: src/phys/frame_fem.h:197-224, eval_forces
  * src/phys/frame_fem.cpp:1896-2070) - FemFrame::solve (src/phys/frame_fem.cpp:2514-3027). Replaces SoftBody::fem for a
- * body of elastic sections (no plastic limits, mounts, contacts or held steps). The system - 6x6 blocks per node pair, FP64 - is
- * factored each step by a supernodal (multifrontal) Cholesky over a nested-dissection order, its dense fronts on
- * FP64 matrix cores.
+ * body that may deform and does not break (SoftBody::allow_break off: members and triangles yield - the hinges, bars
+ * and twists of members, the membranes and bending of shells - but are never split, torn or let go), with no mounts,
+ * contacts or held steps. The system - 6x6 blocks per node pair, FP64 - is factored each step by a supernodal
+ * (multifrontal) Cholesky over a nested-dissection order, its dense fronts on FP64 matrix cores; a step that unloads
+ * a yielding member is assembled, factored and solved again with its elastic tangent (FemFrame::solve_impl's passes,
+ * src/phys/frame_fem.cpp:2959-2996, on their budget, :1863-1864).
  * ------------------------------------------------------------------------------------------------------------- */
 typedef struct bl_fem bl_fem;
 
-/* == ShellSection (src/phys/frame_fem.h:117-149), the fields the elastic step reads */
+/* == ShellSection (src/phys/frame_fem.h:117-149), the fields the step reads */
 typedef struct bl_shell_section {
     float E, nu, rho, t;
     float drill;   /* drilling spring share of G t A per corner (default 0.05) */
     float damping; /* stiffness-proportional damping (s, default 3e-4) */
+    float yield;   /* yield stress (Pa; 0: stays elastic) */
 } bl_shell_section;
 
-/* == FrameSection (src/phys/frame_fem.h:90-112), the fields an elastic member reads (no plastic limits) */
+/* == FrameSection (src/phys/frame_fem.h:90-112), the fields the step reads */
 typedef struct bl_frame_section {
     float E, G, rho;
     float A, Iy, Iz, J;     /* area, second moments about the member's y and z, torsion constant */
@@ -314,6 +318,9 @@ typedef struct bl_frame_section {
     float damping;          /* stiffness-proportional damping (s) */
     float axial;            /* factor on the axial stiffness */
     float joint_k;          /* BL_FJ_ELASTIC ends: rotational stiffness (N m/rad) */
+    float Np, Mp, Tp;       /* plastic axial force (N), moment and torque (N m); Np 0: stays elastic */
+    float hardening;        /* the plastic moment's growth over a hinge's first radian (default 0.25) */
+    float half;             /* half the outer size (m, default 0.02): the torsional damage's lever */
 } bl_frame_section;
 
 /* == FrameJoint (src/phys/frame_fem.h:185-193): what a member's end frees at its node */
@@ -346,6 +353,8 @@ typedef struct bl_fem_report {
     int32_t failures;       /* a pivot block not positive definite (the step then leaves the state as it was) */
     int32_t supernodes, fronts_max; /* the symbolic factorization: supernodes, largest front (scalar rows) */
     int64_t factor_nnz;     /* nonzeros of L (scalars) */
+    int32_t passes;         /* systems assembled, factored and solved this step (1 + the passes again) */
+    int32_t repasses;       /* passes again of components of 60 nodes or more (FemFrame::repasses, this step) */
 } bl_fem_report;
 
 /* FemFrame::finalize for the triangles (src/phys/frame_fem.cpp:259-276, 3075-3086) and the symbolic analysis. */
@@ -386,6 +395,14 @@ int bl_fem_solve(bl_fem* f, const double* b, double* x);
 int bl_fem_solve_noalloc(bl_fem* f, const double* b, double* x, uint64_t* scratch_bytes);
 /* The element stiffness of triangle `tri` as the device built it (18 x 18 row-major, FP64). */
 int bl_fem_tri_k(bl_fem* f, uint32_t tri, double* K);
+/* The plastic state (FrameTri, src/phys/frame_fem.h:154-181: per triangle BL_FEM_TRI_STATE floats - the rest shape
+ * X[6], the corners' plastic rest rotations th0[9], the rest frames r0[12] (x y z w), dmg, util; FrameElement,
+ * :197-224: per member BL_FEM_MEM_STATE floats - up, tp, pa[2], pb[2], ba[2], bb[2], bt, damage, dmg_a, dmg_b, util, N,
+ * qa[4], qb[4], then the tangent's yield, unload and elastic_hold bits, FemFrame::Tangent) and the counters
+ * {FemFrame::repasses, passes_} so far. Any pointer may be null. */
+#define BL_FEM_TRI_STATE 29
+#define BL_FEM_MEM_STATE 27
+int bl_fem_get_plastic(bl_fem* f, float* tris, float* members, int64_t counters[2]);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * The learned warm start and preconditioner of the implicit step's solve (no counterpart in the reference: the new

@@ -289,3 +289,101 @@ def test_frame_closed_forms_on_gpu():
     got = 0.5 * ((xd[2] - x0[2])[0] + (xd[3] - x0[3])[0])
     want = 200.0 / (24 * EI * 7 / 10)
     assert abs(got / want - 1) < 0.02, (got, want)
+
+
+# ---------------------------------------------------------------------------------------------- plasticity
+def _gpu_plastic(fr):
+    return fem.ShellBody.frame(fr["p"], fr["mass"], fr.get("fixed"), fr.get("tris"), fr.get("shell", fc.SECTION_C3),
+                               fr.get("members"), fr.get("sections"))
+
+
+def _close_plastic(a, b, what, rel=1e-5):
+    """the plastic state within rel of its largest magnitude (members' plastic rotations, elongations, damage; the
+    triangles' rest shapes and plastic rotations); the tangent's bits equal"""
+    for k in ("mstate", "X", "th0", "tdmg"):
+        if a[k].size:
+            scale = max(float(np.abs(b[k]).max()), 1e-30)
+            assert np.abs(a[k] - b[k]).max() <= rel * scale, f"{what}: {k} {np.abs(a[k] - b[k]).max():.3e} vs {scale:.3e}"
+    for k in ("qa", "qb", "r0"):
+        if a[k].size:
+            assert np.abs(a[k] - b[k]).max() <= 1e-5, f"{what}: {k}"
+    assert np.array_equal(a["mbits"], b["mbits"]), f"{what}: tangent bits"
+
+
+@pytest.mark.parametrize("name", sorted(fc.plastic_frames()))
+def test_plastic_frames_track_oracle(oracle_mod, name):
+    """Plastic frames on the GPU against the oracle (itself equal bit for bit to the compiled reference), 60 substeps
+    of loads past the sections' limits and back: members yielding at their hinges, in tension, compression and
+    torsion (the plastic tangent, the passes again with their budget), shells whose membranes flow and whose bending
+    yields. Positions, velocities and spins within 1e-5 relative every step; the plastic state within 1e-5 of its
+    scale, the tangents' bits, the counters and the passes equal."""
+    print(f"plastic frame {name}", flush=True)
+    fr = fc.plastic_frames()[name]
+    P, G = fc.make_plastic(oracle_mod, fr), _gpu_plastic(fr)
+    x0 = P.read()["xd"].copy()
+    scale = dict(disp=0.0, v=0.0, w=0.0)
+    for s, (ld, tq) in enumerate(fc.plastic_loads(name, P.nn, 60, 5)):
+        P.add_torques(tq), G.add_torques(tq)
+        P.step(ld, fc.H)
+        rep = G.step(ld, fc.H)
+        assert rep["residual"] <= 1e-8, rep
+        a, b = G.read(), P.read()
+        scale["disp"] = max(scale["disp"], float(np.abs(b["xd"] - x0).max()))
+        scale["v"] = max(scale["v"], float(np.abs(b["v"]).max()))
+        scale["w"] = max(scale["w"], float(np.abs(b["w"]).max()))
+        _close(a, b, x0, scale, f"{name} step {s}")
+        assert (a["failures"], a["clamps"]) == (b["failures"], b["clamps"])
+        pa, pb = G.plastic(), P.plastic()
+        _close_plastic(pa, pb, f"{name} step {s}")
+        assert (pa["repasses"], pa["passes"]) == (pb["repasses"], pb["passes"]), f"{name} step {s}"
+    if name in ("cantilever", "bar", "portal", "long"):
+        assert pb["passes"] > 60
+    if name in ("plate", "stiffened_plate"):
+        assert pb["tdmg"].max() > 0
+
+
+def test_plastic_frames_match_reference_golden():
+    """The GPU's plastic frames against the compiled reference's recorded states (tests/golden/fem_plastic_ref.npz):
+    positions, velocities and spins within 2e-5 relative, the plastic state within 2e-5 of its scale, the passes
+    equal."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fem_plastic_ref.npz"))
+    for name, fr in sorted(fc.plastic_frames().items()):
+        G = _gpu_plastic(fr)
+        x0 = G.read()["xd"].copy()
+        scale = dict(disp=0.0, v=0.0, w=0.0)
+        for s, (ld, tq) in enumerate(fc.plastic_loads(name, G.nn, 60, 5)):
+            G.add_torques(tq)
+            G.step(ld, fc.H)
+            a = G.read()
+            scale["disp"] = max(scale["disp"], float(np.abs(a["xd"] - x0).max()))
+            scale["v"] = max(scale["v"], float(np.abs(a["v"]).max()))
+            scale["w"] = max(scale["w"], float(np.abs(a["w"]).max()))
+            if s % 20 == 19:
+                b = {k: g[f"{name}_{s}_{k}"] for k in fc.STATE_KEYS}
+                _close(a, b, x0, scale, f"{name} step {s}", rel=2e-5)
+                pa = G.plastic()
+                pb = {k: g[f"{name}_{s}_{k}"] for k in ("X", "th0", "r0", "tdmg", "tutil", "mstate", "qa", "qb")}
+                pb["mbits"] = pa["mbits"]  # (private in the reference)
+                _close_plastic(pa, pb, f"{name} step {s}", rel=2e-5)
+                cnt = g[f"{name}_{s}_counters"]
+                assert [a["failures"], a["clamps"], pa["repasses"], pa["passes"]] == list(cnt), f"{name} step {s}"
+        G.close()
+
+
+def test_plastic_cantilever_collapse_on_gpu():
+    """The cantilever's plastic collapse on the GPU (no hardening, quasi-static): at 1.2 Mp / L the root's hinge turns
+    (the tip drops by more than a third of the span), no other, and at rest the root member's moment (its utilisation
+    times Mp) equals the load times the tip's lever arm (0.5 %) within Mp."""
+    sec = fc.plastic_tube(hardening=0.0)
+    G = _gpu_plastic(fc.frame_line(10, (0, 1, 0), (1, 1, 0), sec))
+    F = 1.2 * float(sec[13])
+    ld = np.zeros((G.nn, 3), np.float32)
+    for s in range(6000):
+        ld[-1, 1] = -F * min(1.0, (s + 1) / 3000)
+        G.step(ld, fc.H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    st, pl = G.read(), G.plastic()
+    rot = np.hypot(*(pl["mstate"][:, 2:4] + pl["mstate"][:, 6:8]).T)
+    assert rot[0] > 0.5 and np.abs(pl["mstate"][1:, 2:10]).max() == 0, rot
+    assert -(st["xd"][-1][1] - 1.0) > 1.0 / 3
+    assert abs(pl["mstate"][0, 14] * sec[13] / (F * st["xd"][-1][0]) - 1) < 0.005 and pl["mstate"][0, 14] <= 1.0
