@@ -251,6 +251,14 @@ struct orc_fem {
     uint32_t *comp, *comp_size, ncomp;
     float repass_credit;
     int64_t repasses, passes;
+    /* held steps (FemFrame::held_*, :2342-2512): the last step's prediction of the smooth forces and moments (pred_,
+     * tpred_: all but the elements' and the contacts'), and its factorization kept (the envelope, its inverse
+     * diagonals, the free nodes' indices) while it succeeded */
+    float *pred, *tpred;
+    int factored, nfk;
+    double *Lk, *dk;
+    int *fck, *fik;
+    size_t* roffk;
 };
 
 int orc_fem_rest_shape(const float p[9], float X[6]) {
@@ -370,6 +378,8 @@ orc_fem* orc_fem_create(uint32_t nn, const float* p, const float* mass, const ui
     f->comp = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
     f->comp_size = (uint32_t*)calloc(nn + 1, sizeof(uint32_t));
     f->repass_credit = 8.0f; /* kRepassBurst */
+    f->pred = (float*)calloc(3 * nn + 1, sizeof(float));
+    f->tpred = (float*)calloc(3 * nn + 1, sizeof(float));
     f->nn = nn, f->nt = nt, f->nm = nm, f->nsec = nsec;
     f->xd = (double*)calloc(3 * nn + 1, sizeof(double));
     f->pos0 = (float*)calloc(3 * nn + 1, sizeof(float));
@@ -541,7 +551,8 @@ void orc_fem_destroy(orc_fem* f) {
     if (!f) return;
     free(f->torque), free(f->mem), free(f->mL0), free(f->mqa), free(f->mqb), free(f->mst), free(f->mtw), free(f->mbroken);
     free(f->sec), free(f->X0), free(f->th0), free(f->tdmg), free(f->tutil), free(f->Bc), free(f->tk_ok), free(f->mps);
-    free(f->tan), free(f->comp), free(f->comp_size);
+    free(f->tan), free(f->comp), free(f->comp_size), free(f->pred), free(f->tpred);
+    free(f->Lk), free(f->dk), free(f->fck), free(f->fik), free(f->roffk);
     free(f->xd), free(f->pos0), free(f->p), free(f->inv_mass), free(f->force), free(f->impulse), free(f->v), free(f->w), free(f->q), free(f->mass), free(f->inertia), free(f->fixed);
     free(f->tn), free(f->X), free(f->r0), free(f->Kf), free(f->tmass), free(f->area0);
     free(f);
@@ -1247,12 +1258,10 @@ static void put_block(double* A, int n, int r, int c, const double* B) {
  * (frame_fem.cpp:2899-2919, then L L^T x = b, :2933-2942) over the blocks of the natural order; a block outside the
  * envelope is zero, and its updates (T -= 0) would leave every value as it is, so they are skipped. */
 #define ENV(r, c) (&L[(roff[r] + (size_t)((c) - fc[r])) * 36])
-static int env_solve(int n, const int* fc, const size_t* roff, double* L, double* b) {
-    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)(n ? n : 1));
-    int ok = 1;
-    for (int k = 0; k < n && ok; k++) {
+static int env_factor(int n, const int* fc, const size_t* roff, double* L, double* dinv) {
+    for (int k = 0; k < n; k++) {
         double* Lkk = ENV(k, k);
-        if (!chol6(Lkk, &dinv[6 * k])) { ok = 0; break; }
+        if (!chol6(Lkk, &dinv[6 * k])) return 0;
         for (int r = k + 1; r < n; r++)
             if (fc[r] <= k) right_solve_lt(Lkk, &dinv[6 * k], ENV(r, k));
         for (int r = k + 1; r < n; r++) {
@@ -1261,20 +1270,27 @@ static int env_solve(int n, const int* fc, const size_t* roff, double* L, double
                 if (fc[c] <= k) sub_abt(ENV(r, c), ENV(r, k), ENV(c, k));
         }
     }
-    if (ok) {
-        for (int k = 0; k < n; k++) {
-            forward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
-            for (int r = k + 1; r < n; r++)
-                if (fc[r] <= k) sub_mv(ENV(r, k), &b[6 * k], &b[6 * r]);
-        }
-        for (int k = n - 1; k >= 0; k--) {
-            for (int r = k + 1; r < n; r++)
-                if (fc[r] <= k) sub_mtv(ENV(r, k), &b[6 * r], &b[6 * k]);
-            backward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
-        }
+    return 1;
+}
+
+static void env_subst(int n, const int* fc, const size_t* roff, const double* L, const double* dinv, double* b) {
+    for (int k = 0; k < n; k++) {
+        forward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
+        for (int r = k + 1; r < n; r++)
+            if (fc[r] <= k) sub_mv(ENV(r, k), &b[6 * k], &b[6 * r]);
     }
-    free(dinv);
-    return ok ? 0 : -1;
+    for (int k = n - 1; k >= 0; k--) {
+        for (int r = k + 1; r < n; r++)
+            if (fc[r] <= k) sub_mtv(ENV(r, k), &b[6 * r], &b[6 * k]);
+        backward6(ENV(k, k), &dinv[6 * k], &b[6 * k]);
+    }
+}
+
+/* factor in place (dinv: 6 per block row) and solve; 0, or -1 when a pivot block is not positive definite */
+static int env_solve(int n, const int* fc, const size_t* roff, double* L, double* dinv, double* b) {
+    if (!env_factor(n, fc, roff, L, dinv)) return -1;
+    env_subst(n, fc, roff, L, dinv, b);
+    return 0;
 }
 
 void orc_fem_plastic(const orc_fem* f, float* tris, float* members, int64_t counters[2]) {
@@ -1314,9 +1330,11 @@ int orc_fem_block_solve(int n, double* A, double* b) {
     }
     roff[n] = tot;
     double* L = (double*)malloc(sizeof(double) * 36 * (tot ? tot : 1));
+    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)(n ? n : 1));
     for (int r = 0; r < n; r++)
         for (int c = fc[r]; c <= r; c++) get_block(A, n, r, c, ENV(r, c));
-    const int rc = env_solve(n, fc, roff, L, b);
+    const int rc = env_solve(n, fc, roff, L, dinv, b);
+    free(dinv);
     if (rc == 0)
         for (int r = 0; r < n; r++)
             for (int c = fc[r]; c <= r; c++) put_block(A, n, r, c, ENV(r, c));
@@ -1366,7 +1384,7 @@ static void add_member(const orc_fem* f, uint32_t k, const mem_tan* t, double si
  * torque - unload it: its elastic tangent (held 16 steps), the system corrected and solved again, at most three
  * passes. A pass again of a component of 60 nodes or more is counted and spends the budget. Returns env_solve's. */
 static int solve_passes(orc_fem* f, const mem_out* mo, const int* fi, int nf, const int* fc, const size_t* roff,
-                        size_t tot, double* L, double* r, double h, double h2, double hd) {
+                        size_t tot, double* L, double* dinv, double* r, double h, double h2, double hd) {
     const uint32_t nm = f->nm;
     int any_yield = 0;
     for (uint32_t k = 0; k < nm; k++) any_yield |= mo[k].on && (f->tan[k].yield & ~f->tan[k].unload);
@@ -1394,7 +1412,7 @@ static int solve_passes(orc_fem* f, const mem_out* mo, const int* fi, int nf, co
             nchg = 0;
             memcpy(L, La, sizeof(double) * 36 * tot), memcpy(r, ra, sizeof(double) * 6 * (size_t)nf);
         }
-        rc = env_solve(nf, fc, roff, L, r);
+        rc = env_solve(nf, fc, roff, L, dinv, r);
         if (rc || !any_yield || pass == 2) break;
         memset(again, 0, f->ncomp + 1);
         for (uint32_t k = 0; k < nm; k++) {
@@ -1435,9 +1453,53 @@ static int solve_passes(orc_fem* f, const mem_out* mo, const int* fi, int nf, co
     return rc;
 }
 
+/* SoftBody::integrate_nodes (softbody.cpp:769-795) and FemFrame::sync_positions (frame_fem.cpp:3407-3422) over a
+ * substep of `step`: v += F (m^-1 step), clamped at 400 m/s, a non-finite v reset (counted); p += v step; x += step v in
+ * double, the orientation turned by w step, the float position taken from the double one where within the float's
+ * rounding, else the other way. F is kept as the body's forces. */
+static void integrate_sync(orc_fem* f, const f3* F, float h) {
+    const uint32_t nn = f->nn;
+    for (uint32_t i = 0; i < nn; i++) {
+        f->force[3 * i] = F[i].x, f->force[3 * i + 1] = F[i].y, f->force[3 * i + 2] = F[i].z;
+        if (!(f->inv_mass[i] > 0)) continue;
+        f3 v = add3(F3(f->v[3 * i], f->v[3 * i + 1], f->v[3 * i + 2]), mul3(F[i], f->inv_mass[i] * h));
+        const float v2 = dot3(v, v);
+        if (v2 > 400.0f * 400.0f) {
+            v = mul3(v, 400.0f / sqrtf(v2));
+        } else if (!(v2 >= 0.0f)) {
+            v = F3(0, 0, 0);
+            f->nan_resets++;
+        }
+        f->v[3 * i] = v.x, f->v[3 * i + 1] = v.y, f->v[3 * i + 2] = v.z;
+        const f3 pn = add3(F3(f->p[3 * i], f->p[3 * i + 1], f->p[3 * i + 2]), mul3(v, h));
+        f->p[3 * i] = pn.x, f->p[3 * i + 1] = pn.y, f->p[3 * i + 2] = pn.z;
+    }
+    for (uint32_t i = 0; i < nn; i++) {
+        double* x = &f->xd[3 * i];
+        float* pp = &f->p[3 * i];
+        x[0] += (double)h * f->v[3 * i], x[1] += (double)h * f->v[3 * i + 1], x[2] += (double)h * f->v[3 * i + 2];
+        const f3 wv = F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]);
+        if (dot3(wv, wv) > 0) {
+            const qf qn = qnorm(qmul(qexp(mul3(wv, h)), Q(f->q[4 * i], f->q[4 * i + 1], f->q[4 * i + 2], f->q[4 * i + 3])));
+            f->q[4 * i] = qn.x, f->q[4 * i + 1] = qn.y, f->q[4 * i + 2] = qn.z, f->q[4 * i + 3] = qn.w;
+        }
+        const double tol = 1e-4 * (1.0 + fabsf(pp[0]) + fabsf(pp[1]) + fabsf(pp[2]));
+        if (f->inv_mass[i] > 0 && fabs(x[0] - pp[0]) <= tol && fabs(x[1] - pp[1]) <= tol && fabs(x[2] - pp[2]) <= tol)
+            pp[0] = (float)x[0], pp[1] = (float)x[1], pp[2] = (float)x[2];
+        else
+            x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
+    }
+}
+
 /* ------------------------------------------------------------------ the step */
 int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* Aout,
                  double* rhs_out, double* x_out, uint32_t* free_map, uint32_t* nfree_out) {
+    return orc_fem_step_span(f, loads, h, h, theta, dissipation, g, Aout, rhs_out, x_out, free_map, nfree_out);
+}
+
+int orc_fem_step_span(orc_fem* f, const float* loads, float h, float step, float theta, float dissipation,
+                      const float g[3], double* Aout, double* rhs_out, double* x_out, uint32_t* free_map,
+                      uint32_t* nfree_out) {
     const uint32_t nn = f->nn, nt = f->nt;
     /* theta and the dissipation as FemFrame::solve_begin takes them (frame_fem.cpp:2524-2525) */
     dissipation = dissipation < 0.0f ? 0.0f : (dissipation > 1.0f ? 1.0f : dissipation);
@@ -1468,6 +1530,9 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
      * in triangle order (gather_lists, :862-898); the moments: those from outside, then the same */
     f3* F = (f3*)malloc(sizeof(f3) * (nn + 1));
     f3* T = (f3*)calloc(nn + 1, sizeof(f3));
+    /* (the elements' share alone beside them: member_f, member_t_, gather_forces :2080-2100) */
+    f3* Fm = (f3*)calloc(nn + 1, sizeof(f3));
+    f3* Tm = (f3*)calloc(nn + 1, sizeof(f3));
     for (uint32_t i = 0; i < nn; i++) {
         F[i] = mul3(F3(g[0], g[1], g[2]), f->mass[i]);
         if (loads) F[i] = add3(F[i], F3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]));
@@ -1476,15 +1541,15 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
     for (uint32_t k = 0; k < nm; k++)
         if (mo[k].on) {
             const uint32_t a = f->mem[4 * k], b = f->mem[4 * k + 1];
-            F[a] = add3(F[a], mo[k].fa), T[a] = add3(T[a], mo[k].ta);
-            F[b] = sub3(F[b], mo[k].fa), T[b] = add3(T[b], mo[k].tb);
+            F[a] = add3(F[a], mo[k].fa), Fm[a] = add3(Fm[a], mo[k].fa), T[a] = add3(T[a], mo[k].ta), Tm[a] = add3(Tm[a], mo[k].ta);
+            F[b] = sub3(F[b], mo[k].fa), Fm[b] = sub3(Fm[b], mo[k].fa), T[b] = add3(T[b], mo[k].tb), Tm[b] = add3(Tm[b], mo[k].tb);
         }
     for (uint32_t k = 0; k < nt; k++)
         if (o[k].on)
             for (int c = 0; c < 3; c++) {
                 const uint32_t nd = f->tn[3 * k + c];
-                F[nd] = add3(F[nd], o[k].fo[c]);
-                T[nd] = add3(T[nd], o[k].to[c]);
+                F[nd] = add3(F[nd], o[k].fo[c]), Fm[nd] = add3(Fm[nd], o[k].fo[c]);
+                T[nd] = add3(T[nd], o[k].to[c]), Tm[nd] = add3(Tm[nd], o[k].to[c]);
             }
     /* the free nodes in natural order */
     int* fi = (int*)malloc(sizeof(int) * nn);
@@ -1564,12 +1629,18 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
                 for (int j = 0; j < 6; j++) rd[j] -= cr * o[k].kv[6 * ci + j];
         }
         double* rr = &r[6 * (size_t)fi[i]];
-        /* h f - (h - step) f_c + J with no contact forces and step = h; J, then the held impulse f (-(h - step)): +-0,
-         * or NaN after a non-finite force (frame_fem.cpp:1030-1042) */
+        /* h f - (h - step) f_c + J with no contact forces; then the held impulse (f - f_m - f_c) (-(h - step)) (+-0 for a
+         * step of one substep, or NaN after a non-finite force) and the prediction of the substeps the step spans,
+         * f - f_m - f_c and the moments' m - t_m (frame_fem.cpp:1026-1042) */
+        const double hs = (double)h - (double)step;
         float* J = &f->impulse[3 * i];
-        rr[0] = h * (double)F[i].x - 0.0 * 0.0f + J[0], rr[1] = h * (double)F[i].y - 0.0 * 0.0f + J[1];
-        rr[2] = h * (double)F[i].z - 0.0 * 0.0f + J[2];
-        J[0] = F[i].x * -0.0f, J[1] = F[i].y * -0.0f, J[2] = F[i].z * -0.0f;
+        rr[0] = h * (double)F[i].x - hs * 0.0f + J[0], rr[1] = h * (double)F[i].y - hs * 0.0f + J[1];
+        rr[2] = h * (double)F[i].z - hs * 0.0f + J[2];
+        const f3 pr = sub3(sub3(F[i], Fm[i]), F3(0, 0, 0)), tp = sub3(T[i], Tm[i]);
+        const f3 Jn = mul3(pr, -(float)hs);
+        J[0] = Jn.x, J[1] = Jn.y, J[2] = Jn.z;
+        f->pred[3 * i] = pr.x, f->pred[3 * i + 1] = pr.y, f->pred[3 * i + 2] = pr.z;
+        f->tpred[3 * i] = tp.x, f->tpred[3 * i + 1] = tp.y, f->tpred[3 * i + 2] = tp.z;
         rr[3] = h * (double)T[i].x, rr[4] = h * (double)T[i].y, rr[5] = h * (double)T[i].z;
         for (int j = 0; j < 6; j++) rr[j] += rd[j];
     }
@@ -1605,7 +1676,11 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
             }
     }
     if (rhs_out) memcpy(rhs_out, r, sizeof(double) * N6);
-    int rc = solve_passes(f, mo, fi, nf, fc, roff, tot, L, r, h, h2, hd);
+    double* dinv = (double*)malloc(sizeof(double) * 6 * (size_t)(nf ? nf : 1));
+    int rc = solve_passes(f, mo, fi, nf, fc, roff, tot, L, dinv, r, h, h2, hd);
+    /* the factorization kept for the held steps while it succeeded (comp_factored_) */
+    free(f->Lk), free(f->dk), free(f->fck), free(f->fik), free(f->roffk);
+    f->Lk = L, f->dk = dinv, f->fck = fc, f->fik = fi, f->roffk = roff, f->nfk = nf, f->factored = rc == 0;
     if (rc == 0) {
         if (x_out) memcpy(x_out, r, sizeof(double) * N6);
         /* solve_impl's update (frame_fem.cpp:3000-3027): a clamped node's spin zeroed; a free node's new velocity
@@ -1626,7 +1701,7 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
             const float kMaxDv = 40.0f, kMaxW = 3000.0f;
             float l = len3(dv);
             if (l > kMaxDv) dv = mul3(dv, kMaxDv / l), f->clamps++;
-            F[i] = mul3(dv, f->mass[i] / h);
+            F[i] = mul3(dv, f->mass[i] / step);
             f3 wv = add3(F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]), dw);
             l = len3(wv);
             if (l > kMaxW) wv = mul3(wv, kMaxW / l), f->clamps++;
@@ -1639,40 +1714,73 @@ int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dis
         for (uint32_t i = 0; i < nn; i++)
             if (fi[i] >= 0) f->impulse[3 * i] = f->impulse[3 * i + 1] = f->impulse[3 * i + 2] = 0;
     }
-    /* SoftBody::integrate_nodes (softbody.cpp:769-795) over the step */
-    for (uint32_t i = 0; i < nn; i++) {
-        f->force[3 * i] = F[i].x, f->force[3 * i + 1] = F[i].y, f->force[3 * i + 2] = F[i].z;
-        if (!(f->inv_mass[i] > 0)) continue;
-        f3 v = add3(F3(f->v[3 * i], f->v[3 * i + 1], f->v[3 * i + 2]), mul3(F[i], f->inv_mass[i] * h));
-        const float v2 = dot3(v, v);
-        if (v2 > 400.0f * 400.0f) {
-            v = mul3(v, 400.0f / sqrtf(v2));
-        } else if (!(v2 >= 0.0f)) {
-            v = F3(0, 0, 0);
-            f->nan_resets++;
-        }
-        f->v[3 * i] = v.x, f->v[3 * i + 1] = v.y, f->v[3 * i + 2] = v.z;
-        const f3 pn = add3(F3(f->p[3 * i], f->p[3 * i + 1], f->p[3 * i + 2]), mul3(v, h));
-        f->p[3 * i] = pn.x, f->p[3 * i + 1] = pn.y, f->p[3 * i + 2] = pn.z;
-    }
-    /* FemFrame::sync_positions (frame_fem.cpp:3407-3422): x += h v in double, the orientation turned by w h, the
-     * float position taken from the double one where within the float's rounding, else the other way */
-    for (uint32_t i = 0; i < nn; i++) {
-        double* x = &f->xd[3 * i];
-        float* pp = &f->p[3 * i];
-        x[0] += (double)h * f->v[3 * i], x[1] += (double)h * f->v[3 * i + 1], x[2] += (double)h * f->v[3 * i + 2];
-        const f3 wv = F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]);
-        if (dot3(wv, wv) > 0) {
-            const qf qn = qnorm(qmul(qexp(mul3(wv, h)), Q(f->q[4 * i], f->q[4 * i + 1], f->q[4 * i + 2], f->q[4 * i + 3])));
-            f->q[4 * i] = qn.x, f->q[4 * i + 1] = qn.y, f->q[4 * i + 2] = qn.z, f->q[4 * i + 3] = qn.w;
-        }
-        const double tol = 1e-4 * (1.0 + fabsf(pp[0]) + fabsf(pp[1]) + fabsf(pp[2]));
-        if (f->inv_mass[i] > 0 && fabs(x[0] - pp[0]) <= tol && fabs(x[1] - pp[1]) <= tol && fabs(x[2] - pp[2]) <= tol)
-            pp[0] = (float)x[0], pp[1] = (float)x[1], pp[2] = (float)x[2];
-        else
-            x[0] = pp[0], x[1] = pp[1], x[2] = pp[2];
-    }
+    integrate_sync(f, F, step);
     memset(f->torque, 0, 3 * sizeof(float) * nn); /* (FemFrame::solve_end: the moments from outside used up) */
-    free(mo), free(o), free(F), free(T), free(fi), free(L), free(fc), free(roff), free(r);
+    free(mo), free(o), free(F), free(T), free(Fm), free(Tm), free(r);
     return rc;
+}
+
+/* A held substep (FemFrame::held_begin / held_component / held_end, :2342-2512) of a frame whose implicit step spans
+ * several substeps: the forces (gravity, the loads) and moments from outside beyond the last step's prediction, times
+ * the substep, through that step's factorization; the prediction's share of the held impulse paid; the solution's
+ * velocity changes handed to the integrator as forces (a non-finite one: no force; |dv| <= 40 m/s, |w| <= 3000 rad/s,
+ * counted). Without a factorization (the last step failed, or none yet) FemFrame::hold (:2321-2340): the forces held
+ * over as impulse. Then integrate_nodes and sync_positions. */
+int orc_fem_held(orc_fem* f, const float* loads, float step, const float g[3]) {
+    const uint32_t nn = f->nn;
+    f3* F = (f3*)malloc(sizeof(f3) * (nn + 1));
+    for (uint32_t i = 0; i < nn; i++) {
+        F[i] = mul3(F3(g[0], g[1], g[2]), f->mass[i]);
+        if (loads) F[i] = add3(F[i], F3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]));
+    }
+    if (!f->factored) {
+        for (uint32_t i = 0; i < nn; i++) {
+            if (f->inv_mass[i] <= 0) continue;
+            const f3 fc = F3(0, 0, 0), J = mul3(sub3(F[i], fc), step);
+            f->impulse[3 * i] += J.x, f->impulse[3 * i + 1] += J.y, f->impulse[3 * i + 2] += J.z;
+            const f3 v = add3(F3(f->v[3 * i], f->v[3 * i + 1], f->v[3 * i + 2]), mul3(fc, step * f->inv_mass[i]));
+            f->v[3 * i] = v.x, f->v[3 * i + 1] = v.y, f->v[3 * i + 2] = v.z;
+            F[i] = F3(0, 0, 0);
+        }
+    } else {
+        const int nf = f->nfk;
+        const int* fi = f->fik;
+        double* r = (double*)calloc(6 * (size_t)(nf ? nf : 1), sizeof(double));
+        for (uint32_t i = 0; i < nn; i++) {
+            if (fi[i] < 0) continue;
+            double* rr = &r[6 * (size_t)fi[i]];
+            if (f->inv_mass[i] <= 0) continue;
+            const f3 pr = F3(f->pred[3 * i], f->pred[3 * i + 1], f->pred[3 * i + 2]);
+            const f3 tp = F3(f->tpred[3 * i], f->tpred[3 * i + 1], f->tpred[3 * i + 2]);
+            const f3 d = mul3(sub3(F[i], pr), step);
+            const f3 t = mul3(sub3(F3(f->torque[3 * i], f->torque[3 * i + 1], f->torque[3 * i + 2]), tp), step);
+            rr[0] = d.x, rr[1] = d.y, rr[2] = d.z, rr[3] = t.x, rr[4] = t.y, rr[5] = t.z;
+            const f3 J = mul3(pr, step);
+            f->impulse[3 * i] += J.x, f->impulse[3 * i + 1] += J.y, f->impulse[3 * i + 2] += J.z;
+        }
+        env_subst(nf, f->fck, f->roffk, f->Lk, f->dk, r);
+        for (uint32_t i = 0; i < nn; i++) {
+            if (fi[i] < 0) continue;
+            const double* x = &r[6 * (size_t)fi[i]];
+            f3 dv = F3((float)x[0], (float)x[1], (float)x[2]);
+            const f3 dw = F3((float)x[3], (float)x[4], (float)x[5]);
+            if (!isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
+                F[i] = F3(0, 0, 0);
+                continue;
+            }
+            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
+            float l = len3(dv);
+            if (l > kMaxDv) dv = mul3(dv, kMaxDv / l), f->clamps++;
+            F[i] = mul3(dv, f->mass[i] / step);
+            f3 wv = add3(F3(f->w[3 * i], f->w[3 * i + 1], f->w[3 * i + 2]), dw);
+            l = len3(wv);
+            if (l > kMaxW) wv = mul3(wv, kMaxW / l), f->clamps++;
+            f->w[3 * i] = wv.x, f->w[3 * i + 1] = wv.y, f->w[3 * i + 2] = wv.z;
+        }
+        free(r);
+    }
+    integrate_sync(f, F, step);
+    memset(f->torque, 0, 3 * sizeof(float) * nn); /* (held_end / hold: the moments from outside used up) */
+    free(F);
+    return 0;
 }

@@ -69,6 +69,15 @@ void orc_fem_get_nodes(const orc_fem* f, float* nodes, float* force, int64_t cou
 int orc_fem_step(orc_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], double* A,
                  double* rhs, double* x, uint32_t* free_map, uint32_t* nfree);
 
+/* One implicit step spanning h (a frame stepping every h / step substeps: WorldSettings::frame_every) of which the
+ * first substep, of `step`, is integrated: the forces taken for the whole of h, the held impulse and the prediction
+ * for the held substeps kept (frame_fem.cpp:1026-1042), the update over `step`. orc_fem_step is this with step = h.
+ * The caller passes the dissipation per step as the world does (frame_dissipation / the steps it spans). */
+int orc_fem_step_span(orc_fem* f, const float* loads, float h, float step, float theta, float dissipation,
+                      const float g[3], double* A, double* rhs, double* x, uint32_t* free_map, uint32_t* nfree);
+/* A held substep of `step` between two implicit steps (FemFrame::held_*, or FemFrame::hold without a factorization) */
+int orc_fem_held(orc_fem* f, const float* loads, float step, const float g[3]);
+
 /* the reference's dense block Cholesky on an n x n block SPD matrix (row-major (6n)^2): solves A x = b in place */
 int orc_fem_block_solve(int n, double* A, double* b);
 

@@ -81,6 +81,9 @@ def oracle_lib():
                                 ctypes.c_uint32, _P]),
             "fem_add_torques": (None, [_P, _P]),
             "fem_plastic": (None, [_P, _P, _P, _P]),
+            "fem_step_span": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                             _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
+            "fem_held": (ctypes.c_int, [_P, _P, ctypes.c_float, _P]),
             "fem_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _P, _P, _P,
                                         ctypes.POINTER(ctypes.c_uint32)]),
             "fem_block_solve": (ctypes.c_int, [ctypes.c_int, _P, _P]),
@@ -119,6 +122,8 @@ def ref_lib():
                                ctypes.c_uint32, _P]),
             "frame_add_torques": (None, [_P, _P]),
             "frame_plastic": (None, [_P, _P, _P, _P]),
+            "frame_cycle": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                           ctypes.c_float, _P]),
         })
         atexit.register(_ref.blref_shutdown)
     return _ref
@@ -360,6 +365,13 @@ class RefPlate:
         self._lib.blref_frame_plastic(self._h, _ptr(tr.reshape(-1)), _ptr(me.reshape(-1)), _ptr(cnt))
         return _plastic_record(tr, me, cnt)
 
+    def cycle(self, loads, dt, every, j, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """Substep j of a frame stepping its frame every `every` substeps (blref_frame_cycle): the implicit step over
+        every dt at j % every == 0 (the dissipation divided by every), else the held step; then the integration."""
+        ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
+        gg = np.asarray(g, np.float32)
+        self._lib.blref_frame_cycle(self._h, _ptr(ld), dt, int(every), int(j), theta, dissipation, _ptr(gg))
+
     def substep(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
         """The plate body's whole substep as the reference's world runs it: the step, then integrate_nodes and
         sync_positions."""
@@ -472,6 +484,19 @@ class FemPlate:
     def set_state(self, xd, v, w, q):
         self._lib.orc_fem_set_state(self._h, *[_ptr(np.ascontiguousarray(a, t)) for a, t in
                                               ((xd, np.float64), (v, np.float32), (w, np.float32), (q, np.float32))])
+
+    def cycle(self, loads, dt, every, j, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """Substep j of a frame stepping its frame every `every` substeps, as RefPlate.cycle: orc_fem_step_span over
+        every dt (the dissipation divided by every) at j % every == 0, else orc_fem_held."""
+        ld = np.zeros((self.nn, 3), np.float32) if loads is None else np.ascontiguousarray(loads, np.float32).reshape(self.nn, 3)
+        gg = np.asarray(g, np.float32)
+        if j % every == 0:
+            nf = ctypes.c_uint32(0)
+            self._lib.orc_fem_step_span(self._h, _ptr(ld), np.float32(dt) * np.float32(every), dt, theta,
+                                        np.float32(dissipation) / np.float32(every), _ptr(gg), None, None, None, None,
+                                        ctypes.byref(nf))
+        else:
+            self._lib.orc_fem_held(self._h, _ptr(ld), dt, _ptr(gg))
 
     def step(self, loads, h, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0), system=False):
         """One implicit step; with system=True also (A, rhs, x, free nodes) of the step's linear system."""

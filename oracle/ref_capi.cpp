@@ -370,6 +370,36 @@ int blref_fem_plate_substep(void* h, const float* loads, float dt, float theta, 
     return 0;
 }
 
+// A substep of a frame body stepping its frame every `every` substeps (WorldSettings::frame_every), as
+// World::simulate_island runs it (src/phys/world.cpp:3244-3252, 3573-3580, 3612-3642, 3665, 3812-3822): clear_forces(g)
+// plus the loads; at substep j % every == 0 FemFrame::compute_forces and FemFrame::solve over h = every dt (the
+// dissipation per step divided by every), else the held step (held_begin, held_component for each component,
+// held_end; FemFrame::hold without a factorization); then integrate_nodes and sync_positions over dt.
+int blref_frame_cycle(void* h, const float* loads, float dt, int every, int j, float theta, float dissipation,
+                      const float g[3]) {
+    SoftBody& b = *static_cast<SoftBody*>(h);
+    b.clear_forces(vec3(g[0], g[1], g[2]));
+    if (loads)
+        for (size_t i = 0; i < b.nodes.size(); i++) b.force[i] += vec3(loads[3 * i], loads[3 * i + 1], loads[3 * i + 2]);
+    if (j % every == 0) {
+        b.fem.compute_forces(b);
+        b.fem.solve(b, dt * (float)every, dt, theta, dissipation / (float)every);
+    } else {
+        const int nc = b.fem.held_begin(b, dt);
+        if (nc == 0) {
+            b.fem.hold(b, dt);
+        } else {
+            for (int c = 0; c < nc; c++) b.fem.held_component(b, c);
+            b.fem.held_end(b);
+        }
+    }
+    vec3 mn(1e30f), mx(-1e30f);
+    float max_v2 = 0;
+    b.integrate_nodes(0, b.nodes.size(), dt, mn, mx, max_v2);
+    b.fem.sync_positions(b, dt);
+    return 0;
+}
+
 // The plate's state: the body's nodes (p, inv_mass, v, mass), the frame's double positions, spins and orientations
 // (scattered to the body's node order: the frame numbers its nodes as the triangles first name them), the forces the
 // update left (3 per node), and the counters {solve_failures, clamps}. Any pointer may be null. Returns the frame's

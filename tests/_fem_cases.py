@@ -289,3 +289,73 @@ def plastic_loads(name, nn, steps, seed):
             ld[nn // 2, 1] += -6000.0 if half else 4000.0
         out.append((ld, tq))
     return out
+
+
+# ---------------------------------------------------------------------------------------------- held steps
+def held_cases():
+    """(name, maker(mod, ref), loads(nn, steps)) of the held-step runs: frames stepping their frame every few substeps
+    (WorldSettings::frame_every), elastic and plastic members, shells, the update-path plate."""
+    out = []
+    for name in ("cantilever", "portal", "stiffened_plate", "two_members"):
+        fr = frames()[name]
+        out.append((name, lambda mod, ref=False, fr=fr: make_frame(mod, fr, ref),
+                    lambda nn, steps, name=name: frame_loads(name, nn, steps, 3)))
+    for name in ("cantilever", "long", "plate"):
+        fr = plastic_frames()[name]
+        out.append(("plastic_" + name, lambda mod, ref=False, fr=fr: make_plastic(mod, fr, ref),
+                    lambda nn, steps, name=name: plastic_loads(name, nn, steps, 5)))
+    sc = events()
+    out.append(("events", lambda mod, ref=False, sc=sc: make(mod, sc, ref),
+                lambda nn, steps, sc=sc: [(ld, np.zeros((nn, 3), np.float32)) for ld in sc["loads"][:steps]]))
+    return out
+
+
+def strip_mesh(L, W, nu, nv, y=1.0):
+    """The reference's tri_plate (tests/test_physics.cpp:1499-1512): ShellMesher::grid from (0, y, 0) by (L / nu, 0, 0)
+    and (0, 0, W / nv), the nodes at x = 0 clamped: (positions, triangles, fixed)."""
+    origin = np.array([0, y, 0], np.float32)
+    du = np.array([np.float32(L) / np.float32(nu), 0, 0], np.float32)
+    dv = np.array([0, 0, np.float32(W) / np.float32(nv)], np.float32)
+    p = np.array([(origin + du * np.float32(i)) + dv * np.float32(j) for j in range(nv + 1) for i in range(nu + 1)],
+                 np.float32)
+    tris = []
+    for j in range(nv):
+        for i in range(nu):
+            n00 = j * (nu + 1) + i
+            n10, n01 = n00 + 1, n00 + nu + 1
+            n11 = n01 + 1
+            tris += [(n00, n10, n11), (n00, n11, n01)] if (i + j) & 1 else [(n00, n10, n01), (n10, n11, n01)]
+    return p, np.asarray(tris, np.uint32), (p[:, 0] < 1e-4).astype(np.uint8)
+
+
+def plastic_strip(L=0.5, W=0.2, t=0.01):
+    """The reference's plastic strip (tests/test_physics.cpp:1645-1686): 0.5 m x 0.2 m of 10 mm steel
+    (make_shell_section("Steel", 0.01): yield 350 MPa), 10 x 4 cells, clamped at its root; the masses as ShellMesher
+    puts them. Returns (frame, section, tip nodes, Pe, Pp): the tip's first-yield and collapse loads."""
+    sec = tuple(steel_section(t)) + (STEEL_YIELD,)
+    p, tr, fx = strip_mesh(L, W, 10, 4)
+    mass = np.zeros(len(p), np.float32)
+    mpa = np.float32(sec[2]) * np.float32(sec[3])
+    for c in tr:
+        a, b, d = p[c[0]], p[c[1]], p[c[2]]
+        n = np.cross(b - a, d - a).astype(np.float32)
+        area = np.float32(0.5) * np.float32(np.sqrt(np.float32(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])))
+        for v in c:
+            mass[v] = mass[v] + (mpa * area) / np.float32(3.0)
+    tip = np.nonzero(p[:, 0] > np.float32(L) - np.float32(1e-4))[0]
+    fy, tt = float(np.float32(STEEL_YIELD)), float(np.float32(t))
+    Pe, Pp = fy * tt * tt * W / 6.0 / L, fy * tt * tt * W / 4.0 / L
+    return dict(p=p, mass=mass, fixed=fx, tris=tr, shell=sec), tip, Pe, Pp
+
+
+def run_strip(P, tip, force, steps, damp):
+    """steps substeps of the strip (no gravity, theta 1, dissipation 1) with force/len(tip) down at each tip node and,
+    with damp, the test's drag -40 m v on every node (its pre_substep, from the velocities before the substep)."""
+    for _ in range(steps):
+        st = P.read()
+        ld = np.zeros((P.nn, 3), np.float32)
+        ld[tip, 1] += np.float32(-force) / np.float32(len(tip))
+        if damp:
+            ld -= st["v"] * (st["mass"] * np.float32(40.0))[:, None]
+        P.step(ld, H, theta=1.0, dissipation=1.0, g=(0.0, 0.0, 0.0))
+    return P.read()

@@ -18,7 +18,10 @@ Outputs (data only: inputs are regenerated from the seeds, outputs are the refer
                         at their hinges, a chain that spends the passes-again budget, plates whose shells flow) under
                         loads past their limits: the states and the plastic state after 20, 40, 60 steps
 
-`python tests/golden/make_golden.py plastic` writes fem_plastic_ref.npz alone.
+    fem_held_ref.npz    frames stepping their frame every 2 and 4 substeps (tests/_fem_cases.py's held cases): the
+                        states after 12, 24, 36 substeps
+
+`python tests/golden/make_golden.py plastic` writes fem_plastic_ref.npz and fem_held_ref.npz alone.
 """
 import hashlib
 import os
@@ -121,6 +124,21 @@ def plastic():
         R.close()
     np.savez_compressed(os.path.join(HERE, "fem_plastic_ref.npz"), **out)
     print("written fem_plastic_ref.npz")
+    out = {}
+    for every, start in ((2, 0), (4, 1)):
+        for name, make, loads in fc.held_cases():
+            R = make(oracle, ref=True)
+            for j, (ld, tq) in enumerate(loads(R.nn, 36)):
+                R.add_torques(tq)
+                R.cycle(ld, fc.H, every, j + start)
+                if j % 12 == 11:
+                    st = R.read()
+                    for k in fc.STATE_KEYS:
+                        out[f"{name}_{every}_{j}_{k}"] = st[k]
+                    out[f"{name}_{every}_{j}_counters"] = np.array([st["failures"], st["clamps"]])
+            R.close()
+    np.savez_compressed(os.path.join(HERE, "fem_held_ref.npz"), **out)
+    print("written fem_held_ref.npz")
 
 
 if __name__ == "__main__":
