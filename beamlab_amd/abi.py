@@ -175,6 +175,9 @@ SIGNATURES = {
     "bl_fem_solve": (ctypes.c_int, [_P, _P, _P]),
     "bl_fem_tri_k": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "bl_fem_get_plastic": (ctypes.c_int, [_P, _P, _P, _P]),
+    "bl_fem_step_span": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P,
+                                        ctypes.POINTER(FemReport)]),
+    "bl_fem_step_held": (ctypes.c_int, [_P, _P, ctypes.c_float, _P]),
     "bl_gnn_param_count": (ctypes.c_int, []),
     "bl_gnn_create": (ctypes.c_int, [ctypes.POINTER(_P), _P, ctypes.c_uint32]),
     "bl_gnn_destroy": (None, [_P]),

@@ -272,6 +272,11 @@ struct FemArgs {
     const float* loads;  // 3 per node
     float g[3];
     float h, theta, dissipation;
+    // the substep the step starts with (h: the step spans h / step substeps; the update and the integration are the
+    // substep's), the prediction of the smooth forces and moments for the held substeps (3 per node each)
+    float step;
+    float* pred;
+    float* tpred;
     // a pass again: the held impulses the first pass used (3 per node), the pass, the members' components and the
     // components a pass unloaded
     float* jsave;
@@ -905,6 +910,8 @@ __global__ void k_nodes(FemArgs a) {
     double rd[6] = {0, 0, 0, 0, 0, 0};
     D[0] = D[7] = D[14] = a.mass[i];
     D[21] = D[28] = D[35] = a.inertia[i];
+    // (the elements' share alone beside the node's sums: member_f, member_t_, src/phys/frame_fem.cpp:2080-2100)
+    f3 Fm = mk(0, 0, 0), Tm = mk(0, 0, 0);
     // its members' ends first (member order), then its triangles' corners (gather_lists, src/phys/frame_fem.cpp:862-898)
     for (uint32_t k = a.nc_off[i]; k < a.nc_off[i + 1]; k++) {
         const uint32_t e = a.nc[k];
@@ -913,7 +920,9 @@ __global__ void k_nodes(FemArgs a) {
             if (!a.mon[m]) continue;
             const float* fo = a.mfo + 9 * (size_t)m;
             F = end ? blg::sub(F, mk(fo[0], fo[1], fo[2])) : blg::add(F, mk(fo[0], fo[1], fo[2]));
+            Fm = end ? blg::sub(Fm, mk(fo[0], fo[1], fo[2])) : blg::add(Fm, mk(fo[0], fo[1], fo[2]));
             T = blg::add(T, end ? mk(fo[6], fo[7], fo[8]) : mk(fo[3], fo[4], fo[5]));
+            Tm = blg::add(Tm, end ? mk(fo[6], fo[7], fo[8]) : mk(fo[3], fo[4], fo[5]));
             member_block(a, m, (int)end, 1.0, h2, D);
             const double crm = hd + (double)a.mdamp[m] * h;
             if (crm != 0)
@@ -922,8 +931,10 @@ __global__ void k_nodes(FemArgs a) {
         }
         const uint32_t t = e / 3, corner = e % 3;
         if (!a.ton[t]) continue;
-        F = blg::add(F, mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]));
-        T = blg::add(T, mk(a.tto[9 * t + 3 * corner], a.tto[9 * t + 3 * corner + 1], a.tto[9 * t + 3 * corner + 2]));
+        const f3 tf = mk(a.tfo[9 * t + 3 * corner], a.tfo[9 * t + 3 * corner + 1], a.tfo[9 * t + 3 * corner + 2]);
+        const f3 tt = mk(a.tto[9 * t + 3 * corner], a.tto[9 * t + 3 * corner + 1], a.tto[9 * t + 3 * corner + 2]);
+        F = blg::add(F, tf), Fm = blg::add(Fm, tf);
+        T = blg::add(T, tt), Tm = blg::add(Tm, tt);
         tri_block(a, t, (int)corner, (int)corner, c, h2, D);
         if (cr != 0)
             for (int j = 0; j < 6; j++) rd[j] -= cr * a.tkv[18 * (size_t)t + 6 * corner + j];
@@ -931,15 +942,21 @@ __global__ void k_nodes(FemArgs a) {
     double* Dd = a.Dsys + 36 * (size_t)fi;
     for (int k = 0; k < 36; k++) Dd[k] = D[k];
     double* r = a.rhs + 6 * (size_t)fi;
-    // h f - (h - step) f_c + J, no contact forces, step = h; then the held impulse J = f (-(h - step)): +-0, or NaN
-    // after a non-finite force, carried into the next step (src/phys/frame_fem.cpp:1030-1042)
+    // h f - (h - step) f_c + J with no contact forces; then the held impulse (f - f_m - f_c) (-(h - step)) (+-0 for a
+    // step of one substep, or NaN after a non-finite force, carried into the next step) and the prediction of the
+    // substeps the step spans, f - f_m - f_c and the moments' m - t_m (src/phys/frame_fem.cpp:1026-1042)
     // (a pass again takes the impulses its first pass took: held aside in jsave)
+    const double hs = (double)a.h - (double)a.step;
     float* J = a.pass ? a.jsave + 3 * i : a.imp + 3 * i;
-    r[0] = h * (double)F.x - 0.0 * 0.0f + J[0], r[1] = h * (double)F.y - 0.0 * 0.0f + J[1];
-    r[2] = h * (double)F.z - 0.0 * 0.0f + J[2];
+    r[0] = h * (double)F.x - hs * 0.0f + J[0], r[1] = h * (double)F.y - hs * 0.0f + J[1];
+    r[2] = h * (double)F.z - hs * 0.0f + J[2];
     if (!a.pass) {
         if (a.jsave) a.jsave[3 * i] = J[0], a.jsave[3 * i + 1] = J[1], a.jsave[3 * i + 2] = J[2];
-        J[0] = F.x * -0.0f, J[1] = F.y * -0.0f, J[2] = F.z * -0.0f;
+        const f3 pr = blg::sub(blg::sub(F, Fm), mk(0, 0, 0)), tp = blg::sub(T, Tm);
+        const f3 Jn = blg::mul(pr, -(float)hs);
+        J[0] = Jn.x, J[1] = Jn.y, J[2] = Jn.z;
+        a.pred[3 * i] = pr.x, a.pred[3 * i + 1] = pr.y, a.pred[3 * i + 2] = pr.z;
+        a.tpred[3 * i] = tp.x, a.tpred[3 * i + 1] = tp.y, a.tpred[3 * i + 2] = tp.z;
     }
     r[3] = h * (double)T.x, r[4] = h * (double)T.y, r[5] = h * (double)T.z;
     for (int j = 0; j < 6; j++) r[j] += rd[j];
@@ -1603,38 +1620,12 @@ __global__ void k_sum_parts(const double* part, uint32_t n, double* out) {
 //    taken from the double one when within 1e-4 (1 + |p|), else the double one from it. (So (float)x == p after every
 //    step, and FemFrame::begin_forces's resync of x from p, :1867-1872, changes nothing: the host's set_state keeps
 //    the two in step.)
-__global__ void k_update(FemArgs a, const double* x, int explicit_step) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (explicit_step && i == 0) atomicAdd(&a.cnt[0], 1ull);
-    if (i >= a.nn) return;
-    a.torq[3 * i] = a.torq[3 * i + 1] = a.torq[3 * i + 2] = 0; // (FemFrame::solve_end: the moments from outside used up)
-    const int fi = a.fidx[i];
-    f3 v = mk(a.v[3 * i], a.v[3 * i + 1], a.v[3 * i + 2]);
-    f3 w = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
-    f3 F = mk(a.fnode[3 * i], a.fnode[3 * i + 1], a.fnode[3 * i + 2]);
-    if (fi < 0) {
-        w = mk(0, 0, 0);
-    } else if (explicit_step) {
-        a.imp[3 * i] = a.imp[3 * i + 1] = a.imp[3 * i + 2] = 0;
-    } else {
-        const double* xs = x + 6 * (size_t)fi;
-        f3 dv = mk((float)xs[0], (float)xs[1], (float)xs[2]);
-        const f3 dw = mk((float)xs[3], (float)xs[4], (float)xs[5]);
-        if (!__builtin_isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
-            atomicAdd(&a.cnt[0], 1ull);
-        } else {
-            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
-            float l = blg::length(dv);
-            if (l > kMaxDv) dv = blg::mul(dv, kMaxDv / l), atomicAdd(&a.cnt[1], 1ull);
-            F = blg::mul(dv, a.mass[i] / a.h);
-            w = blg::add(w, dw);
-            l = blg::length(w);
-            if (l > kMaxW) w = blg::mul(w, kMaxW / l), atomicAdd(&a.cnt[1], 1ull);
-        }
-        a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
-    }
+// integrate_nodes and sync_positions for node i over the substep (the force F, the spin w already updated)
+__device__ void integrate_sync(const FemArgs& a, uint32_t i, f3 F, f3 w) {
     a.w[3 * i] = w.x, a.w[3 * i + 1] = w.y, a.w[3 * i + 2] = w.z;
-    const float im = a.invm[i], h = a.h;
+    a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
+    f3 v = mk(a.v[3 * i], a.v[3 * i + 1], a.v[3 * i + 2]);
+    const float im = a.invm[i], h = a.step;
     f3 p = mk(a.pf[3 * i], a.pf[3 * i + 1], a.pf[3 * i + 2]);
     if (im > 0) {
         v = blg::add(v, blg::mul(F, im * h));
@@ -1661,6 +1652,99 @@ __global__ void k_update(FemArgs a, const double* x, int explicit_step) {
     else
         xd[0] = p.x, xd[1] = p.y, xd[2] = p.z;
     a.pf[3 * i] = p.x, a.pf[3 * i + 1] = p.y, a.pf[3 * i + 2] = p.z;
+    a.torq[3 * i] = a.torq[3 * i + 1] = a.torq[3 * i + 2] = 0; // (the moments from outside used up)
+}
+
+__global__ void k_update(FemArgs a, const double* x, int explicit_step) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (explicit_step && i == 0) atomicAdd(&a.cnt[0], 1ull);
+    if (i >= a.nn) return;
+    const int fi = a.fidx[i];
+    f3 w = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
+    f3 F = mk(a.fnode[3 * i], a.fnode[3 * i + 1], a.fnode[3 * i + 2]);
+    if (fi < 0) {
+        w = mk(0, 0, 0);
+    } else if (explicit_step) {
+        a.imp[3 * i] = a.imp[3 * i + 1] = a.imp[3 * i + 2] = 0;
+    } else {
+        const double* xs = x + 6 * (size_t)fi;
+        f3 dv = mk((float)xs[0], (float)xs[1], (float)xs[2]);
+        const f3 dw = mk((float)xs[3], (float)xs[4], (float)xs[5]);
+        if (!__builtin_isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
+            atomicAdd(&a.cnt[0], 1ull);
+        } else {
+            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
+            float l = blg::length(dv);
+            if (l > kMaxDv) dv = blg::mul(dv, kMaxDv / l), atomicAdd(&a.cnt[1], 1ull);
+            F = blg::mul(dv, a.mass[i] / a.step);
+            w = blg::add(w, dw);
+            l = blg::length(w);
+            if (l > kMaxW) w = blg::mul(w, kMaxW / l), atomicAdd(&a.cnt[1], 1ull);
+        }
+    }
+    integrate_sync(a, i, F, w);
+}
+
+// A held substep's right side (FemFrame::held_begin / held_rhs_col, src/phys/frame_fem.cpp:2342-2370): the node's
+// forces (gravity, the load) and moments from outside beyond the last step's prediction, times the substep, the
+// prediction's share of the held impulse paid; the forces kept for the update. r: 6 per free node.
+__global__ void k_held_rhs(FemArgs a, double* r) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nn) return;
+    f3 F = blg::mul(mk(a.g[0], a.g[1], a.g[2]), a.mass[i]);
+    if (a.loads) F = blg::add(F, mk(a.loads[3 * i], a.loads[3 * i + 1], a.loads[3 * i + 2]));
+    a.fnode[3 * i] = F.x, a.fnode[3 * i + 1] = F.y, a.fnode[3 * i + 2] = F.z;
+    const int fi = a.fidx[i];
+    if (fi < 0) return;
+    double* rr = r + 6 * (size_t)fi;
+    if (!(a.invm[i] > 0)) {
+        for (int j = 0; j < 6; j++) rr[j] = 0;
+        return;
+    }
+    const f3 pr = mk(a.pred[3 * i], a.pred[3 * i + 1], a.pred[3 * i + 2]);
+    const f3 tp = mk(a.tpred[3 * i], a.tpred[3 * i + 1], a.tpred[3 * i + 2]);
+    const f3 d = blg::mul(blg::sub(F, pr), a.step);
+    const f3 t = blg::mul(blg::sub(mk(a.torq[3 * i], a.torq[3 * i + 1], a.torq[3 * i + 2]), tp), a.step);
+    rr[0] = d.x, rr[1] = d.y, rr[2] = d.z, rr[3] = t.x, rr[4] = t.y, rr[5] = t.z;
+    const f3 J = blg::mul(pr, a.step);
+    a.imp[3 * i] += J.x, a.imp[3 * i + 1] += J.y, a.imp[3 * i + 2] += J.z;
+}
+
+// The end of a held substep (held_vel_col, src/phys/frame_fem.cpp:2371-2395): a free node's velocity change from the
+// last factorization's solution as the force dv m / step (a non-finite one: no force), its spin turned, |dv| <= 40 m/s
+// and |w| <= 3000 rad/s (counted); without a factorization FemFrame::hold (:2321-2340): the forces held over as
+// impulse. Then integrate_nodes and sync_positions.
+__global__ void k_held_update(FemArgs a, const double* x, int factored) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nn) return;
+    f3 w = mk(a.w[3 * i], a.w[3 * i + 1], a.w[3 * i + 2]);
+    f3 F = mk(a.fnode[3 * i], a.fnode[3 * i + 1], a.fnode[3 * i + 2]);
+    const int fi = a.fidx[i];
+    if (!factored) {
+        if (a.invm[i] > 0) {
+            const f3 fc = mk(0, 0, 0), J = blg::mul(blg::sub(F, fc), a.step);
+            a.imp[3 * i] += J.x, a.imp[3 * i + 1] += J.y, a.imp[3 * i + 2] += J.z;
+            const f3 v = blg::add(mk(a.v[3 * i], a.v[3 * i + 1], a.v[3 * i + 2]), blg::mul(fc, a.step * a.invm[i]));
+            a.v[3 * i] = v.x, a.v[3 * i + 1] = v.y, a.v[3 * i + 2] = v.z;
+            F = mk(0, 0, 0);
+        }
+    } else if (fi >= 0) {
+        const double* xs = x + 6 * (size_t)fi;
+        f3 dv = mk((float)xs[0], (float)xs[1], (float)xs[2]);
+        const f3 dw = mk((float)xs[3], (float)xs[4], (float)xs[5]);
+        if (!__builtin_isfinite(dv.x + dv.y + dv.z + dw.x + dw.y + dw.z)) {
+            F = mk(0, 0, 0);
+        } else {
+            const float kMaxDv = 40.0f, kMaxW = 3000.0f;
+            float l = blg::length(dv);
+            if (l > kMaxDv) dv = blg::mul(dv, kMaxDv / l), atomicAdd(&a.cnt[1], 1ull);
+            F = blg::mul(dv, a.mass[i] / a.step);
+            w = blg::add(w, dw);
+            l = blg::length(w);
+            if (l > kMaxW) w = blg::mul(w, kMaxW / l), atomicAdd(&a.cnt[1], 1ull);
+        }
+    }
+    integrate_sync(a, i, F, w);
 }
 
 } // namespace blf
@@ -1731,6 +1815,8 @@ struct bl_fem {
     // plastic rest rotations, damage, utilisation and curvature rows; the held impulses of a step's first pass; the
     // components (the members' own, their node counts); the passes-again budget and the statistics
     Dv<float> msec, mps, mtan, X0, th0, tdmg, tutil, Bc, jsave;
+    Dv<float> pred, tpred;  // the prediction of the smooth forces and moments for the held substeps
+    Dv<double> hrhs, hx;    // a held substep's right side and solution
     Dv<uint32_t> mbits, nyield, mcomp, again;
     std::vector<uint32_t> comp_size;
     uint32_t ncomp = 0;
@@ -1796,7 +1882,7 @@ struct bl_fem {
         a.nm = nm, a.mem = mem.p, a.mL0 = mL0.p, a.mqa = mqa.p, a.mqb = mqb.p, a.mst = mst.p, a.mtw = mtw.p;
         a.mdamp = mdamp.p, a.mon = mon.p, a.mfo = mfo.p, a.me1 = me1.p, a.mW = mW.p, a.mkv = mkv.p;
         a.msec = msec.p, a.mps = mps.p, a.mtan = mtan.p, a.mbits = mbits.p, a.nyield = nyield.p, a.mcomp = mcomp.p;
-        a.again = again.p, a.jsave = jsave.p;
+        a.again = again.p, a.jsave = jsave.p, a.pred = pred.p, a.tpred = tpred.p;
         a.tri = tri.p, a.X = X.p, a.r0 = r0.p, a.Kf = Kf.p, a.Kd = Kd.p;
         a.X0 = X0.p, a.th0 = th0.p, a.tdmg = tdmg.p, a.tutil = tutil.p, a.Bc = Bc.p;
         a.te = te.p, a.tfo = tfo.p, a.tto = tto.p, a.tg = tg.p, a.tkv = tkv.p, a.ton = ton.p;
@@ -1804,7 +1890,7 @@ struct bl_fem {
         a.Dsys = Dsys.p, a.rhs = rhs.p, a.Osys = Osys.p, a.pairs = pairs.p, a.pcon = pcon.p, a.npairs = npairs;
         a.loads = dloads;
         if (g) a.g[0] = g[0], a.g[1] = g[1], a.g[2] = g[2];
-        a.h = h, a.theta = theta, a.dissipation = dis;
+        a.h = h, a.theta = theta, a.dissipation = dis, a.step = h;
         return a;
     }
     FacArgs fac() const {
@@ -2339,7 +2425,8 @@ int fem_build(bl_fem* f, const bl_fem_desc* d) {
         (rc = f->again.put(std::vector<uint32_t>(f->ncomp, 0u))) || (rc = f->X0.put(X)) ||
         (rc = f->th0.put(std::vector<float>(9 * (size_t)nt, 0.0f))) || (rc = f->tdmg.put(std::vector<float>(nt, 0.0f))) ||
         (rc = f->tutil.put(std::vector<float>(nt, 0.0f))) || (rc = f->Bc.alloc(shell_yield ? 54 * (size_t)nt : 0)) ||
-        (rc = f->jsave.alloc(3 * (size_t)nn)))
+        (rc = f->jsave.alloc(3 * (size_t)nn)) || (rc = f->pred.put(std::vector<float>(3 * (size_t)nn, 0.0f))) ||
+        (rc = f->tpred.put(std::vector<float>(3 * (size_t)nn, 0.0f))))
         return rc;
     if ((rc = f->mem.put(mem)) || (rc = f->mL0.put(mL0)) || (rc = f->mqa.put(mqa)) || (rc = f->mqb.put(mqb)) ||
         (rc = f->mst.put(mst)) || (rc = f->mtw.put(mtw)) || (rc = f->mdamp.put(mdamp)) || (rc = f->mon.alloc(nm)) ||
@@ -2441,7 +2528,8 @@ FemView fem_view(bl_fem* f) {
     return v;
 }
 
-int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], float* ms) {
+int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3], float* ms,
+                 float step) {
     const hipStream_t st = f->st;
     const float* dl = nullptr;
     // (theta and the dissipation as FemFrame::solve_begin takes them, src/phys/frame_fem.cpp:2524-2525)
@@ -2453,6 +2541,7 @@ int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float diss
     }
     FCHK(hipMemcpyAsync(f->xprev.p, f->x.p, sizeof(double) * 6 * f->nf, hipMemcpyDeviceToDevice, st));
     FemArgs a = f->args(dl, h, theta, dissipation, g);
+    if (step > 0) a.step = step;
     // the passes-again budget (FemFrame::begin_forces, src/phys/frame_fem.cpp:1863-1864): one more every fourth step,
     // at most 8; with less than one left the yielding members start from their elastic tangents
     f->repass_credit = std::min(8.0f, f->repass_credit + 0.25f);
@@ -2675,11 +2764,16 @@ void bl_fem_destroy(bl_fem* f) { delete f; }
 
 int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
                 bl_fem_report* rep) {
-    if (!f || !g || !(h > 0)) return BL_ERR_ARG;
+    return bl_fem_step_span(f, loads, h, h, theta, dissipation, g, rep);
+}
+
+int bl_fem_step_span(bl_fem* f, const float* loads, float h, float step, float theta, float dissipation,
+                     const float g[3], bl_fem_report* rep) {
+    if (!f || !g || !(h > 0) || !(step > 0) || !(step <= h)) return BL_ERR_ARG;
     float msa = 0, msf = 0, mss = 0, msu = 0;
     int fail = 0, rc;
     double res = 0;
-    if ((rc = blf::fem_assemble(f, loads, h, theta, dissipation, g, &msa))) return rc;
+    if ((rc = blf::fem_assemble(f, loads, h, theta, dissipation, g, &msa, step))) return rc;
     if ((rc = blf::fem_cholesky(f, &msf, &mss, &fail))) return rc;
     int passes = 1, repasses = 0;
     if (!fail && f->plastic_members && (rc = blf::fem_repasses(f, &msf, &mss, &fail, &passes, &repasses))) return rc;
@@ -2697,6 +2791,44 @@ int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissi
         rep->assemble_ms = msa, rep->factor_ms = msf, rep->solve_ms = mss, rep->update_ms = msu;
         rep->failures = fail ? 1 : 0;
         rep->passes = passes, rep->repasses = repasses;
+    }
+    return BL_OK;
+}
+
+int bl_fem_step_held(bl_fem* f, const float* loads, float step, const float g[3]) {
+    if (!f || !g || !(step > 0)) return BL_ERR_ARG;
+    const hipStream_t st = f->st;
+    const float* dl = nullptr;
+    if (loads) {
+        FCHK(hipMemcpyAsync(f->loads.p, loads, sizeof(float) * 3 * f->nn, hipMemcpyHostToDevice, st));
+        dl = f->loads.p;
+    }
+    if (f->hrhs.n != 6 * (size_t)f->nf) {
+        int rc;
+        if ((rc = f->hrhs.alloc(6 * (size_t)f->nf)) || (rc = f->hx.alloc(6 * (size_t)f->nf))) return rc;
+    }
+    FemArgs a = f->args(dl, step, 0, 0, g);
+    const bool factored = blf::fem_has_factor(f);
+    const uint32_t nb = (f->nn + 255) / 256;
+    if (factored) {
+        hipLaunchKernelGGL(blf::k_held_rhs, dim3(nb), dim3(256), 0, st, a, f->hrhs.p);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, f->fail.p, 1u);
+        enqueue_solve(f, f->hrhs.p, f->hx.p);
+    } else {
+        hipLaunchKernelGGL(blf::k_held_rhs, dim3(nb), dim3(256), 0, st, a, f->hrhs.p); // (the forces alone)
+    }
+    hipLaunchKernelGGL(blf::k_held_update, dim3(nb), dim3(256), 0, st, a, (const double*)f->hx.p, factored ? 1 : 0);
+    FCHK(hipGetLastError());
+    if (factored) {
+        int err = 0;
+        int rc = blf::fem_solve_error(f, &err);
+        if (rc) return rc;
+        if (err) {
+            blg::set_last_error("a held substep's substitution timed out");
+            return BL_ERR_STATE;
+        }
+    } else {
+        FCHK(hipStreamSynchronize(st));
     }
     return BL_OK;
 }

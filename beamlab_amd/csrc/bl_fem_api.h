@@ -26,7 +26,7 @@ struct FemView {
 FemView fem_view(bl_fem* f);
 // the element pass and the assembly of a step (the previous solution kept as xprev)
 int fem_assemble(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
-                 float* ms);
+                 float* ms, float step = 0.0f);
 // x := A^-1 rhs by the supernodal Cholesky; *fail: bit 1 a pivot not positive, bit 2 a wait timed out
 int fem_cholesky(bl_fem* f, float* ms_factor, float* ms_solve, int* fail);
 // whether the factor buffer holds a successful factorization (of this step's system or an earlier one)

@@ -113,6 +113,19 @@ class ShellBody:
                                         ctypes.byref(self.report)), "bl_fem_step")
         return self.report.as_dict()
 
+    def cycle(self, loads, dt, every, j, theta=0.5, dissipation=0.02, g=(0.0, -9.81, 0.0)):
+        """Substep j of a frame stepping its frame every `every` substeps (WorldSettings::frame_every): at
+        j % every == 0 the implicit step over every dt (bl_fem_step_span, the dissipation divided by every), else the
+        held substep (bl_fem_step_held)."""
+        ld = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(-1)
+        gg = np.asarray(g, np.float32)
+        if j % every == 0:
+            abi.check(self._lib.bl_fem_step_span(self._h, abi.ptr(ld), np.float32(dt) * np.float32(every), dt, theta,
+                                                 np.float32(dissipation) / np.float32(every), abi.ptr(gg),
+                                                 ctypes.byref(self.report)), "bl_fem_step_span")
+        else:
+            abi.check(self._lib.bl_fem_step_held(self._h, abi.ptr(ld), dt, abi.ptr(gg)), "bl_fem_step_held")
+
     def state(self):
         x = np.zeros((self.nn, 3), np.float64)
         v = np.zeros((self.nn, 3), np.float32)

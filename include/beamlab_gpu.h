@@ -294,8 +294,9 @@ int bl_time_kernels(bl_ctx* ctx, int32_t substeps, float* total_ms, float* kerne
  * src/phys/frame_fem.h:154-181) and Timoshenko members (FemFrame::elems: src/phys/frame_fem.h:197-224, eval_forces
  * src/phys/frame_fem.cpp:1896-2070) - FemFrame::solve (src/phys/frame_fem.cpp:2514-3027). Replaces SoftBody::fem for a
  * body that may deform and does not break (SoftBody::allow_break off: members and triangles yield - the hinges, bars
- * and twists of members, the membranes and bending of shells - but are never split, torn or let go), with no mounts,
- * contacts or held steps. The system - 6x6 blocks per node pair, FP64 - is factored each step by a supernodal
+ * and twists of members, the membranes and bending of shells - but are never split, torn or let go), with no mounts
+ * or contacts; its implicit step may span several substeps, the held substeps between answered through its
+ * factorization. The system - 6x6 blocks per node pair, FP64 - is factored each step by a supernodal
  * (multifrontal) Cholesky over a nested-dissection order, its dense fronts on FP64 matrix cores; a step that unloads
  * a yielding member is assembled, factored and solved again with its elastic tangent (FemFrame::solve_impl's passes,
  * src/phys/frame_fem.cpp:2959-2996, on their budget, :1863-1864).
@@ -368,6 +369,18 @@ void bl_fem_destroy(bl_fem* f);
  * does; BL_ERR_STATE only for the device's own trouble (a substitution's wait timed out). */
 int bl_fem_step(bl_fem* f, const float* loads, float h, float theta, float dissipation, const float g[3],
                 bl_fem_report* rep);
+/* One implicit step spanning h of which the first substep, of `step`, is integrated: a frame stepping every h / step
+ * substeps (WorldSettings::frame_every; FemFrame::solve(b, h, step, ...), src/phys/frame_fem.cpp:2514-2545). The
+ * forces are taken for the whole of h; the held impulse and the prediction of the smooth forces for the substeps
+ * between are kept (:1026-1042), the update and integration are over `step`. The world passes the dissipation per
+ * step divided by the substeps the step spans (src/phys/world.cpp:3247-3252). bl_fem_step is this with step = h. */
+int bl_fem_step_span(bl_fem* f, const float* loads, float h, float step, float theta, float dissipation,
+                     const float g[3], bl_fem_report* rep);
+/* A held substep of `step` between two implicit steps (FemFrame::held_begin / held_component / held_end,
+ * src/phys/frame_fem.cpp:2342-2512): the forces (gravity g, the loads) and moments from outside beyond the last step's
+ * prediction through that step's factorization, the velocity changes as forces, then integrate_nodes and
+ * sync_positions; without a factorization (none yet, or the last one failed) FemFrame::hold (:2321-2340). */
+int bl_fem_step_held(bl_fem* f, const float* loads, float step, const float g[3]);
 /* The state: positions (double, 3 per node), v, w (3 per node), q (4 per node, x y z w); any pointer may be null. */
 int bl_fem_get_state(bl_fem* f, double* x, float* v, float* w, float* q);
 int bl_fem_set_state(bl_fem* f, const double* x, const float* v, const float* w, const float* q);
