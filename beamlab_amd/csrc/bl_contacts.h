@@ -10,12 +10,17 @@
 namespace blg {
 
 // A narrow-phase hit (src/phys/world.cpp:2807-2962): the candidate's index in the concatenated lists, the capsule
-// parameter, the triangle's barycentric weights, the normal (from the other side towards the first) and the depth.
+// parameter, the triangle's barycentric weights, the normal (from the other side towards the first) and the depth;
+// and what the response reads of its candidate, written by the narrow phase, which has it at hand: the contact's
+// nodes (side a's, then side b's), its friction and its shape (nodes on side a | on side b << 4 | hull push-out << 8).
 struct Hit {
     uint32_t pair;
     float s;
     f3 bary, nrm;
     float pen;
+    uint32_t n[5];
+    float fric;
+    uint32_t shape;
 };
 
 // A pair island's search parameters of the current rebuild (rebuild_pairs, src/phys/world.cpp:1879-1935).

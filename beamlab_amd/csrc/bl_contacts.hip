@@ -1333,29 +1333,16 @@ __device__ __forceinline__ void set_slots(ContactNodes& cn, uint32_t n0, uint32_
     cn.w[0] = w0, cn.w[1] = w1, cn.w[2] = w2, cn.w[3] = w3, cn.w[4] = w4;
 }
 
-__device__ __forceinline__ void contact_nodes(const KArgs& a, const PArgs& p, const Hit& h, ContactNodes& cn) {
-    if (h.pair < p.nnt) {
-        const uint4 pr = p.nt[h.pair];
-        const uint4 t = a.tri_abc[pr.w];
-        cn.na = 1, cn.nb = 3;
-        set_slots(cn, pr.z, t.x, t.y, t.z, 0u, 1.0f, h.bary.x, h.bary.y, h.bary.z, 0.0f);
-        // (a tyre node would take its own friction: tyre nodes are refused when a body is added)
-        cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
-        cn.hull = a.tri_hull[pr.w] && h.pen > a.bodies[pr.x].collision_radius;
-    } else if (h.pair < p.nnt + p.nnc) {
-        const uint4 pr = p.nc[h.pair - p.nnt];
-        const uint4 cap = a.cap_data[pr.w];
-        cn.na = 1, cn.nb = 2, cn.hull = false;
-        set_slots(cn, pr.z, cap.x, cap.y, 0u, 0u, 1.0f, 1 - h.s, h.s, 0.0f, 0.0f);
-        cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
-    } else {
-        const uint4 pr = p.ct[h.pair - p.nnt - p.nnc];
-        const uint4 cap = a.cap_data[pr.z];
-        const uint4 t = a.tri_abc[pr.w];
-        cn.na = 2, cn.nb = 3, cn.hull = false;
-        set_slots(cn, cap.x, cap.y, t.x, t.y, t.z, 1 - h.s, h.s, h.bary.x, h.bary.y, h.bary.z);
-        cn.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
-    }
+// (from the hit alone: the narrow phase wrote the candidate's nodes, friction and shape into it)
+__device__ __forceinline__ void contact_nodes(const Hit& h, ContactNodes& cn) {
+    cn.na = (int)(h.shape & 15u), cn.nb = (int)((h.shape >> 4) & 15u), cn.hull = (h.shape >> 8) & 1u;
+    cn.fric = h.fric;
+    if (cn.na == 1 && cn.nb == 3) // node-triangle
+        set_slots(cn, h.n[0], h.n[1], h.n[2], h.n[3], 0u, 1.0f, h.bary.x, h.bary.y, h.bary.z, 0.0f);
+    else if (cn.na == 1) // node-capsule
+        set_slots(cn, h.n[0], h.n[1], h.n[2], 0u, 0u, 1.0f, 1 - h.s, h.s, 0.0f, 0.0f);
+    else // capsule-triangle
+        set_slots(cn, h.n[0], h.n[1], h.n[2], h.n[3], h.n[4], 1 - h.s, h.s, h.bary.x, h.bary.y, h.bary.z);
 }
 
 // the concrete model of the inter-body contacts, as compile-time constants
@@ -1492,6 +1479,9 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
     __shared__ uint16_t wfirst[kRespBlk];
     __shared__ uint32_t wblk[kRespBlk];
     __shared__ uint32_t wpre[kRespBlk + 1], wtot[kRespBlk / 64], wnb;
+    __shared__ uint4 wcut[8]; // the window's blocks that a run's start or end cuts (at most two a run): block, start, end, hits
+    __shared__ uint2 wcutr[8];
+    __shared__ uint32_t wcut_n;
     PArgs p = pin;
     p.nnt = pin.st->nnt, p.nnc = pin.st->nnc, p.nct = pin.st->nct;
     if (pin.st->nfnc) p.nc = pin.ncm; // (the node-capsule list merged with the fast bodies')
@@ -1512,34 +1502,45 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
             for (uint32_t wb = 0; wb < nv;) {
                 // the next blocks' hits within [lo, hi) (hits are in candidate order in a block), their prefix, and how
                 // many of the blocks fit one window (at least one)
-                uint32_t cntv = 0;
+                // (a block a run's start or end cuts - at most two a run - has its hits before the cut counted by a
+                // wave, all of its hits' candidates read at once: one round of loads, not a binary search's chain)
+                uint32_t cntv = 0, f = 0, e = 0, b = 0;
+                int cut = -1;
+                if (tid == 0) wcut_n = 0;
+                __syncthreads();
                 if (tid < (uint32_t)kRespBlk) {
                     const uint32_t v = wb + tid;
                     const int L = v < sv0[1] ? 0 : (v < sv0[2] ? 1 : 2);
-                    const uint32_t b = sb0[L] + (v - sv0[L]), lo = slo[L], hi = shi[L];
-                    uint32_t f = 0, e = 0;
+                    const uint32_t lo = slo[L], hi = shi[L];
+                    b = sb0[L] + (v - sv0[L]);
                     if (v < nv) {
-                        const Hit* H = p.hraw + (size_t)b * 256;
                         e = p.bcnt[b];
-                        if (b * 256 < lo) {
-                            uint32_t l2 = 0, h2 = e;
-                            while (l2 < h2) {
-                                const uint32_t mid = (l2 + h2) >> 1;
-                                if (H[mid].pair < lo) l2 = mid + 1;
-                                else h2 = mid;
-                            }
-                            f = l2;
-                        }
-                        if (b * 256 + 256 > hi) {
-                            uint32_t l2 = f, h2 = e;
-                            while (l2 < h2) {
-                                const uint32_t mid = (l2 + h2) >> 1;
-                                if (H[mid].pair < hi) l2 = mid + 1;
-                                else h2 = mid;
-                            }
-                            e = l2;
+                        if (b * 256 < lo || b * 256 + 256 > hi) {
+                            cut = (int)atomicAdd(&wcut_n, 1u);
+                            wcut[cut] = make_uint4(b, b * 256 < lo ? lo : 0u, b * 256 + 256 > hi ? hi : 0xffffffffu, e);
                         }
                     }
+                }
+                __syncthreads();
+                for (uint32_t k = wv; k < wcut_n; k += kRespW / 64) {
+                    const uint4 x = wcut[k];
+                    uint32_t pr[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const uint32_t h = lane + 64 * j;
+                        pr[j] = h < x.w ? p.hraw[(size_t)x.x * 256 + h].pair : 0xffffffffu;
+                    }
+                    uint32_t cl = 0, ch = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        cl += (uint32_t)__builtin_popcountll(__ballot(pr[j] < x.y));
+                        ch += (uint32_t)__builtin_popcountll(__ballot(pr[j] < x.z));
+                    }
+                    if (lane == 0) wcutr[k] = make_uint2(cl, ch);
+                }
+                __syncthreads();
+                if (tid < (uint32_t)kRespBlk) {
+                    if (cut >= 0) f = wcutr[cut].x, e = wcutr[cut].y;
                     wfirst[tid] = (uint16_t)f, wblk[tid] = b, cntv = e - f;
                 }
                 uint32_t x = cntv;
@@ -1572,7 +1573,7 @@ __global__ __launch_bounds__(kRespW) void k_response(KArgs a, PArgs pin, int par
                         else h2 = mid - 1;
                     }
                     hit = p.hraw[(size_t)wblk[l2] * 256 + wfirst[l2] + (tid - wpre[l2])];
-                    contact_nodes(a, p, hit, cn);
+                    contact_nodes(hit, cn);
                 }
                 if (m > 0) {
                     // the window's slots hashed by node into buckets (count, scan, scatter): a slot finds the other

@@ -89,6 +89,7 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
     const float4* Vv = a.vm[parity];
     Hit h;
     h.pair = i, h.s = 0, h.bary = mk(0, 0, 0), h.nrm = mk(0, 0, 0), h.pen = 0;
+    h.n[0] = h.n[1] = h.n[2] = h.n[3] = h.n[4] = 0, h.fric = 0, h.shape = 0;
     bool hit = false;
     if (i >= n) {
         // (past the end: no hit; the block's compaction below needs every thread)
@@ -98,6 +99,10 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
             const uint4 tr = a.tri_abc[pr.w];
             const BodyParam& Bt = a.bodies[pr.y];
             const bool two_sided = !a.tri_hull[pr.w];
+            // (collide_pairs' contact of the candidate: src/phys/world.cpp:2997-3017; a tyre node would take its own
+            // friction - tyre nodes are refused when a body is added)
+            h.n[0] = pr.z, h.n[1] = tr.x, h.n[2] = tr.y, h.n[3] = tr.z;
+            h.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, Bt.contact_friction);
             const float r = Bt.faces_only && !two_sided ? Bt.face_skin : a.bodies[pr.x].collision_radius;
             const f3 np = ld3(P[pr.z]), ap = ld3(P[tr.x]), bp = ld3(P[tr.y]), cp3 = ld3(P[tr.z]);
             bool go;
@@ -140,10 +145,14 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
                     hit = true;
                 }
             }
+            // (a node behind a hull face deeper than its contact radius is pushed out: hull_push_out)
+            h.shape = 1u | 3u << 4 | (!two_sided && h.pen > a.bodies[pr.x].collision_radius ? 1u << 8 : 0u);
         }
     } else if (i < p.nnt + p.nnc) {
         const uint4 pr = p.nc[i - p.nnt];
         const uint4 cap = a.cap_data[pr.w];
+        h.n[0] = pr.z, h.n[1] = cap.x, h.n[2] = cap.y, h.shape = 1u | 2u << 4;
+        h.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
         const f3 np = ld3(P[pr.z]), ap = ld3(P[cap.x]), bp = ld3(P[cap.y]);
         const float t = closest_on_segment(np, ap, bp);
         const f3 cp = add(ap, mul(sub(bp, ap), t));
@@ -161,6 +170,8 @@ __device__ __forceinline__ void narrow_block(const KArgs& a, const PArgs& pin, i
         const uint4 pr = p.ct[i - p.nnt - p.nnc];
         const uint4 cap = a.cap_data[pr.z];
         const uint4 tri = a.tri_abc[pr.w];
+        h.n[0] = cap.x, h.n[1] = cap.y, h.n[2] = tri.x, h.n[3] = tri.y, h.n[4] = tri.z, h.shape = 2u | 3u << 4;
+        h.fric = 0.8f * smin(a.bodies[pr.x].contact_friction, a.bodies[pr.y].contact_friction);
         const f3 p0 = ld3(P[cap.x]), p1 = ld3(P[cap.y]);
         const f3 ta = ld3(P[tri.x]), tb = ld3(P[tri.y]), tc = ld3(P[tri.z]);
         float best = 1e30f, best_s = 0;
