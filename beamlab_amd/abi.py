@@ -24,7 +24,15 @@ TRIANGLE = np.dtype([("a", "<u4"), ("b", "<u4"), ("c", "<u4"), ("surface", "u1")
 CAPSULE = np.dtype([("a", "<u4"), ("b", "<u4"), ("radius", "<f4"), ("joint", "<i4")])
 BOX = np.dtype([("center", "<f4", 3), ("rot", "<f4", 9), ("half", "<f4", 3), ("aabb_mn", "<f4", 3),
                 ("aabb_mx", "<f4", 3), ("max_force", "<f4"), ("surface", "<u4")])
+SHOCK = np.dtype([("beam", "<u4"), ("type", "<u4"), ("soft_bump", "<u4")] +
+                 [(n, "<f4") for n in ("spring", "damp", "spring_in", "damp_in", "prog_spring_in", "prog_damp_in",
+                                       "spring_out", "damp_out", "prog_spring_out", "prog_damp_out", "damp_in_slow",
+                                       "split_in", "damp_in_fast", "damp_out_slow", "split_out", "damp_out_fast",
+                                       "short_bound", "long_bound", "bound_spring", "bound_damp")])
+SLIDE = np.dtype([("node", "<u4"), ("rail0", "<u4"), ("nrail", "<u4"), ("seg", "<i4"), ("k", "<f4"),
+                  ("threshold", "<f4"), ("break_force", "<f4"), ("broken", "<u4")])
 assert NODE.itemsize == 32 and NODE_INFO.itemsize == 20 and BEAM.itemsize == 52
+assert SHOCK.itemsize == 92 and SLIDE.itemsize == 32
 assert TRIANGLE.itemsize == 20 and CAPSULE.itemsize == 16 and BOX.itemsize == 92
 
 # NodeFlag (src/phys/softbody.h:29-40)
@@ -60,7 +68,9 @@ class BodyDesc(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_void_p), ("info", ctypes.c_void_p), ("n_nodes", ctypes.c_uint32),
                 ("beams", ctypes.c_void_p), ("n_beams", ctypes.c_uint32), ("tris", ctypes.c_void_p),
                 ("n_tris", ctypes.c_uint32), ("capsules", ctypes.c_void_p), ("n_capsules", ctypes.c_uint32),
-                ("opts", BodyOpts)]
+                ("opts", BodyOpts), ("shocks", ctypes.c_void_p), ("n_shocks", ctypes.c_uint32),
+                ("slides", ctypes.c_void_p), ("n_slides", ctypes.c_uint32), ("rail", ctypes.c_void_p),
+                ("n_rail", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -155,6 +165,7 @@ SIGNATURES = {
     "bl_read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_body_broken_beams": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64)]),
+    "bl_read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "bl_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "bl_reset_stats": (ctypes.c_int, [_P]),
     "bl_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),

@@ -32,8 +32,12 @@ struct HostBody {
     std::vector<bl_beam> beams;
     std::vector<bl_triangle> tris;
     std::vector<bl_capsule> caps;
+    std::vector<bl_shock> shocks;
+    std::vector<bl_slide> slides; // (seg and broken: the step's state)
+    std::vector<uint32_t> rail;
     bl_body_opts opts;
     uint32_t node0 = 0, beam0 = 0, tri0 = 0, cap0 = 0;
+    uint32_t slide0 = 0; // its first slide in the device array
     // as of the next frame's start: top speed of the last frame and bounds (World::build_islands reads them)
     float max_speed = 0;
     f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
@@ -132,6 +136,13 @@ struct bl_ctx {
     DevArr<uint8_t> tri_hull;
     DevArr<unsigned int> cbox;
     DevArr<uint4> wstat; // per workgroup of the node-parallel integration: its counts of the frame
+    // shocks and slide nodes (XArgs): bodies that have them step in the split form (forces, these, integration)
+    DevArr<uint4> xbody, shx;
+    DevArr<bl_shock> xsh;
+    DevArr<bl_slide> xsl;
+    DevArr<uint32_t> xrail;
+    XArgs xargs{};
+    std::map<std::pair<int, int>, hipGraphExec_t> sgraphs; // the split form's captured substeps
     ContactEngine contacts;
     HostView hv;
     bool contact_geometry = false; // some body has triangles, capsules or self collision
@@ -165,6 +176,8 @@ struct bl_ctx {
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
+        for (auto& kv : sgraphs) (void)hipGraphExecDestroy(kv.second);
+        sgraphs.clear();
         drop_segments();
     }
     void release() {
@@ -178,6 +191,8 @@ struct bl_ctx {
         wg_chunk.release();
         F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), tri_hull.release(), cbox.release();
         wstat.release();
+        xbody.release(), shx.release(), xsh.release(), xsl.release(), xrail.release();
+        xargs = XArgs{};
         contacts.release();
         frame_pending = false;
         fin_queued = false;
@@ -213,6 +228,13 @@ bool unsupported_opts(const bl_body_opts& o) {
 }
 
 // Device state -> host copies of every body (nodes, beam state), so that a rebuild starts from the current state.
+// some body has shocks or slide nodes: the world steps in the split form (F holds the node forces between phases)
+bool has_extras(const bl_ctx* c) {
+    for (const HostBody& hb : c->bodies)
+        if (!hb.shocks.empty() || !hb.slides.empty()) return true;
+    return false;
+}
+
 int pull_state(bl_ctx* c) {
     if (!c->built) return BL_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -233,8 +255,12 @@ int pull_state(bl_ctx* c) {
     if (B) HIPCHK(hipMemcpy(S.data(), c->stress.p, sizeof(float) * B, hipMemcpyDeviceToHost));
     std::vector<uint32_t> torn(c->tri_torn.n);
     if (!torn.empty()) HIPCHK(hipMemcpy(torn.data(), c->tri_torn.p, 4 * torn.size(), hipMemcpyDeviceToHost));
+    std::vector<bl_slide> sl(c->xsl.n);
+    if (!sl.empty()) HIPCHK(hipMemcpy(sl.data(), c->xsl.p, sizeof(bl_slide) * sl.size(), hipMemcpyDeviceToHost));
     for (HostBody& hb : c->bodies) {
         for (size_t t = 0; t < hb.tris.size() && hb.tri0 + t < torn.size(); t++) hb.tris[t].torn = torn[hb.tri0 + t] ? 1 : 0;
+        for (size_t q = 0; q < hb.slides.size() && hb.slide0 + q < sl.size(); q++)
+            hb.slides[q].seg = sl[hb.slide0 + q].seg, hb.slides[q].broken = sl[hb.slide0 + q].broken;
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const float4 p = P[hb.node0 + i], v = V[hb.node0 + i];
             bl_node& n = hb.nodes[i];
@@ -328,6 +354,7 @@ int build(bl_ctx* c) {
     std::vector<uint2> iAB(E);
     std::vector<float> stress(B);
     c->a_slot.assign(B, 0xffffffffu);
+    std::vector<uint32_t> b_slot(B, 0xffffffffu);
     for (size_t k = 0; k < c->bodies.size(); k++) {
         const HostBody& hb = c->bodies[k];
         const bl_body_opts& o = hb.opts;
@@ -351,6 +378,7 @@ int build(bl_ctx* c) {
                 iM[e] = make_float4(b.max_pos, b.max_neg, u2f(gb), u2f(tf | (side ? TF_ISB : 0u)));
                 iAB[e] = make_uint2(ga, gbn);
                 if (!side) c->a_slot[gb] = e;
+                else b_slot[gb] = e;
             }
         }
     }
@@ -538,7 +566,7 @@ int build(bl_ctx* c) {
         (rc = c->iloc.upload(iloc)) || (rc = c->flags.alloc(std::max<size_t>(1, nchunks))) ||
         (rc = c->tri_abc.upload(tri_abc)) || (rc = c->tri_rest2.upload(tri_rest2)) || (rc = c->tri_torn.upload(tri_torn)) || (rc = c->tri_hull.upload(tri_hull)) ||
         (rc = c->cap_data.upload(cap_data)) || (rc = c->wg_chunk.upload(wg_chunk)) ||
-        (rc = c->F.alloc(c->contact_geometry ? N : 0)) || (rc = c->wstat.alloc((N + 255) / 256 + 1)) ||
+        (rc = c->F.alloc(c->contact_geometry || has_extras(c) ? N : 0)) || (rc = c->wstat.alloc((N + 255) / 256 + 1)) ||
         (rc = c->cbox.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)))
         return rc;
     if (c->contact_geometry && (rc = c->contacts.setup(c->hv, N, NT, NC))) {
@@ -569,6 +597,38 @@ int build(bl_ctx* c) {
     a.fch_rng = c->fch_rng.p, a.fwg = c->fwg.p, a.nfchunks = nf;
     a.wg_rng = c->wg_rng.p, a.fwg_rng = c->fwg_rng.p;
     c->parity = 0;
+    // shocks and slide nodes: per body with any its ranges; per shock its beam's two incidences and nodes; slides
+    // with global nodes and rails
+    {
+        std::vector<uint4> xb, sx;
+        std::vector<bl_shock> sh;
+        std::vector<bl_slide> sl;
+        std::vector<uint32_t> rail;
+        for (HostBody& hb : c->bodies) {
+            hb.slide0 = (uint32_t)sl.size();
+            if (hb.shocks.empty() && hb.slides.empty()) continue;
+            const uint32_t s0 = (uint32_t)sh.size(), l0 = (uint32_t)sl.size();
+            for (const bl_shock& x : hb.shocks) {
+                const bl_beam& bm = hb.beams[x.beam];
+                const uint32_t gb = hb.beam0 + x.beam;
+                sh.push_back(x);
+                sx.push_back(make_uint4(c->a_slot[gb], b_slot[gb], hb.node0 + bm.a, hb.node0 + bm.b));
+            }
+            for (const bl_slide& x : hb.slides) {
+                bl_slide y = x;
+                y.node = hb.node0 + x.node;
+                y.rail0 = (uint32_t)rail.size() + x.rail0;
+                sl.push_back(y);
+            }
+            for (uint32_t r : hb.rail) rail.push_back(hb.node0 + r);
+            xb.push_back(make_uint4(s0, (uint32_t)sh.size(), l0, (uint32_t)sl.size()));
+        }
+        if ((!xb.empty() && (rc = c->xbody.upload(xb))) || (!sx.empty() && (rc = c->shx.upload(sx))) ||
+            (!sh.empty() && (rc = c->xsh.upload(sh))) || (!sl.empty() && (rc = c->xsl.upload(sl))) ||
+            (!rail.empty() && (rc = c->xrail.upload(rail))))
+            return rc;
+        c->xargs = XArgs{c->xbody.p, (uint32_t)xb.size(), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p};
+    }
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;
     return BL_OK;
@@ -745,15 +805,18 @@ int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk
     for (int q = 0; q < len; q++) {
         if (q > 0 && t.block == 256) {
             if ((rc = launch_forces_narrow(a, c->stream, parity, t, p, nblk))) return rc;
+            if ((rc = launch_extras(a, c->xargs, c->stream, parity))) return rc;
             if ((rc = c->contacts.launch_response(a, p, nblk, c->stream, parity))) return rc;
         } else {
-            if (q > 0 && (rc = launch_forces(a, c->stream, parity, t))) return rc;
+            if (q > 0 && ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity))))
+                return rc;
             if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
         }
         if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p))) return rc;
         parity ^= 1;
     }
-    if (tail && (rc = launch_forces(a, c->stream, parity, t))) return rc;
+    if (tail && ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity))))
+        return rc;
     return BL_OK;
 }
 
@@ -841,6 +904,7 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
     if (kev) HIPCHK(hipEventRecord(kev[0], c->stream));
     if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
     if (kev) HIPCHK(hipEventRecord(kev[1], c->stream));
+    if ((rc = launch_extras(a, c->xargs, c->stream, c->parity))) return rc;
     for (int s = 0; s < substeps;) {
         // each body's top speed so far (the frame's start included) and, after the first substep, its bounds now
         if (s > 0 && (rc = launch_box(ab, c->stream, c->parity))) return rc;
@@ -883,13 +947,16 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
                     if ((rc = c->contacts.narrow_args(np, nnb))) return rc;
                     if (nnb) {
                         c->contacts.count_tests(1);
-                        if ((rc = launch_forces_narrow(a, c->stream, c->parity, t, np, nnb))) return rc;
+                        if ((rc = launch_forces_narrow(a, c->stream, c->parity, t, np, nnb)) ||
+                            (rc = launch_extras(a, c->xargs, c->stream, c->parity)))
+                            return rc;
                         continue;
                     }
                 }
                 if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1)], c->stream));
                 if ((rc = launch_forces(a, c->stream, c->parity, t))) return rc;
                 if (kev) HIPCHK(hipEventRecord(kev[2 * (s + 1) + 1], c->stream));
+                if ((rc = launch_extras(a, c->xargs, c->stream, c->parity))) return rc;
             }
         }
     }
@@ -910,13 +977,61 @@ bool needs_contacts(bl_ctx* c, int substeps) {
     return c->contacts.npair_islands() > 0;
 }
 
+// The split form without contacts, for worlds with shocks or slide nodes (their forces join the beams' between the
+// force phase and the integration): per substep the force phase into F, the shocks and slides, the static contacts
+// and the integration; up to 32 substeps a captured graph.
+int enqueue_split(bl_ctx* c, int substeps) {
+    if (c->F.n < c->args.nnodes || c->wstat.n < (c->args.nnodes + 255) / 256) {
+        g_last_error = "internal: the split form's force and count buffers are not allocated";
+        return BL_ERR_STATE;
+    }
+    bl_tuning t = c->tune;
+    t.block = c->built_block, t.incidences_per_thread = c->built_per;
+    const KArgs& a = c->args;
+    int rc;
+    auto one = [&](int parity) -> int {
+        if ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity)) ||
+            (rc = launch_integrate(a, c->stream, parity, c->wstat.p)))
+            return rc;
+        return BL_OK;
+    };
+    for (int s = 0; s < substeps; s += 32) {
+        const int k = std::min(32, substeps - s);
+        if (!t.use_graph) {
+            for (int q = 0; q < k; q++)
+                if ((rc = one(c->parity ^ (q & 1)))) return rc;
+        } else {
+            auto key = std::make_pair(k, c->parity);
+            auto it = c->sgraphs.find(key);
+            if (it == c->sgraphs.end()) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int r2 = BL_OK;
+                for (int q = 0; q < k && !r2; q++) r2 = one(c->parity ^ (q & 1));
+                const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+                if (r2) return r2;
+                HIPCHK(ce);
+                hipGraphExec_t ex;
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHK(ie);
+                it = c->sgraphs.emplace(key, ex).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second, c->stream));
+        }
+        c->parity ^= (k & 1);
+    }
+    c->substeps += substeps;
+    return launch_wstat_fold(a, c->stream, c->wstat.p);
+}
+
 // Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.
 int enqueue_frame(bl_ctx* c, int substeps) {
     if (substeps <= 0) return BL_OK;
     int rc;
     if (needs_contacts(c, substeps)) return contact_frame(c, substeps);
     if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
-    if ((rc = enqueue(c, substeps))) return rc;
+    if ((rc = c->xargs.nbody ? enqueue_split(c, substeps) : enqueue(c, substeps))) return rc;
     if ((rc = launch_frame_end(c->args, c->stream, c->parity))) return rc;
     return frame_enqueued(c);
 }
@@ -1193,6 +1308,16 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
         const bl_triangle& tr = d->tris[t];
         if (tr.a >= d->n_nodes || tr.b >= d->n_nodes || tr.c >= d->n_nodes) return BL_ERR_ARG;
     }
+    if ((d->n_shocks && !d->shocks) || (d->n_slides && !d->slides) || (d->n_rail && !d->rail)) return BL_ERR_ARG;
+    for (uint32_t q = 0; q < d->n_shocks; q++)
+        if (d->shocks[q].beam >= d->n_beams || d->shocks[q].type < 1 || d->shocks[q].type > 3) return BL_ERR_ARG;
+    for (uint32_t r = 0; r < d->n_rail; r++)
+        if (d->rail[r] >= d->n_nodes) return BL_ERR_ARG;
+    for (uint32_t q = 0; q < d->n_slides; q++) {
+        const bl_slide& s = d->slides[q];
+        if (s.node >= d->n_nodes || (uint64_t)s.rail0 + s.nrail > d->n_rail || (s.nrail >= 2 && s.seg >= (int32_t)s.nrail - 1))
+            return BL_ERR_ARG;
+    }
     for (uint32_t q = 0; q < d->n_capsules; q++) {
         const bl_capsule& cp = d->capsules[q];
         if (cp.a >= d->n_nodes || cp.b >= d->n_nodes) return BL_ERR_ARG;
@@ -1211,6 +1336,9 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     hb.beams.assign(d->beams, d->beams + d->n_beams);
     if (d->n_tris) hb.tris.assign(d->tris, d->tris + d->n_tris);
     if (d->n_capsules) hb.caps.assign(d->capsules, d->capsules + d->n_capsules);
+    if (d->n_shocks) hb.shocks.assign(d->shocks, d->shocks + d->n_shocks);
+    if (d->n_slides) hb.slides.assign(d->slides, d->slides + d->n_slides);
+    if (d->n_rail) hb.rail.assign(d->rail, d->rail + d->n_rail);
     hb.opts = d->opts;
     hb.compute_bounds(); // (SoftBody::compute_aabb at World::add_body, src/phys/world.cpp:283-289)
     c->bodies.push_back(std::move(hb));
@@ -1273,6 +1401,17 @@ int bl_write_body(bl_ctx* c, uint32_t id, const bl_node* nodes, const bl_beam* b
     if ((rc = build(c))) return rc;
     HIPCHK(hipMemcpy(c->counters.p, cnt.data(), sizeof(unsigned long long) * CNT_N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->broken.p, brk.data(), sizeof(unsigned long long) * brk.size(), hipMemcpyHostToDevice));
+    return BL_OK;
+}
+
+int bl_read_slides(bl_ctx* c, uint32_t id, bl_slide* out) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    const HostBody& hb = c->bodies[id];
+    if (out && !hb.slides.empty()) std::memcpy(out, hb.slides.data(), sizeof(bl_slide) * hb.slides.size());
     return BL_OK;
 }
 
@@ -1406,6 +1545,10 @@ int bl_time_kernels(bl_ctx* c, int32_t substeps, float* total_ms, float* kernel_
         *kernel_ms = sum;
         *launches = substeps;
         return post_frame(c);
+    }
+    if (c->xargs.nbody) {
+        g_last_error = "the forms' kernel timing covers worlds without shocks or slide nodes (those step in the split form)";
+        return BL_ERR_UNSUPPORTED;
     }
     const int per = t.mode == BL_MODE_PERSISTENT ? (t.substeps_per_launch > 0 ? t.substeps_per_launch : 64)
                     : (t.mode == BL_MODE_RESIDENT || t.mode == BL_MODE_HALO)

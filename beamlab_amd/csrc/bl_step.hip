@@ -987,6 +987,194 @@ int launch_integrate(const KArgs& a, hipStream_t st, int parity, uint4* wstat) {
     return BL_OK_OR_HIP();
 }
 
+// ---------------------------------------------------------------------------------------------- shocks, slides
+// SoftBody::compute_shock_forces (src/phys/softbody.cpp:472-572), one shock. The beam's L, strength and flags are
+// its a-end incidence's; a break marks both incidences.
+__device__ __forceinline__ void shock_prog(const bl_shock& s, bool out, float diff, float L, float& kk, float& dd) {
+    if (out) {
+        kk = s.spring_out;
+        dd = s.damp_out;
+        const float fr = s.long_bound != 0 ? smin((diff / (s.long_bound * L)) * (diff / (s.long_bound * L)), 1.0f) : 1.0f;
+        kk += s.prog_spring_out * kk * fr;
+        dd += s.prog_damp_out * dd * fr;
+    } else {
+        kk = s.spring_in;
+        dd = s.damp_in;
+        const float fr = s.short_bound != 0 ? smin((diff / (s.short_bound * L)) * (diff / (s.short_bound * L)), 1.0f) : 1.0f;
+        kk += s.prog_spring_in * kk * fr;
+        dd += s.prog_damp_in * dd * fr;
+    }
+}
+
+__device__ void shock_eval(const KArgs& a, const bl_shock& s, const uint4 x, int parity, uint32_t body) {
+    float4 M = a.iM[x.x];
+    const uint32_t tf = f2u(M.w);
+    if (tf & TF_BROKEN) return;
+    const float4 A = a.iA[x.x];
+    const float4 Pa = a.pm[parity][x.z], Pb = a.pm[parity][x.w], Va = a.vm[parity][x.z], Vb = a.vm[parity][x.w];
+    const f3 dis = sub(mk(Pa.x, Pa.y, Pa.z), mk(Pb.x, Pb.y, Pb.z));
+    const float len2 = dot(dis, dis);
+    if (len2 < 1e-12f) return;
+    const float inv_len = 1.0f / fsqrt(len2);
+    const float len = len2 * inv_len;
+    const float L = A.x;
+    const float diff = len - L;
+    const float v = dot(sub(mk(Va.x, Va.y, Va.z), mk(Vb.x, Vb.y, Vb.z)), dis) * inv_len;
+    float k, d;
+    const float lb = s.long_bound * L, sb = s.short_bound * L;
+    if (s.type == 1) {
+        k = s.spring;
+        d = s.damp;
+        float r = 0;
+        if (diff > lb) r = diff - lb;
+        else if (diff < -sb) r = -diff - sb;
+        if (r != 0) {
+            r = smin(r, 1.0f);
+            k += (s.bound_spring - k) * r;
+            d += (s.bound_damp - d) * r;
+        }
+    } else if (s.type == 2) {
+        shock_prog(s, v > 0, diff, L, k, d);
+        if (s.soft_bump) {
+            const float Lp = 0.8f * L, lpre = s.long_bound * Lp, spre = -s.short_bound * Lp;
+            if (diff > lpre) {
+                shock_prog(s, true, diff, L, k, d);
+                const float q = (diff - lpre) * 5.0f / (s.long_bound * L);
+                const float f2 = s.long_bound != 0 ? smin(q * q, 1.0f) : 1.0f;
+                k += (k + 100) * s.prog_spring_out * f2;
+                d += (d + 100) * s.prog_damp_out * f2;
+                if (v < 0) k = s.spring_in, d = s.damp_in;
+            } else if (diff < spre) {
+                shock_prog(s, false, diff, L, k, d);
+                const float q = (diff - spre) * 5.0f / (s.short_bound * L);
+                const float f2 = s.short_bound != 0 ? smin(q * q, 1.0f) : 1.0f;
+                k += (k + 100) * s.prog_spring_out * f2;
+                d += (d + 100) * s.prog_damp_out * f2;
+                if (v > 0) k = s.spring_out, d = s.damp_out;
+            }
+            if (diff > lb || diff < -sb) {
+                k = smax(k, s.bound_spring);
+                d = smax(d, s.bound_damp);
+            }
+        } else if (diff > lb || diff < -sb) {
+            k = s.bound_spring;
+            d = s.bound_damp;
+        }
+    } else {
+        k = s.spring_in;
+        d = s.damp_in;
+        if (diff > lb) {
+            const float r = smin(diff - lb, 1.0f);
+            k += (s.bound_spring - k) * r;
+            d += (s.bound_damp - d) * r;
+        } else if (diff < -sb) {
+            const float r = smin(-diff - sb, 1.0f);
+            k += (s.bound_spring - k) * r;
+            d += (s.bound_damp - d) * r;
+        } else if (v > 0) {
+            const float w = clampf(ffabs(v), 0.15f, 20.0f);
+            k = s.spring_out;
+            d = (s.damp_out * s.damp_out_slow * smin(w, s.split_out) + s.damp_out * s.damp_out_fast * smax(0.0f, w - s.split_out)) / w;
+        } else if (v < 0) {
+            const float w = clampf(ffabs(v), 0.15f, 20.0f);
+            k = s.spring_in;
+            d = (s.damp_in * s.damp_in_slow * smin(w, s.split_in) + s.damp_in * s.damp_in_fast * smax(0.0f, w - s.split_in)) / w;
+        }
+    }
+    const float slen = -k * diff - d * v;
+    if ((tf & TF_BRK) && ffabs(slen) > A.w && !(tf & TF_NOBREAK)) {
+        M.w = u2f(tf | TF_BROKEN);
+        a.iM[x.x] = M;
+        float4 Mb = a.iM[x.y];
+        Mb.w = u2f(f2u(Mb.w) | TF_BROKEN);
+        a.iM[x.y] = Mb;
+        atomicAdd(&a.broken[body], 1ull);
+        return;
+    }
+    a.stress[f2u(M.z)] = slen;
+    const f3 fv = mul(dis, slen * inv_len);
+    float4 Fa = a.F[x.z], Fb = a.F[x.w];
+    Fa = make_float4(Fa.x + fv.x, Fa.y + fv.y, Fa.z + fv.z, Fa.w);
+    a.F[x.z] = Fa;
+    Fb = a.F[x.w]; // (the same node at both ends reads the force just written, as the reference's f[] does)
+    Fb = make_float4(Fb.x - fv.x, Fb.y - fv.y, Fb.z - fv.z, Fb.w);
+    a.F[x.w] = Fb;
+}
+
+// SoftBody::compute_slide_forces (src/phys/softbody.cpp:574-623), one slide node
+__device__ __forceinline__ float slide_seg_d2(const float4* P, const uint32_t* rail, int i, f3 p, float& t) {
+    const f3 a = ld3(P[rail[i]]), b = ld3(P[rail[i + 1]]), ab = sub(b, a);
+    const float l2 = dot(ab, ab);
+    t = l2 > 1e-12f ? clampf(dot(sub(p, a), ab) / l2, 0.0f, 1.0f) : 0.0f;
+    const f3 q = sub(add(a, mul(ab, t)), p);
+    return dot(q, q);
+}
+
+__device__ void slide_eval(const KArgs& a, bl_slide& s, const uint32_t* rail_all, int parity) {
+    if (s.broken || s.nrail < 2) return;
+    const float4* P = a.pm[parity];
+    const uint32_t* rail = rail_all + s.rail0;
+    const f3 p = ld3(P[s.node]);
+    const int nseg = (int)s.nrail - 1;
+    float t, best_t = 0;
+    if (s.seg < 0) {
+        float best = 1e30f;
+        for (int i = 0; i < nseg; i++) {
+            const float dd = slide_seg_d2(P, rail, i, p, t);
+            if (dd < best) best = dd, s.seg = i, best_t = t;
+        }
+        if (s.seg < 0) return; // (every distance not a number: no nearest segment - the reference would read rail[-1])
+    }
+    float best = slide_seg_d2(P, rail, s.seg, p, best_t);
+    int bi = s.seg;
+    if (s.seg > 0) {
+        const float dd = slide_seg_d2(P, rail, s.seg - 1, p, t);
+        if (dd < best) best = dd, bi = s.seg - 1, best_t = t;
+    }
+    if (s.seg + 1 < nseg) {
+        const float dd = slide_seg_d2(P, rail, s.seg + 1, p, t);
+        if (dd < best) best = dd, bi = s.seg + 1, best_t = t;
+    }
+    s.seg = bi;
+    const uint32_t na = rail[bi], nb = rail[bi + 1];
+    const f3 pa = ld3(P[na]);
+    const f3 ideal = add(pa, mul(sub(ld3(P[nb]), pa), best_t));
+    const f3 d = sub(ideal, p);
+    const float dist = length(d);
+    const float stretch = dist - s.threshold;
+    if (stretch <= 0 || dist < 1e-9f) return;
+    const f3 F = mul(d, s.k * stretch / dist);
+    if (s.break_force > 0 && length(F) > s.break_force) {
+        s.broken = 1;
+        return;
+    }
+    float4 f = a.F[s.node];
+    a.F[s.node] = make_float4(f.x + F.x, f.y + F.y, f.z + F.z, f.w);
+    const f3 Fa = mul(F, 1.0f - best_t), Fb = mul(F, best_t);
+    f = a.F[na];
+    a.F[na] = make_float4(f.x - Fa.x, f.y - Fa.y, f.z - Fa.z, f.w);
+    f = a.F[nb];
+    a.F[nb] = make_float4(f.x - Fb.x, f.y - Fb.y, f.z - Fb.z, f.w);
+}
+
+// a thread per body with shocks or slides: its shocks, then its slides, in order
+__global__ void k_extras(KArgs a, XArgs x, int parity) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= x.nbody) return;
+    const uint4 r = x.body[i];
+    for (uint32_t q = r.x; q < r.y; q++) shock_eval(a, x.sh[q], x.shx[q], parity, a.ninfo[x.shx[q].z] >> 16);
+    for (uint32_t q = r.z; q < r.w; q++) {
+        bl_slide s = x.sl[q];
+        slide_eval(a, s, x.rail, parity);
+        x.sl[q] = s;
+    }
+}
+
+int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
+    if (x.nbody) hipLaunchKernelGGL(k_extras, dim3((x.nbody + 63) / 64), dim3(64), 0, st, a, x, parity);
+    return BL_OK_OR_HIP();
+}
+
 int launch_wstat_fold(const KArgs& a, hipStream_t st, uint4* wstat) {
     const uint32_t n = (a.nnodes + 255) / 256;
     if (n) hipLaunchKernelGGL(k_wstat_fold, dim3((n + 255) / 256), dim3(256), 0, st, a, wstat, n);

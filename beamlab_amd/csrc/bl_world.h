@@ -97,6 +97,18 @@ struct KArgs {
     const uint4* fwg_rng;
 };
 
+// The bodies' shocks and slide nodes (the forces that follow the beams in World::simulate_island's base_forces,
+// src/phys/world.cpp:3255-3273): run after the force phase, into F, a thread per body with any, in the body's order.
+struct XArgs {
+    const uint4* body;    // per such body: first shock, end shock, first slide, end slide
+    uint32_t nbody;
+    const bl_shock* sh;   // the shocks (beam: the body's beam index, unused here)
+    const uint4* shx;     // per shock: its beam's a-end and b-end incidences, its a and b nodes
+    bl_slide* sl;         // the slides (node, rail0 global); seg and broken updated here
+    const uint32_t* rail; // the rails' nodes (global)
+};
+int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
+
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
 int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,
                     int* err, uint32_t spin_limit);

@@ -22,6 +22,9 @@ class SoftBody:
         self.beams = np.zeros(0, abi.BEAM)
         self.tris = np.zeros(0, abi.TRIANGLE)
         self.capsules = np.zeros(0, abi.CAPSULE)
+        self.shocks = np.zeros(0, abi.SHOCK)
+        self.slides = np.zeros(0, abi.SLIDE)
+        self.rail = np.zeros(0, np.uint32)
         self.opts = abi.default_body_opts()
         self._pending_nodes, self._pending_beams, self._pending_tris = [], [], []
 
@@ -40,6 +43,30 @@ class SoftBody:
     def add_triangle(self, a, b, c, two_sided=True):
         self._pending_tris.append((a, b, c, two_sided))
         return len(self.tris) + len(self._pending_tris) - 1
+
+    # a shock on a beam (the reference's truck loader makes them: Shock, src/phys/softbody.h:88-101); the beam is
+    # flagged BF_SHOCK, so that the beam loop leaves it to the shock loop. Unnamed fields keep the reference's defaults.
+    def add_shock(self, beam, type_=1, **fields):
+        self._flush()
+        s = np.zeros(1, abi.SHOCK)
+        for n, v in dict(damp_in_slow=1, split_in=1, damp_in_fast=1, damp_out_slow=1, split_out=1, damp_out_fast=1,
+                         bound_spring=9e6, bound_damp=12000).items():
+            s[0][n] = v
+        s[0]["beam"], s[0]["type"] = beam, type_
+        for n, v in fields.items():
+            s[0][n] = v
+        self.beams[beam]["flags"] |= abi.BF_SHOCK
+        self.shocks = np.concatenate([self.shocks, s])
+        return len(self.shocks) - 1
+
+    # a slide node on a rail of nodes (SlideNode, src/phys/softbody.h:559-567)
+    def add_slide(self, node, rail, k=9e6, threshold=0.0, break_force=0.0):
+        s = np.zeros(1, abi.SLIDE)
+        s[0]["node"], s[0]["rail0"], s[0]["nrail"], s[0]["seg"] = node, len(self.rail), len(rail), -1
+        s[0]["k"], s[0]["threshold"], s[0]["break_force"] = k, threshold, break_force
+        self.rail = np.concatenate([self.rail, np.asarray(rail, np.uint32)])
+        self.slides = np.concatenate([self.slides, s])
+        return len(self.slides) - 1
 
     def _flush(self):
         if self._pending_nodes:
@@ -96,6 +123,9 @@ class SoftBody:
         d.tris, d.n_tris = abi.ptr(self.tris), len(self.tris)
         d.capsules, d.n_capsules = abi.ptr(self.capsules), len(self.capsules)
         d.opts = self.opts
+        d.shocks, d.n_shocks = abi.ptr(self.shocks), len(self.shocks)
+        d.slides, d.n_slides = abi.ptr(self.slides), len(self.slides)
+        d.rail, d.n_rail = abi.ptr(self.rail), len(self.rail)
         return d
 
 
@@ -190,6 +220,11 @@ class World:
 
     def write_body(self, bid, nodes=None, beams=None):
         abi.check(self._lib().bl_write_body(self._context(), bid, abi.ptr(nodes), abi.ptr(beams)), "bl_write_body")
+
+    def read_slides(self, bid):
+        out = self.bodies[bid].slides.copy()
+        abi.check(self._lib().bl_read_slides(self._context(), bid, abi.ptr(out)), "bl_read_slides")
+        return out
 
     def broken_beams(self, bid):
         v = ctypes.c_int64(0)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BL_ABI_VERSION 3
+#define BL_ABI_VERSION 4
 
 enum {
     BL_OK = 0,
@@ -114,6 +114,34 @@ typedef struct bl_capsule {
     int32_t joint;
 } bl_capsule;
 
+/* A shock absorber (== bl::phys::Shock, src/phys/softbody.h:88-101, without the frame seats): the beam it drives is
+ * flagged BL_BF_SHOCK, so that the beam loop leaves it to the shock loop (SoftBody::compute_shock_forces,
+ * src/phys/softbody.cpp:472-572), which runs after the beams. */
+typedef struct bl_shock {
+    uint32_t beam;           /* index into the body's beams */
+    uint32_t type;           /* 1 shocks (bounded wheel beams), 2 shocks2, 3 shocks3 */
+    uint32_t soft_bump;      /* shocks2 's' */
+    float spring, damp;      /* type 1, in bounds */
+    float spring_in, damp_in, prog_spring_in, prog_damp_in;
+    float spring_out, damp_out, prog_spring_out, prog_damp_out;
+    float damp_in_slow, split_in, damp_in_fast, damp_out_slow, split_out, damp_out_fast;
+    float short_bound, long_bound; /* fractions of L */
+    float bound_spring, bound_damp; /* stiffness at the bump stops */
+} bl_shock;
+
+/* A slide node (== bl::phys::SlideNode, src/phys/softbody.h:559-567; SoftBody::compute_slide_forces,
+ * src/phys/softbody.cpp:574-623): a spring from `node` to the nearest point of its rail, the rail's nodes being
+ * rail[rail0 .. rail0 + nrail) of the body description; `seg` and `broken` are the state the step updates. */
+typedef struct bl_slide {
+    uint32_t node;
+    uint32_t rail0, nrail;
+    int32_t seg;             /* current segment (its first rail node's index in the rail); -1: search the whole rail */
+    float k;                 /* default 9e6 */
+    float threshold;         /* free play (m) */
+    float break_force;       /* 0: unbreakable */
+    uint32_t broken;
+} bl_slide;
+
 /* A static oriented box (== StaticBox, src/phys/static_world.h:115-126, without hull planes). `rot` holds the
  * columns of the rotation (the box's local axes in world space), as bl::mat3 does; aabb_mn/aabb_mx as
  * StaticBox::update_aabb (src/phys/static_world.cpp:242-246) computes them (bl_box_from_quat fills both). */
@@ -179,6 +207,14 @@ typedef struct bl_body_desc {
     const bl_capsule* capsules;
     uint32_t n_capsules;
     bl_body_opts opts;
+    /* shocks and slide nodes, stepped after the beams in this order (World::simulate_island's base_forces,
+     * src/phys/world.cpp:3255-3273); null with a zero count for none */
+    const bl_shock* shocks;
+    uint32_t n_shocks;
+    const bl_slide* slides;
+    uint32_t n_slides;
+    const uint32_t* rail;    /* the slides' rail nodes */
+    uint32_t n_rail;
 } bl_body_desc;
 
 /* Counters (WorldStats / BodyStats / FemFrame counters: src/phys/world.h:79-132, src/phys/softbody.h:588-593). */
@@ -270,8 +306,12 @@ int bl_read_body(bl_ctx* ctx, uint32_t id, bl_node* nodes_out, bl_beam* beams_ou
 /* Overwrites a body's node and beam state (a snapshot restore; sizes as added; either pointer may be null). */
 int bl_write_body(bl_ctx* ctx, uint32_t id, const bl_node* nodes, const bl_beam* beams);
 
-/* A body's broken beams since it was added (BodyStats::broken_beams, src/phys/softbody.h:589). */
+/* A body's broken beams since it was added (BodyStats::broken_beams, src/phys/softbody.h:589), shocks that broke
+ * included. */
 int bl_body_broken_beams(bl_ctx* ctx, uint32_t id, int64_t* out);
+
+/* A body's slide nodes as the step left them (SlideNode::seg, SlideNode::broken; the other fields as added). */
+int bl_read_slides(bl_ctx* ctx, uint32_t id, bl_slide* slides_out);
 
 /* Counters since bl_create (or the last bl_reset_stats). */
 int bl_get_stats(bl_ctx* ctx, bl_stats* out);

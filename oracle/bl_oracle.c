@@ -115,6 +115,11 @@ typedef struct {
     bl_capsule* caps;
     uint32_t ncap;
     float hull_reach; /* SoftBody::hull_reach (src/phys/softbody.h:751-758): hull_depth when any triangle is one-sided */
+    bl_shock* shocks;
+    uint32_t nsh;
+    bl_slide* slides;
+    uint32_t nsl;
+    uint32_t* rail;
 } orc_body;
 
 struct orc_world {
@@ -144,6 +149,9 @@ void orc_destroy(orc_world* w) {
         free(w->bodies[i].force);
         free(w->bodies[i].tris);
         free(w->bodies[i].caps);
+        free(w->bodies[i].shocks);
+        free(w->bodies[i].slides);
+        free(w->bodies[i].rail);
     }
     free(w->bodies);
     free(w->boxes);
@@ -175,6 +183,13 @@ int orc_add_body(orc_world* w, const bl_body_desc* d) {
     b->caps = (bl_capsule*)malloc(sizeof(bl_capsule) * (b->ncap ? b->ncap : 1));
     if (b->ntri) memcpy(b->tris, d->tris, sizeof(bl_triangle) * b->ntri);
     if (b->ncap) memcpy(b->caps, d->capsules, sizeof(bl_capsule) * b->ncap);
+    b->nsh = d->n_shocks, b->nsl = d->n_slides;
+    b->shocks = (bl_shock*)malloc(sizeof(bl_shock) * (b->nsh ? b->nsh : 1));
+    b->slides = (bl_slide*)malloc(sizeof(bl_slide) * (b->nsl ? b->nsl : 1));
+    b->rail = (uint32_t*)malloc(sizeof(uint32_t) * (d->n_rail ? d->n_rail : 1));
+    if (b->nsh) memcpy(b->shocks, d->shocks, sizeof(bl_shock) * b->nsh);
+    if (b->nsl) memcpy(b->slides, d->slides, sizeof(bl_slide) * b->nsl);
+    if (d->n_rail) memcpy(b->rail, d->rail, sizeof(uint32_t) * d->n_rail);
     int hulls = 0;
     for (uint32_t t = 0; t < b->ntri; t++) hulls += !b->tris[t].two_sided;
     b->hull_reach = hulls > 0 ? b->opts.hull_depth : 0.0f;
@@ -239,6 +254,173 @@ static void beam_forces(orc_body* B) {
         v3 fv = vmul(dis, slen * inv_len);
         f[b->a] = vadd(f[b->a], fv);
         f[b->b] = vsub(f[b->b], fv);
+    }
+}
+
+/* SoftBody::compute_shock_forces (src/phys/softbody.cpp:472-572): each shock's beam in shock order, after the beams */
+static inline float sqrf(float x) { return x * x; }
+typedef struct { const bl_shock* s; float diff, L; } shock_ctx;
+/* shocks2's progressive rate, one way (the prog lambda, src/phys/softbody.cpp:503-517) */
+static void shock_prog(const shock_ctx* c, int out, float* kk, float* dd) {
+    const bl_shock* s = c->s;
+    if (out) {
+        *kk = s->spring_out;
+        *dd = s->damp_out;
+        float fr = s->long_bound != 0 ? smin(sqrf(c->diff / (s->long_bound * c->L)), 1.0f) : 1.0f;
+        *kk += s->prog_spring_out * *kk * fr;
+        *dd += s->prog_damp_out * *dd * fr;
+    } else {
+        *kk = s->spring_in;
+        *dd = s->damp_in;
+        float fr = s->short_bound != 0 ? smin(sqrf(c->diff / (s->short_bound * c->L)), 1.0f) : 1.0f;
+        *kk += s->prog_spring_in * *kk * fr;
+        *dd += s->prog_damp_in * *dd * fr;
+    }
+}
+
+static void shock_forces(orc_body* B) {
+    bl_node* nd = B->nodes;
+    v3* f = B->force;
+    for (uint32_t q = 0; q < B->nsh; q++) {
+        const bl_shock* s = &B->shocks[q];
+        bl_beam* b = &B->beams[s->beam];
+        if (b->flags & BL_BF_BROKEN) continue;
+        const bl_node* na = &nd[b->a];
+        const bl_node* nb = &nd[b->b];
+        v3 dis = vsub(ld3(na->p), ld3(nb->p));
+        float len2 = vdot(dis, dis);
+        if (len2 < 1e-12f) continue;
+        float inv_len = 1.0f / sqrtf(len2);
+        float len = len2 * inv_len;
+        const float L = b->L;
+        float diff = len - L;
+        float v = vdot(vsub(ld3(na->v), ld3(nb->v)), dis) * inv_len;
+        float k, d;
+        const float lb = s->long_bound * L, sb = s->short_bound * L;
+        if (s->type == 1) {
+            k = s->spring;
+            d = s->damp;
+            float r = 0;
+            if (diff > lb) r = diff - lb;
+            else if (diff < -sb) r = -diff - sb;
+            if (r != 0) {
+                r = smin(r, 1.0f);
+                k += (s->bound_spring - k) * r;
+                d += (s->bound_damp - d) * r;
+            }
+        } else if (s->type == 2) {
+            const shock_ctx c = {s, diff, L};
+            shock_prog(&c, v > 0, &k, &d);
+            if (s->soft_bump) {
+                float Lp = 0.8f * L, lpre = s->long_bound * Lp, spre = -s->short_bound * Lp;
+                if (diff > lpre) {
+                    shock_prog(&c, 1, &k, &d);
+                    float f2 = s->long_bound != 0 ? smin(sqrf((diff - lpre) * 5.0f / (s->long_bound * L)), 1.0f) : 1.0f;
+                    k += (k + 100) * s->prog_spring_out * f2;
+                    d += (d + 100) * s->prog_damp_out * f2;
+                    if (v < 0) { k = s->spring_in; d = s->damp_in; }
+                } else if (diff < spre) {
+                    shock_prog(&c, 0, &k, &d);
+                    float f2 = s->short_bound != 0 ? smin(sqrf((diff - spre) * 5.0f / (s->short_bound * L)), 1.0f) : 1.0f;
+                    k += (k + 100) * s->prog_spring_out * f2;
+                    d += (d + 100) * s->prog_damp_out * f2;
+                    if (v > 0) { k = s->spring_out; d = s->damp_out; }
+                }
+                if (diff > lb || diff < -sb) {
+                    k = smax(k, s->bound_spring);
+                    d = smax(d, s->bound_damp);
+                }
+            } else if (diff > lb || diff < -sb) {
+                k = s->bound_spring;
+                d = s->bound_damp;
+            }
+        } else {
+            k = s->spring_in;
+            d = s->damp_in;
+            if (diff > lb) {
+                float r = smin(diff - lb, 1.0f);
+                k += (s->bound_spring - k) * r;
+                d += (s->bound_damp - d) * r;
+            } else if (diff < -sb) {
+                float r = smin(-diff - sb, 1.0f);
+                k += (s->bound_spring - k) * r;
+                d += (s->bound_damp - d) * r;
+            } else if (v > 0) {
+                float wv = clampf_(fabsf(v), 0.15f, 20.0f);
+                k = s->spring_out;
+                d = (s->damp_out * s->damp_out_slow * smin(wv, s->split_out) + s->damp_out * s->damp_out_fast * smax(0.0f, wv - s->split_out)) / wv;
+            } else if (v < 0) {
+                float wv = clampf_(fabsf(v), 0.15f, 20.0f);
+                k = s->spring_in;
+                d = (s->damp_in * s->damp_in_slow * smin(wv, s->split_in) + s->damp_in * s->damp_in_fast * smax(0.0f, wv - s->split_in)) / wv;
+            }
+        }
+        float slen = -k * diff - d * v;
+        if (B->opts.allow_break && fabsf(slen) > b->strength && !(b->flags & BL_BF_NO_BREAK)) {
+            b->flags |= BL_BF_BROKEN;
+            B->broken++;
+            continue;
+        }
+        b->stress = slen;
+        v3 fv = vmul(dis, slen * inv_len);
+        f[b->a] = vadd(f[b->a], fv);
+        f[b->b] = vsub(f[b->b], fv);
+    }
+}
+
+/* SoftBody::compute_slide_forces (src/phys/softbody.cpp:574-623): each slide node pulled onto the nearest point of its
+ * rail segment (the current one and its neighbours; the whole rail the first time), the reaction split over the
+ * segment's ends */
+static float slide_seg_dist2(const bl_node* nd, const uint32_t* rail, int i, v3 p, float* t) {
+    v3 a = ld3(nd[rail[i]].p), b = ld3(nd[rail[i + 1]].p), ab = vsub(b, a);
+    float l2 = vdot(ab, ab);
+    *t = l2 > 1e-12f ? clampf_(vdot(vsub(p, a), ab) / l2, 0.0f, 1.0f) : 0.0f;
+    v3 q = vsub(vadd(a, vmul(ab, *t)), p);
+    return vdot(q, q);
+}
+
+static void slide_forces(orc_body* B) {
+    bl_node* nd = B->nodes;
+    v3* f = B->force;
+    for (uint32_t q = 0; q < B->nsl; q++) {
+        bl_slide* s = &B->slides[q];
+        if (s->broken || s->nrail < 2) continue;
+        const uint32_t* rail = B->rail + s->rail0;
+        const v3 p = ld3(nd[s->node].p);
+        const int nseg = (int)s->nrail - 1;
+        float t, best_t = 0;
+        if (s->seg < 0) {
+            float best = 1e30f;
+            for (int i = 0; i < nseg; i++) {
+                float dd = slide_seg_dist2(nd, rail, i, p, &t);
+                if (dd < best) { best = dd; s->seg = i; best_t = t; }
+            }
+        }
+        float best = slide_seg_dist2(nd, rail, s->seg, p, &best_t);
+        int bi = s->seg;
+        if (s->seg > 0) {
+            float dd = slide_seg_dist2(nd, rail, s->seg - 1, p, &t);
+            if (dd < best) { best = dd; bi = s->seg - 1; best_t = t; }
+        }
+        if (s->seg + 1 < nseg) {
+            float dd = slide_seg_dist2(nd, rail, s->seg + 1, p, &t);
+            if (dd < best) { best = dd; bi = s->seg + 1; best_t = t; }
+        }
+        s->seg = bi;
+        uint32_t na = rail[bi], nb = rail[bi + 1];
+        v3 ideal = vadd(ld3(nd[na].p), vmul(vsub(ld3(nd[nb].p), ld3(nd[na].p)), best_t));
+        v3 d = vsub(ideal, p);
+        float dist = vlen(d);
+        float stretch = dist - s->threshold;
+        if (stretch <= 0 || dist < 1e-9f) continue;
+        v3 F = vmul(d, s->k * stretch / dist);
+        if (s->break_force > 0 && vlen(F) > s->break_force) {
+            s->broken = 1;
+            continue;
+        }
+        f[s->node] = vadd(f[s->node], F);
+        f[na] = vsub(f[na], vmul(F, 1.0f - best_t));
+        f[nb] = vsub(f[nb], vmul(F, best_t));
     }
 }
 
@@ -1435,6 +1617,9 @@ int orc_step(orc_world* w, int substeps) {
                 orc_body* B = &w->bodies[I->bodies[q]];
                 for (uint32_t i = 0; i < B->n; i++) B->force[i] = vmul(g, B->nodes[i].mass);
                 beam_forces(B);
+                /* (World::simulate_island's base_forces, src/phys/world.cpp:3255-3273: beams, shocks, ..., slides) */
+                if (B->nsh) shock_forces(B);
+                if (B->nsl) slide_forces(B);
             }
             if (I->needs_pairs) {
                 if (s == 0 || s >= I->next_rebuild) {
@@ -1477,6 +1662,13 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
     orc_body* B = &w->bodies[id];
     if (nodes && B->n) memcpy(B->nodes, nodes, sizeof(bl_node) * B->n);
     if (beams && B->nb) memcpy(B->beams, beams, sizeof(bl_beam) * B->nb);
+    return 0;
+}
+
+int orc_read_slides(orc_world* w, uint32_t id, bl_slide* out) {
+    if ((int)id >= w->nbodies) return BL_ERR_ARG;
+    orc_body* B = &w->bodies[id];
+    if (out && B->nsl) memcpy(out, B->slides, sizeof(bl_slide) * B->nsl);
     return 0;
 }
 

@@ -28,6 +28,7 @@ int orc_step(orc_world* w, int substeps);
 int orc_read_body(orc_world* w, uint32_t id, bl_node* nodes, bl_beam* beams, int64_t* broken);
 int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_beam* beams);
 int orc_stats(orc_world* w, bl_stats* out);
+int orc_read_slides(orc_world* w, uint32_t id, bl_slide* out);
 /* the fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds */
 int64_t orc_fast_pairs(const orc_world* w);
 /* summed over every rebuild (every island): hashed nodes, triangles queried, NT before and after the near filter,

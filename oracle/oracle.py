@@ -58,6 +58,7 @@ def oracle_lib():
             "read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "stats": (ctypes.c_int, [_P, _P]),
+            "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
             "fast_pairs": (ctypes.c_int64, [_P]),
             "rebuild_counts": (None, [_P, _P]),
             "finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
@@ -108,6 +109,7 @@ def ref_lib():
             "time_step": (ctypes.c_double, [_P, ctypes.c_int, ctypes.c_int]),
             "read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+            "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
             "finalize_stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_int,
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
@@ -162,6 +164,11 @@ class _CpuWorld:
         broken = ctypes.c_int64(0)
         self._fn("read_body")(self._h, bid, _ptr(nodes), _ptr(beams), ctypes.byref(broken))
         return nodes, beams
+
+    def read_slides(self, bid):
+        out = self.bodies[bid].slides.copy()
+        self._fn("read_slides")(self._h, bid, _ptr(out))
+        return out
 
     def broken_beams(self, bid):
         broken = ctypes.c_int64(0)

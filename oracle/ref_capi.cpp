@@ -174,10 +174,40 @@ int blref_add_body(void* h, const bl_body_desc* d) {
         c.a = d->capsules[i].a, c.b = d->capsules[i].b, c.radius = d->capsules[i].radius, c.joint = d->capsules[i].joint;
         b->capsules.push_back(c);
     }
+    for (uint32_t i = 0; i < d->n_shocks; i++) {
+        const bl_shock& x = d->shocks[i];
+        Shock s;
+        s.beam = x.beam, s.type = (uint8_t)x.type, s.soft_bump = x.soft_bump != 0;
+        s.spring = x.spring, s.damp = x.damp;
+        s.spring_in = x.spring_in, s.damp_in = x.damp_in, s.prog_spring_in = x.prog_spring_in, s.prog_damp_in = x.prog_damp_in;
+        s.spring_out = x.spring_out, s.damp_out = x.damp_out, s.prog_spring_out = x.prog_spring_out;
+        s.prog_damp_out = x.prog_damp_out;
+        s.damp_in_slow = x.damp_in_slow, s.split_in = x.split_in, s.damp_in_fast = x.damp_in_fast;
+        s.damp_out_slow = x.damp_out_slow, s.split_out = x.split_out, s.damp_out_fast = x.damp_out_fast;
+        s.short_bound = x.short_bound, s.long_bound = x.long_bound;
+        s.bound_spring = x.bound_spring, s.bound_damp = x.bound_damp;
+        b->shocks.push_back(s);
+    }
+    for (uint32_t i = 0; i < d->n_slides; i++) {
+        const bl_slide& x = d->slides[i];
+        SlideNode s;
+        s.node = x.node;
+        s.rail.assign(d->rail + x.rail0, d->rail + x.rail0 + x.nrail);
+        s.seg = x.seg, s.k = x.k, s.threshold = x.threshold, s.break_force = x.break_force, s.broken = x.broken != 0;
+        b->slides.push_back(s);
+    }
     apply_opts(*b, d->opts);
     SoftBody* p = rw->w.add_body(std::move(b));
     rw->bodies.push_back(p);
     return (int)rw->bodies.size() - 1;
+}
+
+int blref_read_slides(void* h, uint32_t id, bl_slide* out) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    const SoftBody& b = *rw->bodies[id];
+    for (size_t i = 0; i < b.slides.size(); i++) out[i].seg = b.slides[i].seg, out[i].broken = b.slides[i].broken ? 1u : 0u;
+    return 0;
 }
 
 int blref_step(void* h, int n) {

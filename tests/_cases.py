@@ -115,10 +115,88 @@ def case_multi():
     return abi.default_settings(), _boxes(GROUND), bodies, 1000
 
 
+def rig_body(seed, origin=(0.0, 1.2, 0.0)):
+    """A suspension rig: a box frame of nodes on four feet, each foot hung from the frame by a shock (types 1, 2,
+    2 with soft bump stops, 3; bounds small enough that the fall drives them past the stops) and held sideways by
+    stiff beams; two slide nodes riding rails of frame nodes (one with free play, one that breaks off)."""
+    from beamlab_amd.world import SoftBody
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    b = SoftBody("rig")
+    o = np.asarray(origin, f)
+    top = []
+    for i in range(4):
+        for k in range(2):
+            top.append(b.add_node(o + f((i * 0.5, 0.3, k * 0.8)), 8.0))
+    bot = [b.add_node(o + f((i * 0.5, 0.0, k * 0.8)), 8.0) for i in range(4) for k in range(2)]
+    def beam(x, y, k=2e6, d=600):
+        return b.add_beam(x, y, k, d, 1e9, 1e9)
+    for L in (top, bot):
+        for i in range(3):
+            for k in range(2):
+                beam(L[2 * i + k], L[2 * i + 2 + k])
+        for i in range(4):
+            beam(L[2 * i], L[2 * i + 1])
+        for i in range(3):
+            beam(L[2 * i], L[2 * i + 3]), beam(L[2 * i + 1], L[2 * i + 2])
+    for j in range(8):
+        beam(top[j], bot[j])
+        if j + 1 < 8:
+            beam(top[j], bot[j + 1])
+    feet = []
+    corners = [bot[0], bot[1], bot[6], bot[7]]
+    for q, c in enumerate(corners):
+        pc = b._pending_nodes[c][0] if c >= len(b.nodes) else b.nodes[c]["p"]
+        foot = b.add_node(np.asarray(pc, f) + f((0.0, -0.45, 0.0)), 15.0)
+        feet.append(foot)
+        sh = beam(c, foot, 0.0, 0.0)
+        # two stiff side links to the neighbouring bottom nodes keep the foot under its corner
+        nb = bot[2] if q < 2 else bot[4]
+        beam(nb, foot, 4e5, 200)
+        beam(bot[(c + 1) % 8] if c != bot[7] else bot[6], foot, 4e5, 200)
+        b.finalize()
+        if q == 0:
+            b.add_shock(sh, 1, spring=9e4, damp=3e3, short_bound=0.012, long_bound=0.015)
+        elif q == 1:
+            b.add_shock(sh, 2, spring_in=8e4, damp_in=2e3, prog_spring_in=1.5, prog_damp_in=0.5, spring_out=7e4,
+                        damp_out=2.5e3, prog_spring_out=1.2, prog_damp_out=0.8, short_bound=0.015, long_bound=0.02)
+        elif q == 2:
+            b.add_shock(sh, 2, soft_bump=1, spring_in=8e4, damp_in=2e3, prog_spring_in=1.0, prog_damp_in=0.3,
+                        spring_out=7e4, damp_out=2.5e3, prog_spring_out=0.9, prog_damp_out=0.6, short_bound=0.03,
+                        long_bound=0.025)
+        else:
+            b.add_shock(sh, 3, spring_in=9e4, damp_in=2.4e3, spring_out=8e4, damp_out=2.2e3, damp_in_slow=1.2,
+                        split_in=0.3, damp_in_fast=0.6, damp_out_slow=1.1, split_out=0.25, damp_out_fast=0.7,
+                        short_bound=0.015, long_bound=0.012)
+            if seed == 31:
+                b.beams[sh]["strength"] = 1.6e3  # (this rig's fourth shock breaks at the landing)
+    b.finalize()
+    # slides: a node between the feet riding the frame's bottom rail (free play 1 cm), and one that tears off
+    s1 = b.add_node(o + f((0.75, -0.05, 0.4)), 3.0)
+    s2 = b.add_node(o + f((0.25, -0.08, 0.4)), 3.0)
+    beam(s1, feet[0], 2e5, 100), beam(s2, feet[1], 2e5, 100)
+    b.finalize()
+    b.add_slide(s1, [bot[0], bot[2], bot[4], bot[6]], k=4e5, threshold=0.01)
+    b.add_slide(s2, [bot[1], bot[3], bot[5]], k=6e5, break_force=2.5e3)
+    b.stabilize(0.0005)
+    b.nodes["v"][:] = f((0.6, -3.5, 0.2))
+    b.nodes["v"][:, 0] += rng.uniform(-0.2, 0.2, len(b.nodes)).astype(f)
+    return b
+
+
+def case_shocks_slides():
+    """Two rigs dropped onto the ground: every shock type past its bump stops, slide nodes along their rails."""
+    b1 = rig_body(31)
+    b2 = rig_body(32, origin=(3.0, 1.6, 0.5))
+    b2.opts.allow_break = 0
+    return abi.default_settings(), _boxes(GROUND), [b1, b2], 1500
+
+
 CPU_CASES = {
     "c1": case_c1, "plastic": case_plastic, "ductile_nodeform": case_ductile_nodeform,
     "rope_support": case_rope_support, "fixed_kill": case_fixed_kill, "surfaces": case_surfaces,
     "oriented_boxes": case_oriented_boxes, "runaway": case_runaway, "multi": case_multi,
+    "shocks_slides": case_shocks_slides,
 }
 
 

@@ -54,6 +54,34 @@ public:
         d.opts.faces_only = b.faces_only, d.opts.face_skin = b.face_skin;
         d.opts.air_drag = b.air_drag, d.opts.aero_cda = b.aero_cda, d.opts.rest_damp = b.rest_damp;
         d.opts.energy_guard = b.energy_guard;
+        // its shocks and slide nodes (Shock, SlideNode: src/phys/softbody.h:88-101,559-567), field by field
+        std::vector<bl_shock> shocks;
+        for (const Shock& s : b.shocks) {
+            bl_shock x{};
+            x.beam = s.beam, x.type = s.type, x.soft_bump = s.soft_bump ? 1u : 0u;
+            x.spring = s.spring, x.damp = s.damp;
+            x.spring_in = s.spring_in, x.damp_in = s.damp_in, x.prog_spring_in = s.prog_spring_in;
+            x.prog_damp_in = s.prog_damp_in;
+            x.spring_out = s.spring_out, x.damp_out = s.damp_out, x.prog_spring_out = s.prog_spring_out;
+            x.prog_damp_out = s.prog_damp_out;
+            x.damp_in_slow = s.damp_in_slow, x.split_in = s.split_in, x.damp_in_fast = s.damp_in_fast;
+            x.damp_out_slow = s.damp_out_slow, x.split_out = s.split_out, x.damp_out_fast = s.damp_out_fast;
+            x.short_bound = s.short_bound, x.long_bound = s.long_bound;
+            x.bound_spring = s.bound_spring, x.bound_damp = s.bound_damp;
+            shocks.push_back(x);
+        }
+        std::vector<bl_slide> slides;
+        std::vector<uint32_t> rail;
+        for (const SlideNode& s : b.slides) {
+            bl_slide x{};
+            x.node = s.node, x.rail0 = (uint32_t)rail.size(), x.nrail = (uint32_t)s.rail.size(), x.seg = s.seg;
+            x.k = s.k, x.threshold = s.threshold, x.break_force = s.break_force, x.broken = s.broken ? 1u : 0u;
+            rail.insert(rail.end(), s.rail.begin(), s.rail.end());
+            slides.push_back(x);
+        }
+        d.shocks = shocks.data(), d.n_shocks = (uint32_t)shocks.size();
+        d.slides = slides.data(), d.n_slides = (uint32_t)slides.size();
+        d.rail = rail.data(), d.n_rail = (uint32_t)rail.size();
         uint32_t id;
         const int rc = bl_add_body(ctx_, &d, &id);
         if (rc == BL_ERR_UNSUPPORTED) throw Unsupported(bl_error_string(rc)); // keep this body on the CPU
@@ -61,12 +89,17 @@ public:
         bodies_.push_back(&b);
         return id;
     }
-    // World::step_substeps for the GPU bodies; their arrays read back after the frame
+    // World::step_substeps for the GPU bodies; their arrays (and slide nodes' state) read back after the frame
     void step_substeps(int n) {
         check(bl_step(ctx_, n));
-        for (uint32_t i = 0; i < bodies_.size(); i++)
-            check(bl_read_body(ctx_, i, reinterpret_cast<bl_node*>(bodies_[i]->nodes.data()),
-                               reinterpret_cast<bl_beam*>(bodies_[i]->beams.data())));
+        for (uint32_t i = 0; i < bodies_.size(); i++) {
+            SoftBody& b = *bodies_[i];
+            check(bl_read_body(ctx_, i, reinterpret_cast<bl_node*>(b.nodes.data()), reinterpret_cast<bl_beam*>(b.beams.data())));
+            if (b.slides.empty()) continue;
+            std::vector<bl_slide> sl(b.slides.size());
+            check(bl_read_slides(ctx_, i, sl.data()));
+            for (size_t q = 0; q < sl.size(); q++) b.slides[q].seg = sl[q].seg, b.slides[q].broken = sl[q].broken != 0;
+        }
     }
 
     struct Unsupported : std::runtime_error {

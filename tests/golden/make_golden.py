@@ -21,7 +21,8 @@ Outputs (data only: inputs are regenerated from the seeds, outputs are the refer
     fem_held_ref.npz    frames stepping their frame every 2 and 4 substeps (tests/_fem_cases.py's held cases): the
                         states after 12, 24, 36 substeps
 
-`python tests/golden/make_golden.py plastic` writes fem_plastic_ref.npz and fem_held_ref.npz alone.
+`python tests/golden/make_golden.py plastic` writes fem_plastic_ref.npz and fem_held_ref.npz alone, `... cases`
+c1_ref.npz and cases_ref.npz alone.
 """
 import hashlib
 import os
@@ -42,6 +43,12 @@ from oracle import oracle  # noqa: E402
 def main():
     if not oracle.have_ref():
         oracle.build(ref=True)
+    cases()
+    fem()
+    plastic()
+
+
+def cases():
     settings, boxes, bodies, n = CPU_CASES["c1"]()
     cps = [1, 7, n // 3, n]
     st = run_world(oracle.RefWorld(settings, boxes), bodies, n, cps)
@@ -63,6 +70,9 @@ def main():
         digests[f"{name}_broken"] = np.array(st[-1][-1])
     np.savez_compressed(os.path.join(HERE, "cases_ref.npz"), **digests)
     print("written", sorted(digests))
+
+
+def fem():
     rng = np.random.default_rng(17)
     sections = np.array([oracle.SECTION_C3, (2.1e11, 0.3, 7850.0, 1e-3, 0.05, 3e-4), (5e9, 0.45, 1200.0, 4e-3, 0.2, 0.0)],
                         np.float32)
@@ -101,7 +111,6 @@ def main():
         R.close()
     np.savez_compressed(os.path.join(HERE, "fem_frame_ref.npz"), **out)
     print("written fem_frame_ref.npz")
-    plastic()
 
 
 PLASTIC_KEYS = ("X", "th0", "r0", "tdmg", "tutil", "mstate", "qa", "qb")
@@ -144,5 +153,7 @@ def plastic():
 if __name__ == "__main__":
     if sys.argv[1:] == ["plastic"]:
         plastic()
+    elif sys.argv[1:] == ["cases"]:
+        cases()
     else:
         main()
