@@ -31,8 +31,15 @@ SHOCK = np.dtype([("beam", "<u4"), ("type", "<u4"), ("soft_bump", "<u4")] +
                                        "short_bound", "long_bound", "bound_spring", "bound_damp")])
 SLIDE = np.dtype([("node", "<u4"), ("rail0", "<u4"), ("nrail", "<u4"), ("seg", "<i4"), ("k", "<f4"),
                   ("threshold", "<f4"), ("break_force", "<f4"), ("broken", "<u4")])
+FRAME = np.dtype([("node", "<u4"), ("q", "<f4", 4), ("w", "<f4", 3), ("inv_inertia", "<f4"), ("ref_x", "<i4"),
+                  ("ref_y", "<i4")])
+JOINT = np.dtype([("parent_frame", "<u4"), ("child_node", "<u4"), ("child_frame", "<i4"), ("local_offset", "<f4", 3),
+                  ("rel_rot", "<f4", 4)] +
+                 [(n, "<f4") for n in ("k_lin", "d_lin", "k_ang", "d_ang", "break_force", "break_torque",
+                                       "yield_angle", "max_stretch", "max_bend", "plastic_lin", "plastic_ang",
+                                       "stress")] + [("broken", "<u4")])
 assert NODE.itemsize == 32 and NODE_INFO.itemsize == 20 and BEAM.itemsize == 52
-assert SHOCK.itemsize == 92 and SLIDE.itemsize == 32
+assert SHOCK.itemsize == 92 and SLIDE.itemsize == 32 and FRAME.itemsize == 44 and JOINT.itemsize == 92
 assert TRIANGLE.itemsize == 20 and CAPSULE.itemsize == 16 and BOX.itemsize == 92
 
 # NodeFlag (src/phys/softbody.h:29-40)
@@ -70,7 +77,8 @@ class BodyDesc(ctypes.Structure):
                 ("n_tris", ctypes.c_uint32), ("capsules", ctypes.c_void_p), ("n_capsules", ctypes.c_uint32),
                 ("opts", BodyOpts), ("shocks", ctypes.c_void_p), ("n_shocks", ctypes.c_uint32),
                 ("slides", ctypes.c_void_p), ("n_slides", ctypes.c_uint32), ("rail", ctypes.c_void_p),
-                ("n_rail", ctypes.c_uint32)]
+                ("n_rail", ctypes.c_uint32), ("frames", ctypes.c_void_p), ("n_frames", ctypes.c_uint32),
+                ("joints", ctypes.c_void_p), ("n_joints", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -166,6 +174,7 @@ SIGNATURES = {
     "bl_write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_body_broken_beams": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64)]),
     "bl_read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "bl_read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "bl_reset_stats": (ctypes.c_int, [_P]),
     "bl_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),

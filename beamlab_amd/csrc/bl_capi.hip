@@ -35,9 +35,12 @@ struct HostBody {
     std::vector<bl_shock> shocks;
     std::vector<bl_slide> slides; // (seg and broken: the step's state)
     std::vector<uint32_t> rail;
+    std::vector<bl_frame> frames; // (q, w and the joints' state: the step's)
+    std::vector<bl_joint> joints;
     bl_body_opts opts;
     uint32_t node0 = 0, beam0 = 0, tri0 = 0, cap0 = 0;
     uint32_t slide0 = 0; // its first slide in the device array
+    uint32_t frame0 = 0, joint0 = 0; // its first frame and joint in the device arrays
     // as of the next frame's start: top speed of the last frame and bounds (World::build_islands reads them)
     float max_speed = 0;
     f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
@@ -141,6 +144,9 @@ struct bl_ctx {
     DevArr<bl_shock> xsh;
     DevArr<bl_slide> xsl;
     DevArr<uint32_t> xrail;
+    DevArr<bl_joint> xjt;
+    DevArr<float4> fq, fw, ftq;
+    DevArr<uint32_t> fnode;
     XArgs xargs{};
     std::map<std::pair<int, int>, hipGraphExec_t> sgraphs; // the split form's captured substeps
     ContactEngine contacts;
@@ -192,6 +198,7 @@ struct bl_ctx {
         F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), tri_hull.release(), cbox.release();
         wstat.release();
         xbody.release(), shx.release(), xsh.release(), xsl.release(), xrail.release();
+        xjt.release(), fq.release(), fw.release(), ftq.release(), fnode.release();
         xargs = XArgs{};
         contacts.release();
         frame_pending = false;
@@ -231,7 +238,7 @@ bool unsupported_opts(const bl_body_opts& o) {
 // some body has shocks or slide nodes: the world steps in the split form (F holds the node forces between phases)
 bool has_extras(const bl_ctx* c) {
     for (const HostBody& hb : c->bodies)
-        if (!hb.shocks.empty() || !hb.slides.empty()) return true;
+        if (!hb.shocks.empty() || !hb.slides.empty() || !hb.joints.empty() || !hb.frames.empty()) return true;
     return false;
 }
 
@@ -257,10 +264,29 @@ int pull_state(bl_ctx* c) {
     if (!torn.empty()) HIPCHK(hipMemcpy(torn.data(), c->tri_torn.p, 4 * torn.size(), hipMemcpyDeviceToHost));
     std::vector<bl_slide> sl(c->xsl.n);
     if (!sl.empty()) HIPCHK(hipMemcpy(sl.data(), c->xsl.p, sizeof(bl_slide) * sl.size(), hipMemcpyDeviceToHost));
+    std::vector<bl_joint> jt(c->xjt.n);
+    if (!jt.empty()) HIPCHK(hipMemcpy(jt.data(), c->xjt.p, sizeof(bl_joint) * jt.size(), hipMemcpyDeviceToHost));
+    std::vector<float4> fq(c->fq.n), fw(c->fw.n);
+    if (!fq.empty()) {
+        HIPCHK(hipMemcpy(fq.data(), c->fq.p, sizeof(float4) * fq.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(fw.data(), c->fw.p, sizeof(float4) * fw.size(), hipMemcpyDeviceToHost));
+    }
     for (HostBody& hb : c->bodies) {
         for (size_t t = 0; t < hb.tris.size() && hb.tri0 + t < torn.size(); t++) hb.tris[t].torn = torn[hb.tri0 + t] ? 1 : 0;
         for (size_t q = 0; q < hb.slides.size() && hb.slide0 + q < sl.size(); q++)
             hb.slides[q].seg = sl[hb.slide0 + q].seg, hb.slides[q].broken = sl[hb.slide0 + q].broken;
+        for (size_t q = 0; q < hb.frames.size() && hb.frame0 + q < fq.size(); q++) {
+            const float4 a = fq[hb.frame0 + q], b = fw[hb.frame0 + q];
+            bl_frame& f = hb.frames[q];
+            f.q[0] = a.x, f.q[1] = a.y, f.q[2] = a.z, f.q[3] = a.w, f.w[0] = b.x, f.w[1] = b.y, f.w[2] = b.z;
+        }
+        for (size_t q = 0; q < hb.joints.size() && hb.joint0 + q < jt.size(); q++) {
+            const bl_joint& d = jt[hb.joint0 + q];
+            bl_joint& j = hb.joints[q];
+            for (int k = 0; k < 3; k++) j.local_offset[k] = d.local_offset[k];
+            for (int k = 0; k < 4; k++) j.rel_rot[k] = d.rel_rot[k];
+            j.plastic_lin = d.plastic_lin, j.plastic_ang = d.plastic_ang, j.stress = d.stress, j.broken = d.broken;
+        }
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const float4 p = P[hb.node0 + i], v = V[hb.node0 + i];
             bl_node& n = hb.nodes[i];
@@ -334,7 +360,8 @@ int build(bl_ctx* c) {
         const HostBody& hb = c->bodies[k];
         bp[k] = BodyParam{hb.opts.bounce, hb.opts.contact_push_max, hb.opts.contact_slop, hb.opts.collision_radius,
                           hb.opts.contact_friction, hb.opts.hull_depth, hb.opts.face_skin, hull_reach(hb),
-                          hb.opts.collision_group, hb.opts.self_collision, hb.opts.faces_only, 0};
+                          hb.opts.collision_group, hb.opts.self_collision, hb.opts.faces_only, 0,
+                          hb.opts.allow_deform, hb.opts.allow_break, 0, 0};
         const float fric_body = hb.opts.ground_friction * 1.0f; // (resting is off: src/phys/world.cpp:792)
         for (size_t i = 0; i < hb.nodes.size(); i++) {
             const bl_node& n = hb.nodes[i];
@@ -603,11 +630,25 @@ int build(bl_ctx* c) {
         std::vector<uint4> xb, sx;
         std::vector<bl_shock> sh;
         std::vector<bl_slide> sl;
-        std::vector<uint32_t> rail;
+        std::vector<uint32_t> rail, fn;
+        std::vector<bl_joint> jt;
+        std::vector<float4> fq4, fw4;
         for (HostBody& hb : c->bodies) {
-            hb.slide0 = (uint32_t)sl.size();
-            if (hb.shocks.empty() && hb.slides.empty()) continue;
+            hb.slide0 = (uint32_t)sl.size(), hb.frame0 = (uint32_t)fn.size(), hb.joint0 = (uint32_t)jt.size();
+            if (hb.shocks.empty() && hb.slides.empty() && hb.joints.empty() && hb.frames.empty()) continue;
             const uint32_t s0 = (uint32_t)sh.size(), l0 = (uint32_t)sl.size();
+            const uint32_t j0 = (uint32_t)jt.size(), f0 = (uint32_t)fn.size();
+            for (const bl_frame& f : hb.frames) {
+                fn.push_back(hb.node0 + f.node);
+                fq4.push_back(make_float4(f.q[0], f.q[1], f.q[2], f.q[3]));
+                fw4.push_back(make_float4(f.w[0], f.w[1], f.w[2], f.inv_inertia));
+            }
+            for (const bl_joint& j : hb.joints) {
+                bl_joint y = j;
+                y.parent_frame = f0 + j.parent_frame, y.child_node = hb.node0 + j.child_node;
+                y.child_frame = j.child_frame >= 0 ? (int32_t)f0 + j.child_frame : -1;
+                jt.push_back(y);
+            }
             for (const bl_shock& x : hb.shocks) {
                 const bl_beam& bm = hb.beams[x.beam];
                 const uint32_t gb = hb.beam0 + x.beam;
@@ -622,12 +663,16 @@ int build(bl_ctx* c) {
             }
             for (uint32_t r : hb.rail) rail.push_back(hb.node0 + r);
             xb.push_back(make_uint4(s0, (uint32_t)sh.size(), l0, (uint32_t)sl.size()));
+            xb.push_back(make_uint4(j0, (uint32_t)jt.size(), f0, (uint32_t)fn.size()));
         }
         if ((!xb.empty() && (rc = c->xbody.upload(xb))) || (!sx.empty() && (rc = c->shx.upload(sx))) ||
             (!sh.empty() && (rc = c->xsh.upload(sh))) || (!sl.empty() && (rc = c->xsl.upload(sl))) ||
-            (!rail.empty() && (rc = c->xrail.upload(rail))))
+            (!rail.empty() && (rc = c->xrail.upload(rail))) || (!jt.empty() && (rc = c->xjt.upload(jt))) ||
+            (!fn.empty() && ((rc = c->fnode.upload(fn)) || (rc = c->fq.upload(fq4)) || (rc = c->fw.upload(fw4)) ||
+                             (rc = c->ftq.upload(std::vector<float4>(fn.size(), make_float4(0, 0, 0, 0)))))))
             return rc;
-        c->xargs = XArgs{c->xbody.p, (uint32_t)xb.size(), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p};
+        c->xargs = XArgs{c->xbody.p, (uint32_t)(xb.size() / 2), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p,
+                         c->xjt.p, c->fq.p, c->fw.p, c->ftq.p, c->fnode.p, (uint32_t)fn.size()};
     }
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;
@@ -794,6 +839,7 @@ int repair_body(HostBody& b) {
             b.nodes[i].p[0] = med.x, b.nodes[i].p[1] = med.y, b.nodes[i].p[2] = med.z;
             b.nodes[i].v[0] = b.nodes[i].v[1] = b.nodes[i].v[2] = 0;
         }
+    for (bl_frame& f : b.frames) f.w[0] = f.w[1] = f.w[2] = 0;
     return reset;
 }
 
@@ -812,7 +858,7 @@ int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk
                 return rc;
             if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
         }
-        if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p))) return rc;
+        if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt))) return rc;
         parity ^= 1;
     }
     if (tail && ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity))))
@@ -939,7 +985,8 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
                 g_last_error = "contact narrow phase or response failed";
                 return rc;
             }
-            if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p))) return rc;
+            if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
+                return rc;
             c->parity ^= 1;
             nnb = 0;
             if (s + 1 < substeps) {
@@ -991,7 +1038,7 @@ int enqueue_split(bl_ctx* c, int substeps) {
     int rc;
     auto one = [&](int parity) -> int {
         if ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity)) ||
-            (rc = launch_integrate(a, c->stream, parity, c->wstat.p)))
+            (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
             return rc;
         return BL_OK;
     };
@@ -1308,7 +1355,22 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
         const bl_triangle& tr = d->tris[t];
         if (tr.a >= d->n_nodes || tr.b >= d->n_nodes || tr.c >= d->n_nodes) return BL_ERR_ARG;
     }
-    if ((d->n_shocks && !d->shocks) || (d->n_slides && !d->slides) || (d->n_rail && !d->rail)) return BL_ERR_ARG;
+    if ((d->n_shocks && !d->shocks) || (d->n_slides && !d->slides) || (d->n_rail && !d->rail) ||
+        (d->n_frames && !d->frames) || (d->n_joints && !d->joints))
+        return BL_ERR_ARG;
+    for (uint32_t q = 0; q < d->n_frames; q++) {
+        if (d->frames[q].node >= d->n_nodes) return BL_ERR_ARG;
+        if (d->frames[q].ref_x >= 0 || d->frames[q].ref_y >= 0) {
+            g_last_error = "kinematic frames (a node's triad) are not stepped on the GPU";
+            return BL_ERR_UNSUPPORTED;
+        }
+    }
+    for (uint32_t q = 0; q < d->n_joints; q++) {
+        const bl_joint& j = d->joints[q];
+        if (j.parent_frame >= d->n_frames || j.child_node >= d->n_nodes || j.child_frame >= (int32_t)d->n_frames ||
+            j.child_frame < -1)
+            return BL_ERR_ARG;
+    }
     for (uint32_t q = 0; q < d->n_shocks; q++)
         if (d->shocks[q].beam >= d->n_beams || d->shocks[q].type < 1 || d->shocks[q].type > 3) return BL_ERR_ARG;
     for (uint32_t r = 0; r < d->n_rail; r++)
@@ -1339,6 +1401,8 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     if (d->n_shocks) hb.shocks.assign(d->shocks, d->shocks + d->n_shocks);
     if (d->n_slides) hb.slides.assign(d->slides, d->slides + d->n_slides);
     if (d->n_rail) hb.rail.assign(d->rail, d->rail + d->n_rail);
+    if (d->n_frames) hb.frames.assign(d->frames, d->frames + d->n_frames);
+    if (d->n_joints) hb.joints.assign(d->joints, d->joints + d->n_joints);
     hb.opts = d->opts;
     hb.compute_bounds(); // (SoftBody::compute_aabb at World::add_body, src/phys/world.cpp:283-289)
     c->bodies.push_back(std::move(hb));
@@ -1412,6 +1476,18 @@ int bl_read_slides(bl_ctx* c, uint32_t id, bl_slide* out) {
     if (rc) return rc;
     const HostBody& hb = c->bodies[id];
     if (out && !hb.slides.empty()) std::memcpy(out, hb.slides.data(), sizeof(bl_slide) * hb.slides.size());
+    return BL_OK;
+}
+
+int bl_read_joints(bl_ctx* c, uint32_t id, bl_frame* frames_out, bl_joint* joints_out) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    const HostBody& hb = c->bodies[id];
+    if (frames_out && !hb.frames.empty()) std::memcpy(frames_out, hb.frames.data(), sizeof(bl_frame) * hb.frames.size());
+    if (joints_out && !hb.joints.empty()) std::memcpy(joints_out, hb.joints.data(), sizeof(bl_joint) * hb.joints.size());
     return BL_OK;
 }
 

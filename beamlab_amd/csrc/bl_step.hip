@@ -1157,12 +1157,133 @@ __device__ void slide_eval(const KArgs& a, bl_slide& s, const uint32_t* rail_all
     a.F[nb] = make_float4(f.x - Fb.x, f.y - Fb.y, f.z - Fb.z, f.w);
 }
 
-// a thread per body with shocks or slides: its shocks, then its slides, in order
+// quaternions (x, y, z, w) as src/core/math.h:131-222 computes them (sin, cos, atan2 as glibc's)
+struct qd {
+    float x, y, z, w;
+};
+__device__ __forceinline__ qd qmk(float x, float y, float z, float w) { return qd{x, y, z, w}; }
+__device__ __forceinline__ qd qmul(qd a, qd b) {
+    return qmk(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+               a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+__device__ __forceinline__ qd qscale(qd a, float s) { return qmk(a.x * s, a.y * s, a.z * s, a.w * s); }
+__device__ __forceinline__ qd qconj(qd a) { return qmk(-a.x, -a.y, -a.z, a.w); }
+__device__ __forceinline__ qd qnormalize(qd a) {
+    const float l = fsqrt(a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w);
+    return l > 1e-12f ? qscale(a, 1.0f / l) : qmk(0, 0, 0, 1);
+}
+__device__ __forceinline__ f3 qrotate(qd q, f3 p) {
+    const f3 u = mk(q.x, q.y, q.z);
+    const f3 t = mul(cross(u, p), 2.0f);
+    return add(add(p, mul(t, q.w)), cross(u, t));
+}
+__device__ __forceinline__ qd qaxis_angle(f3 axis, float angle) {
+    const float h = angle * 0.5f, sn = bl_sinf(h);
+    const f3 a = normalize(axis);
+    return qmk(a.x * sn, a.y * sn, a.z * sn, bl_cosf(h));
+}
+__device__ __forceinline__ f3 qlog(qd qin) {
+    const qd q = qin.w < 0 ? qscale(qin, -1.0f) : qin;
+    const f3 v = mk(q.x, q.y, q.z);
+    const float sn = length(v);
+    if (sn < 1e-8f) return mul(v, 2.0f);
+    const float angle = 2.0f * bl_atan2f(sn, q.w);
+    return mul(v, angle / sn);
+}
+__device__ __forceinline__ qd ld_q(const float* f) { return qmk(f[0], f[1], f[2], f[3]); }
+__device__ __forceinline__ qd ld_q(float4 f) { return qmk(f.x, f.y, f.z, f.w); }
+
+__device__ __forceinline__ void add_force(const KArgs& a, uint32_t n, f3 x) {
+    const float4 f = a.F[n];
+    a.F[n] = make_float4(f.x + x.x, f.y + x.y, f.z + x.z, f.w);
+}
+__device__ __forceinline__ void sub_force(const KArgs& a, uint32_t n, f3 x) {
+    const float4 f = a.F[n];
+    a.F[n] = make_float4(f.x - x.x, f.y - x.y, f.z - x.z, f.w);
+}
+
+// SoftBody::compute_joint_forces (src/phys/softbody.cpp:625-679), one joint; its frames' torques in x.ftq
+__device__ void joint_eval(const KArgs& a, const XArgs& x, bl_joint& j, int parity, bool deform, bool brk) {
+    if (j.broken) return;
+    const uint32_t pf = j.parent_frame, pnode = x.fnode[pf];
+    const float4 Pp = a.pm[parity][pnode], Pc = a.pm[parity][j.child_node];
+    const float4 Vp = a.vm[parity][pnode], Vc = a.vm[parity][j.child_node];
+    const qd pq = ld_q(x.fq[pf]);
+    const float4 pw4 = x.fw[pf];
+    const f3 pw = mk(pw4.x, pw4.y, pw4.z);
+    f3 r = qrotate(pq, mk(j.local_offset[0], j.local_offset[1], j.local_offset[2]));
+    f3 err = sub(mk(Pc.x, Pc.y, Pc.z), add(mk(Pp.x, Pp.y, Pp.z), r));
+    if (j.break_force > 0 && deform) {
+        const float ylen = j.break_force / j.k_lin;
+        const float el = length(err);
+        if (el > ylen) {
+            const float excess = el - ylen;
+            const f3 dir = divs(err, el);
+            const f3 lo = add(mk(j.local_offset[0], j.local_offset[1], j.local_offset[2]), qrotate(qconj(pq), mul(dir, excess)));
+            j.local_offset[0] = lo.x, j.local_offset[1] = lo.y, j.local_offset[2] = lo.z;
+            r = qrotate(pq, lo);
+            err = mul(dir, ylen);
+            j.plastic_lin += excess;
+        }
+    }
+    const f3 vrel = sub(mk(Vc.x, Vc.y, Vc.z), add(mk(Vp.x, Vp.y, Vp.z), cross(pw, r)));
+    const f3 F = sub(mul(err, -j.k_lin), mul(vrel, j.d_lin));
+    add_force(a, j.child_node, F);
+    sub_force(a, pnode, F);
+    float4 pt = x.ftq[pf];
+    const f3 tq = cross(r, mk(-F.x, -F.y, -F.z));
+    pt = make_float4(pt.x + tq.x, pt.y + tq.y, pt.z + tq.z, 0.0f);
+    float s = j.break_force > 0 ? length(err) * j.k_lin / j.break_force : 0.0f;
+    if (j.child_frame >= 0) {
+        const uint32_t cf = (uint32_t)j.child_frame;
+        qd qe = qmul(ld_q(x.fq[cf]), qconj(qmul(pq, ld_q(j.rel_rot))));
+        f3 theta = qlog(qe);
+        const float ang = length(theta);
+        float yield = j.yield_angle > 0 ? j.yield_angle : 1e9f;
+        if (j.break_torque > 0) yield = smin(yield, j.break_torque / j.k_ang);
+        if (ang > yield && deform) {
+            const float excess = ang - yield;
+            const f3 axis = divs(theta, ang);
+            const qd bend = qaxis_angle(axis, excess);
+            const qd rr = qnormalize(qmul(qmul(qmul(qconj(pq), bend), pq), ld_q(j.rel_rot)));
+            j.rel_rot[0] = rr.x, j.rel_rot[1] = rr.y, j.rel_rot[2] = rr.z, j.rel_rot[3] = rr.w;
+            theta = mul(axis, yield);
+            j.plastic_ang += excess;
+        }
+        const float4 cw4 = x.fw[cf];
+        const f3 T = sub(mul(theta, -j.k_ang), mul(sub(mk(cw4.x, cw4.y, cw4.z), pw), j.d_ang));
+        // (the child's torque first, then the parent's: a joint between a frame and itself sums both in this order)
+        x.ftq[pf] = pt;
+        float4 ct = x.ftq[cf];
+        x.ftq[cf] = make_float4(ct.x + T.x, ct.y + T.y, ct.z + T.z, 0.0f);
+        pt = x.ftq[pf];
+        pt = make_float4(pt.x - T.x, pt.y - T.y, pt.z - T.z, 0.0f);
+        if (j.break_torque > 0) s = smax(s, length(theta) * j.k_ang / j.break_torque);
+    }
+    x.ftq[pf] = pt;
+    j.stress = s;
+    const bool snap = (j.break_force > 0 && j.plastic_lin > j.max_stretch) || (j.break_torque > 0 && j.plastic_ang > j.max_bend);
+    if (brk && snap) j.broken = 1;
+}
+
+// a thread per body with shocks, joints or slides: its frames' torques cleared (clear_forces), its shocks, joints
+// and slides, in order (World::simulate_island's base_forces, src/phys/world.cpp:3255-3273)
 __global__ void k_extras(KArgs a, XArgs x, int parity) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x.nbody) return;
-    const uint4 r = x.body[i];
+    const uint4 r = x.body[2 * i], r2 = x.body[2 * i + 1];
+    for (uint32_t f = r2.z; f < r2.w; f++) x.ftq[f] = make_float4(0, 0, 0, 0);
     for (uint32_t q = r.x; q < r.y; q++) shock_eval(a, x.sh[q], x.shx[q], parity, a.ninfo[x.shx[q].z] >> 16);
+    if (r2.x < r2.y) {
+        // (the body's allow_deform and allow_break: its beams' flags carry them; a body with joints has a frame node)
+        const uint32_t body = a.ninfo[x.fnode[x.jt[r2.x].parent_frame]] >> 16;
+        const BodyParam& bp = a.bodies[body];
+        for (uint32_t q = r2.x; q < r2.y; q++) {
+            bl_joint j = x.jt[q];
+            joint_eval(a, x, j, parity, bp.deform != 0, bp.brk != 0);
+            x.jt[q] = j;
+        }
+    }
     for (uint32_t q = r.z; q < r.w; q++) {
         bl_slide s = x.sl[q];
         slide_eval(a, s, x.rail, parity);
@@ -1172,6 +1293,27 @@ __global__ void k_extras(KArgs a, XArgs x, int parity) {
 
 int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
     if (x.nbody) hipLaunchKernelGGL(k_extras, dim3((x.nbody + 63) / 64), dim3(64), 0, st, a, x, parity);
+    return BL_OK_OR_HIP();
+}
+
+// SoftBody::integrate_frames (src/phys/softbody.cpp:1006-1021), dynamic frames: a thread per frame
+__global__ void k_frames(XArgs x, float dt) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= x.nframes) return;
+    const float4 W = x.fw[f], T = x.ftq[f];
+    if (W.w <= 0) return;
+    f3 w = add(mk(W.x, W.y, W.z), mul(mk(T.x, T.y, T.z), W.w * dt));
+    w = mul(w, 1.0f - 0.5f * dt);
+    x.fw[f] = make_float4(w.x, w.y, w.z, W.w);
+    const qd dq = qmk(w.x, w.y, w.z, 0);
+    const qd q = ld_q(x.fq[f]);
+    const qd s = qscale(qmul(dq, q), 0.5f * dt);
+    const qd n = qnormalize(qmk(q.x + s.x, q.y + s.y, q.z + s.z, q.w + s.w));
+    x.fq[f] = make_float4(n.x, n.y, n.z, n.w);
+}
+
+int launch_frames(const XArgs& x, hipStream_t st, float dt) {
+    if (x.nframes) hipLaunchKernelGGL(k_frames, dim3((x.nframes + 63) / 64), dim3(64), 0, st, x, dt);
     return BL_OK_OR_HIP();
 }
 

@@ -39,6 +39,7 @@ struct BodyParam {
     float hull_depth, face_skin;              // its hull (one-sided) triangles: depth behind, faces-only skin
     float hull_reach;                         // hull_depth when it has a hull triangle, else 0
     int32_t group, self_coll, faces_only, pad;
+    int32_t deform, brk, pad2, pad3; // its allow_deform, allow_break (the joints read them)
 };
 
 struct KArgs {
@@ -100,14 +101,23 @@ struct KArgs {
 // The bodies' shocks and slide nodes (the forces that follow the beams in World::simulate_island's base_forces,
 // src/phys/world.cpp:3255-3273): run after the force phase, into F, a thread per body with any, in the body's order.
 struct XArgs {
-    const uint4* body;    // per such body: first shock, end shock, first slide, end slide
+    const uint4* body;    // per such body, two records: first shock, end shock, first slide, end slide; first joint,
+                          // end joint, first frame, end frame
     uint32_t nbody;
     const bl_shock* sh;   // the shocks (beam: the body's beam index, unused here)
     const uint4* shx;     // per shock: its beam's a-end and b-end incidences, its a and b nodes
     bl_slide* sl;         // the slides (node, rail0 global); seg and broken updated here
     const uint32_t* rail; // the rails' nodes (global)
+    bl_joint* jt;         // the joints (frames and child node global); their plastic state, stress, broken updated
+    float4* fq;           // per frame: orientation (x, y, z, w)
+    float4* fw;           // per frame: angular velocity, inverse inertia
+    float4* ftq;          // per frame: torque of the substep
+    const uint32_t* fnode;
+    uint32_t nframes;
 };
 int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
+// the frames' integration after the nodes' (SoftBody::integrate_frames)
+int launch_frames(const XArgs& x, hipStream_t st, float dt);
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
 int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,

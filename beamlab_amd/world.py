@@ -25,6 +25,8 @@ class SoftBody:
         self.shocks = np.zeros(0, abi.SHOCK)
         self.slides = np.zeros(0, abi.SLIDE)
         self.rail = np.zeros(0, np.uint32)
+        self.frames = np.zeros(0, abi.FRAME)
+        self.joints = np.zeros(0, abi.JOINT)
         self.opts = abi.default_body_opts()
         self._pending_nodes, self._pending_beams, self._pending_tris = [], [], []
 
@@ -67,6 +69,33 @@ class SoftBody:
         self.rail = np.concatenate([self.rail, np.asarray(rail, np.uint32)])
         self.slides = np.concatenate([self.slides, s])
         return len(self.slides) - 1
+
+    # SoftBody::add_frame (src/phys/softbody.cpp:42-53): an oriented node (q = x, y, z, w)
+    def add_frame(self, node, q=(0.0, 0.0, 0.0, 1.0), inertia=1.0):
+        self._flush()
+        f = np.zeros(1, abi.FRAME)
+        f[0]["node"], f[0]["q"], f[0]["ref_x"], f[0]["ref_y"] = node, q, -1, -1
+        f[0]["inv_inertia"] = np.float32(1.0) / np.float32(inertia) if inertia > 0 and self.nodes[node]["inv_mass"] > 0 else 0
+        self.info[node]["flags"] |= abi.NF_FRAME
+        self.info[node]["frame"] = len(self.frames)
+        self.frames = np.concatenate([self.frames, f])
+        return len(self.frames) - 1
+
+    # SoftBody::add_joint (src/phys/softbody.cpp:55-75): the child's rest offset and orientation in the parent frame
+    def add_joint(self, parent_frame, child_node, child_frame, k_lin, d_lin, k_ang, d_ang, **fields):
+        self._flush()
+        pf = self.frames[parent_frame]
+        j = np.zeros(1, abi.JOINT)
+        j[0]["parent_frame"], j[0]["child_node"], j[0]["child_frame"] = parent_frame, child_node, child_frame
+        qc = _qconj(pf["q"])
+        j[0]["local_offset"] = _qrotate(qc, self.nodes[child_node]["p"] - self.nodes[pf["node"]]["p"])
+        j[0]["rel_rot"] = _qnormalize(_qmul(qc, self.frames[child_frame]["q"])) if child_frame >= 0 else (0, 0, 0, 1)
+        j[0]["k_lin"], j[0]["d_lin"], j[0]["k_ang"], j[0]["d_ang"] = k_lin, d_lin, k_ang, d_ang
+        j[0]["max_stretch"], j[0]["max_bend"] = 0.12, 0.8
+        for n, v in fields.items():
+            j[0][n] = v
+        self.joints = np.concatenate([self.joints, j])
+        return len(self.joints) - 1
 
     def _flush(self):
         if self._pending_nodes:
@@ -126,7 +155,35 @@ class SoftBody:
         d.shocks, d.n_shocks = abi.ptr(self.shocks), len(self.shocks)
         d.slides, d.n_slides = abi.ptr(self.slides), len(self.slides)
         d.rail, d.n_rail = abi.ptr(self.rail), len(self.rail)
+        d.frames, d.n_frames = abi.ptr(self.frames), len(self.frames)
+        d.joints, d.n_joints = abi.ptr(self.joints), len(self.joints)
         return d
+
+
+# quaternions (x, y, z, w) in float32, as src/core/math.h:131-170 writes them (for the joints' rest values)
+def _qmul(a, b):
+    f = np.float32
+    ax, ay, az, aw = (f(v) for v in a)
+    bx, by, bz, bw = (f(v) for v in b)
+    return np.array([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz], np.float32)
+
+
+def _qconj(q):
+    return np.array([-q[0], -q[1], -q[2], q[3]], np.float32)
+
+
+def _qnormalize(q):
+    q = np.asarray(q, np.float32)
+    n = np.float32(np.sqrt(np.float32(q @ q)))
+    return q * (np.float32(1.0) / n) if n > 1e-12 else np.array([0, 0, 0, 1], np.float32)
+
+
+def _qrotate(q, p):
+    u = np.asarray(q[:3], np.float32)
+    p = np.asarray(p, np.float32)
+    t = np.cross(u, p).astype(np.float32) * np.float32(2.0)
+    return (p + t * np.float32(q[3]) + np.cross(u, t)).astype(np.float32)
 
 
 def make_box(center, half, quat=(0.0, 0.0, 0.0, 1.0), surface=abi.SURF_CONCRETE):
@@ -220,6 +277,11 @@ class World:
 
     def write_body(self, bid, nodes=None, beams=None):
         abi.check(self._lib().bl_write_body(self._context(), bid, abi.ptr(nodes), abi.ptr(beams)), "bl_write_body")
+
+    def read_joints(self, bid):
+        fr, jt = self.bodies[bid].frames.copy(), self.bodies[bid].joints.copy()
+        abi.check(self._lib().bl_read_joints(self._context(), bid, abi.ptr(fr), abi.ptr(jt)), "bl_read_joints")
+        return fr, jt
 
     def read_slides(self, bid):
         out = self.bodies[bid].slides.copy()

@@ -142,6 +142,32 @@ typedef struct bl_slide {
     uint32_t broken;
 } bl_slide;
 
+/* An oriented node (== bl::phys::Frame, src/phys/softbody.h:104-113): orientation and angular velocity, integrated
+ * after the nodes (SoftBody::integrate_frames, src/phys/softbody.cpp:1006-1021) from the joints' torques. Kinematic
+ * frames (ref_x, ref_y >= 0: the truck format's triads) are not stepped on the GPU. */
+typedef struct bl_frame {
+    uint32_t node;
+    float q[4];              /* x, y, z, w */
+    float w[3];              /* angular velocity (world) */
+    float inv_inertia;       /* isotropic; 0: not integrated */
+    int32_t ref_x, ref_y;    /* must be -1 */
+} bl_frame;
+
+/* An orientation-preserving joint (== bl::phys::Joint, src/phys/softbody.h:116-136; SoftBody::compute_joint_forces,
+ * src/phys/softbody.cpp:625-679): a 6-DOF spring from a parent frame to a child node (and its frame, if any), with
+ * plastic yield and breaking; local_offset, rel_rot, the plastic strains, stress and broken are the step's state. */
+typedef struct bl_joint {
+    uint32_t parent_frame, child_node;
+    int32_t child_frame;     /* -1: the child has no orientation */
+    float local_offset[3];   /* the child's position in the parent frame (rest) */
+    float rel_rot[4];        /* the child's orientation relative to the parent (rest): x, y, z, w */
+    float k_lin, d_lin, k_ang, d_ang;
+    float break_force, break_torque, yield_angle;
+    float max_stretch, max_bend;
+    float plastic_lin, plastic_ang, stress;
+    uint32_t broken;
+} bl_joint;
+
 /* A static oriented box (== StaticBox, src/phys/static_world.h:115-126, without hull planes). `rot` holds the
  * columns of the rotation (the box's local axes in world space), as bl::mat3 does; aabb_mn/aabb_mx as
  * StaticBox::update_aabb (src/phys/static_world.cpp:242-246) computes them (bl_box_from_quat fills both). */
@@ -215,6 +241,11 @@ typedef struct bl_body_desc {
     uint32_t n_slides;
     const uint32_t* rail;    /* the slides' rail nodes */
     uint32_t n_rail;
+    /* oriented nodes and the joints between them, stepped between the shocks and the slides */
+    const bl_frame* frames;
+    uint32_t n_frames;
+    const bl_joint* joints;
+    uint32_t n_joints;
 } bl_body_desc;
 
 /* Counters (WorldStats / BodyStats / FemFrame counters: src/phys/world.h:79-132, src/phys/softbody.h:588-593). */
@@ -312,6 +343,9 @@ int bl_body_broken_beams(bl_ctx* ctx, uint32_t id, int64_t* out);
 
 /* A body's slide nodes as the step left them (SlideNode::seg, SlideNode::broken; the other fields as added). */
 int bl_read_slides(bl_ctx* ctx, uint32_t id, bl_slide* slides_out);
+
+/* A body's frames and joints as the step left them (either pointer may be null). */
+int bl_read_joints(bl_ctx* ctx, uint32_t id, bl_frame* frames_out, bl_joint* joints_out);
 
 /* Counters since bl_create (or the last bl_reset_stats). */
 int bl_get_stats(bl_ctx* ctx, bl_stats* out);

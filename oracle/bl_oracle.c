@@ -120,6 +120,11 @@ typedef struct {
     bl_slide* slides;
     uint32_t nsl;
     uint32_t* rail;
+    bl_frame* frames;
+    v3* torque; /* per frame (Frame::torque): cleared with the forces, summed by the joints */
+    uint32_t nfr;
+    bl_joint* joints;
+    uint32_t nj;
 } orc_body;
 
 struct orc_world {
@@ -152,6 +157,9 @@ void orc_destroy(orc_world* w) {
         free(w->bodies[i].shocks);
         free(w->bodies[i].slides);
         free(w->bodies[i].rail);
+        free(w->bodies[i].frames);
+        free(w->bodies[i].torque);
+        free(w->bodies[i].joints);
     }
     free(w->bodies);
     free(w->boxes);
@@ -190,6 +198,12 @@ int orc_add_body(orc_world* w, const bl_body_desc* d) {
     if (b->nsh) memcpy(b->shocks, d->shocks, sizeof(bl_shock) * b->nsh);
     if (b->nsl) memcpy(b->slides, d->slides, sizeof(bl_slide) * b->nsl);
     if (d->n_rail) memcpy(b->rail, d->rail, sizeof(uint32_t) * d->n_rail);
+    b->nfr = d->n_frames, b->nj = d->n_joints;
+    b->frames = (bl_frame*)malloc(sizeof(bl_frame) * (b->nfr ? b->nfr : 1));
+    b->torque = (v3*)calloc(b->nfr ? b->nfr : 1, sizeof(v3));
+    b->joints = (bl_joint*)malloc(sizeof(bl_joint) * (b->nj ? b->nj : 1));
+    if (b->nfr) memcpy(b->frames, d->frames, sizeof(bl_frame) * b->nfr);
+    if (b->nj) memcpy(b->joints, d->joints, sizeof(bl_joint) * b->nj);
     int hulls = 0;
     for (uint32_t t = 0; t < b->ntri; t++) hulls += !b->tris[t].two_sided;
     b->hull_reach = hulls > 0 ? b->opts.hull_depth : 0.0f;
@@ -424,6 +438,118 @@ static void slide_forces(orc_body* B) {
     }
 }
 
+/* ------------------------------------------------------------------ quaternions (src/core/math.h:131-222) */
+typedef struct { float x, y, z, w; } qt;
+static inline qt Q(float x, float y, float z, float w) { qt r = {x, y, z, w}; return r; }
+static inline qt qmul(qt a, qt b) {
+    return Q(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+static inline qt qscale(qt a, float s) { return Q(a.x * s, a.y * s, a.z * s, a.w * s); }
+static inline qt qadd(qt a, qt b) { return Q(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline qt qconj(qt a) { return Q(-a.x, -a.y, -a.z, a.w); }
+static inline qt qnormalize(qt a) {
+    float l = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w);
+    return l > 1e-12f ? qscale(a, 1.0f / l) : Q(0, 0, 0, 1);
+}
+static inline v3 vcross_(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline v3 qrotate(qt q, v3 p) {
+    v3 u = V(q.x, q.y, q.z);
+    v3 t = vmul(vcross_(u, p), 2.0f);
+    return vadd(vadd(p, vmul(t, q.w)), vcross_(u, t));
+}
+static inline v3 vnormalize_(v3 a) { float l = vlen(a); return l > 1e-12f ? vmul(a, 1.0f / l) : V(0, 0, 0); }
+static qt qaxis_angle(v3 axis, float angle) {
+    float h = angle * 0.5f, sn = sinf(h);
+    v3 a = vnormalize_(axis);
+    return Q(a.x * sn, a.y * sn, a.z * sn, cosf(h));
+}
+static v3 qlog(qt qin) {
+    qt q = qin.w < 0 ? qscale(qin, -1.0f) : qin;
+    v3 v = V(q.x, q.y, q.z);
+    float sn = vlen(v);
+    if (sn < 1e-8f) return vmul(v, 2.0f);
+    float angle = 2.0f * atan2f(sn, q.w);
+    return vmul(v, angle / sn);
+}
+static inline qt qload(const float* f) { return Q(f[0], f[1], f[2], f[3]); }
+static inline void qstore(float* f, qt q) { f[0] = q.x, f[1] = q.y, f[2] = q.z, f[3] = q.w; }
+
+/* SoftBody::compute_joint_forces (src/phys/softbody.cpp:625-679): each joint in order */
+static void joint_forces(orc_body* B) {
+    bl_node* nd = B->nodes;
+    v3* f = B->force;
+    const int deform = B->opts.allow_deform != 0, brk = B->opts.allow_break != 0;
+    for (uint32_t q = 0; q < B->nj; q++) {
+        bl_joint* j = &B->joints[q];
+        if (j->broken) continue;
+        bl_frame* pf = &B->frames[j->parent_frame];
+        const bl_node* pn = &nd[pf->node];
+        const bl_node* cn = &nd[j->child_node];
+        const qt pq = qload(pf->q);
+        v3 r = qrotate(pq, ld3(j->local_offset));
+        v3 err = vsub(ld3(cn->p), vadd(ld3(pn->p), r));
+        if (j->break_force > 0 && deform) {
+            float ylen = j->break_force / j->k_lin;
+            float el = vlen(err);
+            if (el > ylen) {
+                float excess = el - ylen;
+                v3 dir = vdivs(err, el);
+                st3(j->local_offset, vadd(ld3(j->local_offset), qrotate(qconj(pq), vmul(dir, excess))));
+                r = qrotate(pq, ld3(j->local_offset));
+                err = vmul(dir, ylen);
+                j->plastic_lin += excess;
+            }
+        }
+        v3 vrel = vsub(ld3(cn->v), vadd(ld3(pn->v), vcross_(ld3(pf->w), r)));
+        v3 F = vsub(vmul(err, -j->k_lin), vmul(vrel, j->d_lin));
+        f[j->child_node] = vadd(f[j->child_node], F);
+        f[pf->node] = vsub(f[pf->node], F);
+        v3* pt = &B->torque[j->parent_frame];
+        *pt = vadd(*pt, vcross_(r, V(-F.x, -F.y, -F.z)));
+        float sres = j->break_force > 0 ? vlen(err) * j->k_lin / j->break_force : 0.0f;
+        if (j->child_frame >= 0) {
+            bl_frame* cf = &B->frames[j->child_frame];
+            qt qe = qmul(qload(cf->q), qconj(qmul(pq, qload(j->rel_rot))));
+            v3 theta = qlog(qe);
+            float ang = vlen(theta);
+            float yield = j->yield_angle > 0 ? j->yield_angle : 1e9f;
+            if (j->break_torque > 0) yield = smin(yield, j->break_torque / j->k_ang);
+            if (ang > yield && deform) {
+                float excess = ang - yield;
+                v3 axis = vdivs(theta, ang);
+                qt bend = qaxis_angle(axis, excess);
+                qstore(j->rel_rot, qnormalize(qmul(qmul(qmul(qconj(pq), bend), pq), qload(j->rel_rot))));
+                theta = vmul(axis, yield);
+                j->plastic_ang += excess;
+            }
+            v3 T = vsub(vmul(theta, -j->k_ang), vmul(vsub(ld3(cf->w), ld3(pf->w)), j->d_ang));
+            v3* ct = &B->torque[j->child_frame];
+            *ct = vadd(*ct, T);
+            *pt = vsub(*pt, T);
+            if (j->break_torque > 0) sres = smax(sres, vlen(theta) * j->k_ang / j->break_torque);
+        }
+        j->stress = sres;
+        int snap = (j->break_force > 0 && j->plastic_lin > j->max_stretch) || (j->break_torque > 0 && j->plastic_ang > j->max_bend);
+        if (brk && snap) j->broken = 1;
+    }
+}
+
+/* SoftBody::integrate_frames (src/phys/softbody.cpp:1006-1021), dynamic frames (kinematic triads are refused) */
+static void integrate_frames(orc_body* B, float dt) {
+    for (uint32_t k = 0; k < B->nfr; k++) {
+        bl_frame* fr = &B->frames[k];
+        if (fr->inv_inertia <= 0) continue;
+        v3 w = vadd(ld3(fr->w), vmul(B->torque[k], fr->inv_inertia * dt));
+        w = vmul(w, 1.0f - 0.5f * dt);
+        st3(fr->w, w);
+        qt dq = Q(w.x, w.y, w.z, 0);
+        qt q = qload(fr->q);
+        q = qadd(q, qscale(qmul(dq, q), 0.5f * dt));
+        qstore(fr->q, qnormalize(q));
+    }
+}
+
 /* StaticWorld::collide_point, the box part (src/phys/static_world.cpp:364-374,407-454); boxes in index order */
 static int collide_point(const orc_world* w, v3 p, float r, v3* normal, float* depth_out, int* surface,
                          float* max_force) {
@@ -593,6 +719,7 @@ static int repair_nodes(orc_body* B) {
     for (uint32_t i = 0; i < n; i++)
         if (bad[i]) st3(B->nodes[i].p, med), st3(B->nodes[i].v, V(0, 0, 0));
     free(sp), free(sv), free(cnt), free(bad);
+    for (uint32_t k = 0; k < B->nfr; k++) B->frames[k].w[0] = B->frames[k].w[1] = B->frames[k].w[2] = 0;
     /* bounds and top speed of the repaired body (src/phys/world.cpp:4106-4109) */
     B->mn = V(1e30f, 1e30f, 1e30f), B->mx = V(-1e30f, -1e30f, -1e30f);
     B->max_speed = 0;
@@ -1618,7 +1745,9 @@ int orc_step(orc_world* w, int substeps) {
                 for (uint32_t i = 0; i < B->n; i++) B->force[i] = vmul(g, B->nodes[i].mass);
                 beam_forces(B);
                 /* (World::simulate_island's base_forces, src/phys/world.cpp:3255-3273: beams, shocks, ..., slides) */
+                for (uint32_t k = 0; k < B->nfr; k++) B->torque[k] = V(0, 0, 0);
                 if (B->nsh) shock_forces(B);
+                if (B->nj) joint_forces(B);
                 if (B->nsl) slide_forces(B);
             }
             if (I->needs_pairs) {
@@ -1639,6 +1768,8 @@ int orc_step(orc_world* w, int substeps) {
                 float ms = sqrtf(v2);
                 if (ms > B->max_speed) B->max_speed = ms;
             }
+            /* (the frames after the nodes: src/phys/world.cpp:3929) */
+            for (int q = 0; q < I->nb; q++) integrate_frames(&w->bodies[I->bodies[q]], dt);
         }
         free_island(I);
     }
@@ -1662,6 +1793,14 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
     orc_body* B = &w->bodies[id];
     if (nodes && B->n) memcpy(B->nodes, nodes, sizeof(bl_node) * B->n);
     if (beams && B->nb) memcpy(B->beams, beams, sizeof(bl_beam) * B->nb);
+    return 0;
+}
+
+int orc_read_joints(orc_world* w, uint32_t id, bl_frame* frames, bl_joint* joints) {
+    if ((int)id >= w->nbodies) return BL_ERR_ARG;
+    orc_body* B = &w->bodies[id];
+    if (frames && B->nfr) memcpy(frames, B->frames, sizeof(bl_frame) * B->nfr);
+    if (joints && B->nj) memcpy(joints, B->joints, sizeof(bl_joint) * B->nj);
     return 0;
 }
 

@@ -29,6 +29,7 @@ int orc_read_body(orc_world* w, uint32_t id, bl_node* nodes, bl_beam* beams, int
 int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_beam* beams);
 int orc_stats(orc_world* w, bl_stats* out);
 int orc_read_slides(orc_world* w, uint32_t id, bl_slide* out);
+int orc_read_joints(orc_world* w, uint32_t id, bl_frame* frames, bl_joint* joints);
 /* the fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds */
 int64_t orc_fast_pairs(const orc_world* w);
 /* summed over every rebuild (every island): hashed nodes, triangles queried, NT before and after the near filter,

@@ -59,6 +59,7 @@ def oracle_lib():
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "stats": (ctypes.c_int, [_P, _P]),
             "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+            "read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "fast_pairs": (ctypes.c_int64, [_P]),
             "rebuild_counts": (None, [_P, _P]),
             "finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
@@ -110,6 +111,7 @@ def ref_lib():
             "read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+            "read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "finalize_stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_int,
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
@@ -169,6 +171,11 @@ class _CpuWorld:
         out = self.bodies[bid].slides.copy()
         self._fn("read_slides")(self._h, bid, _ptr(out))
         return out
+
+    def read_joints(self, bid):
+        fr, jt = self.bodies[bid].frames.copy(), self.bodies[bid].joints.copy()
+        self._fn("read_joints")(self._h, bid, _ptr(fr), _ptr(jt))
+        return fr, jt
 
     def broken_beams(self, bid):
         broken = ctypes.c_int64(0)

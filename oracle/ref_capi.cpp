@@ -196,6 +196,25 @@ int blref_add_body(void* h, const bl_body_desc* d) {
         s.seg = x.seg, s.k = x.k, s.threshold = x.threshold, s.break_force = x.break_force, s.broken = x.broken != 0;
         b->slides.push_back(s);
     }
+    for (uint32_t i = 0; i < d->n_frames; i++) {
+        const bl_frame& x = d->frames[i];
+        Frame f;
+        f.node = x.node, f.q = quat(x.q[0], x.q[1], x.q[2], x.q[3]), f.w = v3(x.w), f.inv_inertia = x.inv_inertia;
+        f.torque = vec3(0), f.ref_x = x.ref_x, f.ref_y = x.ref_y;
+        b->frames.push_back(f);
+    }
+    for (uint32_t i = 0; i < d->n_joints; i++) {
+        const bl_joint& x = d->joints[i];
+        Joint j{};
+        j.parent_frame = x.parent_frame, j.child_node = x.child_node, j.child_frame = x.child_frame;
+        j.local_offset = v3(x.local_offset), j.rel_rot = quat(x.rel_rot[0], x.rel_rot[1], x.rel_rot[2], x.rel_rot[3]);
+        j.k_lin = x.k_lin, j.d_lin = x.d_lin, j.k_ang = x.k_ang, j.d_ang = x.d_ang;
+        j.break_force = x.break_force, j.break_torque = x.break_torque, j.yield_angle = x.yield_angle;
+        j.max_stretch = x.max_stretch, j.max_bend = x.max_bend;
+        j.plastic_lin = x.plastic_lin, j.plastic_ang = x.plastic_ang, j.stress = x.stress, j.broken = x.broken != 0;
+        j.radius = 0.05f;
+        b->joints.push_back(j);
+    }
     apply_opts(*b, d->opts);
     SoftBody* p = rw->w.add_body(std::move(b));
     rw->bodies.push_back(p);
@@ -207,6 +226,30 @@ int blref_read_slides(void* h, uint32_t id, bl_slide* out) {
     if (id >= rw->bodies.size()) return -1;
     const SoftBody& b = *rw->bodies[id];
     for (size_t i = 0; i < b.slides.size(); i++) out[i].seg = b.slides[i].seg, out[i].broken = b.slides[i].broken ? 1u : 0u;
+    return 0;
+}
+
+int blref_read_joints(void* h, uint32_t id, bl_frame* frames, bl_joint* joints) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    const SoftBody& b = *rw->bodies[id];
+    for (size_t i = 0; frames && i < b.frames.size(); i++) {
+        const Frame& f = b.frames[i];
+        bl_frame& x = frames[i];
+        x.node = f.node, x.q[0] = f.q.x, x.q[1] = f.q.y, x.q[2] = f.q.z, x.q[3] = f.q.w;
+        x.w[0] = f.w.x, x.w[1] = f.w.y, x.w[2] = f.w.z, x.inv_inertia = f.inv_inertia, x.ref_x = f.ref_x, x.ref_y = f.ref_y;
+    }
+    for (size_t i = 0; joints && i < b.joints.size(); i++) {
+        const Joint& j = b.joints[i];
+        bl_joint& x = joints[i];
+        x.parent_frame = j.parent_frame, x.child_node = j.child_node, x.child_frame = j.child_frame;
+        x.local_offset[0] = j.local_offset.x, x.local_offset[1] = j.local_offset.y, x.local_offset[2] = j.local_offset.z;
+        x.rel_rot[0] = j.rel_rot.x, x.rel_rot[1] = j.rel_rot.y, x.rel_rot[2] = j.rel_rot.z, x.rel_rot[3] = j.rel_rot.w;
+        x.k_lin = j.k_lin, x.d_lin = j.d_lin, x.k_ang = j.k_ang, x.d_ang = j.d_ang;
+        x.break_force = j.break_force, x.break_torque = j.break_torque, x.yield_angle = j.yield_angle;
+        x.max_stretch = j.max_stretch, x.max_bend = j.max_bend;
+        x.plastic_lin = j.plastic_lin, x.plastic_ang = j.plastic_ang, x.stress = j.stress, x.broken = j.broken ? 1u : 0u;
+    }
     return 0;
 }
 

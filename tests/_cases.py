@@ -192,11 +192,60 @@ def case_shocks_slides():
     return abi.default_settings(), _boxes(GROUND), [b1, b2], 1500
 
 
+def _unit_quat(rng):
+    q = rng.normal(size=4).astype(np.float32)
+    return q / np.float32(np.sqrt(np.sum(q * q, dtype=np.float32)))
+
+
+def arm_body(seed, origin=(0.0, 1.0, 0.0)):
+    """A beam box carrying two jointed arms (oriented nodes joined by 6-DOF springs, the arms' last nodes without an
+    orientation), a joint tying the second arm back to the box and one coupling the box's two oriented corners; the
+    frames start turned at random and spinning. The arms hang sideways, so the fall onto the ground bends and pulls
+    their joints past their yield limits: some yield, some tear (the first arm's last joint, its middle bend)."""
+    from beamlab_amd.world import SoftBody
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    b = SoftBody("arms")
+    o = np.asarray(origin, f)
+    c = [b.add_node(o + f((x, y, z)), 10.0) for x in (0.0, 1.0) for y in (0.0, 0.3) for z in (0.0, 0.6)]
+    edges = [(i, j) for i in range(8) for j in range(i + 1, 8)]  # every pair: edges, face and body diagonals
+    for i, j in edges:
+        b.add_beam(c[i], c[j], 2e6, 600, 1e9, 1e9)
+    a1, a2, a3 = (b.add_node(o + f((-0.3 * k, 0.0, 0.1 * k)), 3.0) for k in (1, 2, 3))
+    a4, a5 = (b.add_node(o + f((1.0 + 0.35 * k, 0.3, 0.6)), 3.0) for k in (1, 2))
+    b.finalize()
+    F = {n: b.add_frame(n, _unit_quat(rng), inertia) for n, inertia in ((c[0], 0.4), (c[7], 0.4), (a1, 0.05),
+                                                                         (a2, 0.05), (a4, 0.06))}
+    b.add_joint(F[c[0]], a1, F[a1], 8e5, 300, 2.5e3, 6, break_force=5e3, break_torque=250, yield_angle=0.25,
+                max_stretch=0.05, max_bend=0.4)
+    b.add_joint(F[a1], a2, F[a2], 6e5, 200, 1.5e3, 4, break_torque=60, max_bend=0.3)
+    b.add_joint(F[a2], a3, -1, 3e5, 150, 0, 0, break_force=900, max_stretch=0.03)
+    b.add_joint(F[c[7]], a4, F[a4], 7e5, 250, 2e3, 5, yield_angle=0.15)
+    b.add_joint(F[a4], a5, -1, 4e5, 180, 0, 0)
+    b.add_joint(F[a4], c[6], -1, 2e5, 100, 0, 0)
+    b.add_joint(F[c[0]], c[7], F[c[7]], 0, 0, 1e3, 3)
+    b.stabilize(0.0005)
+    b.frames["w"] = rng.uniform(-2.0, 2.0, (len(b.frames), 3)).astype(f)
+    b.nodes["v"][:] = f((0.4, -3.0, 0.1))
+    b.nodes["v"][:, 2] += rng.uniform(-0.2, 0.2, len(b.nodes)).astype(f)
+    return b
+
+
+def case_joints():
+    """Three arm bodies dropped onto the ground: one as built, one that may not break, one that may not deform."""
+    b1 = arm_body(51)
+    b2 = arm_body(52, origin=(4.0, 1.3, 0.5))
+    b2.opts.allow_break = 0
+    b3 = arm_body(53, origin=(-4.0, 1.1, -0.5))
+    b3.opts.allow_deform = 0
+    return abi.default_settings(), _boxes(GROUND), [b1, b2, b3], 1500
+
+
 CPU_CASES = {
     "c1": case_c1, "plastic": case_plastic, "ductile_nodeform": case_ductile_nodeform,
     "rope_support": case_rope_support, "fixed_kill": case_fixed_kill, "surfaces": case_surfaces,
     "oriented_boxes": case_oriented_boxes, "runaway": case_runaway, "multi": case_multi,
-    "shocks_slides": case_shocks_slides,
+    "shocks_slides": case_shocks_slides, "joints": case_joints,
 }
 
 

@@ -79,6 +79,30 @@ public:
             rail.insert(rail.end(), s.rail.begin(), s.rail.end());
             slides.push_back(x);
         }
+        // its frames and joints (Frame, Joint: src/phys/softbody.h:104-136)
+        std::vector<bl_frame> frames;
+        for (const Frame& f : b.frames) {
+            bl_frame x{};
+            x.node = f.node, x.q[0] = f.q.x, x.q[1] = f.q.y, x.q[2] = f.q.z, x.q[3] = f.q.w;
+            x.w[0] = f.w.x, x.w[1] = f.w.y, x.w[2] = f.w.z, x.inv_inertia = f.inv_inertia;
+            x.ref_x = f.ref_x, x.ref_y = f.ref_y;
+            frames.push_back(x);
+        }
+        std::vector<bl_joint> joints;
+        for (const Joint& j : b.joints) {
+            bl_joint x{};
+            x.parent_frame = j.parent_frame, x.child_node = j.child_node, x.child_frame = j.child_frame;
+            x.local_offset[0] = j.local_offset.x, x.local_offset[1] = j.local_offset.y, x.local_offset[2] = j.local_offset.z;
+            x.rel_rot[0] = j.rel_rot.x, x.rel_rot[1] = j.rel_rot.y, x.rel_rot[2] = j.rel_rot.z, x.rel_rot[3] = j.rel_rot.w;
+            x.k_lin = j.k_lin, x.d_lin = j.d_lin, x.k_ang = j.k_ang, x.d_ang = j.d_ang;
+            x.break_force = j.break_force, x.break_torque = j.break_torque, x.yield_angle = j.yield_angle;
+            x.max_stretch = j.max_stretch, x.max_bend = j.max_bend;
+            x.plastic_lin = j.plastic_lin, x.plastic_ang = j.plastic_ang, x.stress = j.stress;
+            x.broken = j.broken ? 1u : 0u;
+            joints.push_back(x);
+        }
+        d.frames = frames.data(), d.n_frames = (uint32_t)frames.size();
+        d.joints = joints.data(), d.n_joints = (uint32_t)joints.size();
         d.shocks = shocks.data(), d.n_shocks = (uint32_t)shocks.size();
         d.slides = slides.data(), d.n_slides = (uint32_t)slides.size();
         d.rail = rail.data(), d.n_rail = (uint32_t)rail.size();
@@ -89,12 +113,28 @@ public:
         bodies_.push_back(&b);
         return id;
     }
-    // World::step_substeps for the GPU bodies; their arrays (and slide nodes' state) read back after the frame
+    // World::step_substeps for the GPU bodies; their arrays (and slide nodes', frames' and joints' state) read back
     void step_substeps(int n) {
         check(bl_step(ctx_, n));
         for (uint32_t i = 0; i < bodies_.size(); i++) {
             SoftBody& b = *bodies_[i];
             check(bl_read_body(ctx_, i, reinterpret_cast<bl_node*>(b.nodes.data()), reinterpret_cast<bl_beam*>(b.beams.data())));
+            if (!b.frames.empty() || !b.joints.empty()) {
+                std::vector<bl_frame> fr(b.frames.size());
+                std::vector<bl_joint> jt(b.joints.size());
+                check(bl_read_joints(ctx_, i, fr.data(), jt.data()));
+                for (size_t q = 0; q < fr.size(); q++) {
+                    Frame& f = b.frames[q];
+                    f.q = quat(fr[q].q[0], fr[q].q[1], fr[q].q[2], fr[q].q[3]), f.w = vec3(fr[q].w[0], fr[q].w[1], fr[q].w[2]);
+                }
+                for (size_t q = 0; q < jt.size(); q++) {
+                    Joint& j = b.joints[q];
+                    j.local_offset = vec3(jt[q].local_offset[0], jt[q].local_offset[1], jt[q].local_offset[2]);
+                    j.rel_rot = quat(jt[q].rel_rot[0], jt[q].rel_rot[1], jt[q].rel_rot[2], jt[q].rel_rot[3]);
+                    j.plastic_lin = jt[q].plastic_lin, j.plastic_ang = jt[q].plastic_ang, j.stress = jt[q].stress;
+                    j.broken = jt[q].broken != 0;
+                }
+            }
             if (b.slides.empty()) continue;
             std::vector<bl_slide> sl(b.slides.size());
             check(bl_read_slides(ctx_, i, sl.data()));
