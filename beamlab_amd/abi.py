@@ -38,6 +38,11 @@ JOINT = np.dtype([("parent_frame", "<u4"), ("child_node", "<u4"), ("child_frame"
                  [(n, "<f4") for n in ("k_lin", "d_lin", "k_ang", "d_ang", "break_force", "break_torque",
                                        "yield_angle", "max_stretch", "max_bend", "plastic_lin", "plastic_ang",
                                        "stress")] + [("broken", "<u4")])
+WHEEL = np.dtype([("axle0", "<u4"), ("axle1", "<u4"), ("node0", "<u4"), ("n_nodes", "<u4"), ("radius", "<f4"),
+                  ("propulsed", "<i4"), ("arm", "<i4"), ("near_attach", "<i4")] +
+                 [(n, "<f4") for n in ("mass", "torque", "drive_torque", "brake", "speed", "avg_speed", "last_torque",
+                                       "last_retorque")] + [("detached", "<u4"), ("ring", "<u4")])
+assert WHEEL.itemsize == 72
 assert NODE.itemsize == 32 and NODE_INFO.itemsize == 20 and BEAM.itemsize == 52
 assert SHOCK.itemsize == 92 and SLIDE.itemsize == 32 and FRAME.itemsize == 44 and JOINT.itemsize == 92
 assert TRIANGLE.itemsize == 20 and CAPSULE.itemsize == 16 and BOX.itemsize == 92
@@ -59,7 +64,8 @@ MODE_AUTO, MODE_STEPPED, MODE_PERSISTENT, MODE_RESIDENT, MODE_HALO = 0, 1, 2, 3,
 class Settings(ctypes.Structure):
     _fields_ = [("gravity", ctypes.c_float * 3), ("dt", ctypes.c_float), ("collision_hz", ctypes.c_float),
                 ("frame_theta", ctypes.c_float), ("frame_dissipation", ctypes.c_float),
-                ("inter_body_collisions", ctypes.c_int32), ("sleeping", ctypes.c_int32), ("kill_y", ctypes.c_float)]
+                ("inter_body_collisions", ctypes.c_int32), ("sleeping", ctypes.c_int32), ("kill_y", ctypes.c_float),
+                ("tyre_grip", ctypes.c_float)]
 
 
 class BodyOpts(ctypes.Structure):
@@ -78,7 +84,8 @@ class BodyDesc(ctypes.Structure):
                 ("opts", BodyOpts), ("shocks", ctypes.c_void_p), ("n_shocks", ctypes.c_uint32),
                 ("slides", ctypes.c_void_p), ("n_slides", ctypes.c_uint32), ("rail", ctypes.c_void_p),
                 ("n_rail", ctypes.c_uint32), ("frames", ctypes.c_void_p), ("n_frames", ctypes.c_uint32),
-                ("joints", ctypes.c_void_p), ("n_joints", ctypes.c_uint32)]
+                ("joints", ctypes.c_void_p), ("n_joints", ctypes.c_uint32), ("wheels", ctypes.c_void_p),
+                ("n_wheels", ctypes.c_uint32), ("wheel_nodes", ctypes.c_void_p), ("n_wheel_nodes", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -175,6 +182,8 @@ SIGNATURES = {
     "bl_body_broken_beams": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64)]),
     "bl_read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "bl_read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+    "bl_read_wheels": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+    "bl_set_wheel_controls": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "bl_reset_stats": (ctypes.c_int, [_P]),
     "bl_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
@@ -279,6 +288,7 @@ def default_settings():
     s.inter_body_collisions = 1
     s.sleeping = 0
     s.kill_y = -200.0
+    s.tyre_grip = 1.0
     return s
 
 

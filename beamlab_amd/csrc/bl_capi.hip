@@ -37,10 +37,17 @@ struct HostBody {
     std::vector<uint32_t> rail;
     std::vector<bl_frame> frames; // (q, w and the joints' state: the step's)
     std::vector<bl_joint> joints;
+    std::vector<bl_wheel> wheels; // (torque, brake and the speeds: the step's)
+    std::vector<uint32_t> wnodes;
+    std::vector<float4> ctl;      // a controller's torque, brake, on (pre_substep) per wheel
+    float wsp[2] = {0, 0};        // SoftBody::wheel_speed, wheel_spin
+    bool tyres = false;           // it has tyre nodes
     bl_body_opts opts;
     uint32_t node0 = 0, beam0 = 0, tri0 = 0, cap0 = 0;
     uint32_t slide0 = 0; // its first slide in the device array
     uint32_t frame0 = 0, joint0 = 0; // its first frame and joint in the device arrays
+    uint32_t wheel0 = 0;             // its first wheel in the device array
+    int32_t xi = -1;                 // its index among the bodies with extras (the device's per-body records)
     // as of the next frame's start: top speed of the last frame and bounds (World::build_islands reads them)
     float max_speed = 0;
     f3 mn = mk(1e30f, 1e30f, 1e30f), mx = mk(-1e30f, -1e30f, -1e30f);
@@ -147,6 +154,12 @@ struct bl_ctx {
     DevArr<bl_joint> xjt;
     DevArr<float4> fq, fw, ftq;
     DevArr<uint32_t> fnode;
+    DevArr<bl_wheel> xwh;
+    DevArr<uint32_t> xwn;
+    DevArr<float4> xctl;
+    DevArr<float2> xwsp;
+    DevArr<uint2> xty;
+    DevArr<float4> xtsum, xtneed, xtslide;
     XArgs xargs{};
     std::map<std::pair<int, int>, hipGraphExec_t> sgraphs; // the split form's captured substeps
     ContactEngine contacts;
@@ -199,6 +212,8 @@ struct bl_ctx {
         wstat.release();
         xbody.release(), shx.release(), xsh.release(), xsl.release(), xrail.release();
         xjt.release(), fq.release(), fw.release(), ftq.release(), fnode.release();
+        xwh.release(), xwn.release(), xctl.release(), xwsp.release(), xty.release(), xtsum.release(), xtneed.release();
+        xtslide.release();
         xargs = XArgs{};
         contacts.release();
         frame_pending = false;
@@ -238,7 +253,9 @@ bool unsupported_opts(const bl_body_opts& o) {
 // some body has shocks or slide nodes: the world steps in the split form (F holds the node forces between phases)
 bool has_extras(const bl_ctx* c) {
     for (const HostBody& hb : c->bodies)
-        if (!hb.shocks.empty() || !hb.slides.empty() || !hb.joints.empty() || !hb.frames.empty()) return true;
+        if (!hb.shocks.empty() || !hb.slides.empty() || !hb.joints.empty() || !hb.frames.empty() || !hb.wheels.empty() ||
+            hb.tyres)
+            return true;
     return false;
 }
 
@@ -264,6 +281,10 @@ int pull_state(bl_ctx* c) {
     if (!torn.empty()) HIPCHK(hipMemcpy(torn.data(), c->tri_torn.p, 4 * torn.size(), hipMemcpyDeviceToHost));
     std::vector<bl_slide> sl(c->xsl.n);
     if (!sl.empty()) HIPCHK(hipMemcpy(sl.data(), c->xsl.p, sizeof(bl_slide) * sl.size(), hipMemcpyDeviceToHost));
+    std::vector<bl_wheel> wh(c->xwh.n);
+    if (!wh.empty()) HIPCHK(hipMemcpy(wh.data(), c->xwh.p, sizeof(bl_wheel) * wh.size(), hipMemcpyDeviceToHost));
+    std::vector<float2> wsp(c->xwsp.n);
+    if (!wsp.empty()) HIPCHK(hipMemcpy(wsp.data(), c->xwsp.p, sizeof(float2) * wsp.size(), hipMemcpyDeviceToHost));
     std::vector<bl_joint> jt(c->xjt.n);
     if (!jt.empty()) HIPCHK(hipMemcpy(jt.data(), c->xjt.p, sizeof(bl_joint) * jt.size(), hipMemcpyDeviceToHost));
     std::vector<float4> fq(c->fq.n), fw(c->fw.n);
@@ -280,6 +301,13 @@ int pull_state(bl_ctx* c) {
             bl_frame& f = hb.frames[q];
             f.q[0] = a.x, f.q[1] = a.y, f.q[2] = a.z, f.q[3] = a.w, f.w[0] = b.x, f.w[1] = b.y, f.w[2] = b.z;
         }
+        for (size_t q = 0; q < hb.wheels.size() && hb.wheel0 + q < wh.size(); q++) {
+            const bl_wheel& d = wh[hb.wheel0 + q];
+            bl_wheel& w = hb.wheels[q];
+            w.torque = d.torque, w.drive_torque = d.drive_torque, w.brake = d.brake, w.speed = d.speed;
+            w.avg_speed = d.avg_speed, w.last_torque = d.last_torque, w.last_retorque = d.last_retorque;
+        }
+        if (hb.xi >= 0 && (size_t)hb.xi < wsp.size()) hb.wsp[0] = wsp[hb.xi].x, hb.wsp[1] = wsp[hb.xi].y;
         for (size_t q = 0; q < hb.joints.size() && hb.joint0 + q < jt.size(); q++) {
             const bl_joint& d = jt[hb.joint0 + q];
             bl_joint& j = hb.joints[q];
@@ -632,10 +660,36 @@ int build(bl_ctx* c) {
         std::vector<bl_slide> sl;
         std::vector<uint32_t> rail, fn;
         std::vector<bl_joint> jt;
-        std::vector<float4> fq4, fw4;
+        std::vector<float4> fq4, fw4, ctl;
+        std::vector<bl_wheel> wh;
+        std::vector<uint32_t> wn;
+        std::vector<uint2> ty;
         for (HostBody& hb : c->bodies) {
             hb.slide0 = (uint32_t)sl.size(), hb.frame0 = (uint32_t)fn.size(), hb.joint0 = (uint32_t)jt.size();
-            if (hb.shocks.empty() && hb.slides.empty() && hb.joints.empty() && hb.frames.empty()) continue;
+            hb.wheel0 = (uint32_t)wh.size(), hb.xi = -1;
+            if (hb.shocks.empty() && hb.slides.empty() && hb.joints.empty() && hb.frames.empty() && hb.wheels.empty() &&
+                !hb.tyres)
+                continue;
+            hb.xi = (int32_t)(xb.size() / 3);
+            const uint32_t w0 = (uint32_t)wh.size(), t0 = (uint32_t)ty.size();
+            for (size_t k = 0; k < hb.wheels.size(); k++) {
+                bl_wheel y = hb.wheels[k];
+                y.axle0 += hb.node0, y.axle1 += hb.node0;
+                y.arm = y.arm >= 0 ? y.arm + (int32_t)hb.node0 : -1;
+                y.near_attach = y.near_attach >= 0 ? y.near_attach + (int32_t)hb.node0 : -1;
+                y.node0 = (uint32_t)wn.size();
+                for (uint32_t j = 0; j < hb.wheels[k].n_nodes; j++) wn.push_back(hb.node0 + hb.wnodes[hb.wheels[k].node0 + j]);
+                wh.push_back(y);
+                ctl.push_back(hb.ctl[k]);
+            }
+            if (hb.tyres) {
+                // (World::collide_static's node_wheel, src/phys/world.cpp:873-877: a later wheel listing a node takes it)
+                std::vector<uint32_t> nw(hb.nodes.size(), 0xffffffffu);
+                for (size_t k = 0; k < hb.wheels.size(); k++)
+                    for (uint32_t j = 0; j < hb.wheels[k].n_nodes; j++) nw[hb.wnodes[hb.wheels[k].node0 + j]] = w0 + (uint32_t)k;
+                for (size_t i = 0; i < hb.nodes.size(); i++)
+                    if (hb.info[i].flags & BL_NF_TYRE) ty.push_back(make_uint2(hb.node0 + (uint32_t)i, nw[i]));
+            }
             const uint32_t s0 = (uint32_t)sh.size(), l0 = (uint32_t)sl.size();
             const uint32_t j0 = (uint32_t)jt.size(), f0 = (uint32_t)fn.size();
             for (const bl_frame& f : hb.frames) {
@@ -664,15 +718,26 @@ int build(bl_ctx* c) {
             for (uint32_t r : hb.rail) rail.push_back(hb.node0 + r);
             xb.push_back(make_uint4(s0, (uint32_t)sh.size(), l0, (uint32_t)sl.size()));
             xb.push_back(make_uint4(j0, (uint32_t)jt.size(), f0, (uint32_t)fn.size()));
+            xb.push_back(make_uint4(w0, (uint32_t)wh.size(), t0, (uint32_t)ty.size()));
         }
+        std::vector<float2> wsp(xb.size() / 3);
+        for (const HostBody& hb : c->bodies)
+            if (hb.xi >= 0) wsp[hb.xi] = make_float2(hb.wsp[0], hb.wsp[1]);
+        if ((!wh.empty() && ((rc = c->xwh.upload(wh)) || (rc = c->xctl.upload(ctl)) || (rc = c->xtsum.alloc(wh.size())))) ||
+            (!wn.empty() && (rc = c->xwn.upload(wn))) || (!wsp.empty() && (rc = c->xwsp.upload(wsp))) ||
+            (!ty.empty() && ((rc = c->xty.upload(ty)) || (rc = c->xtneed.alloc(ty.size())) ||
+                             (rc = c->xtslide.alloc(ty.size())))))
+            return rc;
         if ((!xb.empty() && (rc = c->xbody.upload(xb))) || (!sx.empty() && (rc = c->shx.upload(sx))) ||
             (!sh.empty() && (rc = c->xsh.upload(sh))) || (!sl.empty() && (rc = c->xsl.upload(sl))) ||
             (!rail.empty() && (rc = c->xrail.upload(rail))) || (!jt.empty() && (rc = c->xjt.upload(jt))) ||
             (!fn.empty() && ((rc = c->fnode.upload(fn)) || (rc = c->fq.upload(fq4)) || (rc = c->fw.upload(fw4)) ||
                              (rc = c->ftq.upload(std::vector<float4>(fn.size(), make_float4(0, 0, 0, 0)))))))
             return rc;
-        c->xargs = XArgs{c->xbody.p, (uint32_t)(xb.size() / 2), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p,
-                         c->xjt.p, c->fq.p, c->fw.p, c->ftq.p, c->fnode.p, (uint32_t)fn.size()};
+        c->xargs = XArgs{c->xbody.p, (uint32_t)(xb.size() / 3), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p,
+                         c->xjt.p, c->fq.p, c->fw.p, c->ftq.p, c->fnode.p, (uint32_t)fn.size(),
+                         c->xwh.p, c->xwn.p, c->xctl.p, c->xwsp.p, ty.empty() ? nullptr : c->xty.p, c->xtsum.p,
+                         c->xtneed.p, c->xtslide.p, c->s.tyre_grip};
     }
     c->built_block = T, c->built_per = effective_per(c);
     c->built = true;
@@ -858,7 +923,9 @@ int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk
                 return rc;
             if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
         }
-        if ((rc = launch_integrate(a, c->stream, parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt))) return rc;
+        if ((rc = launch_tyres(a, c->xargs, c->stream, parity)) || (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) ||
+            (rc = launch_frames(c->xargs, c->stream, a.dt)))
+            return rc;
         parity ^= 1;
     }
     if (tail && ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity))))
@@ -985,7 +1052,8 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
                 g_last_error = "contact narrow phase or response failed";
                 return rc;
             }
-            if ((rc = launch_integrate(a, c->stream, c->parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
+            if ((rc = launch_tyres(a, c->xargs, c->stream, c->parity)) ||
+                (rc = launch_integrate(a, c->stream, c->parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
                 return rc;
             c->parity ^= 1;
             nnb = 0;
@@ -1038,7 +1106,8 @@ int enqueue_split(bl_ctx* c, int substeps) {
     int rc;
     auto one = [&](int parity) -> int {
         if ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity)) ||
-            (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
+            (rc = launch_tyres(a, c->xargs, c->stream, parity)) || (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) ||
+            (rc = launch_frames(c->xargs, c->stream, a.dt)))
             return rc;
         return BL_OK;
     };
@@ -1187,6 +1256,7 @@ void bl_default_settings(bl_settings* s) {
     s->inter_body_collisions = 1;
     s->sleeping = 0;
     s->kill_y = -200.0f;
+    s->tyre_grip = 1.0f;
 }
 
 void bl_default_body_opts(bl_body_opts* o) {
@@ -1346,11 +1416,19 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     }
     for (uint32_t j = 0; j < d->n_beams; j++)
         if (d->beams[j].a >= d->n_nodes || d->beams[j].b >= d->n_nodes || d->beams[j].type > BL_BT_SUPPORT) return BL_ERR_ARG;
-    for (uint32_t i = 0; i < d->n_nodes; i++)
-        if (d->info[i].flags & BL_NF_TYRE) {
-            g_last_error = "tyre nodes are not stepped on the GPU";
+    if ((d->n_wheels && !d->wheels) || (d->n_wheel_nodes && !d->wheel_nodes)) return BL_ERR_ARG;
+    for (uint32_t q = 0; q < d->n_wheel_nodes; q++)
+        if (d->wheel_nodes[q] >= d->n_nodes) return BL_ERR_ARG;
+    for (uint32_t q = 0; q < d->n_wheels; q++) {
+        const bl_wheel& w = d->wheels[q];
+        if (w.ring) {
+            g_last_error = "ring tyres are not stepped on the GPU";
             return BL_ERR_UNSUPPORTED;
         }
+        if (w.axle0 >= d->n_nodes || w.axle1 >= d->n_nodes || (uint64_t)w.node0 + w.n_nodes > d->n_wheel_nodes ||
+            w.arm >= (int32_t)d->n_nodes || w.near_attach >= (int32_t)d->n_nodes || d->n_wheels > 32767)
+            return BL_ERR_ARG;
+    }
     for (uint32_t t = 0; t < d->n_tris; t++) {
         const bl_triangle& tr = d->tris[t];
         if (tr.a >= d->n_nodes || tr.b >= d->n_nodes || tr.c >= d->n_nodes) return BL_ERR_ARG;
@@ -1403,6 +1481,10 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
     if (d->n_rail) hb.rail.assign(d->rail, d->rail + d->n_rail);
     if (d->n_frames) hb.frames.assign(d->frames, d->frames + d->n_frames);
     if (d->n_joints) hb.joints.assign(d->joints, d->joints + d->n_joints);
+    if (d->n_wheels) hb.wheels.assign(d->wheels, d->wheels + d->n_wheels);
+    if (d->n_wheel_nodes) hb.wnodes.assign(d->wheel_nodes, d->wheel_nodes + d->n_wheel_nodes);
+    hb.ctl.assign(d->n_wheels, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < d->n_nodes; i++) hb.tyres |= (d->info[i].flags & BL_NF_TYRE) != 0;
     hb.opts = d->opts;
     hb.compute_bounds(); // (SoftBody::compute_aabb at World::add_body, src/phys/world.cpp:283-289)
     c->bodies.push_back(std::move(hb));
@@ -1476,6 +1558,31 @@ int bl_read_slides(bl_ctx* c, uint32_t id, bl_slide* out) {
     if (rc) return rc;
     const HostBody& hb = c->bodies[id];
     if (out && !hb.slides.empty()) std::memcpy(out, hb.slides.data(), sizeof(bl_slide) * hb.slides.size());
+    return BL_OK;
+}
+
+int bl_read_wheels(bl_ctx* c, uint32_t id, bl_wheel* wheels_out, float speed_spin[2]) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    const HostBody& hb = c->bodies[id];
+    if (wheels_out && !hb.wheels.empty()) std::memcpy(wheels_out, hb.wheels.data(), sizeof(bl_wheel) * hb.wheels.size());
+    if (speed_spin) speed_spin[0] = hb.wsp[0], speed_spin[1] = hb.wsp[1];
+    return BL_OK;
+}
+
+int bl_set_wheel_controls(bl_ctx* c, uint32_t id, const float* torque, const float* brake) {
+    if (!c || id >= c->bodies.size()) return BL_ERR_ARG;
+    HostBody& hb = c->bodies[id];
+    const bool on = torque && brake;
+    for (size_t k = 0; k < hb.wheels.size(); k++) hb.ctl[k] = on ? make_float4(torque[k], brake[k], 1.0f, 0) : make_float4(0, 0, 0, 0);
+    if (c->built && !hb.wheels.empty() && c->xctl.n >= hb.wheel0 + hb.wheels.size()) {
+        // (the step in flight reads the controls: it ends first)
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->xctl.p + hb.wheel0, hb.ctl.data(), sizeof(float4) * hb.ctl.size(), hipMemcpyHostToDevice));
+    }
     return BL_OK;
 }
 

@@ -414,4 +414,43 @@ BL_HD f3 primitive_collision(f3 F, f3 vel, float mass, f3 normal, float dt, cons
     return out;
 }
 
+// tyre_contact (src/phys/static_world.cpp:53-80): a tyre node's normal force, the tangential force that would hold it
+// (need) up to its grip (cap), and the force it slides with
+struct tyre_hit {
+    f3 normal_force, need, slide;
+    float cap;
+    bool touching;
+};
+BL_HD tyre_hit tyre_contact(f3 F, f3 vel, float mass, f3 normal, float dt, const ground_model& gm, float pen,
+                            float friction_coef) {
+    tyre_hit t{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
+    const float Vn = dot(vel, normal);
+    const float Fn = dot(F, normal);
+    const float sgl = gm.soft_depth;
+    if (sgl > 0 && pen >= 0) {
+        t.normal_force = add(t.normal_force, mul(vel, -gm.fluid_drag * smin(1.0f, pen / sgl + 0.2f)));
+        float Fb = gm.fluid_density * smin(pen, sgl) * 9.807f * 0.02f;
+        if (Vn >= 0 && Fn < 0 && Fb > -Fn) Fb = -Fn;
+        t.normal_force = add(t.normal_force, mul(normal, Fb));
+    }
+    if (pen < sgl) return t;
+    float R = -Fn;
+    if (Vn < 0) R -= (0.8f * Vn - smin(0.2f * (pen - sgl) / dt, 2.0f)) * mass / dt;
+    if (R <= 0) return t;
+    t.touching = true;
+    t.normal_force = add(t.normal_force, mul(normal, R));
+    const f3 slipF = sub(F, mul(normal, Fn));
+    const f3 slipV = sub(vel, mul(normal, Vn));
+    const float G = R * gm.strength * friction_coef;
+    t.need = sub(mk(-slipF.x, -slipF.y, -slipF.z), mul(slipV, 0.5f * mass / dt));
+    t.cap = gm.ms * G;
+    const float sv = length(slipV);
+    const float mu_slide = smax(gm.mc, 0.85f * gm.ms);
+    const float g = mu_slide + (gm.ms - mu_slide) * bl_expf(-pow_alpha(sv / gm.vs, gm.alpha)) + smin(gm.t2 * sv, 5.0f);
+    const float nl = length(t.need);
+    const f3 dir = sv > 0.05f ? mul(slipV, -1.0f / sv) : (nl > 1e-6f ? mul(t.need, 1.0f / nl) : mk(0, 0, 0));
+    t.slide = mul(dir, g * G);
+    return t;
+}
+
 } // namespace blg

@@ -185,6 +185,8 @@ __device__ __forceinline__ void node_finish(const KArgs& a, uint32_t info, float
                     const float fl = length(fc);
                     if (fl > max_force) fc = mul(fc, max_force / fl);
                     f = add(f, fc);
+                } else if (info & BL_NF_TYRE) {
+                    // (a tyre node: its contact force is in f already, from the tyre pass)
                 } else {
                     f = add(f, primitive_collision(f, v, mass, n, a.dt, gm, depth, aux.y, bp.push_max, bp.slop, bp.bounce));
                 }
@@ -1266,12 +1268,71 @@ __device__ void joint_eval(const KArgs& a, const XArgs& x, bl_joint& j, int pari
     if (brk && snap) j.broken = 1;
 }
 
-// a thread per body with shocks, joints or slides: its frames' torques cleared (clear_forces), its shocks, joints
-// and slides, in order (World::simulate_island's base_forces, src/phys/world.cpp:3255-3273)
+// SoftBody::compute_wheel_forces (src/phys/softbody.cpp:689-767), one node wheel, first = true (the controllers'
+// substep): the drive and brake torques as tangential tread forces, the tread speed, the arm's reaction torque
+__device__ void wheel_eval(const KArgs& a, const XArgs& x, bl_wheel& w, int parity, float& speed_sum, float& spin_sum,
+                           int& nprop) {
+    if (w.detached || w.n_nodes == 0) return;
+    const float dt = a.dt;
+    w.drive_torque = w.torque;
+    if (w.brake > 0) {
+        const float stop = -w.avg_speed * w.radius * w.mass / dt - w.last_retorque;
+        w.torque += w.speed > 0 ? clampf(stop, -w.brake, 0.0f) : clampf(stop, 0.0f, w.brake);
+    }
+    const float4 H0 = a.pm[parity][w.axle0], H1 = a.pm[parity][w.axle1];
+    const float4 U0 = a.vm[parity][w.axle0], U1 = a.vm[parity][w.axle1];
+    const f3 axis = normalize(sub(mk(H1.x, H1.y, H1.z), mk(H0.x, H0.y, H0.z)));
+    const int nn = (int)w.n_nodes;
+    const float per = w.torque / (float)nn;
+    float expected = w.speed;
+    float speed = 0;
+    for (int j = 0; j < nn; j++) {
+        const float4 hp = (j & 1) ? H1 : H0, hv = (j & 1) ? U1 : U0;
+        const uint32_t ni = x.wn[w.node0 + j];
+        const float4 P = a.pm[parity][ni], V = a.vm[parity][ni];
+        f3 rad = sub(mk(P.x, P.y, P.z), mk(hp.x, hp.y, hp.z));
+        const float rl2 = dot(rad, rad);
+        if (rl2 < 1e-8f) continue;
+        const float inv = 1.0f / fsqrt(rl2);
+        if (w.propulsed == 2) rad = mk(-rad.x, -rad.y, -rad.z);
+        const f3 dir = mul(cross(axis, rad), inv);
+        add_force(a, ni, mul(dir, per * inv));
+        speed += dot(sub(mk(V.x, V.y, V.z), mk(hv.x, hv.y, hv.z)), dir);
+    }
+    w.speed = speed / (float)nn;
+    w.avg_speed = 0.99f * w.avg_speed + 0.1f * w.speed;
+    if (w.propulsed == 1) {
+        speed_sum += w.speed;
+        spin_sum += w.speed / smax(0.05f, w.radius);
+        nprop++;
+    }
+    if (w.mass > 0 && w.radius > 0) {
+        expected += (w.last_torque / w.radius) / w.mass * dt;
+        w.last_retorque = w.mass * (w.speed - expected) / dt;
+    }
+    if (w.arm >= 0 && w.near_attach >= 0 && ffabs(w.torque) > 0.01f) {
+        const float4 PA = a.pm[parity][w.arm], PN = a.pm[parity][w.near_attach];
+        const f3 rr = sub(mk(PA.x, PA.y, PA.z), mk(PN.x, PN.y, PN.z));
+        f3 r = sub(rr, mul(axis, dot(rr, axis)));
+        const float off = length(sub(rr, r)), rl = length(r);
+        if (rl > 0.01f && 2 * off < rl) {
+            r = divs(r, rl);
+            const f3 cf = mul(mul(cross(axis, r), 0.5f * w.torque / rl), 1.0f - 2.0f * off / rl);
+            sub_force(a, (uint32_t)w.arm, cf);
+            add_force(a, (uint32_t)w.near_attach, cf);
+        }
+    }
+    w.last_torque = w.torque;
+    w.torque = 0;
+}
+
+// a thread per body with shocks, joints, wheels or slides: its frames' torques cleared (clear_forces), its shocks,
+// joints, wheels (after its controller's inputs) and slides, in order (World::simulate_island's base_forces,
+// src/phys/world.cpp:3255-3273)
 __global__ void k_extras(KArgs a, XArgs x, int parity) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x.nbody) return;
-    const uint4 r = x.body[2 * i], r2 = x.body[2 * i + 1];
+    const uint4 r = x.body[3 * i], r2 = x.body[3 * i + 1], r3 = x.body[3 * i + 2];
     for (uint32_t f = r2.z; f < r2.w; f++) x.ftq[f] = make_float4(0, 0, 0, 0);
     for (uint32_t q = r.x; q < r.y; q++) shock_eval(a, x.sh[q], x.shx[q], parity, a.ninfo[x.shx[q].z] >> 16);
     if (r2.x < r2.y) {
@@ -1284,6 +1345,18 @@ __global__ void k_extras(KArgs a, XArgs x, int parity) {
             x.jt[q] = j;
         }
     }
+    if (r3.x < r3.y) {
+        float speed_sum = 0, spin_sum = 0;
+        int nprop = 0;
+        for (uint32_t q = r3.x; q < r3.y; q++) {
+            bl_wheel w = x.wh[q];
+            const float4 c = x.ctl[q];
+            if (c.z != 0) w.torque = c.x, w.brake = c.y;
+            wheel_eval(a, x, w, parity, speed_sum, spin_sum, nprop);
+            x.wh[q] = w;
+        }
+        if (nprop > 0) x.wsp[i] = make_float2(speed_sum / (float)nprop, spin_sum / (float)nprop);
+    }
     for (uint32_t q = r.z; q < r.w; q++) {
         bl_slide s = x.sl[q];
         slide_eval(a, s, x.rail, parity);
@@ -1293,6 +1366,61 @@ __global__ void k_extras(KArgs a, XArgs x, int parity) {
 
 int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
     if (x.nbody) hipLaunchKernelGGL(k_extras, dim3((x.nbody + 63) / 64), dim3(64), 0, st, a, x, parity);
+    return BL_OK_OR_HIP();
+}
+
+// World::collide_static's tyre nodes (src/phys/world.cpp:799-892, boxes only): a thread per body with tyre nodes, over
+// them in node order - the static contact by tyre_contact (a yielding obstacle's contact is the integration's), its
+// normal force into F; then each wheel's touching tyre nodes share its grip: the patch holds (each node gets the
+// force that would hold it) when its summed need is within its summed grip, else each slides. The integration counts
+// these contacts and leaves their forces as they are.
+__global__ void k_tyres(KArgs a, XArgs x, int parity) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= x.nbody) return;
+    const uint4 r3 = x.body[3 * i + 2];
+    if (r3.z >= r3.w) return;
+    for (uint32_t w = r3.x; w < r3.y; w++) x.tsum[w] = make_float4(0, 0, 0, 0);
+    for (uint32_t q = r3.z; q < r3.w; q++) {
+        const uint2 e = x.ty[q];
+        x.tslide[q] = make_float4(0, 0, 0, 0);
+        const uint32_t info = a.ninfo[e.x];
+        const float4 P = a.pm[parity][e.x], V = a.vm[parity][e.x];
+        if (!(info & BL_NF_GROUND) || !(P.w > 0) || P.y < a.kill_y || a.nbox <= 0) continue;
+        const float2 aux = a.naux[e.x];
+        f3 n;
+        float depth, max_force;
+        uint32_t surf;
+        if (!collide_point(a, mk(P.x, P.y, P.z), aux.x, n, depth, surf, max_force) || max_force > 0) continue;
+        const ground_model& gm = k_gm[surf < BL_SURF_N ? surf : 0];
+        const float4 F = a.F[e.x];
+        const tyre_hit t = tyre_contact(mk(F.x, F.y, F.z), mk(V.x, V.y, V.z), V.w, n, a.dt, gm, depth, aux.y * x.tyre_grip);
+        a.F[e.x] = make_float4(F.x + t.normal_force.x, F.y + t.normal_force.y, F.z + t.normal_force.z, F.w);
+        if (!t.touching) continue;
+        x.tneed[q] = make_float4(t.need.x, t.need.y, t.need.z, t.cap);
+        x.tslide[q] = make_float4(t.slide.x, t.slide.y, t.slide.z, 1.0f);
+        if (e.y != 0xffffffffu) {
+            const float4 s = x.tsum[e.y];
+            x.tsum[e.y] = make_float4(s.x + t.need.x, s.y + t.need.y, s.z + t.need.z, s.w + t.cap);
+        }
+    }
+    for (uint32_t q = r3.z; q < r3.w; q++) {
+        const float4 sl = x.tslide[q];
+        if (sl.w == 0) continue;
+        const uint2 e = x.ty[q];
+        const float4 nd = x.tneed[q];
+        bool stick;
+        if (e.y != 0xffffffffu) {
+            const float4 s = x.tsum[e.y];
+            stick = length(mk(s.x, s.y, s.z)) <= s.w;
+        } else {
+            stick = length(mk(nd.x, nd.y, nd.z)) <= nd.w;
+        }
+        add_force(a, e.x, stick ? mk(nd.x, nd.y, nd.z) : mk(sl.x, sl.y, sl.z));
+    }
+}
+
+int launch_tyres(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
+    if (x.ty && x.nbody) hipLaunchKernelGGL(k_tyres, dim3((x.nbody + 63) / 64), dim3(64), 0, st, a, x, parity);
     return BL_OK_OR_HIP();
 }
 

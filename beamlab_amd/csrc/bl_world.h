@@ -101,8 +101,8 @@ struct KArgs {
 // The bodies' shocks and slide nodes (the forces that follow the beams in World::simulate_island's base_forces,
 // src/phys/world.cpp:3255-3273): run after the force phase, into F, a thread per body with any, in the body's order.
 struct XArgs {
-    const uint4* body;    // per such body, two records: first shock, end shock, first slide, end slide; first joint,
-                          // end joint, first frame, end frame
+    const uint4* body;    // per such body, three records: first shock, end shock, first slide, end slide; first joint,
+                          // end joint, first frame, end frame; first wheel, end wheel, first tyre node, end tyre node
     uint32_t nbody;
     const bl_shock* sh;   // the shocks (beam: the body's beam index, unused here)
     const uint4* shx;     // per shock: its beam's a-end and b-end incidences, its a and b nodes
@@ -114,8 +114,24 @@ struct XArgs {
     float4* ftq;          // per frame: torque of the substep
     const uint32_t* fnode;
     uint32_t nframes;
+    // wheels: the records (axles, arm and near_attach global; node0 into wn), their tyre nodes, a controller's
+    // (torque, brake, on) per wheel, and per such body its wheel_speed, wheel_spin
+    bl_wheel* wh;
+    const uint32_t* wn;
+    const float4* ctl;
+    float2* wsp;
+    // per body, its tyre nodes in node order (node, wheel or ~0), and the tyre patches' scratch: per wheel the need
+    // and cap sums, per tyre node its need (cap in w) and slide (touching in w)
+    const uint2* ty;
+    float4* tsum;
+    float4* tneed;
+    float4* tslide;
+    float tyre_grip;
 };
 int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
+// the tyre nodes' static contacts and their wheels' shared grip (World::collide_static's tyre part), into F before the
+// integration, whose static contact then leaves those nodes' forces as they are
+int launch_tyres(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
 // the frames' integration after the nodes' (SoftBody::integrate_frames)
 int launch_frames(const XArgs& x, hipStream_t st, float dt);
 

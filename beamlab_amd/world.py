@@ -27,6 +27,8 @@ class SoftBody:
         self.rail = np.zeros(0, np.uint32)
         self.frames = np.zeros(0, abi.FRAME)
         self.joints = np.zeros(0, abi.JOINT)
+        self.wheels = np.zeros(0, abi.WHEEL)
+        self.wheel_nodes = np.zeros(0, np.uint32)
         self.opts = abi.default_body_opts()
         self._pending_nodes, self._pending_beams, self._pending_tris = [], [], []
 
@@ -97,6 +99,18 @@ class SoftBody:
         self.joints = np.concatenate([self.joints, j])
         return len(self.joints) - 1
 
+    # a wheel (Wheel, src/phys/softbody.h:468-490; the vehicle builder fills it): its hub's two axle nodes and its
+    # tyre nodes (even: the axle0 side); mass defaults to the tyre nodes' masses
+    def add_wheel(self, axle0, axle1, nodes, radius=0.4, propulsed=0, arm=-1, near_attach=-1, mass=None):
+        self._flush()
+        w = np.zeros(1, abi.WHEEL)
+        w[0]["axle0"], w[0]["axle1"], w[0]["node0"], w[0]["n_nodes"] = axle0, axle1, len(self.wheel_nodes), len(nodes)
+        w[0]["radius"], w[0]["propulsed"], w[0]["arm"], w[0]["near_attach"] = radius, propulsed, arm, near_attach
+        w[0]["mass"] = np.float32(sum(self.nodes[n]["mass"] for n in nodes)) if mass is None else mass
+        self.wheel_nodes = np.concatenate([self.wheel_nodes, np.asarray(nodes, np.uint32)])
+        self.wheels = np.concatenate([self.wheels, w])
+        return len(self.wheels) - 1
+
     def _flush(self):
         if self._pending_nodes:
             n = np.zeros(len(self._pending_nodes), abi.NODE)
@@ -157,6 +171,8 @@ class SoftBody:
         d.rail, d.n_rail = abi.ptr(self.rail), len(self.rail)
         d.frames, d.n_frames = abi.ptr(self.frames), len(self.frames)
         d.joints, d.n_joints = abi.ptr(self.joints), len(self.joints)
+        d.wheels, d.n_wheels = abi.ptr(self.wheels), len(self.wheels)
+        d.wheel_nodes, d.n_wheel_nodes = abi.ptr(self.wheel_nodes), len(self.wheel_nodes)
         return d
 
 
@@ -282,6 +298,21 @@ class World:
         fr, jt = self.bodies[bid].frames.copy(), self.bodies[bid].joints.copy()
         abi.check(self._lib().bl_read_joints(self._context(), bid, abi.ptr(fr), abi.ptr(jt)), "bl_read_joints")
         return fr, jt
+
+    def read_wheels(self, bid):
+        """(the wheels as the step left them, [wheel_speed, wheel_spin])"""
+        wh, ss = self.bodies[bid].wheels.copy(), np.zeros(2, np.float32)
+        abi.check(self._lib().bl_read_wheels(self._context(), bid, abi.ptr(wh), abi.ptr(ss)), "bl_read_wheels")
+        return wh, ss
+
+    def set_wheel_controls(self, bid, torque=None, brake=None):
+        """A controller setting each wheel's torque and brake at every substep from the next step on (None: none)."""
+        if torque is None or brake is None:
+            abi.check(self._lib().bl_set_wheel_controls(self._context(), bid, None, None), "bl_set_wheel_controls")
+            return
+        self._ctl = (np.ascontiguousarray(torque, np.float32), np.ascontiguousarray(brake, np.float32))
+        abi.check(self._lib().bl_set_wheel_controls(self._context(), bid, abi.ptr(self._ctl[0]), abi.ptr(self._ctl[1])),
+                  "bl_set_wheel_controls")
 
     def read_slides(self, bid):
         out = self.bodies[bid].slides.copy()

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BL_ABI_VERSION 4
+#define BL_ABI_VERSION 5
 
 enum {
     BL_OK = 0,
@@ -168,6 +168,26 @@ typedef struct bl_joint {
     uint32_t broken;
 } bl_joint;
 
+/* A wheel (== bl::phys::Wheel, src/phys/softbody.h:468-490, its node-tyre part; SoftBody::compute_wheel_forces,
+ * src/phys/softbody.cpp:689-767): the drive torque spread over its tyre nodes as tangential tread forces, the
+ * brake's stop torque, the tread's speed, the reaction torque on its arm. Its tyre nodes are
+ * wheel_nodes[node0 .. node0 + n_nodes) of the body description (even: the axle0 side, odd: the axle1 side). The
+ * tyre nodes touching the ground share their wheel's grip (World::collide_static, src/phys/world.cpp:852-892).
+ * torque, drive_torque, brake and the state fields are the step's. Ring tyres (Wheel::ring) are not stepped on the
+ * GPU. */
+typedef struct bl_wheel {
+    uint32_t axle0, axle1;
+    uint32_t node0, n_nodes;
+    float radius;
+    int32_t propulsed;        /* 0 none, 1 forward, 2 backward */
+    int32_t arm, near_attach; /* the reaction torque's arm node and the axle node nearest it; -1: none */
+    float mass;               /* the tyre nodes' mass */
+    float torque, drive_torque, brake;
+    float speed, avg_speed, last_torque, last_retorque;
+    uint32_t detached;
+    uint32_t ring;            /* must be 0 */
+} bl_wheel;
+
 /* A static oriented box (== StaticBox, src/phys/static_world.h:115-126, without hull planes). `rot` holds the
  * columns of the rotation (the box's local axes in world space), as bl::mat3 does; aabb_mn/aabb_mx as
  * StaticBox::update_aabb (src/phys/static_world.cpp:242-246) computes them (bl_box_from_quat fills both). */
@@ -196,6 +216,7 @@ typedef struct bl_settings {
     int32_t inter_body_collisions;
     int32_t sleeping;          /* must be 0: sleeping is not stepped on the GPU */
     float kill_y;              /* StaticWorld::kill_y, default -200 */
+    float tyre_grip;           /* WorldSettings::tyre_grip: the tyre nodes' friction multiplier, default 1 */
 } bl_settings;
 
 /* The per-body settings the path reads (SoftBody, src/phys/softbody.h:649-716). */
@@ -246,6 +267,11 @@ typedef struct bl_body_desc {
     uint32_t n_frames;
     const bl_joint* joints;
     uint32_t n_joints;
+    /* wheels (after the joints, before the slides) and their tyre nodes */
+    const bl_wheel* wheels;
+    uint32_t n_wheels;
+    const uint32_t* wheel_nodes;
+    uint32_t n_wheel_nodes;
 } bl_body_desc;
 
 /* Counters (WorldStats / BodyStats / FemFrame counters: src/phys/world.h:79-132, src/phys/softbody.h:588-593). */
@@ -346,6 +372,15 @@ int bl_read_slides(bl_ctx* ctx, uint32_t id, bl_slide* slides_out);
 
 /* A body's frames and joints as the step left them (either pointer may be null). */
 int bl_read_joints(bl_ctx* ctx, uint32_t id, bl_frame* frames_out, bl_joint* joints_out);
+
+/* A body's wheels as the step left them and its SoftBody::wheel_speed, wheel_spin (either pointer may be null). */
+int bl_read_wheels(bl_ctx* ctx, uint32_t id, bl_wheel* wheels_out, float speed_spin[2]);
+
+/* A controller of the body's wheels (a vehicle's pre_substep hook, src/phys/softbody.h:807, run by base_forces once a
+ * substep before the forces, src/phys/world.cpp:3260, setting Wheel::torque and Wheel::brake): from the next bl_step
+ * on, every substep starts with wheel k's torque = torque[k] and brake = brake[k] (n_wheels values each). Null
+ * pointers remove it: the wheels keep what the step leaves (a torque is spent in one substep, a brake stays). */
+int bl_set_wheel_controls(bl_ctx* ctx, uint32_t id, const float* torque, const float* brake);
 
 /* Counters since bl_create (or the last bl_reset_stats). */
 int bl_get_stats(bl_ctx* ctx, bl_stats* out);

@@ -125,6 +125,13 @@ typedef struct {
     uint32_t nfr;
     bl_joint* joints;
     uint32_t nj;
+    bl_wheel* wheels;
+    uint32_t nw;
+    uint32_t* wnodes;    /* the wheels' tyre nodes */
+    int16_t* node_wheel; /* SoftBody::node_wheel: each node's wheel (the last wheel listing it), or -1 */
+    float* ctl;          /* a controller's torque and brake per wheel (pre_substep), when `controlled` */
+    int controlled;
+    float wheel_speed, wheel_spin;
 } orc_body;
 
 struct orc_world {
@@ -160,6 +167,10 @@ void orc_destroy(orc_world* w) {
         free(w->bodies[i].frames);
         free(w->bodies[i].torque);
         free(w->bodies[i].joints);
+        free(w->bodies[i].wheels);
+        free(w->bodies[i].wnodes);
+        free(w->bodies[i].node_wheel);
+        free(w->bodies[i].ctl);
     }
     free(w->bodies);
     free(w->boxes);
@@ -204,6 +215,17 @@ int orc_add_body(orc_world* w, const bl_body_desc* d) {
     b->joints = (bl_joint*)malloc(sizeof(bl_joint) * (b->nj ? b->nj : 1));
     if (b->nfr) memcpy(b->frames, d->frames, sizeof(bl_frame) * b->nfr);
     if (b->nj) memcpy(b->joints, d->joints, sizeof(bl_joint) * b->nj);
+    b->nw = d->n_wheels;
+    b->wheels = (bl_wheel*)malloc(sizeof(bl_wheel) * (b->nw ? b->nw : 1));
+    b->wnodes = (uint32_t*)malloc(sizeof(uint32_t) * (d->n_wheel_nodes ? d->n_wheel_nodes : 1));
+    b->ctl = (float*)calloc(2 * (b->nw ? b->nw : 1), sizeof(float));
+    if (b->nw) memcpy(b->wheels, d->wheels, sizeof(bl_wheel) * b->nw);
+    if (d->n_wheel_nodes) memcpy(b->wnodes, d->wheel_nodes, sizeof(uint32_t) * d->n_wheel_nodes);
+    /* (World::collide_static's node_wheel, src/phys/world.cpp:873-877: a later wheel listing a node takes it) */
+    b->node_wheel = (int16_t*)malloc(sizeof(int16_t) * (b->n ? b->n : 1));
+    for (uint32_t i = 0; i < b->n; i++) b->node_wheel[i] = -1;
+    for (uint32_t k = 0; k < b->nw; k++)
+        for (uint32_t j = 0; j < b->wheels[k].n_nodes; j++) b->node_wheel[b->wnodes[b->wheels[k].node0 + j]] = (int16_t)k;
     int hulls = 0;
     for (uint32_t t = 0; t < b->ntri; t++) hulls += !b->tris[t].two_sided;
     b->hull_reach = hulls > 0 ? b->opts.hull_depth : 0.0f;
@@ -593,9 +615,117 @@ static int collide_point(const orc_world* w, v3 p, float r, v3* normal, float* d
     return hit;
 }
 
-/* World::collide_static for one body's nodes, boxes only (src/phys/world.cpp:781-869) */
+/* SoftBody::compute_wheel_forces (src/phys/softbody.cpp:689-767) with first = true (the controllers' substep), the
+ * node wheels (ring tyres are refused); wheel_speed and wheel_spin over the forward-propelled ones */
+static void wheel_forces(orc_body* B, float dt) {
+    bl_node* nd = B->nodes;
+    v3* f = B->force;
+    float speed_sum = 0, spin_sum = 0;
+    int nprop = 0;
+    for (uint32_t k = 0; k < B->nw; k++) {
+        bl_wheel* w = &B->wheels[k];
+        if (w->detached || w->n_nodes == 0) continue;
+        w->drive_torque = w->torque;
+        if (w->brake > 0) {
+            float stop = -w->avg_speed * w->radius * w->mass / dt - w->last_retorque;
+            w->torque += w->speed > 0 ? clampf_(stop, -w->brake, 0.0f) : clampf_(stop, 0.0f, w->brake);
+        }
+        const bl_node* h0 = &nd[w->axle0];
+        const bl_node* h1 = &nd[w->axle1];
+        const v3 axis = vnormalize_(vsub(ld3(h1->p), ld3(h0->p)));
+        const int nn = (int)w->n_nodes;
+        const float per = w->torque / (float)nn;
+        float expected = w->speed;
+        float speed = 0;
+        for (int j = 0; j < nn; j++) {
+            const bl_node* hub = (j & 1) ? h1 : h0;
+            const uint32_t ni = B->wnodes[w->node0 + j];
+            v3 rad = vsub(ld3(nd[ni].p), ld3(hub->p));
+            float rl2 = vdot(rad, rad);
+            if (rl2 < 1e-8f) continue;
+            float inv = 1.0f / sqrtf(rl2);
+            if (w->propulsed == 2) rad = V(-rad.x, -rad.y, -rad.z);
+            v3 dir = vmul(vcross_(axis, rad), inv);
+            f[ni] = vadd(f[ni], vmul(dir, per * inv));
+            speed += vdot(vsub(ld3(nd[ni].v), ld3(hub->v)), dir);
+        }
+        w->speed = speed / (float)nn;
+        w->avg_speed = 0.99f * w->avg_speed + 0.1f * w->speed;
+        if (w->propulsed == 1) {
+            speed_sum += w->speed;
+            spin_sum += w->speed / smax(0.05f, w->radius);
+            nprop++;
+        }
+        if (w->mass > 0 && w->radius > 0) {
+            expected += (w->last_torque / w->radius) / w->mass * dt;
+            w->last_retorque = w->mass * (w->speed - expected) / dt;
+        }
+        if (w->arm >= 0 && w->near_attach >= 0 && fabsf(w->torque) > 0.01f) {
+            const v3 rr = vsub(ld3(nd[w->arm].p), ld3(nd[w->near_attach].p));
+            v3 r = vsub(rr, vmul(axis, vdot(rr, axis)));
+            const float off = vlen(vsub(rr, r)), rl = vlen(r);
+            if (rl > 0.01f && 2 * off < rl) {
+                r = vdivs(r, rl);
+                const v3 cf = vmul(vmul(vcross_(axis, r), 0.5f * w->torque / rl), 1.0f - 2.0f * off / rl);
+                f[w->arm] = vsub(f[w->arm], cf);
+                f[w->near_attach] = vadd(f[w->near_attach], cf);
+            }
+        }
+        w->last_torque = w->torque;
+        w->torque = 0;
+    }
+    if (nprop > 0) {
+        B->wheel_speed = speed_sum / (float)nprop;
+        B->wheel_spin = spin_sum / (float)nprop;
+    }
+}
+
+/* tyre_contact (src/phys/static_world.cpp:53-80) */
+typedef struct {
+    v3 normal_force, need, slide;
+    float cap;
+    int touching;
+} tyre_hit;
+static tyre_hit tyre_contact(v3 F, v3 vel, float mass, v3 normal, float dt, const ground_model* gm, float pen,
+                             float friction_coef) {
+    tyre_hit t;
+    memset(&t, 0, sizeof(t));
+    float Vn = vdot(vel, normal);
+    float Fn = vdot(F, normal);
+    float sgl = gm->soft_depth;
+    if (sgl > 0 && pen >= 0) {
+        t.normal_force = vadd(t.normal_force, vmul(vel, -gm->fluid_drag * smin(1.0f, pen / sgl + 0.2f)));
+        float Fb = gm->fluid_density * smin(pen, sgl) * 9.807f * 0.02f;
+        if (Vn >= 0 && Fn < 0 && Fb > -Fn) Fb = -Fn;
+        t.normal_force = vadd(t.normal_force, vmul(normal, Fb));
+    }
+    if (pen < sgl) return t;
+    float R = -Fn;
+    if (Vn < 0) R -= (0.8f * Vn - smin(0.2f * (pen - sgl) / dt, 2.0f)) * mass / dt;
+    if (R <= 0) return t;
+    t.touching = 1;
+    t.normal_force = vadd(t.normal_force, vmul(normal, R));
+    v3 slipF = vsub(F, vmul(normal, Fn));
+    v3 slipV = vsub(vel, vmul(normal, Vn));
+    float G = R * gm->strength * friction_coef;
+    t.need = vsub(V(-slipF.x, -slipF.y, -slipF.z), vmul(slipV, 0.5f * mass / dt));
+    t.cap = gm->ms * G;
+    float sv = vlen(slipV);
+    float mu_slide = smax(gm->mc, 0.85f * gm->ms);
+    float g = mu_slide + (gm->ms - mu_slide) * expf(-powf(sv / gm->vs, gm->alpha)) + smin(gm->t2 * sv, 5.0f);
+    float nl = vlen(t.need);
+    v3 dir = sv > 0.05f ? vmul(slipV, -1.0f / sv) : (nl > 1e-6f ? vmul(t.need, 1.0f / nl) : V(0, 0, 0));
+    t.slide = vmul(dir, g * G);
+    return t;
+}
+
+/* World::collide_static for one body's nodes, boxes only (src/phys/world.cpp:781-869); tyre nodes by tyre_contact,
+ * each wheel's touching tyre nodes then sharing its grip (:850-892) */
 static int collide_static(orc_world* w, orc_body* B, float dt) {
     int contacts = 0;
+    uint32_t ntyre = 0;
+    uint32_t* tyre_node = NULL;
+    tyre_hit* tyre = NULL;
     const float fric_body = B->opts.ground_friction * 1.0f; /* resting is set only by the rest damping (off) */
     for (uint32_t i = 0; i < B->n; i++) {
         bl_node* x = &B->nodes[i];
@@ -620,12 +750,37 @@ static int collide_static(orc_world* w, orc_body* B, float dt) {
             float fl = vlen(f);
             if (fl > max_force) f = vmul(f, max_force / fl);
             B->force[i] = vadd(B->force[i], f);
+        } else if (in->flags & BL_NF_TYRE) {
+            tyre_hit t = tyre_contact(B->force[i], ld3(x->v), x->mass, n, dt, gm, depth,
+                                      in->friction * fric_body * w->s.tyre_grip);
+            B->force[i] = vadd(B->force[i], t.normal_force);
+            if (t.touching) {
+                tyre_node = (uint32_t*)realloc(tyre_node, sizeof(uint32_t) * (ntyre + 1));
+                tyre = (tyre_hit*)realloc(tyre, sizeof(tyre_hit) * (ntyre + 1));
+                tyre_node[ntyre] = i, tyre[ntyre] = t;
+                ntyre++;
+            }
         } else {
             v3 f = prim_coll(B->force[i], ld3(x->v), x->mass, n, dt, gm, depth, in->friction * fric_body,
                              B->opts.contact_push_max, B->opts.contact_slop, B->opts.bounce);
             B->force[i] = vadd(B->force[i], f);
         }
     }
+    if (ntyre) {
+        v3* need_sum = (v3*)calloc(B->nw ? B->nw : 1, sizeof(v3));
+        float* cap_sum = (float*)calloc(B->nw ? B->nw : 1, sizeof(float));
+        for (uint32_t q = 0; q < ntyre; q++) {
+            const int wk = B->node_wheel[tyre_node[q]];
+            if (wk >= 0) need_sum[wk] = vadd(need_sum[wk], tyre[q].need), cap_sum[wk] += tyre[q].cap;
+        }
+        for (uint32_t q = 0; q < ntyre; q++) {
+            const int wk = B->node_wheel[tyre_node[q]];
+            const int stick = wk >= 0 ? vlen(need_sum[wk]) <= cap_sum[wk] : vlen(tyre[q].need) <= tyre[q].cap;
+            B->force[tyre_node[q]] = vadd(B->force[tyre_node[q]], stick ? tyre[q].need : tyre[q].slide);
+        }
+        free(need_sum), free(cap_sum);
+    }
+    free(tyre_node), free(tyre);
     return contacts;
 }
 
@@ -1748,6 +1903,11 @@ int orc_step(orc_world* w, int substeps) {
                 for (uint32_t k = 0; k < B->nfr; k++) B->torque[k] = V(0, 0, 0);
                 if (B->nsh) shock_forces(B);
                 if (B->nj) joint_forces(B);
+                if (B->nw) {
+                    if (B->controlled)
+                        for (uint32_t k = 0; k < B->nw; k++) B->wheels[k].torque = B->ctl[2 * k], B->wheels[k].brake = B->ctl[2 * k + 1];
+                    wheel_forces(B, dt);
+                }
                 if (B->nsl) slide_forces(B);
             }
             if (I->needs_pairs) {
@@ -1793,6 +1953,22 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
     orc_body* B = &w->bodies[id];
     if (nodes && B->n) memcpy(B->nodes, nodes, sizeof(bl_node) * B->n);
     if (beams && B->nb) memcpy(B->beams, beams, sizeof(bl_beam) * B->nb);
+    return 0;
+}
+
+int orc_read_wheels(orc_world* w, uint32_t id, bl_wheel* wheels, float* speed_spin) {
+    if ((int)id >= w->nbodies) return BL_ERR_ARG;
+    orc_body* B = &w->bodies[id];
+    if (wheels && B->nw) memcpy(wheels, B->wheels, sizeof(bl_wheel) * B->nw);
+    if (speed_spin) speed_spin[0] = B->wheel_speed, speed_spin[1] = B->wheel_spin;
+    return 0;
+}
+
+int orc_set_wheel_controls(orc_world* w, uint32_t id, const float* torque, const float* brake) {
+    if ((int)id >= w->nbodies) return BL_ERR_ARG;
+    orc_body* B = &w->bodies[id];
+    B->controlled = torque != NULL && brake != NULL;
+    for (uint32_t k = 0; B->controlled && k < B->nw; k++) B->ctl[2 * k] = torque[k], B->ctl[2 * k + 1] = brake[k];
     return 0;
 }
 

@@ -30,6 +30,9 @@ int orc_write_body(orc_world* w, uint32_t id, const bl_node* nodes, const bl_bea
 int orc_stats(orc_world* w, bl_stats* out);
 int orc_read_slides(orc_world* w, uint32_t id, bl_slide* out);
 int orc_read_joints(orc_world* w, uint32_t id, bl_frame* frames, bl_joint* joints);
+int orc_read_wheels(orc_world* w, uint32_t id, bl_wheel* wheels, float* speed_spin);
+/* a controller setting each wheel's torque and brake at every substep (null: none) */
+int orc_set_wheel_controls(orc_world* w, uint32_t id, const float* torque, const float* brake);
 /* the fast bodies' candidate pairs (World::fast_pairs), summed over the rebuilds */
 int64_t orc_fast_pairs(const orc_world* w);
 /* summed over every rebuild (every island): hashed nodes, triangles queried, NT before and after the near filter,

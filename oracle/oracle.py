@@ -60,6 +60,8 @@ def oracle_lib():
             "stats": (ctypes.c_int, [_P, _P]),
             "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
             "read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+            "read_wheels": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+            "set_wheel_controls": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "fast_pairs": (ctypes.c_int64, [_P]),
             "rebuild_counts": (None, [_P, _P]),
             "finalize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
@@ -112,6 +114,8 @@ def ref_lib():
             "write_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
             "read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+            "read_wheels": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+            "set_wheel_controls": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
             "finalize_stabilize": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32, ctypes.c_int,
                                                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_int)]),
@@ -176,6 +180,19 @@ class _CpuWorld:
         fr, jt = self.bodies[bid].frames.copy(), self.bodies[bid].joints.copy()
         self._fn("read_joints")(self._h, bid, _ptr(fr), _ptr(jt))
         return fr, jt
+
+    def read_wheels(self, bid):
+        """(the wheels, [wheel_speed, wheel_spin])"""
+        wh, ss = self.bodies[bid].wheels.copy(), np.zeros(2, np.float32)
+        self._fn("read_wheels")(self._h, bid, _ptr(wh), _ptr(ss))
+        return wh, ss
+
+    def set_wheel_controls(self, bid, torque=None, brake=None):
+        if torque is None or brake is None:
+            self._fn("set_wheel_controls")(self._h, bid, None, None)
+            return
+        t, b = np.ascontiguousarray(torque, np.float32), np.ascontiguousarray(brake, np.float32)
+        self._fn("set_wheel_controls")(self._h, bid, _ptr(t), _ptr(b))
 
     def broken_beams(self, bid):
         broken = ctypes.c_int64(0)

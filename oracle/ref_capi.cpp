@@ -48,6 +48,8 @@ namespace {
 struct RefWorld {
     World w;
     std::vector<SoftBody*> bodies;
+    // per body: a controller's torque and brake per wheel, set by its pre_substep hook when present
+    std::vector<std::unique_ptr<std::vector<float>>> ctl;
 };
 
 int g_threads_inited = 0;
@@ -95,6 +97,7 @@ void* blref_create(const bl_settings* s, int threads) {
     ws.frame_dissipation = s->frame_dissipation;
     ws.inter_body_collisions = s->inter_body_collisions != 0;
     ws.sleeping = s->sleeping != 0;
+    ws.tyre_grip = s->tyre_grip;
     ws.multithreaded = threads > 1;
     rw->w.statics.kill_y = s->kill_y;
     return rw;
@@ -215,9 +218,21 @@ int blref_add_body(void* h, const bl_body_desc* d) {
         j.radius = 0.05f;
         b->joints.push_back(j);
     }
+    for (uint32_t i = 0; i < d->n_wheels; i++) {
+        const bl_wheel& x = d->wheels[i];
+        Wheel wh;
+        wh.axle0 = x.axle0, wh.axle1 = x.axle1;
+        wh.nodes.assign(d->wheel_nodes + x.node0, d->wheel_nodes + x.node0 + x.n_nodes);
+        wh.radius = x.radius, wh.propulsed = x.propulsed, wh.arm = x.arm, wh.near_attach = x.near_attach;
+        wh.mass = x.mass, wh.torque = x.torque, wh.drive_torque = x.drive_torque, wh.brake = x.brake;
+        wh.speed = x.speed, wh.avg_speed = x.avg_speed, wh.last_torque = x.last_torque, wh.last_retorque = x.last_retorque;
+        wh.detached = x.detached != 0, wh.ring = x.ring != 0;
+        b->wheels.push_back(wh);
+    }
     apply_opts(*b, d->opts);
     SoftBody* p = rw->w.add_body(std::move(b));
     rw->bodies.push_back(p);
+    rw->ctl.push_back(std::make_unique<std::vector<float>>());
     return (int)rw->bodies.size() - 1;
 }
 
@@ -226,6 +241,41 @@ int blref_read_slides(void* h, uint32_t id, bl_slide* out) {
     if (id >= rw->bodies.size()) return -1;
     const SoftBody& b = *rw->bodies[id];
     for (size_t i = 0; i < b.slides.size(); i++) out[i].seg = b.slides[i].seg, out[i].broken = b.slides[i].broken ? 1u : 0u;
+    return 0;
+}
+
+int blref_read_wheels(void* h, uint32_t id, bl_wheel* wheels, float* speed_spin) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    const SoftBody& b = *rw->bodies[id];
+    for (size_t i = 0; wheels && i < b.wheels.size(); i++) {
+        const Wheel& w = b.wheels[i];
+        bl_wheel& x = wheels[i];
+        x.axle0 = w.axle0, x.axle1 = w.axle1, x.n_nodes = (uint32_t)w.nodes.size();
+        x.radius = w.radius, x.propulsed = w.propulsed, x.arm = w.arm, x.near_attach = w.near_attach, x.mass = w.mass;
+        x.torque = w.torque, x.drive_torque = w.drive_torque, x.brake = w.brake;
+        x.speed = w.speed, x.avg_speed = w.avg_speed, x.last_torque = w.last_torque, x.last_retorque = w.last_retorque;
+        x.detached = w.detached ? 1u : 0u, x.ring = w.ring ? 1u : 0u;
+    }
+    if (speed_spin) speed_spin[0] = b.wheel_speed, speed_spin[1] = b.wheel_spin;
+    return 0;
+}
+
+// A vehicle controller in the reference's own form: a pre_substep hook setting each wheel's torque and brake.
+int blref_set_wheel_controls(void* h, uint32_t id, const float* torque, const float* brake) {
+    auto* rw = static_cast<RefWorld*>(h);
+    if (id >= rw->bodies.size()) return -1;
+    SoftBody& b = *rw->bodies[id];
+    if (!torque || !brake) {
+        b.pre_substep = nullptr;
+        return 0;
+    }
+    std::vector<float>* c = rw->ctl[id].get();
+    c->assign(2 * b.wheels.size(), 0.0f);
+    for (size_t k = 0; k < b.wheels.size(); k++) (*c)[2 * k] = torque[k], (*c)[2 * k + 1] = brake[k];
+    b.pre_substep = [c](SoftBody& body, float) {
+        for (size_t k = 0; k < body.wheels.size(); k++) body.wheels[k].torque = (*c)[2 * k], body.wheels[k].brake = (*c)[2 * k + 1];
+    };
     return 0;
 }
 

@@ -241,11 +241,68 @@ def case_joints():
     return abi.default_settings(), _boxes(GROUND), [b1, b2, b3], 1500
 
 
+def car_body(seed, origin=(0.0, 0.0, 0.0), nseg=8, radius=0.3):
+    """A beam box on four wheels: each wheel two hub (axle) nodes tied to the box and a tyre of 2 x nseg tyre nodes
+    (NF_TYRE) round them, spoked to both hubs; the rear wheels propelled forward, one front wheel backward, the arm
+    node of each wheel a box corner (the drive's reaction torque). The tyres start a centimetre above the ground."""
+    from beamlab_amd.world import SoftBody
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    b = SoftBody("car")
+    o = np.asarray(origin, f) + f((0.0, radius + 0.01, 0.0))
+    c = [b.add_node(o + f((x, y, z)), 40.0) for x in (-1.0, 1.0) for y in (0.1, 0.5) for z in (-0.6, 0.6)]
+    for i in range(8):
+        for j in range(i + 1, 8):
+            b.add_beam(c[i], c[j], 2e6, 800, 1e9, 1e9)
+    tyre = abi.NF_TYRE | abi.NF_GROUND | abi.NF_CONTACTER
+    wheels = []
+    for q, (x, z) in enumerate([(-0.8, -0.75), (-0.8, 0.75), (0.8, -0.75), (0.8, 0.75)]):
+        s = 1.0 if z > 0 else -1.0
+        a0 = b.add_node(o + f((x, 0.0, z - 0.08)), 10.0)
+        a1 = b.add_node(o + f((x, 0.0, z + 0.08)), 10.0)
+        b.add_beam(a0, a1, 2e6, 500, 1e9, 1e9)
+        near = [k for k in range(8) if (k // 4 == (1 if x > 0 else 0)) and ((k % 2 == 1) == (z > 0))]
+        for k in near:
+            b.add_beam(k, a0, 6e5, 400, 1e9, 1e9), b.add_beam(k, a1, 6e5, 400, 1e9, 1e9)
+        ring = []
+        for i in range(nseg):
+            th = 2 * np.pi * (i + rng.uniform(-0.05, 0.05)) / nseg
+            for zz in (z - 0.1, z + 0.1):
+                ring.append(b.add_node(o + f((x + radius * np.cos(th), radius * np.sin(th), zz)), 2.0, tyre))
+        for i in range(nseg):
+            n0, n1 = ring[2 * i], ring[2 * i + 1]
+            m0, m1 = ring[(2 * i + 2) % (2 * nseg)], ring[(2 * i + 3) % (2 * nseg)]
+            for n in (n0, n1):
+                b.add_beam(n, a0, 3e5, 150, 1e9, 1e9), b.add_beam(n, a1, 3e5, 150, 1e9, 1e9)
+            b.add_beam(n0, n1, 3e5, 150, 1e9, 1e9), b.add_beam(n0, m0, 3e5, 150, 1e9, 1e9)
+            b.add_beam(n1, m1, 3e5, 150, 1e9, 1e9), b.add_beam(n0, m1, 2e5, 100, 1e9, 1e9)
+        arm = near[1]
+        wheels.append((a0, a1, ring, 1 if x < 0 else (2 if q == 2 else 0), arm, a1 if z > 0 else a0))
+    b.finalize()
+    for a0, a1, ring, prop, arm, na in wheels:
+        b.add_wheel(a0, a1, ring, radius=radius, propulsed=prop, arm=arm, near_attach=na)
+    b.stabilize(0.0005)
+    return b
+
+
+def case_wheels():
+    """Two cars on the ground: one rolling on with its drive torque spent in the first substep and no brakes, one
+    braking its front wheels from 3 m/s with a wheel off its hub (detached) and a lone tyre node under the box."""
+    b1 = car_body(81)
+    b1.wheels["torque"][:2] = 150.0
+    b2 = car_body(82, origin=(0.0, 0.0, 4.0))
+    b2.nodes["v"][:, 0] = 3.0
+    b2.wheels["brake"][2:] = 900.0
+    b2.wheels["detached"][1] = 1
+    b2.info["flags"][0] |= abi.NF_TYRE
+    return abi.default_settings(), _boxes(GROUND), [b1, b2], 1500
+
+
 CPU_CASES = {
     "c1": case_c1, "plastic": case_plastic, "ductile_nodeform": case_ductile_nodeform,
     "rope_support": case_rope_support, "fixed_kill": case_fixed_kill, "surfaces": case_surfaces,
     "oriented_boxes": case_oriented_boxes, "runaway": case_runaway, "multi": case_multi,
-    "shocks_slides": case_shocks_slides, "joints": case_joints,
+    "shocks_slides": case_shocks_slides, "joints": case_joints, "wheels": case_wheels,
 }
 
 
