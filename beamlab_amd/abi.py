@@ -43,6 +43,8 @@ WHEEL = np.dtype([("axle0", "<u4"), ("axle1", "<u4"), ("node0", "<u4"), ("n_node
                  [(n, "<f4") for n in ("mass", "torque", "drive_torque", "brake", "speed", "avg_speed", "last_torque",
                                        "last_retorque")] + [("detached", "<u4"), ("ring", "<u4")])
 assert WHEEL.itemsize == 72
+CYLINDER = np.dtype([("base", "<f4", 3), ("radius", "<f4"), ("height", "<f4"), ("surface", "<u4")])
+assert CYLINDER.itemsize == 24
 assert NODE.itemsize == 32 and NODE_INFO.itemsize == 20 and BEAM.itemsize == 52
 assert SHOCK.itemsize == 92 and SLIDE.itemsize == 32 and FRAME.itemsize == 44 and JOINT.itemsize == 92
 assert TRIANGLE.itemsize == 20 and CAPSULE.itemsize == 16 and BOX.itemsize == 92
@@ -183,6 +185,8 @@ SIGNATURES = {
     "bl_read_slides": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "bl_read_joints": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_read_wheels": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
+    "bl_set_cylinders": (ctypes.c_int, [_P, _P, ctypes.c_int32]),
+    "bl_set_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, _P, _P, _P]),
     "bl_set_wheel_controls": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P]),
     "bl_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "bl_reset_stats": (ctypes.c_int, [_P]),

@@ -135,6 +135,16 @@ struct bl_ctx {
     DevArr<float> stress;
     DevArr<BodyParam> bparam;
     DevArr<bl_box> dboxes;
+    // the static world's cylinders and terrain (host copies; the terrain's 16 x 16 vertex blocks' tops), and each
+    // body's terrain top over its reach for the frame (World::build_islands)
+    std::vector<bl_cylinder> cyls;
+    int tnx = 0, tnz = 0, tbx = 0, tbz = 0;
+    float tcell = 1, tox = 0, toz = 0;
+    std::vector<float> th, tblock, htmax;
+    std::vector<uint8_t> ts;
+    DevArr<bl_cylinder> dcyl;
+    DevArr<float> dth, dtmax;
+    DevArr<uint8_t> dts;
     DevArr<unsigned long long> counters, broken;
     DevArr<unsigned int> bounds, body_frame, bar;
     DevArr<int> err;
@@ -204,6 +214,7 @@ struct bl_ctx {
         for (int k = 0; k < 2; k++) pm[k].release(), vm[k].release();
         iA.release(), iM.release(), bcold.release(), ninfo.release(), noff.release(), ch_n0.release(), ch_rng.release(), fch_rng.release(), fwg.release(), wg_rng.release(), fwg_rng.release();
         body_c0.release(), naux.release(), iOth.release(), stress.release(), bparam.release(), dboxes.release();
+        dcyl.release(), dth.release(), dtmax.release(), dts.release();
         counters.release(), broken.release(), bounds.release(), body_frame.release(), bar.release(), err.release();
         chunk_body.release(), cstat.release();
         halo_off.release(), halo.release(), nbr_off.release(), nbr.release(), iloc.release(), flags.release();
@@ -612,7 +623,9 @@ int build(bl_ctx* c) {
         (rc = c->ninfo.upload(ninfo)) || (rc = c->naux.upload(naux)) || (rc = c->noff.upload(noff)) ||
         (rc = c->iA.upload(iA)) || (rc = c->iM.upload(iM)) || (rc = c->iOth.upload(iOth)) || (rc = c->bcold.upload(bcold)) ||
         (rc = c->stress.upload(stress)) || (rc = c->ch_n0.upload(ch_n0)) || (rc = c->ch_rng.upload(ch_rng)) || (rc = c->fch_rng.upload(fch_rng)) || (rc = c->fwg.upload(fwg)) || (rc = c->wg_rng.upload(wg_rng)) || (rc = c->fwg_rng.upload(fwg_rng)) || (rc = c->body_c0.upload(body_c0)) ||
-        (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->counters.alloc(CNT_N)) ||
+        (rc = c->bparam.upload(bp)) || (rc = c->dboxes.upload(c->boxes)) || (rc = c->dcyl.upload(c->cyls)) ||
+        (c->tnx && ((rc = c->dth.upload(c->th)) || (rc = c->dts.upload(c->ts)) || (rc = c->dtmax.alloc(c->bodies.size())))) ||
+        (rc = c->counters.alloc(CNT_N)) ||
         (rc = c->broken.alloc(std::max<size_t>(1, c->bodies.size()))) || (rc = c->bounds.alloc(8)) ||
         (rc = c->body_frame.alloc(std::max<size_t>(1, c->bodies.size()) * BF_N)) || (rc = c->bar.alloc(1)) ||
         (rc = c->err.alloc(1)) || (rc = c->chunk_body.upload(chunk_body)) ||
@@ -647,6 +660,11 @@ int build(bl_ctx* c) {
     a.cap_data = c->cap_data.p, a.ntris = NT, a.ncaps = NC;
     a.g = mk(c->s.gravity[0], c->s.gravity[1], c->s.gravity[2]);
     a.dt = c->s.dt, a.kill_y = c->s.kill_y, a.nbox = (int)c->boxes.size();
+    a.cyls = c->dcyl.p, a.ncyl = (int)c->cyls.size();
+    if (c->tnx) {
+        a.th = c->dth.p, a.ts = c->dts.p, a.tmax = c->dtmax.p;
+        a.tnx = c->tnx, a.tnz = c->tnz, a.tcell = c->tcell, a.tox = c->tox, a.toz = c->toz;
+    }
     a.nnodes = N, a.nchunks = nchunks, a.nbodies = (uint32_t)c->bodies.size();
     a.wg_chunk = c->wg_chunk.p;
     a.fch_rng = c->fch_rng.p, a.fwg = c->fwg.p, a.nfchunks = nf;
@@ -1142,9 +1160,33 @@ int enqueue_split(bl_ctx* c, int substeps) {
 }
 
 // Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.
+// Heightfield::max_height (src/phys/static_world.cpp:220-231) over the terrain's vertex blocks
+float terrain_max_height(const bl_ctx* c, float x0, float z0, float x1, float z1) {
+    int bx0 = (int)std::floor((x0 - c->tox) / c->tcell) / 16, bx1 = (int)std::floor((x1 - c->tox) / c->tcell) / 16;
+    int bz0 = (int)std::floor((z0 - c->toz) / c->tcell) / 16, bz1 = (int)std::floor((z1 - c->toz) / c->tcell) / 16;
+    bx0 = std::max(0, bx0), bz0 = std::max(0, bz0);
+    bx1 = std::min(c->tbx - 1, bx1), bz1 = std::min(c->tbz - 1, bz1);
+    float m = -1e30f;
+    for (int bz = bz0; bz <= bz1; bz++)
+        for (int bx = bx0; bx <= bx1; bx++) m = smax(m, c->tblock[(size_t)bz * c->tbx + bx]);
+    return m;
+}
+
 int enqueue_frame(bl_ctx* c, int substeps) {
     if (substeps <= 0) return BL_OK;
     int rc;
+    if (c->tnx) {
+        // each body's terrain top over its reach this frame: its bounds widened by twice what its top speed covers
+        // in the frame and half a metre (World::build_islands, src/phys/world.cpp:743-756)
+        const float ft = (float)substeps * c->s.dt;
+        c->htmax.resize(c->bodies.size());
+        for (size_t k = 0; k < c->bodies.size(); k++) {
+            const HostBody& hb = c->bodies[k];
+            const float m = smin(hb.max_speed, 200.0f) * ft * 2.0f + 0.5f;
+            c->htmax[k] = terrain_max_height(c, hb.mn.x - m, hb.mn.z - m, hb.mx.x + m, hb.mx.z + m);
+        }
+        HIPCHK(hipMemcpyAsync(c->dtmax.p, c->htmax.data(), sizeof(float) * c->htmax.size(), hipMemcpyHostToDevice, c->stream));
+    }
     if (needs_contacts(c, substeps)) return contact_frame(c, substeps);
     if ((rc = launch_frame_begin(c->args, c->stream, c->parity))) return rc;
     if ((rc = c->xargs.nbody ? enqueue_split(c, substeps) : enqueue(c, substeps))) return rc;
@@ -1400,6 +1442,56 @@ int bl_set_statics(bl_ctx* c, const bl_box* boxes, int32_t n) {
     c->boxes.assign(boxes, boxes + n);
     for (const bl_box& b : c->boxes)
         if (b.surface >= BL_SURF_COUNT) return BL_ERR_ARG;
+    c->release();
+    return BL_OK;
+}
+
+int bl_set_cylinders(bl_ctx* c, const bl_cylinder* cyl, int32_t n) {
+    if (!c || n < 0 || (n > 0 && !cyl)) return BL_ERR_ARG;
+    for (int32_t k = 0; k < n; k++)
+        if (cyl[k].surface >= BL_SURF_COUNT) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    c->cyls.assign(cyl, cyl + n);
+    c->release();
+    return BL_OK;
+}
+
+int bl_set_terrain(bl_ctx* c, int32_t nx, int32_t nz, float cell, const float origin[2], const float* heights,
+                   const uint8_t* surfaces) {
+    if (!c || nx < 0 || nz < 0) return BL_ERR_ARG;
+    if (nx > 0 && (nx < 2 || nz < 2 || !origin || !heights || !(cell > 0) || (size_t)nx * nz > (1u << 28))) return BL_ERR_ARG;
+    const size_t n = (size_t)nx * nz;
+    for (size_t q = 0; surfaces && q < n; q++)
+        if (surfaces[q] >= BL_SURF_COUNT) return BL_ERR_ARG;
+    int rc = post_frame(c);
+    if (rc) return rc;
+    rc = pull_state(c);
+    if (rc) return rc;
+    c->tnx = nx > 0 ? nx : 0, c->tnz = nx > 0 ? nz : 0;
+    c->th.clear(), c->ts.clear(), c->tblock.clear();
+    if (c->tnx) {
+        c->tcell = cell, c->tox = origin[0], c->toz = origin[1];
+        c->th.assign(heights, heights + n);
+        c->ts.assign(n, 0);
+        if (surfaces) c->ts.assign(surfaces, surfaces + n);
+        // Heightfield::update_bounds (src/phys/static_world.cpp:160-181): a vertex counts for up to 4 blocks
+        c->tbx = (nx - 1 + 15) / 16, c->tbz = (nz - 1 + 15) / 16;
+        c->tblock.assign((size_t)c->tbx * c->tbz, -1e30f);
+        for (int z = 0; z < nz; z++)
+            for (int x = 0; x < nx; x++) {
+                const float v = c->th[(size_t)z * nx + x];
+                const int bx0 = std::max(0, (x - 1) / 16), bx1 = std::min(c->tbx - 1, x / 16);
+                const int bz0 = std::max(0, (z - 1) / 16), bz1 = std::min(c->tbz - 1, z / 16);
+                for (int bz = bz0; bz <= bz1; bz++)
+                    for (int bx = bx0; bx <= bx1; bx++) {
+                        float& m = c->tblock[(size_t)bz * c->tbx + bx];
+                        m = std::max(m, v);
+                    }
+            }
+    }
     c->release();
     return BL_OK;
 }

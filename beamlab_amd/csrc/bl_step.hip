@@ -111,12 +111,53 @@ __device__ __forceinline__ bool beam_eval(float4& A, float4& M, const float4 Pa,
 }
 
 // ---------------------------------------------------------------------------------------------- node phase
-// StaticWorld::collide_point, the box part (src/phys/static_world.cpp:364-374,407-454), boxes in index order.
-__device__ __forceinline__ bool collide_point(const KArgs& a, f3 p, float r, f3& normal, float& depth_out,
-                                              uint32_t& surface, float& max_force) {
+// Heightfield::sample (src/phys/static_world.cpp:189-205) on the dense heights
+__device__ __forceinline__ bool terrain_sample(const KArgs& a, float x, float z, float& out_h, f3& n) {
+    const float lx = (x - a.tox) / a.tcell, lz = (z - a.toz) / a.tcell;
+    if (!(lx >= 0 && lz >= 0 && lx < a.tnx - 1 && lz < a.tnz - 1)) return false;
+    const int ix = (int)lx, iz = (int)lz;
+    const float fx = lx - ix, fz = lz - iz;
+    const size_t r0 = (size_t)iz * a.tnx, r1 = r0 + a.tnx;
+    const float h00 = a.th[r0 + ix], h10 = a.th[r0 + ix + 1], h01 = a.th[r1 + ix], h11 = a.th[r1 + ix + 1];
+    if (smin(smin(h00, h10), smin(h01, h11)) < -1e29f) return false;
+    if (fx > fz) {
+        out_h = h00 + (h10 - h00) * fx + (h11 - h10) * fz;
+        n = normalize(mk(-(h10 - h00) / a.tcell, 1.0f, -(h11 - h10) / a.tcell));
+    } else {
+        out_h = h00 + (h11 - h01) * fx + (h01 - h00) * fz;
+        n = normalize(mk(-(h11 - h01) / a.tcell, 1.0f, -(h01 - h00) / a.tcell));
+    }
+    return true;
+}
+
+// Heightfield::surface_at (src/phys/static_world.cpp:213-218): the nearest vertex's (lround: halves away from zero)
+__device__ __forceinline__ uint32_t terrain_surface(const KArgs& a, float x, float z) {
+    int ix = (int)roundf((x - a.tox) / a.tcell), iz = (int)roundf((z - a.toz) / a.tcell);
+    ix = ix < 0 ? 0 : (ix > a.tnx - 1 ? a.tnx - 1 : ix);
+    iz = iz < 0 ? 0 : (iz > a.tnz - 1 ? a.tnz - 1 : iz);
+    return a.ts[(size_t)iz * a.tnx + ix];
+}
+
+// StaticWorld::collide_point (src/phys/static_world.cpp:364-474): the terrain when near it (no road or paving), the
+// boxes in index order, the vertical cylinders (the deepest contact; a cylinder leaves max_force as it was).
+__device__ __forceinline__ bool collide_point(const KArgs& a, f3 p, float r, bool near_terrain, f3& normal,
+                                              float& depth_out, uint32_t& surface, float& max_force) {
     bool hit = false;
     float out_depth = 0;
     max_force = 0;
+    if (near_terrain) {
+        float th;
+        f3 n;
+        if (terrain_sample(a, p.x, p.z, th, n)) {
+            const float d = (th - p.y) * n.y + r;
+            if (d > 0) {
+                out_depth = d;
+                normal = n;
+                surface = terrain_surface(a, p.x, p.z);
+                hit = true;
+            }
+        }
+    }
     for (int k = 0; k < a.nbox; k++) {
         const bl_box& b = a.boxes[k];
         if (p.x < b.aabb_mn[0] - r || p.x > b.aabb_mx[0] + r || p.y < b.aabb_mn[1] - r || p.y > b.aabb_mx[1] + r ||
@@ -152,8 +193,33 @@ __device__ __forceinline__ bool collide_point(const KArgs& a, f3 p, float r, f3&
             hit = true;
         }
     }
+    for (int k = 0; k < a.ncyl; k++) {
+        const bl_cylinder& c = a.cyls[k];
+        const f3 rel = sub(p, mk(c.base[0], c.base[1], c.base[2]));
+        if (rel.y < -r || rel.y > c.height + r) continue;
+        if (ffabs(rel.x) >= c.radius + r || ffabs(rel.z) >= c.radius + r) continue;
+        const float dxz = fsqrt(rel.x * rel.x + rel.z * rel.z);
+        const float side = c.radius + r - dxz;
+        const float top = c.height + r - rel.y;
+        if (side <= 0) continue;
+        float depth;
+        f3 n;
+        if (top < side && rel.y > c.height * 0.5f) depth = top, n = mk(0, 1, 0);
+        else depth = side, n = dxz > 1e-6f ? mk(rel.x / dxz, 0, rel.z / dxz) : mk(1, 0, 0);
+        if (depth > out_depth) {
+            out_depth = depth;
+            normal = n;
+            surface = c.surface;
+            hit = true;
+        }
+    }
     depth_out = out_depth;
     return hit;
+}
+
+// World::collide_static's terrain test (src/phys/world.cpp:812): a node within 5 cm of its body's terrain top
+__device__ __forceinline__ bool near_terrain(const KArgs& a, float py, float r, uint32_t body) {
+    return a.tmax && py - r <= a.tmax[body] + 0.05f;
 }
 
 struct NodeCounts {
@@ -172,11 +238,12 @@ __device__ __forceinline__ void node_finish(const KArgs& a, uint32_t info, float
         if (p.y < a.kill_y) {
             v = mk(0, 0, 0);
             inv_mass = 0;
-        } else if (a.nbox > 0) {
+        } else {
+            const bool nt = near_terrain(a, p.y, aux.x, info >> 16);
             f3 n;
             float depth, max_force;
             uint32_t surf;
-            if (collide_point(a, p, aux.x, n, depth, surf, max_force)) {
+            if ((nt || a.nbox > 0 || a.ncyl > 0) && collide_point(a, p, aux.x, nt, n, depth, surf, max_force)) {
                 cnt.contacts++;
                 const ground_model& gm = k_gm[surf < BL_SURF_N ? surf : 0];
                 const BodyParam& bp = a.bodies[info >> 16];
@@ -1385,12 +1452,14 @@ __global__ void k_tyres(KArgs a, XArgs x, int parity) {
         x.tslide[q] = make_float4(0, 0, 0, 0);
         const uint32_t info = a.ninfo[e.x];
         const float4 P = a.pm[parity][e.x], V = a.vm[parity][e.x];
-        if (!(info & BL_NF_GROUND) || !(P.w > 0) || P.y < a.kill_y || a.nbox <= 0) continue;
+        if (!(info & BL_NF_GROUND) || !(P.w > 0) || P.y < a.kill_y) continue;
         const float2 aux = a.naux[e.x];
+        const bool nt = near_terrain(a, P.y, aux.x, info >> 16);
+        if (!nt && a.nbox <= 0 && a.ncyl <= 0) continue;
         f3 n;
         float depth, max_force;
         uint32_t surf;
-        if (!collide_point(a, mk(P.x, P.y, P.z), aux.x, n, depth, surf, max_force) || max_force > 0) continue;
+        if (!collide_point(a, mk(P.x, P.y, P.z), aux.x, nt, n, depth, surf, max_force) || max_force > 0) continue;
         const ground_model& gm = k_gm[surf < BL_SURF_N ? surf : 0];
         const float4 F = a.F[e.x];
         const tyre_hit t = tyre_contact(mk(F.x, F.y, F.z), mk(V.x, V.y, V.z), V.w, n, a.dt, gm, depth, aux.y * x.tyre_grip);

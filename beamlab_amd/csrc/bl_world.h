@@ -96,6 +96,15 @@ struct KArgs {
     // the ranges in workgroup order: wg_rng[b] = ch_rng[wg_chunk[b]], fwg_rng[b] = fch_rng[fwg[b]]
     const uint4* wg_rng;
     const uint4* fwg_rng;
+    // the static world beyond the boxes: vertical cylinders; the terrain (dense heights and surfaces, z-major) and
+    // each body's terrain top over its reach this frame (null without a terrain)
+    const bl_cylinder* cyls;
+    int ncyl;
+    const float* th;
+    const uint8_t* ts;
+    int tnx, tnz;
+    float tcell, tox, toz;
+    const float* tmax;
 };
 
 // The bodies' shocks and slide nodes (the forces that follow the beams in World::simulate_island's base_forces,

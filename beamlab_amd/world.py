@@ -250,6 +250,23 @@ class World:
         self.bodies.append(body)
         return bid.value
 
+    def set_cylinders(self, cyl):
+        """The static world's vertical cylinders (abi.CYLINDER records: StaticWorld::cylinders)."""
+        self._cyl = np.ascontiguousarray(cyl)
+        abi.check(self._lib().bl_set_cylinders(self._context(), abi.ptr(self._cyl), len(self._cyl)), "bl_set_cylinders")
+
+    def set_terrain(self, heights, cell, origin, surfaces=None):
+        """The static world's heightfield (StaticWorld::terrain): heights[z, x] (-1e30: no ground), `cell` apart from
+        origin (x, z), surfaces[z, x] (None: grass); None removes it."""
+        if heights is None:
+            abi.check(self._lib().bl_set_terrain(self._context(), 0, 0, 0.0, None, None, None), "bl_set_terrain")
+            return
+        h = np.ascontiguousarray(heights, np.float32)
+        o = np.asarray(origin, np.float32)
+        sf = None if surfaces is None else np.ascontiguousarray(surfaces, np.uint8)
+        abi.check(self._lib().bl_set_terrain(self._context(), h.shape[1], h.shape[0], float(cell), abi.ptr(o), abi.ptr(h),
+                                             abi.ptr(sf)), "bl_set_terrain")
+
     def _sync_statics(self):
         boxes = self.statics.boxes
         key = boxes.tobytes()

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BL_ABI_VERSION 5
+#define BL_ABI_VERSION 6
 
 enum {
     BL_OK = 0,
@@ -219,6 +219,13 @@ typedef struct bl_settings {
     float tyre_grip;           /* WorldSettings::tyre_grip: the tyre nodes' friction multiplier, default 1 */
 } bl_settings;
 
+/* A vertical cylinder of the static world (== bl::phys::StaticCylinder, src/phys/static_world.h:128-132). */
+typedef struct bl_cylinder {
+    float base[3];
+    float radius, height;
+    uint32_t surface;
+} bl_cylinder;
+
 /* The per-body settings the path reads (SoftBody, src/phys/softbody.h:649-716). */
 typedef struct bl_body_opts {
     float collision_radius;    /* default 0.05 */
@@ -345,6 +352,16 @@ void bl_destroy(bl_ctx* ctx);
 
 /* The static world (replaces the previous boxes). World::statics (src/phys/world.h:158). */
 int bl_set_statics(bl_ctx* ctx, const bl_box* boxes, int32_t n_boxes);
+
+/* The static world's vertical cylinders (replaces the previous ones): StaticWorld::cylinders. */
+int bl_set_cylinders(bl_ctx* ctx, const bl_cylinder* cylinders, int32_t n_cylinders);
+
+/* The static world's terrain (StaticWorld::terrain, a Heightfield: src/phys/static_world.h:62-112,
+ * src/phys/static_world.cpp:117-230), replacing the previous one: nx x nz vertices `cell` apart from `origin` (x, z),
+ * heights[z * nx + x] (-1e30: no ground there), surfaces[z * nx + x] (null: all 0, grass). nx = 0 removes it. The
+ * terrain's road and paved layers are not stepped on the GPU. */
+int bl_set_terrain(bl_ctx* ctx, int32_t nx, int32_t nz, float cell, const float origin[2], const float* heights,
+                   const uint8_t* surfaces);
 
 /* Adds a body (World::add_body, src/phys/world.cpp:283-289). Bodies are stepped together. *id = its index. */
 int bl_add_body(bl_ctx* ctx, const bl_body_desc* desc, uint32_t* id);

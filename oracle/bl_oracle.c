@@ -132,12 +132,21 @@ typedef struct {
     float* ctl;          /* a controller's torque and brake per wheel (pre_substep), when `controlled` */
     int controlled;
     float wheel_speed, wheel_spin;
+    float terrain_max; /* the terrain's top over the body's reach this frame (World::build_islands) */
 } orc_body;
 
 struct orc_world {
     bl_settings s;
     bl_box* boxes;
     int nbox;
+    bl_cylinder* cyl;
+    int ncyl;
+    /* the terrain (Heightfield) dense: heights and surfaces z-major, and each 16 x 16 block's top (update_bounds) */
+    int tnx, tnz, tbx, tbz;
+    float tcell, tox, toz;
+    float* th;
+    uint8_t* ts;
+    float* tblock;
     orc_body* bodies;
     int nbodies;
     bl_stats st;
@@ -174,6 +183,7 @@ void orc_destroy(orc_world* w) {
     }
     free(w->bodies);
     free(w->boxes);
+    free(w->cyl), free(w->th), free(w->ts), free(w->tblock);
     free(w);
 }
 
@@ -183,6 +193,86 @@ int orc_set_statics(orc_world* w, const bl_box* boxes, int n) {
     if (n > 0) memcpy(w->boxes, boxes, sizeof(bl_box) * n);
     w->nbox = n;
     return 0;
+}
+
+int orc_set_cylinders(orc_world* w, const bl_cylinder* c, int n) {
+    free(w->cyl);
+    w->cyl = (bl_cylinder*)malloc(sizeof(bl_cylinder) * (n > 0 ? n : 1));
+    if (n > 0) memcpy(w->cyl, c, sizeof(bl_cylinder) * n);
+    w->ncyl = n > 0 ? n : 0;
+    return 0;
+}
+
+/* Heightfield::create + the heights and surfaces + update_bounds (src/phys/static_world.cpp:117-180) */
+int orc_set_terrain(orc_world* w, int nx, int nz, float cell, const float* origin, const float* h, const uint8_t* surf) {
+    free(w->th), free(w->ts), free(w->tblock);
+    w->th = NULL, w->ts = NULL, w->tblock = NULL;
+    w->tnx = w->tnz = 0;
+    if (nx < 2 || nz < 2) return 0;
+    const size_t n = (size_t)nx * nz;
+    w->tnx = nx, w->tnz = nz, w->tcell = cell, w->tox = origin[0], w->toz = origin[1];
+    w->th = (float*)malloc(sizeof(float) * n);
+    w->ts = (uint8_t*)calloc(n, 1);
+    memcpy(w->th, h, sizeof(float) * n);
+    if (surf) memcpy(w->ts, surf, n);
+    const int K = 16;
+    w->tbx = (nx - 1 + K - 1) / K, w->tbz = (nz - 1 + K - 1) / K;
+    w->tblock = (float*)malloc(sizeof(float) * w->tbx * w->tbz);
+    for (int q = 0; q < w->tbx * w->tbz; q++) w->tblock[q] = -1e30f;
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+            const float v = w->th[(size_t)z * nx + x];
+            int bx0 = x - 1 < 0 ? 0 : (x - 1) / K, bx1 = x / K < w->tbx - 1 ? x / K : w->tbx - 1;
+            int bz0 = z - 1 < 0 ? 0 : (z - 1) / K, bz1 = z / K < w->tbz - 1 ? z / K : w->tbz - 1;
+            for (int bz = bz0; bz <= bz1; bz++)
+                for (int bx = bx0; bx <= bx1; bx++) {
+                    float* m = &w->tblock[(size_t)bz * w->tbx + bx];
+                    *m = smax(*m, v);
+                }
+        }
+    return 0;
+}
+
+/* Heightfield::max_height (src/phys/static_world.cpp:220-231) */
+static float terrain_max_height(const orc_world* w, float x0, float z0, float x1, float z1) {
+    int bx0 = (int)floorf((x0 - w->tox) / w->tcell) / 16, bx1 = (int)floorf((x1 - w->tox) / w->tcell) / 16;
+    int bz0 = (int)floorf((z0 - w->toz) / w->tcell) / 16, bz1 = (int)floorf((z1 - w->toz) / w->tcell) / 16;
+    bx0 = bx0 < 0 ? 0 : bx0, bz0 = bz0 < 0 ? 0 : bz0;
+    bx1 = bx1 < w->tbx - 1 ? bx1 : w->tbx - 1, bz1 = bz1 < w->tbz - 1 ? bz1 : w->tbz - 1;
+    float m = -1e30f;
+    for (int bz = bz0; bz <= bz1; bz++)
+        for (int bx = bx0; bx <= bx1; bx++) m = smax(m, w->tblock[(size_t)bz * w->tbx + bx]);
+    return m;
+}
+
+static inline v3 vnormalize_(v3 a);
+
+/* Heightfield::sample (src/phys/static_world.cpp:189-205) */
+static int terrain_sample(const orc_world* w, float x, float z, float* out_h, v3* n) {
+    float lx = (x - w->tox) / w->tcell, lz = (z - w->toz) / w->tcell;
+    if (!(lx >= 0 && lz >= 0 && lx < w->tnx - 1 && lz < w->tnz - 1)) return 0;
+    int ix = (int)lx, iz = (int)lz;
+    float fx = lx - ix, fz = lz - iz;
+    const float* H = w->th;
+    const size_t r0 = (size_t)iz * w->tnx, r1 = r0 + w->tnx;
+    float h00 = H[r0 + ix], h10 = H[r0 + ix + 1], h01 = H[r1 + ix], h11 = H[r1 + ix + 1];
+    if (smin(smin(h00, h10), smin(h01, h11)) < -1e29f) return 0;
+    if (fx > fz) {
+        *out_h = h00 + (h10 - h00) * fx + (h11 - h10) * fz;
+        *n = vnormalize_(V(-(h10 - h00) / w->tcell, 1.0f, -(h11 - h10) / w->tcell));
+    } else {
+        *out_h = h00 + (h11 - h01) * fx + (h01 - h00) * fz;
+        *n = vnormalize_(V(-(h11 - h01) / w->tcell, 1.0f, -(h01 - h00) / w->tcell));
+    }
+    return 1;
+}
+
+/* Heightfield::surface_at (src/phys/static_world.cpp:213-218) */
+static int terrain_surface(const orc_world* w, float x, float z) {
+    int ix = (int)lroundf((x - w->tox) / w->tcell), iz = (int)lroundf((z - w->toz) / w->tcell);
+    ix = ix < 0 ? 0 : (ix > w->tnx - 1 ? w->tnx - 1 : ix);
+    iz = iz < 0 ? 0 : (iz > w->tnz - 1 ? w->tnz - 1 : iz);
+    return w->ts[(size_t)iz * w->tnx + ix];
 }
 
 int orc_add_body(orc_world* w, const bl_body_desc* d) {
@@ -572,12 +662,26 @@ static void integrate_frames(orc_body* B, float dt) {
     }
 }
 
-/* StaticWorld::collide_point, the box part (src/phys/static_world.cpp:364-374,407-454); boxes in index order */
-static int collide_point(const orc_world* w, v3 p, float r, v3* normal, float* depth_out, int* surface,
-                         float* max_force) {
+/* StaticWorld::collide_point (src/phys/static_world.cpp:364-474): the terrain (without road or paving) when near it,
+ * the boxes in index order, the cylinders */
+static int collide_point(const orc_world* w, v3 p, float r, int near_terrain, v3* normal, float* depth_out,
+                         int* surface, float* max_force) {
     int hit = 0;
     float out_depth = 0;
     *max_force = 0;
+    if (near_terrain && w->tnx) {
+        float th;
+        v3 n;
+        if (terrain_sample(w, p.x, p.z, &th, &n)) {
+            float d = (th - p.y) * n.y + r;
+            if (d > 0) {
+                out_depth = d;
+                *normal = n;
+                *surface = terrain_surface(w, p.x, p.z);
+                hit = 1;
+            }
+        }
+    }
     for (int k = 0; k < w->nbox; k++) {
         const bl_box* b = &w->boxes[k];
         if (p.x < b->aabb_mn[0] - r || p.x > b->aabb_mx[0] + r || p.y < b->aabb_mn[1] - r || p.y > b->aabb_mx[1] + r ||
@@ -608,6 +712,27 @@ static int collide_point(const orc_world* w, v3 p, float r, v3* normal, float* d
             *normal = mat_mul(b->rot, nl);
             *surface = (int)b->surface;
             *max_force = b->max_force;
+            hit = 1;
+        }
+    }
+    for (int k = 0; k < w->ncyl; k++) {
+        const bl_cylinder* c = &w->cyl[k];
+        v3 rel = vsub(p, ld3(c->base));
+        if (rel.y < -r || rel.y > c->height + r) continue;
+        if (fabsf(rel.x) >= c->radius + r || fabsf(rel.z) >= c->radius + r) continue;
+        float dxz = sqrtf(rel.x * rel.x + rel.z * rel.z);
+        float side = c->radius + r - dxz;
+        float top = c->height + r - rel.y;
+        if (side <= 0) continue;
+        float depth;
+        v3 n;
+        if (top < side && rel.y > c->height * 0.5f) depth = top, n = V(0, 1, 0);
+        else depth = side, n = dxz > 1e-6f ? V(rel.x / dxz, 0, rel.z / dxz) : V(1, 0, 0);
+        /* (the deepest so far: a cylinder leaves max_force as it was) */
+        if (depth > out_depth) {
+            out_depth = depth;
+            *normal = n;
+            *surface = (int)c->surface;
             hit = 1;
         }
     }
@@ -736,12 +861,13 @@ static int collide_static(orc_world* w, orc_body* B, float dt) {
             x->inv_mass = 0;
             continue;
         }
-        if (w->nbox == 0) continue;
         float r = in->radius;
+        const int near_terrain = w->tnx && x->p[1] - r <= B->terrain_max + 0.05f;
+        if (!near_terrain && w->nbox == 0 && w->ncyl == 0) continue;
         v3 n;
         float depth, max_force;
         int surf;
-        if (!collide_point(w, ld3(x->p), r, &n, &depth, &surf, &max_force)) continue;
+        if (!collide_point(w, ld3(x->p), r, near_terrain, &n, &depth, &surf, &max_force)) continue;
         contacts++;
         const ground_model* gm = &k_gm[surf < BL_SURF_COUNT ? surf : 0];
         if (max_force > 0) {
@@ -1881,6 +2007,12 @@ int orc_step(orc_world* w, int substeps) {
     const v3 g = ld3(w->s.gravity);
     if (substeps <= 0) return 0;
     orc_island* isl;
+    /* (World::build_islands, src/phys/world.cpp:743-756: each body's reach this frame over the terrain) */
+    for (int k = 0; k < w->nbodies && w->tnx; k++) {
+        orc_body* B = &w->bodies[k];
+        const float m = smin(B->max_speed, 200.0f) * ((float)substeps * dt) * 2.0f + 0.5f;
+        B->terrain_max = terrain_max_height(w, B->mn.x - m, B->mn.z - m, B->mx.x + m, B->mx.z + m);
+    }
     const int nisl = build_islands(w, (float)substeps * dt, &isl);
     for (int k = 0; k < nisl; k++) {
         orc_island* I = &isl[k];

@@ -23,6 +23,9 @@ typedef struct orc_world orc_world;
 orc_world* orc_create(const bl_settings* s);
 void orc_destroy(orc_world* w);
 int orc_set_statics(orc_world* w, const bl_box* boxes, int n);
+int orc_set_cylinders(orc_world* w, const bl_cylinder* c, int n);
+int orc_set_terrain(orc_world* w, int nx, int nz, float cell, const float* origin, const float* heights,
+                    const uint8_t* surfaces);
 int orc_add_body(orc_world* w, const bl_body_desc* d);
 int orc_step(orc_world* w, int substeps);
 int orc_read_body(orc_world* w, uint32_t id, bl_node* nodes, bl_beam* beams, int64_t* broken);

@@ -53,6 +53,8 @@ def oracle_lib():
             "create": (_P, [_P]),
             "destroy": (None, [_P]),
             "set_statics": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+            "set_cylinders": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+            "set_terrain": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P, _P, _P]),
             "add_body": (ctypes.c_int, [_P, _P]),
             "step": (ctypes.c_int, [_P, ctypes.c_int]),
             "read_body": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
@@ -106,6 +108,8 @@ def ref_lib():
             "threads": (ctypes.c_int, []),
             "shutdown": (None, []),
             "set_statics": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+            "set_cylinders": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+            "set_terrain": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P, _P, _P]),
             "box_from_quat": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
             "add_body": (ctypes.c_int, [_P, _P]),
             "step": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -152,6 +156,21 @@ class _CpuWorld:
 
     def _fn(self, name):
         return getattr(self._lib, self._pre + name)
+
+    def set_cylinders(self, cyl):
+        """The static world's vertical cylinders (abi.CYLINDER records)."""
+        self._cyl = np.ascontiguousarray(cyl)
+        self._fn("set_cylinders")(self._h, _ptr(self._cyl), len(self._cyl))
+
+    def set_terrain(self, heights, cell, origin, surfaces=None):
+        """A heightfield: heights[z, x] (-1e30: no ground), `cell` apart from origin (x, z); None removes it."""
+        if heights is None:
+            self._fn("set_terrain")(self._h, 0, 0, 0.0, None, None, None)
+            return
+        h = np.ascontiguousarray(heights, np.float32)
+        o = np.asarray(origin, np.float32)
+        s = None if surfaces is None else np.ascontiguousarray(surfaces, np.uint8)
+        self._fn("set_terrain")(self._h, h.shape[1], h.shape[0], float(cell), _ptr(o), _ptr(h), _ptr(s))
 
     def add_body(self, body):
         d = body.desc()

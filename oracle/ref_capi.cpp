@@ -127,6 +127,33 @@ int blref_set_statics(void* h, const bl_box* boxes, int n) {
     return 0;
 }
 
+int blref_set_cylinders(void* h, const bl_cylinder* c, int n) {
+    auto* rw = static_cast<RefWorld*>(h);
+    rw->w.statics.cylinders.clear();
+    for (int i = 0; i < n; i++) {
+        StaticCylinder cy;
+        cy.base = v3(c[i].base), cy.radius = c[i].radius, cy.height = c[i].height, cy.surface = (uint8_t)c[i].surface;
+        rw->w.statics.cylinders.push_back(cy);
+    }
+    return 0;
+}
+
+// Heightfield::create, then its heights and surfaces vertex by vertex, update_bounds.
+int blref_set_terrain(void* h, int nx, int nz, float cell, const float* origin, const float* heights, const uint8_t* surf) {
+    auto* rw = static_cast<RefWorld*>(h);
+    StaticWorld& sw = rw->w.statics;
+    sw.has_terrain = nx >= 2 && nz >= 2;
+    if (!sw.has_terrain) return 0;
+    sw.terrain.create(nx, nz, cell, vec2(origin[0], origin[1]));
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+            sw.terrain.h(x, z) = heights[(size_t)z * nx + x];
+            sw.terrain.surf(x, z) = surf ? surf[(size_t)z * nx + x] : (uint8_t)0;
+        }
+    sw.terrain.update_bounds();
+    return 0;
+}
+
 // The reference's own box (StaticWorld::add_box): to check bl_box_from_quat bit for bit.
 int blref_box_from_quat(const float* c, const float* half, const float* q, uint32_t surface, bl_box* out) {
     StaticWorld sw;

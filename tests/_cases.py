@@ -306,8 +306,77 @@ CPU_CASES = {
 }
 
 
-def run_world(world, bodies, substeps, checkpoints):
-    """Steps `world` and returns the states at each checkpoint (a substep count)."""
+def terrain_statics(seed=5):
+    """A heightfield of 49 x 49 vertices 0.5 m apart from (-12, -12): rolling hills, a ridge whose face rises 2 m per
+    metre (63 degrees), a flat strip (z > 7), a hole of no ground, surfaces in patches of every kind; and four vertical
+    cylinders (posts) standing out of it."""
+    rng = np.random.default_rng(seed)
+    n, cell, org = 49, 0.5, (-12.0, -12.0)
+    x = org[0] + cell * np.arange(n)
+    z = org[1] + cell * np.arange(n)
+    X, Z = np.meshgrid(x, z)
+    h = 0.4 * np.sin(0.35 * X) * np.cos(0.3 * Z) + 0.02 * rng.standard_normal(X.shape)
+    h += np.clip((X - 3.0) * 2.0, 0.0, 1.5) * (np.abs(Z - 2.0) < 3.0)
+    h[Z > 7.0] = 0.0
+    h[(np.abs(X + 1.0) < 1.1) & (np.abs(Z + 8.0) < 1.1)] = -1e30
+    surf = ((np.floor(X / 4.0) + np.floor(Z / 4.0)) % (abi.SURF_ICE + 1)).astype(np.uint8)
+    cyl = np.zeros(4, abi.CYLINDER)
+    for k, (b, r, hh, sf) in enumerate([((6.0, -1.0, -6.0), 0.6, 2.5, abi.SURF_METAL), ((0.0, -1.0, 6.0), 0.3, 2.0, abi.SURF_WOOD),
+                                         ((-6.0, -1.0, 0.0), 1.0, 1.4, abi.SURF_ROCK), ((8.0, -1.0, 8.0), 0.2, 3.0, abi.SURF_CONCRETE)]):
+        cyl[k]["base"], cyl[k]["radius"], cyl[k]["height"], cyl[k]["surface"] = b, r, hh, sf
+    return h.astype(np.float32), cell, org, surf, cyl
+
+
+def set_terrain_statics(world, statics):
+    h, cell, org, surf, cyl = statics
+    world.set_terrain(h, cell, org, surf)
+    world.set_cylinders(cyl)
+
+
+def case_terrain():
+    """Lattices on the terrain: over the hills, onto the ridge's face, onto a post's top, into a post's side, over
+    the hole (falling through to kill_y); no boxes."""
+    h, cell, org, _, _ = terrain_statics()
+
+    def top(x, z, w=1.0):  # the terrain's highest vertex within w of (x, z) (holes aside)
+        i0, i1 = int((x - w - org[0]) / cell), int((x + w - org[0]) / cell) + 1
+        j0, j1 = int((z - w - org[1]) / cell), int((z + w - org[1]) / cell) + 1
+        return float(h[max(j0, 0):j1 + 1, max(i0, 0):i1 + 1].max())
+
+    bodies = []
+    for k, (o, v) in enumerate([((-6.0, top(-5.6, -4.8) + 0.1, -5.0), (2.0, -1.0, 0.0)),
+                                ((3.3, top(3.7, 1.7) + 0.2, 1.5), (0.5, -2.0, 0.0)),
+                                ((5.6, 1.55, -6.4), (0.0, 0.0, 0.0)),
+                                ((-2.5, top(-2.1, 5.9) + 0.05, 5.7), (4.0, 0.0, 0.0)),
+                                ((-1.4, 0.5, -8.4), (0.0, 0.0, 0.0))]):
+        b = scenes.lattice_body(4, 3, 3, 0.25, 2.0, o, seed=60 + k, eps=0.1)
+        b.nodes["v"][:] = np.asarray(v, np.float32)
+        b.info["radius"] = np.float32(0.02 * (k % 3))
+        bodies.append(b)
+    s = abi.default_settings()
+    s.kill_y = np.float32(-3.0)
+    return s, np.zeros(0, abi.BOX), bodies, 1500
+
+
+def case_terrain_car():
+    """A car on the terrain's flat strip driving onto grass and dirt, braking at the end, beside a lattice on the
+    hills; a box too (terrain, boxes and cylinders together)."""
+    car = car_body(83, origin=(-9.0, 0.0, 9.0))
+    car.wheels["torque"][:2] = 300.0
+    b = scenes.lattice_body(3, 3, 3, 0.25, 2.0, (-4.0, 0.6, -3.0), seed=70, eps=0.1)
+    s = abi.default_settings()
+    s.tyre_grip = np.float32(0.9)
+    return s, _boxes(((2.0, 0.2, 10.0), (1.0, 0.4, 1.0), (0.0, 0.0, 0.0, 1.0), abi.SURF_ASPHALT)), [car, b], 1500
+
+
+TERRAIN_CASES = {"terrain": case_terrain, "terrain_car": case_terrain_car}
+
+
+def run_world(world, bodies, substeps, checkpoints, statics=None):
+    """Steps `world` and returns the states at each checkpoint (a substep count); `statics` (terrain_statics's) is
+    set on the world first."""
+    if statics is not None:
+        set_terrain_statics(world, statics)
     for b in bodies:
         world.add_body(b)
     states = []

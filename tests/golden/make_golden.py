@@ -6,7 +6,8 @@ tests/_cases.py. Run in the container that holds the reference sources:
 
 Outputs (data only: inputs are regenerated from the seeds, outputs are the reference's states):
     c1_ref.npz          config-1 lattice: node and beam records after 1, 7, 666 and 2000 substeps, broken counts
-    cases_ref.npz       every other case: sha256 of the final node and beam records, broken counts
+    cases_ref.npz       every other case (the terrain cases with tests/_cases.py's terrain and posts): sha256 of the
+                        final node and beam records, broken counts
     fem_ref.npz         the shell element: the reference's FemFrame::tri_stiffness (K, 18 x 18 FP64, and the rest
                         shape X) of 24 seeded triangles under three sections
     fem_step_ref.npz    the plate body's whole substep (compute_forces, solve, integrate_nodes, sync_positions) of the
@@ -35,7 +36,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import _fem_cases as fc  # noqa: E402
-from _cases import CPU_CASES, run_world  # noqa: E402
+from _cases import CPU_CASES, TERRAIN_CASES, run_world, terrain_statics  # noqa: E402
 from _contact_cases import CONTACT_CASES, golden_checkpoints  # noqa: E402
 from oracle import oracle  # noqa: E402
 
@@ -59,9 +60,10 @@ def cases():
     out["broken"] = np.array(out["broken"])
     np.savez_compressed(os.path.join(HERE, "c1_ref.npz"), **out)
     digests = {}
-    for name in sorted(CPU_CASES) + sorted(CONTACT_CASES):
-        settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
-        st = run_world(oracle.RefWorld(settings, boxes), bodies, n, golden_checkpoints(name, n))
+    for name in sorted(CPU_CASES) + sorted(CONTACT_CASES) + sorted(TERRAIN_CASES):
+        settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES.get(name) or TERRAIN_CASES[name])()
+        st = run_world(oracle.RefWorld(settings, boxes), bodies, n, golden_checkpoints(name, n),
+                       terrain_statics() if name in TERRAIN_CASES else None)
         h = hashlib.sha256()
         for nodes, beams in st[-1][:-1]:
             h.update(nodes.tobytes())

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol():
 
 def test_version_and_errors():
     lib = abi.load()
-    assert lib.bl_abi_version() == 5
+    assert lib.bl_abi_version() == 6
     assert lib.bl_error_string(abi.BL_ERR_ARG)
     s = abi.Settings()
     lib.bl_default_settings(ctypes.byref(s))

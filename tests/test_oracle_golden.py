@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from _cases import CPU_CASES, run_world
+from _cases import CPU_CASES, TERRAIN_CASES, run_world, terrain_statics
 from _contact_cases import CONTACT_CASES, golden_checkpoints
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -25,11 +25,12 @@ def test_c1_states(oracle_mod):
     assert int(g["broken"][-1]) > 0, "the fixture covers breaking"
 
 
-@pytest.mark.parametrize("name", sorted(CPU_CASES) + sorted(CONTACT_CASES))
+@pytest.mark.parametrize("name", sorted(CPU_CASES) + sorted(CONTACT_CASES) + sorted(TERRAIN_CASES))
 def test_case_digests(name, oracle_mod):
     g = np.load(os.path.join(GOLD, "cases_ref.npz"))
-    settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES[name])()
-    st = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, golden_checkpoints(name, n))
+    settings, boxes, bodies, n = (CPU_CASES.get(name) or CONTACT_CASES.get(name) or TERRAIN_CASES[name])()
+    st = run_world(oracle_mod.OracleWorld(settings, boxes), bodies, n, golden_checkpoints(name, n),
+                   terrain_statics() if name in TERRAIN_CASES else None)
     h = hashlib.sha256()
     for nodes, beams in st[-1][:-1]:
         h.update(nodes.tobytes())
