@@ -42,6 +42,8 @@ struct HostBody {
     std::vector<float4> ctl;      // a controller's torque, brake, on (pre_substep) per wheel
     float wsp[2] = {0, 0};        // SoftBody::wheel_speed, wheel_spin
     bool tyres = false;           // it has tyre nodes
+    double guard_ke = 0, guard_pe = 0; // the energy guard's motion energy at the frame's start
+    int guard_quiet = 0, guard_cuts = 0;
     bl_body_opts opts;
     uint32_t node0 = 0, beam0 = 0, tri0 = 0, cap0 = 0;
     uint32_t slide0 = 0; // its first slide in the device array
@@ -257,7 +259,7 @@ bool tuning_valid(const bl_tuning& t) {
 constexpr uint32_t kSpinLimit = 1u << 20; // grid-barrier polls (about a second) before a resident launch gives up
 
 bool unsupported_opts(const bl_body_opts& o) {
-    return o.air_drag != 0 || o.aero_cda != 0 || o.rest_damp != 0 || o.energy_guard != 0;
+    return o.air_drag != 0 || o.aero_cda != 0 || o.rest_damp != 0;
 }
 
 // Device state -> host copies of every body (nodes, beam state), so that a rebuild starts from the current state.
@@ -1160,6 +1162,8 @@ int enqueue_split(bl_ctx* c, int substeps) {
 }
 
 // Enqueues one frame: the frame-start speeds, the substeps, the frame-end bounds.
+void motion_energy(const HostBody& hb, const float g[3], double& ke, double& pe);
+
 // Heightfield::max_height (src/phys/static_world.cpp:220-231) over the terrain's vertex blocks
 float terrain_max_height(const bl_ctx* c, float x0, float z0, float x1, float z1) {
     int bx0 = (int)std::floor((x0 - c->tox) / c->tcell) / 16, bx1 = (int)std::floor((x1 - c->tox) / c->tcell) / 16;
@@ -1175,6 +1179,9 @@ float terrain_max_height(const bl_ctx* c, float x0, float z0, float x1, float z1
 int enqueue_frame(bl_ctx* c, int substeps) {
     if (substeps <= 0) return BL_OK;
     int rc;
+    // (the energy guard's start: src/phys/world.cpp:3236; the guarded bodies' host nodes are current)
+    for (HostBody& hb : c->bodies)
+        if (hb.opts.energy_guard) motion_energy(hb, c->s.gravity, hb.guard_ke, hb.guard_pe);
     if (c->tnx) {
         // each body's terrain top over its reach this frame: its bounds widened by twice what its top speed covers
         // in the frame and half a metre (World::build_islands, src/phys/world.cpp:743-756)
@@ -1197,6 +1204,57 @@ int enqueue_frame(bl_ctx* c, int substeps) {
 // The post-frame check of World::simulate_island (src/phys/world.cpp:3962-3979) on the frame just run: a body whose
 // bounds are not finite or wider than 800 m, or whose top speed passed 360 m/s, is repaired on the host, or frozen
 // when it cannot be.
+// SoftBody::motion_energy (src/phys/softbody.cpp:798-803), the nodes' part: in double, in node order
+void motion_energy(const HostBody& hb, const float g[3], double& ke, double& pe) {
+    ke = pe = 0;
+    for (const bl_node& x : hb.nodes)
+        if (x.inv_mass > 0) {
+            ke += 0.5 * x.mass * (x.v[0] * x.v[0] + x.v[1] * x.v[1] + x.v[2] * x.v[2]);
+            pe -= x.mass * (double)(g[0] * x.p[0] + g[1] * x.p[1] + g[2] * x.p[2]);
+        }
+}
+
+// The energy guard after a frame (World::simulate_island, src/phys/world.cpp:3995-3996; SoftBody::guard_energy,
+// src/phys/softbody.cpp:805-815): a body quiet for more than 15 frames (no collision volume or sphere contact - none
+// here) that gained more than 2 J and 5 % of its kinetic energy at the frame's start has its velocities scaled back.
+// The body's nodes on the host are current. True when it scaled them.
+bool guard_body(bl_ctx* c, HostBody& hb) {
+    hb.guard_quiet++;
+    if (!hb.opts.energy_guard || hb.guard_quiet <= 15) return false;
+    double ke, pe;
+    motion_energy(hb, c->s.gravity, ke, pe);
+    const double excess = (ke + pe) - (hb.guard_ke + hb.guard_pe) - (2.0 + 0.05 * hb.guard_ke);
+    if (!(excess > 0) || !(ke > 0)) return false;
+    const float s = (float)std::sqrt(std::max(0.0, ke - excess) / ke);
+    for (bl_node& x : hb.nodes) x.v[0] *= s, x.v[1] *= s, x.v[2] *= s;
+    hb.guard_cuts++;
+    return true;
+}
+
+// A body's nodes from the device's current state into the host copy, or back (p, inv_mass; v, mass)
+int body_nodes(bl_ctx* c, HostBody& hb, bool to_device) {
+    const size_t n = hb.nodes.size();
+    if (!n) return BL_OK;
+    std::vector<float4> P(n), V(n);
+    if (to_device) {
+        for (size_t i = 0; i < n; i++) {
+            const bl_node& x = hb.nodes[i];
+            P[i] = make_float4(x.p[0], x.p[1], x.p[2], x.inv_mass), V[i] = make_float4(x.v[0], x.v[1], x.v[2], x.mass);
+        }
+        HIPCHK(hipMemcpy(c->pm[c->parity].p + hb.node0, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->vm[c->parity].p + hb.node0, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        return BL_OK;
+    }
+    HIPCHK(hipMemcpy(P.data(), c->pm[c->parity].p + hb.node0, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(V.data(), c->vm[c->parity].p + hb.node0, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        bl_node& x = hb.nodes[i];
+        x.p[0] = P[i].x, x.p[1] = P[i].y, x.p[2] = P[i].z, x.inv_mass = P[i].w;
+        x.v[0] = V[i].x, x.v[1] = V[i].y, x.v[2] = V[i].z, x.mass = V[i].w;
+    }
+    return BL_OK;
+}
+
 int post_frame(bl_ctx* c) {
     if (!c->frame_pending) return BL_OK;
     c->frame_pending = false;
@@ -1234,12 +1292,24 @@ int post_frame(bl_ctx* c) {
         if (!std::isfinite(mn.x) || !std::isfinite(mx.y) || smax(e.x, smax(e.y, e.z)) > 800.0f || ms > lim)
             trouble.push_back(k);
     }
-    if (trouble.empty()) return BL_OK;
-    int rc = pull_state(c);
-    if (rc) return rc;
+    int rc;
+    if (trouble.empty()) {
+        // (the guarded bodies' nodes come back each frame: the next frame's start needs their energy too)
+        for (HostBody& hb : c->bodies)
+            if (hb.opts.energy_guard) {
+                if ((rc = body_nodes(c, hb, false))) return rc;
+                if (guard_body(c, hb) && (rc = body_nodes(c, hb, true))) return rc;
+            } else {
+                hb.guard_quiet++;
+            }
+        return BL_OK;
+    }
+    if ((rc = pull_state(c))) return rc;
+    std::vector<char> frozen(nb, 0);
     for (size_t k : trouble) {
         HostBody& hb = c->bodies[k];
         const int fixed = repair_body(hb);
+        frozen[k] = fixed < 0;
         if (fixed > 0) {
             // bounds and top speed of the repaired body (src/phys/world.cpp:4106-4109)
             hb.compute_bounds();
@@ -1256,6 +1326,9 @@ int post_frame(bl_ctx* c) {
         }
         c->repairs++;
     }
+    // (a frozen body skips the rest of the frame's end: src/phys/world.cpp:3967-3976)
+    for (size_t k = 0; k < nb; k++)
+        if (!frozen[k]) guard_body(c, c->bodies[k]);
     // the repaired state goes back; the counters survive the rebuild
     std::vector<unsigned long long> cnt(CNT_N, 0), brk(nb, 0);
     HIPCHK(hipMemcpy(cnt.data(), c->counters.p, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost));

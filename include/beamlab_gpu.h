@@ -247,6 +247,8 @@ typedef struct bl_body_opts {
     float air_drag;
     float aero_cda;
     float rest_damp;
+    /* SoftBody::energy_guard (src/phys/softbody.h:790-799): after 15 quiet frames, a frame's energy gain beyond 2 J
+     * and 5 % of its kinetic energy at the frame's start is scaled back (checked on the host after each frame) */
     int32_t energy_guard;
 } bl_body_opts;
 

@@ -133,6 +133,8 @@ typedef struct {
     int controlled;
     float wheel_speed, wheel_spin;
     float terrain_max; /* the terrain's top over the body's reach this frame (World::build_islands) */
+    double guard_ke, guard_pe; /* the energy guard's motion energy at the frame's start */
+    int guard_quiet, guard_cuts;
 } orc_body;
 
 struct orc_world {
@@ -1013,6 +1015,35 @@ static int repair_nodes(orc_body* B) {
 
 /* The post-frame check of World::simulate_island (src/phys/world.cpp:3962-3979): a body whose bounds are not
  * finite, wider than 800 m, or whose top speed in the frame passed 360 m/s is repaired, or frozen when it cannot be. */
+/* SoftBody::motion_energy (src/phys/softbody.cpp:798-803), the nodes' part */
+static void motion_energy(const orc_body* B, v3 g, double* ke, double* pe) {
+    *ke = *pe = 0;
+    for (uint32_t i = 0; i < B->n; i++) {
+        const bl_node* x = &B->nodes[i];
+        if (x->inv_mass > 0) {
+            const v3 v = ld3(x->v);
+            *ke += 0.5 * x->mass * vdot(v, v);
+            *pe -= x->mass * (double)vdot(g, ld3(x->p));
+        }
+    }
+}
+
+/* SoftBody::guard_energy (src/phys/softbody.cpp:805-815): a frame that gained energy beyond 2 J and 5 % of its
+ * kinetic energy at the start is scaled back */
+static void guard_energy(orc_body* B, v3 g) {
+    double ke, pe;
+    motion_energy(B, g, &ke, &pe);
+    const double excess = (ke + pe) - (B->guard_ke + B->guard_pe) - (2.0 + 0.05 * B->guard_ke);
+    if (!(excess > 0) || !(ke > 0)) return;
+    const double left = ke - excess;
+    const float s = (float)sqrt((0.0 < left ? left : 0.0) / ke);
+    for (uint32_t i = 0; i < B->n; i++) {
+        bl_node* x = &B->nodes[i];
+        x->v[0] *= s, x->v[1] *= s, x->v[2] *= s;
+    }
+    B->guard_cuts++;
+}
+
 static void post_frame(orc_world* w, orc_body* B) {
     const float lim = 400.0f * 0.9f;
     v3 e = vsub(B->mx, B->mn);
@@ -1027,9 +1058,14 @@ static void post_frame(orc_world* w, orc_body* B) {
             }
             B->mn = V(1e30f, 1e30f, 1e30f), B->mx = V(-1e30f, -1e30f, -1e30f);
             for (uint32_t i = 0; i < B->n; i++) B->mn = vmin3(B->mn, ld3(B->nodes[i].p)), B->mx = vmax3(B->mx, ld3(B->nodes[i].p));
+            B->repairs += 1;
+            return; /* (frozen: src/phys/world.cpp:3967-3976 skips the rest) */
         }
         B->repairs += 1;
     }
+    /* (src/phys/world.cpp:3995-3996: no collision volumes or sphere contacts here, so every frame is a quiet one) */
+    B->guard_quiet++;
+    if (B->opts.energy_guard && B->guard_quiet > 15) guard_energy(B, ld3(w->s.gravity));
 }
 
 /* ------------------------------------------------------------------ inter-body contacts (slow bodies) */
@@ -2024,6 +2060,8 @@ int orc_step(orc_world* w, int substeps) {
                 m2 = smax(m2, vdot(v, v));
             }
             B->max_speed = sqrtf(m2);
+            /* (the energy guard's start, src/phys/world.cpp:3236) */
+            if (B->opts.energy_guard) motion_energy(B, g, &B->guard_ke, &B->guard_pe);
         }
         I->next_rebuild = 0;
         for (int s = 0; s < substeps; s++) {
