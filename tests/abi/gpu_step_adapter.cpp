@@ -21,6 +21,7 @@ public:
         s.frame_theta = w.settings.frame_theta;
         s.frame_dissipation = w.settings.frame_dissipation;
         s.sleeping = 0; // sleeping bodies are not stepped on the GPU
+        s.tyre_grip = w.settings.tyre_grip;
         check(bl_create(&ctx_, &s));
     }
     ~GpuStep() { bl_destroy(ctx_); }
@@ -29,6 +30,24 @@ public:
 
     void set_statics(const std::vector<bl_box>& boxes) { // StaticWorld boxes (src/phys/static_world.cpp:298-306)
         check(bl_set_statics(ctx_, boxes.data(), (int32_t)boxes.size()));
+    }
+    // StaticWorld's cylinders and terrain (a Heightfield: its vertices row by row, holes as they read)
+    void set_terrain_and_cylinders(const StaticWorld& sw) {
+        std::vector<bl_cylinder> cyl;
+        for (const StaticCylinder& c : sw.cylinders)
+            cyl.push_back(bl_cylinder{{c.base.x, c.base.y, c.base.z}, c.radius, c.height, c.surface});
+        check(bl_set_cylinders(ctx_, cyl.data(), (int32_t)cyl.size()));
+        if (!sw.has_terrain) {
+            check(bl_set_terrain(ctx_, 0, 0, 0.0f, nullptr, nullptr, nullptr));
+            return;
+        }
+        const Heightfield& t = sw.terrain;
+        std::vector<float> h((size_t)t.nx() * t.nz());
+        std::vector<uint8_t> sf(h.size());
+        for (int z = 0; z < t.nz(); z++)
+            for (int x = 0; x < t.nx(); x++) h[(size_t)z * t.nx() + x] = t.h(x, z), sf[(size_t)z * t.nx() + x] = t.surf(x, z);
+        const float origin[2] = {t.origin().x, t.origin().y};
+        check(bl_set_terrain(ctx_, t.nx(), t.nz(), t.cell(), origin, h.data(), sf.data()));
     }
     // World::add_body (src/phys/world.cpp:280-300): the body's arrays as they are, its options from its fields
     uint32_t add(SoftBody& b) {
@@ -101,6 +120,21 @@ public:
             x.broken = j.broken ? 1u : 0u;
             joints.push_back(x);
         }
+        // its wheels (Wheel: src/phys/softbody.h:468-490), their tyre nodes in one list
+        std::vector<bl_wheel> wheels;
+        std::vector<uint32_t> wnodes;
+        for (const Wheel& w : b.wheels) {
+            bl_wheel x{};
+            x.axle0 = w.axle0, x.axle1 = w.axle1, x.node0 = (uint32_t)wnodes.size(), x.n_nodes = (uint32_t)w.nodes.size();
+            x.radius = w.radius, x.propulsed = w.propulsed, x.arm = w.arm, x.near_attach = w.near_attach, x.mass = w.mass;
+            x.torque = w.torque, x.drive_torque = w.drive_torque, x.brake = w.brake;
+            x.speed = w.speed, x.avg_speed = w.avg_speed, x.last_torque = w.last_torque, x.last_retorque = w.last_retorque;
+            x.detached = w.detached ? 1u : 0u, x.ring = w.ring ? 1u : 0u;
+            wnodes.insert(wnodes.end(), w.nodes.begin(), w.nodes.end());
+            wheels.push_back(x);
+        }
+        d.wheels = wheels.data(), d.n_wheels = (uint32_t)wheels.size();
+        d.wheel_nodes = wnodes.data(), d.n_wheel_nodes = (uint32_t)wnodes.size();
         d.frames = frames.data(), d.n_frames = (uint32_t)frames.size();
         d.joints = joints.data(), d.n_joints = (uint32_t)joints.size();
         d.shocks = shocks.data(), d.n_shocks = (uint32_t)shocks.size();
@@ -113,7 +147,11 @@ public:
         bodies_.push_back(&b);
         return id;
     }
-    // World::step_substeps for the GPU bodies; their arrays (and slide nodes', frames' and joints' state) read back
+    // A vehicle's controller (its pre_substep hook's torque and brake per wheel) for the substeps of the next steps
+    void set_wheel_controls(uint32_t id, const std::vector<float>& torque, const std::vector<float>& brake) {
+        check(bl_set_wheel_controls(ctx_, id, torque.data(), brake.data()));
+    }
+    // World::step_substeps for the GPU bodies; their arrays (and slide nodes', frames', joints' and wheels' state) read back
     void step_substeps(int n) {
         check(bl_step(ctx_, n));
         for (uint32_t i = 0; i < bodies_.size(); i++) {
@@ -134,6 +172,18 @@ public:
                     j.plastic_lin = jt[q].plastic_lin, j.plastic_ang = jt[q].plastic_ang, j.stress = jt[q].stress;
                     j.broken = jt[q].broken != 0;
                 }
+            }
+            if (!b.wheels.empty()) {
+                std::vector<bl_wheel> wh(b.wheels.size());
+                float ss[2];
+                check(bl_read_wheels(ctx_, i, wh.data(), ss));
+                for (size_t q = 0; q < wh.size(); q++) {
+                    Wheel& w = b.wheels[q];
+                    w.torque = wh[q].torque, w.drive_torque = wh[q].drive_torque, w.brake = wh[q].brake;
+                    w.speed = wh[q].speed, w.avg_speed = wh[q].avg_speed;
+                    w.last_torque = wh[q].last_torque, w.last_retorque = wh[q].last_retorque;
+                }
+                b.wheel_speed = ss[0], b.wheel_spin = ss[1];
             }
             if (b.slides.empty()) continue;
             std::vector<bl_slide> sl(b.slides.size());
