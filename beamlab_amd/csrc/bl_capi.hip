@@ -166,6 +166,7 @@ struct bl_ctx {
     DevArr<bl_joint> xjt;
     DevArr<float4> fq, fw, ftq;
     DevArr<uint32_t> fnode;
+    DevArr<int2> fref;
     DevArr<bl_wheel> xwh;
     DevArr<uint32_t> xwn;
     DevArr<float4> xctl;
@@ -224,7 +225,7 @@ struct bl_ctx {
         F.release(), tri_abc.release(), cap_data.release(), tri_rest2.release(), tri_torn.release(), tri_hull.release(), cbox.release();
         wstat.release();
         xbody.release(), shx.release(), xsh.release(), xsl.release(), xrail.release();
-        xjt.release(), fq.release(), fw.release(), ftq.release(), fnode.release();
+        xjt.release(), fq.release(), fw.release(), ftq.release(), fnode.release(), fref.release();
         xwh.release(), xwn.release(), xctl.release(), xwsp.release(), xty.release(), xtsum.release(), xtneed.release();
         xtslide.release();
         xargs = XArgs{};
@@ -681,6 +682,7 @@ int build(bl_ctx* c) {
         std::vector<uint32_t> rail, fn;
         std::vector<bl_joint> jt;
         std::vector<float4> fq4, fw4, ctl;
+        std::vector<int2> fr2;
         std::vector<bl_wheel> wh;
         std::vector<uint32_t> wn;
         std::vector<uint2> ty;
@@ -716,6 +718,9 @@ int build(bl_ctx* c) {
                 fn.push_back(hb.node0 + f.node);
                 fq4.push_back(make_float4(f.q[0], f.q[1], f.q[2], f.q[3]));
                 fw4.push_back(make_float4(f.w[0], f.w[1], f.w[2], f.inv_inertia));
+                const int32_t nn = (int32_t)hb.nodes.size();
+                const bool kin = f.ref_x >= 0 && f.ref_y >= 0 && f.ref_x < nn && f.ref_y < nn;
+                fr2.push_back(kin ? make_int2((int32_t)hb.node0 + f.ref_x, (int32_t)hb.node0 + f.ref_y) : make_int2(-1, -1));
             }
             for (const bl_joint& j : hb.joints) {
                 bl_joint y = j;
@@ -751,11 +756,11 @@ int build(bl_ctx* c) {
         if ((!xb.empty() && (rc = c->xbody.upload(xb))) || (!sx.empty() && (rc = c->shx.upload(sx))) ||
             (!sh.empty() && (rc = c->xsh.upload(sh))) || (!sl.empty() && (rc = c->xsl.upload(sl))) ||
             (!rail.empty() && (rc = c->xrail.upload(rail))) || (!jt.empty() && (rc = c->xjt.upload(jt))) ||
-            (!fn.empty() && ((rc = c->fnode.upload(fn)) || (rc = c->fq.upload(fq4)) || (rc = c->fw.upload(fw4)) ||
+            (!fn.empty() && ((rc = c->fnode.upload(fn)) || (rc = c->fq.upload(fq4)) || (rc = c->fw.upload(fw4)) || (rc = c->fref.upload(fr2)) ||
                              (rc = c->ftq.upload(std::vector<float4>(fn.size(), make_float4(0, 0, 0, 0)))))))
             return rc;
         c->xargs = XArgs{c->xbody.p, (uint32_t)(xb.size() / 3), c->xsh.p, c->shx.p, c->xsl.p, c->xrail.p,
-                         c->xjt.p, c->fq.p, c->fw.p, c->ftq.p, c->fnode.p, (uint32_t)fn.size(),
+                         c->xjt.p, c->fq.p, c->fw.p, c->ftq.p, c->fnode.p, c->fref.p, (uint32_t)fn.size(),
                          c->xwh.p, c->xwn.p, c->xctl.p, c->xwsp.p, ty.empty() ? nullptr : c->xty.p, c->xtsum.p,
                          c->xtneed.p, c->xtslide.p, c->s.tyre_grip};
     }
@@ -944,7 +949,7 @@ int enqueue_segment(bl_ctx* c, const bl_tuning& t, const PArgs& p, uint32_t nblk
             if ((rc = c->contacts.launch_collide(a, p, nblk, c->stream, parity))) return rc;
         }
         if ((rc = launch_tyres(a, c->xargs, c->stream, parity)) || (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) ||
-            (rc = launch_frames(c->xargs, c->stream, a.dt)))
+            (rc = launch_frames(a, c->xargs, c->stream, parity ^ 1)))
             return rc;
         parity ^= 1;
     }
@@ -1073,7 +1078,8 @@ int contact_frame(bl_ctx* c, int substeps, const hipEvent_t* kev = nullptr) {
                 return rc;
             }
             if ((rc = launch_tyres(a, c->xargs, c->stream, c->parity)) ||
-                (rc = launch_integrate(a, c->stream, c->parity, c->wstat.p)) || (rc = launch_frames(c->xargs, c->stream, a.dt)))
+                (rc = launch_integrate(a, c->stream, c->parity, c->wstat.p)) ||
+                (rc = launch_frames(a, c->xargs, c->stream, c->parity ^ 1)))
                 return rc;
             c->parity ^= 1;
             nnb = 0;
@@ -1127,7 +1133,7 @@ int enqueue_split(bl_ctx* c, int substeps) {
     auto one = [&](int parity) -> int {
         if ((rc = launch_forces(a, c->stream, parity, t)) || (rc = launch_extras(a, c->xargs, c->stream, parity)) ||
             (rc = launch_tyres(a, c->xargs, c->stream, parity)) || (rc = launch_integrate(a, c->stream, parity, c->wstat.p)) ||
-            (rc = launch_frames(c->xargs, c->stream, a.dt)))
+            (rc = launch_frames(a, c->xargs, c->stream, parity ^ 1)))
             return rc;
         return BL_OK;
     };
@@ -1603,10 +1609,7 @@ int bl_add_body(bl_ctx* c, const bl_body_desc* d, uint32_t* id) {
         return BL_ERR_ARG;
     for (uint32_t q = 0; q < d->n_frames; q++) {
         if (d->frames[q].node >= d->n_nodes) return BL_ERR_ARG;
-        if (d->frames[q].ref_x >= 0 || d->frames[q].ref_y >= 0) {
-            g_last_error = "kinematic frames (a node's triad) are not stepped on the GPU";
-            return BL_ERR_UNSUPPORTED;
-        }
+        // (a triad's nodes out of range make a dynamic frame, as in SoftBody::integrate_frames)
     }
     for (uint32_t q = 0; q < d->n_joints; q++) {
         const bl_joint& j = d->joints[q];

@@ -1493,11 +1493,54 @@ int launch_tyres(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
     return BL_OK_OR_HIP();
 }
 
-// SoftBody::integrate_frames (src/phys/softbody.cpp:1006-1021), dynamic frames: a thread per frame
-__global__ void k_frames(XArgs x, float dt) {
+__device__ __forceinline__ f3 normalize_or(f3 a, f3 fb) {
+    const float l = length(a);
+    return l > 1e-12f ? mul(a, 1.0f / l) : fb;
+}
+
+// from_mat3 (src/core/math.h:171-201): the rotation of the matrix with columns c0, c1, c2
+__device__ qd quat_from_cols(f3 c0, f3 c1, f3 c2) {
+    const float m00 = c0.x, m11 = c1.y, m22 = c2.z;
+    const float tr = m00 + m11 + m22;
+    qd q;
+    if (tr > 0) {
+        const float s = fsqrt(tr + 1.0f) * 2;
+        q = qmk((c1.z - c2.y) / s, (c2.x - c0.z) / s, (c0.y - c1.x) / s, 0.25f * s);
+    } else if (m00 > m11 && m00 > m22) {
+        const float s = fsqrt(1.0f + m00 - m11 - m22) * 2;
+        q = qmk(0.25f * s, (c1.x + c0.y) / s, (c2.x + c0.z) / s, (c1.z - c2.y) / s);
+    } else if (m11 > m22) {
+        const float s = fsqrt(1.0f + m11 - m00 - m22) * 2;
+        q = qmk((c1.x + c0.y) / s, 0.25f * s, (c2.y + c1.z) / s, (c2.x - c0.z) / s);
+    } else {
+        const float s = fsqrt(1.0f + m22 - m00 - m11) * 2;
+        q = qmk((c2.x + c0.z) / s, (c2.y + c1.z) / s, 0.25f * s, (c0.y - c1.x) / s);
+    }
+    return qnormalize(q);
+}
+
+// SoftBody::integrate_frames (src/phys/softbody.cpp:997-1021): a thread per frame, after the nodes' integration
+// (P: the integrated positions); a kinematic frame takes its triad's orientation (x towards ref_x, z normal to the
+// plane of ref_y), a dynamic one integrates its torque
+__global__ void k_frames(const float4* __restrict__ P, XArgs x, float dt) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= x.nframes) return;
     const float4 W = x.fw[f], T = x.ftq[f];
+    const int2 rf = x.fref[f];
+    if (rf.x >= 0) {
+        const float4 pn = P[x.fnode[f]], px = P[rf.x], py = P[rf.y];
+        const f3 p = mk(pn.x, pn.y, pn.z);
+        const f3 e1 = normalize_or(sub(mk(px.x, px.y, px.z), p), mk(1, 0, 0));
+        f3 e3 = normalize_or(cross(e1, sub(mk(py.x, py.y, py.z), p)), mk(0, 0, 1));
+        const f3 t = cross(e1, e3);
+        if (dot(t, t) < 1e-12f) e3 = normalize_or(cross(e1, mk(0, 1, 0)), mk(0, 0, 1));
+        const f3 e2 = cross(e3, e1);
+        const qd q = qnormalize(quat_from_cols(e1, e2, e3));
+        x.fq[f] = make_float4(q.x, q.y, q.z, q.w);
+        x.fw[f] = make_float4(0, 0, 0, W.w);
+        x.ftq[f] = make_float4(0, 0, 0, 0);
+        return;
+    }
     if (W.w <= 0) return;
     f3 w = add(mk(W.x, W.y, W.z), mul(mk(T.x, T.y, T.z), W.w * dt));
     w = mul(w, 1.0f - 0.5f * dt);
@@ -1509,8 +1552,9 @@ __global__ void k_frames(XArgs x, float dt) {
     x.fq[f] = make_float4(n.x, n.y, n.z, n.w);
 }
 
-int launch_frames(const XArgs& x, hipStream_t st, float dt) {
-    if (x.nframes) hipLaunchKernelGGL(k_frames, dim3((x.nframes + 63) / 64), dim3(64), 0, st, x, dt);
+int launch_frames(const KArgs& a, const XArgs& x, hipStream_t st, int parity_new) {
+    if (x.nframes)
+        hipLaunchKernelGGL(k_frames, dim3((x.nframes + 63) / 64), dim3(64), 0, st, (const float4*)a.pm[parity_new], x, a.dt);
     return BL_OK_OR_HIP();
 }
 

@@ -122,6 +122,7 @@ struct XArgs {
     float4* fw;           // per frame: angular velocity, inverse inertia
     float4* ftq;          // per frame: torque of the substep
     const uint32_t* fnode;
+    const int2* fref;     // per frame: a kinematic frame's two reference nodes (global), or -1
     uint32_t nframes;
     // wheels: the records (axles, arm and near_attach global; node0 into wn), their tyre nodes, a controller's
     // (torque, brake, on) per wheel, and per such body its wheel_speed, wheel_spin
@@ -142,7 +143,7 @@ int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
 // integration, whose static contact then leaves those nodes' forces as they are
 int launch_tyres(const KArgs& a, const XArgs& x, hipStream_t st, int parity);
 // the frames' integration after the nodes' (SoftBody::integrate_frames)
-int launch_frames(const XArgs& x, hipStream_t st, float dt);
+int launch_frames(const KArgs& a, const XArgs& x, hipStream_t st, int parity_new); // parity_new: the integrated state
 
 int launch_substeps(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t);
 int launch_resident(const KArgs& a, hipStream_t st, int parity, int substeps, const bl_tuning& t, unsigned int* bar,

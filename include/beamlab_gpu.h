@@ -143,14 +143,14 @@ typedef struct bl_slide {
 } bl_slide;
 
 /* An oriented node (== bl::phys::Frame, src/phys/softbody.h:104-113): orientation and angular velocity, integrated
- * after the nodes (SoftBody::integrate_frames, src/phys/softbody.cpp:1006-1021) from the joints' torques. Kinematic
- * frames (ref_x, ref_y >= 0: the truck format's triads) are not stepped on the GPU. */
+ * after the nodes (SoftBody::integrate_frames, src/phys/softbody.cpp:997-1021) from the joints' torques; a kinematic
+ * frame (ref_x, ref_y: nodes of the body, the truck format's triads) takes its triad's orientation instead. */
 typedef struct bl_frame {
     uint32_t node;
     float q[4];              /* x, y, z, w */
     float w[3];              /* angular velocity (world) */
     float inv_inertia;       /* isotropic; 0: not integrated */
-    int32_t ref_x, ref_y;    /* must be -1 */
+    int32_t ref_x, ref_y;    /* the triad's nodes (x towards ref_x, z normal to the plane of ref_y); -1: dynamic */
 } bl_frame;
 
 /* An orientation-preserving joint (== bl::phys::Joint, src/phys/softbody.h:116-136; SoftBody::compute_joint_forces,

@@ -649,10 +649,51 @@ static void joint_forces(orc_body* B) {
     }
 }
 
-/* SoftBody::integrate_frames (src/phys/softbody.cpp:1006-1021), dynamic frames (kinematic triads are refused) */
+static inline v3 vnormalize_or_(v3 a, v3 fb) { float l = vlen(a); return l > 1e-12f ? vmul(a, 1.0f / l) : fb; }
+
+/* from_mat3 (src/core/math.h:171-201): the rotation of the matrix with columns c0, c1, c2 */
+static qt quat_from_cols(v3 c0, v3 c1, v3 c2) {
+    const float m00 = c0.x, m11 = c1.y, m22 = c2.z;
+    const float tr = m00 + m11 + m22;
+    qt q;
+    if (tr > 0) {
+        float s = sqrtf(tr + 1.0f) * 2;
+        q = Q((c1.z - c2.y) / s, (c2.x - c0.z) / s, (c0.y - c1.x) / s, 0.25f * s);
+    } else if (m00 > m11 && m00 > m22) {
+        float s = sqrtf(1.0f + m00 - m11 - m22) * 2;
+        q = Q(0.25f * s, (c1.x + c0.y) / s, (c2.x + c0.z) / s, (c1.z - c2.y) / s);
+    } else if (m11 > m22) {
+        float s = sqrtf(1.0f + m11 - m00 - m22) * 2;
+        q = Q((c1.x + c0.y) / s, 0.25f * s, (c2.y + c1.z) / s, (c2.x - c0.z) / s);
+    } else {
+        float s = sqrtf(1.0f + m22 - m00 - m11) * 2;
+        q = Q((c2.x + c0.z) / s, (c2.y + c1.z) / s, 0.25f * s, (c0.y - c1.x) / s);
+    }
+    return qnormalize(q);
+}
+
+/* SoftBody::triad_orientation (src/phys/softbody.cpp:997-1004): x towards ref_x, z normal to the plane of ref_y */
+static qt triad_orientation(const orc_body* B, uint32_t node, int32_t ref_x, int32_t ref_y) {
+    const v3 p = ld3(B->nodes[node].p);
+    const v3 e1 = vnormalize_or_(vsub(ld3(B->nodes[ref_x].p), p), V(1, 0, 0));
+    v3 e3 = vnormalize_or_(vcross_(e1, vsub(ld3(B->nodes[ref_y].p), p)), V(0, 0, 1));
+    const v3 t = vcross_(e1, e3);
+    if (vdot(t, t) < 1e-12f) e3 = vnormalize_or_(vcross_(e1, V(0, 1, 0)), V(0, 0, 1));
+    const v3 e2 = vcross_(e3, e1);
+    return qnormalize(quat_from_cols(e1, e2, e3));
+}
+
+/* SoftBody::integrate_frames (src/phys/softbody.cpp:1006-1021): a kinematic frame takes its triad's orientation,
+ * a dynamic one integrates its torque */
 static void integrate_frames(orc_body* B, float dt) {
     for (uint32_t k = 0; k < B->nfr; k++) {
         bl_frame* fr = &B->frames[k];
+        if (fr->ref_x >= 0 && fr->ref_y >= 0 && fr->ref_x < (int32_t)B->n && fr->ref_y < (int32_t)B->n) {
+            qstore(fr->q, triad_orientation(B, fr->node, fr->ref_x, fr->ref_y));
+            fr->w[0] = fr->w[1] = fr->w[2] = 0;
+            B->torque[k] = V(0, 0, 0);
+            continue;
+        }
         if (fr->inv_inertia <= 0) continue;
         v3 w = vadd(ld3(fr->w), vmul(B->torque[k], fr->inv_inertia * dt));
         w = vmul(w, 1.0f - 0.5f * dt);

@@ -197,11 +197,13 @@ def _unit_quat(rng):
     return q / np.float32(np.sqrt(np.sum(q * q, dtype=np.float32)))
 
 
-def arm_body(seed, origin=(0.0, 1.0, 0.0)):
+def arm_body(seed, origin=(0.0, 1.0, 0.0), kinematic=False):
     """A beam box carrying two jointed arms (oriented nodes joined by 6-DOF springs, the arms' last nodes without an
     orientation), a joint tying the second arm back to the box and one coupling the box's two oriented corners; the
     frames start turned at random and spinning. The arms hang sideways, so the fall onto the ground bends and pulls
-    their joints past their yield limits: some yield, some tear (the first arm's last joint, its middle bend)."""
+    their joints past their yield limits: some yield, some tear (the first arm's last joint, its middle bend).
+    kinematic: the box's two oriented corners follow their triads (Frame::ref_x, ref_y: x towards a box neighbour, z
+    normal to the plane of another), starting at the triads' orientations."""
     from beamlab_amd.world import SoftBody
     f = np.float32
     rng = np.random.default_rng(seed)
@@ -216,6 +218,10 @@ def arm_body(seed, origin=(0.0, 1.0, 0.0)):
     b.finalize()
     F = {n: b.add_frame(n, _unit_quat(rng), inertia) for n, inertia in ((c[0], 0.4), (c[7], 0.4), (a1, 0.05),
                                                                          (a2, 0.05), (a4, 0.06))}
+    if kinematic:
+        # (c0's triad is the identity - x to c4, z out of the plane of c2 -, c7's a half turn about z)
+        for n, rx, ry, q in ((c[0], c[4], c[2], (0.0, 0.0, 0.0, 1.0)), (c[7], c[3], c[5], (0.0, 0.0, 1.0, 0.0))):
+            b.frames[F[n]]["ref_x"], b.frames[F[n]]["ref_y"], b.frames[F[n]]["q"] = rx, ry, q
     b.add_joint(F[c[0]], a1, F[a1], 8e5, 300, 2.5e3, 6, break_force=5e3, break_torque=250, yield_angle=0.25,
                 max_stretch=0.05, max_bend=0.4)
     b.add_joint(F[a1], a2, F[a2], 6e5, 200, 1.5e3, 4, break_torque=60, max_bend=0.3)
@@ -232,13 +238,15 @@ def arm_body(seed, origin=(0.0, 1.0, 0.0)):
 
 
 def case_joints():
-    """Three arm bodies dropped onto the ground: one as built, one that may not break, one that may not deform."""
+    """Four arm bodies dropped onto the ground: one as built, one that may not break, one that may not deform, one
+    whose box corners' frames follow their triads."""
     b1 = arm_body(51)
     b2 = arm_body(52, origin=(4.0, 1.3, 0.5))
     b2.opts.allow_break = 0
     b3 = arm_body(53, origin=(-4.0, 1.1, -0.5))
     b3.opts.allow_deform = 0
-    return abi.default_settings(), _boxes(GROUND), [b1, b2, b3], 1500
+    b4 = arm_body(54, origin=(0.0, 1.0, 4.0), kinematic=True)
+    return abi.default_settings(), _boxes(GROUND), [b1, b2, b3, b4], 1500
 
 
 def car_body(seed, origin=(0.0, 0.0, 0.0), nseg=8, radius=0.3):

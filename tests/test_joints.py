@@ -7,7 +7,7 @@ linear spring carries its weight, a twisted frame on an angular spring swings wi
 decays at the frames' damping, an angular yield moves the rest orientation by the excess about the twist's own axis,
 and a joint pulled past its yield force carries exactly that force (the mass sinks at the speed where the damper
 takes the rest of its weight) until it tears. GPU: the HIP path against the oracle, bit for bit, joints included,
-without and with inter-body contacts, and the refusal of kinematic frames.
+without and with inter-body contacts; kinematic frames follow their triads.
 """
 import numpy as np
 import pytest
@@ -77,6 +77,35 @@ def test_joint_state_matches_reference(ref_mod):
     assert not b["broken"].any() and (b["plastic_ang"] > b["max_bend"]).any()
     assert not joints[2]["plastic_lin"].any() and not joints[2]["plastic_ang"].any()
     assert all((np.abs(orc.read_joints(i)[0]["w"]) > 0).all() for i in range(3))
+    assert not orc.read_joints(3)[0]["w"][:2].any()
+
+
+def test_kinematic_frame_follows_its_triad(oracle_mod):
+    """A kinematic frame's orientation turns x towards its ref_x node and z normal to the plane of its ref_y node (the
+    triad of the nodes as the substep left them), its spin zero."""
+    settings, boxes, bodies, n = CPU_CASES["joints"]()
+    w = oracle_mod.OracleWorld(settings, boxes)
+    body = bodies[3]
+    w.add_body(body)
+    w.step_substeps(700)
+    fr, _ = w.read_joints(0)
+    nodes = w.read_body(0)[0]
+    for f in fr[:2]:
+        q = f["q"].astype(np.float64)
+        p = nodes["p"].astype(np.float64)
+        ex = p[f["ref_x"]] - p[f["node"]]
+        ex /= np.linalg.norm(ex)
+        ez = np.cross(ex, p[f["ref_y"]] - p[f["node"]])
+        ez /= np.linalg.norm(ez)
+
+        def rot(v):
+            u = q[:3]
+            t = 2 * np.cross(u, v)
+            return v + q[3] * t + np.cross(u, t)
+
+        assert np.abs(rot(np.array([1.0, 0, 0])) - ex).max() < 2e-5
+        assert np.abs(rot(np.array([0, 0, 1.0])) - ez).max() < 2e-5
+        assert not f["w"].any()
 
 
 def test_joint_carries_a_hanging_weight(oracle_mod):
@@ -233,13 +262,3 @@ def test_gpu_joints_with_contacts_match_oracle(oracle_mod):
     gw.close()
 
 
-@pytest.mark.gpu
-def test_gpu_kinematic_frames_refused():
-    from beamlab_amd.world import World
-    w = World(abi.default_settings())
-    b = arm_body(70)
-    b.frames[0]["ref_x"], b.frames[0]["ref_y"] = 1, 2
-    with pytest.raises(abi.BeamlabError) as e:
-        w.add_body(b)
-    assert e.value.status == abi.BL_ERR_UNSUPPORTED
-    w.close()
