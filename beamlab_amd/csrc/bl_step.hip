@@ -1436,18 +1436,18 @@ int launch_extras(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
     return BL_OK_OR_HIP();
 }
 
-// World::collide_static's tyre nodes (src/phys/world.cpp:799-892, boxes only): a thread per body with tyre nodes, over
-// them in node order - the static contact by tyre_contact (a yielding obstacle's contact is the integration's), its
-// normal force into F; then each wheel's touching tyre nodes share its grip: the patch holds (each node gets the
-// force that would hold it) when its summed need is within its summed grip, else each slides. The integration counts
-// these contacts and leaves their forces as they are.
-__global__ void k_tyres(KArgs a, XArgs x, int parity) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= x.nbody) return;
+// World::collide_static's tyre nodes (src/phys/world.cpp:799-892): a workgroup per body with tyre nodes. Each tyre
+// node's static contact by tyre_contact (a yielding obstacle's contact is the integration's), its normal force into
+// F, its need, grip and slide force kept (the nodes are independent: a thread each); then each wheel sums its
+// touching tyre nodes' need and grip in node order (a thread per wheel: the order the float sums need); then each
+// node gets the force that would hold it when its wheel's patch holds (summed need within summed grip), else its
+// slide force. The integration counts these contacts and leaves their forces as they are. (Global scratch across
+// the workgroup's barriers: its waves share one CU's L1.)
+__global__ __launch_bounds__(256) void k_tyres(KArgs a, XArgs x, int parity) {
+    const uint32_t i = blockIdx.x;
     const uint4 r3 = x.body[3 * i + 2];
     if (r3.z >= r3.w) return;
-    for (uint32_t w = r3.x; w < r3.y; w++) x.tsum[w] = make_float4(0, 0, 0, 0);
-    for (uint32_t q = r3.z; q < r3.w; q++) {
+    for (uint32_t q = r3.z + threadIdx.x; q < r3.w; q += blockDim.x) {
         const uint2 e = x.ty[q];
         x.tslide[q] = make_float4(0, 0, 0, 0);
         const uint32_t info = a.ninfo[e.x];
@@ -1467,12 +1467,19 @@ __global__ void k_tyres(KArgs a, XArgs x, int parity) {
         if (!t.touching) continue;
         x.tneed[q] = make_float4(t.need.x, t.need.y, t.need.z, t.cap);
         x.tslide[q] = make_float4(t.slide.x, t.slide.y, t.slide.z, 1.0f);
-        if (e.y != 0xffffffffu) {
-            const float4 s = x.tsum[e.y];
-            x.tsum[e.y] = make_float4(s.x + t.need.x, s.y + t.need.y, s.z + t.need.z, s.w + t.cap);
-        }
     }
-    for (uint32_t q = r3.z; q < r3.w; q++) {
+    __syncthreads();
+    for (uint32_t w = r3.x + threadIdx.x; w < r3.y; w += blockDim.x) {
+        float4 s = make_float4(0, 0, 0, 0);
+        for (uint32_t q = r3.z; q < r3.w; q++) {
+            if (x.ty[q].y != w || x.tslide[q].w == 0) continue;
+            const float4 nd = x.tneed[q];
+            s = make_float4(s.x + nd.x, s.y + nd.y, s.z + nd.z, s.w + nd.w);
+        }
+        x.tsum[w] = s;
+    }
+    __syncthreads();
+    for (uint32_t q = r3.z + threadIdx.x; q < r3.w; q += blockDim.x) {
         const float4 sl = x.tslide[q];
         if (sl.w == 0) continue;
         const uint2 e = x.ty[q];
@@ -1489,7 +1496,7 @@ __global__ void k_tyres(KArgs a, XArgs x, int parity) {
 }
 
 int launch_tyres(const KArgs& a, const XArgs& x, hipStream_t st, int parity) {
-    if (x.ty && x.nbody) hipLaunchKernelGGL(k_tyres, dim3((x.nbody + 63) / 64), dim3(64), 0, st, a, x, parity);
+    if (x.ty && x.nbody) hipLaunchKernelGGL(k_tyres, dim3(x.nbody), dim3(256), 0, st, a, x, parity);
     return BL_OK_OR_HIP();
 }
 
